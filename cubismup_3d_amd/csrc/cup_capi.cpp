@@ -1,0 +1,467 @@
+// cup_capi.cpp -- the extern "C" surface of include/cup.h: context lifetime, field I/O,
+// and the operator entry points, each launching the hand-written kernels on the
+// context's stream. No exception crosses this boundary.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "cup_internal.h"
+
+namespace cup {
+
+int set_error(Ctx& c, int code, const std::string& msg)
+{
+  c.err = msg;
+  return code;
+}
+
+int check_hip(Ctx& c, hipError_t e, const char* what)
+{
+  if (e == hipSuccess) return CUP_OK;
+  c.err = std::string(what) + ": " + hipGetErrorString(e);
+  return CUP_E_DEVICE;
+}
+
+int ensure_partials(Ctx& c, size_t ndoubles)
+{
+  if (ndoubles <= c.partials_cap) return CUP_OK;
+  hipStreamSynchronize(c.stream);
+  if (c.partials) hipFree(c.partials);
+  c.partials = nullptr;
+  c.partials_cap = 0;
+  if (check_hip(c, hipMalloc(&c.partials, ndoubles * sizeof(double)), "hipMalloc partials")) return CUP_E_NOMEM;
+  c.partials_cap = ndoubles;
+  return CUP_OK;
+}
+
+static void release(Ctx& c)
+{
+  if (c.stream) hipStreamSynchronize(c.stream);
+  body_free(c);
+  for (auto& p : c.f)
+    if (p) { hipFree(p); p = nullptr; }
+  void** bufs[] = {&c.spec, &c.spec2, &c.spec3, &c.rbuf, &c.tw[0], &c.tw[1], &c.tw[2],
+                   &c.phase[0], &c.phase[1], &c.phase[2]};
+  for (void** b : bufs)
+    if (*b) { hipFree(*b); *b = nullptr; }
+  for (auto& p : c.cos4)
+    if (p) { hipFree(p); p = nullptr; }
+  if (c.partials) hipFree(c.partials);
+  if (c.low_k) hipFree(c.low_k);
+  if (c.low_coef) hipFree(c.low_coef);
+  if (c.sc) hipFree(c.sc);
+  if (c.sc_host) hipHostFree(c.sc_host);
+  if (c.host_red) hipHostFree(c.host_red);
+  c.partials = nullptr; c.low_k = nullptr; c.low_coef = nullptr; c.sc = nullptr; c.sc_host = nullptr;
+  c.host_red = nullptr;
+  if (c.stream) hipStreamDestroy(c.stream);
+  c.stream = nullptr;
+}
+
+// Reads DevScalars back (synchronizes the stream).
+static int read_scalars(Ctx& c)
+{
+  if (check_hip(c, hipMemcpyAsync(c.sc_host, c.sc, sizeof(DevScalars), hipMemcpyDeviceToHost, c.stream),
+                "read scalars"))
+    return CUP_E_DEVICE;
+  return check_hip(c, hipStreamSynchronize(c.stream), "synchronize");
+}
+
+}  // namespace cup
+
+using namespace cup;
+
+#define CUP_GUARD(ctx)                                 \
+  if (!(ctx)) return CUP_E_INVALID;                    \
+  Ctx& c = (ctx)->c;                                   \
+  if (hipSetDevice(c.dev) != hipSuccess) return set_error(c, CUP_E_DEVICE, "hipSetDevice");
+
+extern "C" {
+
+int cup_create(const cup_grid_desc* d, cup_ctx** out)
+{
+  if (!d || !out) return CUP_E_INVALID;
+  *out = nullptr;
+  for (int a = 0; a < 3; ++a)
+    if (d->bpd[a] < 1) return CUP_E_INVALID;
+  if (d->precision != 4 && d->precision != 8) return CUP_E_INVALID;
+  if (d->decomposition != CUP_PLAIN && d->decomposition != CUP_FUSED) return CUP_E_INVALID;
+  if (!(d->maxextent > 0)) return CUP_E_INVALID;
+  cup_ctx* h = new (std::nothrow) cup_ctx();
+  if (!h) return CUP_E_NOMEM;
+  Ctx& c = h->c;
+  c.dev = d->device;
+  c.precision = d->precision;
+  c.esz = (size_t)d->precision;
+  c.decomposition = d->decomposition;
+  int maxN = 0;
+  for (int a = 0; a < 3; ++a) {
+    c.bpd[a] = d->bpd[a];
+    c.N[a] = d->bpd[a] * BS;
+    maxN = c.N[a] > maxN ? c.N[a] : maxN;
+  }
+  c.nb = c.bpd[0] * c.bpd[1] * c.bpd[2];
+  c.ncell = (size_t)c.nb * BS3;
+  c.maxextent = d->maxextent;
+  c.h = d->maxextent / maxN;
+  for (int a = 0; a < 3; ++a) c.extent[a] = ((double)c.N[a] / maxN) * d->maxextent;
+  int rc = CUP_OK;
+  do {
+    if ((rc = check_hip(c, hipSetDevice(c.dev), "hipSetDevice"))) break;
+    if ((rc = check_hip(c, hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking), "hipStreamCreate"))) break;
+    for (int k = 0; k < 8 && !rc; ++k) {
+      if ((rc = check_hip(c, hipMalloc(&c.f[k], c.ncell * c.esz), "hipMalloc field"))) break;
+      rc = check_hip(c, hipMemsetAsync(c.f[k], 0, c.ncell * c.esz, c.stream), "hipMemset field");
+    }
+    if (rc) break;
+    if ((rc = check_hip(c, hipMalloc(&c.spec, c.ncell * c.esz), "hipMalloc spectrum"))) break;
+    if ((rc = check_hip(c, hipMalloc(&c.rbuf, c.ncell * c.esz), "hipMalloc rhs"))) break;
+    if ((rc = check_hip(c, hipMalloc(&c.sc, sizeof(DevScalars)), "hipMalloc scalars"))) break;
+    if ((rc = check_hip(c, hipMemsetAsync(c.sc, 0, sizeof(DevScalars), c.stream), "hipMemset scalars"))) break;
+    if ((rc = check_hip(c, hipHostMalloc((void**)&c.sc_host, sizeof(DevScalars), hipHostMallocDefault),
+                        "hipHostMalloc")))
+      break;
+    if ((rc = check_hip(c, hipHostMalloc((void**)&c.host_red, 64 * sizeof(double), hipHostMallocDefault),
+                        "hipHostMalloc")))
+      break;
+    if ((rc = ensure_partials(c, 20 * (size_t)c.nb + 1024))) break;
+    if ((rc = fft_setup(c))) break;
+    if ((rc = spectral_setup(c))) break;
+    rc = check_hip(c, hipStreamSynchronize(c.stream), "synchronize");
+  } while (0);
+  if (rc) {
+    release(c);
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return CUP_OK;
+}
+
+int cup_destroy(cup_ctx* ctx)
+{
+  if (!ctx) return CUP_E_INVALID;
+  hipSetDevice(ctx->c.dev);
+  release(ctx->c);
+  delete ctx;
+  return CUP_OK;
+}
+
+const char* cup_last_error(const cup_ctx* ctx) { return ctx ? ctx->c.err.c_str() : "null context"; }
+
+int cup_get_geometry(const cup_ctx* ctx, int cells[3], double* h, double extent[3])
+{
+  if (!ctx) return CUP_E_INVALID;
+  const Ctx& c = ctx->c;
+  for (int a = 0; a < 3; ++a) {
+    if (cells) cells[a] = c.N[a];
+    if (extent) extent[a] = c.extent[a];
+  }
+  if (h) *h = c.h;
+  return CUP_OK;
+}
+
+int cup_synchronize(cup_ctx* ctx)
+{
+  CUP_GUARD(ctx);
+  return check_hip(c, hipStreamSynchronize(c.stream), "synchronize");
+}
+
+int cup_stream(cup_ctx* ctx, void** s)
+{
+  CUP_GUARD(ctx);
+  if (!s) return CUP_E_INVALID;
+  *s = (void*)c.stream;
+  return CUP_OK;
+}
+
+int cup_set_field(cup_ctx* ctx, int field, const double* host)
+{
+  CUP_GUARD(ctx);
+  if (field < 0 || field > 7 || !host) return set_error(c, CUP_E_INVALID, "set_field: bad field or pointer");
+  hipStreamSynchronize(c.stream);
+  hipError_t e;
+  if (c.precision == 4) {
+    std::vector<float> tmp(c.ncell);
+    for (size_t i = 0; i < c.ncell; ++i) tmp[i] = (float)host[i];
+    e = hipMemcpy(c.f[field], tmp.data(), c.ncell * sizeof(float), hipMemcpyHostToDevice);
+  } else {
+    e = hipMemcpy(c.f[field], host, c.ncell * sizeof(double), hipMemcpyHostToDevice);
+  }
+  return check_hip(c, e, "set_field");
+}
+
+int cup_get_field(cup_ctx* ctx, int field, double* host)
+{
+  CUP_GUARD(ctx);
+  if (field < 0 || field > 7 || !host) return set_error(c, CUP_E_INVALID, "get_field: bad field or pointer");
+  if (int rc = check_hip(c, hipStreamSynchronize(c.stream), "synchronize")) return rc;
+  hipError_t e;
+  if (c.precision == 4) {
+    std::vector<float> tmp(c.ncell);
+    e = hipMemcpy(tmp.data(), c.f[field], c.ncell * sizeof(float), hipMemcpyDeviceToHost);
+    for (size_t i = 0; i < c.ncell; ++i) host[i] = tmp[i];
+  } else {
+    e = hipMemcpy(host, c.f[field], c.ncell * sizeof(double), hipMemcpyDeviceToHost);
+  }
+  return check_hip(c, e, "get_field");
+}
+
+int cup_field_device_ptr(cup_ctx* ctx, int field, void** dptr)
+{
+  CUP_GUARD(ctx);
+  if (field < 0 || field > 7 || !dptr) return set_error(c, CUP_E_INVALID, "field_device_ptr: bad field");
+  *dptr = c.f[field];
+  return CUP_OK;
+}
+
+int cup_lab_shift(cup_ctx* ctx, int g, int dx, int dy, int dz)
+{
+  CUP_GUARD(ctx);
+  if (int rc = launch_lab_shift(c, g, dx, dy, dz)) return rc;
+  return check_hip(c, hipStreamSynchronize(c.stream), "lab_shift");
+}
+
+int cup_max_velocity(cup_ctx* ctx, const double uinf[3], double* umax)
+{
+  CUP_GUARD(ctx);
+  if (!uinf || !umax) return set_error(c, CUP_E_INVALID, "max_velocity: null argument");
+  if (int rc = launch_max_u(c, uinf)) return rc;
+  if (int rc = launch_reduce(c, c.nb, 2, 1u, c.sc->red)) return rc;
+  if (int rc = read_scalars(c)) return rc;
+  *umax = c.sc_host->red[0];
+  return CUP_OK;
+}
+
+int cup_calc_max_timestep(cup_ctx* ctx, double CFL, double nu, const double uinf[3], int step, int rampup,
+                          double* dt_out, double* umax_out)
+{
+  CUP_GUARD(ctx);
+  if (!(CFL > 0) || nu < 0) return set_error(c, CUP_E_INVALID, "calc_max_timestep: CFL must be > 0 and nu >= 0");
+  double umax = 0;
+  if (int rc = cup_max_velocity(ctx, uinf, &umax)) return rc;
+  const double hMin = c.h;
+  const double dtDif = nu > 0 ? hMin * hMin / nu : INFINITY;
+  const double dtAdv = hMin / (umax + 1e-8);
+  const double m = std::fmin(dtDif, dtAdv);
+  double dt = CFL * m;
+  if (step < rampup) {
+    const double x = (step + 1.0) / rampup;
+    const double rampCFL = std::exp(std::log(1e-3) * (1 - x) + std::log(CFL) * x);
+    dt = rampCFL * m;
+  }
+  if (dt_out) *dt_out = dt;
+  if (umax_out) *umax_out = umax;
+  return launch_set_dt(c, dt);
+}
+
+int cup_advect_diffuse(cup_ctx* ctx, double dt, double nu, const double uinf[3], int scheme)
+{
+  CUP_GUARD(ctx);
+  if (!uinf || !(dt > 0)) return set_error(c, CUP_E_INVALID, "advect_diffuse: dt must be > 0");
+  if (int rc = launch_set_dt(c, dt)) return rc;
+  if (int rc = launch_advdiff(c, nu, uinf, scheme)) return rc;
+  if (c.decomposition == CUP_PLAIN)
+    if (int rc = launch_copy_tmp_to_u(c)) return rc;
+  return CUP_OK;
+}
+
+int cup_pressure_rhs(cup_ctx* ctx, double dt)
+{
+  CUP_GUARD(ctx);
+  if (!(dt > 0)) return set_error(c, CUP_E_INVALID, "pressure_rhs: dt must be > 0");
+  if (int rc = launch_set_dt(c, dt)) return rc;
+  if (c.decomposition == CUP_PLAIN) {
+    if (int rc = launch_fill(c, c.rbuf, c.ncell, 0.0)) return rc;  // PoissonSolver::reset
+    if (int rc = launch_rhs_plain(c)) return rc;
+    if (c.body.present)
+      if (int rc = launch_rhs_obstacle(c, c.rbuf)) return rc;
+  } else if (c.body.present) {
+    if (int rc = launch_rhs_obstacle(c, nullptr)) return rc;
+  }
+  return CUP_OK;
+}
+
+int cup_poisson_periodic(cup_ctx* ctx)
+{
+  CUP_GUARD(ctx);
+  if (c.decomposition == CUP_PLAIN) {
+    if (int rc = launch_poisson_plain(c)) return rc;
+    return launch_copy(c, c.f[CUP_P], c.rbuf, c.ncell);  // PoissonSolver::_fftw2cub
+  }
+  return launch_poisson_fused(c);
+}
+
+int cup_pressure_project(cup_ctx* ctx, double dt)
+{
+  CUP_GUARD(ctx);
+  if (!(dt > 0)) return set_error(c, CUP_E_INVALID, "pressure_project: dt must be > 0");
+  if (int rc = launch_set_dt(c, dt)) return rc;
+  if (c.decomposition == CUP_PLAIN) return launch_gradp_plain(c);
+  const double zero[3] = {0, 0, 0};
+  return launch_gradp_fused(c, zero);
+}
+
+int cup_diagnostics(cup_ctx* ctx, double nu, double out[20])
+{
+  CUP_GUARD(ctx);
+  if (!out) return set_error(c, CUP_E_INVALID, "diagnostics: null output");
+  if (int rc = launch_diagnostics(c, nu)) return rc;
+  if (int rc = launch_reduce(c, c.nb, 20, 0u, c.sc->red)) return rc;
+  if (int rc = read_scalars(c)) return rc;
+  for (int i = 0; i < 20; ++i) out[i] = c.sc_host->red[i];
+  return CUP_OK;
+}
+
+int cup_ic_taylor_green(cup_ctx* ctx, double uMax)
+{
+  CUP_GUARD(ctx);
+  if (int rc = launch_ic_taylor_green(c, uMax)) return rc;
+  return check_hip(c, hipStreamSynchronize(c.stream), "ic_taylor_green");
+}
+
+int cup_ic_hit(cup_ctx* ctx, double k0, double u_rms, uint64_t seed)
+{
+  CUP_GUARD(ctx);
+  if (!(k0 > 0) || !(u_rms > 0)) return set_error(c, CUP_E_INVALID, "ic_hit: k0 and u_rms must be > 0");
+  if (int rc = launch_ic_hit(c, k0, u_rms, seed)) return rc;
+  return check_hip(c, hipStreamSynchronize(c.stream), "ic_hit");
+}
+
+int cup_spectral_forcing(cup_ctx* ctx, double dt, double sim_dt, double nu, double tke_target,
+                         double injection_rate, double stats_out[5])
+{
+  CUP_GUARD(ctx);
+  if (!(dt > 0)) return set_error(c, CUP_E_INVALID, "spectral_forcing: dt must be > 0");
+  double st[5];
+  if (int rc = forcing_plain(c, dt, sim_dt, nu, tke_target, injection_rate, st)) return rc;
+  if (stats_out)
+    for (int i = 0; i < 5; ++i) stats_out[i] = st[i];
+  return CUP_OK;
+}
+
+int cup_chi_from_sphere(cup_ctx* ctx, const double center[3], double radius, double com_mass_out[4])
+{
+  CUP_GUARD(ctx);
+  if (!center || !(radius > 0)) return set_error(c, CUP_E_INVALID, "chi_from_sphere: radius must be > 0");
+  if (int rc = body_create(c, center, radius)) return rc;
+  if (com_mass_out) {
+    for (int a = 0; a < 3; ++a) com_mass_out[a] = c.body.com[a];
+    com_mass_out[3] = c.body.mass;
+  }
+  return CUP_OK;
+}
+
+int cup_penalize(cup_ctx* ctx, double dt, double lambda, int implicit, const double trans_vel[3],
+                 const double ang_vel[3], double ft_out[6])
+{
+  CUP_GUARD(ctx);
+  if (!c.body.present) return set_error(c, CUP_E_STATE, "penalize: no obstacle (call cup_chi_from_sphere)");
+  if (!(dt > 0) || !trans_vel || !ang_vel) return set_error(c, CUP_E_INVALID, "penalize: bad argument");
+  if (int rc = launch_set_dt(c, dt)) return rc;
+  if (int rc = launch_penalize(c, lambda, implicit, trans_vel, ang_vel)) return rc;
+  if (int rc = read_scalars(c)) return rc;
+  if (ft_out)
+    for (int i = 0; i < 6; ++i) ft_out[i] = c.sc_host->red[i];
+  return CUP_OK;
+}
+
+int cup_surface_forces(cup_ctx* ctx, double nu, const double trans_vel[3], const double ang_vel[3], double out[22])
+{
+  CUP_GUARD(ctx);
+  if (!c.body.present) return set_error(c, CUP_E_STATE, "surface_forces: no obstacle");
+  if (!trans_vel || !ang_vel) return set_error(c, CUP_E_INVALID, "surface_forces: bad argument");
+  if (int rc = launch_surface_forces(c, nu, trans_vel, ang_vel)) return rc;
+  if (int rc = read_scalars(c)) return rc;
+  if (out)
+    for (int i = 0; i < 22; ++i) out[i] = c.sc_host->red[i];
+  return CUP_OK;
+}
+
+// One step of the PLAIN decomposition, host-driven like Simulation::run (the reference's
+// operator order, dt from the host each step).
+static int plain_step(cup_ctx* ctx, const cup_step_params* p)
+{
+  Ctx& c = ctx->c;
+  double dt = p->fixed_dt, umax = 0;
+  if (!(dt > 0)) {
+    if (int rc = cup_calc_max_timestep(ctx, p->CFL, p->nu, p->uinf, (int)c.sc_host->step, p->rampup, &dt, &umax))
+      return rc;
+  }
+  if (int rc = cup_advect_diffuse(ctx, dt, p->nu, p->uinf, p->scheme)) return rc;
+  if (p->spectral_forcing)
+    if (int rc = cup_spectral_forcing(ctx, dt, dt, p->nu, p->tke_target, p->injection_rate, nullptr)) return rc;
+  if (int rc = cup_pressure_rhs(ctx, dt)) return rc;
+  if (int rc = cup_poisson_periodic(ctx)) return rc;
+  if (int rc = cup_pressure_project(ctx, dt)) return rc;
+  if (p->has_body && c.body.present)
+    if (int rc = cup_penalize(ctx, dt, p->lambda, p->implicit_penalization, p->body_vel, p->body_angvel, nullptr))
+      return rc;
+  c.sc_host->step += 1;
+  c.sc_host->time += dt;
+  c.sc_host->dt = dt;
+  c.sc_host->last_dt = dt;
+  return check_hip(c, hipMemcpyAsync(c.sc, c.sc_host, sizeof(DevScalars), hipMemcpyHostToDevice, c.stream), "scalars");
+}
+
+int cup_run_steps(cup_ctx* ctx, const cup_step_params* p, int nsteps)
+{
+  CUP_GUARD(ctx);
+  if (!p || nsteps < 0) return set_error(c, CUP_E_INVALID, "run_steps: bad argument");
+  if (!(p->CFL > 0) && !(p->fixed_dt > 0)) return set_error(c, CUP_E_INVALID, "run_steps: need CFL > 0 or fixed_dt");
+  if (p->has_body && !c.body.present) return set_error(c, CUP_E_STATE, "run_steps: has_body without obstacle");
+  if (c.decomposition == CUP_PLAIN) {
+    if (int rc = read_scalars(c)) return rc;
+    for (int s = 0; s < nsteps; ++s)
+      if (int rc = plain_step(ctx, p)) return rc;
+    return check_hip(c, hipStreamSynchronize(c.stream), "run_steps");
+  }
+  // FUSED: measure the current velocity and set dt for the coming step on the device.
+  if (int rc = launch_max_u(c, p->uinf)) return rc;
+  if (int rc = launch_finalize_step(c, c.nb, p->CFL, p->nu, p->rampup, p->fixed_dt, 0)) return rc;
+  for (int s = 0; s < nsteps; ++s) {
+    if (int rc = launch_advdiff(c, p->nu, p->uinf, p->scheme)) return rc;
+    if (p->spectral_forcing) {
+      double st[5];
+      if (int rc = forcing_plain(c, -1.0, -1.0, p->nu, p->tke_target, p->injection_rate, st)) return rc;
+    }
+    if (c.body.present && p->has_body)
+      if (int rc = launch_rhs_obstacle(c, nullptr)) return rc;
+    if (int rc = launch_poisson_fused(c)) return rc;
+    if (int rc = launch_gradp_fused(c, p->uinf)) return rc;
+    if (c.body.present && p->has_body)
+      if (int rc = launch_penalize_step(c, p->lambda, p->implicit_penalization, p->body_vel, p->body_angvel, p->uinf))
+        return rc;
+    if (int rc = launch_finalize_step(c, c.nb, p->CFL, p->nu, p->rampup, p->fixed_dt, 1)) return rc;
+  }
+  return CUP_OK;
+}
+
+int cup_step_state(cup_ctx* ctx, int* step, double* time, double* last_dt, double* umax, double* tke)
+{
+  CUP_GUARD(ctx);
+  if (int rc = read_scalars(c)) return rc;
+  if (step) *step = (int)c.sc_host->step;
+  if (time) *time = c.sc_host->time;
+  if (last_dt) *last_dt = c.sc_host->last_dt;
+  if (umax) *umax = c.sc_host->umax;
+  if (tke) *tke = c.sc_host->ke;
+  return CUP_OK;
+}
+
+int cup_set_step_state(cup_ctx* ctx, int step, double time)
+{
+  CUP_GUARD(ctx);
+  if (int rc = read_scalars(c)) return rc;
+  c.sc_host->step = step;
+  c.sc_host->time = time;
+  if (int rc = check_hip(c, hipMemcpy(c.sc, c.sc_host, sizeof(DevScalars), hipMemcpyHostToDevice), "scalars"))
+    return rc;
+  return CUP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,535 @@
+// cup_fft.hip -- hand-written 3-D real-to-complex FFT for the periodic Poisson solve.
+//
+// Spectral layout ("packed half spectrum"): complex S[z][y][kx], kx in [0, Nx/2). The
+// x transform is real-to-complex; its kx = 0 and kx = Nx/2 outputs are both real and are
+// packed into slot 0 as (X[0], X[Nx/2]). The y and z transforms then act on Nx/2 complex
+// columns. Slot 0 carries A + iB with A, B the kx = 0 and kx = Nx/2 planes; because the
+// grid-consistent Poisson symbol is real, even, and equal at kx = 0 and kx = Nx/2
+// (cos(4 pi k/N) = 1 at both), dividing the packed column by the kx = 0 symbol solves both
+// planes at once.
+//
+// Passes (each one HBM read + write of S; FFTs run in LDS, radix 8/4/2 Stockham):
+//   x-forward : 8 rows (one block row of a z plane) per workgroup, two real rows packed
+//               into one complex FFT; the source is either a real block-layout field or,
+//               fused, the central divergence of the advected velocity (PressureRHS).
+//   y         : TK consecutive kx columns x all y of one z plane per workgroup.
+//   z         : TK columns x all z of one y row; fused: forward, spectral division, inverse.
+//   x-inverse : 8 rows per workgroup, complex-to-real, written into a block-layout field.
+// Unnormalized in both directions like FFTW's r2c/c2r; the 1/N^3 and h factors live in the
+// division (PoissonSolverPeriodic.cpp:23-35).
+#include "cup_internal.h"
+
+#include <cmath>
+
+namespace cup {
+
+template <typename T>
+__device__ __forceinline__ C2<T> cmul(C2<T> a, C2<T> b)
+{
+  return C2<T>{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename T>
+__device__ __forceinline__ C2<T> cadd(C2<T> a, C2<T> b) { return C2<T>{a.x + b.x, a.y + b.y}; }
+template <typename T>
+__device__ __forceinline__ C2<T> csub(C2<T> a, C2<T> b) { return C2<T>{a.x - b.x, a.y - b.y}; }
+// a * e^{DIR i pi/2}: DIR=-1 -> -i a, DIR=+1 -> +i a
+template <int DIR, typename T>
+__device__ __forceinline__ C2<T> rot90(C2<T> a)
+{
+  return DIR < 0 ? C2<T>{a.y, -a.x} : C2<T>{-a.y, a.x};
+}
+
+template <int DIR, typename T>
+__device__ __forceinline__ void dft2(C2<T>* v)
+{
+  const C2<T> a = v[0], b = v[1];
+  v[0] = cadd(a, b);
+  v[1] = csub(a, b);
+}
+
+template <int DIR, typename T>
+__device__ __forceinline__ void dft4(C2<T>& v0, C2<T>& v1, C2<T>& v2, C2<T>& v3)
+{
+  const C2<T> t0 = cadd(v0, v2), t1 = csub(v0, v2), t2 = cadd(v1, v3), t3 = rot90<DIR>(csub(v1, v3));
+  v0 = cadd(t0, t2);
+  v2 = csub(t0, t2);
+  v1 = cadd(t1, t3);
+  v3 = csub(t1, t3);
+}
+
+template <int DIR, typename T>
+__device__ __forceinline__ void dft8(C2<T>* v)
+{
+  C2<T> e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+  C2<T> o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+  dft4<DIR>(e0, e1, e2, e3);
+  dft4<DIR>(o0, o1, o2, o3);
+  const T c = (T)0.70710678118654752440;
+  // W^k, W = e^{DIR 2 pi i / 8}
+  const C2<T> w1o = cmul(o1, C2<T>{c, (T)DIR * c});
+  const C2<T> w2o = rot90<DIR>(o2);
+  const C2<T> w3o = cmul(o3, C2<T>{-c, (T)DIR * c});
+  v[0] = cadd(e0, o0);
+  v[4] = csub(e0, o0);
+  v[1] = cadd(e1, w1o);
+  v[5] = csub(e1, w1o);
+  v[2] = cadd(e2, w2o);
+  v[6] = csub(e2, w2o);
+  v[3] = cadd(e3, w3o);
+  v[7] = csub(e3, w3o);
+}
+
+template <int R, int DIR, typename T>
+__device__ __forceinline__ void dftR(C2<T>* v)
+{
+  if constexpr (R == 8) dft8<DIR>(v);
+  else if constexpr (R == 4) dft4<DIR>(v[0], v[1], v[2], v[3]);
+  else dft2<DIR>(v);
+}
+
+constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
+
+// One Stockham stage (radix R, Ns = product of the earlier radices) over NFFT transforms
+// of length N held in LDS at buf[f + n*NFFT] (columns interleaved: the thread map
+// f-fastest is conflict free). tw[t] = e^{-2 pi i t/N}.
+template <int N, int NFFT, int R, int Ns, int DIR, typename T>
+__device__ __forceinline__ void fft_stage(C2<T>* buf, const C2<T>* __restrict__ tw)
+{
+  constexpr int NB = NFFT * (N / R);
+  constexpr int IT = (NB + NT - 1) / NT;
+  C2<T> v[IT][R];
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int t = threadIdx.x + it * NT;
+    if (NB % NT == 0 || t < NB) {
+      const int f = t % NFFT, j = t / NFFT, k = j % Ns;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[it][r] = buf[f + (j + r * (N / R)) * NFFT];
+      if (Ns > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          C2<T> w = tw[r * k * (N / (Ns * R))];
+          if (DIR > 0) w.y = -w.y;
+          v[it][r] = cmul(v[it][r], w);
+        }
+      }
+      dftR<R, DIR>(v[it]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int t = threadIdx.x + it * NT;
+    if (NB % NT == 0 || t < NB) {
+      const int f = t % NFFT, j = t / NFFT, k = j % Ns;
+      const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[f + (base + r * Ns) * NFFT] = v[it][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <int N, int NFFT, int Ns, int DIR, typename T>
+__device__ __forceinline__ void fft_stages(C2<T>* buf, const C2<T>* __restrict__ tw)
+{
+  if constexpr (Ns < N) {
+    constexpr int rem = N / Ns;
+    constexpr int R = rem >= 8 ? 8 : rem;
+    fft_stage<N, NFFT, R, Ns, DIR>(buf, tw);
+    fft_stages<N, NFFT, Ns * R, DIR>(buf, tw);
+  }
+}
+
+// NFFT complex transforms of length N in LDS, in place; caller synchronizes before.
+template <int N, int NFFT, int DIR, typename T>
+__device__ __forceinline__ void lds_fft(C2<T>* buf, const C2<T>* __restrict__ tw)
+{
+  fft_stages<N, NFFT, 1, DIR>(buf, tw);
+}
+
+// --------------------------------------------------------------------- x passes
+
+constexpr int XROWS = 8;         // rows per workgroup = one block row of a z plane
+constexpr int XFFT = XROWS / 2;  // complex FFTs per workgroup (two real rows each)
+
+enum XSrc { XSRC_FIELD = 0, XSRC_DIV = 1 };
+
+// Real rows -> packed half spectrum. grid = (Ny/8, Nz).
+template <typename T, int N, int SRC>
+__global__ __launch_bounds__(NT) void k_fft_x_fwd(const T* __restrict__ src0, const T* __restrict__ src1,
+                                                  const T* __restrict__ src2, C2<T>* __restrict__ S,
+                                                  const C2<T>* __restrict__ tw, GridArgs g,
+                                                  const DevScalars* __restrict__ sc, const int* __restrict__ ob_slot,
+                                                  const T* __restrict__ ob_rhs)
+{
+  __shared__ C2<T> buf[N * XFFT];
+  const int by = blockIdx.x, z = blockIdx.y;
+  const int bz = z >> 3, iz = z & 7;
+  T fac = 0;
+  if (SRC == XSRC_DIV) fac = (T)(0.5 * g.h * g.h / sc->dt);
+  T* rb = (T*)buf;
+  for (int t = threadIdx.x; t < N * XROWS; t += NT) {
+    const int bx = t >> 6, iy = (t >> 3) & 7, ix = t & 7;
+    const int x = bx * BS + ix, y = by * BS + iy;
+    const int b = bx + g.nbx * (by + g.nby * bz);
+    const size_t o = (size_t)b * BS3 + (iz * BS + iy) * BS + ix;
+    T val;
+    if (SRC == XSRC_FIELD) {
+      val = src0[o];
+    } else {
+      const int slot = ob_slot ? ob_slot[b] : -1;
+      if (slot >= 0) {
+        val = ob_rhs[(size_t)slot * BS3 + (iz * BS + iy) * BS + ix];
+      } else {
+        const T ue = src0[cell_index(g, wrapi(x + 1, g.Nx), y, z)];
+        const T uw = src0[cell_index(g, wrapi(x - 1, g.Nx), y, z)];
+        const T vn = src1[cell_index(g, x, wrapi(y + 1, g.Ny), z)];
+        const T vs = src1[cell_index(g, x, wrapi(y - 1, g.Ny), z)];
+        const T wb = src2[cell_index(g, x, y, wrapi(z + 1, g.Nz))];
+        const T wf = src2[cell_index(g, x, y, wrapi(z - 1, g.Nz))];
+        val = fac * (ue - uw + vn - vs + wb - wf);
+      }
+    }
+    rb[2 * (x * XFFT + (iy >> 1)) + (iy & 1)] = val;
+  }
+  __syncthreads();
+  lds_fft<N, XFFT, -1>(buf, tw);
+  // Separate the two real transforms of each complex FFT and pack kx = N/2 into slot 0.
+  for (int t = threadIdx.x; t < XROWS * (N / 2); t += NT) {
+    const int row = t / (N / 2), k = t % (N / 2);
+    const int q = row >> 1, odd = row & 1;
+    const C2<T> Z = buf[k * XFFT + q];
+    C2<T> out;
+    if (k == 0) {
+      const C2<T> Zn = buf[(N / 2) * XFFT + q];
+      out = odd ? C2<T>{Z.y, Zn.y} : C2<T>{Z.x, Zn.x};
+    } else {
+      const C2<T> Zm = buf[(N - k) * XFFT + q];
+      // Xa = (Z + conj Zm)/2 ; Xb = -i (Z - conj Zm)/2
+      out = odd ? C2<T>{(T)0.5 * (Z.y + Zm.y), (T)0.5 * (Zm.x - Z.x)}
+                : C2<T>{(T)0.5 * (Z.x + Zm.x), (T)0.5 * (Z.y - Zm.y)};
+    }
+    const int y = by * BS + row;
+    S[((size_t)z * g.Ny + y) * (N / 2) + k] = out;
+  }
+}
+
+// Packed half spectrum -> real rows of a block-layout field. grid = (Ny/8, Nz).
+template <typename T, int N>
+__global__ __launch_bounds__(NT) void k_fft_x_inv(const C2<T>* __restrict__ S, T* __restrict__ dst,
+                                                  const C2<T>* __restrict__ tw, GridArgs g)
+{
+  __shared__ C2<T> buf[N * XFFT];
+  const int by = blockIdx.x, z = blockIdx.y;
+  const int bz = z >> 3, iz = z & 7;
+  for (int t = threadIdx.x; t < XFFT * (N / 2); t += NT) {
+    const int q = t / (N / 2), k = t % (N / 2);
+    const int ya = by * BS + 2 * q;
+    const C2<T> A = S[((size_t)z * g.Ny + ya) * (N / 2) + k];
+    const C2<T> B = S[((size_t)z * g.Ny + ya + 1) * (N / 2) + k];
+    if (k == 0) {
+      buf[0 * XFFT + q] = C2<T>{A.x, B.x};
+      buf[(N / 2) * XFFT + q] = C2<T>{A.y, B.y};
+    } else {
+      buf[k * XFFT + q] = C2<T>{A.x - B.y, A.y + B.x};
+      buf[(N - k) * XFFT + q] = C2<T>{A.x + B.y, B.x - A.y};
+    }
+  }
+  __syncthreads();
+  lds_fft<N, XFFT, +1>(buf, tw);
+  const T* rb = (const T*)buf;
+  for (int t = threadIdx.x; t < N * XROWS; t += NT) {
+    const int bx = t >> 6, iy = (t >> 3) & 7, ix = t & 7;
+    const int x = bx * BS + ix;
+    const int b = bx + g.nbx * (by + g.nby * bz);
+    dst[(size_t)b * BS3 + (iz * BS + iy) * BS + ix] = rb[2 * (x * XFFT + (iy >> 1)) + (iy & 1)];
+  }
+}
+
+// ------------------------------------------------------------------ y / z passes
+
+template <typename T, int N>
+struct ColTile {
+  // columns per workgroup: LDS tile at most 64 KiB, at most 16 columns
+  static constexpr int raw = 65536 / (N * (int)sizeof(C2<T>));
+  static constexpr int TK = raw >= 16 ? 16 : (raw < 1 ? 1 : raw);
+};
+
+enum ZMode { ZM_FWD = 0, ZM_INV = 1, ZM_POISSON = 2 };
+
+// Column FFT along y (AXIS=1) or z (AXIS=2). grid = (ceil(ncols/TK), other axis).
+// MODE: forward, inverse, or (z only) forward + grid-consistent division + inverse.
+template <typename T, int N, int AXIS, int MODE>
+__global__ __launch_bounds__(NT) void k_fft_col(C2<T>* __restrict__ S, const C2<T>* __restrict__ tw, int ncols,
+                                                int Ny, int Nz, double norm, const double* __restrict__ cx,
+                                                const double* __restrict__ cy, const double* __restrict__ cz)
+{
+  constexpr int TK = ColTile<T, N>::TK;
+  __shared__ C2<T> buf[N * TK];
+  const int kx0 = blockIdx.x * TK, other = blockIdx.y;
+  const int nc = min(TK, ncols - kx0);
+  // element (f, n): AXIS=1: S[(other*Ny + n)*ncols + kx0 + f]; AXIS=2: S[(n*Ny + other)*ncols + kx0 + f]
+  for (int t = threadIdx.x; t < N * TK; t += NT) {
+    const int f = t % TK, n = t / TK;
+    if (f < nc) {
+      const size_t gi = AXIS == 1 ? ((size_t)other * Ny + n) * ncols + kx0 + f : ((size_t)n * Ny + other) * ncols + kx0 + f;
+      buf[f + n * TK] = S[gi];
+    } else {
+      buf[f + n * TK] = C2<T>{0, 0};
+    }
+  }
+  __syncthreads();
+  if (MODE == ZM_INV) {
+    lds_fft<N, TK, +1>(buf, tw);
+  } else {
+    lds_fft<N, TK, -1>(buf, tw);
+  }
+  if (MODE == ZM_POISSON) {
+    // PoissonSolverPeriodic::_solve grid-consistent branch (PoissonSolverPeriodic.cpp:23-61):
+    // multiply by norm/(2D-6), D = cos(4 pi kx/Nx) + cos(4 pi ky/Ny) + cos(4 pi kz/Nz),
+    // in FP64; zero the modes with every index in {0, N/2} (kx slot 0 carries kx=0 and Nx/2).
+    const int ky = other;
+    for (int t = threadIdx.x; t < N * TK; t += NT) {
+      const int f = t % TK, kz = t / TK;
+      const int kx = kx0 + f;
+      if (f < nc) {
+        const double D = cx[kx] + cy[ky] + cz[kz];
+        double fac = norm / (2 * D - 6);
+        const bool zero = kx == 0 && (ky == 0 || 2 * ky == Ny) && (kz == 0 || 2 * kz == N);
+        if (zero) fac = 0;
+        C2<T> v = buf[f + kz * TK];
+        v.x = (T)((double)v.x * fac);
+        v.y = (T)((double)v.y * fac);
+        if (zero) v = C2<T>{0, 0};
+        buf[f + kz * TK] = v;
+      }
+    }
+    __syncthreads();
+    lds_fft<N, TK, +1>(buf, tw);
+  }
+  for (int t = threadIdx.x; t < N * TK; t += NT) {
+    const int f = t % TK, n = t / TK;
+    if (f < nc) {
+      const size_t gi = AXIS == 1 ? ((size_t)other * Ny + n) * ncols + kx0 + f : ((size_t)n * Ny + other) * ncols + kx0 + f;
+      S[gi] = buf[f + n * TK];
+    }
+  }
+}
+
+// Separate spectral division (the PLAIN decomposition's "FFTW solve" pass).
+template <typename T>
+__global__ void k_divide(C2<T>* __restrict__ S, int ncols, int Ny, int Nz, double norm, const double* __restrict__ cx,
+                         const double* __restrict__ cy, const double* __restrict__ cz)
+{
+  const size_t n = (size_t)ncols * Ny * Nz;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const int kx = (int)(i % ncols);
+    const int ky = (int)((i / ncols) % Ny);
+    const int kz = (int)(i / ((size_t)ncols * Ny));
+    const double D = cx[kx] + cy[ky] + cz[kz];
+    const double fac = norm / (2 * D - 6);
+    const bool zero = kx == 0 && (ky == 0 || 2 * ky == Ny) && (kz == 0 || 2 * kz == Nz);
+    C2<T> v = S[i];
+    v.x = (T)((double)v.x * fac);
+    v.y = (T)((double)v.y * fac);
+    if (zero) v = C2<T>{0, 0};
+    S[i] = v;
+  }
+}
+
+// ------------------------------------------------------------------ dispatch
+
+template <typename T, int N>
+struct XPass {
+  static void fwd(Ctx& c, const T* s0, const T* s1, const T* s2, int src)
+  {
+    dim3 grid(c.N[1] / BS, c.N[2]);
+    const int* slot = c.body.present ? c.body.ob_slot : nullptr;
+    if (src == XSRC_FIELD)
+      hipLaunchKernelGGL((k_fft_x_fwd<T, N, XSRC_FIELD>), grid, dim3(NT), 0, c.stream, s0, s1, s2, (C2<T>*)c.spec,
+                         (const C2<T>*)c.tw[0], c.grid(), c.sc, (const int*)nullptr, (const T*)nullptr);
+    else
+      hipLaunchKernelGGL((k_fft_x_fwd<T, N, XSRC_DIV>), grid, dim3(NT), 0, c.stream, s0, s1, s2, (C2<T>*)c.spec,
+                         (const C2<T>*)c.tw[0], c.grid(), c.sc, slot, (const T*)c.body.ob_rhs);
+  }
+  static void fwd_to(Ctx& c, const T* s0, C2<T>* spec)
+  {
+    dim3 grid(c.N[1] / BS, c.N[2]);
+    hipLaunchKernelGGL((k_fft_x_fwd<T, N, XSRC_FIELD>), grid, dim3(NT), 0, c.stream, s0, s0, s0, spec,
+                       (const C2<T>*)c.tw[0], c.grid(), c.sc, (const int*)nullptr, (const T*)nullptr);
+  }
+  static void inv(Ctx& c, const C2<T>* spec, T* dst)
+  {
+    dim3 grid(c.N[1] / BS, c.N[2]);
+    hipLaunchKernelGGL((k_fft_x_inv<T, N>), grid, dim3(NT), 0, c.stream, spec, dst, (const C2<T>*)c.tw[0], c.grid());
+  }
+};
+
+template <typename T, int N, int AXIS>
+struct ColPass {
+  static void run(Ctx& c, C2<T>* spec, int mode, double norm)
+  {
+    constexpr int TK = ColTile<T, N>::TK;
+    const int ncols = c.N[0] / 2;
+    const int other = AXIS == 1 ? c.N[2] : c.N[1];
+    dim3 grid((ncols + TK - 1) / TK, other);
+    const C2<T>* tw = (const C2<T>*)c.tw[AXIS];
+    if (mode == ZM_FWD)
+      hipLaunchKernelGGL((k_fft_col<T, N, AXIS, ZM_FWD>), grid, dim3(NT), 0, c.stream, spec, tw, ncols, c.N[1], c.N[2],
+                         norm, c.cos4[0], c.cos4[1], c.cos4[2]);
+    else if (mode == ZM_INV)
+      hipLaunchKernelGGL((k_fft_col<T, N, AXIS, ZM_INV>), grid, dim3(NT), 0, c.stream, spec, tw, ncols, c.N[1], c.N[2],
+                         norm, c.cos4[0], c.cos4[1], c.cos4[2]);
+    else if (AXIS == 2)
+      hipLaunchKernelGGL((k_fft_col<T, N, 2, ZM_POISSON>), grid, dim3(NT), 0, c.stream, spec, tw, ncols, c.N[1],
+                         c.N[2], norm, c.cos4[0], c.cos4[1], c.cos4[2]);
+  }
+};
+
+// Runtime N -> template N.
+template <template <typename, int> class F, typename T, typename... A>
+static bool dispatch_n(int n, A&&... a)
+{
+  switch (n) {
+    case 8: F<T, 8>::go(a...); return true;
+    case 16: F<T, 16>::go(a...); return true;
+    case 32: F<T, 32>::go(a...); return true;
+    case 64: F<T, 64>::go(a...); return true;
+    case 128: F<T, 128>::go(a...); return true;
+    case 256: F<T, 256>::go(a...); return true;
+    case 512: F<T, 512>::go(a...); return true;
+    case 1024: F<T, 1024>::go(a...); return true;
+    default: return false;
+  }
+}
+
+template <typename T, int N>
+struct GoXFwd {
+  static void go(Ctx& c, const T* s0, const T* s1, const T* s2, int src) { XPass<T, N>::fwd(c, s0, s1, s2, src); }
+};
+template <typename T, int N>
+struct GoXFwdTo {
+  static void go(Ctx& c, const T* s0, C2<T>* spec) { XPass<T, N>::fwd_to(c, s0, spec); }
+};
+template <typename T, int N>
+struct GoXInv {
+  static void go(Ctx& c, const C2<T>* spec, T* dst) { XPass<T, N>::inv(c, spec, dst); }
+};
+template <typename T, int N>
+struct GoY {
+  static void go(Ctx& c, C2<T>* spec, int mode, double norm) { ColPass<T, N, 1>::run(c, spec, mode, norm); }
+};
+template <typename T, int N>
+struct GoZ {
+  static void go(Ctx& c, C2<T>* spec, int mode, double norm) { ColPass<T, N, 2>::run(c, spec, mode, norm); }
+};
+
+static double poisson_norm(const Ctx& c)
+{
+  return 4.0 / ((double)c.N[0] * c.h * (double)c.N[1] * (double)c.N[2]);
+}
+
+template <typename T>
+static int poisson_fused_t(Ctx& c)
+{
+  C2<T>* S = (C2<T>*)c.spec;
+  const double norm = poisson_norm(c);
+  dispatch_n<GoXFwd, T>(c.N[0], c, (const T*)c.f[CUP_TMPU], (const T*)c.f[CUP_TMPV], (const T*)c.f[CUP_TMPW],
+                        (int)XSRC_DIV);
+  dispatch_n<GoY, T>(c.N[1], c, S, (int)ZM_FWD, norm);
+  dispatch_n<GoZ, T>(c.N[2], c, S, (int)ZM_POISSON, norm);
+  dispatch_n<GoY, T>(c.N[1], c, S, (int)ZM_INV, norm);
+  dispatch_n<GoXInv, T>(c.N[0], c, (const C2<T>*)S, (T*)c.f[CUP_P]);
+  return check_hip(c, hipGetLastError(), "poisson_fused");
+}
+
+template <typename T>
+static int poisson_plain_t(Ctx& c)
+{
+  C2<T>* S = (C2<T>*)c.spec;
+  const double norm = poisson_norm(c);
+  dispatch_n<GoXFwdTo, T>(c.N[0], c, (const T*)c.rbuf, S);
+  dispatch_n<GoY, T>(c.N[1], c, S, (int)ZM_FWD, norm);
+  dispatch_n<GoZ, T>(c.N[2], c, S, (int)ZM_FWD, norm);
+  const size_t n = (size_t)(c.N[0] / 2) * c.N[1] * c.N[2];
+  size_t gsz = (n + NT - 1) / NT;
+  if (gsz > 65536) gsz = 65536;
+  hipLaunchKernelGGL((k_divide<T>), dim3((unsigned)gsz), dim3(NT), 0, c.stream, S, c.N[0] / 2, c.N[1], c.N[2], norm,
+                     (const double*)c.cos4[0], (const double*)c.cos4[1], (const double*)c.cos4[2]);
+  dispatch_n<GoZ, T>(c.N[2], c, S, (int)ZM_INV, norm);
+  dispatch_n<GoY, T>(c.N[1], c, S, (int)ZM_INV, norm);
+  dispatch_n<GoXInv, T>(c.N[0], c, (const C2<T>*)S, (T*)c.rbuf);
+  return check_hip(c, hipGetLastError(), "poisson_plain");
+}
+
+int launch_poisson_fused(Ctx& c)
+{
+  return c.precision == 4 ? poisson_fused_t<float>(c) : poisson_fused_t<double>(c);
+}
+
+int launch_poisson_plain(Ctx& c)
+{
+  return c.precision == 4 ? poisson_plain_t<float>(c) : poisson_plain_t<double>(c);
+}
+
+template <typename T>
+static int fwd_field_t(Ctx& c, const void* real_blocks, void* spec)
+{
+  C2<T>* S = (C2<T>*)spec;
+  dispatch_n<GoXFwdTo, T>(c.N[0], c, (const T*)real_blocks, S);
+  dispatch_n<GoY, T>(c.N[1], c, S, (int)ZM_FWD, 0.0);
+  dispatch_n<GoZ, T>(c.N[2], c, S, (int)ZM_FWD, 0.0);
+  return check_hip(c, hipGetLastError(), "fft_forward_field");
+}
+
+template <typename T>
+static int inv_field_t(Ctx& c, void* spec, void* real_blocks)
+{
+  C2<T>* S = (C2<T>*)spec;
+  dispatch_n<GoZ, T>(c.N[2], c, S, (int)ZM_INV, 0.0);
+  dispatch_n<GoY, T>(c.N[1], c, S, (int)ZM_INV, 0.0);
+  dispatch_n<GoXInv, T>(c.N[0], c, (const C2<T>*)S, (T*)real_blocks);
+  return check_hip(c, hipGetLastError(), "fft_inverse_field");
+}
+
+int launch_fft_forward_field(Ctx& c, const void* real_blocks, void* spec)
+{
+  return c.precision == 4 ? fwd_field_t<float>(c, real_blocks, spec) : fwd_field_t<double>(c, real_blocks, spec);
+}
+
+int launch_fft_inverse_field(Ctx& c, void* spec, void* real_blocks)
+{
+  return c.precision == 4 ? inv_field_t<float>(c, spec, real_blocks) : inv_field_t<double>(c, spec, real_blocks);
+}
+
+// Twiddle tables e^{-2 pi i t/N} (t < N) per axis and the Poisson cosine tables, built in
+// FP64 on the host.
+int fft_setup(Ctx& c)
+{
+  for (int a = 0; a < 3; ++a) {
+    const int n = c.N[a];
+    if (n < 8 || n > 1024 || (n & (n - 1)) != 0)
+      return set_error(c, CUP_E_INVALID, "FFT sizes must be powers of two between 8 and 1024 cells per axis");
+    std::vector<double> tw(2 * (size_t)n), cs((size_t)n);
+    for (int t = 0; t < n; ++t) {
+      const double ang = 2.0 * M_PI * (double)t / (double)n;
+      tw[2 * t] = cos(ang);
+      tw[2 * t + 1] = -sin(ang);
+      cs[t] = cos(4.0 * M_PI * (double)t / (double)n);
+    }
+    if (check_hip(c, hipMalloc(&c.tw[a], (size_t)n * 2 * c.esz), "hipMalloc twiddles")) return CUP_E_DEVICE;
+    if (c.precision == 4) {
+      std::vector<float> twf(2 * (size_t)n);
+      for (size_t i = 0; i < twf.size(); ++i) twf[i] = (float)tw[i];
+      hipMemcpy(c.tw[a], twf.data(), twf.size() * sizeof(float), hipMemcpyHostToDevice);
+    } else {
+      hipMemcpy(c.tw[a], tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice);
+    }
+    if (check_hip(c, hipMalloc(&c.cos4[a], (size_t)n * sizeof(double)), "hipMalloc cos4")) return CUP_E_DEVICE;
+    hipMemcpy(c.cos4[a], cs.data(), cs.size() * sizeof(double), hipMemcpyHostToDevice);
+  }
+  return CUP_OK;
+}
+
+}  // namespace cup

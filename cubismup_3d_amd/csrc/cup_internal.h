@@ -1,0 +1,177 @@
+// cup_internal.h -- device context, shared device helpers and the launch functions each
+// .hip file provides. Internal to libcup.so; the public surface is include/cup.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/cup.h"
+
+namespace cup {
+
+constexpr int BS = 8;     // cells per block edge (the configs' CUP_BLOCK_SIZE)
+constexpr int BS3 = 512;  // cells per block
+constexpr int NT = 256;   // threads per workgroup of the block-store kernels
+
+template <typename T>
+struct C2 {
+  T x, y;
+};
+
+// Step scalars resident on the device. The fused driver never copies them to the host
+// inside a step; the host reads them on request.
+struct DevScalars {
+  double dt;         // dt of the step being taken
+  double last_dt;    // dt of the last completed step
+  double umax;       // max_i |u_i + uinf_i| measured on the current velocity
+  double time;
+  double ke;         // sum over cells of (u^2+v^2+w^2)/2 * h^3 on the current velocity
+  long long step;
+  double forcing_fac;
+  double tke, tke_low, eps, lint;
+  double red[32];    // generic finalized reduction results
+};
+
+// Geometry passed by value to kernels.
+struct GridArgs {
+  int nbx, nby, nbz;  // blocks per dimension
+  int Nx, Ny, Nz;     // cells per dimension
+  int nb;             // blocks
+  double h;
+};
+
+// A sphere turned into chi and surface data (static body: built once).
+struct Body {
+  bool present = false;
+  double radius = 0, center[3] = {0, 0, 0};
+  double com[3] = {0, 0, 0}, mass = 0;
+  int nob = 0;                  // obstacle blocks
+  int* ob_ids = nullptr;        // device: block id of each obstacle block
+  int* ob_slot = nullptr;       // device: per block, its obstacle slot or -1
+  double* ob_chi = nullptr;     // device: nob*512 obstacle chi (FP64)
+  double* ob_sdf = nullptr;     // device: nob*512 SDF
+  double* ob_surf = nullptr;    // device: nob*512*4 (delta, dchidx, dchidy, dchidz)
+  void* ob_rhs = nullptr;       // device: nob*512 corrected RHS (context precision)
+  unsigned char* ob_touch = nullptr;  // device: nob*512 surface flag of the RHS correction
+};
+
+struct Ctx {
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  int precision = 8;  // 4 or 8
+  int decomposition = CUP_FUSED;
+  int bpd[3] = {0, 0, 0}, N[3] = {0, 0, 0}, nb = 0;
+  size_t ncell = 0;
+  double h = 0, maxextent = 0, extent[3] = {0, 0, 0};
+  size_t esz = 8;
+
+  void* f[8] = {};        // block-layout fields chi,u,v,w,p,tmpU,tmpV,tmpW
+  void* spec = nullptr;   // packed half spectrum, complex, [z][y][kx], kx < Nx/2
+  void* spec2 = nullptr;  // extra spectra for the plain forcing path
+  void* spec3 = nullptr;
+  void* rbuf = nullptr;   // block-layout real scratch (plain RHS / solution)
+  void* tw[3] = {};       // FFT twiddles per axis, e^{-2 pi i t / N}, context precision
+  double* cos4[3] = {};   // cos(4 pi k / N) per axis (FP64), the Poisson symbol
+  double* partials = nullptr;
+  size_t partials_cap = 0;  // doubles
+  DevScalars* sc = nullptr;
+  DevScalars* sc_host = nullptr;  // pinned
+  double* host_red = nullptr;     // pinned scratch for reductions
+  // forcing: canonical low modes (integer wavenumbers), count and device tables
+  int nlow = 0;
+  int* low_k = nullptr;        // device, nlow*3 ints
+  double* low_coef = nullptr;  // device, nlow*3 complex: unnormalized DFT of u*,v*,w*
+  void* phase[3] = {};         // device, per axis N complex e^{i 2 pi (i+1/2)/N}
+  Body body;
+  std::string err;
+  GridArgs grid() const {
+    GridArgs g;
+    g.nbx = bpd[0]; g.nby = bpd[1]; g.nbz = bpd[2];
+    g.Nx = N[0]; g.Ny = N[1]; g.Nz = N[2];
+    g.nb = nb; g.h = h;
+    return g;
+  }
+};
+
+// ----------------------------------------------------------------- device helpers
+
+// Workgroup i is dispatched to XCD i % 8. Remap so that each XCD walks a contiguous
+// range of logical ids (neighbouring blocks share that XCD's L2).
+__device__ __forceinline__ int xcd_remap(int i, int n)
+{
+  if ((n & 7) != 0) return i;
+  return (i & 7) * (n >> 3) + (i >> 3);
+}
+
+__device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+__device__ __forceinline__ size_t cell_index(const GridArgs& g, int gx, int gy, int gz)
+{
+  const int b = (gx >> 3) + g.nbx * ((gy >> 3) + g.nby * (gz >> 3));
+  return (size_t)b * BS3 + (((gz & 7) * BS + (gy & 7)) * BS + (gx & 7));
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v)
+{
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_max(T v)
+{
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// ----------------------------------------------------------------- launchers
+// cup_stencil.hip
+int launch_lab_shift(Ctx& c, int g, int dx, int dy, int dz);
+int launch_max_u(Ctx& c, const double uinf[3]);                  // partials[nb]
+int launch_advdiff(Ctx& c, double nu, const double uinf[3], int scheme);  // -> tmp
+int launch_copy_tmp_to_u(Ctx& c);
+int launch_rhs_plain(Ctx& c);                                   // -> rbuf
+int launch_gradp_plain(Ctx& c);                                 // u += -dt/2h grad p
+int launch_gradp_fused(Ctx& c, const double uinf[3]);           // u = tmp - dt/2h grad p; partials[nb][2]
+int launch_diagnostics(Ctx& c, double nu);                      // partials[nb][20]
+int launch_ic_taylor_green(Ctx& c, double uMax);
+int launch_fill(Ctx& c, void* ptr, size_t n, double value);
+int launch_copy(Ctx& c, void* dst, const void* src, size_t n);
+int launch_set_dt(Ctx& c, double dt);
+int launch_reduce(Ctx& c, int n, int width, unsigned maxmask, double* out_dev);
+int launch_finalize_step(Ctx& c, int n, double CFL, double nu, int rampup, double fixed_dt, int advance);
+// cup_fft.hip
+int fft_setup(Ctx& c);
+int launch_poisson_fused(Ctx& c);   // rhs from div(tmp) -> p (block layout)
+int launch_poisson_plain(Ctx& c);   // rhs in rbuf -> solution in rbuf (separate divide pass)
+int launch_fft_forward_field(Ctx& c, const void* real_blocks, void* spec);  // 3 passes
+int launch_fft_inverse_field(Ctx& c, void* spec, void* real_blocks);        // 3 passes
+// cup_spectral.hip
+int spectral_setup(Ctx& c);
+int launch_ic_hit(Ctx& c, double k0, double u_rms, uint64_t seed);
+int forcing_plain(Ctx& c, double dt, double sim_dt, double nu, double tke_target, double inj,
+                  double stats[5]);
+// cup_obstacle.hip
+int body_create(Ctx& c, const double center[3], double radius);
+void body_free(Ctx& c);
+int launch_penalize(Ctx& c, double lambda, int implicit, const double vel[3], const double omega[3]);
+int launch_penalize_step(Ctx& c, double lambda, int implicit, const double vel[3], const double omega[3],
+                         const double uinf[3]);  // also refreshes the fused step's max/energy partials
+int launch_surface_forces(Ctx& c, double nu, const double vel[3], const double omega[3]);
+int launch_rhs_obstacle(Ctx& c, void* rhs_blocks);
+
+// helpers (cup_capi.cpp)
+int ensure_partials(Ctx& c, size_t ndoubles);
+int set_error(Ctx& c, int code, const std::string& msg);
+int check_hip(Ctx& c, hipError_t e, const char* what);
+
+}  // namespace cup
+
+struct cup_ctx {
+  cup::Ctx c;
+};

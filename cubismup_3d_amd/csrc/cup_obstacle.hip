@@ -1,0 +1,516 @@
+// cup_obstacle.hip -- a sphere on the block grid: SDF and chi (Towers' smoothed
+// Heaviside), surface points, the pressure-RHS masking with its perimeter correction,
+// Brinkman penalization and the surface force integrals. All body arithmetic is FP64;
+// only the velocity it updates and the grid chi are stored in the context precision.
+// The body touches few blocks (a few hundred of tens of thousands), so every kernel here
+// runs one workgroup per obstacle block.
+#include "cup_internal.h"
+
+#include <cfloat>
+#include <cmath>
+#include <vector>
+
+namespace cup {
+
+static constexpr double DEPS = DBL_EPSILON;
+static constexpr int SURFDH = 1;  // ObstacleBlock.h:16
+
+__device__ __forceinline__ void bcoords(const GridArgs& g, int b, int& bx, int& by, int& bz)
+{
+  bx = b % g.nbx;
+  by = (b / g.nbx) % g.nby;
+  bz = b / (g.nbx * g.nby);
+}
+
+// FP64 block-wide reduction, thread 0 writes out[0..NV).
+template <int NV>
+__device__ __forceinline__ void breduce(double (&v)[NV], double* out)
+{
+  __shared__ double red[NT / 64][NV];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) v[i] = wave_sum(v[i]);
+  if (lane == 0)
+#pragma unroll
+    for (int i = 0; i < NV; ++i) red[wv][i] = v[i];
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      double a = red[0][i];
+      for (int w = 1; w < NT / 64; ++w) a += red[w][i];
+      out[i] = a;
+    }
+  __syncthreads();
+}
+
+// SDF of the sphere on obstacle blocks: R - |x - c| (Sphere.cpp:44-48).
+__global__ void k_sphere_sdf(const int* __restrict__ ob_ids, double* __restrict__ sdf, GridArgs g, double cx,
+                             double cy, double cz, double R)
+{
+  const int s = blockIdx.x, b = ob_ids[s];
+  int bx, by, bz;
+  bcoords(g, b, bx, by, bz);
+  for (int c = threadIdx.x; c < BS3; c += NT) {
+    const int ix = c & 7, iy = (c >> 3) & 7, iz = c >> 6;
+    const double dx = (bx * BS + ix + 0.5) * g.h - cx, dy = (by * BS + iy + 0.5) * g.h - cy,
+                 dz = (bz * BS + iz + 0.5) * g.h - cz;
+    sdf[(size_t)s * BS3 + c] = R - sqrt(dx * dx + dy * dy + dz * dz);
+  }
+}
+
+// SDF as the reference's grid tmpU after CreateObstacles' reset (-1) and the block fill
+// (max with the body's SDF): the body's SDF on obstacle blocks, -1 elsewhere.
+__device__ __forceinline__ double sdf_at(const GridArgs& g, const int* __restrict__ slot, const double* __restrict__ sdf,
+                                         int gx, int gy, int gz)
+{
+  gx = wrapi(gx, g.Nx); gy = wrapi(gy, g.Ny); gz = wrapi(gz, g.Nz);
+  const int b = (gx >> 3) + g.nbx * ((gy >> 3) + g.nby * (gz >> 3));
+  const int s = slot[b];
+  if (s < 0) return -1.0;
+  return sdf[(size_t)s * BS3 + (((gz & 7) * BS + (gy & 7)) * BS + (gx & 7))];
+}
+
+// KernelCharacteristicFunction (ObstaclesCreate.cpp:51-137) on one obstacle block:
+// chi by Towers' formula inside the +-2h band, grid chi = max, CoM and mass partials,
+// surface points (delta, dchi) where Delta > eps (ObstacleBlock::write, ObstacleBlock.h:157-166).
+template <typename T>
+__global__ __launch_bounds__(NT) void k_sphere_chi(const int* __restrict__ ob_ids, const int* __restrict__ slot,
+                                                   const double* __restrict__ sdf, double* __restrict__ ob_chi,
+                                                   double* __restrict__ surf, T* __restrict__ chi, GridArgs g,
+                                                   double* __restrict__ part)
+{
+  const int s = blockIdx.x, b = ob_ids[s];
+  int bx, by, bz;
+  bcoords(g, b, bx, by, bz);
+  const double h = g.h, inv2h = .5 / h, fac1 = .5 * h * h, vol = h * h * h;
+  double acc[4] = {0, 0, 0, 0};
+  for (int c = threadIdx.x; c < BS3; c += NT) {
+    const int ix = c & 7, iy = (c >> 3) & 7, iz = c >> 6;
+    const int gx = bx * BS + ix, gy = by * BS + iy, gz = bz * BS + iz;
+    const double sdf0 = sdf[(size_t)s * BS3 + c];
+    const double dPx = sdf_at(g, slot, sdf, gx + 1, gy, gz), dMx = sdf_at(g, slot, sdf, gx - 1, gy, gz);
+    const double dPy = sdf_at(g, slot, sdf, gx, gy + 1, gz), dMy = sdf_at(g, slot, sdf, gx, gy - 1, gz);
+    const double dPz = sdf_at(g, slot, sdf, gx, gy, gz + 1), dMz = sdf_at(g, slot, sdf, gx, gy, gz - 1);
+    double X;
+    if (sdf0 > +2 * h || sdf0 < -2 * h) {
+      X = sdf0 > 0 ? 1 : 0;
+    } else {
+      const double gUX = inv2h * (dPx - dMx), gUY = inv2h * (dPy - dMy), gUZ = inv2h * (dPz - dMz);
+      const double gUSq = gUX * gUX + gUY * gUY + gUZ * gUZ + DEPS;
+      const double gIX = inv2h * (fmax(dPx, 0.0) - fmax(dMx, 0.0));
+      const double gIY = inv2h * (fmax(dPy, 0.0) - fmax(dMy, 0.0));
+      const double gIZ = inv2h * (fmax(dPz, 0.0) - fmax(dMz, 0.0));
+      X = (gIX * gUX + gIY * gUY + gIZ * gUZ) / gUSq;
+    }
+    ob_chi[(size_t)s * BS3 + c] = X;
+    const size_t o = (size_t)b * BS3 + c;
+    chi[o] = (T)fmax((double)chi[o], X);
+    const double px = (gx + 0.5) * h, py = (gy + 0.5) * h, pz = (gz + 0.5) * h;
+    acc[0] += X * vol * px;
+    acc[1] += X * vol * py;
+    acc[2] += X * vol * pz;
+    acc[3] += X * vol;
+    double* sd = surf + ((size_t)s * BS3 + c) * 4;
+    sd[0] = sd[1] = sd[2] = sd[3] = 0;
+    const double sdfs = sdf0 + h * SURFDH;
+    if (sdfs > +2 * h || sdfs < -2 * h) continue;
+    const double aPx = dPx + h * SURFDH, aMx = dMx + h * SURFDH, aPy = dPy + h * SURFDH, aMy = dMy + h * SURFDH,
+                 aPz = dPz + h * SURFDH, aMz = dMz + h * SURFDH;
+    const double gUX = inv2h * (aPx - aMx), gUY = inv2h * (aPy - aMy), gUZ = inv2h * (aPz - aMz);
+    const double gUSq = gUX * gUX + gUY * gUY + gUZ * gUZ + DEPS;
+    auto H = [](double d) { return fabs(d) < DEPS ? 0.5 : (d < 0 ? 0.0 : 1.0); };
+    const double gHX = H(aPx) - H(aMx), gHY = H(aPy) - H(aMy), gHZ = H(aPz) - H(aMz);
+    const double Delta = fac1 * (gHX * gUX + gHY * gUY + gHZ * gUZ) / gUSq;
+    if (Delta > DEPS) {
+      sd[0] = Delta;
+      sd[1] = -Delta * gUX;
+      sd[2] = -Delta * gUY;
+      sd[3] = -Delta * gUZ;
+    }
+  }
+  breduce<4>(acc, part + 4 * (size_t)s);
+}
+
+template <typename T>
+__global__ void k_fill_t(T* p, size_t n, T v)
+{
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+void body_free(Ctx& c)
+{
+  Body& B = c.body;
+  if (B.ob_ids) hipFree(B.ob_ids);
+  if (B.ob_slot) hipFree(B.ob_slot);
+  if (B.ob_chi) hipFree(B.ob_chi);
+  if (B.ob_sdf) hipFree(B.ob_sdf);
+  if (B.ob_surf) hipFree(B.ob_surf);
+  if (B.ob_rhs) hipFree(B.ob_rhs);
+  if (B.ob_touch) hipFree(B.ob_touch);
+  B = Body();
+}
+
+template <typename T>
+static int body_chi_t(Ctx& c)
+{
+  Body& B = c.body;
+  const size_t n = c.ncell;
+  size_t gsz = (n + NT - 1) / NT;
+  if (gsz > 65536) gsz = 65536;
+  hipLaunchKernelGGL((k_fill_t<T>), dim3((unsigned)gsz), dim3(NT), 0, c.stream, (T*)c.f[CUP_CHI], n, (T)0);
+  hipLaunchKernelGGL(k_sphere_sdf, dim3(B.nob), dim3(NT), 0, c.stream, B.ob_ids, B.ob_sdf, c.grid(), B.center[0],
+                     B.center[1], B.center[2], B.radius);
+  hipLaunchKernelGGL((k_sphere_chi<T>), dim3(B.nob), dim3(NT), 0, c.stream, B.ob_ids, B.ob_slot, B.ob_sdf, B.ob_chi,
+                     B.ob_surf, (T*)c.f[CUP_CHI], c.grid(), c.partials);
+  return check_hip(c, hipGetLastError(), "k_sphere_chi");
+}
+
+// CreateObstacles for one static sphere (ObstaclesCreate.cpp:444-540).
+int body_create(Ctx& c, const double center[3], double radius)
+{
+  hipStreamSynchronize(c.stream);
+  body_free(c);
+  Body& B = c.body;
+  B.radius = radius;
+  for (int a = 0; a < 3; ++a) B.center[a] = center[a];
+  // Blocks whose cell-centre box strictly intersects the sphere's box of half-width
+  // R + (2+SURFDH) h (Sphere.cpp:21-42, Obstacle.h:186-207).
+  const double safety = (2 + SURFDH) * c.h;
+  std::vector<int> ids, slot(c.nb, -1);
+  for (int b = 0; b < c.nb; ++b) {
+    const int bc[3] = {b % c.bpd[0], (b / c.bpd[0]) % c.bpd[1], b / (c.bpd[0] * c.bpd[1])};
+    bool touching = true;
+    for (int a = 0; a < 3; ++a) {
+      const double mn = (bc[a] * BS + 0.5) * c.h, mx = (bc[a] * BS + BS - 1 + 0.5) * c.h;
+      const double lo = std::fmax(mn, center[a] - (radius + safety)), hi = std::fmin(mx, center[a] + (radius + safety));
+      if (!(hi - lo > 0)) touching = false;
+    }
+    if (touching) {
+      slot[b] = (int)ids.size();
+      ids.push_back(b);
+    }
+  }
+  B.nob = (int)ids.size();
+  if (B.nob == 0) return set_error(c, CUP_E_INVALID, "chi_from_sphere: the sphere touches no block");
+  const size_t nc = (size_t)B.nob * BS3;
+  if (check_hip(c, hipMalloc(&B.ob_ids, B.nob * sizeof(int)), "hipMalloc") ||
+      check_hip(c, hipMalloc(&B.ob_slot, c.nb * sizeof(int)), "hipMalloc") ||
+      check_hip(c, hipMalloc(&B.ob_chi, nc * sizeof(double)), "hipMalloc") ||
+      check_hip(c, hipMalloc(&B.ob_sdf, nc * sizeof(double)), "hipMalloc") ||
+      check_hip(c, hipMalloc(&B.ob_surf, nc * 4 * sizeof(double)), "hipMalloc") ||
+      check_hip(c, hipMalloc(&B.ob_rhs, nc * c.esz), "hipMalloc") ||
+      check_hip(c, hipMalloc(&B.ob_touch, nc), "hipMalloc")) {
+    body_free(c);
+    return CUP_E_NOMEM;
+  }
+  hipMemcpy(B.ob_ids, ids.data(), ids.size() * sizeof(int), hipMemcpyHostToDevice);
+  hipMemcpy(B.ob_slot, slot.data(), slot.size() * sizeof(int), hipMemcpyHostToDevice);
+  B.present = true;
+  if (int rc = ensure_partials(c, 32 * (size_t)B.nob + 2 * (size_t)c.nb)) return rc;
+  int rc = c.precision == 4 ? body_chi_t<float>(c) : body_chi_t<double>(c);
+  if (rc) return rc;
+  std::vector<double> part(4 * (size_t)B.nob);
+  hipMemcpy(part.data(), c.partials, part.size() * sizeof(double), hipMemcpyDeviceToHost);
+  double com[4] = {0, 0, 0, 0};
+  for (int s = 0; s < B.nob; ++s)
+    for (int i = 0; i < 4; ++i) com[i] += part[4 * (size_t)s + i];
+  B.mass = com[3];
+  for (int a = 0; a < 3; ++a) B.com[a] = com[a] / com[3];
+  return check_hip(c, hipStreamSynchronize(c.stream), "chi_from_sphere");
+}
+
+// ------------------------------------------------------- pressure RHS with a body
+
+// KernelPressureRHS's visitor (PressureRHS.cpp:78-112) on one obstacle block: where the
+// grid chi does not exceed the body's chi, rhs *= (1 - chi) with the bulk source summed,
+// and the positive/negative RHS of surface cells (nonzero central grad chi) summed.
+// FUSED (rhs == nullptr): the unmasked RHS of the block is first formed from tmp
+// (h^2/(2dt) times the central divergence) into ob_rhs; PLAIN: rhs is the block buffer.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_rhs_obs_mask(const int* __restrict__ ob_ids, const double* __restrict__ ob_chi,
+                                                     const T* __restrict__ chi, const T* __restrict__ u,
+                                                     const T* __restrict__ v, const T* __restrict__ w,
+                                                     T* __restrict__ rhs, T* __restrict__ ob_rhs,
+                                                     unsigned char* __restrict__ touch, GridArgs g,
+                                                     const DevScalars* __restrict__ sc, double* __restrict__ part)
+{
+  const int s = blockIdx.x, b = ob_ids[s];
+  int bx, by, bz;
+  bcoords(g, b, bx, by, bz);
+  const T fac = (T)(0.5 * g.h * g.h / sc->dt);
+  double acc[3] = {0, 0, 0};
+  for (int c = threadIdx.x; c < BS3; c += NT) {
+    const int ix = c & 7, iy = (c >> 3) & 7, iz = c >> 6;
+    const int gx = bx * BS + ix, gy = by * BS + iy, gz = bz * BS + iz;
+    const size_t o = (size_t)b * BS3 + c;
+    T r;
+    if (rhs) {
+      r = rhs[o];
+    } else {
+      r = fac * (u[cell_index(g, wrapi(gx + 1, g.Nx), gy, gz)] - u[cell_index(g, wrapi(gx - 1, g.Nx), gy, gz)] +
+                 v[cell_index(g, gx, wrapi(gy + 1, g.Ny), gz)] - v[cell_index(g, gx, wrapi(gy - 1, g.Ny), gz)] +
+                 w[cell_index(g, gx, gy, wrapi(gz + 1, g.Nz))] - w[cell_index(g, gx, gy, wrapi(gz - 1, g.Nz))]);
+    }
+    const double X = ob_chi[(size_t)s * BS3 + c];
+    unsigned char t = 0;
+    if (!((double)chi[o] > X)) {
+      const double dXx = (double)chi[cell_index(g, wrapi(gx + 1, g.Nx), gy, gz)] - chi[cell_index(g, wrapi(gx - 1, g.Nx), gy, gz)];
+      const double dXy = (double)chi[cell_index(g, gx, wrapi(gy + 1, g.Ny), gz)] - chi[cell_index(g, gx, wrapi(gy - 1, g.Ny), gz)];
+      const double dXz = (double)chi[cell_index(g, gx, gy, wrapi(gz + 1, g.Nz))] - chi[cell_index(g, gx, gy, wrapi(gz - 1, g.Nz))];
+      const bool surf = dXx * dXx + dXy * dXy + dXz * dXz > 0;
+      t = surf ? 1 : 0;
+      const double srcBulk = -X * (double)r;
+      acc[0] += srcBulk;
+      const T nr = (T)((double)r + srcBulk);
+      r = nr;
+      if (surf) {
+        if (nr > 0) acc[1] += (double)nr;
+        if (nr < 0) acc[2] -= (double)nr;
+      }
+    }
+    touch[(size_t)s * BS3 + c] = t;
+    if (rhs) rhs[o] = r;
+    else ob_rhs[(size_t)s * BS3 + c] = r;
+  }
+  breduce<3>(acc, part + 3 * (size_t)s);
+}
+
+// KernelFinalizePerimeters (PressureRHS.cpp:215-256) after the factor of :331-334.
+// Every workgroup reduces the per-block sums itself (same order everywhere).
+template <typename T>
+__global__ __launch_bounds__(NT) void k_rhs_obs_correct(const int* __restrict__ ob_ids, int nob,
+                                                        const unsigned char* __restrict__ touch, T* __restrict__ rhs,
+                                                        T* __restrict__ ob_rhs, const double* __restrict__ part)
+{
+  __shared__ double corr_s;
+  double acc[3] = {0, 0, 0};
+  for (int j = threadIdx.x; j < nob; j += NT) {
+    acc[0] += part[3 * (size_t)j];
+    acc[1] += part[3 * (size_t)j + 1];
+    acc[2] += part[3 * (size_t)j + 2];
+  }
+  __shared__ double tot[3];
+  breduce<3>(acc, tot);
+  if (threadIdx.x == 0) {
+    const double sumRHS = tot[0], posRHS = tot[1], negRHS = tot[2];
+    const double denom = sumRHS > 0 ? posRHS : negRHS;
+    corr_s = sumRHS / fmax(denom, DEPS);
+  }
+  __syncthreads();
+  const double corr = corr_s;
+  const int s = blockIdx.x, b = ob_ids[s];
+  for (int c = threadIdx.x; c < BS3; c += NT) {
+    if (!touch[(size_t)s * BS3 + c]) continue;
+    T* p = rhs ? &rhs[(size_t)b * BS3 + c] : &ob_rhs[(size_t)s * BS3 + c];
+    const double r = (double)*p;
+    if (r > 0 && corr > 0) *p = (T)(r - corr * r);
+    else if (r < 0 && corr < 0) *p = (T)(r + corr * r);
+  }
+}
+
+template <typename T>
+static int rhs_obs_t(Ctx& c, void* rhs_blocks)
+{
+  Body& B = c.body;
+  const int s0 = c.decomposition == CUP_FUSED ? CUP_TMPU : CUP_U;
+  hipLaunchKernelGGL((k_rhs_obs_mask<T>), dim3(B.nob), dim3(NT), 0, c.stream, B.ob_ids, B.ob_chi,
+                     (const T*)c.f[CUP_CHI], (const T*)c.f[s0], (const T*)c.f[s0 + 1], (const T*)c.f[s0 + 2],
+                     (T*)rhs_blocks, (T*)B.ob_rhs, B.ob_touch, c.grid(), c.sc, c.partials + 2 * (size_t)c.nb);
+  hipLaunchKernelGGL((k_rhs_obs_correct<T>), dim3(B.nob), dim3(NT), 0, c.stream, B.ob_ids, B.nob, B.ob_touch,
+                     (T*)rhs_blocks, (T*)B.ob_rhs, (const double*)(c.partials + 2 * (size_t)c.nb));
+  return check_hip(c, hipGetLastError(), "rhs_obstacle");
+}
+
+int launch_rhs_obstacle(Ctx& c, void* rhs_blocks)
+{
+  if (!c.body.present) return CUP_OK;
+  return c.precision == 4 ? rhs_obs_t<float>(c, rhs_blocks) : rhs_obs_t<double>(c, rhs_blocks);
+}
+
+// ------------------------------------------------------------------ penalization
+
+// KernelPenalization<implicit> (Penalization.cpp:40-98), one body, udef = 0. Also
+// refreshes the fused step's per-block max/energy partials of the blocks it changed.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_penalize(const int* __restrict__ ob_ids, const double* __restrict__ ob_chi,
+                                                 const T* __restrict__ chi, T* __restrict__ u, T* __restrict__ v,
+                                                 T* __restrict__ w, GridArgs g, const DevScalars* __restrict__ sc,
+                                                 double lambda, int implicit, double vx, double vy, double vz,
+                                                 double ox, double oy, double oz, double cmx, double cmy, double cmz,
+                                                 double Ux, double Uy, double Uz, double* __restrict__ part,
+                                                 double* __restrict__ steppart)
+{
+  const int s = blockIdx.x, b = ob_ids[s];
+  int bx, by, bz;
+  bcoords(g, b, bx, by, bz);
+  const double dt = sc->dt, dv = g.h * g.h * g.h;
+  const double lambdaFac = implicit ? lambda : 1.0 / dt;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  double mk[2] = {0, 0};
+  for (int c = threadIdx.x; c < BS3; c += NT) {
+    const int ix = c & 7, iy = (c >> 3) & 7, iz = c >> 6;
+    const size_t o = (size_t)b * BS3 + c;
+    const double X = ob_chi[(size_t)s * BS3 + c];
+    double uu = u[o], vv = v[o], ww = w[o];
+    if (!((double)chi[o] > X) && X > 0) {
+      const double px = (bx * BS + ix + 0.5) * g.h - cmx, py = (by * BS + iy + 0.5) * g.h - cmy,
+                   pz = (bz * BS + iz + 0.5) * g.h - cmz;
+      const double UT0 = vx + oy * pz - oz * py, UT1 = vy + oz * px - ox * pz, UT2 = vz + ox * py - oy * px;
+      const double pf = implicit ? X * lambdaFac / (1 + lambdaFac * dt * X) : X * lambdaFac;
+      const double FX = pf * (UT0 - uu), FY = pf * (UT1 - vv), FZ = pf * (UT2 - ww);
+      uu = uu + dt * FX;
+      vv = vv + dt * FY;
+      ww = ww + dt * FZ;
+      u[o] = (T)uu;
+      v[o] = (T)vv;
+      w[o] = (T)ww;
+      uu = u[o]; vv = v[o]; ww = w[o];
+      acc[0] += dv * FX; acc[1] += dv * FY; acc[2] += dv * FZ;
+      acc[3] += dv * (py * FZ - pz * FY);
+      acc[4] += dv * (pz * FX - px * FZ);
+      acc[5] += dv * (px * FY - py * FX);
+    }
+    mk[0] = fmax(mk[0], fmax(fabs(uu + Ux), fmax(fabs(vv + Uy), fabs(ww + Uz))));
+    mk[1] += uu * uu + vv * vv + ww * ww;
+  }
+  breduce<6>(acc, part + 6 * (size_t)s);
+  if (steppart) {
+    // max reduction for mk[0], sum for mk[1]
+    __shared__ double red[NT / 64][2];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    mk[0] = wave_max(mk[0]);
+    mk[1] = wave_sum(mk[1]);
+    if (lane == 0) { red[wv][0] = mk[0]; red[wv][1] = mk[1]; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double a = red[0][0], bsum = red[0][1];
+      for (int q = 1; q < NT / 64; ++q) { a = fmax(a, red[q][0]); bsum += red[q][1]; }
+      steppart[2 * (size_t)b] = a;
+      steppart[2 * (size_t)b + 1] = bsum;
+    }
+  }
+}
+
+__global__ __launch_bounds__(NT) void k_sum_rows(const double* __restrict__ part, int n, int width,
+                                                 double* __restrict__ out)
+{
+  const int i = blockIdx.x;
+  double a[1] = {0};
+  for (int j = threadIdx.x; j < n; j += NT) a[0] += part[(size_t)j * width + i];
+  breduce<1>(a, out + i);
+}
+
+template <typename T>
+static int penalize_t(Ctx& c, double lambda, int implicit, const double vel[3], const double om[3],
+                      const double uinf[3], bool refresh)
+{
+  Body& B = c.body;
+  double* bp = c.partials + 2 * (size_t)c.nb;
+  hipLaunchKernelGGL((k_penalize<T>), dim3(B.nob), dim3(NT), 0, c.stream, B.ob_ids, B.ob_chi, (const T*)c.f[CUP_CHI],
+                     (T*)c.f[CUP_U], (T*)c.f[CUP_V], (T*)c.f[CUP_W], c.grid(), c.sc, lambda, implicit, vel[0], vel[1],
+                     vel[2], om[0], om[1], om[2], B.com[0], B.com[1], B.com[2], uinf[0], uinf[1], uinf[2], bp,
+                     refresh ? c.partials : (double*)nullptr);
+  hipLaunchKernelGGL(k_sum_rows, dim3(6), dim3(NT), 0, c.stream, (const double*)bp, B.nob, 6, c.sc->red);
+  return check_hip(c, hipGetLastError(), "k_penalize");
+}
+
+int launch_penalize(Ctx& c, double lambda, int implicit, const double vel[3], const double omega[3])
+{
+  const double zero[3] = {0, 0, 0};
+  return c.precision == 4 ? penalize_t<float>(c, lambda, implicit, vel, omega, zero, false)
+                          : penalize_t<double>(c, lambda, implicit, vel, omega, zero, false);
+}
+
+int launch_penalize_step(Ctx& c, double lambda, int implicit, const double vel[3], const double omega[3],
+                         const double uinf[3])
+{
+  return c.precision == 4 ? penalize_t<float>(c, lambda, implicit, vel, omega, uinf, true)
+                          : penalize_t<double>(c, lambda, implicit, vel, omega, uinf, true);
+}
+
+// ---------------------------------------------------------------- surface forces
+
+// KernelComputeForces::visit (FluidSolidForces.cpp:42-157) on one obstacle block:
+// 22 QoIs in ObstacleBlock::sumQoI order (ObstacleBlock.h:81-93), udef = 0.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_surface_forces(const int* __restrict__ ob_ids, const double* __restrict__ surf,
+                                                       const T* __restrict__ u, const T* __restrict__ v,
+                                                       const T* __restrict__ w, const T* __restrict__ p, GridArgs g,
+                                                       double nu, double vx, double vy, double vz, double ox,
+                                                       double oy, double oz, double cmx, double cmy, double cmz,
+                                                       double* __restrict__ part)
+{
+  const int s = blockIdx.x, b = ob_ids[s];
+  int bx, by, bz;
+  bcoords(g, b, bx, by, bz);
+  double Q[22];
+#pragma unroll
+  for (int i = 0; i < 22; ++i) Q[i] = 0;
+  const double vn = sqrt(vx * vx + vy * vy + vz * vz);
+  double vu[3] = {0, 0, 0};
+  if (vn > 1e-9) { vu[0] = vx / vn; vu[1] = vy / vn; vu[2] = vz / vn; }
+  const double _1oH = nu / g.h;
+  for (int c = threadIdx.x; c < BS3; c += NT) {
+    const double* sd = surf + ((size_t)s * BS3 + c) * 4;
+    if (!(sd[0] > DEPS)) continue;
+    const int ix = c & 7, iy = (c >> 3) & 7, iz = c >> 6;
+    const int gx = bx * BS + ix, gy = by * BS + iy, gz = bz * BS + iz;
+    const size_t E = cell_index(g, wrapi(gx + 1, g.Nx), gy, gz), W = cell_index(g, wrapi(gx - 1, g.Nx), gy, gz);
+    const size_t N = cell_index(g, gx, wrapi(gy + 1, g.Ny), gz), S = cell_index(g, gx, wrapi(gy - 1, g.Ny), gz);
+    const size_t Bk = cell_index(g, gx, gy, wrapi(gz + 1, g.Nz)), F = cell_index(g, gx, gy, wrapi(gz - 1, g.Nz));
+    const size_t C = (size_t)b * BS3 + c;
+    const double D11 = _1oH * ((double)u[E] - u[W]);
+    const double D22 = _1oH * ((double)v[N] - v[S]);
+    const double D33 = _1oH * ((double)w[Bk] - w[F]);
+    const double D12 = .5 * _1oH * ((double)u[N] - u[S] + v[E] - v[W]);
+    const double D13 = .5 * _1oH * ((double)u[Bk] - u[F] + w[E] - w[W]);
+    const double D23 = .5 * _1oH * ((double)w[N] - w[S] + v[Bk] - v[F]);
+    const double P = p[C];
+    const double nX = sd[1], nY = sd[2], nZ = sd[3];
+    const double fXV = D11 * nX + D12 * nY + D13 * nZ;
+    const double fYV = D12 * nX + D22 * nY + D23 * nZ;
+    const double fZV = D13 * nX + D23 * nY + D33 * nZ;
+    const double fXP = -P * nX, fYP = -P * nY, fZP = -P * nZ;
+    const double fXT = fXV + fXP, fYT = fYV + fYP, fZT = fZV + fZP;
+    const double uX = u[C], uY = v[C], uZ = w[C];
+    const double px = (gx + 0.5) * g.h, py = (gy + 0.5) * g.h, pz = (gz + 0.5) * g.h;
+    Q[0] += fXT; Q[1] += fYT; Q[2] += fZT;
+    Q[3] += fXP; Q[4] += fYP; Q[5] += fZP;
+    Q[6] += fXV; Q[7] += fYV; Q[8] += fZV;
+    Q[9] += (py - cmy) * fZT - (pz - cmz) * fYT;
+    Q[10] += (pz - cmz) * fXT - (px - cmx) * fZT;
+    Q[11] += (px - cmx) * fYT - (py - cmy) * fXT;
+    Q[12] += nY * uZ - nZ * uY;
+    Q[13] += nZ * uX - nX * uZ;
+    Q[14] += nX * uY - nY * uX;
+    const double fpar = fXT * vu[0] + fYT * vu[1] + fZT * vu[2];
+    Q[16] += .5 * (fpar + fabs(fpar));
+    Q[15] -= .5 * (fpar - fabs(fpar));
+    const double powOut = fXT * uX + fYT * uY + fZT * uZ;
+    Q[17] += powOut;
+    Q[18] += fmin(0.0, powOut);
+    const double rx = px - cmx, ry = py - cmy, rz = pz - cmz;
+    Q[21] += fXT * (vx + oy * rz - ry * oz) + fYT * (vy + oz * rx - rz * ox) + fZT * (vz + ox * ry - rx * oy);
+  }
+  breduce<22>(Q, part + 22 * (size_t)s);
+}
+
+template <typename T>
+static int surface_forces_t(Ctx& c, double nu, const double vel[3], const double om[3])
+{
+  Body& B = c.body;
+  double* bp = c.partials + 2 * (size_t)c.nb;
+  hipLaunchKernelGGL((k_surface_forces<T>), dim3(B.nob), dim3(NT), 0, c.stream, B.ob_ids, B.ob_surf,
+                     (const T*)c.f[CUP_U], (const T*)c.f[CUP_V], (const T*)c.f[CUP_W], (const T*)c.f[CUP_P], c.grid(),
+                     nu, vel[0], vel[1], vel[2], om[0], om[1], om[2], B.com[0], B.com[1], B.com[2], bp);
+  hipLaunchKernelGGL(k_sum_rows, dim3(22), dim3(NT), 0, c.stream, (const double*)bp, B.nob, 22, c.sc->red);
+  return check_hip(c, hipGetLastError(), "k_surface_forces");
+}
+
+int launch_surface_forces(Ctx& c, double nu, const double vel[3], const double omega[3])
+{
+  return c.precision == 4 ? surface_forces_t<float>(c, nu, vel, omega) : surface_forces_t<double>(c, nu, vel, omega);
+}
+
+}  // namespace cup

@@ -1,0 +1,342 @@
+// cup_spectral.hip -- spectral pieces of isotropic turbulence: the seeded initial
+// condition and band-limited linear forcing in the reference's decomposition.
+#include "cup_internal.h"
+
+#include <cmath>
+#include <vector>
+
+namespace cup {
+
+__device__ __forceinline__ int signed_k(int i, int n) { return i <= n / 2 ? i : i - n; }
+
+// Philox-4x32-10 (Salmon, Moraes, Dror, Shaw: "Parallel random numbers: as easy as
+// 1, 2, 3", SC'11): 10 rounds of the 4x32 multiply-xor bijection with a Weyl key schedule.
+__device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                                              uint32_t k1, uint32_t out[4])
+{
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0, n1 = (uint32_t)p1;
+    const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1, n3 = (uint32_t)p0;
+    c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+// Packed half spectrum of the seeded isotropic-turbulence velocity, in FP64. Entry
+// (kz, ky, s): s >= 1 is mode (kx=s, ky, kz); s = 0 is mode (0, ky, kz) (+ i * the
+// kx = Nx/2 plane, which is zero). Construction per mode: SpectralManipFFTW::_compute_IC
+// (SpectralManipFFTW.cpp:254-285) with noise_a[1] for Im(u) and exact Hermitian symmetry.
+__global__ void k_hit_spectrum(C2<double>* __restrict__ su, C2<double>* __restrict__ sv, C2<double>* __restrict__ sw,
+                               int N0, int N1, int N2, double wfx, double wfy, double wfz, double k0, uint32_t key0,
+                               uint32_t key1)
+{
+  const int ncols = N0 / 2;
+  const size_t n = (size_t)ncols * N1 * N2;
+  const double coef = 32.0 / (3 * k0) * sqrt(2.0 / M_PI);
+  for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < n; id += (size_t)gridDim.x * blockDim.x) {
+    const int s = (int)(id % ncols);
+    const int ky = (int)((id / ncols) % N1);
+    const int kz = (int)(id / ((size_t)ncols * N1));
+    int ii = s, jj = signed_k(ky, N1), kk = signed_k(kz, N2);
+    C2<double> U{0, 0}, V{0, 0}, W{0, 0};
+    const bool nyq = 2 * abs(jj) == N1 || 2 * abs(kk) == N2;
+    if (!nyq && !(ii == 0 && jj == 0 && kk == 0)) {
+      const bool canonical = kk > 0 || (kk == 0 && (jj > 0 || (jj == 0 && ii > 0)));
+      if (!canonical) { ii = -ii; jj = -jj; kk = -kk; }
+      const int wi = ii < 0 ? ii + N0 : ii, wj = jj < 0 ? jj + N1 : jj, wk = kk < 0 ? kk + N2 : kk;
+      const uint64_t cid = ((uint64_t)wi * N1 + wj) * N2 + wk;
+      const double kx = ii * wfx, kyv = jj * wfy, kz_ = kk * wfz;
+      const double k2 = kx * kx + kyv * kyv + kz_ * kz_;
+      const double k_xy = sqrt(kx * kx + kyv * kyv), k_norm = sqrt(k2);
+      const double r = k_norm / k0;
+      const double E_k = coef * r * r * r * r * exp(-2 * r * r);
+      const double kn = k_norm / wfx;
+      const double amp = sqrt(E_k / (2 * M_PI * kn * kn));
+      uint32_t o[4];
+      philox4x32_10((uint32_t)cid, (uint32_t)(cid >> 32), 0u, 0u, key0, key1, o);
+      const double inv32 = 1.0 / 4294967296.0;
+      const double theta1 = (o[0] + 0.5) * inv32 * 2 * M_PI;
+      const double theta2 = (o[1] + 0.5) * inv32 * 2 * M_PI;
+      const double phi = (o[2] + 0.5) * inv32 * 2 * M_PI;
+      const double ca = amp * cos(phi), cb = amp * sin(phi);
+      const C2<double> na{ca * cos(theta1), ca * sin(theta1)};
+      const C2<double> nb{cb * cos(theta2), cb * sin(theta2)};
+      const double fac = k_norm * k_xy, invFac = fac <= 0 ? 0 : 1 / fac;
+      U = C2<double>{invFac * (na.x * k_norm * kyv + nb.x * kx * kz_), invFac * (na.y * k_norm * kyv + nb.y * kx * kz_)};
+      V = C2<double>{invFac * (nb.x * kyv * kz_ - na.x * k_norm * kx), invFac * (nb.y * kyv * kz_ - na.y * k_norm * kx)};
+      W = C2<double>{-nb.x * k_xy / k_norm, -nb.y * k_xy / k_norm};
+      if (!canonical) { U.y = -U.y; V.y = -V.y; W.y = -W.y; }
+    }
+    su[id] = U;
+    sv[id] = V;
+    sw[id] = W;
+  }
+}
+
+template <typename T>
+__global__ void k_scale3(T* __restrict__ a, T* __restrict__ b, T* __restrict__ c, size_t n, double s)
+{
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    a[i] = (T)(a[i] * s);
+    b[i] = (T)(b[i] * s);
+    c[i] = (T)(c[i] * s);
+  }
+}
+
+static dim3 grid_for(size_t n)
+{
+  size_t g = (n + NT - 1) / NT;
+  if (g > 65536) g = 65536;
+  return dim3((unsigned)g);
+}
+
+static int alloc_extra_spectra(Ctx& c)
+{
+  if (!c.spec2 && check_hip(c, hipMalloc(&c.spec2, c.ncell * c.esz), "hipMalloc spectrum 2")) return CUP_E_NOMEM;
+  if (!c.spec3 && check_hip(c, hipMalloc(&c.spec3, c.ncell * c.esz), "hipMalloc spectrum 3")) return CUP_E_NOMEM;
+  return CUP_OK;
+}
+
+int launch_ic_hit(Ctx& c, double k0, double u_rms, uint64_t seed)
+{
+  if (c.precision == 4) {
+    // Generate in an FP64 context of the same grid, then store rounded to FP32.
+    cup_grid_desc d;
+    for (int a = 0; a < 3; ++a) d.bpd[a] = c.bpd[a];
+    d.maxextent = c.maxextent;
+    d.precision = 8;
+    d.decomposition = CUP_FUSED;
+    d.device = c.dev;
+    cup_ctx* t = nullptr;
+    if (int rc = cup_create(&d, &t)) return set_error(c, rc, "ic_hit: FP64 work context");
+    int rc = cup_ic_hit(t, k0, u_rms, seed);
+    std::vector<double> host(c.ncell);
+    for (int f = 0; f < 8 && !rc; ++f) {
+      rc = cup_get_field(t, f, host.data());
+      if (!rc) rc = cup_set_field(reinterpret_cast<cup_ctx*>(&c), f, host.data());
+    }
+    if (rc) c.err = std::string("ic_hit: ") + cup_last_error(t);
+    cup_destroy(t);
+    return rc;
+  }
+  if (int rc = alloc_extra_spectra(c)) return rc;
+  for (int k = 0; k < 8; ++k)
+    if (int rc = launch_fill(c, c.f[k], c.ncell, 0.0)) return rc;
+  const size_t n = (size_t)(c.N[0] / 2) * c.N[1] * c.N[2];
+  const double wf[3] = {2 * M_PI / c.extent[0], 2 * M_PI / c.extent[1], 2 * M_PI / c.extent[2]};
+  hipLaunchKernelGGL(k_hit_spectrum, grid_for(n), dim3(NT), 0, c.stream, (C2<double>*)c.spec, (C2<double>*)c.spec2,
+                     (C2<double>*)c.spec3, c.N[0], c.N[1], c.N[2], wf[0], wf[1], wf[2], k0, (uint32_t)seed,
+                     (uint32_t)(seed >> 32));
+  if (int rc = check_hip(c, hipGetLastError(), "k_hit_spectrum")) return rc;
+  if (int rc = launch_fft_inverse_field(c, c.spec, c.f[CUP_U])) return rc;
+  if (int rc = launch_fft_inverse_field(c, c.spec2, c.f[CUP_V])) return rc;
+  if (int rc = launch_fft_inverse_field(c, c.spec3, c.f[CUP_W])) return rc;
+  const double zero[3] = {0, 0, 0};
+  if (int rc = launch_max_u(c, zero)) return rc;
+  if (int rc = launch_reduce(c, c.nb, 2, 1u, c.sc->red)) return rc;
+  double red[2];
+  if (int rc = check_hip(c, hipMemcpyAsync(red, c.sc->red, sizeof(red), hipMemcpyDeviceToHost, c.stream), "ic_hit"))
+    return rc;
+  if (int rc = check_hip(c, hipStreamSynchronize(c.stream), "ic_hit")) return rc;
+  const double scale = u_rms / std::sqrt(red[1] / (3.0 * (double)c.ncell));
+  hipLaunchKernelGGL((k_scale3<double>), grid_for(c.ncell), dim3(NT), 0, c.stream, (double*)c.f[CUP_U],
+                     (double*)c.f[CUP_V], (double*)c.f[CUP_W], c.ncell, scale);
+  return check_hip(c, hipGetLastError(), "k_scale3");
+}
+
+// ------------------------------------------------------------- forcing (plain)
+
+// SpectralManipFFTW::_compute_largeModesForcing (SpectralManipFFTW.cpp:25-108) over the
+// packed half spectra of u, v, w: per-mode energy, central-difference enstrophy, E/|k|,
+// energy of 0 < k^2 <= 4, and every other mode zeroed. The x axis is the halved one here
+// (weight 2 for 0 < kx < Nx/2); slot 0 holds the kx = 0 and kx = Nx/2 planes as A + iB and
+// is handled in (k, -k) pairs. part[4*wg + {0..3}] = tke, eps, tke_filtered, lIntegral sums.
+struct ModeStat {
+  double E, eps, lint;
+  bool keep;
+};
+
+__device__ __forceinline__ ModeStat mode_stat(int ii, int jj, int kk, int N0, int N1, int N2, double wfx, double wfy,
+                                              double wfz, C2<double> U, C2<double> V, C2<double> W)
+{
+  ModeStat m;
+  const double kx = ii * wfx, ky = jj * wfy, kz = kk * wfz;
+  const double k2 = kx * kx + ky * ky + kz * kz;
+  const double dX = 2 * sin(2 * M_PI / N0 * ii), dY = 2 * sin(2 * M_PI / N1 * jj), dZ = 2 * sin(2 * M_PI / N2 * kk);
+  // d/dx -> i*dX: (a+ib)*i*d = (-b d, a d)
+  const double OXr = -W.y * dY + V.y * dZ, OXi = W.x * dY - V.x * dZ;
+  const double OYr = -U.y * dZ + W.y * dX, OYi = U.x * dZ - W.x * dX;
+  const double OZr = -V.y * dX + U.y * dY, OZi = V.x * dX - U.x * dY;
+  m.E = 0.5 * (U.x * U.x + U.y * U.y + V.x * V.x + V.y * V.y + W.x * W.x + W.y * W.y);
+  m.eps = 0.5 * (OXr * OXr + OXi * OXi + OYr * OYr + OYi * OYi + OZr * OZr + OZi * OZi);
+  m.lint = k2 > 0 ? m.E / sqrt(k2) : 0;
+  m.keep = k2 > 0 && k2 <= 4;
+  return m;
+}
+
+template <typename T>
+__device__ __forceinline__ C2<double> ld(const C2<T>* p, size_t i)
+{
+  const C2<T> v = p[i];
+  return C2<double>{(double)v.x, (double)v.y};
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_forcing_filter(C2<T>* __restrict__ su, C2<T>* __restrict__ sv,
+                                                       C2<T>* __restrict__ sw, int N0, int N1, int N2, double wfx,
+                                                       double wfy, double wfz, double* __restrict__ part)
+{
+  const int ncols = N0 / 2;
+  const size_t n = (size_t)ncols * N1 * N2;
+  double acc[4] = {0, 0, 0, 0};
+  for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < n; id += (size_t)gridDim.x * blockDim.x) {
+    const int s = (int)(id % ncols);
+    const int ky = (int)((id / ncols) % N1);
+    const int kz = (int)(id / ((size_t)ncols * N1));
+    const int jj = signed_k(ky, N1), kk = signed_k(kz, N2);
+    if (s > 0) {
+      const ModeStat m = mode_stat(s, jj, kk, N0, N1, N2, wfx, wfy, wfz, ld(su, id), ld(sv, id), ld(sw, id));
+      acc[0] += 2 * m.E; acc[1] += 2 * m.eps; acc[3] += 2 * m.lint;
+      if (m.keep) acc[2] += 2 * m.E;
+      else { su[id] = C2<T>{0, 0}; sv[id] = C2<T>{0, 0}; sw[id] = C2<T>{0, 0}; }
+      continue;
+    }
+    // slot 0: pair (ky,kz) with its mirror (-ky,-kz); the canonical member does both.
+    const int my = ky == 0 ? 0 : N1 - ky, mz = kz == 0 ? 0 : N2 - kz;
+    const bool self = my == ky && mz == kz;
+    const bool canon = kk > 0 || (kk == 0 && jj > 0) || self;
+    if (!canon) continue;
+    const size_t mid = ((size_t)mz * N1 + my) * ncols;
+    C2<double> Z[3] = {ld(su, id), ld(sv, id), ld(sw, id)};
+    C2<double> Zm[3] = {ld(su, mid), ld(sv, mid), ld(sw, mid)};
+    C2<double> A[3], B[3];
+    for (int q = 0; q < 3; ++q) {
+      A[q] = C2<double>{0.5 * (Z[q].x + Zm[q].x), 0.5 * (Z[q].y - Zm[q].y)};
+      B[q] = C2<double>{0.5 * (Z[q].y + Zm[q].y), 0.5 * (Zm[q].x - Z[q].x)};
+    }
+    const double w = self ? 1.0 : 2.0;
+    const ModeStat ma = mode_stat(0, jj, kk, N0, N1, N2, wfx, wfy, wfz, A[0], A[1], A[2]);
+    const ModeStat mb = mode_stat(N0 / 2, jj, kk, N0, N1, N2, wfx, wfy, wfz, B[0], B[1], B[2]);
+    acc[0] += w * (ma.E + mb.E);
+    acc[1] += w * (ma.eps + mb.eps);
+    acc[3] += w * (ma.lint + mb.lint);
+    if (ma.keep) acc[2] += w * ma.E;
+    if (mb.keep) acc[2] += w * mb.E;
+    C2<T>* sp[3] = {su, sv, sw};
+    for (int q = 0; q < 3; ++q) {
+      const C2<double> a = ma.keep ? A[q] : C2<double>{0, 0};
+      const C2<double> bq = mb.keep ? B[q] : C2<double>{0, 0};
+      // Z = a + i b ; Z(-k) = conj(a) + i conj(b)
+      sp[q][id] = C2<T>{(T)(a.x - bq.y), (T)(a.y + bq.x)};
+      if (!self) sp[q][mid] = C2<T>{(T)(a.x + bq.y), (T)(bq.x - a.y)};
+    }
+  }
+  // block reduction
+  __shared__ double red[NT / 64][4];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int i = 0; i < 4; ++i) acc[i] = wave_sum(acc[i]);
+  if (lane == 0)
+    for (int i = 0; i < 4; ++i) red[wv][i] = acc[i];
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int i = 0; i < 4; ++i) {
+      double a = 0;
+      for (int q = 0; q < NT / 64; ++q) a += red[q][i];
+      part[4 * (size_t)blockIdx.x + i] = a;
+    }
+}
+
+template <typename T>
+__global__ void k_axpy_blocks(T* __restrict__ y, const T* __restrict__ x, size_t n, double a)
+{
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    y[i] = (T)((double)y[i] + a * (double)x[i]);
+}
+
+template <typename T>
+static int forcing_plain_t(Ctx& c, double dt, double sim_dt, double nu, double tke_target, double inj_rate,
+                           double stats[5])
+{
+  if (int rc = alloc_extra_spectra(c)) return rc;
+  const int v0 = c.decomposition == CUP_FUSED ? CUP_TMPU : CUP_U;
+  // SpectralForcing::_cub2fftw + runFwd: three forward transforms.
+  if (int rc = launch_fft_forward_field(c, c.f[v0], c.spec)) return rc;
+  if (int rc = launch_fft_forward_field(c, c.f[v0 + 1], c.spec2)) return rc;
+  if (int rc = launch_fft_forward_field(c, c.f[v0 + 2], c.spec3)) return rc;
+  const size_t n = (size_t)(c.N[0] / 2) * c.N[1] * c.N[2];
+  dim3 g = grid_for(n);
+  if (g.x > 4096) g.x = 4096;
+  if (int rc = ensure_partials(c, 4 * (size_t)g.x)) return rc;
+  const double wf[3] = {2 * M_PI / c.extent[0], 2 * M_PI / c.extent[1], 2 * M_PI / c.extent[2]};
+  hipLaunchKernelGGL((k_forcing_filter<T>), g, dim3(NT), 0, c.stream, (C2<T>*)c.spec, (C2<T>*)c.spec2,
+                     (C2<T>*)c.spec3, c.N[0], c.N[1], c.N[2], wf[0], wf[1], wf[2], c.partials);
+  if (int rc = check_hip(c, hipGetLastError(), "k_forcing_filter")) return rc;
+  if (int rc = launch_reduce(c, (int)g.x, 4, 0u, c.sc->red)) return rc;
+  if (int rc = check_hip(c, hipMemcpyAsync(c.sc_host, c.sc, sizeof(DevScalars), hipMemcpyDeviceToHost, c.stream),
+                         "forcing"))
+    return rc;
+  if (int rc = check_hip(c, hipStreamSynchronize(c.stream), "forcing")) return rc;
+  if (dt < 0) dt = c.sc_host->dt;
+  if (sim_dt < 0) sim_dt = c.sc_host->dt;
+  const double normFFT = (double)c.N[0] * c.N[1] * c.N[2];
+  const double* r = c.sc_host->red;
+  const double tke = r[0] / (normFFT * normFFT);
+  const double eps = r[1] * 2 * nu / std::pow(normFFT * 2 * c.h, 2);
+  const double tkeF = r[2] / (normFFT * normFFT);
+  const double lint = r[3] / r[0];
+  double inj = 0;
+  if (tke_target > 0) inj = (tke_target - tke) / dt;
+  else if (inj_rate > 0) inj = inj_rate;
+  const double fac = sim_dt * inj / (2 * tkeF);
+  if (fac > 0) {
+    void* specs[3] = {c.spec, c.spec2, c.spec3};
+    for (int q = 0; q < 3; ++q) {
+      if (int rc = launch_fft_inverse_field(c, specs[q], c.rbuf)) return rc;
+      hipLaunchKernelGGL((k_axpy_blocks<T>), grid_for(c.ncell), dim3(NT), 0, c.stream, (T*)c.f[v0 + q],
+                         (const T*)c.rbuf, c.ncell, fac / normFFT);
+      if (int rc = check_hip(c, hipGetLastError(), "k_axpy_blocks")) return rc;
+    }
+  }
+  stats[0] = tke; stats[1] = eps; stats[2] = tkeF; stats[3] = lint; stats[4] = fac;
+  return CUP_OK;
+}
+
+int forcing_plain(Ctx& c, double dt, double sim_dt, double nu, double tke_target, double inj, double stats[5])
+{
+  return c.precision == 4 ? forcing_plain_t<float>(c, dt, sim_dt, nu, tke_target, inj, stats)
+                          : forcing_plain_t<double>(c, dt, sim_dt, nu, tke_target, inj, stats);
+}
+
+// Low modes of the linear forcing, 0 < |k|^2 <= 4 in physical wavenumbers, canonical
+// member of each conjugate pair (kz > 0, or kz = 0 and ky > 0, or kz = ky = 0 and kx > 0).
+int spectral_setup(Ctx& c)
+{
+  std::vector<int> ks;
+  const double wf[3] = {2 * M_PI / c.extent[0], 2 * M_PI / c.extent[1], 2 * M_PI / c.extent[2]};
+  int lim[3];
+  for (int a = 0; a < 3; ++a) {
+    lim[a] = (int)std::floor(2.0 / wf[a] + 1e-9);
+    if (lim[a] > c.N[a] / 2 - 1) lim[a] = c.N[a] / 2 - 1;
+  }
+  for (int kz = 0; kz <= lim[2]; ++kz)
+    for (int ky = -lim[1]; ky <= lim[1]; ++ky)
+      for (int kx = -lim[0]; kx <= lim[0]; ++kx) {
+        const bool canonical = kz > 0 || (kz == 0 && (ky > 0 || (ky == 0 && kx > 0)));
+        if (!canonical) continue;
+        const double k2 = std::pow(kx * wf[0], 2) + std::pow(ky * wf[1], 2) + std::pow(kz * wf[2], 2);
+        if (k2 > 0 && k2 <= 4) { ks.push_back(kx); ks.push_back(ky); ks.push_back(kz); }
+      }
+  c.nlow = (int)ks.size() / 3;
+  if (c.nlow == 0) return CUP_OK;
+  if (check_hip(c, hipMalloc(&c.low_k, ks.size() * sizeof(int)), "hipMalloc low modes")) return CUP_E_NOMEM;
+  hipMemcpy(c.low_k, ks.data(), ks.size() * sizeof(int), hipMemcpyHostToDevice);
+  if (check_hip(c, hipMalloc(&c.low_coef, (size_t)c.nlow * 6 * sizeof(double)), "hipMalloc low coef"))
+    return CUP_E_NOMEM;
+  return CUP_OK;
+}
+
+}  // namespace cup

@@ -1,0 +1,590 @@
+// cup_stencil.hip -- block-store kernels: LDS-staged periodic halo gather, advection-
+// diffusion, pressure RHS, pressure gradient, reductions, diagnostics, initial conditions.
+//
+// Layout: every field is block-major, block b = bx + nbx*(by + nby*bz) holds 512 cells
+// [iz][iy][ix] contiguously. One workgroup of 256 threads owns one 8^3 block: it stages
+// the block plus the halo its stencil needs in LDS (the device form of Cubism's
+// BlockLab::load with periodic wrap), then computes 2 cells per thread.
+#include "cup_internal.h"
+
+#include <cmath>
+
+namespace cup {
+
+// 10^3 lab (halo 1), faces only: the 7-point stencils never read edges or corners.
+constexpr int L1 = 10;
+constexpr int L1C = L1 * L1 * L1;
+
+template <typename T, int NF>
+__device__ __forceinline__ void gather_g1(const T* const* src, T (*lab)[L1C], const GridArgs& g, int b,
+                                          int bx, int by, int bz)
+{
+  const int tid = threadIdx.x;
+  for (int c = tid; c < BS3; c += NT) {
+    const int ix = c & 7, iy = (c >> 3) & 7, iz = c >> 6;
+    const int l = ((iz + 1) * L1 + iy + 1) * L1 + ix + 1;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) lab[f][l] = src[f][(size_t)b * BS3 + c];
+  }
+  for (int c = tid; c < 6 * 64; c += NT) {
+    const int face = c >> 6, a = (c >> 3) & 7, e = c & 7;
+    int qx = bx, qy = by, qz = bz, sx, sy, sz, lx, ly, lz;
+    switch (face) {
+      case 0: qx = wrapi(bx - 1, g.nbx); sx = 7; sy = e; sz = a; lx = 0; ly = e + 1; lz = a + 1; break;
+      case 1: qx = wrapi(bx + 1, g.nbx); sx = 0; sy = e; sz = a; lx = 9; ly = e + 1; lz = a + 1; break;
+      case 2: qy = wrapi(by - 1, g.nby); sx = e; sy = 7; sz = a; lx = e + 1; ly = 0; lz = a + 1; break;
+      case 3: qy = wrapi(by + 1, g.nby); sx = e; sy = 0; sz = a; lx = e + 1; ly = 9; lz = a + 1; break;
+      case 4: qz = wrapi(bz - 1, g.nbz); sx = e; sy = a; sz = 7; lx = e + 1; ly = a + 1; lz = 0; break;
+      default: qz = wrapi(bz + 1, g.nbz); sx = e; sy = a; sz = 0; lx = e + 1; ly = a + 1; lz = 9; break;
+    }
+    const size_t s = (size_t)(qx + g.nbx * (qy + g.nby * qz)) * BS3 + (sz * BS + sy) * BS + sx;
+    const int l = (lz * L1 + ly) * L1 + lx;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) lab[f][l] = src[f][s];
+  }
+}
+
+__device__ __forceinline__ void block_coords(const GridArgs& g, int b, int& bx, int& by, int& bz)
+{
+  bx = b % g.nbx;
+  by = (b / g.nbx) % g.nby;
+  bz = b / (g.nbx * g.nby);
+}
+
+// Block-wide reduction of NV doubles per thread into out[0..NV) (thread 0 holds them).
+// Fixed shuffle tree and fixed LDS order: deterministic.
+template <int NV>
+__device__ __forceinline__ void block_reduce(double (&v)[NV], const unsigned maxmask, double* out)
+{
+  __shared__ double red[NT / 64][NV > 0 ? NV : 1];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) v[i] = ((maxmask >> i) & 1) ? wave_max(v[i]) : wave_sum(v[i]);
+  if (lane == 0)
+#pragma unroll
+    for (int i = 0; i < NV; ++i) red[wv][i] = v[i];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      double a = red[0][i];
+      for (int w = 1; w < NT / 64; ++w) a = ((maxmask >> i) & 1) ? fmax(a, red[w][i]) : a + red[w][i];
+      out[i] = a;
+    }
+  }
+}
+
+// ------------------------------------------------------------------- lab shift test
+
+template <typename T, int G>
+__global__ __launch_bounds__(NT) void k_lab_shift(const T* __restrict__ u, T* __restrict__ out, GridArgs g, int dx,
+                                                  int dy, int dz)
+{
+  constexpr int L = BS + 2 * G;
+  __shared__ T lab[L * L * L];
+  const int b = blockIdx.x;
+  int bx, by, bz;
+  block_coords(g, b, bx, by, bz);
+  for (int c = threadIdx.x; c < L * L * L; c += NT) {
+    const int lx = c % L, ly = (c / L) % L, lz = c / (L * L);
+    const int gx = wrapi(bx * BS + lx - G, g.Nx), gy = wrapi(by * BS + ly - G, g.Ny), gz = wrapi(bz * BS + lz - G, g.Nz);
+    lab[c] = u[cell_index(g, gx, gy, gz)];
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < BS3; c += NT) {
+    const int ix = c & 7, iy = (c >> 3) & 7, iz = c >> 6;
+    out[(size_t)b * BS3 + c] = lab[((iz + G - dz) * L + (iy + G - dy)) * L + (ix + G - dx)];
+  }
+}
+
+template <typename T>
+static int lab_shift_t(Ctx& c, int G, int dx, int dy, int dz)
+{
+  const T* u = (const T*)c.f[CUP_U];
+  T* o = (T*)c.f[CUP_TMPU];
+  if (G == 1) hipLaunchKernelGGL((k_lab_shift<T, 1>), dim3(c.nb), dim3(NT), 0, c.stream, u, o, c.grid(), dx, dy, dz);
+  else hipLaunchKernelGGL((k_lab_shift<T, 2>), dim3(c.nb), dim3(NT), 0, c.stream, u, o, c.grid(), dx, dy, dz);
+  return check_hip(c, hipGetLastError(), "k_lab_shift");
+}
+
+int launch_lab_shift(Ctx& c, int g, int dx, int dy, int dz)
+{
+  if (g < 1 || g > 2 || abs(dx) > g || abs(dy) > g || abs(dz) > g)
+    return set_error(c, CUP_E_INVALID, "lab_shift: need 1 <= g <= 2 and |shift| <= g");
+  return c.precision == 4 ? lab_shift_t<float>(c, g, dx, dy, dz) : lab_shift_t<double>(c, g, dx, dy, dz);
+}
+
+// ------------------------------------------------------------------- max |u + uinf|
+// findMaxU (ProcessHelpers.h:74-99) per block, with the sum of |u|^2 beside it:
+// part[2b] = max_i |u_i + uinf_i|, part[2b+1] = sum (u^2+v^2+w^2).
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_max_u(const T* __restrict__ u, const T* __restrict__ v,
+                                              const T* __restrict__ w, int nb, double ux, double uy, double uz,
+                                              double* __restrict__ part)
+{
+  const int b = blockIdx.x;
+  double m[2] = {0.0, 0.0};
+  for (int c = threadIdx.x; c < BS3; c += NT) {
+    const size_t i = (size_t)b * BS3 + c;
+    const double a = fabs((double)u[i] + ux), bb = fabs((double)v[i] + uy), cc = fabs((double)w[i] + uz);
+    m[0] = fmax(m[0], fmax(a, fmax(bb, cc)));
+    m[1] += (double)u[i] * u[i] + (double)v[i] * v[i] + (double)w[i] * w[i];
+  }
+  block_reduce<2>(m, 1u, part + 2 * (size_t)b);
+}
+
+template <typename T>
+static int max_u_t(Ctx& c, const double uinf[3])
+{
+  hipLaunchKernelGGL((k_max_u<T>), dim3(c.nb), dim3(NT), 0, c.stream, (const T*)c.f[CUP_U], (const T*)c.f[CUP_V],
+                     (const T*)c.f[CUP_W], c.nb, uinf[0], uinf[1], uinf[2], c.partials);
+  return check_hip(c, hipGetLastError(), "k_max_u");
+}
+
+int launch_max_u(Ctx& c, const double uinf[3])
+{
+  if (int e = ensure_partials(c, 2 * (size_t)c.nb)) return e;
+  return c.precision == 4 ? max_u_t<float>(c, uinf) : max_u_t<double>(c, uinf);
+}
+
+// ------------------------------------------------------------ advection-diffusion
+
+// KernelAdvectDiffuse (AdvectionDiffusion.cpp:106-145), same operation order:
+// tmp = u + facA*((u+U).grad_c u) + facD*(sum of 6 neighbours - 6u), facA = -dt/(2h),
+// facD = (nu/h)*(dt/h).
+template <typename T>
+__global__ __launch_bounds__(NT) void k_advdiff_central(const T* __restrict__ u, const T* __restrict__ v,
+                                                        const T* __restrict__ w, T* __restrict__ ou,
+                                                        T* __restrict__ ov, T* __restrict__ ow, GridArgs g,
+                                                        const DevScalars* __restrict__ sc, T Ux, T Uy, T Uz,
+                                                        double nu)
+{
+  __shared__ T lab[3][L1C];
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  int bx, by, bz;
+  block_coords(g, b, bx, by, bz);
+  const T* src[3] = {u, v, w};
+  gather_g1<T, 3>(src, lab, g, b, bx, by, bz);
+  __syncthreads();
+  const double dt = sc->dt;
+  const T facA = (T)(-dt / (2 * g.h));
+  const T facD = (T)((nu / g.h) * (dt / g.h));
+  for (int c = threadIdx.x; c < BS3; c += NT) {
+    const int ix = c & 7, iy = (c >> 3) & 7, iz = c >> 6;
+    const int l = ((iz + 1) * L1 + iy + 1) * L1 + ix + 1;
+    const T* U = lab[0];
+    const T* V = lab[1];
+    const T* W = lab[2];
+    const T dudx = U[l + 1] - U[l - 1], dvdx = V[l + 1] - V[l - 1], dwdx = W[l + 1] - W[l - 1];
+    const T dudy = U[l + L1] - U[l - L1], dvdy = V[l + L1] - V[l - L1], dwdy = W[l + L1] - W[l - L1];
+    const T dudz = U[l + L1 * L1] - U[l - L1 * L1], dvdz = V[l + L1 * L1] - V[l - L1 * L1],
+            dwdz = W[l + L1 * L1] - W[l - L1 * L1];
+    const T uu = U[l] + Ux, vv = V[l] + Uy, ww = W[l] + Uz;
+    const T duD = U[l + L1] + U[l - L1] + U[l + 1] + U[l - 1] + U[l - L1 * L1] + U[l + L1 * L1] - U[l] * 6;
+    const T dvD = V[l + L1] + V[l - L1] + V[l + 1] + V[l - 1] + V[l - L1 * L1] + V[l + L1 * L1] - V[l] * 6;
+    const T dwD = W[l + L1] + W[l - L1] + W[l + 1] + W[l - 1] + W[l - L1 * L1] + W[l + L1 * L1] - W[l] * 6;
+    const T duA = uu * dudx + vv * dudy + ww * dudz;
+    const T dvA = uu * dvdx + vv * dvdy + ww * dvdz;
+    const T dwA = uu * dwdx + vv * dwdy + ww * dwdz;
+    const size_t o = (size_t)b * BS3 + c;
+    ou[o] = U[l] + facA * duA + facD * duD;
+    ov[o] = V[l] + facA * dvA + facD * dvD;
+    ow[o] = W[l] + facA * dwA + facD * dwD;
+  }
+}
+
+template <typename T>
+static int advdiff_t(Ctx& c, double nu, const double uinf[3])
+{
+  hipLaunchKernelGGL((k_advdiff_central<T>), dim3(c.nb), dim3(NT), 0, c.stream, (const T*)c.f[CUP_U],
+                     (const T*)c.f[CUP_V], (const T*)c.f[CUP_W], (T*)c.f[CUP_TMPU], (T*)c.f[CUP_TMPV],
+                     (T*)c.f[CUP_TMPW], c.grid(), c.sc, (T)uinf[0], (T)uinf[1], (T)uinf[2], nu);
+  return check_hip(c, hipGetLastError(), "k_advdiff_central");
+}
+
+int launch_advdiff(Ctx& c, double nu, const double uinf[3], int scheme)
+{
+  if (scheme != CUP_CENTRAL_EULER) return set_error(c, CUP_E_INVALID, "advect_diffuse: scheme not available");
+  return c.precision == 4 ? advdiff_t<float>(c, nu, uinf) : advdiff_t<double>(c, nu, uinf);
+}
+
+// -------------------------------------------------------------------- copies, fills
+
+template <typename T>
+__global__ void k_copy3(T* __restrict__ a, T* __restrict__ b, T* __restrict__ cc, const T* __restrict__ sa,
+                        const T* __restrict__ sb, const T* __restrict__ scc, size_t n)
+{
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    a[i] = sa[i];
+    b[i] = sb[i];
+    cc[i] = scc[i];
+  }
+}
+
+template <typename T>
+__global__ void k_copy1(T* __restrict__ a, const T* __restrict__ sa, size_t n)
+{
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    a[i] = sa[i];
+}
+
+template <typename T>
+__global__ void k_fill(T* __restrict__ a, size_t n, T value)
+{
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    a[i] = value;
+}
+
+static dim3 ew_grid(size_t n)
+{
+  size_t g = (n + NT - 1) / NT;
+  if (g > 65536) g = 65536;
+  return dim3((unsigned)g);
+}
+
+int launch_copy_tmp_to_u(Ctx& c)
+{
+  if (c.precision == 4)
+    hipLaunchKernelGGL((k_copy3<float>), ew_grid(c.ncell), dim3(NT), 0, c.stream, (float*)c.f[CUP_U],
+                       (float*)c.f[CUP_V], (float*)c.f[CUP_W], (const float*)c.f[CUP_TMPU],
+                       (const float*)c.f[CUP_TMPV], (const float*)c.f[CUP_TMPW], c.ncell);
+  else
+    hipLaunchKernelGGL((k_copy3<double>), ew_grid(c.ncell), dim3(NT), 0, c.stream, (double*)c.f[CUP_U],
+                       (double*)c.f[CUP_V], (double*)c.f[CUP_W], (const double*)c.f[CUP_TMPU],
+                       (const double*)c.f[CUP_TMPV], (const double*)c.f[CUP_TMPW], c.ncell);
+  return check_hip(c, hipGetLastError(), "k_copy3");
+}
+
+int launch_copy(Ctx& c, void* dst, const void* src, size_t n)
+{
+  if (c.precision == 4)
+    hipLaunchKernelGGL((k_copy1<float>), ew_grid(n), dim3(NT), 0, c.stream, (float*)dst, (const float*)src, n);
+  else
+    hipLaunchKernelGGL((k_copy1<double>), ew_grid(n), dim3(NT), 0, c.stream, (double*)dst, (const double*)src, n);
+  return check_hip(c, hipGetLastError(), "k_copy1");
+}
+
+int launch_fill(Ctx& c, void* ptr, size_t n, double value)
+{
+  if (c.precision == 4)
+    hipLaunchKernelGGL((k_fill<float>), ew_grid(n), dim3(NT), 0, c.stream, (float*)ptr, n, (float)value);
+  else
+    hipLaunchKernelGGL((k_fill<double>), ew_grid(n), dim3(NT), 0, c.stream, (double*)ptr, n, value);
+  return check_hip(c, hipGetLastError(), "k_fill");
+}
+
+__global__ void k_set_dt(DevScalars* sc, double dt) { sc->dt = dt; }
+
+int launch_set_dt(Ctx& c, double dt)
+{
+  hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, c.stream, c.sc, dt);
+  return check_hip(c, hipGetLastError(), "k_set_dt");
+}
+
+// ----------------------------------------------------------------- pressure RHS
+
+// KernelPressureRHS (PressureRHS.cpp:51-65): h^2/(2dt) * (u_E-u_W + v_N-v_S + w_B-w_F),
+// written to a block-layout buffer (the solver's input).
+template <typename T>
+__global__ __launch_bounds__(NT) void k_rhs(const T* __restrict__ u, const T* __restrict__ v,
+                                            const T* __restrict__ w, T* __restrict__ rhs, GridArgs g,
+                                            const DevScalars* __restrict__ sc)
+{
+  __shared__ T lab[3][L1C];
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  int bx, by, bz;
+  block_coords(g, b, bx, by, bz);
+  const T* src[3] = {u, v, w};
+  gather_g1<T, 3>(src, lab, g, b, bx, by, bz);
+  __syncthreads();
+  const T fac = (T)(0.5 * g.h * g.h / sc->dt);
+  for (int c = threadIdx.x; c < BS3; c += NT) {
+    const int ix = c & 7, iy = (c >> 3) & 7, iz = c >> 6;
+    const int l = ((iz + 1) * L1 + iy + 1) * L1 + ix + 1;
+    rhs[(size_t)b * BS3 + c] = fac * (lab[0][l + 1] - lab[0][l - 1] + lab[1][l + L1] - lab[1][l - L1] +
+                                      lab[2][l + L1 * L1] - lab[2][l - L1 * L1]);
+  }
+}
+
+int launch_rhs_plain(Ctx& c)
+{
+  const int src0 = c.decomposition == CUP_FUSED ? CUP_TMPU : CUP_U;
+  if (c.precision == 4)
+    hipLaunchKernelGGL((k_rhs<float>), dim3(c.nb), dim3(NT), 0, c.stream, (const float*)c.f[src0],
+                       (const float*)c.f[src0 + 1], (const float*)c.f[src0 + 2], (float*)c.rbuf, c.grid(), c.sc);
+  else
+    hipLaunchKernelGGL((k_rhs<double>), dim3(c.nb), dim3(NT), 0, c.stream, (const double*)c.f[src0],
+                       (const double*)c.f[src0 + 1], (const double*)c.f[src0 + 2], (double*)c.rbuf, c.grid(), c.sc);
+  return check_hip(c, hipGetLastError(), "k_rhs");
+}
+
+// --------------------------------------------------------------- pressure gradient
+
+// KernelGradP (PressureProjection.cpp:38-50): u += -dt/(2h) (p_E - p_W), etc.
+// PLAIN: in place on u (which holds u* after the advection copy).
+// FUSED: u = tmp + fac*grad p, plus the step's reductions: max |u+uinf| and sum |u|^2.
+template <typename T, bool FUSED>
+__global__ __launch_bounds__(NT) void k_gradp(const T* __restrict__ p, const T* __restrict__ su,
+                                              const T* __restrict__ sv, const T* __restrict__ sw, T* __restrict__ u,
+                                              T* __restrict__ v, T* __restrict__ w, GridArgs g,
+                                              const DevScalars* __restrict__ sc, double Ux, double Uy, double Uz,
+                                              double* __restrict__ part)
+{
+  __shared__ T lab[1][L1C];
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  int bx, by, bz;
+  block_coords(g, b, bx, by, bz);
+  const T* src[1] = {p};
+  gather_g1<T, 1>(src, lab, g, b, bx, by, bz);
+  __syncthreads();
+  const T fac = (T)(-0.5 * sc->dt / g.h);
+  double r[2] = {0.0, 0.0};
+  for (int c = threadIdx.x; c < BS3; c += NT) {
+    const int ix = c & 7, iy = (c >> 3) & 7, iz = c >> 6;
+    const int l = ((iz + 1) * L1 + iy + 1) * L1 + ix + 1;
+    const size_t o = (size_t)b * BS3 + c;
+    const T* P = lab[0];
+    const T nu_ = su[o] + fac * (P[l + 1] - P[l - 1]);
+    const T nv_ = sv[o] + fac * (P[l + L1] - P[l - L1]);
+    const T nw_ = sw[o] + fac * (P[l + L1 * L1] - P[l - L1 * L1]);
+    u[o] = nu_;
+    v[o] = nv_;
+    w[o] = nw_;
+    if (FUSED) {
+      const double a = fabs((double)nu_ + Ux), bb = fabs((double)nv_ + Uy), cc = fabs((double)nw_ + Uz);
+      r[0] = fmax(r[0], fmax(a, fmax(bb, cc)));
+      r[1] += (double)nu_ * nu_ + (double)nv_ * nv_ + (double)nw_ * nw_;
+    }
+  }
+  if (FUSED) block_reduce<2>(r, 1u, part + 2 * (size_t)b);
+}
+
+template <typename T>
+static int gradp_t(Ctx& c, bool fused, const double uinf[3])
+{
+  const T* p = (const T*)c.f[CUP_P];
+  const int s = fused ? CUP_TMPU : CUP_U;
+  if (fused)
+    hipLaunchKernelGGL((k_gradp<T, true>), dim3(c.nb), dim3(NT), 0, c.stream, p, (const T*)c.f[s],
+                       (const T*)c.f[s + 1], (const T*)c.f[s + 2], (T*)c.f[CUP_U], (T*)c.f[CUP_V], (T*)c.f[CUP_W],
+                       c.grid(), c.sc, uinf[0], uinf[1], uinf[2], c.partials);
+  else
+    hipLaunchKernelGGL((k_gradp<T, false>), dim3(c.nb), dim3(NT), 0, c.stream, p, (const T*)c.f[s],
+                       (const T*)c.f[s + 1], (const T*)c.f[s + 2], (T*)c.f[CUP_U], (T*)c.f[CUP_V], (T*)c.f[CUP_W],
+                       c.grid(), c.sc, 0.0, 0.0, 0.0, (double*)nullptr);
+  return check_hip(c, hipGetLastError(), "k_gradp");
+}
+
+int launch_gradp_plain(Ctx& c)
+{
+  return c.precision == 4 ? gradp_t<float>(c, false, nullptr) : gradp_t<double>(c, false, nullptr);
+}
+
+int launch_gradp_fused(Ctx& c, const double uinf[3])
+{
+  if (int e = ensure_partials(c, 2 * (size_t)c.nb)) return e;
+  return c.precision == 4 ? gradp_t<float>(c, true, uinf) : gradp_t<double>(c, true, uinf);
+}
+
+// ------------------------------------------------------------------- reductions
+
+// One workgroup per component; component i of partial j at part[j*width + i].
+__global__ __launch_bounds__(NT) void k_reduce(const double* __restrict__ part, int n, int width, unsigned maxmask,
+                                               double* __restrict__ out)
+{
+  const int i = blockIdx.x;
+  const bool ismax = (maxmask >> i) & 1;
+  double a[1] = {ismax ? -INFINITY : 0.0};
+  for (int j = threadIdx.x; j < n; j += NT) {
+    const double x = part[(size_t)j * width + i];
+    a[0] = ismax ? fmax(a[0], x) : a[0] + x;
+  }
+  block_reduce<1>(a, ismax ? 1u : 0u, out + i);
+}
+
+int launch_reduce(Ctx& c, int n, int width, unsigned maxmask, double* out_dev)
+{
+  hipLaunchKernelGGL(k_reduce, dim3(width), dim3(NT), 0, c.stream, c.partials, n, width, maxmask, out_dev);
+  return check_hip(c, hipGetLastError(), "k_reduce");
+}
+
+// Step bookkeeping after the fused gradient: umax and kinetic energy of the new velocity,
+// step += 1 and time += dt when advancing, then the next dt by Simulation::calcMaxTimestep
+// (Simulation.cpp:403-424), ramp included.
+__global__ __launch_bounds__(NT) void k_finalize_step(const double* __restrict__ part, int n, DevScalars* sc,
+                                                      double CFL, double nu, double h, int rampup,
+                                                      double fixed_dt, int advance)
+{
+  double r[2] = {0.0, 0.0};
+  for (int j = threadIdx.x; j < n; j += NT) {
+    r[0] = fmax(r[0], part[2 * (size_t)j]);
+    r[1] += part[2 * (size_t)j + 1];
+  }
+  __shared__ double out[2];
+  block_reduce<2>(r, 1u, out);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double umax = out[0];
+    sc->umax = umax;
+    sc->ke = 0.5 * out[1] * h * h * h;
+    if (advance) {
+      sc->step += 1;
+      sc->time += sc->dt;
+      sc->last_dt = sc->dt;
+    }
+    double dt;
+    if (fixed_dt > 0) {
+      dt = fixed_dt;
+    } else {
+      const double dtDif = nu > 0 ? h * h / nu : INFINITY;
+      const double dtAdv = h / (umax + 1e-8);
+      const double m = fmin(dtDif, dtAdv);
+      dt = CFL * m;
+      if (sc->step < rampup) {
+        const double x = (sc->step + 1.0) / rampup;
+        const double rampCFL = exp(log(1e-3) * (1 - x) + log(CFL) * x);
+        dt = rampCFL * m;
+      }
+    }
+    sc->dt = dt;
+  }
+}
+
+int launch_finalize_step(Ctx& c, int n, double CFL, double nu, int rampup, double fixed_dt, int advance)
+{
+  hipLaunchKernelGGL(k_finalize_step, dim3(1), dim3(NT), 0, c.stream, c.partials, n, c.sc, CFL, nu, c.h, rampup,
+                     fixed_dt, advance);
+  return check_hip(c, hipGetLastError(), "k_finalize_step");
+}
+
+// ------------------------------------------------------------------ diagnostics
+
+// KernelDissipation (ComputeDissipation.cpp:46-115): 20 FP64 integrals per block.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_diagnostics(const T* __restrict__ u, const T* __restrict__ v,
+                                                    const T* __restrict__ w, const T* __restrict__ p,
+                                                    const T* __restrict__ chi, GridArgs g, double nu, double cx,
+                                                    double cy, double cz, double* __restrict__ part)
+{
+  __shared__ T lab[4][L1C];
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  int bx, by, bz;
+  block_coords(g, b, bx, by, bz);
+  const T* src[4] = {u, v, w, p};
+  gather_g1<T, 4>(src, lab, g, b, bx, by, bz);
+  __syncthreads();
+  const double h = g.h, hCube = h * h * h, inv2h = .5 / h, invHh = 1 / (h * h);
+  double R[20];
+#pragma unroll
+  for (int i = 0; i < 20; ++i) R[i] = 0;
+  for (int c = threadIdx.x; c < BS3; c += NT) {
+    const int ix = c & 7, iy = (c >> 3) & 7, iz = c >> 6;
+    const int l = ((iz + 1) * L1 + iy + 1) * L1 + ix + 1;
+    const int E = l + 1, W = l - 1, N = l + L1, S = l - L1, B = l + L1 * L1, F = l - L1 * L1;
+    const T* U = lab[0];
+    const T* V = lab[1];
+    const T* Wv = lab[2];
+    const T* P = lab[3];
+    const double X = (double)chi[(size_t)b * BS3 + c];
+    const double PX = (bx * BS + ix + 0.5) * h - cx, PY = (by * BS + iy + 0.5) * h - cy, PZ = (bz * BS + iz + 0.5) * h - cz;
+    const double Lu = U[l], Lv = V[l], Lw = Wv[l];
+    const double WX = inv2h * (((double)Wv[N] - Wv[S]) - ((double)V[B] - V[F]));
+    const double WY = inv2h * (((double)U[B] - U[F]) - ((double)Wv[E] - Wv[W]));
+    const double WZ = inv2h * (((double)V[E] - V[W]) - ((double)U[N] - U[S]));
+    const double dPdx = inv2h * ((double)P[E] - P[W]);
+    const double dPdy = inv2h * ((double)P[N] - P[S]);
+    const double dPdz = inv2h * ((double)P[B] - P[F]);
+    R[15] -= (1 - X) * hCube * (dPdx * Lu + dPdy * Lv + dPdz * Lw);
+    const double lapU = invHh * ((double)U[E] + U[W] + U[N] + U[S] + U[B] + U[F] - 6 * Lu);
+    const double lapV = invHh * ((double)V[E] + V[W] + V[N] + V[S] + V[B] + V[F] - 6 * Lv);
+    const double lapW = invHh * ((double)Wv[E] + Wv[W] + Wv[N] + Wv[S] + Wv[B] + Wv[F] - 6 * Lw);
+    const double V1 = lapU * Lu + lapV * Lv + lapW * Lw;
+    const double D11 = inv2h * ((double)U[E] - U[W]);
+    const double D22 = inv2h * ((double)V[N] - V[S]);
+    const double D33 = inv2h * ((double)Wv[B] - Wv[F]);
+    const double D12 = inv2h * ((double)U[N] - U[S] + V[E] - V[W]) / 2;
+    const double D13 = inv2h * ((double)U[B] - U[F] + Wv[E] - Wv[W]) / 2;
+    const double D23 = inv2h * ((double)Wv[N] - Wv[S] + V[B] - V[F]) / 2;
+    const double V2 = D11 * D11 + D22 * D22 + D33 * D33 + 2 * (D12 * D12 + D13 * D13 + D23 * D23);
+    R[16] += (1 - X) * hCube * nu * (V1 + 2 * V2);
+    R[0] += hCube * WX;
+    R[1] += hCube * WY;
+    R[2] += hCube * WZ;
+    R[3] += hCube / 2 * (PY * WZ - PZ * WY);
+    R[4] += hCube / 2 * (PZ * WX - PX * WZ);
+    R[5] += hCube / 2 * (PX * WY - PY * WX);
+    R[6] += hCube * Lu;
+    R[7] += hCube * Lv;
+    R[8] += hCube * Lw;
+    R[9] += hCube / 3 * (PX * (PY * WY + PZ * WZ) - WX * (PY * PY + PZ * PZ));
+    R[10] += hCube / 3 * (PY * (PX * WX + PZ * WZ) - WY * (PX * PX + PZ * PZ));
+    R[11] += hCube / 3 * (PZ * (PX * WX + PY * WY) - WZ * (PX * PX + PY * PY));
+    R[12] += hCube * (PY * Lw - PZ * Lv);
+    R[13] += hCube * (PZ * Lu - PX * Lw);
+    R[14] += hCube * (PX * Lv - PY * Lu);
+    R[17] += hCube * (WX * Lu + WY * Lv + WZ * Lw);
+    R[18] += hCube * (Lu * Lu + Lv * Lv + Lw * Lw) / 2;
+    R[19] += sqrt(WX * WX + WY * WY + WZ * WZ);
+  }
+  block_reduce<20>(R, 0u, part + 20 * (size_t)b);
+}
+
+template <typename T>
+static int diagnostics_t(Ctx& c, double nu)
+{
+  hipLaunchKernelGGL((k_diagnostics<T>), dim3(c.nb), dim3(NT), 0, c.stream, (const T*)c.f[CUP_U],
+                     (const T*)c.f[CUP_V], (const T*)c.f[CUP_W], (const T*)c.f[CUP_P], (const T*)c.f[CUP_CHI],
+                     c.grid(), nu, c.extent[0] / 2, c.extent[1] / 2, c.extent[2] / 2, c.partials);
+  return check_hip(c, hipGetLastError(), "k_diagnostics");
+}
+
+int launch_diagnostics(Ctx& c, double nu)
+{
+  if (int e = ensure_partials(c, 20 * (size_t)c.nb)) return e;
+  return c.precision == 4 ? diagnostics_t<float>(c, nu) : diagnostics_t<double>(c, nu);
+}
+
+// ----------------------------------------------------------- initial conditions
+
+// KernelIC_taylorGreen (InitialConditions.cpp:32-52): every field cleared, then
+// u = A cos(ax) sin(by) sin(cz), v = B sin(ax) cos(by) sin(cz), B = -A Ly/Lx.
+template <typename T>
+__global__ void k_ic_tgv(T* const* f, GridArgs g, double a, double bb, double cc, double A, double B)
+{
+  const size_t n = (size_t)g.nb * BS3;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const int b = (int)(i / BS3), c = (int)(i % BS3);
+    int bx, by, bz;
+    block_coords(g, b, bx, by, bz);
+    const int ix = c & 7, iy = (c >> 3) & 7, iz = c >> 6;
+    const double x = (bx * BS + ix + 0.5) * g.h, y = (by * BS + iy + 0.5) * g.h, z = (bz * BS + iz + 0.5) * g.h;
+    for (int k = 0; k < 8; ++k) f[k][i] = 0;
+    f[CUP_U][i] = (T)(A * cos(a * x) * sin(bb * y) * sin(cc * z));
+    f[CUP_V][i] = (T)(B * sin(a * x) * cos(bb * y) * sin(cc * z));
+  }
+}
+
+template <typename T>
+static int ic_tgv_t(Ctx& c, double uMax)
+{
+  T** dptrs = nullptr;
+  if (hipMalloc(&dptrs, 8 * sizeof(T*)) != hipSuccess) return set_error(c, CUP_E_NOMEM, "ic: hipMalloc");
+  T* hp[8];
+  for (int k = 0; k < 8; ++k) hp[k] = (T*)c.f[k];
+  hipMemcpyAsync(dptrs, hp, sizeof(hp), hipMemcpyHostToDevice, c.stream);
+  const double a = 2 * M_PI / c.extent[0], bb = 2 * M_PI / c.extent[1], cc = 2 * M_PI / c.extent[2];
+  hipLaunchKernelGGL((k_ic_tgv<T>), ew_grid(c.ncell), dim3(NT), 0, c.stream, dptrs, c.grid(), a, bb, cc, uMax,
+                     -uMax * c.extent[1] / c.extent[0]);
+  int e = check_hip(c, hipGetLastError(), "k_ic_tgv");
+  hipStreamSynchronize(c.stream);
+  hipFree(dptrs);
+  return e;
+}
+
+int launch_ic_taylor_green(Ctx& c, double uMax)
+{
+  return c.precision == 4 ? ic_tgv_t<float>(c, uMax) : ic_tgv_t<double>(c, uMax);
+}
+
+}  // namespace cup

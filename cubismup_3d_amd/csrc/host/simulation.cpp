@@ -1,0 +1,787 @@
+// simulation.cpp -- the reference's Simulation / SimulationData / Operator classes for the
+// periodic path, in host C++ over the C-ABI of include/cup.h, plus the cup_sim_* C entry
+// points the Python bindings load.
+#include "simulation.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <iomanip>
+#include <limits>
+
+namespace cubismup3d {
+
+BCflag string2BCflag(const std::string& s)
+{
+  if (s == "periodic") return periodic;
+  if (s == "dirichlet") return dirichlet;
+  if (s == "wall") return wall;
+  if (s == "freespace" || s == "unbounded") return freespace;
+  throw std::invalid_argument("BC not recognized " + s);
+}
+
+// ------------------------------------------------------------------ arguments
+
+static std::string strip_dashes(const std::string& k)
+{
+  size_t i = 0;
+  while (i < k.size() && k[i] == '-') ++i;
+  return k.substr(i);
+}
+
+ArgumentParser::ArgumentParser(int argc, const char* const* argv)
+{
+  for (int i = 0; i < argc; ++i) {
+    const std::string a = argv[i];
+    if (a.size() < 2 || a[0] != '-' || (a[1] >= '0' && a[1] <= '9') || a[1] == '.') continue;
+    std::string v = "true";
+    if (i + 1 < argc) {
+      const std::string n = argv[i + 1];
+      const bool nextIsKey = n.size() >= 2 && n[0] == '-' && !(n[1] >= '0' && n[1] <= '9') && n[1] != '.';
+      if (!nextIsKey) {
+        v = n;
+        ++i;
+      }
+    }
+    args_[strip_dashes(a)] = v;
+  }
+}
+
+ArgumentParser::Value ArgumentParser::operator()(const std::string& key) const
+{
+  auto it = args_.find(strip_dashes(key));
+  if (it == args_.end()) return Value(false, "");
+  return Value(true, it->second);
+}
+
+int ArgumentParser::Value::asInt(int def) const { return present_ ? (int)std::stod(s_) : def; }
+int ArgumentParser::Value::asInt() const
+{
+  if (!present_) throw std::invalid_argument("mandatory integer argument missing");
+  return (int)std::stod(s_);
+}
+double ArgumentParser::Value::asDouble(double def) const { return present_ ? std::stod(s_) : def; }
+double ArgumentParser::Value::asDouble() const
+{
+  if (!present_) throw std::invalid_argument("mandatory argument missing");
+  return std::stod(s_);
+}
+std::string ArgumentParser::Value::asString(const std::string& def) const { return present_ ? s_ : def; }
+bool ArgumentParser::Value::asBool(bool def) const
+{
+  if (!present_) return def;
+  if (s_ == "true" || s_ == "1" || s_ == "yes") return true;
+  if (s_ == "false" || s_ == "0" || s_ == "no") return false;
+  return std::stod(s_) != 0;
+}
+
+// ---------------------------------------------------------------------- logger
+
+std::stringstream& BufferedLogger::get_stream(const std::string& filename)
+{
+  auto it = files_.find(filename);
+  if (it != files_.end()) {
+    if (++it->second.requests_since_last_flush == AUTO_FLUSH_COUNT) flush_one(it->first, it->second);
+    return it->second.stream;
+  }
+  return files_[filename].stream;
+}
+
+void BufferedLogger::flush_one(const std::string& name, Stream& s)
+{
+  std::ofstream f(dir_ + "/" + name, std::ios::app | std::ios::out);
+  f << s.stream.rdbuf();
+  s.stream.str("");
+  s.stream.clear();
+  s.requests_since_last_flush = 0;
+}
+
+void BufferedLogger::flush()
+{
+  for (auto& p : files_) flush_one(p.first, p.second);
+}
+
+// ------------------------------------------------------------- SimulationData
+
+SimulationData::SimulationData() : logger(std::make_shared<BufferedLogger>()) {}
+
+SimulationData::SimulationData(const SimulationData& o) { *this = o; }
+
+SimulationData& SimulationData::operator=(const SimulationData& o)
+{
+  if (this == &o) return *this;
+  step = o.step; nsteps = o.nsteps; time = o.time; endTime = o.endTime; dt = o.dt;
+  bpdx = o.bpdx; bpdy = o.bpdy; bpdz = o.bpdz; maxextent = o.maxextent; extent = o.extent;
+  bImplicitPenalization = o.bImplicitPenalization; hmin = o.hmin; hmax = o.hmax; hmean = o.hmean;
+  uinf = o.uinf; nu = o.nu; CFL = o.CFL; lambda = o.lambda; DLM = o.DLM;
+  initCond = o.initCond; spectralIC = o.spectralIC; k0 = o.k0; tke0 = o.tke0; icSeed = o.icSeed; icUrms = o.icUrms;
+  uMax_forced = o.uMax_forced; uMax_measured = o.uMax_measured; spectralForcing = o.spectralForcing;
+  turbKinEn_target = o.turbKinEn_target; enInjectionRate = o.enInjectionRate;
+  freqDiagnostics = o.freqDiagnostics; rampup = o.rampup; verbose = o.verbose; muteAll = o.muteAll;
+  path4serialization = o.path4serialization;
+  BCx_flag = o.BCx_flag; BCy_flag = o.BCy_flag; BCz_flag = o.BCz_flag; bUseFourierBC = o.bUseFourierBC;
+  precision = o.precision; decomposition = o.decomposition; scheme = o.scheme; device = o.device;
+  obstacles.clear();
+  for (const auto& s : o.obstacles) obstacles.push_back(std::make_shared<Sphere>(*s));
+  logger = std::make_shared<BufferedLogger>(o.path4serialization);
+  ctx = nullptr;  // a copy never shares the device context
+  return *this;
+}
+
+SimulationData::~SimulationData()
+{
+  pipeline.clear();
+  if (ctx) cup_destroy(ctx);
+  ctx = nullptr;
+}
+
+// SimulationData(MPI_Comm, ArgumentParser&) (SimulationData.cpp:32-172), periodic subset.
+SimulationData::SimulationData(int argc, const char* const* argv) : SimulationData()
+{
+  ArgumentParser parser(argc, argv);
+  bpdx = parser("-bpdx").asInt();
+  bpdy = parser("-bpdy").asInt();
+  bpdz = parser("-bpdz").asInt();
+  extent[0] = parser("extentx").asDouble(1);
+  extent[1] = parser("extenty").asDouble(0);
+  extent[2] = parser("extentz").asDouble(0);
+  nu = parser("-nu").asDouble();
+  initCond = parser("-initCond").asString("zero");
+  spectralIC = parser("-spectralIC").asString("");
+  k0 = parser("-k0").asDouble(10.0);
+  tke0 = parser("-tke0").asDouble(1.0);
+  icSeed = (uint64_t)parser("-icSeed").asDouble(20260101);
+  icUrms = parser("-icUrms").asDouble(1.0);
+  turbKinEn_target = parser("-turbKinEn_target").asDouble(0);
+  enInjectionRate = parser("-energyInjectionRate").asDouble(0);
+  const bool hint = turbKinEn_target > 0 || enInjectionRate > 0;
+  spectralForcing = parser("-spectralForcing").asBool(hint);
+  if (turbKinEn_target > 0 && enInjectionRate > 0)
+    throw std::invalid_argument("either constant energy injection rate or forcing to fixed energy target");
+  freqDiagnostics = parser("-freqDiagnostics").asInt(100);
+  if (parser("-iterativePenalization").asBool(false))
+    throw std::invalid_argument("-iterativePenalization is not supported");
+  bImplicitPenalization = parser("-implicitPenalization").asBool(false);
+  uMax_forced = parser("-uMax_forced").asDouble(0.0);
+  lambda = parser("-lambda").asDouble(1e6);
+  DLM = parser("-use-dlm").asDouble(0);
+  CFL = parser("-CFL").asDouble(.1);
+  uinf[0] = parser("-uinfx").asDouble(0.0);
+  uinf[1] = parser("-uinfy").asDouble(0.0);
+  uinf[2] = parser("-uinfz").asDouble(0.0);
+  verbose = parser("-verbose").asBool(false);
+  muteAll = parser("-muteAll").asBool(false);
+  rampup = parser("-rampup").asInt(100);
+  nsteps = parser("-nsteps").asInt(0);
+  endTime = parser("-tend").asDouble(0);
+  path4serialization = parser("-serialization").asString("./");
+  const std::string BC_x = parser("-BC_x").asString("dirichlet");
+  const std::string BC_y = parser("-BC_y").asString("dirichlet");
+  const std::string BC_z = parser("-BC_z").asString("dirichlet");
+  BCx_flag = string2BCflag(BC_x);
+  BCy_flag = string2BCflag(BC_y);
+  BCz_flag = string2BCflag(BC_z);
+  bUseFourierBC = BCx_flag == periodic && BCy_flag == periodic && BCz_flag == periodic;
+  const std::string prec = parser("-precision").asString("double");
+  precision = (prec == "float" || prec == "single" || prec == "4" || prec == "fp32") ? 4 : 8;
+  const std::string dec = parser("-decomposition").asString(precision == 4 ? "fused" : "plain");
+  decomposition = dec == "fused" ? CUP_FUSED : CUP_PLAIN;
+  device = parser("-device").asInt(0);
+  logger->set_dir(path4serialization);
+  // Sphere from the factory idiom of launch/settingsSphere.sh:19 as flags.
+  if (parser("-sphereL").exists()) {
+    auto s = std::make_shared<Sphere>();
+    s->radius = 0.5 * parser("-sphereL").asDouble();
+    s->position[0] = parser("-xpos").asDouble();
+    s->position[1] = parser("-ypos").asDouble(-1);
+    s->position[2] = parser("-zpos").asDouble(-1);
+    const bool forced = parser("-bForcedInSimFrame").asBool(false);
+    const bool fix = parser("-bFixFrameOfRef").asBool(false);
+    for (int d = 0; d < 3; ++d) {
+      s->bForcedInSimFrame[d] = forced;
+      s->bFixFrameOfRef[d] = fix;
+    }
+    s->transVel[0] = -parser("-xvel").asDouble(0.0);  // Obstacle.cpp:64
+    s->transVel[1] = -parser("-yvel").asDouble(0.0);
+    s->transVel[2] = -parser("-zvel").asDouble(0.0);
+    obstacles.push_back(s);
+  }
+}
+
+// SimulationData::_preprocessArguments (SimulationData.cpp:174-247), one rank.
+void SimulationData::_preprocessArguments()
+{
+  if (bpdx < 1 || bpdy < 1 || bpdz < 1) throw std::invalid_argument("Invalid bpd");
+  const double NFE[3] = {(double)bpdx * 8, (double)bpdy * 8, (double)bpdz * 8};
+  const double maxbpd = std::max({NFE[0], NFE[1], NFE[2]});
+  maxextent = std::max({extent[0], extent[1], extent[2]});
+  if (extent[0] <= 0 || extent[1] <= 0 || extent[2] <= 0) {
+    for (int d = 0; d < 3; ++d) extent[d] = (NFE[d] / maxbpd) * maxextent;
+  } else {
+    // all three extents given: the reference switches to a stretched grid, which the
+    // periodic FFT path does not support unless the extents match the uniform spacing
+    for (int d = 0; d < 3; ++d)
+      if (std::fabs(extent[d] - (NFE[d] / maxbpd) * maxextent) > 1e-12 * maxextent)
+        throw std::invalid_argument("stretched grids are not supported");
+  }
+  if (!(nu >= 0)) throw std::invalid_argument("nu must be >= 0");
+  if (!(CFL > 0)) throw std::invalid_argument("CFL must be > 0");
+  if (BCx_flag != periodic || BCy_flag != periodic || BCz_flag != periodic)
+    throw std::invalid_argument("only fully periodic boxes are supported (-BC_x/y/z periodic)");
+  hmin = hmax = hmean = maxextent / maxbpd;
+  for (auto& s : obstacles) {
+    if (s->position[1] < 0) s->position[1] = extent[1] / 2;  // ObstacleArguments defaults
+    if (s->position[2] < 0) s->position[2] = extent[2] / 2;
+  }
+}
+
+void SimulationData::setCells(int nx, int ny, int nz)
+{
+  if (nx <= 0 || ny <= 0 || nz <= 0 || nx % 8 != 0 || ny % 8 != 0 || nz % 8 != 0)
+    throw std::invalid_argument("Number of cells must be multiple of block size * number of processes.");
+  bpdx = nx / 8;
+  bpdy = ny / 8;
+  bpdz = nz / 8;
+}
+
+// ------------------------------------------------------------------ operators
+
+void Operator::check(int rc, const char* what)
+{
+  if (rc == CUP_OK) return;
+  throw std::runtime_error(std::string(what) + ": " + cup_last_error(sim.ctx));
+}
+
+// InitialConditions::operator() (InitialConditions.cpp:164-230).
+void InitialConditions::operator()(double)
+{
+  if (sim.initCond == "zero") {
+    for (int f = 0; f < 8; ++f) {
+      std::vector<double> z((size_t)sim.bpdx * sim.bpdy * sim.bpdz * 512, 0.0);
+      check(cup_set_field(sim.ctx, f, z.data()), "IC zero");
+    }
+  } else if (sim.initCond == "taylorGreen") {
+    check(cup_ic_taylor_green(sim.ctx, sim.uMax_forced), "IC taylorGreen");
+  } else if (sim.initCond == "HITurbulence") {
+    check(cup_ic_hit(sim.ctx, sim.k0, sim.icUrms, sim.icSeed), "IC HITurbulence");
+    // SpectralIcGenerator::run (SpectralIcGenerator.cpp:126-130): target = IC energy
+    if (sim.spectralForcing && sim.turbKinEn_target <= 0 && sim.enInjectionRate <= 0) {
+      double d[20];
+      check(cup_diagnostics(sim.ctx, sim.nu, d), "IC energy");
+      const double vol = sim.extent[0] * sim.extent[1] * sim.extent[2];
+      sim.turbKinEn_target = d[18] / vol;
+    }
+  } else {
+    throw std::invalid_argument("initCond not supported: " + sim.initCond);
+  }
+}
+
+// CreateObstacles (ObstaclesCreate.cpp:444-540) for static spheres: the frame follows
+// the body (ObstacleVector::updateUinf, ObstacleVector.h:60-74) and the chi is built once.
+void CreateObstacles::operator()(double)
+{
+  if (sim.obstacles.empty()) return;
+  if (sim.obstacles.size() > 1) throw std::invalid_argument("one obstacle at most");
+  auto& s = *sim.obstacles[0];
+  std::array<double, 3> u = {0, 0, 0};
+  for (int d = 0; d < 3; ++d)
+    if (s.bFixFrameOfRef[d]) u[d] = -s.transVel[d];
+  sim.uinf = u;
+  if (created) return;
+  for (int d = 0; d < 3; ++d)
+    if (!s.bForcedInSimFrame[d] || std::fabs(s.transVel[d] + sim.uinf[d]) > 0)
+      throw std::invalid_argument("only bodies forced in the simulation frame and at rest in it are supported");
+  double cm[4];
+  check(cup_chi_from_sphere(sim.ctx, s.position.data(), s.radius, cm), "CreateObstacles");
+  s.centerOfMass = {cm[0], cm[1], cm[2]};
+  s.mass = cm[3];
+  created = true;
+}
+
+void AdvectionDiffusion::operator()(double dt)
+{
+  check(cup_advect_diffuse(sim.ctx, dt, sim.nu, sim.uinf.data(), sim.scheme), "AdvectionDiffusion");
+}
+
+// SpectralForcing::operator() (SpectralForcing.cpp:23-82) with its forcingData.dat rows.
+void SpectralForcing::operator()(double dt)
+{
+  double st[5];
+  check(cup_spectral_forcing(sim.ctx, dt, sim.dt, sim.nu, sim.turbKinEn_target, sim.enInjectionRate, st),
+        "SpectralForcing");
+  const double totalKinEn = st[0], viscousDissip = st[1], largeModesKinEn = st[2];
+  sim.dissipationRate = (totalKinEnPrev - totalKinEn) / dt;
+  sim.actualInjectionRate = 0;
+  if (sim.turbKinEn_target > 0) sim.actualInjectionRate = (sim.turbKinEn_target - totalKinEn) / dt;
+  else if (sim.enInjectionRate > 0) sim.actualInjectionRate = sim.enInjectionRate;
+  if (!sim.muteAll) {
+    std::stringstream& ssF = sim.logger->get_stream("forcingData.dat");
+    const std::string tab("\t");
+    if (sim.step == 0)
+      ssF << "step \t time \t dt \t totalKinEn \t largeModesKinEn \t viscousDissip \t totalDissipRate \t "
+             "injectionRate \t lIntegral\n";
+    ssF << sim.step << tab;
+    ssF.setf(std::ios::scientific);
+    ssF.precision(std::numeric_limits<float>::digits10 + 1);
+    ssF << sim.time << tab << sim.dt << tab << totalKinEn << tab << largeModesKinEn << tab << viscousDissip << tab
+        << sim.dissipationRate << tab << sim.actualInjectionRate << tab << st[3] << "\n";
+  }
+  totalKinEnPrev = st[4] > 0 ? totalKinEn + dt * sim.actualInjectionRate : totalKinEn;
+}
+
+void PressureRHS::operator()(double dt) { check(cup_pressure_rhs(sim.ctx, dt), "PressureRHS"); }
+
+// PressureProjection::operator() (PressureProjection.cpp:107-129).
+void PressureProjection::operator()(double dt)
+{
+  check(cup_poisson_periodic(sim.ctx), "PressureProjection solve");
+  check(cup_pressure_project(sim.ctx, dt), "PressureProjection gradient");
+}
+
+// UpdateObstacles: every degree of freedom forced, so Obstacle::computeVelocities keeps
+// the imposed velocities and zero rotation (Obstacle.cpp:176-263).
+void UpdateObstacles::operator()(double) {}
+
+void Penalization::operator()(double dt)
+{
+  if (sim.obstacles.empty()) return;
+  auto& s = *sim.obstacles[0];
+  double ft[6];
+  check(cup_penalize(sim.ctx, dt, sim.lambda, sim.bImplicitPenalization ? 1 : 0, s.transVel.data(),
+                     s.angVel.data(), ft),
+        "Penalization");
+  s.force = {ft[0], ft[1], ft[2]};
+  s.torque = {ft[3], ft[4], ft[5]};
+}
+
+// ComputeForces (FluidSolidForces.cpp:205-223) and Obstacle::_writeSurfForcesToFile
+// (Obstacle.cpp:518-545).
+void ComputeForces::operator()(double)
+{
+  if (sim.obstacles.empty()) return;
+  auto& s = *sim.obstacles[0];
+  double q[22];
+  check(cup_surface_forces(sim.ctx, sim.nu, s.transVel.data(), s.angVel.data(), q), "ComputeForces");
+  for (int i = 0; i < 22; ++i) s.surfaceQoI[i] = q[i];
+  if (sim.muteAll) return;
+  std::stringstream& ssF = sim.logger->get_stream("forceValues_surface_" + std::to_string(s.obstacleID) + ".dat");
+  const std::string tab("\t");
+  if (sim.step == 0)
+    ssF << "step" << tab << "time" << tab << "mass" << tab << "force_x" << tab << "force_y" << tab << "force_z" << tab
+        << "torque_x" << tab << "torque_y" << tab << "torque_z" << tab << "presF_x" << tab << "presF_y" << tab
+        << "presF_z" << tab << "viscF_x" << tab << "viscF_y" << tab << "viscF_z" << tab << "gamma_x" << tab
+        << "gamma_y" << tab << "gamma_z" << tab << "drag" << tab << "thrust" << std::endl;
+  ssF << sim.step << tab;
+  ssF.setf(std::ios::scientific);
+  ssF.precision(std::numeric_limits<float>::digits10 + 1);
+  ssF << sim.time << tab << s.mass << tab << q[0] << tab << q[1] << tab << q[2] << tab << q[9] << tab << q[10] << tab
+      << q[11] << tab << q[3] << tab << q[4] << tab << q[5] << tab << q[6] << tab << q[7] << tab << q[8] << tab << q[12]
+      << tab << q[13] << tab << q[14] << tab << q[15] << tab << q[16] << std::endl;
+}
+
+// ComputeDissipation::operator() (ComputeDissipation.cpp:119-178) and diagnostics.dat.
+void ComputeDissipation::operator()(double)
+{
+  if (sim.freqDiagnostics == 0 || sim.step % sim.freqDiagnostics) return;
+  double R[20];
+  check(cup_diagnostics(sim.ctx, sim.nu, R), "ComputeDissipation");
+  for (int i = 0; i < 20; ++i) last[i] = R[i];
+  if (sim.muteAll) return;
+  std::stringstream& f = sim.logger->get_stream("diagnostics.dat");
+  if (sim.step == 0)
+    f << "step_id time circ_x circ_y circ_y linImp_x linImp_y linImp_z linMom_x linMom_y linMom_z angImp_x angImp_y "
+         "angImp_z angMom_x angMom_y angMom_z presPow viscPow helicity kineticEn enstrophy"
+      << std::endl;
+  f << sim.step << " " << sim.time;
+  for (int i = 0; i < 20; ++i) f << " " << R[i];
+  f << std::endl;
+}
+
+// ----------------------------------------------------------------- Simulation
+
+Simulation::Simulation(const SimulationData& sd) : sim(sd)
+{
+  sim._preprocessArguments();
+  setupGrid();
+  setupOperators();
+  _ic();
+}
+
+Simulation::Simulation(int argc, const char* const* argv) : sim(argc, argv)
+{
+  sim._preprocessArguments();
+  setupGrid();
+  setupOperators();
+  _ic();
+}
+
+Simulation::~Simulation()
+{
+  sim.pipeline.clear();
+  if (sim.logger) sim.logger->flush();
+}
+
+void Simulation::setupGrid()
+{
+  cup_grid_desc d;
+  d.bpd[0] = sim.bpdx;
+  d.bpd[1] = sim.bpdy;
+  d.bpd[2] = sim.bpdz;
+  d.maxextent = sim.maxextent;
+  d.precision = sim.precision;
+  d.decomposition = sim.decomposition;
+  d.device = sim.device;
+  const int rc = cup_create(&d, &sim.ctx);
+  if (rc != CUP_OK) throw std::runtime_error("cup_create failed (" + std::to_string(rc) + ")");
+}
+
+void Simulation::addObstacle(std::shared_ptr<Sphere> s)
+{
+  s->obstacleID = (int)sim.obstacles.size();
+  sim.obstacles.push_back(std::move(s));
+  setupOperators();
+}
+
+// Simulation::setupOperators (Simulation.cpp:307-401), periodic, non-iterative pipeline.
+void Simulation::setupOperators()
+{
+  sim.pipeline.clear();
+  auto* create = new CreateObstacles(sim);
+  sim.pipeline.emplace_back(create);
+  sim.pipeline.emplace_back(new AdvectionDiffusion(sim));
+  if (sim.spectralForcing) sim.pipeline.emplace_back(new SpectralForcing(sim));
+  sim.pipeline.emplace_back(new PressureRHS(sim));
+  sim.pipeline.emplace_back(new PressureProjection(sim));
+  sim.pipeline.emplace_back(new UpdateObstacles(sim));
+  sim.pipeline.emplace_back(new Penalization(sim));
+  sim.pipeline.emplace_back(new ComputeForces(sim));
+  sim.pipeline.emplace_back(new ComputeDissipation(sim));
+  (*create)(0);
+}
+
+void Simulation::_ic()
+{
+  InitialConditions ic(sim);
+  ic(0);
+}
+
+void Simulation::reset()
+{
+  _ic();
+  sim.step = 0;
+  sim.time = 0;
+  sim.uinf = {0, 0, 0};
+  cup_set_step_state(sim.ctx, 0, 0.0);
+}
+
+// Simulation::calcMaxTimestep (Simulation.cpp:403-424).
+double Simulation::calcMaxTimestep()
+{
+  double dt = 0, umax = 0;
+  const int rc =
+      cup_calc_max_timestep(sim.ctx, sim.CFL, sim.nu, sim.uinf.data(), sim.step, sim.rampup, &dt, &umax);
+  if (rc != CUP_OK) throw std::runtime_error(std::string("calcMaxTimestep: ") + cup_last_error(sim.ctx));
+  sim.uMax_measured = umax;
+  sim.dt = dt;
+  if (sim.DLM > 0) sim.lambda = sim.DLM / sim.dt;
+  return sim.dt;
+}
+
+// Simulation::timestep (Simulation.cpp:593-623).
+bool Simulation::timestep(double dt)
+{
+  for (auto& op : sim.pipeline) (*op)(dt);
+  sim.step++;
+  sim.time += dt;
+  cup_set_step_state(sim.ctx, sim.step, sim.time);
+  if (sim.verbose) printf("%d : %f uInf {%f %f %f}\n", sim.step, sim.time, sim.uinf[0], sim.uinf[1], sim.uinf[2]);
+  if ((sim.endTime > 0 && sim.time > sim.endTime) || (sim.nsteps != 0 && sim.step >= sim.nsteps)) return true;
+  return false;
+}
+
+void Simulation::syncStepState()
+{
+  int step = 0;
+  double time = 0, last = 0, umax = 0, ke = 0;
+  if (cup_step_state(sim.ctx, &step, &time, &last, &umax, &ke) != CUP_OK)
+    throw std::runtime_error(std::string("step state: ") + cup_last_error(sim.ctx));
+  sim.step = step;
+  sim.time = time;
+  sim.dt = last;
+  sim.uMax_measured = umax;
+}
+
+bool Simulation::advance(int n)
+{
+  if (sim.decomposition != CUP_FUSED) {
+    for (int i = 0; i < n; ++i)
+      if (timestep(calcMaxTimestep())) return true;
+    return false;
+  }
+  for (auto& op : sim.pipeline)
+    if (auto* c = dynamic_cast<CreateObstacles*>(op.get())) (*c)(0);
+  cup_step_params p{};
+  p.CFL = sim.CFL;
+  p.nu = sim.nu;
+  for (int d = 0; d < 3; ++d) p.uinf[d] = sim.uinf[d];
+  p.rampup = sim.rampup;
+  p.scheme = sim.scheme;
+  p.spectral_forcing = sim.spectralForcing ? 1 : 0;
+  p.tke_target = sim.turbKinEn_target;
+  p.injection_rate = sim.enInjectionRate;
+  p.fixed_dt = 0;
+  if (!sim.obstacles.empty()) {
+    auto& s = *sim.obstacles[0];
+    p.has_body = 1;
+    p.lambda = sim.lambda;
+    p.implicit_penalization = sim.bImplicitPenalization ? 1 : 0;
+    for (int d = 0; d < 3; ++d) {
+      p.body_vel[d] = s.transVel[d];
+      p.body_angvel[d] = s.angVel[d];
+    }
+  }
+  if (cup_run_steps(sim.ctx, &p, n) != CUP_OK)
+    throw std::runtime_error(std::string("advance: ") + cup_last_error(sim.ctx));
+  syncStepState();
+  return (sim.endTime > 0 && sim.time > sim.endTime) || (sim.nsteps != 0 && sim.step >= sim.nsteps);
+}
+
+// Simulation::run (Simulation.cpp:582-591).
+void Simulation::run()
+{
+  if (sim.decomposition == CUP_FUSED && sim.nsteps > 0 && sim.endTime <= 0 && sim.freqDiagnostics == 0) {
+    advance(sim.nsteps - sim.step);
+    return;
+  }
+  for (;;) {
+    const double dt = calcMaxTimestep();
+    if (timestep(dt)) break;
+  }
+}
+
+}  // namespace cubismup3d
+
+// ------------------------------------------------------------ C entry points
+
+using namespace cubismup3d;
+
+struct cup_sim {
+  std::unique_ptr<Simulation> S;
+  std::unique_ptr<SimulationData> pending;  // before the Simulation is built
+  std::string err;
+};
+
+template <typename F>
+static int guarded(cup_sim* h, F&& f)
+{
+  if (!h) return CUP_E_INVALID;
+  try {
+    f();
+    return CUP_OK;
+  } catch (const std::invalid_argument& e) {
+    h->err = e.what();
+    return CUP_E_INVALID;
+  } catch (const std::exception& e) {
+    h->err = e.what();
+    return CUP_E_DEVICE;
+  }
+}
+
+extern "C" {
+
+int cup_sim_create(int argc, const char* const* argv, cup_sim** out)
+{
+  if (!out) return CUP_E_INVALID;
+  *out = new cup_sim();
+  return guarded(*out, [&] { (*out)->pending.reset(new SimulationData(argc, argv)); });
+}
+
+int cup_sim_add_sphere(cup_sim* h, double radius, const double position[3], const double trans_vel[3],
+                       int forced, int fix_frame)
+{
+  return guarded(h, [&] {
+    if (!(radius > 0)) throw std::invalid_argument("radius must be > 0");
+    auto s = std::make_shared<Sphere>();
+    s->radius = radius;
+    for (int d = 0; d < 3; ++d) {
+      s->position[d] = position[d];
+      s->transVel[d] = trans_vel[d];
+      s->bForcedInSimFrame[d] = forced != 0;
+      s->bFixFrameOfRef[d] = fix_frame != 0;
+    }
+    if (h->S) h->S->addObstacle(s);
+    else if (h->pending) h->pending->obstacles.push_back(s);
+    else throw std::invalid_argument("no simulation");
+  });
+}
+
+int cup_sim_init(cup_sim* h)
+{
+  return guarded(h, [&] {
+    if (h->S) return;
+    if (!h->pending) throw std::invalid_argument("no simulation data");
+    h->S.reset(new Simulation(*h->pending));
+    h->pending.reset();
+  });
+}
+
+int cup_sim_destroy(cup_sim* h)
+{
+  if (!h) return CUP_E_INVALID;
+  delete h;
+  return CUP_OK;
+}
+
+const char* cup_sim_last_error(const cup_sim* h) { return h ? h->err.c_str() : "null simulation"; }
+
+int cup_sim_run(cup_sim* h)
+{
+  return guarded(h, [&] {
+    if (!h->S) throw std::invalid_argument("simulation not initialised");
+    h->S->run();
+  });
+}
+
+int cup_sim_calc_max_timestep(cup_sim* h, double* dt)
+{
+  return guarded(h, [&] {
+    if (!h->S) throw std::invalid_argument("simulation not initialised");
+    const double v = h->S->calcMaxTimestep();
+    if (dt) *dt = v;
+  });
+}
+
+int cup_sim_timestep(cup_sim* h, double dt, int* finished)
+{
+  return guarded(h, [&] {
+    if (!h->S) throw std::invalid_argument("simulation not initialised");
+    h->S->sim.dt = dt;
+    const bool f = h->S->timestep(dt);
+    if (finished) *finished = f ? 1 : 0;
+  });
+}
+
+int cup_sim_advance(cup_sim* h, int nsteps, int* finished)
+{
+  return guarded(h, [&] {
+    if (!h->S) throw std::invalid_argument("simulation not initialised");
+    const bool f = h->S->advance(nsteps);
+    if (finished) *finished = f ? 1 : 0;
+  });
+}
+
+int cup_sim_context(cup_sim* h, cup_ctx** ctx)
+{
+  return guarded(h, [&] {
+    if (!h->S) throw std::invalid_argument("simulation not initialised");
+    *ctx = h->S->sim.ctx;
+  });
+}
+
+// Scalar members of SimulationData by name.
+static double* sim_field(SimulationData& s, const std::string& n)
+{
+  static double tmp;
+  if (n == "time") return &s.time;
+  if (n == "dt") return &s.dt;
+  if (n == "endTime") return &s.endTime;
+  if (n == "nu") return &s.nu;
+  if (n == "CFL") return &s.CFL;
+  if (n == "lambda") return &s.lambda;
+  if (n == "uinf_x") return &s.uinf[0];
+  if (n == "uinf_y") return &s.uinf[1];
+  if (n == "uinf_z") return &s.uinf[2];
+  if (n == "extent_x") return &s.extent[0];
+  if (n == "extent_y") return &s.extent[1];
+  if (n == "extent_z") return &s.extent[2];
+  if (n == "h") return &s.hmean;
+  if (n == "uMax_measured") return &s.uMax_measured;
+  if (n == "turbKinEn_target") return &s.turbKinEn_target;
+  if (n == "dissipationRate") return &s.dissipationRate;
+  if (n == "actualInjectionRate") return &s.actualInjectionRate;
+  (void)tmp;
+  return nullptr;
+}
+
+static int* sim_ifield(SimulationData& s, const std::string& n)
+{
+  if (n == "step") return &s.step;
+  if (n == "nsteps") return &s.nsteps;
+  if (n == "rampup") return &s.rampup;
+  if (n == "freqDiagnostics") return &s.freqDiagnostics;
+  if (n == "bpdx") return &s.bpdx;
+  if (n == "bpdy") return &s.bpdy;
+  if (n == "bpdz") return &s.bpdz;
+  if (n == "precision") return &s.precision;
+  if (n == "decomposition") return &s.decomposition;
+  if (n == "scheme") return &s.scheme;
+  return nullptr;
+}
+
+int cup_sim_get(cup_sim* h, const char* name, double* v)
+{
+  return guarded(h, [&] {
+    SimulationData* s = h->S ? &h->S->sim : h->pending.get();
+    if (!s || !name || !v) throw std::invalid_argument("bad argument");
+    if (double* p = sim_field(*s, name)) *v = *p;
+    else if (int* q = sim_ifield(*s, name)) *v = *q;
+    else throw std::invalid_argument(std::string("unknown member ") + name);
+  });
+}
+
+int cup_sim_set(cup_sim* h, const char* name, double v)
+{
+  return guarded(h, [&] {
+    SimulationData* s = h->S ? &h->S->sim : h->pending.get();
+    if (!s || !name) throw std::invalid_argument("bad argument");
+    const std::string n = name;
+    if (h->S && (n == "bpdx" || n == "bpdy" || n == "bpdz" || n == "precision" || n == "decomposition"))
+      throw std::invalid_argument(n + " is fixed once the simulation exists");
+    if (double* p = sim_field(*s, n)) *p = v;
+    else if (int* q = sim_ifield(*s, n)) *q = (int)v;
+    else throw std::invalid_argument("unknown member " + n);
+    if (h->S && n == "step") cup_set_step_state(s->ctx, s->step, s->time);
+    if (h->S && n == "time") cup_set_step_state(s->ctx, s->step, s->time);
+  });
+}
+
+int cup_sim_set_cells(cup_sim* h, int nx, int ny, int nz)
+{
+  return guarded(h, [&] {
+    if (!h->pending) throw std::invalid_argument("cells are fixed once the simulation exists");
+    h->pending->setCells(nx, ny, nz);
+  });
+}
+
+// Obstacle state: out = CoM[3], mass, force[3], torque[3], surface QoIs[22] (34 doubles).
+int cup_sim_obstacle(cup_sim* h, int idx, double out[34])
+{
+  return guarded(h, [&] {
+    if (!h->S || idx < 0 || idx >= (int)h->S->sim.obstacles.size()) throw std::invalid_argument("no such obstacle");
+    const Sphere& s = *h->S->sim.obstacles[idx];
+    for (int d = 0; d < 3; ++d) out[d] = s.centerOfMass[d];
+    out[3] = s.mass;
+    for (int d = 0; d < 3; ++d) { out[4 + d] = s.force[d]; out[7 + d] = s.torque[d]; }
+    for (int i = 0; i < 22; ++i) out[10 + i] = s.surfaceQoI[i];
+  });
+}
+
+int cup_sim_diagnostics(cup_sim* h, double out[20])
+{
+  return guarded(h, [&] {
+    if (!h->S) throw std::invalid_argument("simulation not initialised");
+    if (cup_diagnostics(h->S->sim.ctx, h->S->sim.nu, out) != CUP_OK)
+      throw std::runtime_error(cup_last_error(h->S->sim.ctx));
+  });
+}
+
+int cup_sim_flush_logs(cup_sim* h)
+{
+  return guarded(h, [&] {
+    if (h->S && h->S->sim.logger) h->S->sim.logger->flush();
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,269 @@
+// simulation.h -- host C++ layer of cubismup_3d_amd: the reference's
+// Simulation / SimulationData / Operator API (source/Simulation.h:24-71,
+// source/SimulationData.h:46-179, source/operators/Operator.h:16-177) for the periodic
+// per-timestep path, each operator calling the C-ABI of include/cup.h, which launches the
+// HIP kernels. One Simulation owns one device context; calls are not re-entrant.
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../../include/cup.h"
+
+namespace cubismup3d {
+
+using Real = double;
+
+enum BCflag { dirichlet, periodic, wall, freespace };  // Definitions.h:74
+BCflag string2BCflag(const std::string& s);
+
+// "-name value" command-line arguments (the subset of cubism::ArgumentParser used by
+// SimulationData's constructor, SimulationData.cpp:32-172). Keys are matched with or
+// without their leading dash.
+class ArgumentParser
+{
+ public:
+  class Value
+  {
+   public:
+    Value(bool present, std::string s) : present_(present), s_(std::move(s)) {}
+    int asInt(int def) const;
+    int asInt() const;
+    double asDouble(double def) const;
+    double asDouble() const;
+    std::string asString(const std::string& def) const;
+    bool asBool(bool def) const;
+    bool exists() const { return present_; }
+
+   private:
+    bool present_;
+    std::string s_;
+  };
+  ArgumentParser(int argc, const char* const* argv);
+  Value operator()(const std::string& key) const;
+
+ private:
+  std::map<std::string, std::string> args_;
+};
+
+// BufferedLogger (utils/BufferedLogger.cpp:18-76): per-file string streams appended to
+// disk every AUTO_FLUSH_COUNT requests and at destruction.
+class BufferedLogger
+{
+ public:
+  static constexpr int AUTO_FLUSH_COUNT = 100;
+  explicit BufferedLogger(std::string dir = ".") : dir_(std::move(dir)) {}
+  ~BufferedLogger() { flush(); }
+  std::stringstream& get_stream(const std::string& filename);
+  void flush();
+  void set_dir(const std::string& d) { dir_ = d; }
+
+ private:
+  struct Stream {
+    std::stringstream stream;
+    int requests_since_last_flush = 0;
+  };
+  std::string dir_;
+  std::map<std::string, Stream> files_;
+  void flush_one(const std::string& name, Stream& s);
+};
+
+// A sphere (obstacles/Sphere.h, obstacles/Obstacle.cpp:24-151 arguments). Only forced
+// motion is supported: velocities stay at their imposed values
+// (Obstacle::computeVelocities with every degree of freedom forced, Obstacle.cpp:153-264).
+struct Sphere {
+  double radius = 0;
+  std::array<double, 3> position = {0, 0, 0};
+  std::array<bool, 3> bForcedInSimFrame = {false, false, false};
+  std::array<bool, 3> bFixFrameOfRef = {false, false, false};
+  std::array<double, 3> transVel = {0, 0, 0};
+  std::array<double, 3> angVel = {0, 0, 0};
+  std::array<double, 3> centerOfMass = {0, 0, 0};
+  double mass = 0;
+  std::array<double, 3> force = {0, 0, 0}, torque = {0, 0, 0};  // penalization integrals
+  std::array<double, 22> surfaceQoI = {};                        // ObstacleBlock::sumQoI order
+  int obstacleID = 0;
+};
+
+class Operator;
+
+struct SimulationData {
+  // simulation status (SimulationData.h:76-79)
+  int step = 0, nsteps = 0;
+  double time = 0, endTime = 0, dt = 0;
+  // grid (SimulationData.h:86-94)
+  int bpdx = -1, bpdy = -1, bpdz = -1;
+  Real maxextent = 1;
+  std::array<Real, 3> extent = {{1, 0, 0}};
+  bool bImplicitPenalization = false;
+  Real hmin = 0, hmax = 0, hmean = 0;
+  // flow (SimulationData.h:97-98)
+  std::array<Real, 3> uinf = {{0, 0, 0}};
+  double nu = 0, CFL = 0, lambda = -1, DLM = 1;
+  // initial conditions (SimulationData.h:101-106)
+  std::string initCond = "zero";
+  std::string spectralIC = "";
+  double k0 = 0, tke0 = 0;
+  uint64_t icSeed = 20260101;
+  double icUrms = 1.0;
+  // forcing (SimulationData.h:109-115)
+  Real uMax_forced = 0, uMax_measured = 0;
+  bool spectralForcing = false;
+  double turbKinEn_target = 0, enInjectionRate = 0, dissipationRate = 0, actualInjectionRate = 0;
+  // settings (SimulationData.h:141-147)
+  int freqDiagnostics = 0;
+  int rampup = 100;
+  bool verbose = false, muteAll = false;
+  std::string path4serialization = "./";
+  BCflag BCx_flag = dirichlet, BCy_flag = dirichlet, BCz_flag = dirichlet;
+  bool bUseFourierBC = false;
+  // device-specific: context precision (4/8), step decomposition, advection scheme
+  int precision = 8;
+  int decomposition = CUP_PLAIN;
+  int scheme = CUP_CENTRAL_EULER;
+  int device = 0;
+
+  // runtime objects
+  cup_ctx* ctx = nullptr;
+  std::vector<std::shared_ptr<Sphere>> obstacles;
+  std::vector<std::unique_ptr<Operator>> pipeline;
+  std::shared_ptr<BufferedLogger> logger;
+
+  SimulationData();
+  SimulationData(int argc, const char* const* argv);
+  SimulationData(const SimulationData&);
+  SimulationData& operator=(const SimulationData&);
+  ~SimulationData();
+  void _preprocessArguments();
+  void setCells(int nx, int ny, int nz);
+  Real uniformH() const { return hmean; }
+};
+
+class Operator
+{
+ protected:
+  SimulationData& sim;
+  void check(int rc, const char* what);
+
+ public:
+  explicit Operator(SimulationData& s) : sim(s) {}
+  virtual ~Operator() = default;
+  virtual void operator()(double dt) = 0;
+  virtual std::string getName() = 0;
+};
+
+class InitialConditions : public Operator
+{
+ public:
+  using Operator::Operator;
+  void operator()(double dt) override;
+  std::string getName() override { return "IC"; }
+};
+
+class CreateObstacles : public Operator
+{
+ public:
+  using Operator::Operator;
+  void operator()(double dt) override;
+  std::string getName() override { return "CreateObstacles"; }
+  bool created = false;
+};
+
+class AdvectionDiffusion : public Operator
+{
+ public:
+  using Operator::Operator;
+  void operator()(double dt) override;
+  std::string getName() override { return "AdvectionDiffusion"; }
+};
+
+class SpectralForcing : public Operator
+{
+ public:
+  using Operator::Operator;
+  void operator()(double dt) override;
+  std::string getName() override { return "SpectralForcing"; }
+  double totalKinEnPrev = 0;
+};
+
+class PressureRHS : public Operator
+{
+ public:
+  using Operator::Operator;
+  void operator()(double dt) override;
+  std::string getName() override { return "PressureRHS"; }
+};
+
+class PressureProjection : public Operator
+{
+ public:
+  using Operator::Operator;
+  void operator()(double dt) override;
+  std::string getName() override { return "PressureProjection"; }
+};
+
+class UpdateObstacles : public Operator
+{
+ public:
+  using Operator::Operator;
+  void operator()(double dt) override;
+  std::string getName() override { return "UpdateObstacles"; }
+};
+
+class Penalization : public Operator
+{
+ public:
+  using Operator::Operator;
+  void operator()(double dt) override;
+  std::string getName() override { return "Penalization"; }
+};
+
+class ComputeForces : public Operator
+{
+ public:
+  using Operator::Operator;
+  void operator()(double dt) override;
+  std::string getName() override { return "ComputeForces"; }
+};
+
+class ComputeDissipation : public Operator
+{
+ public:
+  using Operator::Operator;
+  void operator()(double dt) override;
+  std::string getName() override { return "ComputeDissipation"; }
+  std::array<double, 20> last = {};
+};
+
+class Simulation
+{
+ public:
+  SimulationData sim;
+
+  explicit Simulation(const SimulationData& sd);
+  Simulation(int argc, const char* const* argv);
+  virtual ~Simulation();
+
+  void addObstacle(std::shared_ptr<Sphere> s);
+  void setupOperators();
+  virtual void run();
+  double calcMaxTimestep();
+  bool timestep(double dt);
+  // The FUSED decomposition's loop: dt, step and time stay on the device; returns true
+  // when the run is finished.
+  bool advance(int nsteps);
+  void reset();
+
+ private:
+  void setupGrid();
+  void _ic();
+  void syncStepState();
+};
+
+}  // namespace cubismup3d

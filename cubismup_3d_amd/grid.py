@@ -1,0 +1,208 @@
+"""Operator-level access to the device block store (the C-ABI of include/cup.h).
+
+``Grid`` owns one device context: a periodic box of bpd[0] x bpd[1] x bpd[2] blocks of
+8^3 cells. Fields cross the boundary as numpy float64 arrays indexed [z][y][x]
+(x fastest); on the device they live block-major in the context precision.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+BS = 8
+
+
+def to_blocks(g: np.ndarray) -> np.ndarray:
+    """[Z][Y][X] field -> block-major [block][z][y][x], block = bx + nbx*(by + nby*bz)."""
+    Z, Y, X = g.shape
+    nbz, nby, nbx = Z // BS, Y // BS, X // BS
+    return np.ascontiguousarray(
+        g.reshape(nbz, BS, nby, BS, nbx, BS).transpose(0, 2, 4, 1, 3, 5).reshape(-1), dtype=np.float64)
+
+
+def from_blocks(b: np.ndarray, shape) -> np.ndarray:
+    Z, Y, X = shape
+    nbz, nby, nbx = Z // BS, Y // BS, X // BS
+    return np.ascontiguousarray(
+        b.reshape(nbz, nby, nbx, BS, BS, BS).transpose(0, 3, 1, 4, 2, 5).reshape(Z, Y, X))
+
+
+class Grid:
+    """A device context of the per-timestep fluid update (cup_create / cup_destroy)."""
+
+    def __init__(self, bpd, maxextent=2 * np.pi, precision=8, decomposition="plain", device=0):
+        self.bpd = tuple(int(b) for b in bpd)
+        d = L.GridDesc()
+        for a in range(3):
+            d.bpd[a] = self.bpd[a]
+        d.maxextent = float(maxextent)
+        d.precision = int(precision)
+        d.decomposition = L.CUP_FUSED if decomposition == "fused" else L.CUP_PLAIN
+        d.device = int(device)
+        self.precision = int(precision)
+        self.decomposition = decomposition
+        h = C.c_void_p()
+        rc = L.lib().cup_create(C.byref(d), C.byref(h))
+        if rc != L.CUP_OK:
+            if rc == L.CUP_E_INVALID:
+                raise ValueError(f"invalid grid {bpd} / precision {precision}")
+            raise L.CupError(rc, "cup_create failed")
+        self._h = h
+        cells = (C.c_int * 3)()
+        hh = C.c_double()
+        ext = (C.c_double * 3)()
+        L.lib().cup_get_geometry(self._h, cells, C.byref(hh), ext)
+        self.cells = tuple(cells)
+        self.shape = (self.cells[2], self.cells[1], self.cells[0])
+        self.h = hh.value
+        self.extent = np.array(list(ext))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            L.lib().cup_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _c(self, rc):
+        L.check(rc, ctx=self._h)
+
+    # ----------------------------------------------------------------- fields
+    def set(self, name: str, g: np.ndarray):
+        g = np.asarray(g, dtype=np.float64)
+        if g.shape != self.shape:
+            raise ValueError(f"field shape {g.shape} != grid {self.shape}")
+        b = to_blocks(g)
+        self._c(L.lib().cup_set_field(self._h, L.FIELD_IDS[name], L.dptr(b)))
+
+    def get(self, name: str) -> np.ndarray:
+        b = np.empty(int(np.prod(self.shape)), dtype=np.float64)
+        self._c(L.lib().cup_get_field(self._h, L.FIELD_IDS[name], L.dptr(b)))
+        return from_blocks(b, self.shape)
+
+    def set_velocity(self, u, v, w):
+        self.set("u", u); self.set("v", v); self.set("w", w)
+
+    def velocity(self):
+        return self.get("u"), self.get("v"), self.get("w")
+
+    def device_ptr(self, name: str) -> int:
+        p = C.c_void_p()
+        self._c(L.lib().cup_field_device_ptr(self._h, L.FIELD_IDS[name], C.byref(p)))
+        return p.value
+
+    def synchronize(self):
+        self._c(L.lib().cup_synchronize(self._h))
+
+    def stream(self) -> int:
+        s = C.c_void_p()
+        self._c(L.lib().cup_stream(self._h, C.byref(s)))
+        return s.value or 0
+
+    # -------------------------------------------------------------- operators
+    def lab_shift(self, g, dx, dy, dz):
+        self._c(L.lib().cup_lab_shift(self._h, int(g), int(dx), int(dy), int(dz)))
+
+    def max_velocity(self, uinf=(0, 0, 0)) -> float:
+        m = C.c_double()
+        self._c(L.lib().cup_max_velocity(self._h, L.vec3(uinf), C.byref(m)))
+        return m.value
+
+    def calc_max_timestep(self, CFL, nu, uinf=(0, 0, 0), step=0, rampup=0):
+        dt, um = C.c_double(), C.c_double()
+        self._c(L.lib().cup_calc_max_timestep(self._h, float(CFL), float(nu), L.vec3(uinf), int(step),
+                                              int(rampup), C.byref(dt), C.byref(um)))
+        return dt.value, um.value
+
+    def advect_diffuse(self, dt, nu, uinf=(0, 0, 0), scheme=L.CUP_CENTRAL_EULER):
+        self._c(L.lib().cup_advect_diffuse(self._h, float(dt), float(nu), L.vec3(uinf), int(scheme)))
+
+    def pressure_rhs(self, dt):
+        self._c(L.lib().cup_pressure_rhs(self._h, float(dt)))
+
+    def poisson_periodic(self):
+        self._c(L.lib().cup_poisson_periodic(self._h))
+
+    def pressure_project(self, dt):
+        self._c(L.lib().cup_pressure_project(self._h, float(dt)))
+
+    def diagnostics(self, nu) -> np.ndarray:
+        out = np.zeros(20)
+        self._c(L.lib().cup_diagnostics(self._h, float(nu), L.dptr(out)))
+        return out
+
+    def ic_taylor_green(self, uMax):
+        self._c(L.lib().cup_ic_taylor_green(self._h, float(uMax)))
+
+    def ic_hit(self, k0=4.0, u_rms=1.0, seed=20260101):
+        self._c(L.lib().cup_ic_hit(self._h, float(k0), float(u_rms), int(seed)))
+
+    def spectral_forcing(self, dt, nu, tke_target=0.0, injection_rate=0.0, sim_dt=None):
+        st = np.zeros(5)
+        self._c(L.lib().cup_spectral_forcing(self._h, float(dt), float(dt if sim_dt is None else sim_dt),
+                                             float(nu), float(tke_target), float(injection_rate), L.dptr(st)))
+        return dict(tke=st[0], eps=st[1], tke_filtered=st[2], l_integral=st[3], fac=st[4])
+
+    def chi_from_sphere(self, center, radius):
+        out = np.zeros(4)
+        self._c(L.lib().cup_chi_from_sphere(self._h, L.vec3(center), float(radius), L.dptr(out)))
+        return dict(com=out[:3].copy(), mass=out[3])
+
+    def penalize(self, dt, lam, implicit, trans_vel, ang_vel=(0, 0, 0)) -> np.ndarray:
+        ft = np.zeros(6)
+        self._c(L.lib().cup_penalize(self._h, float(dt), float(lam), int(bool(implicit)), L.vec3(trans_vel),
+                                     L.vec3(ang_vel), L.dptr(ft)))
+        return ft
+
+    def surface_forces(self, nu, trans_vel, ang_vel=(0, 0, 0)) -> np.ndarray:
+        q = np.zeros(22)
+        self._c(L.lib().cup_surface_forces(self._h, float(nu), L.vec3(trans_vel), L.vec3(ang_vel), L.dptr(q)))
+        return q
+
+    def timestep(self, dt, nu, uinf=(0, 0, 0), spectral_forcing=False, tke_target=0.0, sphere=None):
+        """One step through the operator entry points in the reference pipeline order."""
+        out = {}
+        self.advect_diffuse(dt, nu, uinf)
+        if spectral_forcing:
+            out["forcing"] = self.spectral_forcing(dt, nu, tke_target=tke_target)
+        self.pressure_rhs(dt)
+        self.poisson_periodic()
+        self.pressure_project(dt)
+        if sphere is not None:
+            out["penal_ft"] = self.penalize(dt, sphere["lambda_"], sphere["implicit"], sphere["trans_vel"],
+                                            sphere.get("ang_vel", (0, 0, 0)))
+            out["surface"] = self.surface_forces(nu, sphere["trans_vel"], sphere.get("ang_vel", (0, 0, 0)))
+        return out
+
+    def run_steps(self, nsteps, CFL=0.3, nu=0.0, uinf=(0, 0, 0), rampup=0, fixed_dt=0.0,
+                  spectral_forcing=False, tke_target=0.0, injection_rate=0.0, sphere=None,
+                  scheme=L.CUP_CENTRAL_EULER):
+        p = L.StepParams()
+        p.CFL = float(CFL); p.nu = float(nu)
+        for a in range(3):
+            p.uinf[a] = float(uinf[a])
+        p.rampup = int(rampup); p.scheme = int(scheme)
+        p.spectral_forcing = int(bool(spectral_forcing)); p.tke_target = float(tke_target)
+        p.injection_rate = float(injection_rate); p.fixed_dt = float(fixed_dt)
+        if sphere is not None:
+            p.has_body = 1
+            p.lambda_ = float(sphere["lambda_"]); p.implicit_penalization = int(bool(sphere["implicit"]))
+            for a in range(3):
+                p.body_vel[a] = float(sphere["trans_vel"][a])
+                p.body_angvel[a] = float(sphere.get("ang_vel", (0, 0, 0))[a])
+        self._c(L.lib().cup_run_steps(self._h, C.byref(p), int(nsteps)))
+
+    def step_state(self):
+        s = C.c_int(); t, dt, um, ke = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+        self._c(L.lib().cup_step_state(self._h, C.byref(s), C.byref(t), C.byref(dt), C.byref(um), C.byref(ke)))
+        return dict(step=s.value, time=t.value, last_dt=dt.value, umax=um.value, ke=ke.value)
+
+    def set_step_state(self, step, time):
+        self._c(L.lib().cup_set_step_state(self._h, int(step), float(time)))

@@ -1,0 +1,147 @@
+"""Python surface of the reference's pybind11 module ``cubismup3d``
+(source/bindings/Simulation.cpp:59-94, source/bindings/Obstacles.cpp:79-128) over the
+Simulation / SimulationData host C++ classes of libcup.so.
+
+    SD = SimulationData(cells=[64, 64, 64], CFL=0.1, uinf=[0.1, 0, 0])
+    SD.nsteps = 10
+    S = Simulation(SD)
+    S.add_obstacle(Sphere(position=[0.3, 0.4, 0.5], radius=0.1))
+    S.run()
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+from . import _lib as L
+from .grid import Grid, from_blocks
+
+
+class Sphere:
+    """Sphere arguments (bindings/Obstacles.cpp:79-128): forced in the simulation frame
+    with velocity -xvel, the frame fixed to it (the reference's sphere-in-flow idiom)."""
+
+    def __init__(self, position, radius, xvel=0.0, bForcedInSimFrame=True, bFixFrameOfRef=True):
+        if not radius > 0:
+            raise ValueError("radius must be > 0")
+        self.position = [float(x) for x in position]
+        self.radius = float(radius)
+        self.trans_vel = [-float(xvel), 0.0, 0.0]
+        self.forced = bool(bForcedInSimFrame)
+        self.fix_frame = bool(bFixFrameOfRef)
+
+
+class SimulationData:
+    """SimulationData with the reference's command-line vocabulary
+    (SimulationData.cpp:32-172). ``argv`` takes precedence over keyword settings."""
+
+    _INT = {"nsteps", "rampup", "freqDiagnostics", "bpdx", "bpdy", "bpdz"}
+
+    def __init__(self, cells=None, CFL=None, uinf=(0.0, 0.0, 0.0), argv=None, **kw):
+        if argv is None:
+            if cells is None:
+                raise TypeError("cells are required")
+            if CFL is None:
+                raise TypeError("CFL is required")
+            cells = [int(c) for c in cells]
+            if len(cells) != 3 or any(c <= 0 or c % 8 for c in cells):
+                raise ValueError("Number of cells must be multiple of block size.")
+            argv = ["-bpdx", str(cells[0] // 8), "-bpdy", str(cells[1] // 8), "-bpdz", str(cells[2] // 8),
+                    "-BC_x", "periodic", "-BC_y", "periodic", "-BC_z", "periodic",
+                    "-CFL", repr(float(CFL)), "-nu", repr(float(kw.pop("nu", 0.0))),
+                    "-uinfx", repr(float(uinf[0])), "-uinfy", repr(float(uinf[1])),
+                    "-uinfz", repr(float(uinf[2])), "-extentx", repr(float(kw.pop("extent", 2 * math.pi))),
+                    "-freqDiagnostics", str(kw.pop("freqDiagnostics", 0)), "-muteAll",
+                    str(int(kw.pop("muteAll", True)))]
+            for k, v in kw.items():
+                argv += [f"-{k}", str(v)]
+        self.argv = [str(a) for a in argv]
+        arr = (C.c_char_p * len(self.argv))(*[a.encode() for a in self.argv])
+        h = C.c_void_p()
+        rc = L.lib().cup_sim_create(len(self.argv), arr, C.byref(h))
+        self._h = h
+        L.check(rc, sim=self._h)
+        self._owned = True
+
+    def __getattr__(self, name):
+        if name.startswith("_"):
+            raise AttributeError(name)
+        v = C.c_double()
+        rc = L.lib().cup_sim_get(self.__dict__["_h"], name.encode(), C.byref(v))
+        if rc != L.CUP_OK:
+            raise AttributeError(name)
+        return int(v.value) if name in self._INT or name == "step" else v.value
+
+    def __setattr__(self, name, value):
+        if name.startswith("_") or name == "argv":
+            object.__setattr__(self, name, value)
+            return
+        L.check(L.lib().cup_sim_set(self._h, name.encode(), float(value)), sim=self._h)
+
+    def setCells(self, nx, ny, nz):
+        L.check(L.lib().cup_sim_set_cells(self._h, int(nx), int(ny), int(nz)), sim=self._h)
+
+    def __del__(self):
+        if self.__dict__.get("_owned") and self.__dict__.get("_h"):
+            L.lib().cup_sim_destroy(self._h)
+            object.__setattr__(self, "_h", None)
+
+
+class Simulation:
+    """Simulation(SD): grid, operator pipeline and initial condition on the device."""
+
+    def __init__(self, SD: SimulationData):
+        if not isinstance(SD, SimulationData):
+            raise TypeError("Simulation needs a SimulationData")
+        self.sim = SD
+        self._h = SD._h
+        L.check(L.lib().cup_sim_init(self._h), sim=self._h)
+        ctx = C.c_void_p()
+        L.check(L.lib().cup_sim_context(self._h, C.byref(ctx)), sim=self._h)
+        self._ctx = ctx
+        bpd = (SD.bpdx, SD.bpdy, SD.bpdz)
+        self.shape = (8 * bpd[2], 8 * bpd[1], 8 * bpd[0])
+
+    def add_obstacle(self, s: Sphere):
+        L.check(L.lib().cup_sim_add_sphere(self._h, s.radius, L.vec3(s.position), L.vec3(s.trans_vel),
+                                           int(s.forced), int(s.fix_frame)), sim=self._h)
+
+    def run(self):
+        L.check(L.lib().cup_sim_run(self._h), sim=self._h)
+
+    def calcMaxTimestep(self) -> float:
+        dt = C.c_double()
+        L.check(L.lib().cup_sim_calc_max_timestep(self._h, C.byref(dt)), sim=self._h)
+        return dt.value
+
+    def timestep(self, dt) -> bool:
+        f = C.c_int()
+        L.check(L.lib().cup_sim_timestep(self._h, float(dt), C.byref(f)), sim=self._h)
+        return bool(f.value)
+
+    def advance(self, nsteps) -> bool:
+        f = C.c_int()
+        L.check(L.lib().cup_sim_advance(self._h, int(nsteps), C.byref(f)), sim=self._h)
+        return bool(f.value)
+
+    def field(self, name):
+        import numpy as np
+        b = np.empty(int(np.prod(self.shape)), dtype=np.float64)
+        L.check(L.lib().cup_get_field(self._ctx, L.FIELD_IDS[name], L.dptr(b)), ctx=self._ctx)
+        return from_blocks(b, self.shape)
+
+    def obstacle(self, idx=0):
+        import numpy as np
+        o = np.zeros(34)
+        L.check(L.lib().cup_sim_obstacle(self._h, int(idx), L.dptr(o)), sim=self._h)
+        return dict(com=o[0:3].copy(), mass=o[3], force=o[4:7].copy(), torque=o[7:10].copy(),
+                    surface=o[10:32].copy())
+
+    def diagnostics(self):
+        import numpy as np
+        o = np.zeros(20)
+        L.check(L.lib().cup_sim_diagnostics(self._h, L.dptr(o)), sim=self._h)
+        return o
+
+    def flush_logs(self):
+        L.check(L.lib().cup_sim_flush_logs(self._h), sim=self._h)

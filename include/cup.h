@@ -1,0 +1,207 @@
+/*
+ * cup.h -- the C-ABI of cubismup_3d_amd: the per-timestep periodic fluid update of
+ * CubismUP_3D on a device-resident grid of 8^3-cell blocks, for AMD Instinct MI355X
+ * (gfx950). Plain pointers and sizes only; every entry point returns 0 on success or a
+ * negative CUP_E* code, and cup_last_error(ctx) holds the message (no exception crosses
+ * this boundary).
+ *
+ * Each entry point names the reference code it replaces (paths relative to the
+ * CubismUP_3D source tree). Work is stream-ordered on the context's HIP stream; scalar
+ * results (maxima, integrals) are returned synchronously as doubles.
+ *
+ * Host field format: one double per cell in block-major order
+ * [block][iz][iy][ix], block = bx + bpdx*(by + bpdy*bz) -- the reference's
+ * FluidBlock::data[z][y][x] (source/Definitions.h:115) walked over the grid's blocks.
+ */
+#ifndef CUP_H
+#define CUP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CUP_OK 0
+#define CUP_E_INVALID -1   /* bad argument (the reference throws std::invalid_argument / aborts) */
+#define CUP_E_DEVICE -2    /* HIP runtime error */
+#define CUP_E_STATE -3     /* call out of order (e.g. no obstacle created) */
+#define CUP_E_NOMEM -4
+
+/* Field ids, same order as the reference's FluidElement (Definitions.h:39). */
+enum cup_field {
+  CUP_CHI = 0, CUP_U = 1, CUP_V = 2, CUP_W = 3, CUP_P = 4, CUP_TMPU = 5, CUP_TMPV = 6, CUP_TMPW = 7
+};
+
+/* Decomposition of one step. PLAIN follows the reference's operator decomposition
+ * kernel by kernel (AdvectionDiffusion.cpp:402-428 kernel + copy, PressureRHS.cpp:273-346
+ * memset + kernel, PoissonSolverPeriodic.cpp:129-146 r2c / solve / c2r as separate passes,
+ * PoissonSolver.cpp:116-130 copy, PressureProjection.cpp:107-129 gradient). FUSED keeps
+ * the arithmetic but fuses passes (RHS into the first FFT pass, the spectral division
+ * into the z pass, reductions into the gradient epilogue, dt kept on the device). */
+enum cup_decomposition { CUP_PLAIN = 0, CUP_FUSED = 1 };
+
+/* Advection scheme. CENTRAL_EULER is the reference's (AdvectionDiffusion.cpp:106-145). */
+enum cup_scheme { CUP_CENTRAL_EULER = 0, CUP_UPWIND3_EULER = 1, CUP_UPWIND3_RK2 = 2 };
+
+typedef struct cup_grid_desc {
+  int bpd[3];          /* blocks per dimension (SimulationData::bpdx/y/z, SimulationData.h:88) */
+  double maxextent;    /* longest side; h = maxextent / max cells (SimulationData.cpp:184-195) */
+  int precision;       /* 4: FP32 bulk fields, 8: FP64 */
+  int decomposition;   /* cup_decomposition */
+  int device;          /* HIP device ordinal */
+} cup_grid_desc;
+
+typedef struct cup_ctx cup_ctx;
+
+/* Grid and block store. Replaces the Cubism FluidGridMPI allocation in
+ * Simulation::setupGrid (Simulation.cpp:221-243) and the PoissonSolverPeriodic buffers and
+ * plans (PoissonSolverPeriodic.cpp:90-127). Only fully periodic boxes are supported. */
+int cup_create(const cup_grid_desc* desc, cup_ctx** out);
+int cup_destroy(cup_ctx* ctx);
+const char* cup_last_error(const cup_ctx* ctx);
+/* cells[3], h, extent[3] */
+int cup_get_geometry(const cup_ctx* ctx, int cells[3], double* h, double extent[3]);
+int cup_synchronize(cup_ctx* ctx);
+
+/* Field upload/download in host block-major doubles (converted to the context precision),
+ * replacing direct FluidBlock access through BlockInfo::ptrBlock (Definitions.h:103-167). */
+int cup_set_field(cup_ctx* ctx, int field, const double* host);
+int cup_get_field(cup_ctx* ctx, int field, double* host);
+/* Raw device pointer of a field (context precision), for zero-copy callers. */
+int cup_field_device_ptr(cup_ctx* ctx, int field, void** dptr);
+
+/* TestBoundaries.cpp:19-75 restated on the device: tmpU = lab(-dx,-dy,-dz).u through the
+ * LDS-staged periodic halo gather with halo width g (1 or 2). */
+int cup_lab_shift(cup_ctx* ctx, int g, int dx, int dy, int dz);
+
+/* findMaxU (operators/ProcessHelpers.h:74-99): max over cells of max_i |u_i + uinf_i|. */
+int cup_max_velocity(cup_ctx* ctx, const double uinf[3], double* umax);
+
+/* Simulation::calcMaxTimestep (Simulation.cpp:403-424) with the CFL ramp. */
+int cup_calc_max_timestep(cup_ctx* ctx, double CFL, double nu, const double uinf[3], int step,
+                          int rampup, double* dt_out, double* umax_out);
+
+/* AdvectionDiffusion::operator() (AdvectionDiffusion.cpp:402-428): KernelAdvectDiffuse
+ * (:106-145) into tmpU/V/W and the copy tmp -> u (:315-328; the inflow correction is a
+ * no-op on periodic faces). With CUP_FUSED the copy is skipped: u keeps the old velocity
+ * until cup_pressure_project writes u = tmp - dt grad p. */
+int cup_advect_diffuse(cup_ctx* ctx, double dt, double nu, const double uinf[3], int scheme);
+
+/* PressureRHS::operator() (PressureRHS.cpp:273-346): h^2/(2dt) * central divergence of
+ * the velocity (tmp in FUSED, u in PLAIN), with the obstacle (1-chi) masking and perimeter
+ * correction when a sphere exists. */
+int cup_pressure_rhs(cup_ctx* ctx, double dt);
+
+/* PoissonSolverPeriodic::solve (PoissonSolverPeriodic.cpp:129-146) and _fftw2cub
+ * (PoissonSolver.cpp:116-130): hand-written r2c FFT, grid-consistent spectral division
+ * (:15-61), c2r, result in p. */
+int cup_poisson_periodic(cup_ctx* ctx);
+
+/* KernelGradP (PressureProjection.cpp:27-51): u -= dt/(2h) (p_E - p_W), etc. */
+int cup_pressure_project(cup_ctx* ctx, double dt);
+
+/* A sphere (Sphere.cpp:17-50 SDF, ObstacleLibrary.h:41-69 block fill, Obstacle.h:186-207
+ * block selection) turned into chi by CreateObstacles' KernelCharacteristicFunction
+ * (ObstaclesCreate.cpp:22-137), with surface points and the grid centre of mass
+ * (:230-256). Static body: built once. com_mass_out = CoM[3], mass (may be NULL). */
+int cup_chi_from_sphere(cup_ctx* ctx, const double center[3], double radius, double com_mass_out[4]);
+
+/* Penalization::operator() (Penalization.cpp:129-156): u += dt*pf*(U_body - u) with
+ * pf = chi*lambda/(1+lambda*dt*chi) (implicit) or chi/dt (explicit), and the force/torque
+ * integrals ft_out[6] (KernelFinalizePenalizationForce, :101-124). */
+int cup_penalize(cup_ctx* ctx, double dt, double lambda, int implicit, const double trans_vel[3],
+                 const double ang_vel[3], double ft_out[6]);
+
+/* ComputeForces (FluidSolidForces.cpp:17-158, 205-223) reduced like Obstacle::computeForces
+ * (Obstacle.cpp:266-286): out[22] in ObstacleBlock::sumQoI order (ObstacleBlock.h:81-93). */
+int cup_surface_forces(cup_ctx* ctx, double nu, const double trans_vel[3], const double ang_vel[3],
+                       double out[22]);
+
+/* KernelDissipation (ComputeDissipation.cpp:17-116): out[20] in the RDX order (:136-155). */
+int cup_diagnostics(cup_ctx* ctx, double nu, double out[20]);
+
+/* Initial conditions: KernelIC_taylorGreen (InitialConditions.cpp:32-52). */
+int cup_ic_taylor_green(cup_ctx* ctx, double uMax);
+/* Seeded isotropic turbulence (construction of SpectralManipFFTW::_compute_IC,
+ * SpectralManipFFTW.cpp:217-288, with the "art" spectrum, SpectralIcGenerator.cpp:63-81):
+ * Philox-4x32-10 phases from seed, exact Hermitian symmetry, DC and Nyquist zeroed,
+ * scaled to sqrt(<|u|^2>/3) = u_rms. Generated in FP64 on the device. */
+int cup_ic_hit(cup_ctx* ctx, double k0, double u_rms, uint64_t seed);
+
+/* SpectralForcing::operator() (SpectralForcing.cpp:23-82) with
+ * SpectralManipFFTW::_compute_largeModesForcing (SpectralManipFFTW.cpp:25-108):
+ * u += fac * P_{0<k^2<=4} u, fac = sim_dt*injection/(2 tke_low), injection =
+ * (tke_target - tke)/dt or injection_rate. stats_out = tke, eps, tke_filtered, l_integral, fac. */
+int cup_spectral_forcing(cup_ctx* ctx, double dt, double sim_dt, double nu, double tke_target,
+                         double injection_rate, double stats_out[5]);
+
+/* ------------------------------------------------------------------------------------
+ * Whole-step driver with the step scalars resident on the device (no host round trip
+ * per step): dt from the CFL rule (with ramp) of the maximum velocity measured in the
+ * previous step's gradient epilogue, the reference pipeline order of
+ * Simulation::timestep (Simulation.cpp:593-623), step counter and time kept on the
+ * device. It is the FUSED decomposition's form of Simulation::run (Simulation.cpp:582-591).
+ */
+typedef struct cup_step_params {
+  double CFL;
+  double nu;
+  double uinf[3];
+  int rampup;
+  int scheme;             /* cup_scheme */
+  int spectral_forcing;   /* band-limited linear forcing on/off */
+  double tke_target;      /* forcing target (<=0: use injection_rate) */
+  double injection_rate;
+  double fixed_dt;        /* > 0: use this dt instead of the CFL rule */
+  int has_body;           /* penalize the sphere created by cup_chi_from_sphere */
+  double lambda;
+  int implicit_penalization;
+  double body_vel[3];
+  double body_angvel[3];
+} cup_step_params;
+
+int cup_run_steps(cup_ctx* ctx, const cup_step_params* prm, int nsteps);
+/* step, time, dt of the last step, umax measured after it, tke after it. */
+int cup_step_state(cup_ctx* ctx, int* step, double* time, double* last_dt, double* umax, double* tke);
+int cup_set_step_state(cup_ctx* ctx, int step, double time);
+
+/* Kernel-level timing hooks for measurement: the stream the context launches on. */
+int cup_stream(cup_ctx* ctx, void** hip_stream);
+
+/* ------------------------------------------------------------------------------------
+ * Simulation-level entry points: the reference's Simulation / SimulationData pair
+ * (source/Simulation.h:24-71, source/SimulationData.h:46-179) built by the host C++
+ * layer of this library on top of the calls above, for bindings (the reference's pybind11
+ * module, source/bindings/Simulation.cpp:59-94, binds these same objects).
+ */
+typedef struct cup_sim cup_sim;
+
+/* SimulationData(comm, parser) (SimulationData.cpp:32-172): "-bpdx 8 -nu 0.01 ..."; adds
+ * -precision float|double, -decomposition plain|fused, -device, -icSeed, -icUrms, and a
+ * sphere through -sphereL -xpos -ypos -zpos -xvel -bForcedInSimFrame -bFixFrameOfRef. */
+int cup_sim_create(int argc, const char* const* argv, cup_sim** out);
+/* SimulationData::setCells (SimulationData.cpp:280-291); before cup_sim_init. */
+int cup_sim_set_cells(cup_sim* sim, int nx, int ny, int nz);
+/* Simulation(const SimulationData&) (Simulation.cpp:53-64): grid, pipeline, IC. */
+int cup_sim_init(cup_sim* sim);
+int cup_sim_destroy(cup_sim* sim);
+const char* cup_sim_last_error(const cup_sim* sim);
+/* Simulation_addObstacle with a Sphere (bindings/Obstacles.cpp:79-128). */
+int cup_sim_add_sphere(cup_sim* sim, double radius, const double position[3], const double trans_vel[3],
+                       int forced, int fix_frame);
+int cup_sim_run(cup_sim* sim);                                   /* Simulation::run (Simulation.cpp:582-591) */
+int cup_sim_calc_max_timestep(cup_sim* sim, double* dt);         /* Simulation.cpp:403-424 */
+int cup_sim_timestep(cup_sim* sim, double dt, int* finished);    /* Simulation.cpp:593-623 */
+int cup_sim_advance(cup_sim* sim, int nsteps, int* finished);    /* FUSED device-resident loop */
+int cup_sim_context(cup_sim* sim, cup_ctx** ctx);
+int cup_sim_get(cup_sim* sim, const char* name, double* value);  /* SimulationData members by name */
+int cup_sim_set(cup_sim* sim, const char* name, double value);
+int cup_sim_obstacle(cup_sim* sim, int idx, double out[34]);     /* CoM, mass, force, torque, 22 QoIs */
+int cup_sim_diagnostics(cup_sim* sim, double out[20]);
+int cup_sim_flush_logs(cup_sim* sim);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CUP_H */

@@ -1,0 +1,1152 @@
+/*
+ * cup_oracle.c -- the oracle: an FP64 CPU restatement of CubismUP_3D's per-timestep fluid
+ * update on a uniform, periodic grid of 8^3-cell blocks, with OpenMP over blocks.
+ *
+ * TEST INFRASTRUCTURE ONLY. This file is the checker. It is built into oracle/liboracle.so
+ * and is loaded only by tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline
+ * leg; nothing in the product links or calls it.
+ *
+ * It keeps the reference CPU path's structure on purpose, because it doubles as the CPU
+ * baseline: an AoS cell element of 8 doubles (source/Definitions.h:40-52), blocks stored
+ * data[z][y][x] (Definitions.h:115), a halo'd "lab" copy of every block filled by periodic
+ * wrap before each stencil kernel (Cubism BlockLab + BlockLabBC::_apply_bc,
+ * Definitions.h:349-377, where periodic faces are a no-op), and one OpenMP loop over blocks
+ * per operator (source/operators/Operator.h:113-170). The FFT is written here (radix-2,
+ * full complex, unnormalized like FFTW's r2c/c2r pair) because FFTW is not available.
+ *
+ * What pins each function (tests/test_oracle_*.py):
+ *   lab fill          -- TestBoundaries shift test restated, exact (tests/cubismup3d/TestBoundaries.cpp:19-75)
+ *   advect_diffuse    -- shear-wave closed form (exact one-step factor), advected-mode symbol,
+ *                        brute force against an independent numpy restatement on 16^3
+ *   pressure_rhs      -- single-mode closed form (2i sin(kh) h^2/(2dt) symbol)
+ *   fft3d             -- numpy.fft golden vectors (tests/golden/fft_*.npz), round trip
+ *   poisson_solve     -- closed-form symbol per mode, the 8 zeroed modes, div_c(u_new)=0
+ *   grad_p            -- single-mode closed form, projection idempotence
+ *   find_max_u / dt   -- closed form incl. ramp
+ *   dissipation       -- KE vs analytic TGV energy, linear momentum of a constant field
+ *   spectral_forcing  -- Parseval (tke vs physical-space mean), 32-mode count, injected energy
+ *   ic_hit            -- Philox-4x32-10 known-answer vectors, solenoidality, u_rms, spectrum shape
+ *   sphere chi        -- Towers formula on a planar SDF (closed form), sphere volume vs 4/3 pi R^3
+ *   penalize          -- per-cell closed form; force = sum of per-cell terms
+ *   surface_forces    -- uniform pressure gives zero net force (closed surface), linear shear
+ */
+#include "cup_oracle.h"
+
+#include <complex.h>
+#include <float.h>
+#include <math.h>
+#include <omp.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define BS 8
+#define SURFDH 1 /* ObstacleBlock.h:16 */
+
+typedef struct {
+  double chi, u, v, w, p, tmpU, tmpV, tmpW;
+} FE;
+
+typedef struct {
+  FE d[BS][BS][BS];
+} Block;
+
+/* Per-block obstacle data (ObstacleBlock.h:34-67), sphere only. */
+typedef struct {
+  double chi[BS][BS][BS];
+  double sdf[BS][BS][BS];
+  double CoM[3], mass;
+  int npts, cap;
+  int* pix;
+  double *dchidx, *dchidy, *dchidz, *delta;
+} OBlock;
+
+struct orc_sim {
+  int bpd[3], N[3], nb;
+  double h, maxextent, extent[3];
+  Block* blk;
+  double* rhs; /* solver buffer, global [x][y][z], z fastest (PoissonSolverPeriodic.cpp:112-116) */
+  int has_sphere;
+  double radius, spos[3];
+  OBlock** ob;
+  double CoM[3], mass;
+};
+
+static const double EPS = DBL_EPSILON;
+
+/* ------------------------------------------------------------------------ grid */
+
+orc_sim* orc_create(int bpdx, int bpdy, int bpdz, double maxextent)
+{
+  if (bpdx < 1 || bpdy < 1 || bpdz < 1) return NULL;
+  orc_sim* s = (orc_sim*)calloc(1, sizeof(orc_sim));
+  s->bpd[0] = bpdx; s->bpd[1] = bpdy; s->bpd[2] = bpdz;
+  for (int d = 0; d < 3; d++) s->N[d] = s->bpd[d] * BS;
+  s->nb = bpdx * bpdy * bpdz;
+  int maxN = s->N[0];
+  if (s->N[1] > maxN) maxN = s->N[1];
+  if (s->N[2] > maxN) maxN = s->N[2];
+  s->maxextent = maxextent;
+  s->h = maxextent / maxN;
+  for (int d = 0; d < 3; d++) s->extent[d] = ((double)s->N[d] / maxN) * maxextent;
+  s->blk = (Block*)calloc((size_t)s->nb, sizeof(Block));
+  s->rhs = (double*)calloc((size_t)s->N[0] * s->N[1] * s->N[2], sizeof(double));
+  return s;
+}
+
+static void free_sphere(orc_sim* s)
+{
+  if (!s->ob) return;
+  for (int b = 0; b < s->nb; b++) {
+    OBlock* o = s->ob[b];
+    if (!o) continue;
+    free(o->pix); free(o->dchidx); free(o->dchidy); free(o->dchidz); free(o->delta);
+    free(o);
+  }
+  free(s->ob);
+  s->ob = NULL;
+  s->has_sphere = 0;
+}
+
+void orc_destroy(orc_sim* s)
+{
+  if (!s) return;
+  free_sphere(s);
+  free(s->blk);
+  free(s->rhs);
+  free(s);
+}
+
+double orc_h(const orc_sim* s) { return s->h; }
+void orc_extent(const orc_sim* s, double out[3]) { for (int d = 0; d < 3; d++) out[d] = s->extent[d]; }
+void orc_set_threads(int n) { if (n > 0) omp_set_num_threads(n); }
+
+static inline double* fe_field(FE* e, int f) { return &e->chi + f; }
+
+void orc_set_field(orc_sim* s, int f, const double* src)
+{
+  #pragma omp parallel for schedule(static)
+  for (int b = 0; b < s->nb; b++) {
+    FE* e = &s->blk[b].d[0][0][0];
+    const double* q = src + (size_t)b * BS * BS * BS;
+    for (int i = 0; i < BS * BS * BS; i++) *fe_field(&e[i], f) = q[i];
+  }
+}
+
+void orc_get_field(const orc_sim* s, int f, double* dst)
+{
+  #pragma omp parallel for schedule(static)
+  for (int b = 0; b < s->nb; b++) {
+    FE* e = &s->blk[b].d[0][0][0];
+    double* q = dst + (size_t)b * BS * BS * BS;
+    for (int i = 0; i < BS * BS * BS; i++) q[i] = *fe_field(&e[i], f);
+  }
+}
+
+/* Block coordinates of block id b (block = bx + bpdx*(by + bpdy*bz)). */
+static inline void bcoords(const orc_sim* s, int b, int bc[3])
+{
+  bc[0] = b % s->bpd[0];
+  bc[1] = (b / s->bpd[0]) % s->bpd[1];
+  bc[2] = b / (s->bpd[0] * s->bpd[1]);
+}
+
+/* Cell-centre position x = (index*BS + i + 1/2) h (LinearInterpolation.h:155-166). */
+static inline void cell_pos(const orc_sim* s, const int bc[3], int ix, int iy, int iz, double p[3])
+{
+  p[0] = (bc[0] * BS + ix + 0.5) * s->h;
+  p[1] = (bc[1] * BS + iy + 0.5) * s->h;
+  p[2] = (bc[2] * BS + iz + 0.5) * s->h;
+}
+
+static inline int wrap(int i, int n) { i %= n; return i < 0 ? i + n : i; }
+
+static inline const FE* gcell(const orc_sim* s, int gx, int gy, int gz)
+{
+  gx = wrap(gx, s->N[0]); gy = wrap(gy, s->N[1]); gz = wrap(gz, s->N[2]);
+  const int b = gx / BS + s->bpd[0] * (gy / BS + s->bpd[1] * (gz / BS));
+  return &s->blk[b].d[gz % BS][gy % BS][gx % BS];
+}
+
+/* Lab: a (BS+2g)^3 halo'd copy of block b, periodic wrap everywhere (tensorial). */
+typedef struct {
+  FE* c;
+  int g, L;
+} Lab;
+
+static Lab lab_alloc(int g)
+{
+  Lab l;
+  l.g = g;
+  l.L = BS + 2 * g;
+  l.c = (FE*)malloc(sizeof(FE) * l.L * l.L * l.L);
+  return l;
+}
+
+static void lab_load(const orc_sim* s, int b, Lab* l)
+{
+  int bc[3];
+  bcoords(s, b, bc);
+  const int g = l->g, L = l->L;
+  for (int iz = -g; iz < BS + g; iz++)
+    for (int iy = -g; iy < BS + g; iy++)
+      for (int ix = -g; ix < BS + g; ix++)
+        l->c[((iz + g) * L + (iy + g)) * L + (ix + g)] =
+            *gcell(s, bc[0] * BS + ix, bc[1] * BS + iy, bc[2] * BS + iz);
+}
+
+static inline const FE* LAB(const Lab* l, int ix, int iy, int iz)
+{
+  return &l->c[((iz + l->g) * l->L + (iy + l->g)) * l->L + (ix + l->g)];
+}
+
+void orc_lab_shift(orc_sim* s, int g, int dx, int dy, int dz)
+{
+  #pragma omp parallel
+  {
+    Lab l = lab_alloc(g);
+    #pragma omp for schedule(static)
+    for (int b = 0; b < s->nb; b++) {
+      lab_load(s, b, &l);
+      for (int iz = 0; iz < BS; iz++)
+        for (int iy = 0; iy < BS; iy++)
+          for (int ix = 0; ix < BS; ix++)
+            s->blk[b].d[iz][iy][ix].tmpU = LAB(&l, ix - dx, iy - dy, iz - dz)->u;
+    }
+    free(l.c);
+  }
+}
+
+/* ------------------------------------------------------------ reductions, dt */
+
+/* findMaxU (ProcessHelpers.h:74-99). */
+double orc_find_max_u(const orc_sim* s, const double uinf[3])
+{
+  double maxU = 0;
+  #pragma omp parallel for schedule(static) reduction(max : maxU)
+  for (int b = 0; b < s->nb; b++) {
+    const FE* e = &s->blk[b].d[0][0][0];
+    for (int i = 0; i < BS * BS * BS; i++) {
+      const double au = fabs(e[i].u + uinf[0]);
+      const double av = fabs(e[i].v + uinf[1]);
+      const double aw = fabs(e[i].w + uinf[2]);
+      double m = au > av ? au : av;
+      m = m > aw ? m : aw;
+      maxU = maxU > m ? maxU : m;
+    }
+  }
+  return maxU;
+}
+
+/* calcMaxTimestep (Simulation.cpp:403-424), uniform grid so hMin = h. */
+double orc_calc_max_timestep(const orc_sim* s, double CFL, double nu, const double uinf[3],
+                             int step, int rampup, double* umax_out)
+{
+  const double hMin = s->h;
+  const double umax = orc_find_max_u(s, uinf);
+  if (umax_out) *umax_out = umax;
+  const double dtDif = hMin * hMin / nu;
+  const double dtAdv = hMin / (umax + 1e-8);
+  const double m = dtDif < dtAdv ? dtDif : dtAdv;
+  double dt = CFL * m;
+  if (step < rampup) {
+    const double x = (step + 1.0) / rampup;
+    const double rampCFL = exp(log(1e-3) * (1 - x) + log(CFL) * x);
+    dt = rampCFL * m;
+  }
+  return dt;
+}
+
+/* ---------------------------------------------------------------- operators */
+
+/* KernelAdvectDiffuse (AdvectionDiffusion.cpp:106-145) into tmpU/V/W, then the copy
+ * tmp -> u of UpdateAndCorrectInflow_nonUniform::operate (:315-328); with every face
+ * periodic the inflow sums are zero and the correction returns early (:363-365). */
+void orc_advect_diffuse(orc_sim* s, double dt, double nu, const double uinf[3])
+{
+  const double h = s->h;
+  const double facA = -dt / (2 * h);
+  const double facD = (nu / h) * (dt / h);
+  #pragma omp parallel
+  {
+    Lab l = lab_alloc(1);
+    #pragma omp for schedule(static)
+    for (int b = 0; b < s->nb; b++) {
+      lab_load(s, b, &l);
+      Block* o = &s->blk[b];
+      for (int iz = 0; iz < BS; iz++)
+        for (int iy = 0; iy < BS; iy++)
+          for (int ix = 0; ix < BS; ix++) {
+            const FE* L = LAB(&l, ix, iy, iz);
+            const FE *LW = LAB(&l, ix - 1, iy, iz), *LE = LAB(&l, ix + 1, iy, iz);
+            const FE *LS = LAB(&l, ix, iy - 1, iz), *LN = LAB(&l, ix, iy + 1, iz);
+            const FE *LF = LAB(&l, ix, iy, iz - 1), *LB = LAB(&l, ix, iy, iz + 1);
+            const double dudx = LE->u - LW->u, dvdx = LE->v - LW->v, dwdx = LE->w - LW->w;
+            const double dudy = LN->u - LS->u, dvdy = LN->v - LS->v, dwdy = LN->w - LS->w;
+            const double dudz = LB->u - LF->u, dvdz = LB->v - LF->v, dwdz = LB->w - LF->w;
+            const double u = L->u + uinf[0], v = L->v + uinf[1], w = L->w + uinf[2];
+            const double duD = LN->u + LS->u + LE->u + LW->u + LF->u + LB->u - L->u * 6;
+            const double dvD = LN->v + LS->v + LE->v + LW->v + LF->v + LB->v - L->v * 6;
+            const double dwD = LN->w + LS->w + LE->w + LW->w + LF->w + LB->w - L->w * 6;
+            const double duA = u * dudx + v * dudy + w * dudz;
+            const double dvA = u * dvdx + v * dvdy + w * dvdz;
+            const double dwA = u * dwdx + v * dwdy + w * dwdz;
+            o->d[iz][iy][ix].tmpU = L->u + facA * duA + facD * duD;
+            o->d[iz][iy][ix].tmpV = L->v + facA * dvA + facD * dvD;
+            o->d[iz][iy][ix].tmpW = L->w + facA * dwA + facD * dwD;
+          }
+    }
+    free(l.c);
+  }
+  #pragma omp parallel for schedule(static)
+  for (int b = 0; b < s->nb; b++) {
+    FE* e = &s->blk[b].d[0][0][0];
+    for (int i = 0; i < BS * BS * BS; i++) {
+      e[i].u = e[i].tmpU;
+      e[i].v = e[i].tmpV;
+      e[i].w = e[i].tmpW;
+    }
+  }
+}
+
+static inline size_t ridx(const orc_sim* s, int gx, int gy, int gz)
+{
+  return ((size_t)gx * s->N[1] + gy) * s->N[2] + gz;
+}
+
+/* KernelPressureRHS (PressureRHS.cpp:29-113) after the solver reset (:277, memset), then
+ * with a sphere the perimeter correction (:322-343, KernelFinalizePerimeters :215-256). */
+void orc_pressure_rhs(orc_sim* s, double dt)
+{
+  memset(s->rhs, 0, sizeof(double) * (size_t)s->N[0] * s->N[1] * s->N[2]);
+  const double h = s->h, fac = 0.5 * h * h / dt;
+  unsigned char* touch = NULL;
+  if (s->has_sphere) touch = (unsigned char*)calloc((size_t)s->N[0] * s->N[1] * s->N[2], 1);
+  double sumRHS = 0, posRHS = 0, negRHS = 0;
+  #pragma omp parallel reduction(+ : sumRHS, posRHS, negRHS)
+  {
+    Lab l = lab_alloc(1);
+    #pragma omp for schedule(static)
+    for (int b = 0; b < s->nb; b++) {
+      lab_load(s, b, &l);
+      int bc[3];
+      bcoords(s, b, bc);
+      for (int iz = 0; iz < BS; iz++)
+        for (int iy = 0; iy < BS; iy++)
+          for (int ix = 0; ix < BS; ix++) {
+            const FE *LW = LAB(&l, ix - 1, iy, iz), *LE = LAB(&l, ix + 1, iy, iz);
+            const FE *LS = LAB(&l, ix, iy - 1, iz), *LN = LAB(&l, ix, iy + 1, iz);
+            const FE *LF = LAB(&l, ix, iy, iz - 1), *LB = LAB(&l, ix, iy, iz + 1);
+            s->rhs[ridx(s, bc[0] * BS + ix, bc[1] * BS + iy, bc[2] * BS + iz)] =
+                fac * (LE->u - LW->u + LN->v - LS->v + LB->w - LF->w);
+          }
+      if (!s->has_sphere || s->ob[b] == NULL) continue;
+      const OBlock* o = s->ob[b];
+      for (int iz = 0; iz < BS; iz++)
+        for (int iy = 0; iy < BS; iy++)
+          for (int ix = 0; ix < BS; ix++) {
+            if (LAB(&l, ix, iy, iz)->chi > o->chi[iz][iy][ix]) continue;
+            const size_t idx = ridx(s, bc[0] * BS + ix, bc[1] * BS + iy, bc[2] * BS + iz);
+            const FE *LW = LAB(&l, ix - 1, iy, iz), *LE = LAB(&l, ix + 1, iy, iz);
+            const FE *LS = LAB(&l, ix, iy - 1, iz), *LN = LAB(&l, ix, iy + 1, iz);
+            const FE *LF = LAB(&l, ix, iy, iz - 1), *LB = LAB(&l, ix, iy, iz + 1);
+            const double dXx = LE->chi - LW->chi, dXy = LN->chi - LS->chi, dXz = LB->chi - LF->chi;
+            const int bSurface = dXx * dXx + dXy * dXy + dXz * dXz > 0;
+            if (bSurface) touch[idx] = 1;
+            const double srcBulk = -o->chi[iz][iy][ix] * s->rhs[idx];
+            sumRHS += srcBulk;
+            s->rhs[idx] += srcBulk;
+            posRHS += bSurface * s->rhs[idx] * (s->rhs[idx] > 0);
+            negRHS -= bSurface * s->rhs[idx] * (s->rhs[idx] < 0);
+          }
+    }
+    free(l.c);
+  }
+  if (!s->has_sphere) return;
+  const double corrDenom = sumRHS > 0 ? posRHS : negRHS;
+  const double corr = sumRHS / (corrDenom > EPS ? corrDenom : EPS);
+  #pragma omp parallel for schedule(static)
+  for (int b = 0; b < s->nb; b++) {
+    if (s->ob[b] == NULL) continue;
+    int bc[3];
+    bcoords(s, b, bc);
+    for (int iz = 0; iz < BS; iz++)
+      for (int iy = 0; iy < BS; iy++)
+        for (int ix = 0; ix < BS; ix++) {
+          const size_t idx = ridx(s, bc[0] * BS + ix, bc[1] * BS + iy, bc[2] * BS + iz);
+          if (!touch[idx]) continue;
+          if (s->rhs[idx] > 0 && corr > 0) s->rhs[idx] -= corr * s->rhs[idx];
+          else if (s->rhs[idx] < 0 && corr < 0) s->rhs[idx] += corr * s->rhs[idx];
+        }
+  }
+  free(touch);
+}
+
+void orc_get_rhs(const orc_sim* s, double* dst)
+{
+  #pragma omp parallel for schedule(static)
+  for (int b = 0; b < s->nb; b++) {
+    int bc[3];
+    bcoords(s, b, bc);
+    double* q = dst + (size_t)b * BS * BS * BS;
+    for (int iz = 0; iz < BS; iz++)
+      for (int iy = 0; iy < BS; iy++)
+        for (int ix = 0; ix < BS; ix++)
+          q[(iz * BS + iy) * BS + ix] = s->rhs[ridx(s, bc[0] * BS + ix, bc[1] * BS + iy, bc[2] * BS + iz)];
+  }
+}
+
+void orc_set_rhs(orc_sim* s, const double* src)
+{
+  #pragma omp parallel for schedule(static)
+  for (int b = 0; b < s->nb; b++) {
+    int bc[3];
+    bcoords(s, b, bc);
+    const double* q = src + (size_t)b * BS * BS * BS;
+    for (int iz = 0; iz < BS; iz++)
+      for (int iy = 0; iy < BS; iy++)
+        for (int ix = 0; ix < BS; ix++)
+          s->rhs[ridx(s, bc[0] * BS + ix, bc[1] * BS + iy, bc[2] * BS + iz)] = q[(iz * BS + iy) * BS + ix];
+  }
+}
+
+/* ---------------------------------------------------------------------- FFT */
+
+/* In-place radix-2 decimation-in-time FFT of a contiguous line, unnormalized;
+ * tw[k] = exp(sign*2*pi*i*k/n), k < n/2. */
+static void fft_line(double complex* a, int n, const double complex* tw)
+{
+  for (int i = 1, j = 0; i < n; i++) {
+    int bit = n >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) {
+      const double complex t = a[i];
+      a[i] = a[j];
+      a[j] = t;
+    }
+  }
+  for (int len = 2; len <= n; len <<= 1) {
+    const int half = len >> 1, step = n / len;
+    for (int i = 0; i < n; i += len)
+      for (int k = 0; k < half; k++) {
+        const double complex w = tw[k * step];
+        const double complex x = a[i + k], y = a[i + k + half] * w;
+        a[i + k] = x + y;
+        a[i + k + half] = x - y;
+      }
+  }
+}
+
+static double complex* make_tw(int n, int sign)
+{
+  double complex* tw = (double complex*)malloc(sizeof(double complex) * (n / 2 + 1));
+  for (int k = 0; k < n / 2 + 1; k++) {
+    const double ang = 2.0 * M_PI * (double)k / (double)n;
+    tw[k] = cos(ang) + I * (sign * sin(ang));
+  }
+  return tw;
+}
+
+/* 3-D complex FFT of c[x][y][z] (z fastest), unnormalized. */
+static void fft3d_c(double complex* c, int nx, int ny, int nz, int sign)
+{
+  double complex* twx = make_tw(nx, sign);
+  double complex* twy = make_tw(ny, sign);
+  double complex* twz = make_tw(nz, sign);
+  #pragma omp parallel
+  {
+    int nmax = nx > ny ? nx : ny;
+    nmax = nmax > nz ? nmax : nz;
+    double complex* line = (double complex*)malloc(sizeof(double complex) * nmax);
+    #pragma omp for schedule(static)
+    for (int xy = 0; xy < nx * ny; xy++) fft_line(c + (size_t)xy * nz, nz, twz);
+    #pragma omp for schedule(static)
+    for (int xz = 0; xz < nx * nz; xz++) {
+      const int x = xz / nz, z = xz % nz;
+      double complex* base = c + (size_t)x * ny * nz + z;
+      for (int y = 0; y < ny; y++) line[y] = base[(size_t)y * nz];
+      fft_line(line, ny, twy);
+      for (int y = 0; y < ny; y++) base[(size_t)y * nz] = line[y];
+    }
+    #pragma omp for schedule(static)
+    for (int yz = 0; yz < ny * nz; yz++) {
+      double complex* base = c + yz;
+      for (int x = 0; x < nx; x++) line[x] = base[(size_t)x * ny * nz];
+      fft_line(line, nx, twx);
+      for (int x = 0; x < nx; x++) base[(size_t)x * ny * nz] = line[x];
+    }
+    free(line);
+  }
+  free(twx); free(twy); free(twz);
+}
+
+void orc_fft3d(int nx, int ny, int nz, double* re, double* im, int sign)
+{
+  const size_t n = (size_t)nx * ny * nz;
+  double complex* c = (double complex*)malloc(sizeof(double complex) * n);
+  for (size_t i = 0; i < n; i++) c[i] = re[i] + I * im[i];
+  fft3d_c(c, nx, ny, nz, sign < 0 ? -1 : +1);
+  for (size_t i = 0; i < n; i++) { re[i] = creal(c[i]); im[i] = cimag(c[i]); }
+  free(c);
+}
+
+/* PoissonSolverPeriodic::solve (PoissonSolverPeriodic.cpp:129-146) with _solve's
+ * grid-consistent branch (:15-61): r2c, multiply by 4/(N0 h N1 N2) / (2D - 6) with
+ * D = cos(4 pi i/N0) + cos(4 pi l/N1) + cos(4 pi k/N2), zero the 8 modes whose indices
+ * are all in {0, N/2}, c2r; then _fftw2cub (PoissonSolver.cpp:116-130) copies into p. */
+void orc_poisson_solve(orc_sim* s)
+{
+  const int N0 = s->N[0], N1 = s->N[1], N2 = s->N[2];
+  const size_t n = (size_t)N0 * N1 * N2;
+  double complex* c = (double complex*)malloc(sizeof(double complex) * n);
+  #pragma omp parallel for schedule(static)
+  for (size_t i = 0; i < n; i++) c[i] = s->rhs[i];
+  fft3d_c(c, N0, N1, N2, -1);
+  const double h = s->h;
+  const double norm_factor = 4 / ((double)N0 * h * N1 * N2);
+  const double wFacX = 4 * M_PI / N0, wFacY = 4 * M_PI / N1, wFacZ = 4 * M_PI / N2;
+  #pragma omp parallel for schedule(static)
+  for (int i = 0; i < N0; i++)
+    for (int l = 0; l < N1; l++)
+      for (int k = 0; k < N2; k++) {
+        const double D = cos(wFacX * i) + cos(wFacY * l) + cos(wFacZ * k);
+        const double solutionFactor = norm_factor / (2 * D - 6);
+        c[((size_t)i * N1 + l) * N2 + k] *= solutionFactor;
+      }
+  const int I0[2] = {0, N0 / 2}, L0[2] = {0, N1 / 2}, K0[2] = {0, N2 / 2};
+  for (int a = 0; a < 2; a++)
+    for (int b = 0; b < 2; b++)
+      for (int d = 0; d < 2; d++) c[((size_t)I0[a] * N1 + L0[b]) * N2 + K0[d]] = 0;
+  fft3d_c(c, N0, N1, N2, +1);
+  #pragma omp parallel for schedule(static)
+  for (size_t i = 0; i < n; i++) s->rhs[i] = creal(c[i]);
+  free(c);
+  #pragma omp parallel for schedule(static)
+  for (int b = 0; b < s->nb; b++) {
+    int bc[3];
+    bcoords(s, b, bc);
+    for (int iz = 0; iz < BS; iz++)
+      for (int iy = 0; iy < BS; iy++)
+        for (int ix = 0; ix < BS; ix++)
+          s->blk[b].d[iz][iy][ix].p = s->rhs[ridx(s, bc[0] * BS + ix, bc[1] * BS + iy, bc[2] * BS + iz)];
+  }
+}
+
+/* KernelGradP (PressureProjection.cpp:27-51). */
+void orc_grad_p(orc_sim* s, double dt)
+{
+  const double fac = -0.5 * dt / s->h;
+  #pragma omp parallel
+  {
+    Lab l = lab_alloc(1);
+    #pragma omp for schedule(static)
+    for (int b = 0; b < s->nb; b++) {
+      lab_load(s, b, &l);
+      Block* o = &s->blk[b];
+      for (int iz = 0; iz < BS; iz++)
+        for (int iy = 0; iy < BS; iy++)
+          for (int ix = 0; ix < BS; ix++) {
+            o->d[iz][iy][ix].u += fac * (LAB(&l, ix + 1, iy, iz)->p - LAB(&l, ix - 1, iy, iz)->p);
+            o->d[iz][iy][ix].v += fac * (LAB(&l, ix, iy + 1, iz)->p - LAB(&l, ix, iy - 1, iz)->p);
+            o->d[iz][iy][ix].w += fac * (LAB(&l, ix, iy, iz + 1)->p - LAB(&l, ix, iy, iz - 1)->p);
+          }
+    }
+    free(l.c);
+  }
+}
+
+/* KernelDissipation (ComputeDissipation.cpp:17-116); out[] in RDX order (:136-155). */
+void orc_dissipation(const orc_sim* s, double nu, double out[20])
+{
+  double R[20] = {0};
+  const double h = s->h, hCube = h * h * h, inv2h = .5 / h, invHh = 1 / (h * h);
+  const double center[3] = {s->extent[0] / 2, s->extent[1] / 2, s->extent[2] / 2};
+  #pragma omp parallel reduction(+ : R[:20])
+  {
+    Lab l = lab_alloc(1);
+    #pragma omp for schedule(static)
+    for (int b = 0; b < s->nb; b++) {
+      lab_load(s, b, &l);
+      int bc[3];
+      bcoords(s, b, bc);
+      for (int iz = 0; iz < BS; iz++)
+        for (int iy = 0; iy < BS; iy++)
+          for (int ix = 0; ix < BS; ix++) {
+            const FE* L = LAB(&l, ix, iy, iz);
+            const FE *LW = LAB(&l, ix - 1, iy, iz), *LE = LAB(&l, ix + 1, iy, iz);
+            const FE *LS = LAB(&l, ix, iy - 1, iz), *LN = LAB(&l, ix, iy + 1, iz);
+            const FE *LF = LAB(&l, ix, iy, iz - 1), *LB = LAB(&l, ix, iy, iz + 1);
+            const double X = L->chi;
+            double p[3];
+            cell_pos(s, bc, ix, iy, iz, p);
+            const double PX = p[0] - center[0], PY = p[1] - center[1], PZ = p[2] - center[2];
+            const double WX = inv2h * ((LN->w - LS->w) - (LB->v - LF->v));
+            const double WY = inv2h * ((LB->u - LF->u) - (LE->w - LW->w));
+            const double WZ = inv2h * ((LE->v - LW->v) - (LN->u - LS->u));
+            const double dPdx = inv2h * (LE->p - LW->p);
+            const double dPdy = inv2h * (LN->p - LS->p);
+            const double dPdz = inv2h * (LB->p - LF->p);
+            R[15] -= (1 - X) * hCube * (dPdx * L->u + dPdy * L->v + dPdz * L->w);
+            const double lapU = invHh * (LE->u + LW->u + LN->u + LS->u + LB->u + LF->u - 6 * L->u);
+            const double lapV = invHh * (LE->v + LW->v + LN->v + LS->v + LB->v + LF->v - 6 * L->v);
+            const double lapW = invHh * (LE->w + LW->w + LN->w + LS->w + LB->w + LF->w - 6 * L->w);
+            const double V1 = lapU * L->u + lapV * L->v + lapW * L->w;
+            const double D11 = inv2h * (LE->u - LW->u);
+            const double D22 = inv2h * (LN->v - LS->v);
+            const double D33 = inv2h * (LB->w - LF->w);
+            const double D12 = inv2h * (LN->u - LS->u + LE->v - LW->v) / 2;
+            const double D13 = inv2h * (LB->u - LF->u + LE->w - LW->w) / 2;
+            const double D23 = inv2h * (LN->w - LS->w + LB->v - LF->v) / 2;
+            const double V2 = D11 * D11 + D22 * D22 + D33 * D33 + 2 * (D12 * D12 + D13 * D13 + D23 * D23);
+            R[16] += (1 - X) * hCube * nu * (V1 + 2 * V2);
+            R[0] += hCube * WX;
+            R[1] += hCube * WY;
+            R[2] += hCube * WZ;
+            R[3] += hCube / 2 * (PY * WZ - PZ * WY);
+            R[4] += hCube / 2 * (PZ * WX - PX * WZ);
+            R[5] += hCube / 2 * (PX * WY - PY * WX);
+            R[6] += hCube * L->u;
+            R[7] += hCube * L->v;
+            R[8] += hCube * L->w;
+            R[9] += hCube / 3 * (PX * (PY * WY + PZ * WZ) - WX * (PY * PY + PZ * PZ));
+            R[10] += hCube / 3 * (PY * (PX * WX + PZ * WZ) - WY * (PX * PX + PZ * PZ));
+            R[11] += hCube / 3 * (PZ * (PX * WX + PY * WY) - WZ * (PX * PX + PY * PY));
+            R[12] += hCube * (PY * L->w - PZ * L->v);
+            R[13] += hCube * (PZ * L->u - PX * L->w);
+            R[14] += hCube * (PX * L->v - PY * L->u);
+            R[17] += hCube * (WX * L->u + WY * L->v + WZ * L->w);
+            R[18] += hCube * (L->u * L->u + L->v * L->v + L->w * L->w) / 2;
+            R[19] += sqrt(WX * WX + WY * WY + WZ * WZ);
+          }
+    }
+    free(l.c);
+  }
+  for (int i = 0; i < 20; i++) out[i] = R[i];
+}
+
+/* ------------------------------------------------------------ spectral forcing */
+
+static void field_to_grid(const orc_sim* s, int f, double complex* c)
+{
+  #pragma omp parallel for schedule(static)
+  for (int b = 0; b < s->nb; b++) {
+    int bc[3];
+    bcoords(s, b, bc);
+    for (int iz = 0; iz < BS; iz++)
+      for (int iy = 0; iy < BS; iy++)
+        for (int ix = 0; ix < BS; ix++)
+          c[ridx(s, bc[0] * BS + ix, bc[1] * BS + iy, bc[2] * BS + iz)] =
+              *fe_field((FE*)&s->blk[b].d[iz][iy][ix], f);
+  }
+}
+
+static inline int signed_k(int i, int n) { return i <= n / 2 ? i : -(n - i); }
+
+/* SpectralForcing::operator() (SpectralForcing.cpp:23-82) with
+ * _compute_largeModesForcing (SpectralManipFFTW.cpp:25-108). The half-spectrum sums with
+ * weight mult/2 equal full-spectrum sums with weight 1/2. */
+void orc_spectral_forcing(orc_sim* s, double dt, double sim_dt, double nu,
+                          double tke_target, double injection_rate, double stats_out[5])
+{
+  const int N0 = s->N[0], N1 = s->N[1], N2 = s->N[2];
+  const size_t n = (size_t)N0 * N1 * N2;
+  double complex* cu = (double complex*)malloc(sizeof(double complex) * n);
+  double complex* cv = (double complex*)malloc(sizeof(double complex) * n);
+  double complex* cw = (double complex*)malloc(sizeof(double complex) * n);
+  field_to_grid(s, ORC_U, cu);
+  field_to_grid(s, ORC_V, cv);
+  field_to_grid(s, ORC_W, cw);
+  fft3d_c(cu, N0, N1, N2, -1);
+  fft3d_c(cv, N0, N1, N2, -1);
+  fft3d_c(cw, N0, N1, N2, -1);
+  const double wfx = 2.0 * M_PI / s->extent[0], wfy = 2.0 * M_PI / s->extent[1], wfz = 2.0 * M_PI / s->extent[2];
+  const double dfx = 2 * M_PI / N0, dfy = 2 * M_PI / N1, dfz = 2 * M_PI / N2;
+  double eps = 0, tke = 0, tkeF = 0, lInt = 0;
+  #pragma omp parallel for schedule(static) reduction(+ : eps, tke, tkeF, lInt)
+  for (int i = 0; i < N0; i++)
+    for (int l = 0; l < N1; l++)
+      for (int k = 0; k < N2; k++) {
+        const size_t id = ((size_t)i * N1 + l) * N2 + k;
+        const int ii = signed_k(i, N0), jj = signed_k(l, N1), kk = signed_k(k, N2);
+        const double kx = ii * wfx, ky = jj * wfy, kz = kk * wfz;
+        const double k2 = kx * kx + ky * ky + kz * kz;
+        const double dX = 2 * sin(dfx * ii), dY = 2 * sin(dfy * jj), dZ = 2 * sin(dfz * kk);
+        const double complex U = cu[id], V = cv[id], W = cw[id];
+        /* i*d applied: d(.)/dx -> i * dX * (.) */
+        const double complex OX = I * dY * W - I * dZ * V;
+        const double complex OY = I * dZ * U - I * dX * W;
+        const double complex OZ = I * dX * V - I * dY * U;
+        const double E = 0.5 * (creal(U * conj(U)) + creal(V * conj(V)) + creal(W * conj(W)));
+        tke += E;
+        eps += 0.5 * (creal(OX * conj(OX)) + creal(OY * conj(OY)) + creal(OZ * conj(OZ)));
+        lInt += (k2 > 0) ? E / sqrt(k2) : 0;
+        if (k2 > 0 && k2 <= 4) tkeF += E;
+        else cu[id] = cv[id] = cw[id] = 0;
+      }
+  const double normFFT = (double)N0 * N1 * N2;
+  const double st_tkeF = tkeF / (normFFT * normFFT);
+  const double st_tke = tke / (normFFT * normFFT);
+  const double st_eps = eps * 2 * nu / pow(normFFT * 2 * s->h, 2);
+  const double st_lint = lInt / tke;
+  double inj = 0;
+  if (tke_target > 0) inj = (tke_target - st_tke) / dt;
+  else if (injection_rate > 0) inj = injection_rate;
+  const double fac = sim_dt * inj / (2 * st_tkeF);
+  if (fac > 0) {
+    fft3d_c(cu, N0, N1, N2, +1);
+    fft3d_c(cv, N0, N1, N2, +1);
+    fft3d_c(cw, N0, N1, N2, +1);
+    const double f = fac / normFFT;
+    #pragma omp parallel for schedule(static)
+    for (int b = 0; b < s->nb; b++) {
+      int bc[3];
+      bcoords(s, b, bc);
+      for (int iz = 0; iz < BS; iz++)
+        for (int iy = 0; iy < BS; iy++)
+          for (int ix = 0; ix < BS; ix++) {
+            const size_t id = ridx(s, bc[0] * BS + ix, bc[1] * BS + iy, bc[2] * BS + iz);
+            FE* e = &s->blk[b].d[iz][iy][ix];
+            e->u += f * creal(cu[id]);
+            e->v += f * creal(cv[id]);
+            e->w += f * creal(cw[id]);
+          }
+    }
+  }
+  if (stats_out) {
+    stats_out[0] = st_tke;
+    stats_out[1] = st_eps;
+    stats_out[2] = st_tkeF;
+    stats_out[3] = st_lint;
+    stats_out[4] = fac;
+  }
+  free(cu); free(cv); free(cw);
+}
+
+/* ---------------------------------------------------------- initial conditions */
+
+/* KernelIC_taylorGreen (InitialConditions.cpp:32-52). */
+void orc_ic_taylor_green(orc_sim* s, double uMax)
+{
+  const double a = 2 * M_PI / s->extent[0], b_ = 2 * M_PI / s->extent[1], c = 2 * M_PI / s->extent[2];
+  const double A = uMax, B = -uMax * s->extent[1] / s->extent[0];
+  #pragma omp parallel for schedule(static)
+  for (int b = 0; b < s->nb; b++) {
+    int bc[3];
+    bcoords(s, b, bc);
+    for (int iz = 0; iz < BS; iz++)
+      for (int iy = 0; iy < BS; iy++)
+        for (int ix = 0; ix < BS; ix++) {
+          FE* e = &s->blk[b].d[iz][iy][ix];
+          memset(e, 0, sizeof(FE));
+          double p[3];
+          cell_pos(s, bc, ix, iy, iz, p);
+          e->u = A * cos(a * p[0]) * sin(b_ * p[1]) * sin(c * p[2]);
+          e->v = B * sin(a * p[0]) * cos(b_ * p[1]) * sin(c * p[2]);
+        }
+  }
+}
+
+/* Philox-4x32-10 (Salmon et al., SC'11), the counter-based generator of the seeded
+ * isotropic-turbulence initial condition. */
+static void philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+  uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3], k0 = key[0], k1 = key[1];
+  for (int r = 0; r < 10; r++) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0, n1 = (uint32_t)p1;
+    const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1, n3 = (uint32_t)p0;
+    c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+void orc_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) { philox4x32_10(ctr, key, out); }
+
+/* Seeded isotropic-turbulence IC. Construction of each mode follows
+ * SpectralManipFFTW::_compute_IC (SpectralManipFFTW.cpp:217-288) -- Rogallo's solenoidal
+ * combination of two random complex amplitudes -- with the "art" spectrum
+ * E(k) = 32/(3 k0) sqrt(2/pi) (k/k0)^4 exp(-2 (k/k0)^2) (SpectralIcGenerator.cpp:63-81)
+ * evaluated at |k|, with these fixed differences: phases from Philox-4x32-10 keyed by the
+ * seed and counted by the canonical mode index, the imaginary part of u built from
+ * noise_a[1] (the reference reads noise_a[0] there, :277), exact Hermitian symmetry
+ * (every mode with a negative partner takes the conjugate of its partner), DC and every
+ * Nyquist index zeroed, and the field scaled so that sqrt(<u^2+v^2+w^2>/3) = u_rms. */
+void orc_ic_hit(orc_sim* s, double k0, double u_rms, uint64_t seed)
+{
+  const int N0 = s->N[0], N1 = s->N[1], N2 = s->N[2];
+  const size_t n = (size_t)N0 * N1 * N2;
+  double complex* cu = (double complex*)calloc(n, sizeof(double complex));
+  double complex* cv = (double complex*)calloc(n, sizeof(double complex));
+  double complex* cw = (double complex*)calloc(n, sizeof(double complex));
+  const double wfx = 2.0 * M_PI / s->extent[0], wfy = 2.0 * M_PI / s->extent[1], wfz = 2.0 * M_PI / s->extent[2];
+  const double coef = 32.0 / (3 * k0) * sqrt(2.0 / M_PI);
+  const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  #pragma omp parallel for schedule(static)
+  for (int i = 0; i < N0; i++)
+    for (int l = 0; l < N1; l++)
+      for (int k = 0; k < N2; k++) {
+        const size_t id = ((size_t)i * N1 + l) * N2 + k;
+        int ii = signed_k(i, N0), jj = signed_k(l, N1), kk = signed_k(k, N2);
+        if (2 * abs(ii) == N0 || 2 * abs(jj) == N1 || 2 * abs(kk) == N2) continue;
+        if (ii == 0 && jj == 0 && kk == 0) continue;
+        const int canonical = kk > 0 || (kk == 0 && (jj > 0 || (jj == 0 && ii > 0)));
+        if (!canonical) { ii = -ii; jj = -jj; kk = -kk; }
+        const size_t cid = ((size_t)wrap(ii, N0) * N1 + wrap(jj, N1)) * N2 + wrap(kk, N2);
+        const double kx = ii * wfx, ky = jj * wfy, kz = kk * wfz;
+        const double k2 = kx * kx + ky * ky + kz * kz;
+        const double k_xy = sqrt(kx * kx + ky * ky), k_norm = sqrt(k2);
+        const double r = k_norm / k0;
+        const double E_k = coef * r * r * r * r * exp(-2 * r * r);
+        const double amp = sqrt(E_k / (2 * M_PI * pow(k_norm / wfx, 2)));
+        const uint32_t ctr[4] = {(uint32_t)cid, (uint32_t)((uint64_t)cid >> 32), 0, 0};
+        uint32_t o[4];
+        philox4x32_10(ctr, key, o);
+        const double theta1 = (o[0] + 0.5) * (1.0 / 4294967296.0) * 2 * M_PI;
+        const double theta2 = (o[1] + 0.5) * (1.0 / 4294967296.0) * 2 * M_PI;
+        const double phi = (o[2] + 0.5) * (1.0 / 4294967296.0) * 2 * M_PI;
+        const double complex na = amp * cos(phi) * (cos(theta1) + I * sin(theta1));
+        const double complex nb = amp * sin(phi) * (cos(theta2) + I * sin(theta2));
+        const double fac = k_norm * k_xy, invFac = fac <= 0 ? 0 : 1 / fac;
+        double complex U = invFac * (na * k_norm * ky + nb * kx * kz);
+        double complex V = invFac * (nb * ky * kz - na * k_norm * kx);
+        double complex W = -nb * k_xy / k_norm;
+        if (!canonical) { U = conj(U); V = conj(V); W = conj(W); }
+        cu[id] = U; cv[id] = V; cw[id] = W;
+      }
+  fft3d_c(cu, N0, N1, N2, +1);
+  fft3d_c(cv, N0, N1, N2, +1);
+  fft3d_c(cw, N0, N1, N2, +1);
+  double ssq = 0;
+  #pragma omp parallel for schedule(static) reduction(+ : ssq)
+  for (size_t i = 0; i < n; i++) {
+    const double a = creal(cu[i]), b = creal(cv[i]), c = creal(cw[i]);
+    ssq += a * a + b * b + c * c;
+  }
+  const double scale = u_rms / sqrt(ssq / (3.0 * n));
+  #pragma omp parallel for schedule(static)
+  for (int b = 0; b < s->nb; b++) {
+    int bc[3];
+    bcoords(s, b, bc);
+    for (int iz = 0; iz < BS; iz++)
+      for (int iy = 0; iy < BS; iy++)
+        for (int ix = 0; ix < BS; ix++) {
+          const size_t id = ridx(s, bc[0] * BS + ix, bc[1] * BS + iy, bc[2] * BS + iz);
+          FE* e = &s->blk[b].d[iz][iy][ix];
+          memset(e, 0, sizeof(FE));
+          e->u = scale * creal(cu[id]);
+          e->v = scale * creal(cv[id]);
+          e->w = scale * creal(cw[id]);
+        }
+  }
+  free(cu); free(cv); free(cw);
+}
+
+/* ------------------------------------------------------------------ sphere */
+
+static void ob_write(OBlock* o, int ix, int iy, int iz, double delta, double gx, double gy, double gz)
+{
+  if (o->npts == o->cap) {
+    o->cap = o->cap ? 2 * o->cap : 64;
+    o->pix = (int*)realloc(o->pix, sizeof(int) * o->cap);
+    o->dchidx = (double*)realloc(o->dchidx, sizeof(double) * o->cap);
+    o->dchidy = (double*)realloc(o->dchidy, sizeof(double) * o->cap);
+    o->dchidz = (double*)realloc(o->dchidz, sizeof(double) * o->cap);
+    o->delta = (double*)realloc(o->delta, sizeof(double) * o->cap);
+  }
+  const int k = o->npts++;
+  o->pix[k] = (iz * BS + iy) * BS + ix;
+  o->dchidx[k] = -delta * gx; /* ObstacleBlock::write (ObstacleBlock.h:157-166) */
+  o->dchidy[k] = -delta * gy;
+  o->dchidz[k] = -delta * gz;
+  o->delta[k] = delta;
+}
+
+/* CreateObstacles for one sphere (ObstaclesCreate.cpp:444-540): reset chi=0 and
+ * tmpU=-1, SphereObstacle::FillBlocks over the blocks its box touches (Sphere.cpp:17-50,
+ * ObstacleLibrary.h:41-69, Obstacle.h:186-207), KernelCharacteristicFunction
+ * (ObstaclesCreate.cpp:22-137), KernelComputeGridCoM (:230-256). The sphere's udef is
+ * zero, so the udef-momentum steps (:258-440) leave everything unchanged. */
+void orc_sphere_create(orc_sim* s, double radius, const double pos[3])
+{
+  free_sphere(s);
+  s->has_sphere = 1;
+  s->radius = radius;
+  for (int d = 0; d < 3; d++) s->spos[d] = pos[d];
+  s->ob = (OBlock**)calloc((size_t)s->nb, sizeof(OBlock*));
+  const double h = s->h, safety = (2 + SURFDH) * h;
+  double box[3][2];
+  for (int d = 0; d < 3; d++) {
+    box[d][0] = pos[d] - (radius + safety);
+    box[d][1] = pos[d] + (radius + safety);
+  }
+  #pragma omp parallel for schedule(static)
+  for (int b = 0; b < s->nb; b++) {
+    FE* e = &s->blk[b].d[0][0][0];
+    for (int i = 0; i < BS * BS * BS; i++) { e[i].chi = 0; e[i].tmpU = -1; }
+  }
+  #pragma omp parallel for schedule(static)
+  for (int b = 0; b < s->nb; b++) {
+    int bc[3];
+    bcoords(s, b, bc);
+    double mn[3], mx[3];
+    cell_pos(s, bc, 0, 0, 0, mn);
+    cell_pos(s, bc, BS - 1, BS - 1, BS - 1, mx);
+    int touching = 1;
+    for (int d = 0; d < 3; d++) {
+      const double lo = mn[d] > box[d][0] ? mn[d] : box[d][0];
+      const double hi = mx[d] < box[d][1] ? mx[d] : box[d][1];
+      if (!(hi - lo > 0)) touching = 0;
+    }
+    if (!touching) continue;
+    OBlock* o = (OBlock*)calloc(1, sizeof(OBlock));
+    s->ob[b] = o;
+    for (int iz = 0; iz < BS; iz++)
+      for (int iy = 0; iy < BS; iy++)
+        for (int ix = 0; ix < BS; ix++) {
+          double p[3];
+          cell_pos(s, bc, ix, iy, iz, p);
+          const double dx = p[0] - pos[0], dy = p[1] - pos[1], dz = p[2] - pos[2];
+          const double dist = radius - sqrt(dx * dx + dy * dy + dz * dz);
+          o->sdf[iz][iy][ix] = dist;
+          FE* e = &s->blk[b].d[iz][iy][ix];
+          e->tmpU = dist > e->tmpU ? dist : e->tmpU;
+        }
+  }
+  const double inv2h = .5 / h, fac1 = .5 * h * h, vol = h * h * h;
+  #pragma omp parallel
+  {
+    Lab l = lab_alloc(1);
+    #pragma omp for schedule(static)
+    for (int b = 0; b < s->nb; b++) {
+      OBlock* o = s->ob[b];
+      if (!o) continue;
+      lab_load(s, b, &l);
+      int bc[3];
+      bcoords(s, b, bc);
+      o->CoM[0] = o->CoM[1] = o->CoM[2] = o->mass = 0;
+      for (int iz = 0; iz < BS; iz++)
+        for (int iy = 0; iy < BS; iy++)
+          for (int ix = 0; ix < BS; ix++) {
+            const double sdf0 = o->sdf[iz][iy][ix];
+            if (sdf0 > +2 * h || sdf0 < -2 * h) {
+              o->chi[iz][iy][ix] = sdf0 > 0 ? 1 : 0;
+            } else {
+              const double distPx = LAB(&l, ix + 1, iy, iz)->tmpU, distMx = LAB(&l, ix - 1, iy, iz)->tmpU;
+              const double distPy = LAB(&l, ix, iy + 1, iz)->tmpU, distMy = LAB(&l, ix, iy - 1, iz)->tmpU;
+              const double distPz = LAB(&l, ix, iy, iz + 1)->tmpU, distMz = LAB(&l, ix, iy, iz - 1)->tmpU;
+              const double gradUX = inv2h * (distPx - distMx);
+              const double gradUY = inv2h * (distPy - distMy);
+              const double gradUZ = inv2h * (distPz - distMz);
+              const double gradUSq = gradUX * gradUX + gradUY * gradUY + gradUZ * gradUZ + EPS;
+              const double IplusX = distPx < 0 ? 0 : distPx, IminuX = distMx < 0 ? 0 : distMx;
+              const double IplusY = distPy < 0 ? 0 : distPy, IminuY = distMy < 0 ? 0 : distMy;
+              const double IplusZ = distPz < 0 ? 0 : distPz, IminuZ = distMz < 0 ? 0 : distMz;
+              const double gradIX = inv2h * (IplusX - IminuX);
+              const double gradIY = inv2h * (IplusY - IminuY);
+              const double gradIZ = inv2h * (IplusZ - IminuZ);
+              const double numH = gradIX * gradUX + gradIY * gradUY + gradIZ * gradUZ;
+              o->chi[iz][iy][ix] = numH / gradUSq;
+            }
+            double p[3];
+            cell_pos(s, bc, ix, iy, iz, p);
+            FE* e = &s->blk[b].d[iz][iy][ix];
+            const double X = o->chi[iz][iy][ix];
+            e->chi = X > e->chi ? X : e->chi;
+            o->CoM[0] += X * vol * p[0];
+            o->CoM[1] += X * vol * p[1];
+            o->CoM[2] += X * vol * p[2];
+            o->mass += X * vol;
+            const double sdf = sdf0 + h * SURFDH;
+            if (sdf > +2 * h || sdf < -2 * h) continue;
+            const double distPx = LAB(&l, ix + 1, iy, iz)->tmpU + h * SURFDH;
+            const double distMx = LAB(&l, ix - 1, iy, iz)->tmpU + h * SURFDH;
+            const double distPy = LAB(&l, ix, iy + 1, iz)->tmpU + h * SURFDH;
+            const double distMy = LAB(&l, ix, iy - 1, iz)->tmpU + h * SURFDH;
+            const double distPz = LAB(&l, ix, iy, iz + 1)->tmpU + h * SURFDH;
+            const double distMz = LAB(&l, ix, iy, iz - 1)->tmpU + h * SURFDH;
+            const double gradUX = inv2h * (distPx - distMx);
+            const double gradUY = inv2h * (distPy - distMy);
+            const double gradUZ = inv2h * (distPz - distMz);
+            const double gradUSq = gradUX * gradUX + gradUY * gradUY + gradUZ * gradUZ + EPS;
+            const double HplusX = fabs(distPx) < EPS ? 0.5 : (distPx < 0 ? 0 : 1);
+            const double HminuX = fabs(distMx) < EPS ? 0.5 : (distMx < 0 ? 0 : 1);
+            const double HplusY = fabs(distPy) < EPS ? 0.5 : (distPy < 0 ? 0 : 1);
+            const double HminuY = fabs(distMy) < EPS ? 0.5 : (distMy < 0 ? 0 : 1);
+            const double HplusZ = fabs(distPz) < EPS ? 0.5 : (distPz < 0 ? 0 : 1);
+            const double HminuZ = fabs(distMz) < EPS ? 0.5 : (distMz < 0 ? 0 : 1);
+            const double gradHX = HplusX - HminuX, gradHY = HplusY - HminuY, gradHZ = HplusZ - HminuZ;
+            const double numD = gradHX * gradUX + gradHY * gradUY + gradHZ * gradUZ;
+            const double Delta = fac1 * numD / gradUSq;
+            if (Delta > EPS) ob_write(o, ix, iy, iz, Delta, gradUX, gradUY, gradUZ);
+          }
+    }
+    free(l.c);
+  }
+  double com[4] = {0, 0, 0, 0};
+  for (int b = 0; b < s->nb; b++) {
+    if (!s->ob[b]) continue;
+    com[0] += s->ob[b]->mass;
+    com[1] += s->ob[b]->CoM[0];
+    com[2] += s->ob[b]->CoM[1];
+    com[3] += s->ob[b]->CoM[2];
+  }
+  s->mass = com[0];
+  for (int d = 0; d < 3; d++) s->CoM[d] = com[d + 1] / com[0];
+}
+
+void orc_sphere_info(const orc_sim* s, double out[8])
+{
+  int nbk = 0, npts = 0;
+  for (int b = 0; s->ob && b < s->nb; b++)
+    if (s->ob[b]) { nbk++; npts += s->ob[b]->npts; }
+  out[0] = s->CoM[0]; out[1] = s->CoM[1]; out[2] = s->CoM[2];
+  out[3] = s->mass; out[4] = nbk; out[5] = npts; out[6] = s->radius; out[7] = 0;
+}
+
+/* KernelPenalization<implicit> + KernelFinalizePenalizationForce
+ * (Penalization.cpp:20-124), one body with udef = 0 and lambda_factor = 1. */
+void orc_penalize(orc_sim* s, double dt, double lambda, int implicit,
+                  const double vel[3], const double omega[3], double ft_out[6])
+{
+  double M[6] = {0};
+  if (!s->has_sphere) { for (int i = 0; i < 6; i++) ft_out[i] = 0; return; }
+  const double dv = pow(s->h, 3);
+  const double lambdaFac = implicit ? lambda : 1.0 / dt;
+  #pragma omp parallel for schedule(static) reduction(+ : M[:6])
+  for (int b = 0; b < s->nb; b++) {
+    const OBlock* o = s->ob[b];
+    if (!o) continue;
+    int bc[3];
+    bcoords(s, b, bc);
+    for (int iz = 0; iz < BS; iz++)
+      for (int iy = 0; iy < BS; iy++)
+        for (int ix = 0; ix < BS; ix++) {
+          FE* e = &s->blk[b].d[iz][iy][ix];
+          const double X = o->chi[iz][iy][ix];
+          if (e->chi > X) continue;
+          if (X <= 0) continue;
+          double p[3];
+          cell_pos(s, bc, ix, iy, iz, p);
+          p[0] -= s->CoM[0]; p[1] -= s->CoM[1]; p[2] -= s->CoM[2];
+          const double U_TOT[3] = {vel[0] + omega[1] * p[2] - omega[2] * p[1],
+                                   vel[1] + omega[2] * p[0] - omega[0] * p[2],
+                                   vel[2] + omega[0] * p[1] - omega[1] * p[0]};
+          const double penalFac = implicit ? X * lambdaFac / (1 + lambdaFac * dt * X) : X * lambdaFac;
+          const double FPX = penalFac * (U_TOT[0] - e->u);
+          const double FPY = penalFac * (U_TOT[1] - e->v);
+          const double FPZ = penalFac * (U_TOT[2] - e->w);
+          e->u = e->u + dt * FPX;
+          e->v = e->v + dt * FPY;
+          e->w = e->w + dt * FPZ;
+          M[0] += dv * FPX; M[1] += dv * FPY; M[2] += dv * FPZ;
+          M[3] += dv * (p[1] * FPZ - p[2] * FPY);
+          M[4] += dv * (p[2] * FPX - p[0] * FPZ);
+          M[5] += dv * (p[0] * FPY - p[1] * FPX);
+        }
+  }
+  for (int i = 0; i < 6; i++) ft_out[i] = M[i];
+}
+
+/* KernelComputeForces (FluidSolidForces.cpp:17-158) summed as in Obstacle::computeForces
+ * (Obstacle.cpp:266-286); out[] in ObstacleBlock::sumQoI order (ObstacleBlock.h:81-93). */
+void orc_surface_forces(const orc_sim* s, double nu, const double uTrans[3], const double omega[3],
+                        double out[22])
+{
+  double Q[22] = {0};
+  if (!s->has_sphere) { for (int i = 0; i < 22; i++) out[i] = 0; return; }
+  double velUnit[3] = {0, 0, 0};
+  const double vel_norm = sqrt(uTrans[0] * uTrans[0] + uTrans[1] * uTrans[1] + uTrans[2] * uTrans[2]);
+  if (vel_norm > 1e-9)
+    for (int d = 0; d < 3; d++) velUnit[d] = uTrans[d] / vel_norm;
+  const double _1oH = nu / s->h;
+  const double* CM = s->CoM;
+  #pragma omp parallel reduction(+ : Q[:22])
+  {
+    Lab l = lab_alloc(1);
+    #pragma omp for schedule(static)
+    for (int b = 0; b < s->nb; b++) {
+      const OBlock* o = s->ob[b];
+      if (!o || o->npts == 0) continue;
+      lab_load(s, b, &l);
+      int bc[3];
+      bcoords(s, b, bc);
+      for (int i = 0; i < o->npts; i++) {
+        const int ix = o->pix[i] % BS, iy = (o->pix[i] / BS) % BS, iz = o->pix[i] / (BS * BS);
+        double p[3];
+        cell_pos(s, bc, ix, iy, iz, p);
+        const double D11 = _1oH * (LAB(&l, ix + 1, iy, iz)->u - LAB(&l, ix - 1, iy, iz)->u);
+        const double D22 = _1oH * (LAB(&l, ix, iy + 1, iz)->v - LAB(&l, ix, iy - 1, iz)->v);
+        const double D33 = _1oH * (LAB(&l, ix, iy, iz + 1)->w - LAB(&l, ix, iy, iz - 1)->w);
+        const double D12 = .5 * _1oH * (LAB(&l, ix, iy + 1, iz)->u - LAB(&l, ix, iy - 1, iz)->u +
+                                        LAB(&l, ix + 1, iy, iz)->v - LAB(&l, ix - 1, iy, iz)->v);
+        const double D13 = .5 * _1oH * (LAB(&l, ix, iy, iz + 1)->u - LAB(&l, ix, iy, iz - 1)->u +
+                                        LAB(&l, ix + 1, iy, iz)->w - LAB(&l, ix - 1, iy, iz)->w);
+        const double D23 = .5 * _1oH * (LAB(&l, ix, iy + 1, iz)->w - LAB(&l, ix, iy - 1, iz)->w +
+                                        LAB(&l, ix, iy, iz + 1)->v - LAB(&l, ix, iy, iz - 1)->v);
+        const FE* C = LAB(&l, ix, iy, iz);
+        const double P = C->p;
+        const double normX = o->dchidx[i], normY = o->dchidy[i], normZ = o->dchidz[i];
+        const double fXV = D11 * normX + D12 * normY + D13 * normZ;
+        const double fYV = D12 * normX + D22 * normY + D23 * normZ;
+        const double fZV = D13 * normX + D23 * normY + D33 * normZ;
+        const double fXP = -P * normX, fYP = -P * normY, fZP = -P * normZ;
+        const double fXT = fXV + fXP, fYT = fYV + fYP, fZT = fZV + fZP;
+        const double vX = C->u, vY = C->v, vZ = C->w;
+        Q[0] += fXT; Q[1] += fYT; Q[2] += fZT;
+        Q[3] += fXP; Q[4] += fYP; Q[5] += fZP;
+        Q[6] += fXV; Q[7] += fYV; Q[8] += fZV;
+        Q[9] += (p[1] - CM[1]) * fZT - (p[2] - CM[2]) * fYT;
+        Q[10] += (p[2] - CM[2]) * fXT - (p[0] - CM[0]) * fZT;
+        Q[11] += (p[0] - CM[0]) * fYT - (p[1] - CM[1]) * fXT;
+        Q[12] += normY * vZ - normZ * vY;
+        Q[13] += normZ * vX - normX * vZ;
+        Q[14] += normX * vY - normY * vX;
+        const double forcePar = fXT * velUnit[0] + fYT * velUnit[1] + fZT * velUnit[2];
+        Q[16] += .5 * (forcePar + fabs(forcePar));
+        Q[15] -= .5 * (forcePar - fabs(forcePar));
+        const double powOut = fXT * vX + fYT * vY + fZT * vZ;
+        const double powDef = 0; /* udef = 0 */
+        Q[17] += powOut;
+        Q[18] += powOut < 0 ? powOut : 0;
+        Q[19] += powDef;
+        Q[20] += powDef < 0 ? powDef : 0;
+        const double rV[3] = {p[0] - CM[0], p[1] - CM[1], p[2] - CM[2]};
+        const double uS[3] = {uTrans[0] + omega[1] * rV[2] - rV[1] * omega[2],
+                              uTrans[1] + omega[2] * rV[0] - rV[2] * omega[0],
+                              uTrans[2] + omega[0] * rV[1] - rV[0] * omega[1]};
+        Q[21] += fXT * uS[0] + fYT * uS[1] + fZT * uS[2];
+      }
+    }
+    free(l.c);
+  }
+  for (int i = 0; i < 22; i++) out[i] = Q[i];
+}
+
+/* ---------------------------------------------------------------- timestep */
+
+/* Simulation::timestep (Simulation.cpp:593-623) over the pipeline of setupOperators
+ * (:307-401) for the periodic, non-iterative configuration: [CreateObstacles: a static
+ * sphere is created once by orc_sphere_create], AdvectionDiffusion, [SpectralForcing],
+ * PressureRHS, PressureProjection, [UpdateObstacles: forced body, velocities unchanged],
+ * [Penalization], [ComputeForces]. ComputeDissipation is called separately. */
+void orc_timestep(orc_sim* s, double dt, const orc_params* prm, double forcing_stats[5],
+                  double penal_ft[6], double surf_qoi[22])
+{
+  orc_advect_diffuse(s, dt, prm->nu, prm->uinf);
+  if (prm->spectral_forcing)
+    orc_spectral_forcing(s, dt, dt, prm->nu, prm->tke_target, prm->injection_rate, forcing_stats);
+  orc_pressure_rhs(s, dt);
+  orc_poisson_solve(s);
+  orc_grad_p(s, dt);
+  if (prm->has_sphere && s->has_sphere) {
+    double ft[6];
+    orc_penalize(s, dt, prm->lambda, prm->implicit_penalization, prm->trans_vel, prm->ang_vel, ft);
+    if (penal_ft) for (int i = 0; i < 6; i++) penal_ft[i] = ft[i];
+    double q[22];
+    orc_surface_forces(s, prm->nu, prm->trans_vel, prm->ang_vel, q);
+    if (surf_qoi) for (int i = 0; i < 22; i++) surf_qoi[i] = q[i];
+  }
+}

@@ -1,0 +1,96 @@
+/*
+ * cup_oracle.h -- the oracle: an FP64 CPU restatement of CubismUP_3D's per-timestep
+ * fluid update on a uniform, periodic grid of 8^3-cell blocks.
+ *
+ * TEST INFRASTRUCTURE ONLY. This is the checker. It is built into oracle/liboracle.so
+ * and is loaded only by tests/, by __graft_entry__.smoke() and by bench.py's
+ * cpu_baseline leg. The product (cubismup_3d_amd) never links, loads or calls it.
+ *
+ * Field exchange format ("block-major SoA"): one double per cell, ordered
+ * [block][iz][iy][ix] with block = bx + bpdx*(by + bpdy*bz) and 8 cells per block edge.
+ * This is the order of the reference's FluidBlock data[z][y][x] (Definitions.h:115)
+ * walked over the grid's block list.
+ */
+#ifndef CUP_ORACLE_H
+#define CUP_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_CHI = 0, ORC_U, ORC_V, ORC_W, ORC_P, ORC_TMPU, ORC_TMPV, ORC_TMPW };
+
+typedef struct orc_sim orc_sim;
+
+/* Grid: bpd blocks of 8^3 per dimension, h = maxextent / max(cells)
+ * (SimulationData.cpp:184-195). */
+orc_sim* orc_create(int bpdx, int bpdy, int bpdz, double maxextent);
+void orc_destroy(orc_sim* s);
+double orc_h(const orc_sim* s);
+void orc_extent(const orc_sim* s, double out[3]);
+void orc_set_threads(int n);
+
+void orc_set_field(orc_sim* s, int f, const double* src);
+void orc_get_field(const orc_sim* s, int f, double* dst);
+
+/* Periodic lab test (TestBoundaries.cpp:19-75): tmpU = lab(-dx,-dy,-dz).u for a lab
+ * with halo width g. */
+void orc_lab_shift(orc_sim* s, int g, int dx, int dy, int dz);
+
+/* Reductions / dt (ProcessHelpers.h:74-99, Simulation.cpp:403-424). */
+double orc_find_max_u(const orc_sim* s, const double uinf[3]);
+double orc_calc_max_timestep(const orc_sim* s, double CFL, double nu, const double uinf[3],
+                             int step, int rampup, double* umax_out);
+
+/* Operators. */
+void orc_advect_diffuse(orc_sim* s, double dt, double nu, const double uinf[3]);
+void orc_pressure_rhs(orc_sim* s, double dt);
+void orc_poisson_solve(orc_sim* s);
+void orc_grad_p(orc_sim* s, double dt);
+void orc_get_rhs(const orc_sim* s, double* dst_blockmajor);
+void orc_set_rhs(orc_sim* s, const double* src_blockmajor);
+void orc_dissipation(const orc_sim* s, double nu, double out[20]);
+
+/* Spectral forcing (SpectralForcing.cpp:23-82, SpectralManipFFTW.cpp:25-108).
+ * stats_out: tke, eps, tke_filtered, l_integral, fac. */
+void orc_spectral_forcing(orc_sim* s, double dt, double sim_dt, double nu,
+                          double tke_target, double injection_rate, double stats_out[5]);
+
+/* Initial conditions. */
+void orc_ic_taylor_green(orc_sim* s, double uMax);
+void orc_ic_hit(orc_sim* s, double k0, double u_rms, uint64_t seed);
+
+/* Sphere (Sphere.cpp:17-50, ObstaclesCreate.cpp:22-137,230-256, Penalization.cpp,
+ * FluidSolidForces.cpp:17-158). */
+void orc_sphere_create(orc_sim* s, double radius, const double pos[3]);
+void orc_sphere_info(const orc_sim* s, double out[8]); /* CoM[3], mass, nblocks, npoints */
+void orc_penalize(orc_sim* s, double dt, double lambda, int implicit,
+                  const double transVel[3], const double angVel[3], double ft_out[6]);
+void orc_surface_forces(const orc_sim* s, double nu, const double transVel[3],
+                        const double angVel[3], double out[22]);
+
+/* One full step in the reference pipeline order (Simulation.cpp:307-401,593-623). */
+typedef struct {
+  double nu;
+  double uinf[3];
+  int spectral_forcing;
+  double tke_target;
+  double injection_rate;
+  int has_sphere;
+  double lambda;
+  int implicit_penalization;
+  double trans_vel[3];
+  double ang_vel[3];
+} orc_params;
+void orc_timestep(orc_sim* s, double dt, const orc_params* prm, double forcing_stats[5],
+                  double penal_ft[6], double surf_qoi[22]);
+
+/* Plain 3-D complex FFT, unnormalized, sign -1 forward / +1 backward; arrays [x][y][z]. */
+void orc_fft3d(int nx, int ny, int nz, double* re, double* im, int sign);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

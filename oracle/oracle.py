@@ -1,0 +1,245 @@
+"""ctypes front end of the oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()`` and by
+bench.py's cpu_baseline leg, never by the product package. The oracle is an FP64 CPU
+restatement of CubismUP_3D's per-timestep update (see cup_oracle.c for the reference
+file:line each function follows and for what pins it).
+
+Arrays handed to and from this module are numpy float64 "global" fields indexed
+[z][y][x] (x fastest). The C side stores blocks; ``to_blocks``/``from_blocks`` convert.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+BS = 8
+FIELDS = {"chi": 0, "u": 1, "v": 2, "w": 3, "p": 4, "tmpU": 5, "tmpV": 6, "tmpW": 7}
+
+_lib = None
+
+
+def build() -> str:
+    """Compile oracle/liboracle.so from cup_oracle.c with the Makefile beside it."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        build()
+    L = C.CDLL(_LIB_PATH)
+    d, i, p = C.c_double, C.c_int, C.c_void_p
+    dp = C.POINTER(C.c_double)
+    sig = {
+        "orc_create": (p, [i, i, i, d]),
+        "orc_destroy": (None, [p]),
+        "orc_h": (d, [p]),
+        "orc_extent": (None, [p, dp]),
+        "orc_set_threads": (None, [i]),
+        "orc_set_field": (None, [p, i, dp]),
+        "orc_get_field": (None, [p, i, dp]),
+        "orc_lab_shift": (None, [p, i, i, i, i]),
+        "orc_find_max_u": (d, [p, dp]),
+        "orc_calc_max_timestep": (d, [p, d, d, dp, i, i, dp]),
+        "orc_advect_diffuse": (None, [p, d, d, dp]),
+        "orc_pressure_rhs": (None, [p, d]),
+        "orc_poisson_solve": (None, [p]),
+        "orc_grad_p": (None, [p, d]),
+        "orc_get_rhs": (None, [p, dp]),
+        "orc_set_rhs": (None, [p, dp]),
+        "orc_dissipation": (None, [p, d, dp]),
+        "orc_spectral_forcing": (None, [p, d, d, d, d, d, dp]),
+        "orc_ic_taylor_green": (None, [p, d]),
+        "orc_ic_hit": (None, [p, d, d, C.c_uint64]),
+        "orc_sphere_create": (None, [p, d, dp]),
+        "orc_sphere_info": (None, [p, dp]),
+        "orc_penalize": (None, [p, d, d, i, dp, dp, dp]),
+        "orc_surface_forces": (None, [p, d, dp, dp, dp]),
+        "orc_fft3d": (None, [i, i, i, dp, dp, i]),
+        "orc_philox": (None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _vec(x, n=3):
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.float64).reshape(n))
+    return a
+
+
+def to_blocks(g: np.ndarray) -> np.ndarray:
+    """[Z][Y][X] global field -> block-major [block][z][y][x], block = bx + nbx*(by + nby*bz)."""
+    Z, Y, X = g.shape
+    nbz, nby, nbx = Z // BS, Y // BS, X // BS
+    return np.ascontiguousarray(
+        g.reshape(nbz, BS, nby, BS, nbx, BS).transpose(0, 2, 4, 1, 3, 5).reshape(-1))
+
+
+def from_blocks(b: np.ndarray, shape) -> np.ndarray:
+    Z, Y, X = shape
+    nbz, nby, nbx = Z // BS, Y // BS, X // BS
+    return np.ascontiguousarray(
+        b.reshape(nbz, nby, nbx, BS, BS, BS).transpose(0, 3, 1, 4, 2, 5).reshape(Z, Y, X))
+
+
+def philox4x32_10(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().orc_philox(c, k, o)
+    return [int(v) for v in o]
+
+
+def fft3d(a: np.ndarray, sign: int = -1) -> np.ndarray:
+    """Unnormalized 3-D complex DFT of a (A,B,C) array: sign -1 forward, +1 backward."""
+    re = np.ascontiguousarray(np.real(a), dtype=np.float64)
+    im = np.ascontiguousarray(np.imag(a), dtype=np.float64)
+    A, B, Cc = a.shape
+    lib().orc_fft3d(A, B, Cc, _dp(re), _dp(im), int(sign))
+    return re + 1j * im
+
+
+class OracleSim:
+    """FP64 CPU restatement of the reference's periodic block grid and operators."""
+
+    def __init__(self, bpd, maxextent: float = 2 * np.pi):
+        self.bpd = tuple(int(b) for b in bpd)
+        self.cells = tuple(BS * b for b in self.bpd)
+        self.shape = (self.cells[2], self.cells[1], self.cells[0])  # [z][y][x]
+        self._h = lib().orc_create(self.bpd[0], self.bpd[1], self.bpd[2], float(maxextent))
+        if not self._h:
+            raise ValueError(f"invalid blocks per dimension {bpd}")
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.orc_destroy(h)
+            self._h = None
+
+    @property
+    def h(self) -> float:
+        return lib().orc_h(self._h)
+
+    @property
+    def extent(self):
+        e = np.zeros(3)
+        lib().orc_extent(self._h, _dp(e))
+        return e
+
+    def set(self, name: str, g: np.ndarray):
+        b = to_blocks(np.asarray(g, dtype=np.float64))
+        lib().orc_set_field(self._h, FIELDS[name], _dp(b))
+
+    def get(self, name: str) -> np.ndarray:
+        b = np.empty(int(np.prod(self.shape)), dtype=np.float64)
+        lib().orc_get_field(self._h, FIELDS[name], _dp(b))
+        return from_blocks(b, self.shape)
+
+    def set_velocity(self, u, v, w):
+        self.set("u", u); self.set("v", v); self.set("w", w)
+
+    def velocity(self):
+        return self.get("u"), self.get("v"), self.get("w")
+
+    def lab_shift(self, g, dx, dy, dz):
+        lib().orc_lab_shift(self._h, int(g), int(dx), int(dy), int(dz))
+
+    def find_max_u(self, uinf=(0, 0, 0)) -> float:
+        return lib().orc_find_max_u(self._h, _dp(_vec(uinf)))
+
+    def calc_max_timestep(self, CFL, nu, uinf=(0, 0, 0), step=0, rampup=0):
+        um = C.c_double(0)
+        dt = lib().orc_calc_max_timestep(self._h, float(CFL), float(nu), _dp(_vec(uinf)),
+                                         int(step), int(rampup), C.byref(um))
+        return dt, um.value
+
+    def advect_diffuse(self, dt, nu, uinf=(0, 0, 0)):
+        lib().orc_advect_diffuse(self._h, float(dt), float(nu), _dp(_vec(uinf)))
+
+    def pressure_rhs(self, dt):
+        lib().orc_pressure_rhs(self._h, float(dt))
+
+    def rhs(self) -> np.ndarray:
+        b = np.empty(int(np.prod(self.shape)), dtype=np.float64)
+        lib().orc_get_rhs(self._h, _dp(b))
+        return from_blocks(b, self.shape)
+
+    def set_rhs(self, g):
+        b = to_blocks(np.asarray(g, dtype=np.float64))
+        lib().orc_set_rhs(self._h, _dp(b))
+
+    def poisson_solve(self):
+        lib().orc_poisson_solve(self._h)
+
+    def grad_p(self, dt):
+        lib().orc_grad_p(self._h, float(dt))
+
+    def dissipation(self, nu) -> np.ndarray:
+        out = np.zeros(20)
+        lib().orc_dissipation(self._h, float(nu), _dp(out))
+        return out
+
+    def spectral_forcing(self, dt, nu, tke_target=0.0, injection_rate=0.0, sim_dt=None):
+        st = np.zeros(5)
+        lib().orc_spectral_forcing(self._h, float(dt), float(dt if sim_dt is None else sim_dt),
+                                   float(nu), float(tke_target), float(injection_rate), _dp(st))
+        return dict(tke=st[0], eps=st[1], tke_filtered=st[2], l_integral=st[3], fac=st[4])
+
+    def ic_taylor_green(self, uMax):
+        lib().orc_ic_taylor_green(self._h, float(uMax))
+
+    def ic_hit(self, k0=4.0, u_rms=1.0, seed=20260101):
+        lib().orc_ic_hit(self._h, float(k0), float(u_rms), int(seed))
+
+    def sphere_create(self, radius, pos):
+        lib().orc_sphere_create(self._h, float(radius), _dp(_vec(pos)))
+
+    def sphere_info(self):
+        o = np.zeros(8)
+        lib().orc_sphere_info(self._h, _dp(o))
+        return dict(com=o[0:3].copy(), mass=o[3], nblocks=int(o[4]), npoints=int(o[5]))
+
+    def penalize(self, dt, lam, implicit, trans_vel, ang_vel=(0, 0, 0)):
+        ft = np.zeros(6)
+        lib().orc_penalize(self._h, float(dt), float(lam), int(bool(implicit)),
+                           _dp(_vec(trans_vel)), _dp(_vec(ang_vel)), _dp(ft))
+        return ft
+
+    def surface_forces(self, nu, trans_vel, ang_vel=(0, 0, 0)):
+        q = np.zeros(22)
+        lib().orc_surface_forces(self._h, float(nu), _dp(_vec(trans_vel)), _dp(_vec(ang_vel)), _dp(q))
+        return q
+
+    def timestep(self, dt, nu, uinf=(0, 0, 0), spectral_forcing=False, tke_target=0.0,
+                 sphere=None):
+        """One step in the reference pipeline order. ``sphere``: dict(lambda_, implicit,
+        trans_vel, ang_vel) when a sphere was created with sphere_create."""
+        out = {}
+        self.advect_diffuse(dt, nu, uinf)
+        if spectral_forcing:
+            out["forcing"] = self.spectral_forcing(dt, nu, tke_target=tke_target)
+        self.pressure_rhs(dt)
+        self.poisson_solve()
+        self.grad_p(dt)
+        if sphere is not None:
+            out["penal_ft"] = self.penalize(dt, sphere["lambda_"], sphere["implicit"],
+                                            sphere["trans_vel"], sphere.get("ang_vel", (0, 0, 0)))
+            out["surface"] = self.surface_forces(nu, sphere["trans_vel"], sphere.get("ang_vel", (0, 0, 0)))
+        return out
