@@ -1,0 +1,179 @@
+"""Benchmark of the per-timestep periodic fluid update on one MI355X.
+
+Workload (the headline configuration): forced isotropic turbulence, periodic 2pi box at
+256^3 cells (32^3 blocks of 8^3), seeded solenoidal initial condition with the
+E(k) ~ k^4 exp(-2 (k/4)^2) spectrum scaled to u_rms = 1, band-limited linear forcing
+holding the kinetic energy at its initial value, nu = 0.005, CFL 0.3, no ramp. A "step"
+is one full timestep of the path: CFL dt from the previous step's max velocity,
+advection-diffusion, forcing, pressure RHS, FFT Poisson solve, projection.
+
+The FP32 form (fused decomposition, FP64 reductions and spectral division) is the
+measured line; the FP64 plain form (the reference's operator decomposition) is timed
+beside it on the same GPU, and the FP64 CPU oracle (OpenMP) on a bounded sample of the
+same workload on the host cores.
+
+    python bench.py [--gpus 1] [--steps 90] [--warmup 10]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+# Algorithmic bytes per cell and step, FP32 form (4-byte fields): every field materialised
+# between pipeline stages read once and written once, each 3-D FFT one read plus one write:
+# advection 24, pressure RHS 16, FFT r2c + division + c2r 16, gradient (+ reductions) 28.
+ALGO_BYTES_PER_CELL_FP32 = 84.0
+# Per-launch algorithmic bytes per cell of each stage of the fused FP32 step.
+STAGE_BYTES_PER_CELL = {"advdiff": 24.0, "xfwd": 16.0, "yfwd": 8.0, "zpass": 8.0, "yinv": 8.0, "xinv": 8.0,
+                        "gradp": 28.0}
+SEED = 20260101
+NU = 0.005
+CFL = 0.3
+
+
+def make_grid(cup, n, precision, decomposition):
+    G = cup.Grid((n // 8,) * 3, precision=precision, decomposition=decomposition)
+    G.ic_hit(4.0, 1.0, SEED)
+    return G
+
+
+def target_tke(G):
+    return G.diagnostics(NU)[18] / float(G.extent.prod())
+
+
+def time_steps(G, steps, tke, **kw):
+    G.synchronize()
+    t0 = time.perf_counter()
+    G.run_steps(steps, CFL=CFL, nu=NU, rampup=0, spectral_forcing=True, tke_target=tke, **kw)
+    G.synchronize()
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(n, warm=1, timed=3):
+    """FP64 oracle (OpenMP over blocks, the reference CPU path's structure) on the same
+    workload: `warm` + `timed` steps at n^3 on every host core."""
+    from oracle import oracle as O
+    O.build()
+    threads = os.cpu_count() or 1
+    O.lib().orc_set_threads(threads)
+    s = O.OracleSim((n // 8,) * 3)
+    s.ic_hit(4.0, 1.0, SEED)
+    u, v, w = s.velocity()
+    tke = 0.5 * float((u * u + v * v + w * w).mean())
+    del u, v, w
+    step = 0
+    for _ in range(warm):
+        dt, _ = s.calc_max_timestep(CFL, NU, (0, 0, 0), step, 0)
+        s.timestep(dt, NU, spectral_forcing=True, tke_target=tke)
+        step += 1
+    t0 = time.perf_counter()
+    for _ in range(timed):
+        dt, _ = s.calc_max_timestep(CFL, NU, (0, 0, 0), step, 0)
+        s.timestep(dt, NU, spectral_forcing=True, tke_target=tke)
+        step += 1
+    el = time.perf_counter() - t0
+    return {"value": n ** 3 * timed / el / 1e6, "unit": "Mcell-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{n}^3 forced isotropic turbulence, FP64 oracle, {warm} warm-up + {timed} timed steps "
+                      f"({el:.1f} s), OpenMP threads={threads}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=90)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--size", type=int, default=256, help="cells per side")
+    ap.add_argument("--plain-steps", type=int, default=20, help="timed steps of the FP64 plain form")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU decomposition is described, not built)", file=sys.stderr)
+        sys.exit(2)
+
+    import cubismup_3d_amd as cup
+    n = args.size
+    ncell = n ** 3
+
+    # FP32 form, fused decomposition
+    G = make_grid(cup, n, 4, "fused")
+    tke = target_tke(G)
+    time_steps(G, args.warmup, tke)
+    # per-stage launch durations over a few steps (HIP events on the context's stream)
+    stage_ms = {}
+    for st in STAGE_BYTES_PER_CELL:
+        G.stage_timing_begin(st, 3)
+        time_steps(G, 3, tke)
+        stage_ms[st] = G.stage_timing_end()[0]
+    dominant = max(stage_ms, key=lambda k: stage_ms[k])
+    # timed region: K steps, events around the dominant kernel in every one of them
+    G.stage_timing_begin(dominant, args.steps)
+    el = time_steps(G, args.steps, tke)
+    dom_ms, dom_n = G.stage_timing_end()
+    state = G.step_state()
+    ms = el / args.steps * 1e3
+    value = ncell * args.steps / el / 1e6
+    step_frac = ALGO_BYTES_PER_CELL_FP32 * ncell / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9)
+    achieved = STAGE_BYTES_PER_CELL[dominant] * ncell / (dom_ms * 1e-3) / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", f"traffic_{dominant}_{n}.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    del G
+
+    # FP64 plain form (reference decomposition), same workload
+    P = make_grid(cup, n, 8, "plain")
+    tke64 = target_tke(P)
+    time_steps(P, 3, tke64)
+    el64 = time_steps(P, args.plain_steps, tke64)
+    v64 = ncell * args.plain_steps / el64 / 1e6
+    del P
+
+    out = {
+        "metric": "Mcell-steps/s at 256^3 HIT, FP32 form vs FP64 plain on GPU; % HBM roofline",
+        "value": round(value, 1),
+        "unit": "Mcell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"forced isotropic turbulence, periodic 2pi box {n}^3 ({n // 8}^3 blocks of 8^3), "
+                               f"seeded art-spectrum IC (k0=4, u_rms=1), linear forcing to the initial energy, "
+                               f"nu={NU}, CFL {CFL}, central advection + Euler, FFT Poisson",
+                   "cells": ncell, "decomposition": "fused", "precision": "fp32 fields, fp64 reductions"},
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "algorithmic_bytes_per_launch": STAGE_BYTES_PER_CELL[dominant] * ncell,
+                     "avg_launch_ms": round(dom_ms, 5), "launches_timed": dom_n},
+        "step_roofline": {"algorithmic_bytes_per_cell": ALGO_BYTES_PER_CELL_FP32,
+                          "achieved_GBs": round(ALGO_BYTES_PER_CELL_FP32 * ncell / (ms * 1e-3) / 1e9, 1),
+                          "frac": round(step_frac, 4)},
+        "stage_ms": {k: round(v, 5) for k, v in stage_ms.items()},
+        "fp64_plain": {"value": round(v64, 1), "unit": "Mcell-steps/s", "steps": args.plain_steps,
+                       "ms_per_step": round(el64 / args.plain_steps * 1e3, 4)},
+        "speedup_vs_fp64_plain": round(value / v64, 3),
+        "final_state": {"step": state["step"], "time": state["time"], "umax": state["umax"]},
+    }
+    if not args.no_cpu_baseline:
+        cb = cpu_baseline(n, 1, args.cpu_steps)
+        out["cpu_baseline"] = cb
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -69,6 +69,8 @@ SIGNATURES = {
     "cup_step_state": (_i, [_p, _ip, _dp, _dp, _dp, _dp]),
     "cup_set_step_state": (_i, [_p, _i, _d]),
     "cup_stream": (_i, [_p, C.POINTER(C.c_void_p)]),
+    "cup_stage_timing_begin": (_i, [_p, _i, _i]),
+    "cup_stage_timing_end": (_i, [_p, _dp, _ip]),
     "cup_sim_create": (_i, [_i, C.POINTER(C.c_char_p), C.POINTER(C.c_void_p)]),
     "cup_sim_set_cells": (_i, [_p, _i, _i, _i]),
     "cup_sim_init": (_i, [_p]),

@@ -38,9 +38,12 @@ int ensure_partials(Ctx& c, size_t ndoubles)
   return CUP_OK;
 }
 
+static void timing_release(Ctx& c);
+
 static void release(Ctx& c)
 {
   if (c.stream) hipStreamSynchronize(c.stream);
+  timing_release(c);
   body_free(c);
   for (auto& p : c.f)
     if (p) { hipFree(p); p = nullptr; }
@@ -51,12 +54,17 @@ static void release(Ctx& c)
   for (auto& p : c.cos4)
     if (p) { hipFree(p); p = nullptr; }
   if (c.partials) hipFree(c.partials);
-  if (c.low_k) hipFree(c.low_k);
+  if (c.ke_part) hipFree(c.ke_part);
+  c.ke_part = nullptr;
+  if (c.low_part) hipFree(c.low_part);
+  if (c.low_red) hipFree(c.low_red);
   if (c.low_coef) hipFree(c.low_coef);
+  if (c.low_counter) hipFree(c.low_counter);
+  c.low_part = nullptr; c.low_red = nullptr; c.low_counter = nullptr;
   if (c.sc) hipFree(c.sc);
   if (c.sc_host) hipHostFree(c.sc_host);
   if (c.host_red) hipHostFree(c.host_red);
-  c.partials = nullptr; c.low_k = nullptr; c.low_coef = nullptr; c.sc = nullptr; c.sc_host = nullptr;
+  c.partials = nullptr; c.low_coef = nullptr; c.sc = nullptr; c.sc_host = nullptr;
   c.host_red = nullptr;
   if (c.stream) hipStreamDestroy(c.stream);
   c.stream = nullptr;
@@ -69,6 +77,21 @@ static int read_scalars(Ctx& c)
                 "read scalars"))
     return CUP_E_DEVICE;
   return check_hip(c, hipStreamSynchronize(c.stream), "synchronize");
+}
+
+void stage_mark(Ctx& c, int stage, bool end)
+{
+  if (stage != c.timing_stage || c.timing_count >= c.timing_cap) return;
+  hipEventRecord(c.tev[2 * c.timing_count + (end ? 1 : 0)], c.stream);
+  if (end) c.timing_count++;
+}
+
+static void timing_release(Ctx& c)
+{
+  for (auto e : c.tev) hipEventDestroy(e);
+  c.tev.clear();
+  c.timing_stage = -1;
+  c.timing_cap = c.timing_count = 0;
 }
 
 }  // namespace cup
@@ -129,6 +152,8 @@ int cup_create(const cup_grid_desc* d, cup_ctx** out)
                         "hipHostMalloc")))
       break;
     if ((rc = ensure_partials(c, 20 * (size_t)c.nb + 1024))) break;
+    if ((rc = check_hip(c, hipMalloc(&c.ke_part, (size_t)c.nb * sizeof(double)), "hipMalloc ke"))) break;
+    if ((rc = check_hip(c, hipMemsetAsync(c.ke_part, 0, (size_t)c.nb * sizeof(double), c.stream), "memset ke"))) break;
     if ((rc = fft_setup(c))) break;
     if ((rc = spectral_setup(c))) break;
     rc = check_hip(c, hipStreamSynchronize(c.stream), "synchronize");
@@ -303,7 +328,7 @@ int cup_pressure_project(cup_ctx* ctx, double dt)
   if (int rc = launch_set_dt(c, dt)) return rc;
   if (c.decomposition == CUP_PLAIN) return launch_gradp_plain(c);
   const double zero[3] = {0, 0, 0};
-  return launch_gradp_fused(c, zero);
+  return launch_gradp_fused(c, zero, nullptr);
 }
 
 int cup_diagnostics(cup_ctx* ctx, double nu, double out[20])
@@ -421,30 +446,75 @@ int cup_run_steps(cup_ctx* ctx, const cup_step_params* p, int nsteps)
     return check_hip(c, hipStreamSynchronize(c.stream), "run_steps");
   }
   // FUSED: measure the current velocity and set dt for the coming step on the device.
-  if (int rc = launch_max_u(c, p->uinf)) return rc;
-  if (int rc = launch_finalize_step(c, c.nb, p->CFL, p->nu, p->rampup, p->fixed_dt, 0)) return rc;
+  if (int rc = launch_reset_umax(c)) return rc;
+  if (int rc = launch_max_u(c, p->uinf, true)) return rc;
+  if (int rc = launch_finalize_step(c, p->CFL, p->nu, p->rampup, p->fixed_dt, 0)) return rc;
+  const bool body = c.body.present && p->has_body;
+  // The band-limited forcing stays on the device when its low-mode set fits the fused
+  // kernels; otherwise the reference decomposition of SpectralForcing runs (host round trip).
+  const bool force_fused = p->spectral_forcing && c.low_fused_ok;
   for (int s = 0; s < nsteps; ++s) {
+    stage_mark(c, ST_ADVDIFF, false);
     if (int rc = launch_advdiff(c, p->nu, p->uinf, p->scheme)) return rc;
-    if (p->spectral_forcing) {
+    stage_mark(c, ST_ADVDIFF, true);
+    if (p->spectral_forcing && !force_fused) {
       double st[5];
       if (int rc = forcing_plain(c, -1.0, -1.0, p->nu, p->tke_target, p->injection_rate, st)) return rc;
     }
-    if (c.body.present && p->has_body)
+    if (body)
       if (int rc = launch_rhs_obstacle(c, nullptr)) return rc;
-    if (int rc = launch_poisson_fused(c)) return rc;
-    if (int rc = launch_gradp_fused(c, p->uinf)) return rc;
-    if (c.body.present && p->has_body)
+    if (int rc = launch_poisson_fused(c, force_fused, p->tke_target, p->injection_rate)) return rc;
+    stage_mark(c, ST_GRADP, false);
+    if (int rc = launch_gradp_fused(c, p->uinf, body ? c.body.ob_slot : nullptr, force_fused)) return rc;
+    stage_mark(c, ST_GRADP, true);
+    if (body)
       if (int rc = launch_penalize_step(c, p->lambda, p->implicit_penalization, p->body_vel, p->body_angvel, p->uinf))
         return rc;
-    if (int rc = launch_finalize_step(c, c.nb, p->CFL, p->nu, p->rampup, p->fixed_dt, 1)) return rc;
+    if (int rc = launch_finalize_step(c, p->CFL, p->nu, p->rampup, p->fixed_dt, 1)) return rc;
   }
+  return CUP_OK;
+}
+
+int cup_stage_timing_begin(cup_ctx* ctx, int stage, int max_steps)
+{
+  CUP_GUARD(ctx);
+  if (stage < 0 || stage > ST_FINAL || max_steps < 1) return set_error(c, CUP_E_INVALID, "stage timing: bad argument");
+  hipStreamSynchronize(c.stream);
+  timing_release(c);
+  c.tev.resize(2 * (size_t)max_steps);
+  for (auto& e : c.tev)
+    if (int rc = check_hip(c, hipEventCreate(&e), "hipEventCreate")) return rc;
+  c.timing_stage = stage;
+  c.timing_cap = max_steps;
+  c.timing_count = 0;
+  return CUP_OK;
+}
+
+int cup_stage_timing_end(cup_ctx* ctx, double* avg_ms, int* count)
+{
+  CUP_GUARD(ctx);
+  if (int rc = check_hip(c, hipStreamSynchronize(c.stream), "synchronize")) return rc;
+  double tot = 0;
+  for (int i = 0; i < c.timing_count; ++i) {
+    float ms = 0;
+    if (int rc = check_hip(c, hipEventElapsedTime(&ms, c.tev[2 * i], c.tev[2 * i + 1]), "hipEventElapsedTime"))
+      return rc;
+    tot += ms;
+  }
+  if (count) *count = c.timing_count;
+  if (avg_ms) *avg_ms = c.timing_count ? tot / c.timing_count : 0.0;
+  timing_release(c);
   return CUP_OK;
 }
 
 int cup_step_state(cup_ctx* ctx, int* step, double* time, double* last_dt, double* umax, double* tke)
 {
   CUP_GUARD(ctx);
+  if (c.decomposition == CUP_FUSED) {
+    if (int rc = launch_reduce(c, c.nb, 1, 0u, &c.sc->ke, c.ke_part)) return rc;
+  }
   if (int rc = read_scalars(c)) return rc;
+  if (c.decomposition == CUP_FUSED) c.sc_host->ke *= 0.5 * c.h * c.h * c.h;
   if (step) *step = (int)c.sc_host->step;
   if (time) *time = c.sc_host->time;
   if (last_dt) *last_dt = c.sc_host->last_dt;

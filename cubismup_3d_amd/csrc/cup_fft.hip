@@ -156,19 +156,32 @@ constexpr int XFFT = XROWS / 2;  // complex FFTs per workgroup (two real rows ea
 enum XSrc { XSRC_FIELD = 0, XSRC_DIV = 1 };
 
 // Real rows -> packed half spectrum. grid = (Ny/8, Nz).
-template <typename T, int N, int SRC>
+// FORCE (fused linear forcing): the same pass also sums, for the workgroup's 8 rows, the
+// DFT of the advected velocity at every canonical low mode and sum |u*|^2, written as
+// component-major partials fpart[c*nwg + wg] (6 per mode: re/im of u, v, w; then 1).
+// Each thread's cells lie in one row (t -> bx = t/64, iy, ix), so row sums are lane
+// shuffles over ix plus an LDS sum over the 4 waves.
+template <typename T, int N, int SRC, bool FORCE>
 __global__ __launch_bounds__(NT) void k_fft_x_fwd(const T* __restrict__ src0, const T* __restrict__ src1,
                                                   const T* __restrict__ src2, C2<T>* __restrict__ S,
                                                   const C2<T>* __restrict__ tw, GridArgs g,
                                                   const DevScalars* __restrict__ sc, const int* __restrict__ ob_slot,
-                                                  const T* __restrict__ ob_rhs)
+                                                  const T* __restrict__ ob_rhs, LowModes lm,
+                                                  const C2<T>* __restrict__ twy, const C2<T>* __restrict__ twz,
+                                                  double* __restrict__ fpart)
 {
   __shared__ C2<T> buf[N * XFFT];
+  constexpr int NA = 1 + 2 * LOW_KX;  // row DFT values per component: kx=0 (real), kx=1..: re, im
+  __shared__ double rowacc[FORCE ? NT / 64 : 1][FORCE ? XROWS : 1][FORCE ? 3 * NA + 1 : 1];
   const int by = blockIdx.x, z = blockIdx.y;
   const int bz = z >> 3, iz = z & 7;
   T fac = 0;
   if (SRC == XSRC_DIV) fac = (T)(0.5 * g.h * g.h / sc->dt);
   T* rb = (T*)buf;
+  T acc[FORCE ? 3 * NA + 1 : 1];
+  if (FORCE)
+#pragma unroll
+    for (int q = 0; q < 3 * NA + 1; ++q) acc[q] = 0;
   for (int t = threadIdx.x; t < N * XROWS; t += NT) {
     const int bx = t >> 6, iy = (t >> 3) & 7, ix = t & 7;
     const int x = bx * BS + ix, y = by * BS + iy;
@@ -190,10 +203,69 @@ __global__ __launch_bounds__(NT) void k_fft_x_fwd(const T* __restrict__ src0, co
         const T wf = src2[cell_index(g, x, y, wrapi(z - 1, g.Nz))];
         val = fac * (ue - uw + vn - vs + wb - wf);
       }
+      if (FORCE) {
+        const T cu[3] = {src0[o], src1[o], src2[o]};
+        acc[3 * NA] += cu[0] * cu[0] + cu[1] * cu[1] + cu[2] * cu[2];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) acc[q * NA] += cu[q];
+#pragma unroll
+        for (int k = 1; k <= LOW_KX; ++k) {
+          const C2<T> e = tw[(k * x) & (N - 1)];  // e^{-2 pi i k x / Nx}
+#pragma unroll
+          for (int q = 0; q < 3; ++q) {
+            acc[q * NA + 2 * k - 1] += cu[q] * e.x;
+            acc[q * NA + 2 * k] += cu[q] * e.y;
+          }
+        }
+      }
     }
     rb[2 * (x * XFFT + (iy >> 1)) + (iy & 1)] = val;
   }
+  if (FORCE) {
+    // rows: lanes with the same iy differ in ix = lane & 7
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, iy = (lane >> 3) & 7;
+#pragma unroll
+    for (int q = 0; q < 3 * NA + 1; ++q) {
+      double v = (double)acc[q];
+      v += __shfl_xor(v, 1, 64);
+      v += __shfl_xor(v, 2, 64);
+      v += __shfl_xor(v, 4, 64);
+      if ((lane & 7) == 0) rowacc[wv][iy][q] = v;
+    }
+  }
   __syncthreads();
+  if (FORCE) {
+    const int nwg = gridDim.x * gridDim.y, wg = blockIdx.x + gridDim.x * blockIdx.y;
+    for (int t = threadIdx.x; t < 6 * lm.n + 1; t += NT) {
+      double outv = 0;
+      if (t == 6 * lm.n) {
+        for (int r = 0; r < XROWS; ++r)
+          for (int w = 0; w < NT / 64; ++w) outv += rowacc[w][r][3 * NA];
+      } else {
+        const int m = t / 6, q = (t % 6) >> 1, im = t & 1;
+        const int kx = lm.k[m][0], ky = lm.k[m][1], kz = lm.k[m][2];
+        const int ak = kx < 0 ? -kx : kx;
+        const C2<T> ez = twz[(((kz * z) % g.Nz) + g.Nz) % g.Nz];
+        for (int r = 0; r < XROWS; ++r) {
+          double Ur = 0, Ui = 0;
+          for (int w = 0; w < NT / 64; ++w) {
+            if (ak == 0) {
+              Ur += rowacc[w][r][q * NA];
+            } else {
+              Ur += rowacc[w][r][q * NA + 2 * ak - 1];
+              Ui += rowacc[w][r][q * NA + 2 * ak];
+            }
+          }
+          if (kx < 0) Ui = -Ui;
+          const int y = by * BS + r;
+          const C2<T> ey = twy[(((ky * y) % g.Ny) + g.Ny) % g.Ny];
+          const double er = (double)ey.x * ez.x - (double)ey.y * ez.y, ei = (double)ey.x * ez.y + (double)ey.y * ez.x;
+          outv += im ? (Ur * ei + Ui * er) : (Ur * er - Ui * ei);
+        }
+      }
+      fpart[(size_t)t * nwg + wg] = outv;
+    }
+  }
   lds_fft<N, XFFT, -1>(buf, tw);
   // Separate the two real transforms of each complex FFT and pack kx = N/2 into slot 0.
   for (int t = threadIdx.x; t < XROWS * (N / 2); t += NT) {
@@ -260,10 +332,11 @@ enum ZMode { ZM_FWD = 0, ZM_INV = 1, ZM_POISSON = 2 };
 
 // Column FFT along y (AXIS=1) or z (AXIS=2). grid = (ceil(ncols/TK), other axis).
 // MODE: forward, inverse, or (z only) forward + grid-consistent division + inverse.
-template <typename T, int N, int AXIS, int MODE>
+template <typename T, int N, int AXIS, int MODE, bool FORCE = false>
 __global__ __launch_bounds__(NT) void k_fft_col(C2<T>* __restrict__ S, const C2<T>* __restrict__ tw, int ncols,
                                                 int Ny, int Nz, double norm, const double* __restrict__ cx,
-                                                const double* __restrict__ cy, const double* __restrict__ cz)
+                                                const double* __restrict__ cy, const double* __restrict__ cz,
+                                                LowModes lm, const double* __restrict__ coef)
 {
   constexpr int TK = ColTile<T, N>::TK;
   __shared__ C2<T> buf[N * TK];
@@ -284,6 +357,19 @@ __global__ __launch_bounds__(NT) void k_fft_col(C2<T>* __restrict__ S, const C2<
     lds_fft<N, TK, +1>(buf, tw);
   } else {
     lds_fft<N, TK, -1>(buf, tw);
+  }
+  if (FORCE) {
+    // fused forcing: add the forcing's RHS modes (coefficients from k_forcing_reduce)
+    for (int e = threadIdx.x; e < lm.npos; e += NT) {
+      const int slot = lm.pos[e][0], ky = lm.pos[e][1], kz = lm.pos[e][2], mc = lm.pos[e][3];
+      if (ky != other || slot < kx0 || slot >= kx0 + nc) continue;
+      const double* cm = coef + 8 * (mc >> 1);
+      const double ci = (mc & 1) ? -cm[1] : cm[1];
+      C2<T>& v = buf[(slot - kx0) + kz * TK];
+      v.x = (T)((double)v.x + cm[0]);
+      v.y = (T)((double)v.y + ci);
+    }
+    __syncthreads();
   }
   if (MODE == ZM_POISSON) {
     // PoissonSolverPeriodic::_solve grid-consistent branch (PoissonSolverPeriodic.cpp:23-61):
@@ -342,22 +428,31 @@ __global__ void k_divide(C2<T>* __restrict__ S, int ncols, int Ny, int Nz, doubl
 
 template <typename T, int N>
 struct XPass {
-  static void fwd(Ctx& c, const T* s0, const T* s1, const T* s2, int src)
+  static void fwd(Ctx& c, const T* s0, const T* s1, const T* s2, int src, bool force)
   {
     dim3 grid(c.N[1] / BS, c.N[2]);
     const int* slot = c.body.present ? c.body.ob_slot : nullptr;
+    const C2<T>* twy = (const C2<T>*)c.tw[1];
+    const C2<T>* twz = (const C2<T>*)c.tw[2];
     if (src == XSRC_FIELD)
-      hipLaunchKernelGGL((k_fft_x_fwd<T, N, XSRC_FIELD>), grid, dim3(NT), 0, c.stream, s0, s1, s2, (C2<T>*)c.spec,
-                         (const C2<T>*)c.tw[0], c.grid(), c.sc, (const int*)nullptr, (const T*)nullptr);
+      hipLaunchKernelGGL((k_fft_x_fwd<T, N, XSRC_FIELD, false>), grid, dim3(NT), 0, c.stream, s0, s1, s2,
+                         (C2<T>*)c.spec, (const C2<T>*)c.tw[0], c.grid(), c.sc, (const int*)nullptr,
+                         (const T*)nullptr, c.low, twy, twz, (double*)nullptr);
+    else if (force)
+      hipLaunchKernelGGL((k_fft_x_fwd<T, N, XSRC_DIV, true>), grid, dim3(NT), 0, c.stream, s0, s1, s2, (C2<T>*)c.spec,
+                         (const C2<T>*)c.tw[0], c.grid(), c.sc, slot, (const T*)c.body.ob_rhs, c.low, twy, twz,
+                         c.low_part);
     else
-      hipLaunchKernelGGL((k_fft_x_fwd<T, N, XSRC_DIV>), grid, dim3(NT), 0, c.stream, s0, s1, s2, (C2<T>*)c.spec,
-                         (const C2<T>*)c.tw[0], c.grid(), c.sc, slot, (const T*)c.body.ob_rhs);
+      hipLaunchKernelGGL((k_fft_x_fwd<T, N, XSRC_DIV, false>), grid, dim3(NT), 0, c.stream, s0, s1, s2,
+                         (C2<T>*)c.spec, (const C2<T>*)c.tw[0], c.grid(), c.sc, slot, (const T*)c.body.ob_rhs, c.low,
+                         twy, twz, (double*)nullptr);
   }
   static void fwd_to(Ctx& c, const T* s0, C2<T>* spec)
   {
     dim3 grid(c.N[1] / BS, c.N[2]);
-    hipLaunchKernelGGL((k_fft_x_fwd<T, N, XSRC_FIELD>), grid, dim3(NT), 0, c.stream, s0, s0, s0, spec,
-                       (const C2<T>*)c.tw[0], c.grid(), c.sc, (const int*)nullptr, (const T*)nullptr);
+    hipLaunchKernelGGL((k_fft_x_fwd<T, N, XSRC_FIELD, false>), grid, dim3(NT), 0, c.stream, s0, s0, s0, spec,
+                       (const C2<T>*)c.tw[0], c.grid(), c.sc, (const int*)nullptr, (const T*)nullptr, c.low,
+                       (const C2<T>*)c.tw[1], (const C2<T>*)c.tw[2], (double*)nullptr);
   }
   static void inv(Ctx& c, const C2<T>* spec, T* dst)
   {
@@ -368,7 +463,7 @@ struct XPass {
 
 template <typename T, int N, int AXIS>
 struct ColPass {
-  static void run(Ctx& c, C2<T>* spec, int mode, double norm)
+  static void run(Ctx& c, C2<T>* spec, int mode, double norm, bool force = false)
   {
     constexpr int TK = ColTile<T, N>::TK;
     const int ncols = c.N[0] / 2;
@@ -377,13 +472,16 @@ struct ColPass {
     const C2<T>* tw = (const C2<T>*)c.tw[AXIS];
     if (mode == ZM_FWD)
       hipLaunchKernelGGL((k_fft_col<T, N, AXIS, ZM_FWD>), grid, dim3(NT), 0, c.stream, spec, tw, ncols, c.N[1], c.N[2],
-                         norm, c.cos4[0], c.cos4[1], c.cos4[2]);
+                         norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low, (const double*)nullptr);
     else if (mode == ZM_INV)
       hipLaunchKernelGGL((k_fft_col<T, N, AXIS, ZM_INV>), grid, dim3(NT), 0, c.stream, spec, tw, ncols, c.N[1], c.N[2],
-                         norm, c.cos4[0], c.cos4[1], c.cos4[2]);
+                         norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low, (const double*)nullptr);
+    else if (AXIS == 2 && force)
+      hipLaunchKernelGGL((k_fft_col<T, N, 2, ZM_POISSON, true>), grid, dim3(NT), 0, c.stream, spec, tw, ncols, c.N[1],
+                         c.N[2], norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low, (const double*)c.low_coef);
     else if (AXIS == 2)
-      hipLaunchKernelGGL((k_fft_col<T, N, 2, ZM_POISSON>), grid, dim3(NT), 0, c.stream, spec, tw, ncols, c.N[1],
-                         c.N[2], norm, c.cos4[0], c.cos4[1], c.cos4[2]);
+      hipLaunchKernelGGL((k_fft_col<T, N, 2, ZM_POISSON, false>), grid, dim3(NT), 0, c.stream, spec, tw, ncols, c.N[1],
+                         c.N[2], norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low, (const double*)nullptr);
   }
 };
 
@@ -406,7 +504,10 @@ static bool dispatch_n(int n, A&&... a)
 
 template <typename T, int N>
 struct GoXFwd {
-  static void go(Ctx& c, const T* s0, const T* s1, const T* s2, int src) { XPass<T, N>::fwd(c, s0, s1, s2, src); }
+  static void go(Ctx& c, const T* s0, const T* s1, const T* s2, int src, bool force)
+  {
+    XPass<T, N>::fwd(c, s0, s1, s2, src, force);
+  }
 };
 template <typename T, int N>
 struct GoXFwdTo {
@@ -424,6 +525,10 @@ template <typename T, int N>
 struct GoZ {
   static void go(Ctx& c, C2<T>* spec, int mode, double norm) { ColPass<T, N, 2>::run(c, spec, mode, norm); }
 };
+template <typename T, int N>
+struct GoZF {
+  static void go(Ctx& c, C2<T>* spec, int mode, double norm) { ColPass<T, N, 2>::run(c, spec, mode, norm, true); }
+};
 
 static double poisson_norm(const Ctx& c)
 {
@@ -431,16 +536,32 @@ static double poisson_norm(const Ctx& c)
 }
 
 template <typename T>
-static int poisson_fused_t(Ctx& c)
+static int poisson_fused_t(Ctx& c, bool force, double tke_target, double inj_rate)
 {
   C2<T>* S = (C2<T>*)c.spec;
   const double norm = poisson_norm(c);
+  stage_mark(c, ST_XFWD, false);
   dispatch_n<GoXFwd, T>(c.N[0], c, (const T*)c.f[CUP_TMPU], (const T*)c.f[CUP_TMPV], (const T*)c.f[CUP_TMPW],
-                        (int)XSRC_DIV);
+                        (int)XSRC_DIV, force);
+  stage_mark(c, ST_XFWD, true);
+  if (force) {
+    stage_mark(c, ST_FREDUCE, false);
+    if (int rc = launch_forcing_reduce(c, (c.N[1] / BS) * c.N[2], tke_target, inj_rate)) return rc;
+    stage_mark(c, ST_FREDUCE, true);
+  }
+  stage_mark(c, ST_YFWD, false);
   dispatch_n<GoY, T>(c.N[1], c, S, (int)ZM_FWD, norm);
-  dispatch_n<GoZ, T>(c.N[2], c, S, (int)ZM_POISSON, norm);
+  stage_mark(c, ST_YFWD, true);
+  stage_mark(c, ST_ZPASS, false);
+  if (force) dispatch_n<GoZF, T>(c.N[2], c, S, (int)ZM_POISSON, norm);
+  else dispatch_n<GoZ, T>(c.N[2], c, S, (int)ZM_POISSON, norm);
+  stage_mark(c, ST_ZPASS, true);
+  stage_mark(c, ST_YINV, false);
   dispatch_n<GoY, T>(c.N[1], c, S, (int)ZM_INV, norm);
+  stage_mark(c, ST_YINV, true);
+  stage_mark(c, ST_XINV, false);
   dispatch_n<GoXInv, T>(c.N[0], c, (const C2<T>*)S, (T*)c.f[CUP_P]);
+  stage_mark(c, ST_XINV, true);
   return check_hip(c, hipGetLastError(), "poisson_fused");
 }
 
@@ -463,9 +584,11 @@ static int poisson_plain_t(Ctx& c)
   return check_hip(c, hipGetLastError(), "poisson_plain");
 }
 
-int launch_poisson_fused(Ctx& c)
+int launch_poisson_fused(Ctx& c, bool force, double tke_target, double inj_rate)
 {
-  return c.precision == 4 ? poisson_fused_t<float>(c) : poisson_fused_t<double>(c);
+  if (force && !c.low_fused_ok) return set_error(c, CUP_E_INVALID, "fused forcing: low-mode set not supported");
+  return c.precision == 4 ? poisson_fused_t<float>(c, force, tke_target, inj_rate)
+                          : poisson_fused_t<double>(c, force, tke_target, inj_rate);
 }
 
 int launch_poisson_plain(Ctx& c)

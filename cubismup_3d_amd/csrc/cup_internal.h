@@ -34,6 +34,18 @@ struct DevScalars {
   double forcing_fac;
   double tke, tke_low, eps, lint;
   double red[32];    // generic finalized reduction results
+  unsigned long long umax_bits;  // atomic max of |u+uinf| (bit pattern of a non-negative double)
+};
+
+// Canonical low modes of the band-limited forcing (0 < |k|^2 <= 4, one member of each
+// conjugate pair) and their positions in the packed half spectrum; passed by value.
+constexpr int LOW_MAX = 32;  // modes supported by the fused forcing
+constexpr int LOW_KX = 2;    // largest |kx| among them
+struct LowModes {
+  int n;                  // canonical modes
+  int k[LOW_MAX][3];      // (kx, ky, kz) integer wavenumbers
+  int npos;               // packed positions: 1 per mode with kx != 0, 2 with kx == 0
+  int pos[2 * LOW_MAX][4];  // slot (|kx|), ky index, kz index, (mode << 1) | conjugate
 };
 
 // Geometry passed by value to kernels.
@@ -78,16 +90,24 @@ struct Ctx {
   double* cos4[3] = {};   // cos(4 pi k / N) per axis (FP64), the Poisson symbol
   double* partials = nullptr;
   size_t partials_cap = 0;  // doubles
+  double* ke_part = nullptr;  // per-block sum |u|^2 of the fused step's velocity
   DevScalars* sc = nullptr;
   DevScalars* sc_host = nullptr;  // pinned
   double* host_red = nullptr;     // pinned scratch for reductions
-  // forcing: canonical low modes (integer wavenumbers), count and device tables
+  // forcing: canonical low modes and the fused path's device tables
   int nlow = 0;
-  int* low_k = nullptr;        // device, nlow*3 ints
-  double* low_coef = nullptr;  // device, nlow*3 complex: unnormalized DFT of u*,v*,w*
-  void* phase[3] = {};         // device, per axis N complex e^{i 2 pi (i+1/2)/N}
+  bool low_fused_ok = false;   // modes fit LowModes (count, |kx|)
+  LowModes low{};
+  double* low_part = nullptr;  // per x-pass workgroup: 6 per mode (re/im of u,v,w DFT) + 1 (sum |u|^2), component-major
+  double* low_red = nullptr;   // reduced sums
+  double* low_coef = nullptr;  // per mode: RHS addition (re,im), synthesis of u,v,w (re,im) x3
+  unsigned* low_counter = nullptr;
+  void* phase[3] = {};         // unused
   Body body;
   std::string err;
+  // measurement hook: HIP events around one stage of the fused step
+  int timing_stage = -1, timing_cap = 0, timing_count = 0;
+  std::vector<hipEvent_t> tev;
   GridArgs grid() const {
     GridArgs g;
     g.nbx = bpd[0]; g.nby = bpd[1]; g.nbz = bpd[2];
@@ -132,22 +152,23 @@ __device__ __forceinline__ T wave_max(T v)
 // ----------------------------------------------------------------- launchers
 // cup_stencil.hip
 int launch_lab_shift(Ctx& c, int g, int dx, int dy, int dz);
-int launch_max_u(Ctx& c, const double uinf[3]);                  // partials[nb]
+int launch_max_u(Ctx& c, const double uinf[3], bool prime = false);  // partials[nb][2]; prime: atomic max + ke_part
 int launch_advdiff(Ctx& c, double nu, const double uinf[3], int scheme);  // -> tmp
 int launch_copy_tmp_to_u(Ctx& c);
 int launch_rhs_plain(Ctx& c);                                   // -> rbuf
 int launch_gradp_plain(Ctx& c);                                 // u += -dt/2h grad p
-int launch_gradp_fused(Ctx& c, const double uinf[3]);           // u = tmp - dt/2h grad p; partials[nb][2]
+int launch_gradp_fused(Ctx& c, const double uinf[3], const int* skip_slot, bool force = false);  // u = tmp - dt/2h grad p; max, ke_part
 int launch_diagnostics(Ctx& c, double nu);                      // partials[nb][20]
 int launch_ic_taylor_green(Ctx& c, double uMax);
 int launch_fill(Ctx& c, void* ptr, size_t n, double value);
 int launch_copy(Ctx& c, void* dst, const void* src, size_t n);
 int launch_set_dt(Ctx& c, double dt);
-int launch_reduce(Ctx& c, int n, int width, unsigned maxmask, double* out_dev);
-int launch_finalize_step(Ctx& c, int n, double CFL, double nu, int rampup, double fixed_dt, int advance);
+int launch_reduce(Ctx& c, int n, int width, unsigned maxmask, double* out_dev, const double* part = nullptr);
+int launch_reset_umax(Ctx& c);
+int launch_finalize_step(Ctx& c, double CFL, double nu, int rampup, double fixed_dt, int advance);
 // cup_fft.hip
 int fft_setup(Ctx& c);
-int launch_poisson_fused(Ctx& c);   // rhs from div(tmp) -> p (block layout)
+int launch_poisson_fused(Ctx& c, bool force = false, double tke_target = 0, double inj_rate = 0);  // div(tmp) -> p
 int launch_poisson_plain(Ctx& c);   // rhs in rbuf -> solution in rbuf (separate divide pass)
 int launch_fft_forward_field(Ctx& c, const void* real_blocks, void* spec);  // 3 passes
 int launch_fft_inverse_field(Ctx& c, void* spec, void* real_blocks);        // 3 passes
@@ -156,6 +177,7 @@ int spectral_setup(Ctx& c);
 int launch_ic_hit(Ctx& c, double k0, double u_rms, uint64_t seed);
 int forcing_plain(Ctx& c, double dt, double sim_dt, double nu, double tke_target, double inj,
                   double stats[5]);
+int launch_forcing_reduce(Ctx& c, int nwg, double tke_target, double inj_rate);
 // cup_obstacle.hip
 int body_create(Ctx& c, const double center[3], double radius);
 void body_free(Ctx& c);
@@ -164,6 +186,10 @@ int launch_penalize_step(Ctx& c, double lambda, int implicit, const double vel[3
                          const double uinf[3]);  // also refreshes the fused step's max/energy partials
 int launch_surface_forces(Ctx& c, double nu, const double vel[3], const double omega[3]);
 int launch_rhs_obstacle(Ctx& c, void* rhs_blocks);
+
+// Stages of the fused step, for the measurement hook.
+enum Stage { ST_ADVDIFF = 0, ST_XFWD, ST_FREDUCE, ST_YFWD, ST_ZPASS, ST_YINV, ST_XINV, ST_GRADP, ST_PENAL, ST_FINAL };
+void stage_mark(Ctx& c, int stage, bool end);
 
 // helpers (cup_capi.cpp)
 int ensure_partials(Ctx& c, size_t ndoubles);

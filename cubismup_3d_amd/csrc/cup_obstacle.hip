@@ -211,7 +211,10 @@ int body_create(Ctx& c, const double center[3], double radius)
   int rc = c.precision == 4 ? body_chi_t<float>(c) : body_chi_t<double>(c);
   if (rc) return rc;
   std::vector<double> part(4 * (size_t)B.nob);
-  hipMemcpy(part.data(), c.partials, part.size() * sizeof(double), hipMemcpyDeviceToHost);
+  if (int rc2 = check_hip(c, hipMemcpyAsync(part.data(), c.partials, part.size() * sizeof(double),
+                                            hipMemcpyDeviceToHost, c.stream), "chi_from_sphere"))
+    return rc2;
+  if (int rc2 = check_hip(c, hipStreamSynchronize(c.stream), "chi_from_sphere")) return rc2;
   double com[4] = {0, 0, 0, 0};
   for (int s = 0; s < B.nob; ++s)
     for (int i = 0; i < 4; ++i) com[i] += part[4 * (size_t)s + i];
@@ -330,16 +333,16 @@ int launch_rhs_obstacle(Ctx& c, void* rhs_blocks)
 
 // ------------------------------------------------------------------ penalization
 
-// KernelPenalization<implicit> (Penalization.cpp:40-98), one body, udef = 0. Also
-// refreshes the fused step's per-block max/energy partials of the blocks it changed.
+// KernelPenalization<implicit> (Penalization.cpp:40-98), one body, udef = 0. In the fused
+// step it also folds the penalized blocks into the step's max |u+uinf| and energy.
 template <typename T>
 __global__ __launch_bounds__(NT) void k_penalize(const int* __restrict__ ob_ids, const double* __restrict__ ob_chi,
                                                  const T* __restrict__ chi, T* __restrict__ u, T* __restrict__ v,
-                                                 T* __restrict__ w, GridArgs g, const DevScalars* __restrict__ sc,
+                                                 T* __restrict__ w, GridArgs g, DevScalars* __restrict__ sc,
                                                  double lambda, int implicit, double vx, double vy, double vz,
                                                  double ox, double oy, double oz, double cmx, double cmy, double cmz,
                                                  double Ux, double Uy, double Uz, double* __restrict__ part,
-                                                 double* __restrict__ steppart)
+                                                 double* __restrict__ ke_part)
 {
   const int s = blockIdx.x, b = ob_ids[s];
   int bx, by, bz;
@@ -375,8 +378,7 @@ __global__ __launch_bounds__(NT) void k_penalize(const int* __restrict__ ob_ids,
     mk[1] += uu * uu + vv * vv + ww * ww;
   }
   breduce<6>(acc, part + 6 * (size_t)s);
-  if (steppart) {
-    // max reduction for mk[0], sum for mk[1]
+  if (ke_part) {
     __shared__ double red[NT / 64][2];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     mk[0] = wave_max(mk[0]);
@@ -386,8 +388,8 @@ __global__ __launch_bounds__(NT) void k_penalize(const int* __restrict__ ob_ids,
     if (threadIdx.x == 0) {
       double a = red[0][0], bsum = red[0][1];
       for (int q = 1; q < NT / 64; ++q) { a = fmax(a, red[q][0]); bsum += red[q][1]; }
-      steppart[2 * (size_t)b] = a;
-      steppart[2 * (size_t)b + 1] = bsum;
+      atomicMax(&sc->umax_bits, (unsigned long long)__double_as_longlong(a));
+      ke_part[b] = bsum;
     }
   }
 }
@@ -410,7 +412,7 @@ static int penalize_t(Ctx& c, double lambda, int implicit, const double vel[3], 
   hipLaunchKernelGGL((k_penalize<T>), dim3(B.nob), dim3(NT), 0, c.stream, B.ob_ids, B.ob_chi, (const T*)c.f[CUP_CHI],
                      (T*)c.f[CUP_U], (T*)c.f[CUP_V], (T*)c.f[CUP_W], c.grid(), c.sc, lambda, implicit, vel[0], vel[1],
                      vel[2], om[0], om[1], om[2], B.com[0], B.com[1], B.com[2], uinf[0], uinf[1], uinf[2], bp,
-                     refresh ? c.partials : (double*)nullptr);
+                     refresh ? c.ke_part : (double*)nullptr);
   hipLaunchKernelGGL(k_sum_rows, dim3(6), dim3(NT), 0, c.stream, (const double*)bp, B.nob, 6, c.sc->red);
   return check_hip(c, hipGetLastError(), "k_penalize");
 }

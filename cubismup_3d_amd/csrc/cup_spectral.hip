@@ -206,9 +206,11 @@ __global__ __launch_bounds__(NT) void k_forcing_filter(C2<T>* __restrict__ su, C
       continue;
     }
     // slot 0: pair (ky,kz) with its mirror (-ky,-kz); the canonical member does both.
+    // Exactly one member of each pair is canonical: order pairs by raw (kz, ky) index
+    // (signed indices are ambiguous on the Nyquist planes, where k and -k share a sign).
     const int my = ky == 0 ? 0 : N1 - ky, mz = kz == 0 ? 0 : N2 - kz;
     const bool self = my == ky && mz == kz;
-    const bool canon = kk > 0 || (kk == 0 && jj > 0) || self;
+    const bool canon = kz < mz || (kz == mz && ky <= my);
     if (!canon) continue;
     const size_t mid = ((size_t)mz * N1 + my) * ncols;
     C2<double> Z[3] = {ld(su, id), ld(sv, id), ld(sw, id)};
@@ -312,7 +314,10 @@ int forcing_plain(Ctx& c, double dt, double sim_dt, double nu, double tke_target
 }
 
 // Low modes of the linear forcing, 0 < |k|^2 <= 4 in physical wavenumbers, canonical
-// member of each conjugate pair (kz > 0, or kz = 0 and ky > 0, or kz = ky = 0 and kx > 0).
+// member of each conjugate pair (kz > 0, or kz = 0 and ky > 0, or kz = ky = 0 and kx > 0),
+// and their positions in the packed half spectrum (x halved): kx > 0 at (kx, ky, kz);
+// kx < 0 through the partner (-kx, -ky, -kz) conjugated; kx = 0 in slot 0 at (ky, kz) and,
+// conjugated, at (-ky, -kz).
 int spectral_setup(Ctx& c)
 {
   std::vector<int> ks;
@@ -331,12 +336,113 @@ int spectral_setup(Ctx& c)
         if (k2 > 0 && k2 <= 4) { ks.push_back(kx); ks.push_back(ky); ks.push_back(kz); }
       }
   c.nlow = (int)ks.size() / 3;
-  if (c.nlow == 0) return CUP_OK;
-  if (check_hip(c, hipMalloc(&c.low_k, ks.size() * sizeof(int)), "hipMalloc low modes")) return CUP_E_NOMEM;
-  hipMemcpy(c.low_k, ks.data(), ks.size() * sizeof(int), hipMemcpyHostToDevice);
-  if (check_hip(c, hipMalloc(&c.low_coef, (size_t)c.nlow * 6 * sizeof(double)), "hipMalloc low coef"))
+  c.low = LowModes{};
+  c.low_fused_ok = c.nlow > 0 && c.nlow <= LOW_MAX;
+  for (int m = 0; m < c.nlow && c.low_fused_ok; ++m)
+    if (std::abs(ks[3 * m]) > LOW_KX) c.low_fused_ok = false;
+  if (!c.low_fused_ok) return CUP_OK;
+  LowModes& L = c.low;
+  L.n = c.nlow;
+  auto wrapn = [](int k, int n) { return ((k % n) + n) % n; };
+  for (int m = 0; m < c.nlow; ++m) {
+    const int kx = ks[3 * m], ky = ks[3 * m + 1], kz = ks[3 * m + 2];
+    L.k[m][0] = kx; L.k[m][1] = ky; L.k[m][2] = kz;
+    auto add = [&](int slot, int yy, int zz, int conj) {
+      int* q = L.pos[L.npos++];
+      q[0] = slot; q[1] = wrapn(yy, c.N[1]); q[2] = wrapn(zz, c.N[2]); q[3] = (m << 1) | conj;
+    };
+    if (kx > 0) add(kx, ky, kz, 0);
+    else if (kx < 0) add(-kx, -ky, -kz, 1);
+    else { add(0, ky, kz, 0); add(0, -ky, -kz, 1); }
+  }
+  const int W = 6 * c.nlow + 1;
+  const size_t nwg = (size_t)(c.N[1] / BS) * c.N[2];
+  if (check_hip(c, hipMalloc(&c.low_part, (size_t)W * nwg * sizeof(double)), "hipMalloc low partials") ||
+      check_hip(c, hipMalloc(&c.low_red, (size_t)W * sizeof(double)), "hipMalloc low red") ||
+      check_hip(c, hipMalloc(&c.low_coef, (size_t)c.nlow * 8 * sizeof(double)), "hipMalloc low coef") ||
+      check_hip(c, hipMalloc(&c.low_counter, sizeof(unsigned)), "hipMalloc counter"))
     return CUP_E_NOMEM;
+  hipMemsetAsync(c.low_counter, 0, sizeof(unsigned), c.stream);
+  hipMemsetAsync(c.low_coef, 0, (size_t)c.nlow * 8 * sizeof(double), c.stream);
   return CUP_OK;
+}
+
+// Fused forcing, stage 2: one workgroup per reduced component of the x-pass partials
+// (coalesced, component-major); the last workgroup to finish computes, on the device,
+// SpectralForcing's injection and factor (SpectralForcing.cpp:37-77) from
+// tke = <|u*|^2>/2 (Parseval) and tke_low = sum over the low modes, and the coefficient
+// tables: the forcing's pressure-RHS modes fac*(h^2/dt)*i*sum_d sin(2 pi k_d/N_d) u_d(k)
+// (added in the z pass) and the velocity modes fac/N^3 u(k) (added in the gradient pass).
+__global__ __launch_bounds__(NT) void k_forcing_reduce(const double* __restrict__ part, int nwg, int W,
+                                                       double* __restrict__ red, unsigned* __restrict__ counter,
+                                                       LowModes lm, DevScalars* __restrict__ sc,
+                                                       double* __restrict__ coef, double tke_target, double inj_rate,
+                                                       double h, double ncell, int N0, int N1, int N2)
+{
+  const int i = blockIdx.x;
+  double a = 0;
+  for (int j = threadIdx.x; j < nwg; j += NT) a += part[(size_t)i * nwg + j];
+  __shared__ double wsum[NT / 64];
+  a = wave_sum(a);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = a;
+  __syncthreads();
+  __shared__ bool last;
+  if (threadIdx.x == 0) {
+    double t = 0;
+    for (int w = 0; w < NT / 64; ++w) t += wsum[w];
+    red[i] = t;
+    __threadfence();
+    last = atomicAdd(counter, 1u) == (unsigned)(W - 1);
+  }
+  __syncthreads();
+  if (!last || threadIdx.x != 0) return;
+  __threadfence();
+  *counter = 0u;
+  const volatile double* r = red;
+  const double dt = sc->dt;
+  const double norm = (double)N0 * N1 * N2;
+  const double tke = 0.5 * r[W - 1] / ncell;
+  double tke_low = 0;
+  for (int m = 0; m < lm.n; ++m)
+    for (int q = 0; q < 3; ++q) {
+      const double re = r[2 * (3 * m + q)], im = r[2 * (3 * m + q) + 1];
+      tke_low += re * re + im * im;
+    }
+  tke_low /= norm * norm;
+  double inj = 0;
+  if (tke_target > 0) inj = (tke_target - tke) / dt;
+  else if (inj_rate > 0) inj = inj_rate;
+  const double fac = dt * inj / (2 * tke_low);
+  sc->tke = tke;
+  sc->tke_low = tke_low;
+  sc->forcing_fac = fac;
+  for (int m = 0; m < lm.n; ++m) {
+    double* cm = coef + 8 * m;
+    if (!(fac > 0)) {
+      for (int q = 0; q < 8; ++q) cm[q] = 0;
+      continue;
+    }
+    const double s0 = sin(2 * M_PI * lm.k[m][0] / N0), s1 = sin(2 * M_PI * lm.k[m][1] / N1),
+                 s2 = sin(2 * M_PI * lm.k[m][2] / N2);
+    const double dr = s0 * r[6 * m] + s1 * r[6 * m + 2] + s2 * r[6 * m + 4];
+    const double di = s0 * r[6 * m + 1] + s1 * r[6 * m + 3] + s2 * r[6 * m + 5];
+    const double g = fac * h * h / dt;
+    cm[0] = -g * di;  // i * (dr + i di) = -di + i dr
+    cm[1] = g * dr;
+    for (int q = 0; q < 3; ++q) {
+      cm[2 + 2 * q] = fac / norm * r[6 * m + 2 * q];
+      cm[3 + 2 * q] = fac / norm * r[6 * m + 2 * q + 1];
+    }
+  }
+}
+
+int launch_forcing_reduce(Ctx& c, int nwg, double tke_target, double inj_rate)
+{
+  const int W = 6 * c.nlow + 1;
+  hipLaunchKernelGGL(k_forcing_reduce, dim3(W), dim3(NT), 0, c.stream, (const double*)c.low_part, nwg, W, c.low_red,
+                     c.low_counter, c.low, c.sc, c.low_coef, tke_target, inj_rate, c.h, (double)c.ncell, c.N[0],
+                     c.N[1], c.N[2]);
+  return check_hip(c, hipGetLastError(), "k_forcing_reduce");
 }
 
 }  // namespace cup

@@ -121,7 +121,8 @@ int launch_lab_shift(Ctx& c, int g, int dx, int dy, int dz)
 template <typename T>
 __global__ __launch_bounds__(NT) void k_max_u(const T* __restrict__ u, const T* __restrict__ v,
                                               const T* __restrict__ w, int nb, double ux, double uy, double uz,
-                                              double* __restrict__ part)
+                                              double* __restrict__ part, DevScalars* __restrict__ sc,
+                                              double* __restrict__ ke_part)
 {
   const int b = blockIdx.x;
   double m[2] = {0.0, 0.0};
@@ -131,21 +132,40 @@ __global__ __launch_bounds__(NT) void k_max_u(const T* __restrict__ u, const T* 
     m[0] = fmax(m[0], fmax(a, fmax(bb, cc)));
     m[1] += (double)u[i] * u[i] + (double)v[i] * v[i] + (double)w[i] * w[i];
   }
-  block_reduce<2>(m, 1u, part + 2 * (size_t)b);
+  __shared__ double out[2];
+  block_reduce<2>(m, 1u, out);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    part[2 * (size_t)b] = out[0];
+    part[2 * (size_t)b + 1] = out[1];
+    if (sc) {
+      atomicMax(&sc->umax_bits, (unsigned long long)__double_as_longlong(out[0]));
+      ke_part[b] = out[1];
+    }
+  }
 }
 
 template <typename T>
-static int max_u_t(Ctx& c, const double uinf[3])
+static int max_u_t(Ctx& c, const double uinf[3], bool prime)
 {
   hipLaunchKernelGGL((k_max_u<T>), dim3(c.nb), dim3(NT), 0, c.stream, (const T*)c.f[CUP_U], (const T*)c.f[CUP_V],
-                     (const T*)c.f[CUP_W], c.nb, uinf[0], uinf[1], uinf[2], c.partials);
+                     (const T*)c.f[CUP_W], c.nb, uinf[0], uinf[1], uinf[2], c.partials, prime ? c.sc : nullptr,
+                     c.ke_part);
   return check_hip(c, hipGetLastError(), "k_max_u");
 }
 
-int launch_max_u(Ctx& c, const double uinf[3])
+int launch_max_u(Ctx& c, const double uinf[3], bool prime)
 {
   if (int e = ensure_partials(c, 2 * (size_t)c.nb)) return e;
-  return c.precision == 4 ? max_u_t<float>(c, uinf) : max_u_t<double>(c, uinf);
+  return c.precision == 4 ? max_u_t<float>(c, uinf, prime) : max_u_t<double>(c, uinf, prime);
+}
+
+__global__ void k_reset_umax(DevScalars* sc) { sc->umax_bits = 0ull; }
+
+int launch_reset_umax(Ctx& c)
+{
+  hipLaunchKernelGGL(k_reset_umax, dim3(1), dim3(1), 0, c.stream, c.sc);
+  return check_hip(c, hipGetLastError(), "k_reset_umax");
 }
 
 // ------------------------------------------------------------ advection-diffusion
@@ -323,20 +343,58 @@ int launch_rhs_plain(Ctx& c)
 
 // KernelGradP (PressureProjection.cpp:38-50): u += -dt/(2h) (p_E - p_W), etc.
 // PLAIN: in place on u (which holds u* after the advection copy).
-// FUSED: u = tmp + fac*grad p, plus the step's reductions: max |u+uinf| and sum |u|^2.
-template <typename T, bool FUSED>
+// FUSED: u = tmp + fac*grad p, plus the step's reductions: max |u+uinf| folded into the
+// device scalar by an atomic max (order independent, so deterministic) and the block's
+// sum |u|^2 into ke_part. Blocks a penalized body covers (skip_slot >= 0) leave the max to
+// the penalization kernel, which changes their velocity afterwards.
+template <typename T, bool FUSED, bool FORCE>
 __global__ __launch_bounds__(NT) void k_gradp(const T* __restrict__ p, const T* __restrict__ su,
                                               const T* __restrict__ sv, const T* __restrict__ sw, T* __restrict__ u,
                                               T* __restrict__ v, T* __restrict__ w, GridArgs g,
-                                              const DevScalars* __restrict__ sc, double Ux, double Uy, double Uz,
-                                              double* __restrict__ part)
+                                              DevScalars* __restrict__ sc, double Ux, double Uy, double Uz,
+                                              double* __restrict__ ke_part, const int* __restrict__ skip_slot,
+                                              LowModes lm, const double* __restrict__ coef,
+                                              const C2<T>* __restrict__ twx, const C2<T>* __restrict__ twy,
+                                              const C2<T>* __restrict__ twz)
 {
   __shared__ T lab[1][L1C];
+  // fused forcing: per row (iy, iz) of the block, the synthesis coefficients of
+  // fac*u_low = R0 + 2 Re(R1 e^{i a x}) + 2 Re(R2 e^{2 i a x}) for u, v, w
+  constexpr int NA = 1 + 2 * LOW_KX;
+  __shared__ double rows[FORCE ? 4 : 1][FORCE ? 64 : 1][FORCE ? 3 * NA : 1];
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   int bx, by, bz;
   block_coords(g, b, bx, by, bz);
   const T* src[1] = {p};
   gather_g1<T, 1>(src, lab, g, b, bx, by, bz);
+  if (FORCE) {
+    const int row = threadIdx.x & 63, grp = threadIdx.x >> 6;
+    const int y = by * BS + (row & 7), z = bz * BS + (row >> 3);
+    double r[3 * NA];
+#pragma unroll
+    for (int q = 0; q < 3 * NA; ++q) r[q] = 0;
+    for (int m = grp; m < lm.n; m += 4) {
+      const int kx = lm.k[m][0], ky = lm.k[m][1], kz = lm.k[m][2];
+      const C2<T> ey = twy[(((ky * y) % g.Ny) + g.Ny) % g.Ny], ez = twz[(((kz * z) % g.Nz) + g.Nz) % g.Nz];
+      // e^{+i(...)} = conj of the forward twiddles' product
+      const double er = (double)ey.x * ez.x - (double)ey.y * ez.y, ei = -((double)ey.x * ez.y + (double)ey.y * ez.x);
+      const double* cm = coef + 8 * m;
+      const int ak = kx < 0 ? -kx : kx;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const double cr = cm[2 + 2 * q], ci = cm[3 + 2 * q];
+        const double rr = cr * er - ci * ei, ri = cr * ei + ci * er;
+        if (ak == 0) {
+          r[q * NA] += 2 * rr;
+        } else {
+          r[q * NA + 2 * ak - 1] += rr;
+          r[q * NA + 2 * ak] += kx > 0 ? ri : -ri;
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 3 * NA; ++q) rows[grp][row][q] = r[q];
+  }
   __syncthreads();
   const T fac = (T)(-0.5 * sc->dt / g.h);
   double r[2] = {0.0, 0.0};
@@ -345,9 +403,31 @@ __global__ __launch_bounds__(NT) void k_gradp(const T* __restrict__ p, const T* 
     const int l = ((iz + 1) * L1 + iy + 1) * L1 + ix + 1;
     const size_t o = (size_t)b * BS3 + c;
     const T* P = lab[0];
-    const T nu_ = su[o] + fac * (P[l + 1] - P[l - 1]);
-    const T nv_ = sv[o] + fac * (P[l + L1] - P[l - L1]);
-    const T nw_ = sw[o] + fac * (P[l + L1 * L1] - P[l - L1 * L1]);
+    T nu_ = su[o] + fac * (P[l + 1] - P[l - 1]);
+    T nv_ = sv[o] + fac * (P[l + L1] - P[l - L1]);
+    T nw_ = sw[o] + fac * (P[l + L1 * L1] - P[l - L1 * L1]);
+    if (FORCE) {
+      const int row = iz * 8 + iy, x = bx * BS + ix;
+      const C2<T> e1 = twx[x & (g.Nx - 1)], e2 = twx[(2 * x) & (g.Nx - 1)];  // e^{-i a x}, e^{-2 i a x}
+      T add[3];
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        double R0 = 0, R1r = 0, R1i = 0, R2r = 0, R2i = 0;
+#pragma unroll
+        for (int gg = 0; gg < 4; ++gg) {
+          R0 += rows[gg][row][q * NA];
+          R1r += rows[gg][row][q * NA + 1];
+          R1i += rows[gg][row][q * NA + 2];
+          R2r += rows[gg][row][q * NA + 3];
+          R2i += rows[gg][row][q * NA + 4];
+        }
+        // Re(R e^{+i a x}) with e^{+i a x} = conj(e1)
+        add[q] = (T)(R0 + 2 * (R1r * (double)e1.x + R1i * (double)e1.y) + 2 * (R2r * (double)e2.x + R2i * (double)e2.y));
+      }
+      nu_ += add[0];
+      nv_ += add[1];
+      nw_ += add[2];
+    }
     u[o] = nu_;
     v[o] = nv_;
     w[o] = nw_;
@@ -357,34 +437,54 @@ __global__ __launch_bounds__(NT) void k_gradp(const T* __restrict__ p, const T* 
       r[1] += (double)nu_ * nu_ + (double)nv_ * nv_ + (double)nw_ * nw_;
     }
   }
-  if (FUSED) block_reduce<2>(r, 1u, part + 2 * (size_t)b);
+  if (FUSED) {
+    __shared__ double out[2];
+    block_reduce<2>(r, 1u, out);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      if (!skip_slot || skip_slot[b] < 0)
+        atomicMax(&sc->umax_bits, (unsigned long long)__double_as_longlong(out[0]));
+      ke_part[b] = out[1];
+    }
+  }
 }
 
 template <typename T>
-static int gradp_t(Ctx& c, bool fused, const double uinf[3])
+static int gradp_t(Ctx& c, bool fused, const double uinf[3], const int* skip, bool force)
 {
   const T* p = (const T*)c.f[CUP_P];
   const int s = fused ? CUP_TMPU : CUP_U;
-  if (fused)
-    hipLaunchKernelGGL((k_gradp<T, true>), dim3(c.nb), dim3(NT), 0, c.stream, p, (const T*)c.f[s],
+  const C2<T>* tx = (const C2<T>*)c.tw[0];
+  const C2<T>* ty = (const C2<T>*)c.tw[1];
+  const C2<T>* tz = (const C2<T>*)c.tw[2];
+  if (fused && force)
+    hipLaunchKernelGGL((k_gradp<T, true, true>), dim3(c.nb), dim3(NT), 0, c.stream, p, (const T*)c.f[s],
                        (const T*)c.f[s + 1], (const T*)c.f[s + 2], (T*)c.f[CUP_U], (T*)c.f[CUP_V], (T*)c.f[CUP_W],
-                       c.grid(), c.sc, uinf[0], uinf[1], uinf[2], c.partials);
+                       c.grid(), c.sc, uinf[0], uinf[1], uinf[2], c.ke_part, skip, c.low, (const double*)c.low_coef,
+                       tx, ty, tz);
+  else if (fused)
+    hipLaunchKernelGGL((k_gradp<T, true, false>), dim3(c.nb), dim3(NT), 0, c.stream, p, (const T*)c.f[s],
+                       (const T*)c.f[s + 1], (const T*)c.f[s + 2], (T*)c.f[CUP_U], (T*)c.f[CUP_V], (T*)c.f[CUP_W],
+                       c.grid(), c.sc, uinf[0], uinf[1], uinf[2], c.ke_part, skip, c.low, (const double*)nullptr,
+                       tx, ty, tz);
   else
-    hipLaunchKernelGGL((k_gradp<T, false>), dim3(c.nb), dim3(NT), 0, c.stream, p, (const T*)c.f[s],
+    hipLaunchKernelGGL((k_gradp<T, false, false>), dim3(c.nb), dim3(NT), 0, c.stream, p, (const T*)c.f[s],
                        (const T*)c.f[s + 1], (const T*)c.f[s + 2], (T*)c.f[CUP_U], (T*)c.f[CUP_V], (T*)c.f[CUP_W],
-                       c.grid(), c.sc, 0.0, 0.0, 0.0, (double*)nullptr);
+                       c.grid(), c.sc, 0.0, 0.0, 0.0, (double*)nullptr, (const int*)nullptr, c.low,
+                       (const double*)nullptr, tx, ty, tz);
   return check_hip(c, hipGetLastError(), "k_gradp");
 }
 
 int launch_gradp_plain(Ctx& c)
 {
-  return c.precision == 4 ? gradp_t<float>(c, false, nullptr) : gradp_t<double>(c, false, nullptr);
+  return c.precision == 4 ? gradp_t<float>(c, false, nullptr, nullptr, false)
+                          : gradp_t<double>(c, false, nullptr, nullptr, false);
 }
 
-int launch_gradp_fused(Ctx& c, const double uinf[3])
+int launch_gradp_fused(Ctx& c, const double uinf[3], const int* skip_slot, bool force)
 {
-  if (int e = ensure_partials(c, 2 * (size_t)c.nb)) return e;
-  return c.precision == 4 ? gradp_t<float>(c, true, uinf) : gradp_t<double>(c, true, uinf);
+  return c.precision == 4 ? gradp_t<float>(c, true, uinf, skip_slot, force)
+                          : gradp_t<double>(c, true, uinf, skip_slot, force);
 }
 
 // ------------------------------------------------------------------- reductions
@@ -403,58 +503,47 @@ __global__ __launch_bounds__(NT) void k_reduce(const double* __restrict__ part, 
   block_reduce<1>(a, ismax ? 1u : 0u, out + i);
 }
 
-int launch_reduce(Ctx& c, int n, int width, unsigned maxmask, double* out_dev)
+int launch_reduce(Ctx& c, int n, int width, unsigned maxmask, double* out_dev, const double* part)
 {
-  hipLaunchKernelGGL(k_reduce, dim3(width), dim3(NT), 0, c.stream, c.partials, n, width, maxmask, out_dev);
+  hipLaunchKernelGGL(k_reduce, dim3(width), dim3(NT), 0, c.stream, part ? part : c.partials, n, width, maxmask,
+                     out_dev);
   return check_hip(c, hipGetLastError(), "k_reduce");
 }
 
-// Step bookkeeping after the fused gradient: umax and kinetic energy of the new velocity,
-// step += 1 and time += dt when advancing, then the next dt by Simulation::calcMaxTimestep
-// (Simulation.cpp:403-424), ramp included.
-__global__ __launch_bounds__(NT) void k_finalize_step(const double* __restrict__ part, int n, DevScalars* sc,
-                                                      double CFL, double nu, double h, int rampup,
-                                                      double fixed_dt, int advance)
+// Step bookkeeping after the fused gradient: umax of the new velocity from the atomic
+// max (then cleared for the next step), step += 1 and time += dt when advancing, then the
+// next dt by Simulation::calcMaxTimestep (Simulation.cpp:403-424), ramp included.
+__global__ void k_finalize_step(DevScalars* sc, double CFL, double nu, double h, int rampup, double fixed_dt,
+                                int advance)
 {
-  double r[2] = {0.0, 0.0};
-  for (int j = threadIdx.x; j < n; j += NT) {
-    r[0] = fmax(r[0], part[2 * (size_t)j]);
-    r[1] += part[2 * (size_t)j + 1];
+  const double umax = __longlong_as_double((long long)sc->umax_bits);
+  sc->umax_bits = 0ull;
+  sc->umax = umax;
+  if (advance) {
+    sc->step += 1;
+    sc->time += sc->dt;
+    sc->last_dt = sc->dt;
   }
-  __shared__ double out[2];
-  block_reduce<2>(r, 1u, out);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const double umax = out[0];
-    sc->umax = umax;
-    sc->ke = 0.5 * out[1] * h * h * h;
-    if (advance) {
-      sc->step += 1;
-      sc->time += sc->dt;
-      sc->last_dt = sc->dt;
+  double dt;
+  if (fixed_dt > 0) {
+    dt = fixed_dt;
+  } else {
+    const double dtDif = nu > 0 ? h * h / nu : INFINITY;
+    const double dtAdv = h / (umax + 1e-8);
+    const double m = fmin(dtDif, dtAdv);
+    dt = CFL * m;
+    if (sc->step < rampup) {
+      const double x = (sc->step + 1.0) / rampup;
+      const double rampCFL = exp(log(1e-3) * (1 - x) + log(CFL) * x);
+      dt = rampCFL * m;
     }
-    double dt;
-    if (fixed_dt > 0) {
-      dt = fixed_dt;
-    } else {
-      const double dtDif = nu > 0 ? h * h / nu : INFINITY;
-      const double dtAdv = h / (umax + 1e-8);
-      const double m = fmin(dtDif, dtAdv);
-      dt = CFL * m;
-      if (sc->step < rampup) {
-        const double x = (sc->step + 1.0) / rampup;
-        const double rampCFL = exp(log(1e-3) * (1 - x) + log(CFL) * x);
-        dt = rampCFL * m;
-      }
-    }
-    sc->dt = dt;
   }
+  sc->dt = dt;
 }
 
-int launch_finalize_step(Ctx& c, int n, double CFL, double nu, int rampup, double fixed_dt, int advance)
+int launch_finalize_step(Ctx& c, double CFL, double nu, int rampup, double fixed_dt, int advance)
 {
-  hipLaunchKernelGGL(k_finalize_step, dim3(1), dim3(NT), 0, c.stream, c.partials, n, c.sc, CFL, nu, c.h, rampup,
-                     fixed_dt, advance);
+  hipLaunchKernelGGL(k_finalize_step, dim3(1), dim3(1), 0, c.stream, c.sc, CFL, nu, c.h, rampup, fixed_dt, advance);
   return check_hip(c, hipGetLastError(), "k_finalize_step");
 }
 

@@ -204,5 +204,16 @@ class Grid:
         self._c(L.lib().cup_step_state(self._h, C.byref(s), C.byref(t), C.byref(dt), C.byref(um), C.byref(ke)))
         return dict(step=s.value, time=t.value, last_dt=dt.value, umax=um.value, ke=ke.value)
 
+    STAGES = {"advdiff": 0, "xfwd": 1, "forcing_reduce": 2, "yfwd": 3, "zpass": 4, "yinv": 5, "xinv": 6,
+              "gradp": 7}
+
+    def stage_timing_begin(self, stage, max_steps):
+        self._c(L.lib().cup_stage_timing_begin(self._h, int(self.STAGES.get(stage, stage)), int(max_steps)))
+
+    def stage_timing_end(self):
+        ms, n = C.c_double(), C.c_int()
+        self._c(L.lib().cup_stage_timing_end(self._h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
     def set_step_state(self, step, time):
         self._c(L.lib().cup_set_step_state(self._h, int(step), float(time)))

@@ -166,8 +166,15 @@ int cup_run_steps(cup_ctx* ctx, const cup_step_params* prm, int nsteps);
 int cup_step_state(cup_ctx* ctx, int* step, double* time, double* last_dt, double* umax, double* tke);
 int cup_set_step_state(cup_ctx* ctx, int step, double time);
 
-/* Kernel-level timing hooks for measurement: the stream the context launches on. */
+/* Measurement: the stream the context launches on, and HIP events recorded on it around
+ * one stage of the FUSED step (0 advection-diffusion, 1 x-forward FFT with the RHS,
+ * 2 forcing reduction, 3 y forward, 4 z pass with the division, 5 y inverse, 6 x inverse,
+ * 7 gradient) for up to max_steps steps of the following cup_run_steps calls; _end
+ * synchronizes and returns the average duration (ms) and the number of launches timed.
+ * No counterpart in the reference (its cubism::Profiler regions time host phases). */
 int cup_stream(cup_ctx* ctx, void** hip_stream);
+int cup_stage_timing_begin(cup_ctx* ctx, int stage, int max_steps);
+int cup_stage_timing_end(cup_ctx* ctx, double* avg_ms, int* count);
 
 /* ------------------------------------------------------------------------------------
  * Simulation-level entry points: the reference's Simulation / SimulationData pair

@@ -1,0 +1,190 @@
+"""Device kernels against the oracle for isotropic turbulence (seeded IC, band-limited
+forcing), diagnostics, and the sphere (chi, surface points, RHS masking, penalization,
+surface forces). FP64 device results within 1e-11 relative of the FP64 oracle."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300)
+
+
+@pytest.mark.parametrize("bpd", [(4, 4, 4), (2, 4, 8)])
+def test_hit_ic_matches_oracle(cup, oracle, bpd):
+    print(f"HIT IC bpd={bpd}", flush=True)
+    G = cup.Grid(bpd, precision=8)
+    O = oracle.OracleSim(bpd)
+    G.ic_hit(4.0, 1.0, 20260101)
+    O.ic_hit(4.0, 1.0, 20260101)
+    for a, b in zip(G.velocity(), O.velocity()):
+        assert rel(a, b) < 1e-12
+
+
+def test_hit_ic_fp32_context(cup, oracle):
+    G = cup.Grid((4, 4, 4), precision=4)
+    O = oracle.OracleSim((4, 4, 4))
+    G.ic_hit(4.0, 1.0, 11)
+    O.ic_hit(4.0, 1.0, 11)
+    for a, b in zip(G.velocity(), O.velocity()):
+        assert rel(a, b) < 1e-7
+
+
+@pytest.mark.parametrize("decomp", ["plain", "fused"])
+def test_forcing_matches_oracle(cup, oracle, decomp):
+    """One forcing call on a field with energy on every Nyquist plane (random field)."""
+    print(f"forcing decomp={decomp}", flush=True)
+    bpd = (4, 4, 4)
+    G = cup.Grid(bpd, precision=8, decomposition=decomp)
+    O = oracle.OracleSim(bpd)
+    O.ic_hit(4.0, 1.0, 5)
+    rng = np.random.default_rng(1)
+    u, v, w = (f + 0.01 * rng.standard_normal(f.shape) for f in O.velocity())
+    O.set_velocity(u, v, w)
+    if decomp == "fused":
+        G.set("tmpU", u); G.set("tmpV", v); G.set("tmpW", w)
+    else:
+        G.set_velocity(u, v, w)
+    dt, nu = 1e-3, 0.005
+    tgt = 1.2 * 0.5 * (u ** 2 + v ** 2 + w ** 2).mean()
+    sg = G.spectral_forcing(dt, nu, tke_target=tgt)
+    so = O.spectral_forcing(dt, nu, tke_target=tgt)
+    for k in ("tke", "eps", "tke_filtered", "l_integral", "fac"):
+        assert sg[k] == pytest.approx(so[k], rel=1e-11), k
+    names = ("tmpU", "tmpV", "tmpW") if decomp == "fused" else ("u", "v", "w")
+    for n, b in zip(names, O.velocity()):
+        assert rel(G.get(n), b) < 1e-12
+
+
+def test_c2_style_forced_steps_fp64(cup, oracle):
+    """Forced isotropic turbulence, CFL-driven, 8 steps on 32^3: plain device path vs oracle."""
+    bpd = (4, 4, 4)
+    G = cup.Grid(bpd, precision=8, decomposition="plain")
+    O = oracle.OracleSim(bpd)
+    G.ic_hit(4.0, 1.0, 20260101)
+    O.ic_hit(4.0, 1.0, 20260101)
+    u, v, w = O.velocity()
+    tgt = 0.5 * (u ** 2 + v ** 2 + w ** 2).mean()
+    nu, CFL = 0.005, 0.3
+    G.run_steps(8, CFL=CFL, nu=nu, spectral_forcing=True, tke_target=tgt)
+    t = 0
+    for step in range(8):
+        dt, _ = O.calc_max_timestep(CFL, nu, (0, 0, 0), step, 0)
+        O.timestep(dt, nu, spectral_forcing=True, tke_target=tgt)
+        t += dt
+    assert G.step_state()["time"] == pytest.approx(t, rel=1e-12)
+    for a, b in zip(G.velocity(), O.velocity()):
+        assert rel(a, b) < 1e-11
+
+
+def test_diagnostics_match_oracle(cup, oracle):
+    bpd = (4, 4, 4)
+    G = cup.Grid(bpd, precision=8)
+    O = oracle.OracleSim(bpd)
+    rng = np.random.default_rng(8)
+    f = [rng.standard_normal(G.shape) for _ in range(5)]
+    for n, a in zip(("u", "v", "w", "p", "chi"), f):
+        G.set(n, a); O.set(n, a)
+    dg, do = G.diagnostics(0.01), O.dissipation(0.01)
+    np.testing.assert_allclose(dg, do, rtol=1e-11, atol=1e-11 * np.abs(do).max())
+
+
+@pytest.fixture
+def sphere_pair(cup, oracle):
+    bpd = (8, 4, 4)
+    G = cup.Grid(bpd, precision=8, maxextent=2.0)
+    O = oracle.OracleSim(bpd, maxextent=2.0)
+    c, R = (0.61, 0.52, 0.47), 0.2
+    mg = G.chi_from_sphere(c, R)
+    O.sphere_create(R, c)
+    return G, O, mg
+
+
+def test_sphere_chi_matches_oracle(sphere_pair):
+    G, O, mg = sphere_pair
+    info = O.sphere_info()
+    assert np.abs(G.get("chi") - O.get("chi")).max() < 1e-13
+    assert mg["mass"] == pytest.approx(info["mass"], rel=1e-13)
+    np.testing.assert_allclose(mg["com"], info["com"], rtol=1e-13)
+
+
+def test_sphere_rhs_penalization_forces_match_oracle(sphere_pair):
+    G, O, _ = sphere_pair
+    rng = np.random.default_rng(6)
+    u, v, w = (0.3 * rng.standard_normal(G.shape) for _ in range(3))
+    G.set_velocity(u, v, w); O.set_velocity(u, v, w)
+    dt, nu, lam = 1e-3, 1e-3, 1e6
+    V = (-1.0, 0.0, 0.0)
+    G.pressure_rhs(dt); O.pressure_rhs(dt)
+    G.poisson_periodic(); O.poisson_solve()
+    assert rel(G.get("p"), O.get("p")) < 1e-12
+    G.pressure_project(dt); O.grad_p(dt)
+    fg = G.penalize(dt, lam, True, V)
+    fo = O.penalize(dt, lam, True, V)
+    np.testing.assert_allclose(fg, fo, rtol=1e-11, atol=1e-13)
+    for a, b in zip(G.velocity(), O.velocity()):
+        assert rel(a, b) < 1e-12
+    qg = G.surface_forces(nu, V)
+    qo = O.surface_forces(nu, V)
+    np.testing.assert_allclose(qg, qo, rtol=1e-10, atol=1e-12 * np.abs(qo).max())
+
+
+@pytest.mark.parametrize("decomp", ["plain", "fused"])
+def test_sphere_steps_match_oracle(cup, oracle, decomp):
+    """C3-style steps (sphere at rest in a uniform inflow, implicit penalization) on a
+    small grid: device (both decompositions) vs oracle, velocity and drag."""
+    print(f"sphere steps decomp={decomp}", flush=True)
+    bpd = (8, 4, 4)
+    G = cup.Grid(bpd, precision=8, maxextent=2.0, decomposition=decomp)
+    O = oracle.OracleSim(bpd, maxextent=2.0)
+    c, R = (0.5, 0.5, 0.5), 0.15
+    G.chi_from_sphere(c, R); O.sphere_create(R, c)
+    nu, dt, U = 1e-3, 1e-3, (1.0, 0.0, 0.0)
+    sph = dict(lambda_=1e6, implicit=True, trans_vel=(-1.0, 0.0, 0.0))
+    for _ in range(5):
+        O.timestep(dt, nu, uinf=U, sphere=sph)
+    G.run_steps(5, nu=nu, uinf=U, fixed_dt=dt, sphere=sph)
+    for a, b in zip(G.velocity(), O.velocity()):
+        assert rel(a, b) < 1e-11
+    qg = G.surface_forces(nu, sph["trans_vel"])
+    qo = O.surface_forces(nu, sph["trans_vel"])
+    assert qg[15] == pytest.approx(qo[15], rel=1e-9)
+
+
+@pytest.mark.parametrize("bpd", [(4, 4, 4), (8, 4, 2)])
+def test_fused_forcing_steps_match_oracle(cup, oracle, bpd):
+    """The device-resident forcing (low-mode DFT in the x pass, RHS modes added in the z
+    pass, velocity modes in the gradient pass) reproduces SpectralForcing's step."""
+    print(f"fused forcing bpd={bpd}", flush=True)
+    G = cup.Grid(bpd, precision=8, decomposition="fused")
+    O = oracle.OracleSim(bpd)
+    O.ic_hit(4.0, 1.0, 20260101)
+    G.set_velocity(*O.velocity())
+    u, v, w = O.velocity()
+    tgt = 1.05 * 0.5 * (u ** 2 + v ** 2 + w ** 2).mean()
+    nu, CFL = 0.005, 0.3
+    G.run_steps(8, CFL=CFL, nu=nu, spectral_forcing=True, tke_target=tgt)
+    t = 0
+    for step in range(8):
+        dt, _ = O.calc_max_timestep(CFL, nu, (0, 0, 0), step, 0)
+        O.timestep(dt, nu, spectral_forcing=True, tke_target=tgt)
+        t += dt
+    st = G.step_state()
+    assert st["time"] == pytest.approx(t, rel=1e-11)
+    for a, b in zip(G.velocity(), O.velocity()):
+        assert rel(a, b) < 1e-11
+    assert rel(G.get("p"), O.get("p")) < 1e-9
+
+
+def test_fused_forcing_fp32_vs_fp64(cup):
+    bpd = (4, 4, 4)
+    G8 = cup.Grid(bpd, precision=8, decomposition="fused")
+    G4 = cup.Grid(bpd, precision=4, decomposition="fused")
+    G8.ic_hit(4.0, 1.0, 3)
+    G4.set_velocity(*G8.velocity())
+    tgt = G8.diagnostics(0.005)[18] / (2 * np.pi) ** 3
+    for G in (G8, G4):
+        G.run_steps(10, CFL=0.3, nu=0.005, spectral_forcing=True, tke_target=tgt)
+    for a, b in zip(G4.velocity(), G8.velocity()):
+        assert rel(a, b) < 2e-6
