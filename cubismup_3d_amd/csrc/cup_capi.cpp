@@ -60,6 +60,10 @@ static void release(Ctx& c)
   if (c.low_red) hipFree(c.low_red);
   if (c.low_coef) hipFree(c.low_coef);
   if (c.low_counter) hipFree(c.low_counter);
+  if (c.low_dev) hipFree(c.low_dev);
+  if (c.low_rows) hipFree(c.low_rows);
+  c.low_dev = nullptr;
+  c.low_rows = nullptr;
   c.low_part = nullptr; c.low_red = nullptr; c.low_counter = nullptr;
   if (c.sc) hipFree(c.sc);
   if (c.sc_host) hipHostFree(c.sc_host);

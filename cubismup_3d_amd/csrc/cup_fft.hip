@@ -158,7 +158,7 @@ enum XSrc { XSRC_FIELD = 0, XSRC_DIV = 1 };
 // Real rows -> packed half spectrum. grid = (Ny/8, Nz).
 // FORCE (fused linear forcing): the same pass also sums, for the workgroup's 8 rows, the
 // DFT of the advected velocity at every canonical low mode and sum |u*|^2, written as
-// component-major partials fpart[c*nwg + wg] (6 per mode: re/im of u, v, w; then 1).
+// partials fpart[wg*W + c], W = 6 per mode (re/im of u, v, w) + 1.
 // Each thread's cells lie in one row (t -> bx = t/64, iy, ix), so row sums are lane
 // shuffles over ix plus an LDS sum over the 4 waves.
 template <typename T, int N, int SRC, bool FORCE>
@@ -166,7 +166,7 @@ __global__ __launch_bounds__(NT) void k_fft_x_fwd(const T* __restrict__ src0, co
                                                   const T* __restrict__ src2, C2<T>* __restrict__ S,
                                                   const C2<T>* __restrict__ tw, GridArgs g,
                                                   const DevScalars* __restrict__ sc, const int* __restrict__ ob_slot,
-                                                  const T* __restrict__ ob_rhs, LowModes lm,
+                                                  const T* __restrict__ ob_rhs, const LowModes* __restrict__ lmp,
                                                   const C2<T>* __restrict__ twy, const C2<T>* __restrict__ twz,
                                                   double* __restrict__ fpart)
 {
@@ -195,13 +195,15 @@ __global__ __launch_bounds__(NT) void k_fft_x_fwd(const T* __restrict__ src0, co
       if (slot >= 0) {
         val = ob_rhs[(size_t)slot * BS3 + (iz * BS + iy) * BS + ix];
       } else {
-        const T ue = src0[cell_index(g, wrapi(x + 1, g.Nx), y, z)];
-        const T uw = src0[cell_index(g, wrapi(x - 1, g.Nx), y, z)];
-        const T vn = src1[cell_index(g, x, wrapi(y + 1, g.Ny), z)];
-        const T vs = src1[cell_index(g, x, wrapi(y - 1, g.Ny), z)];
-        const T wb = src2[cell_index(g, x, y, wrapi(z + 1, g.Nz))];
-        const T wf = src2[cell_index(g, x, y, wrapi(z - 1, g.Nz))];
-        val = fac * (ue - uw + vn - vs + wb - wf);
+        // neighbour offsets inside the block, or into the periodic neighbour block
+        const long sx = BS3, sy = (long)g.nbx * BS3, sz = (long)g.nbx * g.nby * BS3;
+        const long oe = ix < 7 ? 1 : (bx + 1 < g.nbx ? sx : sx - (long)g.nbx * sx) - 7;
+        const long ow = ix > 0 ? -1 : (bx > 0 ? -sx : (long)(g.nbx - 1) * sx) + 7;
+        const long on = iy < 7 ? BS : (by + 1 < g.nby ? sy : sy - (long)g.nby * sy) - 7 * BS;
+        const long os = iy > 0 ? -BS : (by > 0 ? -sy : (long)(g.nby - 1) * sy) + 7 * BS;
+        const long ob = iz < 7 ? BS * BS : (bz + 1 < g.nbz ? sz : sz - (long)g.nbz * sz) - 7 * BS * BS;
+        const long of = iz > 0 ? -BS * BS : (bz > 0 ? -sz : (long)(g.nbz - 1) * sz) + 7 * BS * BS;
+        val = fac * (src0[o + oe] - src0[o + ow] + src1[o + on] - src1[o + os] + src2[o + ob] - src2[o + of]);
       }
       if (FORCE) {
         const T cu[3] = {src0[o], src1[o], src2[o]};
@@ -226,44 +228,41 @@ __global__ __launch_bounds__(NT) void k_fft_x_fwd(const T* __restrict__ src0, co
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, iy = (lane >> 3) & 7;
 #pragma unroll
     for (int q = 0; q < 3 * NA + 1; ++q) {
-      double v = (double)acc[q];
+      T v = acc[q];
       v += __shfl_xor(v, 1, 64);
       v += __shfl_xor(v, 2, 64);
       v += __shfl_xor(v, 4, 64);
-      if ((lane & 7) == 0) rowacc[wv][iy][q] = v;
+      if ((lane & 7) == 0) rowacc[wv][iy][q] = (double)v;
     }
   }
   __syncthreads();
   if (FORCE) {
-    const int nwg = gridDim.x * gridDim.y, wg = blockIdx.x + gridDim.x * blockIdx.y;
-    for (int t = threadIdx.x; t < 6 * lm.n + 1; t += NT) {
+    __shared__ double rowsum[XROWS][3 * NA + 1];
+    for (int t = threadIdx.x; t < XROWS * (3 * NA + 1); t += NT) {
+      const int r = t / (3 * NA + 1), q = t % (3 * NA + 1);
+      rowsum[r][q] = rowacc[0][r][q] + rowacc[1][r][q] + rowacc[2][r][q] + rowacc[3][r][q];
+    }
+    __syncthreads();
+    const LowModes& lm = *lmp;
+    const int wg = blockIdx.x + gridDim.x * blockIdx.y, W = 6 * lm.n + 1;
+    for (int t = threadIdx.x; t < W; t += NT) {
       double outv = 0;
       if (t == 6 * lm.n) {
-        for (int r = 0; r < XROWS; ++r)
-          for (int w = 0; w < NT / 64; ++w) outv += rowacc[w][r][3 * NA];
+        for (int r = 0; r < XROWS; ++r) outv += rowsum[r][3 * NA];
       } else {
         const int m = t / 6, q = (t % 6) >> 1, im = t & 1;
         const int kx = lm.k[m][0], ky = lm.k[m][1], kz = lm.k[m][2];
         const int ak = kx < 0 ? -kx : kx;
-        const C2<T> ez = twz[(((kz * z) % g.Nz) + g.Nz) % g.Nz];
+        const C2<T> ez = twz[(kz * z) & (g.Nz - 1)];
         for (int r = 0; r < XROWS; ++r) {
-          double Ur = 0, Ui = 0;
-          for (int w = 0; w < NT / 64; ++w) {
-            if (ak == 0) {
-              Ur += rowacc[w][r][q * NA];
-            } else {
-              Ur += rowacc[w][r][q * NA + 2 * ak - 1];
-              Ui += rowacc[w][r][q * NA + 2 * ak];
-            }
-          }
-          if (kx < 0) Ui = -Ui;
-          const int y = by * BS + r;
-          const C2<T> ey = twy[(((ky * y) % g.Ny) + g.Ny) % g.Ny];
+          const double Ur = ak == 0 ? rowsum[r][q * NA] : rowsum[r][q * NA + 2 * ak - 1];
+          const double Ui = ak == 0 ? 0.0 : (kx < 0 ? -rowsum[r][q * NA + 2 * ak] : rowsum[r][q * NA + 2 * ak]);
+          const C2<T> ey = twy[(ky * (by * BS + r)) & (g.Ny - 1)];
           const double er = (double)ey.x * ez.x - (double)ey.y * ez.y, ei = (double)ey.x * ez.y + (double)ey.y * ez.x;
           outv += im ? (Ur * ei + Ui * er) : (Ur * er - Ui * ei);
         }
       }
-      fpart[(size_t)t * nwg + wg] = outv;
+      fpart[(size_t)wg * W + t] = outv;
     }
   }
   lds_fft<N, XFFT, -1>(buf, tw);
@@ -336,7 +335,7 @@ template <typename T, int N, int AXIS, int MODE, bool FORCE = false>
 __global__ __launch_bounds__(NT) void k_fft_col(C2<T>* __restrict__ S, const C2<T>* __restrict__ tw, int ncols,
                                                 int Ny, int Nz, double norm, const double* __restrict__ cx,
                                                 const double* __restrict__ cy, const double* __restrict__ cz,
-                                                LowModes lm, const double* __restrict__ coef)
+                                                const LowModes* __restrict__ lmp, const double* __restrict__ coef)
 {
   constexpr int TK = ColTile<T, N>::TK;
   __shared__ C2<T> buf[N * TK];
@@ -360,6 +359,7 @@ __global__ __launch_bounds__(NT) void k_fft_col(C2<T>* __restrict__ S, const C2<
   }
   if (FORCE) {
     // fused forcing: add the forcing's RHS modes (coefficients from k_forcing_reduce)
+    const LowModes& lm = *lmp;
     for (int e = threadIdx.x; e < lm.npos; e += NT) {
       const int slot = lm.pos[e][0], ky = lm.pos[e][1], kz = lm.pos[e][2], mc = lm.pos[e][3];
       if (ky != other || slot < kx0 || slot >= kx0 + nc) continue;
@@ -437,21 +437,21 @@ struct XPass {
     if (src == XSRC_FIELD)
       hipLaunchKernelGGL((k_fft_x_fwd<T, N, XSRC_FIELD, false>), grid, dim3(NT), 0, c.stream, s0, s1, s2,
                          (C2<T>*)c.spec, (const C2<T>*)c.tw[0], c.grid(), c.sc, (const int*)nullptr,
-                         (const T*)nullptr, c.low, twy, twz, (double*)nullptr);
+                         (const T*)nullptr, c.low_dev, twy, twz, (double*)nullptr);
     else if (force)
       hipLaunchKernelGGL((k_fft_x_fwd<T, N, XSRC_DIV, true>), grid, dim3(NT), 0, c.stream, s0, s1, s2, (C2<T>*)c.spec,
-                         (const C2<T>*)c.tw[0], c.grid(), c.sc, slot, (const T*)c.body.ob_rhs, c.low, twy, twz,
+                         (const C2<T>*)c.tw[0], c.grid(), c.sc, slot, (const T*)c.body.ob_rhs, c.low_dev, twy, twz,
                          c.low_part);
     else
       hipLaunchKernelGGL((k_fft_x_fwd<T, N, XSRC_DIV, false>), grid, dim3(NT), 0, c.stream, s0, s1, s2,
-                         (C2<T>*)c.spec, (const C2<T>*)c.tw[0], c.grid(), c.sc, slot, (const T*)c.body.ob_rhs, c.low,
+                         (C2<T>*)c.spec, (const C2<T>*)c.tw[0], c.grid(), c.sc, slot, (const T*)c.body.ob_rhs, c.low_dev,
                          twy, twz, (double*)nullptr);
   }
   static void fwd_to(Ctx& c, const T* s0, C2<T>* spec)
   {
     dim3 grid(c.N[1] / BS, c.N[2]);
     hipLaunchKernelGGL((k_fft_x_fwd<T, N, XSRC_FIELD, false>), grid, dim3(NT), 0, c.stream, s0, s0, s0, spec,
-                       (const C2<T>*)c.tw[0], c.grid(), c.sc, (const int*)nullptr, (const T*)nullptr, c.low,
+                       (const C2<T>*)c.tw[0], c.grid(), c.sc, (const int*)nullptr, (const T*)nullptr, c.low_dev,
                        (const C2<T>*)c.tw[1], (const C2<T>*)c.tw[2], (double*)nullptr);
   }
   static void inv(Ctx& c, const C2<T>* spec, T* dst)
@@ -472,16 +472,16 @@ struct ColPass {
     const C2<T>* tw = (const C2<T>*)c.tw[AXIS];
     if (mode == ZM_FWD)
       hipLaunchKernelGGL((k_fft_col<T, N, AXIS, ZM_FWD>), grid, dim3(NT), 0, c.stream, spec, tw, ncols, c.N[1], c.N[2],
-                         norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low, (const double*)nullptr);
+                         norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low_dev, (const double*)nullptr);
     else if (mode == ZM_INV)
       hipLaunchKernelGGL((k_fft_col<T, N, AXIS, ZM_INV>), grid, dim3(NT), 0, c.stream, spec, tw, ncols, c.N[1], c.N[2],
-                         norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low, (const double*)nullptr);
+                         norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low_dev, (const double*)nullptr);
     else if (AXIS == 2 && force)
       hipLaunchKernelGGL((k_fft_col<T, N, 2, ZM_POISSON, true>), grid, dim3(NT), 0, c.stream, spec, tw, ncols, c.N[1],
-                         c.N[2], norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low, (const double*)c.low_coef);
+                         c.N[2], norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low_dev, (const double*)c.low_coef);
     else if (AXIS == 2)
       hipLaunchKernelGGL((k_fft_col<T, N, 2, ZM_POISSON, false>), grid, dim3(NT), 0, c.stream, spec, tw, ncols, c.N[1],
-                         c.N[2], norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low, (const double*)nullptr);
+                         c.N[2], norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low_dev, (const double*)nullptr);
   }
 };
 
@@ -547,6 +547,7 @@ static int poisson_fused_t(Ctx& c, bool force, double tke_target, double inj_rat
   if (force) {
     stage_mark(c, ST_FREDUCE, false);
     if (int rc = launch_forcing_reduce(c, (c.N[1] / BS) * c.N[2], tke_target, inj_rate)) return rc;
+    if (int rc = launch_forcing_rows(c)) return rc;
     stage_mark(c, ST_FREDUCE, true);
   }
   stage_mark(c, ST_YFWD, false);

@@ -34,8 +34,11 @@ struct DevScalars {
   double forcing_fac;
   double tke, tke_low, eps, lint;
   double red[32];    // generic finalized reduction results
-  unsigned long long umax_bits;  // atomic max of |u+uinf| (bit pattern of a non-negative double)
+  // max |u+uinf| of the fused step, folded per workgroup by atomic max on the bit pattern
+  // of a non-negative double, spread over UMAX_SLOTS addresses to avoid contention
+  unsigned long long umax_slots[256];
 };
+constexpr int UMAX_SLOTS = 256;
 
 // Canonical low modes of the band-limited forcing (0 < |k|^2 <= 4, one member of each
 // conjugate pair) and their positions in the packed half spectrum; passed by value.
@@ -98,10 +101,12 @@ struct Ctx {
   int nlow = 0;
   bool low_fused_ok = false;   // modes fit LowModes (count, |kx|)
   LowModes low{};
+  LowModes* low_dev = nullptr;  // device copy (kernels index it per lane)
   double* low_part = nullptr;  // per x-pass workgroup: 6 per mode (re/im of u,v,w DFT) + 1 (sum |u|^2), component-major
   double* low_red = nullptr;   // reduced sums
   double* low_coef = nullptr;  // per mode: RHS addition (re,im), synthesis of u,v,w (re,im) x3
   unsigned* low_counter = nullptr;
+  void* low_rows = nullptr;    // per grid row (y, z): 16 synthesis coefficients (context precision)
   void* phase[3] = {};         // unused
   Body body;
   std::string err;
@@ -178,6 +183,7 @@ int launch_ic_hit(Ctx& c, double k0, double u_rms, uint64_t seed);
 int forcing_plain(Ctx& c, double dt, double sim_dt, double nu, double tke_target, double inj,
                   double stats[5]);
 int launch_forcing_reduce(Ctx& c, int nwg, double tke_target, double inj_rate);
+int launch_forcing_rows(Ctx& c);
 // cup_obstacle.hip
 int body_create(Ctx& c, const double center[3], double radius);
 void body_free(Ctx& c);

@@ -388,7 +388,7 @@ __global__ __launch_bounds__(NT) void k_penalize(const int* __restrict__ ob_ids,
     if (threadIdx.x == 0) {
       double a = red[0][0], bsum = red[0][1];
       for (int q = 1; q < NT / 64; ++q) { a = fmax(a, red[q][0]); bsum += red[q][1]; }
-      atomicMax(&sc->umax_bits, (unsigned long long)__double_as_longlong(a));
+      atomicMax(&sc->umax_slots[b & (UMAX_SLOTS - 1)], (unsigned long long)__double_as_longlong(a));
       ke_part[b] = bsum;
     }
   }

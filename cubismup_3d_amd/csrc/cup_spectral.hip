@@ -360,28 +360,31 @@ int spectral_setup(Ctx& c)
   if (check_hip(c, hipMalloc(&c.low_part, (size_t)W * nwg * sizeof(double)), "hipMalloc low partials") ||
       check_hip(c, hipMalloc(&c.low_red, (size_t)W * sizeof(double)), "hipMalloc low red") ||
       check_hip(c, hipMalloc(&c.low_coef, (size_t)c.nlow * 8 * sizeof(double)), "hipMalloc low coef") ||
-      check_hip(c, hipMalloc(&c.low_counter, sizeof(unsigned)), "hipMalloc counter"))
+      check_hip(c, hipMalloc(&c.low_counter, sizeof(unsigned)), "hipMalloc counter") ||
+      check_hip(c, hipMalloc(&c.low_dev, sizeof(LowModes)), "hipMalloc low modes") ||
+      check_hip(c, hipMalloc(&c.low_rows, (size_t)c.N[1] * c.N[2] * 16 * c.esz), "hipMalloc low rows"))
     return CUP_E_NOMEM;
+  hipMemcpy(c.low_dev, &c.low, sizeof(LowModes), hipMemcpyHostToDevice);
   hipMemsetAsync(c.low_counter, 0, sizeof(unsigned), c.stream);
   hipMemsetAsync(c.low_coef, 0, (size_t)c.nlow * 8 * sizeof(double), c.stream);
   return CUP_OK;
 }
 
 // Fused forcing, stage 2: one workgroup per reduced component of the x-pass partials
-// (coalesced, component-major); the last workgroup to finish computes, on the device,
+// (workgroup-major rows of W values); the last workgroup to finish computes, on the device,
 // SpectralForcing's injection and factor (SpectralForcing.cpp:37-77) from
 // tke = <|u*|^2>/2 (Parseval) and tke_low = sum over the low modes, and the coefficient
 // tables: the forcing's pressure-RHS modes fac*(h^2/dt)*i*sum_d sin(2 pi k_d/N_d) u_d(k)
 // (added in the z pass) and the velocity modes fac/N^3 u(k) (added in the gradient pass).
 __global__ __launch_bounds__(NT) void k_forcing_reduce(const double* __restrict__ part, int nwg, int W,
                                                        double* __restrict__ red, unsigned* __restrict__ counter,
-                                                       LowModes lm, DevScalars* __restrict__ sc,
+                                                       const LowModes* __restrict__ lmp, DevScalars* __restrict__ sc,
                                                        double* __restrict__ coef, double tke_target, double inj_rate,
                                                        double h, double ncell, int N0, int N1, int N2)
 {
   const int i = blockIdx.x;
   double a = 0;
-  for (int j = threadIdx.x; j < nwg; j += NT) a += part[(size_t)i * nwg + j];
+  for (int j = threadIdx.x; j < nwg; j += NT) a += part[(size_t)j * W + i];
   __shared__ double wsum[NT / 64];
   a = wave_sum(a);
   if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = a;
@@ -399,6 +402,7 @@ __global__ __launch_bounds__(NT) void k_forcing_reduce(const double* __restrict_
   __threadfence();
   *counter = 0u;
   const volatile double* r = red;
+  const LowModes& lm = *lmp;
   const double dt = sc->dt;
   const double norm = (double)N0 * N1 * N2;
   const double tke = 0.5 * r[W - 1] / ncell;
@@ -440,9 +444,60 @@ int launch_forcing_reduce(Ctx& c, int nwg, double tke_target, double inj_rate)
 {
   const int W = 6 * c.nlow + 1;
   hipLaunchKernelGGL(k_forcing_reduce, dim3(W), dim3(NT), 0, c.stream, (const double*)c.low_part, nwg, W, c.low_red,
-                     c.low_counter, c.low, c.sc, c.low_coef, tke_target, inj_rate, c.h, (double)c.ncell, c.N[0],
+                     c.low_counter, c.low_dev, c.sc, c.low_coef, tke_target, inj_rate, c.h, (double)c.ncell, c.N[0],
                      c.N[1], c.N[2]);
   return check_hip(c, hipGetLastError(), "k_forcing_reduce");
+}
+
+// Fused forcing, stage 3: for every grid row (y, z) the coefficients of the velocity
+// modes' synthesis along x, fac*u_low(x) = R0 + 2 Re(R1 e^{2 pi i x/Nx}) + 2 Re(R2 e^{4 pi i x/Nx})
+// per component (5 values x 3 = 15, padded to 16), so the gradient pass evaluates the
+// forcing with a handful of FMAs per cell.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_forcing_rows(const LowModes* __restrict__ lmp, const double* __restrict__ coef,
+                                                     T* __restrict__ rows, int Ny, int Nz)
+{
+  const int row = blockIdx.x * NT + threadIdx.x;
+  if (row >= Ny * Nz) return;
+  const int y = row % Ny, z = row / Ny;
+  const LowModes& lm = *lmp;
+  double r[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) r[q] = 0;
+  for (int m = 0; m < lm.n; ++m) {
+    const int kx = lm.k[m][0], ky = lm.k[m][1], kz = lm.k[m][2];
+    double si, co;
+    sincospi(2.0 * (double)(((ky * y) & (Ny - 1))) / Ny + 2.0 * (double)(((kz * z) & (Nz - 1))) / Nz, &si, &co);
+    const double* cm = coef + 8 * m;
+    const int ak = kx < 0 ? -kx : kx;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const double cr = cm[2 + 2 * q], ci = cm[3 + 2 * q];
+      const double rr = cr * co - ci * si, ri = cr * si + ci * co;
+      if (ak == 0) {
+        r[5 * q] += 2 * rr;
+      } else {
+        r[5 * q + 2 * ak - 1] += rr;
+        r[5 * q + 2 * ak] += kx > 0 ? ri : -ri;
+      }
+    }
+  }
+  T* out = rows + (size_t)row * 16;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) out[q] = (T)r[q];
+}
+
+int launch_forcing_rows(Ctx& c)
+{
+  const int nrows = c.N[1] * c.N[2];
+  const dim3 grid((nrows + NT - 1) / NT);
+  if (c.precision == 4)
+    hipLaunchKernelGGL((k_forcing_rows<float>), grid, dim3(NT), 0, c.stream, (const LowModes*)c.low_dev,
+                       (const double*)c.low_coef, (float*)c.low_rows, c.N[1], c.N[2]);
+  else
+    hipLaunchKernelGGL((k_forcing_rows<double>), grid, dim3(NT), 0, c.stream, (const LowModes*)c.low_dev,
+                       (const double*)c.low_coef, (double*)c.low_rows, c.N[1], c.N[2]);
+  return check_hip(c, hipGetLastError(), "k_forcing_rows");
 }
 
 }  // namespace cup

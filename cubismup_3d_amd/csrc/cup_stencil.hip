@@ -15,16 +15,34 @@ namespace cup {
 constexpr int L1 = 10;
 constexpr int L1C = L1 * L1 * L1;
 
+template <typename T>
+struct Vec2;
+template <>
+struct Vec2<float> {
+  using type = float2;
+};
+template <>
+struct Vec2<double> {
+  using type = double2;
+};
+
+
 template <typename T, int NF>
 __device__ __forceinline__ void gather_g1(const T* const* src, T (*lab)[L1C], const GridArgs& g, int b,
                                           int bx, int by, int bz)
 {
   const int tid = threadIdx.x;
-  for (int c = tid; c < BS3; c += NT) {
-    const int ix = c & 7, iy = (c >> 3) & 7, iz = c >> 6;
+  static_assert(2 * NT == BS3, "two cells per thread");
+  {
+    const int c = 2 * tid, ix = c & 7, iy = (c >> 3) & 7, iz = c >> 6;
     const int l = ((iz + 1) * L1 + iy + 1) * L1 + ix + 1;
 #pragma unroll
-    for (int f = 0; f < NF; ++f) lab[f][l] = src[f][(size_t)b * BS3 + c];
+    for (int f = 0; f < NF; ++f) {
+      const T* q = src[f] + (size_t)b * BS3 + c;
+      const T a0 = q[0], a1 = q[1];
+      lab[f][l] = a0;
+      lab[f][l + 1] = a1;
+    }
   }
   for (int c = tid; c < 6 * 64; c += NT) {
     const int face = c >> 6, a = (c >> 3) & 7, e = c & 7;
@@ -139,7 +157,7 @@ __global__ __launch_bounds__(NT) void k_max_u(const T* __restrict__ u, const T* 
     part[2 * (size_t)b] = out[0];
     part[2 * (size_t)b + 1] = out[1];
     if (sc) {
-      atomicMax(&sc->umax_bits, (unsigned long long)__double_as_longlong(out[0]));
+      atomicMax(&sc->umax_slots[b & (UMAX_SLOTS - 1)], (unsigned long long)__double_as_longlong(out[0]));
       ke_part[b] = out[1];
     }
   }
@@ -160,11 +178,11 @@ int launch_max_u(Ctx& c, const double uinf[3], bool prime)
   return c.precision == 4 ? max_u_t<float>(c, uinf, prime) : max_u_t<double>(c, uinf, prime);
 }
 
-__global__ void k_reset_umax(DevScalars* sc) { sc->umax_bits = 0ull; }
+__global__ void k_reset_umax(DevScalars* sc) { sc->umax_slots[threadIdx.x] = 0ull; }
 
 int launch_reset_umax(Ctx& c)
 {
-  hipLaunchKernelGGL(k_reset_umax, dim3(1), dim3(1), 0, c.stream, c.sc);
+  hipLaunchKernelGGL(k_reset_umax, dim3(1), dim3(UMAX_SLOTS), 0, c.stream, c.sc);
   return check_hip(c, hipGetLastError(), "k_reset_umax");
 }
 
@@ -190,9 +208,11 @@ __global__ __launch_bounds__(NT) void k_advdiff_central(const T* __restrict__ u,
   const double dt = sc->dt;
   const T facA = (T)(-dt / (2 * g.h));
   const T facD = (T)((nu / g.h) * (dt / g.h));
-  for (int c = threadIdx.x; c < BS3; c += NT) {
-    const int ix = c & 7, iy = (c >> 3) & 7, iz = c >> 6;
-    const int l = ((iz + 1) * L1 + iy + 1) * L1 + ix + 1;
+  const int c0 = 2 * threadIdx.x, ix0 = c0 & 7, iy = (c0 >> 3) & 7, iz = c0 >> 6;
+  T r[3][2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int l = ((iz + 1) * L1 + iy + 1) * L1 + ix0 + k + 1;
     const T* U = lab[0];
     const T* V = lab[1];
     const T* W = lab[2];
@@ -207,11 +227,15 @@ __global__ __launch_bounds__(NT) void k_advdiff_central(const T* __restrict__ u,
     const T duA = uu * dudx + vv * dudy + ww * dudz;
     const T dvA = uu * dvdx + vv * dvdy + ww * dvdz;
     const T dwA = uu * dwdx + vv * dwdy + ww * dwdz;
-    const size_t o = (size_t)b * BS3 + c;
-    ou[o] = U[l] + facA * duA + facD * duD;
-    ov[o] = V[l] + facA * dvA + facD * dvD;
-    ow[o] = W[l] + facA * dwA + facD * dwD;
+    r[0][k] = U[l] + facA * duA + facD * duD;
+    r[1][k] = V[l] + facA * dvA + facD * dvD;
+    r[2][k] = W[l] + facA * dwA + facD * dwD;
   }
+  using V2 = typename Vec2<T>::type;
+  const size_t o = (size_t)b * BS3 + c0;
+  *reinterpret_cast<V2*>(ou + o) = V2{r[0][0], r[0][1]};
+  *reinterpret_cast<V2*>(ov + o) = V2{r[1][0], r[1][1]};
+  *reinterpret_cast<V2*>(ow + o) = V2{r[2][0], r[2][1]};
 }
 
 template <typename T>
@@ -347,103 +371,79 @@ int launch_rhs_plain(Ctx& c)
 // device scalar by an atomic max (order independent, so deterministic) and the block's
 // sum |u|^2 into ke_part. Blocks a penalized body covers (skip_slot >= 0) leave the max to
 // the penalization kernel, which changes their velocity afterwards.
+// Each thread owns two x-adjacent cells c = 2t, 2t+1 of the block (vector loads/stores).
+// FORCE adds the fused forcing's velocity modes, fac*u_low = R0 + 2 Re(R1 e^{i a x}) +
+// 2 Re(R2 e^{2 i a x}) per component with the row coefficients R of k_forcing_rows.
 template <typename T, bool FUSED, bool FORCE>
 __global__ __launch_bounds__(NT) void k_gradp(const T* __restrict__ p, const T* __restrict__ su,
                                               const T* __restrict__ sv, const T* __restrict__ sw, T* __restrict__ u,
                                               T* __restrict__ v, T* __restrict__ w, GridArgs g,
                                               DevScalars* __restrict__ sc, double Ux, double Uy, double Uz,
                                               double* __restrict__ ke_part, const int* __restrict__ skip_slot,
-                                              LowModes lm, const double* __restrict__ coef,
-                                              const C2<T>* __restrict__ twx, const C2<T>* __restrict__ twy,
-                                              const C2<T>* __restrict__ twz)
+                                              const T* __restrict__ rows, const C2<T>* __restrict__ twx)
 {
+  using V2 = typename Vec2<T>::type;
   __shared__ T lab[1][L1C];
-  // fused forcing: per row (iy, iz) of the block, the synthesis coefficients of
-  // fac*u_low = R0 + 2 Re(R1 e^{i a x}) + 2 Re(R2 e^{2 i a x}) for u, v, w
-  constexpr int NA = 1 + 2 * LOW_KX;
-  __shared__ double rows[FORCE ? 4 : 1][FORCE ? 64 : 1][FORCE ? 3 * NA : 1];
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   int bx, by, bz;
   block_coords(g, b, bx, by, bz);
   const T* src[1] = {p};
   gather_g1<T, 1>(src, lab, g, b, bx, by, bz);
-  if (FORCE) {
-    const int row = threadIdx.x & 63, grp = threadIdx.x >> 6;
-    const int y = by * BS + (row & 7), z = bz * BS + (row >> 3);
-    double r[3 * NA];
-#pragma unroll
-    for (int q = 0; q < 3 * NA; ++q) r[q] = 0;
-    for (int m = grp; m < lm.n; m += 4) {
-      const int kx = lm.k[m][0], ky = lm.k[m][1], kz = lm.k[m][2];
-      const C2<T> ey = twy[(((ky * y) % g.Ny) + g.Ny) % g.Ny], ez = twz[(((kz * z) % g.Nz) + g.Nz) % g.Nz];
-      // e^{+i(...)} = conj of the forward twiddles' product
-      const double er = (double)ey.x * ez.x - (double)ey.y * ez.y, ei = -((double)ey.x * ez.y + (double)ey.y * ez.x);
-      const double* cm = coef + 8 * m;
-      const int ak = kx < 0 ? -kx : kx;
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        const double cr = cm[2 + 2 * q], ci = cm[3 + 2 * q];
-        const double rr = cr * er - ci * ei, ri = cr * ei + ci * er;
-        if (ak == 0) {
-          r[q * NA] += 2 * rr;
-        } else {
-          r[q * NA + 2 * ak - 1] += rr;
-          r[q * NA + 2 * ak] += kx > 0 ? ri : -ri;
-        }
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < 3 * NA; ++q) rows[grp][row][q] = r[q];
-  }
+  const int t = threadIdx.x;
   __syncthreads();
   const T fac = (T)(-0.5 * sc->dt / g.h);
-  double r[2] = {0.0, 0.0};
-  for (int c = threadIdx.x; c < BS3; c += NT) {
-    const int ix = c & 7, iy = (c >> 3) & 7, iz = c >> 6;
-    const int l = ((iz + 1) * L1 + iy + 1) * L1 + ix + 1;
-    const size_t o = (size_t)b * BS3 + c;
-    const T* P = lab[0];
-    T nu_ = su[o] + fac * (P[l + 1] - P[l - 1]);
-    T nv_ = sv[o] + fac * (P[l + L1] - P[l - L1]);
-    T nw_ = sw[o] + fac * (P[l + L1 * L1] - P[l - L1 * L1]);
-    if (FORCE) {
-      const int row = iz * 8 + iy, x = bx * BS + ix;
+  const int c0 = 2 * t, ix0 = c0 & 7, iy = (c0 >> 3) & 7, iz = c0 >> 6;
+  const size_t o = (size_t)b * BS3 + c0;
+  const V2 a = *reinterpret_cast<const V2*>(su + o);
+  const V2 bb = *reinterpret_cast<const V2*>(sv + o);
+  const V2 cc = *reinterpret_cast<const V2*>(sw + o);
+  T nu_[2] = {a.x, a.y}, nv_[2] = {bb.x, bb.y}, nw_[2] = {cc.x, cc.y};
+  const T* P = lab[0];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int l = ((iz + 1) * L1 + iy + 1) * L1 + ix0 + k + 1;
+    nu_[k] += fac * (P[l + 1] - P[l - 1]);
+    nv_[k] += fac * (P[l + L1] - P[l - L1]);
+    nw_[k] += fac * (P[l + L1 * L1] - P[l - L1 * L1]);
+  }
+  if (FORCE) {
+    // this thread's row (iy, iz): 15 synthesis coefficients from k_forcing_rows
+    const T* Rr = rows + ((size_t)(bz * BS + iz) * g.Ny + (by * BS + iy)) * 16;
+    T R[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) R[q] = Rr[q];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int x = bx * BS + ix0 + k;
       const C2<T> e1 = twx[x & (g.Nx - 1)], e2 = twx[(2 * x) & (g.Nx - 1)];  // e^{-i a x}, e^{-2 i a x}
-      T add[3];
+      // Re(R e^{+i a x}) with e^{+i a x} = conj(e1)
 #pragma unroll
       for (int q = 0; q < 3; ++q) {
-        double R0 = 0, R1r = 0, R1i = 0, R2r = 0, R2i = 0;
-#pragma unroll
-        for (int gg = 0; gg < 4; ++gg) {
-          R0 += rows[gg][row][q * NA];
-          R1r += rows[gg][row][q * NA + 1];
-          R1i += rows[gg][row][q * NA + 2];
-          R2r += rows[gg][row][q * NA + 3];
-          R2i += rows[gg][row][q * NA + 4];
-        }
-        // Re(R e^{+i a x}) with e^{+i a x} = conj(e1)
-        add[q] = (T)(R0 + 2 * (R1r * (double)e1.x + R1i * (double)e1.y) + 2 * (R2r * (double)e2.x + R2i * (double)e2.y));
+        const T* Q = R + 5 * q;
+        const T add = Q[0] + 2 * (Q[1] * e1.x + Q[2] * e1.y) + 2 * (Q[3] * e2.x + Q[4] * e2.y);
+        if (q == 0) nu_[k] += add;
+        else if (q == 1) nv_[k] += add;
+        else nw_[k] += add;
       }
-      nu_ += add[0];
-      nv_ += add[1];
-      nw_ += add[2];
-    }
-    u[o] = nu_;
-    v[o] = nv_;
-    w[o] = nw_;
-    if (FUSED) {
-      const double a = fabs((double)nu_ + Ux), bb = fabs((double)nv_ + Uy), cc = fabs((double)nw_ + Uz);
-      r[0] = fmax(r[0], fmax(a, fmax(bb, cc)));
-      r[1] += (double)nu_ * nu_ + (double)nv_ * nv_ + (double)nw_ * nw_;
     }
   }
+  *reinterpret_cast<V2*>(u + o) = V2{nu_[0], nu_[1]};
+  *reinterpret_cast<V2*>(v + o) = V2{nv_[0], nv_[1]};
+  *reinterpret_cast<V2*>(w + o) = V2{nw_[0], nw_[1]};
   if (FUSED) {
+    double r[2] = {0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const double aa = fabs((double)nu_[k] + Ux), b2 = fabs((double)nv_[k] + Uy), c2 = fabs((double)nw_[k] + Uz);
+      r[0] = fmax(r[0], fmax(aa, fmax(b2, c2)));
+      r[1] += (double)nu_[k] * nu_[k] + (double)nv_[k] * nv_[k] + (double)nw_[k] * nw_[k];
+    }
     __shared__ double out[2];
     block_reduce<2>(r, 1u, out);
     __syncthreads();
     if (threadIdx.x == 0) {
       if (!skip_slot || skip_slot[b] < 0)
-        atomicMax(&sc->umax_bits, (unsigned long long)__double_as_longlong(out[0]));
+        atomicMax(&sc->umax_slots[b & (UMAX_SLOTS - 1)], (unsigned long long)__double_as_longlong(out[0]));
       ke_part[b] = out[1];
     }
   }
@@ -460,18 +460,15 @@ static int gradp_t(Ctx& c, bool fused, const double uinf[3], const int* skip, bo
   if (fused && force)
     hipLaunchKernelGGL((k_gradp<T, true, true>), dim3(c.nb), dim3(NT), 0, c.stream, p, (const T*)c.f[s],
                        (const T*)c.f[s + 1], (const T*)c.f[s + 2], (T*)c.f[CUP_U], (T*)c.f[CUP_V], (T*)c.f[CUP_W],
-                       c.grid(), c.sc, uinf[0], uinf[1], uinf[2], c.ke_part, skip, c.low, (const double*)c.low_coef,
-                       tx, ty, tz);
+                       c.grid(), c.sc, uinf[0], uinf[1], uinf[2], c.ke_part, skip, (const T*)c.low_rows, tx);
   else if (fused)
     hipLaunchKernelGGL((k_gradp<T, true, false>), dim3(c.nb), dim3(NT), 0, c.stream, p, (const T*)c.f[s],
                        (const T*)c.f[s + 1], (const T*)c.f[s + 2], (T*)c.f[CUP_U], (T*)c.f[CUP_V], (T*)c.f[CUP_W],
-                       c.grid(), c.sc, uinf[0], uinf[1], uinf[2], c.ke_part, skip, c.low, (const double*)nullptr,
-                       tx, ty, tz);
+                       c.grid(), c.sc, uinf[0], uinf[1], uinf[2], c.ke_part, skip, (const T*)nullptr, tx);
   else
     hipLaunchKernelGGL((k_gradp<T, false, false>), dim3(c.nb), dim3(NT), 0, c.stream, p, (const T*)c.f[s],
                        (const T*)c.f[s + 1], (const T*)c.f[s + 2], (T*)c.f[CUP_U], (T*)c.f[CUP_V], (T*)c.f[CUP_W],
-                       c.grid(), c.sc, 0.0, 0.0, 0.0, (double*)nullptr, (const int*)nullptr, c.low,
-                       (const double*)nullptr, tx, ty, tz);
+                       c.grid(), c.sc, 0.0, 0.0, 0.0, (double*)nullptr, (const int*)nullptr, (const T*)nullptr, tx);
   return check_hip(c, hipGetLastError(), "k_gradp");
 }
 
@@ -513,11 +510,16 @@ int launch_reduce(Ctx& c, int n, int width, unsigned maxmask, double* out_dev, c
 // Step bookkeeping after the fused gradient: umax of the new velocity from the atomic
 // max (then cleared for the next step), step += 1 and time += dt when advancing, then the
 // next dt by Simulation::calcMaxTimestep (Simulation.cpp:403-424), ramp included.
-__global__ void k_finalize_step(DevScalars* sc, double CFL, double nu, double h, int rampup, double fixed_dt,
-                                int advance)
+__global__ __launch_bounds__(UMAX_SLOTS) void k_finalize_step(DevScalars* sc, double CFL, double nu, double h,
+                                                               int rampup, double fixed_dt, int advance)
 {
-  const double umax = __longlong_as_double((long long)sc->umax_bits);
-  sc->umax_bits = 0ull;
+  double m[1] = {__longlong_as_double((long long)sc->umax_slots[threadIdx.x])};
+  sc->umax_slots[threadIdx.x] = 0ull;
+  __shared__ double out[1];
+  block_reduce<1>(m, 1u, out);
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const double umax = out[0];
   sc->umax = umax;
   if (advance) {
     sc->step += 1;
@@ -530,12 +532,12 @@ __global__ void k_finalize_step(DevScalars* sc, double CFL, double nu, double h,
   } else {
     const double dtDif = nu > 0 ? h * h / nu : INFINITY;
     const double dtAdv = h / (umax + 1e-8);
-    const double m = fmin(dtDif, dtAdv);
-    dt = CFL * m;
+    const double mm = fmin(dtDif, dtAdv);
+    dt = CFL * mm;
     if (sc->step < rampup) {
       const double x = (sc->step + 1.0) / rampup;
       const double rampCFL = exp(log(1e-3) * (1 - x) + log(CFL) * x);
-      dt = rampCFL * m;
+      dt = rampCFL * mm;
     }
   }
   sc->dt = dt;
@@ -543,7 +545,8 @@ __global__ void k_finalize_step(DevScalars* sc, double CFL, double nu, double h,
 
 int launch_finalize_step(Ctx& c, double CFL, double nu, int rampup, double fixed_dt, int advance)
 {
-  hipLaunchKernelGGL(k_finalize_step, dim3(1), dim3(1), 0, c.stream, c.sc, CFL, nu, c.h, rampup, fixed_dt, advance);
+  hipLaunchKernelGGL(k_finalize_step, dim3(1), dim3(UMAX_SLOTS), 0, c.stream, c.sc, CFL, nu, c.h, rampup, fixed_dt,
+                     advance);
   return check_hip(c, hipGetLastError(), "k_finalize_step");
 }
 
