@@ -47,8 +47,7 @@ static void release(Ctx& c)
   body_free(c);
   for (auto& p : c.f)
     if (p) { hipFree(p); p = nullptr; }
-  void** bufs[] = {&c.spec, &c.spec2, &c.spec3, &c.rbuf, &c.tw[0], &c.tw[1], &c.tw[2],
-                   &c.phase[0], &c.phase[1], &c.phase[2]};
+  void** bufs[] = {&c.spec, &c.spec2, &c.spec3, &c.rbuf, &c.tw[0], &c.tw[1], &c.tw[2]};
   for (void** b : bufs)
     if (*b) { hipFree(*b); *b = nullptr; }
   for (auto& p : c.cos4)
@@ -107,19 +106,30 @@ using namespace cup;
   Ctx& c = (ctx)->c;                                   \
   if (hipSetDevice(c.dev) != hipSuccess) return set_error(c, CUP_E_DEVICE, "hipSetDevice");
 
+// Message of this thread's last failed cup_create (no context exists to hold it).
+static thread_local std::string g_create_error;
+
+static int create_error(int code, const std::string& msg)
+{
+  g_create_error = msg;
+  return code;
+}
+
 extern "C" {
 
 int cup_create(const cup_grid_desc* d, cup_ctx** out)
 {
-  if (!d || !out) return CUP_E_INVALID;
+  if (!d || !out) return create_error(CUP_E_INVALID, "cup_create: null argument");
   *out = nullptr;
   for (int a = 0; a < 3; ++a)
-    if (d->bpd[a] < 1) return CUP_E_INVALID;
-  if (d->precision != 4 && d->precision != 8) return CUP_E_INVALID;
-  if (d->decomposition != CUP_PLAIN && d->decomposition != CUP_FUSED) return CUP_E_INVALID;
-  if (!(d->maxextent > 0)) return CUP_E_INVALID;
+    if (d->bpd[a] < 1) return create_error(CUP_E_INVALID, "cup_create: blocks per dimension must be >= 1");
+  if (d->precision != 4 && d->precision != 8)
+    return create_error(CUP_E_INVALID, "cup_create: precision must be 4 (FP32) or 8 (FP64)");
+  if (d->decomposition != CUP_PLAIN && d->decomposition != CUP_FUSED)
+    return create_error(CUP_E_INVALID, "cup_create: decomposition must be CUP_PLAIN or CUP_FUSED");
+  if (!(d->maxextent > 0)) return create_error(CUP_E_INVALID, "cup_create: maxextent must be > 0");
   cup_ctx* h = new (std::nothrow) cup_ctx();
-  if (!h) return CUP_E_NOMEM;
+  if (!h) return create_error(CUP_E_NOMEM, "cup_create: out of host memory");
   Ctx& c = h->c;
   c.dev = d->device;
   c.precision = d->precision;
@@ -163,9 +173,10 @@ int cup_create(const cup_grid_desc* d, cup_ctx** out)
     rc = check_hip(c, hipStreamSynchronize(c.stream), "synchronize");
   } while (0);
   if (rc) {
+    const std::string msg = c.err;
     release(c);
     delete h;
-    return rc;
+    return create_error(rc, msg);
   }
   *out = h;
   return CUP_OK;
@@ -180,7 +191,11 @@ int cup_destroy(cup_ctx* ctx)
   return CUP_OK;
 }
 
-const char* cup_last_error(const cup_ctx* ctx) { return ctx ? ctx->c.err.c_str() : "null context"; }
+const char* cup_last_error(const cup_ctx* ctx)
+{
+  if (ctx) return ctx->c.err.c_str();
+  return g_create_error.empty() ? "null context" : g_create_error.c_str();
+}
 
 int cup_get_geometry(const cup_ctx* ctx, int cells[3], double* h, double extent[3])
 {
@@ -294,8 +309,8 @@ int cup_advect_diffuse(cup_ctx* ctx, double dt, double nu, const double uinf[3],
   if (!uinf || !(dt > 0)) return set_error(c, CUP_E_INVALID, "advect_diffuse: dt must be > 0");
   if (int rc = launch_set_dt(c, dt)) return rc;
   if (int rc = launch_advdiff(c, nu, uinf, scheme)) return rc;
-  if (c.decomposition == CUP_PLAIN)
-    if (int rc = launch_copy_tmp_to_u(c)) return rc;
+  if (c.decomposition == CUP_PLAIN)  // u = tmp = u*, as after the reference's copy
+    if (int rc = c.ustar == CUP_U ? launch_copy_u_to_tmp(c) : launch_copy_tmp_to_u(c)) return rc;
   return CUP_OK;
 }
 

@@ -541,8 +541,8 @@ static int poisson_fused_t(Ctx& c, bool force, double tke_target, double inj_rat
   C2<T>* S = (C2<T>*)c.spec;
   const double norm = poisson_norm(c);
   stage_mark(c, ST_XFWD, false);
-  dispatch_n<GoXFwd, T>(c.N[0], c, (const T*)c.f[CUP_TMPU], (const T*)c.f[CUP_TMPV], (const T*)c.f[CUP_TMPW],
-                        (int)XSRC_DIV, force);
+  dispatch_n<GoXFwd, T>(c.N[0], c, (const T*)c.f[c.ustar], (const T*)c.f[c.ustar + 1],
+                        (const T*)c.f[c.ustar + 2], (int)XSRC_DIV, force);
   stage_mark(c, ST_XFWD, true);
   if (force) {
     stage_mark(c, ST_FREDUCE, false);

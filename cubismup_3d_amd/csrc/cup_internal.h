@@ -85,6 +85,7 @@ struct Ctx {
   size_t esz = 8;
 
   void* f[8] = {};        // block-layout fields chi,u,v,w,p,tmpU,tmpV,tmpW
+  int ustar = CUP_TMPU;   // first of the three fields holding the step's u* after advection
   void* spec = nullptr;   // packed half spectrum, complex, [z][y][kx], kx < Nx/2
   void* spec2 = nullptr;  // extra spectra for the plain forcing path
   void* spec3 = nullptr;
@@ -102,12 +103,11 @@ struct Ctx {
   bool low_fused_ok = false;   // modes fit LowModes (count, |kx|)
   LowModes low{};
   LowModes* low_dev = nullptr;  // device copy (kernels index it per lane)
-  double* low_part = nullptr;  // per x-pass workgroup: 6 per mode (re/im of u,v,w DFT) + 1 (sum |u|^2), component-major
-  double* low_red = nullptr;   // reduced sums
+  double* low_part = nullptr;  // per x-pass workgroup, workgroup-major: 6 per mode (re/im of u,v,w DFT) + 1 (sum |u|^2)
+  double* low_red = nullptr;   // reduced sums, then the per-chunk sums of the first reduction stage
   double* low_coef = nullptr;  // per mode: RHS addition (re,im), synthesis of u,v,w (re,im) x3
   unsigned* low_counter = nullptr;
   void* low_rows = nullptr;    // per grid row (y, z): 16 synthesis coefficients (context precision)
-  void* phase[3] = {};         // unused
   Body body;
   std::string err;
   // measurement hook: HIP events around one stage of the fused step
@@ -160,6 +160,7 @@ int launch_lab_shift(Ctx& c, int g, int dx, int dy, int dz);
 int launch_max_u(Ctx& c, const double uinf[3], bool prime = false);  // partials[nb][2]; prime: atomic max + ke_part
 int launch_advdiff(Ctx& c, double nu, const double uinf[3], int scheme);  // -> tmp
 int launch_copy_tmp_to_u(Ctx& c);
+int launch_copy_u_to_tmp(Ctx& c);
 int launch_rhs_plain(Ctx& c);                                   // -> rbuf
 int launch_gradp_plain(Ctx& c);                                 // u += -dt/2h grad p
 int launch_gradp_fused(Ctx& c, const double uinf[3], const int* skip_slot, bool force = false);  // u = tmp - dt/2h grad p; max, ke_part

@@ -316,7 +316,7 @@ template <typename T>
 static int rhs_obs_t(Ctx& c, void* rhs_blocks)
 {
   Body& B = c.body;
-  const int s0 = c.decomposition == CUP_FUSED ? CUP_TMPU : CUP_U;
+  const int s0 = c.decomposition == CUP_FUSED ? c.ustar : CUP_U;
   hipLaunchKernelGGL((k_rhs_obs_mask<T>), dim3(B.nob), dim3(NT), 0, c.stream, B.ob_ids, B.ob_chi,
                      (const T*)c.f[CUP_CHI], (const T*)c.f[s0], (const T*)c.f[s0 + 1], (const T*)c.f[s0 + 2],
                      (T*)rhs_blocks, (T*)B.ob_rhs, B.ob_touch, c.grid(), c.sc, c.partials + 2 * (size_t)c.nb);

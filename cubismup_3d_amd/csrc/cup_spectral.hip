@@ -264,7 +264,7 @@ static int forcing_plain_t(Ctx& c, double dt, double sim_dt, double nu, double t
                            double stats[5])
 {
   if (int rc = alloc_extra_spectra(c)) return rc;
-  const int v0 = c.decomposition == CUP_FUSED ? CUP_TMPU : CUP_U;
+  const int v0 = c.decomposition == CUP_FUSED ? c.ustar : CUP_U;
   // SpectralForcing::_cub2fftw + runFwd: three forward transforms.
   if (int rc = launch_fft_forward_field(c, c.f[v0], c.spec)) return rc;
   if (int rc = launch_fft_forward_field(c, c.f[v0 + 1], c.spec2)) return rc;
@@ -313,6 +313,9 @@ int forcing_plain(Ctx& c, double dt, double sim_dt, double nu, double tke_target
                           : forcing_plain_t<double>(c, dt, sim_dt, nu, tke_target, inj, stats);
 }
 
+// Chunks of x-pass workgroup partials summed by the first stage of the forcing reduction.
+constexpr int FR_CHUNKS = 256;
+
 // Low modes of the linear forcing, 0 < |k|^2 <= 4 in physical wavenumbers, canonical
 // member of each conjugate pair (kz > 0, or kz = 0 and ky > 0, or kz = ky = 0 and kx > 0),
 // and their positions in the packed half spectrum (x halved): kx > 0 at (kx, ky, kz);
@@ -358,7 +361,7 @@ int spectral_setup(Ctx& c)
   const int W = 6 * c.nlow + 1;
   const size_t nwg = (size_t)(c.N[1] / BS) * c.N[2];
   if (check_hip(c, hipMalloc(&c.low_part, (size_t)W * nwg * sizeof(double)), "hipMalloc low partials") ||
-      check_hip(c, hipMalloc(&c.low_red, (size_t)W * sizeof(double)), "hipMalloc low red") ||
+      check_hip(c, hipMalloc(&c.low_red, (size_t)W * (FR_CHUNKS + 1) * sizeof(double)), "hipMalloc low red") ||
       check_hip(c, hipMalloc(&c.low_coef, (size_t)c.nlow * 8 * sizeof(double)), "hipMalloc low coef") ||
       check_hip(c, hipMalloc(&c.low_counter, sizeof(unsigned)), "hipMalloc counter") ||
       check_hip(c, hipMalloc(&c.low_dev, sizeof(LowModes)), "hipMalloc low modes") ||
@@ -370,38 +373,56 @@ int spectral_setup(Ctx& c)
   return CUP_OK;
 }
 
-// Fused forcing, stage 2: one workgroup per reduced component of the x-pass partials
-// (workgroup-major rows of W values); the last workgroup to finish computes, on the device,
-// SpectralForcing's injection and factor (SpectralForcing.cpp:37-77) from
-// tke = <|u*|^2>/2 (Parseval) and tke_low = sum over the low modes, and the coefficient
-// tables: the forcing's pressure-RHS modes fac*(h^2/dt)*i*sum_d sin(2 pi k_d/N_d) u_d(k)
-// (added in the z pass) and the velocity modes fac/N^3 u(k) (added in the gradient pass).
+// Fused forcing, stage 2: the x-pass partials (workgroup-major rows of W values) are summed
+// in FR_CHUNKS contiguous chunks of rows, one workgroup per chunk, R = NT / W rows in flight
+// so that consecutive lanes read consecutive values; the last workgroup to finish sums the
+// chunk totals in a fixed order and computes, on the device, SpectralForcing's injection and
+// factor (SpectralForcing.cpp:37-77) from tke = <|u*|^2>/2 (Parseval) and tke_low = sum over
+// the low modes, and the coefficient tables: the forcing's pressure-RHS modes
+// fac*(h^2/dt)*i*sum_d sin(2 pi k_d/N_d) u_d(k) (added in the z pass) and the velocity modes
+// fac/N^3 u(k) (added in the gradient pass). red holds W totals then FR_CHUNKS*W chunk sums.
+
 __global__ __launch_bounds__(NT) void k_forcing_reduce(const double* __restrict__ part, int nwg, int W,
                                                        double* __restrict__ red, unsigned* __restrict__ counter,
                                                        const LowModes* __restrict__ lmp, DevScalars* __restrict__ sc,
                                                        double* __restrict__ coef, double tke_target, double inj_rate,
                                                        double h, double ncell, int N0, int N1, int N2)
 {
-  const int i = blockIdx.x;
-  double a = 0;
-  for (int j = threadIdx.x; j < nwg; j += NT) a += part[(size_t)j * W + i];
-  __shared__ double wsum[NT / 64];
-  a = wave_sum(a);
-  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = a;
-  __syncthreads();
+  __shared__ double s[NT];
   __shared__ bool last;
-  if (threadIdx.x == 0) {
-    double t = 0;
-    for (int w = 0; w < NT / 64; ++w) t += wsum[w];
-    red[i] = t;
-    __threadfence();
-    last = atomicAdd(counter, 1u) == (unsigned)(W - 1);
+  const int t = threadIdx.x;
+  const int R = NT / W;  // W <= 6 * LOW_MAX + 1 < NT
+  const int g = t / W, i = t - g * W;
+  const int lo = (int)((long long)nwg * blockIdx.x / gridDim.x);
+  const int hi = (int)((long long)nwg * (blockIdx.x + 1) / gridDim.x);
+  double a = 0;
+  if (g < R)
+    for (int j = lo + g; j < hi; j += R) a += part[(size_t)j * W + i];
+  s[t] = a;
+  __syncthreads();
+  double* chunk = red + W;
+  if (t < W) {
+    double tot = 0;
+    for (int k = 0; k < R; ++k) tot += s[k * W + t];
+    chunk[(size_t)blockIdx.x * W + t] = tot;
+  }
+  __threadfence();
+  __syncthreads();
+  if (t == 0) last = atomicAdd(counter, 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  if (t < W) {
+    const volatile double* vc = chunk;
+    double tot = 0;
+    for (unsigned k = 0; k < gridDim.x; ++k) tot += vc[(size_t)k * W + t];
+    s[t] = tot;
+    red[t] = tot;
   }
   __syncthreads();
-  if (!last || threadIdx.x != 0) return;
-  __threadfence();
+  if (t != 0) return;
   *counter = 0u;
-  const volatile double* r = red;
+  const double* r = s;
   const LowModes& lm = *lmp;
   const double dt = sc->dt;
   const double norm = (double)N0 * N1 * N2;
@@ -430,9 +451,9 @@ __global__ __launch_bounds__(NT) void k_forcing_reduce(const double* __restrict_
                  s2 = sin(2 * M_PI * lm.k[m][2] / N2);
     const double dr = s0 * r[6 * m] + s1 * r[6 * m + 2] + s2 * r[6 * m + 4];
     const double di = s0 * r[6 * m + 1] + s1 * r[6 * m + 3] + s2 * r[6 * m + 5];
-    const double g = fac * h * h / dt;
-    cm[0] = -g * di;  // i * (dr + i di) = -di + i dr
-    cm[1] = g * dr;
+    const double gg = fac * h * h / dt;
+    cm[0] = -gg * di;  // i * (dr + i di) = -di + i dr
+    cm[1] = gg * dr;
     for (int q = 0; q < 3; ++q) {
       cm[2 + 2 * q] = fac / norm * r[6 * m + 2 * q];
       cm[3 + 2 * q] = fac / norm * r[6 * m + 2 * q + 1];
@@ -443,7 +464,8 @@ __global__ __launch_bounds__(NT) void k_forcing_reduce(const double* __restrict_
 int launch_forcing_reduce(Ctx& c, int nwg, double tke_target, double inj_rate)
 {
   const int W = 6 * c.nlow + 1;
-  hipLaunchKernelGGL(k_forcing_reduce, dim3(W), dim3(NT), 0, c.stream, (const double*)c.low_part, nwg, W, c.low_red,
+  const int nch = nwg < FR_CHUNKS ? nwg : FR_CHUNKS;
+  hipLaunchKernelGGL(k_forcing_reduce, dim3(nch), dim3(NT), 0, c.stream, (const double*)c.low_part, nwg, W, c.low_red,
                      c.low_counter, c.low_dev, c.sc, c.low_coef, tke_target, inj_rate, c.h, (double)c.ncell, c.N[0],
                      c.N[1], c.N[2]);
   return check_hip(c, hipGetLastError(), "k_forcing_reduce");
@@ -452,22 +474,28 @@ int launch_forcing_reduce(Ctx& c, int nwg, double tke_target, double inj_rate)
 // Fused forcing, stage 3: for every grid row (y, z) the coefficients of the velocity
 // modes' synthesis along x, fac*u_low(x) = R0 + 2 Re(R1 e^{2 pi i x/Nx}) + 2 Re(R2 e^{4 pi i x/Nx})
 // per component (5 values x 3 = 15, padded to 16), so the gradient pass evaluates the
-// forcing with a handful of FMAs per cell.
+// forcing with a handful of FMAs per cell. The row phase e^{2 pi i (ky y/Ny + kz z/Nz)} is
+// the conjugate of the product of two twiddles e^{-2 pi i t/N}.
+constexpr int FROWS_NT = 64;
+
 template <typename T>
-__global__ __launch_bounds__(NT) void k_forcing_rows(const LowModes* __restrict__ lmp, const double* __restrict__ coef,
-                                                     T* __restrict__ rows, int Ny, int Nz)
+__global__ __launch_bounds__(FROWS_NT) void k_forcing_rows(const LowModes* __restrict__ lmp,
+                                                           const double* __restrict__ coef, const C2<T>* __restrict__ twy,
+                                                           const C2<T>* __restrict__ twz, T* __restrict__ rows, int Ny,
+                                                           int Nz)
 {
-  const int row = blockIdx.x * NT + threadIdx.x;
+  const int row = blockIdx.x * FROWS_NT + threadIdx.x;
   if (row >= Ny * Nz) return;
-  const int y = row % Ny, z = row / Ny;
+  const int y = row & (Ny - 1), z = row / Ny;
   const LowModes& lm = *lmp;
   double r[16];
 #pragma unroll
   for (int q = 0; q < 16; ++q) r[q] = 0;
   for (int m = 0; m < lm.n; ++m) {
     const int kx = lm.k[m][0], ky = lm.k[m][1], kz = lm.k[m][2];
-    double si, co;
-    sincospi(2.0 * (double)(((ky * y) & (Ny - 1))) / Ny + 2.0 * (double)(((kz * z) & (Nz - 1))) / Nz, &si, &co);
+    const C2<T> ey = twy[(ky * y) & (Ny - 1)], ez = twz[(kz * z) & (Nz - 1)];
+    const double co = (double)ey.x * ez.x - (double)ey.y * ez.y;
+    const double si = -((double)ey.x * ez.y + (double)ey.y * ez.x);
     const double* cm = coef + 8 * m;
     const int ak = kx < 0 ? -kx : kx;
 #pragma unroll
@@ -490,13 +518,15 @@ __global__ __launch_bounds__(NT) void k_forcing_rows(const LowModes* __restrict_
 int launch_forcing_rows(Ctx& c)
 {
   const int nrows = c.N[1] * c.N[2];
-  const dim3 grid((nrows + NT - 1) / NT);
+  const dim3 grid((nrows + FROWS_NT - 1) / FROWS_NT);
   if (c.precision == 4)
-    hipLaunchKernelGGL((k_forcing_rows<float>), grid, dim3(NT), 0, c.stream, (const LowModes*)c.low_dev,
-                       (const double*)c.low_coef, (float*)c.low_rows, c.N[1], c.N[2]);
+    hipLaunchKernelGGL((k_forcing_rows<float>), grid, dim3(FROWS_NT), 0, c.stream, (const LowModes*)c.low_dev,
+                       (const double*)c.low_coef, (const C2<float>*)c.tw[1], (const C2<float>*)c.tw[2],
+                       (float*)c.low_rows, c.N[1], c.N[2]);
   else
-    hipLaunchKernelGGL((k_forcing_rows<double>), grid, dim3(NT), 0, c.stream, (const LowModes*)c.low_dev,
-                       (const double*)c.low_coef, (double*)c.low_rows, c.N[1], c.N[2]);
+    hipLaunchKernelGGL((k_forcing_rows<double>), grid, dim3(FROWS_NT), 0, c.stream, (const LowModes*)c.low_dev,
+                       (const double*)c.low_coef, (const C2<double>*)c.tw[1], (const C2<double>*)c.tw[2],
+                       (double*)c.low_rows, c.N[1], c.N[2]);
   return check_hip(c, hipGetLastError(), "k_forcing_rows");
 }
 

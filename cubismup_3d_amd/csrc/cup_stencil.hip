@@ -69,6 +69,102 @@ __device__ __forceinline__ void block_coords(const GridArgs& g, int b, int& bx, 
   bz = b / (g.nbx * g.nby);
 }
 
+// ------------------------------------------------------- pipelined lab staging
+// The fused step's stencil kernels walk nbw consecutive blocks per workgroup and fetch
+// block j+1's cells into registers while block j is computed from LDS, which keeps about
+// twice the bytes in flight per CU of a load-sync-compute workgroup.
+// A thread's cells: interior c = 2t, 2t+1; halo (faces only, 6*64 cells) c = t (faces
+// x-, x+, y-, y+: one face per wave) and, for t < 128, c = NT + t (faces z-, z+).
+struct LabSlots {
+  int li;       // LDS index of interior cell 2t
+  int l[2];     // LDS index of the halo cells
+  int off[2];   // their offsets inside the neighbour block
+  int face[2];  // 0..5 = x-, x+, y-, y+, z-, z+
+};
+
+__device__ __forceinline__ bool has_halo2() { return threadIdx.x < 6 * 64 - NT; }
+
+__device__ __forceinline__ LabSlots lab_slots()
+{
+  LabSlots s;
+  const int t = threadIdx.x, c0 = 2 * t;
+  s.li = (((c0 >> 6) + 1) * L1 + ((c0 >> 3) & 7) + 1) * L1 + (c0 & 7) + 1;
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int c = k == 0 ? t : (has_halo2() ? NT + t : 4 * 64);
+    const int face = c >> 6, a = (c >> 3) & 7, e = c & 7;
+    int sx, sy, sz, lx, ly, lz;
+    switch (face) {
+      case 0: sx = 7; sy = e; sz = a; lx = 0; ly = e + 1; lz = a + 1; break;
+      case 1: sx = 0; sy = e; sz = a; lx = 9; ly = e + 1; lz = a + 1; break;
+      case 2: sx = e; sy = 7; sz = a; lx = e + 1; ly = 0; lz = a + 1; break;
+      case 3: sx = e; sy = 0; sz = a; lx = e + 1; ly = 9; lz = a + 1; break;
+      case 4: sx = e; sy = a; sz = 7; lx = e + 1; ly = a + 1; lz = 0; break;
+      default: sx = e; sy = a; sz = 0; lx = e + 1; ly = a + 1; lz = 9; break;
+    }
+    s.face[k] = face;
+    s.off[k] = (sz * BS + sy) * BS + sx;
+    s.l[k] = (lz * L1 + ly) * L1 + lx;
+  }
+  return s;
+}
+
+// Periodic neighbour of block (bx, by, bz) across face 0..5.
+__device__ __forceinline__ int face_block(const GridArgs& g, int bx, int by, int bz, int face)
+{
+  const int d = (face & 1) ? 1 : -1;
+  if (face < 2) bx = wrapi(bx + d, g.nbx);
+  else if (face < 4) by = wrapi(by + d, g.nby);
+  else bz = wrapi(bz + d, g.nbz);
+  return bx + g.nbx * (by + g.nby * bz);
+}
+
+template <typename T, int NF>
+struct LabRegs {
+  T in[NF][2];
+  T hv[NF][2];
+};
+
+template <typename T, int NF>
+__device__ __forceinline__ void lab_fetch(LabRegs<T, NF>& r, const T* const* src, const LabSlots& s,
+                                          const GridArgs& g, int b)
+{
+  using V2 = typename Vec2<T>::type;
+  int bx, by, bz;
+  block_coords(g, b, bx, by, bz);
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    const V2 a = *reinterpret_cast<const V2*>(src[f] + (size_t)b * BS3 + 2 * threadIdx.x);
+    r.in[f][0] = a.x;
+    r.in[f][1] = a.y;
+  }
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+    if (k == 0 || has_halo2()) {
+      const size_t q = (size_t)face_block(g, bx, by, bz, s.face[k]) * BS3 + s.off[k];
+#pragma unroll
+      for (int f = 0; f < NF; ++f) r.hv[f][k] = src[f][q];
+    }
+}
+
+template <typename T, int NF>
+__device__ __forceinline__ void lab_put(const LabRegs<T, NF>& r, T (*lab)[L1C], const LabSlots& s)
+{
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    lab[f][s.li] = r.in[f][0];
+    lab[f][s.li + 1] = r.in[f][1];
+  }
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+    if (k == 0 || has_halo2())
+#pragma unroll
+      for (int f = 0; f < NF; ++f) lab[f][s.l[k]] = r.hv[f][k];
+}
+
+// Blocks per workgroup of the pipelined kernels: 4 when the block count allows it.
+inline int blocks_per_wg(int nb) { return nb % 4 == 0 ? 4 : (nb % 2 == 0 ? 2 : 1); }
+
 // Block-wide reduction of NV doubles per thread into out[0..NV) (thread 0 holds them).
 // Fixed shuffle tree and fixed LDS order: deterministic.
 template <int NV>
@@ -196,61 +292,192 @@ __global__ __launch_bounds__(NT) void k_advdiff_central(const T* __restrict__ u,
                                                         const T* __restrict__ w, T* __restrict__ ou,
                                                         T* __restrict__ ov, T* __restrict__ ow, GridArgs g,
                                                         const DevScalars* __restrict__ sc, T Ux, T Uy, T Uz,
-                                                        double nu)
+                                                        double nu, int nbw)
 {
   __shared__ T lab[3][L1C];
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
-  int bx, by, bz;
-  block_coords(g, b, bx, by, bz);
+  const LabSlots s = lab_slots();
   const T* src[3] = {u, v, w};
-  gather_g1<T, 3>(src, lab, g, b, bx, by, bz);
-  __syncthreads();
+  const int b0 = xcd_remap(blockIdx.x, gridDim.x) * nbw;
+  LabRegs<T, 3> r;
+  lab_fetch(r, src, s, g, b0);
   const double dt = sc->dt;
   const T facA = (T)(-dt / (2 * g.h));
   const T facD = (T)((nu / g.h) * (dt / g.h));
-  const int c0 = 2 * threadIdx.x, ix0 = c0 & 7, iy = (c0 >> 3) & 7, iz = c0 >> 6;
-  T r[3][2];
-#pragma unroll
-  for (int k = 0; k < 2; ++k) {
-    const int l = ((iz + 1) * L1 + iy + 1) * L1 + ix0 + k + 1;
-    const T* U = lab[0];
-    const T* V = lab[1];
-    const T* W = lab[2];
-    const T dudx = U[l + 1] - U[l - 1], dvdx = V[l + 1] - V[l - 1], dwdx = W[l + 1] - W[l - 1];
-    const T dudy = U[l + L1] - U[l - L1], dvdy = V[l + L1] - V[l - L1], dwdy = W[l + L1] - W[l - L1];
-    const T dudz = U[l + L1 * L1] - U[l - L1 * L1], dvdz = V[l + L1 * L1] - V[l - L1 * L1],
-            dwdz = W[l + L1 * L1] - W[l - L1 * L1];
-    const T uu = U[l] + Ux, vv = V[l] + Uy, ww = W[l] + Uz;
-    const T duD = U[l + L1] + U[l - L1] + U[l + 1] + U[l - 1] + U[l - L1 * L1] + U[l + L1 * L1] - U[l] * 6;
-    const T dvD = V[l + L1] + V[l - L1] + V[l + 1] + V[l - 1] + V[l - L1 * L1] + V[l + L1 * L1] - V[l] * 6;
-    const T dwD = W[l + L1] + W[l - L1] + W[l + 1] + W[l - 1] + W[l - L1 * L1] + W[l + L1 * L1] - W[l] * 6;
-    const T duA = uu * dudx + vv * dudy + ww * dudz;
-    const T dvA = uu * dvdx + vv * dvdy + ww * dvdz;
-    const T dwA = uu * dwdx + vv * dwdy + ww * dwdz;
-    r[0][k] = U[l] + facA * duA + facD * duD;
-    r[1][k] = V[l] + facA * dvA + facD * dvD;
-    r[2][k] = W[l] + facA * dwA + facD * dwD;
-  }
+  const int c0 = 2 * threadIdx.x;
   using V2 = typename Vec2<T>::type;
-  const size_t o = (size_t)b * BS3 + c0;
-  *reinterpret_cast<V2*>(ou + o) = V2{r[0][0], r[0][1]};
-  *reinterpret_cast<V2*>(ov + o) = V2{r[1][0], r[1][1]};
-  *reinterpret_cast<V2*>(ow + o) = V2{r[2][0], r[2][1]};
+  for (int j = 0; j < nbw; ++j) {
+    const int b = b0 + j;
+    if (j) __syncthreads();  // the previous block's compute is done with the lab
+    lab_put(r, lab, s);
+    __syncthreads();
+    if (j + 1 < nbw) lab_fetch(r, src, s, g, b + 1);
+    T o[3][2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int l = s.li + k;
+      const T* U = lab[0];
+      const T* V = lab[1];
+      const T* W = lab[2];
+      const T dudx = U[l + 1] - U[l - 1], dvdx = V[l + 1] - V[l - 1], dwdx = W[l + 1] - W[l - 1];
+      const T dudy = U[l + L1] - U[l - L1], dvdy = V[l + L1] - V[l - L1], dwdy = W[l + L1] - W[l - L1];
+      const T dudz = U[l + L1 * L1] - U[l - L1 * L1], dvdz = V[l + L1 * L1] - V[l - L1 * L1],
+              dwdz = W[l + L1 * L1] - W[l - L1 * L1];
+      const T uu = U[l] + Ux, vv = V[l] + Uy, ww = W[l] + Uz;
+      const T duD = U[l + L1] + U[l - L1] + U[l + 1] + U[l - 1] + U[l - L1 * L1] + U[l + L1 * L1] - U[l] * 6;
+      const T dvD = V[l + L1] + V[l - L1] + V[l + 1] + V[l - 1] + V[l - L1 * L1] + V[l + L1 * L1] - V[l] * 6;
+      const T dwD = W[l + L1] + W[l - L1] + W[l + 1] + W[l - 1] + W[l - L1 * L1] + W[l + L1 * L1] - W[l] * 6;
+      const T duA = uu * dudx + vv * dudy + ww * dudz;
+      const T dvA = uu * dvdx + vv * dvdy + ww * dvdz;
+      const T dwA = uu * dwdx + vv * dwdy + ww * dwdz;
+      o[0][k] = U[l] + facA * duA + facD * duD;
+      o[1][k] = V[l] + facA * dvA + facD * dvD;
+      o[2][k] = W[l] + facA * dwA + facD * dwD;
+    }
+    const size_t q = (size_t)b * BS3 + c0;
+    *reinterpret_cast<V2*>(ou + q) = V2{o[0][0], o[0][1]};
+    *reinterpret_cast<V2*>(ov + q) = V2{o[1][0], o[1][1]};
+    *reinterpret_cast<V2*>(ow + q) = V2{o[2][0], o[2][1]};
+  }
 }
 
 template <typename T>
 static int advdiff_t(Ctx& c, double nu, const double uinf[3])
 {
-  hipLaunchKernelGGL((k_advdiff_central<T>), dim3(c.nb), dim3(NT), 0, c.stream, (const T*)c.f[CUP_U],
+  const int nbw = blocks_per_wg(c.nb);
+  hipLaunchKernelGGL((k_advdiff_central<T>), dim3(c.nb / nbw), dim3(NT), 0, c.stream, (const T*)c.f[CUP_U],
                      (const T*)c.f[CUP_V], (const T*)c.f[CUP_W], (T*)c.f[CUP_TMPU], (T*)c.f[CUP_TMPV],
-                     (T*)c.f[CUP_TMPW], c.grid(), c.sc, (T)uinf[0], (T)uinf[1], (T)uinf[2], nu);
+                     (T*)c.f[CUP_TMPW], c.grid(), c.sc, (T)uinf[0], (T)uinf[1], (T)uinf[2], nu, nbw);
   return check_hip(c, hipGetLastError(), "k_advdiff_central");
 }
 
+// ------------------------------------------------ third-order upwind advection
+// The upwind scheme (no reference counterpart; the reference's advection is the central
+// kernel above): per axis d with advecting velocity a = (s + U)_d at the cell, 6h times the
+// first derivative is 2f(+1) + 3f(0) - 6f(-1) + f(-2) for a > 0 and
+// -f(+2) + 6f(+1) - 3f(0) - 2f(-1) otherwise; diffusion is the 7-point Laplacian.
+// out = base + mult*dt * (-(s + U).grad_up3 s + nu lap s), with s read through a lab of
+// halo 2 (faces only, two layers deep: 12^3 in LDS) and base read at the cell only, so
+// out may alias base (the midpoint stage u* = u + dt L(u_half) updates u in place).
+constexpr int L2 = 12;
+constexpr int L2C = L2 * L2 * L2;
+
+template <typename T>
+__device__ __forceinline__ void gather_g2(const T* const* src, T (*lab)[L2C], const GridArgs& g, int b, int bx,
+                                          int by, int bz)
+{
+  const int tid = threadIdx.x;
+  {
+    const int c = 2 * tid, ix = c & 7, iy = (c >> 3) & 7, iz = c >> 6;
+    const int l = ((iz + 2) * L2 + iy + 2) * L2 + ix + 2;
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      const T* q = src[f] + (size_t)b * BS3 + c;
+      lab[f][l] = q[0];
+      lab[f][l + 1] = q[1];
+    }
+  }
+  for (int c = tid; c < 6 * 128; c += NT) {
+    const int face = c >> 7, layer = (c >> 6) & 1, a = (c >> 3) & 7, e = c & 7;
+    const bool plus = face & 1;
+    const int sd = plus ? layer : 6 + layer;      // source index along the face normal
+    const int ld = plus ? 10 + layer : layer;     // lab index along the normal
+    int qx = bx, qy = by, qz = bz, sx, sy, sz, lx, ly, lz;
+    if (face < 2) {
+      qx = wrapi(bx + (plus ? 1 : -1), g.nbx); sx = sd; sy = e; sz = a; lx = ld; ly = e + 2; lz = a + 2;
+    } else if (face < 4) {
+      qy = wrapi(by + (plus ? 1 : -1), g.nby); sx = e; sy = sd; sz = a; lx = e + 2; ly = ld; lz = a + 2;
+    } else {
+      qz = wrapi(bz + (plus ? 1 : -1), g.nbz); sx = e; sy = a; sz = sd; lx = e + 2; ly = a + 2; lz = ld;
+    }
+    const size_t sidx = (size_t)(qx + g.nbx * (qy + g.nby * qz)) * BS3 + (sz * BS + sy) * BS + sx;
+    const int l = (lz * L2 + ly) * L2 + lx;
+#pragma unroll
+    for (int f = 0; f < 3; ++f) lab[f][l] = src[f][sidx];
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ T up3(T a, T m2, T m1, T c, T p1, T p2)
+{
+  return a > 0 ? 2 * p1 + 3 * c - 6 * m1 + m2 : -p2 + 6 * p1 - 3 * c - 2 * m1;
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_advdiff_up3(const T* __restrict__ su, const T* __restrict__ sv,
+                                                    const T* __restrict__ sw, const T* bu, const T* bv, const T* bw,
+                                                    T* ou, T* ov, T* ow, GridArgs g,
+                                                    const DevScalars* __restrict__ sc, T Ux, T Uy, T Uz, double nu,
+                                                    double mult)
+{
+  __shared__ T lab[3][L2C];
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  int bx, by, bz;
+  block_coords(g, b, bx, by, bz);
+  const T* src[3] = {su, sv, sw};
+  gather_g2<T>(src, lab, g, b, bx, by, bz);
+  __syncthreads();
+  const double fac = mult * sc->dt;
+  const T facA = (T)(-fac / (6 * g.h));
+  const T facD = (T)(fac * nu / (g.h * g.h));
+  const int c0 = 2 * threadIdx.x, ix0 = c0 & 7, iy = (c0 >> 3) & 7, iz = c0 >> 6;
+  const size_t o = (size_t)b * BS3 + c0;
+  const T* base[3] = {bu, bv, bw};
+  T* out[3] = {ou, ov, ow};
+  T r[3][2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int l = ((iz + 2) * L2 + iy + 2) * L2 + ix0 + k + 2;
+    const T a0 = lab[0][l] + Ux, a1 = lab[1][l] + Uy, a2 = lab[2][l] + Uz;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const T* F = lab[q];
+      const T fc = F[l];
+      const T adv = a0 * up3(a0, F[l - 2], F[l - 1], fc, F[l + 1], F[l + 2]) +
+                    a1 * up3(a1, F[l - 2 * L2], F[l - L2], fc, F[l + L2], F[l + 2 * L2]) +
+                    a2 * up3(a2, F[l - 2 * L2 * L2], F[l - L2 * L2], fc, F[l + L2 * L2], F[l + 2 * L2 * L2]);
+      const T lap = (F[l - 1] + F[l + 1] - 2 * fc) + (F[l - L2] + F[l + L2] - 2 * fc) +
+                    (F[l - L2 * L2] + F[l + L2 * L2] - 2 * fc);
+      r[q][k] = facA * adv + facD * lap;
+    }
+  }
+  using V2 = typename Vec2<T>::type;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const V2 bb = *reinterpret_cast<const V2*>(base[q] + o);
+    *reinterpret_cast<V2*>(out[q] + o) = V2{bb.x + r[q][0], bb.y + r[q][1]};
+  }
+}
+
+template <typename T>
+static int up3_stage(Ctx& c, int src0, int base0, int out0, double nu, const double uinf[3], double mult)
+{
+  hipLaunchKernelGGL((k_advdiff_up3<T>), dim3(c.nb), dim3(NT), 0, c.stream, (const T*)c.f[src0],
+                     (const T*)c.f[src0 + 1], (const T*)c.f[src0 + 2], (const T*)c.f[base0], (const T*)c.f[base0 + 1],
+                     (const T*)c.f[base0 + 2], (T*)c.f[out0], (T*)c.f[out0 + 1], (T*)c.f[out0 + 2], c.grid(), c.sc,
+                     (T)uinf[0], (T)uinf[1], (T)uinf[2], nu, mult);
+  return check_hip(c, hipGetLastError(), "k_advdiff_up3");
+}
+
+// Advection-diffusion of the scheme into the step's intermediate velocity u*: the central
+// and upwind Euler forms write u* to tmp; the midpoint form writes u_half to tmp and then u*
+// over u. c.ustar names the field set holding u* afterwards (the fused step's consumers read
+// it; the plain decomposition copies it so that u = tmp = u* as after the reference's copy).
 int launch_advdiff(Ctx& c, double nu, const double uinf[3], int scheme)
 {
-  if (scheme != CUP_CENTRAL_EULER) return set_error(c, CUP_E_INVALID, "advect_diffuse: scheme not available");
-  return c.precision == 4 ? advdiff_t<float>(c, nu, uinf) : advdiff_t<double>(c, nu, uinf);
+  c.ustar = CUP_TMPU;
+  if (scheme == CUP_CENTRAL_EULER)
+    return c.precision == 4 ? advdiff_t<float>(c, nu, uinf) : advdiff_t<double>(c, nu, uinf);
+  if (scheme != CUP_UPWIND3_EULER && scheme != CUP_UPWIND3_RK2)
+    return set_error(c, CUP_E_INVALID, "advect_diffuse: unknown scheme");
+  auto stage = [&](int src0, int base0, int out0, double mult) {
+    return c.precision == 4 ? up3_stage<float>(c, src0, base0, out0, nu, uinf, mult)
+                            : up3_stage<double>(c, src0, base0, out0, nu, uinf, mult);
+  };
+  if (scheme == CUP_UPWIND3_EULER) return stage(CUP_U, CUP_U, CUP_TMPU, 1.0);
+  if (int rc = stage(CUP_U, CUP_U, CUP_TMPU, 0.5)) return rc;
+  if (int rc = stage(CUP_TMPU, CUP_U, CUP_U, 1.0)) return rc;
+  c.ustar = CUP_U;
+  return CUP_OK;
 }
 
 // -------------------------------------------------------------------- copies, fills
@@ -285,6 +512,20 @@ static dim3 ew_grid(size_t n)
   size_t g = (n + NT - 1) / NT;
   if (g > 65536) g = 65536;
   return dim3((unsigned)g);
+}
+
+// u -> tmp (after the midpoint scheme, which leaves u* in u).
+int launch_copy_u_to_tmp(Ctx& c)
+{
+  if (c.precision == 4)
+    hipLaunchKernelGGL((k_copy3<float>), ew_grid(c.ncell), dim3(NT), 0, c.stream, (float*)c.f[CUP_TMPU],
+                       (float*)c.f[CUP_TMPV], (float*)c.f[CUP_TMPW], (const float*)c.f[CUP_U],
+                       (const float*)c.f[CUP_V], (const float*)c.f[CUP_W], c.ncell);
+  else
+    hipLaunchKernelGGL((k_copy3<double>), ew_grid(c.ncell), dim3(NT), 0, c.stream, (double*)c.f[CUP_TMPU],
+                       (double*)c.f[CUP_TMPV], (double*)c.f[CUP_TMPW], (const double*)c.f[CUP_U],
+                       (const double*)c.f[CUP_V], (const double*)c.f[CUP_W], c.ncell);
+  return check_hip(c, hipGetLastError(), "k_copy3");
 }
 
 int launch_copy_tmp_to_u(Ctx& c)
@@ -353,7 +594,7 @@ __global__ __launch_bounds__(NT) void k_rhs(const T* __restrict__ u, const T* __
 
 int launch_rhs_plain(Ctx& c)
 {
-  const int src0 = c.decomposition == CUP_FUSED ? CUP_TMPU : CUP_U;
+  const int src0 = c.decomposition == CUP_FUSED ? c.ustar : CUP_U;
   if (c.precision == 4)
     hipLaunchKernelGGL((k_rhs<float>), dim3(c.nb), dim3(NT), 0, c.stream, (const float*)c.f[src0],
                        (const float*)c.f[src0 + 1], (const float*)c.f[src0 + 2], (float*)c.rbuf, c.grid(), c.sc);
@@ -371,80 +612,104 @@ int launch_rhs_plain(Ctx& c)
 // device scalar by an atomic max (order independent, so deterministic) and the block's
 // sum |u|^2 into ke_part. Blocks a penalized body covers (skip_slot >= 0) leave the max to
 // the penalization kernel, which changes their velocity afterwards.
-// Each thread owns two x-adjacent cells c = 2t, 2t+1 of the block (vector loads/stores).
+// Each thread owns two x-adjacent cells c = 2t, 2t+1 of the block (vector loads/stores);
+// a workgroup walks nbw consecutive blocks, fetching the next block's p lab, tmp velocity
+// and forcing rows while the current one is computed.
 // FORCE adds the fused forcing's velocity modes, fac*u_low = R0 + 2 Re(R1 e^{i a x}) +
-// 2 Re(R2 e^{2 i a x}) per component with the row coefficients R of k_forcing_rows.
+// 2 Re(R2 e^{2 i a x}) per component with the row coefficients R of k_forcing_rows; the
+// block's 64 rows x 16 coefficients are staged in LDS by one coalesced 4-value load per thread.
 template <typename T, bool FUSED, bool FORCE>
 __global__ __launch_bounds__(NT) void k_gradp(const T* __restrict__ p, const T* __restrict__ su,
                                               const T* __restrict__ sv, const T* __restrict__ sw, T* __restrict__ u,
                                               T* __restrict__ v, T* __restrict__ w, GridArgs g,
                                               DevScalars* __restrict__ sc, double Ux, double Uy, double Uz,
                                               double* __restrict__ ke_part, const int* __restrict__ skip_slot,
-                                              const T* __restrict__ rows, const C2<T>* __restrict__ twx)
+                                              const T* __restrict__ rows, const C2<T>* __restrict__ twx, int nbw)
 {
   using V2 = typename Vec2<T>::type;
   __shared__ T lab[1][L1C];
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
-  int bx, by, bz;
-  block_coords(g, b, bx, by, bz);
-  const T* src[1] = {p};
-  gather_g1<T, 1>(src, lab, g, b, bx, by, bz);
+  __shared__ T rl[FORCE ? 64 * 16 : 1];
   const int t = threadIdx.x;
-  __syncthreads();
-  const T fac = (T)(-0.5 * sc->dt / g.h);
+  const LabSlots s = lab_slots();
+  const T* src[1] = {p};
+  const int b0 = xcd_remap(blockIdx.x, gridDim.x) * nbw;
   const int c0 = 2 * t, ix0 = c0 & 7, iy = (c0 >> 3) & 7, iz = c0 >> 6;
-  const size_t o = (size_t)b * BS3 + c0;
-  const V2 a = *reinterpret_cast<const V2*>(su + o);
-  const V2 bb = *reinterpret_cast<const V2*>(sv + o);
-  const V2 cc = *reinterpret_cast<const V2*>(sw + o);
-  T nu_[2] = {a.x, a.y}, nv_[2] = {bb.x, bb.y}, nw_[2] = {cc.x, cc.y};
-  const T* P = lab[0];
+  LabRegs<T, 1> r;
+  T us[3][2];
+  T rr[4];
+  auto fetch = [&](int b) {
+    lab_fetch(r, src, s, g, b);
+    const size_t o = (size_t)b * BS3 + c0;
+    const V2 a = *reinterpret_cast<const V2*>(su + o);
+    const V2 bb = *reinterpret_cast<const V2*>(sv + o);
+    const V2 cc = *reinterpret_cast<const V2*>(sw + o);
+    us[0][0] = a.x; us[0][1] = a.y; us[1][0] = bb.x; us[1][1] = bb.y; us[2][0] = cc.x; us[2][1] = cc.y;
+    if (FORCE) {
+      // rows (y = by*8 + iy', z = bz*8 + t/32): 8 runs of 128 contiguous values
+      int bx, by, bz;
+      block_coords(g, b, bx, by, bz);
+      const T* q = rows + ((size_t)(bz * BS + (t >> 5)) * g.Ny + by * BS) * 16 + (t & 31) * 4;
+      const V2 q0 = *reinterpret_cast<const V2*>(q), q1 = *reinterpret_cast<const V2*>(q + 2);
+      rr[0] = q0.x; rr[1] = q0.y; rr[2] = q1.x; rr[3] = q1.y;
+    }
+  };
+  fetch(b0);
+  const T fac = (T)(-0.5 * sc->dt / g.h);
+  for (int j = 0; j < nbw; ++j) {
+    const int b = b0 + j;
+    if (j) __syncthreads();  // the previous block is done with the lab
+    lab_put(r, lab, s);
+    if (FORCE)
 #pragma unroll
-  for (int k = 0; k < 2; ++k) {
-    const int l = ((iz + 1) * L1 + iy + 1) * L1 + ix0 + k + 1;
-    nu_[k] += fac * (P[l + 1] - P[l - 1]);
-    nv_[k] += fac * (P[l + L1] - P[l - L1]);
-    nw_[k] += fac * (P[l + L1 * L1] - P[l - L1 * L1]);
-  }
-  if (FORCE) {
-    // this thread's row (iy, iz): 15 synthesis coefficients from k_forcing_rows
-    const T* Rr = rows + ((size_t)(bz * BS + iz) * g.Ny + (by * BS + iy)) * 16;
-    T R[16];
-#pragma unroll
-    for (int q = 0; q < 16; ++q) R[q] = Rr[q];
+      for (int i = 0; i < 4; ++i) rl[4 * t + i] = rr[i];
+    T nu_[2] = {us[0][0], us[0][1]}, nv_[2] = {us[1][0], us[1][1]}, nw_[2] = {us[2][0], us[2][1]};
+    __syncthreads();
+    if (j + 1 < nbw) fetch(b + 1);
+    const T* P = lab[0];
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
-      const int x = bx * BS + ix0 + k;
-      const C2<T> e1 = twx[x & (g.Nx - 1)], e2 = twx[(2 * x) & (g.Nx - 1)];  // e^{-i a x}, e^{-2 i a x}
-      // Re(R e^{+i a x}) with e^{+i a x} = conj(e1)
+      const int l = s.li + k;
+      nu_[k] += fac * (P[l + 1] - P[l - 1]);
+      nv_[k] += fac * (P[l + L1] - P[l - L1]);
+      nw_[k] += fac * (P[l + L1 * L1] - P[l - L1 * L1]);
+    }
+    if (FORCE) {
+      const int x0 = (b % g.nbx) * BS + ix0;
+      const T* R = rl + (iz * BS + iy) * 16;
 #pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        const T* Q = R + 5 * q;
-        const T add = Q[0] + 2 * (Q[1] * e1.x + Q[2] * e1.y) + 2 * (Q[3] * e2.x + Q[4] * e2.y);
-        if (q == 0) nu_[k] += add;
-        else if (q == 1) nv_[k] += add;
-        else nw_[k] += add;
+      for (int k = 0; k < 2; ++k) {
+        const C2<T> e1 = twx[(x0 + k) & (g.Nx - 1)];  // e^{-i a x}
+        const C2<T> e2 = {e1.x * e1.x - e1.y * e1.y, 2 * e1.x * e1.y};  // e^{-2 i a x}
+        // Re(R e^{+i a x}) with e^{+i a x} = conj(e1)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          const T* Q = R + 5 * q;
+          const T add = Q[0] + 2 * (Q[1] * e1.x + Q[2] * e1.y) + 2 * (Q[3] * e2.x + Q[4] * e2.y);
+          if (q == 0) nu_[k] += add;
+          else if (q == 1) nv_[k] += add;
+          else nw_[k] += add;
+        }
       }
     }
-  }
-  *reinterpret_cast<V2*>(u + o) = V2{nu_[0], nu_[1]};
-  *reinterpret_cast<V2*>(v + o) = V2{nv_[0], nv_[1]};
-  *reinterpret_cast<V2*>(w + o) = V2{nw_[0], nw_[1]};
-  if (FUSED) {
-    double r[2] = {0.0, 0.0};
+    const size_t o = (size_t)b * BS3 + c0;
+    *reinterpret_cast<V2*>(u + o) = V2{nu_[0], nu_[1]};
+    *reinterpret_cast<V2*>(v + o) = V2{nv_[0], nv_[1]};
+    *reinterpret_cast<V2*>(w + o) = V2{nw_[0], nw_[1]};
+    if (FUSED) {
+      double rd[2] = {0.0, 0.0};
 #pragma unroll
-    for (int k = 0; k < 2; ++k) {
-      const double aa = fabs((double)nu_[k] + Ux), b2 = fabs((double)nv_[k] + Uy), c2 = fabs((double)nw_[k] + Uz);
-      r[0] = fmax(r[0], fmax(aa, fmax(b2, c2)));
-      r[1] += (double)nu_[k] * nu_[k] + (double)nv_[k] * nv_[k] + (double)nw_[k] * nw_[k];
-    }
-    __shared__ double out[2];
-    block_reduce<2>(r, 1u, out);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      if (!skip_slot || skip_slot[b] < 0)
-        atomicMax(&sc->umax_slots[b & (UMAX_SLOTS - 1)], (unsigned long long)__double_as_longlong(out[0]));
-      ke_part[b] = out[1];
+      for (int k = 0; k < 2; ++k) {
+        const double aa = fabs((double)nu_[k] + Ux), b2 = fabs((double)nv_[k] + Uy), c2 = fabs((double)nw_[k] + Uz);
+        rd[0] = fmax(rd[0], fmax(aa, fmax(b2, c2)));
+        rd[1] += (double)nu_[k] * nu_[k] + (double)nv_[k] * nv_[k] + (double)nw_[k] * nw_[k];
+      }
+      __shared__ double out[2];
+      block_reduce<2>(rd, 1u, out);
+      if (threadIdx.x == 0) {
+        if (!skip_slot || skip_slot[b] < 0)
+          atomicMax(&sc->umax_slots[b & (UMAX_SLOTS - 1)], (unsigned long long)__double_as_longlong(out[0]));
+        ke_part[b] = out[1];
+      }
     }
   }
 }
@@ -453,22 +718,23 @@ template <typename T>
 static int gradp_t(Ctx& c, bool fused, const double uinf[3], const int* skip, bool force)
 {
   const T* p = (const T*)c.f[CUP_P];
-  const int s = fused ? CUP_TMPU : CUP_U;
+  const int s = fused ? c.ustar : CUP_U;
   const C2<T>* tx = (const C2<T>*)c.tw[0];
-  const C2<T>* ty = (const C2<T>*)c.tw[1];
-  const C2<T>* tz = (const C2<T>*)c.tw[2];
+  const int nbw = blocks_per_wg(c.nb);
+  const dim3 grid(c.nb / nbw);
   if (fused && force)
-    hipLaunchKernelGGL((k_gradp<T, true, true>), dim3(c.nb), dim3(NT), 0, c.stream, p, (const T*)c.f[s],
+    hipLaunchKernelGGL((k_gradp<T, true, true>), grid, dim3(NT), 0, c.stream, p, (const T*)c.f[s],
                        (const T*)c.f[s + 1], (const T*)c.f[s + 2], (T*)c.f[CUP_U], (T*)c.f[CUP_V], (T*)c.f[CUP_W],
-                       c.grid(), c.sc, uinf[0], uinf[1], uinf[2], c.ke_part, skip, (const T*)c.low_rows, tx);
+                       c.grid(), c.sc, uinf[0], uinf[1], uinf[2], c.ke_part, skip, (const T*)c.low_rows, tx, nbw);
   else if (fused)
-    hipLaunchKernelGGL((k_gradp<T, true, false>), dim3(c.nb), dim3(NT), 0, c.stream, p, (const T*)c.f[s],
+    hipLaunchKernelGGL((k_gradp<T, true, false>), grid, dim3(NT), 0, c.stream, p, (const T*)c.f[s],
                        (const T*)c.f[s + 1], (const T*)c.f[s + 2], (T*)c.f[CUP_U], (T*)c.f[CUP_V], (T*)c.f[CUP_W],
-                       c.grid(), c.sc, uinf[0], uinf[1], uinf[2], c.ke_part, skip, (const T*)nullptr, tx);
+                       c.grid(), c.sc, uinf[0], uinf[1], uinf[2], c.ke_part, skip, (const T*)nullptr, tx, nbw);
   else
-    hipLaunchKernelGGL((k_gradp<T, false, false>), dim3(c.nb), dim3(NT), 0, c.stream, p, (const T*)c.f[s],
+    hipLaunchKernelGGL((k_gradp<T, false, false>), grid, dim3(NT), 0, c.stream, p, (const T*)c.f[s],
                        (const T*)c.f[s + 1], (const T*)c.f[s + 2], (T*)c.f[CUP_U], (T*)c.f[CUP_V], (T*)c.f[CUP_W],
-                       c.grid(), c.sc, 0.0, 0.0, 0.0, (double*)nullptr, (const int*)nullptr, (const T*)nullptr, tx);
+                       c.grid(), c.sc, 0.0, 0.0, 0.0, (double*)nullptr, (const int*)nullptr, (const T*)nullptr, tx,
+                       nbw);
   return check_hip(c, hipGetLastError(), "k_gradp");
 }
 

@@ -42,7 +42,11 @@ enum cup_field {
  * into the z pass, reductions into the gradient epilogue, dt kept on the device). */
 enum cup_decomposition { CUP_PLAIN = 0, CUP_FUSED = 1 };
 
-/* Advection scheme. CENTRAL_EULER is the reference's (AdvectionDiffusion.cpp:106-145). */
+/* Advection scheme. CENTRAL_EULER is the reference's (AdvectionDiffusion.cpp:106-145).
+ * UPWIND3_* (no reference counterpart) advect with the third-order upwind-biased stencil
+ * (6h f' = 2f(+1) + 3f(0) - 6f(-1) + f(-2) for positive advecting velocity, mirrored
+ * otherwise) and diffuse with the 7-point Laplacian, stepped by Euler or by the explicit
+ * midpoint rule u* = u + dt L(u + dt/2 L(u)). */
 enum cup_scheme { CUP_CENTRAL_EULER = 0, CUP_UPWIND3_EULER = 1, CUP_UPWIND3_RK2 = 2 };
 
 typedef struct cup_grid_desc {
@@ -60,6 +64,8 @@ typedef struct cup_ctx cup_ctx;
  * plans (PoissonSolverPeriodic.cpp:90-127). Only fully periodic boxes are supported. */
 int cup_create(const cup_grid_desc* desc, cup_ctx** out);
 int cup_destroy(cup_ctx* ctx);
+/* The context's last error message; with ctx == NULL, the message of the calling thread's
+ * last failed cup_create. */
 const char* cup_last_error(const cup_ctx* ctx);
 /* cells[3], h, extent[3] */
 int cup_get_geometry(const cup_ctx* ctx, int cells[3], double* h, double extent[3]);

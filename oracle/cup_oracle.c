@@ -309,6 +309,104 @@ void orc_advect_diffuse(orc_sim* s, double dt, double nu, const double uinf[3])
   }
 }
 
+/* Third-order upwind-biased scheme with central diffusion (no reference counterpart: the
+ * reference advects with the central scheme above). Per axis d, with the advecting velocity
+ * a = (u + U)_d at the cell, the first derivative times 6h is taken on the upwind side:
+ *   a > 0:  2 f(+1) + 3 f(0) - 6 f(-1) + f(-2)
+ *   a <= 0: -f(+2) + 6 f(+1) - 3 f(0) - 2 f(-1)
+ * (symbol i k h - (k h)^4 / 12 sign(a) + O((kh)^5): third order, dissipative). Diffusion is
+ * the 7-point Laplacian. L(f) = -(f + U).grad_up3 f + nu lap f. */
+static inline double up3(double a, double m2, double m1, double c, double p1, double p2)
+{
+  return a > 0 ? 2 * p1 + 3 * c - 6 * m1 + m2 : -p2 + 6 * p1 - 3 * c - 2 * m1;
+}
+
+static inline double fe_comp(const FE* e, int q, int tmp)
+{
+  if (tmp) return q == 0 ? e->tmpU : (q == 1 ? e->tmpV : e->tmpW);
+  return q == 0 ? e->u : (q == 1 ? e->v : e->w);
+}
+
+/* out[q][cell] = u_q + fac * L(src)_q, src = (u,v,w) or (tmpU,tmpV,tmpW); out is
+ * block-major, 3 x ncell. */
+static void upwind3_update(orc_sim* s, int src_tmp, double fac, double nu, const double uinf[3], double* out)
+{
+  const double h = s->h;
+  const size_t nc = (size_t)s->nb * BS * BS * BS;
+  #pragma omp parallel
+  {
+    Lab l = lab_alloc(2);
+    #pragma omp for schedule(static)
+    for (int b = 0; b < s->nb; b++) {
+      lab_load(s, b, &l);
+      for (int iz = 0; iz < BS; iz++)
+        for (int iy = 0; iy < BS; iy++)
+          for (int ix = 0; ix < BS; ix++) {
+            const FE* C = LAB(&l, ix, iy, iz);
+            double a[3];
+            for (int d = 0; d < 3; d++) a[d] = fe_comp(C, d, src_tmp) + uinf[d];
+            const size_t cell = (size_t)b * BS * BS * BS + (iz * BS + iy) * BS + ix;
+            for (int q = 0; q < 3; q++) {
+              double f[3][5];
+              for (int k = -2; k <= 2; k++) {
+                f[0][k + 2] = fe_comp(LAB(&l, ix + k, iy, iz), q, src_tmp);
+                f[1][k + 2] = fe_comp(LAB(&l, ix, iy + k, iz), q, src_tmp);
+                f[2][k + 2] = fe_comp(LAB(&l, ix, iy, iz + k), q, src_tmp);
+              }
+              double adv = 0, lap = 0;
+              for (int d = 0; d < 3; d++) {
+                adv += a[d] * up3(a[d], f[d][0], f[d][1], f[d][2], f[d][3], f[d][4]);
+                lap += f[d][1] + f[d][3] - 2 * f[d][2];
+              }
+              const double rhs = -adv / (6 * h) + nu * lap / (h * h);
+              out[q * nc + cell] = fe_comp(&s->blk[b].d[iz][iy][ix], q, 0) + fac * rhs;
+            }
+          }
+    }
+    free(l.c);
+  }
+}
+
+static void store_uvw_tmp(orc_sim* s, const double* in, int also_u)
+{
+  const size_t nc = (size_t)s->nb * BS * BS * BS;
+  #pragma omp parallel for schedule(static)
+  for (int b = 0; b < s->nb; b++) {
+    FE* e = &s->blk[b].d[0][0][0];
+    for (int i = 0; i < BS * BS * BS; i++) {
+      const size_t c = (size_t)b * BS * BS * BS + i;
+      e[i].tmpU = in[c];
+      e[i].tmpV = in[nc + c];
+      e[i].tmpW = in[2 * nc + c];
+      if (also_u) {
+        e[i].u = e[i].tmpU;
+        e[i].v = e[i].tmpV;
+        e[i].w = e[i].tmpW;
+      }
+    }
+  }
+}
+
+void orc_advect_diffuse_scheme(orc_sim* s, double dt, double nu, const double uinf[3], int scheme)
+{
+  if (scheme == 0) {
+    orc_advect_diffuse(s, dt, nu, uinf);
+    return;
+  }
+  const size_t nc = (size_t)s->nb * BS * BS * BS;
+  double* out = (double*)malloc(3 * nc * sizeof(double));
+  if (scheme == 1) {  /* Euler: u* = u + dt L(u) */
+    upwind3_update(s, 0, dt, nu, uinf, out);
+    store_uvw_tmp(s, out, 1);
+  } else {            /* explicit midpoint: u1/2 = u + dt/2 L(u); u* = u + dt L(u1/2) */
+    upwind3_update(s, 0, 0.5 * dt, nu, uinf, out);
+    store_uvw_tmp(s, out, 0);
+    upwind3_update(s, 1, dt, nu, uinf, out);
+    store_uvw_tmp(s, out, 1);
+  }
+  free(out);
+}
+
 static inline size_t ridx(const orc_sim* s, int gx, int gy, int gz)
 {
   return ((size_t)gx * s->N[1] + gy) * s->N[2] + gz;
@@ -1135,7 +1233,7 @@ void orc_surface_forces(const orc_sim* s, double nu, const double uTrans[3], con
 void orc_timestep(orc_sim* s, double dt, const orc_params* prm, double forcing_stats[5],
                   double penal_ft[6], double surf_qoi[22])
 {
-  orc_advect_diffuse(s, dt, prm->nu, prm->uinf);
+  orc_advect_diffuse_scheme(s, dt, prm->nu, prm->uinf, prm->scheme);
   if (prm->spectral_forcing)
     orc_spectral_forcing(s, dt, dt, prm->nu, prm->tke_target, prm->injection_rate, forcing_stats);
   orc_pressure_rhs(s, dt);

@@ -46,6 +46,10 @@ double orc_calc_max_timestep(const orc_sim* s, double CFL, double nu, const doub
 
 /* Operators. */
 void orc_advect_diffuse(orc_sim* s, double dt, double nu, const double uinf[3]);
+/* scheme 0: the reference's central/Euler (orc_advect_diffuse); 1: third-order upwind
+ * advection + central diffusion, Euler; 2: the same with the explicit midpoint RK2.
+ * Schemes 1-2 have no reference counterpart; u = tmp = u* afterwards in every case. */
+void orc_advect_diffuse_scheme(orc_sim* s, double dt, double nu, const double uinf[3], int scheme);
 void orc_pressure_rhs(orc_sim* s, double dt);
 void orc_poisson_solve(orc_sim* s);
 void orc_grad_p(orc_sim* s, double dt);
@@ -83,6 +87,7 @@ typedef struct {
   int implicit_penalization;
   double trans_vel[3];
   double ang_vel[3];
+  int scheme; /* as orc_advect_diffuse_scheme */
 } orc_params;
 void orc_timestep(orc_sim* s, double dt, const orc_params* prm, double forcing_stats[5],
                   double penal_ft[6], double surf_qoi[22]);

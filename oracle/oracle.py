@@ -51,6 +51,7 @@ def lib():
         "orc_find_max_u": (d, [p, dp]),
         "orc_calc_max_timestep": (d, [p, d, d, dp, i, i, dp]),
         "orc_advect_diffuse": (None, [p, d, d, dp]),
+        "orc_advect_diffuse_scheme": (None, [p, d, d, dp, i]),
         "orc_pressure_rhs": (None, [p, d]),
         "orc_poisson_solve": (None, [p]),
         "orc_grad_p": (None, [p, d]),
@@ -170,8 +171,9 @@ class OracleSim:
                                          int(step), int(rampup), C.byref(um))
         return dt, um.value
 
-    def advect_diffuse(self, dt, nu, uinf=(0, 0, 0)):
-        lib().orc_advect_diffuse(self._h, float(dt), float(nu), _dp(_vec(uinf)))
+    def advect_diffuse(self, dt, nu, uinf=(0, 0, 0), scheme=0):
+        """scheme 0: central/Euler (the reference); 1: upwind3/Euler; 2: upwind3/RK2 midpoint."""
+        lib().orc_advect_diffuse_scheme(self._h, float(dt), float(nu), _dp(_vec(uinf)), int(scheme))
 
     def pressure_rhs(self, dt):
         lib().orc_pressure_rhs(self._h, float(dt))
@@ -228,11 +230,11 @@ class OracleSim:
         return q
 
     def timestep(self, dt, nu, uinf=(0, 0, 0), spectral_forcing=False, tke_target=0.0,
-                 sphere=None):
+                 sphere=None, scheme=0):
         """One step in the reference pipeline order. ``sphere``: dict(lambda_, implicit,
         trans_vel, ang_vel) when a sphere was created with sphere_create."""
         out = {}
-        self.advect_diffuse(dt, nu, uinf)
+        self.advect_diffuse(dt, nu, uinf, scheme)
         if spectral_forcing:
             out["forcing"] = self.spectral_forcing(dt, nu, tke_target=tke_target)
         self.pressure_rhs(dt)

@@ -1,0 +1,113 @@
+"""CPU tests of the boundary: libcup.so loads and exports exactly the C ABI that
+include/cup.h declares, argument errors come back as codes with messages (no device is
+touched), and the host C++ layer parses the reference's arguments
+(SimulationData.cpp:32-172, tests/test_grid.py, tests/cubismup3d/TestBufferedLogger.cpp)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "cup.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    return set(re.findall(r"^(?:int|const char\*)\s+(cup_\w+)\s*\(", text, flags=re.M))
+
+
+def test_header_matches_binding_table(cup):
+    from cubismup_3d_amd import _lib
+    assert declared_symbols() == set(_lib.SIGNATURES)
+
+
+def test_library_exports_every_declared_symbol(cup):
+    from cubismup_3d_amd import _lib
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = {ln.split()[-1] for ln in out.splitlines() if " T " in ln}
+    missing = declared_symbols() - exported
+    assert not missing, f"declared in cup.h but not exported: {sorted(missing)}"
+    L = _lib.lib()
+    for name in declared_symbols():
+        assert getattr(L, name) is not None
+
+
+def test_create_rejects_bad_descriptors_with_messages(cup):
+    from cubismup_3d_amd import _lib
+    L = _lib.lib()
+    h = C.c_void_p()
+    assert L.cup_create(None, C.byref(h)) == _lib.CUP_E_INVALID
+    assert b"null" in L.cup_last_error(None)
+    cases = [((0, 4, 4), 1.0, 8, 0, b"blocks"), ((4, 4, 4), 1.0, 3, 0, b"precision"),
+             ((4, 4, 4), 1.0, 8, 7, b"decomposition"), ((4, 4, 4), 0.0, 8, 0, b"maxextent")]
+    for bpd, ext, prec, dec, word in cases:
+        d = _lib.GridDesc((C.c_int * 3)(*bpd), ext, prec, dec, 0)
+        assert L.cup_create(C.byref(d), C.byref(h)) == _lib.CUP_E_INVALID
+        assert h.value is None
+        assert word in L.cup_last_error(None)
+
+
+def test_entry_points_reject_null_context(cup):
+    from cubismup_3d_amd import _lib
+    L = _lib.lib()
+    assert L.cup_destroy(None) == _lib.CUP_E_INVALID
+    assert L.cup_synchronize(None) == _lib.CUP_E_INVALID
+    assert L.cup_pressure_rhs(None, 1e-3) == _lib.CUP_E_INVALID
+    assert L.cup_poisson_periodic(None) == _lib.CUP_E_INVALID
+    assert L.cup_sim_run(None) == _lib.CUP_E_INVALID
+
+
+# tests/test_grid.py of the reference, against the Python surface of the host layer
+def test_cells_are_required(cup):
+    with pytest.raises(TypeError):
+        cup.SimulationData()
+    with pytest.raises(TypeError):
+        cup.SimulationData(cells=[64, 64, 64])
+
+
+def test_cell_multiple_of_block_size(cup):
+    for cells in ([128, 128, 129], [128, 129, 128], [129, 128, 128]):
+        with pytest.raises(ValueError):
+            cup.SimulationData(cells=cells, CFL=0.1)
+    sd = cup.SimulationData(cells=[64, 32, 32], CFL=0.1)
+    assert (sd.bpdx, sd.bpdy, sd.bpdz) == (8, 4, 4)
+
+
+def test_reference_defaults_and_flags(cup):
+    # defaults of SimulationData.cpp:32-172
+    sd = cup.SimulationData(argv=["-bpdx", "2", "-bpdy", "2", "-bpdz", "2", "-nu", "0.01"])
+    assert sd.CFL == 0.1 and getattr(sd, "lambda") == 1e6
+    assert sd.rampup == 100 and sd.freqDiagnostics == 100 and sd.nsteps == 0
+    assert sd.nu == 0.01 and sd.uinf_x == 0.0
+    sd2 = cup.SimulationData(argv=["-bpdx", "4", "-bpdy", "2", "-bpdz", "2", "-extentx", "2", "-CFL", "0.3",
+                                   "-uinfx", "0.5", "-rampup", "7", "-nsteps", "12", "-lambda", "1e5",
+                                   "-nu", "0.001"])
+    assert (sd2.CFL, sd2.uinf_x, sd2.rampup, sd2.nsteps, getattr(sd2, "lambda")) == (0.3, 0.5, 7, 12, 1e5)
+    with pytest.raises(ValueError):  # -nu has no default (SimulationData.cpp:50)
+        cup.SimulationData(argv=["-bpdx", "2", "-bpdy", "2", "-bpdz", "2"])
+    sd2.nsteps = 3
+    assert sd2.nsteps == 3
+    with pytest.raises(AttributeError):
+        sd2.no_such_member
+
+
+def test_sphere_arguments(cup):
+    with pytest.raises(ValueError):
+        cup.Sphere(position=[0.3, 0.4, 0.5], radius=0.0)
+    s = cup.Sphere(position=[0.3, 0.4, 0.5], radius=0.1, xvel=0.2)
+    assert s.trans_vel == [-0.2, 0.0, 0.0]  # enforcedVelocity = -xvel (Obstacle.cpp:64)
+
+
+def test_buffered_logger_flushes_like_the_reference(cup, tmp_path):
+    from cubismup_3d_amd import _lib
+    src = os.path.join(ROOT, "tests", "cpp", "buffered_logger_main.cpp")
+    exe = str(tmp_path / "buffered_logger")
+    libdir = os.path.dirname(_lib.LIB_PATH)
+    subprocess.run(["g++", "-std=c++17", "-O1", src, "-I", os.path.join(ROOT, "cubismup_3d_amd", "csrc", "host"),
+                    "-L", libdir, "-lcup", f"-Wl,-rpath,{libdir}", "-o", exe], check=True)
+    r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.strip() == "ok"
