@@ -59,12 +59,25 @@ def time_steps(G, steps, tke, **kw):
     return time.perf_counter() - t0
 
 
-def cpu_baseline(n, warm=1, timed=3):
-    """FP64 oracle (OpenMP over blocks, the reference CPU path's structure) on the same
-    workload: `warm` + `timed` steps at n^3 on every host core."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(n, warm=2, timed=5):
+    """FP64 oracle (OpenMP over blocks, the reference CPU path's structure: AoS blocks,
+    per-block labs, its own FFT) on the same workload: `warm` + `timed` steps at n^3 on the
+    host cores this process is given (OMP_NUM_THREADS, else every core), threads bound close."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")
     from oracle import oracle as O
     O.build()
-    threads = os.cpu_count() or 1
+    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
     O.lib().orc_set_threads(threads)
     s = O.OracleSim((n // 8,) * 3)
     s.ic_hit(4.0, 1.0, SEED)
@@ -82,9 +95,9 @@ def cpu_baseline(n, warm=1, timed=3):
         s.timestep(dt, NU, spectral_forcing=True, tke_target=tke)
         step += 1
     el = time.perf_counter() - t0
-    return {"value": n ** 3 * timed / el / 1e6, "unit": "Mcell-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{n}^3 forced isotropic turbulence, FP64 oracle, {warm} warm-up + {timed} timed steps "
-                      f"({el:.1f} s), OpenMP threads={threads}"}
+    return {"value": round(n ** 3 * timed / el / 1e6, 2), "unit": "Mcell-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{n}^3 forced isotropic turbulence, FP64 oracle (OpenMP, {threads} threads, {cpu_model()}), "
+                      f"{warm} warm-up + {timed} timed steps ({el:.1f} s)"}
 
 
 def main():
@@ -95,7 +108,7 @@ def main():
     ap.add_argument("--size", type=int, default=256, help="cells per side")
     ap.add_argument("--plain-steps", type=int, default=20, help="timed steps of the FP64 plain form")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -172,7 +185,7 @@ def main():
         "final_state": {"step": state["step"], "time": state["time"], "umax": state["umax"]},
     }
     if not args.no_cpu_baseline:
-        cb = cpu_baseline(n, 1, args.cpu_steps)
+        cb = cpu_baseline(n, 2, args.cpu_steps)
         out["cpu_baseline"] = cb
     print(json.dumps(out), flush=True)
 

@@ -19,7 +19,8 @@ CUP_E_STATE = -3
 CUP_E_NOMEM = -4
 
 CUP_CHI, CUP_U, CUP_V, CUP_W, CUP_P, CUP_TMPU, CUP_TMPV, CUP_TMPW = range(8)
-FIELD_IDS = {"chi": 0, "u": 1, "v": 2, "w": 3, "p": 4, "tmpU": 5, "tmpV": 6, "tmpW": 7}
+FIELD_IDS = {"chi": 0, "u": 1, "v": 2, "w": 3, "p": 4, "tmpU": 5, "tmpV": 6, "tmpW": 7,
+             "adjU": 8, "adjV": 9, "adjW": 10}
 CUP_PLAIN, CUP_FUSED = 0, 1
 CUP_CENTRAL_EULER, CUP_UPWIND3_EULER, CUP_UPWIND3_RK2 = 0, 1, 2
 
@@ -51,6 +52,8 @@ SIGNATURES = {
     "cup_set_field": (_i, [_p, _i, _dp]),
     "cup_get_field": (_i, [_p, _i, _dp]),
     "cup_field_device_ptr": (_i, [_p, _i, C.POINTER(C.c_void_p)]),
+    "cup_set_field_device": (_i, [_p, _i, _p]),
+    "cup_get_field_device": (_i, [_p, _i, _p]),
     "cup_lab_shift": (_i, [_p, _i, _i, _i, _i]),
     "cup_max_velocity": (_i, [_p, _dp, _dp]),
     "cup_calc_max_timestep": (_i, [_p, _d, _d, _dp, _i, _i, _dp, _dp]),
@@ -68,6 +71,7 @@ SIGNATURES = {
     "cup_run_steps": (_i, [_p, C.POINTER(StepParams), _i]),
     "cup_step_state": (_i, [_p, _ip, _dp, _dp, _dp, _dp]),
     "cup_set_step_state": (_i, [_p, _i, _d]),
+    "cup_adjoint_step": (_i, [_p, _d, _d, _dp, _i]),
     "cup_stream": (_i, [_p, C.POINTER(C.c_void_p)]),
     "cup_stage_timing_begin": (_i, [_p, _i, _i]),
     "cup_stage_timing_end": (_i, [_p, _dp, _ip]),

@@ -1,0 +1,82 @@
+"""Differentiable rollouts of the periodic timestep for torch.
+
+``rollout(u0, grid, nsteps, nu, ...)`` advances a [3][Z][Y][X] velocity tensor through
+nsteps steps of the device path (central advection with Euler, FFT projection; no forcing,
+no obstacle) and is a ``torch.autograd.Function``: its backward is the hand-written
+adjoint of the step (cup_adjoint_step: the transposed linearised advection-diffusion
+stencil and the symmetric projection), run backwards over the steps with
+checkpoint-and-recompute: the forward keeps the velocity every ``every`` steps, and the
+backward recomputes each segment's states from its checkpoint before stepping the adjoint
+through it. Timesteps are those of the forward pass, held fixed (the CFL's max over the
+velocity is not differentiated). No counterpart in the reference.
+"""
+from __future__ import annotations
+
+import torch
+
+from .grid import Grid
+
+
+class Rollout(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, u0, grid: Grid, nsteps: int, nu: float, uinf, dts, every: int, CFL: float, dts_out: list):
+        """dts: fixed timesteps, or None for CFL timesteps (appended to dts_out)."""
+        grid.set_torch_velocity(u0.detach())
+        ckpts = {0: u0.detach().clone()}
+        used = []
+        for n in range(nsteps):
+            if dts is None:
+                grid.run_steps(1, CFL=CFL, nu=nu, uinf=uinf)
+                used.append(grid.step_state()["last_dt"])
+            else:
+                grid.run_steps(1, nu=nu, uinf=uinf, fixed_dt=dts[n])
+                used.append(float(dts[n]))
+            if (n + 1) % every == 0 and n + 1 < nsteps:
+                ckpts[n + 1] = grid.get_torch_velocity()
+        dts_out.extend(used)
+        ctx.grid, ctx.nsteps, ctx.nu, ctx.uinf, ctx.dts, ctx.every = grid, nsteps, nu, uinf, used, every
+        ctx.ckpts = ckpts
+        ctx.u0_dtype = u0.dtype
+        return grid.get_torch_velocity()
+
+    @staticmethod
+    def backward(ctx, g):
+        grid, every = ctx.grid, ctx.every
+        lam = g.detach().contiguous().to(grid._torch_dtype())
+        for s0 in sorted(ctx.ckpts, reverse=True):
+            s1 = min(s0 + every, ctx.nsteps)
+            states = [ctx.ckpts[s0]]
+            grid.set_torch_velocity(ctx.ckpts[s0])
+            for n in range(s0, s1 - 1):
+                grid.run_steps(1, nu=ctx.nu, uinf=ctx.uinf, fixed_dt=ctx.dts[n])
+                states.append(grid.get_torch_velocity())
+            for n in reversed(range(s0, s1)):
+                grid.set_torch_velocity(states[n - s0])
+                grid.set_torch_adjoint(lam)
+                grid.adjoint_step(ctx.dts[n], ctx.nu, ctx.uinf)
+                lam = grid.get_torch_adjoint()
+        ctx.ckpts = None
+        return lam.to(ctx.u0_dtype), None, None, None, None, None, None, None, None
+
+
+def rollout(u0, grid: Grid, nsteps: int, nu: float, uinf=(0.0, 0.0, 0.0), dts=None, CFL=0.3, every=10,
+            dts_out=None):
+    """u_N of a rollout, differentiable with respect to u0 ([3][Z][Y][X], cuda). With dts
+    None the steps take CFL timesteps (rampup 0), which are appended to dts_out and held
+    fixed by the backward."""
+    out = dts_out if dts_out is not None else []
+    return Rollout.apply(u0, grid, int(nsteps), float(nu), tuple(float(x) for x in uinf),
+                         None if dts is None else list(dts), int(every), float(CFL), out)
+
+
+def kinetic_energy(u, h):
+    """KE = h^3/2 sum |u|^2 (ComputeDissipation.cpp:109)."""
+    return 0.5 * h ** 3 * (u * u).sum()
+
+
+def recompute_memory_bytes(grid: Grid, nsteps: int, every: int) -> int:
+    """Peak device bytes the rollout keeps for its backward: the checkpoints plus one
+    recomputed segment of states, three components each."""
+    ncell = grid.cells[0] * grid.cells[1] * grid.cells[2]
+    nck = (nsteps + every - 1) // every
+    return (nck + every) * 3 * ncell * grid.precision

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "cup_internal.h"
@@ -69,6 +70,12 @@ static void release(Ctx& c)
   if (c.host_red) hipHostFree(c.host_red);
   c.partials = nullptr; c.low_coef = nullptr; c.sc = nullptr; c.sc_host = nullptr;
   c.host_red = nullptr;
+  if (c.side) hipStreamSynchronize(c.side);
+  if (c.ev_fork) hipEventDestroy(c.ev_fork);
+  if (c.ev_join) hipEventDestroy(c.ev_join);
+  if (c.side) hipStreamDestroy(c.side);
+  c.ev_fork = c.ev_join = nullptr;
+  c.side = nullptr;
   if (c.stream) hipStreamDestroy(c.stream);
   c.stream = nullptr;
 }
@@ -223,10 +230,20 @@ int cup_stream(cup_ctx* ctx, void** s)
   return CUP_OK;
 }
 
+// Valid field id; the adjoint fields are allocated (zeroed) on first use.
+static int field_ready(Ctx& c, int field, const char* what)
+{
+  if (field < 0 || field > CUP_ADJ_W) return set_error(c, CUP_E_INVALID, std::string(what) + ": bad field");
+  if (c.f[field]) return CUP_OK;
+  if (check_hip(c, hipMalloc(&c.f[field], c.ncell * c.esz), "hipMalloc adjoint field")) return CUP_E_NOMEM;
+  return check_hip(c, hipMemsetAsync(c.f[field], 0, c.ncell * c.esz, c.stream), "hipMemset adjoint field");
+}
+
 int cup_set_field(cup_ctx* ctx, int field, const double* host)
 {
   CUP_GUARD(ctx);
-  if (field < 0 || field > 7 || !host) return set_error(c, CUP_E_INVALID, "set_field: bad field or pointer");
+  if (!host) return set_error(c, CUP_E_INVALID, "set_field: null pointer");
+  if (int rc = field_ready(c, field, "set_field")) return rc;
   hipStreamSynchronize(c.stream);
   hipError_t e;
   if (c.precision == 4) {
@@ -242,7 +259,8 @@ int cup_set_field(cup_ctx* ctx, int field, const double* host)
 int cup_get_field(cup_ctx* ctx, int field, double* host)
 {
   CUP_GUARD(ctx);
-  if (field < 0 || field > 7 || !host) return set_error(c, CUP_E_INVALID, "get_field: bad field or pointer");
+  if (!host) return set_error(c, CUP_E_INVALID, "get_field: null pointer");
+  if (int rc = field_ready(c, field, "get_field")) return rc;
   if (int rc = check_hip(c, hipStreamSynchronize(c.stream), "synchronize")) return rc;
   hipError_t e;
   if (c.precision == 4) {
@@ -258,9 +276,28 @@ int cup_get_field(cup_ctx* ctx, int field, double* host)
 int cup_field_device_ptr(cup_ctx* ctx, int field, void** dptr)
 {
   CUP_GUARD(ctx);
-  if (field < 0 || field > 7 || !dptr) return set_error(c, CUP_E_INVALID, "field_device_ptr: bad field");
+  if (!dptr) return set_error(c, CUP_E_INVALID, "field_device_ptr: null pointer");
+  if (int rc = field_ready(c, field, "field_device_ptr")) return rc;
   *dptr = c.f[field];
   return CUP_OK;
+}
+
+int cup_set_field_device(cup_ctx* ctx, int field, const void* src)
+{
+  CUP_GUARD(ctx);
+  if (!src) return set_error(c, CUP_E_INVALID, "set_field_device: null pointer");
+  if (int rc = field_ready(c, field, "set_field_device")) return rc;
+  return check_hip(c, hipMemcpyAsync(c.f[field], src, c.ncell * c.esz, hipMemcpyDeviceToDevice, c.stream),
+                   "set_field_device");
+}
+
+int cup_get_field_device(cup_ctx* ctx, int field, void* dst)
+{
+  CUP_GUARD(ctx);
+  if (!dst) return set_error(c, CUP_E_INVALID, "get_field_device: null pointer");
+  if (int rc = field_ready(c, field, "get_field_device")) return rc;
+  return check_hip(c, hipMemcpyAsync(dst, c.f[field], c.ncell * c.esz, hipMemcpyDeviceToDevice, c.stream),
+                   "get_field_device");
 }
 
 int cup_lab_shift(cup_ctx* ctx, int g, int dx, int dy, int dz)
@@ -550,6 +587,29 @@ int cup_set_step_state(cup_ctx* ctx, int step, double time)
   c.sc_host->time = time;
   if (int rc = check_hip(c, hipMemcpy(c.sc, c.sc_host, sizeof(DevScalars), hipMemcpyHostToDevice), "scalars"))
     return rc;
+  return CUP_OK;
+}
+
+int cup_adjoint_step(cup_ctx* ctx, double dt, double nu, const double uinf[3], int scheme)
+{
+  CUP_GUARD(ctx);
+  if (!uinf || !(dt > 0)) return set_error(c, CUP_E_INVALID, "adjoint_step: dt must be > 0");
+  if (scheme != CUP_CENTRAL_EULER)
+    return set_error(c, CUP_E_INVALID, "adjoint_step: only the central Euler scheme has an adjoint");
+  if (c.body.present) return set_error(c, CUP_E_STATE, "adjoint_step: obstacles are not supported");
+  for (int f = CUP_ADJ_U; f <= CUP_ADJ_W; ++f)
+    if (int rc = field_ready(c, f, "adjoint_step")) return rc;
+  if (int rc = launch_set_dt(c, dt)) return rc;
+  // lam* = P lam: the forward projection on the adjoint fields (P is symmetric)
+  const int saved = c.ustar;
+  c.ustar = CUP_ADJ_U;
+  int rc = launch_poisson_fused(c, false, 0.0, 0.0);
+  c.ustar = saved;
+  if (rc) return rc;
+  if (int e = launch_gradp_inplace(c, CUP_ADJ_U)) return e;
+  // lam_n = A'(u_n)^T lam* into tmp, then tmp and the adjoint fields trade buffers
+  if (int e = launch_adjoint_advdiff(c, nu, uinf)) return e;
+  for (int q = 0; q < 3; ++q) std::swap(c.f[CUP_TMPU + q], c.f[CUP_ADJ_U + q]);
   return CUP_OK;
 }
 

@@ -158,7 +158,8 @@ enum XSrc { XSRC_FIELD = 0, XSRC_DIV = 1 };
 // Real rows -> packed half spectrum. grid = (Ny/8, Nz).
 // FORCE (fused linear forcing): the same pass also sums, for the workgroup's 8 rows, the
 // DFT of the advected velocity at every canonical low mode and sum |u*|^2, written as
-// partials fpart[wg*W + c], W = 6 per mode (re/im of u, v, w) + 1.
+// partials fpart[wg*W + c], W = 6 per mode (re/im of u, v, w) + 1: each row's x-DFT at
+// kx = 0..2, times the row's y-z phase of the mode.
 // Each thread's cells lie in one row (t -> bx = t/64, iy, ix), so row sums are lane
 // shuffles over ix plus an LDS sum over the 4 waves.
 template <typename T, int N, int SRC, bool FORCE>
@@ -175,6 +176,16 @@ __global__ __launch_bounds__(NT) void k_fft_x_fwd(const T* __restrict__ src0, co
   __shared__ double rowacc[FORCE ? NT / 64 : 1][FORCE ? XROWS : 1][FORCE ? 3 * NA + 1 : 1];
   const int by = blockIdx.x, z = blockIdx.y;
   const int bz = z >> 3, iz = z & 7;
+  // FORCE: thread t < XROWS*LOW_MAX fetches, before the row loop so that the loads' latency
+  // hides behind it, the twiddles of row r = t / LOW_MAX and mode m = t % LOW_MAX
+  C2<T> pey{1, 0}, pez{1, 0};
+  if (FORCE && threadIdx.x < XROWS * LOW_MAX) {
+    const int r = threadIdx.x / LOW_MAX, m = threadIdx.x % LOW_MAX;
+    if (m < lmp->n) {
+      pey = twy[(lmp->k[m][1] * (by * BS + r)) & (g.Ny - 1)];
+      pez = twz[(lmp->k[m][2] * z) & (g.Nz - 1)];
+    }
+  }
   T fac = 0;
   if (SRC == XSRC_DIV) fac = (T)(0.5 * g.h * g.h / sc->dt);
   T* rb = (T*)buf;
@@ -237,28 +248,35 @@ __global__ __launch_bounds__(NT) void k_fft_x_fwd(const T* __restrict__ src0, co
   }
   __syncthreads();
   if (FORCE) {
+    // rows' sums over the 4 waves (in wave order), the (row, mode) phases e^{-i(ky y + kz z)},
+    // then the workgroup's partial of every mode: column c = 6m + 2q + {re, im} (mode m,
+    // component q) and c = W-1 for sum |u|^2, written to fpart[wg*W + c]
     __shared__ double rowsum[XROWS][3 * NA + 1];
+    __shared__ double ph[XROWS][LOW_MAX][2];
     for (int t = threadIdx.x; t < XROWS * (3 * NA + 1); t += NT) {
       const int r = t / (3 * NA + 1), q = t % (3 * NA + 1);
       rowsum[r][q] = rowacc[0][r][q] + rowacc[1][r][q] + rowacc[2][r][q] + rowacc[3][r][q];
+    }
+    if (threadIdx.x < XROWS * LOW_MAX) {
+      const int r = threadIdx.x / LOW_MAX, m = threadIdx.x % LOW_MAX;
+      ph[r][m][0] = (double)pey.x * pez.x - (double)pey.y * pez.y;
+      ph[r][m][1] = (double)pey.x * pez.y + (double)pey.y * pez.x;
     }
     __syncthreads();
     const LowModes& lm = *lmp;
     const int wg = blockIdx.x + gridDim.x * blockIdx.y, W = 6 * lm.n + 1;
     for (int t = threadIdx.x; t < W; t += NT) {
       double outv = 0;
-      if (t == 6 * lm.n) {
+      if (t == W - 1) {
         for (int r = 0; r < XROWS; ++r) outv += rowsum[r][3 * NA];
       } else {
         const int m = t / 6, q = (t % 6) >> 1, im = t & 1;
-        const int kx = lm.k[m][0], ky = lm.k[m][1], kz = lm.k[m][2];
-        const int ak = kx < 0 ? -kx : kx;
-        const C2<T> ez = twz[(kz * z) & (g.Nz - 1)];
+        const int kx = lm.k[m][0], ak = kx < 0 ? -kx : kx;
+        const int cr = ak == 0 ? q * NA : q * NA + 2 * ak - 1;
+        const double sg = kx < 0 ? -1.0 : 1.0;
         for (int r = 0; r < XROWS; ++r) {
-          const double Ur = ak == 0 ? rowsum[r][q * NA] : rowsum[r][q * NA + 2 * ak - 1];
-          const double Ui = ak == 0 ? 0.0 : (kx < 0 ? -rowsum[r][q * NA + 2 * ak] : rowsum[r][q * NA + 2 * ak]);
-          const C2<T> ey = twy[(ky * (by * BS + r)) & (g.Ny - 1)];
-          const double er = (double)ey.x * ez.x - (double)ey.y * ez.y, ei = (double)ey.x * ez.y + (double)ey.y * ez.x;
+          const double Ur = rowsum[r][cr], Ui = ak == 0 ? 0.0 : sg * rowsum[r][cr + 1];
+          const double er = ph[r][m][0], ei = ph[r][m][1];
           outv += im ? (Ur * ei + Ui * er) : (Ur * er - Ui * ei);
         }
       }
@@ -444,8 +462,8 @@ struct XPass {
                          c.low_part);
     else
       hipLaunchKernelGGL((k_fft_x_fwd<T, N, XSRC_DIV, false>), grid, dim3(NT), 0, c.stream, s0, s1, s2,
-                         (C2<T>*)c.spec, (const C2<T>*)c.tw[0], c.grid(), c.sc, slot, (const T*)c.body.ob_rhs, c.low_dev,
-                         twy, twz, (double*)nullptr);
+                         (C2<T>*)c.spec, (const C2<T>*)c.tw[0], c.grid(), c.sc, slot, (const T*)c.body.ob_rhs,
+                         c.low_dev, twy, twz, (double*)nullptr);
   }
   static void fwd_to(Ctx& c, const T* s0, C2<T>* spec)
   {
@@ -545,14 +563,25 @@ static int poisson_fused_t(Ctx& c, bool force, double tke_target, double inj_rat
                         (const T*)c.f[c.ustar + 2], (int)XSRC_DIV, force);
   stage_mark(c, ST_XFWD, true);
   if (force) {
+    // The forcing reduction needs only the x pass's row sums and its output is first read by
+    // the z pass: it runs on the side stream beside the y pass (fork after x, join before z).
+    if (int rc = check_hip(c, hipEventRecord(c.ev_fork, c.stream), "fork")) return rc;
+    if (int rc = check_hip(c, hipStreamWaitEvent(c.side, c.ev_fork, 0), "fork")) return rc;
+    hipStream_t main = c.stream;
+    c.stream = c.side;
     stage_mark(c, ST_FREDUCE, false);
-    if (int rc = launch_forcing_reduce(c, (c.N[1] / BS) * c.N[2], tke_target, inj_rate)) return rc;
-    if (int rc = launch_forcing_rows(c)) return rc;
+    int rc = launch_forcing_reduce(c, tke_target, inj_rate);
+    if (!rc) rc = launch_forcing_rows(c);
     stage_mark(c, ST_FREDUCE, true);
+    c.stream = main;
+    if (rc) return rc;
+    if (int e = check_hip(c, hipEventRecord(c.ev_join, c.side), "join")) return e;
   }
   stage_mark(c, ST_YFWD, false);
   dispatch_n<GoY, T>(c.N[1], c, S, (int)ZM_FWD, norm);
   stage_mark(c, ST_YFWD, true);
+  if (force)
+    if (int rc = check_hip(c, hipStreamWaitEvent(c.stream, c.ev_join, 0), "join")) return rc;
   stage_mark(c, ST_ZPASS, false);
   if (force) dispatch_n<GoZF, T>(c.N[2], c, S, (int)ZM_POISSON, norm);
   else dispatch_n<GoZ, T>(c.N[2], c, S, (int)ZM_POISSON, norm);

@@ -84,7 +84,7 @@ struct Ctx {
   double h = 0, maxextent = 0, extent[3] = {0, 0, 0};
   size_t esz = 8;
 
-  void* f[8] = {};        // block-layout fields chi,u,v,w,p,tmpU,tmpV,tmpW
+  void* f[11] = {};       // block-layout fields chi,u,v,w,p,tmpU,tmpV,tmpW; adjoint u,v,w (lazy)
   int ustar = CUP_TMPU;   // first of the three fields holding the step's u* after advection
   void* spec = nullptr;   // packed half spectrum, complex, [z][y][kx], kx < Nx/2
   void* spec2 = nullptr;  // extra spectra for the plain forcing path
@@ -108,6 +108,8 @@ struct Ctx {
   double* low_coef = nullptr;  // per mode: RHS addition (re,im), synthesis of u,v,w (re,im) x3
   unsigned* low_counter = nullptr;
   void* low_rows = nullptr;    // per grid row (y, z): 16 synthesis coefficients (context precision)
+  hipStream_t side = nullptr;  // the forcing reduction's stream (default priority), beside the y pass
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   Body body;
   std::string err;
   // measurement hook: HIP events around one stage of the fused step
@@ -161,6 +163,8 @@ int launch_max_u(Ctx& c, const double uinf[3], bool prime = false);  // partials
 int launch_advdiff(Ctx& c, double nu, const double uinf[3], int scheme);  // -> tmp
 int launch_copy_tmp_to_u(Ctx& c);
 int launch_copy_u_to_tmp(Ctx& c);
+int launch_adjoint_advdiff(Ctx& c, double nu, const double uinf[3]);
+int launch_gradp_inplace(Ctx& c, int field0);
 int launch_rhs_plain(Ctx& c);                                   // -> rbuf
 int launch_gradp_plain(Ctx& c);                                 // u += -dt/2h grad p
 int launch_gradp_fused(Ctx& c, const double uinf[3], const int* skip_slot, bool force = false);  // u = tmp - dt/2h grad p; max, ke_part
@@ -183,7 +187,7 @@ int spectral_setup(Ctx& c);
 int launch_ic_hit(Ctx& c, double k0, double u_rms, uint64_t seed);
 int forcing_plain(Ctx& c, double dt, double sim_dt, double nu, double tke_target, double inj,
                   double stats[5]);
-int launch_forcing_reduce(Ctx& c, int nwg, double tke_target, double inj_rate);
+int launch_forcing_reduce(Ctx& c, double tke_target, double inj_rate);
 int launch_forcing_rows(Ctx& c);
 // cup_obstacle.hip
 int body_create(Ctx& c, const double center[3], double radius);

@@ -314,7 +314,7 @@ int forcing_plain(Ctx& c, double dt, double sim_dt, double nu, double tke_target
 }
 
 // Chunks of x-pass workgroup partials summed by the first stage of the forcing reduction.
-constexpr int FR_CHUNKS = 256;
+constexpr int FR_CHUNKS = 128;
 
 // Low modes of the linear forcing, 0 < |k|^2 <= 4 in physical wavenumbers, canonical
 // member of each conjugate pair (kz > 0, or kz = 0 and ky > 0, or kz = ky = 0 and kx > 0),
@@ -367,6 +367,10 @@ int spectral_setup(Ctx& c)
       check_hip(c, hipMalloc(&c.low_dev, sizeof(LowModes)), "hipMalloc low modes") ||
       check_hip(c, hipMalloc(&c.low_rows, (size_t)c.N[1] * c.N[2] * 16 * c.esz), "hipMalloc low rows"))
     return CUP_E_NOMEM;
+  if (check_hip(c, hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking), "hipStreamCreate side") ||
+      check_hip(c, hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming), "hipEventCreate") ||
+      check_hip(c, hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming), "hipEventCreate"))
+    return CUP_E_DEVICE;
   hipMemcpy(c.low_dev, &c.low, sizeof(LowModes), hipMemcpyHostToDevice);
   hipMemsetAsync(c.low_counter, 0, sizeof(unsigned), c.stream);
   hipMemsetAsync(c.low_coef, 0, (size_t)c.nlow * 8 * sizeof(double), c.stream);
@@ -374,13 +378,41 @@ int spectral_setup(Ctx& c)
 }
 
 // Fused forcing, stage 2: the x-pass partials (workgroup-major rows of W values) are summed
-// in FR_CHUNKS contiguous chunks of rows, one workgroup per chunk, R = NT / W rows in flight
-// so that consecutive lanes read consecutive values; the last workgroup to finish sums the
-// chunk totals in a fixed order and computes, on the device, SpectralForcing's injection and
+// in FR_CHUNKS contiguous chunks of rows, one workgroup per chunk; the last workgroup to
+// finish sums the chunk totals, computes on the device SpectralForcing's injection and
 // factor (SpectralForcing.cpp:37-77) from tke = <|u*|^2>/2 (Parseval) and tke_low = sum over
 // the low modes, and the coefficient tables: the forcing's pressure-RHS modes
 // fac*(h^2/dt)*i*sum_d sin(2 pi k_d/N_d) u_d(k) (added in the z pass) and the velocity modes
 // fac/N^3 u(k) (added in the gradient pass). red holds W totals then FR_CHUNKS*W chunk sums.
+// Rows are summed in a fixed order (deterministic). Each tile of FR_TILE rows is staged in
+// LDS by coalesced loads that are all issued before the first is used, so a chunk costs one
+// memory latency per tile rather than one per row.
+constexpr int FR_TILE = 32;
+constexpr int FR_LOADS = (FR_TILE * (6 * LOW_MAX + 1) + NT - 1) / NT;
+
+// acc (thread t < W) += sum over rows [r0, r1) of src[row * W + t].
+__device__ __forceinline__ void sum_rows_tiled(const double* src, int r0, int r1, int W, double* tile, double& acc)
+{
+  for (int r = r0; r < r1; r += FR_TILE) {
+    const int n = (r1 - r < FR_TILE ? r1 - r : FR_TILE) * W;
+    const double* p = src + (size_t)r * W;
+    double v[FR_LOADS];
+#pragma unroll
+    for (int q = 0; q < FR_LOADS; ++q) {
+      const int e = threadIdx.x + q * NT;
+      v[q] = e < n ? p[e] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < FR_LOADS; ++q) {
+      const int e = threadIdx.x + q * NT;
+      if (e < n) tile[e] = v[q];
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < W)
+      for (int k = 0; k < n / W; ++k) acc += tile[k * W + threadIdx.x];
+    __syncthreads();
+  }
+}
 
 __global__ __launch_bounds__(NT) void k_forcing_reduce(const double* __restrict__ part, int nwg, int W,
                                                        double* __restrict__ red, unsigned* __restrict__ counter,
@@ -388,82 +420,83 @@ __global__ __launch_bounds__(NT) void k_forcing_reduce(const double* __restrict_
                                                        double* __restrict__ coef, double tke_target, double inj_rate,
                                                        double h, double ncell, int N0, int N1, int N2)
 {
-  __shared__ double s[NT];
+  __shared__ double tile[FR_TILE * (6 * LOW_MAX + 1)];
+  __shared__ double fin[6 * LOW_MAX + 1];
+  __shared__ double facs[1];
   __shared__ bool last;
   const int t = threadIdx.x;
-  const int R = NT / W;  // W <= 6 * LOW_MAX + 1 < NT
-  const int g = t / W, i = t - g * W;
-  const int lo = (int)((long long)nwg * blockIdx.x / gridDim.x);
-  const int hi = (int)((long long)nwg * (blockIdx.x + 1) / gridDim.x);
-  double a = 0;
-  if (g < R)
-    for (int j = lo + g; j < hi; j += R) a += part[(size_t)j * W + i];
-  s[t] = a;
-  __syncthreads();
   double* chunk = red + W;
-  if (t < W) {
-    double tot = 0;
-    for (int k = 0; k < R; ++k) tot += s[k * W + t];
-    chunk[(size_t)blockIdx.x * W + t] = tot;
+  {
+    const int lo = (int)((long long)nwg * blockIdx.x / gridDim.x);
+    const int hi = (int)((long long)nwg * (blockIdx.x + 1) / gridDim.x);
+    double a = 0;
+    sum_rows_tiled(part, lo, hi, W, tile, a);
+    if (t < W) chunk[(size_t)blockIdx.x * W + t] = a;
   }
   __threadfence();
   __syncthreads();
   if (t == 0) last = atomicAdd(counter, 1u) == gridDim.x - 1;
   __syncthreads();
   if (!last) return;
-  __threadfence();
-  if (t < W) {
-    const volatile double* vc = chunk;
-    double tot = 0;
-    for (unsigned k = 0; k < gridDim.x; ++k) tot += vc[(size_t)k * W + t];
-    s[t] = tot;
-    red[t] = tot;
+  __threadfence();  // acquire: the other workgroups' chunk sums are visible to plain loads
+  {
+    double a = 0;
+    sum_rows_tiled(chunk, 0, (int)gridDim.x, W, tile, a);
+    if (t < W) {
+      fin[t] = a;
+      red[t] = a;
+    }
   }
   __syncthreads();
-  if (t != 0) return;
-  *counter = 0u;
-  const double* r = s;
   const LowModes& lm = *lmp;
+  const double* r = fin;
   const double dt = sc->dt;
   const double norm = (double)N0 * N1 * N2;
-  const double tke = 0.5 * r[W - 1] / ncell;
-  double tke_low = 0;
-  for (int m = 0; m < lm.n; ++m)
-    for (int q = 0; q < 3; ++q) {
-      const double re = r[2 * (3 * m + q)], im = r[2 * (3 * m + q) + 1];
-      tke_low += re * re + im * im;
-    }
-  tke_low /= norm * norm;
-  double inj = 0;
-  if (tke_target > 0) inj = (tke_target - tke) / dt;
-  else if (inj_rate > 0) inj = inj_rate;
-  const double fac = dt * inj / (2 * tke_low);
-  sc->tke = tke;
-  sc->tke_low = tke_low;
-  sc->forcing_fac = fac;
-  for (int m = 0; m < lm.n; ++m) {
-    double* cm = coef + 8 * m;
-    if (!(fac > 0)) {
-      for (int q = 0; q < 8; ++q) cm[q] = 0;
-      continue;
-    }
-    const double s0 = sin(2 * M_PI * lm.k[m][0] / N0), s1 = sin(2 * M_PI * lm.k[m][1] / N1),
-                 s2 = sin(2 * M_PI * lm.k[m][2] / N2);
-    const double dr = s0 * r[6 * m] + s1 * r[6 * m + 2] + s2 * r[6 * m + 4];
-    const double di = s0 * r[6 * m + 1] + s1 * r[6 * m + 3] + s2 * r[6 * m + 5];
-    const double gg = fac * h * h / dt;
-    cm[0] = -gg * di;  // i * (dr + i di) = -di + i dr
-    cm[1] = gg * dr;
-    for (int q = 0; q < 3; ++q) {
-      cm[2 + 2 * q] = fac / norm * r[6 * m + 2 * q];
-      cm[3 + 2 * q] = fac / norm * r[6 * m + 2 * q + 1];
-    }
+  if (t == 0) {
+    *counter = 0u;
+    const double tke = 0.5 * r[W - 1] / ncell;
+    double tke_low = 0;
+    for (int m = 0; m < lm.n; ++m)
+      for (int q = 0; q < 3; ++q) {
+        const double re = r[2 * (3 * m + q)], im = r[2 * (3 * m + q) + 1];
+        tke_low += re * re + im * im;
+      }
+    tke_low /= norm * norm;
+    double inj = 0;
+    if (tke_target > 0) inj = (tke_target - tke) / dt;
+    else if (inj_rate > 0) inj = inj_rate;
+    const double fac = dt * inj / (2 * tke_low);
+    sc->tke = tke;
+    sc->tke_low = tke_low;
+    sc->forcing_fac = fac;
+    facs[0] = fac;
+  }
+  __syncthreads();
+  const double fac = facs[0];
+  if (t >= lm.n) return;
+  const int m = t;  // one thread per mode
+  double* cm = coef + 8 * m;
+  if (!(fac > 0)) {
+    for (int q = 0; q < 8; ++q) cm[q] = 0;
+    return;
+  }
+  const double s0 = sin(2 * M_PI * lm.k[m][0] / N0), s1 = sin(2 * M_PI * lm.k[m][1] / N1),
+               s2 = sin(2 * M_PI * lm.k[m][2] / N2);
+  const double dr = s0 * r[6 * m] + s1 * r[6 * m + 2] + s2 * r[6 * m + 4];
+  const double di = s0 * r[6 * m + 1] + s1 * r[6 * m + 3] + s2 * r[6 * m + 5];
+  const double gg = fac * h * h / dt;
+  cm[0] = -gg * di;  // i * (dr + i di) = -di + i dr
+  cm[1] = gg * dr;
+  for (int q = 0; q < 3; ++q) {
+    cm[2 + 2 * q] = fac / norm * r[6 * m + 2 * q];
+    cm[3 + 2 * q] = fac / norm * r[6 * m + 2 * q + 1];
   }
 }
 
-int launch_forcing_reduce(Ctx& c, int nwg, double tke_target, double inj_rate)
+int launch_forcing_reduce(Ctx& c, double tke_target, double inj_rate)
 {
   const int W = 6 * c.nlow + 1;
+  const int nwg = (c.N[1] / BS) * c.N[2];  // x-pass workgroups
   const int nch = nwg < FR_CHUNKS ? nwg : FR_CHUNKS;
   hipLaunchKernelGGL(k_forcing_reduce, dim3(nch), dim3(NT), 0, c.stream, (const double*)c.low_part, nwg, W, c.low_red,
                      c.low_counter, c.low_dev, c.sc, c.low_coef, tke_target, inj_rate, c.h, (double)c.ncell, c.N[0],

@@ -738,6 +738,96 @@ static int gradp_t(Ctx& c, bool fused, const double uinf[3], const int* skip, bo
   return check_hip(c, hipGetLastError(), "k_gradp");
 }
 
+// Gradient step in place on the three fields from field0 (no reductions): the adjoint
+// projection's last pass.
+int launch_gradp_inplace(Ctx& c, int field0)
+{
+  const int nbw = blocks_per_wg(c.nb);
+  const dim3 grid(c.nb / nbw);
+  if (c.precision == 4) {
+    float* f[3] = {(float*)c.f[field0], (float*)c.f[field0 + 1], (float*)c.f[field0 + 2]};
+    hipLaunchKernelGGL((k_gradp<float, false, false>), grid, dim3(NT), 0, c.stream, (const float*)c.f[CUP_P], f[0],
+                       f[1], f[2], f[0], f[1], f[2], c.grid(), c.sc, 0.0, 0.0, 0.0, (double*)nullptr,
+                       (const int*)nullptr, (const float*)nullptr, (const C2<float>*)c.tw[0], nbw);
+  } else {
+    double* f[3] = {(double*)c.f[field0], (double*)c.f[field0 + 1], (double*)c.f[field0 + 2]};
+    hipLaunchKernelGGL((k_gradp<double, false, false>), grid, dim3(NT), 0, c.stream, (const double*)c.f[CUP_P],
+                       f[0], f[1], f[2], f[0], f[1], f[2], c.grid(), c.sc, 0.0, 0.0, 0.0, (double*)nullptr,
+                       (const int*)nullptr, (const double*)nullptr, (const C2<double>*)c.tw[0], nbw);
+  }
+  return check_hip(c, hipGetLastError(), "k_gradp");
+}
+
+// ------------------------------------------------------------------------ adjoint
+// Transposed linearisation of the central advection-diffusion step
+// A(u)_q = u_q + facA sum_d a_d (u_q(+e_d) - u_q(-e_d)) + facD lap u_q, a = u + U
+// (AdvectionDiffusion.cpp:106-145), applied to the adjoint lam* (no reference counterpart):
+// out_r = lam*_r + facD lap lam*_r - facA sum_d [(a_d lam*_r)(+e_d) - (a_d lam*_r)(-e_d)]
+//         + facA sum_q lam*_q (u_q(+e_r) - u_q(-e_r)).
+// One lab of six fields (u, v, w of the forward state and the three adjoint components).
+template <typename T>
+__global__ __launch_bounds__(NT) void k_adj_advdiff(const T* __restrict__ u, const T* __restrict__ v,
+                                                    const T* __restrict__ w, const T* __restrict__ lu,
+                                                    const T* __restrict__ lv, const T* __restrict__ lw,
+                                                    T* __restrict__ ou, T* __restrict__ ov, T* __restrict__ ow,
+                                                    GridArgs g, const DevScalars* __restrict__ sc, T Ux, T Uy, T Uz,
+                                                    double nu)
+{
+  __shared__ T lab[6][L1C];
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  int bx, by, bz;
+  block_coords(g, b, bx, by, bz);
+  const T* src[6] = {u, v, w, lu, lv, lw};
+  gather_g1<T, 6>(src, lab, g, b, bx, by, bz);
+  __syncthreads();
+  const double dt = sc->dt;
+  const T facA = (T)(-dt / (2 * g.h));
+  const T facD = (T)((nu / g.h) * (dt / g.h));
+  const T Uc[3] = {Ux, Uy, Uz};
+  constexpr int st[3] = {1, L1, L1 * L1};
+  const int c0 = 2 * threadIdx.x, ix0 = c0 & 7, iy = (c0 >> 3) & 7, iz = c0 >> 6;
+  T r[3][2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int l = ((iz + 1) * L1 + iy + 1) * L1 + ix0 + k + 1;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const T* Lq = lab[3 + q];
+      const T lc = Lq[l];
+      T lap = 0, adv = 0, srcs = 0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const T* A = lab[d];
+        lap += Lq[l - st[d]] + Lq[l + st[d]] - 2 * lc;
+        adv += (A[l + st[d]] + Uc[d]) * Lq[l + st[d]] - (A[l - st[d]] + Uc[d]) * Lq[l - st[d]];
+      }
+#pragma unroll
+      for (int p = 0; p < 3; ++p) srcs += lab[3 + p][l] * (lab[p][l + st[q]] - lab[p][l - st[q]]);
+      r[q][k] = lc + facD * lap - facA * adv + facA * srcs;
+    }
+  }
+  using V2 = typename Vec2<T>::type;
+  const size_t o = (size_t)b * BS3 + c0;
+  *reinterpret_cast<V2*>(ou + o) = V2{r[0][0], r[0][1]};
+  *reinterpret_cast<V2*>(ov + o) = V2{r[1][0], r[1][1]};
+  *reinterpret_cast<V2*>(ow + o) = V2{r[2][0], r[2][1]};
+}
+
+template <typename T>
+static int adj_advdiff_t(Ctx& c, double nu, const double uinf[3])
+{
+  hipLaunchKernelGGL((k_adj_advdiff<T>), dim3(c.nb), dim3(NT), 0, c.stream, (const T*)c.f[CUP_U],
+                     (const T*)c.f[CUP_V], (const T*)c.f[CUP_W], (const T*)c.f[CUP_ADJ_U], (const T*)c.f[CUP_ADJ_V],
+                     (const T*)c.f[CUP_ADJ_W], (T*)c.f[CUP_TMPU], (T*)c.f[CUP_TMPV], (T*)c.f[CUP_TMPW], c.grid(),
+                     c.sc, (T)uinf[0], (T)uinf[1], (T)uinf[2], nu);
+  return check_hip(c, hipGetLastError(), "k_adj_advdiff");
+}
+
+int launch_adjoint_advdiff(Ctx& c, double nu, const double uinf[3])
+{
+  return c.precision == 4 ? adj_advdiff_t<float>(c, nu, uinf) : adj_advdiff_t<double>(c, nu, uinf);
+}
+
 int launch_gradp_plain(Ctx& c)
 {
   return c.precision == 4 ? gradp_t<float>(c, false, nullptr, nullptr, false)

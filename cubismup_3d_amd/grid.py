@@ -47,9 +47,10 @@ class Grid:
         h = C.c_void_p()
         rc = L.lib().cup_create(C.byref(d), C.byref(h))
         if rc != L.CUP_OK:
+            msg = (L.lib().cup_last_error(None) or b"").decode()
             if rc == L.CUP_E_INVALID:
-                raise ValueError(f"invalid grid {bpd} / precision {precision}")
-            raise L.CupError(rc, "cup_create failed")
+                raise ValueError(f"invalid grid {bpd} / precision {precision}: {msg}")
+            raise L.CupError(rc, msg or "cup_create failed")
         self._h = h
         cells = (C.c_int * 3)()
         hh = C.c_double()
@@ -100,6 +101,54 @@ class Grid:
 
     def synchronize(self):
         self._c(L.lib().cup_synchronize(self._h))
+
+    # ---------------------------------------------------- torch device tensors
+    # A field as a torch tensor [Z][Y][X] on the GPU in the context's precision. Copies are
+    # device to device; both streams are synchronized around them, so the tensors may be
+    # used on torch's current stream right away.
+    def _torch_dtype(self):
+        import torch
+        return torch.float32 if self.precision == 4 else torch.float64
+
+    def set_torch(self, name: str, t):
+        import torch
+        if tuple(t.shape) != self.shape or t.dtype != self._torch_dtype() or t.device.type != "cuda":
+            raise ValueError(f"expected a cuda {self._torch_dtype()} tensor of shape {self.shape}")
+        Z, Y, X = self.shape
+        blk = t.detach().reshape(Z // BS, BS, Y // BS, BS, X // BS, BS).permute(0, 2, 4, 1, 3, 5).contiguous()
+        torch.cuda.current_stream().synchronize()
+        self._c(L.lib().cup_set_field_device(self._h, L.FIELD_IDS[name], C.c_void_p(blk.data_ptr())))
+        self.synchronize()
+
+    def get_torch(self, name: str):
+        import torch
+        Z, Y, X = self.shape
+        blk = torch.empty((Z // BS, Y // BS, X // BS, BS, BS, BS), dtype=self._torch_dtype(), device="cuda")
+        torch.cuda.current_stream().synchronize()
+        self._c(L.lib().cup_get_field_device(self._h, L.FIELD_IDS[name], C.c_void_p(blk.data_ptr())))
+        self.synchronize()
+        return blk.permute(0, 3, 1, 4, 2, 5).reshape(Z, Y, X)
+
+    def set_torch_velocity(self, u):
+        """u: [3][Z][Y][X] tensor -> (u, v, w)."""
+        for q, n in enumerate("uvw"):
+            self.set_torch(n, u[q])
+
+    def get_torch_velocity(self):
+        import torch
+        return torch.stack([self.get_torch(n) for n in "uvw"])
+
+    def set_torch_adjoint(self, lam):
+        for q, n in enumerate(("adjU", "adjV", "adjW")):
+            self.set_torch(n, lam[q])
+
+    def get_torch_adjoint(self):
+        import torch
+        return torch.stack([self.get_torch(n) for n in ("adjU", "adjV", "adjW")])
+
+    def adjoint_step(self, dt, nu, uinf=(0, 0, 0), scheme=L.CUP_CENTRAL_EULER):
+        """adj <- dJ/du_n from adj = dJ/du_{n+1}, with u_n the current velocity."""
+        self._c(L.lib().cup_adjoint_step(self._h, float(dt), float(nu), L.vec3(uinf), int(scheme)))
 
     def stream(self) -> int:
         s = C.c_void_p()

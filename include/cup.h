@@ -29,9 +29,11 @@ extern "C" {
 #define CUP_E_STATE -3     /* call out of order (e.g. no obstacle created) */
 #define CUP_E_NOMEM -4
 
-/* Field ids, same order as the reference's FluidElement (Definitions.h:39). */
+/* Field ids, same order as the reference's FluidElement (Definitions.h:39), then the
+ * adjoint velocity of cup_adjoint_step (no reference counterpart; allocated on first use). */
 enum cup_field {
-  CUP_CHI = 0, CUP_U = 1, CUP_V = 2, CUP_W = 3, CUP_P = 4, CUP_TMPU = 5, CUP_TMPV = 6, CUP_TMPW = 7
+  CUP_CHI = 0, CUP_U = 1, CUP_V = 2, CUP_W = 3, CUP_P = 4, CUP_TMPU = 5, CUP_TMPV = 6, CUP_TMPW = 7,
+  CUP_ADJ_U = 8, CUP_ADJ_V = 9, CUP_ADJ_W = 10
 };
 
 /* Decomposition of one step. PLAIN follows the reference's operator decomposition
@@ -77,6 +79,11 @@ int cup_set_field(cup_ctx* ctx, int field, const double* host);
 int cup_get_field(cup_ctx* ctx, int field, double* host);
 /* Raw device pointer of a field (context precision), for zero-copy callers. */
 int cup_field_device_ptr(cup_ctx* ctx, int field, void** dptr);
+/* Device-to-device copies of one field in the context's precision and block-major order,
+ * stream-ordered on the context's stream (the caller orders its own stream before, and
+ * calls cup_synchronize before reading dst elsewhere). */
+int cup_set_field_device(cup_ctx* ctx, int field, const void* src);
+int cup_get_field_device(cup_ctx* ctx, int field, void* dst);
 
 /* TestBoundaries.cpp:19-75 restated on the device: tmpU = lab(-dx,-dy,-dz).u through the
  * LDS-staged periodic halo gather with halo width g (1 or 2). */
@@ -171,6 +178,16 @@ int cup_run_steps(cup_ctx* ctx, const cup_step_params* prm, int nsteps);
 /* step, time, dt of the last step, umax measured after it, tke after it. */
 int cup_step_state(cup_ctx* ctx, int* step, double* time, double* last_dt, double* umax, double* tke);
 int cup_set_step_state(cup_ctx* ctx, int step, double time);
+
+/* Adjoint of one step (no reference counterpart): with the forward state u_n in
+ * (CUP_U, CUP_V, CUP_W) and dJ/du_{n+1} in (CUP_ADJ_U, _V, _W), replaces the adjoint
+ * fields by dJ/du_n for the step u_{n+1} = P(A(u_n)) of the given dt: central advection
+ * with Euler (the only scheme accepted), no forcing, no obstacle (CUP_E_STATE if one
+ * exists). P (PressureRHS + PoissonSolverPeriodic + KernelGradP) is symmetric and is
+ * applied to the adjoint; A'(u_n)^T is the transposed linearised advection-diffusion
+ * stencil. Overwrites p and tmp; the device pointers of tmp and the adjoint fields may
+ * change (cup_field_device_ptr again after this call). */
+int cup_adjoint_step(cup_ctx* ctx, double dt, double nu, const double uinf[3], int scheme);
 
 /* Measurement: the stream the context launches on, and HIP events recorded on it around
  * one stage of the FUSED step (0 advection-diffusion, 1 x-forward FFT with the RHS,

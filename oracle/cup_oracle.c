@@ -1223,6 +1223,78 @@ void orc_surface_forces(const orc_sim* s, double nu, const double uTrans[3], con
   for (int i = 0; i < 22; i++) out[i] = Q[i];
 }
 
+/* ------------------------------------------------------------------ adjoint */
+
+/* Adjoint (vector-Jacobian product) of one step u_{n+1} = P(A(u_n)) without forcing or
+ * body, central advection with Euler (AdvectionDiffusion.cpp:106-145, then PressureRHS,
+ * PressureProjection). No reference counterpart. The forward state u_n is the sim's
+ * (u,v,w); lam (3 x ncell, block-major) holds dJ/du_{n+1} on entry and dJ/du_n on exit.
+ *  - P = I - G S D with central D and G = -D^T and a real even symbol S: P^T = P, so
+ *    lam* = P lam is the forward projection applied to lam (dt cancels).
+ *  - A(u)_q = u_q + facA sum_d a_d (u_q(+e_d) - u_q(-e_d)) + facD lap u_q, a = u + U:
+ *    A'(u)^T lam*_r = lam*_r + facD lap lam*_r - facA sum_d [(a_d lam*_r)(+e_d) - (a_d lam*_r)(-e_d)]
+ *                     + facA sum_q lam*_q (u_q(+e_r) - u_q(-e_r)). */
+void orc_adjoint_step(orc_sim* s, double dt, double nu, const double uinf[3], double* lam)
+{
+  const size_t nc = (size_t)s->nb * BS * BS * BS;
+  double* save = (double*)malloc(4 * nc * sizeof(double));
+  const int keep[4] = {ORC_U, ORC_V, ORC_W, ORC_P};
+  for (int q = 0; q < 4; q++) orc_get_field(s, keep[q], save + q * nc);
+  for (int q = 0; q < 3; q++) orc_set_field(s, ORC_U + q, lam + q * nc);
+  const int sphere = s->has_sphere;
+  s->has_sphere = 0;
+  orc_pressure_rhs(s, dt);
+  orc_poisson_solve(s);
+  orc_grad_p(s, dt);
+  s->has_sphere = sphere;
+  for (int q = 0; q < 3; q++) orc_get_field(s, ORC_U + q, lam + q * nc);  /* lam* */
+  for (int q = 0; q < 4; q++) orc_set_field(s, keep[q], save + q * nc);
+  free(save);
+  /* lam* into tmp (so that a lab carries both u and lam*), then the transposed operator */
+  #pragma omp parallel for schedule(static)
+  for (int b = 0; b < s->nb; b++) {
+    FE* e = &s->blk[b].d[0][0][0];
+    for (int i = 0; i < BS * BS * BS; i++) {
+      const size_t c = (size_t)b * BS * BS * BS + i;
+      e[i].tmpU = lam[c];
+      e[i].tmpV = lam[nc + c];
+      e[i].tmpW = lam[2 * nc + c];
+    }
+  }
+  const double h = s->h;
+  const double facA = -dt / (2 * h);
+  const double facD = (nu / h) * (dt / h);
+  #pragma omp parallel
+  {
+    Lab l = lab_alloc(1);
+    #pragma omp for schedule(static)
+    for (int b = 0; b < s->nb; b++) {
+      lab_load(s, b, &l);
+      for (int iz = 0; iz < BS; iz++)
+        for (int iy = 0; iy < BS; iy++)
+          for (int ix = 0; ix < BS; ix++) {
+            const FE* C = LAB(&l, ix, iy, iz);
+            const FE* nbr[3][2] = {{LAB(&l, ix - 1, iy, iz), LAB(&l, ix + 1, iy, iz)},
+                                   {LAB(&l, ix, iy - 1, iz), LAB(&l, ix, iy + 1, iz)},
+                                   {LAB(&l, ix, iy, iz - 1), LAB(&l, ix, iy, iz + 1)}};
+            const size_t cell = (size_t)b * BS * BS * BS + (iz * BS + iy) * BS + ix;
+            for (int r = 0; r < 3; r++) {
+              const double lc = fe_comp(C, r, 1);
+              double lap = 0, adv = 0, src = 0;
+              for (int d = 0; d < 3; d++) {
+                const FE *m = nbr[d][0], *p = nbr[d][1];
+                lap += fe_comp(m, r, 1) + fe_comp(p, r, 1) - 2 * lc;
+                adv += (fe_comp(p, d, 0) + uinf[d]) * fe_comp(p, r, 1) - (fe_comp(m, d, 0) + uinf[d]) * fe_comp(m, r, 1);
+              }
+              for (int q = 0; q < 3; q++) src += fe_comp(C, q, 1) * (fe_comp(nbr[r][1], q, 0) - fe_comp(nbr[r][0], q, 0));
+              lam[r * nc + cell] = lc + facD * lap - facA * adv + facA * src;
+            }
+          }
+    }
+    free(l.c);
+  }
+}
+
 /* ---------------------------------------------------------------- timestep */
 
 /* Simulation::timestep (Simulation.cpp:593-623) over the pipeline of setupOperators

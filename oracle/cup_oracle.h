@@ -92,6 +92,11 @@ typedef struct {
 void orc_timestep(orc_sim* s, double dt, const orc_params* prm, double forcing_stats[5],
                   double penal_ft[6], double surf_qoi[22]);
 
+/* Adjoint of one step without forcing or body (central advection, Euler): the forward
+ * state u_n is the sim's velocity; lam (3 x ncell, block-major) holds dJ/du_{n+1} on entry,
+ * dJ/du_n on exit. Overwrites tmp; u, v, w, p are kept. No reference counterpart. */
+void orc_adjoint_step(orc_sim* s, double dt, double nu, const double uinf[3], double* lam);
+
 /* Plain 3-D complex FFT, unnormalized, sign -1 forward / +1 backward; arrays [x][y][z]. */
 void orc_fft3d(int nx, int ny, int nz, double* re, double* im, int sign);
 

@@ -52,6 +52,7 @@ def lib():
         "orc_calc_max_timestep": (d, [p, d, d, dp, i, i, dp]),
         "orc_advect_diffuse": (None, [p, d, d, dp]),
         "orc_advect_diffuse_scheme": (None, [p, d, d, dp, i]),
+        "orc_adjoint_step": (None, [p, d, d, dp, dp]),
         "orc_pressure_rhs": (None, [p, d]),
         "orc_poisson_solve": (None, [p]),
         "orc_grad_p": (None, [p, d]),
@@ -174,6 +175,14 @@ class OracleSim:
     def advect_diffuse(self, dt, nu, uinf=(0, 0, 0), scheme=0):
         """scheme 0: central/Euler (the reference); 1: upwind3/Euler; 2: upwind3/RK2 midpoint."""
         lib().orc_advect_diffuse_scheme(self._h, float(dt), float(nu), _dp(_vec(uinf)), int(scheme))
+
+    def adjoint_step(self, dt, nu, lam, uinf=(0, 0, 0)):
+        """dJ/du_n from dJ/du_{n+1} = lam (a tuple of three [Z][Y][X] arrays), with the forward
+        state u_n in this sim's velocity; central advection, Euler, no forcing or body."""
+        buf = np.concatenate([to_blocks(np.asarray(f, dtype=np.float64)) for f in lam])
+        lib().orc_adjoint_step(self._h, float(dt), float(nu), _dp(_vec(uinf)), _dp(buf))
+        n = buf.size // 3
+        return tuple(from_blocks(buf[q * n:(q + 1) * n], self.shape) for q in range(3))
 
     def pressure_rhs(self, dt):
         lib().orc_pressure_rhs(self._h, float(dt))
