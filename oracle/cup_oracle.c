@@ -421,7 +421,8 @@ void orc_pressure_rhs(orc_sim* s, double dt)
   unsigned char* touch = NULL;
   if (s->has_sphere) touch = (unsigned char*)calloc((size_t)s->N[0] * s->N[1] * s->N[2], 1);
   double sumRHS = 0, posRHS = 0, negRHS = 0;
-  #pragma omp parallel reduction(+ : sumRHS, posRHS, negRHS)
+  double* part = (double*)calloc(3 * (size_t)s->nb, sizeof(double));  /* per block, summed in block order */
+  #pragma omp parallel
   {
     Lab l = lab_alloc(1);
     #pragma omp for schedule(static)
@@ -440,6 +441,7 @@ void orc_pressure_rhs(orc_sim* s, double dt)
           }
       if (!s->has_sphere || s->ob[b] == NULL) continue;
       const OBlock* o = s->ob[b];
+      double sumRHS = 0, posRHS = 0, negRHS = 0;
       for (int iz = 0; iz < BS; iz++)
         for (int iy = 0; iy < BS; iy++)
           for (int ix = 0; ix < BS; ix++) {
@@ -457,9 +459,18 @@ void orc_pressure_rhs(orc_sim* s, double dt)
             posRHS += bSurface * s->rhs[idx] * (s->rhs[idx] > 0);
             negRHS -= bSurface * s->rhs[idx] * (s->rhs[idx] < 0);
           }
+      part[3 * b] = sumRHS;
+      part[3 * b + 1] = posRHS;
+      part[3 * b + 2] = negRHS;
     }
     free(l.c);
   }
+  for (int b = 0; b < s->nb; b++) {
+    sumRHS += part[3 * b];
+    posRHS += part[3 * b + 1];
+    negRHS += part[3 * b + 2];
+  }
+  free(part);
   if (!s->has_sphere) return;
   const double corrDenom = sumRHS > 0 ? posRHS : negRHS;
   const double corr = sumRHS / (corrDenom > EPS ? corrDenom : EPS);
@@ -660,7 +671,8 @@ void orc_dissipation(const orc_sim* s, double nu, double out[20])
   double R[20] = {0};
   const double h = s->h, hCube = h * h * h, inv2h = .5 / h, invHh = 1 / (h * h);
   const double center[3] = {s->extent[0] / 2, s->extent[1] / 2, s->extent[2] / 2};
-  #pragma omp parallel reduction(+ : R[:20])
+  double* part = (double*)calloc(20 * (size_t)s->nb, sizeof(double));  /* per block, summed in block order */
+  #pragma omp parallel
   {
     Lab l = lab_alloc(1);
     #pragma omp for schedule(static)
@@ -668,6 +680,7 @@ void orc_dissipation(const orc_sim* s, double nu, double out[20])
       lab_load(s, b, &l);
       int bc[3];
       bcoords(s, b, bc);
+      double R[20] = {0};
       for (int iz = 0; iz < BS; iz++)
         for (int iy = 0; iy < BS; iy++)
           for (int ix = 0; ix < BS; ix++) {
@@ -717,9 +730,13 @@ void orc_dissipation(const orc_sim* s, double nu, double out[20])
             R[18] += hCube * (L->u * L->u + L->v * L->v + L->w * L->w) / 2;
             R[19] += sqrt(WX * WX + WY * WY + WZ * WZ);
           }
+      for (int i = 0; i < 20; i++) part[20 * (size_t)b + i] = R[i];
     }
     free(l.c);
   }
+  for (int b = 0; b < s->nb; b++)
+    for (int i = 0; i < 20; i++) R[i] += part[20 * (size_t)b + i];
+  free(part);
   for (int i = 0; i < 20; i++) out[i] = R[i];
 }
 
@@ -761,8 +778,10 @@ void orc_spectral_forcing(orc_sim* s, double dt, double sim_dt, double nu,
   const double wfx = 2.0 * M_PI / s->extent[0], wfy = 2.0 * M_PI / s->extent[1], wfz = 2.0 * M_PI / s->extent[2];
   const double dfx = 2 * M_PI / N0, dfy = 2 * M_PI / N1, dfz = 2 * M_PI / N2;
   double eps = 0, tke = 0, tkeF = 0, lInt = 0;
-  #pragma omp parallel for schedule(static) reduction(+ : eps, tke, tkeF, lInt)
-  for (int i = 0; i < N0; i++)
+  double* part = (double*)calloc(4 * (size_t)N0, sizeof(double));  /* per x plane, summed in plane order */
+  #pragma omp parallel for schedule(static)
+  for (int i = 0; i < N0; i++) {
+    double eps = 0, tke = 0, tkeF = 0, lInt = 0;
     for (int l = 0; l < N1; l++)
       for (int k = 0; k < N2; k++) {
         const size_t id = ((size_t)i * N1 + l) * N2 + k;
@@ -782,6 +801,12 @@ void orc_spectral_forcing(orc_sim* s, double dt, double sim_dt, double nu,
         if (k2 > 0 && k2 <= 4) tkeF += E;
         else cu[id] = cv[id] = cw[id] = 0;
       }
+    part[4 * i] = eps; part[4 * i + 1] = tke; part[4 * i + 2] = tkeF; part[4 * i + 3] = lInt;
+  }
+  for (int i = 0; i < N0; i++) {
+    eps += part[4 * i]; tke += part[4 * i + 1]; tkeF += part[4 * i + 2]; lInt += part[4 * i + 3];
+  }
+  free(part);
   const double normFFT = (double)N0 * N1 * N2;
   const double st_tkeF = tkeF / (normFFT * normFFT);
   const double st_tke = tke / (normFFT * normFFT);
@@ -918,11 +943,18 @@ void orc_ic_hit(orc_sim* s, double k0, double u_rms, uint64_t seed)
   fft3d_c(cv, N0, N1, N2, +1);
   fft3d_c(cw, N0, N1, N2, +1);
   double ssq = 0;
-  #pragma omp parallel for schedule(static) reduction(+ : ssq)
-  for (size_t i = 0; i < n; i++) {
-    const double a = creal(cu[i]), b = creal(cv[i]), c = creal(cw[i]);
-    ssq += a * a + b * b + c * c;
+  double* part = (double*)calloc((size_t)N0, sizeof(double));  /* per x plane, summed in plane order */
+  #pragma omp parallel for schedule(static)
+  for (int i = 0; i < N0; i++) {
+    double a2 = 0;
+    for (size_t j = (size_t)i * N1 * N2; j < (size_t)(i + 1) * N1 * N2; j++) {
+      const double a = creal(cu[j]), b = creal(cv[j]), c = creal(cw[j]);
+      a2 += a * a + b * b + c * c;
+    }
+    part[i] = a2;
   }
+  for (int i = 0; i < N0; i++) ssq += part[i];
+  free(part);
   const double scale = u_rms / sqrt(ssq / (3.0 * n));
   #pragma omp parallel for schedule(static)
   for (int b = 0; b < s->nb; b++) {
@@ -1113,12 +1145,14 @@ void orc_penalize(orc_sim* s, double dt, double lambda, int implicit,
   if (!s->has_sphere) { for (int i = 0; i < 6; i++) ft_out[i] = 0; return; }
   const double dv = pow(s->h, 3);
   const double lambdaFac = implicit ? lambda : 1.0 / dt;
-  #pragma omp parallel for schedule(static) reduction(+ : M[:6])
+  double* part = (double*)calloc(6 * (size_t)s->nb, sizeof(double));  /* per block, summed in block order */
+  #pragma omp parallel for schedule(static)
   for (int b = 0; b < s->nb; b++) {
     const OBlock* o = s->ob[b];
     if (!o) continue;
     int bc[3];
     bcoords(s, b, bc);
+    double M[6] = {0};
     for (int iz = 0; iz < BS; iz++)
       for (int iy = 0; iy < BS; iy++)
         for (int ix = 0; ix < BS; ix++) {
@@ -1144,7 +1178,11 @@ void orc_penalize(orc_sim* s, double dt, double lambda, int implicit,
           M[4] += dv * (p[2] * FPX - p[0] * FPZ);
           M[5] += dv * (p[0] * FPY - p[1] * FPX);
         }
+    for (int i = 0; i < 6; i++) part[6 * (size_t)b + i] = M[i];
   }
+  for (int b = 0; b < s->nb; b++)
+    for (int i = 0; i < 6; i++) M[i] += part[6 * (size_t)b + i];
+  free(part);
   for (int i = 0; i < 6; i++) ft_out[i] = M[i];
 }
 
@@ -1161,7 +1199,8 @@ void orc_surface_forces(const orc_sim* s, double nu, const double uTrans[3], con
     for (int d = 0; d < 3; d++) velUnit[d] = uTrans[d] / vel_norm;
   const double _1oH = nu / s->h;
   const double* CM = s->CoM;
-  #pragma omp parallel reduction(+ : Q[:22])
+  double* part = (double*)calloc(22 * (size_t)s->nb, sizeof(double));  /* per block, summed in block order */
+  #pragma omp parallel
   {
     Lab l = lab_alloc(1);
     #pragma omp for schedule(static)
@@ -1171,6 +1210,7 @@ void orc_surface_forces(const orc_sim* s, double nu, const double uTrans[3], con
       lab_load(s, b, &l);
       int bc[3];
       bcoords(s, b, bc);
+      double Q[22] = {0};
       for (int i = 0; i < o->npts; i++) {
         const int ix = o->pix[i] % BS, iy = (o->pix[i] / BS) % BS, iz = o->pix[i] / (BS * BS);
         double p[3];
@@ -1217,9 +1257,13 @@ void orc_surface_forces(const orc_sim* s, double nu, const double uTrans[3], con
                               uTrans[2] + omega[0] * rV[1] - rV[0] * omega[1]};
         Q[21] += fXT * uS[0] + fYT * uS[1] + fZT * uS[2];
       }
+      for (int q = 0; q < 22; q++) part[22 * (size_t)b + q] = Q[q];
     }
     free(l.c);
   }
+  for (int b = 0; b < s->nb; b++)
+    for (int q = 0; q < 22; q++) Q[q] += part[22 * (size_t)b + q];
+  free(part);
   for (int i = 0; i < 22; i++) out[i] = Q[i];
 }
 
