@@ -257,7 +257,7 @@ __global__ __launch_bounds__(NT) void k_rhs_obs_mask(const int* __restrict__ ob_
     }
     const double X = ob_chi[(size_t)s * BS3 + c];
     unsigned char t = 0;
-    if (!((double)chi[o] > X)) {
+    if (!(chi[o] > (T)X)) {  // in the grid's precision: the grid chi holds the body's chi rounded to T
       const double dXx = (double)chi[cell_index(g, wrapi(gx + 1, g.Nx), gy, gz)] - chi[cell_index(g, wrapi(gx - 1, g.Nx), gy, gz)];
       const double dXy = (double)chi[cell_index(g, gx, wrapi(gy + 1, g.Ny), gz)] - chi[cell_index(g, gx, wrapi(gy - 1, g.Ny), gz)];
       const double dXz = (double)chi[cell_index(g, gx, gy, wrapi(gz + 1, g.Nz))] - chi[cell_index(g, gx, gy, wrapi(gz - 1, g.Nz))];
@@ -356,7 +356,7 @@ __global__ __launch_bounds__(NT) void k_penalize(const int* __restrict__ ob_ids,
     const size_t o = (size_t)b * BS3 + c;
     const double X = ob_chi[(size_t)s * BS3 + c];
     double uu = u[o], vv = v[o], ww = w[o];
-    if (!((double)chi[o] > X) && X > 0) {
+    if (!(chi[o] > (T)X) && X > 0) {  // compared in the grid's precision, as the grid chi is stored
       const double px = (bx * BS + ix + 0.5) * g.h - cmx, py = (by * BS + iy + 0.5) * g.h - cmy,
                    pz = (bz * BS + iz + 0.5) * g.h - cmz;
       const double UT0 = vx + oy * pz - oz * py, UT1 = vy + oz * px - ox * pz, UT2 = vz + ox * py - oy * px;

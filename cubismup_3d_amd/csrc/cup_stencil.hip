@@ -75,6 +75,13 @@ __device__ __forceinline__ void block_coords(const GridArgs& g, int b, int& bx, 
 // twice the bytes in flight per CU of a load-sync-compute workgroup.
 // A thread's cells: interior c = 2t, 2t+1; halo (faces only, 6*64 cells) c = t (faces
 // x-, x+, y-, y+: one face per wave) and, for t < 128, c = NT + t (faces z-, z+).
+// The pipelined lab pads x to PX = 12 (interior at x = 2..9, halo at 1 and 10) so that a
+// thread's x-adjacent cell pair, and its y and z neighbour pairs, are 8-byte aligned: each is
+// one 64-bit LDS access.
+constexpr int PX = 12;
+constexpr int PY = PX;            // y stride
+constexpr int PZ = PX * L1;       // z stride
+constexpr int PLC = PX * L1 * L1;
 struct LabSlots {
   int li;       // LDS index of interior cell 2t
   int l[2];     // LDS index of the halo cells
@@ -88,7 +95,7 @@ __device__ __forceinline__ LabSlots lab_slots()
 {
   LabSlots s;
   const int t = threadIdx.x, c0 = 2 * t;
-  s.li = (((c0 >> 6) + 1) * L1 + ((c0 >> 3) & 7) + 1) * L1 + (c0 & 7) + 1;
+  s.li = (((c0 >> 6) + 1) * L1 + ((c0 >> 3) & 7) + 1) * PX + (c0 & 7) + 2;
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
     const int c = k == 0 ? t : (has_halo2() ? NT + t : 4 * 64);
@@ -104,7 +111,7 @@ __device__ __forceinline__ LabSlots lab_slots()
     }
     s.face[k] = face;
     s.off[k] = (sz * BS + sy) * BS + sx;
-    s.l[k] = (lz * L1 + ly) * L1 + lx;
+    s.l[k] = (lz * L1 + ly) * PX + lx + 1;
   }
   return s;
 }
@@ -148,13 +155,11 @@ __device__ __forceinline__ void lab_fetch(LabRegs<T, NF>& r, const T* const* src
 }
 
 template <typename T, int NF>
-__device__ __forceinline__ void lab_put(const LabRegs<T, NF>& r, T (*lab)[L1C], const LabSlots& s)
+__device__ __forceinline__ void lab_put(const LabRegs<T, NF>& r, T (*lab)[PLC], const LabSlots& s)
 {
+  using V2 = typename Vec2<T>::type;
 #pragma unroll
-  for (int f = 0; f < NF; ++f) {
-    lab[f][s.li] = r.in[f][0];
-    lab[f][s.li + 1] = r.in[f][1];
-  }
+  for (int f = 0; f < NF; ++f) *reinterpret_cast<V2*>(&lab[f][s.li]) = V2{r.in[f][0], r.in[f][1]};
 #pragma unroll
   for (int k = 0; k < 2; ++k)
     if (k == 0 || has_halo2())
@@ -294,7 +299,7 @@ __global__ __launch_bounds__(NT) void k_advdiff_central(const T* __restrict__ u,
                                                         const DevScalars* __restrict__ sc, T Ux, T Uy, T Uz,
                                                         double nu, int nbw)
 {
-  __shared__ T lab[3][L1C];
+  __shared__ __align__(16) T lab[3][PLC];
   const LabSlots s = lab_slots();
   const T* src[3] = {u, v, w};
   const int b0 = xcd_remap(blockIdx.x, gridDim.x) * nbw;
@@ -311,27 +316,39 @@ __global__ __launch_bounds__(NT) void k_advdiff_central(const T* __restrict__ u,
     lab_put(r, lab, s);
     __syncthreads();
     if (j + 1 < nbw) lab_fetch(r, src, s, g, b + 1);
-    T o[3][2];
+    T o[3][2], fc[3][2], fe[3][2], fw[3][2], fn[3][2], fs[3][2], fb[3][2], ff[3][2];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      // the pair's values: centre c (x, x+1), west of x, east of x+1, and the y/z neighbour pairs
+      const T* F = lab[q];
+      const int l = s.li;
+      const V2 C = *reinterpret_cast<const V2*>(F + l);
+      const V2 Nn = *reinterpret_cast<const V2*>(F + l + PY), Sn = *reinterpret_cast<const V2*>(F + l - PY);
+      const V2 Bn = *reinterpret_cast<const V2*>(F + l + PZ), Fn = *reinterpret_cast<const V2*>(F + l - PZ);
+      const T Wx = F[l - 1], Ex = F[l + 2];
+      fc[q][0] = C.x; fc[q][1] = C.y;
+      fe[q][0] = C.y; fe[q][1] = Ex;
+      fw[q][0] = Wx;  fw[q][1] = C.x;
+      fn[q][0] = Nn.x; fn[q][1] = Nn.y;
+      fs[q][0] = Sn.x; fs[q][1] = Sn.y;
+      fb[q][0] = Bn.x; fb[q][1] = Bn.y;
+      ff[q][0] = Fn.x; ff[q][1] = Fn.y;
+    }
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
-      const int l = s.li + k;
-      const T* U = lab[0];
-      const T* V = lab[1];
-      const T* W = lab[2];
-      const T dudx = U[l + 1] - U[l - 1], dvdx = V[l + 1] - V[l - 1], dwdx = W[l + 1] - W[l - 1];
-      const T dudy = U[l + L1] - U[l - L1], dvdy = V[l + L1] - V[l - L1], dwdy = W[l + L1] - W[l - L1];
-      const T dudz = U[l + L1 * L1] - U[l - L1 * L1], dvdz = V[l + L1 * L1] - V[l - L1 * L1],
-              dwdz = W[l + L1 * L1] - W[l - L1 * L1];
-      const T uu = U[l] + Ux, vv = V[l] + Uy, ww = W[l] + Uz;
-      const T duD = U[l + L1] + U[l - L1] + U[l + 1] + U[l - 1] + U[l - L1 * L1] + U[l + L1 * L1] - U[l] * 6;
-      const T dvD = V[l + L1] + V[l - L1] + V[l + 1] + V[l - 1] + V[l - L1 * L1] + V[l + L1 * L1] - V[l] * 6;
-      const T dwD = W[l + L1] + W[l - L1] + W[l + 1] + W[l - 1] + W[l - L1 * L1] + W[l + L1 * L1] - W[l] * 6;
+      const T dudx = fe[0][k] - fw[0][k], dvdx = fe[1][k] - fw[1][k], dwdx = fe[2][k] - fw[2][k];
+      const T dudy = fn[0][k] - fs[0][k], dvdy = fn[1][k] - fs[1][k], dwdy = fn[2][k] - fs[2][k];
+      const T dudz = fb[0][k] - ff[0][k], dvdz = fb[1][k] - ff[1][k], dwdz = fb[2][k] - ff[2][k];
+      const T uu = fc[0][k] + Ux, vv = fc[1][k] + Uy, ww = fc[2][k] + Uz;
+      const T duD = fn[0][k] + fs[0][k] + fe[0][k] + fw[0][k] + ff[0][k] + fb[0][k] - fc[0][k] * 6;
+      const T dvD = fn[1][k] + fs[1][k] + fe[1][k] + fw[1][k] + ff[1][k] + fb[1][k] - fc[1][k] * 6;
+      const T dwD = fn[2][k] + fs[2][k] + fe[2][k] + fw[2][k] + ff[2][k] + fb[2][k] - fc[2][k] * 6;
       const T duA = uu * dudx + vv * dudy + ww * dudz;
       const T dvA = uu * dvdx + vv * dvdy + ww * dvdz;
       const T dwA = uu * dwdx + vv * dwdy + ww * dwdz;
-      o[0][k] = U[l] + facA * duA + facD * duD;
-      o[1][k] = V[l] + facA * dvA + facD * dvD;
-      o[2][k] = W[l] + facA * dwA + facD * dwD;
+      o[0][k] = fc[0][k] + facA * duA + facD * duD;
+      o[1][k] = fc[1][k] + facA * dvA + facD * dvD;
+      o[2][k] = fc[2][k] + facA * dwA + facD * dwD;
     }
     const size_t q = (size_t)b * BS3 + c0;
     *reinterpret_cast<V2*>(ou + q) = V2{o[0][0], o[0][1]};
@@ -627,7 +644,7 @@ __global__ __launch_bounds__(NT) void k_gradp(const T* __restrict__ p, const T* 
                                               const T* __restrict__ rows, const C2<T>* __restrict__ twx, int nbw)
 {
   using V2 = typename Vec2<T>::type;
-  __shared__ T lab[1][L1C];
+  __shared__ __align__(16) T lab[1][PLC];
   __shared__ T rl[FORCE ? 64 * 16 : 1];
   const int t = threadIdx.x;
   const LabSlots s = lab_slots();
@@ -665,13 +682,19 @@ __global__ __launch_bounds__(NT) void k_gradp(const T* __restrict__ p, const T* 
     T nu_[2] = {us[0][0], us[0][1]}, nv_[2] = {us[1][0], us[1][1]}, nw_[2] = {us[2][0], us[2][1]};
     __syncthreads();
     if (j + 1 < nbw) fetch(b + 1);
-    const T* P = lab[0];
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-      const int l = s.li + k;
-      nu_[k] += fac * (P[l + 1] - P[l - 1]);
-      nv_[k] += fac * (P[l + L1] - P[l - L1]);
-      nw_[k] += fac * (P[l + L1 * L1] - P[l - L1 * L1]);
+    {
+      const T* P = lab[0];
+      const int l = s.li;
+      const V2 C = *reinterpret_cast<const V2*>(P + l);
+      const V2 Nn = *reinterpret_cast<const V2*>(P + l + PY), Sn = *reinterpret_cast<const V2*>(P + l - PY);
+      const V2 Bn = *reinterpret_cast<const V2*>(P + l + PZ), Fn = *reinterpret_cast<const V2*>(P + l - PZ);
+      const T Wx = P[l - 1], Ex = P[l + 2];
+      nu_[0] += fac * (C.y - Wx);
+      nu_[1] += fac * (Ex - C.x);
+      nv_[0] += fac * (Nn.x - Sn.x);
+      nv_[1] += fac * (Nn.y - Sn.y);
+      nw_[0] += fac * (Bn.x - Fn.x);
+      nw_[1] += fac * (Bn.y - Fn.y);
     }
     if (FORCE) {
       const int x0 = (b % g.nbx) * BS + ix0;
