@@ -173,6 +173,7 @@ __global__ __launch_bounds__(NT) void k_fft_x_fwd(const T* __restrict__ src0, co
 {
   __shared__ C2<T> buf[N * XFFT];
   constexpr int NA = 1 + 2 * LOW_KX;  // row DFT values per component: kx=0 (real), kx=1..: re, im
+  static_assert(LOW_KX == 2, "the row DFT below evaluates kx = 1, 2");
   __shared__ double rowacc[FORCE ? NT / 64 : 1][FORCE ? XROWS : 1][FORCE ? 3 * NA + 1 : 1];
   const int by = blockIdx.x, z = blockIdx.y;
   const int bz = z >> 3, iz = z & 7;
@@ -193,57 +194,82 @@ __global__ __launch_bounds__(NT) void k_fft_x_fwd(const T* __restrict__ src0, co
   if (FORCE)
 #pragma unroll
     for (int q = 0; q < 3 * NA + 1; ++q) acc[q] = 0;
-  for (int t = threadIdx.x; t < N * XROWS; t += NT) {
-    const int bx = t >> 6, iy = (t >> 3) & 7, ix = t & 7;
-    const int x = bx * BS + ix, y = by * BS + iy;
+  // Each thread takes quads of 4 x-adjacent cells (half a block row): q -> bx = q/16,
+  // iy = (q/2)%8, x0 = 8 bx + 4 (q%2), with 16-byte loads of u, of v at y +- 1, of w at
+  // z +- 1, and single loads of u across the quad's two x ends.
+  using V4 = typename Vec4<T>::type;
+  for (int q4 = threadIdx.x; q4 < N * XROWS / 4; q4 += NT) {
+    const int bx = q4 >> 4, iy = (q4 >> 1) & 7, half = q4 & 1, ix0 = 4 * half;
+    const int x0 = bx * BS + ix0;
     const int b = bx + g.nbx * (by + g.nby * bz);
-    const size_t o = (size_t)b * BS3 + (iz * BS + iy) * BS + ix;
-    T val;
+    const size_t o = (size_t)b * BS3 + (iz * BS + iy) * BS + ix0;
+    T val[4];
     if (SRC == XSRC_FIELD) {
-      val = src0[o];
+      const V4 a = *reinterpret_cast<const V4*>(src0 + o);
+      val[0] = a.x; val[1] = a.y; val[2] = a.z; val[3] = a.w;
     } else {
       const int slot = ob_slot ? ob_slot[b] : -1;
+      // neighbour offsets inside the block, or into the periodic neighbour block
+      const long sx = BS3, sy = (long)g.nbx * BS3, sz = (long)g.nbx * g.nby * BS3;
+      const long on = iy < 7 ? BS : (by + 1 < g.nby ? sy : sy - (long)g.nby * sy) - 7 * BS;
+      const long os = iy > 0 ? -BS : (by > 0 ? -sy : (long)(g.nby - 1) * sy) + 7 * BS;
+      const long ob = iz < 7 ? BS * BS : (bz + 1 < g.nbz ? sz : sz - (long)g.nbz * sz) - 7 * BS * BS;
+      const long of = iz > 0 ? -BS * BS : (bz > 0 ? -sz : (long)(g.nbz - 1) * sz) + 7 * BS * BS;
+      // u west of x0 and east of x0 + 3
+      const long ow = half ? -1 : (bx > 0 ? -sx : (long)(g.nbx - 1) * sx) + 7;
+      const long oe = half ? 3 + (bx + 1 < g.nbx ? sx : sx - (long)g.nbx * sx) - 7 : 4;
+      const V4 uc = *reinterpret_cast<const V4*>(src0 + o);
+      const T uu[6] = {src0[o + ow], uc.x, uc.y, uc.z, uc.w, src0[o + oe]};
       if (slot >= 0) {
-        val = ob_rhs[(size_t)slot * BS3 + (iz * BS + iy) * BS + ix];
+        const V4 r = *reinterpret_cast<const V4*>(ob_rhs + (size_t)slot * BS3 + (iz * BS + iy) * BS + ix0);
+        val[0] = r.x; val[1] = r.y; val[2] = r.z; val[3] = r.w;
       } else {
-        // neighbour offsets inside the block, or into the periodic neighbour block
-        const long sx = BS3, sy = (long)g.nbx * BS3, sz = (long)g.nbx * g.nby * BS3;
-        const long oe = ix < 7 ? 1 : (bx + 1 < g.nbx ? sx : sx - (long)g.nbx * sx) - 7;
-        const long ow = ix > 0 ? -1 : (bx > 0 ? -sx : (long)(g.nbx - 1) * sx) + 7;
-        const long on = iy < 7 ? BS : (by + 1 < g.nby ? sy : sy - (long)g.nby * sy) - 7 * BS;
-        const long os = iy > 0 ? -BS : (by > 0 ? -sy : (long)(g.nby - 1) * sy) + 7 * BS;
-        const long ob = iz < 7 ? BS * BS : (bz + 1 < g.nbz ? sz : sz - (long)g.nbz * sz) - 7 * BS * BS;
-        const long of = iz > 0 ? -BS * BS : (bz > 0 ? -sz : (long)(g.nbz - 1) * sz) + 7 * BS * BS;
-        val = fac * (src0[o + oe] - src0[o + ow] + src1[o + on] - src1[o + os] + src2[o + ob] - src2[o + of]);
+        const V4 vn = *reinterpret_cast<const V4*>(src1 + o + on), vs = *reinterpret_cast<const V4*>(src1 + o + os);
+        const V4 wb = *reinterpret_cast<const V4*>(src2 + o + ob), wf = *reinterpret_cast<const V4*>(src2 + o + of);
+        const T vN[4] = {vn.x, vn.y, vn.z, vn.w}, vS[4] = {vs.x, vs.y, vs.z, vs.w};
+        const T wB[4] = {wb.x, wb.y, wb.z, wb.w}, wF[4] = {wf.x, wf.y, wf.z, wf.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) val[k] = fac * (uu[k + 2] - uu[k] + vN[k] - vS[k] + wB[k] - wF[k]);
       }
       if (FORCE) {
-        const T cu[3] = {src0[o], src1[o], src2[o]};
-        acc[3 * NA] += cu[0] * cu[0] + cu[1] * cu[1] + cu[2] * cu[2];
+        const V4 vc = *reinterpret_cast<const V4*>(src1 + o), wc = *reinterpret_cast<const V4*>(src2 + o);
+        const T cv[4] = {vc.x, vc.y, vc.z, vc.w}, cw[4] = {wc.x, wc.y, wc.z, wc.w};
+        // e^{-2 pi i x/Nx} at x0, rotated by the table's first entries for x0 + 1..3
+        const C2<T> e0 = tw[x0 & (N - 1)];
 #pragma unroll
-        for (int q = 0; q < 3; ++q) acc[q * NA] += cu[q];
-#pragma unroll
-        for (int k = 1; k <= LOW_KX; ++k) {
-          const C2<T> e = tw[(k * x) & (N - 1)];  // e^{-2 pi i k x / Nx}
+        for (int k = 0; k < 4; ++k) {
+          C2<T> e1 = e0;
+          if (k > 0) {
+            const C2<T> r = tw[k];
+            e1 = C2<T>{e0.x * r.x - e0.y * r.y, e0.x * r.y + e0.y * r.x};
+          }
+          const C2<T> e2 = {e1.x * e1.x - e1.y * e1.y, 2 * e1.x * e1.y};  // e^{-4 pi i x/Nx}
+          const T cu[3] = {uu[k + 1], cv[k], cw[k]};
+          acc[3 * NA] += cu[0] * cu[0] + cu[1] * cu[1] + cu[2] * cu[2];
 #pragma unroll
           for (int q = 0; q < 3; ++q) {
-            acc[q * NA + 2 * k - 1] += cu[q] * e.x;
-            acc[q * NA + 2 * k] += cu[q] * e.y;
+            acc[q * NA] += cu[q];
+            acc[q * NA + 1] += cu[q] * e1.x;
+            acc[q * NA + 2] += cu[q] * e1.y;
+            acc[q * NA + 3] += cu[q] * e2.x;
+            acc[q * NA + 4] += cu[q] * e2.y;
           }
         }
       }
     }
-    rb[2 * (x * XFFT + (iy >> 1)) + (iy & 1)] = val;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) rb[2 * ((x0 + k) * XFFT + (iy >> 1)) + (iy & 1)] = val[k];
   }
   if (FORCE) {
-    // rows: lanes with the same iy differ in ix = lane & 7
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, iy = (lane >> 3) & 7;
+    // rows: lanes with the same iy = (lane >> 1) & 7 differ in bits 0 (half) and 4-5 (bx)
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, iy = (lane >> 1) & 7;
 #pragma unroll
     for (int q = 0; q < 3 * NA + 1; ++q) {
       T v = acc[q];
       v += __shfl_xor(v, 1, 64);
-      v += __shfl_xor(v, 2, 64);
-      v += __shfl_xor(v, 4, 64);
-      if ((lane & 7) == 0) rowacc[wv][iy][q] = (double)v;
+      v += __shfl_xor(v, 16, 64);
+      v += __shfl_xor(v, 32, 64);
+      if ((lane & 0x31) == 0) rowacc[wv][iy][q] = (double)v;
     }
   }
   __syncthreads();
@@ -356,18 +382,21 @@ __global__ __launch_bounds__(NT) void k_fft_col(C2<T>* __restrict__ S, const C2<
                                                 const LowModes* __restrict__ lmp, const double* __restrict__ coef)
 {
   constexpr int TK = ColTile<T, N>::TK;
-  __shared__ C2<T> buf[N * TK];
+  static_assert(TK % 2 == 0, "columns move in pairs");
+  __shared__ __align__(32) C2<T> buf[N * TK];
   const int kx0 = blockIdx.x * TK, other = blockIdx.y;
   const int nc = min(TK, ncols - kx0);
-  // element (f, n): AXIS=1: S[(other*Ny + n)*ncols + kx0 + f]; AXIS=2: S[(n*Ny + other)*ncols + kx0 + f]
-  for (int t = threadIdx.x; t < N * TK; t += NT) {
-    const int f = t % TK, n = t / TK;
+  // element (f, n): AXIS=1: S[(other*Ny + n)*ncols + kx0 + f]; AXIS=2: S[(n*Ny + other)*ncols + kx0 + f].
+  // Columns move in pairs (ncols, TK and so nc are even): one 2-element vector access each.
+  using V4 = typename Vec4<T>::type;
+  for (int t = threadIdx.x; t < N * TK / 2; t += NT) {
+    const int f = 2 * (t % (TK / 2)), n = t / (TK / 2);
+    V4 a = {0, 0, 0, 0};
     if (f < nc) {
       const size_t gi = AXIS == 1 ? ((size_t)other * Ny + n) * ncols + kx0 + f : ((size_t)n * Ny + other) * ncols + kx0 + f;
-      buf[f + n * TK] = S[gi];
-    } else {
-      buf[f + n * TK] = C2<T>{0, 0};
+      a = *reinterpret_cast<const V4*>(S + gi);
     }
+    *reinterpret_cast<V4*>(buf + f + n * TK) = a;
   }
   __syncthreads();
   if (MODE == ZM_INV) {
@@ -412,11 +441,11 @@ __global__ __launch_bounds__(NT) void k_fft_col(C2<T>* __restrict__ S, const C2<
     __syncthreads();
     lds_fft<N, TK, +1>(buf, tw);
   }
-  for (int t = threadIdx.x; t < N * TK; t += NT) {
-    const int f = t % TK, n = t / TK;
+  for (int t = threadIdx.x; t < N * TK / 2; t += NT) {
+    const int f = 2 * (t % (TK / 2)), n = t / (TK / 2);
     if (f < nc) {
       const size_t gi = AXIS == 1 ? ((size_t)other * Ny + n) * ncols + kx0 + f : ((size_t)n * Ny + other) * ncols + kx0 + f;
-      S[gi] = buf[f + n * TK];
+      *reinterpret_cast<V4*>(S + gi) = *reinterpret_cast<const V4*>(buf + f + n * TK);
     }
   }
 }
@@ -567,15 +596,17 @@ static int poisson_fused_t(Ctx& c, bool force, double tke_target, double inj_rat
     // the z pass: it runs on the side stream beside the y pass (fork after x, join before z).
     if (int rc = check_hip(c, hipEventRecord(c.ev_fork, c.stream), "fork")) return rc;
     if (int rc = check_hip(c, hipStreamWaitEvent(c.side, c.ev_fork, 0), "fork")) return rc;
+    // The row coefficients are first read by the gradient: their own join comes later.
     hipStream_t main = c.stream;
     c.stream = c.side;
     stage_mark(c, ST_FREDUCE, false);
     int rc = launch_forcing_reduce(c, tke_target, inj_rate);
-    if (!rc) rc = launch_forcing_rows(c);
     stage_mark(c, ST_FREDUCE, true);
+    if (!rc) rc = check_hip(c, hipEventRecord(c.ev_join, c.side), "join");
+    if (!rc) rc = launch_forcing_rows(c);
+    if (!rc) rc = check_hip(c, hipEventRecord(c.ev_rows, c.side), "join");
     c.stream = main;
     if (rc) return rc;
-    if (int e = check_hip(c, hipEventRecord(c.ev_join, c.side), "join")) return e;
   }
   stage_mark(c, ST_YFWD, false);
   dispatch_n<GoY, T>(c.N[1], c, S, (int)ZM_FWD, norm);
@@ -592,6 +623,8 @@ static int poisson_fused_t(Ctx& c, bool force, double tke_target, double inj_rat
   stage_mark(c, ST_XINV, false);
   dispatch_n<GoXInv, T>(c.N[0], c, (const C2<T>*)S, (T*)c.f[CUP_P]);
   stage_mark(c, ST_XINV, true);
+  if (force)
+    if (int rc = check_hip(c, hipStreamWaitEvent(c.stream, c.ev_rows, 0), "join")) return rc;
   return check_hip(c, hipGetLastError(), "poisson_fused");
 }
 

@@ -22,6 +22,18 @@ struct C2 {
   T x, y;
 };
 
+// Four-wide vector of the context precision (16- or 32-byte loads of 4 consecutive cells).
+template <typename T>
+struct Vec4;
+template <>
+struct Vec4<float> {
+  using type = float4;
+};
+template <>
+struct Vec4<double> {
+  using type = double4;
+};
+
 // Step scalars resident on the device. The fused driver never copies them to the host
 // inside a step; the host reads them on request.
 struct DevScalars {
@@ -109,7 +121,7 @@ struct Ctx {
   unsigned* low_counter = nullptr;
   void* low_rows = nullptr;    // per grid row (y, z): 16 synthesis coefficients (context precision)
   hipStream_t side = nullptr;  // the forcing reduction's stream (default priority), beside the y pass
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_rows = nullptr;
   Body body;
   std::string err;
   // measurement hook: HIP events around one stage of the fused step

@@ -369,7 +369,8 @@ int spectral_setup(Ctx& c)
     return CUP_E_NOMEM;
   if (check_hip(c, hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking), "hipStreamCreate side") ||
       check_hip(c, hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming), "hipEventCreate") ||
-      check_hip(c, hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming), "hipEventCreate"))
+      check_hip(c, hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming), "hipEventCreate") ||
+      check_hip(c, hipEventCreateWithFlags(&c.ev_rows, hipEventDisableTiming), "hipEventCreate"))
     return CUP_E_DEVICE;
   hipMemcpy(c.low_dev, &c.low, sizeof(LowModes), hipMemcpyHostToDevice);
   hipMemsetAsync(c.low_counter, 0, sizeof(unsigned), c.stream);
@@ -420,7 +421,7 @@ __global__ __launch_bounds__(NT) void k_forcing_reduce(const double* __restrict_
                                                        double* __restrict__ coef, double tke_target, double inj_rate,
                                                        double h, double ncell, int N0, int N1, int N2)
 {
-  __shared__ double tile[FR_TILE * (6 * LOW_MAX + 1)];
+  extern __shared__ double tile[];  // FR_TILE * W doubles, sized at launch
   __shared__ double fin[6 * LOW_MAX + 1];
   __shared__ double facs[1];
   __shared__ bool last;
@@ -498,7 +499,8 @@ int launch_forcing_reduce(Ctx& c, double tke_target, double inj_rate)
   const int W = 6 * c.nlow + 1;
   const int nwg = (c.N[1] / BS) * c.N[2];  // x-pass workgroups
   const int nch = nwg < FR_CHUNKS ? nwg : FR_CHUNKS;
-  hipLaunchKernelGGL(k_forcing_reduce, dim3(nch), dim3(NT), 0, c.stream, (const double*)c.low_part, nwg, W, c.low_red,
+  const size_t lds = (size_t)FR_TILE * W * sizeof(double);
+  hipLaunchKernelGGL(k_forcing_reduce, dim3(nch), dim3(NT), lds, c.stream, (const double*)c.low_part, nwg, W, c.low_red,
                      c.low_counter, c.low_dev, c.sc, c.low_coef, tke_target, inj_rate, c.h, (double)c.ncell, c.N[0],
                      c.N[1], c.N[2]);
   return check_hip(c, hipGetLastError(), "k_forcing_reduce");

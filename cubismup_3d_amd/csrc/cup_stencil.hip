@@ -70,7 +70,7 @@ __device__ __forceinline__ void block_coords(const GridArgs& g, int b, int& bx, 
 }
 
 // ------------------------------------------------------- pipelined lab staging
-// The fused step's stencil kernels walk nbw consecutive blocks per workgroup and fetch
+// The fused step's stencil kernels walk nbw blocks per workgroup (stacked in z) and fetch
 // block j+1's cells into registers while block j is computed from LDS, which keeps about
 // twice the bytes in flight per CU of a load-sync-compute workgroup.
 // A thread's cells: interior c = 2t, 2t+1; halo (faces only, 6*64 cells) c = t (faces
@@ -167,8 +167,17 @@ __device__ __forceinline__ void lab_put(const LabRegs<T, NF>& r, T (*lab)[PLC], 
       for (int f = 0; f < NF; ++f) lab[f][s.l[k]] = r.hv[f][k];
 }
 
-// Blocks per workgroup of the pipelined kernels: 4 when the block count allows it.
-inline int blocks_per_wg(int nb) { return nb % 4 == 0 ? 4 : (nb % 2 == 0 ? 2 : 1); }
+// The pipelined kernels' walk: workgroup w takes nbw blocks stacked in z,
+// b_j = bxy + nxy*(zg*nbw + j) with bxy = w % nxy, zg = w / nxy (nxy = nbx*nby), so that each
+// block after the first finds its lower z-face in the block just loaded (still in L2).
+// nbw = 4 when the blocks along z allow it.
+inline int blocks_per_wg(int nbz) { return nbz % 4 == 0 ? 4 : (nbz % 2 == 0 ? 2 : 1); }
+
+__device__ __forceinline__ int walk_first(const GridArgs& g, int w, int nbw)
+{
+  const int nxy = g.nbx * g.nby;
+  return w % nxy + nxy * (w / nxy) * nbw;
+}
 
 // Block-wide reduction of NV doubles per thread into out[0..NV) (thread 0 holds them).
 // Fixed shuffle tree and fixed LDS order: deterministic.
@@ -302,7 +311,8 @@ __global__ __launch_bounds__(NT) void k_advdiff_central(const T* __restrict__ u,
   __shared__ __align__(16) T lab[3][PLC];
   const LabSlots s = lab_slots();
   const T* src[3] = {u, v, w};
-  const int b0 = xcd_remap(blockIdx.x, gridDim.x) * nbw;
+  const int nxy = g.nbx * g.nby;
+  const int b0 = walk_first(g, xcd_remap(blockIdx.x, gridDim.x), nbw);
   LabRegs<T, 3> r;
   lab_fetch(r, src, s, g, b0);
   const double dt = sc->dt;
@@ -311,11 +321,11 @@ __global__ __launch_bounds__(NT) void k_advdiff_central(const T* __restrict__ u,
   const int c0 = 2 * threadIdx.x;
   using V2 = typename Vec2<T>::type;
   for (int j = 0; j < nbw; ++j) {
-    const int b = b0 + j;
+    const int b = b0 + j * nxy;
     if (j) __syncthreads();  // the previous block's compute is done with the lab
     lab_put(r, lab, s);
     __syncthreads();
-    if (j + 1 < nbw) lab_fetch(r, src, s, g, b + 1);
+    if (j + 1 < nbw) lab_fetch(r, src, s, g, b + nxy);
     T o[3][2], fc[3][2], fe[3][2], fw[3][2], fn[3][2], fs[3][2], fb[3][2], ff[3][2];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
@@ -360,7 +370,7 @@ __global__ __launch_bounds__(NT) void k_advdiff_central(const T* __restrict__ u,
 template <typename T>
 static int advdiff_t(Ctx& c, double nu, const double uinf[3])
 {
-  const int nbw = blocks_per_wg(c.nb);
+  const int nbw = blocks_per_wg(c.bpd[2]);
   hipLaunchKernelGGL((k_advdiff_central<T>), dim3(c.nb / nbw), dim3(NT), 0, c.stream, (const T*)c.f[CUP_U],
                      (const T*)c.f[CUP_V], (const T*)c.f[CUP_W], (T*)c.f[CUP_TMPU], (T*)c.f[CUP_TMPV],
                      (T*)c.f[CUP_TMPW], c.grid(), c.sc, (T)uinf[0], (T)uinf[1], (T)uinf[2], nu, nbw);
@@ -630,7 +640,7 @@ int launch_rhs_plain(Ctx& c)
 // sum |u|^2 into ke_part. Blocks a penalized body covers (skip_slot >= 0) leave the max to
 // the penalization kernel, which changes their velocity afterwards.
 // Each thread owns two x-adjacent cells c = 2t, 2t+1 of the block (vector loads/stores);
-// a workgroup walks nbw consecutive blocks, fetching the next block's p lab, tmp velocity
+// a workgroup walks nbw blocks stacked in z, fetching the next block's p lab, tmp velocity
 // and forcing rows while the current one is computed.
 // FORCE adds the fused forcing's velocity modes, fac*u_low = R0 + 2 Re(R1 e^{i a x}) +
 // 2 Re(R2 e^{2 i a x}) per component with the row coefficients R of k_forcing_rows; the
@@ -649,7 +659,8 @@ __global__ __launch_bounds__(NT) void k_gradp(const T* __restrict__ p, const T* 
   const int t = threadIdx.x;
   const LabSlots s = lab_slots();
   const T* src[1] = {p};
-  const int b0 = xcd_remap(blockIdx.x, gridDim.x) * nbw;
+  const int nxy = g.nbx * g.nby;
+  const int b0 = walk_first(g, xcd_remap(blockIdx.x, gridDim.x), nbw);
   const int c0 = 2 * t, ix0 = c0 & 7, iy = (c0 >> 3) & 7, iz = c0 >> 6;
   LabRegs<T, 1> r;
   T us[3][2];
@@ -673,7 +684,7 @@ __global__ __launch_bounds__(NT) void k_gradp(const T* __restrict__ p, const T* 
   fetch(b0);
   const T fac = (T)(-0.5 * sc->dt / g.h);
   for (int j = 0; j < nbw; ++j) {
-    const int b = b0 + j;
+    const int b = b0 + j * nxy;
     if (j) __syncthreads();  // the previous block is done with the lab
     lab_put(r, lab, s);
     if (FORCE)
@@ -681,7 +692,7 @@ __global__ __launch_bounds__(NT) void k_gradp(const T* __restrict__ p, const T* 
       for (int i = 0; i < 4; ++i) rl[4 * t + i] = rr[i];
     T nu_[2] = {us[0][0], us[0][1]}, nv_[2] = {us[1][0], us[1][1]}, nw_[2] = {us[2][0], us[2][1]};
     __syncthreads();
-    if (j + 1 < nbw) fetch(b + 1);
+    if (j + 1 < nbw) fetch(b + nxy);
     {
       const T* P = lab[0];
       const int l = s.li;
@@ -743,7 +754,7 @@ static int gradp_t(Ctx& c, bool fused, const double uinf[3], const int* skip, bo
   const T* p = (const T*)c.f[CUP_P];
   const int s = fused ? c.ustar : CUP_U;
   const C2<T>* tx = (const C2<T>*)c.tw[0];
-  const int nbw = blocks_per_wg(c.nb);
+  const int nbw = blocks_per_wg(c.bpd[2]);
   const dim3 grid(c.nb / nbw);
   if (fused && force)
     hipLaunchKernelGGL((k_gradp<T, true, true>), grid, dim3(NT), 0, c.stream, p, (const T*)c.f[s],
@@ -765,7 +776,7 @@ static int gradp_t(Ctx& c, bool fused, const double uinf[3], const int* skip, bo
 // projection's last pass.
 int launch_gradp_inplace(Ctx& c, int field0)
 {
-  const int nbw = blocks_per_wg(c.nb);
+  const int nbw = blocks_per_wg(c.bpd[2]);
   const dim3 grid(c.nb / nbw);
   if (c.precision == 4) {
     float* f[3] = {(float*)c.f[field0], (float*)c.f[field0 + 1], (float*)c.f[field0 + 2]};
