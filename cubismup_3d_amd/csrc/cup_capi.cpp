@@ -72,10 +72,9 @@ static void release(Ctx& c)
   c.host_red = nullptr;
   if (c.side) hipStreamSynchronize(c.side);
   if (c.ev_fork) hipEventDestroy(c.ev_fork);
-  if (c.ev_join) hipEventDestroy(c.ev_join);
   if (c.ev_rows) hipEventDestroy(c.ev_rows);
   if (c.side) hipStreamDestroy(c.side);
-  c.ev_fork = c.ev_join = c.ev_rows = nullptr;
+  c.ev_fork = c.ev_rows = nullptr;
   c.side = nullptr;
   if (c.stream) hipStreamDestroy(c.stream);
   c.stream = nullptr;

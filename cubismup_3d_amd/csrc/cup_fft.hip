@@ -158,8 +158,8 @@ enum XSrc { XSRC_FIELD = 0, XSRC_DIV = 1 };
 // Real rows -> packed half spectrum. grid = (Ny/8, Nz).
 // FORCE (fused linear forcing): the same pass also sums, for the workgroup's 8 rows, the
 // DFT of the advected velocity at every canonical low mode and sum |u*|^2, written as
-// partials fpart[wg*W + c], W = 6 per mode (re/im of u, v, w) + 1: each row's x-DFT at
-// kx = 0..2, times the row's y-z phase of the mode.
+// partials fpart[c*nwg + slot], c < W = 6 per mode (re/im of u, v, w) + 1: each row's
+// x-DFT at kx = 0..2, times the row's y-z phase of the mode.
 // Each thread's cells lie in one row (t -> bx = t/64, iy, ix), so row sums are lane
 // shuffles over ix plus an LDS sum over the 4 waves.
 template <typename T, int N, int SRC, bool FORCE>
@@ -290,7 +290,10 @@ __global__ __launch_bounds__(NT) void k_fft_x_fwd(const T* __restrict__ src0, co
     }
     __syncthreads();
     const LowModes& lm = *lmp;
-    const int wg = blockIdx.x + gridDim.x * blockIdx.y, W = 6 * lm.n + 1;
+    // column-major partials, fpart[c*nwg + slot]: the slot is the XCD-remapped workgroup
+    // index, so the workgroups of one XCD fill contiguous runs of each column in its L2
+    const int nwg = gridDim.x * gridDim.y, W = 6 * lm.n + 1;
+    const int slot = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, nwg);
     for (int t = threadIdx.x; t < W; t += NT) {
       double outv = 0;
       if (t == W - 1) {
@@ -306,7 +309,7 @@ __global__ __launch_bounds__(NT) void k_fft_x_fwd(const T* __restrict__ src0, co
           outv += im ? (Ur * ei + Ui * er) : (Ur * er - Ui * ei);
         }
       }
-      fpart[(size_t)wg * W + t] = outv;
+      fpart[(size_t)t * nwg + slot] = outv;
     }
   }
   lds_fft<N, XFFT, -1>(buf, tw);
@@ -592,18 +595,17 @@ static int poisson_fused_t(Ctx& c, bool force, double tke_target, double inj_rat
                         (const T*)c.f[c.ustar + 2], (int)XSRC_DIV, force);
   stage_mark(c, ST_XFWD, true);
   if (force) {
-    // The forcing reduction needs only the x pass's row sums and its output is first read by
-    // the z pass: it runs on the side stream beside the y pass (fork after x, join before z).
+    // The forcing reduction (a few microseconds) runs in order; the per-row synthesis
+    // coefficients are first read by the gradient, so they are made on the side stream
+    // (default priority) beside the z, inverse y and inverse x passes and joined after them.
+    stage_mark(c, ST_FREDUCE, false);
+    if (int rc = launch_forcing_reduce(c, tke_target, inj_rate)) return rc;
+    stage_mark(c, ST_FREDUCE, true);
     if (int rc = check_hip(c, hipEventRecord(c.ev_fork, c.stream), "fork")) return rc;
     if (int rc = check_hip(c, hipStreamWaitEvent(c.side, c.ev_fork, 0), "fork")) return rc;
-    // The row coefficients are first read by the gradient: their own join comes later.
     hipStream_t main = c.stream;
     c.stream = c.side;
-    stage_mark(c, ST_FREDUCE, false);
-    int rc = launch_forcing_reduce(c, tke_target, inj_rate);
-    stage_mark(c, ST_FREDUCE, true);
-    if (!rc) rc = check_hip(c, hipEventRecord(c.ev_join, c.side), "join");
-    if (!rc) rc = launch_forcing_rows(c);
+    int rc = launch_forcing_rows(c);
     if (!rc) rc = check_hip(c, hipEventRecord(c.ev_rows, c.side), "join");
     c.stream = main;
     if (rc) return rc;
@@ -611,8 +613,6 @@ static int poisson_fused_t(Ctx& c, bool force, double tke_target, double inj_rat
   stage_mark(c, ST_YFWD, false);
   dispatch_n<GoY, T>(c.N[1], c, S, (int)ZM_FWD, norm);
   stage_mark(c, ST_YFWD, true);
-  if (force)
-    if (int rc = check_hip(c, hipStreamWaitEvent(c.stream, c.ev_join, 0), "join")) return rc;
   stage_mark(c, ST_ZPASS, false);
   if (force) dispatch_n<GoZF, T>(c.N[2], c, S, (int)ZM_POISSON, norm);
   else dispatch_n<GoZ, T>(c.N[2], c, S, (int)ZM_POISSON, norm);

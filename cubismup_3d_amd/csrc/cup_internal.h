@@ -115,13 +115,13 @@ struct Ctx {
   bool low_fused_ok = false;   // modes fit LowModes (count, |kx|)
   LowModes low{};
   LowModes* low_dev = nullptr;  // device copy (kernels index it per lane)
-  double* low_part = nullptr;  // per x-pass workgroup, workgroup-major: 6 per mode (re/im of u,v,w DFT) + 1 (sum |u|^2)
+  double* low_part = nullptr;  // x-pass partials, column-major: 6 per mode (re/im of u,v,w DFT) + 1 (sum |u|^2)
   double* low_red = nullptr;   // reduced sums, then the per-chunk sums of the first reduction stage
   double* low_coef = nullptr;  // per mode: RHS addition (re,im), synthesis of u,v,w (re,im) x3
   unsigned* low_counter = nullptr;
   void* low_rows = nullptr;    // per grid row (y, z): 16 synthesis coefficients (context precision)
-  hipStream_t side = nullptr;  // the forcing reduction's stream (default priority), beside the y pass
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_rows = nullptr;
+  hipStream_t side = nullptr;  // the forcing row coefficients' stream (default priority)
+  hipEvent_t ev_fork = nullptr, ev_rows = nullptr;
   Body body;
   std::string err;
   // measurement hook: HIP events around one stage of the fused step

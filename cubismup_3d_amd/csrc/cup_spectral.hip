@@ -313,9 +313,6 @@ int forcing_plain(Ctx& c, double dt, double sim_dt, double nu, double tke_target
                           : forcing_plain_t<double>(c, dt, sim_dt, nu, tke_target, inj, stats);
 }
 
-// Chunks of x-pass workgroup partials summed by the first stage of the forcing reduction.
-constexpr int FR_CHUNKS = 128;
-
 // Low modes of the linear forcing, 0 < |k|^2 <= 4 in physical wavenumbers, canonical
 // member of each conjugate pair (kz > 0, or kz = 0 and ky > 0, or kz = ky = 0 and kx > 0),
 // and their positions in the packed half spectrum (x halved): kx > 0 at (kx, ky, kz);
@@ -361,7 +358,7 @@ int spectral_setup(Ctx& c)
   const int W = 6 * c.nlow + 1;
   const size_t nwg = (size_t)(c.N[1] / BS) * c.N[2];
   if (check_hip(c, hipMalloc(&c.low_part, (size_t)W * nwg * sizeof(double)), "hipMalloc low partials") ||
-      check_hip(c, hipMalloc(&c.low_red, (size_t)W * (FR_CHUNKS + 1) * sizeof(double)), "hipMalloc low red") ||
+      check_hip(c, hipMalloc(&c.low_red, (size_t)W * sizeof(double)), "hipMalloc low red") ||
       check_hip(c, hipMalloc(&c.low_coef, (size_t)c.nlow * 8 * sizeof(double)), "hipMalloc low coef") ||
       check_hip(c, hipMalloc(&c.low_counter, sizeof(unsigned)), "hipMalloc counter") ||
       check_hip(c, hipMalloc(&c.low_dev, sizeof(LowModes)), "hipMalloc low modes") ||
@@ -369,7 +366,6 @@ int spectral_setup(Ctx& c)
     return CUP_E_NOMEM;
   if (check_hip(c, hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking), "hipStreamCreate side") ||
       check_hip(c, hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming), "hipEventCreate") ||
-      check_hip(c, hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming), "hipEventCreate") ||
       check_hip(c, hipEventCreateWithFlags(&c.ev_rows, hipEventDisableTiming), "hipEventCreate"))
     return CUP_E_DEVICE;
   hipMemcpy(c.low_dev, &c.low, sizeof(LowModes), hipMemcpyHostToDevice);
@@ -378,42 +374,17 @@ int spectral_setup(Ctx& c)
   return CUP_OK;
 }
 
-// Fused forcing, stage 2: the x-pass partials (workgroup-major rows of W values) are summed
-// in FR_CHUNKS contiguous chunks of rows, one workgroup per chunk; the last workgroup to
-// finish sums the chunk totals, computes on the device SpectralForcing's injection and
-// factor (SpectralForcing.cpp:37-77) from tke = <|u*|^2>/2 (Parseval) and tke_low = sum over
-// the low modes, and the coefficient tables: the forcing's pressure-RHS modes
+// Fused forcing, stage 2: one workgroup per column of the x-pass partials (column-major, a
+// column is nwg contiguous values). Each thread sums its entries j = t, t + NT, ... in order,
+// issuing FR_BATCH independent coalesced loads at a time so that a column costs a few memory
+// latencies; the
+// workgroup's sums are combined by a fixed shuffle tree (deterministic). The last workgroup
+// to finish computes, on the device, SpectralForcing's injection and factor
+// (SpectralForcing.cpp:37-77) from tke = <|u*|^2>/2 (Parseval) and tke_low = sum over the
+// low modes, and the coefficient tables: the forcing's pressure-RHS modes
 // fac*(h^2/dt)*i*sum_d sin(2 pi k_d/N_d) u_d(k) (added in the z pass) and the velocity modes
-// fac/N^3 u(k) (added in the gradient pass). red holds W totals then FR_CHUNKS*W chunk sums.
-// Rows are summed in a fixed order (deterministic). Each tile of FR_TILE rows is staged in
-// LDS by coalesced loads that are all issued before the first is used, so a chunk costs one
-// memory latency per tile rather than one per row.
-constexpr int FR_TILE = 32;
-constexpr int FR_LOADS = (FR_TILE * (6 * LOW_MAX + 1) + NT - 1) / NT;
-
-// acc (thread t < W) += sum over rows [r0, r1) of src[row * W + t].
-__device__ __forceinline__ void sum_rows_tiled(const double* src, int r0, int r1, int W, double* tile, double& acc)
-{
-  for (int r = r0; r < r1; r += FR_TILE) {
-    const int n = (r1 - r < FR_TILE ? r1 - r : FR_TILE) * W;
-    const double* p = src + (size_t)r * W;
-    double v[FR_LOADS];
-#pragma unroll
-    for (int q = 0; q < FR_LOADS; ++q) {
-      const int e = threadIdx.x + q * NT;
-      v[q] = e < n ? p[e] : 0.0;
-    }
-#pragma unroll
-    for (int q = 0; q < FR_LOADS; ++q) {
-      const int e = threadIdx.x + q * NT;
-      if (e < n) tile[e] = v[q];
-    }
-    __syncthreads();
-    if ((int)threadIdx.x < W)
-      for (int k = 0; k < n / W; ++k) acc += tile[k * W + threadIdx.x];
-    __syncthreads();
-  }
-}
+// fac/N^3 u(k) (added in the gradient pass).
+constexpr int FR_BATCH = 16;
 
 __global__ __launch_bounds__(NT) void k_forcing_reduce(const double* __restrict__ part, int nwg, int W,
                                                        double* __restrict__ red, unsigned* __restrict__ counter,
@@ -421,36 +392,39 @@ __global__ __launch_bounds__(NT) void k_forcing_reduce(const double* __restrict_
                                                        double* __restrict__ coef, double tke_target, double inj_rate,
                                                        double h, double ncell, int N0, int N1, int N2)
 {
-  extern __shared__ double tile[];  // FR_TILE * W doubles, sized at launch
+  __shared__ double wsum[NT / 64];
   __shared__ double fin[6 * LOW_MAX + 1];
   __shared__ double facs[1];
   __shared__ bool last;
-  const int t = threadIdx.x;
-  double* chunk = red + W;
-  {
-    const int lo = (int)((long long)nwg * blockIdx.x / gridDim.x);
-    const int hi = (int)((long long)nwg * (blockIdx.x + 1) / gridDim.x);
-    double a = 0;
-    sum_rows_tiled(part, lo, hi, W, tile, a);
-    if (t < W) chunk[(size_t)blockIdx.x * W + t] = a;
+  const int t = threadIdx.x, col = blockIdx.x;
+  double a = 0;
+  for (int j0 = t; j0 < nwg; j0 += FR_BATCH * NT) {
+    double v[FR_BATCH];
+#pragma unroll
+    for (int q = 0; q < FR_BATCH; ++q) {
+      const int j = j0 + q * NT;
+      v[q] = j < nwg ? part[(size_t)col * nwg + j] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < FR_BATCH; ++q) a += v[q];
   }
-  __threadfence();
+  a = wave_sum(a);
+  if ((t & 63) == 0) wsum[t >> 6] = a;
   __syncthreads();
-  if (t == 0) last = atomicAdd(counter, 1u) == gridDim.x - 1;
+  if (t == 0) {
+    double tot = 0;
+    for (int w = 0; w < NT / 64; ++w) tot += wsum[w];
+    red[col] = tot;
+    __threadfence();
+    last = atomicAdd(counter, 1u) == (unsigned)(W - 1);
+  }
   __syncthreads();
   if (!last) return;
-  __threadfence();  // acquire: the other workgroups' chunk sums are visible to plain loads
-  {
-    double a = 0;
-    sum_rows_tiled(chunk, 0, (int)gridDim.x, W, tile, a);
-    if (t < W) {
-      fin[t] = a;
-      red[t] = a;
-    }
-  }
+  __threadfence();  // acquire: the other workgroups' column sums are visible to plain loads
+  if (t < W) fin[t] = red[t];
   __syncthreads();
-  const LowModes& lm = *lmp;
   const double* r = fin;
+  const LowModes& lm = *lmp;
   const double dt = sc->dt;
   const double norm = (double)N0 * N1 * N2;
   if (t == 0) {
@@ -498,9 +472,7 @@ int launch_forcing_reduce(Ctx& c, double tke_target, double inj_rate)
 {
   const int W = 6 * c.nlow + 1;
   const int nwg = (c.N[1] / BS) * c.N[2];  // x-pass workgroups
-  const int nch = nwg < FR_CHUNKS ? nwg : FR_CHUNKS;
-  const size_t lds = (size_t)FR_TILE * W * sizeof(double);
-  hipLaunchKernelGGL(k_forcing_reduce, dim3(nch), dim3(NT), lds, c.stream, (const double*)c.low_part, nwg, W, c.low_red,
+  hipLaunchKernelGGL(k_forcing_reduce, dim3(W), dim3(NT), 0, c.stream, (const double*)c.low_part, nwg, W, c.low_red,
                      c.low_counter, c.low_dev, c.sc, c.low_coef, tke_target, inj_rate, c.h, (double)c.ncell, c.N[0],
                      c.N[1], c.N[2]);
   return check_hip(c, hipGetLastError(), "k_forcing_reduce");

@@ -75,9 +75,9 @@ __device__ __forceinline__ void block_coords(const GridArgs& g, int b, int& bx, 
 // twice the bytes in flight per CU of a load-sync-compute workgroup.
 // A thread's cells: interior c = 2t, 2t+1; halo (faces only, 6*64 cells) c = t (faces
 // x-, x+, y-, y+: one face per wave) and, for t < 128, c = NT + t (faces z-, z+).
-// The pipelined lab pads x to PX = 12 (interior at x = 2..9, halo at 1 and 10) so that a
-// thread's x-adjacent cell pair, and its y and z neighbour pairs, are 8-byte aligned: each is
-// one 64-bit LDS access.
+// The pipelined lab pads x to PX = 12 (interior at x = 2..9, halo at 1 and 10): a thread's
+// x-adjacent cell pair, and its y and z neighbour pairs, are 8-byte aligned (one 64-bit LDS
+// access each).
 constexpr int PX = 12;
 constexpr int PY = PX;            // y stride
 constexpr int PZ = PX * L1;       // z stride
@@ -148,7 +148,9 @@ __device__ __forceinline__ void lab_fetch(LabRegs<T, NF>& r, const T* const* src
 #pragma unroll
   for (int k = 0; k < 2; ++k)
     if (k == 0 || has_halo2()) {
-      const size_t q = (size_t)face_block(g, bx, by, bz, s.face[k]) * BS3 + s.off[k];
+      // a halo slot's face is the same across its wave: the neighbour block is scalar work
+      const int face = __builtin_amdgcn_readfirstlane(s.face[k]);
+      const size_t q = (size_t)face_block(g, bx, by, bz, face) * BS3 + s.off[k];
 #pragma unroll
       for (int f = 0; f < NF; ++f) r.hv[f][k] = src[f][q];
     }
@@ -313,57 +315,64 @@ __global__ __launch_bounds__(NT) void k_advdiff_central(const T* __restrict__ u,
   const T* src[3] = {u, v, w};
   const int nxy = g.nbx * g.nby;
   const int b0 = walk_first(g, xcd_remap(blockIdx.x, gridDim.x), nbw);
-  LabRegs<T, 3> r;
-  lab_fetch(r, src, s, g, b0);
+  // two blocks in flight: block j+2's loads are issued while block j is computed, so each
+  // block's loads have a whole iteration to arrive
+  LabRegs<T, 3> ra, rb;
+  lab_fetch(ra, src, s, g, b0);
+  if (nbw > 1) lab_fetch(rb, src, s, g, b0 + nxy);
   const double dt = sc->dt;
   const T facA = (T)(-dt / (2 * g.h));
   const T facD = (T)((nu / g.h) * (dt / g.h));
   const int c0 = 2 * threadIdx.x;
   using V2 = typename Vec2<T>::type;
-  for (int j = 0; j < nbw; ++j) {
-    const int b = b0 + j * nxy;
-    if (j) __syncthreads();  // the previous block's compute is done with the lab
-    lab_put(r, lab, s);
-    __syncthreads();
-    if (j + 1 < nbw) lab_fetch(r, src, s, g, b + nxy);
-    T o[3][2], fc[3][2], fe[3][2], fw[3][2], fn[3][2], fs[3][2], fb[3][2], ff[3][2];
+  auto compute_store = [&](int b) {
+    // advecting velocity of the pair (centre values of u, v, w), then one component at a
+    // time: its 7-point stencil for both cells (few live registers)
+    T av[3][2];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-      // the pair's values: centre c (x, x+1), west of x, east of x+1, and the y/z neighbour pairs
+      const V2 C = *reinterpret_cast<const V2*>(lab[q] + s.li);
+      av[q][0] = C.x;
+      av[q][1] = C.y;
+    }
+    const T uu[2] = {av[0][0] + Ux, av[0][1] + Ux}, vv[2] = {av[1][0] + Uy, av[1][1] + Uy},
+            ww[2] = {av[2][0] + Uz, av[2][1] + Uz};
+    T o[3][2];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
       const T* F = lab[q];
       const int l = s.li;
-      const V2 C = *reinterpret_cast<const V2*>(F + l);
       const V2 Nn = *reinterpret_cast<const V2*>(F + l + PY), Sn = *reinterpret_cast<const V2*>(F + l - PY);
       const V2 Bn = *reinterpret_cast<const V2*>(F + l + PZ), Fn = *reinterpret_cast<const V2*>(F + l - PZ);
       const T Wx = F[l - 1], Ex = F[l + 2];
-      fc[q][0] = C.x; fc[q][1] = C.y;
-      fe[q][0] = C.y; fe[q][1] = Ex;
-      fw[q][0] = Wx;  fw[q][1] = C.x;
-      fn[q][0] = Nn.x; fn[q][1] = Nn.y;
-      fs[q][0] = Sn.x; fs[q][1] = Sn.y;
-      fb[q][0] = Bn.x; fb[q][1] = Bn.y;
-      ff[q][0] = Fn.x; ff[q][1] = Fn.y;
-    }
+      const T fc[2] = {av[q][0], av[q][1]};
+      const T fe[2] = {av[q][1], Ex}, fw[2] = {Wx, av[q][0]};
+      const T fn[2] = {Nn.x, Nn.y}, fs[2] = {Sn.x, Sn.y}, fb[2] = {Bn.x, Bn.y}, ff[2] = {Fn.x, Fn.y};
 #pragma unroll
-    for (int k = 0; k < 2; ++k) {
-      const T dudx = fe[0][k] - fw[0][k], dvdx = fe[1][k] - fw[1][k], dwdx = fe[2][k] - fw[2][k];
-      const T dudy = fn[0][k] - fs[0][k], dvdy = fn[1][k] - fs[1][k], dwdy = fn[2][k] - fs[2][k];
-      const T dudz = fb[0][k] - ff[0][k], dvdz = fb[1][k] - ff[1][k], dwdz = fb[2][k] - ff[2][k];
-      const T uu = fc[0][k] + Ux, vv = fc[1][k] + Uy, ww = fc[2][k] + Uz;
-      const T duD = fn[0][k] + fs[0][k] + fe[0][k] + fw[0][k] + ff[0][k] + fb[0][k] - fc[0][k] * 6;
-      const T dvD = fn[1][k] + fs[1][k] + fe[1][k] + fw[1][k] + ff[1][k] + fb[1][k] - fc[1][k] * 6;
-      const T dwD = fn[2][k] + fs[2][k] + fe[2][k] + fw[2][k] + ff[2][k] + fb[2][k] - fc[2][k] * 6;
-      const T duA = uu * dudx + vv * dudy + ww * dudz;
-      const T dvA = uu * dvdx + vv * dvdy + ww * dvdz;
-      const T dwA = uu * dwdx + vv * dwdy + ww * dwdz;
-      o[0][k] = fc[0][k] + facA * duA + facD * duD;
-      o[1][k] = fc[1][k] + facA * dvA + facD * dvD;
-      o[2][k] = fc[2][k] + facA * dwA + facD * dwD;
+      for (int k = 0; k < 2; ++k) {
+        const T ddx = fe[k] - fw[k], ddy = fn[k] - fs[k], ddz = fb[k] - ff[k];
+        const T dD = fn[k] + fs[k] + fe[k] + fw[k] + ff[k] + fb[k] - fc[k] * 6;
+        const T dA = uu[k] * ddx + vv[k] * ddy + ww[k] * ddz;
+        o[q][k] = fc[k] + facA * dA + facD * dD;
+      }
     }
     const size_t q = (size_t)b * BS3 + c0;
     *reinterpret_cast<V2*>(ou + q) = V2{o[0][0], o[0][1]};
     *reinterpret_cast<V2*>(ov + q) = V2{o[1][0], o[1][1]};
     *reinterpret_cast<V2*>(ow + q) = V2{o[2][0], o[2][1]};
+  };
+  for (int j = 0; j < nbw; j += 2) {
+    if (j) __syncthreads();  // the previous block's compute is done with the lab
+    lab_put(ra, lab, s);
+    __syncthreads();
+    if (j + 2 < nbw) lab_fetch(ra, src, s, g, b0 + (j + 2) * nxy);
+    compute_store(b0 + j * nxy);
+    if (j + 1 >= nbw) break;
+    __syncthreads();
+    lab_put(rb, lab, s);
+    __syncthreads();
+    if (j + 3 < nbw) lab_fetch(rb, src, s, g, b0 + (j + 3) * nxy);
+    compute_store(b0 + (j + 1) * nxy);
   }
 }
 
