@@ -69,6 +69,58 @@ def rollout(u0, grid: Grid, nsteps: int, nu: float, uinf=(0.0, 0.0, 0.0), dts=No
                          None if dts is None else list(dts), int(every), float(CFL), out)
 
 
+class PenalizedRollout(torch.autograd.Function):
+    """Time-averaged drag of a rollout with the penalized sphere (forced at the body velocity
+    V, the grid's chi from Grid.chi_from_sphere), J = sum_n (dt_n / T) F_x^n with F^n the step's
+    penalization force; differentiable with respect to u0 and V. Fixed timesteps."""
+
+    @staticmethod
+    def forward(ctx, u0, vel, grid: Grid, nsteps: int, nu: float, uinf, dts, every: int, lambda_: float,
+                implicit: bool):
+        sphere = dict(lambda_=lambda_, implicit=implicit, trans_vel=tuple(float(x) for x in vel.tolist()))
+        grid.set_torch_velocity(u0.detach())
+        T = float(sum(dts))
+        ckpts = {0: u0.detach().clone()}
+        J = 0.0
+        for n in range(nsteps):
+            grid.run_steps(1, nu=nu, uinf=uinf, fixed_dt=dts[n], sphere=sphere)
+            J += dts[n] / T * grid.last_body_force()[0]
+            if (n + 1) % every == 0 and n + 1 < nsteps:
+                ckpts[n + 1] = grid.get_torch_velocity()
+        ctx.grid, ctx.nsteps, ctx.nu, ctx.uinf, ctx.dts, ctx.every = grid, nsteps, nu, uinf, list(dts), every
+        ctx.sphere, ctx.T, ctx.ckpts, ctx.u0_dtype, ctx.vdev = sphere, T, ckpts, u0.dtype, vel.device
+        return torch.tensor(J, dtype=torch.float64, device=u0.device)
+
+    @staticmethod
+    def backward(ctx, gJ):
+        grid, every, sphere = ctx.grid, ctx.every, ctx.sphere
+        g = float(gJ)
+        lam = torch.zeros((3,) + grid.shape, dtype=grid._torch_dtype(), device="cuda")
+        dV = torch.zeros(3, dtype=torch.float64)
+        for s0 in sorted(ctx.ckpts, reverse=True):
+            s1 = min(s0 + every, ctx.nsteps)
+            states = [ctx.ckpts[s0]]
+            grid.set_torch_velocity(ctx.ckpts[s0])
+            for n in range(s0, s1 - 1):
+                grid.run_steps(1, nu=ctx.nu, uinf=ctx.uinf, fixed_dt=ctx.dts[n], sphere=sphere)
+                states.append(grid.get_torch_velocity())
+            for n in reversed(range(s0, s1)):
+                grid.set_torch_velocity(states[n - s0])
+                grid.set_torch_adjoint(lam)
+                dV += torch.from_numpy(grid.adjoint_step_body(ctx.dts[n], ctx.nu, ctx.uinf, sphere,
+                                                              g * ctx.dts[n] / ctx.T))
+                lam = grid.get_torch_adjoint()
+        ctx.ckpts = None
+        return lam.to(ctx.u0_dtype), dV.to(ctx.vdev), None, None, None, None, None, None, None, None
+
+
+def mean_drag(u0, vel, grid: Grid, nsteps: int, nu: float, uinf, dts, every=10, lambda_=1e6, implicit=True):
+    """Time-averaged drag (penalization force, x) of a rollout with the sphere; differentiable
+    with respect to u0 ([3][Z][Y][X], cuda) and vel (the body velocity, 3 values)."""
+    return PenalizedRollout.apply(u0, vel, grid, int(nsteps), float(nu), tuple(float(x) for x in uinf),
+                                  list(dts), int(every), float(lambda_), bool(implicit))
+
+
 def kinetic_energy(u, h):
     """KE = h^3/2 sum |u|^2 (ComputeDissipation.cpp:109)."""
     return 0.5 * h ** 3 * (u * u).sum()

@@ -613,4 +613,44 @@ int cup_adjoint_step(cup_ctx* ctx, double dt, double nu, const double uinf[3], i
   return CUP_OK;
 }
 
+int cup_adjoint_step_body(cup_ctx* ctx, double dt, double nu, const double uinf[3], double lambda, int implicit,
+                          const double body_vel[3], double wdrag, double dJdV[3])
+{
+  CUP_GUARD(ctx);
+  if (!uinf || !body_vel || !dJdV || !(dt > 0)) return set_error(c, CUP_E_INVALID, "adjoint_step_body: bad argument");
+  if (!c.body.present) return set_error(c, CUP_E_STATE, "adjoint_step_body: no obstacle");
+  for (int f = CUP_ADJ_U; f <= CUP_ADJ_W; ++f)
+    if (int rc = field_ready(c, f, "adjoint_step_body")) return rc;
+  if (int rc = launch_set_dt(c, dt)) return rc;
+  // forward state at this step: u* = A(u_n) in tmp, and its masked RHS on the obstacle blocks
+  if (int rc = launch_advdiff(c, nu, uinf, CUP_CENTRAL_EULER)) return rc;
+  if (int rc = launch_adjoint_body(c, 0, lambda, implicit, wdrag)) return rc;
+  // penalization and the drag's weight; dJ/dV of this step
+  if (int rc = launch_adjoint_body(c, 1, lambda, implicit, wdrag)) return rc;
+  // transposed projection: y = L^{-1} R lam' (no obstacle RHS), y <- M'^T y, lam* = lam' - (dt/2h) grad y
+  c.ustar = CUP_ADJ_U;
+  c.plain_rhs = true;
+  int rc = launch_poisson_fused(c, false, 0.0, 0.0);
+  c.plain_rhs = false;
+  c.ustar = CUP_TMPU;
+  if (rc) return rc;
+  if (int e = launch_adjoint_body(c, 2, lambda, implicit, wdrag)) return e;
+  if (int e = launch_gradp_inplace(c, CUP_ADJ_U)) return e;
+  if (int e = launch_adjoint_advdiff(c, nu, uinf)) return e;
+  for (int q = 0; q < 3; ++q) std::swap(c.f[CUP_TMPU + q], c.f[CUP_ADJ_U + q]);
+  if (int e = read_scalars(c)) return e;
+  for (int d = 0; d < 3; ++d) dJdV[d] = c.sc_host->red[8 + d];
+  return CUP_OK;
+}
+
+int cup_last_body_force(cup_ctx* ctx, double out[6])
+{
+  CUP_GUARD(ctx);
+  if (!out) return set_error(c, CUP_E_INVALID, "last_body_force: null pointer");
+  if (!c.body.present) return set_error(c, CUP_E_STATE, "last_body_force: no obstacle");
+  if (int rc = read_scalars(c)) return rc;
+  for (int i = 0; i < 6; ++i) out[i] = c.sc_host->red[i];
+  return CUP_OK;
+}
+
 }  // extern "C"

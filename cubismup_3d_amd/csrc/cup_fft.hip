@@ -481,7 +481,7 @@ struct XPass {
   static void fwd(Ctx& c, const T* s0, const T* s1, const T* s2, int src, bool force)
   {
     dim3 grid(c.N[1] / BS, c.N[2]);
-    const int* slot = c.body.present ? c.body.ob_slot : nullptr;
+    const int* slot = c.body.present && !c.plain_rhs ? c.body.ob_slot : nullptr;
     const C2<T>* twy = (const C2<T>*)c.tw[1];
     const C2<T>* twz = (const C2<T>*)c.tw[2];
     if (src == XSRC_FIELD)

@@ -98,6 +98,7 @@ struct Ctx {
 
   void* f[11] = {};       // block-layout fields chi,u,v,w,p,tmpU,tmpV,tmpW; adjoint u,v,w (lazy)
   int ustar = CUP_TMPU;   // first of the three fields holding the step's u* after advection
+  bool plain_rhs = false; // the fused x pass ignores the obstacle RHS (the adjoint's projection)
   void* spec = nullptr;   // packed half spectrum, complex, [z][y][kx], kx < Nx/2
   void* spec2 = nullptr;  // extra spectra for the plain forcing path
   void* spec3 = nullptr;
@@ -177,6 +178,7 @@ int launch_copy_tmp_to_u(Ctx& c);
 int launch_copy_u_to_tmp(Ctx& c);
 int launch_adjoint_advdiff(Ctx& c, double nu, const double uinf[3]);
 int launch_gradp_inplace(Ctx& c, int field0);
+int launch_adjoint_body(Ctx& c, int phase, double lambda, int implicit, double wdrag);
 int launch_rhs_plain(Ctx& c);                                   // -> rbuf
 int launch_gradp_plain(Ctx& c);                                 // u += -dt/2h grad p
 int launch_gradp_fused(Ctx& c, const double uinf[3], const int* skip_slot, bool force = false);  // u = tmp - dt/2h grad p; max, ke_part

@@ -394,6 +394,159 @@ __global__ __launch_bounds__(NT) void k_penalize(const int* __restrict__ ob_ids,
   }
 }
 
+__global__ void k_sum_rows(const double* __restrict__ part, int n, int width, double* __restrict__ out);
+
+// ------------------------------------------------------------ adjoint with a body
+// (no reference counterpart) The transposes of the penalization, of its drag F_x and of the
+// RHS masking with its perimeter correction, for the adjoint step with a forced sphere
+// (velocity V, no rotation). Partials per obstacle block, summed by every workgroup in the
+// same order where a total is needed (deterministic).
+
+// lam' = (1 - dt pf) lam - wdrag h^3 pf e_x on penalized cells; part[3s+d] += this block's
+// dJ/dV_d = sum dt pf lam_d + wdrag h^3 pf [d = x].
+template <typename T>
+__global__ __launch_bounds__(NT) void k_adj_penalize(const int* __restrict__ ob_ids, const double* __restrict__ ob_chi,
+                                                     const T* __restrict__ chi, T* __restrict__ lu, T* __restrict__ lv,
+                                                     T* __restrict__ lw, GridArgs g, const DevScalars* __restrict__ sc,
+                                                     double lambda, int implicit, double wdrag,
+                                                     double* __restrict__ part)
+{
+  const int s = blockIdx.x, b = ob_ids[s];
+  const double dt = sc->dt, h3 = g.h * g.h * g.h;
+  const double lf = implicit ? lambda : 1.0 / dt;
+  T* L[3] = {lu, lv, lw};
+  double acc[3] = {0, 0, 0};
+  for (int c = threadIdx.x; c < BS3; c += NT) {
+    const size_t o = (size_t)b * BS3 + c;
+    const double X = ob_chi[(size_t)s * BS3 + c];
+    if (chi[o] > (T)X || !(X > 0)) continue;
+    const double pf = implicit ? X * lf / (1 + lf * dt * X) : X * lf;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double l = L[d][o];
+      const double dr = d == 0 ? wdrag * h3 * pf : 0.0;
+      acc[d] += dt * pf * l + dr;
+      L[d][o] = (T)((1 - dt * pf) * l - dr);
+    }
+  }
+  breduce<3>(acc, part + 3 * (size_t)s);
+}
+
+// Totals of the forward masking sums (S, pos, neg per block) and the correction factor,
+// as k_rhs_obs_correct forms them.
+__device__ __forceinline__ void mask_totals(const double* part, int nob, double& S, double& D, double& corr,
+                                            bool& live)
+{
+  __shared__ double tot[3];
+  double acc[3] = {0, 0, 0};
+  for (int j = threadIdx.x; j < nob; j += NT) {
+    acc[0] += part[3 * (size_t)j];
+    acc[1] += part[3 * (size_t)j + 1];
+    acc[2] += part[3 * (size_t)j + 2];
+  }
+  breduce<3>(acc, tot);
+  __syncthreads();
+  S = tot[0];
+  const double denom = S > 0 ? tot[1] : tot[2];
+  D = fmax(denom, DEPS);
+  live = denom > DEPS;  // the denominator varies with the RHS (else it is the floor)
+  corr = S / D;
+}
+
+// g = sum over corrected cells Q of y * dr_final/dcorr = y * (-sign(corr) r_new), per block.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_adj_mask_g(const int* __restrict__ ob_ids, const double* __restrict__ ob_chi,
+                                                   const T* __restrict__ chi, const T* __restrict__ rnew,
+                                                   const unsigned char* __restrict__ touch, const T* __restrict__ y,
+                                                   const double* __restrict__ msum, int nob, double* __restrict__ gpart)
+{
+  double S, D, corr;
+  bool live;
+  mask_totals(msum, nob, S, D, corr, live);
+  const double sg = corr > 0 ? 1.0 : (corr < 0 ? -1.0 : 0.0);
+  const int s = blockIdx.x, b = ob_ids[s];
+  double acc[1] = {0};
+  for (int c = threadIdx.x; c < BS3; c += NT) {
+    const size_t o = (size_t)b * BS3 + c, q = (size_t)s * BS3 + c;
+    if (chi[o] > (T)ob_chi[q] || !touch[q]) continue;
+    const double r = rnew[q];
+    if ((r > 0 && corr > 0) || (r < 0 && corr < 0)) acc[0] += (double)y[o] * (-sg * r);
+  }
+  breduce<1>(acc, gpart + s);
+}
+
+// y <- M'^T y on the masked cells of the obstacle blocks: with a_S = g/D, a_D = -g corr/D,
+// the adjoint of r_new is y (1 - sg corr) + a_D sg on corrected cells and y elsewhere, and
+// that of the unmasked RHS is (1 - X) adj(r_new) - X a_S.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_adj_mask_apply(const int* __restrict__ ob_ids,
+                                                       const double* __restrict__ ob_chi, const T* __restrict__ chi,
+                                                       const T* __restrict__ rnew,
+                                                       const unsigned char* __restrict__ touch, T* __restrict__ y,
+                                                       const double* __restrict__ msum, int nob,
+                                                       const double* __restrict__ gpart)
+{
+  double S, D, corr;
+  bool live;
+  mask_totals(msum, nob, S, D, corr, live);
+  __shared__ double gtot;
+  {
+    double acc[1] = {0};
+    for (int j = threadIdx.x; j < nob; j += NT) acc[0] += gpart[j];
+    __shared__ double t1[1];
+    breduce<1>(acc, t1);
+    __syncthreads();
+    if (threadIdx.x == 0) gtot = t1[0];
+    __syncthreads();
+  }
+  const double sg = corr > 0 ? 1.0 : (corr < 0 ? -1.0 : 0.0);
+  const double aS = gtot / D, aD = live ? -gtot * corr / D : 0.0;
+  const int s = blockIdx.x, b = ob_ids[s];
+  for (int c = threadIdx.x; c < BS3; c += NT) {
+    const size_t o = (size_t)b * BS3 + c, q = (size_t)s * BS3 + c;
+    const double X = ob_chi[q];
+    if (chi[o] > (T)X) continue;
+    const double r = rnew[q], yy = y[o];
+    const bool inQ = touch[q] && ((r > 0 && corr > 0) || (r < 0 && corr < 0));
+    const double an = inQ ? yy * (1 - sg * corr) + aD * sg : yy;
+    y[o] = (T)((1 - X) * an - X * aS);
+  }
+}
+
+template <typename T>
+static int adj_body_t(Ctx& c, int phase, double lambda, int implicit, double wdrag)
+{
+  Body& B = c.body;
+  double* msum = c.partials + 2 * (size_t)c.nb;  // written by k_rhs_obs_mask
+  double* gpart = msum + 3 * (size_t)B.nob;
+  double* vpart = gpart + B.nob;
+  if (phase == 0) {  // the forward masking of u* (tmp): r_new in ob_rhs, touch flags, sums
+    const int s0 = c.ustar;
+    hipLaunchKernelGGL((k_rhs_obs_mask<T>), dim3(B.nob), dim3(NT), 0, c.stream, B.ob_ids, B.ob_chi,
+                       (const T*)c.f[CUP_CHI], (const T*)c.f[s0], (const T*)c.f[s0 + 1], (const T*)c.f[s0 + 2],
+                       (T*)nullptr, (T*)B.ob_rhs, B.ob_touch, c.grid(), c.sc, msum);
+  } else if (phase == 1) {  // penalization and drag, dJ/dV into sc->red[8..10]
+    hipLaunchKernelGGL((k_adj_penalize<T>), dim3(B.nob), dim3(NT), 0, c.stream, B.ob_ids, B.ob_chi,
+                       (const T*)c.f[CUP_CHI], (T*)c.f[CUP_ADJ_U], (T*)c.f[CUP_ADJ_V], (T*)c.f[CUP_ADJ_W], c.grid(),
+                       c.sc, lambda, implicit, wdrag, vpart);
+    hipLaunchKernelGGL(k_sum_rows, dim3(3), dim3(NT), 0, c.stream, (const double*)vpart, B.nob, 3, c.sc->red + 8);
+  } else {  // the masking's transpose on the adjoint pressure
+    hipLaunchKernelGGL((k_adj_mask_g<T>), dim3(B.nob), dim3(NT), 0, c.stream, B.ob_ids, B.ob_chi,
+                       (const T*)c.f[CUP_CHI], (const T*)B.ob_rhs, B.ob_touch, (const T*)c.f[CUP_P], msum, B.nob,
+                       gpart);
+    hipLaunchKernelGGL((k_adj_mask_apply<T>), dim3(B.nob), dim3(NT), 0, c.stream, B.ob_ids, B.ob_chi,
+                       (const T*)c.f[CUP_CHI], (const T*)B.ob_rhs, B.ob_touch, (T*)c.f[CUP_P], msum, B.nob, gpart);
+  }
+  return check_hip(c, hipGetLastError(), "adjoint body");
+}
+
+int launch_adjoint_body(Ctx& c, int phase, double lambda, int implicit, double wdrag)
+{
+  if (!c.body.present) return set_error(c, CUP_E_STATE, "adjoint: no obstacle");
+  return c.precision == 4 ? adj_body_t<float>(c, phase, lambda, implicit, wdrag)
+                          : adj_body_t<double>(c, phase, lambda, implicit, wdrag);
+}
+
 __global__ __launch_bounds__(NT) void k_sum_rows(const double* __restrict__ part, int n, int width,
                                                  double* __restrict__ out)
 {

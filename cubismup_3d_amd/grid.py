@@ -150,6 +150,21 @@ class Grid:
         """adj <- dJ/du_n from adj = dJ/du_{n+1}, with u_n the current velocity."""
         self._c(L.lib().cup_adjoint_step(self._h, float(dt), float(nu), L.vec3(uinf), int(scheme)))
 
+    def adjoint_step_body(self, dt, nu, uinf, sphere, wdrag=0.0) -> np.ndarray:
+        """With the sphere: adj <- dJ/du_n; returns this step's dJ/d(body velocity), J holding
+        wdrag times the step's drag (penalization force, x)."""
+        out = np.zeros(3)
+        self._c(L.lib().cup_adjoint_step_body(self._h, float(dt), float(nu), L.vec3(uinf), float(sphere["lambda_"]),
+                                              int(bool(sphere["implicit"])), L.vec3(sphere["trans_vel"]),
+                                              float(wdrag), L.dptr(out)))
+        return out
+
+    def last_body_force(self) -> np.ndarray:
+        """Penalization force and torque of the last run_steps step with the sphere."""
+        out = np.zeros(6)
+        self._c(L.lib().cup_last_body_force(self._h, L.dptr(out)))
+        return out
+
     def stream(self) -> int:
         s = C.c_void_p()
         self._c(L.lib().cup_stream(self._h, C.byref(s)))

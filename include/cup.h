@@ -188,6 +188,16 @@ int cup_set_step_state(cup_ctx* ctx, int step, double time);
  * stencil. Overwrites p and tmp; the device pointers of tmp and the adjoint fields may
  * change (cup_field_device_ptr again after this call). */
 int cup_adjoint_step(cup_ctx* ctx, double dt, double nu, const double uinf[3], int scheme);
+/* The same with the sphere (forced at body_vel, no rotation; chi from cup_chi_from_sphere),
+ * no reference counterpart. The step adds the implicit (or explicit) penalization of
+ * Penalization.cpp:40-98 after the projection, whose RHS is masked and corrected on the
+ * body as in PressureRHS.cpp:78-112,322-343. The objective may hold wdrag times this step's
+ * drag F_x = sum h^3 pf (V_x - u_x) (the penalization force, Penalization.cpp:93-96). The
+ * masking's sign pattern is held at the forward state. Returns this step's dJ/dV in dJdV. */
+int cup_adjoint_step_body(cup_ctx* ctx, double dt, double nu, const double uinf[3], double lambda, int implicit,
+                          const double body_vel[3], double wdrag, double dJdV[3]);
+/* Penalization force and torque (6 values) of the last step cup_run_steps made with a body. */
+int cup_last_body_force(cup_ctx* ctx, double out[6]);
 
 /* Measurement: the stream the context launches on, and HIP events recorded on it around
  * one stage of the FUSED step (0 advection-diffusion, 1 x-forward FFT with the RHS,

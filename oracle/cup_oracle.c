@@ -1278,22 +1278,10 @@ void orc_surface_forces(const orc_sim* s, double nu, const double uTrans[3], con
  *  - A(u)_q = u_q + facA sum_d a_d (u_q(+e_d) - u_q(-e_d)) + facD lap u_q, a = u + U:
  *    A'(u)^T lam*_r = lam*_r + facD lap lam*_r - facA sum_d [(a_d lam*_r)(+e_d) - (a_d lam*_r)(-e_d)]
  *                     + facA sum_q lam*_q (u_q(+e_r) - u_q(-e_r)). */
-void orc_adjoint_step(orc_sim* s, double dt, double nu, const double uinf[3], double* lam)
+/* lam <- A'(u_n)^T lam with u_n the sim's velocity (overwrites tmp). */
+static void adj_advect(orc_sim* s, double dt, double nu, const double uinf[3], double* lam)
 {
   const size_t nc = (size_t)s->nb * BS * BS * BS;
-  double* save = (double*)malloc(4 * nc * sizeof(double));
-  const int keep[4] = {ORC_U, ORC_V, ORC_W, ORC_P};
-  for (int q = 0; q < 4; q++) orc_get_field(s, keep[q], save + q * nc);
-  for (int q = 0; q < 3; q++) orc_set_field(s, ORC_U + q, lam + q * nc);
-  const int sphere = s->has_sphere;
-  s->has_sphere = 0;
-  orc_pressure_rhs(s, dt);
-  orc_poisson_solve(s);
-  orc_grad_p(s, dt);
-  s->has_sphere = sphere;
-  for (int q = 0; q < 3; q++) orc_get_field(s, ORC_U + q, lam + q * nc);  /* lam* */
-  for (int q = 0; q < 4; q++) orc_set_field(s, keep[q], save + q * nc);
-  free(save);
   /* lam* into tmp (so that a lab carries both u and lam*), then the transposed operator */
   #pragma omp parallel for schedule(static)
   for (int b = 0; b < s->nb; b++) {
@@ -1337,6 +1325,157 @@ void orc_adjoint_step(orc_sim* s, double dt, double nu, const double uinf[3], do
     }
     free(l.c);
   }
+}
+
+void orc_adjoint_step(orc_sim* s, double dt, double nu, const double uinf[3], double* lam)
+{
+  const size_t nc = (size_t)s->nb * BS * BS * BS;
+  double* save = (double*)malloc(4 * nc * sizeof(double));
+  const int keep[4] = {ORC_U, ORC_V, ORC_W, ORC_P};
+  for (int q = 0; q < 4; q++) orc_get_field(s, keep[q], save + q * nc);
+  for (int q = 0; q < 3; q++) orc_set_field(s, ORC_U + q, lam + q * nc);
+  const int sphere = s->has_sphere;
+  s->has_sphere = 0;
+  orc_pressure_rhs(s, dt);
+  orc_poisson_solve(s);
+  orc_grad_p(s, dt);
+  s->has_sphere = sphere;
+  for (int q = 0; q < 3; q++) orc_get_field(s, ORC_U + q, lam + q * nc);  /* lam* */
+  for (int q = 0; q < 4; q++) orc_set_field(s, keep[q], save + q * nc);
+  free(save);
+  adj_advect(s, dt, nu, uinf, lam);
+}
+
+/* Adjoint of one step with the sphere (forced, velocity V, no rotation), central advection
+ * with Euler, no forcing. No reference counterpart. The step is
+ *   u* = A(u_n); r = M(R u*) (PressureRHS.cpp:78-112 masking and :322-343 perimeter
+ *   correction); u' = u* - (dt/2h) grad_c L^{-1} r; u_{n+1} = u' + dt pf (V - u')
+ *   (Penalization.cpp:40-98), with the step's drag F_x = sum h^3 pf (V_x - u'_x) (:93-96).
+ * With J = sum_n w_n F_x^n + g(u_N): lam holds dJ/du_{n+1} on entry and dJ/du_n on exit, and
+ * dJdV accumulates dJ/dV of this step (wdrag = w_n). The masking's sign pattern (which
+ * surface cells the correction scales) is held at the forward state; the transposed
+ * projection is the forward one with M'^T applied to the pressure instead of M to the RHS. */
+void orc_adjoint_step_body(orc_sim* s, double dt, double nu, const double uinf[3], double lambda, int implicit,
+                           const double V[3], double wdrag, double* lam, double dJdV[3])
+{
+  if (!s->has_sphere) { orc_adjoint_step(s, dt, nu, uinf, lam); return; }
+  const size_t nc = (size_t)s->nb * BS * BS * BS;
+  const double h = s->h, h3 = h * h * h;
+  /* forward state u_n kept aside; u* = A(u_n) for the masking's linearisation point */
+  double* save = (double*)malloc(4 * nc * sizeof(double));
+  const int keep[4] = {ORC_U, ORC_V, ORC_W, ORC_P};
+  for (int q = 0; q < 4; q++) orc_get_field(s, keep[q], save + q * nc);
+  orc_advect_diffuse(s, dt, nu, uinf);
+  /* masked RHS of u* on obstacle blocks, as orc_pressure_rhs forms it: per cell the flags
+   * (masked, surface) and rhs_new; per block the sums S, pos, neg (block order) */
+  double* rnew = (double*)calloc(nc, sizeof(double));
+  unsigned char* flag = (unsigned char*)calloc(nc, 1);  /* bit 0 masked, bit 1 surface */
+  double* part = (double*)calloc(3 * (size_t)s->nb, sizeof(double));
+  const double fac = 0.5 * h * h / dt;
+  #pragma omp parallel
+  {
+    Lab l = lab_alloc(1);
+    #pragma omp for schedule(static)
+    for (int b = 0; b < s->nb; b++) {
+      const OBlock* o = s->ob[b];
+      if (!o) continue;
+      lab_load(s, b, &l);
+      double S = 0, pos = 0, neg = 0;
+      for (int iz = 0; iz < BS; iz++)
+        for (int iy = 0; iy < BS; iy++)
+          for (int ix = 0; ix < BS; ix++) {
+            const size_t c = (size_t)b * BS * BS * BS + (iz * BS + iy) * BS + ix;
+            const FE *LW = LAB(&l, ix - 1, iy, iz), *LE = LAB(&l, ix + 1, iy, iz);
+            const FE *LS = LAB(&l, ix, iy - 1, iz), *LN = LAB(&l, ix, iy + 1, iz);
+            const FE *LF = LAB(&l, ix, iy, iz - 1), *LB = LAB(&l, ix, iy, iz + 1);
+            if (LAB(&l, ix, iy, iz)->chi > o->chi[iz][iy][ix]) continue;
+            const double r0 = fac * (LE->u - LW->u + LN->v - LS->v + LB->w - LF->w);
+            const double dXx = LE->chi - LW->chi, dXy = LN->chi - LS->chi, dXz = LB->chi - LF->chi;
+            const int surf = dXx * dXx + dXy * dXy + dXz * dXz > 0;
+            const double srcBulk = -o->chi[iz][iy][ix] * r0;
+            S += srcBulk;
+            const double r1 = r0 + srcBulk;
+            rnew[c] = r1;
+            flag[c] = 1 | (surf ? 2 : 0);
+            pos += surf * r1 * (r1 > 0);
+            neg -= surf * r1 * (r1 < 0);
+          }
+      part[3 * b] = S;
+      part[3 * b + 1] = pos;
+      part[3 * b + 2] = neg;
+    }
+    free(l.c);
+  }
+  double S = 0, pos = 0, neg = 0;
+  for (int b = 0; b < s->nb; b++) {
+    S += part[3 * b];
+    pos += part[3 * b + 1];
+    neg += part[3 * b + 2];
+  }
+  free(part);
+  const double denom = S > 0 ? pos : neg;
+  const double D = denom > EPS ? denom : EPS;
+  const double corr = S / D;
+  const double sg = corr > 0 ? 1.0 : (corr < 0 ? -1.0 : 0.0);
+  for (int q = 0; q < 4; q++) orc_set_field(s, keep[q], save + q * nc);  /* u_n back */
+  /* penalization and drag: lam' = (1 - dt pf) lam - wdrag h^3 pf e_x on penalized cells */
+  const double lf = implicit ? lambda : 1.0 / dt;
+  double* pv = (double*)calloc(3 * (size_t)s->nb, sizeof(double));
+  #pragma omp parallel for schedule(static)
+  for (int b = 0; b < s->nb; b++) {
+    const OBlock* o = s->ob[b];
+    if (!o) continue;
+    double a[3] = {0, 0, 0};
+    for (int iz = 0; iz < BS; iz++)
+      for (int iy = 0; iy < BS; iy++)
+        for (int ix = 0; ix < BS; ix++) {
+          const double X = o->chi[iz][iy][ix];
+          if (s->blk[b].d[iz][iy][ix].chi > X || X <= 0) continue;
+          const double pf = implicit ? X * lf / (1 + lf * dt * X) : X * lf;
+          const size_t c = (size_t)b * BS * BS * BS + (iz * BS + iy) * BS + ix;
+          for (int d = 0; d < 3; d++) {
+            a[d] += dt * pf * lam[d * nc + c] + (d == 0 ? wdrag * h3 * pf : 0.0);
+            lam[d * nc + c] = (1 - dt * pf) * lam[d * nc + c] - (d == 0 ? wdrag * h3 * pf : 0.0);
+          }
+        }
+    for (int d = 0; d < 3; d++) pv[3 * b + d] = a[d];
+  }
+  for (int b = 0; b < s->nb; b++)
+    for (int d = 0; d < 3; d++) dJdV[d] += pv[3 * b + d];
+  free(pv);
+  (void)V;
+  /* transposed projection: y = L^{-1} R lam' (plain RHS), q = M'^T y, lam* = lam' - (dt/2h) grad_c q */
+  for (int q = 0; q < 3; q++) orc_set_field(s, ORC_U + q, lam + q * nc);
+  const int sphere = s->has_sphere;
+  s->has_sphere = 0;
+  orc_pressure_rhs(s, dt);
+  orc_poisson_solve(s);
+  double* y = (double*)malloc(nc * sizeof(double));
+  orc_get_field(s, ORC_P, y);
+  double g = 0;
+  for (size_t c = 0; c < nc; c++)
+    if ((flag[c] & 2) && ((rnew[c] > 0 && corr > 0) || (rnew[c] < 0 && corr < 0))) g += y[c] * (-sg * rnew[c]);
+  const double aS = g / D, aD = denom > EPS ? -g * corr / D : 0.0;
+  #pragma omp parallel for schedule(static)
+  for (int b = 0; b < s->nb; b++) {
+    const OBlock* o = s->ob[b];
+    if (!o) continue;
+    for (int i = 0; i < BS * BS * BS; i++) {
+      const size_t c = (size_t)b * BS * BS * BS + i;
+      if (!(flag[c] & 1)) continue;
+      const double X = (&o->chi[0][0][0])[i];
+      const int inQ = (flag[c] & 2) && ((rnew[c] > 0 && corr > 0) || (rnew[c] < 0 && corr < 0));
+      const double an = inQ ? y[c] * (1 - sg * corr) + aD * sg : y[c];
+      y[c] = (1 - X) * an - X * aS;
+    }
+  }
+  orc_set_field(s, ORC_P, y);
+  orc_grad_p(s, dt);
+  s->has_sphere = sphere;
+  for (int q = 0; q < 3; q++) orc_get_field(s, ORC_U + q, lam + q * nc);  /* lam* */
+  for (int q = 0; q < 4; q++) orc_set_field(s, keep[q], save + q * nc);
+  free(save); free(y); free(rnew); free(flag);
+  adj_advect(s, dt, nu, uinf, lam);
 }
 
 /* ---------------------------------------------------------------- timestep */

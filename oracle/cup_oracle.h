@@ -97,6 +97,12 @@ void orc_timestep(orc_sim* s, double dt, const orc_params* prm, double forcing_s
  * dJ/du_n on exit. Overwrites tmp; u, v, w, p are kept. No reference counterpart. */
 void orc_adjoint_step(orc_sim* s, double dt, double nu, const double uinf[3], double* lam);
 
+/* Adjoint of one step with the sphere (forced at velocity V): lam as orc_adjoint_step,
+ * J = sum_n wdrag_n F_x^n + g(u_N) with F the step's penalization force; dJdV accumulates
+ * (+=) this step's dJ/dV. The masking's sign pattern is held at the forward state. */
+void orc_adjoint_step_body(orc_sim* s, double dt, double nu, const double uinf[3], double lambda, int implicit,
+                           const double V[3], double wdrag, double* lam, double dJdV[3]);
+
 /* Plain 3-D complex FFT, unnormalized, sign -1 forward / +1 backward; arrays [x][y][z]. */
 void orc_fft3d(int nx, int ny, int nz, double* re, double* im, int sign);
 

@@ -53,6 +53,7 @@ def lib():
         "orc_advect_diffuse": (None, [p, d, d, dp]),
         "orc_advect_diffuse_scheme": (None, [p, d, d, dp, i]),
         "orc_adjoint_step": (None, [p, d, d, dp, dp]),
+        "orc_adjoint_step_body": (None, [p, d, d, dp, d, i, dp, d, dp, dp]),
         "orc_pressure_rhs": (None, [p, d]),
         "orc_poisson_solve": (None, [p]),
         "orc_grad_p": (None, [p, d]),
@@ -183,6 +184,16 @@ class OracleSim:
         lib().orc_adjoint_step(self._h, float(dt), float(nu), _dp(_vec(uinf)), _dp(buf))
         n = buf.size // 3
         return tuple(from_blocks(buf[q * n:(q + 1) * n], self.shape) for q in range(3))
+
+    def adjoint_step_body(self, dt, nu, lam, vel, lambda_, implicit, wdrag, uinf=(0, 0, 0)):
+        """With the sphere: (dJ/du_n, dJ/dV of this step) from lam = dJ/du_{n+1}, J containing
+        wdrag * F_x of this step (F the penalization force); u_n is this sim's velocity."""
+        buf = np.concatenate([to_blocks(np.asarray(f, dtype=np.float64)) for f in lam])
+        dv = np.zeros(3)
+        lib().orc_adjoint_step_body(self._h, float(dt), float(nu), _dp(_vec(uinf)), float(lambda_),
+                                    int(bool(implicit)), _dp(_vec(vel)), float(wdrag), _dp(buf), _dp(dv))
+        n = buf.size // 3
+        return tuple(from_blocks(buf[q * n:(q + 1) * n], self.shape) for q in range(3)), dv
 
     def pressure_rhs(self, dt):
         lib().orc_pressure_rhs(self._h, float(dt))

@@ -60,3 +60,41 @@ def test_adjoint_of_pure_projection_is_symmetric(oracle):
     assert abs(dot(pa, b) - dot(a, pb)) < 1e-12 * abs(dot(pa, b))
     ppa = proj(pa)
     assert max(np.max(np.abs(x - y)) for x, y in zip(ppa, pa)) < 1e-12 * max(np.max(np.abs(x)) for x in pa)
+
+
+def fwd_step_body(oracle, bpd, u, dt, nu, U, V, lam_, ext):
+    s = oracle.OracleSim(bpd, maxextent=ext)
+    s.sphere_create(0.3, (0.9, 0.5, 0.5))
+    s.set_velocity(*u)
+    out = s.timestep(dt, nu, uinf=U, sphere=dict(lambda_=lam_, implicit=True, trans_vel=V))
+    return s.velocity(), out["penal_ft"][0]
+
+
+@pytest.mark.parametrize("lam_", [1e4, 1e6])
+def test_adjoint_with_sphere_dot_product_identity(oracle, lam_):
+    """Step with the penalized sphere, J = <lam, u_{n+1}> + w F_x: the adjoint's dJ/du_n and
+    dJ/dV against central differences of the forward oracle step (the masking's correction
+    makes the step rational in u: eps = 1e-5, agreement 1e-7)."""
+    rng = np.random.default_rng(41)
+    bpd, ext = (4, 2, 2), 2.0
+    s = oracle.OracleSim(bpd, maxextent=ext)
+    s.sphere_create(0.3, (0.9, 0.5, 0.5))
+    shape = s.shape
+    U, V = (1.0, 0.0, 0.0), (-1.0, 0.1, 0.0)
+    u = tuple(0.3 * rng.standard_normal(shape) + (1.0 if q == 0 else 0.0) for q in range(3))
+    d = tuple(rng.standard_normal(shape) for _ in range(3))
+    lam = tuple(rng.standard_normal(shape) for _ in range(3))
+    dt, nu, w, eps = 0.2 * s.h, 1e-3, 3.7, 1e-5
+
+    def J(uu, VV):
+        u1, F = fwd_step_body(oracle, bpd, uu, dt, nu, U, VV, lam_, ext)
+        return dot(lam, u1) + w * F
+    s.set_velocity(*u)
+    lam0, dJdV = s.adjoint_step_body(dt, nu, lam, V, lam_, True, w, U)
+    fd = (J(tuple(a + eps * b for a, b in zip(u, d)), V) - J(tuple(a - eps * b for a, b in zip(u, d)), V)) / (2 * eps)
+    ad = dot(d, lam0)
+    assert abs(fd - ad) < 1e-7 * abs(fd)
+    for k in range(3):
+        e = np.eye(3)[k] * eps
+        fdv = (J(u, tuple(np.add(V, e))) - J(u, tuple(np.subtract(V, e)))) / (2 * eps)
+        assert abs(fdv - dJdV[k]) < 1e-7 * max(abs(fdv), 1e-3)
