@@ -62,6 +62,7 @@ SIGNATURES = {
     "cup_poisson_periodic": (_i, [_p]),
     "cup_pressure_project": (_i, [_p, _d]),
     "cup_chi_from_sphere": (_i, [_p, _dp, _d, _dp]),
+    "cup_chi_from_sphere_mollified": (_i, [_p, _dp, _d, _d, _dp]),
     "cup_penalize": (_i, [_p, _d, _d, _i, _dp, _dp, _dp]),
     "cup_surface_forces": (_i, [_p, _d, _dp, _dp, _dp]),
     "cup_diagnostics": (_i, [_p, _d, _dp]),
@@ -72,7 +73,7 @@ SIGNATURES = {
     "cup_step_state": (_i, [_p, _ip, _dp, _dp, _dp, _dp]),
     "cup_set_step_state": (_i, [_p, _i, _d]),
     "cup_adjoint_step": (_i, [_p, _d, _d, _dp, _i]),
-    "cup_adjoint_step_body": (_i, [_p, _d, _d, _dp, _d, _i, _dp, _d, _dp]),
+    "cup_adjoint_step_body": (_i, [_p, _d, _d, _dp, _d, _i, _dp, _d, _dp, _dp]),
     "cup_last_body_force": (_i, [_p, _dp]),
     "cup_stream": (_i, [_p, C.POINTER(C.c_void_p)]),
     "cup_stage_timing_begin": (_i, [_p, _i, _i]),

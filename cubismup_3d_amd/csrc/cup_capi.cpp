@@ -429,7 +429,22 @@ int cup_chi_from_sphere(cup_ctx* ctx, const double center[3], double radius, dou
 {
   CUP_GUARD(ctx);
   if (!center || !(radius > 0)) return set_error(c, CUP_E_INVALID, "chi_from_sphere: radius must be > 0");
-  if (int rc = body_create(c, center, radius)) return rc;
+  if (int rc = body_create(c, center, radius, 0.0)) return rc;
+  if (com_mass_out) {
+    for (int a = 0; a < 3; ++a) com_mass_out[a] = c.body.com[a];
+    com_mass_out[3] = c.body.mass;
+  }
+  return CUP_OK;
+}
+
+int cup_chi_from_sphere_mollified(cup_ctx* ctx, const double center[3], double radius, double width,
+                                  double com_mass_out[4])
+{
+  CUP_GUARD(ctx);
+  if (!center || !(radius > 0)) return set_error(c, CUP_E_INVALID, "chi_from_sphere_mollified: radius must be > 0");
+  if (!(width > 0 && width <= 2))
+    return set_error(c, CUP_E_INVALID, "chi_from_sphere_mollified: the half-width must lie in (0, 2] cells");
+  if (int rc = body_create(c, center, radius, width)) return rc;
   if (com_mass_out) {
     for (int a = 0; a < 3; ++a) com_mass_out[a] = c.body.com[a];
     com_mass_out[3] = c.body.mass;
@@ -614,19 +629,38 @@ int cup_adjoint_step(cup_ctx* ctx, double dt, double nu, const double uinf[3], i
 }
 
 int cup_adjoint_step_body(cup_ctx* ctx, double dt, double nu, const double uinf[3], double lambda, int implicit,
-                          const double body_vel[3], double wdrag, double dJdV[3])
+                          const double body_vel[3], double wdrag, double dJdV[3], double dJdc[3])
 {
   CUP_GUARD(ctx);
   if (!uinf || !body_vel || !dJdV || !(dt > 0)) return set_error(c, CUP_E_INVALID, "adjoint_step_body: bad argument");
   if (!c.body.present) return set_error(c, CUP_E_STATE, "adjoint_step_body: no obstacle");
+  if (dJdc && !(c.body.moll > 0))
+    return set_error(c, CUP_E_STATE,
+                     "adjoint_step_body: the centre gradient needs the mollified chi (cup_chi_from_sphere_mollified)");
   for (int f = CUP_ADJ_U; f <= CUP_ADJ_W; ++f)
     if (int rc = field_ready(c, f, "adjoint_step_body")) return rc;
+  AdjBodyArgs a;
+  a.dt = dt;
+  a.lambda = lambda;
+  a.implicit = implicit;
+  a.wdrag = wdrag;
+  for (int d = 0; d < 3; ++d) a.vel[d] = body_vel[d];
+  a.centre = dJdc != nullptr;
   if (int rc = launch_set_dt(c, dt)) return rc;
-  // forward state at this step: u* = A(u_n) in tmp, and its masked RHS on the obstacle blocks
+  // forward state at this step: u* = A(u_n) in tmp
   if (int rc = launch_advdiff(c, nu, uinf, CUP_CENTRAL_EULER)) return rc;
-  if (int rc = launch_adjoint_body(c, 0, lambda, implicit, wdrag)) return rc;
+  if (a.centre) {
+    // the penalization's input P_M(u*) needs the forward pressure: masked and corrected RHS,
+    // Poisson solve into P; then the chi sensitivity of the penalization and its drag
+    if (int rc = launch_adjoint_body(c, ADJB_FWD_MASK, a)) return rc;
+    if (int rc = launch_adjoint_body(c, ADJB_FWD_CORRECT, a)) return rc;
+    if (int rc = launch_poisson_fused(c, false, 0.0, 0.0)) return rc;
+    if (int rc = launch_adjoint_body(c, ADJB_CHI_PEN, a)) return rc;
+  }
+  // the masked RHS of u* on the obstacle blocks (the masking's linearisation point)
+  if (int rc = launch_adjoint_body(c, ADJB_FWD_MASK, a)) return rc;
   // penalization and the drag's weight; dJ/dV of this step
-  if (int rc = launch_adjoint_body(c, 1, lambda, implicit, wdrag)) return rc;
+  if (int rc = launch_adjoint_body(c, ADJB_PENALIZE, a)) return rc;
   // transposed projection: y = L^{-1} R lam' (no obstacle RHS), y <- M'^T y, lam* = lam' - (dt/2h) grad y
   c.ustar = CUP_ADJ_U;
   c.plain_rhs = true;
@@ -634,12 +668,14 @@ int cup_adjoint_step_body(cup_ctx* ctx, double dt, double nu, const double uinf[
   c.plain_rhs = false;
   c.ustar = CUP_TMPU;
   if (rc) return rc;
-  if (int e = launch_adjoint_body(c, 2, lambda, implicit, wdrag)) return e;
+  if (int e = launch_adjoint_body(c, ADJB_MASK, a)) return e;
   if (int e = launch_gradp_inplace(c, CUP_ADJ_U)) return e;
   if (int e = launch_adjoint_advdiff(c, nu, uinf)) return e;
   for (int q = 0; q < 3; ++q) std::swap(c.f[CUP_TMPU + q], c.f[CUP_ADJ_U + q]);
   if (int e = read_scalars(c)) return e;
   for (int d = 0; d < 3; ++d) dJdV[d] = c.sc_host->red[8 + d];
+  if (dJdc)
+    for (int d = 0; d < 3; ++d) dJdc[d] = c.sc_host->red[11 + d] + c.sc_host->red[14 + d];
   return CUP_OK;
 }
 

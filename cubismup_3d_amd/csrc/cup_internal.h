@@ -75,6 +75,7 @@ struct GridArgs {
 struct Body {
   bool present = false;
   double radius = 0, center[3] = {0, 0, 0};
+  double moll = 0;              // half-width in cells of the mollified chi; 0: Towers' chi
   double com[3] = {0, 0, 0}, mass = 0;
   int nob = 0;                  // obstacle blocks
   int* ob_ids = nullptr;        // device: block id of each obstacle block
@@ -84,6 +85,7 @@ struct Body {
   double* ob_surf = nullptr;    // device: nob*512*4 (delta, dchidx, dchidy, dchidz)
   void* ob_rhs = nullptr;       // device: nob*512 corrected RHS (context precision)
   unsigned char* ob_touch = nullptr;  // device: nob*512 surface flag of the RHS correction
+  void* ob_r0 = nullptr;        // device: nob*512 unmasked RHS (adjoint with the centre; lazy)
 };
 
 struct Ctx {
@@ -178,7 +180,20 @@ int launch_copy_tmp_to_u(Ctx& c);
 int launch_copy_u_to_tmp(Ctx& c);
 int launch_adjoint_advdiff(Ctx& c, double nu, const double uinf[3]);
 int launch_gradp_inplace(Ctx& c, int field0);
-int launch_adjoint_body(Ctx& c, int phase, double lambda, int implicit, double wdrag);
+// The adjoint step with the sphere, in phases (cup_adjoint_step_body orders them).
+enum AdjBodyPhase {
+  ADJB_FWD_MASK,     // forward masking of u*: r_new in ob_rhs, flags, sums (and r0 with the centre)
+  ADJB_FWD_CORRECT,  // the forward perimeter correction of ob_rhs (for the forward re-solve)
+  ADJB_CHI_PEN,      // chi sensitivity of the penalization and its drag, contracted with dchi/dc
+  ADJB_PENALIZE,     // penalization and drag transpose on the adjoint; dJ/dV
+  ADJB_MASK          // transposed masking on the adjoint pressure (and its chi sensitivity)
+};
+struct AdjBodyArgs {
+  double dt = 0, lambda = 0, wdrag = 0, vel[3] = {0, 0, 0};
+  int implicit = 1;
+  bool centre = false;
+};
+int launch_adjoint_body(Ctx& c, AdjBodyPhase phase, const AdjBodyArgs& a);
 int launch_rhs_plain(Ctx& c);                                   // -> rbuf
 int launch_gradp_plain(Ctx& c);                                 // u += -dt/2h grad p
 int launch_gradp_fused(Ctx& c, const double uinf[3], const int* skip_slot, bool force = false);  // u = tmp - dt/2h grad p; max, ke_part
@@ -204,7 +219,7 @@ int forcing_plain(Ctx& c, double dt, double sim_dt, double nu, double tke_target
 int launch_forcing_reduce(Ctx& c, double tke_target, double inj_rate);
 int launch_forcing_rows(Ctx& c);
 // cup_obstacle.hip
-int body_create(Ctx& c, const double center[3], double radius);
+int body_create(Ctx& c, const double center[3], double radius, double moll);
 void body_free(Ctx& c);
 int launch_penalize(Ctx& c, double lambda, int implicit, const double vel[3], const double omega[3]);
 int launch_penalize_step(Ctx& c, double lambda, int implicit, const double vel[3], const double omega[3],

@@ -71,14 +71,39 @@ __device__ __forceinline__ double sdf_at(const GridArgs& g, const int* __restric
   return sdf[(size_t)s * BS3 + (((gz & 7) * BS + (gy & 7)) * BS + (gx & 7))];
 }
 
+// Mollified Heaviside of the signed distance d with half-width e (no reference counterpart):
+// 0 for d <= -e, 1 for d >= e, (1 + d/e + sin(pi d/e)/pi)/2 between. It is C1, which makes chi
+// differentiable in the sphere centre; dH/dd = (1 + cos(pi d/e))/(2e) vanishes at d = +-e.
+__device__ __forceinline__ double moll_H(double d, double e)
+{
+  if (d <= -e) return 0.0;
+  if (d >= e) return 1.0;
+  return 0.5 * (1.0 + d / e + sin(M_PI * d / e) / M_PI);
+}
+
+// d chi / d centre at the cell (gx, gy, gz) for the mollified chi H(R - |x - c|):
+// dH (x - c)/|x - c|.
+__device__ __forceinline__ void moll_dchi_dc(const GridArgs& g, int gx, int gy, int gz, double cx, double cy,
+                                             double cz, double R, double e, double out[3])
+{
+  const double rx = (gx + 0.5) * g.h - cx, ry = (gy + 0.5) * g.h - cy, rz = (gz + 0.5) * g.h - cz;
+  const double dist = sqrt(rx * rx + ry * ry + rz * rz);
+  const double d = R - dist;
+  const double dH = (dist > 0 && d > -e && d < e) ? (1.0 + cos(M_PI * d / e)) / (2.0 * e) / dist : 0.0;
+  out[0] = dH * rx;
+  out[1] = dH * ry;
+  out[2] = dH * rz;
+}
+
 // KernelCharacteristicFunction (ObstaclesCreate.cpp:51-137) on one obstacle block:
 // chi by Towers' formula inside the +-2h band, grid chi = max, CoM and mass partials,
 // surface points (delta, dchi) where Delta > eps (ObstacleBlock::write, ObstacleBlock.h:157-166).
+// With moll_eps > 0 chi is instead moll_H(sdf, moll_eps); the surface points are unchanged.
 template <typename T>
 __global__ __launch_bounds__(NT) void k_sphere_chi(const int* __restrict__ ob_ids, const int* __restrict__ slot,
                                                    const double* __restrict__ sdf, double* __restrict__ ob_chi,
                                                    double* __restrict__ surf, T* __restrict__ chi, GridArgs g,
-                                                   double* __restrict__ part)
+                                                   double* __restrict__ part, double moll_eps)
 {
   const int s = blockIdx.x, b = ob_ids[s];
   int bx, by, bz;
@@ -93,7 +118,9 @@ __global__ __launch_bounds__(NT) void k_sphere_chi(const int* __restrict__ ob_id
     const double dPy = sdf_at(g, slot, sdf, gx, gy + 1, gz), dMy = sdf_at(g, slot, sdf, gx, gy - 1, gz);
     const double dPz = sdf_at(g, slot, sdf, gx, gy, gz + 1), dMz = sdf_at(g, slot, sdf, gx, gy, gz - 1);
     double X;
-    if (sdf0 > +2 * h || sdf0 < -2 * h) {
+    if (moll_eps > 0) {
+      X = moll_H(sdf0, moll_eps);
+    } else if (sdf0 > +2 * h || sdf0 < -2 * h) {
       X = sdf0 > 0 ? 1 : 0;
     } else {
       const double gUX = inv2h * (dPx - dMx), gUY = inv2h * (dPy - dMy), gUZ = inv2h * (dPz - dMz);
@@ -148,6 +175,7 @@ void body_free(Ctx& c)
   if (B.ob_surf) hipFree(B.ob_surf);
   if (B.ob_rhs) hipFree(B.ob_rhs);
   if (B.ob_touch) hipFree(B.ob_touch);
+  if (B.ob_r0) hipFree(B.ob_r0);
   B = Body();
 }
 
@@ -162,17 +190,19 @@ static int body_chi_t(Ctx& c)
   hipLaunchKernelGGL(k_sphere_sdf, dim3(B.nob), dim3(NT), 0, c.stream, B.ob_ids, B.ob_sdf, c.grid(), B.center[0],
                      B.center[1], B.center[2], B.radius);
   hipLaunchKernelGGL((k_sphere_chi<T>), dim3(B.nob), dim3(NT), 0, c.stream, B.ob_ids, B.ob_slot, B.ob_sdf, B.ob_chi,
-                     B.ob_surf, (T*)c.f[CUP_CHI], c.grid(), c.partials);
+                     B.ob_surf, (T*)c.f[CUP_CHI], c.grid(), c.partials, B.moll * c.h);
   return check_hip(c, hipGetLastError(), "k_sphere_chi");
 }
 
-// CreateObstacles for one static sphere (ObstaclesCreate.cpp:444-540).
-int body_create(Ctx& c, const double center[3], double radius)
+// CreateObstacles for one static sphere (ObstaclesCreate.cpp:444-540); moll > 0 selects the
+// mollified chi of that half-width in cells.
+int body_create(Ctx& c, const double center[3], double radius, double moll)
 {
   hipStreamSynchronize(c.stream);
   body_free(c);
   Body& B = c.body;
   B.radius = radius;
+  B.moll = moll;
   for (int a = 0; a < 3; ++a) B.center[a] = center[a];
   // Blocks whose cell-centre box strictly intersects the sphere's box of half-width
   // R + (2+SURFDH) h (Sphere.cpp:21-42, Obstacle.h:186-207).
@@ -236,7 +266,8 @@ __global__ __launch_bounds__(NT) void k_rhs_obs_mask(const int* __restrict__ ob_
                                                      const T* __restrict__ v, const T* __restrict__ w,
                                                      T* __restrict__ rhs, T* __restrict__ ob_rhs,
                                                      unsigned char* __restrict__ touch, GridArgs g,
-                                                     const DevScalars* __restrict__ sc, double* __restrict__ part)
+                                                     const DevScalars* __restrict__ sc, double* __restrict__ part,
+                                                     T* __restrict__ r0_out)
 {
   const int s = blockIdx.x, b = ob_ids[s];
   int bx, by, bz;
@@ -256,6 +287,7 @@ __global__ __launch_bounds__(NT) void k_rhs_obs_mask(const int* __restrict__ ob_
                  w[cell_index(g, gx, gy, wrapi(gz + 1, g.Nz))] - w[cell_index(g, gx, gy, wrapi(gz - 1, g.Nz))]);
     }
     const double X = ob_chi[(size_t)s * BS3 + c];
+    if (r0_out) r0_out[(size_t)s * BS3 + c] = r;
     unsigned char t = 0;
     if (!(chi[o] > (T)X)) {  // in the grid's precision: the grid chi holds the body's chi rounded to T
       const double dXx = (double)chi[cell_index(g, wrapi(gx + 1, g.Nx), gy, gz)] - chi[cell_index(g, wrapi(gx - 1, g.Nx), gy, gz)];
@@ -319,7 +351,8 @@ static int rhs_obs_t(Ctx& c, void* rhs_blocks)
   const int s0 = c.decomposition == CUP_FUSED ? c.ustar : CUP_U;
   hipLaunchKernelGGL((k_rhs_obs_mask<T>), dim3(B.nob), dim3(NT), 0, c.stream, B.ob_ids, B.ob_chi,
                      (const T*)c.f[CUP_CHI], (const T*)c.f[s0], (const T*)c.f[s0 + 1], (const T*)c.f[s0 + 2],
-                     (T*)rhs_blocks, (T*)B.ob_rhs, B.ob_touch, c.grid(), c.sc, c.partials + 2 * (size_t)c.nb);
+                     (T*)rhs_blocks, (T*)B.ob_rhs, B.ob_touch, c.grid(), c.sc, c.partials + 2 * (size_t)c.nb,
+                     (T*)nullptr);
   hipLaunchKernelGGL((k_rhs_obs_correct<T>), dim3(B.nob), dim3(NT), 0, c.stream, B.ob_ids, B.nob, B.ob_touch,
                      (T*)rhs_blocks, (T*)B.ob_rhs, (const double*)(c.partials + 2 * (size_t)c.nb));
   return check_hip(c, hipGetLastError(), "rhs_obstacle");
@@ -478,13 +511,19 @@ __global__ __launch_bounds__(NT) void k_adj_mask_g(const int* __restrict__ ob_id
 // y <- M'^T y on the masked cells of the obstacle blocks: with a_S = g/D, a_D = -g corr/D,
 // the adjoint of r_new is y (1 - sg corr) + a_D sg on corrected cells and y elsewhere, and
 // that of the unmasked RHS is (1 - X) adj(r_new) - X a_S.
+// With cpart (the mollified chi): r_new = (1 - X) r0 and S = -sum X r0 give the chi
+// sensitivity -r0 (adj(r_new) + a_S), times yscale = dt^2/h^3 because y is the RHS adjoint
+// scaled by the plain RHS operator's h^2/(2dt) in place of the gradient step's dt/(2h);
+// contracted with dchi/dc into cpart[3s..3s+2].
 template <typename T>
 __global__ __launch_bounds__(NT) void k_adj_mask_apply(const int* __restrict__ ob_ids,
                                                        const double* __restrict__ ob_chi, const T* __restrict__ chi,
                                                        const T* __restrict__ rnew,
                                                        const unsigned char* __restrict__ touch, T* __restrict__ y,
                                                        const double* __restrict__ msum, int nob,
-                                                       const double* __restrict__ gpart)
+                                                       const double* __restrict__ gpart, GridArgs g,
+                                                       const T* __restrict__ r0, double yscale, double cx, double cy,
+                                                       double cz, double R, double e, double* __restrict__ cpart)
 {
   double S, D, corr;
   bool live;
@@ -502,6 +541,9 @@ __global__ __launch_bounds__(NT) void k_adj_mask_apply(const int* __restrict__ o
   const double sg = corr > 0 ? 1.0 : (corr < 0 ? -1.0 : 0.0);
   const double aS = gtot / D, aD = live ? -gtot * corr / D : 0.0;
   const int s = blockIdx.x, b = ob_ids[s];
+  int bx, by, bz;
+  bcoords(g, b, bx, by, bz);
+  double acc[3] = {0, 0, 0};
   for (int c = threadIdx.x; c < BS3; c += NT) {
     const size_t o = (size_t)b * BS3 + c, q = (size_t)s * BS3 + c;
     const double X = ob_chi[q];
@@ -510,41 +552,123 @@ __global__ __launch_bounds__(NT) void k_adj_mask_apply(const int* __restrict__ o
     const bool inQ = touch[q] && ((r > 0 && corr > 0) || (r < 0 && corr < 0));
     const double an = inQ ? yy * (1 - sg * corr) + aD * sg : yy;
     y[o] = (T)((1 - X) * an - X * aS);
+    if (cpart) {
+      double dX[3];
+      moll_dchi_dc(g, bx * BS + (c & 7), by * BS + ((c >> 3) & 7), bz * BS + (c >> 6), cx, cy, cz, R, e, dX);
+      const double gX = -yscale * (double)r0[q] * (an + aS);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) acc[k] += gX * dX[k];
+    }
   }
+  if (cpart) breduce<3>(acc, cpart + 3 * (size_t)s);
+}
+
+// The chi sensitivity of the penalization and its drag, contracted with dchi/dc (the
+// mollified chi): per block sum dpf/dX (dt sum_d lam_d (V_d - u_in,d) + wdrag h^3 (V_x - u_in,x))
+// dchi/dc over penalized cells, lam the adjoint before the penalization's transpose and
+// u_in = u* + fac (p(+e_d) - p(-e_d)), fac = -dt/(2h), formed as the gradient kernel forms it.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_adj_chi_pen(const int* __restrict__ ob_ids, const double* __restrict__ ob_chi,
+                                                    const T* __restrict__ chi, const T* __restrict__ us,
+                                                    const T* __restrict__ vs, const T* __restrict__ ws,
+                                                    const T* __restrict__ p, const T* __restrict__ lu,
+                                                    const T* __restrict__ lv, const T* __restrict__ lw, GridArgs g,
+                                                    const DevScalars* __restrict__ sc, double lambda, int implicit,
+                                                    double wdrag, double vx, double vy, double vz, double cx,
+                                                    double cy, double cz, double R, double e,
+                                                    double* __restrict__ part)
+{
+  const int s = blockIdx.x, b = ob_ids[s];
+  int bx, by, bz;
+  bcoords(g, b, bx, by, bz);
+  const double dt = sc->dt, h3 = g.h * g.h * g.h;
+  const double lf = implicit ? lambda : 1.0 / dt;
+  const T fac = (T)(-0.5 * dt / g.h);
+  double acc[3] = {0, 0, 0};
+  for (int c = threadIdx.x; c < BS3; c += NT) {
+    const int gx = bx * BS + (c & 7), gy = by * BS + ((c >> 3) & 7), gz = bz * BS + (c >> 6);
+    const size_t o = (size_t)b * BS3 + c;
+    const double X = ob_chi[(size_t)s * BS3 + c];
+    if (chi[o] > (T)X || !(X > 0)) continue;
+    const T ui = us[o] + fac * (p[cell_index(g, wrapi(gx + 1, g.Nx), gy, gz)] - p[cell_index(g, wrapi(gx - 1, g.Nx), gy, gz)]);
+    const T vi = vs[o] + fac * (p[cell_index(g, gx, wrapi(gy + 1, g.Ny), gz)] - p[cell_index(g, gx, wrapi(gy - 1, g.Ny), gz)]);
+    const T wi = ws[o] + fac * (p[cell_index(g, gx, gy, wrapi(gz + 1, g.Nz))] - p[cell_index(g, gx, gy, wrapi(gz - 1, g.Nz))]);
+    const double den = 1 + lf * dt * X;
+    const double dpf = implicit ? lf / (den * den) : lf;
+    const double gpen = wdrag * h3 * (vx - (double)ui) +
+                        dt * ((double)lu[o] * (vx - (double)ui) + (double)lv[o] * (vy - (double)vi) +
+                              (double)lw[o] * (vz - (double)wi));
+    double dX[3];
+    moll_dchi_dc(g, gx, gy, gz, cx, cy, cz, R, e, dX);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) acc[k] += dpf * gpen * dX[k];
+  }
+  breduce<3>(acc, part + 3 * (size_t)s);
 }
 
 template <typename T>
-static int adj_body_t(Ctx& c, int phase, double lambda, int implicit, double wdrag)
+static int adj_body_t(Ctx& c, AdjBodyPhase phase, const AdjBodyArgs& a)
 {
   Body& B = c.body;
   double* msum = c.partials + 2 * (size_t)c.nb;  // written by k_rhs_obs_mask
   double* gpart = msum + 3 * (size_t)B.nob;
   double* vpart = gpart + B.nob;
-  if (phase == 0) {  // the forward masking of u* (tmp): r_new in ob_rhs, touch flags, sums
-    const int s0 = c.ustar;
-    hipLaunchKernelGGL((k_rhs_obs_mask<T>), dim3(B.nob), dim3(NT), 0, c.stream, B.ob_ids, B.ob_chi,
-                       (const T*)c.f[CUP_CHI], (const T*)c.f[s0], (const T*)c.f[s0 + 1], (const T*)c.f[s0 + 2],
-                       (T*)nullptr, (T*)B.ob_rhs, B.ob_touch, c.grid(), c.sc, msum);
-  } else if (phase == 1) {  // penalization and drag, dJ/dV into sc->red[8..10]
-    hipLaunchKernelGGL((k_adj_penalize<T>), dim3(B.nob), dim3(NT), 0, c.stream, B.ob_ids, B.ob_chi,
-                       (const T*)c.f[CUP_CHI], (T*)c.f[CUP_ADJ_U], (T*)c.f[CUP_ADJ_V], (T*)c.f[CUP_ADJ_W], c.grid(),
-                       c.sc, lambda, implicit, wdrag, vpart);
-    hipLaunchKernelGGL(k_sum_rows, dim3(3), dim3(NT), 0, c.stream, (const double*)vpart, B.nob, 3, c.sc->red + 8);
-  } else {  // the masking's transpose on the adjoint pressure
-    hipLaunchKernelGGL((k_adj_mask_g<T>), dim3(B.nob), dim3(NT), 0, c.stream, B.ob_ids, B.ob_chi,
-                       (const T*)c.f[CUP_CHI], (const T*)B.ob_rhs, B.ob_touch, (const T*)c.f[CUP_P], msum, B.nob,
-                       gpart);
-    hipLaunchKernelGGL((k_adj_mask_apply<T>), dim3(B.nob), dim3(NT), 0, c.stream, B.ob_ids, B.ob_chi,
-                       (const T*)c.f[CUP_CHI], (const T*)B.ob_rhs, B.ob_touch, (T*)c.f[CUP_P], msum, B.nob, gpart);
+  double* cpart = vpart + 3 * (size_t)B.nob;
+  const double e = B.moll * c.h;
+  switch (phase) {
+    case ADJB_FWD_MASK: {  // the forward masking of u*: r_new in ob_rhs, touch flags, sums
+      if (a.centre && !B.ob_r0)
+        if (int rc = check_hip(c, hipMalloc(&B.ob_r0, (size_t)B.nob * BS3 * c.esz), "hipMalloc")) return rc;
+      const int s0 = c.ustar;
+      hipLaunchKernelGGL((k_rhs_obs_mask<T>), dim3(B.nob), dim3(NT), 0, c.stream, B.ob_ids, B.ob_chi,
+                         (const T*)c.f[CUP_CHI], (const T*)c.f[s0], (const T*)c.f[s0 + 1], (const T*)c.f[s0 + 2],
+                         (T*)nullptr, (T*)B.ob_rhs, B.ob_touch, c.grid(), c.sc, msum,
+                         a.centre ? (T*)B.ob_r0 : (T*)nullptr);
+      break;
+    }
+    case ADJB_FWD_CORRECT:  // r_new -> the corrected RHS the forward Poisson solve reads
+      hipLaunchKernelGGL((k_rhs_obs_correct<T>), dim3(B.nob), dim3(NT), 0, c.stream, B.ob_ids, B.nob, B.ob_touch,
+                         (T*)nullptr, (T*)B.ob_rhs, (const double*)msum);
+      break;
+    case ADJB_CHI_PEN:  // u* in c.ustar, the forward pressure in P, lam in the adjoint fields
+      hipLaunchKernelGGL((k_adj_chi_pen<T>), dim3(B.nob), dim3(NT), 0, c.stream, B.ob_ids, B.ob_chi,
+                         (const T*)c.f[CUP_CHI], (const T*)c.f[c.ustar], (const T*)c.f[c.ustar + 1],
+                         (const T*)c.f[c.ustar + 2], (const T*)c.f[CUP_P], (const T*)c.f[CUP_ADJ_U],
+                         (const T*)c.f[CUP_ADJ_V], (const T*)c.f[CUP_ADJ_W], c.grid(), c.sc, a.lambda, a.implicit,
+                         a.wdrag, a.vel[0], a.vel[1], a.vel[2], B.center[0], B.center[1], B.center[2], B.radius, e,
+                         cpart);
+      hipLaunchKernelGGL(k_sum_rows, dim3(3), dim3(NT), 0, c.stream, (const double*)cpart, B.nob, 3, c.sc->red + 11);
+      break;
+    case ADJB_PENALIZE:  // penalization and drag, dJ/dV into sc->red[8..10]
+      hipLaunchKernelGGL((k_adj_penalize<T>), dim3(B.nob), dim3(NT), 0, c.stream, B.ob_ids, B.ob_chi,
+                         (const T*)c.f[CUP_CHI], (T*)c.f[CUP_ADJ_U], (T*)c.f[CUP_ADJ_V], (T*)c.f[CUP_ADJ_W],
+                         c.grid(), c.sc, a.lambda, a.implicit, a.wdrag, vpart);
+      hipLaunchKernelGGL(k_sum_rows, dim3(3), dim3(NT), 0, c.stream, (const double*)vpart, B.nob, 3, c.sc->red + 8);
+      break;
+    case ADJB_MASK: {  // the masking's transpose on the adjoint pressure; with the centre, red[14..16]
+      const double yscale = a.dt * a.dt / (c.h * c.h * c.h);
+      hipLaunchKernelGGL((k_adj_mask_g<T>), dim3(B.nob), dim3(NT), 0, c.stream, B.ob_ids, B.ob_chi,
+                         (const T*)c.f[CUP_CHI], (const T*)B.ob_rhs, B.ob_touch, (const T*)c.f[CUP_P], msum, B.nob,
+                         gpart);
+      hipLaunchKernelGGL((k_adj_mask_apply<T>), dim3(B.nob), dim3(NT), 0, c.stream, B.ob_ids, B.ob_chi,
+                         (const T*)c.f[CUP_CHI], (const T*)B.ob_rhs, B.ob_touch, (T*)c.f[CUP_P], msum, B.nob, gpart,
+                         c.grid(), (const T*)B.ob_r0, yscale, B.center[0], B.center[1], B.center[2], B.radius, e,
+                         a.centre ? cpart : (double*)nullptr);
+      if (a.centre)
+        hipLaunchKernelGGL(k_sum_rows, dim3(3), dim3(NT), 0, c.stream, (const double*)cpart, B.nob, 3,
+                           c.sc->red + 14);
+      break;
+    }
   }
   return check_hip(c, hipGetLastError(), "adjoint body");
 }
 
-int launch_adjoint_body(Ctx& c, int phase, double lambda, int implicit, double wdrag)
+int launch_adjoint_body(Ctx& c, AdjBodyPhase phase, const AdjBodyArgs& a)
 {
   if (!c.body.present) return set_error(c, CUP_E_STATE, "adjoint: no obstacle");
-  return c.precision == 4 ? adj_body_t<float>(c, phase, lambda, implicit, wdrag)
-                          : adj_body_t<double>(c, phase, lambda, implicit, wdrag);
+  if (a.centre && !(c.body.moll > 0))
+    return set_error(c, CUP_E_STATE, "adjoint: the centre gradient needs the mollified chi");
+  return c.precision == 4 ? adj_body_t<float>(c, phase, a) : adj_body_t<double>(c, phase, a);
 }
 
 __global__ __launch_bounds__(NT) void k_sum_rows(const double* __restrict__ part, int n, int width,

@@ -150,14 +150,15 @@ class Grid:
         """adj <- dJ/du_n from adj = dJ/du_{n+1}, with u_n the current velocity."""
         self._c(L.lib().cup_adjoint_step(self._h, float(dt), float(nu), L.vec3(uinf), int(scheme)))
 
-    def adjoint_step_body(self, dt, nu, uinf, sphere, wdrag=0.0) -> np.ndarray:
+    def adjoint_step_body(self, dt, nu, uinf, sphere, wdrag=0.0, centre=False):
         """With the sphere: adj <- dJ/du_n; returns this step's dJ/d(body velocity), J holding
-        wdrag times the step's drag (penalization force, x)."""
-        out = np.zeros(3)
+        wdrag times the step's drag (penalization force, x). centre=True (mollified chi) returns
+        (dJ/dV, dJ/d(sphere centre))."""
+        out, dc = np.zeros(3), np.zeros(3)
         self._c(L.lib().cup_adjoint_step_body(self._h, float(dt), float(nu), L.vec3(uinf), float(sphere["lambda_"]),
                                               int(bool(sphere["implicit"])), L.vec3(sphere["trans_vel"]),
-                                              float(wdrag), L.dptr(out)))
-        return out
+                                              float(wdrag), L.dptr(out), L.dptr(dc) if centre else None))
+        return (out, dc) if centre else out
 
     def last_body_force(self) -> np.ndarray:
         """Penalization force and torque of the last run_steps step with the sphere."""
@@ -214,9 +215,15 @@ class Grid:
                                              float(nu), float(tke_target), float(injection_rate), L.dptr(st)))
         return dict(tke=st[0], eps=st[1], tke_filtered=st[2], l_integral=st[3], fac=st[4])
 
-    def chi_from_sphere(self, center, radius):
+    def chi_from_sphere(self, center, radius, mollify=0.0):
+        """The sphere's chi by Towers' formula, or with mollify (a half-width in cells, in
+        (0, 2]) by the mollified Heaviside of its SDF, which is C1 in the centre."""
         out = np.zeros(4)
-        self._c(L.lib().cup_chi_from_sphere(self._h, L.vec3(center), float(radius), L.dptr(out)))
+        if mollify:
+            self._c(L.lib().cup_chi_from_sphere_mollified(self._h, L.vec3(center), float(radius), float(mollify),
+                                                          L.dptr(out)))
+        else:
+            self._c(L.lib().cup_chi_from_sphere(self._h, L.vec3(center), float(radius), L.dptr(out)))
         return dict(com=out[:3].copy(), mass=out[3])
 
     def penalize(self, dt, lam, implicit, trans_vel, ang_vel=(0, 0, 0)) -> np.ndarray:

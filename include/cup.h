@@ -120,6 +120,13 @@ int cup_pressure_project(cup_ctx* ctx, double dt);
  * (ObstaclesCreate.cpp:22-137), with surface points and the grid centre of mass
  * (:230-256). Static body: built once. com_mass_out = CoM[3], mass (may be NULL). */
 int cup_chi_from_sphere(cup_ctx* ctx, const double center[3], double radius, double com_mass_out[4]);
+/* The same sphere with a mollified chi, no reference counterpart (it replaces Towers' formula
+ * of ObstaclesCreate.cpp:51-137 so that chi is C1 in the centre, for cup_adjoint_step_body's
+ * centre gradient): chi = H(R - |x - c|) with H(d) = 0 below -e, 1 above e,
+ * (1 + d/e + sin(pi d/e)/pi)/2 between, e = width*h, 0 < width <= 2. The SDF, surface points,
+ * block selection and centre of mass are as cup_chi_from_sphere forms them. */
+int cup_chi_from_sphere_mollified(cup_ctx* ctx, const double center[3], double radius, double width,
+                                  double com_mass_out[4]);
 
 /* Penalization::operator() (Penalization.cpp:129-156): u += dt*pf*(U_body - u) with
  * pf = chi*lambda/(1+lambda*dt*chi) (implicit) or chi/dt (explicit), and the force/torque
@@ -193,9 +200,12 @@ int cup_adjoint_step(cup_ctx* ctx, double dt, double nu, const double uinf[3], i
  * Penalization.cpp:40-98 after the projection, whose RHS is masked and corrected on the
  * body as in PressureRHS.cpp:78-112,322-343. The objective may hold wdrag times this step's
  * drag F_x = sum h^3 pf (V_x - u_x) (the penalization force, Penalization.cpp:93-96). The
- * masking's sign pattern is held at the forward state. Returns this step's dJ/dV in dJdV. */
+ * masking's sign pattern is held at the forward state. Returns this step's dJ/dV in dJdV.
+ * With dJdc non-NULL (the chi from cup_chi_from_sphere_mollified, else CUP_E_STATE) it also
+ * returns this step's dJ/d(sphere centre): chi enters the penalization, its force and the
+ * RHS masking, whose surface flags are held; this re-solves the forward pressure once. */
 int cup_adjoint_step_body(cup_ctx* ctx, double dt, double nu, const double uinf[3], double lambda, int implicit,
-                          const double body_vel[3], double wdrag, double dJdV[3]);
+                          const double body_vel[3], double wdrag, double dJdV[3], double dJdc[3]);
 /* Penalization force and torque (6 values) of the last step cup_run_steps made with a body. */
 int cup_last_body_force(cup_ctx* ctx, double out[6]);
 

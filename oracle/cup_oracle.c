@@ -68,6 +68,7 @@ struct orc_sim {
   double* rhs; /* solver buffer, global [x][y][z], z fastest (PoissonSolverPeriodic.cpp:112-116) */
   int has_sphere;
   double radius, spos[3];
+  double moll; /* half-width in cells of the mollified chi, 0 for Towers' chi */
   OBlock** ob;
   double CoM[3], mass;
 };
@@ -999,7 +1000,49 @@ static void ob_write(OBlock* o, int ix, int iy, int iz, double delta, double gx,
  * ObstacleLibrary.h:41-69, Obstacle.h:186-207), KernelCharacteristicFunction
  * (ObstaclesCreate.cpp:22-137), KernelComputeGridCoM (:230-256). The sphere's udef is
  * zero, so the udef-momentum steps (:258-440) leave everything unchanged. */
+/* Mollified Heaviside of the signed distance d with half-width e (no reference
+ * counterpart): 0 for d <= -e, 1 for d >= e, (1 + d/e + sin(pi d/e)/pi)/2 between. It is
+ * C1: dH/dd = (1 + cos(pi d/e))/(2e) vanishes at d = +-e. */
+static double moll_H(double d, double e)
+{
+  if (d <= -e) return 0.0;
+  if (d >= e) return 1.0;
+  return 0.5 * (1.0 + d / e + sin(M_PI * d / e) / M_PI);
+}
+
+static double moll_dH(double d, double e)
+{
+  if (d <= -e || d >= e) return 0.0;
+  return (1.0 + cos(M_PI * d / e)) / (2.0 * e);
+}
+
+/* d chi / d centre at a cell of position p for the mollified chi: dH(R - |p - c|) (p - c)/|p - c|. */
+static void moll_dchi_dc(const orc_sim* s, const double p[3], double g[3])
+{
+  const double r[3] = {p[0] - s->spos[0], p[1] - s->spos[1], p[2] - s->spos[2]};
+  const double dist = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+  const double dH = dist > 0 ? moll_dH(s->radius - dist, s->moll * s->h) / dist : 0.0;
+  for (int k = 0; k < 3; k++) g[k] = dH * r[k];
+}
+
+static void sphere_build(orc_sim* s, double radius, const double pos[3]);
+
 void orc_sphere_create(orc_sim* s, double radius, const double pos[3])
+{
+  s->moll = 0;
+  sphere_build(s, radius, pos);
+}
+
+/* The sphere with chi = moll_H(sdf, width h) instead of Towers' formula (width in (0, 2]
+ * cells keeps the chi band inside the obstacle blocks); SDF, surface points and the rest
+ * as orc_sphere_create. */
+void orc_sphere_create_mollified(orc_sim* s, double radius, const double pos[3], double width)
+{
+  s->moll = width;
+  sphere_build(s, radius, pos);
+}
+
+static void sphere_build(orc_sim* s, double radius, const double pos[3])
 {
   free_sphere(s);
   s->has_sphere = 1;
@@ -1061,7 +1104,9 @@ void orc_sphere_create(orc_sim* s, double radius, const double pos[3])
         for (int iy = 0; iy < BS; iy++)
           for (int ix = 0; ix < BS; ix++) {
             const double sdf0 = o->sdf[iz][iy][ix];
-            if (sdf0 > +2 * h || sdf0 < -2 * h) {
+            if (s->moll > 0) {
+              o->chi[iz][iy][ix] = moll_H(sdf0, s->moll * h);
+            } else if (sdf0 > +2 * h || sdf0 < -2 * h) {
               o->chi[iz][iy][ix] = sdf0 > 0 ? 1 : 0;
             } else {
               const double distPx = LAB(&l, ix + 1, iy, iz)->tmpU, distMx = LAB(&l, ix - 1, iy, iz)->tmpU;
@@ -1356,9 +1401,11 @@ void orc_adjoint_step(orc_sim* s, double dt, double nu, const double uinf[3], do
  * surface cells the correction scales) is held at the forward state; the transposed
  * projection is the forward one with M'^T applied to the pressure instead of M to the RHS. */
 void orc_adjoint_step_body(orc_sim* s, double dt, double nu, const double uinf[3], double lambda, int implicit,
-                           const double V[3], double wdrag, double* lam, double dJdV[3])
+                           const double V[3], double wdrag, double* lam, double dJdV[3], double dJdc[3])
 {
   if (!s->has_sphere) { orc_adjoint_step(s, dt, nu, uinf, lam); return; }
+  const int want_c = dJdc != NULL && s->moll > 0;
+  if (dJdc && !want_c) dJdc[0] = dJdc[1] = dJdc[2] = NAN;  /* Towers' chi: no centre gradient */
   const size_t nc = (size_t)s->nb * BS * BS * BS;
   const double h = s->h, h3 = h * h * h;
   /* forward state u_n kept aside; u* = A(u_n) for the masking's linearisation point */
@@ -1367,8 +1414,9 @@ void orc_adjoint_step_body(orc_sim* s, double dt, double nu, const double uinf[3
   for (int q = 0; q < 4; q++) orc_get_field(s, keep[q], save + q * nc);
   orc_advect_diffuse(s, dt, nu, uinf);
   /* masked RHS of u* on obstacle blocks, as orc_pressure_rhs forms it: per cell the flags
-   * (masked, surface) and rhs_new; per block the sums S, pos, neg (block order) */
+   * (masked, surface), the unmasked rhs r0 and rhs_new; per block the sums S, pos, neg */
   double* rnew = (double*)calloc(nc, sizeof(double));
+  double* r0c = want_c ? (double*)calloc(nc, sizeof(double)) : NULL;
   unsigned char* flag = (unsigned char*)calloc(nc, 1);  /* bit 0 masked, bit 1 surface */
   double* part = (double*)calloc(3 * (size_t)s->nb, sizeof(double));
   const double fac = 0.5 * h * h / dt;
@@ -1396,6 +1444,7 @@ void orc_adjoint_step_body(orc_sim* s, double dt, double nu, const double uinf[3
             S += srcBulk;
             const double r1 = r0 + srcBulk;
             rnew[c] = r1;
+            if (r0c) r0c[c] = r0;
             flag[c] = 1 | (surf ? 2 : 0);
             pos += surf * r1 * (r1 > 0);
             neg -= surf * r1 * (r1 < 0);
@@ -1417,15 +1466,27 @@ void orc_adjoint_step_body(orc_sim* s, double dt, double nu, const double uinf[3
   const double D = denom > EPS ? denom : EPS;
   const double corr = S / D;
   const double sg = corr > 0 ? 1.0 : (corr < 0 ? -1.0 : 0.0);
+  /* the penalization's input u_in = P_M(u*), for its chi sensitivity */
+  double* uin = NULL;
+  if (want_c) {
+    orc_pressure_rhs(s, dt);
+    orc_poisson_solve(s);
+    orc_grad_p(s, dt);
+    uin = (double*)malloc(3 * nc * sizeof(double));
+    for (int q = 0; q < 3; q++) orc_get_field(s, ORC_U + q, uin + q * nc);
+  }
   for (int q = 0; q < 4; q++) orc_set_field(s, keep[q], save + q * nc);  /* u_n back */
-  /* penalization and drag: lam' = (1 - dt pf) lam - wdrag h^3 pf e_x on penalized cells */
+  /* penalization and drag: lam' = (1 - dt pf) lam - wdrag h^3 pf e_x on penalized cells; with
+   * the centre, dJ/dchi = dpf/dchi (dt sum_d lam_d (V_d - u_in,d) + wdrag h^3 (V_x - u_in,x)) */
   const double lf = implicit ? lambda : 1.0 / dt;
-  double* pv = (double*)calloc(3 * (size_t)s->nb, sizeof(double));
+  double* pv = (double*)calloc(6 * (size_t)s->nb, sizeof(double));
   #pragma omp parallel for schedule(static)
   for (int b = 0; b < s->nb; b++) {
     const OBlock* o = s->ob[b];
     if (!o) continue;
-    double a[3] = {0, 0, 0};
+    int bc[3];
+    bcoords(s, b, bc);
+    double a[6] = {0, 0, 0, 0, 0, 0};
     for (int iz = 0; iz < BS; iz++)
       for (int iy = 0; iy < BS; iy++)
         for (int ix = 0; ix < BS; ix++) {
@@ -1433,17 +1494,31 @@ void orc_adjoint_step_body(orc_sim* s, double dt, double nu, const double uinf[3
           if (s->blk[b].d[iz][iy][ix].chi > X || X <= 0) continue;
           const double pf = implicit ? X * lf / (1 + lf * dt * X) : X * lf;
           const size_t c = (size_t)b * BS * BS * BS + (iz * BS + iy) * BS + ix;
+          if (want_c) {
+            const double den = 1 + lf * dt * X;
+            const double dpf = implicit ? lf / (den * den) : lf;
+            double gx = wdrag * h3 * (V[0] - uin[c]);
+            for (int d = 0; d < 3; d++) gx += dt * lam[d * nc + c] * (V[d] - uin[d * nc + c]);
+            double p[3], dX[3];
+            cell_pos(s, bc, ix, iy, iz, p);
+            moll_dchi_dc(s, p, dX);
+            for (int k = 0; k < 3; k++) a[3 + k] += dpf * gx * dX[k];
+          }
           for (int d = 0; d < 3; d++) {
             a[d] += dt * pf * lam[d * nc + c] + (d == 0 ? wdrag * h3 * pf : 0.0);
             lam[d * nc + c] = (1 - dt * pf) * lam[d * nc + c] - (d == 0 ? wdrag * h3 * pf : 0.0);
           }
         }
-    for (int d = 0; d < 3; d++) pv[3 * b + d] = a[d];
+    for (int d = 0; d < 6; d++) pv[6 * b + d] = a[d];
   }
+  double dc[3] = {0, 0, 0};
   for (int b = 0; b < s->nb; b++)
-    for (int d = 0; d < 3; d++) dJdV[d] += pv[3 * b + d];
+    for (int d = 0; d < 3; d++) {
+      dJdV[d] += pv[6 * b + d];
+      dc[d] += pv[6 * b + 3 + d];
+    }
   free(pv);
-  (void)V;
+  free(uin);
   /* transposed projection: y = L^{-1} R lam' (plain RHS), q = M'^T y, lam* = lam' - (dt/2h) grad_c q */
   for (int q = 0; q < 3; q++) orc_set_field(s, ORC_U + q, lam + q * nc);
   const int sphere = s->has_sphere;
@@ -1456,25 +1531,46 @@ void orc_adjoint_step_body(orc_sim* s, double dt, double nu, const double uinf[3
   for (size_t c = 0; c < nc; c++)
     if ((flag[c] & 2) && ((rnew[c] > 0 && corr > 0) || (rnew[c] < 0 && corr < 0))) g += y[c] * (-sg * rnew[c]);
   const double aS = g / D, aD = denom > EPS ? -g * corr / D : 0.0;
+  /* with the centre: rhs_new = (1 - X) r0 and S = -sum X r0 give dJ/dX = -r0 (adj(rhs_new) + aS).
+   * y is the adjoint of the masked RHS times h^3/dt^2: the transposed projection applies the
+   * RHS operator (h^2/2dt) div where the transpose of the gradient step is (dt/2h) div, and the
+   * gradient step after it applies the inverse factor, so lam* is exact; the chi term is scaled. */
+  const double yscale = dt * dt / (h * h * h);
+  double* pm = (double*)calloc(3 * (size_t)s->nb, sizeof(double));
   #pragma omp parallel for schedule(static)
   for (int b = 0; b < s->nb; b++) {
     const OBlock* o = s->ob[b];
     if (!o) continue;
+    int bc[3];
+    bcoords(s, b, bc);
+    double a[3] = {0, 0, 0};
     for (int i = 0; i < BS * BS * BS; i++) {
       const size_t c = (size_t)b * BS * BS * BS + i;
       if (!(flag[c] & 1)) continue;
       const double X = (&o->chi[0][0][0])[i];
       const int inQ = (flag[c] & 2) && ((rnew[c] > 0 && corr > 0) || (rnew[c] < 0 && corr < 0));
       const double an = inQ ? y[c] * (1 - sg * corr) + aD * sg : y[c];
+      if (want_c) {
+        double p[3], dX[3];
+        cell_pos(s, bc, i % BS, (i / BS) % BS, i / (BS * BS), p);
+        moll_dchi_dc(s, p, dX);
+        for (int k = 0; k < 3; k++) a[k] += -yscale * r0c[c] * (an + aS) * dX[k];
+      }
       y[c] = (1 - X) * an - X * aS;
     }
+    for (int k = 0; k < 3; k++) pm[3 * b + k] = a[k];
   }
+  for (int b = 0; b < s->nb; b++)
+    for (int k = 0; k < 3; k++) dc[k] += pm[3 * b + k];
+  free(pm);
+  if (want_c)
+    for (int k = 0; k < 3; k++) dJdc[k] += dc[k];
   orc_set_field(s, ORC_P, y);
   orc_grad_p(s, dt);
   s->has_sphere = sphere;
   for (int q = 0; q < 3; q++) orc_get_field(s, ORC_U + q, lam + q * nc);  /* lam* */
   for (int q = 0; q < 4; q++) orc_set_field(s, keep[q], save + q * nc);
-  free(save); free(y); free(rnew); free(flag);
+  free(save); free(y); free(rnew); free(flag); free(r0c);
   adj_advect(s, dt, nu, uinf, lam);
 }
 

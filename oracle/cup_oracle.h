@@ -69,6 +69,8 @@ void orc_ic_hit(orc_sim* s, double k0, double u_rms, uint64_t seed);
 /* Sphere (Sphere.cpp:17-50, ObstaclesCreate.cpp:22-137,230-256, Penalization.cpp,
  * FluidSolidForces.cpp:17-158). */
 void orc_sphere_create(orc_sim* s, double radius, const double pos[3]);
+/* The same sphere with a mollified chi: H(sdf) over a half-width of width cells (0 < width <= 2). */
+void orc_sphere_create_mollified(orc_sim* s, double radius, const double pos[3], double width);
 void orc_sphere_info(const orc_sim* s, double out[8]); /* CoM[3], mass, nblocks, npoints */
 void orc_penalize(orc_sim* s, double dt, double lambda, int implicit,
                   const double transVel[3], const double angVel[3], double ft_out[6]);
@@ -99,9 +101,12 @@ void orc_adjoint_step(orc_sim* s, double dt, double nu, const double uinf[3], do
 
 /* Adjoint of one step with the sphere (forced at velocity V): lam as orc_adjoint_step,
  * J = sum_n wdrag_n F_x^n + g(u_N) with F the step's penalization force; dJdV accumulates
- * (+=) this step's dJ/dV. The masking's sign pattern is held at the forward state. */
+ * (+=) this step's dJ/dV. The masking's sign pattern is held at the forward state. With the
+ * mollified chi and dJdc non-NULL, dJdc accumulates this step's dJ/d(sphere centre) (chi
+ * enters the penalization, its force and the RHS masking; the masking's flags are held);
+ * with Towers' chi dJdc is set to NaN. */
 void orc_adjoint_step_body(orc_sim* s, double dt, double nu, const double uinf[3], double lambda, int implicit,
-                           const double V[3], double wdrag, double* lam, double dJdV[3]);
+                           const double V[3], double wdrag, double* lam, double dJdV[3], double dJdc[3]);
 
 /* Plain 3-D complex FFT, unnormalized, sign -1 forward / +1 backward; arrays [x][y][z]. */
 void orc_fft3d(int nx, int ny, int nz, double* re, double* im, int sign);

@@ -53,7 +53,7 @@ def lib():
         "orc_advect_diffuse": (None, [p, d, d, dp]),
         "orc_advect_diffuse_scheme": (None, [p, d, d, dp, i]),
         "orc_adjoint_step": (None, [p, d, d, dp, dp]),
-        "orc_adjoint_step_body": (None, [p, d, d, dp, d, i, dp, d, dp, dp]),
+        "orc_adjoint_step_body": (None, [p, d, d, dp, d, i, dp, d, dp, dp, dp]),
         "orc_pressure_rhs": (None, [p, d]),
         "orc_poisson_solve": (None, [p]),
         "orc_grad_p": (None, [p, d]),
@@ -64,6 +64,7 @@ def lib():
         "orc_ic_taylor_green": (None, [p, d]),
         "orc_ic_hit": (None, [p, d, d, C.c_uint64]),
         "orc_sphere_create": (None, [p, d, dp]),
+        "orc_sphere_create_mollified": (None, [p, d, dp, d]),
         "orc_sphere_info": (None, [p, dp]),
         "orc_penalize": (None, [p, d, d, i, dp, dp, dp]),
         "orc_surface_forces": (None, [p, d, dp, dp, dp]),
@@ -185,15 +186,18 @@ class OracleSim:
         n = buf.size // 3
         return tuple(from_blocks(buf[q * n:(q + 1) * n], self.shape) for q in range(3))
 
-    def adjoint_step_body(self, dt, nu, lam, vel, lambda_, implicit, wdrag, uinf=(0, 0, 0)):
+    def adjoint_step_body(self, dt, nu, lam, vel, lambda_, implicit, wdrag, uinf=(0, 0, 0), centre=False):
         """With the sphere: (dJ/du_n, dJ/dV of this step) from lam = dJ/du_{n+1}, J containing
-        wdrag * F_x of this step (F the penalization force); u_n is this sim's velocity."""
+        wdrag * F_x of this step (F the penalization force); u_n is this sim's velocity. With
+        centre=True (mollified chi) also this step's dJ/d(sphere centre) as a third value."""
         buf = np.concatenate([to_blocks(np.asarray(f, dtype=np.float64)) for f in lam])
-        dv = np.zeros(3)
+        dv, dc = np.zeros(3), np.zeros(3)
         lib().orc_adjoint_step_body(self._h, float(dt), float(nu), _dp(_vec(uinf)), float(lambda_),
-                                    int(bool(implicit)), _dp(_vec(vel)), float(wdrag), _dp(buf), _dp(dv))
+                                    int(bool(implicit)), _dp(_vec(vel)), float(wdrag), _dp(buf), _dp(dv),
+                                    _dp(dc) if centre else None)
         n = buf.size // 3
-        return tuple(from_blocks(buf[q * n:(q + 1) * n], self.shape) for q in range(3)), dv
+        lam0 = tuple(from_blocks(buf[q * n:(q + 1) * n], self.shape) for q in range(3))
+        return (lam0, dv, dc) if centre else (lam0, dv)
 
     def pressure_rhs(self, dt):
         lib().orc_pressure_rhs(self._h, float(dt))
@@ -230,8 +234,15 @@ class OracleSim:
     def ic_hit(self, k0=4.0, u_rms=1.0, seed=20260101):
         lib().orc_ic_hit(self._h, float(k0), float(u_rms), int(seed))
 
-    def sphere_create(self, radius, pos):
-        lib().orc_sphere_create(self._h, float(radius), _dp(_vec(pos)))
+    def sphere_create(self, radius, pos, mollify=0.0):
+        """The sphere's chi by Towers' formula, or (mollify = half-width in cells, in (0, 2])
+        by the mollified Heaviside of the SDF."""
+        if mollify:
+            if not 0 < mollify <= 2:
+                raise ValueError("mollify: the half-width must lie in (0, 2] cells")
+            lib().orc_sphere_create_mollified(self._h, float(radius), _dp(_vec(pos)), float(mollify))
+        else:
+            lib().orc_sphere_create(self._h, float(radius), _dp(_vec(pos)))
 
     def sphere_info(self):
         o = np.zeros(8)

@@ -38,6 +38,53 @@ def test_adjoint_step_body_matches_oracle(cup, oracle, decomp):
     np.testing.assert_allclose(dv, dvo, rtol=1e-11, atol=1e-12 * np.abs(dvo).max())
 
 
+@pytest.mark.parametrize("decomp,mollify", [("plain", 1.0), ("fused", 1.0), ("fused", 2.0)])
+def test_adjoint_step_body_centre_matches_oracle(cup, oracle, decomp, mollify):
+    print(f"adjoint step with the mollified sphere decomp={decomp} mollify={mollify}", flush=True)
+    rng = np.random.default_rng(43)
+    bpd, ext, c0 = (4, 2, 2), 2.0, (0.913, 0.487, 0.502)
+    G = cup.Grid(bpd, maxextent=ext, precision=8, decomposition=decomp)
+    O = oracle.OracleSim(bpd, maxextent=ext)
+    G.chi_from_sphere(c0, 0.3, mollify=mollify)
+    O.sphere_create(0.3, c0, mollify=mollify)
+    np.testing.assert_allclose(G.get("chi"), O.get("chi"), rtol=0, atol=1e-15)
+    shape = G.shape
+    U, V = (1.0, 0.0, 0.0), (-1.0, 0.1, 0.0)
+    u = tuple(0.3 * rng.standard_normal(shape) + (1.0 if q == 0 else 0.0) for q in range(3))
+    lam = tuple(rng.standard_normal(shape) for _ in range(3))
+    dt, nu, w = 0.2 * G.h, 1e-3, 3.7
+    sph = dict(lambda_=1e6, implicit=True, trans_vel=V)
+    G.set_velocity(*u); O.set_velocity(*u)
+    for n, f in zip(("adjU", "adjV", "adjW"), lam):
+        G.set(n, f)
+    dv, dc = G.adjoint_step_body(dt, nu, U, sph, w, centre=True)
+    lam0, dvo, dco = O.adjoint_step_body(dt, nu, lam, V, 1e6, True, w, U, centre=True)
+    for n, b in zip(("adjU", "adjV", "adjW"), lam0):
+        assert rel(G.get(n), b) < 1e-11
+    np.testing.assert_allclose(dv, dvo, rtol=1e-11, atol=1e-12 * np.abs(dvo).max())
+    print(f"dJ/dc gpu {dc} oracle {dco}", flush=True)
+    np.testing.assert_allclose(dc, dco, rtol=1e-10, atol=1e-11 * np.abs(dco).max())
+    # the velocity step is unchanged by asking for the centre
+    G.set_velocity(*u)
+    for n, f in zip(("adjU", "adjV", "adjW"), lam):
+        G.set(n, f)
+    dv2 = G.adjoint_step_body(dt, nu, U, sph, w)
+    for n, b in zip(("adjU", "adjV", "adjW"), lam0):
+        assert rel(G.get(n), b) < 1e-11
+    np.testing.assert_allclose(dv2, dvo, rtol=1e-11, atol=1e-12 * np.abs(dvo).max())
+
+
+def test_centre_gradient_needs_mollified_chi(cup):
+    G = cup.Grid((2, 2, 2), maxextent=1.0, precision=8, decomposition="fused")
+    G.chi_from_sphere((0.5, 0.5, 0.5), 0.2)
+    G.set_velocity(*(np.zeros(G.shape) for _ in range(3)))
+    for n in ("adjU", "adjV", "adjW"):
+        G.set(n, np.zeros(G.shape))
+    with pytest.raises(RuntimeError, match="mollified"):
+        G.adjoint_step_body(1e-3, 1e-3, (1, 0, 0), dict(lambda_=1e6, implicit=True, trans_vel=(0, 0, 0)), 1.0,
+                            centre=True)
+
+
 def test_last_body_force_matches_oracle(cup, oracle):
     rng = np.random.default_rng(5)
     bpd, ext = (4, 2, 2), 2.0
@@ -54,35 +101,45 @@ def test_last_body_force_matches_oracle(cup, oracle):
     np.testing.assert_allclose(G.last_body_force(), out["penal_ft"], rtol=1e-11, atol=1e-13)
 
 
-def test_mean_drag_velocity_gradient_vs_finite_differences(cup):
-    """Sphere D = 0.2 at (0.5, 0.5, 0.5) in a periodic 2 x 1 x 1 box at 256 x 128 x 128, inflow
-    U = 1, Re = 200, implicit penalization lambda = 1e6, 100 steps of dt = 1e-3 from rest,
-    checkpoints every 10: the gradient of the time-averaged drag with respect to the body
-    velocity, FP32 and FP64 adjoints against fourth-order central differences (eps = 1e-3)
-    of the FP64 forward."""
+def test_mean_drag_gradient_vs_finite_differences(cup):
+    """Sphere D = 0.2 in a periodic 2 x 1 x 1 box at 256 x 128 x 128, mollified chi (half-width
+    one cell), inflow U = 1, Re = 200, implicit penalization lambda = 1e6, 100 steps of
+    dt = 1e-3 from rest, checkpoints every 10: the gradient of the time-averaged drag with
+    respect to the body velocity and the sphere centre, FP32 and FP64 adjoints against
+    central differences of the FP64 forward (velocity: fourth order, eps = 1e-3; centre:
+    second order, eps = 5e-7, every cell of the chi band staying more than 10 eps away from
+    its edges |x - c| = R +- h, where a cell's masking flags change)."""
     import torch
     from cubismup_3d_amd.adjoint import mean_drag, recompute_memory_bytes
-    bpd, ext, R, c = (32, 16, 16), 2.0, 0.1, (0.5, 0.5, 0.5)
+    bpd, ext, R, moll = (32, 16, 16), 2.0, 0.1, 1.0
+    c0 = np.array([0.4982, 0.4988, 0.4999])
     U, nu, nsteps, every = (1.0, 0.0, 0.0), 1.0 * 2 * R / 200, 100, 10
     V0 = np.array([-1.0, 0.0, 0.0])
     dts = [1e-3] * nsteps
-    grads, vals = {}, {}
+    h = ext / 256
+    ax = [(np.arange(n) + 0.5) * h for n in (256, 128, 128)]
+    sl = [a[(a > ci - R - 3 * h) & (a < ci + R + 3 * h)] for a, ci in zip(ax, c0)]
+    X, Y, Z = np.meshgrid(*sl, indexing="ij")
+    r = np.sqrt((X - c0[0]) ** 2 + (Y - c0[1]) ** 2 + (Z - c0[2]) ** 2)
+    ec = 5e-7
+    assert min(np.abs(r - (R + moll * h)).min(), np.abs(r - (R - moll * h)).min()) > 10 * ec
+    grads, gradc, vals = {}, {}, {}
     for prec in (4, 8):
         G = cup.Grid(bpd, maxextent=ext, precision=prec, decomposition="fused")
-        G.chi_from_sphere(c, R)
         dtype = torch.float32 if prec == 4 else torch.float64
         u0 = torch.zeros((3,) + G.shape, dtype=dtype, device="cuda", requires_grad=True)
         V = torch.tensor(V0, dtype=torch.float64, requires_grad=True)
-        J = mean_drag(u0, V, G, nsteps, nu, U, dts, every=every)
+        C = torch.tensor(c0, dtype=torch.float64, requires_grad=True)
+        J = mean_drag(u0, V, G, nsteps, nu, U, dts, every=every, centre=C, radius=R, mollify=moll)
         J.backward()
-        grads[prec], vals[prec] = V.grad.numpy().copy(), J.item()
+        grads[prec], gradc[prec], vals[prec] = V.grad.numpy().copy(), C.grad.numpy().copy(), J.item()
         print(f"precision {prec}: mean drag {J.item():.8e} (Cd {J.item() / (0.5 * np.pi * R * R):.5f}), "
-              f"dJ/dV {grads[prec]}, recompute memory {recompute_memory_bytes(G, nsteps, every) / 2**20:.0f} MiB",
-              flush=True)
+              f"dJ/dV {grads[prec]}, dJ/dc {gradc[prec]}, "
+              f"recompute memory {recompute_memory_bytes(G, nsteps, every) / 2**20:.0f} MiB", flush=True)
     G8 = cup.Grid(bpd, maxextent=ext, precision=8, decomposition="fused")
-    G8.chi_from_sphere(c, R)
 
-    def J64(V):
+    def J64(V, c):
+        G8.chi_from_sphere(tuple(c), R, mollify=moll)
         G8.set_velocity(*(np.zeros(G8.shape) for _ in range(3)))
         sph = dict(lambda_=1e6, implicit=True, trans_vel=tuple(V))
         J = 0.0
@@ -90,11 +147,24 @@ def test_mean_drag_velocity_gradient_vs_finite_differences(cup):
             G8.run_steps(1, nu=nu, uinf=U, fixed_dt=dt, sphere=sph)
             J += dt / sum(dts) * G8.last_body_force()[0]
         return J
+    assert abs(J64(V0, c0) - vals[8]) < 1e-12 * abs(vals[8])
+    # each component against the gradient's norm: the finite differences carry an absolute
+    # error of about 1e-8 (the masking's sign pattern changes along the trajectory, which
+    # puts kinks in J), the size of the small components themselves
     eps = 1e-3
+    vscale = np.linalg.norm(grads[8])
     for k in range(3):
         e = np.eye(3)[k] * eps
-        fd = (8 * (J64(V0 + e) - J64(V0 - e)) - (J64(V0 + 2 * e) - J64(V0 - 2 * e))) / (12 * eps)
-        scale = max(abs(fd), 1e-3 * abs(vals[8]))
-        print(f"dJ/dV_{k}: fd {fd:.10e} fp64 adjoint {grads[8][k]:.10e} fp32 adjoint {grads[4][k]:.10e}", flush=True)
-        assert abs(grads[8][k] - fd) < 1e-6 * scale
-        assert abs(grads[4][k] - fd) < 1e-3 * scale
+        fd = (8 * (J64(V0 + e, c0) - J64(V0 - e, c0)) - (J64(V0 + 2 * e, c0) - J64(V0 - 2 * e, c0))) / (12 * eps)
+        print(f"dJ/dV_{k}: fd {fd:.10e} fp64 adjoint {grads[8][k]:.10e} ({abs(grads[8][k] - fd) / vscale:.1e}) "
+              f"fp32 adjoint {grads[4][k]:.10e} ({abs(grads[4][k] - fd) / vscale:.1e} of |dJ/dV|)", flush=True)
+        assert abs(grads[8][k] - fd) < 1e-6 * vscale
+        assert abs(grads[4][k] - fd) < 1e-4 * vscale
+    cscale = np.linalg.norm(gradc[8])
+    for k in range(3):
+        e = np.eye(3)[k] * ec
+        fd = (J64(V0, c0 + e) - J64(V0, c0 - e)) / (2 * ec)
+        print(f"dJ/dc_{k}: fd {fd:.10e} fp64 adjoint {gradc[8][k]:.10e} ({abs(gradc[8][k] - fd) / cscale:.1e}) "
+              f"fp32 adjoint {gradc[4][k]:.10e} ({abs(gradc[4][k] - fd) / cscale:.1e} of |dJ/dc|)", flush=True)
+        assert abs(gradc[8][k] - fd) < 1e-5 * cscale
+        assert abs(gradc[4][k] - fd) < 1e-4 * cscale

@@ -62,9 +62,9 @@ def test_adjoint_of_pure_projection_is_symmetric(oracle):
     assert max(np.max(np.abs(x - y)) for x, y in zip(ppa, pa)) < 1e-12 * max(np.max(np.abs(x)) for x in pa)
 
 
-def fwd_step_body(oracle, bpd, u, dt, nu, U, V, lam_, ext):
+def fwd_step_body(oracle, bpd, u, dt, nu, U, V, lam_, ext, centre=(0.9, 0.5, 0.5), mollify=0.0):
     s = oracle.OracleSim(bpd, maxextent=ext)
-    s.sphere_create(0.3, (0.9, 0.5, 0.5))
+    s.sphere_create(0.3, centre, mollify=mollify)
     s.set_velocity(*u)
     out = s.timestep(dt, nu, uinf=U, sphere=dict(lambda_=lam_, implicit=True, trans_vel=V))
     return s.velocity(), out["penal_ft"][0]
@@ -98,3 +98,51 @@ def test_adjoint_with_sphere_dot_product_identity(oracle, lam_):
         e = np.eye(3)[k] * eps
         fdv = (J(u, tuple(np.add(V, e))) - J(u, tuple(np.subtract(V, e)))) / (2 * eps)
         assert abs(fdv - dJdV[k]) < 1e-7 * max(abs(fdv), 1e-3)
+
+
+@pytest.mark.parametrize("mollify", [1.0, 2.0])
+def test_adjoint_sphere_centre_dot_product_identity(oracle, mollify):
+    """The mollified chi is C1 in the sphere centre: dJ/dc of one step (J = <lam, u_{n+1}> +
+    w F_x, chi entering the penalization, its force and the RHS masking) against central
+    differences of the forward oracle step in the centre (eps = 1e-6 h-scale offsets, far
+    below the spacing at which a cell's masking flags change), and dJ/du_n, dJ/dV as with
+    Towers' chi."""
+    rng = np.random.default_rng(43)
+    bpd, ext = (4, 2, 2), 2.0
+    c0 = np.array([0.913, 0.487, 0.502])
+    s = oracle.OracleSim(bpd, maxextent=ext)
+    s.sphere_create(0.3, c0, mollify=mollify)
+    shape = s.shape
+    U, V = (1.0, 0.0, 0.0), (-1.0, 0.1, 0.0)
+    u = tuple(0.3 * rng.standard_normal(shape) + (1.0 if q == 0 else 0.0) for q in range(3))
+    d = tuple(rng.standard_normal(shape) for _ in range(3))
+    lam = tuple(rng.standard_normal(shape) for _ in range(3))
+    dt, nu, w, lam_ = 0.2 * s.h, 1e-3, 3.7, 1e6
+
+    def J(uu, VV, cc):
+        u1, F = fwd_step_body(oracle, bpd, uu, dt, nu, U, VV, lam_, ext, centre=tuple(cc), mollify=mollify)
+        return dot(lam, u1) + w * F
+    s.set_velocity(*u)
+    lam0, dJdV, dJdc = s.adjoint_step_body(dt, nu, lam, V, lam_, True, w, U, centre=True)
+    eps = 1e-5
+    fd = (J(tuple(a + eps * b for a, b in zip(u, d)), V, c0) - J(tuple(a - eps * b for a, b in zip(u, d)), V, c0)) / (2 * eps)
+    assert abs(fd - dot(d, lam0)) < 1e-7 * abs(fd)
+    for k in range(3):
+        e = np.eye(3)[k] * eps
+        fdv = (J(u, tuple(np.add(V, e)), c0) - J(u, tuple(np.subtract(V, e)), c0)) / (2 * eps)
+        assert abs(fdv - dJdV[k]) < 1e-7 * max(abs(fdv), 1e-3)
+    ec = 1e-6 * s.h
+    for k in range(3):
+        e = np.eye(3)[k] * ec
+        fdc = (J(u, V, c0 + e) - J(u, V, c0 - e)) / (2 * ec)
+        print(f"mollify {mollify} dJ/dc_{k}: fd {fdc:.10e} adjoint {dJdc[k]:.10e}")
+        assert abs(fdc - dJdc[k]) < 1e-6 * max(abs(fdc), 1.0)
+
+
+def test_adjoint_centre_is_nan_with_towers_chi(oracle):
+    s = oracle.OracleSim((2, 2, 2), maxextent=1.0)
+    s.sphere_create(0.2, (0.5, 0.5, 0.5))
+    s.set_velocity(*(np.zeros(s.shape) for _ in range(3)))
+    lam = tuple(np.zeros(s.shape) for _ in range(3))
+    _, _, dc = s.adjoint_step_body(1e-3, 1e-3, lam, (0, 0, 0), 1e6, True, 1.0, centre=True)
+    assert np.isnan(dc).all()
