@@ -8,7 +8,8 @@
 // (cos(4 pi k/N) = 1 at both), dividing the packed column by the kx = 0 symbol solves both
 // planes at once.
 //
-// Passes (each one HBM read + write of S; FFTs run in LDS, radix 8/4/2 Stockham):
+// Passes (each one HBM read + write of S; x FFTs run in LDS, radix 8/4/2 Stockham; y and z
+// FFTs of 64 to 256 points in registers with one LDS transpose, others as the x FFTs):
 //   x-forward : 8 rows (one block row of a z plane) per workgroup, two real rows packed
 //               into one complex FFT; the source is either a real block-layout field or,
 //               fused, the central divergence of the advected velocity (PressureRHS).
@@ -77,6 +78,42 @@ __device__ __forceinline__ void dft8(C2<T>* v)
   v[6] = csub(e2, w2o);
   v[3] = cadd(e3, w3o);
   v[7] = csub(e3, w3o);
+}
+
+// 16-point DFT in registers, natural order in and out: radix 4 x 4 with n = q + 4p,
+// k = r + 4s, X[r + 4s] = sum_q W16^{qr} W4^{qs} sum_p v[q + 4p] W4^{pr}, W16 = e^{DIR 2 pi i/16}.
+template <int DIR, typename T>
+__device__ __forceinline__ void dft16(C2<T>* v)
+{
+  C2<T> b[4][4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    b[q][0] = v[q];
+    b[q][1] = v[q + 4];
+    b[q][2] = v[q + 8];
+    b[q][3] = v[q + 12];
+    dft4<DIR>(b[q][0], b[q][1], b[q][2], b[q][3]);
+  }
+  const T c1 = (T)0.92387953251128675613, s1 = (T)0.38268343236508977173, h = (T)0.70710678118654752440;
+  const T d = (T)DIR;
+  // W16^m for m = q r: 1, 2, 3, 4 (a quarter turn), 6, 9
+  b[1][1] = cmul(b[1][1], C2<T>{c1, d * s1});
+  b[1][2] = cmul(b[1][2], C2<T>{h, d * h});
+  b[1][3] = cmul(b[1][3], C2<T>{s1, d * c1});
+  b[2][1] = cmul(b[2][1], C2<T>{h, d * h});
+  b[2][2] = rot90<DIR>(b[2][2]);
+  b[2][3] = cmul(b[2][3], C2<T>{-h, d * h});
+  b[3][1] = cmul(b[3][1], C2<T>{s1, d * c1});
+  b[3][2] = cmul(b[3][2], C2<T>{-h, d * h});
+  b[3][3] = cmul(b[3][3], C2<T>{-c1, -d * s1});
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    dft4<DIR>(b[0][r], b[1][r], b[2][r], b[3][r]);
+    v[r] = b[0][r];
+    v[r + 4] = b[1][r];
+    v[r + 8] = b[2][r];
+    v[r + 12] = b[3][r];
+  }
 }
 
 template <int R, int DIR, typename T>
@@ -453,6 +490,163 @@ __global__ __launch_bounds__(NT) void k_fft_col(C2<T>* __restrict__ S, const C2<
   }
 }
 
+// ------------------------------------------- y / z passes, register four-step form
+// N = 16 NTH (NTH = 4, 8, 16 threads per column), TK = NT/NTH columns per workgroup.
+// Layout A: thread (f, j) holds v[m] = x[j + NTH m], m < 16. Layout B: thread (f, j) holds
+// v[i NTH + k2] = X[k1 + 16 k2] for its RPT = 16/NTH rows k1 = j RPT + i. With
+// W_N^{(j + NTH m)(k1 + 16 k2)} = W_N^{j k1} W_NTH^{j k2} W_16^{m k1}, a transform is a register
+// DFT16 over m, the twiddle W_N^{j k1}, one LDS transpose, and register DFTs of length NTH
+// over j (A -> B); B -> A runs the same steps in reverse order. Column f is the workgroup's
+// f-th kx column; lanes run f fastest, so each global access covers 16 columns of a row.
+template <typename T, int N>
+struct ColReg {
+  static constexpr bool ok = N >= 64 && N <= 256;
+  static constexpr int NTH = ok ? N / 16 : 1, RPT = 16 / NTH, TK = NT / NTH;
+  // LDS stride of one k1 row. Unpadded, the tile is 32 KiB (FP32), so five workgroups share
+  // a CU; the transposes' writes are conflict free and their reads cost less than a lost slot.
+  static constexpr int ROWP = NTH * TK;
+};
+
+template <int L, int DIR, typename T>
+__device__ __forceinline__ void dftL(C2<T>* v)
+{
+  if constexpr (L == 16) dft16<DIR>(v);
+  else if constexpr (L == 8) dft8<DIR>(v);
+  else dft4<DIR>(v[0], v[1], v[2], v[3]);
+}
+
+// A -> B. The caller synchronizes before when buf was read since its last barrier.
+template <typename T, int N, int DIR>
+__device__ __forceinline__ void fft_a_to_b(C2<T> (&v)[16], C2<T>* buf, const C2<T>* __restrict__ tw, int f, int j)
+{
+  using CR = ColReg<T, N>;
+  dft16<DIR>(v);
+#pragma unroll
+  for (int k1 = 1; k1 < 16; ++k1) {
+    C2<T> w = tw[(j * k1) & (N - 1)];
+    if (DIR > 0) w.y = -w.y;
+    v[k1] = cmul(v[k1], w);
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < 16; ++k1) buf[k1 * CR::ROWP + j * CR::TK + f] = v[k1];
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < CR::RPT; ++i)
+#pragma unroll
+    for (int jp = 0; jp < CR::NTH; ++jp) v[i * CR::NTH + jp] = buf[(j * CR::RPT + i) * CR::ROWP + jp * CR::TK + f];
+#pragma unroll
+  for (int i = 0; i < CR::RPT; ++i) dftL<CR::NTH, DIR>(v + i * CR::NTH);
+}
+
+// B -> A. The caller synchronizes before when buf was read since its last barrier.
+template <typename T, int N, int DIR>
+__device__ __forceinline__ void fft_b_to_a(C2<T> (&v)[16], C2<T>* buf, const C2<T>* __restrict__ tw, int f, int j)
+{
+  using CR = ColReg<T, N>;
+#pragma unroll
+  for (int i = 0; i < CR::RPT; ++i) {
+    dftL<CR::NTH, DIR>(v + i * CR::NTH);
+    const int k1 = j * CR::RPT + i;
+#pragma unroll
+    for (int jn = 1; jn < CR::NTH; ++jn) {
+      C2<T> w = tw[(jn * k1) & (N - 1)];
+      if (DIR > 0) w.y = -w.y;
+      v[i * CR::NTH + jn] = cmul(v[i * CR::NTH + jn], w);
+    }
+#pragma unroll
+    for (int jn = 0; jn < CR::NTH; ++jn) buf[k1 * CR::ROWP + jn * CR::TK + f] = v[i * CR::NTH + jn];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k1 = 0; k1 < 16; ++k1) v[k1] = buf[k1 * CR::ROWP + j * CR::TK + f];
+  dft16<DIR>(v);
+}
+
+// The column pass of k_fft_col in the register form (same arguments and modes).
+template <typename T, int N, int AXIS, int MODE, bool FORCE = false>
+__global__ __launch_bounds__(NT) void k_fft_colr(C2<T>* __restrict__ S, const C2<T>* __restrict__ tw, int ncols,
+                                                 int Ny, int Nz, double norm, const double* __restrict__ cx,
+                                                 const double* __restrict__ cy, const double* __restrict__ cz,
+                                                 const LowModes* __restrict__ lmp, const double* __restrict__ coef)
+{
+  using CR = ColReg<T, N>;
+  constexpr int TK = CR::TK, NTH = CR::NTH, RPT = CR::RPT;
+  __shared__ C2<T> buf[16 * CR::ROWP];
+  const int kx0 = blockIdx.x * TK, other = blockIdx.y;
+  const int f = threadIdx.x % TK, j = threadIdx.x / TK;
+  const bool live = kx0 + f < ncols;
+  // element n of column kx0 + f
+  auto gi = [&](int n) -> size_t {
+    return AXIS == 1 ? ((size_t)other * Ny + n) * ncols + kx0 + f : ((size_t)n * Ny + other) * ncols + kx0 + f;
+  };
+  C2<T> v[16];
+  if (MODE == ZM_INV) {
+#pragma unroll
+    for (int i = 0; i < RPT; ++i)
+#pragma unroll
+      for (int k2 = 0; k2 < NTH; ++k2) v[i * NTH + k2] = live ? S[gi(j * RPT + i + 16 * k2)] : C2<T>{0, 0};
+    fft_b_to_a<T, N, +1>(v, buf, tw, f, j);
+    if (live)
+#pragma unroll
+      for (int m = 0; m < 16; ++m) S[gi(j + NTH * m)] = v[m];
+    return;
+  }
+#pragma unroll
+  for (int m = 0; m < 16; ++m) v[m] = live ? S[gi(j + NTH * m)] : C2<T>{0, 0};
+  fft_a_to_b<T, N, -1>(v, buf, tw, f, j);
+  if (FORCE && kx0 == 0) {
+    // fused forcing: add the forcing's RHS modes (coefficients from k_forcing_reduce) to the
+    // entries this thread holds (column f, k = k1 + 16 k2 with k1 / RPT = j)
+    const LowModes& lm = *lmp;
+    for (int e = 0; e < lm.npos; ++e) {
+      const int slot = lm.pos[e][0], ky = lm.pos[e][1], kz = lm.pos[e][2], mc = lm.pos[e][3];
+      if (ky != other || slot != f || (kz & 15) / RPT != j) continue;
+      const int r = ((kz & 15) % RPT) * NTH + (kz >> 4);
+      const double* cm = coef + 8 * (mc >> 1);
+      const double ci = (mc & 1) ? -cm[1] : cm[1];
+#pragma unroll
+      for (int q = 0; q < 16; ++q)
+        if (q == r) {
+          v[q].x = (T)((double)v[q].x + cm[0]);
+          v[q].y = (T)((double)v[q].y + ci);
+        }
+    }
+  }
+  if (MODE == ZM_FWD) {
+    if (live)
+#pragma unroll
+      for (int i = 0; i < RPT; ++i)
+#pragma unroll
+        for (int k2 = 0; k2 < NTH; ++k2) S[gi(j * RPT + i + 16 * k2)] = v[i * NTH + k2];
+    return;
+  }
+  // ZM_POISSON (z): PoissonSolverPeriodic::_solve's grid-consistent branch
+  // (PoissonSolverPeriodic.cpp:23-61), multiply by norm/(2D-6) in FP64, zero the modes with
+  // every index in {0, N/2}; then the inverse transform.
+  {
+    const int kx = kx0 + f, ky = other;
+    const double dxy = live ? cx[kx] + cy[ky] : 0.0;
+    const bool zxy = kx == 0 && (ky == 0 || 2 * ky == Ny);
+#pragma unroll
+    for (int i = 0; i < RPT; ++i)
+#pragma unroll
+      for (int k2 = 0; k2 < NTH; ++k2) {
+        const int kz = j * RPT + i + 16 * k2;
+        const double D = dxy + cz[kz];
+        double fac = norm / (2 * D - 6);
+        if (zxy && (kz == 0 || 2 * kz == N)) fac = 0;
+        C2<T>& q = v[i * NTH + k2];
+        q.x = (T)((double)q.x * fac);
+        q.y = (T)((double)q.y * fac);
+      }
+  }
+  __syncthreads();  // the forward transpose's reads of buf are done
+  fft_b_to_a<T, N, +1>(v, buf, tw, f, j);
+  if (live)
+#pragma unroll
+    for (int m = 0; m < 16; ++m) S[gi(j + NTH * m)] = v[m];
+}
+
 // Separate spectral division (the PLAIN decomposition's "FFTW solve" pass).
 template <typename T>
 __global__ void k_divide(C2<T>* __restrict__ S, int ncols, int Ny, int Nz, double norm, const double* __restrict__ cx,
@@ -515,6 +709,27 @@ template <typename T, int N, int AXIS>
 struct ColPass {
   static void run(Ctx& c, C2<T>* spec, int mode, double norm, bool force = false)
   {
+    if constexpr (ColReg<T, N>::ok) {
+      constexpr int TK = ColReg<T, N>::TK;
+      const int ncols = c.N[0] / 2;
+      const int other = AXIS == 1 ? c.N[2] : c.N[1];
+      dim3 grid((ncols + TK - 1) / TK, other);
+      const C2<T>* tw = (const C2<T>*)c.tw[AXIS];
+      if (mode == ZM_FWD)
+        hipLaunchKernelGGL((k_fft_colr<T, N, AXIS, ZM_FWD>), grid, dim3(NT), 0, c.stream, spec, tw, ncols, c.N[1],
+                           c.N[2], norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low_dev, (const double*)nullptr);
+      else if (mode == ZM_INV)
+        hipLaunchKernelGGL((k_fft_colr<T, N, AXIS, ZM_INV>), grid, dim3(NT), 0, c.stream, spec, tw, ncols, c.N[1],
+                           c.N[2], norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low_dev, (const double*)nullptr);
+      else if (AXIS == 2 && force)
+        hipLaunchKernelGGL((k_fft_colr<T, N, 2, ZM_POISSON, true>), grid, dim3(NT), 0, c.stream, spec, tw, ncols,
+                           c.N[1], c.N[2], norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low_dev,
+                           (const double*)c.low_coef);
+      else if (AXIS == 2)
+        hipLaunchKernelGGL((k_fft_colr<T, N, 2, ZM_POISSON, false>), grid, dim3(NT), 0, c.stream, spec, tw, ncols,
+                           c.N[1], c.N[2], norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low_dev, (const double*)nullptr);
+      return;
+    }
     constexpr int TK = ColTile<T, N>::TK;
     const int ncols = c.N[0] / 2;
     const int other = AXIS == 1 ? c.N[2] : c.N[1];

@@ -79,6 +79,8 @@ __device__ __forceinline__ void block_coords(const GridArgs& g, int b, int& bx, 
 // x-adjacent cell pair, and its y and z neighbour pairs, are 8-byte aligned (one 64-bit LDS
 // access each).
 constexpr int PX = 12;
+constexpr bool ADV_XWALK = true;  // advection walks blocks along x (x faces from registers)
+constexpr int ADV_PD = 3;         // blocks in flight in that walk
 constexpr int PY = PX;            // y stride
 constexpr int PZ = PX * L1;       // z stride
 constexpr int PLC = PX * L1 * L1;
@@ -376,9 +378,139 @@ __global__ __launch_bounds__(NT) void k_advdiff_central(const T* __restrict__ u,
   }
 }
 
+// The same stencil over a walk of NBW blocks along x: block j's x- halo is block j-1's
+// ix = 7 plane and its x+ halo block j+1's ix = 0 plane, both taken from the registers the
+// pipeline holds (threads t % 4 == 3 hold ix = 6, 7; t % 4 == 0 hold ix = 0, 1), so only
+// the walk's two end faces are gathered. PD blocks are in flight (register sets rotate
+// through PD + 1 slots; the loop unrolls fully, so the slot indices are constants).
+template <typename T, int NF>
+__device__ __forceinline__ void lab_fetch_xw(LabRegs<T, NF>& r, const T* const* src, const LabSlots& s,
+                                             const GridArgs& g, int b, bool x_lo, bool x_hi)
+{
+  using V2 = typename Vec2<T>::type;
+  int bx, by, bz;
+  block_coords(g, b, bx, by, bz);
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    const V2 a = *reinterpret_cast<const V2*>(src[f] + (size_t)b * BS3 + 2 * threadIdx.x);
+    r.in[f][0] = a.x;
+    r.in[f][1] = a.y;
+  }
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+    if (k == 0 || has_halo2()) {
+      const int face = __builtin_amdgcn_readfirstlane(s.face[k]);
+      if ((face == 0 && !x_lo) || (face == 1 && !x_hi)) continue;
+      const size_t q = (size_t)face_block(g, bx, by, bz, face) * BS3 + s.off[k];
+#pragma unroll
+      for (int f = 0; f < NF; ++f) r.hv[f][k] = src[f][q];
+    }
+}
+
+template <typename T, int NF>
+__device__ __forceinline__ void lab_put_xw(const LabRegs<T, NF>& r, const LabRegs<T, NF>& prev,
+                                           const LabRegs<T, NF>& next, T (*lab)[PLC], const LabSlots& s, bool x_lo,
+                                           bool x_hi)
+{
+  using V2 = typename Vec2<T>::type;
+#pragma unroll
+  for (int f = 0; f < NF; ++f) *reinterpret_cast<V2*>(&lab[f][s.li]) = V2{r.in[f][0], r.in[f][1]};
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+    if (k == 0 || has_halo2()) {
+      const int face = __builtin_amdgcn_readfirstlane(s.face[k]);
+      if ((face == 0 && !x_lo) || (face == 1 && !x_hi)) continue;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) lab[f][s.l[k]] = r.hv[f][k];
+    }
+  const int t4 = threadIdx.x & 3;
+  if (!x_lo && t4 == 3)
+#pragma unroll
+    for (int f = 0; f < NF; ++f) lab[f][s.li - 7] = prev.in[f][1];
+  if (!x_hi && t4 == 0)
+#pragma unroll
+    for (int f = 0; f < NF; ++f) lab[f][s.li + 8] = next.in[f][0];
+}
+
+template <typename T, int NBW, int PD>
+__global__ __launch_bounds__(NT) void k_advdiff_xwalk(const T* __restrict__ u, const T* __restrict__ v,
+                                                      const T* __restrict__ w, T* __restrict__ ou,
+                                                      T* __restrict__ ov, T* __restrict__ ow, GridArgs g,
+                                                      const DevScalars* __restrict__ sc, T Ux, T Uy, T Uz, double nu)
+{
+  __shared__ __align__(16) T lab[3][PLC];
+  const LabSlots s = lab_slots();
+  const T* src[3] = {u, v, w};
+  const int b0 = xcd_remap(blockIdx.x, gridDim.x) * NBW;
+  constexpr int NS = PD + 1;
+  LabRegs<T, 3> r[NS];
+#pragma unroll
+  for (int p = 0; p < PD && p < NBW; ++p) lab_fetch_xw(r[p], src, s, g, b0 + p, p == 0, p == NBW - 1);
+  const double dt = sc->dt;
+  const T facA = (T)(-dt / (2 * g.h));
+  const T facD = (T)((nu / g.h) * (dt / g.h));
+  const int c0 = 2 * threadIdx.x;
+  using V2 = typename Vec2<T>::type;
+#pragma unroll
+  for (int j = 0; j < NBW; ++j) {
+    if (j) __syncthreads();  // the previous block's compute is done with the lab
+    lab_put_xw(r[j % NS], r[(j + NS - 1) % NS], r[(j + 1) % NS], lab, s, j == 0, j == NBW - 1);
+    __syncthreads();
+    if (j + PD < NBW) lab_fetch_xw(r[(j + PD) % NS], src, s, g, b0 + j + PD, false, j + PD == NBW - 1);
+    T av[3][2];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const V2 C = *reinterpret_cast<const V2*>(lab[q] + s.li);
+      av[q][0] = C.x;
+      av[q][1] = C.y;
+    }
+    const T uu[2] = {av[0][0] + Ux, av[0][1] + Ux}, vv[2] = {av[1][0] + Uy, av[1][1] + Uy},
+            ww[2] = {av[2][0] + Uz, av[2][1] + Uz};
+    T o[3][2];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const T* F = lab[q];
+      const int l = s.li;
+      const V2 Nn = *reinterpret_cast<const V2*>(F + l + PY), Sn = *reinterpret_cast<const V2*>(F + l - PY);
+      const V2 Bn = *reinterpret_cast<const V2*>(F + l + PZ), Fn = *reinterpret_cast<const V2*>(F + l - PZ);
+      const T Wx = F[l - 1], Ex = F[l + 2];
+      const T fc[2] = {av[q][0], av[q][1]};
+      const T fe[2] = {av[q][1], Ex}, fw[2] = {Wx, av[q][0]};
+      const T fn[2] = {Nn.x, Nn.y}, fs[2] = {Sn.x, Sn.y}, fb[2] = {Bn.x, Bn.y}, ff[2] = {Fn.x, Fn.y};
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const T ddx = fe[k] - fw[k], ddy = fn[k] - fs[k], ddz = fb[k] - ff[k];
+        const T dD = fn[k] + fs[k] + fe[k] + fw[k] + ff[k] + fb[k] - fc[k] * 6;
+        const T dA = uu[k] * ddx + vv[k] * ddy + ww[k] * ddz;
+        o[q][k] = fc[k] + facA * dA + facD * dD;
+      }
+    }
+    const size_t q = (size_t)(b0 + j) * BS3 + c0;
+    *reinterpret_cast<V2*>(ou + q) = V2{o[0][0], o[0][1]};
+    *reinterpret_cast<V2*>(ov + q) = V2{o[1][0], o[1][1]};
+    *reinterpret_cast<V2*>(ow + q) = V2{o[2][0], o[2][1]};
+  }
+}
+
+template <typename T, int NBW>
+static void launch_advdiff_xwalk(Ctx& c, double nu, const double uinf[3])
+{
+  hipLaunchKernelGGL((k_advdiff_xwalk<T, NBW, ADV_PD>), dim3(c.nb / NBW), dim3(NT), 0, c.stream,
+                     (const T*)c.f[CUP_U], (const T*)c.f[CUP_V], (const T*)c.f[CUP_W], (T*)c.f[CUP_TMPU],
+                     (T*)c.f[CUP_TMPV], (T*)c.f[CUP_TMPW], c.grid(), c.sc, (T)uinf[0], (T)uinf[1], (T)uinf[2], nu);
+}
+
 template <typename T>
 static int advdiff_t(Ctx& c, double nu, const double uinf[3])
 {
+  if (ADV_XWALK) {
+    const int nbx = c.bpd[0];
+    if (nbx % 8 == 0) launch_advdiff_xwalk<T, 8>(c, nu, uinf);
+    else if (nbx % 4 == 0) launch_advdiff_xwalk<T, 4>(c, nu, uinf);
+    else if (nbx % 2 == 0) launch_advdiff_xwalk<T, 2>(c, nu, uinf);
+    else launch_advdiff_xwalk<T, 1>(c, nu, uinf);
+    return check_hip(c, hipGetLastError(), "k_advdiff_xwalk");
+  }
   const int nbw = blocks_per_wg(c.bpd[2]);
   hipLaunchKernelGGL((k_advdiff_central<T>), dim3(c.nb / nbw), dim3(NT), 0, c.stream, (const T*)c.f[CUP_U],
                      (const T*)c.f[CUP_V], (const T*)c.f[CUP_W], (T*)c.f[CUP_TMPU], (T*)c.f[CUP_TMPV],
