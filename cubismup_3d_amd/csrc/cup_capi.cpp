@@ -173,8 +173,9 @@ int cup_create(const cup_grid_desc* d, cup_ctx** out)
                         "hipHostMalloc")))
       break;
     if ((rc = ensure_partials(c, 20 * (size_t)c.nb + 1024))) break;
-    if ((rc = check_hip(c, hipMalloc(&c.ke_part, (size_t)c.nb * sizeof(double)), "hipMalloc ke"))) break;
-    if ((rc = check_hip(c, hipMemsetAsync(c.ke_part, 0, (size_t)c.nb * sizeof(double), c.stream), "memset ke"))) break;
+    if ((rc = check_hip(c, hipMalloc(&c.ke_part, 4 * (size_t)c.nb * sizeof(double)), "hipMalloc ke"))) break;
+    if ((rc = check_hip(c, hipMemsetAsync(c.ke_part, 0, 4 * (size_t)c.nb * sizeof(double), c.stream), "memset ke")))
+      break;
     if ((rc = fft_setup(c))) break;
     if ((rc = spectral_setup(c))) break;
     rc = check_hip(c, hipStreamSynchronize(c.stream), "synchronize");
@@ -582,7 +583,7 @@ int cup_step_state(cup_ctx* ctx, int* step, double* time, double* last_dt, doubl
 {
   CUP_GUARD(ctx);
   if (c.decomposition == CUP_FUSED) {
-    if (int rc = launch_reduce(c, c.nb, 1, 0u, &c.sc->ke, c.ke_part)) return rc;
+    if (int rc = launch_reduce(c, 4 * c.nb, 1, 0u, &c.sc->ke, c.ke_part)) return rc;
   }
   if (int rc = read_scalars(c)) return rc;
   if (c.decomposition == CUP_FUSED) c.sc_host->ke *= 0.5 * c.h * c.h * c.h;

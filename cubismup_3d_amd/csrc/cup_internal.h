@@ -109,7 +109,7 @@ struct Ctx {
   double* cos4[3] = {};   // cos(4 pi k / N) per axis (FP64), the Poisson symbol
   double* partials = nullptr;
   size_t partials_cap = 0;  // doubles
-  double* ke_part = nullptr;  // per-block sum |u|^2 of the fused step's velocity
+  double* ke_part = nullptr;  // [nb][4] partial sums |u|^2 of the fused step's velocity (per wave)
   DevScalars* sc = nullptr;
   DevScalars* sc_host = nullptr;  // pinned
   double* host_red = nullptr;     // pinned scratch for reductions

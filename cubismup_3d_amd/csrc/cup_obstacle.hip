@@ -422,7 +422,8 @@ __global__ __launch_bounds__(NT) void k_penalize(const int* __restrict__ ob_ids,
       double a = red[0][0], bsum = red[0][1];
       for (int q = 1; q < NT / 64; ++q) { a = fmax(a, red[q][0]); bsum += red[q][1]; }
       atomicMax(&sc->umax_slots[b & (UMAX_SLOTS - 1)], (unsigned long long)__double_as_longlong(a));
-      ke_part[b] = bsum;
+      ke_part[4 * (size_t)b] = bsum;  // the block's partials: [4b] its sum, [4b+1..3] zero
+      ke_part[4 * (size_t)b + 1] = ke_part[4 * (size_t)b + 2] = ke_part[4 * (size_t)b + 3] = 0.0;
     }
   }
 }

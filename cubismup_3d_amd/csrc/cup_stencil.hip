@@ -272,7 +272,8 @@ __global__ __launch_bounds__(NT) void k_max_u(const T* __restrict__ u, const T* 
     part[2 * (size_t)b + 1] = out[1];
     if (sc) {
       atomicMax(&sc->umax_slots[b & (UMAX_SLOTS - 1)], (unsigned long long)__double_as_longlong(out[0]));
-      ke_part[b] = out[1];
+      ke_part[4 * (size_t)b] = out[1];
+      ke_part[4 * (size_t)b + 1] = ke_part[4 * (size_t)b + 2] = ke_part[4 * (size_t)b + 3] = 0.0;
     }
   }
 }
@@ -776,16 +777,19 @@ int launch_rhs_plain(Ctx& c)
 
 // KernelGradP (PressureProjection.cpp:38-50): u += -dt/(2h) (p_E - p_W), etc.
 // PLAIN: in place on u (which holds u* after the advection copy).
-// FUSED: u = tmp + fac*grad p, plus the step's reductions: max |u+uinf| folded into the
-// device scalar by an atomic max (order independent, so deterministic) and the block's
-// sum |u|^2 into ke_part. Blocks a penalized body covers (skip_slot >= 0) leave the max to
-// the penalization kernel, which changes their velocity afterwards.
+// FUSED: u = tmp + fac*grad p, plus the step's reductions: max |u+uinf| over the
+// workgroup's blocks folded into the device scalar by one atomic max (order independent, so
+// deterministic), and each wave's sum |u|^2 of each block into ke_part[4b + wave]. Blocks a
+// penalized body covers (skip_slot >= 0) leave the max to the penalization kernel, which
+// changes their velocity afterwards.
 // Each thread owns two x-adjacent cells c = 2t, 2t+1 of the block (vector loads/stores);
 // a workgroup walks nbw blocks stacked in z, fetching the next block's p lab, tmp velocity
 // and forcing rows while the current one is computed.
 // FORCE adds the fused forcing's velocity modes, fac*u_low = R0 + 2 Re(R1 e^{i a x}) +
 // 2 Re(R2 e^{2 i a x}) per component with the row coefficients R of k_forcing_rows; the
-// block's 64 rows x 16 coefficients are staged in LDS by one coalesced 4-value load per thread.
+// block's 64 rows x 16 coefficients are staged in LDS by one coalesced 4-value load per
+// thread. A thread's x is the same in every block of its walk, so its e^{-i a x} and
+// e^{-2 i a x} are formed once.
 template <typename T, bool FUSED, bool FORCE>
 __global__ __launch_bounds__(NT) void k_gradp(const T* __restrict__ p, const T* __restrict__ su,
                                               const T* __restrict__ sv, const T* __restrict__ sw, T* __restrict__ u,
@@ -795,8 +799,9 @@ __global__ __launch_bounds__(NT) void k_gradp(const T* __restrict__ p, const T* 
                                               const T* __restrict__ rows, const C2<T>* __restrict__ twx, int nbw)
 {
   using V2 = typename Vec2<T>::type;
+  using V4 = typename Vec4<T>::type;
   __shared__ __align__(16) T lab[1][PLC];
-  __shared__ T rl[FORCE ? 64 * 16 : 1];
+  __shared__ __align__(16) T rl[FORCE ? 64 * 16 : 4];
   const int t = threadIdx.x;
   const LabSlots s = lab_slots();
   const T* src[1] = {p};
@@ -823,14 +828,22 @@ __global__ __launch_bounds__(NT) void k_gradp(const T* __restrict__ p, const T* 
     }
   };
   fetch(b0);
+  C2<T> e1[2], e2[2];
+  if (FORCE) {
+    const int x0 = (b0 % g.nbx) * BS + ix0;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      e1[k] = twx[(x0 + k) & (g.Nx - 1)];                                      // e^{-i a x}
+      e2[k] = C2<T>{e1[k].x * e1[k].x - e1[k].y * e1[k].y, 2 * e1[k].x * e1[k].y};  // e^{-2 i a x}
+    }
+  }
   const T fac = (T)(-0.5 * sc->dt / g.h);
+  double mx = 0.0;
   for (int j = 0; j < nbw; ++j) {
     const int b = b0 + j * nxy;
     if (j) __syncthreads();  // the previous block is done with the lab
     lab_put(r, lab, s);
-    if (FORCE)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) rl[4 * t + i] = rr[i];
+    if (FORCE) *reinterpret_cast<V4*>(rl + 4 * t) = V4{rr[0], rr[1], rr[2], rr[3]};
     T nu_[2] = {us[0][0], us[0][1]}, nv_[2] = {us[1][0], us[1][1]}, nw_[2] = {us[2][0], us[2][1]};
     __syncthreads();
     if (j + 1 < nbw) fetch(b + nxy);
@@ -849,17 +862,18 @@ __global__ __launch_bounds__(NT) void k_gradp(const T* __restrict__ p, const T* 
       nw_[1] += fac * (Bn.y - Fn.y);
     }
     if (FORCE) {
-      const int x0 = (b % g.nbx) * BS + ix0;
-      const T* R = rl + (iz * BS + iy) * 16;
+      // the row's 15 coefficients (3 components x {R0, Re R1, Im R1, Re R2, Im R2}) as four
+      // 16-byte LDS reads
+      const V4* R4 = reinterpret_cast<const V4*>(rl + (iz * BS + iy) * 16);
+      const V4 A = R4[0], B = R4[1], Cc = R4[2], D = R4[3];
+      const T Q[16] = {A.x, A.y, A.z, A.w, B.x, B.y, B.z, B.w, Cc.x, Cc.y, Cc.z, Cc.w, D.x, D.y, D.z, D.w};
 #pragma unroll
       for (int k = 0; k < 2; ++k) {
-        const C2<T> e1 = twx[(x0 + k) & (g.Nx - 1)];  // e^{-i a x}
-        const C2<T> e2 = {e1.x * e1.x - e1.y * e1.y, 2 * e1.x * e1.y};  // e^{-2 i a x}
         // Re(R e^{+i a x}) with e^{+i a x} = conj(e1)
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-          const T* Q = R + 5 * q;
-          const T add = Q[0] + 2 * (Q[1] * e1.x + Q[2] * e1.y) + 2 * (Q[3] * e2.x + Q[4] * e2.y);
+          const T* R = Q + 5 * q;
+          const T add = R[0] + 2 * (R[1] * e1[k].x + R[2] * e1[k].y) + 2 * (R[3] * e2[k].x + R[4] * e2[k].y);
           if (q == 0) nu_[k] += add;
           else if (q == 1) nv_[k] += add;
           else nw_[k] += add;
@@ -871,21 +885,22 @@ __global__ __launch_bounds__(NT) void k_gradp(const T* __restrict__ p, const T* 
     *reinterpret_cast<V2*>(v + o) = V2{nv_[0], nv_[1]};
     *reinterpret_cast<V2*>(w + o) = V2{nw_[0], nw_[1]};
     if (FUSED) {
-      double rd[2] = {0.0, 0.0};
+      double ke = 0.0;
 #pragma unroll
       for (int k = 0; k < 2; ++k) {
         const double aa = fabs((double)nu_[k] + Ux), b2 = fabs((double)nv_[k] + Uy), c2 = fabs((double)nw_[k] + Uz);
-        rd[0] = fmax(rd[0], fmax(aa, fmax(b2, c2)));
-        rd[1] += (double)nu_[k] * nu_[k] + (double)nv_[k] * nv_[k] + (double)nw_[k] * nw_[k];
+        if (!skip_slot || skip_slot[b] < 0) mx = fmax(mx, fmax(aa, fmax(b2, c2)));
+        ke += (double)nu_[k] * nu_[k] + (double)nv_[k] * nv_[k] + (double)nw_[k] * nw_[k];
       }
-      __shared__ double out[2];
-      block_reduce<2>(rd, 1u, out);
-      if (threadIdx.x == 0) {
-        if (!skip_slot || skip_slot[b] < 0)
-          atomicMax(&sc->umax_slots[b & (UMAX_SLOTS - 1)], (unsigned long long)__double_as_longlong(out[0]));
-        ke_part[b] = out[1];
-      }
+      ke = wave_sum(ke);
+      if ((t & 63) == 0) ke_part[4 * (size_t)b + (t >> 6)] = ke;
     }
+  }
+  if (FUSED) {
+    double m[1] = {mx};
+    __shared__ double out[1];
+    block_reduce<1>(m, 1u, out);
+    if (t == 0) atomicMax(&sc->umax_slots[b0 & (UMAX_SLOTS - 1)], (unsigned long long)__double_as_longlong(out[0]));
   }
 }
 
