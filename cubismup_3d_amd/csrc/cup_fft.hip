@@ -193,10 +193,12 @@ constexpr int XFFT = XROWS / 2;  // complex FFTs per workgroup (two real rows ea
 enum XSrc { XSRC_FIELD = 0, XSRC_DIV = 1 };
 
 // Real rows -> packed half spectrum. grid = (Ny/8, Nz).
-// FORCE (fused linear forcing): the same pass also sums, for the workgroup's 8 rows, the
-// DFT of the advected velocity at every canonical low mode and sum |u*|^2, written as
-// partials fpart[c*nwg + slot], c < W = 6 per mode (re/im of u, v, w) + 1: each row's
-// x-DFT at kx = 0..2, times the row's y-z phase of the mode.
+// FORCE (fused linear forcing): the same pass also forms, per grid row, the x-DFT at
+// kx = 0, 1, 2 of each advected velocity component and sum |u*|^2: 16 values per row
+// (component q's kx = 0, then re/im of kx = 1, 2 at 5q..5q+4; |u*|^2 at 15), written to
+// rowbuf[(z*Ny + y)*16 + slot]. The divergence already loads v on row y+1 and w on plane
+// z+1, so the workgroup of row (y, z) takes u of (y, z), v of (y+1, z) and w of (y, z+1);
+// every row of every component is summed by exactly one workgroup.
 // Each thread's cells lie in one row (t -> bx = t/64, iy, ix), so row sums are lane
 // shuffles over ix plus an LDS sum over the 4 waves.
 template <typename T, int N, int SRC, bool FORCE>
@@ -204,26 +206,15 @@ __global__ __launch_bounds__(NT) void k_fft_x_fwd(const T* __restrict__ src0, co
                                                   const T* __restrict__ src2, C2<T>* __restrict__ S,
                                                   const C2<T>* __restrict__ tw, GridArgs g,
                                                   const DevScalars* __restrict__ sc, const int* __restrict__ ob_slot,
-                                                  const T* __restrict__ ob_rhs, const LowModes* __restrict__ lmp,
-                                                  const C2<T>* __restrict__ twy, const C2<T>* __restrict__ twz,
-                                                  double* __restrict__ fpart)
+                                                  const T* __restrict__ ob_rhs, double* __restrict__ rowbuf)
 {
   __shared__ C2<T> buf[N * XFFT];
   constexpr int NA = 1 + 2 * LOW_KX;  // row DFT values per component: kx=0 (real), kx=1..: re, im
   static_assert(LOW_KX == 2, "the row DFT below evaluates kx = 1, 2");
+  static_assert(3 * NA + 1 == 16, "16 row values");
   __shared__ double rowacc[FORCE ? NT / 64 : 1][FORCE ? XROWS : 1][FORCE ? 3 * NA + 1 : 1];
   const int by = blockIdx.x, z = blockIdx.y;
   const int bz = z >> 3, iz = z & 7;
-  // FORCE: thread t < XROWS*LOW_MAX fetches, before the row loop so that the loads' latency
-  // hides behind it, the twiddles of row r = t / LOW_MAX and mode m = t % LOW_MAX
-  C2<T> pey{1, 0}, pez{1, 0};
-  if (FORCE && threadIdx.x < XROWS * LOW_MAX) {
-    const int r = threadIdx.x / LOW_MAX, m = threadIdx.x % LOW_MAX;
-    if (m < lmp->n) {
-      pey = twy[(lmp->k[m][1] * (by * BS + r)) & (g.Ny - 1)];
-      pez = twz[(lmp->k[m][2] * z) & (g.Nz - 1)];
-    }
-  }
   T fac = 0;
   if (SRC == XSRC_DIV) fac = (T)(0.5 * g.h * g.h / sc->dt);
   T* rb = (T*)buf;
@@ -257,20 +248,21 @@ __global__ __launch_bounds__(NT) void k_fft_x_fwd(const T* __restrict__ src0, co
       const long oe = half ? 3 + (bx + 1 < g.nbx ? sx : sx - (long)g.nbx * sx) - 7 : 4;
       const V4 uc = *reinterpret_cast<const V4*>(src0 + o);
       const T uu[6] = {src0[o + ow], uc.x, uc.y, uc.z, uc.w, src0[o + oe]};
+      const V4 vn = *reinterpret_cast<const V4*>(src1 + o + on), wb = *reinterpret_cast<const V4*>(src2 + o + ob);
+      const T vN[4] = {vn.x, vn.y, vn.z, vn.w}, wB[4] = {wb.x, wb.y, wb.z, wb.w};
       if (slot >= 0) {
         const V4 r = *reinterpret_cast<const V4*>(ob_rhs + (size_t)slot * BS3 + (iz * BS + iy) * BS + ix0);
         val[0] = r.x; val[1] = r.y; val[2] = r.z; val[3] = r.w;
       } else {
-        const V4 vn = *reinterpret_cast<const V4*>(src1 + o + on), vs = *reinterpret_cast<const V4*>(src1 + o + os);
-        const V4 wb = *reinterpret_cast<const V4*>(src2 + o + ob), wf = *reinterpret_cast<const V4*>(src2 + o + of);
-        const T vN[4] = {vn.x, vn.y, vn.z, vn.w}, vS[4] = {vs.x, vs.y, vs.z, vs.w};
-        const T wB[4] = {wb.x, wb.y, wb.z, wb.w}, wF[4] = {wf.x, wf.y, wf.z, wf.w};
+        const V4 vs = *reinterpret_cast<const V4*>(src1 + o + os), wf = *reinterpret_cast<const V4*>(src2 + o + of);
+        const T vS[4] = {vs.x, vs.y, vs.z, vs.w}, wF[4] = {wf.x, wf.y, wf.z, wf.w};
 #pragma unroll
         for (int k = 0; k < 4; ++k) val[k] = fac * (uu[k + 2] - uu[k] + vN[k] - vS[k] + wB[k] - wF[k]);
       }
       if (FORCE) {
-        const V4 vc = *reinterpret_cast<const V4*>(src1 + o), wc = *reinterpret_cast<const V4*>(src2 + o);
-        const T cv[4] = {vc.x, vc.y, vc.z, vc.w}, cw[4] = {wc.x, wc.y, wc.z, wc.w};
+        // u of this row, v of row y+1, w of plane z+1 (see above)
+        const T* cv = vN;
+        const T* cw = wB;
         // e^{-2 pi i x/Nx} at x0, rotated by the table's first entries for x0 + 1..3
         const C2<T> e0 = tw[x0 & (N - 1)];
 #pragma unroll
@@ -310,44 +302,14 @@ __global__ __launch_bounds__(NT) void k_fft_x_fwd(const T* __restrict__ src0, co
     }
   }
   __syncthreads();
-  if (FORCE) {
-    // rows' sums over the 4 waves (in wave order), the (row, mode) phases e^{-i(ky y + kz z)},
-    // then the workgroup's partial of every mode: column c = 6m + 2q + {re, im} (mode m,
-    // component q) and c = W-1 for sum |u|^2, written to fpart[wg*W + c]
-    __shared__ double rowsum[XROWS][3 * NA + 1];
-    __shared__ double ph[XROWS][LOW_MAX][2];
-    for (int t = threadIdx.x; t < XROWS * (3 * NA + 1); t += NT) {
-      const int r = t / (3 * NA + 1), q = t % (3 * NA + 1);
-      rowsum[r][q] = rowacc[0][r][q] + rowacc[1][r][q] + rowacc[2][r][q] + rowacc[3][r][q];
-    }
-    if (threadIdx.x < XROWS * LOW_MAX) {
-      const int r = threadIdx.x / LOW_MAX, m = threadIdx.x % LOW_MAX;
-      ph[r][m][0] = (double)pey.x * pez.x - (double)pey.y * pez.y;
-      ph[r][m][1] = (double)pey.x * pez.y + (double)pey.y * pez.x;
-    }
-    __syncthreads();
-    const LowModes& lm = *lmp;
-    // column-major partials, fpart[c*nwg + slot]: the slot is the XCD-remapped workgroup
-    // index, so the workgroups of one XCD fill contiguous runs of each column in its L2
-    const int nwg = gridDim.x * gridDim.y, W = 6 * lm.n + 1;
-    const int slot = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, nwg);
-    for (int t = threadIdx.x; t < W; t += NT) {
-      double outv = 0;
-      if (t == W - 1) {
-        for (int r = 0; r < XROWS; ++r) outv += rowsum[r][3 * NA];
-      } else {
-        const int m = t / 6, q = (t % 6) >> 1, im = t & 1;
-        const int kx = lm.k[m][0], ak = kx < 0 ? -kx : kx;
-        const int cr = ak == 0 ? q * NA : q * NA + 2 * ak - 1;
-        const double sg = kx < 0 ? -1.0 : 1.0;
-        for (int r = 0; r < XROWS; ++r) {
-          const double Ur = rowsum[r][cr], Ui = ak == 0 ? 0.0 : sg * rowsum[r][cr + 1];
-          const double er = ph[r][m][0], ei = ph[r][m][1];
-          outv += im ? (Ur * ei + Ui * er) : (Ur * er - Ui * ei);
-        }
-      }
-      fpart[(size_t)t * nwg + slot] = outv;
-    }
+  if (FORCE && threadIdx.x < XROWS * (3 * NA + 1)) {
+    // the rows' sums over the 4 waves, in wave order, to the row each component belongs to
+    const int r = threadIdx.x >> 4, q = threadIdx.x & 15;
+    const double v = rowacc[0][r][q] + rowacc[1][r][q] + rowacc[2][r][q] + rowacc[3][r][q];
+    const int y = by * BS + r;
+    const int yy = (q >= NA && q < 2 * NA) ? (y + 1 < g.Ny ? y + 1 : 0) : y;
+    const int zz = (q >= 2 * NA && q < 3 * NA) ? (z + 1 < g.Nz ? z + 1 : 0) : z;
+    rowbuf[((size_t)zz * g.Ny + yy) * 16 + q] = v;
   }
   lds_fft<N, XFFT, -1>(buf, tw);
   // Separate the two real transforms of each complex FFT and pack kx = N/2 into slot 0.
@@ -676,27 +638,24 @@ struct XPass {
   {
     dim3 grid(c.N[1] / BS, c.N[2]);
     const int* slot = c.body.present && !c.plain_rhs ? c.body.ob_slot : nullptr;
-    const C2<T>* twy = (const C2<T>*)c.tw[1];
-    const C2<T>* twz = (const C2<T>*)c.tw[2];
     if (src == XSRC_FIELD)
       hipLaunchKernelGGL((k_fft_x_fwd<T, N, XSRC_FIELD, false>), grid, dim3(NT), 0, c.stream, s0, s1, s2,
                          (C2<T>*)c.spec, (const C2<T>*)c.tw[0], c.grid(), c.sc, (const int*)nullptr,
-                         (const T*)nullptr, c.low_dev, twy, twz, (double*)nullptr);
+                         (const T*)nullptr, (double*)nullptr);
     else if (force)
       hipLaunchKernelGGL((k_fft_x_fwd<T, N, XSRC_DIV, true>), grid, dim3(NT), 0, c.stream, s0, s1, s2, (C2<T>*)c.spec,
-                         (const C2<T>*)c.tw[0], c.grid(), c.sc, slot, (const T*)c.body.ob_rhs, c.low_dev, twy, twz,
-                         c.low_part);
+                         (const C2<T>*)c.tw[0], c.grid(), c.sc, slot, (const T*)c.body.ob_rhs, c.low_part);
     else
       hipLaunchKernelGGL((k_fft_x_fwd<T, N, XSRC_DIV, false>), grid, dim3(NT), 0, c.stream, s0, s1, s2,
                          (C2<T>*)c.spec, (const C2<T>*)c.tw[0], c.grid(), c.sc, slot, (const T*)c.body.ob_rhs,
-                         c.low_dev, twy, twz, (double*)nullptr);
+                         (double*)nullptr);
   }
   static void fwd_to(Ctx& c, const T* s0, C2<T>* spec)
   {
     dim3 grid(c.N[1] / BS, c.N[2]);
     hipLaunchKernelGGL((k_fft_x_fwd<T, N, XSRC_FIELD, false>), grid, dim3(NT), 0, c.stream, s0, s0, s0, spec,
-                       (const C2<T>*)c.tw[0], c.grid(), c.sc, (const int*)nullptr, (const T*)nullptr, c.low_dev,
-                       (const C2<T>*)c.tw[1], (const C2<T>*)c.tw[2], (double*)nullptr);
+                       (const C2<T>*)c.tw[0], c.grid(), c.sc, (const int*)nullptr, (const T*)nullptr,
+                       (double*)nullptr);
   }
   static void inv(Ctx& c, const C2<T>* spec, T* dst)
   {

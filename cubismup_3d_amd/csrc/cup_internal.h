@@ -118,8 +118,8 @@ struct Ctx {
   bool low_fused_ok = false;   // modes fit LowModes (count, |kx|)
   LowModes low{};
   LowModes* low_dev = nullptr;  // device copy (kernels index it per lane)
-  double* low_part = nullptr;  // x-pass partials, column-major: 6 per mode (re/im of u,v,w DFT) + 1 (sum |u|^2)
-  double* low_red = nullptr;   // reduced sums, then the per-chunk sums of the first reduction stage
+  double* low_part = nullptr;  // x-pass row values [Nz][Ny][16]: per component the row's x-DFT at kx 0..2, |u|^2
+  double* low_red = nullptr;   // plane DFTs in y [Nz][16][5], then the W mode sums (6 per mode: re/im of u,v,w; |u|^2)
   double* low_coef = nullptr;  // per mode: RHS addition (re,im), synthesis of u,v,w (re,im) x3
   unsigned* low_counter = nullptr;
   void* low_rows = nullptr;    // per grid row (y, z): 16 synthesis coefficients (context precision)

@@ -356,9 +356,9 @@ int spectral_setup(Ctx& c)
     else { add(0, ky, kz, 0); add(0, -ky, -kz, 1); }
   }
   const int W = 6 * c.nlow + 1;
-  const size_t nwg = (size_t)(c.N[1] / BS) * c.N[2];
-  if (check_hip(c, hipMalloc(&c.low_part, (size_t)W * nwg * sizeof(double)), "hipMalloc low partials") ||
-      check_hip(c, hipMalloc(&c.low_red, (size_t)W * sizeof(double)), "hipMalloc low red") ||
+  const size_t nrow = (size_t)c.N[1] * c.N[2];
+  if (check_hip(c, hipMalloc(&c.low_part, nrow * 16 * sizeof(double)), "hipMalloc low row values") ||
+      check_hip(c, hipMalloc(&c.low_red, ((size_t)c.N[2] * 16 * 5 + W) * sizeof(double)), "hipMalloc low red") ||
       check_hip(c, hipMalloc(&c.low_coef, (size_t)c.nlow * 8 * sizeof(double)), "hipMalloc low coef") ||
       check_hip(c, hipMalloc(&c.low_counter, sizeof(unsigned)), "hipMalloc counter") ||
       check_hip(c, hipMalloc(&c.low_dev, sizeof(LowModes)), "hipMalloc low modes") ||
@@ -374,57 +374,121 @@ int spectral_setup(Ctx& c)
   return CUP_OK;
 }
 
-// Fused forcing, stage 2: one workgroup per column of the x-pass partials (column-major, a
-// column is nwg contiguous values). Each thread sums its entries j = t, t + NT, ... in order,
-// issuing FR_BATCH independent coalesced loads at a time so that a column costs a few memory
-// latencies; the
-// workgroup's sums are combined by a fixed shuffle tree (deterministic). The last workgroup
-// to finish computes, on the device, SpectralForcing's injection and factor
-// (SpectralForcing.cpp:37-77) from tke = <|u*|^2>/2 (Parseval) and tke_low = sum over the
-// low modes, and the coefficient tables: the forcing's pressure-RHS modes
+// Fused forcing, stage 2 (plane DFTs in y): one workgroup per z plane over the x pass's
+// row values R(y, z, s) (16 real values per row, rowbuf[(z*Ny + y)*16 + s]). For each slot s
+// it forms P(z, s) = sum_y R e^{-i ky y} at ky = 0, 1, 2 (5 real values: the ky = 0 sum, then
+// re/im at ky = 1 and 2; ky < 0 are their conjugates, the slots being real) into
+// plane[(z*16 + s)*5 + k]. Thread t takes slot t % 16 and rows y = t/16 (mod 16); the 16
+// row groups are added in a fixed order.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_forcing_planes(const double* __restrict__ rowbuf, double* __restrict__ plane,
+                                                       int Ny, const C2<T>* __restrict__ twy)
+{
+  static_assert(NT == 256, "16 slots x 16 row groups");
+  __shared__ double acc[16][16][5];
+  const int t = threadIdx.x, z = blockIdx.x, sl = t & 15, grp = t >> 4;
+  double a[5] = {0, 0, 0, 0, 0};
+#pragma unroll 4
+  for (int y = grp; y < Ny; y += 16) {
+    const double R = rowbuf[((size_t)z * Ny + y) * 16 + sl];
+    const C2<T> e1 = twy[y], e2 = twy[(2 * y) & (Ny - 1)];  // e^{-i a y}, e^{-2 i a y}
+    a[0] += R;
+    a[1] += R * (double)e1.x;
+    a[2] += R * (double)e1.y;
+    a[3] += R * (double)e2.x;
+    a[4] += R * (double)e2.y;
+  }
+#pragma unroll
+  for (int k = 0; k < 5; ++k) acc[grp][sl][k] = a[k];
+  __syncthreads();
+  if (t < 16 * 5) {
+    const int s2 = t / 5, k = t % 5;
+    double v = 0;
+    for (int g2 = 0; g2 < 16; ++g2) v += acc[g2][s2][k];
+    plane[((size_t)z * 16 + s2) * 5 + k] = v;
+  }
+}
+
+// Fused forcing, stage 3 (the z sums): one workgroup per column owner, (mode m, component q)
+// for o < 3n and the sum |u*|^2 for o = 3n. For mode m = (kx, ky, kz) and component q, the
+// plane value is U(z) = Pr + i sg Pi, with Pr, Pi the plane DFTs (at ky, conjugated for
+// ky < 0) of the slots holding re and im of the x-DFT at |kx| (sg = -1 for kx < 0, the
+// conjugate; Pi = 0 at kx = 0); the owner sums U(z) e^{-i kz z} over z, threads over planes
+// and a fixed shuffle tree (deterministic), into red[6m + 2q + {re, im}] (red[W-1] for
+// |u*|^2). The last workgroup to finish computes, on the device, SpectralForcing's injection
+// and factor (SpectralForcing.cpp:37-77) from tke = <|u*|^2>/2 (Parseval) and tke_low = sum
+// over the low modes, and the coefficient tables: the forcing's pressure-RHS modes
 // fac*(h^2/dt)*i*sum_d sin(2 pi k_d/N_d) u_d(k) (added in the z pass) and the velocity modes
 // fac/N^3 u(k) (added in the gradient pass).
-constexpr int FR_BATCH = 16;
-
-__global__ __launch_bounds__(NT) void k_forcing_reduce(const double* __restrict__ part, int nwg, int W,
+template <typename T>
+__global__ __launch_bounds__(NT) void k_forcing_reduce(const double* __restrict__ plane, int W,
                                                        double* __restrict__ red, unsigned* __restrict__ counter,
                                                        const LowModes* __restrict__ lmp, DevScalars* __restrict__ sc,
                                                        double* __restrict__ coef, double tke_target, double inj_rate,
-                                                       double h, double ncell, int N0, int N1, int N2)
+                                                       double h, double ncell, int N0, int N1, int N2,
+                                                       const C2<T>* __restrict__ twz)
 {
-  __shared__ double wsum[NT / 64];
+  __shared__ double wsum[NT / 64][2];
   __shared__ double fin[6 * LOW_MAX + 1];
   __shared__ double facs[1];
   __shared__ bool last;
-  const int t = threadIdx.x, col = blockIdx.x;
-  double a = 0;
-  for (int j0 = t; j0 < nwg; j0 += FR_BATCH * NT) {
-    double v[FR_BATCH];
-#pragma unroll
-    for (int q = 0; q < FR_BATCH; ++q) {
-      const int j = j0 + q * NT;
-      v[q] = j < nwg ? part[(size_t)col * nwg + j] : 0.0;
+  const LowModes& lm = *lmp;
+  const int t = threadIdx.x, o = blockIdx.x, nq = 3 * lm.n;
+  double ar = 0, ai = 0;
+  if (o == nq) {
+    for (int z = t; z < N2; z += NT) ar += plane[((size_t)z * 16 + 15) * 5];
+  } else {
+    const int m = o / 3, q = o % 3;
+    const int kx = lm.k[m][0], ky = lm.k[m][1], kz = lm.k[m][2];
+    const int ak = kx < 0 ? -kx : kx, aky = ky < 0 ? -ky : ky;
+    const int sre = ak == 0 ? 5 * q : 5 * q + 2 * ak - 1;
+    const double sg = kx < 0 ? -1.0 : 1.0, cy = ky < 0 ? -1.0 : 1.0;
+    for (int z = t; z < N2; z += NT) {
+      const double* P = plane + (size_t)z * 16 * 5;
+      // plane DFT of slot s at ky: (P[s][0], 0) for ky = 0, else (P[s][2|ky|-1], cy P[s][2|ky|])
+      const double rr = aky == 0 ? P[sre * 5] : P[sre * 5 + 2 * aky - 1];
+      const double ri = aky == 0 ? 0.0 : cy * P[sre * 5 + 2 * aky];
+      double ir = 0, ii = 0;
+      if (ak != 0) {
+        ir = aky == 0 ? P[(sre + 1) * 5] : P[(sre + 1) * 5 + 2 * aky - 1];
+        ii = aky == 0 ? 0.0 : cy * P[(sre + 1) * 5 + 2 * aky];
+      }
+      // U = (rr + i ri) + i sg (ir + i ii)
+      const double Ur = rr - sg * ii, Ui = ri + sg * ir;
+      const C2<T> e = twz[(kz * z) & (N2 - 1)];
+      const double er = (double)e.x, ei = (double)e.y;
+      ar += Ur * er - Ui * ei;
+      ai += Ur * ei + Ui * er;
     }
-#pragma unroll
-    for (int q = 0; q < FR_BATCH; ++q) a += v[q];
   }
-  a = wave_sum(a);
-  if ((t & 63) == 0) wsum[t >> 6] = a;
+  ar = wave_sum(ar);
+  ai = wave_sum(ai);
+  if ((t & 63) == 0) {
+    wsum[t >> 6][0] = ar;
+    wsum[t >> 6][1] = ai;
+  }
   __syncthreads();
   if (t == 0) {
-    double tot = 0;
-    for (int w = 0; w < NT / 64; ++w) tot += wsum[w];
-    red[col] = tot;
+    double sr = 0, si = 0;
+    for (int w = 0; w < NT / 64; ++w) {
+      sr += wsum[w][0];
+      si += wsum[w][1];
+    }
+    if (o == nq) {
+      red[W - 1] = sr;
+    } else {
+      red[2 * o] = sr;  // column 6m + 2q = 2o
+      red[2 * o + 1] = si;
+    }
     __threadfence();
-    last = atomicAdd(counter, 1u) == (unsigned)(W - 1);
+    last = atomicAdd(counter, 1u) == (unsigned)(gridDim.x - 1);
   }
   __syncthreads();
   if (!last) return;
-  __threadfence();  // acquire: the other workgroups' column sums are visible to plain loads
+  __threadfence();  // acquire: the other workgroups' sums are visible to plain loads
   if (t < W) fin[t] = red[t];
   __syncthreads();
   const double* r = fin;
-  const LowModes& lm = *lmp;
   const double dt = sc->dt;
   const double norm = (double)N0 * N1 * N2;
   if (t == 0) {
@@ -471,10 +535,22 @@ __global__ __launch_bounds__(NT) void k_forcing_reduce(const double* __restrict_
 int launch_forcing_reduce(Ctx& c, double tke_target, double inj_rate)
 {
   const int W = 6 * c.nlow + 1;
-  const int nwg = (c.N[1] / BS) * c.N[2];  // x-pass workgroups
-  hipLaunchKernelGGL(k_forcing_reduce, dim3(W), dim3(NT), 0, c.stream, (const double*)c.low_part, nwg, W, c.low_red,
-                     c.low_counter, c.low_dev, c.sc, c.low_coef, tke_target, inj_rate, c.h, (double)c.ncell, c.N[0],
-                     c.N[1], c.N[2]);
+  const int nz = c.N[2];
+  double* plane = c.low_red;  // [Nz][16][5] plane DFTs, then the W column sums
+  double* red = plane + (size_t)nz * 16 * 5;
+  if (c.precision == 4) {
+    hipLaunchKernelGGL((k_forcing_planes<float>), dim3(nz), dim3(NT), 0, c.stream, (const double*)c.low_part, plane,
+                       c.N[1], (const C2<float>*)c.tw[1]);
+    hipLaunchKernelGGL((k_forcing_reduce<float>), dim3(3 * c.nlow + 1), dim3(NT), 0, c.stream, (const double*)plane, W,
+                       red, c.low_counter, c.low_dev, c.sc, c.low_coef, tke_target, inj_rate, c.h, (double)c.ncell,
+                       c.N[0], c.N[1], c.N[2], (const C2<float>*)c.tw[2]);
+  } else {
+    hipLaunchKernelGGL((k_forcing_planes<double>), dim3(nz), dim3(NT), 0, c.stream, (const double*)c.low_part, plane,
+                       c.N[1], (const C2<double>*)c.tw[1]);
+    hipLaunchKernelGGL((k_forcing_reduce<double>), dim3(3 * c.nlow + 1), dim3(NT), 0, c.stream, (const double*)plane,
+                       W, red, c.low_counter, c.low_dev, c.sc, c.low_coef, tke_target, inj_rate, c.h,
+                       (double)c.ncell, c.N[0], c.N[1], c.N[2], (const C2<double>*)c.tw[2]);
+  }
   return check_hip(c, hipGetLastError(), "k_forcing_reduce");
 }
 
