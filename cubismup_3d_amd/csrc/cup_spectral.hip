@@ -7,6 +7,15 @@
 
 namespace cup {
 
+// The forcing's plane stage splits each z plane into up to FR_PARTS_MAX row ranges of at
+// least 64 rows (Ny is a power of two), one workgroup each.
+constexpr int FR_PARTS_MAX = 4;
+inline int forcing_plane_parts(int ny)
+{
+  const int p = ny / 64;
+  return p < 1 ? 1 : (p > FR_PARTS_MAX ? FR_PARTS_MAX : p);
+}
+
 __device__ __forceinline__ int signed_k(int i, int n) { return i <= n / 2 ? i : i - n; }
 
 // Philox-4x32-10 (Salmon, Moraes, Dror, Shaw: "Parallel random numbers: as easy as
@@ -358,7 +367,8 @@ int spectral_setup(Ctx& c)
   const int W = 6 * c.nlow + 1;
   const size_t nrow = (size_t)c.N[1] * c.N[2];
   if (check_hip(c, hipMalloc(&c.low_part, nrow * 16 * sizeof(double)), "hipMalloc low row values") ||
-      check_hip(c, hipMalloc(&c.low_red, ((size_t)c.N[2] * 16 * 5 + W) * sizeof(double)), "hipMalloc low red") ||
+      check_hip(c, hipMalloc(&c.low_red, ((size_t)c.N[2] * FR_PARTS_MAX * 16 * 5 + W) * sizeof(double)),
+                "hipMalloc low red") ||
       check_hip(c, hipMalloc(&c.low_coef, (size_t)c.nlow * 8 * sizeof(double)), "hipMalloc low coef") ||
       check_hip(c, hipMalloc(&c.low_counter, sizeof(unsigned)), "hipMalloc counter") ||
       check_hip(c, hipMalloc(&c.low_dev, sizeof(LowModes)), "hipMalloc low modes") ||
@@ -374,12 +384,12 @@ int spectral_setup(Ctx& c)
   return CUP_OK;
 }
 
-// Fused forcing, stage 2 (plane DFTs in y): one workgroup per z plane over the x pass's
-// row values R(y, z, s) (16 real values per row, rowbuf[(z*Ny + y)*16 + s]). For each slot s
-// it forms P(z, s) = sum_y R e^{-i ky y} at ky = 0, 1, 2 (5 real values: the ky = 0 sum, then
-// re/im at ky = 1 and 2; ky < 0 are their conjugates, the slots being real) into
-// plane[(z*16 + s)*5 + k]. Thread t takes slot t % 16 and rows y = t/16 (mod 16); the 16
-// row groups are added in a fixed order.
+// Fused forcing, stage 2 (plane DFTs in y): workgroup (z, part) over the x pass's row values
+// R(y, z, s) (16 real values per row, rowbuf[(z*Ny + y)*16 + s]) of rows y in part's quarter
+// (gridDim.y parts). For each slot s it forms P(z, part, s) = sum_y R e^{-i ky y} at
+// ky = 0, 1, 2 (5 real values: the ky = 0 sum, then re/im at ky = 1 and 2; ky < 0 are their
+// conjugates, the slots being real) into plane[((z*parts + part)*16 + s)*5 + k]. Thread t takes
+// slot t % 16 and every 16th row from t/16; the 16 row groups are added in a fixed order.
 template <typename T>
 __global__ __launch_bounds__(NT) void k_forcing_planes(const double* __restrict__ rowbuf, double* __restrict__ plane,
                                                        int Ny, const C2<T>* __restrict__ twy)
@@ -387,9 +397,10 @@ __global__ __launch_bounds__(NT) void k_forcing_planes(const double* __restrict_
   static_assert(NT == 256, "16 slots x 16 row groups");
   __shared__ double acc[16][16][5];
   const int t = threadIdx.x, z = blockIdx.x, sl = t & 15, grp = t >> 4;
+  const int parts = gridDim.y, ny = Ny / parts, y0 = blockIdx.y * ny;
   double a[5] = {0, 0, 0, 0, 0};
 #pragma unroll 4
-  for (int y = grp; y < Ny; y += 16) {
+  for (int y = y0 + grp; y < y0 + ny; y += 16) {
     const double R = rowbuf[((size_t)z * Ny + y) * 16 + sl];
     const C2<T> e1 = twy[y], e2 = twy[(2 * y) & (Ny - 1)];  // e^{-i a y}, e^{-2 i a y}
     a[0] += R;
@@ -405,7 +416,7 @@ __global__ __launch_bounds__(NT) void k_forcing_planes(const double* __restrict_
     const int s2 = t / 5, k = t % 5;
     double v = 0;
     for (int g2 = 0; g2 < 16; ++g2) v += acc[g2][s2][k];
-    plane[((size_t)z * 16 + s2) * 5 + k] = v;
+    plane[(((size_t)z * parts + blockIdx.y) * 16 + s2) * 5 + k] = v;
   }
 }
 
@@ -426,7 +437,7 @@ __global__ __launch_bounds__(NT) void k_forcing_reduce(const double* __restrict_
                                                        const LowModes* __restrict__ lmp, DevScalars* __restrict__ sc,
                                                        double* __restrict__ coef, double tke_target, double inj_rate,
                                                        double h, double ncell, int N0, int N1, int N2,
-                                                       const C2<T>* __restrict__ twz)
+                                                       const C2<T>* __restrict__ twz, int parts)
 {
   __shared__ double wsum[NT / 64][2];
   __shared__ double fin[6 * LOW_MAX + 1];
@@ -435,16 +446,19 @@ __global__ __launch_bounds__(NT) void k_forcing_reduce(const double* __restrict_
   const LowModes& lm = *lmp;
   const int t = threadIdx.x, o = blockIdx.x, nq = 3 * lm.n;
   double ar = 0, ai = 0;
+  // plane partials zp = z*parts + part
+  const int nzp = N2 * parts;
   if (o == nq) {
-    for (int z = t; z < N2; z += NT) ar += plane[((size_t)z * 16 + 15) * 5];
+    for (int zp = t; zp < nzp; zp += NT) ar += plane[((size_t)zp * 16 + 15) * 5];
   } else {
     const int m = o / 3, q = o % 3;
     const int kx = lm.k[m][0], ky = lm.k[m][1], kz = lm.k[m][2];
     const int ak = kx < 0 ? -kx : kx, aky = ky < 0 ? -ky : ky;
     const int sre = ak == 0 ? 5 * q : 5 * q + 2 * ak - 1;
     const double sg = kx < 0 ? -1.0 : 1.0, cy = ky < 0 ? -1.0 : 1.0;
-    for (int z = t; z < N2; z += NT) {
-      const double* P = plane + (size_t)z * 16 * 5;
+    for (int zp = t; zp < nzp; zp += NT) {
+      const int z = zp / parts;
+      const double* P = plane + (size_t)zp * 16 * 5;
       // plane DFT of slot s at ky: (P[s][0], 0) for ky = 0, else (P[s][2|ky|-1], cy P[s][2|ky|])
       const double rr = aky == 0 ? P[sre * 5] : P[sre * 5 + 2 * aky - 1];
       const double ri = aky == 0 ? 0.0 : cy * P[sre * 5 + 2 * aky];
@@ -536,20 +550,21 @@ int launch_forcing_reduce(Ctx& c, double tke_target, double inj_rate)
 {
   const int W = 6 * c.nlow + 1;
   const int nz = c.N[2];
-  double* plane = c.low_red;  // [Nz][16][5] plane DFTs, then the W column sums
-  double* red = plane + (size_t)nz * 16 * 5;
+  const int parts = forcing_plane_parts(c.N[1]);
+  double* plane = c.low_red;  // [Nz][parts][16][5] plane DFTs, then the W column sums
+  double* red = plane + (size_t)nz * FR_PARTS_MAX * 16 * 5;
   if (c.precision == 4) {
-    hipLaunchKernelGGL((k_forcing_planes<float>), dim3(nz), dim3(NT), 0, c.stream, (const double*)c.low_part, plane,
-                       c.N[1], (const C2<float>*)c.tw[1]);
+    hipLaunchKernelGGL((k_forcing_planes<float>), dim3(nz, parts), dim3(NT), 0, c.stream, (const double*)c.low_part,
+                       plane, c.N[1], (const C2<float>*)c.tw[1]);
     hipLaunchKernelGGL((k_forcing_reduce<float>), dim3(3 * c.nlow + 1), dim3(NT), 0, c.stream, (const double*)plane, W,
                        red, c.low_counter, c.low_dev, c.sc, c.low_coef, tke_target, inj_rate, c.h, (double)c.ncell,
-                       c.N[0], c.N[1], c.N[2], (const C2<float>*)c.tw[2]);
+                       c.N[0], c.N[1], c.N[2], (const C2<float>*)c.tw[2], parts);
   } else {
-    hipLaunchKernelGGL((k_forcing_planes<double>), dim3(nz), dim3(NT), 0, c.stream, (const double*)c.low_part, plane,
-                       c.N[1], (const C2<double>*)c.tw[1]);
+    hipLaunchKernelGGL((k_forcing_planes<double>), dim3(nz, parts), dim3(NT), 0, c.stream, (const double*)c.low_part,
+                       plane, c.N[1], (const C2<double>*)c.tw[1]);
     hipLaunchKernelGGL((k_forcing_reduce<double>), dim3(3 * c.nlow + 1), dim3(NT), 0, c.stream, (const double*)plane,
                        W, red, c.low_counter, c.low_dev, c.sc, c.low_coef, tke_target, inj_rate, c.h,
-                       (double)c.ncell, c.N[0], c.N[1], c.N[2], (const C2<double>*)c.tw[2]);
+                       (double)c.ncell, c.N[0], c.N[1], c.N[2], (const C2<double>*)c.tw[2], parts);
   }
   return check_hip(c, hipGetLastError(), "k_forcing_reduce");
 }
