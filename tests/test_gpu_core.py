@@ -52,8 +52,10 @@ def test_advdiff_fp64_parity(cup, oracle, bpd):
 
 
 @pytest.mark.parametrize("decomp", ["plain", "fused"])
-@pytest.mark.parametrize("bpd", [(2, 2, 2), (4, 2, 1), (1, 2, 4), (8, 8, 8)])
+@pytest.mark.parametrize("bpd", [(2, 2, 2), (4, 2, 1), (1, 2, 4), (8, 8, 8), (4, 16, 16), (2, 32, 32)])
 def test_projection_fp64_parity(cup, oracle, decomp, bpd):
+    """RHS, Poisson and gradient against the oracle; y/z FFTs of 8 to 256 points (64 and up
+    take the register four-step form: 4, 8, 16 threads per column)."""
     print(f"projection parity decomp={decomp} bpd={bpd}", flush=True)
     rng = np.random.default_rng(5)
     G = cup.Grid(bpd, precision=8, decomposition=decomp)

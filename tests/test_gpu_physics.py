@@ -59,7 +59,7 @@ def test_forcing_matches_oracle(cup, oracle, decomp):
 
 def test_c2_style_forced_steps_fp64(cup, oracle):
     """Forced isotropic turbulence, CFL-driven, 8 steps on 32^3: plain device path vs oracle."""
-    bpd = (4, 4, 4)
+    bpd, nsteps = (4, 4, 4), 8
     G = cup.Grid(bpd, precision=8, decomposition="plain")
     O = oracle.OracleSim(bpd)
     G.ic_hit(4.0, 1.0, 20260101)
@@ -67,9 +67,9 @@ def test_c2_style_forced_steps_fp64(cup, oracle):
     u, v, w = O.velocity()
     tgt = 0.5 * (u ** 2 + v ** 2 + w ** 2).mean()
     nu, CFL = 0.005, 0.3
-    G.run_steps(8, CFL=CFL, nu=nu, spectral_forcing=True, tke_target=tgt)
+    G.run_steps(nsteps, CFL=CFL, nu=nu, spectral_forcing=True, tke_target=tgt)
     t = 0
-    for step in range(8):
+    for step in range(nsteps):
         dt, _ = O.calc_max_timestep(CFL, nu, (0, 0, 0), step, 0)
         O.timestep(dt, nu, spectral_forcing=True, tke_target=tgt)
         t += dt
@@ -152,10 +152,12 @@ def test_sphere_steps_match_oracle(cup, oracle, decomp):
     assert qg[15] == pytest.approx(qo[15], rel=1e-9)
 
 
-@pytest.mark.parametrize("bpd", [(4, 4, 4), (8, 4, 2)])
-def test_fused_forcing_steps_match_oracle(cup, oracle, bpd):
-    """The device-resident forcing (low-mode DFT in the x pass, RHS modes added in the z
-    pass, velocity modes in the gradient pass) reproduces SpectralForcing's step."""
+@pytest.mark.parametrize("bpd,nsteps", [((4, 4, 4), 8), ((8, 4, 2), 8), ((8, 8, 8), 4), ((4, 16, 16), 3)])
+def test_fused_forcing_steps_match_oracle(cup, oracle, bpd, nsteps):
+    """The device-resident forcing (row DFT values in the x pass, plane and mode sums, RHS
+    modes added in the z pass, velocity modes in the gradient pass) reproduces
+    SpectralForcing's step; 64^3 and 32 x 128 x 128 take the register column FFTs (4 and 8
+    threads per column) with the forcing modes added there."""
     print(f"fused forcing bpd={bpd}", flush=True)
     G = cup.Grid(bpd, precision=8, decomposition="fused")
     O = oracle.OracleSim(bpd)
@@ -164,9 +166,9 @@ def test_fused_forcing_steps_match_oracle(cup, oracle, bpd):
     u, v, w = O.velocity()
     tgt = 1.05 * 0.5 * (u ** 2 + v ** 2 + w ** 2).mean()
     nu, CFL = 0.005, 0.3
-    G.run_steps(8, CFL=CFL, nu=nu, spectral_forcing=True, tke_target=tgt)
+    G.run_steps(nsteps, CFL=CFL, nu=nu, spectral_forcing=True, tke_target=tgt)
     t = 0
-    for step in range(8):
+    for step in range(nsteps):
         dt, _ = O.calc_max_timestep(CFL, nu, (0, 0, 0), step, 0)
         O.timestep(dt, nu, spectral_forcing=True, tke_target=tgt)
         t += dt
