@@ -28,3 +28,22 @@ def test_fp32_within_1e5_of_fp64_after_100_steps(cup, scheme):
     assert s4["step"] == s8["step"]
     assert abs(s4["time"] - s8["time"]) < 1e-5 * s8["time"]
     assert rel < 1e-5
+    # every output against the FP64 form: fields (max error over max |ref|, relative L2) and
+    # the scalars (relative error): step time, max |u|, kinetic energy and the 20 dissipation
+    # integrals of ComputeDissipation
+    print(f"scheme {scheme}: FP32 vs FP64 after {nsteps} steps, 256^3", flush=True)
+    for n in ("u", "v", "w", "p"):
+        a, b = G4.get(n), G8.get(n)
+        emax = np.abs(a - b).max() / np.abs(b).max()
+        el2 = np.linalg.norm(a - b) / np.linalg.norm(b)
+        print(f"  {n:8s} max {emax:.2e}  L2 {el2:.2e}", flush=True)
+        assert el2 < (1e-5 if n != "p" else 1e-4) and emax < 1e-4
+    for k in ("time", "umax", "ke"):
+        e = abs(s4[k] - s8[k]) / abs(s8[k])
+        print(f"  {k:8s} rel {e:.2e}", flush=True)
+        assert e < 1e-5
+    d4, d8 = G4.diagnostics(0.005), G8.diagnostics(0.005)
+    scale = np.abs(d8).max()
+    for i, (x, y) in enumerate(zip(d4, d8)):
+        print(f"  diag[{i:2d}] fp64 {y: .6e} rel {abs(x - y) / max(abs(y), 1e-12 * scale):.2e}", flush=True)
+    assert np.all(np.abs(d4 - d8) <= 1e-4 * np.maximum(np.abs(d8), 1e-6 * scale))
