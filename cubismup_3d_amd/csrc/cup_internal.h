@@ -61,6 +61,7 @@ struct LowModes {
   int k[LOW_MAX][3];      // (kx, ky, kz) integer wavenumbers
   int npos;               // packed positions: 1 per mode with kx != 0, 2 with kx == 0
   int pos[2 * LOW_MAX][4];  // slot (|kx|), ky index, kz index, (mode << 1) | conjugate
+  double sk[LOW_MAX][3];    // sin(2 pi k_d / N_d): the central-difference symbol of each mode
 };
 
 // Geometry passed by value to kernels.

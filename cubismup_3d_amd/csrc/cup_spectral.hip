@@ -356,6 +356,7 @@ int spectral_setup(Ctx& c)
   for (int m = 0; m < c.nlow; ++m) {
     const int kx = ks[3 * m], ky = ks[3 * m + 1], kz = ks[3 * m + 2];
     L.k[m][0] = kx; L.k[m][1] = ky; L.k[m][2] = kz;
+    for (int d = 0; d < 3; ++d) L.sk[m][d] = std::sin(2.0 * M_PI * ks[3 * m + d] / c.N[d]);
     auto add = [&](int slot, int yy, int zz, int conj) {
       int* q = L.pos[L.npos++];
       q[0] = slot; q[1] = wrapn(yy, c.N[1]); q[2] = wrapn(zz, c.N[2]); q[3] = (m << 1) | conj;
@@ -533,8 +534,7 @@ __global__ __launch_bounds__(NT) void k_forcing_reduce(const double* __restrict_
     for (int q = 0; q < 8; ++q) cm[q] = 0;
     return;
   }
-  const double s0 = sin(2 * M_PI * lm.k[m][0] / N0), s1 = sin(2 * M_PI * lm.k[m][1] / N1),
-               s2 = sin(2 * M_PI * lm.k[m][2] / N2);
+  const double s0 = lm.sk[m][0], s1 = lm.sk[m][1], s2 = lm.sk[m][2];
   const double dr = s0 * r[6 * m] + s1 * r[6 * m + 2] + s2 * r[6 * m + 4];
   const double di = s0 * r[6 * m + 1] + s1 * r[6 * m + 3] + s2 * r[6 * m + 5];
   const double gg = fac * h * h / dt;

@@ -70,20 +70,20 @@ __device__ __forceinline__ void block_coords(const GridArgs& g, int b, int& bx, 
 }
 
 // ------------------------------------------------------- pipelined lab staging
-// The fused step's stencil kernels walk nbw blocks per workgroup (stacked in z) and fetch
-// block j+1's cells into registers while block j is computed from LDS, which keeps about
-// twice the bytes in flight per CU of a load-sync-compute workgroup.
+// The fused step's stencil kernels walk several blocks per workgroup (the gradient and the
+// adjoint stacked in z, the advection along x) and fetch the next blocks' cells into
+// registers while the current one is computed from LDS, which keeps about twice the bytes in
+// flight per CU of a load-sync-compute workgroup.
 // A thread's cells: interior c = 2t, 2t+1; halo (faces only, 6*64 cells) c = t (faces
 // x-, x+, y-, y+: one face per wave) and, for t < 128, c = NT + t (faces z-, z+).
 // The pipelined lab pads x to PX = 12 (interior at x = 2..9, halo at 1 and 10): a thread's
 // x-adjacent cell pair, and its y and z neighbour pairs, are 8-byte aligned (one 64-bit LDS
 // access each).
 constexpr int PX = 12;
-constexpr bool ADV_XWALK = true;  // advection walks blocks along x (x faces from registers)
-constexpr int ADV_PD = 3;         // blocks in flight in that walk
 constexpr int PY = PX;            // y stride
 constexpr int PZ = PX * L1;       // z stride
 constexpr int PLC = PX * L1 * L1;
+constexpr int ADV_PD = 3;  // blocks in flight in the advection's walk along x
 struct LabSlots {
   int li;       // LDS index of interior cell 2t
   int l[2];     // LDS index of the halo cells
@@ -306,80 +306,7 @@ int launch_reset_umax(Ctx& c)
 // KernelAdvectDiffuse (AdvectionDiffusion.cpp:106-145), same operation order:
 // tmp = u + facA*((u+U).grad_c u) + facD*(sum of 6 neighbours - 6u), facA = -dt/(2h),
 // facD = (nu/h)*(dt/h).
-template <typename T>
-__global__ __launch_bounds__(NT) void k_advdiff_central(const T* __restrict__ u, const T* __restrict__ v,
-                                                        const T* __restrict__ w, T* __restrict__ ou,
-                                                        T* __restrict__ ov, T* __restrict__ ow, GridArgs g,
-                                                        const DevScalars* __restrict__ sc, T Ux, T Uy, T Uz,
-                                                        double nu, int nbw)
-{
-  __shared__ __align__(16) T lab[3][PLC];
-  const LabSlots s = lab_slots();
-  const T* src[3] = {u, v, w};
-  const int nxy = g.nbx * g.nby;
-  const int b0 = walk_first(g, xcd_remap(blockIdx.x, gridDim.x), nbw);
-  // two blocks in flight: block j+2's loads are issued while block j is computed, so each
-  // block's loads have a whole iteration to arrive
-  LabRegs<T, 3> ra, rb;
-  lab_fetch(ra, src, s, g, b0);
-  if (nbw > 1) lab_fetch(rb, src, s, g, b0 + nxy);
-  const double dt = sc->dt;
-  const T facA = (T)(-dt / (2 * g.h));
-  const T facD = (T)((nu / g.h) * (dt / g.h));
-  const int c0 = 2 * threadIdx.x;
-  using V2 = typename Vec2<T>::type;
-  auto compute_store = [&](int b) {
-    // advecting velocity of the pair (centre values of u, v, w), then one component at a
-    // time: its 7-point stencil for both cells (few live registers)
-    T av[3][2];
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      const V2 C = *reinterpret_cast<const V2*>(lab[q] + s.li);
-      av[q][0] = C.x;
-      av[q][1] = C.y;
-    }
-    const T uu[2] = {av[0][0] + Ux, av[0][1] + Ux}, vv[2] = {av[1][0] + Uy, av[1][1] + Uy},
-            ww[2] = {av[2][0] + Uz, av[2][1] + Uz};
-    T o[3][2];
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      const T* F = lab[q];
-      const int l = s.li;
-      const V2 Nn = *reinterpret_cast<const V2*>(F + l + PY), Sn = *reinterpret_cast<const V2*>(F + l - PY);
-      const V2 Bn = *reinterpret_cast<const V2*>(F + l + PZ), Fn = *reinterpret_cast<const V2*>(F + l - PZ);
-      const T Wx = F[l - 1], Ex = F[l + 2];
-      const T fc[2] = {av[q][0], av[q][1]};
-      const T fe[2] = {av[q][1], Ex}, fw[2] = {Wx, av[q][0]};
-      const T fn[2] = {Nn.x, Nn.y}, fs[2] = {Sn.x, Sn.y}, fb[2] = {Bn.x, Bn.y}, ff[2] = {Fn.x, Fn.y};
-#pragma unroll
-      for (int k = 0; k < 2; ++k) {
-        const T ddx = fe[k] - fw[k], ddy = fn[k] - fs[k], ddz = fb[k] - ff[k];
-        const T dD = fn[k] + fs[k] + fe[k] + fw[k] + ff[k] + fb[k] - fc[k] * 6;
-        const T dA = uu[k] * ddx + vv[k] * ddy + ww[k] * ddz;
-        o[q][k] = fc[k] + facA * dA + facD * dD;
-      }
-    }
-    const size_t q = (size_t)b * BS3 + c0;
-    *reinterpret_cast<V2*>(ou + q) = V2{o[0][0], o[0][1]};
-    *reinterpret_cast<V2*>(ov + q) = V2{o[1][0], o[1][1]};
-    *reinterpret_cast<V2*>(ow + q) = V2{o[2][0], o[2][1]};
-  };
-  for (int j = 0; j < nbw; j += 2) {
-    if (j) __syncthreads();  // the previous block's compute is done with the lab
-    lab_put(ra, lab, s);
-    __syncthreads();
-    if (j + 2 < nbw) lab_fetch(ra, src, s, g, b0 + (j + 2) * nxy);
-    compute_store(b0 + j * nxy);
-    if (j + 1 >= nbw) break;
-    __syncthreads();
-    lab_put(rb, lab, s);
-    __syncthreads();
-    if (j + 3 < nbw) lab_fetch(rb, src, s, g, b0 + (j + 3) * nxy);
-    compute_store(b0 + (j + 1) * nxy);
-  }
-}
-
-// The same stencil over a walk of NBW blocks along x: block j's x- halo is block j-1's
+// A workgroup walks NBW blocks along x: block j's x- halo is block j-1's
 // ix = 7 plane and its x+ halo block j+1's ix = 0 plane, both taken from the registers the
 // pipeline holds (threads t % 4 == 3 hold ix = 6, 7; t % 4 == 0 hold ix = 0, 1), so only
 // the walk's two end faces are gathered. PD blocks are in flight (register sets rotate
@@ -504,19 +431,12 @@ static void launch_advdiff_xwalk(Ctx& c, double nu, const double uinf[3])
 template <typename T>
 static int advdiff_t(Ctx& c, double nu, const double uinf[3])
 {
-  if (ADV_XWALK) {
-    const int nbx = c.bpd[0];
-    if (nbx % 8 == 0) launch_advdiff_xwalk<T, 8>(c, nu, uinf);
-    else if (nbx % 4 == 0) launch_advdiff_xwalk<T, 4>(c, nu, uinf);
-    else if (nbx % 2 == 0) launch_advdiff_xwalk<T, 2>(c, nu, uinf);
-    else launch_advdiff_xwalk<T, 1>(c, nu, uinf);
-    return check_hip(c, hipGetLastError(), "k_advdiff_xwalk");
-  }
-  const int nbw = blocks_per_wg(c.bpd[2]);
-  hipLaunchKernelGGL((k_advdiff_central<T>), dim3(c.nb / nbw), dim3(NT), 0, c.stream, (const T*)c.f[CUP_U],
-                     (const T*)c.f[CUP_V], (const T*)c.f[CUP_W], (T*)c.f[CUP_TMPU], (T*)c.f[CUP_TMPV],
-                     (T*)c.f[CUP_TMPW], c.grid(), c.sc, (T)uinf[0], (T)uinf[1], (T)uinf[2], nu, nbw);
-  return check_hip(c, hipGetLastError(), "k_advdiff_central");
+  const int nbx = c.bpd[0];
+  if (nbx % 8 == 0) launch_advdiff_xwalk<T, 8>(c, nu, uinf);
+  else if (nbx % 4 == 0) launch_advdiff_xwalk<T, 4>(c, nu, uinf);
+  else if (nbx % 2 == 0) launch_advdiff_xwalk<T, 2>(c, nu, uinf);
+  else launch_advdiff_xwalk<T, 1>(c, nu, uinf);
+  return check_hip(c, hipGetLastError(), "k_advdiff_xwalk");
 }
 
 // ------------------------------------------------ third-order upwind advection
