@@ -392,8 +392,9 @@ int cup_diagnostics(cup_ctx* ctx, double nu, double out[20])
 {
   CUP_GUARD(ctx);
   if (!out) return set_error(c, CUP_E_INVALID, "diagnostics: null output");
-  if (int rc = launch_diagnostics(c, nu)) return rc;
-  if (int rc = launch_reduce(c, c.nb, 20, 0u, c.sc->red)) return rc;
+  int nparts = 0;
+  if (int rc = launch_diagnostics(c, nu, &nparts)) return rc;
+  if (int rc = launch_reduce(c, nparts, 20, 0u, c.sc->red)) return rc;
   if (int rc = read_scalars(c)) return rc;
   for (int i = 0; i < 20; ++i) out[i] = c.sc_host->red[i];
   return CUP_OK;

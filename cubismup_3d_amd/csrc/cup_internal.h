@@ -198,7 +198,7 @@ int launch_adjoint_body(Ctx& c, AdjBodyPhase phase, const AdjBodyArgs& a);
 int launch_rhs_plain(Ctx& c);                                   // -> rbuf
 int launch_gradp_plain(Ctx& c);                                 // u += -dt/2h grad p
 int launch_gradp_fused(Ctx& c, const double uinf[3], const int* skip_slot, bool force = false);  // u = tmp - dt/2h grad p; max, ke_part
-int launch_diagnostics(Ctx& c, double nu);                      // partials[nb][20]
+int launch_diagnostics(Ctx& c, double nu, int* nparts);  // partials[nparts][20]
 int launch_ic_taylor_green(Ctx& c, double uMax);
 int launch_fill(Ctx& c, void* ptr, size_t n, double value);
 int launch_copy(Ctx& c, void* dst, const void* src, size_t n);

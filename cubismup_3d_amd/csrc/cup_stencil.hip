@@ -1018,88 +1018,98 @@ int launch_finalize_step(Ctx& c, double CFL, double nu, int rampup, double fixed
 
 // ------------------------------------------------------------------ diagnostics
 
-// KernelDissipation (ComputeDissipation.cpp:46-115): 20 FP64 integrals per block.
+// KernelDissipation (ComputeDissipation.cpp:46-115): 20 FP64 integrals. A workgroup sums
+// nbw consecutive blocks (one block reduction per workgroup) into part[20*wg].
 template <typename T>
 __global__ __launch_bounds__(NT) void k_diagnostics(const T* __restrict__ u, const T* __restrict__ v,
                                                     const T* __restrict__ w, const T* __restrict__ p,
                                                     const T* __restrict__ chi, GridArgs g, double nu, double cx,
-                                                    double cy, double cz, double* __restrict__ part)
+                                                    double cy, double cz, double* __restrict__ part, int nbw)
 {
   __shared__ T lab[4][L1C];
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
-  int bx, by, bz;
-  block_coords(g, b, bx, by, bz);
+  const int wg = xcd_remap(blockIdx.x, gridDim.x);
   const T* src[4] = {u, v, w, p};
-  gather_g1<T, 4>(src, lab, g, b, bx, by, bz);
-  __syncthreads();
   const double h = g.h, hCube = h * h * h, inv2h = .5 / h, invHh = 1 / (h * h);
   double R[20];
 #pragma unroll
   for (int i = 0; i < 20; ++i) R[i] = 0;
-  for (int c = threadIdx.x; c < BS3; c += NT) {
-    const int ix = c & 7, iy = (c >> 3) & 7, iz = c >> 6;
-    const int l = ((iz + 1) * L1 + iy + 1) * L1 + ix + 1;
-    const int E = l + 1, W = l - 1, N = l + L1, S = l - L1, B = l + L1 * L1, F = l - L1 * L1;
-    const T* U = lab[0];
-    const T* V = lab[1];
-    const T* Wv = lab[2];
-    const T* P = lab[3];
-    const double X = (double)chi[(size_t)b * BS3 + c];
-    const double PX = (bx * BS + ix + 0.5) * h - cx, PY = (by * BS + iy + 0.5) * h - cy, PZ = (bz * BS + iz + 0.5) * h - cz;
-    const double Lu = U[l], Lv = V[l], Lw = Wv[l];
-    const double WX = inv2h * (((double)Wv[N] - Wv[S]) - ((double)V[B] - V[F]));
-    const double WY = inv2h * (((double)U[B] - U[F]) - ((double)Wv[E] - Wv[W]));
-    const double WZ = inv2h * (((double)V[E] - V[W]) - ((double)U[N] - U[S]));
-    const double dPdx = inv2h * ((double)P[E] - P[W]);
-    const double dPdy = inv2h * ((double)P[N] - P[S]);
-    const double dPdz = inv2h * ((double)P[B] - P[F]);
-    R[15] -= (1 - X) * hCube * (dPdx * Lu + dPdy * Lv + dPdz * Lw);
-    const double lapU = invHh * ((double)U[E] + U[W] + U[N] + U[S] + U[B] + U[F] - 6 * Lu);
-    const double lapV = invHh * ((double)V[E] + V[W] + V[N] + V[S] + V[B] + V[F] - 6 * Lv);
-    const double lapW = invHh * ((double)Wv[E] + Wv[W] + Wv[N] + Wv[S] + Wv[B] + Wv[F] - 6 * Lw);
-    const double V1 = lapU * Lu + lapV * Lv + lapW * Lw;
-    const double D11 = inv2h * ((double)U[E] - U[W]);
-    const double D22 = inv2h * ((double)V[N] - V[S]);
-    const double D33 = inv2h * ((double)Wv[B] - Wv[F]);
-    const double D12 = inv2h * ((double)U[N] - U[S] + V[E] - V[W]) / 2;
-    const double D13 = inv2h * ((double)U[B] - U[F] + Wv[E] - Wv[W]) / 2;
-    const double D23 = inv2h * ((double)Wv[N] - Wv[S] + V[B] - V[F]) / 2;
-    const double V2 = D11 * D11 + D22 * D22 + D33 * D33 + 2 * (D12 * D12 + D13 * D13 + D23 * D23);
-    R[16] += (1 - X) * hCube * nu * (V1 + 2 * V2);
-    R[0] += hCube * WX;
-    R[1] += hCube * WY;
-    R[2] += hCube * WZ;
-    R[3] += hCube / 2 * (PY * WZ - PZ * WY);
-    R[4] += hCube / 2 * (PZ * WX - PX * WZ);
-    R[5] += hCube / 2 * (PX * WY - PY * WX);
-    R[6] += hCube * Lu;
-    R[7] += hCube * Lv;
-    R[8] += hCube * Lw;
-    R[9] += hCube / 3 * (PX * (PY * WY + PZ * WZ) - WX * (PY * PY + PZ * PZ));
-    R[10] += hCube / 3 * (PY * (PX * WX + PZ * WZ) - WY * (PX * PX + PZ * PZ));
-    R[11] += hCube / 3 * (PZ * (PX * WX + PY * WY) - WZ * (PX * PX + PY * PY));
-    R[12] += hCube * (PY * Lw - PZ * Lv);
-    R[13] += hCube * (PZ * Lu - PX * Lw);
-    R[14] += hCube * (PX * Lv - PY * Lu);
-    R[17] += hCube * (WX * Lu + WY * Lv + WZ * Lw);
-    R[18] += hCube * (Lu * Lu + Lv * Lv + Lw * Lw) / 2;
-    R[19] += sqrt(WX * WX + WY * WY + WZ * WZ);
+  for (int j = 0; j < nbw; ++j) {
+    const int b = wg * nbw + j;
+    int bx, by, bz;
+    block_coords(g, b, bx, by, bz);
+    if (j) __syncthreads();  // the previous block is done with the lab
+    gather_g1<T, 4>(src, lab, g, b, bx, by, bz);
+    __syncthreads();
+    for (int c = threadIdx.x; c < BS3; c += NT) {
+      const int ix = c & 7, iy = (c >> 3) & 7, iz = c >> 6;
+      const int l = ((iz + 1) * L1 + iy + 1) * L1 + ix + 1;
+      const int E = l + 1, W = l - 1, N = l + L1, S = l - L1, B = l + L1 * L1, F = l - L1 * L1;
+      const T* U = lab[0];
+      const T* V = lab[1];
+      const T* Wv = lab[2];
+      const T* P = lab[3];
+      const double X = (double)chi[(size_t)b * BS3 + c];
+      const double PX = (bx * BS + ix + 0.5) * h - cx, PY = (by * BS + iy + 0.5) * h - cy, PZ = (bz * BS + iz + 0.5) * h - cz;
+      const double Lu = U[l], Lv = V[l], Lw = Wv[l];
+      const double WX = inv2h * (((double)Wv[N] - Wv[S]) - ((double)V[B] - V[F]));
+      const double WY = inv2h * (((double)U[B] - U[F]) - ((double)Wv[E] - Wv[W]));
+      const double WZ = inv2h * (((double)V[E] - V[W]) - ((double)U[N] - U[S]));
+      const double dPdx = inv2h * ((double)P[E] - P[W]);
+      const double dPdy = inv2h * ((double)P[N] - P[S]);
+      const double dPdz = inv2h * ((double)P[B] - P[F]);
+      R[15] -= (1 - X) * hCube * (dPdx * Lu + dPdy * Lv + dPdz * Lw);
+      const double lapU = invHh * ((double)U[E] + U[W] + U[N] + U[S] + U[B] + U[F] - 6 * Lu);
+      const double lapV = invHh * ((double)V[E] + V[W] + V[N] + V[S] + V[B] + V[F] - 6 * Lv);
+      const double lapW = invHh * ((double)Wv[E] + Wv[W] + Wv[N] + Wv[S] + Wv[B] + Wv[F] - 6 * Lw);
+      const double V1 = lapU * Lu + lapV * Lv + lapW * Lw;
+      const double D11 = inv2h * ((double)U[E] - U[W]);
+      const double D22 = inv2h * ((double)V[N] - V[S]);
+      const double D33 = inv2h * ((double)Wv[B] - Wv[F]);
+      const double D12 = inv2h * ((double)U[N] - U[S] + V[E] - V[W]) / 2;
+      const double D13 = inv2h * ((double)U[B] - U[F] + Wv[E] - Wv[W]) / 2;
+      const double D23 = inv2h * ((double)Wv[N] - Wv[S] + V[B] - V[F]) / 2;
+      const double V2 = D11 * D11 + D22 * D22 + D33 * D33 + 2 * (D12 * D12 + D13 * D13 + D23 * D23);
+      R[16] += (1 - X) * hCube * nu * (V1 + 2 * V2);
+      R[0] += hCube * WX;
+      R[1] += hCube * WY;
+      R[2] += hCube * WZ;
+      R[3] += hCube / 2 * (PY * WZ - PZ * WY);
+      R[4] += hCube / 2 * (PZ * WX - PX * WZ);
+      R[5] += hCube / 2 * (PX * WY - PY * WX);
+      R[6] += hCube * Lu;
+      R[7] += hCube * Lv;
+      R[8] += hCube * Lw;
+      R[9] += hCube / 3 * (PX * (PY * WY + PZ * WZ) - WX * (PY * PY + PZ * PZ));
+      R[10] += hCube / 3 * (PY * (PX * WX + PZ * WZ) - WY * (PX * PX + PZ * PZ));
+      R[11] += hCube / 3 * (PZ * (PX * WX + PY * WY) - WZ * (PX * PX + PY * PY));
+      R[12] += hCube * (PY * Lw - PZ * Lv);
+      R[13] += hCube * (PZ * Lu - PX * Lw);
+      R[14] += hCube * (PX * Lv - PY * Lu);
+      R[17] += hCube * (WX * Lu + WY * Lv + WZ * Lw);
+      R[18] += hCube * (Lu * Lu + Lv * Lv + Lw * Lw) / 2;
+      R[19] += sqrt(WX * WX + WY * WY + WZ * WZ);
+    }
   }
-  block_reduce<20>(R, 0u, part + 20 * (size_t)b);
+  block_reduce<20>(R, 0u, part + 20 * (size_t)wg);
 }
+
+// Blocks per diagnostics workgroup: 8 when the x rows allow it.
+inline int diag_nbw(const Ctx& c) { return c.bpd[0] % 8 == 0 ? 8 : (c.bpd[0] % 4 == 0 ? 4 : (c.bpd[0] % 2 == 0 ? 2 : 1)); }
 
 template <typename T>
 static int diagnostics_t(Ctx& c, double nu)
 {
-  hipLaunchKernelGGL((k_diagnostics<T>), dim3(c.nb), dim3(NT), 0, c.stream, (const T*)c.f[CUP_U],
+  const int nbw = diag_nbw(c);
+  hipLaunchKernelGGL((k_diagnostics<T>), dim3(c.nb / nbw), dim3(NT), 0, c.stream, (const T*)c.f[CUP_U],
                      (const T*)c.f[CUP_V], (const T*)c.f[CUP_W], (const T*)c.f[CUP_P], (const T*)c.f[CUP_CHI],
-                     c.grid(), nu, c.extent[0] / 2, c.extent[1] / 2, c.extent[2] / 2, c.partials);
+                     c.grid(), nu, c.extent[0] / 2, c.extent[1] / 2, c.extent[2] / 2, c.partials, nbw);
   return check_hip(c, hipGetLastError(), "k_diagnostics");
 }
 
-int launch_diagnostics(Ctx& c, double nu)
+int launch_diagnostics(Ctx& c, double nu, int* nparts)
 {
   if (int e = ensure_partials(c, 20 * (size_t)c.nb)) return e;
+  *nparts = c.nb / diag_nbw(c);
   return c.precision == 4 ? diagnostics_t<float>(c, nu) : diagnostics_t<double>(c, nu);
 }
 
