@@ -62,6 +62,7 @@ SIGNATURES = {
     "cup_poisson_periodic": (_i, [_p]),
     "cup_pressure_project": (_i, [_p, _d]),
     "cup_chi_from_sphere": (_i, [_p, _dp, _d, _dp]),
+    "cup_hit_analysis": (_i, [_p, _d, _dp, _dp, _i]),
     "cup_chi_from_sphere_mollified": (_i, [_p, _dp, _d, _d, _dp]),
     "cup_penalize": (_i, [_p, _d, _d, _i, _dp, _dp, _dp]),
     "cup_surface_forces": (_i, [_p, _d, _dp, _dp, _dp]),

@@ -12,6 +12,8 @@
 
 #include "cup_internal.h"
 
+#include <algorithm>
+
 namespace cup {
 
 int set_error(Ctx& c, int code, const std::string& msg)
@@ -48,6 +50,8 @@ static void release(Ctx& c)
   body_free(c);
   for (auto& p : c.f)
     if (p) { hipFree(p); p = nullptr; }
+  if (c.ana) hipFree(c.ana);
+  c.ana = nullptr;
   void** bufs[] = {&c.spec, &c.spec2, &c.spec3, &c.rbuf, &c.tw[0], &c.tw[1], &c.tw[2]};
   for (void** b : bufs)
     if (*b) { hipFree(*b); *b = nullptr; }
@@ -397,6 +401,37 @@ int cup_diagnostics(cup_ctx* ctx, double nu, double out[20])
   if (int rc = launch_reduce(c, nparts, 20, 0u, c.sc->red)) return rc;
   if (int rc = read_scalars(c)) return rc;
   for (int i = 0; i < 20; ++i) out[i] = c.sc_host->red[i];
+  return CUP_OK;
+}
+
+int cup_hit_analysis(cup_ctx* ctx, double nu, double out[9], double* spectrum, int nbin)
+{
+  CUP_GUARD(ctx);
+  const int maxN = std::max(c.N[0], std::max(c.N[1], c.N[2]));
+  const int nb = maxN / 2 - 1;
+  if (!out || !(nu > 0)) return set_error(c, CUP_E_INVALID, "hit_analysis: bad argument");
+  if (nbin < nb || !spectrum)
+    return set_error(c, CUP_E_INVALID, "hit_analysis: the spectrum needs max(N)/2 - 1 entries");
+  double sums[3], grad[2];
+  std::vector<double> E(nb);
+  if (int rc = launch_hit_analysis(c, sums, E.data(), nb, grad)) return rc;
+  // SpectralManipFFTW.cpp:191-206 (normalization by the FFT size) and Analysis.cpp:171-187
+  const double norm = (double)c.N[0] * c.N[1] * c.N[2];
+  for (int b = 0; b < nb; ++b) spectrum[b] = E[b] / (norm * norm);
+  const double tke = sums[0] / (norm * norm);
+  const double eps = sums[1] * 2 * nu / ((norm * 2 * c.h) * (norm * 2 * c.h));
+  const double uprime = std::sqrt(2 * tke / 3);
+  const double lambda = std::sqrt(15 * nu / eps) * uprime;
+  const double gmean = grad[0] / norm;
+  out[0] = tke;
+  out[1] = eps;
+  out[2] = uprime;
+  out[3] = lambda;
+  out[4] = uprime * lambda / nu;
+  out[5] = sums[2] / sums[0];
+  out[6] = out[5] / uprime;
+  out[7] = gmean;
+  out[8] = std::sqrt(grad[1] / norm - gmean * gmean);
   return CUP_OK;
 }
 

@@ -103,6 +103,7 @@ struct Ctx {
   int ustar = CUP_TMPU;   // first of the three fields holding the step's u* after advection
   bool plain_rhs = false; // the fused x pass ignores the obstacle RHS (the adjoint's projection)
   void* spec = nullptr;   // packed half spectrum, complex, [z][y][kx], kx < Nx/2
+  double* ana = nullptr;  // HIT analysis: per-line sums [3 + nbin][Ny*Nz], then the row totals (lazy)
   void* spec2 = nullptr;  // extra spectra for the plain forcing path
   void* spec3 = nullptr;
   void* rbuf = nullptr;   // block-layout real scratch (plain RHS / solution)
@@ -211,6 +212,10 @@ int fft_setup(Ctx& c);
 int launch_poisson_fused(Ctx& c, bool force = false, double tke_target = 0, double inj_rate = 0);  // div(tmp) -> p
 int launch_poisson_plain(Ctx& c);   // rhs in rbuf -> solution in rbuf (separate divide pass)
 int launch_fft_forward_field(Ctx& c, const void* real_blocks, void* spec);  // 3 passes
+// HIT analysis: out[0..2] = raw sums tke, eps, l_integral numerator; E[nbin] raw shell sums;
+// grad[2] = sum sqrt(grad2), sum grad2 (see cup_hit_analysis)
+int launch_hit_analysis(Ctx& c, double* tke_eps_lint, double* E, int nbin, double grad[2]);
+int launch_grad_stats(Ctx& c, double out[2]);  // sums of sqrt(grad2) and grad2 (synchronous)
 int launch_fft_inverse_field(Ctx& c, void* spec, void* real_blocks);        // 3 passes
 // cup_spectral.hip
 int spectral_setup(Ctx& c);

@@ -2,6 +2,7 @@
 // condition and band-limited linear forcing in the reference's decomposition.
 #include "cup_internal.h"
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -626,6 +627,150 @@ int launch_forcing_rows(Ctx& c)
                        (const double*)c.low_coef, (const C2<double>*)c.tw[1], (const C2<double>*)c.tw[2],
                        (double*)c.low_rows, c.N[1], c.N[2]);
   return check_hip(c, hipGetLastError(), "k_forcing_rows");
+}
+
+// ------------------------------------------------------------ HIT analysis
+// SpectralManipFFTW::_compute_analysis (SpectralManipFFTW.cpp:110-207) on the packed half
+// spectra of u, v, w (x halved): one thread per (ky, kz) line walks kx = 0 .. Nx/2. Slots
+// kx = 1 .. Nx/2-1 stand for kx and -kx (weight 2); slot 0 packs the kx = 0 and kx = Nx/2
+// planes as A + iB, unpacked with the mirrored entry Z(-ky, -kz): A = (Z + conj Zm)/2,
+// B = -i (Z - conj Zm)/2 (weight 1 each). Per mode, with E = w (|U|^2+|V|^2+|W|^2)/2:
+// tke += E, eps += w |Omega|^2/2 (Omega from the central-difference symbols 2 sin(2 pi k/N)),
+// lint += E/|k|, and E into shell floor(sqrt(kx^2+ky^2+kz^2) (nyq-1)/nyq) when
+// kx^2+ky^2+kz^2 < nyq^2 (nyq = maxN/2). Along a line the shells are non-decreasing, so
+// the thread writes run sums: ana[(3 + shell)*nlines + line], and ana[q*nlines + line] for
+// the three totals; the rows are then summed line by line in order (deterministic).
+template <typename T>
+__global__ __launch_bounds__(NT) void k_hit_analysis_lines(const C2<T>* __restrict__ su, const C2<T>* __restrict__ sv,
+                                                          const C2<T>* __restrict__ sw, double* __restrict__ ana,
+                                                          int N0, int N1, int N2, double wfx, double wfy,
+                                                          double wfz, int nyq)
+{
+  const int nlines = N1 * N2;
+  const int line = blockIdx.x * blockDim.x + threadIdx.x;
+  if (line >= nlines) return;
+  const int ly = line % N1, lz = line / N1;
+  const int jj = ly <= N1 / 2 ? ly : ly - N1, kk = lz <= N2 / 2 ? lz : lz - N2;
+  const int ncols = N0 / 2;
+  const size_t row = ((size_t)lz * N1 + ly) * ncols;
+  const size_t mrow = ((size_t)((N2 - lz) % N2) * N1 + (N1 - ly) % N1) * ncols;
+  const double dY = 2 * sin(2 * M_PI * jj / N1), dZ = 2 * sin(2 * M_PI * kk / N2);
+  const double nyq_scale = (nyq - 1) / (double)nyq;
+  const long nyq2 = (long)nyq * nyq;
+  double tke = 0, eps = 0, lint = 0, run = 0;
+  int cur = -1;
+  // mode m = 0 .. ncols: ii = m (m = ncols is kx = Nx/2), weight 1 at m = 0 and m = ncols
+  for (int m = 0; m <= ncols; ++m) {
+    double U[3][2];
+    const C2<T>* S[3] = {su, sv, sw};
+    double wgt;
+    if (m == 0 || m == ncols) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const C2<T> Z = S[q][row], Zm = S[q][mrow];
+        if (m == 0) {  // A = (Z + conj Zm)/2
+          U[q][0] = 0.5 * ((double)Z.x + (double)Zm.x);
+          U[q][1] = 0.5 * ((double)Z.y - (double)Zm.y);
+        } else {  // B = -i (Z - conj Zm)/2
+          U[q][0] = 0.5 * ((double)Z.y + (double)Zm.y);
+          U[q][1] = -0.5 * ((double)Z.x - (double)Zm.x);
+        }
+      }
+      wgt = 1;
+    } else {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const C2<T> Z = S[q][row + m];
+        U[q][0] = Z.x;
+        U[q][1] = Z.y;
+      }
+      wgt = 2;
+    }
+    const int ii = m;
+    const double kx = ii * wfx, ky = jj * wfy, kz = kk * wfz;
+    const double k2 = kx * kx + ky * ky + kz * kz;
+    const double dX = 2 * sin(2 * M_PI * ii / N0);
+    // Omega = i (dY W - dZ V, dZ U - dX W, dX V - dY U); |i z|^2 = |z|^2
+    const double ox0 = dY * U[2][0] - dZ * U[1][0], ox1 = dY * U[2][1] - dZ * U[1][1];
+    const double oy0 = dZ * U[0][0] - dX * U[2][0], oy1 = dZ * U[0][1] - dX * U[2][1];
+    const double oz0 = dX * U[1][0] - dY * U[0][0], oz1 = dX * U[1][1] - dY * U[0][1];
+    const double e = wgt * 0.5 *
+                     (U[0][0] * U[0][0] + U[0][1] * U[0][1] + U[1][0] * U[1][0] + U[1][1] * U[1][1] +
+                      U[2][0] * U[2][0] + U[2][1] * U[2][1]);
+    tke += e;
+    eps += wgt * 0.5 * (ox0 * ox0 + ox1 * ox1 + oy0 * oy0 + oy1 * oy1 + oz0 * oz0 + oz1 * oz1);
+    lint += k2 > 0 ? e / sqrt(k2) : 0.0;
+    const long kind = (long)ii * ii + (long)jj * jj + (long)kk * kk;
+    if (kind < nyq2) {
+      const int b = (int)floor(sqrt((double)kind) * nyq_scale);
+      if (b != cur) {
+        if (cur >= 0) ana[(size_t)(3 + cur) * nlines + line] = run;
+        cur = b;
+        run = 0;
+      }
+      run += e;
+    }
+  }
+  if (cur >= 0) ana[(size_t)(3 + cur) * nlines + line] = run;
+  ana[line] = tke;
+  ana[(size_t)nlines + line] = eps;
+  ana[2 * (size_t)nlines + line] = lint;
+}
+
+// Row r of a [nrows][len] array summed in a fixed order (thread strides, shuffle tree,
+// waves in order) into out[r].
+__global__ __launch_bounds__(NT) void k_rows_sum(const double* __restrict__ a, int len, double* __restrict__ out)
+{
+  __shared__ double ws[NT / 64];
+  const int r = blockIdx.x, t = threadIdx.x;
+  double acc = 0;
+  for (int i = t; i < len; i += NT) acc += a[(size_t)r * len + i];
+  acc = wave_sum(acc);
+  if ((t & 63) == 0) ws[t >> 6] = acc;
+  __syncthreads();
+  if (t == 0) {
+    double v = 0;
+    for (int w = 0; w < NT / 64; ++w) v += ws[w];
+    out[r] = v;
+  }
+}
+
+template <typename T>
+static int hit_analysis_t(Ctx& c, double* sums, double* E, int nbin)
+{
+  if (int rc = alloc_extra_spectra(c)) return rc;
+  const int nlines = c.N[1] * c.N[2];
+  const size_t nrows = 3 + (size_t)nbin;
+  if (!c.ana && check_hip(c, hipMalloc(&c.ana, (nrows * nlines + nrows) * sizeof(double)), "hipMalloc analysis"))
+    return CUP_E_NOMEM;
+  if (int rc = launch_fft_forward_field(c, c.f[CUP_U], c.spec)) return rc;
+  if (int rc = launch_fft_forward_field(c, c.f[CUP_V], c.spec2)) return rc;
+  if (int rc = launch_fft_forward_field(c, c.f[CUP_W], c.spec3)) return rc;
+  if (int rc = check_hip(c, hipMemsetAsync(c.ana, 0, nrows * nlines * sizeof(double), c.stream), "memset analysis"))
+    return rc;
+  int maxN = std::max(c.N[0], std::max(c.N[1], c.N[2]));
+  hipLaunchKernelGGL((k_hit_analysis_lines<T>), dim3((nlines + NT - 1) / NT), dim3(NT), 0, c.stream,
+                     (const C2<T>*)c.spec, (const C2<T>*)c.spec2, (const C2<T>*)c.spec3, c.ana, c.N[0], c.N[1],
+                     c.N[2], 2 * M_PI / c.extent[0], 2 * M_PI / c.extent[1], 2 * M_PI / c.extent[2], maxN / 2);
+  double* tot = c.ana + nrows * nlines;
+  hipLaunchKernelGGL(k_rows_sum, dim3((unsigned)nrows), dim3(NT), 0, c.stream, (const double*)c.ana, nlines, tot);
+  if (int rc = check_hip(c, hipGetLastError(), "k_hit_analysis")) return rc;
+  std::vector<double> h(nrows);
+  if (int rc = check_hip(c, hipMemcpyAsync(h.data(), tot, nrows * sizeof(double), hipMemcpyDeviceToHost, c.stream),
+                         "hipMemcpy analysis"))
+    return rc;
+  if (int rc = check_hip(c, hipStreamSynchronize(c.stream), "synchronize")) return rc;
+  for (int q = 0; q < 3; ++q) sums[q] = h[q];
+  for (int b = 0; b < nbin; ++b) E[b] = h[3 + b];
+  return CUP_OK;
+}
+
+int launch_hit_analysis(Ctx& c, double* tke_eps_lint, double* E, int nbin, double grad[2])
+{
+  int rc = c.precision == 4 ? hit_analysis_t<float>(c, tke_eps_lint, E, nbin)
+                            : hit_analysis_t<double>(c, tke_eps_lint, E, nbin);
+  if (rc) return rc;
+  return launch_grad_stats(c, grad);
 }
 
 }  // namespace cup

@@ -1106,6 +1106,61 @@ static int diagnostics_t(Ctx& c, double nu)
   return check_hip(c, hipGetLastError(), "k_diagnostics");
 }
 
+// KernelAnalysis_gradStats (Analysis.cpp:18-52): per cell grad2 = sum of the nine squared
+// central differences of u, v, w over 4h^2; sums of sqrt(grad2) and grad2, nbw blocks per
+// workgroup, into part[2*wg].
+template <typename T>
+__global__ __launch_bounds__(NT) void k_grad_stats(const T* __restrict__ u, const T* __restrict__ v,
+                                                   const T* __restrict__ w, GridArgs g, double* __restrict__ part,
+                                                   int nbw)
+{
+  __shared__ T lab[3][L1C];
+  const int wg = xcd_remap(blockIdx.x, gridDim.x);
+  const T* src[3] = {u, v, w};
+  const double fac = 1 / (4 * g.h * g.h);
+  double R[2] = {0, 0};
+  for (int j = 0; j < nbw; ++j) {
+    const int b = wg * nbw + j;
+    int bx, by, bz;
+    block_coords(g, b, bx, by, bz);
+    if (j) __syncthreads();  // the previous block is done with the lab
+    gather_g1<T, 3>(src, lab, g, b, bx, by, bz);
+    __syncthreads();
+    for (int c = threadIdx.x; c < BS3; c += NT) {
+      const int l = (((c >> 6) + 1) * L1 + ((c >> 3) & 7) + 1) * L1 + (c & 7) + 1;
+      double g2 = 0;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const T* F = lab[q];
+        const double dx = (double)F[l + 1] - F[l - 1], dy = (double)F[l + L1] - F[l - L1],
+                     dz = (double)F[l + L1 * L1] - F[l - L1 * L1];
+        g2 += dx * dx + dy * dy + dz * dz;
+      }
+      g2 *= fac;
+      R[0] += sqrt(g2);
+      R[1] += g2;
+    }
+  }
+  block_reduce<2>(R, 0u, part + 2 * (size_t)wg);
+}
+
+int launch_grad_stats(Ctx& c, double out[2])
+{
+  if (int e = ensure_partials(c, 2 * (size_t)c.nb)) return e;
+  const int nbw = diag_nbw(c);
+  if (c.precision == 4)
+    hipLaunchKernelGGL((k_grad_stats<float>), dim3(c.nb / nbw), dim3(NT), 0, c.stream, (const float*)c.f[CUP_U],
+                       (const float*)c.f[CUP_V], (const float*)c.f[CUP_W], c.grid(), c.partials, nbw);
+  else
+    hipLaunchKernelGGL((k_grad_stats<double>), dim3(c.nb / nbw), dim3(NT), 0, c.stream, (const double*)c.f[CUP_U],
+                       (const double*)c.f[CUP_V], (const double*)c.f[CUP_W], c.grid(), c.partials, nbw);
+  if (int rc = launch_reduce(c, c.nb / nbw, 2, 0u, c.sc->red)) return rc;
+  if (int rc = check_hip(c, hipMemcpyAsync(out, c.sc->red, 2 * sizeof(double), hipMemcpyDeviceToHost, c.stream),
+                         "hipMemcpy grad stats"))
+    return rc;
+  return check_hip(c, hipStreamSynchronize(c.stream), "synchronize");
+}
+
 int launch_diagnostics(Ctx& c, double nu, int* nparts)
 {
   if (int e = ensure_partials(c, 20 * (size_t)c.nb)) return e;

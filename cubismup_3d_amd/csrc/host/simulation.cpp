@@ -10,6 +10,7 @@
 #include <fstream>
 #include <iomanip>
 #include <limits>
+#include <sys/stat.h>
 
 namespace cubismup3d {
 
@@ -120,6 +121,9 @@ SimulationData& SimulationData::operator=(const SimulationData& o)
   uMax_forced = o.uMax_forced; uMax_measured = o.uMax_measured; spectralForcing = o.spectralForcing;
   turbKinEn_target = o.turbKinEn_target; enInjectionRate = o.enInjectionRate;
   freqDiagnostics = o.freqDiagnostics; rampup = o.rampup; verbose = o.verbose; muteAll = o.muteAll;
+  analysis = o.analysis; timeAnalysis = o.timeAnalysis; nextAnalysisTime = o.nextAnalysisTime;
+  freqAnalysis = o.freqAnalysis; grad_mean = o.grad_mean; grad_std = o.grad_std; nu_sgs = o.nu_sgs;
+  cs2_avg = o.cs2_avg;
   path4serialization = o.path4serialization;
   BCx_flag = o.BCx_flag; BCy_flag = o.BCy_flag; BCz_flag = o.BCz_flag; bUseFourierBC = o.bUseFourierBC;
   precision = o.precision; decomposition = o.decomposition; scheme = o.scheme; device = o.device;
@@ -161,6 +165,9 @@ SimulationData::SimulationData(int argc, const char* const* argv) : SimulationDa
   if (turbKinEn_target > 0 && enInjectionRate > 0)
     throw std::invalid_argument("either constant energy injection rate or forcing to fixed energy target");
   freqDiagnostics = parser("-freqDiagnostics").asInt(100);
+  analysis = parser("-analysis").asString("");
+  timeAnalysis = parser("-tAnalysis").asDouble(0.0);
+  freqAnalysis = parser("-fAnalysis").asInt(0);
   if (parser("-iterativePenalization").asBool(false))
     throw std::invalid_argument("-iterativePenalization is not supported");
   bImplicitPenalization = parser("-implicitPenalization").asBool(false);
@@ -228,6 +235,9 @@ void SimulationData::_preprocessArguments()
   }
   if (!(nu >= 0)) throw std::invalid_argument("nu must be >= 0");
   if (!(CFL > 0)) throw std::invalid_argument("CFL must be > 0");
+  if (analysis == "channel") throw std::invalid_argument("channel analysis needs walls; this build has periodic boxes");
+  if (analysis == "HIT" && (freqAnalysis > 0 || timeAnalysis > 0) && !(nu > 0))
+    throw std::invalid_argument("HIT analysis needs nu > 0");
   if (BCx_flag != periodic || BCy_flag != periodic || BCz_flag != periodic)
     throw std::invalid_argument("only fully periodic boxes are supported (-BC_x/y/z periodic)");
   hmin = hmax = hmean = maxextent / maxbpd;
@@ -399,6 +409,63 @@ void ComputeDissipation::operator()(double)
   f << std::endl;
 }
 
+// Analysis::operator() (Analysis.cpp:96-112 trigger and file number, :159-196 HIT branch)
+// and SpectralAnalysis::dump2File (SpectralAnalysis.cpp:124-205): the same rows, names and
+// column widths, so the reference's post-processing reads these files. The directory is
+// made when a file is due (the reference makes it at step 0 of every run).
+void Analysis::operator()(double dt)
+{
+  if (sim.analysis != "HIT") return;
+  const bool bFreq = sim.freqAnalysis > 0 && (sim.step + 1) % sim.freqAnalysis == 0;
+  const bool bTime = sim.timeAnalysis > 0 && sim.time + dt > sim.nextAnalysisTime;
+  if (!bFreq && !bTime) return;
+  int nFile = sim.step;
+  if (bTime) {
+    sim.nextAnalysisTime += sim.timeAnalysis;
+    nFile = sim.time <= 0 ? 0 : 1 + (int)(sim.time / sim.timeAnalysis);
+  }
+  const int nb = std::max({sim.bpdx, sim.bpdy, sim.bpdz}) * 4 - 1;  // max(N)/2 - 1 shells
+  spectrum.assign(nb, 0.0);
+  check(cup_hit_analysis(sim.ctx, sim.nu, last.data(), spectrum.data(), nb), "Analysis");
+  sim.grad_mean = last[7];
+  sim.grad_std = last[8];
+  ++files;
+  const double nu = sim.nu, eps = last[1];
+  if (sim.verbose)
+    printf("step:%d time:%e totalKinEn:%e viscousDissip:%e totalDissipRate:%e injectionRate:%e lIntegral:%e\n",
+           sim.step, sim.time, last[0], eps, sim.dissipationRate, sim.actualInjectionRate, last[5]);
+  struct Row {
+    const char* name;
+    double value;
+    const char* what;
+  };
+  const Row rows[] = {{"time", sim.time, "simulation time"},
+                      {"lBox", sim.maxextent, "simulation box length"},
+                      {"tke", last[0], "turbulent kinetic energy"},
+                      {"eps", eps, "Viscous dissipation rate"},
+                      {"eps_f", sim.dissipationRate, "Total dissipation rate"},
+                      {"lambda", last[3], "Taylor microscale"},
+                      {"Re_lambda", last[4], "Turbulent Reynolds based on lambda"},
+                      {"eta", std::pow(nu * nu * nu / eps, 0.25), "Kolmogorov length scale"},
+                      {"tau_integral", last[6], "Integral time scale"},
+                      {"l_integral", last[5], "Integral length scale"},
+                      {"tau_eta", std::sqrt(nu / eps), "Kolmogorov time scale"},
+                      {"nu_sgs", sim.nu_sgs, "Average SGS viscosity"},
+                      {"cs2_avg", sim.cs2_avg, "Average Cs2 if dynamic model"},
+                      {"mean_grad", sim.grad_mean, "Average gradient magnitude"},
+                      {"std_grad", sim.grad_std, "Stdev gradient magnitude"}};
+  mkdir("analysis", 0775);  // EEXIST when it is there already
+  char name[64];
+  snprintf(name, sizeof name, "analysis/spectralAnalysis_%09d", nFile);
+  std::ofstream f(name);
+  if (!f) throw std::runtime_error(std::string("Analysis: cannot write ") + name);
+  f << std::left << "Spectral Analysis :" << "\n";
+  for (const Row& r : rows) f << std::setw(15) << r.name << std::setw(15) << r.value << " #" << r.what << "\n";
+  f << "\n" << std::setw(15) << "k * (lBox/2pi)" << std::setw(15) << "E_k" << "\n";
+  for (int i = 0; i < nb; ++i)
+    f << std::setw(15) << (i + 1) * 2 * M_PI / sim.maxextent << std::setw(15) << spectrum[i] << "\n";
+}
+
 // ----------------------------------------------------------------- Simulation
 
 Simulation::Simulation(const SimulationData& sd) : sim(sd)
@@ -458,6 +525,7 @@ void Simulation::setupOperators()
   sim.pipeline.emplace_back(new Penalization(sim));
   sim.pipeline.emplace_back(new ComputeForces(sim));
   sim.pipeline.emplace_back(new ComputeDissipation(sim));
+  sim.pipeline.emplace_back(new Analysis(sim));
   (*create)(0);
 }
 
@@ -465,6 +533,14 @@ void Simulation::_ic()
 {
   InitialConditions ic(sim);
   ic(0);
+  // The zero and Taylor-Green ICs clear every field, chi included. A static body's chi is
+  // built once (the reference rebuilds it in every step's CreateObstacles), so it is built
+  // again here.
+  for (auto& op : sim.pipeline)
+    if (auto* c = dynamic_cast<CreateObstacles*>(op.get())) {
+      c->created = false;
+      (*c)(0);
+    }
 }
 
 void Simulation::reset()
@@ -545,13 +621,27 @@ bool Simulation::advance(int n)
   if (cup_run_steps(sim.ctx, &p, n) != CUP_OK)
     throw std::runtime_error(std::string("advance: ") + cup_last_error(sim.ctx));
   syncStepState();
+  if (!sim.obstacles.empty()) {  // the body's last penalization force and current surface QoIs
+    auto& s = *sim.obstacles[0];
+    double ft[6], q[22];
+    if (cup_last_body_force(sim.ctx, ft) != CUP_OK ||
+        cup_surface_forces(sim.ctx, sim.nu, s.transVel.data(), s.angVel.data(), q) != CUP_OK)
+      throw std::runtime_error(std::string("advance: ") + cup_last_error(sim.ctx));
+    s.force = {ft[0], ft[1], ft[2]};
+    s.torque = {ft[3], ft[4], ft[5]};
+    for (int i = 0; i < 22; ++i) s.surfaceQoI[i] = q[i];
+  }
   return (sim.endTime > 0 && sim.time > sim.endTime) || (sim.nsteps != 0 && sim.step >= sim.nsteps);
 }
 
 // Simulation::run (Simulation.cpp:582-591).
 void Simulation::run()
 {
-  if (sim.decomposition == CUP_FUSED && sim.nsteps > 0 && sim.endTime <= 0 && sim.freqDiagnostics == 0) {
+  const bool analysis = sim.analysis == "HIT" && (sim.freqAnalysis > 0 || sim.timeAnalysis > 0);
+  // the device-resident loop writes no per-step files: taken when none is asked for
+  const bool logs = !sim.muteAll && (!sim.obstacles.empty() || sim.spectralForcing);
+  if (sim.decomposition == CUP_FUSED && sim.nsteps > 0 && sim.endTime <= 0 && sim.freqDiagnostics == 0 && !analysis &&
+      !logs) {
     advance(sim.nsteps - sim.step);
     return;
   }
@@ -701,6 +791,10 @@ static double* sim_field(SimulationData& s, const std::string& n)
   if (n == "turbKinEn_target") return &s.turbKinEn_target;
   if (n == "dissipationRate") return &s.dissipationRate;
   if (n == "actualInjectionRate") return &s.actualInjectionRate;
+  if (n == "timeAnalysis") return &s.timeAnalysis;
+  if (n == "nextAnalysisTime") return &s.nextAnalysisTime;
+  if (n == "grad_mean") return &s.grad_mean;
+  if (n == "grad_std") return &s.grad_std;
   (void)tmp;
   return nullptr;
 }
@@ -711,6 +805,7 @@ static int* sim_ifield(SimulationData& s, const std::string& n)
   if (n == "nsteps") return &s.nsteps;
   if (n == "rampup") return &s.rampup;
   if (n == "freqDiagnostics") return &s.freqDiagnostics;
+  if (n == "freqAnalysis") return &s.freqAnalysis;
   if (n == "bpdx") return &s.bpdx;
   if (n == "bpdy") return &s.bpdy;
   if (n == "bpdz") return &s.bpdz;

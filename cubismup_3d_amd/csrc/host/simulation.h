@@ -119,6 +119,11 @@ struct SimulationData {
   // settings (SimulationData.h:141-147)
   int freqDiagnostics = 0;
   int rampup = 100;
+  // analysis (SimulationData.h:122-131); nu_sgs and cs2_avg stay 0 (no SGS model here)
+  std::string analysis = "";
+  double timeAnalysis = 0, nextAnalysisTime = 0;
+  int freqAnalysis = 0;
+  double grad_mean = 0, grad_std = 0, nu_sgs = 0, cs2_avg = 0;
   bool verbose = false, muteAll = false;
   std::string path4serialization = "./";
   BCflag BCx_flag = dirichlet, BCy_flag = dirichlet, BCz_flag = dirichlet;
@@ -239,6 +244,20 @@ class ComputeDissipation : public Operator
   void operator()(double dt) override;
   std::string getName() override { return "ComputeDissipation"; }
   std::array<double, 20> last = {};
+};
+
+// Analysis (Analysis.cpp:96-196), HIT branch: gradient statistics and the spectral
+// analysis every fAnalysis steps or tAnalysis time units, written to
+// analysis/spectralAnalysis_<nFile, 9 digits> (SpectralAnalysis.cpp:124-205).
+class Analysis : public Operator
+{
+ public:
+  using Operator::Operator;
+  void operator()(double dt) override;
+  std::string getName() override { return "Analysis"; }
+  std::array<double, 9> last = {};  // tke eps uprime lambda Re_lambda l_integral tau_integral mean std
+  std::vector<double> spectrum;
+  int files = 0;
 };
 
 class Simulation

@@ -203,6 +203,22 @@ class Grid:
         self._c(L.lib().cup_diagnostics(self._h, float(nu), L.dptr(out)))
         return out
 
+    def hit_analysis(self, nu):
+        """The HIT analysis of Analysis.cpp (gradient statistics and the spectral analysis of
+        SpectralManipFFTW::_compute_analysis): dict of tke, eps, uprime, lambda, Re_lambda,
+        l_integral, tau_integral, grad_mean, grad_std, and E (the shell spectrum, max(N)/2 - 1
+        values) with k (its wavenumbers (i+1) 2 pi / max extent)."""
+        out = np.zeros(9)
+        nb = max(self.cells) // 2 - 1
+        E = np.zeros(nb)
+        self._c(L.lib().cup_hit_analysis(self._h, float(nu), L.dptr(out), L.dptr(E), int(nb)))
+        keys = ("tke", "eps", "uprime", "lambda", "Re_lambda", "l_integral", "tau_integral", "grad_mean",
+                "grad_std")
+        r = dict(zip(keys, out.tolist()))
+        r["E"] = E
+        r["k"] = (np.arange(nb) + 1) * 2 * np.pi / float(max(self.extent))
+        return r
+
     def ic_taylor_green(self, uMax):
         self._c(L.lib().cup_ic_taylor_green(self._h, float(uMax)))
 

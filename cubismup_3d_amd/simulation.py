@@ -35,7 +35,7 @@ class SimulationData:
     """SimulationData with the reference's command-line vocabulary
     (SimulationData.cpp:32-172). ``argv`` takes precedence over keyword settings."""
 
-    _INT = {"nsteps", "rampup", "freqDiagnostics", "bpdx", "bpdy", "bpdz"}
+    _INT = {"nsteps", "rampup", "freqDiagnostics", "freqAnalysis", "bpdx", "bpdy", "bpdz"}
 
     def __init__(self, cells=None, CFL=None, uinf=(0.0, 0.0, 0.0), argv=None, **kw):
         if argv is None:

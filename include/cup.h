@@ -142,6 +142,15 @@ int cup_surface_forces(cup_ctx* ctx, double nu, const double trans_vel[3], const
 /* KernelDissipation (ComputeDissipation.cpp:17-116): out[20] in the RDX order (:136-155). */
 int cup_diagnostics(cup_ctx* ctx, double nu, double out[20]);
 
+/* The HIT branch of Analysis::operator() (Analysis.cpp:159-196): KernelAnalysis_gradStats
+ * (:18-52) and SpectralAnalysis::run, i.e. SpectralManipFFTW::_compute_analysis
+ * (SpectralManipFFTW.cpp:110-207) on the velocity's forward FFT. out[9] = tke, eps (viscous
+ * dissipation from the central-difference vorticity), uprime, lambda (Taylor microscale),
+ * Re_lambda, l_integral, tau_integral, mean_grad, std_grad; spectrum[max(N)/2 - 1] the shell
+ * energy spectrum E_k of HITstatistics (SpectralManip.h:51-110; shell i at k = (i+1) 2 pi / L).
+ * nbin is the capacity of spectrum. Deterministic. */
+int cup_hit_analysis(cup_ctx* ctx, double nu, double out[9], double* spectrum, int nbin);
+
 /* Initial conditions: KernelIC_taylorGreen (InitialConditions.cpp:32-52). */
 int cup_ic_taylor_green(cup_ctx* ctx, double uMax);
 /* Seeded isotropic turbulence (construction of SpectralManipFFTW::_compute_IC,

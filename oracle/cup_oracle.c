@@ -762,6 +762,111 @@ static inline int signed_k(int i, int n) { return i <= n / 2 ? i : -(n - i); }
 /* SpectralForcing::operator() (SpectralForcing.cpp:23-82) with
  * _compute_largeModesForcing (SpectralManipFFTW.cpp:25-108). The half-spectrum sums with
  * weight mult/2 equal full-spectrum sums with weight 1/2. */
+/* The HIT branch of Analysis::operator() (Analysis.cpp:159-196): KernelAnalysis_gradStats
+ * (:18-52) and SpectralAnalysis::run (SpectralAnalysis.cpp:99-106), i.e. the forward FFT of
+ * u, v, w and SpectralManipFFTW::_compute_analysis (SpectralManipFFTW.cpp:110-207) with the
+ * HITstatistics histogram (SpectralManip.h:51-110: nyquist = maxN/2, nBin = nyquist - 1).
+ * Full complex spectra with weight 1 per mode, which by Hermitian symmetry equals the
+ * reference's half spectrum with weights mult/2. out[9] = tke, eps, uprime, lambda, Re_lambda,
+ * l_integral, tau_integral, grad_mean, grad_std; E (nBin values) the shell spectrum. Sums are
+ * per x plane, added in plane order. */
+void orc_hit_analysis(orc_sim* s, double nu, double out[9], double* E)
+{
+  const int N0 = s->N[0], N1 = s->N[1], N2 = s->N[2];
+  const size_t n = (size_t)N0 * N1 * N2;
+  int maxN = N0 > N1 ? N0 : N1;
+  if (N2 > maxN) maxN = N2;
+  const int nyq = maxN / 2, nbin = nyq - 1;
+  const double nyq_scale = (nyq - 1) / (double)nyq;
+  double complex* cu = (double complex*)malloc(sizeof(double complex) * n);
+  double complex* cv = (double complex*)malloc(sizeof(double complex) * n);
+  double complex* cw = (double complex*)malloc(sizeof(double complex) * n);
+  field_to_grid(s, ORC_U, cu);
+  field_to_grid(s, ORC_V, cv);
+  field_to_grid(s, ORC_W, cw);
+  fft3d_c(cu, N0, N1, N2, -1);
+  fft3d_c(cv, N0, N1, N2, -1);
+  fft3d_c(cw, N0, N1, N2, -1);
+  const double wfx = 2.0 * M_PI / s->extent[0], wfy = 2.0 * M_PI / s->extent[1], wfz = 2.0 * M_PI / s->extent[2];
+  const double dfx = 2 * M_PI / N0, dfy = 2 * M_PI / N1, dfz = 2 * M_PI / N2;
+  double* part = (double*)calloc((3 + (size_t)nbin) * N0, sizeof(double));
+  #pragma omp parallel for schedule(static)
+  for (int i = 0; i < N0; i++) {
+    double* P = part + (3 + (size_t)nbin) * i;
+    for (int l = 0; l < N1; l++)
+      for (int k = 0; k < N2; k++) {
+        const size_t id = ((size_t)i * N1 + l) * N2 + k;
+        const int ii = signed_k(i, N0), jj = signed_k(l, N1), kk = signed_k(k, N2);
+        const double kx = ii * wfx, ky = jj * wfy, kz = kk * wfz;
+        const double k2 = kx * kx + ky * ky + kz * kz;
+        const double dX = 2 * sin(dfx * ii), dY = 2 * sin(dfy * jj), dZ = 2 * sin(dfz * kk);
+        const double complex U = cu[id], V = cv[id], W = cw[id];
+        const double complex OX = I * dY * W - I * dZ * V;
+        const double complex OY = I * dZ * U - I * dX * W;
+        const double complex OZ = I * dX * V - I * dY * U;
+        const double e = 0.5 * (creal(U * conj(U)) + creal(V * conj(V)) + creal(W * conj(W)));
+        P[0] += e;
+        P[1] += 0.5 * (creal(OX * conj(OX)) + creal(OY * conj(OY)) + creal(OZ * conj(OZ)));
+        P[2] += k2 > 0 ? e / sqrt(k2) : 0;
+        const long kind = (long)ii * ii + (long)jj * jj + (long)kk * kk;
+        if (kind < (long)nyq * nyq) P[3 + (size_t)floor(sqrt((double)kind) * nyq_scale)] += e;
+      }
+  }
+  double tke = 0, eps = 0, lint = 0;
+  for (int b = 0; b < nbin; b++) E[b] = 0;
+  for (int i = 0; i < N0; i++) {
+    const double* P = part + (3 + (size_t)nbin) * i;
+    tke += P[0]; eps += P[1]; lint += P[2];
+    for (int b = 0; b < nbin; b++) E[b] += P[3 + b];
+  }
+  free(part); free(cu); free(cv); free(cw);
+  const double normFFT = (double)n;
+  for (int b = 0; b < nbin; b++) E[b] /= normFFT * normFFT;
+  out[0] = tke / (normFFT * normFFT);
+  out[1] = eps * 2 * nu / pow(normFFT * 2 * s->h, 2);
+  out[2] = sqrt(2 * out[0] / 3);
+  out[3] = sqrt(15 * nu / out[1]) * out[2];
+  out[4] = out[2] * out[3] / nu;
+  out[5] = lint / tke;
+  out[6] = out[5] / out[2];
+  /* gradient statistics: fac = 1/(4h^2), grad2 = fac sum of squared central differences */
+  double* gp = (double*)calloc(2 * (size_t)s->nb, sizeof(double));
+  const double fac = 1 / (4 * s->h * s->h);
+  #pragma omp parallel
+  {
+    Lab l = lab_alloc(1);
+    #pragma omp for schedule(static)
+    for (int b = 0; b < s->nb; b++) {
+      lab_load(s, b, &l);
+      double gm = 0, gs = 0;
+      for (int iz = 0; iz < BS; iz++)
+        for (int iy = 0; iy < BS; iy++)
+          for (int ix = 0; ix < BS; ix++) {
+            const FE *LW = LAB(&l, ix - 1, iy, iz), *LE = LAB(&l, ix + 1, iy, iz);
+            const FE *LS = LAB(&l, ix, iy - 1, iz), *LN = LAB(&l, ix, iy + 1, iz);
+            const FE *LF = LAB(&l, ix, iy, iz - 1), *LB = LAB(&l, ix, iy, iz + 1);
+            const double ux = LE->u - LW->u, vx = LE->v - LW->v, wx = LE->w - LW->w;
+            const double uy = LN->u - LS->u, vy = LN->v - LS->v, wy = LN->w - LS->w;
+            const double uz = LB->u - LF->u, vz = LB->v - LF->v, wz = LB->w - LF->w;
+            const double g2 = fac * (ux * ux + vx * vx + wx * wx + uy * uy + vy * vy + wy * wy +
+                                     uz * uz + vz * vz + wz * wz);
+            gm += sqrt(g2);
+            gs += g2;
+          }
+      gp[2 * b] = gm;
+      gp[2 * b + 1] = gs;
+    }
+    free(l.c);
+  }
+  double gm = 0, gs = 0;
+  for (int b = 0; b < s->nb; b++) { gm += gp[2 * b]; gs += gp[2 * b + 1]; }
+  free(gp);
+  gm /= normFFT;
+  gs /= normFFT;
+  out[7] = gm;
+  out[8] = sqrt(gs - gm * gm);
+}
+
 void orc_spectral_forcing(orc_sim* s, double dt, double sim_dt, double nu,
                           double tke_target, double injection_rate, double stats_out[5])
 {

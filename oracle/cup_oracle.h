@@ -56,6 +56,9 @@ void orc_grad_p(orc_sim* s, double dt);
 void orc_get_rhs(const orc_sim* s, double* dst_blockmajor);
 void orc_set_rhs(orc_sim* s, const double* src_blockmajor);
 void orc_dissipation(const orc_sim* s, double nu, double out[20]);
+/* HIT analysis (Analysis.cpp:159-196, SpectralManipFFTW.cpp:110-207): out[9] = tke, eps, uprime,
+ * lambda, Re_lambda, l_integral, tau_integral, grad_mean, grad_std; E[maxN/2 - 1] the shell spectrum. */
+void orc_hit_analysis(orc_sim* s, double nu, double out[9], double* E);
 
 /* Spectral forcing (SpectralForcing.cpp:23-82, SpectralManipFFTW.cpp:25-108).
  * stats_out: tke, eps, tke_filtered, l_integral, fac. */

@@ -60,6 +60,7 @@ def lib():
         "orc_get_rhs": (None, [p, dp]),
         "orc_set_rhs": (None, [p, dp]),
         "orc_dissipation": (None, [p, d, dp]),
+        "orc_hit_analysis": (None, [p, d, dp, dp]),
         "orc_spectral_forcing": (None, [p, d, d, d, d, d, dp]),
         "orc_ic_taylor_green": (None, [p, d]),
         "orc_ic_hit": (None, [p, d, d, C.c_uint64]),
@@ -198,6 +199,17 @@ class OracleSim:
         n = buf.size // 3
         lam0 = tuple(from_blocks(buf[q * n:(q + 1) * n], self.shape) for q in range(3))
         return (lam0, dv, dc) if centre else (lam0, dv)
+
+    def hit_analysis(self, nu):
+        """HIT analysis: dict of tke, eps, uprime, lambda, Re_lambda, l_integral, tau_integral,
+        grad_mean, grad_std and E (the shell spectrum, maxN/2 - 1 bins)."""
+        out = np.zeros(9)
+        E = np.zeros(max(self.shape) // 2 - 1)
+        lib().orc_hit_analysis(self._h, float(nu), _dp(out), _dp(E))
+        keys = ("tke", "eps", "uprime", "lambda", "Re_lambda", "l_integral", "tau_integral", "grad_mean", "grad_std")
+        r = dict(zip(keys, out.tolist()))
+        r["E"] = E
+        return r
 
     def pressure_rhs(self, dt):
         lib().orc_pressure_rhs(self._h, float(dt))

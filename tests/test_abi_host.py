@@ -90,6 +90,11 @@ def test_reference_defaults_and_flags(cup):
         cup.SimulationData(argv=["-bpdx", "2", "-bpdy", "2", "-bpdz", "2"])
     sd2.nsteps = 3
     assert sd2.nsteps == 3
+    # analysis flags (SimulationData.cpp:100-102): off by default
+    assert (sd.freqAnalysis, sd.timeAnalysis, sd.nextAnalysisTime) == (0, 0.0, 0.0)
+    sd3 = cup.SimulationData(argv=["-bpdx", "2", "-bpdy", "2", "-bpdz", "2", "-nu", "0.01", "-analysis", "HIT",
+                                   "-fAnalysis", "5", "-tAnalysis", "0.25"])
+    assert (sd3.freqAnalysis, sd3.timeAnalysis) == (5, 0.25)
     with pytest.raises(AttributeError):
         sd2.no_such_member
 
