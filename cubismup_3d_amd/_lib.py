@@ -66,6 +66,7 @@ SIGNATURES = {
     "cup_chi_from_sphere_mollified": (_i, [_p, _dp, _d, _d, _dp]),
     "cup_penalize": (_i, [_p, _d, _d, _i, _dp, _dp, _dp]),
     "cup_surface_forces": (_i, [_p, _d, _dp, _dp, _dp]),
+    "cup_obstacle_momenta": (_i, [_p, _d, _d, _i, _dp]),
     "cup_diagnostics": (_i, [_p, _d, _dp]),
     "cup_ic_taylor_green": (_i, [_p, _d]),
     "cup_ic_hit": (_i, [_p, _d, _d, _u64]),
@@ -93,6 +94,7 @@ SIGNATURES = {
     "cup_sim_get": (_i, [_p, C.c_char_p, _dp]),
     "cup_sim_set": (_i, [_p, C.c_char_p, _d]),
     "cup_sim_obstacle": (_i, [_p, _i, _dp]),
+    "cup_sim_obstacle_state": (_i, [_p, _i, _dp]),
     "cup_sim_diagnostics": (_i, [_p, _dp]),
     "cup_sim_flush_logs": (_i, [_p]),
 }

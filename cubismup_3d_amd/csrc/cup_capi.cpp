@@ -503,6 +503,17 @@ int cup_penalize(cup_ctx* ctx, double dt, double lambda, int implicit, const dou
   return CUP_OK;
 }
 
+int cup_obstacle_momenta(cup_ctx* ctx, double dt, double lambda, int implicit, double out[29])
+{
+  CUP_GUARD(ctx);
+  if (!c.body.present) return set_error(c, CUP_E_STATE, "obstacle_momenta: no obstacle");
+  if (!out || !(dt > 0) || !(lambda > 0)) return set_error(c, CUP_E_INVALID, "obstacle_momenta: bad argument");
+  if (int rc = launch_obstacle_momenta(c, dt, lambda, implicit ? 1 : 0)) return rc;
+  if (int rc = read_scalars(c)) return rc;
+  for (int i = 0; i < 29; ++i) out[i] = c.sc_host->red[i];
+  return CUP_OK;
+}
+
 int cup_surface_forces(cup_ctx* ctx, double nu, const double trans_vel[3], const double ang_vel[3], double out[22])
 {
   CUP_GUARD(ctx);

@@ -87,6 +87,7 @@ struct Body {
   void* ob_rhs = nullptr;       // device: nob*512 corrected RHS (context precision)
   unsigned char* ob_touch = nullptr;  // device: nob*512 surface flag of the RHS correction
   void* ob_r0 = nullptr;        // device: nob*512 unmasked RHS (adjoint with the centre; lazy)
+  std::vector<int> ids_host;    // the obstacle blocks' ids, to reuse the buffers when a moved body keeps them
 };
 
 struct Ctx {
@@ -231,6 +232,7 @@ int launch_penalize(Ctx& c, double lambda, int implicit, const double vel[3], co
 int launch_penalize_step(Ctx& c, double lambda, int implicit, const double vel[3], const double omega[3],
                          const double uinf[3]);  // also refreshes the fused step's max/energy partials
 int launch_surface_forces(Ctx& c, double nu, const double vel[3], const double omega[3]);
+int launch_obstacle_momenta(Ctx& c, double dt, double lambda, int implicit);
 int launch_rhs_obstacle(Ctx& c, void* rhs_blocks);
 
 // Stages of the fused step, for the measurement hook.

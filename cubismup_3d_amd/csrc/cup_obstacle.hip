@@ -194,16 +194,13 @@ static int body_chi_t(Ctx& c)
   return check_hip(c, hipGetLastError(), "k_sphere_chi");
 }
 
-// CreateObstacles for one static sphere (ObstaclesCreate.cpp:444-540); moll > 0 selects the
-// mollified chi of that half-width in cells.
+// CreateObstacles for one sphere (ObstaclesCreate.cpp:444-540); moll > 0 selects the
+// mollified chi of that half-width in cells. Called again for a body that moved: the
+// per-block buffers are kept when the set of obstacle blocks is unchanged.
 int body_create(Ctx& c, const double center[3], double radius, double moll)
 {
   hipStreamSynchronize(c.stream);
-  body_free(c);
   Body& B = c.body;
-  B.radius = radius;
-  B.moll = moll;
-  for (int a = 0; a < 3; ++a) B.center[a] = center[a];
   // Blocks whose cell-centre box strictly intersects the sphere's box of half-width
   // R + (2+SURFDH) h (Sphere.cpp:21-42, Obstacle.h:186-207).
   const double safety = (2 + SURFDH) * c.h;
@@ -221,21 +218,32 @@ int body_create(Ctx& c, const double center[3], double radius, double moll)
       ids.push_back(b);
     }
   }
+  if (ids.empty()) {
+    body_free(c);
+    return set_error(c, CUP_E_INVALID, "chi_from_sphere: the sphere touches no block");
+  }
+  const bool reuse = B.present && B.radius == radius && B.moll == moll && B.ids_host == ids;
+  if (!reuse) body_free(c);
+  B.radius = radius;
+  B.moll = moll;
+  for (int a = 0; a < 3; ++a) B.center[a] = center[a];
   B.nob = (int)ids.size();
-  if (B.nob == 0) return set_error(c, CUP_E_INVALID, "chi_from_sphere: the sphere touches no block");
   const size_t nc = (size_t)B.nob * BS3;
-  if (check_hip(c, hipMalloc(&B.ob_ids, B.nob * sizeof(int)), "hipMalloc") ||
+  if (!reuse && (check_hip(c, hipMalloc(&B.ob_ids, B.nob * sizeof(int)), "hipMalloc") ||
       check_hip(c, hipMalloc(&B.ob_slot, c.nb * sizeof(int)), "hipMalloc") ||
       check_hip(c, hipMalloc(&B.ob_chi, nc * sizeof(double)), "hipMalloc") ||
       check_hip(c, hipMalloc(&B.ob_sdf, nc * sizeof(double)), "hipMalloc") ||
       check_hip(c, hipMalloc(&B.ob_surf, nc * 4 * sizeof(double)), "hipMalloc") ||
       check_hip(c, hipMalloc(&B.ob_rhs, nc * c.esz), "hipMalloc") ||
-      check_hip(c, hipMalloc(&B.ob_touch, nc), "hipMalloc")) {
+      check_hip(c, hipMalloc(&B.ob_touch, nc), "hipMalloc"))) {
     body_free(c);
     return CUP_E_NOMEM;
   }
-  hipMemcpy(B.ob_ids, ids.data(), ids.size() * sizeof(int), hipMemcpyHostToDevice);
-  hipMemcpy(B.ob_slot, slot.data(), slot.size() * sizeof(int), hipMemcpyHostToDevice);
+  if (!reuse) {
+    hipMemcpy(B.ob_ids, ids.data(), ids.size() * sizeof(int), hipMemcpyHostToDevice);
+    hipMemcpy(B.ob_slot, slot.data(), slot.size() * sizeof(int), hipMemcpyHostToDevice);
+    B.ids_host = ids;
+  }
   B.present = true;
   if (int rc = ensure_partials(c, 32 * (size_t)B.nob + 2 * (size_t)c.nb)) return rc;
   int rc = c.precision == 4 ? body_chi_t<float>(c) : body_chi_t<double>(c);
@@ -707,6 +715,89 @@ int launch_penalize_step(Ctx& c, double lambda, int implicit, const double vel[3
 {
   return c.precision == 4 ? penalize_t<float>(c, lambda, implicit, vel, omega, uinf, true)
                           : penalize_t<double>(c, lambda, implicit, vel, omega, uinf, true);
+}
+
+// ------------------------------------------------------------ body momenta
+
+// KernelIntegrateFluidMomenta (ObstaclesUpdate.cpp:15-131) on one obstacle block, udef = 0:
+// part[29 s + q] in KernelFinalizeObstacleVel's order (:139-158): V; sum chi u dv (3);
+// sum chi (p x u) dv (3); J0..J5; and with implicit penalization GfX, Gp (3), Gj (6),
+// Gu (3), Ga (3) weighted by penalFac = dv lambda dt chi / (1 + chi lambda dt), zero
+// otherwise. p = x - CoM; the obstacle's chi, cells with chi <= 0 skipped.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_obstacle_momenta(const int* __restrict__ ob_ids,
+                                                         const double* __restrict__ ob_chi, const T* __restrict__ u,
+                                                         const T* __restrict__ v, const T* __restrict__ w, GridArgs g,
+                                                         double lambdt, int implicit, double cmx, double cmy,
+                                                         double cmz, double* __restrict__ part)
+{
+  const int s = blockIdx.x, b = ob_ids[s];
+  int bx, by, bz;
+  bcoords(g, b, bx, by, bz);
+  const double dv = g.h * g.h * g.h;
+  double M[29];
+#pragma unroll
+  for (int q = 0; q < 29; ++q) M[q] = 0;
+  for (int c = threadIdx.x; c < BS3; c += NT) {
+    const double X = ob_chi[(size_t)s * BS3 + c];
+    if (!(X > 0)) continue;
+    const int ix = c & 7, iy = (c >> 3) & 7, iz = c >> 6;
+    const double px = (bx * BS + ix + 0.5) * g.h - cmx, py = (by * BS + iy + 0.5) * g.h - cmy,
+                 pz = (bz * BS + iz + 0.5) * g.h - cmz;
+    const size_t o = (size_t)b * BS3 + c;
+    const double U = u[o], V = v[o], W = w[o];
+    const double Xd = X * dv;
+    M[0] += Xd;
+    M[1] += Xd * U;
+    M[2] += Xd * V;
+    M[3] += Xd * W;
+    M[4] += Xd * (py * W - pz * V);
+    M[5] += Xd * (pz * U - px * W);
+    M[6] += Xd * (px * V - py * U);
+    M[7] += Xd * (py * py + pz * pz);
+    M[8] += Xd * (px * px + pz * pz);
+    M[9] += Xd * (px * px + py * py);
+    M[10] -= Xd * px * py;
+    M[11] -= Xd * px * pz;
+    M[12] -= Xd * py * pz;
+    if (implicit) {
+      const double pf = dv * lambdt * X / (1 + X * lambdt);
+      M[13] += pf;
+      M[14] += pf * px;
+      M[15] += pf * py;
+      M[16] += pf * pz;
+      M[17] += pf * (py * py + pz * pz);
+      M[18] += pf * (px * px + pz * pz);
+      M[19] += pf * (px * px + py * py);
+      M[20] -= pf * px * py;
+      M[21] -= pf * px * pz;
+      M[22] -= pf * py * pz;
+      M[23] += pf * U;
+      M[24] += pf * V;
+      M[25] += pf * W;
+      M[26] += pf * (py * W - pz * V);
+      M[27] += pf * (pz * U - px * W);
+      M[28] += pf * (px * V - py * U);
+    }
+  }
+  breduce<29>(M, part + 29 * (size_t)s);
+}
+
+template <typename T>
+static int momenta_t(Ctx& c, double lambdt, int implicit)
+{
+  Body& B = c.body;
+  double* bp = c.partials + 2 * (size_t)c.nb;
+  hipLaunchKernelGGL((k_obstacle_momenta<T>), dim3(B.nob), dim3(NT), 0, c.stream, B.ob_ids, B.ob_chi,
+                     (const T*)c.f[CUP_U], (const T*)c.f[CUP_V], (const T*)c.f[CUP_W], c.grid(), lambdt, implicit,
+                     B.com[0], B.com[1], B.com[2], bp);
+  hipLaunchKernelGGL(k_sum_rows, dim3(29), dim3(NT), 0, c.stream, (const double*)bp, B.nob, 29, c.sc->red);
+  return check_hip(c, hipGetLastError(), "k_obstacle_momenta");
+}
+
+int launch_obstacle_momenta(Ctx& c, double dt, double lambda, int implicit)
+{
+  return c.precision == 4 ? momenta_t<float>(c, lambda * dt, implicit) : momenta_t<double>(c, lambda * dt, implicit);
 }
 
 // ---------------------------------------------------------------- surface forces

@@ -210,9 +210,8 @@ SimulationData::SimulationData(int argc, const char* const* argv) : SimulationDa
       s->bForcedInSimFrame[d] = forced;
       s->bFixFrameOfRef[d] = fix;
     }
-    s->transVel[0] = -parser("-xvel").asDouble(0.0);  // Obstacle.cpp:64
-    s->transVel[1] = -parser("-yvel").asDouble(0.0);
-    s->transVel[2] = -parser("-zvel").asDouble(0.0);
+    // enforcedVelocity = -(xvel, yvel, zvel) (Obstacle.cpp:64)
+    s->setVelocities({-parser("-xvel").asDouble(0.0), -parser("-yvel").asDouble(0.0), -parser("-zvel").asDouble(0.0)});
     obstacles.push_back(s);
   }
 }
@@ -288,9 +287,114 @@ void InitialConditions::operator()(double)
   }
 }
 
-// CreateObstacles (ObstaclesCreate.cpp:444-540) for static spheres: the frame follows
-// the body (ObstacleVector::updateUinf, ObstacleVector.h:60-74) and the chi is built once.
-void CreateObstacles::operator()(double)
+// ------------------------------------------------------------------- Sphere
+
+void Sphere::setVelocities(const std::array<double, 3>& imposed)
+{
+  bool any = false;
+  for (int d = 0; d < 3; ++d) {
+    transVel_imposed[d] = bForcedInSimFrame[d] ? imposed[d] : 0.0;
+    transVel[d] = transVel_imposed[d];
+    any = any || bForcedInSimFrame[d];
+  }
+  bBlockRotation = {any, any, any};
+  angVel = {0, 0, 0};
+}
+
+bool Sphere::isStatic() const
+{
+  for (int d = 0; d < 3; ++d)
+    if (!bForcedInSimFrame[d] || !(bFixFrameOfRef[d] || transVel_imposed[d] == 0)) return false;
+  return true;
+}
+
+void Sphere::update(double dt, const std::array<double, 3>& uinf)
+{
+  for (int d = 0; d < 3; ++d) {
+    position[d] += dt * (transVel[d] + uinf[d]);
+    absPos[d] += dt * transVel[d];
+  }
+  // the quaternion moves along the unit sphere by the rotation of dt * dq/dt
+  const auto& Q = quaternion;
+  const double w0 = angVel[0], w1 = angVel[1], w2 = angVel[2];
+  const double DQ[4] = {0.5 * dt * (-w0 * Q[1] - w1 * Q[2] - w2 * Q[3]), 0.5 * dt * (w0 * Q[0] + w1 * Q[3] - w2 * Q[2]),
+                        0.5 * dt * (-w0 * Q[3] + w1 * Q[0] + w2 * Q[1]), 0.5 * dt * (w0 * Q[2] - w1 * Q[1] + w2 * Q[0])};
+  const double n = std::sqrt(DQ[0] * DQ[0] + DQ[1] * DQ[1] + DQ[2] * DQ[2] + DQ[3] * DQ[3]);
+  if (n > std::numeric_limits<double>::epsilon()) {
+    const double f = std::tan(n) / n;
+    double D[4], m = 0;
+    for (int i = 0; i < 4; ++i) {
+      D[i] = Q[i] + f * DQ[i];
+      m += D[i] * D[i];
+    }
+    m = 1 / std::sqrt(m);
+    for (int i = 0; i < 4; ++i) quaternion[i] = D[i] * m;
+  }
+}
+
+// Gaussian elimination with partial pivoting on a 6x6 system (the LU solve of the reference).
+static void solve6(double A[6][6], double b[6], double x[6])
+{
+  for (int k = 0; k < 6; ++k) {
+    int p = k;
+    for (int i = k + 1; i < 6; ++i)
+      if (std::fabs(A[i][k]) > std::fabs(A[p][k])) p = i;
+    if (!(std::fabs(A[p][k]) > 0)) throw std::runtime_error("computeVelocities: singular momentum system");
+    if (p != k) {
+      for (int j = 0; j < 6; ++j) std::swap(A[k][j], A[p][j]);
+      std::swap(b[k], b[p]);
+    }
+    for (int i = k + 1; i < 6; ++i) {
+      const double f = A[i][k] / A[k][k];
+      for (int j = k; j < 6; ++j) A[i][j] -= f * A[k][j];
+      b[i] -= f * b[k];
+    }
+  }
+  for (int i = 5; i >= 0; --i) {
+    double r = b[i];
+    for (int j = i + 1; j < 6; ++j) r -= A[i][j] * x[j];
+    x[i] = r / A[i][i];
+  }
+}
+
+void Sphere::computeVelocities(const double M[29], bool implicit)
+{
+  // the penalized momenta (KernelFinalizeObstacleVel, ObstaclesUpdate.cpp:172-187)
+  const double pm = implicit ? M[13] : M[0];
+  const double cm[3] = {implicit ? M[14] : 0.0, implicit ? M[15] : 0.0, implicit ? M[16] : 0.0};
+  const double* J = implicit ? M + 17 : M + 7;
+  const double* L = implicit ? M + 23 : M + 1;
+  const double* Am = implicit ? M + 26 : M + 4;
+  double A[6][6] = {{pm, 0, 0, 0, cm[2], -cm[1]},       {0, pm, 0, -cm[2], 0, cm[0]},
+                    {0, 0, pm, cm[1], -cm[0], 0},       {0, -cm[2], cm[1], J[0], J[3], J[4]},
+                    {cm[2], 0, -cm[0], J[3], J[1], J[5]}, {-cm[1], cm[0], 0, J[4], J[5], J[2]}};
+  double b[6] = {L[0], L[1], L[2], Am[0], Am[1], Am[2]};
+  // a forced axis or a blocked rotation keeps only its diagonal (Obstacle.cpp:172-200)
+  for (int d = 0; d < 3; ++d) {
+    if (bForcedInSimFrame[d]) {
+      for (int j = 0; j < 6; ++j)
+        if (j != d) A[d][j] = 0;
+      b[d] = pm * transVel_imposed[d];
+    }
+    if (bBlockRotation[d]) {
+      for (int j = 0; j < 6; ++j)
+        if (j != 3 + d) A[3 + d][j] = 0;
+      b[3 + d] = 0;
+    }
+  }
+  double x[6];
+  solve6(A, b, x);
+  for (int d = 0; d < 3; ++d) {
+    transVel[d] = bForcedInSimFrame[d] ? transVel_imposed[d] : x[d];
+    angVel[d] = bBlockRotation[d] ? 0.0 : x[3 + d];
+  }
+}
+
+// CreateObstacles (ObstaclesCreate.cpp:444-540) for one sphere: the frame velocity
+// (ObstacleVector::updateUinf, ObstacleVector.h:60-74), the body's update with the step's dt
+// (Obstacle::update), then chi, surface points and centre of mass at the new position. A
+// body that has not moved keeps its chi.
+void CreateObstacles::operator()(double dt)
 {
   if (sim.obstacles.empty()) return;
   if (sim.obstacles.size() > 1) throw std::invalid_argument("one obstacle at most");
@@ -299,10 +403,9 @@ void CreateObstacles::operator()(double)
   for (int d = 0; d < 3; ++d)
     if (s.bFixFrameOfRef[d]) u[d] = -s.transVel[d];
   sim.uinf = u;
-  if (created) return;
-  for (int d = 0; d < 3; ++d)
-    if (!s.bForcedInSimFrame[d] || std::fabs(s.transVel[d] + sim.uinf[d]) > 0)
-      throw std::invalid_argument("only bodies forced in the simulation frame and at rest in it are supported");
+  const auto before = s.position;
+  s.update(dt, sim.uinf);
+  if (created && s.position == before) return;
   double cm[4];
   check(cup_chi_from_sphere(sim.ctx, s.position.data(), s.radius, cm), "CreateObstacles");
   s.centerOfMass = {cm[0], cm[1], cm[2]};
@@ -350,9 +453,18 @@ void PressureProjection::operator()(double dt)
   check(cup_pressure_project(sim.ctx, dt), "PressureProjection gradient");
 }
 
-// UpdateObstacles: every degree of freedom forced, so Obstacle::computeVelocities keeps
-// the imposed velocities and zero rotation (Obstacle.cpp:176-263).
-void UpdateObstacles::operator()(double) {}
+// UpdateObstacles (ObstaclesUpdate.cpp:202-236): the body's momentum integrals on the
+// projected velocity, then Obstacle::computeVelocities. A static body keeps its velocities
+// (every axis forced, rotation blocked), so its step skips the integrals.
+void UpdateObstacles::operator()(double dt)
+{
+  if (sim.obstacles.empty()) return;
+  auto& s = *sim.obstacles[0];
+  if (s.isStatic()) return;
+  double M[29];
+  check(cup_obstacle_momenta(sim.ctx, dt, sim.lambda, sim.bImplicitPenalization ? 1 : 0, M), "UpdateObstacles");
+  s.computeVelocities(M, sim.bImplicitPenalization);
+}
 
 void Penalization::operator()(double dt)
 {
@@ -591,7 +703,10 @@ void Simulation::syncStepState()
 
 bool Simulation::advance(int n)
 {
-  if (sim.decomposition != CUP_FUSED) {
+  // the device-resident loop holds a body still: a body that moves takes the pipeline
+  bool moving = false;
+  for (const auto& s : sim.obstacles) moving = moving || !s->isStatic();
+  if (sim.decomposition != CUP_FUSED || moving) {
     for (int i = 0; i < n; ++i)
       if (timestep(calcMaxTimestep())) return true;
     return false;
@@ -697,10 +812,10 @@ int cup_sim_add_sphere(cup_sim* h, double radius, const double position[3], cons
     s->radius = radius;
     for (int d = 0; d < 3; ++d) {
       s->position[d] = position[d];
-      s->transVel[d] = trans_vel[d];
       s->bForcedInSimFrame[d] = forced != 0;
       s->bFixFrameOfRef[d] = fix_frame != 0;
     }
+    s->setVelocities({trans_vel[0], trans_vel[1], trans_vel[2]});
     if (h->S) h->S->addObstacle(s);
     else if (h->pending) h->pending->obstacles.push_back(s);
     else throw std::invalid_argument("no simulation");
@@ -851,6 +966,20 @@ int cup_sim_set_cells(cup_sim* h, int nx, int ny, int nz)
 }
 
 // Obstacle state: out = CoM[3], mass, force[3], torque[3], surface QoIs[22] (34 doubles).
+int cup_sim_obstacle_state(cup_sim* h, int idx, double out[13])
+{
+  return guarded(h, [&] {
+    if (!h->S || idx < 0 || idx >= (int)h->S->sim.obstacles.size()) throw std::invalid_argument("no such obstacle");
+    const Sphere& s = *h->S->sim.obstacles[idx];
+    for (int d = 0; d < 3; ++d) out[d] = s.position[d];
+    for (int i = 0; i < 4; ++i) out[3 + i] = s.quaternion[i];
+    for (int d = 0; d < 3; ++d) {
+      out[7 + d] = s.transVel[d];
+      out[10 + d] = s.angVel[d];
+    }
+  });
+}
+
 int cup_sim_obstacle(cup_sim* h, int idx, double out[34])
 {
   return guarded(h, [&] {

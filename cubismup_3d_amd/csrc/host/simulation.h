@@ -79,9 +79,12 @@ class BufferedLogger
 // (Obstacle::computeVelocities with every degree of freedom forced, Obstacle.cpp:153-264).
 struct Sphere {
   double radius = 0;
-  std::array<double, 3> position = {0, 0, 0};
+  std::array<double, 3> position = {0, 0, 0}, absPos = {0, 0, 0};
+  std::array<double, 4> quaternion = {1, 0, 0, 0};
   std::array<bool, 3> bForcedInSimFrame = {false, false, false};
   std::array<bool, 3> bFixFrameOfRef = {false, false, false};
+  std::array<bool, 3> bBlockRotation = {false, false, false};
+  std::array<double, 3> transVel_imposed = {0, 0, 0};
   std::array<double, 3> transVel = {0, 0, 0};
   std::array<double, 3> angVel = {0, 0, 0};
   std::array<double, 3> centerOfMass = {0, 0, 0};
@@ -89,6 +92,16 @@ struct Sphere {
   std::array<double, 3> force = {0, 0, 0}, torque = {0, 0, 0};  // penalization integrals
   std::array<double, 22> surfaceQoI = {};                        // ObstacleBlock::sumQoI order
   int obstacleID = 0;
+
+  // The Obstacle constructor's velocity set-up (Obstacle.cpp:119-137): forced axes take the
+  // imposed velocity, the others start at rest; any forced axis blocks the rotation.
+  void setVelocities(const std::array<double, 3>& imposed);
+  // Every axis forced and at rest in the frame: the body never moves and keeps its velocity.
+  bool isStatic() const;
+  // Obstacle::update (Obstacle.cpp:314-345): position, absolute position and quaternion.
+  void update(double dt, const std::array<double, 3>& uinf);
+  // Obstacle::computeVelocities (Obstacle.cpp:153-264) on UpdateObstacles' 29 momentum sums.
+  void computeVelocities(const double M[29], bool implicit);
 };
 
 class Operator;

@@ -248,6 +248,12 @@ class Grid:
                                      L.vec3(ang_vel), L.dptr(ft)))
         return ft
 
+    def obstacle_momenta(self, dt, lam, implicit) -> np.ndarray:
+        """UpdateObstacles' 29 momentum sums over the sphere's blocks (ObstaclesUpdate.cpp:15-170)."""
+        m = np.zeros(29)
+        self._c(L.lib().cup_obstacle_momenta(self._h, float(dt), float(lam), int(bool(implicit)), L.dptr(m)))
+        return m
+
     def surface_forces(self, nu, trans_vel, ang_vel=(0, 0, 0)) -> np.ndarray:
         q = np.zeros(22)
         self._c(L.lib().cup_surface_forces(self._h, float(nu), L.vec3(trans_vel), L.vec3(ang_vel), L.dptr(q)))

@@ -18,8 +18,9 @@ from .grid import Grid, from_blocks
 
 
 class Sphere:
-    """Sphere arguments (bindings/Obstacles.cpp:79-128): forced in the simulation frame
-    with velocity -xvel, the frame fixed to it (the reference's sphere-in-flow idiom)."""
+    """Sphere arguments (bindings/Obstacles.cpp:79-128). Forced in the simulation frame, it
+    moves with velocity -xvel (the frame fixed to it by default: the reference's sphere-in-flow
+    idiom); not forced, it starts at rest and moves with the fluid's momentum."""
 
     def __init__(self, position, radius, xvel=0.0, bForcedInSimFrame=True, bFixFrameOfRef=True):
         if not radius > 0:
@@ -134,8 +135,11 @@ class Simulation:
         import numpy as np
         o = np.zeros(34)
         L.check(L.lib().cup_sim_obstacle(self._h, int(idx), L.dptr(o)), sim=self._h)
+        st = np.zeros(13)
+        L.check(L.lib().cup_sim_obstacle_state(self._h, int(idx), L.dptr(st)), sim=self._h)
         return dict(com=o[0:3].copy(), mass=o[3], force=o[4:7].copy(), torque=o[7:10].copy(),
-                    surface=o[10:32].copy())
+                    surface=o[10:32].copy(), position=st[0:3].copy(), quaternion=st[3:7].copy(),
+                    trans_vel=st[7:10].copy(), ang_vel=st[10:13].copy())
 
     def diagnostics(self):
         import numpy as np

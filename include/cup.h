@@ -118,7 +118,8 @@ int cup_pressure_project(cup_ctx* ctx, double dt);
 /* A sphere (Sphere.cpp:17-50 SDF, ObstacleLibrary.h:41-69 block fill, Obstacle.h:186-207
  * block selection) turned into chi by CreateObstacles' KernelCharacteristicFunction
  * (ObstaclesCreate.cpp:22-137), with surface points and the grid centre of mass
- * (:230-256). Static body: built once. com_mass_out = CoM[3], mass (may be NULL). */
+ * (:230-256). A body that moved is built again; its buffers are kept while it covers the
+ * same blocks. com_mass_out = CoM[3], mass (may be NULL). */
 int cup_chi_from_sphere(cup_ctx* ctx, const double center[3], double radius, double com_mass_out[4]);
 /* The same sphere with a mollified chi, no reference counterpart (it replaces Towers' formula
  * of ObstaclesCreate.cpp:51-137 so that chi is C1 in the centre, for cup_adjoint_step_body's
@@ -138,6 +139,14 @@ int cup_penalize(cup_ctx* ctx, double dt, double lambda, int implicit, const dou
  * (Obstacle.cpp:266-286): out[22] in ObstacleBlock::sumQoI order (ObstacleBlock.h:81-93). */
 int cup_surface_forces(cup_ctx* ctx, double nu, const double trans_vel[3], const double ang_vel[3],
                        double out[22]);
+
+/* UpdateObstacles' momentum integrals (KernelIntegrateFluidMomenta, ObstaclesUpdate.cpp:15-131,
+ * summed over the obstacle blocks as KernelFinalizeObstacleVel does, :133-170) for the sphere
+ * (udef = 0), positions relative to its grid centre of mass: out[29] = V, sum chi u dv (3),
+ * sum chi (p x u) dv (3), J0..J5, then for implicit penalization GfX, Gp (3), Gj (6), Gu (3),
+ * Ga (3) with penalFac = dv lambda dt chi / (1 + chi lambda dt), zero otherwise. The input of
+ * Obstacle::computeVelocities (Obstacle.cpp:153-264). */
+int cup_obstacle_momenta(cup_ctx* ctx, double dt, double lambda, int implicit, double out[29]);
 
 /* KernelDissipation (ComputeDissipation.cpp:17-116): out[20] in the RDX order (:136-155). */
 int cup_diagnostics(cup_ctx* ctx, double nu, double out[20]);
@@ -258,6 +267,8 @@ int cup_sim_get(cup_sim* sim, const char* name, double* value);  /* SimulationDa
 int cup_sim_set(cup_sim* sim, const char* name, double value);
 int cup_sim_obstacle(cup_sim* sim, int idx, double out[34]);     /* CoM, mass, force, torque, 22 QoIs */
 int cup_sim_diagnostics(cup_sim* sim, double out[20]);
+/* An obstacle's kinematic state (Obstacle.h:99-104): position[3], quaternion[4], transVel[3], angVel[3]. */
+int cup_sim_obstacle_state(cup_sim* sim, int idx, double out[13]);
 int cup_sim_flush_logs(cup_sim* sim);
 
 #ifdef __cplusplus

@@ -1336,6 +1336,69 @@ void orc_penalize(orc_sim* s, double dt, double lambda, int implicit,
   for (int i = 0; i < 6; i++) ft_out[i] = M[i];
 }
 
+/* KernelIntegrateFluidMomenta (ObstaclesUpdate.cpp:15-131) for the sphere (udef = 0), each
+ * obstacle block summed over its cells in (iz, iy, ix) order and the blocks in block order
+ * (KernelFinalizeObstacleVel, :133-170). out[0] V = sum chi dv; [1..3] sum chi u dv;
+ * [4..6] sum chi (p x u) dv; [7..12] J0..J5; with implicit penalization also [13] GfX,
+ * [14..16] Gp, [17..22] Gj, [23..25] Gu, [26..28] Ga weighted by
+ * penalFac = dv lambda dt chi / (1 + chi lambda dt), else zero. p = x - CoM; the obstacle's
+ * chi, cells with chi <= 0 skipped (:82). */
+void orc_obstacle_momenta(const orc_sim* s, double dt, double lambda, int implicit, double out[29])
+{
+  for (int i = 0; i < 29; i++) out[i] = 0;
+  if (!s->has_sphere) return;
+  const double dv = pow(s->h, 3), lambdt = lambda * dt;
+  double* part = (double*)calloc(29 * (size_t)s->nb, sizeof(double));
+  #pragma omp parallel for schedule(static)
+  for (int b = 0; b < s->nb; b++) {
+    const OBlock* o = s->ob[b];
+    if (!o) continue;
+    int bc[3];
+    bcoords(s, b, bc);
+    double* M = part + 29 * (size_t)b;
+    for (int iz = 0; iz < BS; iz++)
+      for (int iy = 0; iy < BS; iy++)
+        for (int ix = 0; ix < BS; ix++) {
+          const double X = o->chi[iz][iy][ix];
+          if (X <= 0) continue;
+          const FE* e = &s->blk[b].d[iz][iy][ix];
+          double p[3];
+          cell_pos(s, bc, ix, iy, iz, p);
+          p[0] -= s->CoM[0]; p[1] -= s->CoM[1]; p[2] -= s->CoM[2];
+          const double Xd = X * dv;
+          M[0] += Xd;
+          M[1] += Xd * e->u; M[2] += Xd * e->v; M[3] += Xd * e->w;
+          M[4] += Xd * (p[1] * e->w - p[2] * e->v);
+          M[5] += Xd * (p[2] * e->u - p[0] * e->w);
+          M[6] += Xd * (p[0] * e->v - p[1] * e->u);
+          M[7] += Xd * (p[1] * p[1] + p[2] * p[2]);
+          M[8] += Xd * (p[0] * p[0] + p[2] * p[2]);
+          M[9] += Xd * (p[0] * p[0] + p[1] * p[1]);
+          M[10] -= Xd * p[0] * p[1];
+          M[11] -= Xd * p[0] * p[2];
+          M[12] -= Xd * p[1] * p[2];
+          if (implicit) {
+            const double pf = dv * lambdt * X / (1 + X * lambdt);
+            M[13] += pf;
+            M[14] += pf * p[0]; M[15] += pf * p[1]; M[16] += pf * p[2];
+            M[17] += pf * (p[1] * p[1] + p[2] * p[2]);
+            M[18] += pf * (p[0] * p[0] + p[2] * p[2]);
+            M[19] += pf * (p[0] * p[0] + p[1] * p[1]);
+            M[20] -= pf * p[0] * p[1];
+            M[21] -= pf * p[0] * p[2];
+            M[22] -= pf * p[1] * p[2];
+            M[23] += pf * e->u; M[24] += pf * e->v; M[25] += pf * e->w;
+            M[26] += pf * (p[1] * e->w - p[2] * e->v);
+            M[27] += pf * (p[2] * e->u - p[0] * e->w);
+            M[28] += pf * (p[0] * e->v - p[1] * e->u);
+          }
+        }
+  }
+  for (int b = 0; b < s->nb; b++)
+    for (int i = 0; i < 29; i++) out[i] += part[29 * (size_t)b + i];
+  free(part);
+}
+
 /* KernelComputeForces (FluidSolidForces.cpp:17-158) summed as in Obstacle::computeForces
  * (Obstacle.cpp:266-286); out[] in ObstacleBlock::sumQoI order (ObstacleBlock.h:81-93). */
 void orc_surface_forces(const orc_sim* s, double nu, const double uTrans[3], const double omega[3],

@@ -77,6 +77,7 @@ void orc_sphere_create_mollified(orc_sim* s, double radius, const double pos[3],
 void orc_sphere_info(const orc_sim* s, double out[8]); /* CoM[3], mass, nblocks, npoints */
 void orc_penalize(orc_sim* s, double dt, double lambda, int implicit,
                   const double transVel[3], const double angVel[3], double ft_out[6]);
+void orc_obstacle_momenta(const orc_sim* s, double dt, double lambda, int implicit, double out[29]);
 void orc_surface_forces(const orc_sim* s, double nu, const double transVel[3],
                         const double angVel[3], double out[22]);
 

@@ -69,6 +69,7 @@ def lib():
         "orc_sphere_info": (None, [p, dp]),
         "orc_penalize": (None, [p, d, d, i, dp, dp, dp]),
         "orc_surface_forces": (None, [p, d, dp, dp, dp]),
+        "orc_obstacle_momenta": (None, [p, d, d, i, dp]),
         "orc_fft3d": (None, [i, i, i, dp, dp, i]),
         "orc_philox": (None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     }
@@ -119,6 +120,44 @@ def fft3d(a: np.ndarray, sign: int = -1) -> np.ndarray:
     A, B, Cc = a.shape
     lib().orc_fft3d(A, B, Cc, _dp(re), _dp(im), int(sign))
     return re + 1j * im
+
+
+def compute_velocities(M, implicit, forced, imposed, block_rot):
+    """Obstacle::computeVelocities (Obstacle.cpp:153-212) on the sums of obstacle_momenta:
+    solve the 6x6 system of the penalized momenta for (V, Omega). A forced translation keeps
+    only its diagonal with right-hand side penalM * imposed; a blocked rotation keeps only
+    its diagonal with right-hand side 0 (numpy's LU with partial pivoting in place of GSL's)."""
+    M = np.asarray(M, dtype=np.float64)
+    if implicit:
+        pm, cm, J, L, A_ = M[13], M[14:17], M[17:23], M[23:26], M[26:29]
+    else:
+        pm, cm, J, L, A_ = M[0], np.zeros(3), M[7:13], M[1:4], M[4:7]
+    A = np.array([[pm, 0, 0, 0, cm[2], -cm[1]],
+                  [0, pm, 0, -cm[2], 0, cm[0]],
+                  [0, 0, pm, cm[1], -cm[0], 0],
+                  [0, -cm[2], cm[1], J[0], J[3], J[4]],
+                  [cm[2], 0, -cm[0], J[3], J[1], J[5]],
+                  [-cm[1], cm[0], 0, J[4], J[5], J[2]]], dtype=np.float64)
+    b = np.concatenate([L, A_]).astype(np.float64)
+    for d in range(3):
+        if forced[d]:
+            diag = A[d, d]
+            A[d, :] = 0
+            A[d, d] = diag
+            b[d] = pm * imposed[d]
+        if block_rot[d]:
+            diag = A[3 + d, 3 + d]
+            A[3 + d, :] = 0
+            A[3 + d, 3 + d] = diag
+            b[3 + d] = 0
+    x = np.linalg.solve(A, b)
+    V, Om = x[:3].copy(), x[3:].copy()
+    for d in range(3):
+        if forced[d]:
+            V[d] = imposed[d]
+        if block_rot[d]:
+            Om[d] = 0.0
+    return V, Om
 
 
 class OracleSim:
@@ -267,10 +306,39 @@ class OracleSim:
                            _dp(_vec(trans_vel)), _dp(_vec(ang_vel)), _dp(ft))
         return ft
 
+    def obstacle_momenta(self, dt, lam, implicit):
+        """The 29 sums of KernelIntegrateFluidMomenta / KernelFinalizeObstacleVel
+        (ObstaclesUpdate.cpp:15-170) over the sphere's obstacle blocks."""
+        m = np.zeros(29)
+        lib().orc_obstacle_momenta(self._h, float(dt), float(lam), int(bool(implicit)), _dp(m))
+        return m
+
     def surface_forces(self, nu, trans_vel, ang_vel=(0, 0, 0)):
         q = np.zeros(22)
         lib().orc_surface_forces(self._h, float(nu), _dp(_vec(trans_vel)), _dp(_vec(ang_vel)), _dp(q))
         return q
+
+    def body_timestep(self, dt, nu, body, lambda_, implicit):
+        """One step with a sphere that may move, in the reference pipeline order
+        (Simulation.cpp:307-401): CreateObstacles (frame velocity from the fixed-frame axes,
+        ObstacleVector.h:60-74; position += dt (V + U_inf), Obstacle.cpp:314-319; chi at the new
+        position), advection-diffusion, RHS, Poisson, gradient, UpdateObstacles (momenta and
+        computeVelocities), Penalization, ComputeForces. ``body``: dict with radius,
+        position, transVel, angVel, forced, fix, block_rot, imposed; position and velocities
+        are updated in place. Returns (U_inf, penalization force/torque, surface QoIs)."""
+        uinf = [-body["transVel"][d] if body["fix"][d] else 0.0 for d in range(3)]
+        body["position"] = [body["position"][d] + dt * (body["transVel"][d] + uinf[d]) for d in range(3)]
+        self.sphere_create(body["radius"], body["position"])
+        self.advect_diffuse(dt, nu, uinf)
+        self.pressure_rhs(dt)
+        self.poisson_solve()
+        self.grad_p(dt)
+        M = self.obstacle_momenta(dt, lambda_, implicit)
+        V, Om = compute_velocities(M, implicit, body["forced"], body["imposed"], body["block_rot"])
+        body["transVel"], body["angVel"] = V, Om
+        ft = self.penalize(dt, lambda_, implicit, V, Om)
+        q = self.surface_forces(nu, V, Om)
+        return uinf, ft, q
 
     def timestep(self, dt, nu, uinf=(0, 0, 0), spectral_forcing=False, tke_target=0.0,
                  sphere=None, scheme=0):
