@@ -85,7 +85,7 @@ SIGNATURES = {
     "cup_sim_init": (_i, [_p]),
     "cup_sim_destroy": (_i, [_p]),
     "cup_sim_last_error": (C.c_char_p, [_p]),
-    "cup_sim_add_sphere": (_i, [_p, _d, _dp, _dp, _i, _i]),
+    "cup_sim_add_sphere": (_i, [_p, _d, _dp, _dp, _i, _i, _i, _d, _d]),
     "cup_sim_run": (_i, [_p]),
     "cup_sim_calc_max_timestep": (_i, [_p, _dp]),
     "cup_sim_timestep": (_i, [_p, _d, _ip]),

@@ -212,6 +212,13 @@ SimulationData::SimulationData(int argc, const char* const* argv) : SimulationDa
     }
     // enforcedVelocity = -(xvel, yvel, zvel) (Obstacle.cpp:64)
     s->setVelocities({-parser("-xvel").asDouble(0.0), -parser("-yvel").asDouble(0.0), -parser("-zvel").asDouble(0.0)});
+    // the ramped x velocity (Sphere.cpp:90-98): umax = xvel, tmax = T
+    s->accel_decel = parser("-accel").asBool(false);
+    if (s->accel_decel) {
+      s->umax = parser("-xvel").asDouble(0.0);
+      s->tmax = parser("-T").asDouble(1.0);
+    }
+    if (parser("-hemisphere").asBool(false)) throw std::invalid_argument("the hemisphere is not supported");
     obstacles.push_back(s);
   }
 }
@@ -303,6 +310,7 @@ void Sphere::setVelocities(const std::array<double, 3>& imposed)
 
 bool Sphere::isStatic() const
 {
+  if (accel_decel) return false;
   for (int d = 0; d < 3; ++d)
     if (!bForcedInSimFrame[d] || !(bFixFrameOfRef[d] || transVel_imposed[d] == 0)) return false;
   return true;
@@ -357,8 +365,13 @@ static void solve6(double A[6][6], double b[6], double x[6])
   }
 }
 
-void Sphere::computeVelocities(const double M[29], bool implicit)
+void Sphere::computeVelocities(const double M[29], bool implicit, double time)
 {
+  if (accel_decel) {  // Sphere::computeVelocities (Sphere.cpp:147-159)
+    if (time < tmax) transVel_imposed[0] = umax * time / tmax;
+    else if (time < 2 * tmax) transVel_imposed[0] = umax * (2 * tmax - time) / tmax;
+    else transVel_imposed[0] = 0;
+  }
   // the penalized momenta (KernelFinalizeObstacleVel, ObstaclesUpdate.cpp:172-187)
   const double pm = implicit ? M[13] : M[0];
   const double cm[3] = {implicit ? M[14] : 0.0, implicit ? M[15] : 0.0, implicit ? M[16] : 0.0};
@@ -397,7 +410,6 @@ void Sphere::computeVelocities(const double M[29], bool implicit)
 void CreateObstacles::operator()(double dt)
 {
   if (sim.obstacles.empty()) return;
-  if (sim.obstacles.size() > 1) throw std::invalid_argument("one obstacle at most");
   auto& s = *sim.obstacles[0];
   std::array<double, 3> u = {0, 0, 0};
   for (int d = 0; d < 3; ++d)
@@ -405,7 +417,15 @@ void CreateObstacles::operator()(double dt)
   sim.uinf = u;
   const auto before = s.position;
   s.update(dt, sim.uinf);
-  if (created && s.position == before) return;
+  if (!created || s.position != before) build();
+}
+
+// chi, surface points and centre of mass of the body where it is (set-up, and after an IC)
+void CreateObstacles::build()
+{
+  if (sim.obstacles.empty()) return;
+  if (sim.obstacles.size() > 1) throw std::invalid_argument("one obstacle at most");
+  auto& s = *sim.obstacles[0];
   double cm[4];
   check(cup_chi_from_sphere(sim.ctx, s.position.data(), s.radius, cm), "CreateObstacles");
   s.centerOfMass = {cm[0], cm[1], cm[2]};
@@ -463,7 +483,7 @@ void UpdateObstacles::operator()(double dt)
   if (s.isStatic()) return;
   double M[29];
   check(cup_obstacle_momenta(sim.ctx, dt, sim.lambda, sim.bImplicitPenalization ? 1 : 0, M), "UpdateObstacles");
-  s.computeVelocities(M, sim.bImplicitPenalization);
+  s.computeVelocities(M, sim.bImplicitPenalization, sim.time);
 }
 
 void Penalization::operator()(double dt)
@@ -638,21 +658,18 @@ void Simulation::setupOperators()
   sim.pipeline.emplace_back(new ComputeForces(sim));
   sim.pipeline.emplace_back(new ComputeDissipation(sim));
   sim.pipeline.emplace_back(new Analysis(sim));
-  (*create)(0);
+  create->build();
 }
 
 void Simulation::_ic()
 {
   InitialConditions ic(sim);
   ic(0);
-  // The zero and Taylor-Green ICs clear every field, chi included. A static body's chi is
-  // built once (the reference rebuilds it in every step's CreateObstacles), so it is built
+  // The zero and Taylor-Green ICs clear every field, chi included. A body that does not move
+  // keeps its chi (the reference rebuilds it in every step's CreateObstacles), so it is built
   // again here.
   for (auto& op : sim.pipeline)
-    if (auto* c = dynamic_cast<CreateObstacles*>(op.get())) {
-      c->created = false;
-      (*c)(0);
-    }
+    if (auto* c = dynamic_cast<CreateObstacles*>(op.get())) c->build();
 }
 
 void Simulation::reset()
@@ -711,8 +728,17 @@ bool Simulation::advance(int n)
       if (timestep(calcMaxTimestep())) return true;
     return false;
   }
-  for (auto& op : sim.pipeline)
-    if (auto* c = dynamic_cast<CreateObstacles*>(op.get())) (*c)(0);
+  // The frame velocity is set by the first step's CreateObstacles, after that step's dt
+  // was computed with the previous one: that step goes through the pipeline when they differ.
+  if (!sim.obstacles.empty() && n > 0) {
+    const auto& s = *sim.obstacles[0];
+    bool same = true;
+    for (int d = 0; d < 3; ++d) same = same && sim.uinf[d] == (s.bFixFrameOfRef[d] ? -s.transVel[d] : 0.0);
+    if (!same) {
+      if (timestep(calcMaxTimestep())) return true;
+      --n;
+    }
+  }
   cup_step_params p{};
   p.CFL = sim.CFL;
   p.nu = sim.nu;
@@ -803,19 +829,24 @@ int cup_sim_create(int argc, const char* const* argv, cup_sim** out)
   return guarded(*out, [&] { (*out)->pending.reset(new SimulationData(argc, argv)); });
 }
 
-int cup_sim_add_sphere(cup_sim* h, double radius, const double position[3], const double trans_vel[3],
-                       int forced, int fix_frame)
+int cup_sim_add_sphere(cup_sim* h, double radius, const double position[3], const double enforced_vel[3],
+                       int forced_mask, int fix_mask, int accel_decel, double umax, double tmax)
 {
   return guarded(h, [&] {
     if (!(radius > 0)) throw std::invalid_argument("radius must be > 0");
+    if (!position || !enforced_vel) throw std::invalid_argument("null argument");
+    if (accel_decel && !(tmax > 0)) throw std::invalid_argument("tmax must be > 0");
     auto s = std::make_shared<Sphere>();
     s->radius = radius;
     for (int d = 0; d < 3; ++d) {
       s->position[d] = position[d];
-      s->bForcedInSimFrame[d] = forced != 0;
-      s->bFixFrameOfRef[d] = fix_frame != 0;
+      s->bForcedInSimFrame[d] = (forced_mask >> d) & 1;
+      s->bFixFrameOfRef[d] = (fix_mask >> d) & 1;
     }
-    s->setVelocities({trans_vel[0], trans_vel[1], trans_vel[2]});
+    s->setVelocities({enforced_vel[0], enforced_vel[1], enforced_vel[2]});
+    s->accel_decel = accel_decel != 0;
+    s->umax = umax;
+    s->tmax = tmax;
     if (h->S) h->S->addObstacle(s);
     else if (h->pending) h->pending->obstacles.push_back(s);
     else throw std::invalid_argument("no simulation");

@@ -92,6 +92,9 @@ struct Sphere {
   std::array<double, 3> force = {0, 0, 0}, torque = {0, 0, 0};  // penalization integrals
   std::array<double, 22> surfaceQoI = {};                        // ObstacleBlock::sumQoI order
   int obstacleID = 0;
+  // SphereArguments (Sphere.h:17-29): the ramped x velocity, up to umax at tmax, 0 at 2 tmax
+  bool accel_decel = false;
+  double umax = 0, tmax = 1;
 
   // The Obstacle constructor's velocity set-up (Obstacle.cpp:119-137): forced axes take the
   // imposed velocity, the others start at rest; any forced axis blocks the rotation.
@@ -100,8 +103,9 @@ struct Sphere {
   bool isStatic() const;
   // Obstacle::update (Obstacle.cpp:314-345): position, absolute position and quaternion.
   void update(double dt, const std::array<double, 3>& uinf);
-  // Obstacle::computeVelocities (Obstacle.cpp:153-264) on UpdateObstacles' 29 momentum sums.
-  void computeVelocities(const double M[29], bool implicit);
+  // Sphere::computeVelocities then Obstacle::computeVelocities (Sphere.cpp:147-159,
+  // Obstacle.cpp:153-264) on UpdateObstacles' 29 momentum sums at simulation time `time`.
+  void computeVelocities(const double M[29], bool implicit, double time);
 };
 
 class Operator;
@@ -189,6 +193,7 @@ class CreateObstacles : public Operator
  public:
   using Operator::Operator;
   void operator()(double dt) override;
+  void build();
   std::string getName() override { return "CreateObstacles"; }
   bool created = false;
 };

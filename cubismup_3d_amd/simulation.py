@@ -18,18 +18,52 @@ from .grid import Grid, from_blocks
 
 
 class Sphere:
-    """Sphere arguments (bindings/Obstacles.cpp:79-128). Forced in the simulation frame, it
-    moves with velocity -xvel (the frame fixed to it by default: the reference's sphere-in-flow
-    idiom); not forced, it starts at rest and moves with the fluid's momentum."""
+    """ObstacleAndSphereArguments of the bindings (bindings/Obstacles.cpp:12-71, 79-128):
+    ``radius`` or ``length``; ``position``; ``enforcedVelocity``, the velocity of the axes
+    forced in the simulation frame; ``bForcedInSimFrame`` and ``bFixFrameOfRef``, a bool or one
+    per axis (default: free, frame at rest); ``umax``, ``tmax``, ``accel_decel``, the ramped x
+    velocity of Sphere::computeVelocities (Sphere.cpp:147-159). ``xvel`` is the command-line
+    shorthand: every axis forced, the frame fixed to the body, enforcedVelocity = (-xvel, 0, 0)
+    (Obstacle.cpp:64). ``bHemi``, ``bFixToPlanar``, a non-identity ``quaternion`` and
+    ``bComputeForces=False`` are not supported and raise ValueError, as unknown keywords do."""
 
-    def __init__(self, position, radius, xvel=0.0, bForcedInSimFrame=True, bFixFrameOfRef=True):
-        if not radius > 0:
-            raise ValueError("radius must be > 0")
-        self.position = [float(x) for x in position]
-        self.radius = float(radius)
-        self.trans_vel = [-float(xvel), 0.0, 0.0]
-        self.forced = bool(bForcedInSimFrame)
-        self.fix_frame = bool(bFixFrameOfRef)
+    def __init__(self, radius=0.0, umax=0.0, tmax=1.0, accel_decel=False, bHemi=False, **kwargs):
+        length = float(kwargs.pop("length", 0.0))
+        radius = float(radius)
+        if radius > 0 and length > 0 and radius != 0.5 * length:
+            raise ValueError("Cannot specify both `radius` and `length`.")
+        if radius <= 0 and length <= 0:
+            raise ValueError("Expected a `radius` or `length`.")
+        self.radius = radius if radius > 0 else 0.5 * length
+        self.position = [float(x) for x in kwargs.pop("position", (0.0, 0.0, 0.0))]
+        xvel = kwargs.pop("xvel", None)
+        forced, fix = kwargs.pop("bForcedInSimFrame", None), kwargs.pop("bFixFrameOfRef", None)
+        vel = kwargs.pop("enforcedVelocity", None)
+        if xvel is not None:
+            if vel is not None:
+                raise ValueError("give xvel or enforcedVelocity, not both")
+            vel = (-float(xvel), 0.0, 0.0)
+            forced = True if forced is None else forced
+            fix = True if fix is None else fix
+
+        def axes(v):
+            v = False if v is None else v
+            return [bool(v)] * 3 if isinstance(v, (bool, int)) else [bool(a) for a in v]
+
+        self.forced, self.fix_frame = axes(forced), axes(fix)
+        self.trans_vel = [float(x) for x in (vel if vel is not None else (0.0, 0.0, 0.0))]
+        if len(self.position) != 3 or len(self.trans_vel) != 3 or len(self.forced) != 3 or len(self.fix_frame) != 3:
+            raise ValueError("position, enforcedVelocity and the axis flags take three values")
+        q = [float(x) for x in kwargs.pop("quaternion", (1.0, 0.0, 0.0, 0.0))]
+        if bHemi or kwargs.pop("bFixToPlanar", False) or q != [1.0, 0.0, 0.0, 0.0] \
+                or not kwargs.pop("bComputeForces", True):
+            raise ValueError("bHemi, bFixToPlanar, a rotated quaternion and bComputeForces=False are not supported")
+        if kwargs:
+            raise ValueError(f"unknown sphere arguments: {sorted(kwargs)}")
+        self.umax, self.tmax, self.accel_decel = float(umax), float(tmax), bool(accel_decel)
+
+    def masks(self):
+        return (sum(1 << d for d in range(3) if self.forced[d]), sum(1 << d for d in range(3) if self.fix_frame[d]))
 
 
 class SimulationData:
@@ -104,8 +138,11 @@ class Simulation:
         self.shape = (8 * bpd[2], 8 * bpd[1], 8 * bpd[0])
 
     def add_obstacle(self, s: Sphere):
-        L.check(L.lib().cup_sim_add_sphere(self._h, s.radius, L.vec3(s.position), L.vec3(s.trans_vel),
-                                           int(s.forced), int(s.fix_frame)), sim=self._h)
+        if not isinstance(s, Sphere):  # Simulation_addObstacle (bindings/Obstacles.cpp:120-128)
+            raise ValueError(str(s))
+        fm, xm = s.masks()
+        L.check(L.lib().cup_sim_add_sphere(self._h, s.radius, L.vec3(s.position), L.vec3(s.trans_vel), fm, xm,
+                                           int(s.accel_decel), s.umax, s.tmax), sim=self._h)
 
     def run(self):
         L.check(L.lib().cup_sim_run(self._h), sim=self._h)

@@ -247,7 +247,8 @@ typedef struct cup_sim cup_sim;
 
 /* SimulationData(comm, parser) (SimulationData.cpp:32-172): "-bpdx 8 -nu 0.01 ..."; adds
  * -precision float|double, -decomposition plain|fused, -device, -icSeed, -icUrms, and a
- * sphere through -sphereL -xpos -ypos -zpos -xvel -bForcedInSimFrame -bFixFrameOfRef. */
+ * sphere through -sphereL -xpos -ypos -zpos -xvel -yvel -zvel -bForcedInSimFrame
+ * -bFixFrameOfRef -accel -T (Sphere.cpp:87-99). */
 int cup_sim_create(int argc, const char* const* argv, cup_sim** out);
 /* SimulationData::setCells (SimulationData.cpp:280-291); before cup_sim_init. */
 int cup_sim_set_cells(cup_sim* sim, int nx, int ny, int nz);
@@ -255,9 +256,12 @@ int cup_sim_set_cells(cup_sim* sim, int nx, int ny, int nz);
 int cup_sim_init(cup_sim* sim);
 int cup_sim_destroy(cup_sim* sim);
 const char* cup_sim_last_error(const cup_sim* sim);
-/* Simulation_addObstacle with a Sphere (bindings/Obstacles.cpp:79-128). */
-int cup_sim_add_sphere(cup_sim* sim, double radius, const double position[3], const double trans_vel[3],
-                       int forced, int fix_frame);
+/* Simulation_addObstacle with a Sphere (bindings/Obstacles.cpp:12-128): forced_mask and
+ * fix_mask hold bForcedInSimFrame and bFixFrameOfRef per axis (bit d for axis d);
+ * enforced_vel is the velocity of the forced axes; accel_decel, umax, tmax select the
+ * ramped x velocity of Sphere::computeVelocities (Sphere.cpp:147-159). */
+int cup_sim_add_sphere(cup_sim* sim, double radius, const double position[3], const double enforced_vel[3],
+                       int forced_mask, int fix_mask, int accel_decel, double umax, double tmax);
 int cup_sim_run(cup_sim* sim);                                   /* Simulation::run (Simulation.cpp:582-591) */
 int cup_sim_calc_max_timestep(cup_sim* sim, double* dt);         /* Simulation.cpp:403-424 */
 int cup_sim_timestep(cup_sim* sim, double dt, int* finished);    /* Simulation.cpp:593-623 */

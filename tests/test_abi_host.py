@@ -104,6 +104,22 @@ def test_sphere_arguments(cup):
         cup.Sphere(position=[0.3, 0.4, 0.5], radius=0.0)
     s = cup.Sphere(position=[0.3, 0.4, 0.5], radius=0.1, xvel=0.2)
     assert s.trans_vel == [-0.2, 0.0, 0.0]  # enforcedVelocity = -xvel (Obstacle.cpp:64)
+    assert s.masks() == (7, 7)
+    # the bindings' keywords and defaults (bindings/Obstacles.cpp:12-71, Obstacle.h:60-76)
+    s = cup.Sphere(position=[0.3, 0.4, 0.5], radius=0.1)
+    assert s.masks() == (0, 0) and s.trans_vel == [0.0, 0.0, 0.0] and not s.accel_decel
+    s = cup.Sphere(length=0.3, position=[0.3, 0.4, 0.5], enforcedVelocity=[0.5, 0, 0],
+                   bForcedInSimFrame=[True, False, False], bFixFrameOfRef=[False, True, False], umax=1.0,
+                   tmax=2.0, accel_decel=True)
+    assert s.radius == 0.15 and s.masks() == (1, 2) and (s.umax, s.tmax) == (1.0, 2.0)
+    with pytest.raises(ValueError):
+        cup.Sphere(radius=0.1, length=0.3)
+    with pytest.raises(ValueError):
+        cup.Sphere(position=[0.3, 0.4, 0.5])
+    with pytest.raises(ValueError):
+        cup.Sphere(radius=0.1, bHemi=True)
+    with pytest.raises(ValueError):
+        cup.Sphere(radius=0.1, no_such_argument=1)
 
 
 def test_buffered_logger_flushes_like_the_reference(cup, tmp_path):

@@ -35,7 +35,7 @@ def test_obstacle_momenta_match_oracle(cup, oracle, implicit):
     np.testing.assert_allclose(mg, mo, rtol=1e-11, atol=1e-13 * np.abs(mo).max())
 
 
-def run_body(cup, oracle, extra, body, n, ic, nu=0.01, CFL=0.3, lam=1e6):
+def run_body(cup, oracle, extra, body, n, ic, nu=0.01, CFL=0.3, lam=1e6, ramp=None):
     argv = ["-bpdx", "4", "-bpdy", "4", "-bpdz", "4", "-extentx", "1.0", "-CFL", str(CFL), "-nu", str(nu),
             "-nsteps", str(n), "-rampup", "0", "-lambda", str(lam), "-implicitPenalization", "1",
             "-decomposition", "plain"] + extra + PERIODIC
@@ -45,10 +45,13 @@ def run_body(cup, oracle, extra, body, n, ic, nu=0.01, CFL=0.3, lam=1e6):
     O = oracle.OracleSim((4, 4, 4), maxextent=1.0)
     ic(O)
     O.sphere_create(body["radius"], body["position"])
-    uinf = [-body["transVel"][d] if body["fix"][d] else 0.0 for d in range(3)]
+    uinf, t = [0.0] * 3, 0.0  # the first dt sees the flags' frame velocity (0)
     for step in range(n):
         dt, _ = O.calc_max_timestep(CFL, nu, uinf, step, 0)
+        if ramp is not None:
+            body["imposed"][0] = ramp(t)
         uinf, ft, q = O.body_timestep(dt, nu, body, lam, True)
+        t += dt
     return S, O, ft, q
 
 
@@ -94,3 +97,47 @@ def test_forced_sphere_moving_through_the_box(cup, oracle):
     for name in ("u", "v", "w"):
         assert rel(S.field(name), O.get(name)) < 1e-10, name
     np.testing.assert_allclose(ob["force"], ft[:3], rtol=1e-9, atol=1e-11 * np.abs(ft).max())
+
+
+def test_sphere_with_ramped_velocity(cup, oracle):
+    """-accel: the forced x velocity rises to xvel at T and falls to 0 at 2T
+    (Sphere.cpp:90-98, 147-159); the frame follows the body, so U_inf = -V changes every step."""
+    print("accel/decel sphere", flush=True)
+    R, pos, T, umax = 0.12, [0.5, 0.5, 0.5], 0.1, 1.5
+    extra = ["-initCond", "zero", "-sphereL", str(2 * R), "-xpos", str(pos[0]), "-ypos", str(pos[1]), "-zpos",
+             str(pos[2]), "-xvel", str(umax), "-accel", "1", "-T", str(T), "-bForcedInSimFrame", "1",
+             "-bFixFrameOfRef", "1"]
+
+    def ramp(t):
+        return umax * t / T if t < T else (umax * (2 * T - t) / T if t < 2 * T else 0.0)
+
+    body = dict(radius=R, position=list(pos), transVel=[-umax, 0.0, 0.0], angVel=[0.0] * 3, forced=[True] * 3,
+                fix=[True] * 3, block_rot=[True] * 3, imposed=[-umax, 0.0, 0.0])
+    S, O, ft, q = run_body(cup, oracle, extra, body, 6, lambda O: None, ramp=ramp)
+    ob = S.obstacle(0)
+    assert 0 < ob["trans_vel"][0] < umax
+    np.testing.assert_allclose(ob["trans_vel"], body["transVel"], rtol=1e-12, atol=0)
+    np.testing.assert_allclose(ob["position"], pos, rtol=0, atol=0)  # the frame follows the body
+    for name in ("u", "v", "w"):
+        assert rel(S.field(name), O.get(name)) < 1e-10, name
+    np.testing.assert_allclose(ob["force"], ft[:3], rtol=1e-9, atol=1e-11 * np.abs(ft).max())
+
+
+def test_python_sphere_like_the_reference_test(cup):
+    """tests/python/test_obstacles.py of the reference: a 64^3 box with U_inf = 0.1, a sphere
+    added with the bindings' defaults (free, frame at rest), 10 steps. CreateObstacles sets
+    U_inf from the bodies' fixed-frame axes, none here, so the flow stays at rest and so does
+    the body."""
+    print("python sphere as in the reference test", flush=True)
+    SD = cup.SimulationData(cells=[64, 64, 64], CFL=0.1, uinf=[0.1, 0.0, 0.0])
+    SD.nsteps = 10
+    S = cup.Simulation(SD)
+    S.add_obstacle(cup.Sphere(position=[0.3, 0.4, 0.5], radius=0.1))
+    S.run()
+    assert SD.step == 10 and SD.uinf_x == 0.0
+    ob = S.obstacle(0)
+    assert not np.asarray(ob["trans_vel"]).any() and not np.asarray(ob["ang_vel"]).any()
+    np.testing.assert_array_equal(ob["position"], [0.3, 0.4, 0.5])
+    assert ob["mass"] > 0  # a radius of about one cell (h = 2 pi / 64)
+    for name in ("u", "v", "w"):
+        assert not S.field(name).any(), name
