@@ -17,6 +17,7 @@ CUP_E_INVALID = -1
 CUP_E_DEVICE = -2
 CUP_E_STATE = -3
 CUP_E_NOMEM = -4
+CUP_E_IO = -5
 
 CUP_CHI, CUP_U, CUP_V, CUP_W, CUP_P, CUP_TMPU, CUP_TMPV, CUP_TMPW = range(8)
 FIELD_IDS = {"chi": 0, "u": 1, "v": 2, "w": 3, "p": 4, "tmpU": 5, "tmpV": 6, "tmpW": 7,
@@ -138,6 +139,8 @@ def check(rc: int, ctx=None, sim=None):
     msg = msg.decode() if msg else ""
     if rc == CUP_E_INVALID:
         raise ValueError(msg or "invalid argument")
+    if rc == CUP_E_IO:
+        raise OSError(msg or "file error")
     raise CupError(rc, msg)
 
 

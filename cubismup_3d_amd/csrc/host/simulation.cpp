@@ -10,6 +10,7 @@
 #include <fstream>
 #include <iomanip>
 #include <limits>
+#include <cstring>
 #include <sys/stat.h>
 
 namespace cubismup3d {
@@ -121,6 +122,8 @@ SimulationData& SimulationData::operator=(const SimulationData& o)
   uMax_forced = o.uMax_forced; uMax_measured = o.uMax_measured; spectralForcing = o.spectralForcing;
   turbKinEn_target = o.turbKinEn_target; enInjectionRate = o.enInjectionRate;
   freqDiagnostics = o.freqDiagnostics; rampup = o.rampup; verbose = o.verbose; muteAll = o.muteAll;
+  saveFreq = o.saveFreq; saveTime = o.saveTime; nextSaveTime = o.nextSaveTime; bDump = o.bDump;
+  b3Ddump = o.b3Ddump; bRestart = o.bRestart;
   analysis = o.analysis; timeAnalysis = o.timeAnalysis; nextAnalysisTime = o.nextAnalysisTime;
   freqAnalysis = o.freqAnalysis; grad_mean = o.grad_mean; grad_std = o.grad_std; nu_sgs = o.nu_sgs;
   cs2_avg = o.cs2_avg;
@@ -165,6 +168,16 @@ SimulationData::SimulationData(int argc, const char* const* argv) : SimulationDa
   if (turbKinEn_target > 0 && enInjectionRate > 0)
     throw std::invalid_argument("either constant energy injection rate or forcing to fixed energy target");
   freqDiagnostics = parser("-freqDiagnostics").asInt(100);
+  b3Ddump = parser("-dump3D").asBool(true);
+  {
+    const int dumpFreq = (int)parser("-fdump").asDouble(0);
+    const double dumpTime = parser("-tdump").asDouble(0.0);
+    saveFreq = parser("-fsave").asInt(0);
+    saveTime = parser("-tsave").asDouble(0.0);
+    if (saveFreq <= 0 && dumpFreq > 0) saveFreq = dumpFreq;
+    if (saveTime <= 0 && dumpTime > 0) saveTime = dumpTime;
+  }
+  bRestart = parser("-restart").asBool(false);
   analysis = parser("-analysis").asString("");
   timeAnalysis = parser("-tAnalysis").asDouble(0.0);
   freqAnalysis = parser("-fAnalysis").asInt(0);
@@ -338,6 +351,41 @@ void Sphere::update(double dt, const std::array<double, 3>& uinf)
     m = 1 / std::sqrt(m);
     for (int i = 0; i < 4; ++i) quaternion[i] = D[i] * m;
   }
+  _2Dangle = 2 * std::atan2(quaternion[3], quaternion[0]);
+}
+
+void Sphere::save(const std::string& filename, double time) const
+{
+  std::ofstream f(filename + ".txt");
+  if (!f) throw IOError("Couldn't open " + filename + ".txt");
+  f.setf(std::ios::scientific);
+  f.precision(std::numeric_limits<double>::digits10 + 1);
+  f << time << "\n";
+  f << position[0] << "\t" << position[1] << "\t" << position[2] << "\n";
+  f << absPos[0] << "\t" << absPos[1] << "\t" << absPos[2] << "\n";
+  f << quaternion[0] << "\t" << quaternion[1] << "\t" << quaternion[2] << "\t" << quaternion[3] << "\n";
+  f << transVel[0] << "\t" << transVel[1] << "\t" << transVel[2] << "\n";
+  f << angVel[0] << "\t" << angVel[1] << "\t" << angVel[2] << "\n";
+  f << _2Dangle << "\n";
+}
+
+bool Sphere::restart(const std::string& filename)
+{
+  std::ifstream f(filename + ".txt");
+  if (!f.good()) {
+    printf("Could not restart from file\n");
+    return false;
+  }
+  double t = 0;
+  f >> t;
+  f >> position[0] >> position[1] >> position[2];
+  f >> absPos[0] >> absPos[1] >> absPos[2];
+  f >> quaternion[0] >> quaternion[1] >> quaternion[2] >> quaternion[3];
+  f >> transVel[0] >> transVel[1] >> transVel[2];
+  f >> angVel[0] >> angVel[1] >> angVel[2];
+  f >> _2Dangle;
+  if (!f) throw IOError("Error reading " + filename + ".txt");
+  return true;
 }
 
 // Gaussian elimination with partial pivoting on a 6x6 system (the LU solve of the reference).
@@ -590,7 +638,7 @@ void Analysis::operator()(double dt)
   char name[64];
   snprintf(name, sizeof name, "analysis/spectralAnalysis_%09d", nFile);
   std::ofstream f(name);
-  if (!f) throw std::runtime_error(std::string("Analysis: cannot write ") + name);
+  if (!f) throw IOError(std::string("Analysis: cannot write ") + name);
   f << std::left << "Spectral Analysis :" << "\n";
   for (const Row& r : rows) f << std::setw(15) << r.name << std::setw(15) << r.value << " #" << r.what << "\n";
   f << "\n" << std::setw(15) << "k * (lBox/2pi)" << std::setw(15) << "E_k" << "\n";
@@ -598,14 +646,148 @@ void Analysis::operator()(double dt)
     f << std::setw(15) << (i + 1) * 2 * M_PI / sim.maxextent << std::setw(15) << spectrum[i] << "\n";
 }
 
+// ------------------------------------------------------------- field files
+
+// The saves' field container (the reference writes HDF5 through the Cubism library, whose
+// layout is not available here): a 64-byte header -- the 8 bytes "CUPFIELD", int32 version
+// (1), int32 ncomp, int32 cells x, y, z, int32 step, int32 0, FP64 time, 20 bytes of zeros --
+// then ncomp FP64 arrays in [comp][z][y][x] order, little-endian.
+static void write_field_file(const std::string& path, const SimulationData& sim, const int* fields, int ncomp)
+{
+  const int N[3] = {sim.bpdx * 8, sim.bpdy * 8, sim.bpdz * 8};
+  const size_t n = (size_t)N[0] * N[1] * N[2];
+  std::vector<double> blocks(n), grid(n);
+  std::ofstream f(path, std::ios::binary);
+  if (!f) throw IOError("cannot write " + path);
+  char head[64] = {};
+  std::memcpy(head, "CUPFIELD", 8);
+  const int32_t ints[7] = {1, ncomp, N[0], N[1], N[2], sim.step, 0};
+  std::memcpy(head + 8, ints, sizeof ints);
+  std::memcpy(head + 36, &sim.time, sizeof(double));
+  f.write(head, sizeof head);
+  for (int q = 0; q < ncomp; ++q) {
+    if (cup_get_field(sim.ctx, fields[q], blocks.data()) != CUP_OK)
+      throw std::runtime_error(std::string("save: ") + cup_last_error(sim.ctx));
+    for (size_t b = 0; b < n / 512; ++b) {
+      const int bx = (int)(b % sim.bpdx), by = (int)((b / sim.bpdx) % sim.bpdy), bz = (int)(b / ((size_t)sim.bpdx * sim.bpdy));
+      for (int c = 0; c < 512; ++c) {
+        const size_t x = bx * 8 + (c & 7), y = by * 8 + ((c >> 3) & 7), z = bz * 8 + (c >> 6);
+        grid[(z * N[1] + y) * N[0] + x] = blocks[b * 512 + c];
+      }
+    }
+    f.write(reinterpret_cast<const char*>(grid.data()), n * sizeof(double));
+  }
+  if (!f) throw IOError("cannot write " + path);
+}
+
+static void read_field_file(const std::string& path, const SimulationData& sim, const int* fields, int ncomp)
+{
+  const int N[3] = {sim.bpdx * 8, sim.bpdy * 8, sim.bpdz * 8};
+  const size_t n = (size_t)N[0] * N[1] * N[2];
+  std::ifstream f(path, std::ios::binary);
+  if (!f) throw IOError("Unable to restart: cannot read " + path);
+  char head[64];
+  f.read(head, sizeof head);
+  int32_t ints[7];
+  std::memcpy(ints, head + 8, sizeof ints);
+  if (!f || std::memcmp(head, "CUPFIELD", 8) != 0 || ints[0] != 1 || ints[1] != ncomp || ints[2] != N[0] ||
+      ints[3] != N[1] || ints[4] != N[2])
+    throw IOError("Unable to restart: " + path + " does not hold this grid's fields");
+  std::vector<double> blocks(n), grid(n);
+  for (int q = 0; q < ncomp; ++q) {
+    f.read(reinterpret_cast<char*>(grid.data()), n * sizeof(double));
+    if (!f) throw IOError("Unable to restart: " + path + " is truncated");
+    for (size_t b = 0; b < n / 512; ++b) {
+      const int bx = (int)(b % sim.bpdx), by = (int)((b / sim.bpdx) % sim.bpdy), bz = (int)(b / ((size_t)sim.bpdx * sim.bpdy));
+      for (int c = 0; c < 512; ++c) {
+        const size_t x = bx * 8 + (c & 7), y = by * 8 + ((c >> 3) & 7), z = bz * 8 + (c >> 6);
+        blocks[b * 512 + c] = grid[(z * N[1] + y) * N[0] + x];
+      }
+    }
+    if (cup_set_field(sim.ctx, fields[q], blocks.data()) != CUP_OK)
+      throw std::runtime_error(std::string("restart: ") + cup_last_error(sim.ctx));
+  }
+}
+
 // ----------------------------------------------------------------- Simulation
+
+// Simulation::_serialize (Simulation.cpp:426-532): <path>/<name>.status, the obstacles'
+// <name>_<i>.txt, the fields, then <path>/restart.status; name = restart_ or the given prefix,
+// followed by the step in 9 digits.
+void Simulation::_serialize(const std::string& append)
+{
+  char num[16];
+  snprintf(num, sizeof num, "%09d", sim.step);
+  const std::string name = (append.empty() ? std::string("restart_") : append) + num;
+  mkdir(sim.path4serialization.c_str(), 0775);  // EEXIST when it is there already
+  const std::string fpath = sim.path4serialization + "/" + name;
+  for (size_t i = 0; i < sim.obstacles.size(); ++i) sim.obstacles[i]->save(fpath + "_" + std::to_string(i), sim.time);
+  auto status = [&](const std::string& file) {
+    FILE* f = fopen(file.c_str(), "w");
+    if (!f) throw IOError("cannot write " + file);
+    fprintf(f, "time: %20.20e\n", sim.time);
+    fprintf(f, "stepid: %d\n", sim.step);
+    fprintf(f, "uinfx: %20.20e\n", sim.uinf[0]);
+    fprintf(f, "uinfy: %20.20e\n", sim.uinf[1]);
+    fprintf(f, "uinfz: %20.20e\n", sim.uinf[2]);
+    fclose(f);
+  };
+  status(fpath + ".status");
+  if (sim.b3Ddump) {
+    const int vel[3] = {CUP_U, CUP_V, CUP_W}, pres[1] = {CUP_P}, chi[1] = {CUP_CHI};
+    write_field_file(sim.path4serialization + "/vel_" + name + ".cupf", sim, vel, 3);
+    write_field_file(sim.path4serialization + "/pres_" + name + ".cupf", sim, pres, 1);
+    write_field_file(sim.path4serialization + "/chi_" + name + ".cupf", sim, chi, 1);
+  }
+  status(sim.path4serialization + "/restart.status");
+}
+
+// Simulation::_deserialize (Simulation.cpp:534-580): restart.status, then the velocity of
+// restart_<step> and the bodies' state; chi is built again where the bodies are. Without a
+// restart.status the run starts afresh from the IC (the reference starts from the fields as
+// allocated).
+void Simulation::_deserialize()
+{
+  const std::string file = sim.path4serialization + "/restart.status";
+  FILE* f = fopen(file.c_str(), "r");
+  if (!f) {
+    printf("Could not restart... starting a new sim.\n");
+    _ic();
+    return;
+  }
+  double t = -1, u[3] = {0, 0, 0};
+  int step = -1;
+  bool ok = fscanf(f, "time: %le\n", &t) == 1;
+  ok = ok && fscanf(f, "stepid: %d\n", &step) == 1;
+  ok = ok && fscanf(f, "uinfx: %le\n", &u[0]) == 1;
+  ok = ok && fscanf(f, "uinfy: %le\n", &u[1]) == 1;
+  ok = ok && fscanf(f, "uinfz: %le\n", &u[2]) == 1;
+  fclose(f);
+  if (!ok || step < 0 || t < 0) throw IOError("Error reading restart file " + file);
+  sim.time = t;
+  sim.step = step;
+  sim.uinf = {u[0], u[1], u[2]};
+  char num[16];
+  snprintf(num, sizeof num, "%09d", step);
+  const std::string name = std::string("restart_") + num;
+  const int vel[3] = {CUP_U, CUP_V, CUP_W};
+  read_field_file(sim.path4serialization + "/vel_" + name + ".cupf", sim, vel, 3);
+  for (size_t i = 0; i < sim.obstacles.size(); ++i)
+    sim.obstacles[i]->restart(sim.path4serialization + "/" + name + "_" + std::to_string(i));
+  for (auto& op : sim.pipeline)
+    if (auto* c = dynamic_cast<CreateObstacles*>(op.get())) c->build();
+  if (cup_set_step_state(sim.ctx, sim.step, sim.time) != CUP_OK)
+    throw std::runtime_error(std::string("restart: ") + cup_last_error(sim.ctx));
+  sim.nextSaveTime = sim.time + sim.saveTime;
+  printf("DESERIALIZATION: time is %f and step id is %d\n", sim.time, sim.step);
+}
 
 Simulation::Simulation(const SimulationData& sd) : sim(sd)
 {
   sim._preprocessArguments();
   setupGrid();
   setupOperators();
-  _ic();
+  _init();
 }
 
 Simulation::Simulation(int argc, const char* const* argv) : sim(argc, argv)
@@ -613,7 +795,16 @@ Simulation::Simulation(int argc, const char* const* argv) : sim(argc, argv)
   sim._preprocessArguments();
   setupGrid();
   setupOperators();
-  _ic();
+  _init();
+}
+
+// Simulation::_init (Simulation.cpp:154-165): restart or IC, then the "init" save. The init
+// save is written only when saves are asked for (-fsave/-tsave, or -fdump/-tdump).
+void Simulation::_init()
+{
+  if (sim.bRestart) _deserialize();
+  else _ic();
+  if (sim.saveFreq > 0 || sim.saveTime > 0) _serialize("init");
 }
 
 Simulation::~Simulation()
@@ -697,10 +888,15 @@ double Simulation::calcMaxTimestep()
 // Simulation::timestep (Simulation.cpp:593-623).
 bool Simulation::timestep(double dt)
 {
+  const bool bDumpFreq = sim.saveFreq > 0 && (sim.step + 1) % sim.saveFreq == 0;
+  const bool bDumpTime = sim.saveTime > 0 && sim.time + dt > sim.nextSaveTime;
+  if (bDumpTime) sim.nextSaveTime += sim.saveTime;
+  sim.bDump = bDumpFreq || bDumpTime;
   for (auto& op : sim.pipeline) (*op)(dt);
   sim.step++;
   sim.time += dt;
   cup_set_step_state(sim.ctx, sim.step, sim.time);
+  if (sim.bDump) _serialize();
   if (sim.verbose) printf("%d : %f uInf {%f %f %f}\n", sim.step, sim.time, sim.uinf[0], sim.uinf[1], sim.uinf[2]);
   if ((sim.endTime > 0 && sim.time > sim.endTime) || (sim.nsteps != 0 && sim.step >= sim.nsteps)) return true;
   return false;
@@ -780,7 +976,8 @@ void Simulation::run()
 {
   const bool analysis = sim.analysis == "HIT" && (sim.freqAnalysis > 0 || sim.timeAnalysis > 0);
   // the device-resident loop writes no per-step files: taken when none is asked for
-  const bool logs = !sim.muteAll && (!sim.obstacles.empty() || sim.spectralForcing);
+  const bool logs = (!sim.muteAll && (!sim.obstacles.empty() || sim.spectralForcing)) || sim.saveFreq > 0 ||
+                    sim.saveTime > 0;
   if (sim.decomposition == CUP_FUSED && sim.nsteps > 0 && sim.endTime <= 0 && sim.freqDiagnostics == 0 && !analysis &&
       !logs) {
     advance(sim.nsteps - sim.step);
@@ -814,6 +1011,9 @@ static int guarded(cup_sim* h, F&& f)
   } catch (const std::invalid_argument& e) {
     h->err = e.what();
     return CUP_E_INVALID;
+  } catch (const IOError& e) {
+    h->err = e.what();
+    return CUP_E_IO;
   } catch (const std::exception& e) {
     h->err = e.what();
     return CUP_E_DEVICE;

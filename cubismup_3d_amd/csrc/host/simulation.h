@@ -26,6 +26,11 @@ BCflag string2BCflag(const std::string& s);
 // "-name value" command-line arguments (the subset of cubism::ArgumentParser used by
 // SimulationData's constructor, SimulationData.cpp:32-172). Keys are matched with or
 // without their leading dash.
+// A file that could not be read or written (CUP_E_IO at the C ABI).
+struct IOError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
 class ArgumentParser
 {
  public:
@@ -81,6 +86,7 @@ struct Sphere {
   double radius = 0;
   std::array<double, 3> position = {0, 0, 0}, absPos = {0, 0, 0};
   std::array<double, 4> quaternion = {1, 0, 0, 0};
+  double _2Dangle = 0;
   std::array<bool, 3> bForcedInSimFrame = {false, false, false};
   std::array<bool, 3> bFixFrameOfRef = {false, false, false};
   std::array<bool, 3> bBlockRotation = {false, false, false};
@@ -103,6 +109,10 @@ struct Sphere {
   bool isStatic() const;
   // Obstacle::update (Obstacle.cpp:314-345): position, absolute position and quaternion.
   void update(double dt, const std::array<double, 3>& uinf);
+  // Obstacle::save / restart (Obstacle.cpp:402-450): <filename>.txt with time, position,
+  // absPos, quaternion, transVel, angVel and the 2-D angle.
+  void save(const std::string& filename, double time) const;
+  bool restart(const std::string& filename);
   // Sphere::computeVelocities then Obstacle::computeVelocities (Sphere.cpp:147-159,
   // Obstacle.cpp:153-264) on UpdateObstacles' 29 momentum sums at simulation time `time`.
   void computeVelocities(const double M[29], bool implicit, double time);
@@ -136,6 +146,10 @@ struct SimulationData {
   // settings (SimulationData.h:141-147)
   int freqDiagnostics = 0;
   int rampup = 100;
+  // serialization (SimulationData.h:148-153, SimulationData.cpp:96-115)
+  int saveFreq = 0;
+  double saveTime = 0, nextSaveTime = 0;
+  bool bDump = false, b3Ddump = true, bRestart = false;
   // analysis (SimulationData.h:122-131); nu_sgs and cs2_avg stay 0 (no SGS model here)
   std::string analysis = "";
   double timeAnalysis = 0, nextAnalysisTime = 0;
@@ -299,8 +313,16 @@ class Simulation
 
  private:
   void setupGrid();
+  void _init();
   void _ic();
   void syncStepState();
+
+ public:
+  // Simulation::_serialize / _deserialize (Simulation.cpp:426-580): <name>.status and
+  // restart.status as the reference writes them, the obstacles' .txt files, and the fields in
+  // vel_/pres_/chi_<name>.cupf (write_field_file); a restart reads the velocity and the bodies.
+  void _serialize(const std::string& append = "");
+  void _deserialize();
 };
 
 }  // namespace cubismup3d

@@ -17,6 +17,26 @@ from . import _lib as L
 from .grid import Grid, from_blocks
 
 
+def read_field_file(path):
+    """A field file of the saves (vel_/pres_/chi_<name>.cupf, written by Simulation::_serialize):
+    a 64-byte header ("CUPFIELD", int32 version 1, ncomp, cells x, y, z, step, 0, FP64 time)
+    then ncomp FP64 arrays in [comp][z][y][x] order. Returns dict(time, step, data)."""
+    import struct
+    import numpy as np
+    with open(path, "rb") as f:
+        head = f.read(64)
+        if len(head) != 64 or head[:8] != b"CUPFIELD":
+            raise ValueError(f"{path}: not a field file")
+        version, ncomp, nx, ny, nz, step, _ = struct.unpack("<7i", head[8:36])
+        if version != 1:
+            raise ValueError(f"{path}: version {version}")
+        (time,) = struct.unpack("<d", head[36:44])
+        data = np.fromfile(f, dtype="<f8", count=ncomp * nx * ny * nz)
+    if data.size != ncomp * nx * ny * nz:
+        raise ValueError(f"{path}: truncated")
+    return dict(time=time, step=step, data=data.reshape(ncomp, nz, ny, nx))
+
+
 class Sphere:
     """ObstacleAndSphereArguments of the bindings (bindings/Obstacles.cpp:12-71, 79-128):
     ``radius`` or ``length``; ``position``; ``enforcedVelocity``, the velocity of the axes

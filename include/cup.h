@@ -28,6 +28,7 @@ extern "C" {
 #define CUP_E_DEVICE -2    /* HIP runtime error */
 #define CUP_E_STATE -3     /* call out of order (e.g. no obstacle created) */
 #define CUP_E_NOMEM -4
+#define CUP_E_IO -5        /* a file could not be read or written (saves, restarts, analysis files) */
 
 /* Field ids, same order as the reference's FluidElement (Definitions.h:39), then the
  * adjoint velocity of cup_adjoint_step (no reference counterpart; allocated on first use). */
