@@ -430,6 +430,16 @@ void Sphere::computeVelocities(const double M[29], bool implicit, double time)
                     {0, 0, pm, cm[1], -cm[0], 0},       {0, -cm[2], cm[1], J[0], J[3], J[4]},
                     {cm[2], 0, -cm[0], J[3], J[1], J[5]}, {-cm[1], cm[0], 0, J[4], J[5], J[2]}};
   double b[6] = {L[0], L[1], L[2], Am[0], Am[1], Am[2]};
+  penalM = pm;
+  for (int d = 0; d < 3; ++d) {
+    penalCM[d] = cm[d];
+    penalLmom[d] = L[d];
+    penalAmom[d] = Am[d];
+  }
+  for (int i = 0; i < 6; ++i) {
+    penalJ[i] = J[i];
+    this->J[i] = M[7 + i];
+  }
   // a forced axis or a blocked rotation keeps only its diagonal (Obstacle.cpp:172-200)
   for (int d = 0; d < 3; ++d) {
     if (bForcedInSimFrame[d]) {
@@ -446,6 +456,8 @@ void Sphere::computeVelocities(const double M[29], bool implicit, double time)
   double x[6];
   solve6(A, b, x);
   for (int d = 0; d < 3; ++d) {
+    transVel_computed[d] = x[d];
+    angVel_computed[d] = x[3 + d];
     transVel[d] = bForcedInSimFrame[d] ? transVel_imposed[d] : x[d];
     angVel[d] = bBlockRotation[d] ? 0.0 : x[3 + d];
   }
@@ -465,6 +477,28 @@ void CreateObstacles::operator()(double dt)
   sim.uinf = u;
   const auto before = s.position;
   s.update(dt, sim.uinf);
+  if (dt > 0 && !sim.muteAll) {  // Obstacle::_writeComputedVelToFile (Obstacle.cpp:491-515)
+    std::stringstream& f = sim.logger->get_stream("computedVelocity_" + std::to_string(s.obstacleID) + ".dat");
+    const std::string tab("\t");
+    if (sim.step == 0 && !velHeader) {
+      velHeader = true;
+      f << "step" << tab << "time" << tab << "CMx" << tab << "CMy" << tab << "CMz" << tab << "quat_0" << tab
+        << "quat_1" << tab << "quat_2" << tab << "quat_3" << tab << "vel_x" << tab << "vel_y" << tab << "vel_z" << tab
+        << "angvel_x" << tab << "angvel_y" << tab << "angvel_z" << tab << "mass" << tab << "J0" << tab << "J1" << tab
+        << "J2" << tab << "J3" << tab << "J4" << tab << "J5" << std::endl;
+    }
+    f << sim.step << tab;
+    f.setf(std::ios::scientific);
+    f.precision(std::numeric_limits<float>::digits10 + 1);
+    f << sim.time;
+    for (double v : s.position) f << tab << v;
+    for (double v : s.quaternion) f << tab << v;
+    for (double v : s.transVel) f << tab << v;
+    for (double v : s.angVel) f << tab << v;
+    f << tab << s.mass;
+    for (double v : s.J) f << tab << v;
+    f << std::endl;
+  }
   if (!created || s.position != before) build();
 }
 
@@ -523,12 +557,12 @@ void PressureProjection::operator()(double dt)
 
 // UpdateObstacles (ObstaclesUpdate.cpp:202-236): the body's momentum integrals on the
 // projected velocity, then Obstacle::computeVelocities. A static body keeps its velocities
-// (every axis forced, rotation blocked), so its step skips the integrals.
+// (every axis forced, rotation blocked), so without log files its step skips the integrals.
 void UpdateObstacles::operator()(double dt)
 {
   if (sim.obstacles.empty()) return;
   auto& s = *sim.obstacles[0];
-  if (s.isStatic()) return;
+  if (s.isStatic() && sim.muteAll) return;
   double M[29];
   check(cup_obstacle_momenta(sim.ctx, dt, sim.lambda, sim.bImplicitPenalization ? 1 : 0, M), "UpdateObstacles");
   s.computeVelocities(M, sim.bImplicitPenalization, sim.time);
@@ -569,6 +603,48 @@ void ComputeForces::operator()(double)
   ssF << sim.time << tab << s.mass << tab << q[0] << tab << q[1] << tab << q[2] << tab << q[9] << tab << q[10] << tab
       << q[11] << tab << q[3] << tab << q[4] << tab << q[5] << tab << q[6] << tab << q[7] << tab << q[8] << tab << q[12]
       << tab << q[13] << tab << q[14] << tab << q[15] << tab << q[16] << std::endl;
+
+  // powerValues_surface_<id>.dat: the derived powers of Obstacle::computeForces (:288-296)
+  const double EPS = std::numeric_limits<double>::epsilon();
+  const double vn = std::sqrt(s.transVel[0] * s.transVel[0] + s.transVel[1] * s.transVel[1] + s.transVel[2] * s.transVel[2]);
+  const double Pthrust = q[16] * vn, Pdrag = q[15] * vn, Pout = q[17], PoutBnd = q[18], defPower = q[19],
+               defPowerBnd = q[20], pLocom = q[21];
+  const double EffPDef = Pthrust / (Pthrust - std::min(defPower, 0.0) + EPS);
+  const double EffPDefBnd = Pthrust / (Pthrust - defPowerBnd + EPS);
+  std::stringstream& ssP = sim.logger->get_stream("powerValues_surface_" + std::to_string(s.obstacleID) + ".dat");
+  if (sim.step == 0)
+    ssP << "step" << tab << "time" << tab << "Pthrust" << tab << "Pdrag" << tab << "Pout" << tab << "pDef" << tab
+        << "etaPDef" << tab << "pLocom" << tab << "PoutBnd" << tab << "defPowerBnd" << tab << "etaPDefBnd" << std::endl;
+  ssP << sim.step << tab;
+  ssP.setf(std::ios::scientific);
+  ssP.precision(std::numeric_limits<float>::digits10 + 1);
+  ssP << sim.time << tab << Pthrust << tab << Pdrag << tab << Pout << tab << -defPower << tab << EffPDef << tab << pLocom
+      << tab << PoutBnd << tab << -defPowerBnd << tab << EffPDefBnd << std::endl;
+
+  // forceValues_<id>.dat: the penalization force and the momentum solve (Obstacle.cpp:557-589)
+  std::stringstream& ssD = sim.logger->get_stream("forceValues_" + std::to_string(s.obstacleID) + ".dat");
+  if (sim.step == 0)
+    ssD << "step" << tab << "time" << tab << "mass" << tab << "force_x" << tab << "force_y" << tab << "force_z" << tab
+        << "torque_x" << tab << "torque_y" << tab << "torque_z" << tab << "penalLmom_x" << tab << "penalLmom_y" << tab
+        << "penalLmom_z" << tab << "penalAmom_x" << tab << "penalAmom_y" << tab << "penalAmom_z" << tab << "penalCM_x"
+        << tab << "penalCM_y" << tab << "penalCM_z" << tab << "linVel_comp_x" << tab << "linVel_comp_y" << tab
+        << "linVel_comp_z" << tab << "angVel_comp_x" << tab << "angVel_comp_y" << tab << "angVel_comp_z" << tab
+        << "penalM" << tab << "penalJ0" << tab << "penalJ1" << tab << "penalJ2" << tab << "penalJ3" << tab << "penalJ4"
+        << tab << "penalJ5" << std::endl;
+  ssD << sim.step << tab;
+  ssD.setf(std::ios::scientific);
+  ssD.precision(std::numeric_limits<float>::digits10 + 1);
+  ssD << sim.time << tab << s.mass;
+  for (double v : s.force) ssD << tab << v;
+  for (double v : s.torque) ssD << tab << v;
+  for (double v : s.penalLmom) ssD << tab << v;
+  for (double v : s.penalAmom) ssD << tab << v;
+  for (double v : s.penalCM) ssD << tab << v;
+  for (double v : s.transVel_computed) ssD << tab << v;
+  for (double v : s.angVel_computed) ssD << tab << v;
+  ssD << tab << s.penalM;
+  for (double v : s.penalJ) ssD << tab << v;
+  ssD << std::endl;
 }
 
 // ComputeDissipation::operator() (ComputeDissipation.cpp:119-178) and diagnostics.dat.

@@ -97,6 +97,11 @@ struct Sphere {
   double mass = 0;
   std::array<double, 3> force = {0, 0, 0}, torque = {0, 0, 0};  // penalization integrals
   std::array<double, 22> surfaceQoI = {};                        // ObstacleBlock::sumQoI order
+  // UpdateObstacles' penalized momenta and the velocities they solve to (Obstacle.h:120-140)
+  double penalM = 0;
+  std::array<double, 3> penalLmom = {0, 0, 0}, penalAmom = {0, 0, 0}, penalCM = {0, 0, 0};
+  std::array<double, 6> penalJ = {0, 0, 0, 0, 0, 0}, J = {0, 0, 0, 0, 0, 0};
+  std::array<double, 3> transVel_computed = {0, 0, 0}, angVel_computed = {0, 0, 0};
   int obstacleID = 0;
   // SphereArguments (Sphere.h:17-29): the ramped x velocity, up to umax at tmax, 0 at 2 tmax
   bool accel_decel = false;
@@ -210,6 +215,7 @@ class CreateObstacles : public Operator
   void build();
   std::string getName() override { return "CreateObstacles"; }
   bool created = false;
+  bool velHeader = false;
 };
 
 class AdvectionDiffusion : public Operator

@@ -146,3 +146,45 @@ def test_analysis_files_every_t_time(cup, tmp_path, monkeypatch):
     assert SD.nextAnalysisTime == pytest.approx(nxt, rel=1e-12)
     assert sorted(os.listdir("analysis")) == sorted(f"spectralAnalysis_{i:09d}" for i in expect)
     assert len(expect) >= 3
+
+
+def test_body_log_files(cup, tmp_path):
+    """The per-body files of Obstacle::computeForces and Obstacle::update
+    (Obstacle.cpp:491-589): forceValues_surface_0, powerValues_surface_0, forceValues_0 and
+    computedVelocity_0, one row per step after the header, consistent with each other."""
+    print("body log files", flush=True)
+    argv = ["-bpdx", "8", "-bpdy", "4", "-bpdz", "4", "-extentx", "2.0", "-CFL", "0.3", "-nu", "1e-3", "-nsteps", "3",
+            "-rampup", "0", "-sphereL", "0.3", "-xpos", "0.5", "-ypos", "0.5", "-zpos", "0.5", "-xvel", "1.0",
+            "-bForcedInSimFrame", "1", "-bFixFrameOfRef", "1", "-lambda", "1e6", "-implicitPenalization", "1",
+            "-decomposition", "plain", "-BC_x", "periodic", "-BC_y", "periodic", "-BC_z", "periodic",
+            "-freqDiagnostics", "0", "-precision", "double", "-muteAll", "0", "-serialization", str(tmp_path)]
+    SD = cup.SimulationData(argv=argv)
+    S = cup.Simulation(SD)
+    S.run()
+    S.flush_logs()
+
+    def table(name):
+        with open(tmp_path / name) as f:
+            rows = [ln.split() for ln in f.read().splitlines() if ln.strip()]
+        head = rows[0]
+        data = np.array([[float(x) for x in r] for r in rows[1:]])
+        return head, data
+
+    hs, fs = table("forceValues_surface_0.dat")
+    hp, fp = table("powerValues_surface_0.dat")
+    hd, fd = table("forceValues_0.dat")
+    hv, fv = table("computedVelocity_0.dat")
+    assert hs[:3] == ["step", "time", "mass"] and hs[-2:] == ["drag", "thrust"] and len(hs) == 20
+    assert hp == ["step", "time", "Pthrust", "Pdrag", "Pout", "pDef", "etaPDef", "pLocom", "PoutBnd", "defPowerBnd",
+                  "etaPDefBnd"]
+    assert len(hd) == 31 and hd[-1] == "penalJ5" and len(hv) == 22 and hv[-1] == "J5"
+    assert fs.shape[0] == fp.shape[0] == fd.shape[0] == 3 and fv.shape[0] == 3
+    np.testing.assert_array_equal(fs[:, 0], [0, 1, 2])
+    # |V| = 1: Pdrag = drag, Pthrust = thrust
+    np.testing.assert_allclose(fp[:, 3], fs[:, hs.index("drag")], rtol=1e-6)
+    np.testing.assert_allclose(fp[:, 2], fs[:, hs.index("thrust")], rtol=1e-6, atol=1e-12)
+    # the forced axis solves to the imposed velocity; the penalization force is the body's
+    assert (fd[:, hd.index("linVel_comp_x")] == -1.0).all() and (fd[:, hd.index("penalM")] > 0).all()
+    ob = S.obstacle(0)
+    np.testing.assert_allclose(fd[-1, 3:6], ob["force"], rtol=1e-6, atol=1e-9)
+    assert (fv[:, hv.index("vel_x")] == -1.0).all() and (fv[:, hv.index("CMx")] == 0.5).all()
