@@ -1,10 +1,11 @@
 """Differentiable rollouts of the periodic timestep for torch.
 
 ``rollout(u0, grid, nsteps, nu, ...)`` advances a [3][Z][Y][X] velocity tensor through
-nsteps steps of the device path (central advection with Euler, FFT projection; no forcing,
-no obstacle) and is a ``torch.autograd.Function``: its backward is the hand-written
-adjoint of the step (cup_adjoint_step: the transposed linearised advection-diffusion
-stencil and the symmetric projection), run backwards over the steps with
+nsteps steps of the device path (central advection with Euler, or the third-order upwind
+scheme with Euler or the midpoint rule; FFT projection; no forcing, no obstacle) and is a
+``torch.autograd.Function``: its backward is the hand-written adjoint of the step
+(cup_adjoint_step: the transposed linearised advection-diffusion stencil and the
+symmetric projection), run backwards over the steps with
 checkpoint-and-recompute: the forward keeps the velocity every ``every`` steps, and the
 backward recomputes each segment's states from its checkpoint before stepping the adjoint
 through it. Timesteps are those of the forward pass, held fixed (the CFL's max over the
@@ -19,22 +20,24 @@ from .grid import Grid
 
 class Rollout(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, u0, grid: Grid, nsteps: int, nu: float, uinf, dts, every: int, CFL: float, dts_out: list):
+    def forward(ctx, u0, grid: Grid, nsteps: int, nu: float, uinf, dts, every: int, CFL: float, dts_out: list,
+                scheme: int):
         """dts: fixed timesteps, or None for CFL timesteps (appended to dts_out)."""
         grid.set_torch_velocity(u0.detach())
         ckpts = {0: u0.detach().clone()}
         used = []
         for n in range(nsteps):
             if dts is None:
-                grid.run_steps(1, CFL=CFL, nu=nu, uinf=uinf)
+                grid.run_steps(1, CFL=CFL, nu=nu, uinf=uinf, scheme=scheme)
                 used.append(grid.step_state()["last_dt"])
             else:
-                grid.run_steps(1, nu=nu, uinf=uinf, fixed_dt=dts[n])
+                grid.run_steps(1, nu=nu, uinf=uinf, fixed_dt=dts[n], scheme=scheme)
                 used.append(float(dts[n]))
             if (n + 1) % every == 0 and n + 1 < nsteps:
                 ckpts[n + 1] = grid.get_torch_velocity()
         dts_out.extend(used)
         ctx.grid, ctx.nsteps, ctx.nu, ctx.uinf, ctx.dts, ctx.every = grid, nsteps, nu, uinf, used, every
+        ctx.scheme = scheme
         ctx.ckpts = ckpts
         ctx.u0_dtype = u0.dtype
         return grid.get_torch_velocity()
@@ -48,25 +51,25 @@ class Rollout(torch.autograd.Function):
             states = [ctx.ckpts[s0]]
             grid.set_torch_velocity(ctx.ckpts[s0])
             for n in range(s0, s1 - 1):
-                grid.run_steps(1, nu=ctx.nu, uinf=ctx.uinf, fixed_dt=ctx.dts[n])
+                grid.run_steps(1, nu=ctx.nu, uinf=ctx.uinf, fixed_dt=ctx.dts[n], scheme=ctx.scheme)
                 states.append(grid.get_torch_velocity())
             for n in reversed(range(s0, s1)):
                 grid.set_torch_velocity(states[n - s0])
                 grid.set_torch_adjoint(lam)
-                grid.adjoint_step(ctx.dts[n], ctx.nu, ctx.uinf)
+                grid.adjoint_step(ctx.dts[n], ctx.nu, ctx.uinf, scheme=ctx.scheme)
                 lam = grid.get_torch_adjoint()
         ctx.ckpts = None
-        return lam.to(ctx.u0_dtype), None, None, None, None, None, None, None, None
+        return lam.to(ctx.u0_dtype), None, None, None, None, None, None, None, None, None
 
 
 def rollout(u0, grid: Grid, nsteps: int, nu: float, uinf=(0.0, 0.0, 0.0), dts=None, CFL=0.3, every=10,
-            dts_out=None):
+            dts_out=None, scheme=0):
     """u_N of a rollout, differentiable with respect to u0 ([3][Z][Y][X], cuda). With dts
     None the steps take CFL timesteps (rampup 0), which are appended to dts_out and held
-    fixed by the backward."""
+    fixed by the backward. scheme: 0 central Euler, 1 upwind Euler, 2 upwind midpoint."""
     out = dts_out if dts_out is not None else []
     return Rollout.apply(u0, grid, int(nsteps), float(nu), tuple(float(x) for x in uinf),
-                         None if dts is None else list(dts), int(every), float(CFL), out)
+                         None if dts is None else list(dts), int(every), float(CFL), out, int(scheme))
 
 
 class PenalizedRollout(torch.autograd.Function):

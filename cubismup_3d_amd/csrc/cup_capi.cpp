@@ -657,9 +657,14 @@ int cup_adjoint_step(cup_ctx* ctx, double dt, double nu, const double uinf[3], i
 {
   CUP_GUARD(ctx);
   if (!uinf || !(dt > 0)) return set_error(c, CUP_E_INVALID, "adjoint_step: dt must be > 0");
-  if (scheme != CUP_CENTRAL_EULER)
-    return set_error(c, CUP_E_INVALID, "adjoint_step: only the central Euler scheme has an adjoint");
+  if (scheme != CUP_CENTRAL_EULER && scheme != CUP_UPWIND3_EULER && scheme != CUP_UPWIND3_RK2)
+    return set_error(c, CUP_E_INVALID, "adjoint_step: unknown scheme");
   if (c.body.present) return set_error(c, CUP_E_STATE, "adjoint_step: obstacles are not supported");
+  if (scheme == CUP_UPWIND3_RK2)
+    for (int q = 0; q < 3; ++q)
+      if (!c.f[F_ADJ2 + q] &&
+          check_hip(c, hipMalloc(&c.f[F_ADJ2 + q], c.ncell * c.esz), "hipMalloc adjoint scratch"))
+        return CUP_E_NOMEM;
   for (int f = CUP_ADJ_U; f <= CUP_ADJ_W; ++f)
     if (int rc = field_ready(c, f, "adjoint_step")) return rc;
   if (int rc = launch_set_dt(c, dt)) return rc;
@@ -671,7 +676,17 @@ int cup_adjoint_step(cup_ctx* ctx, double dt, double nu, const double uinf[3], i
   if (rc) return rc;
   if (int e = launch_gradp_inplace(c, CUP_ADJ_U)) return e;
   // lam_n = A'(u_n)^T lam* into tmp, then tmp and the adjoint fields trade buffers
-  if (int e = launch_adjoint_advdiff(c, nu, uinf)) return e;
+  if (scheme == CUP_CENTRAL_EULER) {
+    if (int e = launch_adjoint_advdiff(c, nu, uinf)) return e;
+  } else if (scheme == CUP_UPWIND3_EULER) {
+    // lam_n = lam* + dt L'(u_n)^T lam*
+    if (int e = launch_adjoint_up3(c, CUP_U, CUP_ADJ_U, CUP_ADJ_U, -1, CUP_TMPU, nu, uinf, 1.0)) return e;
+  } else {
+    // midpoint: u_h = u_n + dt/2 L(u_n); mu = dt L'(u_h)^T lam*; lam_n = lam* + mu + dt/2 L'(u_n)^T mu
+    if (int e = launch_up3_stage(c, CUP_U, CUP_U, CUP_TMPU, nu, uinf, 0.5)) return e;
+    if (int e = launch_adjoint_up3(c, CUP_TMPU, CUP_ADJ_U, -1, -1, F_ADJ2, nu, uinf, 1.0)) return e;
+    if (int e = launch_adjoint_up3(c, CUP_U, F_ADJ2, CUP_ADJ_U, F_ADJ2, CUP_TMPU, nu, uinf, 0.5)) return e;
+  }
   for (int q = 0; q < 3; ++q) std::swap(c.f[CUP_TMPU + q], c.f[CUP_ADJ_U + q]);
   return CUP_OK;
 }

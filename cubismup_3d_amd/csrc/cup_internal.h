@@ -100,7 +100,8 @@ struct Ctx {
   double h = 0, maxextent = 0, extent[3] = {0, 0, 0};
   size_t esz = 8;
 
-  void* f[11] = {};       // block-layout fields chi,u,v,w,p,tmpU,tmpV,tmpW; adjoint u,v,w (lazy)
+  void* f[14] = {};       // block-layout fields chi,u,v,w,p,tmpU,tmpV,tmpW; adjoint u,v,w (lazy);
+                          // F_ADJ2..+2: the upwind midpoint adjoint's scratch (lazy)
   int ustar = CUP_TMPU;   // first of the three fields holding the step's u* after advection
   bool plain_rhs = false; // the fused x pass ignores the obstacle RHS (the adjoint's projection)
   void* spec = nullptr;   // packed half spectrum, complex, [z][y][kx], kx < Nx/2
@@ -182,6 +183,10 @@ int launch_advdiff(Ctx& c, double nu, const double uinf[3], int scheme);  // -> 
 int launch_copy_tmp_to_u(Ctx& c);
 int launch_copy_u_to_tmp(Ctx& c);
 int launch_adjoint_advdiff(Ctx& c, double nu, const double uinf[3]);
+constexpr int F_ADJ2 = 11;
+int launch_adjoint_up3(Ctx& c, int s0, int l0, int b10, int b20, int out0, double nu, const double uinf[3],
+                       double mult);
+int launch_up3_stage(Ctx& c, int src0, int base0, int out0, double nu, const double uinf[3], double mult);
 int launch_gradp_inplace(Ctx& c, int field0);
 // The adjoint step with the sphere, in phases (cup_adjoint_step_body orders them).
 enum AdjBodyPhase {

@@ -938,6 +938,115 @@ int launch_adjoint_advdiff(Ctx& c, double nu, const double uinf[3])
   return c.precision == 4 ? adj_advdiff_t<float>(c, nu, uinf) : adj_advdiff_t<double>(c, nu, uinf);
 }
 
+// Transposed linearisation of the upwind operator L(s) = -(s + U).grad_up3 s + nu lap s at s
+// (k_advdiff_up3), applied to lam (no reference counterpart):
+// out = b1 + b2 + mult*dt * L'(s)^T lam, b1 and b2 optional (null: zero), with a = s + U,
+// S^{+/-} f = sum_k c^{+/-}_k f(+k) along an axis, c^+ = (1, -6, 3, 2, 0), c^- = (0, -2, -3, 6, -1):
+// (L'^T lam)_p(y) = -(1/6h) [sum_q lam_q(y) S_p^{sign a_p(y)} s_q (y)
+//                   + sum_d sum_k lam_p(y - k e_d) a_d(y - k e_d) c^{sign a_d(y - k e_d)}_k]
+//                   + nu/h^2 lap lam_p(y).
+// Two labs of halo 2 (s and lam); every term moves along one axis, so faces suffice.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_adj_up3(const T* __restrict__ su, const T* __restrict__ sv,
+                                                const T* __restrict__ sw, const T* __restrict__ lu,
+                                                const T* __restrict__ lv, const T* __restrict__ lw, const T* b1u,
+                                                const T* b1v, const T* b1w, const T* b2u, const T* b2v,
+                                                const T* b2w, T* __restrict__ ou, T* __restrict__ ov,
+                                                T* __restrict__ ow, GridArgs g, const DevScalars* __restrict__ sc,
+                                                T Ux, T Uy, T Uz, double nu, double mult)
+{
+  __shared__ T labs[3][L2C];
+  __shared__ T labl[3][L2C];
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  int bx, by, bz;
+  block_coords(g, b, bx, by, bz);
+  const T* src[3] = {su, sv, sw};
+  const T* lsrc[3] = {lu, lv, lw};
+  gather_g2<T>(src, labs, g, b, bx, by, bz);
+  gather_g2<T>(lsrc, labl, g, b, bx, by, bz);
+  __syncthreads();
+  const double fac = mult * sc->dt;
+  const T fA = (T)(-fac / (6 * g.h));
+  const T fD = (T)(fac * nu / (g.h * g.h));
+  const T Uc[3] = {Ux, Uy, Uz};
+  constexpr int st[3] = {1, L2, L2 * L2};
+  constexpr T cp[5] = {1, -6, 3, 2, 0}, cm[5] = {0, -2, -3, 6, -1};
+  const int c0 = 2 * threadIdx.x, ix0 = c0 & 7, iy = (c0 >> 3) & 7, iz = c0 >> 6;
+  T r[3][2];
+#pragma unroll
+  for (int k2 = 0; k2 < 2; ++k2) {
+    const int l = ((iz + 2) * L2 + iy + 2) * L2 + ix0 + k2 + 2;
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+      const T ap = labs[p][l] + Uc[p];
+      T t1 = 0;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const T* F = labs[q];
+        t1 += labl[q][l] * up3(ap, F[l - 2 * st[p]], F[l - st[p]], F[l], F[l + st[p]], F[l + 2 * st[p]]);
+      }
+      T t2 = 0, lap = 0;
+      const T* Lp = labl[p];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+#pragma unroll
+        for (int k = -2; k <= 2; ++k) {
+          const int x = l - k * st[d];
+          const T a = labs[d][x] + Uc[d];
+          t2 += Lp[x] * a * (a > 0 ? cp[k + 2] : cm[k + 2]);
+        }
+        lap += Lp[l - st[d]] + Lp[l + st[d]] - 2 * Lp[l];
+      }
+      r[p][k2] = fA * (t1 + t2) + fD * lap;
+    }
+  }
+  using V2 = typename Vec2<T>::type;
+  const size_t o = (size_t)b * BS3 + c0;
+  const T* b1[3] = {b1u, b1v, b1w};
+  const T* b2[3] = {b2u, b2v, b2w};
+  T* out[3] = {ou, ov, ow};
+#pragma unroll
+  for (int p = 0; p < 3; ++p) {
+    T v0 = r[p][0], v1 = r[p][1];
+    if (b1[p]) {
+      const V2 a = *reinterpret_cast<const V2*>(b1[p] + o);
+      v0 += a.x;
+      v1 += a.y;
+    }
+    if (b2[p]) {
+      const V2 a = *reinterpret_cast<const V2*>(b2[p] + o);
+      v0 += a.x;
+      v1 += a.y;
+    }
+    *reinterpret_cast<V2*>(out[p] + o) = V2{v0, v1};
+  }
+}
+
+// out0 = b1 + b2 + mult*dt * L'(s)^T lam on field triples (b1, b2 may be -1: zero).
+template <typename T>
+static int adj_up3_t(Ctx& c, int s0, int l0, int b10, int b20, int out0, double nu, const double uinf[3], double mult)
+{
+  auto f = [&](int base, int q) { return base < 0 ? (const T*)nullptr : (const T*)c.f[base + q]; };
+  hipLaunchKernelGGL((k_adj_up3<T>), dim3(c.nb), dim3(NT), 0, c.stream, f(s0, 0), f(s0, 1), f(s0, 2), f(l0, 0),
+                     f(l0, 1), f(l0, 2), f(b10, 0), f(b10, 1), f(b10, 2), f(b20, 0), f(b20, 1), f(b20, 2),
+                     (T*)c.f[out0], (T*)c.f[out0 + 1], (T*)c.f[out0 + 2], c.grid(), c.sc, (T)uinf[0], (T)uinf[1],
+                     (T)uinf[2], nu, mult);
+  return check_hip(c, hipGetLastError(), "k_adj_up3");
+}
+
+int launch_adjoint_up3(Ctx& c, int s0, int l0, int b10, int b20, int out0, double nu, const double uinf[3],
+                       double mult)
+{
+  return c.precision == 4 ? adj_up3_t<float>(c, s0, l0, b10, b20, out0, nu, uinf, mult)
+                          : adj_up3_t<double>(c, s0, l0, b10, b20, out0, nu, uinf, mult);
+}
+
+int launch_up3_stage(Ctx& c, int src0, int base0, int out0, double nu, const double uinf[3], double mult)
+{
+  return c.precision == 4 ? up3_stage<float>(c, src0, base0, out0, nu, uinf, mult)
+                          : up3_stage<double>(c, src0, base0, out0, nu, uinf, mult);
+}
+
 int launch_gradp_plain(Ctx& c)
 {
   return c.precision == 4 ? gradp_t<float>(c, false, nullptr, nullptr, false)

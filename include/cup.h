@@ -207,12 +207,13 @@ int cup_set_step_state(cup_ctx* ctx, int step, double time);
 
 /* Adjoint of one step (no reference counterpart): with the forward state u_n in
  * (CUP_U, CUP_V, CUP_W) and dJ/du_{n+1} in (CUP_ADJ_U, _V, _W), replaces the adjoint
- * fields by dJ/du_n for the step u_{n+1} = P(A(u_n)) of the given dt: central advection
- * with Euler (the only scheme accepted), no forcing, no obstacle (CUP_E_STATE if one
+ * fields by dJ/du_n for the step u_{n+1} = P(A(u_n)) of the given dt and scheme (central
+ * Euler, upwind Euler, upwind midpoint), no forcing, no obstacle (CUP_E_STATE if one
  * exists). P (PressureRHS + PoissonSolverPeriodic + KernelGradP) is symmetric and is
  * applied to the adjoint; A'(u_n)^T is the transposed linearised advection-diffusion
- * stencil. Overwrites p and tmp; the device pointers of tmp and the adjoint fields may
- * change (cup_field_device_ptr again after this call). */
+ * stencil, for the midpoint form applied at the recomputed half step and at u_n. Overwrites
+ * p and tmp; the device pointers of tmp and the adjoint fields may change
+ * (cup_field_device_ptr again after this call). */
 int cup_adjoint_step(cup_ctx* ctx, double dt, double nu, const double uinf[3], int scheme);
 /* The same with the sphere (forced at body_vel, no rotation; chi from cup_chi_from_sphere),
  * no reference counterpart. The step adds the implicit (or explicit) penalization of

@@ -1540,7 +1540,8 @@ static void adj_advect(orc_sim* s, double dt, double nu, const double uinf[3], d
   }
 }
 
-void orc_adjoint_step(orc_sim* s, double dt, double nu, const double uinf[3], double* lam)
+/* lam <- P lam: the forward projection (without the sphere) on the adjoint, P symmetric. */
+static void adj_project(orc_sim* s, double dt, double* lam)
 {
   const size_t nc = (size_t)s->nb * BS * BS * BS;
   double* save = (double*)malloc(4 * nc * sizeof(double));
@@ -1553,9 +1554,112 @@ void orc_adjoint_step(orc_sim* s, double dt, double nu, const double uinf[3], do
   orc_poisson_solve(s);
   orc_grad_p(s, dt);
   s->has_sphere = sphere;
-  for (int q = 0; q < 3; q++) orc_get_field(s, ORC_U + q, lam + q * nc);  /* lam* */
+  for (int q = 0; q < 3; q++) orc_get_field(s, ORC_U + q, lam + q * nc);
   for (int q = 0; q < 4; q++) orc_set_field(s, keep[q], save + q * nc);
   free(save);
+}
+
+/* Block-major index of the periodic cell (gx, gy, gz). */
+static inline size_t bm_index(const orc_sim* s, int gx, int gy, int gz)
+{
+  gx = (gx % s->N[0] + s->N[0]) % s->N[0];
+  gy = (gy % s->N[1] + s->N[1]) % s->N[1];
+  gz = (gz % s->N[2] + s->N[2]) % s->N[2];
+  const int b = gx / BS + s->bpd[0] * (gy / BS + s->bpd[1] * (gz / BS));
+  return (size_t)b * BS * BS * BS + ((gz % BS) * BS + gy % BS) * BS + gx % BS;
+}
+
+/* Transposed linearisation of the upwind operator L (upwind3_update) at phi, applied to lam
+ * (no reference counterpart): with a = phi + U and S^{+/-} f = sum_k c^{+/-}_k f(+k) along an
+ * axis (c^+ = (1, -6, 3, 2, 0), c^- = (0, -2, -3, 6, -1) for k = -2..2),
+ *   out_p(y) = -(1/6h) [ sum_q lam_q(y) S^{sign a_p(y)}_p phi_q (y)
+ *                        + sum_d sum_k lam_p(y - k e_d) a_d(y - k e_d) c^{sign a_d(y - k e_d)}_k ]
+ *              + nu/h^2 lap lam_p(y).
+ * The first sum differentiates the advecting velocity, the second the advected field; the
+ * upwind side is held where it is (a.e. differentiable). Fields block-major, 3 x ncell. */
+static void up3_jt(const orc_sim* s, const double* phi, const double* lam, double nu, const double uinf[3],
+                   double* out)
+{
+  static const double cp[5] = {1, -6, 3, 2, 0}, cm[5] = {0, -2, -3, 6, -1};
+  const size_t nc = (size_t)s->nb * BS * BS * BS;
+  const double h = s->h;
+  #pragma omp parallel for collapse(2) schedule(static)
+  for (int gz = 0; gz < s->N[2]; gz++)
+    for (int gy = 0; gy < s->N[1]; gy++)
+      for (int gx = 0; gx < s->N[0]; gx++) {
+        const int g0[3] = {gx, gy, gz};
+        const size_t y = bm_index(s, gx, gy, gz);
+        for (int p = 0; p < 3; p++) {
+          double t1 = 0, t2 = 0, lap = 0;
+          /* sum_q lam_q(y) S_p phi_q (y), upwind side of a_p(y) */
+          const double ap = phi[p * nc + y] + uinf[p];
+          const double* c = ap > 0 ? cp : cm;
+          for (int q = 0; q < 3; q++) {
+            double S = 0;
+            for (int k = -2; k <= 2; k++) {
+              int o[3] = {gx, gy, gz};
+              o[p] += k;
+              S += c[k + 2] * phi[q * nc + bm_index(s, o[0], o[1], o[2])];
+            }
+            t1 += lam[q * nc + y] * S;
+          }
+          for (int d = 0; d < 3; d++) {
+            for (int k = -2; k <= 2; k++) {
+              int o[3] = {gx, gy, gz};
+              o[d] -= k;
+              const size_t x = bm_index(s, o[0], o[1], o[2]);
+              const double a = phi[d * nc + x] + uinf[d];
+              t2 += lam[p * nc + x] * a * (a > 0 ? cp[k + 2] : cm[k + 2]);
+            }
+            int e[3] = {gx, gy, gz}, f[3] = {gx, gy, gz};
+            e[d] += 1;
+            f[d] -= 1;
+            lap += lam[p * nc + bm_index(s, e[0], e[1], e[2])] + lam[p * nc + bm_index(s, f[0], f[1], f[2])] -
+                   2 * lam[p * nc + y];
+          }
+          (void)g0;
+          out[p * nc + y] = -(t1 + t2) / (6 * h) + nu * lap / (h * h);
+        }
+      }
+}
+
+/* Adjoint of one step of the upwind scheme (no reference counterpart): lam holds
+ * dJ/du_{n+1} on entry, dJ/du_n on exit, with u_n the sim's velocity. lam* = P lam, then
+ * Euler:    lam_n = lam* + dt L'(u_n)^T lam*;
+ * midpoint: u_h = u_n + dt/2 L(u_n), mu = dt L'(u_h)^T lam*, lam_n = lam* + mu + dt/2 L'(u_n)^T mu.
+ * scheme 0 is the central Euler step (orc_adjoint_step). */
+void orc_adjoint_step_scheme(orc_sim* s, double dt, double nu, const double uinf[3], int scheme, double* lam)
+{
+  if (scheme == 0) {
+    orc_adjoint_step(s, dt, nu, uinf, lam);
+    return;
+  }
+  const size_t nc = (size_t)s->nb * BS * BS * BS;
+  adj_project(s, dt, lam);
+  double* phi = (double*)malloc(3 * nc * sizeof(double));
+  double* t = (double*)malloc(3 * nc * sizeof(double));
+  for (int q = 0; q < 3; q++) orc_get_field(s, ORC_U + q, phi + q * nc);
+  if (scheme == 1) {
+    up3_jt(s, phi, lam, nu, uinf, t);
+    for (size_t i = 0; i < 3 * nc; i++) lam[i] += dt * t[i];
+  } else {
+    double* uh = (double*)malloc(3 * nc * sizeof(double));
+    double* mu = (double*)malloc(3 * nc * sizeof(double));
+    upwind3_update(s, 0, 0.5 * dt, nu, uinf, uh);
+    up3_jt(s, uh, lam, nu, uinf, mu);
+    for (size_t i = 0; i < 3 * nc; i++) mu[i] *= dt;
+    up3_jt(s, phi, mu, nu, uinf, t);
+    for (size_t i = 0; i < 3 * nc; i++) lam[i] += mu[i] + 0.5 * dt * t[i];
+    free(uh);
+    free(mu);
+  }
+  free(phi);
+  free(t);
+}
+
+void orc_adjoint_step(orc_sim* s, double dt, double nu, const double uinf[3], double* lam)
+{
+  adj_project(s, dt, lam);  /* lam* */
   adj_advect(s, dt, nu, uinf, lam);
 }
 

@@ -33,6 +33,7 @@ void orc_extent(const orc_sim* s, double out[3]);
 void orc_set_threads(int n);
 
 void orc_set_field(orc_sim* s, int f, const double* src);
+void orc_adjoint_step_scheme(orc_sim* s, double dt, double nu, const double uinf[3], int scheme, double* lam);
 void orc_get_field(const orc_sim* s, int f, double* dst);
 
 /* Periodic lab test (TestBoundaries.cpp:19-75): tmpU = lab(-dx,-dy,-dz).u for a lab

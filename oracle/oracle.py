@@ -53,6 +53,7 @@ def lib():
         "orc_advect_diffuse": (None, [p, d, d, dp]),
         "orc_advect_diffuse_scheme": (None, [p, d, d, dp, i]),
         "orc_adjoint_step": (None, [p, d, d, dp, dp]),
+        "orc_adjoint_step_scheme": (None, [p, d, d, dp, i, dp]),
         "orc_adjoint_step_body": (None, [p, d, d, dp, d, i, dp, d, dp, dp, dp]),
         "orc_pressure_rhs": (None, [p, d]),
         "orc_poisson_solve": (None, [p]),
@@ -218,11 +219,12 @@ class OracleSim:
         """scheme 0: central/Euler (the reference); 1: upwind3/Euler; 2: upwind3/RK2 midpoint."""
         lib().orc_advect_diffuse_scheme(self._h, float(dt), float(nu), _dp(_vec(uinf)), int(scheme))
 
-    def adjoint_step(self, dt, nu, lam, uinf=(0, 0, 0)):
+    def adjoint_step(self, dt, nu, lam, uinf=(0, 0, 0), scheme=0):
         """dJ/du_n from dJ/du_{n+1} = lam (a tuple of three [Z][Y][X] arrays), with the forward
-        state u_n in this sim's velocity; central advection, Euler, no forcing or body."""
+        state u_n in this sim's velocity; central Euler (scheme 0) or upwind Euler/midpoint (1, 2),
+        no forcing or body."""
         buf = np.concatenate([to_blocks(np.asarray(f, dtype=np.float64)) for f in lam])
-        lib().orc_adjoint_step(self._h, float(dt), float(nu), _dp(_vec(uinf)), _dp(buf))
+        lib().orc_adjoint_step_scheme(self._h, float(dt), float(nu), _dp(_vec(uinf)), int(scheme), _dp(buf))
         n = buf.size // 3
         return tuple(from_blocks(buf[q * n:(q + 1) * n], self.shape) for q in range(3))
 

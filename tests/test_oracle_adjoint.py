@@ -42,6 +42,44 @@ def test_adjoint_dot_product_identity(oracle, bpd, U):
         assert np.array_equal(a, b)
 
 
+def fwd_step_scheme(oracle, bpd, u, dt, nu, U, scheme):
+    s = oracle.OracleSim(bpd)
+    s.set_velocity(*u)
+    s.advect_diffuse(dt, nu, U, scheme)
+    s.pressure_rhs(dt)
+    s.poisson_solve()
+    s.grad_p(dt)
+    return s.velocity()
+
+
+@pytest.mark.parametrize("scheme", [1, 2])
+def test_upwind_adjoint_dot_product_identity(oracle, scheme):
+    """The upwind step is polynomial in u (degree 2 for Euler, 4 for the midpoint form) away
+    from the cells where an advecting velocity changes sign, so the five-point central
+    difference with a small step is its exact directional derivative; the adjoint of
+    upwind Euler and midpoint (scheme 1, 2) must satisfy the dot-product identity."""
+    rng = np.random.default_rng(31)
+    bpd, U = (2, 2, 2), (0.3, -0.2, 0.1)
+    s = oracle.OracleSim(bpd)
+    shape = s.shape
+    u = tuple(rng.standard_normal(shape) for _ in range(3))
+    d = tuple(rng.standard_normal(shape) for _ in range(3))
+    lam = tuple(rng.standard_normal(shape) for _ in range(3))
+    dt, nu, eps = 0.3 * s.h, 0.02, 1e-5
+
+    def f(e):
+        return fwd_step_scheme(oracle, bpd, tuple(a + e * b for a, b in zip(u, d)), dt, nu, U, scheme)
+    f1, f2, m1, m2 = f(eps), f(2 * eps), f(-eps), f(-2 * eps)
+    jd = tuple((8 * (a - b) - (c - e)) / (12 * eps) for a, b, c, e in zip(f1, m1, f2, m2))
+    s.set_velocity(*u)
+    lam0 = s.adjoint_step(dt, nu, lam, U, scheme=scheme)
+    lhs, rhs = dot(jd, lam), dot(d, lam0)
+    assert abs(lhs - rhs) < 1e-9 * abs(lhs), (lhs, rhs)
+    # not the central adjoint: the upwind scheme's transpose differs
+    lam_c = s.adjoint_step(dt, nu, lam, U, scheme=0)
+    assert abs(dot(d, lam_c) - rhs) > 1e-4 * abs(rhs)
+
+
 def test_adjoint_of_pure_projection_is_symmetric(oracle):
     """With nu = 0, dt -> tiny advection, the adjoint reduces to P lam + O(dt); check P
     itself: <P a, b> = <a, P b> and P P = P through the oracle's own projection."""
