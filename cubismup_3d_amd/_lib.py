@@ -63,6 +63,7 @@ SIGNATURES = {
     "cup_poisson_periodic": (_i, [_p]),
     "cup_pressure_project": (_i, [_p, _d]),
     "cup_chi_from_sphere": (_i, [_p, _dp, _d, _dp]),
+    "cup_chi_from_hemisphere": (_i, [_p, _dp, _d, _dp]),
     "cup_hit_analysis": (_i, [_p, _d, _dp, _dp, _i]),
     "cup_chi_from_sphere_mollified": (_i, [_p, _dp, _d, _d, _dp]),
     "cup_penalize": (_i, [_p, _d, _d, _i, _dp, _dp, _dp]),
@@ -86,7 +87,7 @@ SIGNATURES = {
     "cup_sim_init": (_i, [_p]),
     "cup_sim_destroy": (_i, [_p]),
     "cup_sim_last_error": (C.c_char_p, [_p]),
-    "cup_sim_add_sphere": (_i, [_p, _d, _dp, _dp, _i, _i, _i, _d, _d]),
+    "cup_sim_add_sphere": (_i, [_p, _d, _dp, _dp, _i, _i, _i, _d, _d, _i]),
     "cup_sim_run": (_i, [_p]),
     "cup_sim_calc_max_timestep": (_i, [_p, _dp]),
     "cup_sim_timestep": (_i, [_p, _d, _ip]),

@@ -474,6 +474,18 @@ int cup_chi_from_sphere(cup_ctx* ctx, const double center[3], double radius, dou
   return CUP_OK;
 }
 
+int cup_chi_from_hemisphere(cup_ctx* ctx, const double center[3], double radius, double com_mass_out[4])
+{
+  CUP_GUARD(ctx);
+  if (!center || !(radius > 0)) return set_error(c, CUP_E_INVALID, "chi_from_hemisphere: radius must be > 0");
+  if (int rc = body_create(c, center, radius, 0.0, true)) return rc;
+  if (com_mass_out) {
+    for (int a = 0; a < 3; ++a) com_mass_out[a] = c.body.com[a];
+    com_mass_out[3] = c.body.mass;
+  }
+  return CUP_OK;
+}
+
 int cup_chi_from_sphere_mollified(cup_ctx* ctx, const double center[3], double radius, double width,
                                   double com_mass_out[4])
 {

@@ -77,6 +77,7 @@ struct Body {
   bool present = false;
   double radius = 0, center[3] = {0, 0, 0};
   double moll = 0;              // half-width in cells of the mollified chi; 0: Towers' chi
+  bool hemi = false;            // the hemisphere (HemiSphereObstacle): the ball's part at x < c_x
   double com[3] = {0, 0, 0}, mass = 0;
   int nob = 0;                  // obstacle blocks
   int* ob_ids = nullptr;        // device: block id of each obstacle block
@@ -231,7 +232,7 @@ int forcing_plain(Ctx& c, double dt, double sim_dt, double nu, double tke_target
 int launch_forcing_reduce(Ctx& c, double tke_target, double inj_rate);
 int launch_forcing_rows(Ctx& c);
 // cup_obstacle.hip
-int body_create(Ctx& c, const double center[3], double radius, double moll);
+int body_create(Ctx& c, const double center[3], double radius, double moll, bool hemi = false);
 void body_free(Ctx& c);
 int launch_penalize(Ctx& c, double lambda, int implicit, const double vel[3], const double omega[3]);
 int launch_penalize_step(Ctx& c, double lambda, int implicit, const double vel[3], const double omega[3],

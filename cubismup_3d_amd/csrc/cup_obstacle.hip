@@ -44,9 +44,10 @@ __device__ __forceinline__ void breduce(double (&v)[NV], double* out)
   __syncthreads();
 }
 
-// SDF of the sphere on obstacle blocks: R - |x - c| (Sphere.cpp:44-48).
+// SDF of the sphere on obstacle blocks: R - |x - c| (Sphere.cpp:44-48); of the hemisphere
+// min(-(x - c_x), R - |x - c|) (Sphere.cpp:78-83).
 __global__ void k_sphere_sdf(const int* __restrict__ ob_ids, double* __restrict__ sdf, GridArgs g, double cx,
-                             double cy, double cz, double R)
+                             double cy, double cz, double R, int hemi)
 {
   const int s = blockIdx.x, b = ob_ids[s];
   int bx, by, bz;
@@ -55,7 +56,8 @@ __global__ void k_sphere_sdf(const int* __restrict__ ob_ids, double* __restrict_
     const int ix = c & 7, iy = (c >> 3) & 7, iz = c >> 6;
     const double dx = (bx * BS + ix + 0.5) * g.h - cx, dy = (by * BS + iy + 0.5) * g.h - cy,
                  dz = (bz * BS + iz + 0.5) * g.h - cz;
-    sdf[(size_t)s * BS3 + c] = R - sqrt(dx * dx + dy * dy + dz * dz);
+    const double ball = R - sqrt(dx * dx + dy * dy + dz * dz);
+    sdf[(size_t)s * BS3 + c] = hemi && -dx < ball ? -dx : ball;
   }
 }
 
@@ -188,7 +190,7 @@ static int body_chi_t(Ctx& c)
   if (gsz > 65536) gsz = 65536;
   hipLaunchKernelGGL((k_fill_t<T>), dim3((unsigned)gsz), dim3(NT), 0, c.stream, (T*)c.f[CUP_CHI], n, (T)0);
   hipLaunchKernelGGL(k_sphere_sdf, dim3(B.nob), dim3(NT), 0, c.stream, B.ob_ids, B.ob_sdf, c.grid(), B.center[0],
-                     B.center[1], B.center[2], B.radius);
+                     B.center[1], B.center[2], B.radius, B.hemi ? 1 : 0);
   hipLaunchKernelGGL((k_sphere_chi<T>), dim3(B.nob), dim3(NT), 0, c.stream, B.ob_ids, B.ob_slot, B.ob_sdf, B.ob_chi,
                      B.ob_surf, (T*)c.f[CUP_CHI], c.grid(), c.partials, B.moll * c.h);
   return check_hip(c, hipGetLastError(), "k_sphere_chi");
@@ -197,7 +199,7 @@ static int body_chi_t(Ctx& c)
 // CreateObstacles for one sphere (ObstaclesCreate.cpp:444-540); moll > 0 selects the
 // mollified chi of that half-width in cells. Called again for a body that moved: the
 // per-block buffers are kept when the set of obstacle blocks is unchanged.
-int body_create(Ctx& c, const double center[3], double radius, double moll)
+int body_create(Ctx& c, const double center[3], double radius, double moll, bool hemi)
 {
   hipStreamSynchronize(c.stream);
   Body& B = c.body;
@@ -210,7 +212,9 @@ int body_create(Ctx& c, const double center[3], double radius, double moll)
     bool touching = true;
     for (int a = 0; a < 3; ++a) {
       const double mn = (bc[a] * BS + 0.5) * c.h, mx = (bc[a] * BS + BS - 1 + 0.5) * c.h;
-      const double lo = std::fmax(mn, center[a] - (radius + safety)), hi = std::fmin(mx, center[a] + (radius + safety));
+      // the hemisphere's box ends at c_x + safety in x (Sphere.cpp:59-63)
+      const double top = hemi && a == 0 ? center[a] + safety : center[a] + (radius + safety);
+      const double lo = std::fmax(mn, center[a] - (radius + safety)), hi = std::fmin(mx, top);
       if (!(hi - lo > 0)) touching = false;
     }
     if (touching) {
@@ -222,10 +226,11 @@ int body_create(Ctx& c, const double center[3], double radius, double moll)
     body_free(c);
     return set_error(c, CUP_E_INVALID, "chi_from_sphere: the sphere touches no block");
   }
-  const bool reuse = B.present && B.radius == radius && B.moll == moll && B.ids_host == ids;
+  const bool reuse = B.present && B.radius == radius && B.moll == moll && B.hemi == hemi && B.ids_host == ids;
   if (!reuse) body_free(c);
   B.radius = radius;
   B.moll = moll;
+  B.hemi = hemi;
   for (int a = 0; a < 3; ++a) B.center[a] = center[a];
   B.nob = (int)ids.size();
   const size_t nc = (size_t)B.nob * BS3;

@@ -231,7 +231,7 @@ SimulationData::SimulationData(int argc, const char* const* argv) : SimulationDa
       s->umax = parser("-xvel").asDouble(0.0);
       s->tmax = parser("-T").asDouble(1.0);
     }
-    if (parser("-hemisphere").asBool(false)) throw std::invalid_argument("the hemisphere is not supported");
+    s->bHemi = parser("-hemisphere").asBool(false);
     obstacles.push_back(s);
   }
 }
@@ -509,7 +509,9 @@ void CreateObstacles::build()
   if (sim.obstacles.size() > 1) throw std::invalid_argument("one obstacle at most");
   auto& s = *sim.obstacles[0];
   double cm[4];
-  check(cup_chi_from_sphere(sim.ctx, s.position.data(), s.radius, cm), "CreateObstacles");
+  check(s.bHemi ? cup_chi_from_hemisphere(sim.ctx, s.position.data(), s.radius, cm)
+                : cup_chi_from_sphere(sim.ctx, s.position.data(), s.radius, cm),
+        "CreateObstacles");
   s.centerOfMass = {cm[0], cm[1], cm[2]};
   s.mass = cm[3];
   created = true;
@@ -1106,7 +1108,7 @@ int cup_sim_create(int argc, const char* const* argv, cup_sim** out)
 }
 
 int cup_sim_add_sphere(cup_sim* h, double radius, const double position[3], const double enforced_vel[3],
-                       int forced_mask, int fix_mask, int accel_decel, double umax, double tmax)
+                       int forced_mask, int fix_mask, int accel_decel, double umax, double tmax, int hemi)
 {
   return guarded(h, [&] {
     if (!(radius > 0)) throw std::invalid_argument("radius must be > 0");
@@ -1123,6 +1125,7 @@ int cup_sim_add_sphere(cup_sim* h, double radius, const double position[3], cons
     s->accel_decel = accel_decel != 0;
     s->umax = umax;
     s->tmax = tmax;
+    s->bHemi = hemi != 0;
     if (h->S) h->S->addObstacle(s);
     else if (h->pending) h->pending->obstacles.push_back(s);
     else throw std::invalid_argument("no simulation");

@@ -104,7 +104,7 @@ struct Sphere {
   std::array<double, 3> transVel_computed = {0, 0, 0}, angVel_computed = {0, 0, 0};
   int obstacleID = 0;
   // SphereArguments (Sphere.h:17-29): the ramped x velocity, up to umax at tmax, 0 at 2 tmax
-  bool accel_decel = false;
+  bool accel_decel = false, bHemi = false;
   double umax = 0, tmax = 1;
 
   // The Obstacle constructor's velocity set-up (Obstacle.cpp:119-137): forced axes take the

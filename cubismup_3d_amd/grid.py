@@ -231,11 +231,16 @@ class Grid:
                                              float(nu), float(tke_target), float(injection_rate), L.dptr(st)))
         return dict(tke=st[0], eps=st[1], tke_filtered=st[2], l_integral=st[3], fac=st[4])
 
-    def chi_from_sphere(self, center, radius, mollify=0.0):
+    def chi_from_sphere(self, center, radius, mollify=0.0, hemi=False):
         """The sphere's chi by Towers' formula, or with mollify (a half-width in cells, in
-        (0, 2]) by the mollified Heaviside of its SDF, which is C1 in the centre."""
+        (0, 2]) by the mollified Heaviside of its SDF, which is C1 in the centre. hemi: the
+        hemisphere (the ball's part below the centre in x)."""
         out = np.zeros(4)
-        if mollify:
+        if hemi:
+            if mollify:
+                raise ValueError("the hemisphere takes Towers' chi")
+            self._c(L.lib().cup_chi_from_hemisphere(self._h, L.vec3(center), float(radius), L.dptr(out)))
+        elif mollify:
             self._c(L.lib().cup_chi_from_sphere_mollified(self._h, L.vec3(center), float(radius), float(mollify),
                                                           L.dptr(out)))
         else:

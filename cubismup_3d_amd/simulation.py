@@ -42,10 +42,11 @@ class Sphere:
     ``radius`` or ``length``; ``position``; ``enforcedVelocity``, the velocity of the axes
     forced in the simulation frame; ``bForcedInSimFrame`` and ``bFixFrameOfRef``, a bool or one
     per axis (default: free, frame at rest); ``umax``, ``tmax``, ``accel_decel``, the ramped x
-    velocity of Sphere::computeVelocities (Sphere.cpp:147-159). ``xvel`` is the command-line
-    shorthand: every axis forced, the frame fixed to the body, enforcedVelocity = (-xvel, 0, 0)
-    (Obstacle.cpp:64). ``bHemi``, ``bFixToPlanar``, a non-identity ``quaternion`` and
-    ``bComputeForces=False`` are not supported and raise ValueError, as unknown keywords do."""
+    velocity of Sphere::computeVelocities (Sphere.cpp:147-159); ``bHemi``, the hemisphere
+    (Sphere.cpp:53-84). ``xvel`` is the command-line shorthand: every axis forced, the frame
+    fixed to the body, enforcedVelocity = (-xvel, 0, 0) (Obstacle.cpp:64). ``bFixToPlanar``, a
+    non-identity ``quaternion`` and ``bComputeForces=False`` are not supported and raise
+    ValueError, as unknown keywords do."""
 
     def __init__(self, radius=0.0, umax=0.0, tmax=1.0, accel_decel=False, bHemi=False, **kwargs):
         length = float(kwargs.pop("length", 0.0))
@@ -75,9 +76,9 @@ class Sphere:
         if len(self.position) != 3 or len(self.trans_vel) != 3 or len(self.forced) != 3 or len(self.fix_frame) != 3:
             raise ValueError("position, enforcedVelocity and the axis flags take three values")
         q = [float(x) for x in kwargs.pop("quaternion", (1.0, 0.0, 0.0, 0.0))]
-        if bHemi or kwargs.pop("bFixToPlanar", False) or q != [1.0, 0.0, 0.0, 0.0] \
-                or not kwargs.pop("bComputeForces", True):
-            raise ValueError("bHemi, bFixToPlanar, a rotated quaternion and bComputeForces=False are not supported")
+        if kwargs.pop("bFixToPlanar", False) or q != [1.0, 0.0, 0.0, 0.0] or not kwargs.pop("bComputeForces", True):
+            raise ValueError("bFixToPlanar, a rotated quaternion and bComputeForces=False are not supported")
+        self.bHemi = bool(bHemi)
         if kwargs:
             raise ValueError(f"unknown sphere arguments: {sorted(kwargs)}")
         self.umax, self.tmax, self.accel_decel = float(umax), float(tmax), bool(accel_decel)
@@ -162,7 +163,7 @@ class Simulation:
             raise ValueError(str(s))
         fm, xm = s.masks()
         L.check(L.lib().cup_sim_add_sphere(self._h, s.radius, L.vec3(s.position), L.vec3(s.trans_vel), fm, xm,
-                                           int(s.accel_decel), s.umax, s.tmax), sim=self._h)
+                                           int(s.accel_decel), s.umax, s.tmax, int(s.bHemi)), sim=self._h)
 
     def run(self):
         L.check(L.lib().cup_sim_run(self._h), sim=self._h)

@@ -122,6 +122,10 @@ int cup_pressure_project(cup_ctx* ctx, double dt);
  * (:230-256). A body that moved is built again; its buffers are kept while it covers the
  * same blocks. com_mass_out = CoM[3], mass (may be NULL). */
 int cup_chi_from_sphere(cup_ctx* ctx, const double center[3], double radius, double com_mass_out[4]);
+/* The hemisphere (Sphere with bHemi, HemiSphereObstacle::FillBlocks, Sphere.cpp:53-84): the
+ * ball's part with x < center[0], SDF min(center[0] - x, R - |x - c|); otherwise as
+ * cup_chi_from_sphere. */
+int cup_chi_from_hemisphere(cup_ctx* ctx, const double center[3], double radius, double com_mass_out[4]);
 /* The same sphere with a mollified chi, no reference counterpart (it replaces Towers' formula
  * of ObstaclesCreate.cpp:51-137 so that chi is C1 in the centre, for cup_adjoint_step_body's
  * centre gradient): chi = H(R - |x - c|) with H(d) = 0 below -e, 1 above e,
@@ -261,9 +265,9 @@ const char* cup_sim_last_error(const cup_sim* sim);
 /* Simulation_addObstacle with a Sphere (bindings/Obstacles.cpp:12-128): forced_mask and
  * fix_mask hold bForcedInSimFrame and bFixFrameOfRef per axis (bit d for axis d);
  * enforced_vel is the velocity of the forced axes; accel_decel, umax, tmax select the
- * ramped x velocity of Sphere::computeVelocities (Sphere.cpp:147-159). */
+ * ramped x velocity of Sphere::computeVelocities (Sphere.cpp:147-159); hemi the hemisphere. */
 int cup_sim_add_sphere(cup_sim* sim, double radius, const double position[3], const double enforced_vel[3],
-                       int forced_mask, int fix_mask, int accel_decel, double umax, double tmax);
+                       int forced_mask, int fix_mask, int accel_decel, double umax, double tmax, int hemi);
 int cup_sim_run(cup_sim* sim);                                   /* Simulation::run (Simulation.cpp:582-591) */
 int cup_sim_calc_max_timestep(cup_sim* sim, double* dt);         /* Simulation.cpp:403-424 */
 int cup_sim_timestep(cup_sim* sim, double dt, int* finished);    /* Simulation.cpp:593-623 */

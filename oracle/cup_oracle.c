@@ -69,6 +69,7 @@ struct orc_sim {
   int has_sphere;
   double radius, spos[3];
   double moll; /* half-width in cells of the mollified chi, 0 for Towers' chi */
+  int hemi;    /* the hemisphere of HemiSphereObstacle (flat face at x = c_x, body at x < c_x) */
   OBlock** ob;
   double CoM[3], mass;
 };
@@ -1135,6 +1136,16 @@ static void sphere_build(orc_sim* s, double radius, const double pos[3]);
 void orc_sphere_create(orc_sim* s, double radius, const double pos[3])
 {
   s->moll = 0;
+  s->hemi = 0;
+  sphere_build(s, radius, pos);
+}
+
+/* HemiSphereObstacle::FillBlocks (Sphere.cpp:53-84): SDF min(-(x - c_x), R - |x - c|) and the
+ * block box [c_x - R - safety, c_x + safety] in x; chi by Towers' formula. */
+void orc_hemisphere_create(orc_sim* s, double radius, const double pos[3])
+{
+  s->moll = 0;
+  s->hemi = 1;
   sphere_build(s, radius, pos);
 }
 
@@ -1144,6 +1155,7 @@ void orc_sphere_create(orc_sim* s, double radius, const double pos[3])
 void orc_sphere_create_mollified(orc_sim* s, double radius, const double pos[3], double width)
 {
   s->moll = width;
+  s->hemi = 0;
   sphere_build(s, radius, pos);
 }
 
@@ -1160,6 +1172,7 @@ static void sphere_build(orc_sim* s, double radius, const double pos[3])
     box[d][0] = pos[d] - (radius + safety);
     box[d][1] = pos[d] + (radius + safety);
   }
+  if (s->hemi) box[0][1] = pos[0] + safety;
   #pragma omp parallel for schedule(static)
   for (int b = 0; b < s->nb; b++) {
     FE* e = &s->blk[b].d[0][0][0];
@@ -1187,7 +1200,8 @@ static void sphere_build(orc_sim* s, double radius, const double pos[3])
           double p[3];
           cell_pos(s, bc, ix, iy, iz, p);
           const double dx = p[0] - pos[0], dy = p[1] - pos[1], dz = p[2] - pos[2];
-          const double dist = radius - sqrt(dx * dx + dy * dy + dz * dz);
+          const double ball = radius - sqrt(dx * dx + dy * dy + dz * dz);
+          const double dist = s->hemi && -dx < ball ? -dx : ball;
           o->sdf[iz][iy][ix] = dist;
           FE* e = &s->blk[b].d[iz][iy][ix];
           e->tmpU = dist > e->tmpU ? dist : e->tmpU;

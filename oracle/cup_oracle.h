@@ -33,6 +33,7 @@ void orc_extent(const orc_sim* s, double out[3]);
 void orc_set_threads(int n);
 
 void orc_set_field(orc_sim* s, int f, const double* src);
+void orc_hemisphere_create(orc_sim* s, double radius, const double pos[3]);
 void orc_adjoint_step_scheme(orc_sim* s, double dt, double nu, const double uinf[3], int scheme, double* lam);
 void orc_get_field(const orc_sim* s, int f, double* dst);
 

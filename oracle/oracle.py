@@ -71,6 +71,7 @@ def lib():
         "orc_penalize": (None, [p, d, d, i, dp, dp, dp]),
         "orc_surface_forces": (None, [p, d, dp, dp, dp]),
         "orc_obstacle_momenta": (None, [p, d, d, i, dp]),
+        "orc_hemisphere_create": (None, [p, d, dp]),
         "orc_fft3d": (None, [i, i, i, dp, dp, i]),
         "orc_philox": (None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     }
@@ -287,10 +288,14 @@ class OracleSim:
     def ic_hit(self, k0=4.0, u_rms=1.0, seed=20260101):
         lib().orc_ic_hit(self._h, float(k0), float(u_rms), int(seed))
 
-    def sphere_create(self, radius, pos, mollify=0.0):
+    def sphere_create(self, radius, pos, mollify=0.0, hemi=False):
         """The sphere's chi by Towers' formula, or (mollify = half-width in cells, in (0, 2])
-        by the mollified Heaviside of the SDF."""
-        if mollify:
+        by the mollified Heaviside of the SDF; hemi: the hemisphere (x below the centre)."""
+        if hemi:
+            if mollify:
+                raise ValueError("the hemisphere takes Towers' chi")
+            lib().orc_hemisphere_create(self._h, float(radius), _dp(_vec(pos)))
+        elif mollify:
             if not 0 < mollify <= 2:
                 raise ValueError("mollify: the half-width must lie in (0, 2] cells")
             lib().orc_sphere_create_mollified(self._h, float(radius), _dp(_vec(pos)), float(mollify))

@@ -116,8 +116,9 @@ def test_sphere_arguments(cup):
         cup.Sphere(radius=0.1, length=0.3)
     with pytest.raises(ValueError):
         cup.Sphere(position=[0.3, 0.4, 0.5])
+    assert cup.Sphere(radius=0.1, bHemi=True).bHemi
     with pytest.raises(ValueError):
-        cup.Sphere(radius=0.1, bHemi=True)
+        cup.Sphere(radius=0.1, bFixToPlanar=True)
     with pytest.raises(ValueError):
         cup.Sphere(radius=0.1, no_such_argument=1)
 

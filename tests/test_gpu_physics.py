@@ -90,14 +90,14 @@ def test_diagnostics_match_oracle(cup, oracle):
     np.testing.assert_allclose(dg, do, rtol=1e-11, atol=1e-11 * np.abs(do).max())
 
 
-@pytest.fixture
-def sphere_pair(cup, oracle):
+@pytest.fixture(params=[False, True], ids=["sphere", "hemisphere"])
+def sphere_pair(cup, oracle, request):
     bpd = (8, 4, 4)
     G = cup.Grid(bpd, precision=8, maxextent=2.0)
     O = oracle.OracleSim(bpd, maxextent=2.0)
     c, R = (0.61, 0.52, 0.47), 0.2
-    mg = G.chi_from_sphere(c, R)
-    O.sphere_create(R, c)
+    mg = G.chi_from_sphere(c, R, hemi=request.param)
+    O.sphere_create(R, c, hemi=request.param)
     return G, O, mg
 
 

@@ -42,22 +42,22 @@ def test_run_taylor_green_matches_oracle(cup, oracle, decomp):
     assert rel(S.field("p"), O.get("p")) < 1e-10
 
 
-@pytest.mark.parametrize("decomp", ["plain", "fused"])
-def test_run_sphere_matches_oracle(cup, oracle, decomp):
+@pytest.mark.parametrize("decomp,hemi", [("plain", False), ("fused", False), ("plain", True)])
+def test_run_sphere_matches_oracle(cup, oracle, decomp, hemi):
     """A sphere at rest in a uniform inflow (the frame follows the body, -xvel 1), implicit
     penalization, CFL-driven steps: the pipeline's velocity, penalization force and surface
     QoIs against the oracle's step."""
-    print(f"Simulation.run sphere decomp={decomp}", flush=True)
+    print(f"Simulation.run sphere decomp={decomp} hemisphere={hemi}", flush=True)
     nu, CFL, n, R = 1e-3, 0.3, 4, 0.15
     argv = ["-bpdx", "8", "-bpdy", "4", "-bpdz", "4", "-extentx", "2.0", "-CFL", str(CFL), "-nu", str(nu),
             "-nsteps", str(n), "-rampup", "0", "-sphereL", str(2 * R), "-xpos", "0.5", "-ypos", "0.5", "-zpos",
             "0.5", "-xvel", "1.0", "-bForcedInSimFrame", "1", "-bFixFrameOfRef", "1", "-lambda", "1e6",
-            "-implicitPenalization", "1", "-decomposition", decomp] + PERIODIC
+            "-implicitPenalization", "1", "-decomposition", decomp, "-hemisphere", str(int(hemi))] + PERIODIC
     SD = cup.SimulationData(argv=argv)
     S = cup.Simulation(SD)
     S.run()
     O = oracle.OracleSim((8, 4, 4), maxextent=2.0)
-    O.sphere_create(R, (0.5, 0.5, 0.5))
+    O.sphere_create(R, (0.5, 0.5, 0.5), hemi=hemi)
     U = (1.0, 0.0, 0.0)
     sph = dict(lambda_=1e6, implicit=True, trans_vel=(-1.0, 0.0, 0.0))
     # the first dt sees the frame velocity of the flags (0): CreateObstacles sets U after it

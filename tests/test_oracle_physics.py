@@ -161,6 +161,37 @@ def test_sphere_volume(oracle):
     assert info["npoints"] > 0
 
 
+def test_hemisphere_volume_centroid_and_flat_face(oracle):
+    """The hemisphere (HemiSphereObstacle, Sphere.cpp:53-84): sum chi h^3 converges to
+    2/3 pi R^3 and the centroid to 3R/8 behind the flat face at second order (the rim, where
+    the SDF min(-dx, R - r) has a kink, costs O(h^2): -3.0e-3 and -6.7e-4 of the volume at
+    R/h = 12.8 and 25.6); along the axis through the centre the flat face's chi is Towers'
+    planar closed form clip(1/2 - (x - c_x)/(2h))."""
+    R = 0.2
+    errs = []
+    for n in (8, 16):
+        s = oracle.OracleSim((n, n, n), maxextent=1.0)
+        c = np.array([0.5 + 0.3 / (8 * n), 0.5, 0.5])
+        s.sphere_create(R, c, hemi=True)
+        info = s.sphere_info()
+        errs.append((info["mass"] / (2 / 3 * np.pi * R ** 3) - 1, info["com"][0] - (c[0] - 3 * R / 8)))
+        np.testing.assert_allclose(info["com"][1:], [0.5, 0.5], atol=1e-12)
+    assert abs(errs[1][0]) < 1e-3 and abs(errs[0][0]) > 3 * abs(errs[1][0])
+    assert abs(errs[1][1]) < 2e-5 and abs(errs[0][1]) > 3 * abs(errs[1][1])
+    s = oracle.OracleSim((8, 8, 8), maxextent=1.0)
+    c = np.array([0.5 + 0.3 / 64, 0.5, 0.5])
+    s.sphere_create(R, c, hemi=True)
+    h = s.h
+    chi = s.get("chi")
+    x, _, _ = cell_centres(s.shape, h)
+    row, xr = chi[32, 32, :], x[32, 32, :]   # y, z at 32.5 h: near the axis
+    dx = xr - c[0]
+    band = np.abs(dx) < 1.9 * h
+    assert band.sum() >= 3
+    np.testing.assert_allclose(row[band], np.clip(0.5 - dx[band] / (2 * h), 0.0, 1.0), atol=1e-12)
+    assert (row[dx > 2 * h] == 0).all() and (row[(dx < -2 * h) & (dx > -R + 3 * h)] == 1).all()
+
+
 def test_penalization_closed_form(oracle):
     s = oracle.OracleSim((4, 4, 4), maxextent=1.0)
     s.sphere_create(0.2, (0.5, 0.5, 0.5))
