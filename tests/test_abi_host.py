@@ -133,3 +133,13 @@ def test_buffered_logger_flushes_like_the_reference(cup, tmp_path):
     r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, r.stderr
     assert r.stdout.strip() == "ok"
+
+
+def test_command_line_program_rejects_bad_flags(cup):
+    """cup_simulation (source/main.cpp) ends with status 1 and the reason on stderr where the
+    reference aborts; here the box is not periodic (the -BC_x default is dirichlet), which is
+    found before any device call."""
+    exe = os.path.join(os.path.dirname(cup.__file__), "cup_simulation")
+    r = subprocess.run([exe, "-bpdx", "2", "-bpdy", "2", "-bpdz", "2", "-nu", "0.01"], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 1 and "periodic" in r.stderr

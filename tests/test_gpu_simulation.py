@@ -188,3 +188,32 @@ def test_body_log_files(cup, tmp_path):
     ob = S.obstacle(0)
     np.testing.assert_allclose(fd[-1, 3:6], ob["force"], rtol=1e-6, atol=1e-9)
     assert (fv[:, hv.index("vel_x")] == -1.0).all() and (fv[:, hv.index("CMx")] == 0.5).all()
+
+
+def test_command_line_program(cup, oracle, tmp_path):
+    """cup_simulation (the reference's source/main.cpp): a Taylor-Green run from the flags
+    writes diagnostics.dat every step; its kinetic energy column against the oracle's steps
+    with the same CFL timesteps (the file's 6 significant digits)."""
+    import subprocess
+    print("cup_simulation", flush=True)
+    exe = os.path.join(os.path.dirname(cup.__file__), "cup_simulation")
+    flags = ["-bpdx", "4", "-bpdy", "4", "-bpdz", "4", "-extentx", repr(2 * math.pi), "-BC_x", "periodic", "-BC_y",
+             "periodic", "-BC_z", "periodic", "-initCond", "taylorGreen", "-uMax_forced", "-1", "-nu", "0.01",
+             "-CFL", "0.3", "-rampup", "0", "-nsteps", "5", "-freqDiagnostics", "1", "-muteAll", "0",
+             "-serialization", str(tmp_path)]
+    r = subprocess.run([exe] + flags, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    with open(tmp_path / "diagnostics.dat") as f:
+        rows = [ln.split() for ln in f.read().splitlines()]
+    assert rows[0][:2] == ["step_id", "time"] and rows[0][-2:] == ["kineticEn", "enstrophy"]
+    data = np.array([[float(x) for x in r_] for r_ in rows[1:]])
+    assert data.shape == (5, 22)
+    O = oracle.OracleSim((4, 4, 4))
+    O.ic_taylor_green(-1.0)
+    t = 0.0
+    for step in range(5):  # row k: ComputeDissipation inside step k, after its update, at time t_k
+        dt, _ = O.calc_max_timestep(0.3, 0.01, (0, 0, 0), step, 0)
+        O.timestep(dt, 0.01)
+        assert data[step, 0] == step and data[step, 1] == pytest.approx(t, rel=1e-5, abs=1e-12)
+        assert data[step, 2 + 18] == pytest.approx(O.dissipation(0.01)[18], rel=1e-5)
+        t += dt
