@@ -69,6 +69,7 @@ SIGNATURES = {
     "cup_penalize": (_i, [_p, _d, _d, _i, _dp, _dp, _dp]),
     "cup_surface_forces": (_i, [_p, _d, _dp, _dp, _dp]),
     "cup_obstacle_momenta": (_i, [_p, _d, _d, _i, _dp]),
+    "cup_add_uniform_velocity": (_i, [_p, _dp]),
     "cup_diagnostics": (_i, [_p, _d, _dp]),
     "cup_ic_taylor_green": (_i, [_p, _d]),
     "cup_ic_hit": (_i, [_p, _d, _d, _u64]),

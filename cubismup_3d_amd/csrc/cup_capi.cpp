@@ -356,6 +356,13 @@ int cup_advect_diffuse(cup_ctx* ctx, double dt, double nu, const double uinf[3],
   return CUP_OK;
 }
 
+int cup_add_uniform_velocity(cup_ctx* ctx, const double du[3])
+{
+  CUP_GUARD(ctx);
+  if (!du) return set_error(c, CUP_E_INVALID, "add_uniform_velocity: null pointer");
+  return launch_add_uniform(c, du);
+}
+
 int cup_pressure_rhs(cup_ctx* ctx, double dt)
 {
   CUP_GUARD(ctx);

@@ -239,6 +239,7 @@ int launch_penalize_step(Ctx& c, double lambda, int implicit, const double vel[3
                          const double uinf[3]);  // also refreshes the fused step's max/energy partials
 int launch_surface_forces(Ctx& c, double nu, const double vel[3], const double omega[3]);
 int launch_obstacle_momenta(Ctx& c, double dt, double lambda, int implicit);
+int launch_add_uniform(Ctx& c, const double du[3]);
 int launch_rhs_obstacle(Ctx& c, void* rhs_blocks);
 
 // Stages of the fused step, for the measurement hook.

@@ -603,6 +603,32 @@ static dim3 ew_grid(size_t n)
   return dim3((unsigned)g);
 }
 
+template <typename T>
+__global__ void k_add_const(T* __restrict__ a, size_t n, double value)
+{
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    a[i] = (T)((double)a[i] + value);
+}
+
+// KernelExternalForcing (ExternalForcing.cpp:17-31): du added to every cell of u, and of u* when
+// the advection left u* apart from u.
+int launch_add_uniform(Ctx& c, const double du[3])
+{
+  for (int q = 0; q < 3; ++q) {
+    if (du[q] == 0) continue;
+    const int fields[2] = {CUP_U + q, c.ustar + q};
+    for (int k = 0; k < (c.ustar == CUP_U ? 1 : 2); ++k) {
+      if (c.precision == 4)
+        hipLaunchKernelGGL((k_add_const<float>), ew_grid(c.ncell), dim3(NT), 0, c.stream, (float*)c.f[fields[k]],
+                           c.ncell, du[q]);
+      else
+        hipLaunchKernelGGL((k_add_const<double>), ew_grid(c.ncell), dim3(NT), 0, c.stream, (double*)c.f[fields[k]],
+                           c.ncell, du[q]);
+    }
+  }
+  return check_hip(c, hipGetLastError(), "k_add_const");
+}
+
 // u -> tmp (after the midpoint scheme, which leaves u* in u).
 int launch_copy_u_to_tmp(Ctx& c)
 {

@@ -120,6 +120,7 @@ SimulationData& SimulationData::operator=(const SimulationData& o)
   uinf = o.uinf; nu = o.nu; CFL = o.CFL; lambda = o.lambda; DLM = o.DLM;
   initCond = o.initCond; spectralIC = o.spectralIC; k0 = o.k0; tke0 = o.tke0; icSeed = o.icSeed; icUrms = o.icUrms;
   uMax_forced = o.uMax_forced; uMax_measured = o.uMax_measured; spectralForcing = o.spectralForcing;
+  bChannelFixedMassFlux = o.bChannelFixedMassFlux;
   turbKinEn_target = o.turbKinEn_target; enInjectionRate = o.enInjectionRate;
   freqDiagnostics = o.freqDiagnostics; rampup = o.rampup; verbose = o.verbose; muteAll = o.muteAll;
   saveFreq = o.saveFreq; saveTime = o.saveTime; nextSaveTime = o.nextSaveTime; bDump = o.bDump;
@@ -185,6 +186,7 @@ SimulationData::SimulationData(int argc, const char* const* argv) : SimulationDa
     throw std::invalid_argument("-iterativePenalization is not supported");
   bImplicitPenalization = parser("-implicitPenalization").asBool(false);
   uMax_forced = parser("-uMax_forced").asDouble(0.0);
+  bChannelFixedMassFlux = parser("-channelFixedMassFlux").asBool(false);
   lambda = parser("-lambda").asDouble(1e6);
   DLM = parser("-use-dlm").asDouble(0);
   CFL = parser("-CFL").asDouble(.1);
@@ -255,6 +257,8 @@ void SimulationData::_preprocessArguments()
   if (!(nu >= 0)) throw std::invalid_argument("nu must be >= 0");
   if (!(CFL > 0)) throw std::invalid_argument("CFL must be > 0");
   if (analysis == "channel") throw std::invalid_argument("channel analysis needs walls; this build has periodic boxes");
+  if (bChannelFixedMassFlux)
+    throw std::invalid_argument("the fixed mass flux of channel flows needs walls; this build has periodic boxes");
   if (analysis == "HIT" && (freqAnalysis > 0 || timeAnalysis > 0) && !(nu > 0))
     throw std::invalid_argument("HIT analysis needs nu > 0");
   if (BCx_flag != periodic || BCy_flag != periodic || BCz_flag != periodic)
@@ -515,6 +519,14 @@ void CreateObstacles::build()
   s.centerOfMass = {cm[0], cm[1], cm[2]};
   s.mass = cm[3];
   created = true;
+}
+
+void ExternalForcing::operator()(double dt)
+{
+  const int dir = sim.BCy_flag == wall ? 1 : 2;
+  const double H = sim.extent[dir];
+  const double du[3] = {8 * sim.uMax_forced * sim.nu / H / H * dt, 0, 0};
+  check(cup_add_uniform_velocity(sim.ctx, du), "ExternalForcing");
 }
 
 void AdvectionDiffusion::operator()(double dt)
@@ -919,6 +931,9 @@ void Simulation::setupOperators()
   auto* create = new CreateObstacles(sim);
   sim.pipeline.emplace_back(create);
   sim.pipeline.emplace_back(new AdvectionDiffusion(sim));
+  // Simulation.cpp:336-339: the uniform forcing unless the TGV (which also reads uMax_forced)
+  if (sim.uMax_forced > 0 && sim.initCond != "taylorGreen" && !sim.bChannelFixedMassFlux)
+    sim.pipeline.emplace_back(new ExternalForcing(sim));
   if (sim.spectralForcing) sim.pipeline.emplace_back(new SpectralForcing(sim));
   sim.pipeline.emplace_back(new PressureRHS(sim));
   sim.pipeline.emplace_back(new PressureProjection(sim));
@@ -1053,11 +1068,12 @@ bool Simulation::advance(int n)
 void Simulation::run()
 {
   const bool analysis = sim.analysis == "HIT" && (sim.freqAnalysis > 0 || sim.timeAnalysis > 0);
+  const bool external = sim.uMax_forced > 0 && sim.initCond != "taylorGreen";  // not in the device loop
   // the device-resident loop writes no per-step files: taken when none is asked for
   const bool logs = (!sim.muteAll && (!sim.obstacles.empty() || sim.spectralForcing)) || sim.saveFreq > 0 ||
                     sim.saveTime > 0;
   if (sim.decomposition == CUP_FUSED && sim.nsteps > 0 && sim.endTime <= 0 && sim.freqDiagnostics == 0 && !analysis &&
-      !logs) {
+      !logs && !external) {
     advance(sim.nsteps - sim.step);
     return;
   }

@@ -146,6 +146,7 @@ struct SimulationData {
   double icUrms = 1.0;
   // forcing (SimulationData.h:109-115)
   Real uMax_forced = 0, uMax_measured = 0;
+  bool bChannelFixedMassFlux = false;
   bool spectralForcing = false;
   double turbKinEn_target = 0, enInjectionRate = 0, dissipationRate = 0, actualInjectionRate = 0;
   // settings (SimulationData.h:141-147)
@@ -224,6 +225,16 @@ class AdvectionDiffusion : public Operator
   using Operator::Operator;
   void operator()(double dt) override;
   std::string getName() override { return "AdvectionDiffusion"; }
+};
+
+// ExternalForcing (ExternalForcing.cpp:35-48): the uniform pressure gradient that drives a
+// mean flow of centreline velocity uMax_forced, u += 8 uMax_forced nu dt / H^2.
+class ExternalForcing : public Operator
+{
+ public:
+  using Operator::Operator;
+  void operator()(double dt) override;
+  std::string getName() override { return "ExternalForcing"; }
 };
 
 class SpectralForcing : public Operator

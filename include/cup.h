@@ -153,6 +153,11 @@ int cup_surface_forces(cup_ctx* ctx, double nu, const double trans_vel[3], const
  * Obstacle::computeVelocities (Obstacle.cpp:153-264). */
 int cup_obstacle_momenta(cup_ctx* ctx, double dt, double lambda, int implicit, double out[29]);
 
+/* ExternalForcing (ExternalForcing.cpp:35-48): the uniform velocity increment du (the
+ * reference's 8 uMax_forced nu dt / H^2 along x) added to every cell of the step's velocity:
+ * u, and u* when the advection left it apart from u. */
+int cup_add_uniform_velocity(cup_ctx* ctx, const double du[3]);
+
 /* KernelDissipation (ComputeDissipation.cpp:17-116): out[20] in the RDX order (:136-155). */
 int cup_diagnostics(cup_ctx* ctx, double nu, double out[20]);
 

@@ -217,3 +217,22 @@ def test_command_line_program(cup, oracle, tmp_path):
         assert data[step, 0] == step and data[step, 1] == pytest.approx(t, rel=1e-5, abs=1e-12)
         assert data[step, 2 + 18] == pytest.approx(O.dissipation(0.01)[18], rel=1e-5)
         t += dt
+
+
+@pytest.mark.parametrize("decomp", ["plain", "fused"])
+def test_external_forcing_uniform_acceleration(cup, decomp):
+    """ExternalForcing (ExternalForcing.cpp:35-48, added by Simulation.cpp:336-339 when
+    uMax_forced > 0 and the IC is not the Taylor-Green vortex): from rest, a uniform field is
+    neither advected, diffused nor projected, so after the run u = 8 uMax nu t / H^2 with
+    H = extent_z, exactly, and v = w = 0."""
+    print(f"external forcing decomp={decomp}", flush=True)
+    argv = ["-bpdx", "4", "-bpdy", "4", "-bpdz", "2", "-extentx", "2.0", "-initCond", "zero", "-uMax_forced", "0.5",
+            "-nu", "0.01", "-CFL", "0.3", "-rampup", "0", "-nsteps", "3", "-decomposition", decomp] + PERIODIC
+    SD = cup.SimulationData(argv=argv)
+    S = cup.Simulation(SD)
+    S.run()
+    H = SD.extent_z
+    assert H == pytest.approx(1.0)
+    u = S.field("u")
+    np.testing.assert_allclose(u, 8 * 0.5 * 0.01 / H ** 2 * SD.time, rtol=1e-13)
+    assert not S.field("v").any() and not S.field("w").any()
