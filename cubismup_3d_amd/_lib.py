@@ -70,6 +70,7 @@ SIGNATURES = {
     "cup_surface_forces": (_i, [_p, _d, _dp, _dp, _dp]),
     "cup_obstacle_momenta": (_i, [_p, _d, _d, _i, _dp]),
     "cup_add_uniform_velocity": (_i, [_p, _dp]),
+    "cup_ic_hit_spectrum": (_i, [_p, _dp, _dp, _i, _d, _u64]),
     "cup_diagnostics": (_i, [_p, _d, _dp]),
     "cup_ic_taylor_green": (_i, [_p, _d]),
     "cup_ic_hit": (_i, [_p, _d, _d, _u64]),

@@ -453,8 +453,18 @@ int cup_ic_hit(cup_ctx* ctx, double k0, double u_rms, uint64_t seed)
 {
   CUP_GUARD(ctx);
   if (!(k0 > 0) || !(u_rms > 0)) return set_error(c, CUP_E_INVALID, "ic_hit: k0 and u_rms must be > 0");
-  if (int rc = launch_ic_hit(c, k0, u_rms, seed)) return rc;
+  if (int rc = launch_ic_hit(c, k0, nullptr, nullptr, 0, u_rms, seed)) return rc;
   return check_hip(c, hipStreamSynchronize(c.stream), "ic_hit");
+}
+
+int cup_ic_hit_spectrum(cup_ctx* ctx, const double* K, const double* E, int n, double u_rms, uint64_t seed)
+{
+  CUP_GUARD(ctx);
+  if (!K || !E || n < 2) return set_error(c, CUP_E_INVALID, "ic_hit_spectrum: the table needs two entries or more");
+  for (int i = 1; i < n; ++i)
+    if (!(K[i] > K[i - 1])) return set_error(c, CUP_E_INVALID, "ic_hit_spectrum: K must increase");
+  if (int rc = launch_ic_hit(c, 0.0, K, E, n, u_rms, seed)) return rc;
+  return check_hip(c, hipStreamSynchronize(c.stream), "ic_hit_spectrum");
 }
 
 int cup_spectral_forcing(cup_ctx* ctx, double dt, double sim_dt, double nu, double tke_target,

@@ -226,7 +226,7 @@ int launch_grad_stats(Ctx& c, double out[2]);  // sums of sqrt(grad2) and grad2 
 int launch_fft_inverse_field(Ctx& c, void* spec, void* real_blocks);        // 3 passes
 // cup_spectral.hip
 int spectral_setup(Ctx& c);
-int launch_ic_hit(Ctx& c, double k0, double u_rms, uint64_t seed);
+int launch_ic_hit(Ctx& c, double k0, const double* K, const double* E, int tn, double u_rms, uint64_t seed);
 int forcing_plain(Ctx& c, double dt, double sim_dt, double nu, double tke_target, double inj,
                   double stats[5]);
 int launch_forcing_reduce(Ctx& c, double tke_target, double inj_rate);

@@ -40,13 +40,33 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t
 // (kz, ky, s): s >= 1 is mode (kx=s, ky, kz); s = 0 is mode (0, ky, kz) (+ i * the
 // kx = Nx/2 plane, which is zero). Construction per mode: SpectralManipFFTW::_compute_IC
 // (SpectralManipFFTW.cpp:254-285) with noise_a[1] for Im(u) and exact Hermitian symmetry.
+// EnergySpectrum::interpE (SpectralManip.cpp:96-116) on a table with K increasing: with i the
+// first index of K above k (a binary search), the line through (K[i-1], E[i-1]) and
+// (K[i], E[i]); below K[0] the reference's E[0] - (E[1] - E[0]) / (K[1] - K[0]) (K[1] - k);
+// beyond the table the last segment's line; negative values give 0.
+__device__ double spectrum_interp(const double* __restrict__ K, const double* __restrict__ E, int n, double k)
+{
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (k < K[mid]) hi = mid;
+    else lo = mid + 1;
+  }
+  const int i = lo;
+  double e;
+  if (i > 0 && i < n) e = E[i] + (E[i - 1] - E[i]) / (K[i] - K[i - 1]) * (K[i] - k);
+  else if (i == 0) e = E[0] - (E[1] - E[0]) / (K[1] - K[0]) * (K[1] - k);
+  else e = E[n - 1] + (E[n - 2] - E[n - 1]) / (K[n - 1] - K[n - 2]) * (K[n - 1] - k);
+  return e < 0 ? 0.0 : e;
+}
+
 __global__ void k_hit_spectrum(C2<double>* __restrict__ su, C2<double>* __restrict__ sv, C2<double>* __restrict__ sw,
                                int N0, int N1, int N2, double wfx, double wfy, double wfz, double k0, uint32_t key0,
-                               uint32_t key1)
+                               uint32_t key1, const double* __restrict__ tK, const double* __restrict__ tE, int tn)
 {
   const int ncols = N0 / 2;
   const size_t n = (size_t)ncols * N1 * N2;
-  const double coef = 32.0 / (3 * k0) * sqrt(2.0 / M_PI);
+  const double coef = tn ? 0.0 : 32.0 / (3 * k0) * sqrt(2.0 / M_PI);
   for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < n; id += (size_t)gridDim.x * blockDim.x) {
     const int s = (int)(id % ncols);
     const int ky = (int)((id / ncols) % N1);
@@ -62,8 +82,8 @@ __global__ void k_hit_spectrum(C2<double>* __restrict__ su, C2<double>* __restri
       const double kx = ii * wfx, kyv = jj * wfy, kz_ = kk * wfz;
       const double k2 = kx * kx + kyv * kyv + kz_ * kz_;
       const double k_xy = sqrt(kx * kx + kyv * kyv), k_norm = sqrt(k2);
-      const double r = k_norm / k0;
-      const double E_k = coef * r * r * r * r * exp(-2 * r * r);
+      const double r = tn ? 0.0 : k_norm / k0;
+      const double E_k = tn ? spectrum_interp(tK, tE, tn, k_norm) : coef * r * r * r * r * exp(-2 * r * r);
       const double kn = k_norm / wfx;
       const double amp = sqrt(E_k / (2 * M_PI * kn * kn));
       uint32_t o[4];
@@ -111,7 +131,7 @@ static int alloc_extra_spectra(Ctx& c)
   return CUP_OK;
 }
 
-int launch_ic_hit(Ctx& c, double k0, double u_rms, uint64_t seed)
+int launch_ic_hit(Ctx& c, double k0, const double* K, const double* E, int tn, double u_rms, uint64_t seed)
 {
   if (c.precision == 4) {
     // Generate in an FP64 context of the same grid, then store rounded to FP32.
@@ -123,7 +143,7 @@ int launch_ic_hit(Ctx& c, double k0, double u_rms, uint64_t seed)
     d.device = c.dev;
     cup_ctx* t = nullptr;
     if (int rc = cup_create(&d, &t)) return set_error(c, rc, "ic_hit: FP64 work context");
-    int rc = cup_ic_hit(t, k0, u_rms, seed);
+    int rc = tn ? cup_ic_hit_spectrum(t, K, E, tn, u_rms, seed) : cup_ic_hit(t, k0, u_rms, seed);
     std::vector<double> host(c.ncell);
     for (int f = 0; f < 8 && !rc; ++f) {
       rc = cup_get_field(t, f, host.data());
@@ -138,10 +158,21 @@ int launch_ic_hit(Ctx& c, double k0, double u_rms, uint64_t seed)
     if (int rc = launch_fill(c, c.f[k], c.ncell, 0.0)) return rc;
   const size_t n = (size_t)(c.N[0] / 2) * c.N[1] * c.N[2];
   const double wf[3] = {2 * M_PI / c.extent[0], 2 * M_PI / c.extent[1], 2 * M_PI / c.extent[2]};
+  double* table = nullptr;  // the target table on the device, freed once the spectrum is built
+  if (tn) {
+    if (check_hip(c, hipMalloc(&table, 2 * (size_t)tn * sizeof(double)), "hipMalloc spectrum table")) return CUP_E_NOMEM;
+    hipMemcpy(table, K, tn * sizeof(double), hipMemcpyHostToDevice);
+    hipMemcpy(table + tn, E, tn * sizeof(double), hipMemcpyHostToDevice);
+  }
   hipLaunchKernelGGL(k_hit_spectrum, grid_for(n), dim3(NT), 0, c.stream, (C2<double>*)c.spec, (C2<double>*)c.spec2,
                      (C2<double>*)c.spec3, c.N[0], c.N[1], c.N[2], wf[0], wf[1], wf[2], k0, (uint32_t)seed,
-                     (uint32_t)(seed >> 32));
-  if (int rc = check_hip(c, hipGetLastError(), "k_hit_spectrum")) return rc;
+                     (uint32_t)(seed >> 32), (const double*)table, (const double*)(table ? table + tn : nullptr), tn);
+  const int krc = check_hip(c, hipGetLastError(), "k_hit_spectrum");
+  if (table) {
+    hipStreamSynchronize(c.stream);
+    hipFree(table);
+  }
+  if (krc) return krc;
   if (int rc = launch_fft_inverse_field(c, c.spec, c.f[CUP_U])) return rc;
   if (int rc = launch_fft_inverse_field(c, c.spec2, c.f[CUP_V])) return rc;
   if (int rc = launch_fft_inverse_field(c, c.spec3, c.f[CUP_W])) return rc;
@@ -152,6 +183,7 @@ int launch_ic_hit(Ctx& c, double k0, double u_rms, uint64_t seed)
   if (int rc = check_hip(c, hipMemcpyAsync(red, c.sc->red, sizeof(red), hipMemcpyDeviceToHost, c.stream), "ic_hit"))
     return rc;
   if (int rc = check_hip(c, hipStreamSynchronize(c.stream), "ic_hit")) return rc;
+  if (!(u_rms > 0)) return CUP_OK;  // a target table's energy is kept
   const double scale = u_rms / std::sqrt(red[1] / (3.0 * (double)c.ncell));
   hipLaunchKernelGGL((k_scale3<double>), grid_for(c.ncell), dim3(NT), 0, c.stream, (double*)c.f[CUP_U],
                      (double*)c.f[CUP_V], (double*)c.f[CUP_W], c.ncell, scale);

@@ -161,7 +161,7 @@ SimulationData::SimulationData(int argc, const char* const* argv) : SimulationDa
   k0 = parser("-k0").asDouble(10.0);
   tke0 = parser("-tke0").asDouble(1.0);
   icSeed = (uint64_t)parser("-icSeed").asDouble(20260101);
-  icUrms = parser("-icUrms").asDouble(1.0);
+  icUrms = parser("-icUrms").asDouble(0.0);  // 0: the target spectrum's energy
   turbKinEn_target = parser("-turbKinEn_target").asDouble(0);
   enInjectionRate = parser("-energyInjectionRate").asDouble(0);
   const bool hint = turbKinEn_target > 0 || enInjectionRate > 0;
@@ -288,6 +288,56 @@ void Operator::check(int rc, const char* what)
 }
 
 // InitialConditions::operator() (InitialConditions.cpp:164-230).
+// SpectralIcGenerator::_generateTarget (SpectralIcGenerator.cpp:21-98): the target spectrum
+// table of -spectralIC. "art" (and no value): E = 32 tke0/(3 k0) sqrt(2/pi) (k/k0)^4
+// exp(-2 (k/k0)^2) at bin centres; "cbc": the Comte-Bellot and Corrsin measurements in the
+// reference's units; "fromFit": HITstatistics::getTargetSpectrumFit (SpectralManip.cpp:26-46)
+// for the injection rate and nu. The table values are data and fit constants of the reference.
+void spectralTarget(const SimulationData& sim, std::vector<double>& K, std::vector<double>& E)
+{
+  const int maxN = 8 * std::max({sim.bpdx, sim.bpdy, sim.bpdz});
+  const double maxL = sim.maxextent;
+  const int nBins = (int)std::ceil(std::sqrt(3.0) * maxN / 2.0) + 1;
+  if (sim.spectralIC == "" || sim.spectralIC == "art") {
+    const double binSize = M_PI * std::sqrt(3.0) * maxN / (nBins * maxL);
+    const double coef = 32 * sim.tke0 / (3 * sim.k0) * std::sqrt(2.0 / M_PI);
+    K.resize(nBins);
+    E.resize(nBins);
+    for (int i = 0; i < nBins; ++i) {
+      K[i] = (i + 0.5) * binSize;
+      E[i] = coef * std::pow(K[i] / sim.k0, 4) * std::exp(-2 * std::pow(K[i] / sim.k0, 2));
+    }
+  } else if (sim.spectralIC == "cbc") {
+    // wavenumbers in 1/cm and energies in cm^3/s^2, grid mesh M = 5.08 cm
+    K = {0.20, 0.25, 0.30, 0.40, 0.50, 0.70, 1.00, 1.50, 2.00, 2.50, 3.00, 4.00, 6.00, 8.00, 10.00, 12.50, 15.00,
+         17.50, 20.00};
+    E = {129.0, 230.0, 322.0, 435.0, 457.0, 380.0, 270.0, 168.0, 120.0, 89.0, 70.3, 47.0, 24.7, 12.6, 7.42, 3.96, 2.33,
+         1.34, 0.8};
+    const double M = 5.08, u_ref = std::sqrt(1.5) * 22.2, L_ref = 10.8 * M;
+    for (size_t i = 0; i < K.size(); ++i) {
+      K[i] *= L_ref * 0.01 * 2 * M_PI;
+      E[i] /= u_ref * u_ref * L_ref;
+    }
+  } else if (sim.spectralIC == "fromFit") {
+    const double eps = sim.enInjectionRate, nu = sim.nu;
+    if (!(eps > 0) || !(nu > 0)) throw std::invalid_argument("spectralIC fromFit needs -energyInjectionRate and -nu");
+    const double Lint = 0.74885397 * std::pow(eps, -0.0233311) * std::pow(nu, 0.07192009);
+    const double Lk = std::pow(eps, -0.25) * std::pow(nu, 0.75);
+    const double C = 5.7, CI = 1e-3, CE = 0, BE = 4.9, P0 = 600;
+    K.resize(nBins);
+    E.resize(nBins);
+    for (int i = 0; i < nBins; ++i) {
+      K[i] = (i + 0.5) * 2 * M_PI / maxL;
+      const double KI = K[i] * Lint, KE4 = std::pow(K[i] * Lk, 4);
+      const double FL = std::pow(KI / (KI + CI), 5 / 3.0 + P0);
+      const double FE = std::exp(-BE * (std::pow(KE4 + std::pow(CE, 4), 0.25) - CE));
+      E[i] = C * std::pow(eps, 2 / 3.0) * std::pow(K[i], -5 / 3.0) * FL * FE;
+    }
+  } else {
+    throw std::invalid_argument("spectralIC not supported: " + sim.spectralIC + " (art, cbc, fromFit)");
+  }
+}
+
 void InitialConditions::operator()(double)
 {
   if (sim.initCond == "zero") {
@@ -298,7 +348,12 @@ void InitialConditions::operator()(double)
   } else if (sim.initCond == "taylorGreen") {
     check(cup_ic_taylor_green(sim.ctx, sim.uMax_forced), "IC taylorGreen");
   } else if (sim.initCond == "HITurbulence") {
-    check(cup_ic_hit(sim.ctx, sim.k0, sim.icUrms, sim.icSeed), "IC HITurbulence");
+    // SpectralIcGenerator::run (SpectralIcGenerator.cpp:118-135) on the target of -spectralIC;
+    // the field keeps the target's energy unless -icUrms asks for a u_rms
+    std::vector<double> K, E;
+    spectralTarget(sim, K, E);
+    check(cup_ic_hit_spectrum(sim.ctx, K.data(), E.data(), (int)K.size(), sim.icUrms, sim.icSeed),
+          "IC HITurbulence");
     // SpectralIcGenerator::run (SpectralIcGenerator.cpp:126-130): target = IC energy
     if (sim.spectralForcing && sim.turbKinEn_target <= 0 && sim.enInjectionRate <= 0) {
       double d[20];

@@ -143,7 +143,7 @@ struct SimulationData {
   std::string spectralIC = "";
   double k0 = 0, tke0 = 0;
   uint64_t icSeed = 20260101;
-  double icUrms = 1.0;
+  double icUrms = 0.0;  // > 0: the HIT IC scaled to this u_rms; 0: the target spectrum's energy
   // forcing (SimulationData.h:109-115)
   Real uMax_forced = 0, uMax_measured = 0;
   bool bChannelFixedMassFlux = false;
@@ -308,6 +308,9 @@ class Analysis : public Operator
   std::vector<double> spectrum;
   int files = 0;
 };
+
+// SpectralIcGenerator::_generateTarget: the target spectrum table (K, E) of -spectralIC.
+void spectralTarget(const SimulationData& sim, std::vector<double>& K, std::vector<double>& E);
 
 class Simulation
 {

@@ -225,6 +225,15 @@ class Grid:
     def ic_hit(self, k0=4.0, u_rms=1.0, seed=20260101):
         self._c(L.lib().cup_ic_hit(self._h, float(k0), float(u_rms), int(seed)))
 
+    def ic_hit_spectrum(self, K, E, u_rms=0.0, seed=20260101):
+        """The seeded isotropic IC with E(|k|) interpolated from the target table (K, E);
+        u_rms <= 0 keeps the table's energy."""
+        K = np.ascontiguousarray(K, dtype=np.float64)
+        E = np.ascontiguousarray(E, dtype=np.float64)
+        if K.shape != E.shape:
+            raise ValueError("K and E differ in length")
+        self._c(L.lib().cup_ic_hit_spectrum(self._h, L.dptr(K), L.dptr(E), int(K.size), float(u_rms), int(seed)))
+
     def spectral_forcing(self, dt, nu, tke_target=0.0, injection_rate=0.0, sim_dt=None):
         st = np.zeros(5)
         self._c(L.lib().cup_spectral_forcing(self._h, float(dt), float(dt if sim_dt is None else sim_dt),

@@ -177,6 +177,11 @@ int cup_ic_taylor_green(cup_ctx* ctx, double uMax);
  * Philox-4x32-10 phases from seed, exact Hermitian symmetry, DC and Nyquist zeroed,
  * scaled to sqrt(<|u|^2>/3) = u_rms. Generated in FP64 on the device. */
 int cup_ic_hit(cup_ctx* ctx, double k0, double u_rms, uint64_t seed);
+/* The same construction with E(|k|) from a target table (SpectralIcGenerator::_generateTarget,
+ * SpectralIcGenerator.cpp:21-98, read through EnergySpectrum::interpE, SpectralManip.cpp:96-116,
+ * in SpectralManipFFTW::_compute_IC): K[i], E[i], i < n, n >= 2, K increasing. u_rms > 0 scales
+ * the field to it; u_rms <= 0 keeps the table's energy, as the reference does. */
+int cup_ic_hit_spectrum(cup_ctx* ctx, const double* K, const double* E, int n, double u_rms, uint64_t seed);
 
 /* SpectralForcing::operator() (SpectralForcing.cpp:23-82) with
  * SpectralManipFFTW::_compute_largeModesForcing (SpectralManipFFTW.cpp:25-108):

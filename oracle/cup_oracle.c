@@ -1004,7 +1004,36 @@ void orc_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
  * noise_a[1] (the reference reads noise_a[0] there, :277), exact Hermitian symmetry
  * (every mode with a negative partner takes the conjugate of its partner), DC and every
  * Nyquist index zeroed, and the field scaled so that sqrt(<u^2+v^2+w^2>/3) = u_rms. */
-void orc_ic_hit(orc_sim* s, double k0, double u_rms, uint64_t seed)
+/* EnergySpectrum::interpE (SpectralManip.cpp:96-116): with i the first index of K above k,
+ * the line through (K[i-1], E[i-1]) and (K[i], E[i]); below K[0] the reference's form
+ * E[0] - (E[1] - E[0]) / (K[1] - K[0]) (K[1] - k); beyond the table the last segment's line;
+ * negative values give 0. */
+double orc_spectrum_interp(const double* K, const double* E, int n, double k)
+{
+  int i = 0;
+  while (i < n && !(k < K[i])) i++;
+  double e;
+  if (i > 0 && i < n) e = E[i] + (E[i - 1] - E[i]) / (K[i] - K[i - 1]) * (K[i] - k);
+  else if (i == 0) e = E[0] - (E[1] - E[0]) / (K[1] - K[0]) * (K[1] - k);
+  else e = E[n - 1] + (E[n - 2] - E[n - 1]) / (K[n - 1] - K[n - 2]) * (K[n - 1] - k);
+  return e < 0 ? 0.0 : e;
+}
+
+static void ic_hit_core(orc_sim* s, double k0, const double* tK, const double* tE, int tn, double u_rms,
+                        uint64_t seed);
+
+void orc_ic_hit(orc_sim* s, double k0, double u_rms, uint64_t seed) { ic_hit_core(s, k0, NULL, NULL, 0, u_rms, seed); }
+
+/* The same construction with E(|k|) from a target table (SpectralIcGenerator::_generateTarget,
+ * SpectralIcGenerator.cpp:21-98, read by interpE in _compute_IC); u_rms <= 0 keeps the
+ * table's energy (the reference does not rescale). */
+void orc_ic_hit_spectrum(orc_sim* s, const double* K, const double* E, int n, double u_rms, uint64_t seed)
+{
+  ic_hit_core(s, 0.0, K, E, n, u_rms, seed);
+}
+
+static void ic_hit_core(orc_sim* s, double k0, const double* tK, const double* tE, int tn, double u_rms,
+                        uint64_t seed)
 {
   const int N0 = s->N[0], N1 = s->N[1], N2 = s->N[2];
   const size_t n = (size_t)N0 * N1 * N2;
@@ -1012,7 +1041,7 @@ void orc_ic_hit(orc_sim* s, double k0, double u_rms, uint64_t seed)
   double complex* cv = (double complex*)calloc(n, sizeof(double complex));
   double complex* cw = (double complex*)calloc(n, sizeof(double complex));
   const double wfx = 2.0 * M_PI / s->extent[0], wfy = 2.0 * M_PI / s->extent[1], wfz = 2.0 * M_PI / s->extent[2];
-  const double coef = 32.0 / (3 * k0) * sqrt(2.0 / M_PI);
+  const double coef = tn ? 0.0 : 32.0 / (3 * k0) * sqrt(2.0 / M_PI);
   const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
   #pragma omp parallel for schedule(static)
   for (int i = 0; i < N0; i++)
@@ -1028,8 +1057,8 @@ void orc_ic_hit(orc_sim* s, double k0, double u_rms, uint64_t seed)
         const double kx = ii * wfx, ky = jj * wfy, kz = kk * wfz;
         const double k2 = kx * kx + ky * ky + kz * kz;
         const double k_xy = sqrt(kx * kx + ky * ky), k_norm = sqrt(k2);
-        const double r = k_norm / k0;
-        const double E_k = coef * r * r * r * r * exp(-2 * r * r);
+        const double r = tn ? 0.0 : k_norm / k0;
+        const double E_k = tn ? orc_spectrum_interp(tK, tE, tn, k_norm) : coef * r * r * r * r * exp(-2 * r * r);
         const double amp = sqrt(E_k / (2 * M_PI * pow(k_norm / wfx, 2)));
         const uint32_t ctr[4] = {(uint32_t)cid, (uint32_t)((uint64_t)cid >> 32), 0, 0};
         uint32_t o[4];
@@ -1062,7 +1091,7 @@ void orc_ic_hit(orc_sim* s, double k0, double u_rms, uint64_t seed)
   }
   for (int i = 0; i < N0; i++) ssq += part[i];
   free(part);
-  const double scale = u_rms / sqrt(ssq / (3.0 * n));
+  const double scale = u_rms > 0 ? u_rms / sqrt(ssq / (3.0 * n)) : 1.0;
   #pragma omp parallel for schedule(static)
   for (int b = 0; b < s->nb; b++) {
     int bc[3];

@@ -65,6 +65,8 @@ def lib():
         "orc_spectral_forcing": (None, [p, d, d, d, d, d, dp]),
         "orc_ic_taylor_green": (None, [p, d]),
         "orc_ic_hit": (None, [p, d, d, C.c_uint64]),
+        "orc_ic_hit_spectrum": (None, [p, dp, dp, i, d, C.c_uint64]),
+        "orc_spectrum_interp": (d, [dp, dp, i, d]),
         "orc_sphere_create": (None, [p, d, dp]),
         "orc_sphere_create_mollified": (None, [p, d, dp, d]),
         "orc_sphere_info": (None, [p, dp]),
@@ -122,6 +124,13 @@ def fft3d(a: np.ndarray, sign: int = -1) -> np.ndarray:
     A, B, Cc = a.shape
     lib().orc_fft3d(A, B, Cc, _dp(re), _dp(im), int(sign))
     return re + 1j * im
+
+
+def spectrum_interp(K, E, k):
+    """EnergySpectrum::interpE (SpectralManip.cpp:96-116) of the oracle."""
+    K = np.ascontiguousarray(K, dtype=np.float64)
+    E = np.ascontiguousarray(E, dtype=np.float64)
+    return lib().orc_spectrum_interp(_dp(K), _dp(E), int(K.size), float(k))
 
 
 def compute_velocities(M, implicit, forced, imposed, block_rot):
@@ -287,6 +296,12 @@ class OracleSim:
 
     def ic_hit(self, k0=4.0, u_rms=1.0, seed=20260101):
         lib().orc_ic_hit(self._h, float(k0), float(u_rms), int(seed))
+
+    def ic_hit_spectrum(self, K, E, u_rms=0.0, seed=20260101):
+        """The seeded IC with E(|k|) interpolated from the table (K, E); u_rms <= 0 keeps its energy."""
+        K = np.ascontiguousarray(K, dtype=np.float64)
+        E = np.ascontiguousarray(E, dtype=np.float64)
+        lib().orc_ic_hit_spectrum(self._h, _dp(K), _dp(E), int(K.size), float(u_rms), int(seed))
 
     def sphere_create(self, radius, pos, mollify=0.0, hemi=False):
         """The sphere's chi by Towers' formula, or (mollify = half-width in cells, in (0, 2])

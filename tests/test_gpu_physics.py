@@ -190,3 +190,52 @@ def test_fused_forcing_fp32_vs_fp64(cup):
         G.run_steps(10, CFL=0.3, nu=0.005, spectral_forcing=True, tke_target=tgt)
     for a, b in zip(G4.velocity(), G8.velocity()):
         assert rel(a, b) < 2e-6
+
+
+@pytest.mark.parametrize("bpd,u_rms", [((4, 4, 4), 0.0), ((2, 4, 8), 0.5)])
+def test_table_ic_matches_oracle(cup, oracle, bpd, u_rms):
+    """The IC from a target table (interpE of SpectralManip.cpp:96-116): device FP64 against
+    the oracle, with the table's energy kept (u_rms 0) or scaled to u_rms."""
+    print(f"table IC bpd={bpd} u_rms={u_rms}", flush=True)
+    K = np.linspace(0.3, 40.0, 60)
+    E = K ** 4 * np.exp(-2 * (K / 3.0) ** 2) / 50.0
+    G = cup.Grid(bpd, precision=8)
+    O = oracle.OracleSim(bpd)
+    G.ic_hit_spectrum(K, E, u_rms, 11)
+    O.ic_hit_spectrum(K, E, u_rms, 11)
+    for a, b in zip(G.velocity(), O.velocity()):
+        assert rel(a, b) < 1e-12
+
+
+def _mirror_ic(cup, extra):
+    import math
+    argv = ["-bpdx", "4", "-bpdy", "4", "-bpdz", "4", "-extentx", repr(2 * math.pi), "-BC_x", "periodic", "-BC_y",
+            "periodic", "-BC_z", "periodic", "-muteAll", "1", "-freqDiagnostics", "0", "-precision", "double",
+            "-initCond", "HITurbulence", "-nu", "0.005", "-CFL", "0.3"] + extra
+    SD = cup.SimulationData(argv=argv)
+    return cup.Simulation(SD)
+
+
+def test_mirror_spectral_ic_targets(cup):
+    """-spectralIC art -tke0 0.67 -k0 4 (launch/settingsHIT.sh:23): the field's energy is the
+    closed form of the art table of SpectralIcGenerator::_generateTarget (bins of width
+    pi sqrt(3) N / (nBins L), nBins = ceil(sqrt(3) N/2) + 1) and near tke0; cbc and fromFit
+    build; an unknown target is an error."""
+    import math
+    S = _mirror_ic(cup, ["-spectralIC", "art", "-tke0", "0.67", "-k0", "4"])
+    u, v, w = (S.field(n) for n in "uvw")
+    tke = 0.5 * np.mean(u ** 2 + v ** 2 + w ** 2)
+    N, L = 32, 2 * math.pi
+    nb = math.ceil(math.sqrt(3) * N / 2) + 1
+    K = (np.arange(nb) + 0.5) * math.pi * math.sqrt(3) * N / (nb * L)
+    E = 32 * 0.67 / (3 * 4) * math.sqrt(2 / math.pi) * (K / 4) ** 4 * np.exp(-2 * (K / 4) ** 2)
+    from test_oracle_physics import hit_table_energy
+    h = L / N
+    assert tke == pytest.approx(hit_table_energy(u.shape, h, K, E), rel=1e-10)
+    assert tke == pytest.approx(0.67, rel=0.1)
+    for extra in (["-spectralIC", "cbc"], ["-spectralIC", "fromFit", "-energyInjectionRate", "0.1"]):
+        S = _mirror_ic(cup, extra)
+        e = 0.5 * np.mean(sum(S.field(n) ** 2 for n in "uvw"))
+        assert np.isfinite(e) and e > 0, extra
+    with pytest.raises(ValueError, match="spectralIC"):
+        _mirror_ic(cup, ["-spectralIC", "no_such_target"])

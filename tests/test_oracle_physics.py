@@ -225,3 +225,48 @@ def test_surface_force_of_uniform_pressure_vanishes(oracle):
     area = 4 * np.pi * 0.2 ** 2
     assert abs(q[0]) < 1e-10 * 3 * area and abs(q[1]) < 1e-10 * 3 * area
     assert np.all(q[6:9] == 0)
+
+
+def test_spectrum_interp_pins(oracle):
+    """EnergySpectrum::interpE (SpectralManip.cpp:96-116): exact on a linear table inside and
+    beyond it, the reference's form below the first node (E0 - slope (K1 - k)), 0 for
+    negative values."""
+    K = np.array([1.0, 2.0, 4.0, 7.0])
+    E = 3.0 + 2.0 * K
+    for k in (1.0, 1.3, 2.0, 3.7, 6.99, 7.0, 9.5):
+        assert oracle.spectrum_interp(K, E, k) == pytest.approx(3.0 + 2.0 * k, rel=1e-14)
+    assert oracle.spectrum_interp(K, E, 0.5) == pytest.approx(E[0] - 2.0 * (K[1] - 0.5), rel=1e-14)
+    assert oracle.spectrum_interp(K, 5.0 - K, 6.0) == 0.0  # 5 - 6 < 0
+
+
+def hit_table_energy(shape, h, K, E):
+    """Closed form of the table IC's energy: every mode with k_xy != 0 (kx = ky = 0 modes carry
+    none) and no Nyquist index holds |u_k|^2 = amp^2 = E(|k|) / (2 pi (|k|/wf)^2), and the
+    unnormalized inverse transform gives tke = 1/2 sum_k amp^2 (Parseval)."""
+    Z, Y, X = shape
+    wf = 2 * np.pi / (X * h)
+    kx = np.fft.fftfreq(X, 1.0 / X)
+    ky = np.fft.fftfreq(Y, 1.0 / Y)
+    kz = np.fft.fftfreq(Z, 1.0 / Z)
+    KX, KY, KZ = np.meshgrid(kx, ky, kz, indexing="ij")
+    ok = (2 * np.abs(KX) != X) & (2 * np.abs(KY) != Y) & (2 * np.abs(KZ) != Z) & ((KX != 0) | (KY != 0))
+    wfx, wfy, wfz = 2 * np.pi / (X * h), 2 * np.pi / (Y * h), 2 * np.pi / (Z * h)
+    kn = np.sqrt((KX * wfx) ** 2 + (KY * wfy) ** 2 + (KZ * wfz) ** 2)[ok]
+    Ek = np.interp(kn, K, E)  # the tables here span every |k| of the grid
+    return 0.5 * np.sum(Ek / (2 * np.pi * (kn / wf) ** 2))
+
+
+def test_table_ic_energy_and_divergence(oracle):
+    """The IC from a target table keeps the table's energy (closed form above) and is
+    spectrally solenoidal (central divergence small against |grad u|)."""
+    s = oracle.OracleSim((4, 4, 4))
+    K = np.linspace(0.5, 30.0, 40)
+    E = K ** 4 * np.exp(-2 * (K / 4.0) ** 2) / 100.0
+    s.ic_hit_spectrum(K, E, u_rms=0.0, seed=7)
+    u, v, w = s.velocity()
+    tke = 0.5 * np.mean(u ** 2 + v ** 2 + w ** 2)
+    assert tke == pytest.approx(hit_table_energy(s.shape, s.h, K, E), rel=1e-12)
+    s.ic_hit_spectrum(K, E, u_rms=0.7, seed=7)
+    u2, v2, w2 = s.velocity()
+    assert np.sqrt(np.mean(u2 ** 2 + v2 ** 2 + w2 ** 2) / 3) == pytest.approx(0.7, rel=1e-12)
+    np.testing.assert_allclose(u2 / u, 0.7 / np.sqrt(2 * tke / 3), rtol=1e-10)
