@@ -239,3 +239,37 @@ def test_mirror_spectral_ic_targets(cup):
         assert np.isfinite(e) and e > 0, extra
     with pytest.raises(ValueError, match="spectralIC"):
         _mirror_ic(cup, ["-spectralIC", "no_such_target"])
+
+
+@pytest.mark.parametrize("decomp", ["plain", "fused"])
+def test_forcing_constant_injection_matches_oracle(cup, oracle, decomp):
+    """The constant injection rate (-energyInjectionRate, SpectralForcing.cpp:41-46): one
+    forcing call (on u, or on u* in tmp for the fused decomposition, whose forcing acts on
+    the advected velocity) and 4 forced steps against the oracle, FP64."""
+    print(f"constant injection decomp={decomp}", flush=True)
+    bpd = (4, 4, 4)
+    G = cup.Grid(bpd, precision=8, decomposition=decomp)
+    O = oracle.OracleSim(bpd)
+    O.ic_hit(4.0, 1.0, 20260101)
+    u0 = O.velocity()
+    names = ("tmpU", "tmpV", "tmpW") if decomp == "fused" else ("u", "v", "w")
+    for n, f in zip(names, u0):
+        G.set(n, f)
+    sg = G.spectral_forcing(2e-3, 0.005, injection_rate=0.3)
+    so = O.spectral_forcing(2e-3, 0.005, injection_rate=0.3)
+    assert sg["fac"] == pytest.approx(so["fac"], rel=1e-12)
+    for n, b in zip(names, O.velocity()):
+        assert rel(G.get(n), b) < 1e-12
+    O.set_velocity(*u0)
+    G.set_velocity(*u0)
+    nu, CFL = 0.005, 0.3
+    G.run_steps(4, CFL=CFL, nu=nu, spectral_forcing=True, injection_rate=0.3)
+    for step in range(4):
+        dt, _ = O.calc_max_timestep(CFL, nu, (0, 0, 0), step, 0)
+        O.advect_diffuse(dt, nu)
+        O.spectral_forcing(dt, nu, injection_rate=0.3)
+        O.pressure_rhs(dt)
+        O.poisson_solve()
+        O.grad_p(dt)
+    for a, b in zip(G.velocity(), O.velocity()):
+        assert rel(a, b) < 1e-11

@@ -95,6 +95,26 @@ def test_forcing_stats_parseval_and_injection(oracle):
         assert np.abs((Y - X)[~low]).max() < 1e-8 * np.abs(X).max()
 
 
+def test_forcing_constant_injection_rate(oracle):
+    """Without a target the injection is the constant rate r (SpectralForcing.cpp:41-46):
+    fac = dt r / (2 tke_low) (:71), and the low modes gain ((1 + fac)^2 - 1) tke_low."""
+    s = oracle.OracleSim((4, 4, 4))
+    s.ic_hit(k0=4.0, u_rms=1.0, seed=9)
+    u, v, w = s.velocity()
+    N = u.shape[0]
+    k = np.fft.fftfreq(N, 1.0 / N)
+    kz, ky, kx = np.meshgrid(k, k, k, indexing="ij")
+    k2 = kx ** 2 + ky ** 2 + kz ** 2
+    low = (k2 > 0) & (k2 <= 4)
+    tke_low = 0.5 * sum((np.abs(np.fft.fftn(f)[low]) ** 2).sum() for f in (u, v, w)) / N ** 6
+    dt, r = 2e-3, 0.3
+    st = s.spectral_forcing(dt, 0.005, injection_rate=r)
+    assert st["fac"] == pytest.approx(dt * r / (2 * tke_low), rel=1e-12)
+    u2, v2, w2 = s.velocity()
+    tke_low2 = 0.5 * sum((np.abs(np.fft.fftn(f)[low]) ** 2).sum() for f in (u2, v2, w2)) / N ** 6
+    assert tke_low2 == pytest.approx((1 + st["fac"]) ** 2 * tke_low, rel=1e-11)
+
+
 def test_forcing_negative_injection_is_a_no_op(oracle):
     s = oracle.OracleSim((2, 2, 2))
     s.ic_hit(k0=2.0, u_rms=1.0, seed=5)
