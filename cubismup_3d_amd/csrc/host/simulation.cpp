@@ -1064,10 +1064,12 @@ void Simulation::syncStepState()
 
 bool Simulation::advance(int n)
 {
-  // the device-resident loop holds a body still: a body that moves takes the pipeline
+  // the device-resident loop holds a body still and has no uniform forcing: those steps take
+  // the pipeline (it writes no per-step files either)
   bool moving = false;
   for (const auto& s : sim.obstacles) moving = moving || !s->isStatic();
-  if (sim.decomposition != CUP_FUSED || moving) {
+  const bool external = sim.uMax_forced > 0 && sim.initCond != "taylorGreen";
+  if (sim.decomposition != CUP_FUSED || moving || external) {
     for (int i = 0; i < n; ++i)
       if (timestep(calcMaxTimestep())) return true;
     return false;
@@ -1103,6 +1105,7 @@ bool Simulation::advance(int n)
       p.body_angvel[d] = s.angVel[d];
     }
   }
+  if (n <= 0) return (sim.endTime > 0 && sim.time > sim.endTime) || (sim.nsteps != 0 && sim.step >= sim.nsteps);
   if (cup_run_steps(sim.ctx, &p, n) != CUP_OK)
     throw std::runtime_error(std::string("advance: ") + cup_last_error(sim.ctx));
   syncStepState();
