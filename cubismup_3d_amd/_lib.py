@@ -69,6 +69,12 @@ SIGNATURES = {
     "cup_penalize": (_i, [_p, _d, _d, _i, _dp, _dp, _dp]),
     "cup_surface_forces": (_i, [_p, _d, _dp, _dp, _dp]),
     "cup_obstacle_momenta": (_i, [_p, _d, _d, _i, _dp]),
+    "cup_body_count": (_i, [_p, C.POINTER(C.c_int)]),
+    "cup_add_sphere": (_i, [_p, _dp, _d, _i, _dp, C.POINTER(C.c_int)]),
+    "cup_move_body": (_i, [_p, _i, _dp, _dp]),
+    "cup_penalize_body": (_i, [_p, _i, _d, _d, _i, _dp, _dp, _dp]),
+    "cup_surface_forces_body": (_i, [_p, _i, _d, _dp, _dp, _dp]),
+    "cup_obstacle_momenta_body": (_i, [_p, _i, _d, _d, _i, _dp]),
     "cup_add_uniform_velocity": (_i, [_p, _dp]),
     "cup_ic_hit_spectrum": (_i, [_p, _dp, _dp, _i, _d, _u64]),
     "cup_diagnostics": (_i, [_p, _d, _dp]),

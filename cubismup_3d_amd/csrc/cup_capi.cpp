@@ -47,7 +47,7 @@ static void release(Ctx& c)
 {
   if (c.stream) hipStreamSynchronize(c.stream);
   timing_release(c);
-  body_free(c);
+  bodies_free(c);
   for (auto& p : c.f)
     if (p) { hipFree(p); p = nullptr; }
   if (c.ana) hipFree(c.ana);
@@ -479,10 +479,21 @@ int cup_spectral_forcing(cup_ctx* ctx, double dt, double sim_dt, double nu, doub
   return CUP_OK;
 }
 
+// The bodies other than body 0 are dropped (the single-body entry points replace every body).
+static void drop_other_bodies(Ctx& c)
+{
+  for (size_t j = 0; j < c.more.size(); ++j) {
+    BodyScope scope(c, (int)j + 1);
+    body_free(c);
+  }
+  c.more.clear();
+}
+
 int cup_chi_from_sphere(cup_ctx* ctx, const double center[3], double radius, double com_mass_out[4])
 {
   CUP_GUARD(ctx);
   if (!center || !(radius > 0)) return set_error(c, CUP_E_INVALID, "chi_from_sphere: radius must be > 0");
+  drop_other_bodies(c);
   if (int rc = body_create(c, center, radius, 0.0)) return rc;
   if (com_mass_out) {
     for (int a = 0; a < 3; ++a) com_mass_out[a] = c.body.com[a];
@@ -495,6 +506,7 @@ int cup_chi_from_hemisphere(cup_ctx* ctx, const double center[3], double radius,
 {
   CUP_GUARD(ctx);
   if (!center || !(radius > 0)) return set_error(c, CUP_E_INVALID, "chi_from_hemisphere: radius must be > 0");
+  drop_other_bodies(c);
   if (int rc = body_create(c, center, radius, 0.0, true)) return rc;
   if (com_mass_out) {
     for (int a = 0; a < 3; ++a) com_mass_out[a] = c.body.com[a];
@@ -510,12 +522,94 @@ int cup_chi_from_sphere_mollified(cup_ctx* ctx, const double center[3], double r
   if (!center || !(radius > 0)) return set_error(c, CUP_E_INVALID, "chi_from_sphere_mollified: radius must be > 0");
   if (!(width > 0 && width <= 2))
     return set_error(c, CUP_E_INVALID, "chi_from_sphere_mollified: the half-width must lie in (0, 2] cells");
+  drop_other_bodies(c);
   if (int rc = body_create(c, center, radius, width)) return rc;
   if (com_mass_out) {
     for (int a = 0; a < 3; ++a) com_mass_out[a] = c.body.com[a];
     com_mass_out[3] = c.body.mass;
   }
   return CUP_OK;
+}
+
+int cup_body_count(cup_ctx* ctx, int* n)
+{
+  CUP_GUARD(ctx);
+  if (!n) return set_error(c, CUP_E_INVALID, "body_count: null pointer");
+  *n = body_count(c);
+  return CUP_OK;
+}
+
+int cup_add_sphere(cup_ctx* ctx, const double center[3], double radius, int hemi, double com_mass_out[4], int* id)
+{
+  CUP_GUARD(ctx);
+  if (!center || !(radius > 0)) return set_error(c, CUP_E_INVALID, "add_sphere: radius must be > 0");
+  int k = 0;
+  if (c.body.present) {
+    c.more.emplace_back();
+    k = (int)c.more.size();
+  }
+  int rc;
+  {
+    BodyScope scope(c, k);
+    rc = body_create(c, center, radius, 0.0, hemi != 0);
+    if (!rc && com_mass_out) {
+      for (int a = 0; a < 3; ++a) com_mass_out[a] = c.body.com[a];
+      com_mass_out[3] = c.body.mass;
+    }
+  }
+  if (rc) {
+    if (k > 0) c.more.pop_back();
+    return rc;
+  }
+  if (id) *id = k;
+  return CUP_OK;
+}
+
+static int body_arg(Ctx& c, int id, const char* what)
+{
+  if (id < 0 || id >= body_count(c)) return set_error(c, CUP_E_INVALID, std::string(what) + ": no such body");
+  return CUP_OK;
+}
+
+int cup_move_body(cup_ctx* ctx, int id, const double center[3], double com_mass_out[4])
+{
+  CUP_GUARD(ctx);
+  if (int rc = body_arg(c, id, "move_body")) return rc;
+  if (!center) return set_error(c, CUP_E_INVALID, "move_body: null pointer");
+  BodyScope scope(c, id);
+  if (c.body.moll > 0) return set_error(c, CUP_E_STATE, "move_body: the mollified sphere is built by its own call");
+  if (int rc = body_create(c, center, c.body.radius, 0.0, c.body.hemi)) return rc;
+  if (com_mass_out) {
+    for (int a = 0; a < 3; ++a) com_mass_out[a] = c.body.com[a];
+    com_mass_out[3] = c.body.mass;
+  }
+  return CUP_OK;
+}
+
+int cup_penalize_body(cup_ctx* ctx, int id, double dt, double lambda, int implicit, const double trans_vel[3],
+                      const double ang_vel[3], double ft_out[6])
+{
+  CUP_GUARD(ctx);
+  if (int rc = body_arg(c, id, "penalize_body")) return rc;
+  BodyScope scope(c, id);
+  return cup_penalize(ctx, dt, lambda, implicit, trans_vel, ang_vel, ft_out);
+}
+
+int cup_surface_forces_body(cup_ctx* ctx, int id, double nu, const double trans_vel[3], const double ang_vel[3],
+                            double out[22])
+{
+  CUP_GUARD(ctx);
+  if (int rc = body_arg(c, id, "surface_forces_body")) return rc;
+  BodyScope scope(c, id);
+  return cup_surface_forces(ctx, nu, trans_vel, ang_vel, out);
+}
+
+int cup_obstacle_momenta_body(cup_ctx* ctx, int id, double dt, double lambda, int implicit, double out[29])
+{
+  CUP_GUARD(ctx);
+  if (int rc = body_arg(c, id, "obstacle_momenta_body")) return rc;
+  BodyScope scope(c, id);
+  return cup_obstacle_momenta(ctx, dt, lambda, implicit, out);
 }
 
 int cup_penalize(cup_ctx* ctx, double dt, double lambda, int implicit, const double trans_vel[3],
@@ -587,6 +681,8 @@ int cup_run_steps(cup_ctx* ctx, const cup_step_params* p, int nsteps)
   if (!p || nsteps < 0) return set_error(c, CUP_E_INVALID, "run_steps: bad argument");
   if (!(p->CFL > 0) && !(p->fixed_dt > 0)) return set_error(c, CUP_E_INVALID, "run_steps: need CFL > 0 or fixed_dt");
   if (p->has_body && !c.body.present) return set_error(c, CUP_E_STATE, "run_steps: has_body without obstacle");
+  if (p->has_body && body_count(c) > 1)
+    return set_error(c, CUP_E_STATE, "run_steps: the device-resident loop takes one body");
   if (c.decomposition == CUP_PLAIN) {
     if (int rc = read_scalars(c)) return rc;
     for (int s = 0; s < nsteps; ++s)
@@ -613,7 +709,7 @@ int cup_run_steps(cup_ctx* ctx, const cup_step_params* p, int nsteps)
       if (int rc = launch_rhs_obstacle(c, nullptr)) return rc;
     if (int rc = launch_poisson_fused(c, force_fused, p->tke_target, p->injection_rate)) return rc;
     stage_mark(c, ST_GRADP, false);
-    if (int rc = launch_gradp_fused(c, p->uinf, body ? c.body.ob_slot : nullptr, force_fused)) return rc;
+    if (int rc = launch_gradp_fused(c, p->uinf, body ? c.ob_slot_all : nullptr, force_fused)) return rc;
     stage_mark(c, ST_GRADP, true);
     if (body)
       if (int rc = launch_penalize_step(c, p->lambda, p->implicit_penalization, p->body_vel, p->body_angvel, p->uinf))
@@ -726,6 +822,7 @@ int cup_adjoint_step_body(cup_ctx* ctx, double dt, double nu, const double uinf[
   CUP_GUARD(ctx);
   if (!uinf || !body_vel || !dJdV || !(dt > 0)) return set_error(c, CUP_E_INVALID, "adjoint_step_body: bad argument");
   if (!c.body.present) return set_error(c, CUP_E_STATE, "adjoint_step_body: no obstacle");
+  if (body_count(c) > 1) return set_error(c, CUP_E_STATE, "adjoint_step_body: one body only");
   if (dJdc && !(c.body.moll > 0))
     return set_error(c, CUP_E_STATE,
                      "adjoint_step_body: the centre gradient needs the mollified chi (cup_chi_from_sphere_mollified)");

@@ -637,17 +637,17 @@ struct XPass {
   static void fwd(Ctx& c, const T* s0, const T* s1, const T* s2, int src, bool force)
   {
     dim3 grid(c.N[1] / BS, c.N[2]);
-    const int* slot = c.body.present && !c.plain_rhs ? c.body.ob_slot : nullptr;
+    const int* slot = c.body.present && !c.plain_rhs ? c.ob_slot_all : nullptr;
     if (src == XSRC_FIELD)
       hipLaunchKernelGGL((k_fft_x_fwd<T, N, XSRC_FIELD, false>), grid, dim3(NT), 0, c.stream, s0, s1, s2,
                          (C2<T>*)c.spec, (const C2<T>*)c.tw[0], c.grid(), c.sc, (const int*)nullptr,
                          (const T*)nullptr, (double*)nullptr);
     else if (force)
       hipLaunchKernelGGL((k_fft_x_fwd<T, N, XSRC_DIV, true>), grid, dim3(NT), 0, c.stream, s0, s1, s2, (C2<T>*)c.spec,
-                         (const C2<T>*)c.tw[0], c.grid(), c.sc, slot, (const T*)c.body.ob_rhs, c.low_part);
+                         (const C2<T>*)c.tw[0], c.grid(), c.sc, slot, (const T*)c.ob_rhs_all, c.low_part);
     else
       hipLaunchKernelGGL((k_fft_x_fwd<T, N, XSRC_DIV, false>), grid, dim3(NT), 0, c.stream, s0, s1, s2,
-                         (C2<T>*)c.spec, (const C2<T>*)c.tw[0], c.grid(), c.sc, slot, (const T*)c.body.ob_rhs,
+                         (C2<T>*)c.spec, (const C2<T>*)c.tw[0], c.grid(), c.sc, slot, (const T*)c.ob_rhs_all,
                          (double*)nullptr);
   }
   static void fwd_to(Ctx& c, const T* s0, C2<T>* spec)

@@ -7,6 +7,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/cup.h"
@@ -130,7 +131,11 @@ struct Ctx {
   void* low_rows = nullptr;    // per grid row (y, z): 16 synthesis coefficients (context precision)
   hipStream_t side = nullptr;  // the forcing row coefficients' stream (default priority)
   hipEvent_t ev_fork = nullptr, ev_rows = nullptr;
-  Body body;
+  Body body;                // the active body: body 0, or body k inside a body-indexed call
+  std::vector<Body> more;   // the other bodies (more[k-1] holds body k while body 0 is active)
+  int* ob_slot_all = nullptr;     // device [nb]: every body's obstacle blocks, slot into ob_rhs_all
+  void* ob_rhs_all = nullptr;     // device: the corrected RHS of every body's blocks (ob_rhs slices)
+  size_t ob_rhs_all_cap = 0;      // bytes
   std::string err;
   // measurement hook: HIP events around one stage of the fused step
   int timing_stage = -1, timing_cap = 0, timing_count = 0;
@@ -233,6 +238,21 @@ int launch_forcing_reduce(Ctx& c, double tke_target, double inj_rate);
 int launch_forcing_rows(Ctx& c);
 // cup_obstacle.hip
 int body_create(Ctx& c, const double center[3], double radius, double moll, bool hemi = false);
+// Body k made the active body for the scope (bodies are swapped, not copied).
+struct BodyScope {
+  Ctx& c;
+  int k;
+  BodyScope(Ctx& ctx, int id) : c(ctx), k(id)
+  {
+    if (k > 0) std::swap(c.body, c.more[k - 1]);
+  }
+  ~BodyScope()
+  {
+    if (k > 0) std::swap(c.body, c.more[k - 1]);
+  }
+};
+inline int body_count(const Ctx& c) { return c.body.present ? 1 + (int)c.more.size() : 0; }
+void bodies_free(Ctx& c);
 void body_free(Ctx& c);
 int launch_penalize(Ctx& c, double lambda, int implicit, const double vel[3], const double omega[3]);
 int launch_penalize_step(Ctx& c, double lambda, int implicit, const double vel[3], const double omega[3],

@@ -175,20 +175,66 @@ void body_free(Ctx& c)
   if (B.ob_chi) hipFree(B.ob_chi);
   if (B.ob_sdf) hipFree(B.ob_sdf);
   if (B.ob_surf) hipFree(B.ob_surf);
-  if (B.ob_rhs) hipFree(B.ob_rhs);
   if (B.ob_touch) hipFree(B.ob_touch);
   if (B.ob_r0) hipFree(B.ob_r0);
   B = Body();
 }
 
+// Every body's buffers and the merged tables.
+void bodies_free(Ctx& c)
+{
+  for (size_t j = 0; j < c.more.size(); ++j) {
+    BodyScope scope(c, (int)j + 1);
+    body_free(c);
+  }
+  c.more.clear();
+  body_free(c);
+  if (c.ob_slot_all) hipFree(c.ob_slot_all);
+  if (c.ob_rhs_all) hipFree(c.ob_rhs_all);
+  c.ob_slot_all = nullptr;
+  c.ob_rhs_all = nullptr;
+  c.ob_rhs_all_cap = 0;
+}
+
+// The merged obstacle-slot table and RHS buffer of all bodies (their blocks are disjoint):
+// body j's blocks take slots [offset_j, offset_j + nob_j), and its ob_rhs is that slice.
+static int bodies_merge(Ctx& c)
+{
+  std::vector<Body*> all;
+  if (c.body.present) all.push_back(&c.body);
+  for (auto& b : c.more)
+    if (b.present) all.push_back(&b);
+  size_t total = 0;
+  for (Body* b : all) total += (size_t)b->nob;
+  const size_t bytes = total * BS3 * c.esz;
+  if (bytes > c.ob_rhs_all_cap) {
+    if (c.ob_rhs_all) hipFree(c.ob_rhs_all);
+    c.ob_rhs_all = nullptr;
+    c.ob_rhs_all_cap = 0;
+    if (check_hip(c, hipMalloc(&c.ob_rhs_all, bytes), "hipMalloc obstacle RHS")) return CUP_E_NOMEM;
+    c.ob_rhs_all_cap = bytes;
+  }
+  if (!c.ob_slot_all && check_hip(c, hipMalloc(&c.ob_slot_all, c.nb * sizeof(int)), "hipMalloc obstacle slots"))
+    return CUP_E_NOMEM;
+  std::vector<int> slot(c.nb, -1);
+  size_t off = 0;
+  for (Body* b : all) {
+    b->ob_rhs = (char*)c.ob_rhs_all + off * BS3 * c.esz;
+    for (int s = 0; s < b->nob; ++s) slot[b->ids_host[s]] = (int)(off + s);
+    off += b->nob;
+  }
+  return check_hip(c, hipMemcpy(c.ob_slot_all, slot.data(), c.nb * sizeof(int), hipMemcpyHostToDevice),
+                   "obstacle slots");
+}
+
 template <typename T>
-static int body_chi_t(Ctx& c)
+static int body_chi_t(Ctx& c, bool fill = true)
 {
   Body& B = c.body;
   const size_t n = c.ncell;
   size_t gsz = (n + NT - 1) / NT;
   if (gsz > 65536) gsz = 65536;
-  hipLaunchKernelGGL((k_fill_t<T>), dim3((unsigned)gsz), dim3(NT), 0, c.stream, (T*)c.f[CUP_CHI], n, (T)0);
+  if (fill) hipLaunchKernelGGL((k_fill_t<T>), dim3((unsigned)gsz), dim3(NT), 0, c.stream, (T*)c.f[CUP_CHI], n, (T)0);
   hipLaunchKernelGGL(k_sphere_sdf, dim3(B.nob), dim3(NT), 0, c.stream, B.ob_ids, B.ob_sdf, c.grid(), B.center[0],
                      B.center[1], B.center[2], B.radius, B.hemi ? 1 : 0);
   hipLaunchKernelGGL((k_sphere_chi<T>), dim3(B.nob), dim3(NT), 0, c.stream, B.ob_ids, B.ob_slot, B.ob_sdf, B.ob_chi,
@@ -226,6 +272,14 @@ int body_create(Ctx& c, const double center[3], double radius, double moll, bool
     body_free(c);
     return set_error(c, CUP_E_INVALID, "chi_from_sphere: the sphere touches no block");
   }
+  // the other bodies' obstacle blocks must be apart from this one's
+  for (const Body& o : c.more)
+    if (o.present)
+      for (int b : ids)
+        if (std::find(o.ids_host.begin(), o.ids_host.end(), b) != o.ids_host.end()) {
+          body_free(c);
+          return set_error(c, CUP_E_INVALID, "add_sphere: the bodies' obstacle blocks overlap (bodies too close)");
+        }
   const bool reuse = B.present && B.radius == radius && B.moll == moll && B.hemi == hemi && B.ids_host == ids;
   if (!reuse) body_free(c);
   B.radius = radius;
@@ -239,7 +293,6 @@ int body_create(Ctx& c, const double center[3], double radius, double moll, bool
       check_hip(c, hipMalloc(&B.ob_chi, nc * sizeof(double)), "hipMalloc") ||
       check_hip(c, hipMalloc(&B.ob_sdf, nc * sizeof(double)), "hipMalloc") ||
       check_hip(c, hipMalloc(&B.ob_surf, nc * 4 * sizeof(double)), "hipMalloc") ||
-      check_hip(c, hipMalloc(&B.ob_rhs, nc * c.esz), "hipMalloc") ||
       check_hip(c, hipMalloc(&B.ob_touch, nc), "hipMalloc"))) {
     body_free(c);
     return CUP_E_NOMEM;
@@ -250,8 +303,17 @@ int body_create(Ctx& c, const double center[3], double radius, double moll, bool
     B.ids_host = ids;
   }
   B.present = true;
+  if (int rc = bodies_merge(c)) return rc;
   if (int rc = ensure_partials(c, 32 * (size_t)B.nob + 2 * (size_t)c.nb)) return rc;
   int rc = c.precision == 4 ? body_chi_t<float>(c) : body_chi_t<double>(c);
+  if (rc) return rc;
+  // the grid chi was cleared: the other bodies write theirs again (their partials are not read)
+  for (size_t j = 0; j < c.more.size() && !rc; ++j) {
+    if (!c.more[j].present) continue;
+    std::swap(c.body, c.more[j]);
+    rc = c.precision == 4 ? body_chi_t<float>(c, false) : body_chi_t<double>(c, false);
+    std::swap(c.body, c.more[j]);
+  }
   if (rc) return rc;
   std::vector<double> part(4 * (size_t)B.nob);
   if (int rc2 = check_hip(c, hipMemcpyAsync(part.data(), c.partials, part.size() * sizeof(double),
@@ -371,10 +433,15 @@ static int rhs_obs_t(Ctx& c, void* rhs_blocks)
   return check_hip(c, hipGetLastError(), "rhs_obstacle");
 }
 
+// PressureRHS's obstacle visitor (PressureRHS.cpp:78-112, 322-343) for every body in turn.
 int launch_rhs_obstacle(Ctx& c, void* rhs_blocks)
 {
-  if (!c.body.present) return CUP_OK;
-  return c.precision == 4 ? rhs_obs_t<float>(c, rhs_blocks) : rhs_obs_t<double>(c, rhs_blocks);
+  const int n = body_count(c);
+  for (int k = 0; k < n; ++k) {
+    BodyScope scope(c, k);
+    if (int rc = c.precision == 4 ? rhs_obs_t<float>(c, rhs_blocks) : rhs_obs_t<double>(c, rhs_blocks)) return rc;
+  }
+  return CUP_OK;
 }
 
 // ------------------------------------------------------------------ penalization

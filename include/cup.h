@@ -134,6 +134,22 @@ int cup_chi_from_hemisphere(cup_ctx* ctx, const double center[3], double radius,
 int cup_chi_from_sphere_mollified(cup_ctx* ctx, const double center[3], double radius, double width,
                                   double com_mass_out[4]);
 
+/* Several bodies (ObstacleVector, ObstacleVector.h:20-80): the calls above and below act on
+ * body 0 (cup_chi_from_sphere and its variants replace every body by one); cup_add_sphere
+ * adds a sphere (hemi: the hemisphere) whose obstacle blocks must be apart from the others'
+ * (CUP_E_INVALID otherwise) and returns its index; cup_move_body builds body id again at a
+ * new centre. PressureRHS masks and corrects for every body; the _body variants penalize,
+ * integrate forces and momenta for one. The device-resident loop (cup_run_steps with a body)
+ * and the adjoint with a body take a single body. */
+int cup_body_count(cup_ctx* ctx, int* n);
+int cup_add_sphere(cup_ctx* ctx, const double center[3], double radius, int hemi, double com_mass_out[4], int* id);
+int cup_move_body(cup_ctx* ctx, int id, const double center[3], double com_mass_out[4]);
+int cup_penalize_body(cup_ctx* ctx, int id, double dt, double lambda, int implicit, const double trans_vel[3],
+                      const double ang_vel[3], double ft_out[6]);
+int cup_surface_forces_body(cup_ctx* ctx, int id, double nu, const double trans_vel[3], const double ang_vel[3],
+                            double out[22]);
+int cup_obstacle_momenta_body(cup_ctx* ctx, int id, double dt, double lambda, int implicit, double out[29]);
+
 /* Penalization::operator() (Penalization.cpp:129-156): u += dt*pf*(U_body - u) with
  * pf = chi*lambda/(1+lambda*dt*chi) (implicit) or chi/dt (explicit), and the force/torque
  * integrals ft_out[6] (KernelFinalizePenalizationForce, :101-124). */
