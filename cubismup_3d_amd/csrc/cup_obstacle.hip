@@ -307,14 +307,6 @@ int body_create(Ctx& c, const double center[3], double radius, double moll, bool
   if (int rc = ensure_partials(c, 32 * (size_t)B.nob + 2 * (size_t)c.nb)) return rc;
   int rc = c.precision == 4 ? body_chi_t<float>(c) : body_chi_t<double>(c);
   if (rc) return rc;
-  // the grid chi was cleared: the other bodies write theirs again (their partials are not read)
-  for (size_t j = 0; j < c.more.size() && !rc; ++j) {
-    if (!c.more[j].present) continue;
-    std::swap(c.body, c.more[j]);
-    rc = c.precision == 4 ? body_chi_t<float>(c, false) : body_chi_t<double>(c, false);
-    std::swap(c.body, c.more[j]);
-  }
-  if (rc) return rc;
   std::vector<double> part(4 * (size_t)B.nob);
   if (int rc2 = check_hip(c, hipMemcpyAsync(part.data(), c.partials, part.size() * sizeof(double),
                                             hipMemcpyDeviceToHost, c.stream), "chi_from_sphere"))
@@ -325,6 +317,15 @@ int body_create(Ctx& c, const double center[3], double radius, double moll, bool
     for (int i = 0; i < 4; ++i) com[i] += part[4 * (size_t)s + i];
   B.mass = com[3];
   for (int a = 0; a < 3; ++a) B.com[a] = com[a] / com[3];
+  // the grid chi was cleared: the other bodies write theirs again (after this body's centre of
+  // mass was read, since they reuse the partials)
+  for (size_t j = 0; j < c.more.size() && !rc; ++j) {
+    if (!c.more[j].present) continue;
+    std::swap(c.body, c.more[j]);
+    rc = c.precision == 4 ? body_chi_t<float>(c, false) : body_chi_t<double>(c, false);
+    std::swap(c.body, c.more[j]);
+  }
+  if (rc) return rc;
   return check_hip(c, hipStreamSynchronize(c.stream), "chi_from_sphere");
 }
 

@@ -262,6 +262,36 @@ class Grid:
                                      L.vec3(ang_vel), L.dptr(ft)))
         return ft
 
+    def add_sphere(self, center, radius, hemi=False):
+        """Another body (its obstacle blocks apart from the others'); returns (index, com, mass)."""
+        out = np.zeros(4)
+        k = C.c_int()
+        self._c(L.lib().cup_add_sphere(self._h, L.vec3(center), float(radius), int(bool(hemi)), L.dptr(out),
+                                       C.byref(k)))
+        return k.value, out[:3].copy(), out[3]
+
+    def body_count(self) -> int:
+        n = C.c_int()
+        self._c(L.lib().cup_body_count(self._h, C.byref(n)))
+        return n.value
+
+    def move_body(self, k, center):
+        out = np.zeros(4)
+        self._c(L.lib().cup_move_body(self._h, int(k), L.vec3(center), L.dptr(out)))
+        return dict(com=out[:3].copy(), mass=out[3])
+
+    def penalize_body(self, k, dt, lam, implicit, trans_vel, ang_vel=(0, 0, 0)) -> np.ndarray:
+        ft = np.zeros(6)
+        self._c(L.lib().cup_penalize_body(self._h, int(k), float(dt), float(lam), int(bool(implicit)),
+                                          L.vec3(trans_vel), L.vec3(ang_vel), L.dptr(ft)))
+        return ft
+
+    def surface_forces_body(self, k, nu, trans_vel, ang_vel=(0, 0, 0)) -> np.ndarray:
+        q = np.zeros(22)
+        self._c(L.lib().cup_surface_forces_body(self._h, int(k), float(nu), L.vec3(trans_vel), L.vec3(ang_vel),
+                                                L.dptr(q)))
+        return q
+
     def obstacle_momenta(self, dt, lam, implicit) -> np.ndarray:
         """UpdateObstacles' 29 momentum sums over the sphere's blocks (ObstaclesUpdate.cpp:15-170)."""
         m = np.zeros(29)

@@ -61,8 +61,20 @@ typedef struct {
   double *dchidx, *dchidy, *dchidz, *delta;
 } OBlock;
 
+/* A body kept aside while another is the sim's active one (orc_select_body swaps them). */
+typedef struct {
+  int has_sphere;
+  double radius, spos[3];
+  double moll;
+  int hemi;
+  OBlock** ob;
+  double CoM[3], mass;
+} OBody;
+
 struct orc_sim {
   int bpd[3], N[3], nb;
+  OBody* more; /* bodies 1.. (more[k-1] holds body k while body 0 is active) */
+  int nmore;
   double h, maxextent, extent[3];
   Block* blk;
   double* rhs; /* solver buffer, global [x][y][z], z fastest (PoissonSolverPeriodic.cpp:112-116) */
@@ -110,8 +122,37 @@ static void free_sphere(orc_sim* s)
   s->has_sphere = 0;
 }
 
+static void body_swap(orc_sim* s, OBody* b)
+{
+  OBody t = {s->has_sphere, s->radius, {s->spos[0], s->spos[1], s->spos[2]}, s->moll, s->hemi, s->ob,
+             {s->CoM[0], s->CoM[1], s->CoM[2]}, s->mass};
+  s->has_sphere = b->has_sphere; s->radius = b->radius; s->moll = b->moll; s->hemi = b->hemi; s->ob = b->ob;
+  s->mass = b->mass;
+  for (int d = 0; d < 3; d++) { s->spos[d] = b->spos[d]; s->CoM[d] = b->CoM[d]; }
+  *b = t;
+}
+
+/* Body k becomes the active one; calling again with the same k swaps it back. */
+void orc_select_body(orc_sim* s, int k)
+{
+  if (k > 0 && k <= s->nmore) body_swap(s, &s->more[k - 1]);
+}
+
+static void free_other_bodies(orc_sim* s)
+{
+  for (int k = 1; k <= s->nmore; k++) {
+    orc_select_body(s, k);
+    free_sphere(s);
+    orc_select_body(s, k);
+  }
+  free(s->more);
+  s->more = NULL;
+  s->nmore = 0;
+}
+
 void orc_destroy(orc_sim* s)
 {
+  free_other_bodies(s);
   if (!s) return;
   free_sphere(s);
   free(s->blk);
@@ -416,14 +457,13 @@ static inline size_t ridx(const orc_sim* s, int gx, int gy, int gz)
 
 /* KernelPressureRHS (PressureRHS.cpp:29-113) after the solver reset (:277, memset), then
  * with a sphere the perimeter correction (:322-343, KernelFinalizePerimeters :215-256). */
+static void rhs_obstacle(orc_sim* s);
+int orc_body_count(const orc_sim* s);
+
 void orc_pressure_rhs(orc_sim* s, double dt)
 {
   memset(s->rhs, 0, sizeof(double) * (size_t)s->N[0] * s->N[1] * s->N[2]);
   const double h = s->h, fac = 0.5 * h * h / dt;
-  unsigned char* touch = NULL;
-  if (s->has_sphere) touch = (unsigned char*)calloc((size_t)s->N[0] * s->N[1] * s->N[2], 1);
-  double sumRHS = 0, posRHS = 0, negRHS = 0;
-  double* part = (double*)calloc(3 * (size_t)s->nb, sizeof(double));  /* per block, summed in block order */
   #pragma omp parallel
   {
     Lab l = lab_alloc(1);
@@ -441,7 +481,35 @@ void orc_pressure_rhs(orc_sim* s, double dt)
             s->rhs[ridx(s, bc[0] * BS + ix, bc[1] * BS + iy, bc[2] * BS + iz)] =
                 fac * (LE->u - LW->u + LN->v - LS->v + LB->w - LF->w);
           }
-      if (!s->has_sphere || s->ob[b] == NULL) continue;
+    }
+    free(l.c);
+  }
+  /* the obstacles' masking and perimeter correction, body by body (their blocks are apart,
+   * so this is the reference's all-masks-then-all-corrections order) */
+  const int nbody = orc_body_count(s);
+  for (int k = 0; k < nbody; k++) {
+    orc_select_body(s, k);
+    rhs_obstacle(s);
+    orc_select_body(s, k);
+  }
+}
+
+/* The active body's part of PressureRHS: rhs -= chi rhs where the grid chi does not exceed the
+ * body's (PressureRHS.cpp:78-112), then the perimeter correction (:322-343, :215-256). */
+static void rhs_obstacle(orc_sim* s)
+{
+  unsigned char* touch = (unsigned char*)calloc((size_t)s->N[0] * s->N[1] * s->N[2], 1);
+  double sumRHS = 0, posRHS = 0, negRHS = 0;
+  double* part = (double*)calloc(3 * (size_t)s->nb, sizeof(double));  /* per block, summed in block order */
+  #pragma omp parallel
+  {
+    Lab l = lab_alloc(1);
+    #pragma omp for schedule(static)
+    for (int b = 0; b < s->nb; b++) {
+      if (s->ob[b] == NULL) continue;
+      lab_load(s, b, &l);
+      int bc[3];
+      bcoords(s, b, bc);
       const OBlock* o = s->ob[b];
       double sumRHS = 0, posRHS = 0, negRHS = 0;
       for (int iz = 0; iz < BS; iz++)
@@ -473,7 +541,6 @@ void orc_pressure_rhs(orc_sim* s, double dt)
     negRHS += part[3 * b + 2];
   }
   free(part);
-  if (!s->has_sphere) return;
   const double corrDenom = sumRHS > 0 ? posRHS : negRHS;
   const double corr = sumRHS / (corrDenom > EPS ? corrDenom : EPS);
   #pragma omp parallel for schedule(static)
@@ -1161,18 +1228,50 @@ static void moll_dchi_dc(const orc_sim* s, const double p[3], double g[3])
 }
 
 static void sphere_build(orc_sim* s, double radius, const double pos[3]);
+static void sphere_build_blocks(orc_sim* s, double radius, const double pos[3], int reset);
 
 void orc_sphere_create(orc_sim* s, double radius, const double pos[3])
 {
+  free_other_bodies(s);
   s->moll = 0;
   s->hemi = 0;
   sphere_build(s, radius, pos);
 }
 
+/* ObstacleVector with several spheres (ObstaclesCreate.cpp:444-480 for each): the grid chi
+ * and SDF field are reset once and every body writes its blocks (which must be apart, as the
+ * device requires); the new body takes index nmore + 1 (0 when it is the first). */
+int orc_add_sphere(orc_sim* s, double radius, const double pos[3], int hemi)
+{
+  if (!s->has_sphere) {
+    s->moll = 0;
+    s->hemi = hemi;
+    sphere_build(s, radius, pos);
+    return 0;
+  }
+  s->more = (OBody*)realloc(s->more, (size_t)(s->nmore + 1) * sizeof(OBody));
+  OBody nb = {0, radius, {pos[0], pos[1], pos[2]}, 0.0, hemi, NULL, {0, 0, 0}, 0};
+  s->more[s->nmore++] = nb;
+  for (int b = 0; b < s->nb; b++) {
+    FE* e = &s->blk[b].d[0][0][0];
+    for (int i = 0; i < BS * BS * BS; i++) { e[i].chi = 0; e[i].tmpU = -1; }
+  }
+  for (int k = 0; k <= s->nmore; k++) {
+    orc_select_body(s, k);
+    const double r = s->radius, p[3] = {s->spos[0], s->spos[1], s->spos[2]};
+    sphere_build_blocks(s, r, p, 0);
+    orc_select_body(s, k);
+  }
+  return s->nmore;
+}
+
+int orc_body_count(const orc_sim* s) { return s->has_sphere ? 1 + s->nmore : 0; }
+
 /* HemiSphereObstacle::FillBlocks (Sphere.cpp:53-84): SDF min(-(x - c_x), R - |x - c|) and the
  * block box [c_x - R - safety, c_x + safety] in x; chi by Towers' formula. */
 void orc_hemisphere_create(orc_sim* s, double radius, const double pos[3])
 {
+  free_other_bodies(s);
   s->moll = 0;
   s->hemi = 1;
   sphere_build(s, radius, pos);
@@ -1183,12 +1282,15 @@ void orc_hemisphere_create(orc_sim* s, double radius, const double pos[3])
  * as orc_sphere_create. */
 void orc_sphere_create_mollified(orc_sim* s, double radius, const double pos[3], double width)
 {
+  free_other_bodies(s);
   s->moll = width;
   s->hemi = 0;
   sphere_build(s, radius, pos);
 }
 
-static void sphere_build(orc_sim* s, double radius, const double pos[3])
+static void sphere_build(orc_sim* s, double radius, const double pos[3]) { sphere_build_blocks(s, radius, pos, 1); }
+
+static void sphere_build_blocks(orc_sim* s, double radius, const double pos[3], int reset)
 {
   free_sphere(s);
   s->has_sphere = 1;
@@ -1202,10 +1304,12 @@ static void sphere_build(orc_sim* s, double radius, const double pos[3])
     box[d][1] = pos[d] + (radius + safety);
   }
   if (s->hemi) box[0][1] = pos[0] + safety;
-  #pragma omp parallel for schedule(static)
-  for (int b = 0; b < s->nb; b++) {
-    FE* e = &s->blk[b].d[0][0][0];
-    for (int i = 0; i < BS * BS * BS; i++) { e[i].chi = 0; e[i].tmpU = -1; }
+  if (reset) {
+    #pragma omp parallel for schedule(static)
+    for (int b = 0; b < s->nb; b++) {
+      FE* e = &s->blk[b].d[0][0][0];
+      for (int i = 0; i < BS * BS * BS; i++) { e[i].chi = 0; e[i].tmpU = -1; }
+    }
   }
   #pragma omp parallel for schedule(static)
   for (int b = 0; b < s->nb; b++) {

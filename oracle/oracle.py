@@ -74,6 +74,9 @@ def lib():
         "orc_surface_forces": (None, [p, d, dp, dp, dp]),
         "orc_obstacle_momenta": (None, [p, d, d, i, dp]),
         "orc_hemisphere_create": (None, [p, d, dp]),
+        "orc_add_sphere": (i, [p, d, dp, i]),
+        "orc_body_count": (i, [p]),
+        "orc_select_body": (None, [p, i]),
         "orc_fft3d": (None, [i, i, i, dp, dp, i]),
         "orc_philox": (None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     }
@@ -316,6 +319,27 @@ class OracleSim:
             lib().orc_sphere_create_mollified(self._h, float(radius), _dp(_vec(pos)), float(mollify))
         else:
             lib().orc_sphere_create(self._h, float(radius), _dp(_vec(pos)))
+
+    def add_sphere(self, radius, pos, hemi=False):
+        """Another sphere (its obstacle blocks apart from the others'); returns its index."""
+        return lib().orc_add_sphere(self._h, float(radius), _dp(_vec(pos)), int(bool(hemi)))
+
+    def body_count(self):
+        return lib().orc_body_count(self._h)
+
+    def body(self, k):
+        """Context manager: body k is the active body inside (penalize, surface_forces,
+        obstacle_momenta and sphere_info act on the active body)."""
+        import contextlib
+
+        @contextlib.contextmanager
+        def scope():
+            lib().orc_select_body(self._h, int(k))
+            try:
+                yield self
+            finally:
+                lib().orc_select_body(self._h, int(k))
+        return scope()
 
     def sphere_info(self):
         o = np.zeros(8)
