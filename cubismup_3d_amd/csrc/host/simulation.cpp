@@ -522,57 +522,85 @@ void Sphere::computeVelocities(const double M[29], bool implicit, double time)
   }
 }
 
-// CreateObstacles (ObstaclesCreate.cpp:444-540) for one sphere: the frame velocity
-// (ObstacleVector::updateUinf, ObstacleVector.h:60-74), the body's update with the step's dt
-// (Obstacle::update), then chi, surface points and centre of mass at the new position. A
-// body that has not moved keeps its chi.
+// CreateObstacles (ObstaclesCreate.cpp:444-540) for the spheres: the frame velocity
+// (ObstacleVector::updateUinf, ObstacleVector.h:60-74: minus the mean velocity of the bodies
+// fixed to the frame, per axis), each body's update with the step's dt (Obstacle::update), then
+// chi, surface points and centre of mass where a body moved. Body k of the device is
+// sim.obstacles[k].
 void CreateObstacles::operator()(double dt)
 {
   if (sim.obstacles.empty()) return;
-  auto& s = *sim.obstacles[0];
   std::array<double, 3> u = {0, 0, 0};
+  int nfix[3] = {0, 0, 0};
+  for (const auto& sp : sim.obstacles)
+    for (int d = 0; d < 3; ++d)
+      if (sp->bFixFrameOfRef[d]) {
+        ++nfix[d];
+        u[d] -= sp->transVel[d];
+      }
   for (int d = 0; d < 3; ++d)
-    if (s.bFixFrameOfRef[d]) u[d] = -s.transVel[d];
+    if (nfix[d]) u[d] /= nfix[d];
   sim.uinf = u;
-  const auto before = s.position;
-  s.update(dt, sim.uinf);
-  if (dt > 0 && !sim.muteAll) {  // Obstacle::_writeComputedVelToFile (Obstacle.cpp:491-515)
-    std::stringstream& f = sim.logger->get_stream("computedVelocity_" + std::to_string(s.obstacleID) + ".dat");
-    const std::string tab("\t");
-    if (sim.step == 0 && !velHeader) {
-      velHeader = true;
-      f << "step" << tab << "time" << tab << "CMx" << tab << "CMy" << tab << "CMz" << tab << "quat_0" << tab
-        << "quat_1" << tab << "quat_2" << tab << "quat_3" << tab << "vel_x" << tab << "vel_y" << tab << "vel_z" << tab
-        << "angvel_x" << tab << "angvel_y" << tab << "angvel_z" << tab << "mass" << tab << "J0" << tab << "J1" << tab
-        << "J2" << tab << "J3" << tab << "J4" << tab << "J5" << std::endl;
+  velHeader.resize(sim.obstacles.size(), 0);
+  std::vector<int> moved;
+  for (size_t k = 0; k < sim.obstacles.size(); ++k) {
+    auto& s = *sim.obstacles[k];
+    const auto before = s.position;
+    s.update(dt, sim.uinf);
+    if (s.position != before) moved.push_back((int)k);
+    if (dt > 0 && !sim.muteAll) {  // Obstacle::_writeComputedVelToFile (Obstacle.cpp:491-515)
+      std::stringstream& f = sim.logger->get_stream("computedVelocity_" + std::to_string(s.obstacleID) + ".dat");
+      const std::string tab("\t");
+      if (sim.step == 0 && !velHeader[k]) {
+        velHeader[k] = 1;
+        f << "step" << tab << "time" << tab << "CMx" << tab << "CMy" << tab << "CMz" << tab << "quat_0" << tab
+          << "quat_1" << tab << "quat_2" << tab << "quat_3" << tab << "vel_x" << tab << "vel_y" << tab << "vel_z"
+          << tab << "angvel_x" << tab << "angvel_y" << tab << "angvel_z" << tab << "mass" << tab << "J0" << tab
+          << "J1" << tab << "J2" << tab << "J3" << tab << "J4" << tab << "J5" << std::endl;
+      }
+      f << sim.step << tab;
+      f.setf(std::ios::scientific);
+      f.precision(std::numeric_limits<float>::digits10 + 1);
+      f << sim.time;
+      for (double v : s.position) f << tab << v;
+      for (double v : s.quaternion) f << tab << v;
+      for (double v : s.transVel) f << tab << v;
+      for (double v : s.angVel) f << tab << v;
+      f << tab << s.mass;
+      for (double v : s.J) f << tab << v;
+      f << std::endl;
     }
-    f << sim.step << tab;
-    f.setf(std::ios::scientific);
-    f.precision(std::numeric_limits<float>::digits10 + 1);
-    f << sim.time;
-    for (double v : s.position) f << tab << v;
-    for (double v : s.quaternion) f << tab << v;
-    for (double v : s.transVel) f << tab << v;
-    for (double v : s.angVel) f << tab << v;
-    f << tab << s.mass;
-    for (double v : s.J) f << tab << v;
-    f << std::endl;
   }
-  if (!created || s.position != before) build();
+  if (!created) {
+    build();
+    return;
+  }
+  for (int k : moved) {
+    auto& s = *sim.obstacles[k];
+    double cm[4];
+    check(cup_move_body(sim.ctx, k, s.position.data(), cm), "CreateObstacles");
+    s.centerOfMass = {cm[0], cm[1], cm[2]};
+    s.mass = cm[3];
+  }
 }
 
-// chi, surface points and centre of mass of the body where it is (set-up, and after an IC)
+// chi, surface points and centre of mass of every body where it is (set-up, and after an IC)
 void CreateObstacles::build()
 {
-  if (sim.obstacles.empty()) return;
-  if (sim.obstacles.size() > 1) throw std::invalid_argument("one obstacle at most");
-  auto& s = *sim.obstacles[0];
-  double cm[4];
-  check(s.bHemi ? cup_chi_from_hemisphere(sim.ctx, s.position.data(), s.radius, cm)
-                : cup_chi_from_sphere(sim.ctx, s.position.data(), s.radius, cm),
-        "CreateObstacles");
-  s.centerOfMass = {cm[0], cm[1], cm[2]};
-  s.mass = cm[3];
+  for (size_t k = 0; k < sim.obstacles.size(); ++k) {
+    auto& s = *sim.obstacles[k];
+    double cm[4];
+    if (k == 0) {  // replaces every body of the device
+      check(s.bHemi ? cup_chi_from_hemisphere(sim.ctx, s.position.data(), s.radius, cm)
+                    : cup_chi_from_sphere(sim.ctx, s.position.data(), s.radius, cm),
+            "CreateObstacles");
+    } else {
+      int id = -1;
+      check(cup_add_sphere(sim.ctx, s.position.data(), s.radius, s.bHemi ? 1 : 0, cm, &id), "CreateObstacles");
+    }
+    s.centerOfMass = {cm[0], cm[1], cm[2]};
+    s.mass = cm[3];
+  }
   created = true;
 }
 
@@ -629,34 +657,42 @@ void PressureProjection::operator()(double dt)
 // (every axis forced, rotation blocked), so without log files its step skips the integrals.
 void UpdateObstacles::operator()(double dt)
 {
-  if (sim.obstacles.empty()) return;
-  auto& s = *sim.obstacles[0];
-  if (s.isStatic() && sim.muteAll) return;
-  double M[29];
-  check(cup_obstacle_momenta(sim.ctx, dt, sim.lambda, sim.bImplicitPenalization ? 1 : 0, M), "UpdateObstacles");
-  s.computeVelocities(M, sim.bImplicitPenalization, sim.time);
+  for (size_t k = 0; k < sim.obstacles.size(); ++k) {
+    auto& s = *sim.obstacles[k];
+    if (s.isStatic() && sim.muteAll) continue;
+    double M[29];
+    check(cup_obstacle_momenta_body(sim.ctx, (int)k, dt, sim.lambda, sim.bImplicitPenalization ? 1 : 0, M),
+          "UpdateObstacles");
+    s.computeVelocities(M, sim.bImplicitPenalization, sim.time);
+  }
 }
 
+// Penalization::operator() (Penalization.cpp:129-156), body by body.
 void Penalization::operator()(double dt)
 {
-  if (sim.obstacles.empty()) return;
-  auto& s = *sim.obstacles[0];
-  double ft[6];
-  check(cup_penalize(sim.ctx, dt, sim.lambda, sim.bImplicitPenalization ? 1 : 0, s.transVel.data(),
-                     s.angVel.data(), ft),
-        "Penalization");
-  s.force = {ft[0], ft[1], ft[2]};
-  s.torque = {ft[3], ft[4], ft[5]};
+  for (size_t k = 0; k < sim.obstacles.size(); ++k) {
+    auto& s = *sim.obstacles[k];
+    double ft[6];
+    check(cup_penalize_body(sim.ctx, (int)k, dt, sim.lambda, sim.bImplicitPenalization ? 1 : 0, s.transVel.data(),
+                            s.angVel.data(), ft),
+          "Penalization");
+    s.force = {ft[0], ft[1], ft[2]};
+    s.torque = {ft[3], ft[4], ft[5]};
+  }
 }
 
 // ComputeForces (FluidSolidForces.cpp:205-223) and Obstacle::_writeSurfForcesToFile
 // (Obstacle.cpp:518-545).
 void ComputeForces::operator()(double)
 {
-  if (sim.obstacles.empty()) return;
-  auto& s = *sim.obstacles[0];
+  for (size_t k = 0; k < sim.obstacles.size(); ++k) write((int)k);
+}
+
+void ComputeForces::write(int k)
+{
+  auto& s = *sim.obstacles[k];
   double q[22];
-  check(cup_surface_forces(sim.ctx, sim.nu, s.transVel.data(), s.angVel.data(), q), "ComputeForces");
+  check(cup_surface_forces_body(sim.ctx, k, sim.nu, s.transVel.data(), s.angVel.data(), q), "ComputeForces");
   for (int i = 0; i < 22; ++i) s.surfaceQoI[i] = q[i];
   if (sim.muteAll) return;
   std::stringstream& ssF = sim.logger->get_stream("forceValues_surface_" + std::to_string(s.obstacleID) + ".dat");
@@ -1066,7 +1102,7 @@ bool Simulation::advance(int n)
 {
   // the device-resident loop holds a body still and has no uniform forcing: those steps take
   // the pipeline (it writes no per-step files either)
-  bool moving = false;
+  bool moving = sim.obstacles.size() > 1;  // the device-resident loop takes one body
   for (const auto& s : sim.obstacles) moving = moving || !s->isStatic();
   const bool external = sim.uMax_forced > 0 && sim.initCond != "taylorGreen";
   if (sim.decomposition != CUP_FUSED || moving || external) {

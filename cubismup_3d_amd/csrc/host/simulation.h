@@ -216,7 +216,7 @@ class CreateObstacles : public Operator
   void build();
   std::string getName() override { return "CreateObstacles"; }
   bool created = false;
-  bool velHeader = false;
+  std::vector<char> velHeader;  // per body: computedVelocity's header written
 };
 
 class AdvectionDiffusion : public Operator
@@ -283,6 +283,7 @@ class ComputeForces : public Operator
  public:
   using Operator::Operator;
   void operator()(double dt) override;
+  void write(int k);  // body k's surface forces and its files
   std::string getName() override { return "ComputeForces"; }
 };
 

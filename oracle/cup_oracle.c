@@ -1241,6 +1241,20 @@ void orc_sphere_create(orc_sim* s, double radius, const double pos[3])
 /* ObstacleVector with several spheres (ObstaclesCreate.cpp:444-480 for each): the grid chi
  * and SDF field are reset once and every body writes its blocks (which must be apart, as the
  * device requires); the new body takes index nmore + 1 (0 when it is the first). */
+static void bodies_rebuild(orc_sim* s)
+{
+  for (int b = 0; b < s->nb; b++) {
+    FE* e = &s->blk[b].d[0][0][0];
+    for (int i = 0; i < BS * BS * BS; i++) { e[i].chi = 0; e[i].tmpU = -1; }
+  }
+  for (int k = 0; k <= s->nmore; k++) {
+    orc_select_body(s, k);
+    const double r = s->radius, p[3] = {s->spos[0], s->spos[1], s->spos[2]};
+    sphere_build_blocks(s, r, p, 0);
+    orc_select_body(s, k);
+  }
+}
+
 int orc_add_sphere(orc_sim* s, double radius, const double pos[3], int hemi)
 {
   if (!s->has_sphere) {
@@ -1252,17 +1266,19 @@ int orc_add_sphere(orc_sim* s, double radius, const double pos[3], int hemi)
   s->more = (OBody*)realloc(s->more, (size_t)(s->nmore + 1) * sizeof(OBody));
   OBody nb = {0, radius, {pos[0], pos[1], pos[2]}, 0.0, hemi, NULL, {0, 0, 0}, 0};
   s->more[s->nmore++] = nb;
-  for (int b = 0; b < s->nb; b++) {
-    FE* e = &s->blk[b].d[0][0][0];
-    for (int i = 0; i < BS * BS * BS; i++) { e[i].chi = 0; e[i].tmpU = -1; }
-  }
-  for (int k = 0; k <= s->nmore; k++) {
-    orc_select_body(s, k);
-    const double r = s->radius, p[3] = {s->spos[0], s->spos[1], s->spos[2]};
-    sphere_build_blocks(s, r, p, 0);
-    orc_select_body(s, k);
-  }
+  bodies_rebuild(s);
   return s->nmore;
+}
+
+/* Body k at a new centre: CreateObstacles resets chi and the SDF field and every body fills
+ * its blocks again (ObstaclesCreate.cpp:449-466). Returns -1 for no such body. */
+int orc_move_body(orc_sim* s, int k, const double pos[3])
+{
+  if (k < 0 || k >= orc_body_count(s)) return -1;
+  double* c = k == 0 ? s->spos : s->more[k - 1].spos;
+  for (int d = 0; d < 3; d++) c[d] = pos[d];
+  bodies_rebuild(s);
+  return 0;
 }
 
 int orc_body_count(const orc_sim* s) { return s->has_sphere ? 1 + s->nmore : 0; }

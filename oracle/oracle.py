@@ -76,6 +76,7 @@ def lib():
         "orc_hemisphere_create": (None, [p, d, dp]),
         "orc_add_sphere": (i, [p, d, dp, i]),
         "orc_body_count": (i, [p]),
+        "orc_move_body": (i, [p, i, dp]),
         "orc_select_body": (None, [p, i]),
         "orc_fft3d": (None, [i, i, i, dp, dp, i]),
         "orc_philox": (None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
@@ -327,6 +328,11 @@ class OracleSim:
     def body_count(self):
         return lib().orc_body_count(self._h)
 
+    def move_body(self, k, pos):
+        """Body k at centre pos; chi and the SDF of every body are built again."""
+        if lib().orc_move_body(self._h, int(k), _dp(_vec(pos))) != 0:
+            raise IndexError(f"no body {k}")
+
     def body(self, k):
         """Context manager: body k is the active body inside (penalize, surface_forces,
         obstacle_momenta and sphere_info act on the active body)."""
@@ -384,6 +390,39 @@ class OracleSim:
         body["transVel"], body["angVel"] = V, Om
         ft = self.penalize(dt, lambda_, implicit, V, Om)
         q = self.surface_forces(nu, V, Om)
+        return uinf, ft, q
+
+    def bodies_timestep(self, dt, nu, bodies, lambda_, implicit):
+        """body_timestep for the bodies added with add_sphere (bodies[k] is body k): the frame
+        velocity is minus the mean velocity of the bodies fixed to the frame on each axis
+        (ObstacleVector::updateUinf, ObstacleVector.h:60-74), every body moves and is built
+        again where it moved, then one step; the momenta of every body are taken before any
+        is penalized (UpdateObstacles, then Penalization). Returns (U_inf, [force/torque of
+        each body], [surface QoIs of each body])."""
+        uinf = [0.0] * 3
+        for d in range(3):
+            fixed = [b["transVel"][d] for b in bodies if b["fix"][d]]
+            uinf[d] = -sum(fixed) / len(fixed) if fixed else 0.0
+        for k, b in enumerate(bodies):
+            new = [b["position"][d] + dt * (b["transVel"][d] + uinf[d]) for d in range(3)]
+            if new != list(b["position"]):
+                self.move_body(k, new)
+            b["position"] = new
+        self.advect_diffuse(dt, nu, uinf)
+        self.pressure_rhs(dt)
+        self.poisson_solve()
+        self.grad_p(dt)
+        for k, b in enumerate(bodies):
+            with self.body(k):
+                M = self.obstacle_momenta(dt, lambda_, implicit)
+            b["transVel"], b["angVel"] = compute_velocities(M, implicit, b["forced"], b["imposed"], b["block_rot"])
+        ft, q = [], []
+        for k, b in enumerate(bodies):
+            with self.body(k):
+                ft.append(self.penalize(dt, lambda_, implicit, b["transVel"], b["angVel"]))
+        for k, b in enumerate(bodies):
+            with self.body(k):
+                q.append(self.surface_forces(nu, b["transVel"], b["angVel"]))
         return uinf, ft, q
 
     def timestep(self, dt, nu, uinf=(0, 0, 0), spectral_forcing=False, tke_target=0.0,

@@ -98,3 +98,50 @@ def test_overlapping_bodies_are_rejected(cup):
     with pytest.raises(cup.CupError):  # the device-resident loop takes one body
         G.add_sphere((1.5, 0.5, 0.5), 0.1)
         G.run_steps(1, nu=1e-3, fixed_dt=1e-3, sphere=dict(lambda_=1e6, implicit=True, trans_vel=(0, 0, 0)))
+
+
+PERIODIC = ["-BC_x", "periodic", "-BC_y", "periodic", "-BC_z", "periodic", "-muteAll", "1",
+            "-freqDiagnostics", "0", "-precision", "double"]
+
+
+@pytest.mark.parametrize("decomp", ["plain", "fused"])
+def test_two_spheres_in_the_simulation(cup, oracle, decomp):
+    """The drop-in Simulation with two forced spheres (the second a hemisphere) in a
+    Taylor-Green flow, both fixed to the frame on x: U_inf is minus their mean x velocity
+    (ObstacleVector.h:60-74), so each moves relative to the grid and is built again every
+    step; 4 CFL steps against the oracle's pipeline with both bodies."""
+    print(f"two spheres in the Simulation decomp={decomp}", flush=True)
+    nu, CFL, n, lam = 1e-3, 0.3, 4, 1e6
+    argv = ["-bpdx", str(BPD[0]), "-bpdy", str(BPD[1]), "-bpdz", str(BPD[2]), "-extentx", str(EXT), "-CFL",
+            str(CFL), "-nu", str(nu), "-nsteps", str(n), "-rampup", "0", "-lambda", str(lam),
+            "-implicitPenalization", "1", "-decomposition", decomp, "-initCond", "taylorGreen",
+            "-uMax_forced", "1.0"] + PERIODIC
+    vel = [(-1.0, 0.0, 0.0), (-0.5, 0.2, 0.0)]
+    fix = [(True, True, True), (True, False, False)]
+    S = cup.Simulation(cup.SimulationData(argv=argv))
+    for (c, R, hemi), v, f in zip(BODIES, vel, fix):
+        S.add_obstacle(cup.Sphere(radius=R, position=c, enforcedVelocity=v, bForcedInSimFrame=True,
+                                  bFixFrameOfRef=f, bHemi=hemi))
+    S.run()
+    O = oracle.OracleSim(BPD, maxextent=EXT)
+    O.ic_taylor_green(1.0)
+    for c, R, hemi in BODIES:
+        O.add_sphere(R, c, hemi=hemi)
+    bodies = [dict(radius=R, position=list(c), transVel=list(v), angVel=[0.0] * 3, forced=[True] * 3,
+                   fix=list(f), block_rot=[True] * 3, imposed=list(v))
+              for (c, R, _), v, f in zip(BODIES, vel, fix)]
+    uinf = [0.0] * 3  # the first dt sees the flags' frame velocity (0)
+    for step in range(n):
+        dt, _ = O.calc_max_timestep(CFL, nu, uinf, step, 0)
+        uinf, ft, q = O.bodies_timestep(dt, nu, bodies, lam, True)
+    assert uinf == pytest.approx([0.75, 0.0, 0.0])
+    for name in ("u", "v", "w"):
+        assert rel(S.field(name), O.get(name)) < 1e-10, name
+    assert np.abs(S.field("chi") - O.get("chi")).max() < 1e-13
+    for k, b in enumerate(bodies):
+        ob = S.obstacle(k)
+        np.testing.assert_allclose(ob["position"], b["position"], rtol=1e-14)
+        assert np.abs(np.asarray(ob["position"]) - BODIES[k][0]).max() > 0
+        np.testing.assert_allclose(ob["trans_vel"], b["transVel"], rtol=0, atol=0)
+        np.testing.assert_allclose(ob["force"], ft[k][:3], rtol=1e-9, atol=1e-11 * np.abs(ft[k]).max())
+        np.testing.assert_allclose(ob["surface"], q[k], rtol=1e-8, atol=1e-10 * np.abs(q[k]).max())

@@ -53,3 +53,63 @@ def test_forced_and_blocked_degrees_of_freedom(oracle):
     Vs, Os = oracle.compute_velocities(M, False, [False] * 3, [0.0] * 3, [True] * 3)
     np.testing.assert_allclose(Vs, V, rtol=0, atol=1e-12)
     assert not Os.any()
+
+
+def test_moving_one_of_two_bodies_equals_building_them_there(oracle):
+    """orc_move_body rebuilds every body's chi and SDF: the pool after a move equals a pool
+    built with the body at its new centre."""
+    O = oracle.OracleSim((8, 4, 4), maxextent=2.0)
+    O.add_sphere(0.15, (0.55, 0.5, 0.48))
+    O.add_sphere(0.12, (1.45, 0.52, 0.5), hemi=True)
+    chi0 = O.get("chi").copy()
+    O.move_body(1, (1.47, 0.5, 0.5))
+    P = oracle.OracleSim((8, 4, 4), maxextent=2.0)
+    P.add_sphere(0.15, (0.55, 0.5, 0.48))
+    P.add_sphere(0.12, (1.47, 0.5, 0.5), hemi=True)
+    assert np.array_equal(O.get("chi"), P.get("chi")) and not np.array_equal(O.get("chi"), chi0)
+    for k in range(2):
+        with O.body(k):
+            a = O.sphere_info()
+        with P.body(k):
+            b = P.sphere_info()
+        assert np.array_equal(a["com"], b["com"]) and a["mass"] == b["mass"] and a["npoints"] == b["npoints"]
+    with pytest.raises(IndexError):
+        O.move_body(2, (1.0, 0.5, 0.5))
+
+
+def test_two_bodies_superpose_single_bodies(oracle):
+    """Bodies whose obstacle blocks are apart act each on its own blocks: the pool's chi is the
+    sum of the single-body chis, and since the pressure RHS is linear in its per-body
+    corrections and the solve is linear, p(A, B) = p(A) + p(B) - p(no body); each body's
+    penalization force in the pool equals the single-body one."""
+    bpd, ext = (8, 4, 4), 2.0
+    A, B = ((0.55, 0.5, 0.48), 0.15), ((1.45, 0.52, 0.5), 0.12)
+    rng = np.random.default_rng(11)
+    u = tuple(0.3 * rng.standard_normal((32, 32, 64)) for _ in range(3))
+    dt = 1e-3
+
+    def solved(sim):
+        sim.set_velocity(*u)
+        sim.pressure_rhs(dt)
+        sim.poisson_solve()
+        return sim.get("p").copy()
+
+    P = oracle.OracleSim(bpd, maxextent=ext)
+    P.add_sphere(A[1], A[0])
+    P.add_sphere(B[1], B[0], hemi=True)
+    SA = oracle.OracleSim(bpd, maxextent=ext)
+    SA.sphere_create(A[1], A[0])
+    SB = oracle.OracleSim(bpd, maxextent=ext)
+    SB.sphere_create(B[1], B[0], hemi=True)
+    N = oracle.OracleSim(bpd, maxextent=ext)
+    assert np.array_equal(P.get("chi"), SA.get("chi") + SB.get("chi"))
+    pp, pa, pb, p0 = solved(P), solved(SA), solved(SB), solved(N)
+    assert np.abs(pp - (pa + pb - p0)).max() < 1e-12 * np.abs(pp).max()
+    assert np.abs(pa - p0).max() > 1e-3 * np.abs(p0).max()  # the corrections are not negligible
+    for k, S in ((0, SA), (1, SB)):
+        for sim in (P, S):
+            sim.set_velocity(*u)
+        with P.body(k):
+            fp = P.penalize(dt, 1e6, True, (-1.0, 0.2, 0.0))
+        fs = S.penalize(dt, 1e6, True, (-1.0, 0.2, 0.0))
+        assert np.array_equal(fp, fs)
