@@ -284,7 +284,9 @@ void SimulationData::setCells(int nx, int ny, int nz)
 void Operator::check(int rc, const char* what)
 {
   if (rc == CUP_OK) return;
-  throw std::runtime_error(std::string(what) + ": " + cup_last_error(sim.ctx));
+  const std::string m = std::string(what) + ": " + cup_last_error(sim.ctx);
+  if (rc == CUP_E_INVALID) throw std::invalid_argument(m);
+  throw std::runtime_error(m);
 }
 
 // InitialConditions::operator() (InitialConditions.cpp:164-230).
@@ -1012,7 +1014,13 @@ void Simulation::addObstacle(std::shared_ptr<Sphere> s)
 {
   s->obstacleID = (int)sim.obstacles.size();
   sim.obstacles.push_back(std::move(s));
-  setupOperators();
+  try {
+    setupOperators();
+  } catch (...) {  // e.g. obstacle blocks overlapping another body's: the body is not added
+    sim.obstacles.pop_back();
+    setupOperators();
+    throw;
+  }
 }
 
 // Simulation::setupOperators (Simulation.cpp:307-401), periodic, non-iterative pipeline.

@@ -145,3 +145,21 @@ def test_two_spheres_in_the_simulation(cup, oracle, decomp):
         np.testing.assert_allclose(ob["trans_vel"], b["transVel"], rtol=0, atol=0)
         np.testing.assert_allclose(ob["force"], ft[k][:3], rtol=1e-9, atol=1e-11 * np.abs(ft[k]).max())
         np.testing.assert_allclose(ob["surface"], q[k], rtol=1e-8, atol=1e-10 * np.abs(q[k]).max())
+
+
+def test_simulation_rejects_an_overlapping_sphere(cup):
+    """A sphere whose obstacle blocks would overlap another body's is refused with ValueError and
+    leaves the simulation with its bodies as before."""
+    print("Simulation overlapping sphere", flush=True)
+    argv = ["-bpdx", str(BPD[0]), "-bpdy", str(BPD[1]), "-bpdz", str(BPD[2]), "-extentx", str(EXT), "-CFL", "0.3",
+            "-nu", "1e-3", "-nsteps", "2", "-initCond", "zero"] + PERIODIC
+    S = cup.Simulation(cup.SimulationData(argv=argv))
+    S.add_obstacle(cup.Sphere(radius=0.15, position=(0.5, 0.5, 0.5), xvel=1.0))
+    chi = S.field("chi").copy()
+    with pytest.raises(ValueError, match="overlap"):
+        S.add_obstacle(cup.Sphere(radius=0.15, position=(0.7, 0.5, 0.5), xvel=1.0))
+    assert np.array_equal(S.field("chi"), chi)
+    with pytest.raises(ValueError):
+        S.obstacle(1)
+    S.run()
+    assert np.isfinite(S.obstacle(0)["force"]).all()
