@@ -2,7 +2,8 @@
 
 ``rollout(u0, grid, nsteps, nu, ...)`` advances a [3][Z][Y][X] velocity tensor through
 nsteps steps of the device path (central advection with Euler, or the third-order upwind
-scheme with Euler or the midpoint rule; FFT projection; no forcing, no obstacle) and is a
+scheme with Euler or the midpoint rule; optionally the linear forcing, to a target energy or at
+a constant injection rate; FFT projection; no obstacle) and is a
 ``torch.autograd.Function``: its backward is the hand-written adjoint of the step
 (cup_adjoint_step: the transposed linearised advection-diffusion stencil and the
 symmetric projection), run backwards over the steps with
@@ -21,23 +22,26 @@ from .grid import Grid
 class Rollout(torch.autograd.Function):
     @staticmethod
     def forward(ctx, u0, grid: Grid, nsteps: int, nu: float, uinf, dts, every: int, CFL: float, dts_out: list,
-                scheme: int):
-        """dts: fixed timesteps, or None for CFL timesteps (appended to dts_out)."""
+                scheme: int, forcing: tuple):
+        """dts: fixed timesteps, or None for CFL timesteps (appended to dts_out); forcing:
+        (tke_target, injection_rate), both 0 for none."""
         grid.set_torch_velocity(u0.detach())
         ckpts = {0: u0.detach().clone()}
         used = []
+        fkw = dict(spectral_forcing=any(f > 0 for f in forcing), tke_target=forcing[0], injection_rate=forcing[1])
         for n in range(nsteps):
             if dts is None:
-                grid.run_steps(1, CFL=CFL, nu=nu, uinf=uinf, scheme=scheme)
+                grid.run_steps(1, CFL=CFL, nu=nu, uinf=uinf, scheme=scheme, **fkw)
                 used.append(grid.step_state()["last_dt"])
             else:
-                grid.run_steps(1, nu=nu, uinf=uinf, fixed_dt=dts[n], scheme=scheme)
+                grid.run_steps(1, nu=nu, uinf=uinf, fixed_dt=dts[n], scheme=scheme, **fkw)
                 used.append(float(dts[n]))
             if (n + 1) % every == 0 and n + 1 < nsteps:
                 ckpts[n + 1] = grid.get_torch_velocity()
         dts_out.extend(used)
         ctx.grid, ctx.nsteps, ctx.nu, ctx.uinf, ctx.dts, ctx.every = grid, nsteps, nu, uinf, used, every
         ctx.scheme = scheme
+        ctx.forcing, ctx.fkw = forcing, fkw
         ctx.ckpts = ckpts
         ctx.u0_dtype = u0.dtype
         return grid.get_torch_velocity()
@@ -51,25 +55,31 @@ class Rollout(torch.autograd.Function):
             states = [ctx.ckpts[s0]]
             grid.set_torch_velocity(ctx.ckpts[s0])
             for n in range(s0, s1 - 1):
-                grid.run_steps(1, nu=ctx.nu, uinf=ctx.uinf, fixed_dt=ctx.dts[n], scheme=ctx.scheme)
+                grid.run_steps(1, nu=ctx.nu, uinf=ctx.uinf, fixed_dt=ctx.dts[n], scheme=ctx.scheme, **ctx.fkw)
                 states.append(grid.get_torch_velocity())
             for n in reversed(range(s0, s1)):
                 grid.set_torch_velocity(states[n - s0])
                 grid.set_torch_adjoint(lam)
-                grid.adjoint_step(ctx.dts[n], ctx.nu, ctx.uinf, scheme=ctx.scheme)
+                grid.adjoint_step(ctx.dts[n], ctx.nu, ctx.uinf, scheme=ctx.scheme, tke_target=ctx.forcing[0],
+                                  injection_rate=ctx.forcing[1])
                 lam = grid.get_torch_adjoint()
         ctx.ckpts = None
-        return lam.to(ctx.u0_dtype), None, None, None, None, None, None, None, None, None
+        return lam.to(ctx.u0_dtype), None, None, None, None, None, None, None, None, None, None
 
 
 def rollout(u0, grid: Grid, nsteps: int, nu: float, uinf=(0.0, 0.0, 0.0), dts=None, CFL=0.3, every=10,
-            dts_out=None, scheme=0):
+            dts_out=None, scheme=0, tke_target=0.0, injection_rate=0.0):
     """u_N of a rollout, differentiable with respect to u0 ([3][Z][Y][X], cuda). With dts
     None the steps take CFL timesteps (rampup 0), which are appended to dts_out and held
-    fixed by the backward. scheme: 0 central Euler, 1 upwind Euler, 2 upwind midpoint."""
+    fixed by the backward. scheme: 0 central Euler, 1 upwind Euler, 2 upwind midpoint.
+    tke_target or injection_rate > 0: each step applies the linear forcing after the
+    advection (SpectralForcing), differentiated through its energy statistics too."""
     out = dts_out if dts_out is not None else []
+    if tke_target < 0 or injection_rate < 0:
+        raise ValueError("tke_target and injection_rate must be >= 0")
     return Rollout.apply(u0, grid, int(nsteps), float(nu), tuple(float(x) for x in uinf),
-                         None if dts is None else list(dts), int(every), float(CFL), out, int(scheme))
+                         None if dts is None else list(dts), int(every), float(CFL), out, int(scheme),
+                         (float(tke_target), float(injection_rate)))
 
 
 class PenalizedRollout(torch.autograd.Function):

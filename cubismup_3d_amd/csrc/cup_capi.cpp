@@ -66,6 +66,8 @@ static void release(Ctx& c)
   if (c.low_counter) hipFree(c.low_counter);
   if (c.low_dev) hipFree(c.low_dev);
   if (c.low_rows) hipFree(c.low_rows);
+  if (c.adjf) hipFree(c.adjf);
+  c.adjf = nullptr;
   c.low_dev = nullptr;
   c.low_rows = nullptr;
   c.low_part = nullptr; c.low_red = nullptr; c.low_counter = nullptr;
@@ -780,8 +782,16 @@ int cup_set_step_state(cup_ctx* ctx, int step, double time)
 
 int cup_adjoint_step(cup_ctx* ctx, double dt, double nu, const double uinf[3], int scheme)
 {
+  return cup_adjoint_step_forcing(ctx, dt, nu, uinf, scheme, 0.0, 0.0);
+}
+
+int cup_adjoint_step_forcing(cup_ctx* ctx, double dt, double nu, const double uinf[3], int scheme,
+                             double tke_target, double injection_rate)
+{
   CUP_GUARD(ctx);
   if (!uinf || !(dt > 0)) return set_error(c, CUP_E_INVALID, "adjoint_step: dt must be > 0");
+  if (tke_target < 0 || injection_rate < 0)
+    return set_error(c, CUP_E_INVALID, "adjoint_step: tke_target and injection_rate must be >= 0");
   if (scheme != CUP_CENTRAL_EULER && scheme != CUP_UPWIND3_EULER && scheme != CUP_UPWIND3_RK2)
     return set_error(c, CUP_E_INVALID, "adjoint_step: unknown scheme");
   if (c.body.present) return set_error(c, CUP_E_STATE, "adjoint_step: obstacles are not supported");
@@ -800,6 +810,26 @@ int cup_adjoint_step(cup_ctx* ctx, double dt, double nu, const double uinf[3], i
   c.ustar = saved;
   if (rc) return rc;
   if (int e = launch_gradp_inplace(c, CUP_ADJ_U)) return e;
+  if (tke_target > 0 || injection_rate > 0) {
+    // the forcing's transpose at u* = A(u_n); the midpoint form computes u* in place of u_n,
+    // which is kept aside and put back
+    const size_t fb = c.ncell * c.esz;
+    if (!c.adjf && check_hip(c, hipMalloc(&c.adjf, 9 * fb), "hipMalloc forcing adjoint")) return CUP_E_NOMEM;
+    void* keep = nullptr;
+    if (scheme == CUP_UPWIND3_RK2) {
+      keep = (char*)c.adjf + 6 * fb;
+      for (int q = 0; q < 3; ++q)
+        hipMemcpyAsync((char*)keep + q * fb, c.f[CUP_U + q], fb, hipMemcpyDeviceToDevice, c.stream);
+    }
+    int e = launch_advdiff(c, nu, uinf, scheme);
+    const int us0 = c.ustar;
+    c.ustar = saved;
+    if (!e) e = launch_adjoint_forcing(c, us0, dt, tke_target, injection_rate);
+    if (keep)
+      for (int q = 0; q < 3; ++q)
+        hipMemcpyAsync(c.f[CUP_U + q], (char*)keep + q * fb, fb, hipMemcpyDeviceToDevice, c.stream);
+    if (e) return e;
+  }
   // lam_n = A'(u_n)^T lam* into tmp, then tmp and the adjoint fields trade buffers
   if (scheme == CUP_CENTRAL_EULER) {
     if (int e = launch_adjoint_advdiff(c, nu, uinf)) return e;

@@ -129,6 +129,7 @@ struct Ctx {
   double* low_coef = nullptr;  // per mode: RHS addition (re,im), synthesis of u,v,w (re,im) x3
   unsigned* low_counter = nullptr;
   void* low_rows = nullptr;    // per grid row (y, z): 16 synthesis coefficients (context precision)
+  void* adjf = nullptr;        // the forcing adjoint's 9 fields: n P lam, n P u*, u_n kept aside (lazy)
   hipStream_t side = nullptr;  // the forcing row coefficients' stream (default priority)
   hipEvent_t ev_fork = nullptr, ev_rows = nullptr;
   Body body;                // the active body: body 0, or body k inside a body-indexed call
@@ -236,6 +237,8 @@ int forcing_plain(Ctx& c, double dt, double sim_dt, double nu, double tke_target
                   double stats[5]);
 int launch_forcing_reduce(Ctx& c, double tke_target, double inj_rate);
 int launch_forcing_rows(Ctx& c);
+// adjoint fields += the forcing's transpose at u* (fields us0..us0+2); synchronous
+int launch_adjoint_forcing(Ctx& c, int us0, double dt, double tke_target, double inj_rate);
 // cup_obstacle.hip
 int body_create(Ctx& c, const double center[3], double radius, double moll, bool hemi = false);
 // Body k made the active body for the scope (bodies are swapped, not copied).

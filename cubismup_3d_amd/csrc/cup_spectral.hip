@@ -355,6 +355,112 @@ int forcing_plain(Ctx& c, double dt, double sim_dt, double nu, double tke_target
                           : forcing_plain_t<double>(c, dt, sim_dt, nu, tke_target, inj, stats);
 }
 
+// sum over cells of a0 b0 + a1 b1 + a2 b2 into part[blockIdx.x] (fixed grid: deterministic)
+template <typename T>
+__global__ __launch_bounds__(NT) void k_dot3(const T* __restrict__ a0, const T* __restrict__ a1,
+                                             const T* __restrict__ a2, const T* __restrict__ b0,
+                                             const T* __restrict__ b1, const T* __restrict__ b2, size_t n,
+                                             double* __restrict__ part)
+{
+  double acc = 0;
+  for (size_t i = (size_t)blockIdx.x * NT + threadIdx.x; i < n; i += (size_t)gridDim.x * NT)
+    acc += (double)a0[i] * b0[i] + (double)a1[i] * b1[i] + (double)a2[i] * b2[i];
+  acc = wave_sum(acc);
+  __shared__ double red[NT / 64];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0;
+    for (int q = 0; q < NT / 64; ++q) s += red[q];
+    part[blockIdx.x] = s;
+  }
+}
+
+// Transpose of the linear forcing at u* (no reference counterpart). The forward
+// (SpectralForcing.cpp:37-77) is u** = u* + fac P u* for fac > 0, with P the band filter
+// (k_forcing_filter's mask, an orthogonal projector), tke = <|u*|^2>/2, tke_F = <u*, P u*>/2
+// and fac = (tke_target - tke)/(2 tke_F), or dt r/(2 tke_F) at the constant injection rate r.
+// With s = <P u*, lam> over the n cells and three components:
+//   lam += fac P lam + s grad fac,  grad fac = -u*/(2 n tke_F) [target only] - fac P u*/(n tke_F).
+// P u* and P lam come from forward transforms, the filter and inverse transforms (n times P,
+// unnormalized); the switch fac > 0 is held at the forward state.
+template <typename T>
+static int adjoint_forcing_t(Ctx& c, int us0, double dt, double tke_target, double inj_rate)
+{
+  if (int rc = alloc_extra_spectra(c)) return rc;
+  const size_t fb = c.ncell * c.esz;
+  if (!c.adjf && check_hip(c, hipMalloc(&c.adjf, 9 * fb), "hipMalloc forcing adjoint")) return CUP_E_NOMEM;
+  T* X = (T*)c.adjf;                 // n P lam, three fields
+  T* Y = (T*)((char*)c.adjf + 3 * fb);  // n P u*
+  const size_t nh = (size_t)(c.N[0] / 2) * c.N[1] * c.N[2];
+  dim3 g = grid_for(nh);
+  if (g.x > 4096) g.x = 4096;
+  if (int rc = ensure_partials(c, 4 * (size_t)g.x)) return rc;
+  const double wf[3] = {2 * M_PI / c.extent[0], 2 * M_PI / c.extent[1], 2 * M_PI / c.extent[2]};
+  void* specs[3] = {c.spec, c.spec2, c.spec3};
+  auto filtered = [&](int f0) {  // spectra of fields f0.. filtered, band sums in c.partials
+    for (int q = 0; q < 3; ++q)
+      if (int rc = launch_fft_forward_field(c, c.f[f0 + q], specs[q])) return rc;
+    hipLaunchKernelGGL((k_forcing_filter<T>), g, dim3(NT), 0, c.stream, (C2<T>*)c.spec, (C2<T>*)c.spec2,
+                       (C2<T>*)c.spec3, c.N[0], c.N[1], c.N[2], wf[0], wf[1], wf[2], c.partials);
+    return check_hip(c, hipGetLastError(), "k_forcing_filter");
+  };
+  auto inverse = [&](T* dst) {
+    for (int q = 0; q < 3; ++q)
+      if (int rc = launch_fft_inverse_field(c, specs[q], dst + q * c.ncell)) return rc;
+    return CUP_OK;
+  };
+  auto scalars = [&]() {
+    if (int rc = check_hip(c, hipMemcpyAsync(c.sc_host, c.sc, sizeof(DevScalars), hipMemcpyDeviceToHost, c.stream),
+                           "forcing adjoint"))
+      return rc;
+    return check_hip(c, hipStreamSynchronize(c.stream), "forcing adjoint");
+  };
+  // the forward's statistics at u*
+  if (int rc = filtered(us0)) return rc;
+  if (int rc = launch_reduce(c, (int)g.x, 4, 0u, c.sc->red)) return rc;
+  if (int rc = scalars()) return rc;
+  const double n = (double)c.N[0] * c.N[1] * c.N[2];
+  const double tke = c.sc_host->red[0] / (n * n), tkeF = c.sc_host->red[2] / (n * n);
+  const double inj = tke_target > 0 ? (tke_target - tke) / dt : inj_rate;
+  const double fac = dt * inj / (2 * tkeF);
+  if (!(fac > 0)) return CUP_OK;
+  if (int rc = inverse(Y)) return rc;
+  if (int rc = filtered(CUP_ADJ_U)) return rc;
+  if (int rc = inverse(X)) return rc;
+  // s = <P u*, lam> = <Y, lam>/n
+  dim3 gd = grid_for(c.ncell);
+  if (gd.x > 4096) gd.x = 4096;
+  hipLaunchKernelGGL((k_dot3<T>), gd, dim3(NT), 0, c.stream, Y, Y + c.ncell, Y + 2 * c.ncell,
+                     (const T*)c.f[CUP_ADJ_U], (const T*)c.f[CUP_ADJ_V], (const T*)c.f[CUP_ADJ_W], c.ncell,
+                     c.partials);
+  if (int rc = check_hip(c, hipGetLastError(), "k_dot3")) return rc;
+  if (int rc = launch_reduce(c, (int)gd.x, 1, 0u, c.sc->red)) return rc;
+  if (int rc = scalars()) return rc;
+  const double s = c.sc_host->red[0] / n;
+  const double a_x = fac / n;                                            // on n P lam
+  const double a_u = tke_target > 0 ? -s / (2 * n * tkeF) : 0.0;          // on u*
+  const double a_y = -s * fac / (n * tkeF) / n;                          // on n P u*
+  for (int q = 0; q < 3; ++q) {
+    T* lam = (T*)c.f[CUP_ADJ_U + q];
+    hipLaunchKernelGGL((k_axpy_blocks<T>), grid_for(c.ncell), dim3(NT), 0, c.stream, lam,
+                       (const T*)(X + q * c.ncell), c.ncell, a_x);
+    if (a_u != 0)
+      hipLaunchKernelGGL((k_axpy_blocks<T>), grid_for(c.ncell), dim3(NT), 0, c.stream, lam,
+                         (const T*)c.f[us0 + q], c.ncell, a_u);
+    hipLaunchKernelGGL((k_axpy_blocks<T>), grid_for(c.ncell), dim3(NT), 0, c.stream, lam,
+                       (const T*)(Y + q * c.ncell), c.ncell, a_y);
+    if (int rc = check_hip(c, hipGetLastError(), "k_axpy_blocks")) return rc;
+  }
+  return CUP_OK;
+}
+
+int launch_adjoint_forcing(Ctx& c, int us0, double dt, double tke_target, double inj_rate)
+{
+  return c.precision == 4 ? adjoint_forcing_t<float>(c, us0, dt, tke_target, inj_rate)
+                          : adjoint_forcing_t<double>(c, us0, dt, tke_target, inj_rate);
+}
+
 // Low modes of the linear forcing, 0 < |k|^2 <= 4 in physical wavenumbers, canonical
 // member of each conjugate pair (kz > 0, or kz = 0 and ky > 0, or kz = ky = 0 and kx > 0),
 // and their positions in the packed half spectrum (x halved): kx > 0 at (kx, ky, kz);

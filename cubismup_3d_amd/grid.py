@@ -146,9 +146,11 @@ class Grid:
         import torch
         return torch.stack([self.get_torch(n) for n in ("adjU", "adjV", "adjW")])
 
-    def adjoint_step(self, dt, nu, uinf=(0, 0, 0), scheme=L.CUP_CENTRAL_EULER):
-        """adj <- dJ/du_n from adj = dJ/du_{n+1}, with u_n the current velocity."""
-        self._c(L.lib().cup_adjoint_step(self._h, float(dt), float(nu), L.vec3(uinf), int(scheme)))
+    def adjoint_step(self, dt, nu, uinf=(0, 0, 0), scheme=L.CUP_CENTRAL_EULER, tke_target=0.0, injection_rate=0.0):
+        """adj <- dJ/du_n from adj = dJ/du_{n+1}, with u_n the current velocity; with the linear
+        forcing's transpose when tke_target or injection_rate > 0."""
+        self._c(L.lib().cup_adjoint_step_forcing(self._h, float(dt), float(nu), L.vec3(uinf), int(scheme),
+                                                 float(tke_target), float(injection_rate)))
 
     def adjoint_step_body(self, dt, nu, uinf, sphere, wdrag=0.0, centre=False):
         """With the sphere: adj <- dJ/du_n; returns this step's dJ/d(body velocity), J holding

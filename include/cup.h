@@ -245,6 +245,15 @@ int cup_set_step_state(cup_ctx* ctx, int step, double time);
  * p and tmp; the device pointers of tmp and the adjoint fields may change
  * (cup_field_device_ptr again after this call). */
 int cup_adjoint_step(cup_ctx* ctx, double dt, double nu, const double uinf[3], int scheme);
+/* The same for the step with the linear forcing after the advection (SpectralForcing.cpp:37-77,
+ * cup_spectral_forcing: to the energy tke_target when > 0, else at the constant injection
+ * rate injection_rate), no reference counterpart. Between the projection's transpose and the
+ * advection's it applies the forcing's: lam += fac P lam + <P u*, lam> grad fac at u* = A(u_n),
+ * P the forcing's band filter, grad fac = -u* / (2 n tke_F) (target only) - fac P u* / (n tke_F)
+ * (n cells, tke_F = <u*, P u*>/2); nothing when fac <= 0, as the forward. Both zero: exactly
+ * cup_adjoint_step. Synchronous (two scalar read-backs). */
+int cup_adjoint_step_forcing(cup_ctx* ctx, double dt, double nu, const double uinf[3], int scheme,
+                             double tke_target, double injection_rate);
 /* The same with the sphere (forced at body_vel, no rotation; chi from cup_chi_from_sphere),
  * no reference counterpart. The step adds the implicit (or explicit) penalization of
  * Penalization.cpp:40-98 after the projection, whose RHS is masked and corrected on the

@@ -1791,14 +1791,106 @@ static void up3_jt(const orc_sim* s, const double* phi, const double* lam, doubl
  * Euler:    lam_n = lam* + dt L'(u_n)^T lam*;
  * midpoint: u_h = u_n + dt/2 L(u_n), mu = dt L'(u_h)^T lam*, lam_n = lam* + mu + dt/2 L'(u_n)^T mu.
  * scheme 0 is the central Euler step (orc_adjoint_step). */
+static void adj_advection(orc_sim* s, double dt, double nu, const double uinf[3], int scheme, double* lam);
+
 void orc_adjoint_step_scheme(orc_sim* s, double dt, double nu, const double uinf[3], int scheme, double* lam)
 {
   if (scheme == 0) {
     orc_adjoint_step(s, dt, nu, uinf, lam);
     return;
   }
-  const size_t nc = (size_t)s->nb * BS * BS * BS;
   adj_project(s, dt, lam);
+  adj_advection(s, dt, nu, uinf, scheme, lam);
+}
+
+/* out = P in for three block-major components: the band filter of SpectralForcing, keeping
+ * the modes 0 < |k|^2 <= 4 in physical wavenumbers (the forward's filter), an orthogonal
+ * projector on the grid. */
+static void low_project(const orc_sim* s, const double* in, double* out)
+{
+  const int N0 = s->N[0], N1 = s->N[1], N2 = s->N[2];
+  const size_t n = (size_t)N0 * N1 * N2, nc = (size_t)s->nb * BS * BS * BS;
+  const double wf[3] = {2.0 * M_PI / s->extent[0], 2.0 * M_PI / s->extent[1], 2.0 * M_PI / s->extent[2]};
+  double complex* g = (double complex*)malloc(sizeof(double complex) * n);
+  for (int q = 0; q < 3; q++) {
+    for (int gz = 0; gz < N2; gz++)
+      for (int gy = 0; gy < N1; gy++)
+        for (int gx = 0; gx < N0; gx++) g[ridx(s, gx, gy, gz)] = in[q * nc + bm_index(s, gx, gy, gz)];
+    fft3d_c(g, N0, N1, N2, -1);
+    for (int i = 0; i < N0; i++)
+      for (int l = 0; l < N1; l++)
+        for (int k = 0; k < N2; k++) {
+          const double kx = signed_k(i, N0) * wf[0], ky = signed_k(l, N1) * wf[1], kz = signed_k(k, N2) * wf[2];
+          const double k2 = kx * kx + ky * ky + kz * kz;
+          if (!(k2 > 0 && k2 <= 4)) g[((size_t)i * N1 + l) * N2 + k] = 0;
+        }
+    fft3d_c(g, N0, N1, N2, +1);
+    for (int gz = 0; gz < N2; gz++)
+      for (int gy = 0; gy < N1; gy++)
+        for (int gx = 0; gx < N0; gx++) out[q * nc + bm_index(s, gx, gy, gz)] = creal(g[ridx(s, gx, gy, gz)]) / n;
+  }
+  free(g);
+}
+
+/* Adjoint of one step with the linear forcing (no reference counterpart). The forward step
+ * is u* = A(u_n) (the scheme's advection), u** = u* + fac(u*) P u* when fac > 0
+ * (SpectralForcing.cpp:37-77: P the band filter, tke = <|u*|^2>/2, tke_F = <u*, P u*>/2
+ * (Parseval), fac = (tke_target - tke)/(2 tke_F), or dt r/(2 tke_F) for the constant
+ * injection rate r), then the projection. With lam* = P_proj lam and s = <P u*, lam*> summed
+ * over cells and components (n cells):
+ *   lam** = lam* + fac P lam* + s grad fac,
+ *   grad fac = -u* / (2 n tke_F) [target only] - fac P u* / (n tke_F),
+ * then lam_n = A'(u_n)^T lam**. The switch fac > 0 is held at the forward state. */
+void orc_adjoint_step_forcing(orc_sim* s, double dt, double nu, const double uinf[3], int scheme,
+                              double tke_target, double injection_rate, double* lam)
+{
+  if (!(tke_target > 0) && !(injection_rate > 0)) {
+    orc_adjoint_step_scheme(s, dt, nu, uinf, scheme, lam);
+    return;
+  }
+  const size_t nc = (size_t)s->nb * BS * BS * BS;
+  const double n = (double)s->N[0] * s->N[1] * s->N[2];
+  adj_project(s, dt, lam);
+  double* un = (double*)malloc(3 * nc * sizeof(double));
+  double* us = (double*)malloc(3 * nc * sizeof(double));
+  double* Pu = (double*)malloc(3 * nc * sizeof(double));
+  double* Pl = (double*)malloc(3 * nc * sizeof(double));
+  for (int q = 0; q < 3; q++) orc_get_field(s, ORC_U + q, un + q * nc);
+  orc_advect_diffuse_scheme(s, dt, nu, uinf, scheme);
+  for (int q = 0; q < 3; q++) orc_get_field(s, ORC_U + q, us + q * nc);
+  for (int q = 0; q < 3; q++) orc_set_field(s, ORC_U + q, un + q * nc);
+  low_project(s, us, Pu);
+  low_project(s, lam, Pl);
+  double tke = 0, tkeF = 0, sl = 0;
+  for (size_t i = 0; i < 3 * nc; i++) {
+    tke += us[i] * us[i];
+    tkeF += us[i] * Pu[i];
+    sl += Pu[i] * lam[i];
+  }
+  tke *= 0.5 / n;
+  tkeF *= 0.5 / n;
+  const double inj = tke_target > 0 ? (tke_target - tke) / dt : injection_rate;
+  const double fac = dt * inj / (2 * tkeF);
+  if (fac > 0) {
+    const double a_u = tke_target > 0 ? -sl / (2 * n * tkeF) : 0.0;
+    const double a_p = -sl * fac / (n * tkeF);
+    for (size_t i = 0; i < 3 * nc; i++) lam[i] += fac * Pl[i] + a_u * us[i] + a_p * Pu[i];
+  }
+  free(un);
+  free(us);
+  free(Pu);
+  free(Pl);
+  adj_advection(s, dt, nu, uinf, scheme, lam);
+}
+
+/* lam <- A'(u_n)^T lam for the scheme's advection, u_n the sim's velocity. */
+static void adj_advection(orc_sim* s, double dt, double nu, const double uinf[3], int scheme, double* lam)
+{
+  if (scheme == 0) {
+    adj_advect(s, dt, nu, uinf, lam);
+    return;
+  }
+  const size_t nc = (size_t)s->nb * BS * BS * BS;
   double* phi = (double*)malloc(3 * nc * sizeof(double));
   double* t = (double*)malloc(3 * nc * sizeof(double));
   for (int q = 0; q < 3; q++) orc_get_field(s, ORC_U + q, phi + q * nc);

@@ -41,6 +41,8 @@ void orc_select_body(orc_sim* s, int k);
 double orc_spectrum_interp(const double* K, const double* E, int n, double k);
 void orc_ic_hit_spectrum(orc_sim* s, const double* K, const double* E, int n, double u_rms, uint64_t seed);
 void orc_adjoint_step_scheme(orc_sim* s, double dt, double nu, const double uinf[3], int scheme, double* lam);
+void orc_adjoint_step_forcing(orc_sim* s, double dt, double nu, const double uinf[3], int scheme,
+                              double tke_target, double injection_rate, double* lam);
 void orc_get_field(const orc_sim* s, int f, double* dst);
 
 /* Periodic lab test (TestBoundaries.cpp:19-75): tmpU = lab(-dx,-dy,-dz).u for a lab

@@ -54,6 +54,7 @@ def lib():
         "orc_advect_diffuse_scheme": (None, [p, d, d, dp, i]),
         "orc_adjoint_step": (None, [p, d, d, dp, dp]),
         "orc_adjoint_step_scheme": (None, [p, d, d, dp, i, dp]),
+        "orc_adjoint_step_forcing": (None, [p, d, d, dp, i, d, d, dp]),
         "orc_adjoint_step_body": (None, [p, d, d, dp, d, i, dp, d, dp, dp, dp]),
         "orc_pressure_rhs": (None, [p, d]),
         "orc_poisson_solve": (None, [p]),
@@ -233,12 +234,13 @@ class OracleSim:
         """scheme 0: central/Euler (the reference); 1: upwind3/Euler; 2: upwind3/RK2 midpoint."""
         lib().orc_advect_diffuse_scheme(self._h, float(dt), float(nu), _dp(_vec(uinf)), int(scheme))
 
-    def adjoint_step(self, dt, nu, lam, uinf=(0, 0, 0), scheme=0):
+    def adjoint_step(self, dt, nu, lam, uinf=(0, 0, 0), scheme=0, tke_target=0.0, injection_rate=0.0):
         """dJ/du_n from dJ/du_{n+1} = lam (a tuple of three [Z][Y][X] arrays), with the forward
-        state u_n in this sim's velocity; central Euler (scheme 0) or upwind Euler/midpoint (1, 2),
-        no forcing or body."""
+        state u_n in this sim's velocity; central Euler (scheme 0) or upwind Euler/midpoint (1, 2);
+        with the linear forcing after the advection when tke_target or injection_rate > 0; no body."""
         buf = np.concatenate([to_blocks(np.asarray(f, dtype=np.float64)) for f in lam])
-        lib().orc_adjoint_step_scheme(self._h, float(dt), float(nu), _dp(_vec(uinf)), int(scheme), _dp(buf))
+        lib().orc_adjoint_step_forcing(self._h, float(dt), float(nu), _dp(_vec(uinf)), int(scheme),
+                                       float(tke_target), float(injection_rate), _dp(buf))
         n = buf.size // 3
         return tuple(from_blocks(buf[q * n:(q + 1) * n], self.shape) for q in range(3))
 

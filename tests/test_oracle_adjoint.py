@@ -184,3 +184,74 @@ def test_adjoint_centre_is_nan_with_towers_chi(oracle):
     lam = tuple(np.zeros(s.shape) for _ in range(3))
     _, _, dc = s.adjoint_step_body(1e-3, 1e-3, lam, (0, 0, 0), 1e6, True, 1.0, centre=True)
     assert np.isnan(dc).all()
+
+
+def fwd_step_forced(oracle, bpd, u, dt, nu, U, scheme, target, rate):
+    s = oracle.OracleSim(bpd)
+    s.set_velocity(*u)
+    s.advect_diffuse(dt, nu, U, scheme)
+    st = s.spectral_forcing(dt, nu, tke_target=target, injection_rate=rate)
+    s.pressure_rhs(dt)
+    s.poisson_solve()
+    s.grad_p(dt)
+    return s.velocity(), st
+
+
+def low_mode_field(shape, h, rng):
+    """A velocity dominated by modes |k| <= 2 (so the forcing's band holds most of the energy)
+    plus noise on every mode."""
+    nz, ny, nx = shape
+    z, y, x = np.meshgrid(*(h * (np.arange(m) + 0.5) for m in shape), indexing="ij")
+    u = np.sin(x) * np.cos(y) * np.cos(2 * z) + 0.3 * rng.standard_normal(shape)
+    v = -np.cos(x) * np.sin(y) * np.cos(z) + 0.3 * rng.standard_normal(shape)
+    w = 0.5 * np.sin(2 * y + z) + 0.3 * rng.standard_normal(shape)
+    return u, v, w
+
+
+@pytest.mark.parametrize("scheme,bpd,target,rate", [(0, (2, 2, 2), 1.1, 0.0), (1, (2, 2, 2), 0.0, 0.7),
+                                                    (2, (4, 2, 2), 1.1, 0.0), (0, (4, 2, 2), 0.0, 0.7)])
+def test_forcing_adjoint_dot_product_identity(oracle, scheme, bpd, target, rate):
+    """The step with the linear forcing (to a target energy, or at a constant injection rate)
+    is rational in u: the five-point central difference with a small step is its directional
+    derivative to about 1e-12, and the adjoint with the forcing's transpose (the band filter,
+    symmetric, and the derivative of fac through tke and tke_F) must satisfy the dot-product
+    identity. Dropping either term of grad fac, or the factor fac on P lam, fails it."""
+    rng = np.random.default_rng(41)
+    s = oracle.OracleSim(bpd)
+    u = low_mode_field(s.shape, s.h, rng)
+    tke0 = 0.5 * sum(float(np.mean(a * a)) for a in u)
+    target = target * tke0 if target else 0.0
+    d = low_mode_field(s.shape, s.h, rng)  # a direction and a seed with energy in the band
+    lam = tuple(a + 0.5 * b for a, b in zip(low_mode_field(s.shape, s.h, rng), u))
+    # the upwind step's kinks (sign changes of the advecting velocity) stay out of a small step
+    dt, nu, eps, U = 0.3 * s.h, 0.02, 1e-4 if scheme == 0 else 1e-5, (0.3, -0.2, 0.1)
+
+    def f(e):
+        return fwd_step_forced(oracle, bpd, tuple(a + e * b for a, b in zip(u, d)), dt, nu, U, scheme, target,
+                               rate)[0]
+    _, st = fwd_step_forced(oracle, bpd, u, dt, nu, U, scheme, target, rate)
+    assert 1e-3 < st["fac"] < 5  # the forcing acts
+    f1, f2, m1, m2 = f(eps), f(2 * eps), f(-eps), f(-2 * eps)
+    jd = tuple((8 * (a - b) - (c - e)) / (12 * eps) for a, b, c, e in zip(f1, m1, f2, m2))
+    s.set_velocity(*u)
+    lam0 = s.adjoint_step(dt, nu, lam, U, scheme=scheme, tke_target=target, injection_rate=rate)
+    lhs, rhs = dot(jd, lam), dot(d, lam0)
+    tol = 1e-9 if scheme == 0 else 1e-7  # the upwind step: a.e. smooth, as above
+    assert abs(lhs - rhs) < tol * abs(lhs), (lhs, rhs)
+    for a, b in zip(s.velocity(), u):  # the forward state is left as it was
+        assert np.array_equal(a, b)
+    plain = s.adjoint_step(dt, nu, lam, U, scheme=scheme)
+    assert abs(dot(d, plain) - rhs) > 1e-3 * abs(rhs)  # the forcing's transpose is not negligible
+
+
+def test_forcing_adjoint_is_the_plain_one_when_the_forcing_is_off(oracle):
+    """Above the target energy fac <= 0 and the forward forcing does nothing: so does its adjoint."""
+    rng = np.random.default_rng(43)
+    s = oracle.OracleSim((2, 2, 2))
+    u = low_mode_field(s.shape, s.h, rng)
+    lam = tuple(rng.standard_normal(s.shape) for _ in range(3))
+    s.set_velocity(*u)
+    a = s.adjoint_step(0.01, 0.02, lam, tke_target=1e-3)
+    b = s.adjoint_step(0.01, 0.02, lam)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
