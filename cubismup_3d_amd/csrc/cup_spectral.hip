@@ -393,9 +393,10 @@ static int adjoint_forcing_t(Ctx& c, int us0, double dt, double tke_target, doub
   T* X = (T*)c.adjf;                 // n P lam, three fields
   T* Y = (T*)((char*)c.adjf + 3 * fb);  // n P u*
   const size_t nh = (size_t)(c.N[0] / 2) * c.N[1] * c.N[2];
-  dim3 g = grid_for(nh);
+  dim3 g = grid_for(nh), gd = grid_for(c.ncell);
   if (g.x > 4096) g.x = 4096;
-  if (int rc = ensure_partials(c, 4 * (size_t)g.x)) return rc;
+  if (gd.x > 4096) gd.x = 4096;
+  if (int rc = ensure_partials(c, std::max(4 * (size_t)g.x, (size_t)gd.x))) return rc;
   const double wf[3] = {2 * M_PI / c.extent[0], 2 * M_PI / c.extent[1], 2 * M_PI / c.extent[2]};
   void* specs[3] = {c.spec, c.spec2, c.spec3};
   auto filtered = [&](int f0) {  // spectra of fields f0.. filtered, band sums in c.partials
@@ -429,8 +430,6 @@ static int adjoint_forcing_t(Ctx& c, int us0, double dt, double tke_target, doub
   if (int rc = filtered(CUP_ADJ_U)) return rc;
   if (int rc = inverse(X)) return rc;
   // s = <P u*, lam> = <Y, lam>/n
-  dim3 gd = grid_for(c.ncell);
-  if (gd.x > 4096) gd.x = 4096;
   hipLaunchKernelGGL((k_dot3<T>), gd, dim3(NT), 0, c.stream, Y, Y + c.ncell, Y + 2 * c.ncell,
                      (const T*)c.f[CUP_ADJ_U], (const T*)c.f[CUP_ADJ_V], (const T*)c.f[CUP_ADJ_W], c.ncell,
                      c.partials);
