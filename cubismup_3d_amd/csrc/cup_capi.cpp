@@ -819,15 +819,22 @@ int cup_adjoint_step_forcing(cup_ctx* ctx, double dt, double nu, const double ui
     if (scheme == CUP_UPWIND3_RK2) {
       keep = (char*)c.adjf + 6 * fb;
       for (int q = 0; q < 3; ++q)
-        hipMemcpyAsync((char*)keep + q * fb, c.f[CUP_U + q], fb, hipMemcpyDeviceToDevice, c.stream);
+        if (int e = check_hip(c, hipMemcpyAsync((char*)keep + q * fb, c.f[CUP_U + q], fb, hipMemcpyDeviceToDevice,
+                                                c.stream),
+                              "forcing adjoint"))
+          return e;
     }
     int e = launch_advdiff(c, nu, uinf, scheme);
     const int us0 = c.ustar;
     c.ustar = saved;
     if (!e) e = launch_adjoint_forcing(c, us0, dt, tke_target, injection_rate);
-    if (keep)
-      for (int q = 0; q < 3; ++q)
-        hipMemcpyAsync(c.f[CUP_U + q], (char*)keep + q * fb, fb, hipMemcpyDeviceToDevice, c.stream);
+    if (keep)  // u_n back in place, also after a failure
+      for (int q = 0; q < 3; ++q) {
+        const int r = check_hip(c, hipMemcpyAsync(c.f[CUP_U + q], (char*)keep + q * fb, fb, hipMemcpyDeviceToDevice,
+                                                  c.stream),
+                                "forcing adjoint");
+        if (!e) e = r;
+      }
     if (e) return e;
   }
   // lam_n = A'(u_n)^T lam* into tmp, then tmp and the adjoint fields trade buffers
