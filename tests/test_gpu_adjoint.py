@@ -223,3 +223,23 @@ def test_forced_rollout_gradient_vs_finite_differences(cup):
         ad = (grad * d).sum().item()
         print(f"direction {k}: adjoint {ad:.12e} fd {fd:.12e} rel {abs(ad - fd) / abs(fd):.2e}", flush=True)
         assert abs(ad - fd) < 1e-6 * abs(fd)
+
+
+def test_forcing_adjoint_step_fp32_vs_fp64(cup):
+    """The forced adjoint step in FP32 against FP64 on a 32^3 field (fused), to a target energy."""
+    print("forcing adjoint fp32 vs fp64", flush=True)
+    rng = np.random.default_rng(9)
+    bpd = (4, 4, 4)
+    G8 = cup.Grid(bpd, precision=8, decomposition="fused")
+    u = low_mode_field(G8.shape, G8.h, rng)
+    lam = tuple(a + 0.5 * b for a, b in zip(low_mode_field(G8.shape, G8.h, rng), u))
+    target = 1.1 * 0.5 * sum(float(np.mean(a * a)) for a in u)
+    out = []
+    for prec in (8, 4):
+        G = cup.Grid(bpd, precision=prec, decomposition="fused")
+        G.set_velocity(*u)
+        set_adjoint(G, lam)
+        G.adjoint_step(0.2 * G.h, 0.01, (0.1, 0.0, 0.0), tke_target=target)
+        out.append(get_adjoint(G))
+    for a, b in zip(out[1], out[0]):
+        assert rel(a, b) < 1e-5

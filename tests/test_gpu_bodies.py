@@ -163,3 +163,37 @@ def test_simulation_rejects_an_overlapping_sphere(cup):
         S.obstacle(1)
     S.run()
     assert np.isfinite(S.obstacle(0)["force"]).all()
+
+
+def test_two_spheres_write_their_own_files(cup, tmp_path):
+    """Each body writes its own forceValues_surface_<id>, powerValues_surface_<id>,
+    forceValues_<id> and computedVelocity_<id> (Obstacle.cpp:491-589), one row per step, with
+    its own velocity and centre."""
+    print("two spheres log files", flush=True)
+    argv = ["-bpdx", str(BPD[0]), "-bpdy", str(BPD[1]), "-bpdz", str(BPD[2]), "-extentx", str(EXT), "-CFL", "0.3",
+            "-nu", "1e-3", "-nsteps", "3", "-rampup", "0", "-lambda", "1e6", "-implicitPenalization", "1",
+            "-initCond", "zero", "-BC_x", "periodic", "-BC_y", "periodic", "-BC_z", "periodic",
+            "-freqDiagnostics", "0", "-precision", "double", "-muteAll", "0", "-serialization", str(tmp_path)]
+    S = cup.Simulation(cup.SimulationData(argv=argv))
+    vel = [(-1.0, 0.0, 0.0), (-0.5, 0.2, 0.0)]
+    for (c, R, hemi), v in zip(BODIES, vel):
+        S.add_obstacle(cup.Sphere(radius=R, position=c, enforcedVelocity=v, bForcedInSimFrame=True,
+                                  bFixFrameOfRef=False, bHemi=hemi))
+    S.run()
+    S.flush_logs()
+    for k in range(2):
+        for name in (f"forceValues_surface_{k}.dat", f"powerValues_surface_{k}.dat", f"forceValues_{k}.dat",
+                     f"computedVelocity_{k}.dat"):
+            with open(tmp_path / name) as f:
+                rows = [ln.split() for ln in f.read().splitlines() if ln.strip()]
+            assert len(rows) == 4, name  # header and 3 steps
+        with open(tmp_path / f"computedVelocity_{k}.dat") as f:
+            rows = [ln.split() for ln in f.read().splitlines() if ln.strip()]
+        head, data = rows[0], np.array([[float(x) for x in r] for r in rows[1:]])
+        np.testing.assert_array_equal(data[:, head.index("vel_x")], vel[k][0])
+        np.testing.assert_array_equal(data[:, head.index("vel_y")], vel[k][1])
+        assert data[-1, head.index("CMx")] != BODIES[k][0][0]  # the body moved with the frame at rest
+        ob = S.obstacle(k)
+        with open(tmp_path / f"forceValues_{k}.dat") as f:
+            rows = [ln.split() for ln in f.read().splitlines() if ln.strip()]
+        np.testing.assert_allclose([float(x) for x in rows[-1][3:6]], ob["force"], rtol=1e-6, atol=1e-9)
