@@ -588,6 +588,61 @@ int cup_move_body(cup_ctx* ctx, int id, const double center[3], double com_mass_
   return CUP_OK;
 }
 
+int cup_body_obstacle_blocks(cup_ctx* ctx, int id, int* ids_out, int cap, int* n_out)
+{
+  CUP_GUARD(ctx);
+  if (int rc = body_arg(c, id, "body_obstacle_blocks")) return rc;
+  if (!n_out) return set_error(c, CUP_E_INVALID, "body_obstacle_blocks: null pointer");
+  BodyScope scope(c, id);
+  const int n = c.body.nob;
+  *n_out = n;
+  if (ids_out) {
+    if (cap < n) return set_error(c, CUP_E_INVALID, "body_obstacle_blocks: cap below the number of obstacle blocks");
+    std::copy(c.body.ids_host.begin(), c.body.ids_host.end(), ids_out);
+  }
+  return CUP_OK;
+}
+
+int cup_body_set_udef(cup_ctx* ctx, int id, const double* udef, int n)
+{
+  CUP_GUARD(ctx);
+  if (int rc = body_arg(c, id, "body_set_udef")) return rc;
+  if (!udef) return set_error(c, CUP_E_INVALID, "body_set_udef: null pointer");
+  BodyScope scope(c, id);
+  Body& B = c.body;
+  if (n != B.nob) return set_error(c, CUP_E_INVALID, "body_set_udef: n must be the number of obstacle blocks");
+  const size_t bytes = (size_t)n * 3 * BS3 * sizeof(double);
+  if (!B.ob_udef && check_hip(c, hipMalloc(&B.ob_udef, bytes), "hipMalloc udef")) return CUP_E_NOMEM;
+  if (int rc = check_hip(c, hipStreamSynchronize(c.stream), "body_set_udef")) return rc;
+  return check_hip(c, hipMemcpy(B.ob_udef, udef, bytes, hipMemcpyHostToDevice), "body_set_udef");
+}
+
+int cup_body_get_udef(cup_ctx* ctx, int id, double* udef, int n)
+{
+  CUP_GUARD(ctx);
+  if (int rc = body_arg(c, id, "body_get_udef")) return rc;
+  if (!udef) return set_error(c, CUP_E_INVALID, "body_get_udef: null pointer");
+  BodyScope scope(c, id);
+  Body& B = c.body;
+  if (n != B.nob) return set_error(c, CUP_E_INVALID, "body_get_udef: n must be the number of obstacle blocks");
+  const size_t cnt = (size_t)n * 3 * BS3;
+  if (!B.ob_udef) {
+    std::fill(udef, udef + cnt, 0.0);
+    return CUP_OK;
+  }
+  if (int rc = check_hip(c, hipStreamSynchronize(c.stream), "body_get_udef")) return rc;
+  return check_hip(c, hipMemcpy(udef, B.ob_udef, cnt * sizeof(double), hipMemcpyDeviceToHost), "body_get_udef");
+}
+
+int cup_body_remove_udef_momenta(cup_ctx* ctx, int id, double out[13])
+{
+  CUP_GUARD(ctx);
+  if (int rc = body_arg(c, id, "body_remove_udef_momenta")) return rc;
+  if (!out) return set_error(c, CUP_E_INVALID, "body_remove_udef_momenta: null pointer");
+  BodyScope scope(c, id);
+  return launch_udef_remove(c, out);
+}
+
 int cup_penalize_body(cup_ctx* ctx, int id, double dt, double lambda, int implicit, const double trans_vel[3],
                       const double ang_vel[3], double ft_out[6])
 {

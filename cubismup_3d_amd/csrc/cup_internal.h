@@ -90,6 +90,8 @@ struct Body {
   unsigned char* ob_touch = nullptr;  // device: nob*512 surface flag of the RHS correction
   void* ob_r0 = nullptr;        // device: nob*512 unmasked RHS (adjoint with the centre; lazy)
   std::vector<int> ids_host;    // the obstacle blocks' ids, to reuse the buffers when a moved body keeps them
+  double* ob_udef = nullptr;    // device: nob*3*512 deformation velocity [slot][component][cell] (lazy; 0 when absent)
+  double corr[6] = {0, 0, 0, 0, 0, 0};  // transVel_correction, angVel_correction of the last udef removal
 };
 
 struct Ctx {
@@ -262,6 +264,7 @@ int launch_penalize_step(Ctx& c, double lambda, int implicit, const double vel[3
                          const double uinf[3]);  // also refreshes the fused step's max/energy partials
 int launch_surface_forces(Ctx& c, double nu, const double vel[3], const double omega[3]);
 int launch_obstacle_momenta(Ctx& c, double dt, double lambda, int implicit);
+int launch_udef_remove(Ctx& c, double out[13]);  // the active body's udef momentum removal (synchronous)
 int launch_add_uniform(Ctx& c, const double du[3]);
 int launch_rhs_obstacle(Ctx& c, void* rhs_blocks);
 

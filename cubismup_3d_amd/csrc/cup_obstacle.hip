@@ -177,6 +177,7 @@ void body_free(Ctx& c)
   if (B.ob_surf) hipFree(B.ob_surf);
   if (B.ob_touch) hipFree(B.ob_touch);
   if (B.ob_r0) hipFree(B.ob_r0);
+  if (B.ob_udef) hipFree(B.ob_udef);
   B = Body();
 }
 
@@ -281,7 +282,14 @@ int body_create(Ctx& c, const double center[3], double radius, double moll, bool
           return set_error(c, CUP_E_INVALID, "add_sphere: the bodies' obstacle blocks overlap (bodies too close)");
         }
   const bool reuse = B.present && B.radius == radius && B.moll == moll && B.hemi == hemi && B.ids_host == ids;
+  // a rebuilt body has no deformation velocity until it is set again; the last udef
+  // corrections stay (they are the body's, as Obstacle::transVel_correction)
+  double corr[6];
+  std::copy(B.corr, B.corr + 6, corr);
+  if (B.ob_udef) hipFree(B.ob_udef);
+  B.ob_udef = nullptr;
   if (!reuse) body_free(c);
+  std::copy(corr, corr + 6, B.corr);
   B.radius = radius;
   B.moll = moll;
   B.hemi = hemi;
@@ -871,6 +879,117 @@ static int momenta_t(Ctx& c, double lambdt, int implicit)
 int launch_obstacle_momenta(Ctx& c, double dt, double lambda, int implicit)
 {
   return c.precision == 4 ? momenta_t<float>(c, lambda * dt, implicit) : momenta_t<double>(c, lambda * dt, implicit);
+}
+
+// ---------------------------------------------------------------- udef momenta
+
+// KernelIntegrateUdefMomenta::visit (ObstaclesCreate.cpp:280-331) on one obstacle block:
+// over cells with chi > 0, p = x - CoM and the previous linear correction c_old,
+// part[13 s + q] = V, F (3), T = p x (udef - c_old) (3), J0..J5, each weighted by chi dv.
+__global__ __launch_bounds__(NT) void k_udef_momenta(const int* __restrict__ ob_ids, const double* __restrict__ ob_chi,
+                                                     const double* __restrict__ ob_udef, GridArgs g, double cmx,
+                                                     double cmy, double cmz, double ocx, double ocy, double ocz,
+                                                     double* __restrict__ part)
+{
+  const int s = blockIdx.x, b = ob_ids[s];
+  int bx, by, bz;
+  bcoords(g, b, bx, by, bz);
+  const double dv = g.h * g.h * g.h;
+  const double* U = ob_udef + (size_t)s * 3 * BS3;
+  double M[13];
+#pragma unroll
+  for (int q = 0; q < 13; ++q) M[q] = 0;
+  for (int c = threadIdx.x; c < BS3; c += NT) {
+    const double X = ob_chi[(size_t)s * BS3 + c];
+    if (X <= 0) continue;
+    const int ix = c & 7, iy = (c >> 3) & 7, iz = c >> 6;
+    const double p0 = (bx * BS + ix + 0.5) * g.h - cmx, p1 = (by * BS + iy + 0.5) * g.h - cmy,
+                 p2 = (bz * BS + iz + 0.5) * g.h - cmz;
+    const double ux = U[c], uy = U[BS3 + c], uz = U[2 * BS3 + c];
+    const double dU = ux - ocx, dV = uy - ocy, dW = uz - ocz;
+    M[0] += X * dv;
+    M[1] += X * ux * dv;
+    M[2] += X * uy * dv;
+    M[3] += X * uz * dv;
+    M[4] += X * (p1 * dW - p2 * dV) * dv;
+    M[5] += X * (p2 * dU - p0 * dW) * dv;
+    M[6] += X * (p0 * dV - p1 * dU) * dv;
+    M[7] += X * (p1 * p1 + p2 * p2) * dv;
+    M[8] += X * (p0 * p0 + p2 * p2) * dv;
+    M[9] += X * (p0 * p0 + p1 * p1) * dv;
+    M[10] -= X * p0 * p1 * dv;
+    M[11] -= X * p0 * p2 * dv;
+    M[12] -= X * p1 * p2 * dv;
+  }
+  breduce<13>(M, part + 13 * (size_t)s);
+}
+
+// KernelRemoveUdefMomenta::visit (ObstaclesCreate.cpp:392-439): udef -= ct + cw x (x - CoM) on
+// every cell of the obstacle blocks.
+__global__ __launch_bounds__(NT) void k_udef_remove(const int* __restrict__ ob_ids, double* __restrict__ ob_udef,
+                                                    GridArgs g, double cmx, double cmy, double cmz, double ctx,
+                                                    double cty, double ctz, double cwx, double cwy, double cwz)
+{
+  const int s = blockIdx.x, b = ob_ids[s];
+  int bx, by, bz;
+  bcoords(g, b, bx, by, bz);
+  double* U = ob_udef + (size_t)s * 3 * BS3;
+  for (int c = threadIdx.x; c < BS3; c += NT) {
+    const int ix = c & 7, iy = (c >> 3) & 7, iz = c >> 6;
+    const double p0 = (bx * BS + ix + 0.5) * g.h - cmx, p1 = (by * BS + iy + 0.5) * g.h - cmy,
+                 p2 = (bz * BS + iz + 0.5) * g.h - cmz;
+    U[c] -= ctx + (cwy * p2 - cwz * p1);
+    U[BS3 + c] -= cty + (cwz * p0 - cwx * p2);
+    U[2 * BS3 + c] -= ctz + (cwx * p1 - cwy * p0);
+  }
+}
+
+// KernelIntegrateUdefMomenta, KernelAccumulateUdefMomenta (ObstaclesCreate.cpp:334-384: the
+// corrections F/V and J^-1 T, J inverted by cofactors on the host) and KernelRemoveUdefMomenta
+// for the active body. out: mass, linear correction (3), angular correction (3), J0..J5.
+int launch_udef_remove(Ctx& c, double out[13])
+{
+  Body& B = c.body;
+  const size_t n = (size_t)B.nob * 3 * BS3;
+  if (!B.ob_udef) {  // no deformation velocity set: zero
+    if (check_hip(c, hipMalloc(&B.ob_udef, n * sizeof(double)), "hipMalloc udef")) return CUP_E_NOMEM;
+    const std::vector<double> zero(n, 0.0);
+    if (int rc = check_hip(c, hipMemcpy(B.ob_udef, zero.data(), n * sizeof(double), hipMemcpyHostToDevice), "udef"))
+      return rc;
+  }
+  if (int rc = ensure_partials(c, 32 * (size_t)B.nob + 2 * (size_t)c.nb)) return rc;
+  double* bp = c.partials + 2 * (size_t)c.nb;
+  hipLaunchKernelGGL(k_udef_momenta, dim3(B.nob), dim3(NT), 0, c.stream, B.ob_ids, B.ob_chi, B.ob_udef, c.grid(),
+                     B.com[0], B.com[1], B.com[2], B.corr[0], B.corr[1], B.corr[2], bp);
+  hipLaunchKernelGGL(k_sum_rows, dim3(13), dim3(NT), 0, c.stream, (const double*)bp, B.nob, 13, c.sc->red);
+  if (int rc = check_hip(c, hipGetLastError(), "k_udef_momenta")) return rc;
+  double M[13];
+  if (int rc = check_hip(c, hipMemcpyAsync(M, c.sc->red, sizeof(M), hipMemcpyDeviceToHost, c.stream), "udef"))
+    return rc;
+  if (int rc = check_hip(c, hipStreamSynchronize(c.stream), "udef")) return rc;
+  if (!(M[0] > 0)) return set_error(c, CUP_E_STATE, "remove_udef_momenta: the body has no volume on the grid");
+  const double* J = M + 7;  // J0..J2 diagonal, J3 = xy, J4 = xz, J5 = yz
+  const double c0 = J[1] * J[2] - J[5] * J[5], c1 = J[0] * J[2] - J[4] * J[4], c2 = J[0] * J[1] - J[3] * J[3];
+  const double c3 = J[4] * J[5] - J[3] * J[2], c4 = J[3] * J[5] - J[4] * J[1], c5 = J[3] * J[4] - J[0] * J[5];
+  const double det = J[0] * c0 + J[3] * c3 + J[4] * c4;
+  const double iJ[6] = {c0 / det, c1 / det, c2 / det, c3 / det, c4 / det, c5 / det};
+  const double ct[3] = {M[1] / M[0], M[2] / M[0], M[3] / M[0]};
+  const double cw[3] = {iJ[0] * M[4] + iJ[3] * M[5] + iJ[4] * M[6], iJ[3] * M[4] + iJ[1] * M[5] + iJ[5] * M[6],
+                        iJ[4] * M[4] + iJ[5] * M[5] + iJ[2] * M[6]};
+  hipLaunchKernelGGL(k_udef_remove, dim3(B.nob), dim3(NT), 0, c.stream, B.ob_ids, B.ob_udef, c.grid(), B.com[0],
+                     B.com[1], B.com[2], ct[0], ct[1], ct[2], cw[0], cw[1], cw[2]);
+  if (int rc = check_hip(c, hipGetLastError(), "k_udef_remove")) return rc;
+  for (int d = 0; d < 3; ++d) {
+    B.corr[d] = ct[d];
+    B.corr[3 + d] = cw[d];
+  }
+  out[0] = M[0];
+  for (int d = 0; d < 3; ++d) {
+    out[1 + d] = ct[d];
+    out[4 + d] = cw[d];
+  }
+  for (int q = 0; q < 6; ++q) out[7 + q] = J[q];
+  return check_hip(c, hipStreamSynchronize(c.stream), "udef");
 }
 
 // ---------------------------------------------------------------- surface forces

@@ -282,6 +282,39 @@ class Grid:
         self._c(L.lib().cup_move_body(self._h, int(k), L.vec3(center), L.dptr(out)))
         return dict(com=out[:3].copy(), mass=out[3])
 
+    def obstacle_blocks(self, k=0) -> np.ndarray:
+        """Block ids of body k's obstacle blocks, in the order its udef uses."""
+        n = C.c_int()
+        self._c(L.lib().cup_body_obstacle_blocks(self._h, int(k), None, 0, C.byref(n)))
+        ids = (C.c_int * max(n.value, 1))()
+        self._c(L.lib().cup_body_obstacle_blocks(self._h, int(k), ids, n.value, C.byref(n)))
+        return np.array(ids[:n.value], dtype=np.int64)
+
+    def set_udef(self, k, ux, uy, uz):
+        """Body k's deformation velocity from three [Z][Y][X] arrays (its obstacle blocks are kept)."""
+        ids = self.obstacle_blocks(k)
+        blk = np.stack([to_blocks(np.asarray(f, dtype=np.float64)).reshape(-1, 512)[ids] for f in (ux, uy, uz)], 1)
+        buf = np.ascontiguousarray(blk)
+        self._c(L.lib().cup_body_set_udef(self._h, int(k), L.dptr(buf), len(ids)))
+
+    def get_udef(self, k=0):
+        """Body k's deformation velocity as three [Z][Y][X] arrays, 0 outside its obstacle blocks."""
+        ids = self.obstacle_blocks(k)
+        buf = np.zeros((len(ids), 3, 512))
+        self._c(L.lib().cup_body_get_udef(self._h, int(k), L.dptr(buf), len(ids)))
+        out = []
+        for q in range(3):
+            full = np.zeros((int(np.prod(self.shape)) // 512, 512))
+            full[ids] = buf[:, q]
+            out.append(from_blocks(full.ravel(), self.shape))
+        return tuple(out)
+
+    def remove_udef_momenta(self, k=0):
+        """CreateObstacles' udef momentum removal for body k: dict(mass, trans, ang, J)."""
+        o = np.zeros(13)
+        self._c(L.lib().cup_body_remove_udef_momenta(self._h, int(k), L.dptr(o)))
+        return dict(mass=o[0], trans=o[1:4].copy(), ang=o[4:7].copy(), J=o[7:13].copy())
+
     def penalize_body(self, k, dt, lam, implicit, trans_vel, ang_vel=(0, 0, 0)) -> np.ndarray:
         ft = np.zeros(6)
         self._c(L.lib().cup_penalize_body(self._h, int(k), float(dt), float(lam), int(bool(implicit)),

@@ -144,6 +144,21 @@ int cup_chi_from_sphere_mollified(cup_ctx* ctx, const double center[3], double r
 int cup_body_count(cup_ctx* ctx, int* n);
 int cup_add_sphere(cup_ctx* ctx, const double center[3], double radius, int hemi, double com_mass_out[4], int* id);
 int cup_move_body(cup_ctx* ctx, int id, const double center[3], double com_mass_out[4]);
+/* Body id's deformation velocity udef (ObstacleBlock::udef, ObstacleBlock.h:41-45) on its
+ * obstacle blocks: cup_body_obstacle_blocks gives their number (*n_out) and, with ids_out of
+ * cap >= n entries, their block ids in the order udef uses; udef is n x 3 x 512 FP64,
+ * [block][component][cell] with cell = ix + 8 (iy + 8 iz). A body built again (cup_move_body,
+ * cup_chi_from_sphere) has udef = 0 until it is set again. cup_body_remove_udef_momenta is
+ * CreateObstacles' momentum removal (ObstaclesCreate.cpp:258-440): over cells with chi > 0,
+ * V = sum chi dv, F = sum chi udef dv, T = sum chi (x - CoM) x (udef - c_old) dv with c_old the
+ * body's previous linear correction, J the inertia tensor; corrections F/V and J^-1 T;
+ * udef -= F/V + (J^-1 T) x (x - CoM) on every cell of the obstacle blocks. out: mass,
+ * the linear correction (3), the angular correction (3), J0..J5 (xx, yy, zz, xy, xz, yz).
+ * The penalization, momentum and force calls take the body's udef as zero. */
+int cup_body_obstacle_blocks(cup_ctx* ctx, int id, int* ids_out, int cap, int* n_out);
+int cup_body_set_udef(cup_ctx* ctx, int id, const double* udef, int n);
+int cup_body_get_udef(cup_ctx* ctx, int id, double* udef, int n);
+int cup_body_remove_udef_momenta(cup_ctx* ctx, int id, double out[13]);
 int cup_penalize_body(cup_ctx* ctx, int id, double dt, double lambda, int implicit, const double trans_vel[3],
                       const double ang_vel[3], double ft_out[6]);
 int cup_surface_forces_body(cup_ctx* ctx, int id, double nu, const double trans_vel[3], const double ang_vel[3],

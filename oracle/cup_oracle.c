@@ -55,6 +55,7 @@ typedef struct {
 typedef struct {
   double chi[BS][BS][BS];
   double sdf[BS][BS][BS];
+  double udef[3][BS][BS][BS]; /* the deformation velocity (0 until orc_set_udef) */
   double CoM[3], mass;
   int npts, cap;
   int* pix;
@@ -69,6 +70,7 @@ typedef struct {
   int hemi;
   OBlock** ob;
   double CoM[3], mass;
+  double corr[6]; /* transVel_correction, angVel_correction of the last udef removal */
 } OBody;
 
 struct orc_sim {
@@ -84,6 +86,7 @@ struct orc_sim {
   int hemi;    /* the hemisphere of HemiSphereObstacle (flat face at x = c_x, body at x < c_x) */
   OBlock** ob;
   double CoM[3], mass;
+  double corr[6];
 };
 
 static const double EPS = DBL_EPSILON;
@@ -129,6 +132,7 @@ static void body_swap(orc_sim* s, OBody* b)
   s->has_sphere = b->has_sphere; s->radius = b->radius; s->moll = b->moll; s->hemi = b->hemi; s->ob = b->ob;
   s->mass = b->mass;
   for (int d = 0; d < 3; d++) { s->spos[d] = b->spos[d]; s->CoM[d] = b->CoM[d]; }
+  for (int q = 0; q < 6; q++) { t.corr[q] = s->corr[q]; s->corr[q] = b->corr[q]; }
   *b = t;
 }
 
@@ -1438,6 +1442,107 @@ static void sphere_build_blocks(orc_sim* s, double radius, const double pos[3], 
   }
   s->mass = com[0];
   for (int d = 0; d < 3; d++) s->CoM[d] = com[d + 1] / com[0];
+}
+
+/* The active body's udef from three block-major fields (3 x ncell, component-major); cells
+ * outside its obstacle blocks are ignored. */
+void orc_set_udef(orc_sim* s, const double* udef)
+{
+  const size_t nc = (size_t)s->nb * BS * BS * BS;
+  for (int b = 0; s->ob && b < s->nb; b++) {
+    OBlock* o = s->ob[b];
+    if (!o) continue;
+    for (int q = 0; q < 3; q++) memcpy(o->udef[q], udef + q * nc + (size_t)b * BS * BS * BS, sizeof(o->udef[q]));
+  }
+}
+
+/* The active body's udef as three block-major fields, 0 outside its obstacle blocks. */
+void orc_get_udef(const orc_sim* s, double* udef)
+{
+  const size_t nc = (size_t)s->nb * BS * BS * BS;
+  memset(udef, 0, 3 * nc * sizeof(double));
+  for (int b = 0; s->ob && b < s->nb; b++) {
+    const OBlock* o = s->ob[b];
+    if (!o) continue;
+    for (int q = 0; q < 3; q++) memcpy(udef + q * nc + (size_t)b * BS * BS * BS, o->udef[q], sizeof(o->udef[q]));
+  }
+}
+
+/* The momentum removal of the deformation velocity, for the active body:
+ * KernelIntegrateUdefMomenta (ObstaclesCreate.cpp:258-332): per obstacle block, over cells with
+ * chi > 0 and p = x - CoM, V = sum chi dv, F = sum chi udef dv, T = sum chi p x (udef - c_old) dv
+ * (c_old the previous removal's linear correction) and the inertia J; KernelAccumulate
+ * UdefMomenta (:334-384): transVel_correction = F / V, angVel_correction = J^-1 T;
+ * KernelRemoveUdefMomenta (:386-440): udef -= transVel_correction + angVel_correction x p on
+ * every cell of the obstacle blocks. out: mass, the two corrections, J0..J5. The block sums
+ * are added in block order. */
+void orc_remove_udef_momenta(orc_sim* s, double out[13])
+{
+  double M[13] = {0};
+  const double dv = s->h * s->h * s->h;
+  for (int b = 0; s->ob && b < s->nb; b++) {
+    const OBlock* o = s->ob[b];
+    if (!o) continue;
+    int bc[3];
+    bcoords(s, b, bc);
+    double m[13] = {0};
+    for (int iz = 0; iz < BS; iz++)
+      for (int iy = 0; iy < BS; iy++)
+        for (int ix = 0; ix < BS; ix++) {
+          const double X = o->chi[iz][iy][ix];
+          if (X <= 0) continue;
+          double p[3];
+          cell_pos(s, bc, ix, iy, iz, p);
+          for (int d = 0; d < 3; d++) p[d] -= s->CoM[d];
+          const double U = o->udef[0][iz][iy][ix], V = o->udef[1][iz][iy][ix], W = o->udef[2][iz][iy][ix];
+          const double dU = U - s->corr[0], dV = V - s->corr[1], dW = W - s->corr[2];
+          m[0] += X * dv;
+          m[1] += X * U * dv;
+          m[2] += X * V * dv;
+          m[3] += X * W * dv;
+          m[4] += X * (p[1] * dW - p[2] * dV) * dv;
+          m[5] += X * (p[2] * dU - p[0] * dW) * dv;
+          m[6] += X * (p[0] * dV - p[1] * dU) * dv;
+          m[7] += X * (p[1] * p[1] + p[2] * p[2]) * dv;
+          m[8] += X * (p[0] * p[0] + p[2] * p[2]) * dv;
+          m[9] += X * (p[0] * p[0] + p[1] * p[1]) * dv;
+          m[10] -= X * p[0] * p[1] * dv;
+          m[11] -= X * p[0] * p[2] * dv;
+          m[12] -= X * p[1] * p[2] * dv;
+        }
+    for (int q = 0; q < 13; q++) M[q] += m[q];
+  }
+  /* inverse of the symmetric J (diagonal J0..J2, off-diagonal J3 = xy, J4 = xz, J5 = yz) by cofactors */
+  const double* J = M + 7;
+  const double c0 = J[1] * J[2] - J[5] * J[5], c1 = J[0] * J[2] - J[4] * J[4], c2 = J[0] * J[1] - J[3] * J[3];
+  const double c3 = J[4] * J[5] - J[3] * J[2], c4 = J[3] * J[5] - J[4] * J[1], c5 = J[3] * J[4] - J[0] * J[5];
+  const double det = J[0] * c0 + J[3] * c3 + J[4] * c4;
+  const double iJ[6] = {c0 / det, c1 / det, c2 / det, c3 / det, c4 / det, c5 / det};
+  const double T[3] = {M[4], M[5], M[6]};
+  double ct[3], cw[3];
+  for (int d = 0; d < 3; d++) ct[d] = M[1 + d] / M[0];
+  cw[0] = iJ[0] * T[0] + iJ[3] * T[1] + iJ[4] * T[2];
+  cw[1] = iJ[3] * T[0] + iJ[1] * T[1] + iJ[5] * T[2];
+  cw[2] = iJ[4] * T[0] + iJ[5] * T[1] + iJ[2] * T[2];
+  for (int b = 0; s->ob && b < s->nb; b++) {
+    OBlock* o = s->ob[b];
+    if (!o) continue;
+    int bc[3];
+    bcoords(s, b, bc);
+    for (int iz = 0; iz < BS; iz++)
+      for (int iy = 0; iy < BS; iy++)
+        for (int ix = 0; ix < BS; ix++) {
+          double p[3];
+          cell_pos(s, bc, ix, iy, iz, p);
+          for (int d = 0; d < 3; d++) p[d] -= s->CoM[d];
+          const double r[3] = {cw[1] * p[2] - cw[2] * p[1], cw[2] * p[0] - cw[0] * p[2], cw[0] * p[1] - cw[1] * p[0]};
+          for (int d = 0; d < 3; d++) o->udef[d][iz][iy][ix] -= ct[d] + r[d];
+        }
+  }
+  for (int d = 0; d < 3; d++) { s->corr[d] = ct[d]; s->corr[3 + d] = cw[d]; }
+  out[0] = M[0];
+  for (int d = 0; d < 3; d++) { out[1 + d] = ct[d]; out[4 + d] = cw[d]; }
+  for (int q = 0; q < 6; q++) out[7 + q] = J[q];
 }
 
 void orc_sphere_info(const orc_sim* s, double out[8])

@@ -37,6 +37,9 @@ void orc_hemisphere_create(orc_sim* s, double radius, const double pos[3]);
 int orc_add_sphere(orc_sim* s, double radius, const double pos[3], int hemi);
 int orc_body_count(const orc_sim* s);
 int orc_move_body(orc_sim* s, int k, const double pos[3]);
+void orc_set_udef(orc_sim* s, const double* udef);
+void orc_get_udef(const orc_sim* s, double* udef);
+void orc_remove_udef_momenta(orc_sim* s, double out[13]);
 void orc_select_body(orc_sim* s, int k);
 double orc_spectrum_interp(const double* K, const double* E, int n, double k);
 void orc_ic_hit_spectrum(orc_sim* s, const double* K, const double* E, int n, double u_rms, uint64_t seed);

@@ -78,6 +78,9 @@ def lib():
         "orc_add_sphere": (i, [p, d, dp, i]),
         "orc_body_count": (i, [p]),
         "orc_move_body": (i, [p, i, dp]),
+        "orc_set_udef": (None, [p, dp]),
+        "orc_get_udef": (None, [p, dp]),
+        "orc_remove_udef_momenta": (None, [p, dp]),
         "orc_select_body": (None, [p, i]),
         "orc_fft3d": (None, [i, i, i, dp, dp, i]),
         "orc_philox": (None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
@@ -329,6 +332,24 @@ class OracleSim:
 
     def body_count(self):
         return lib().orc_body_count(self._h)
+
+    def set_udef(self, ux, uy, uz):
+        """The active body's deformation velocity (three [Z][Y][X] arrays; only its obstacle
+        blocks are kept)."""
+        buf = np.concatenate([to_blocks(np.asarray(f, dtype=np.float64)) for f in (ux, uy, uz)])
+        lib().orc_set_udef(self._h, _dp(buf))
+
+    def get_udef(self):
+        n = int(np.prod(self.shape))
+        buf = np.zeros(3 * n)
+        lib().orc_get_udef(self._h, _dp(buf))
+        return tuple(from_blocks(buf[q * n:(q + 1) * n], self.shape) for q in range(3))
+
+    def remove_udef_momenta(self):
+        """The active body's udef momentum removal; returns dict(mass, trans, ang, J)."""
+        o = np.zeros(13)
+        lib().orc_remove_udef_momenta(self._h, _dp(o))
+        return dict(mass=o[0], trans=o[1:4].copy(), ang=o[4:7].copy(), J=o[7:13].copy())
 
     def move_body(self, k, pos):
         """Body k at centre pos; chi and the SDF of every body are built again."""

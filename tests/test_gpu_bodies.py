@@ -197,3 +197,41 @@ def test_two_spheres_write_their_own_files(cup, tmp_path):
         with open(tmp_path / f"forceValues_{k}.dat") as f:
             rows = [ln.split() for ln in f.read().splitlines() if ln.strip()]
         np.testing.assert_allclose([float(x) for x in rows[-1][3:6]], ob["force"], rtol=1e-6, atol=1e-9)
+
+
+def test_udef_momentum_removal_matches_oracle(cup, oracle):
+    """CreateObstacles' udef momentum removal (ObstaclesCreate.cpp:258-440) on the device against
+    the oracle, for body 1 of two (the other keeps udef = 0), twice in a row (the second call
+    uses the first's linear correction), FP64; a moved body starts again from udef = 0."""
+    print("udef momentum removal", flush=True)
+    G, O = pair(cup, oracle)
+    rng = np.random.default_rng(21)
+    udef = tuple(rng.standard_normal(G.shape) for _ in range(3))
+    G.set_udef(1, *udef)
+    with O.body(1):
+        O.set_udef(*udef)
+    scale = {}
+    for it in range(2):
+        rg = G.remove_udef_momenta(1)
+        with O.body(1):
+            ro = O.remove_udef_momenta()
+        assert rg["mass"] == pytest.approx(ro["mass"], rel=1e-13)
+        for key in ("trans", "ang", "J"):
+            if it == 0:
+                scale[key] = np.abs(ro[key]).max()
+            # the second call removes rounding only: compared at the first call's scale
+            np.testing.assert_allclose(rg[key], ro[key], rtol=1e-11, atol=1e-13 * scale[key])
+        with O.body(1):
+            uo = O.get_udef()
+        for a, b in zip(G.get_udef(1), uo):
+            assert rel(a, b) < 1e-12
+    assert np.abs(ro["trans"]).max() < 1e-12 * scale["trans"]  # nothing left to remove
+    for a in G.get_udef(0):
+        assert not a.any()
+    r0 = G.remove_udef_momenta(0)  # a body without udef: nothing to remove
+    assert not np.abs(r0["trans"]).any() and not np.abs(r0["ang"]).any()
+    G.move_body(1, (1.47, 0.5, 0.5))
+    for a in G.get_udef(1):
+        assert not a.any()
+    with pytest.raises(ValueError):
+        G.set_udef(2, *udef)

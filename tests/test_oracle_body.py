@@ -113,3 +113,63 @@ def test_two_bodies_superpose_single_bodies(oracle):
             fp = P.penalize(dt, 1e6, True, (-1.0, 0.2, 0.0))
         fs = S.penalize(dt, 1e6, True, (-1.0, 0.2, 0.0))
         assert np.array_equal(fp, fs)
+
+
+def _rigid_udef(O, V, W):
+    """V + W x (x - CoM) on every cell, and the cell positions relative to the centre of mass."""
+    with_info = O.sphere_info()
+    nz, ny, nx = O.shape
+    h = O.h
+    z, y, x = np.meshgrid(*(h * (np.arange(m) + 0.5) for m in (nz, ny, nx)), indexing="ij")
+    p = (x - with_info["com"][0], y - with_info["com"][1], z - with_info["com"][2])
+    u = V[0] + W[1] * p[2] - W[2] * p[1]
+    v = V[1] + W[2] * p[0] - W[0] * p[2]
+    w = V[2] + W[0] * p[1] - W[1] * p[0]
+    return (u, v, w), p
+
+
+def test_udef_rigid_motion_is_removed_entirely(oracle):
+    """A deformation velocity that is a rigid motion V + W x (x - CoM) is all momentum:
+    F/V gives V (the centre of mass is chi's), J^-1 T gives W (T = J W about the centre of
+    mass), and the removal leaves udef = 0 on every cell of the obstacle blocks. A second
+    removal finds nothing (its T uses udef minus the last linear correction, whose moment
+    vanishes about the centre of mass). A wrong sign in a cross product, a misplaced J entry
+    or a transposed inverse fails it."""
+    O = oracle.OracleSim((4, 4, 4), maxextent=1.0)
+    O.sphere_create(0.2, (0.47, 0.52, 0.55))
+    V, W = np.array([0.3, -0.7, 0.2]), np.array([1.1, 0.4, -0.9])
+    udef, _ = _rigid_udef(O, V, W)
+    O.set_udef(*udef)
+    r = O.remove_udef_momenta()
+    np.testing.assert_allclose(r["trans"], V, rtol=1e-12)
+    np.testing.assert_allclose(r["ang"], W, rtol=1e-9)
+    assert r["mass"] == pytest.approx(O.sphere_info()["mass"], rel=1e-14)
+    after = O.get_udef()
+    assert max(np.abs(a).max() for a in after) < 1e-9
+    r2 = O.remove_udef_momenta()
+    assert np.abs(r2["trans"]).max() < 1e-9 and np.abs(r2["ang"]).max() < 1e-8
+
+
+def test_udef_removal_leaves_no_momentum(oracle):
+    """For any udef the removed field has zero linear and angular momentum about the centre of
+    mass (sum chi udef dv and sum chi p x udef dv), the defining property of the operator."""
+    rng = np.random.default_rng(17)
+    O = oracle.OracleSim((4, 4, 4), maxextent=1.0)
+    O.sphere_create(0.2, (0.47, 0.52, 0.55))
+    udef = tuple(rng.standard_normal(O.shape) for _ in range(3))
+    O.set_udef(*udef)
+    r = O.remove_udef_momenta()
+    assert np.abs(r["trans"]).max() > 1e-3 and np.abs(r["ang"]).max() > 1e-3
+    after = O.get_udef()
+    chi = O.get("chi")
+    _, p = _rigid_udef(O, np.zeros(3), np.zeros(3))
+    dv = O.h ** 3
+    lin = [float((chi * a).sum() * dv) for a in after]
+    ang = [float((chi * (p[1] * after[2] - p[2] * after[1])).sum() * dv),
+           float((chi * (p[2] * after[0] - p[0] * after[2])).sum() * dv),
+           float((chi * (p[0] * after[1] - p[1] * after[0])).sum() * dv)]
+    scale = float(sum((chi * np.abs(a)).sum() for a in udef) * dv)
+    assert max(abs(x) for x in lin) < 1e-13 * scale and max(abs(x) for x in ang) < 1e-13 * scale
+    # cells outside the body's obstacle blocks hold no udef
+    far = after[0][0, 0, 0], after[0][-1, -1, -1]
+    assert far == (0.0, 0.0)
