@@ -915,6 +915,7 @@ int cup_adjoint_step_body(cup_ctx* ctx, double dt, double nu, const double uinf[
   if (!uinf || !body_vel || !dJdV || !(dt > 0)) return set_error(c, CUP_E_INVALID, "adjoint_step_body: bad argument");
   if (!c.body.present) return set_error(c, CUP_E_STATE, "adjoint_step_body: no obstacle");
   if (body_count(c) > 1) return set_error(c, CUP_E_STATE, "adjoint_step_body: one body only");
+  if (c.body.ob_udef) return set_error(c, CUP_E_STATE, "adjoint_step_body: the body's udef is not differentiated");
   if (dJdc && !(c.body.moll > 0))
     return set_error(c, CUP_E_STATE,
                      "adjoint_step_body: the centre gradient needs the mollified chi (cup_chi_from_sphere_mollified)");

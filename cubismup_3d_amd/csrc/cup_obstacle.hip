@@ -459,7 +459,7 @@ int launch_rhs_obstacle(Ctx& c, void* rhs_blocks)
 // step it also folds the penalized blocks into the step's max |u+uinf| and energy.
 template <typename T>
 __global__ __launch_bounds__(NT) void k_penalize(const int* __restrict__ ob_ids, const double* __restrict__ ob_chi,
-                                                 const T* __restrict__ chi, T* __restrict__ u, T* __restrict__ v,
+                                                 const double* __restrict__ ob_udef, const T* __restrict__ chi, T* __restrict__ u, T* __restrict__ v,
                                                  T* __restrict__ w, GridArgs g, DevScalars* __restrict__ sc,
                                                  double lambda, int implicit, double vx, double vy, double vz,
                                                  double ox, double oy, double oz, double cmx, double cmy, double cmz,
@@ -481,7 +481,16 @@ __global__ __launch_bounds__(NT) void k_penalize(const int* __restrict__ ob_ids,
     if (!(chi[o] > (T)X) && X > 0) {  // compared in the grid's precision, as the grid chi is stored
       const double px = (bx * BS + ix + 0.5) * g.h - cmx, py = (by * BS + iy + 0.5) * g.h - cmy,
                    pz = (bz * BS + iz + 0.5) * g.h - cmz;
-      const double UT0 = vx + oy * pz - oz * py, UT1 = vy + oz * px - ox * pz, UT2 = vz + ox * py - oy * px;
+      // U_tot = V + Omega x p + udef (Penalization.cpp:74-80); no udef: + 0, the same values
+      double d0 = 0, d1 = 0, d2 = 0;
+      if (ob_udef) {
+        const double* D = ob_udef + (size_t)s * 3 * BS3;
+        d0 = D[c];
+        d1 = D[BS3 + c];
+        d2 = D[2 * BS3 + c];
+      }
+      const double UT0 = vx + oy * pz - oz * py + d0, UT1 = vy + oz * px - ox * pz + d1,
+                   UT2 = vz + ox * py - oy * px + d2;
       const double pf = implicit ? X * lambdaFac / (1 + lambdaFac * dt * X) : X * lambdaFac;
       const double FX = pf * (UT0 - uu), FY = pf * (UT1 - vv), FZ = pf * (UT2 - ww);
       uu = uu + dt * FX;
@@ -776,7 +785,8 @@ static int penalize_t(Ctx& c, double lambda, int implicit, const double vel[3], 
 {
   Body& B = c.body;
   double* bp = c.partials + 2 * (size_t)c.nb;
-  hipLaunchKernelGGL((k_penalize<T>), dim3(B.nob), dim3(NT), 0, c.stream, B.ob_ids, B.ob_chi, (const T*)c.f[CUP_CHI],
+  hipLaunchKernelGGL((k_penalize<T>), dim3(B.nob), dim3(NT), 0, c.stream, B.ob_ids, B.ob_chi,
+                     (const double*)B.ob_udef, (const T*)c.f[CUP_CHI],
                      (T*)c.f[CUP_U], (T*)c.f[CUP_V], (T*)c.f[CUP_W], c.grid(), c.sc, lambda, implicit, vel[0], vel[1],
                      vel[2], om[0], om[1], om[2], B.com[0], B.com[1], B.com[2], uinf[0], uinf[1], uinf[2], bp,
                      refresh ? c.ke_part : (double*)nullptr);
@@ -807,7 +817,8 @@ int launch_penalize_step(Ctx& c, double lambda, int implicit, const double vel[3
 // otherwise. p = x - CoM; the obstacle's chi, cells with chi <= 0 skipped.
 template <typename T>
 __global__ __launch_bounds__(NT) void k_obstacle_momenta(const int* __restrict__ ob_ids,
-                                                         const double* __restrict__ ob_chi, const T* __restrict__ u,
+                                                         const double* __restrict__ ob_chi,
+                                                         const double* __restrict__ ob_udef, const T* __restrict__ u,
                                                          const T* __restrict__ v, const T* __restrict__ w, GridArgs g,
                                                          double lambdt, int implicit, double cmx, double cmy,
                                                          double cmz, double* __restrict__ part)
@@ -853,12 +864,21 @@ __global__ __launch_bounds__(NT) void k_obstacle_momenta(const int* __restrict__
       M[20] -= pf * px * py;
       M[21] -= pf * px * pz;
       M[22] -= pf * py * pz;
-      M[23] += pf * U;
-      M[24] += pf * V;
-      M[25] += pf * W;
-      M[26] += pf * (py * W - pz * V);
-      M[27] += pf * (pz * U - px * W);
-      M[28] += pf * (px * V - py * U);
+      // u - udef (ObstaclesUpdate.cpp:117-126); no udef: - 0, the same values
+      double d0 = 0, d1 = 0, d2 = 0;
+      if (ob_udef) {
+        const double* D = ob_udef + (size_t)s * 3 * BS3;
+        d0 = D[c];
+        d1 = D[BS3 + c];
+        d2 = D[2 * BS3 + c];
+      }
+      const double DU = U - d0, DV = V - d1, DW = W - d2;
+      M[23] += pf * DU;
+      M[24] += pf * DV;
+      M[25] += pf * DW;
+      M[26] += pf * (py * DW - pz * DV);
+      M[27] += pf * (pz * DU - px * DW);
+      M[28] += pf * (px * DV - py * DU);
     }
   }
   breduce<29>(M, part + 29 * (size_t)s);
@@ -870,7 +890,7 @@ static int momenta_t(Ctx& c, double lambdt, int implicit)
   Body& B = c.body;
   double* bp = c.partials + 2 * (size_t)c.nb;
   hipLaunchKernelGGL((k_obstacle_momenta<T>), dim3(B.nob), dim3(NT), 0, c.stream, B.ob_ids, B.ob_chi,
-                     (const T*)c.f[CUP_U], (const T*)c.f[CUP_V], (const T*)c.f[CUP_W], c.grid(), lambdt, implicit,
+                     (const double*)B.ob_udef, (const T*)c.f[CUP_U], (const T*)c.f[CUP_V], (const T*)c.f[CUP_W], c.grid(), lambdt, implicit,
                      B.com[0], B.com[1], B.com[2], bp);
   hipLaunchKernelGGL(k_sum_rows, dim3(29), dim3(NT), 0, c.stream, (const double*)bp, B.nob, 29, c.sc->red);
   return check_hip(c, hipGetLastError(), "k_obstacle_momenta");

@@ -154,7 +154,10 @@ int cup_move_body(cup_ctx* ctx, int id, const double center[3], double com_mass_
  * body's previous linear correction, J the inertia tensor; corrections F/V and J^-1 T;
  * udef -= F/V + (J^-1 T) x (x - CoM) on every cell of the obstacle blocks. out: mass,
  * the linear correction (3), the angular correction (3), J0..J5 (xx, yy, zz, xy, xz, yz).
- * The penalization, momentum and force calls take the body's udef as zero. */
+ * Once set, udef enters the penalization's target velocity V + Omega x p + udef
+ * (Penalization.cpp:74-80, also in cup_run_steps) and the penalized momenta's u - udef
+ * (ObstaclesUpdate.cpp:117-126); the surface forces' deformation powers take it as zero, and
+ * cup_adjoint_step_body refuses a body that has one (CUP_E_STATE). */
 int cup_body_obstacle_blocks(cup_ctx* ctx, int id, int* ids_out, int cap, int* n_out);
 int cup_body_set_udef(cup_ctx* ctx, int id, const double* udef, int n);
 int cup_body_get_udef(cup_ctx* ctx, int id, double* udef, int n);

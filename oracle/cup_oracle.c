@@ -1581,9 +1581,9 @@ void orc_penalize(orc_sim* s, double dt, double lambda, int implicit,
           double p[3];
           cell_pos(s, bc, ix, iy, iz, p);
           p[0] -= s->CoM[0]; p[1] -= s->CoM[1]; p[2] -= s->CoM[2];
-          const double U_TOT[3] = {vel[0] + omega[1] * p[2] - omega[2] * p[1],
-                                   vel[1] + omega[2] * p[0] - omega[0] * p[2],
-                                   vel[2] + omega[0] * p[1] - omega[1] * p[0]};
+          const double U_TOT[3] = {vel[0] + omega[1] * p[2] - omega[2] * p[1] + o->udef[0][iz][iy][ix],
+                                   vel[1] + omega[2] * p[0] - omega[0] * p[2] + o->udef[1][iz][iy][ix],
+                                   vel[2] + omega[0] * p[1] - omega[1] * p[0] + o->udef[2][iz][iy][ix]};
           const double penalFac = implicit ? X * lambdaFac / (1 + lambdaFac * dt * X) : X * lambdaFac;
           const double FPX = penalFac * (U_TOT[0] - e->u);
           const double FPY = penalFac * (U_TOT[1] - e->v);
@@ -1655,10 +1655,12 @@ void orc_obstacle_momenta(const orc_sim* s, double dt, double lambda, int implic
             M[20] -= pf * p[0] * p[1];
             M[21] -= pf * p[0] * p[2];
             M[22] -= pf * p[1] * p[2];
-            M[23] += pf * e->u; M[24] += pf * e->v; M[25] += pf * e->w;
-            M[26] += pf * (p[1] * e->w - p[2] * e->v);
-            M[27] += pf * (p[2] * e->u - p[0] * e->w);
-            M[28] += pf * (p[0] * e->v - p[1] * e->u);
+            const double DU = e->u - o->udef[0][iz][iy][ix], DV = e->v - o->udef[1][iz][iy][ix],
+                         DW = e->w - o->udef[2][iz][iy][ix];  /* ObstaclesUpdate.cpp:117-126 */
+            M[23] += pf * DU; M[24] += pf * DV; M[25] += pf * DW;
+            M[26] += pf * (p[1] * DW - p[2] * DV);
+            M[27] += pf * (p[2] * DU - p[0] * DW);
+            M[28] += pf * (p[0] * DV - p[1] * DU);
           }
         }
   }

@@ -235,3 +235,37 @@ def test_udef_momentum_removal_matches_oracle(cup, oracle):
         assert not a.any()
     with pytest.raises(ValueError):
         G.set_udef(2, *udef)
+
+
+@pytest.mark.parametrize("decomp", ["plain", "fused"])
+def test_udef_in_penalization_and_momenta(cup, oracle, decomp):
+    """A sphere with a deformation velocity: the penalization's target V + Omega x p + udef and
+    the penalized momenta's u - udef against the oracle, then 3 steps of the device loop with the
+    body (the loop's penalization takes udef too), FP64."""
+    print(f"udef penalization decomp={decomp}", flush=True)
+    rng = np.random.default_rng(23)
+    bpd, ext, c, R = (4, 4, 4), 1.0, (0.47, 0.52, 0.55), 0.2
+    G = cup.Grid(bpd, precision=8, maxextent=ext, decomposition=decomp)
+    O = oracle.OracleSim(bpd, maxextent=ext)
+    G.chi_from_sphere(c, R)
+    O.sphere_create(R, c)
+    udef = tuple(0.3 * rng.standard_normal(G.shape) for _ in range(3))
+    u = tuple(rng.standard_normal(G.shape) for _ in range(3))
+    G.set_udef(0, *udef); O.set_udef(*udef)
+    G.set_velocity(*u); O.set_velocity(*u)
+    mg, mo = G.obstacle_momenta(1e-3, 1e6, True), O.obstacle_momenta(1e-3, 1e6, True)
+    np.testing.assert_allclose(mg, mo, rtol=1e-11, atol=1e-13 * np.abs(mo).max())
+    V, W = (0.3, -0.2, 0.1), (0.5, 0.0, -0.4)
+    fg = G.penalize(1e-3, 1e6, True, V, W)
+    fo = O.penalize(1e-3, 1e6, True, V, W)
+    np.testing.assert_allclose(fg, fo, rtol=1e-11, atol=1e-13 * np.abs(fo).max())
+    for a, b in zip(G.velocity(), O.velocity()):
+        assert rel(a, b) < 1e-13
+    sph = dict(lambda_=1e6, implicit=True, trans_vel=V, ang_vel=W)
+    G.run_steps(3, nu=1e-3, fixed_dt=2e-3, sphere=sph)
+    for _ in range(3):
+        O.timestep(2e-3, 1e-3, sphere=sph)
+    for a, b in zip(G.velocity(), O.velocity()):
+        assert rel(a, b) < 1e-11
+    with pytest.raises(cup.CupError, match="udef"):  # the body adjoint does not differentiate udef
+        G.adjoint_step_body(1e-3, 1e-3, (0, 0, 0), sph)

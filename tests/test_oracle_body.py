@@ -173,3 +173,31 @@ def test_udef_removal_leaves_no_momentum(oracle):
     # cells outside the body's obstacle blocks hold no udef
     far = after[0][0, 0, 0], after[0][-1, -1, -1]
     assert far == (0.0, 0.0)
+
+
+def test_udef_is_the_penalization_target_in_the_stiff_limit(oracle):
+    """Implicit penalization with a huge lambda drives u to the target V + udef on cells with
+    chi = 1 (u_new = u + dt pf (U_tot - u), pf = lambda/(1 + lambda dt) -> 1/dt): a dropped
+    udef or a wrong sign leaves u at V or V - udef there. The penalized momenta see u - udef:
+    the Gu terms for (u, udef) equal those for (u - udef, 0)."""
+    rng = np.random.default_rng(19)
+    O = oracle.OracleSim((4, 4, 4), maxextent=1.0)
+    O.sphere_create(0.25, (0.5, 0.5, 0.5))
+    udef = tuple(0.5 * rng.standard_normal(O.shape) for _ in range(3))
+    u = tuple(rng.standard_normal(O.shape) for _ in range(3))
+    O.set_udef(*udef)
+    O.set_velocity(*u)
+    V = (0.3, -0.2, 0.1)
+    O.penalize(1e-3, 1e15, True, V)
+    chi = O.get("chi")
+    inside = chi == 1.0
+    assert inside.sum() > 500
+    for q, a in enumerate(O.velocity()):
+        assert np.abs(a[inside] - (V[q] + udef[q][inside])).max() < 1e-9
+    O.set_velocity(*u)
+    m1 = O.obstacle_momenta(1e-3, 1e6, True)
+    O.set_udef(*(np.zeros(O.shape),) * 3)
+    O.set_velocity(*(a - b for a, b in zip(u, udef)))
+    m0 = O.obstacle_momenta(1e-3, 1e6, True)
+    np.testing.assert_allclose(m1[23:29], m0[23:29], rtol=1e-12, atol=1e-14 * np.abs(m0[23:29]).max())
+    np.testing.assert_allclose(m1[13:23], m0[13:23], rtol=1e-14)
