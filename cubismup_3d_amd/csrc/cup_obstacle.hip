@@ -1015,10 +1015,11 @@ int launch_udef_remove(Ctx& c, double out[13])
 // ---------------------------------------------------------------- surface forces
 
 // KernelComputeForces::visit (FluidSolidForces.cpp:42-157) on one obstacle block:
-// 22 QoIs in ObstacleBlock::sumQoI order (ObstacleBlock.h:81-93), udef = 0.
+// 22 QoIs in ObstacleBlock::sumQoI order (ObstacleBlock.h:81-93); the deformation powers
+// (defPower, defPowerBnd, FluidSolidForces.cpp:110-112,145-147) from the body's udef when set.
 template <typename T>
 __global__ __launch_bounds__(NT) void k_surface_forces(const int* __restrict__ ob_ids, const double* __restrict__ surf,
-                                                       const T* __restrict__ u, const T* __restrict__ v,
+                                                       const double* __restrict__ ob_udef, const T* __restrict__ u, const T* __restrict__ v,
                                                        const T* __restrict__ w, const T* __restrict__ p, GridArgs g,
                                                        double nu, double vx, double vy, double vz, double ox,
                                                        double oy, double oz, double cmx, double cmy, double cmz,
@@ -1073,6 +1074,12 @@ __global__ __launch_bounds__(NT) void k_surface_forces(const int* __restrict__ o
     const double powOut = fXT * uX + fYT * uY + fZT * uZ;
     Q[17] += powOut;
     Q[18] += fmin(0.0, powOut);
+    if (ob_udef) {
+      const double* D = ob_udef + (size_t)s * 3 * BS3;
+      const double powDef = fXT * D[c] + fYT * D[BS3 + c] + fZT * D[2 * BS3 + c];
+      Q[19] += powDef;
+      Q[20] += fmin(0.0, powDef);
+    }
     const double rx = px - cmx, ry = py - cmy, rz = pz - cmz;
     Q[21] += fXT * (vx + oy * rz - ry * oz) + fYT * (vy + oz * rx - rz * ox) + fZT * (vz + ox * ry - rx * oy);
   }
@@ -1085,7 +1092,7 @@ static int surface_forces_t(Ctx& c, double nu, const double vel[3], const double
   Body& B = c.body;
   double* bp = c.partials + 2 * (size_t)c.nb;
   hipLaunchKernelGGL((k_surface_forces<T>), dim3(B.nob), dim3(NT), 0, c.stream, B.ob_ids, B.ob_surf,
-                     (const T*)c.f[CUP_U], (const T*)c.f[CUP_V], (const T*)c.f[CUP_W], (const T*)c.f[CUP_P], c.grid(),
+                     (const double*)B.ob_udef, (const T*)c.f[CUP_U], (const T*)c.f[CUP_V], (const T*)c.f[CUP_W], (const T*)c.f[CUP_P], c.grid(),
                      nu, vel[0], vel[1], vel[2], om[0], om[1], om[2], B.com[0], B.com[1], B.com[2], bp);
   hipLaunchKernelGGL(k_sum_rows, dim3(22), dim3(NT), 0, c.stream, (const double*)bp, B.nob, 22, c.sc->red);
   return check_hip(c, hipGetLastError(), "k_surface_forces");

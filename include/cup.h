@@ -156,8 +156,9 @@ int cup_move_body(cup_ctx* ctx, int id, const double center[3], double com_mass_
  * the linear correction (3), the angular correction (3), J0..J5 (xx, yy, zz, xy, xz, yz).
  * Once set, udef enters the penalization's target velocity V + Omega x p + udef
  * (Penalization.cpp:74-80, also in cup_run_steps) and the penalized momenta's u - udef
- * (ObstaclesUpdate.cpp:117-126); the surface forces' deformation powers take it as zero, and
- * cup_adjoint_step_body refuses a body that has one (CUP_E_STATE). */
+ * (ObstaclesUpdate.cpp:117-126) and the surface forces' deformation powers f . udef
+ * (FluidSolidForces.cpp:110-112,145-147); cup_adjoint_step_body refuses a body that has one
+ * (CUP_E_STATE). */
 int cup_body_obstacle_blocks(cup_ctx* ctx, int id, int* ids_out, int cap, int* n_out);
 int cup_body_set_udef(cup_ctx* ctx, int id, const double* udef, int n);
 int cup_body_get_udef(cup_ctx* ctx, int id, double* udef, int n);

@@ -1729,7 +1729,8 @@ void orc_surface_forces(const orc_sim* s, double nu, const double uTrans[3], con
         Q[16] += .5 * (forcePar + fabs(forcePar));
         Q[15] -= .5 * (forcePar - fabs(forcePar));
         const double powOut = fXT * vX + fYT * vY + fZT * vZ;
-        const double powDef = 0; /* udef = 0 */
+        const double powDef = fXT * o->udef[0][iz][iy][ix] + fYT * o->udef[1][iz][iy][ix] +
+                              fZT * o->udef[2][iz][iy][ix]; /* FluidSolidForces.cpp:110-112,145 */
         Q[17] += powOut;
         Q[18] += powOut < 0 ? powOut : 0;
         Q[19] += powDef;

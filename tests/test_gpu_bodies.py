@@ -261,6 +261,11 @@ def test_udef_in_penalization_and_momenta(cup, oracle, decomp):
     np.testing.assert_allclose(fg, fo, rtol=1e-11, atol=1e-13 * np.abs(fo).max())
     for a, b in zip(G.velocity(), O.velocity()):
         assert rel(a, b) < 1e-13
+    P = rng.standard_normal(G.shape)  # a pressure field on both (the fused RHS reads u*, not u)
+    G.set("p", P); O.set("p", P)
+    qg, qo = G.surface_forces(1e-3, V, W), O.surface_forces(1e-3, V, W)
+    assert abs(qo[19]) > 0 and qo[20] < 0  # the deformation powers f . udef
+    np.testing.assert_allclose(qg, qo, rtol=1e-10, atol=1e-12 * np.abs(qo).max())
     sph = dict(lambda_=1e6, implicit=True, trans_vel=V, ang_vel=W)
     G.run_steps(3, nu=1e-3, fixed_dt=2e-3, sphere=sph)
     for _ in range(3):

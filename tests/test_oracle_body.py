@@ -201,3 +201,23 @@ def test_udef_is_the_penalization_target_in_the_stiff_limit(oracle):
     m0 = O.obstacle_momenta(1e-3, 1e6, True)
     np.testing.assert_allclose(m1[23:29], m0[23:29], rtol=1e-12, atol=1e-14 * np.abs(m0[23:29]).max())
     np.testing.assert_allclose(m1[13:23], m0[13:23], rtol=1e-14)
+
+
+def test_deformation_power_of_a_uniform_udef(oracle):
+    """With a uniform udef D the deformation power sum f . udef is the total force dotted with D
+    (QoI 19 = QoIs 0..2 . D), and its bounded variant (QoI 20) is the sum of the negative parts,
+    between it and 0; without udef both are 0."""
+    rng = np.random.default_rng(29)
+    O = oracle.OracleSim((4, 4, 4), maxextent=1.0)
+    O.sphere_create(0.2, (0.47, 0.52, 0.55))
+    O.set_velocity(*(rng.standard_normal(O.shape) for _ in range(3)))
+    O.pressure_rhs(1e-3)
+    O.poisson_solve()
+    q0 = O.surface_forces(1e-3, (0.3, 0, 0))
+    assert q0[19] == 0 and q0[20] == 0
+    D = np.array([0.7, -1.3, 0.4])
+    O.set_udef(*(np.full(O.shape, d) for d in D))
+    q = O.surface_forces(1e-3, (0.3, 0, 0))
+    assert q[19] == pytest.approx(float(np.dot(q[0:3], D)), rel=1e-12)
+    assert min(q[19], 0.0) >= q[20] and q[20] < 0
+    np.testing.assert_array_equal(q[:19], q0[:19])
