@@ -72,6 +72,8 @@ SIGNATURES = {
     "cup_body_count": (_i, [_p, C.POINTER(C.c_int)]),
     "cup_add_sphere": (_i, [_p, _dp, _d, _i, _dp, C.POINTER(C.c_int)]),
     "cup_move_body": (_i, [_p, _i, _dp, _dp]),
+    "cup_body_velocity": (_i, [_p, _i, _dp, _dp]),
+    "cup_last_body_force_body": (_i, [_p, _i, _dp]),
     "cup_body_obstacle_blocks": (_i, [_p, _i, C.POINTER(C.c_int), _i, C.POINTER(C.c_int)]),
     "cup_body_set_udef": (_i, [_p, _i, _dp, _i]),
     "cup_body_get_udef": (_i, [_p, _i, _dp, _i]),

@@ -68,6 +68,9 @@ static void release(Ctx& c)
   if (c.low_rows) hipFree(c.low_rows);
   if (c.adjf) hipFree(c.adjf);
   c.adjf = nullptr;
+  if (c.body_ft) hipFree(c.body_ft);
+  c.body_ft = nullptr;
+  c.body_ft_n = 0;
   c.low_dev = nullptr;
   c.low_rows = nullptr;
   c.low_part = nullptr; c.low_red = nullptr; c.low_counter = nullptr;
@@ -722,9 +725,16 @@ static int plain_step(cup_ctx* ctx, const cup_step_params* p)
   if (int rc = cup_pressure_rhs(ctx, dt)) return rc;
   if (int rc = cup_poisson_periodic(ctx)) return rc;
   if (int rc = cup_pressure_project(ctx, dt)) return rc;
-  if (p->has_body && c.body.present)
+  if (p->has_body && c.body.present) {
     if (int rc = cup_penalize(ctx, dt, p->lambda, p->implicit_penalization, p->body_vel, p->body_angvel, nullptr))
       return rc;
+    for (int k = 1; k < body_count(c) && k < c.body_ft_n; ++k) {  // dt on the device from body 0's call
+      BodyScope scope(c, k);
+      if (int rc = launch_penalize(c, p->lambda, p->implicit_penalization, c.body.vel, c.body.vel + 3,
+                                   c.body_ft + 6 * k))
+        return rc;
+    }
+  }
   c.sc_host->step += 1;
   c.sc_host->time += dt;
   c.sc_host->dt = dt;
@@ -738,8 +748,15 @@ int cup_run_steps(cup_ctx* ctx, const cup_step_params* p, int nsteps)
   if (!p || nsteps < 0) return set_error(c, CUP_E_INVALID, "run_steps: bad argument");
   if (!(p->CFL > 0) && !(p->fixed_dt > 0)) return set_error(c, CUP_E_INVALID, "run_steps: need CFL > 0 or fixed_dt");
   if (p->has_body && !c.body.present) return set_error(c, CUP_E_STATE, "run_steps: has_body without obstacle");
-  if (p->has_body && body_count(c) > 1)
-    return set_error(c, CUP_E_STATE, "run_steps: the device-resident loop takes one body");
+  const int nbody = p->has_body ? body_count(c) : 0;
+  if (nbody > 1 && c.body_ft_n < nbody) {  // the other bodies' forces of the last step
+    if (c.body_ft) hipFree(c.body_ft);
+    c.body_ft = nullptr;
+    c.body_ft_n = 0;
+    if (check_hip(c, hipMalloc(&c.body_ft, 6 * (size_t)nbody * sizeof(double)), "hipMalloc body forces"))
+      return CUP_E_NOMEM;
+    c.body_ft_n = nbody;
+  }
   if (c.decomposition == CUP_PLAIN) {
     if (int rc = read_scalars(c)) return rc;
     for (int s = 0; s < nsteps; ++s)
@@ -771,6 +788,12 @@ int cup_run_steps(cup_ctx* ctx, const cup_step_params* p, int nsteps)
     if (body)
       if (int rc = launch_penalize_step(c, p->lambda, p->implicit_penalization, p->body_vel, p->body_angvel, p->uinf))
         return rc;
+    for (int k = 1; k < nbody; ++k) {  // the other bodies, each with its cup_body_velocity
+      BodyScope scope(c, k);
+      if (int rc = launch_penalize_step(c, p->lambda, p->implicit_penalization, c.body.vel, c.body.vel + 3, p->uinf,
+                                        c.body_ft + 6 * k))
+        return rc;
+    }
     if (int rc = launch_finalize_step(c, p->CFL, p->nu, p->rampup, p->fixed_dt, 1)) return rc;
   }
   return CUP_OK;
@@ -959,6 +982,32 @@ int cup_adjoint_step_body(cup_ctx* ctx, double dt, double nu, const double uinf[
   if (dJdc)
     for (int d = 0; d < 3; ++d) dJdc[d] = c.sc_host->red[11 + d] + c.sc_host->red[14 + d];
   return CUP_OK;
+}
+
+int cup_body_velocity(cup_ctx* ctx, int id, const double vel[3], const double angvel[3])
+{
+  CUP_GUARD(ctx);
+  if (int rc = body_arg(c, id, "body_velocity")) return rc;
+  if (!vel || !angvel) return set_error(c, CUP_E_INVALID, "body_velocity: null pointer");
+  BodyScope scope(c, id);
+  for (int d = 0; d < 3; ++d) {
+    c.body.vel[d] = vel[d];
+    c.body.vel[3 + d] = angvel[d];
+  }
+  return CUP_OK;
+}
+
+int cup_last_body_force_body(cup_ctx* ctx, int id, double out[6])
+{
+  CUP_GUARD(ctx);
+  if (int rc = body_arg(c, id, "last_body_force_body")) return rc;
+  if (!out) return set_error(c, CUP_E_INVALID, "last_body_force_body: null pointer");
+  if (id == 0) return cup_last_body_force(ctx, out);
+  if (id >= c.body_ft_n) return set_error(c, CUP_E_STATE, "last_body_force_body: no loop step with this body yet");
+  if (int rc = check_hip(c, hipMemcpyAsync(out, c.body_ft + 6 * id, 6 * sizeof(double), hipMemcpyDeviceToHost,
+                                           c.stream), "last_body_force_body"))
+    return rc;
+  return check_hip(c, hipStreamSynchronize(c.stream), "last_body_force_body");
 }
 
 int cup_last_body_force(cup_ctx* ctx, double out[6])

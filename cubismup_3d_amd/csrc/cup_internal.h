@@ -92,6 +92,7 @@ struct Body {
   std::vector<int> ids_host;    // the obstacle blocks' ids, to reuse the buffers when a moved body keeps them
   double* ob_udef = nullptr;    // device: nob*3*512 deformation velocity [slot][component][cell] (lazy; 0 when absent)
   double corr[6] = {0, 0, 0, 0, 0, 0};  // transVel_correction, angVel_correction of the last udef removal
+  double vel[6] = {0, 0, 0, 0, 0, 0};   // velocity and rotation the device-resident loop penalizes with (bodies >= 1)
 };
 
 struct Ctx {
@@ -131,6 +132,8 @@ struct Ctx {
   double* low_coef = nullptr;  // per mode: RHS addition (re,im), synthesis of u,v,w (re,im) x3
   unsigned* low_counter = nullptr;
   void* low_rows = nullptr;    // per grid row (y, z): 16 synthesis coefficients (context precision)
+  double* body_ft = nullptr;   // device: 6 per body, each body's last penalization force/torque in the loop
+  int body_ft_n = 0;
   void* adjf = nullptr;        // the forcing adjoint's 9 fields: n P lam, n P u*, u_n kept aside (lazy)
   hipStream_t side = nullptr;  // the forcing row coefficients' stream (default priority)
   hipEvent_t ev_fork = nullptr, ev_rows = nullptr;
@@ -259,9 +262,10 @@ struct BodyScope {
 inline int body_count(const Ctx& c) { return c.body.present ? 1 + (int)c.more.size() : 0; }
 void bodies_free(Ctx& c);
 void body_free(Ctx& c);
-int launch_penalize(Ctx& c, double lambda, int implicit, const double vel[3], const double omega[3]);
+int launch_penalize(Ctx& c, double lambda, int implicit, const double vel[3], const double omega[3],
+                    double* ft_dev = nullptr);
 int launch_penalize_step(Ctx& c, double lambda, int implicit, const double vel[3], const double omega[3],
-                         const double uinf[3]);  // also refreshes the fused step's max/energy partials
+                         const double uinf[3], double* ft_dev = nullptr);  // also refreshes the fused step's max/energy partials
 int launch_surface_forces(Ctx& c, double nu, const double vel[3], const double omega[3]);
 int launch_obstacle_momenta(Ctx& c, double dt, double lambda, int implicit);
 int launch_udef_remove(Ctx& c, double out[13]);  // the active body's udef momentum removal (synchronous)

@@ -781,7 +781,7 @@ __global__ __launch_bounds__(NT) void k_sum_rows(const double* __restrict__ part
 
 template <typename T>
 static int penalize_t(Ctx& c, double lambda, int implicit, const double vel[3], const double om[3],
-                      const double uinf[3], bool refresh)
+                      const double uinf[3], bool refresh, double* ft_dev = nullptr)
 {
   Body& B = c.body;
   double* bp = c.partials + 2 * (size_t)c.nb;
@@ -790,22 +790,23 @@ static int penalize_t(Ctx& c, double lambda, int implicit, const double vel[3], 
                      (T*)c.f[CUP_U], (T*)c.f[CUP_V], (T*)c.f[CUP_W], c.grid(), c.sc, lambda, implicit, vel[0], vel[1],
                      vel[2], om[0], om[1], om[2], B.com[0], B.com[1], B.com[2], uinf[0], uinf[1], uinf[2], bp,
                      refresh ? c.ke_part : (double*)nullptr);
-  hipLaunchKernelGGL(k_sum_rows, dim3(6), dim3(NT), 0, c.stream, (const double*)bp, B.nob, 6, c.sc->red);
+  hipLaunchKernelGGL(k_sum_rows, dim3(6), dim3(NT), 0, c.stream, (const double*)bp, B.nob, 6,
+                     ft_dev ? ft_dev : c.sc->red);
   return check_hip(c, hipGetLastError(), "k_penalize");
 }
 
-int launch_penalize(Ctx& c, double lambda, int implicit, const double vel[3], const double omega[3])
+int launch_penalize(Ctx& c, double lambda, int implicit, const double vel[3], const double omega[3], double* ft_dev)
 {
   const double zero[3] = {0, 0, 0};
-  return c.precision == 4 ? penalize_t<float>(c, lambda, implicit, vel, omega, zero, false)
-                          : penalize_t<double>(c, lambda, implicit, vel, omega, zero, false);
+  return c.precision == 4 ? penalize_t<float>(c, lambda, implicit, vel, omega, zero, false, ft_dev)
+                          : penalize_t<double>(c, lambda, implicit, vel, omega, zero, false, ft_dev);
 }
 
 int launch_penalize_step(Ctx& c, double lambda, int implicit, const double vel[3], const double omega[3],
-                         const double uinf[3])
+                         const double uinf[3], double* ft_dev)
 {
-  return c.precision == 4 ? penalize_t<float>(c, lambda, implicit, vel, omega, uinf, true)
-                          : penalize_t<double>(c, lambda, implicit, vel, omega, uinf, true);
+  return c.precision == 4 ? penalize_t<float>(c, lambda, implicit, vel, omega, uinf, true, ft_dev)
+                          : penalize_t<double>(c, lambda, implicit, vel, omega, uinf, true, ft_dev);
 }
 
 // ------------------------------------------------------------ body momenta

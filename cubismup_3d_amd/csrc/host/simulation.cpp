@@ -1110,7 +1110,7 @@ bool Simulation::advance(int n)
 {
   // the device-resident loop holds a body still and has no uniform forcing: those steps take
   // the pipeline (it writes no per-step files either)
-  bool moving = sim.obstacles.size() > 1;  // the device-resident loop takes one body
+  bool moving = false;
   for (const auto& s : sim.obstacles) moving = moving || !s->isStatic();
   const bool external = sim.uMax_forced > 0 && sim.initCond != "taylorGreen";
   if (sim.decomposition != CUP_FUSED || moving || external) {
@@ -1121,10 +1121,17 @@ bool Simulation::advance(int n)
   // The frame velocity is set by the first step's CreateObstacles, after that step's dt
   // was computed with the previous one: that step goes through the pipeline when they differ.
   if (!sim.obstacles.empty() && n > 0) {
-    const auto& s = *sim.obstacles[0];
-    bool same = true;
-    for (int d = 0; d < 3; ++d) same = same && sim.uinf[d] == (s.bFixFrameOfRef[d] ? -s.transVel[d] : 0.0);
-    if (!same) {
+    std::array<double, 3> u = {0, 0, 0};  // CreateObstacles' frame velocity (ObstacleVector.h:60-74)
+    int nfix[3] = {0, 0, 0};
+    for (const auto& sp : sim.obstacles)
+      for (int d = 0; d < 3; ++d)
+        if (sp->bFixFrameOfRef[d]) {
+          ++nfix[d];
+          u[d] -= sp->transVel[d];
+        }
+    for (int d = 0; d < 3; ++d)
+      if (nfix[d]) u[d] /= nfix[d];
+    if (u != sim.uinf) {
       if (timestep(calcMaxTimestep())) return true;
       --n;
     }
@@ -1150,14 +1157,18 @@ bool Simulation::advance(int n)
     }
   }
   if (n <= 0) return (sim.endTime > 0 && sim.time > sim.endTime) || (sim.nsteps != 0 && sim.step >= sim.nsteps);
+  for (size_t k = 1; k < sim.obstacles.size(); ++k)
+    if (cup_body_velocity(sim.ctx, (int)k, sim.obstacles[k]->transVel.data(), sim.obstacles[k]->angVel.data()) !=
+        CUP_OK)
+      throw std::runtime_error(std::string("advance: ") + cup_last_error(sim.ctx));
   if (cup_run_steps(sim.ctx, &p, n) != CUP_OK)
     throw std::runtime_error(std::string("advance: ") + cup_last_error(sim.ctx));
   syncStepState();
-  if (!sim.obstacles.empty()) {  // the body's last penalization force and current surface QoIs
-    auto& s = *sim.obstacles[0];
+  for (size_t k = 0; k < sim.obstacles.size(); ++k) {  // each body's last penalization force and surface QoIs
+    auto& s = *sim.obstacles[k];
     double ft[6], q[22];
-    if (cup_last_body_force(sim.ctx, ft) != CUP_OK ||
-        cup_surface_forces(sim.ctx, sim.nu, s.transVel.data(), s.angVel.data(), q) != CUP_OK)
+    if (cup_last_body_force_body(sim.ctx, (int)k, ft) != CUP_OK ||
+        cup_surface_forces_body(sim.ctx, (int)k, sim.nu, s.transVel.data(), s.angVel.data(), q) != CUP_OK)
       throw std::runtime_error(std::string("advance: ") + cup_last_error(sim.ctx));
     s.force = {ft[0], ft[1], ft[2]};
     s.torque = {ft[3], ft[4], ft[5]};

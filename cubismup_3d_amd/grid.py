@@ -282,6 +282,15 @@ class Grid:
         self._c(L.lib().cup_move_body(self._h, int(k), L.vec3(center), L.dptr(out)))
         return dict(com=out[:3].copy(), mass=out[3])
 
+    def body_velocity(self, k, trans_vel, ang_vel=(0, 0, 0)):
+        """The velocity and rotation run_steps penalizes body k >= 1 with (body 0 takes sphere=)."""
+        self._c(L.lib().cup_body_velocity(self._h, int(k), L.vec3(trans_vel), L.vec3(ang_vel)))
+
+    def last_body_force_body(self, k) -> np.ndarray:
+        out = np.zeros(6)
+        self._c(L.lib().cup_last_body_force_body(self._h, int(k), L.dptr(out)))
+        return out
+
     def obstacle_blocks(self, k=0) -> np.ndarray:
         """Block ids of body k's obstacle blocks, in the order its udef uses."""
         n = C.c_int()

@@ -140,7 +140,7 @@ int cup_chi_from_sphere_mollified(cup_ctx* ctx, const double center[3], double r
  * (CUP_E_INVALID otherwise) and returns its index; cup_move_body builds body id again at a
  * new centre. PressureRHS masks and corrects for every body; the _body variants penalize,
  * integrate forces and momenta for one. The device-resident loop (cup_run_steps with a body)
- * and the adjoint with a body take a single body. */
+ * penalizes every body (cup_body_velocity); the adjoint with a body takes a single body. */
 int cup_body_count(cup_ctx* ctx, int* n);
 int cup_add_sphere(cup_ctx* ctx, const double center[3], double radius, int hemi, double com_mass_out[4], int* id);
 int cup_move_body(cup_ctx* ctx, int id, const double center[3], double com_mass_out[4]);
@@ -286,6 +286,13 @@ int cup_adjoint_step_body(cup_ctx* ctx, double dt, double nu, const double uinf[
                           const double body_vel[3], double wdrag, double dJdV[3], double dJdc[3]);
 /* Penalization force and torque (6 values) of the last step cup_run_steps made with a body. */
 int cup_last_body_force(cup_ctx* ctx, double out[6]);
+/* Several bodies in cup_run_steps (has_body): body 0 is penalized with the params' body_vel and
+ * body_angvel, every other body with the velocity and rotation last given here (default 0),
+ * each after the projection (Penalization::operator(), Penalization.cpp:129-156, for each
+ * obstacle). cup_last_body_force_body gives body id's force and torque of the loop's last
+ * step (id 0: cup_last_body_force). */
+int cup_body_velocity(cup_ctx* ctx, int id, const double vel[3], const double angvel[3]);
+int cup_last_body_force_body(cup_ctx* ctx, int id, double out[6]);
 
 /* Measurement: the stream the context launches on, and HIP events recorded on it around
  * one stage of the FUSED step (0 advection-diffusion, 1 x-forward FFT with the RHS,

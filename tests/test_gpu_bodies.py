@@ -95,9 +95,7 @@ def test_overlapping_bodies_are_rejected(cup):
     with pytest.raises(ValueError, match="overlap"):
         G.add_sphere((0.7, 0.5, 0.5), 0.15)
     assert G.body_count() == 1
-    with pytest.raises(cup.CupError):  # the device-resident loop takes one body
-        G.add_sphere((1.5, 0.5, 0.5), 0.1)
-        G.run_steps(1, nu=1e-3, fixed_dt=1e-3, sphere=dict(lambda_=1e6, implicit=True, trans_vel=(0, 0, 0)))
+
 
 
 PERIODIC = ["-BC_x", "periodic", "-BC_y", "periodic", "-BC_z", "periodic", "-muteAll", "1",
@@ -274,3 +272,38 @@ def test_udef_in_penalization_and_momenta(cup, oracle, decomp):
         assert rel(a, b) < 1e-11
     with pytest.raises(cup.CupError, match="udef"):  # the body adjoint does not differentiate udef
         G.adjoint_step_body(1e-3, 1e-3, (0, 0, 0), sph)
+
+
+@pytest.mark.parametrize("decomp", ["plain", "fused"])
+def test_two_static_spheres_in_the_device_loop(cup, oracle, decomp):
+    """Two spheres held in place in a uniform inflow (both forced at -1 on x with the frame fixed
+    to them, so U_inf = 1 and neither moves): Simulation.run takes the device-resident loop
+    (cup_run_steps penalizing both, the second with cup_body_velocity) after the first step;
+    fields, each body's penalization force and surface QoIs against the oracle's pipeline."""
+    print(f"two static spheres device loop decomp={decomp}", flush=True)
+    nu, CFL, n, lam = 1e-3, 0.3, 4, 1e6
+    argv = ["-bpdx", str(BPD[0]), "-bpdy", str(BPD[1]), "-bpdz", str(BPD[2]), "-extentx", str(EXT), "-CFL",
+            str(CFL), "-nu", str(nu), "-nsteps", str(n), "-rampup", "0", "-lambda", str(lam),
+            "-implicitPenalization", "1", "-decomposition", decomp, "-initCond", "zero"] + PERIODIC
+    S = cup.Simulation(cup.SimulationData(argv=argv))
+    for c, R, hemi in BODIES:
+        S.add_obstacle(cup.Sphere(radius=R, position=c, enforcedVelocity=(-1.0, 0.0, 0.0), bForcedInSimFrame=True,
+                                  bFixFrameOfRef=True, bHemi=hemi))
+    S.run()
+    O = oracle.OracleSim(BPD, maxextent=EXT)
+    for c, R, hemi in BODIES:
+        O.add_sphere(R, c, hemi=hemi)
+    bodies = [dict(radius=R, position=list(c), transVel=[-1.0, 0.0, 0.0], angVel=[0.0] * 3, forced=[True] * 3,
+                   fix=[True] * 3, block_rot=[True] * 3, imposed=[-1.0, 0.0, 0.0]) for c, R, _ in BODIES]
+    uinf = [0.0] * 3
+    for step in range(n):
+        dt, _ = O.calc_max_timestep(CFL, nu, uinf, step, 0)
+        uinf, ft, q = O.bodies_timestep(dt, nu, bodies, lam, True)
+    assert uinf == [1.0, 0.0, 0.0]
+    for name in ("u", "v", "w"):
+        assert rel(S.field(name), O.get(name)) < 1e-10, name
+    for k in range(2):
+        ob = S.obstacle(k)
+        np.testing.assert_array_equal(ob["position"], BODIES[k][0])
+        np.testing.assert_allclose(ob["force"], ft[k][:3], rtol=1e-9, atol=1e-11 * np.abs(ft[k]).max())
+        np.testing.assert_allclose(ob["surface"], q[k], rtol=1e-8, atol=1e-10 * np.abs(q[k]).max())
