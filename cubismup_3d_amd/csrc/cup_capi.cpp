@@ -494,12 +494,24 @@ static void drop_other_bodies(Ctx& c)
   c.more.clear();
 }
 
+// A call that replaces every body by one and fails leaves no body: the buffers and merged tables
+// are freed and the grid chi cleared, so no later call sees a body half replaced.
+static int replace_failed(Ctx& c, int rc)
+{
+  const std::string msg = c.err;
+  bodies_free(c);
+  launch_fill(c, c.f[CUP_CHI], c.ncell, 0.0);
+  hipStreamSynchronize(c.stream);
+  c.err = msg;
+  return rc;
+}
+
 int cup_chi_from_sphere(cup_ctx* ctx, const double center[3], double radius, double com_mass_out[4])
 {
   CUP_GUARD(ctx);
   if (!center || !(radius > 0)) return set_error(c, CUP_E_INVALID, "chi_from_sphere: radius must be > 0");
   drop_other_bodies(c);
-  if (int rc = body_create(c, center, radius, 0.0)) return rc;
+  if (int rc = body_create(c, center, radius, 0.0)) return replace_failed(c, rc);
   if (com_mass_out) {
     for (int a = 0; a < 3; ++a) com_mass_out[a] = c.body.com[a];
     com_mass_out[3] = c.body.mass;
@@ -512,7 +524,7 @@ int cup_chi_from_hemisphere(cup_ctx* ctx, const double center[3], double radius,
   CUP_GUARD(ctx);
   if (!center || !(radius > 0)) return set_error(c, CUP_E_INVALID, "chi_from_hemisphere: radius must be > 0");
   drop_other_bodies(c);
-  if (int rc = body_create(c, center, radius, 0.0, true)) return rc;
+  if (int rc = body_create(c, center, radius, 0.0, true)) return replace_failed(c, rc);
   if (com_mass_out) {
     for (int a = 0; a < 3; ++a) com_mass_out[a] = c.body.com[a];
     com_mass_out[3] = c.body.mass;
@@ -528,7 +540,7 @@ int cup_chi_from_sphere_mollified(cup_ctx* ctx, const double center[3], double r
   if (!(width > 0 && width <= 2))
     return set_error(c, CUP_E_INVALID, "chi_from_sphere_mollified: the half-width must lie in (0, 2] cells");
   drop_other_bodies(c);
-  if (int rc = body_create(c, center, radius, width)) return rc;
+  if (int rc = body_create(c, center, radius, width)) return replace_failed(c, rc);
   if (com_mass_out) {
     for (int a = 0; a < 3; ++a) com_mass_out[a] = c.body.com[a];
     com_mass_out[3] = c.body.mass;

@@ -8,6 +8,7 @@
 
 #include <cfloat>
 #include <cmath>
+#include <limits>
 #include <vector>
 
 namespace cup {
@@ -269,18 +270,15 @@ int body_create(Ctx& c, const double center[3], double radius, double moll, bool
       ids.push_back(b);
     }
   }
-  if (ids.empty()) {
-    body_free(c);
-    return set_error(c, CUP_E_INVALID, "chi_from_sphere: the sphere touches no block");
-  }
+  // Both checks come before B is touched: a rejected placement (a new body, or a move of an
+  // existing one) leaves the body, its buffers and the merged tables as they were.
+  if (ids.empty()) return set_error(c, CUP_E_INVALID, "chi_from_sphere: the sphere touches no block");
   // the other bodies' obstacle blocks must be apart from this one's
   for (const Body& o : c.more)
     if (o.present)
       for (int b : ids)
-        if (std::find(o.ids_host.begin(), o.ids_host.end(), b) != o.ids_host.end()) {
-          body_free(c);
+        if (std::find(o.ids_host.begin(), o.ids_host.end(), b) != o.ids_host.end())
           return set_error(c, CUP_E_INVALID, "add_sphere: the bodies' obstacle blocks overlap (bodies too close)");
-        }
   const bool reuse = B.present && B.radius == radius && B.moll == moll && B.hemi == hemi && B.ids_host == ids;
   // a rebuilt body has no deformation velocity until it is set again; the last udef
   // corrections stay (they are the body's, as Obstacle::transVel_correction)
@@ -993,7 +991,14 @@ int launch_udef_remove(Ctx& c, double out[13])
   const double c0 = J[1] * J[2] - J[5] * J[5], c1 = J[0] * J[2] - J[4] * J[4], c2 = J[0] * J[1] - J[3] * J[3];
   const double c3 = J[4] * J[5] - J[3] * J[2], c4 = J[3] * J[5] - J[4] * J[1], c5 = J[3] * J[4] - J[0] * J[5];
   const double det = J[0] * c0 + J[3] * c3 + J[4] * c4;
-  const double iJ[6] = {c0 / det, c1 / det, c2 / det, c3 / det, c4 / det, c5 / det};
+  // a singular J (a body of one cell, or of collinear cells) has no angular correction, as
+  // invertSym returns a zero inverse for |det| <= DBL_MIN (MatArrayMath.h:18-38)
+  const bool singular = std::fabs(det) <= std::numeric_limits<double>::min();
+  double iJ[6] = {0, 0, 0, 0, 0, 0};
+  if (!singular) {
+    iJ[0] = c0 / det; iJ[1] = c1 / det; iJ[2] = c2 / det;
+    iJ[3] = c3 / det; iJ[4] = c4 / det; iJ[5] = c5 / det;
+  }
   const double ct[3] = {M[1] / M[0], M[2] / M[0], M[3] / M[0]};
   const double cw[3] = {iJ[0] * M[4] + iJ[3] * M[5] + iJ[4] * M[6], iJ[3] * M[4] + iJ[1] * M[5] + iJ[5] * M[6],
                         iJ[4] * M[4] + iJ[5] * M[5] + iJ[2] * M[6]};

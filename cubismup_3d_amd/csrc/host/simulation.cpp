@@ -382,11 +382,19 @@ void Sphere::setVelocities(const std::array<double, 3>& imposed)
   angVel = {0, 0, 0};
 }
 
-bool Sphere::isStatic() const
+bool Sphere::constantVelocity() const
 {
   if (accel_decel) return false;
   for (int d = 0; d < 3; ++d)
-    if (!bForcedInSimFrame[d] || !(bFixFrameOfRef[d] || transVel_imposed[d] == 0)) return false;
+    if (!bForcedInSimFrame[d]) return false;
+  return true;
+}
+
+bool Sphere::isStatic() const
+{
+  if (!constantVelocity()) return false;
+  for (int d = 0; d < 3; ++d)
+    if (!(bFixFrameOfRef[d] || transVel_imposed[d] == 0)) return false;
   return true;
 }
 
@@ -529,12 +537,13 @@ void Sphere::computeVelocities(const double M[29], bool implicit, double time)
 // fixed to the frame, per axis), each body's update with the step's dt (Obstacle::update), then
 // chi, surface points and centre of mass where a body moved. Body k of the device is
 // sim.obstacles[k].
-void CreateObstacles::operator()(double dt)
+// ObstacleVector::updateUinf (ObstacleVector.h:60-74): per axis, minus the mean velocity of the
+// bodies fixed to the frame on that axis (0 when none is).
+static std::array<double, 3> frameVelocity(const std::vector<std::shared_ptr<Sphere>>& obstacles)
 {
-  if (sim.obstacles.empty()) return;
   std::array<double, 3> u = {0, 0, 0};
   int nfix[3] = {0, 0, 0};
-  for (const auto& sp : sim.obstacles)
+  for (const auto& sp : obstacles)
     for (int d = 0; d < 3; ++d)
       if (sp->bFixFrameOfRef[d]) {
         ++nfix[d];
@@ -542,7 +551,13 @@ void CreateObstacles::operator()(double dt)
       }
   for (int d = 0; d < 3; ++d)
     if (nfix[d]) u[d] /= nfix[d];
-  sim.uinf = u;
+  return u;
+}
+
+void CreateObstacles::operator()(double dt)
+{
+  if (sim.obstacles.empty()) return;
+  sim.uinf = frameVelocity(sim.obstacles);
   velHeader.resize(sim.obstacles.size(), 0);
   std::vector<int> moved;
   for (size_t k = 0; k < sim.obstacles.size(); ++k) {
@@ -1108,10 +1123,17 @@ void Simulation::syncStepState()
 
 bool Simulation::advance(int n)
 {
-  // the device-resident loop holds a body still and has no uniform forcing: those steps take
-  // the pipeline (it writes no per-step files either)
+  // The device-resident loop neither moves a body nor builds its chi again, and has no uniform
+  // forcing: it is taken only when every body keeps its imposed velocity and is at rest in the
+  // frame, transVel + uinf = 0 on every axis with uinf the bodies' frame velocity (a body at rest
+  // beside a towed one that fixes the frame moves, and so do frame-fixing bodies of different
+  // velocities); other steps take the pipeline, which moves the bodies.
+  const std::array<double, 3> uframe = frameVelocity(sim.obstacles);
   bool moving = false;
-  for (const auto& s : sim.obstacles) moving = moving || !s->isStatic();
+  for (const auto& s : sim.obstacles) {
+    moving = moving || !s->constantVelocity();
+    for (int d = 0; d < 3; ++d) moving = moving || s->transVel[d] + uframe[d] != 0;
+  }
   const bool external = sim.uMax_forced > 0 && sim.initCond != "taylorGreen";
   if (sim.decomposition != CUP_FUSED || moving || external) {
     for (int i = 0; i < n; ++i)
@@ -1121,17 +1143,7 @@ bool Simulation::advance(int n)
   // The frame velocity is set by the first step's CreateObstacles, after that step's dt
   // was computed with the previous one: that step goes through the pipeline when they differ.
   if (!sim.obstacles.empty() && n > 0) {
-    std::array<double, 3> u = {0, 0, 0};  // CreateObstacles' frame velocity (ObstacleVector.h:60-74)
-    int nfix[3] = {0, 0, 0};
-    for (const auto& sp : sim.obstacles)
-      for (int d = 0; d < 3; ++d)
-        if (sp->bFixFrameOfRef[d]) {
-          ++nfix[d];
-          u[d] -= sp->transVel[d];
-        }
-    for (int d = 0; d < 3; ++d)
-      if (nfix[d]) u[d] /= nfix[d];
-    if (u != sim.uinf) {
+    if (uframe != sim.uinf) {
       if (timestep(calcMaxTimestep())) return true;
       --n;
     }

@@ -110,7 +110,10 @@ struct Sphere {
   // The Obstacle constructor's velocity set-up (Obstacle.cpp:119-137): forced axes take the
   // imposed velocity, the others start at rest; any forced axis blocks the rotation.
   void setVelocities(const std::array<double, 3>& imposed);
-  // Every axis forced and at rest in the frame: the body never moves and keeps its velocity.
+  // Every axis forced, no ramp: the velocity stays at its imposed value and the rotation at 0.
+  bool constantVelocity() const;
+  // constantVelocity, and at rest in the frame when it is the only body: each axis is fixed to
+  // the frame or imposed at 0 (UpdateObstacles skips its momenta when logs are muted).
   bool isStatic() const;
   // Obstacle::update (Obstacle.cpp:314-345): position, absolute position and quaternion.
   void update(double dt, const std::array<double, 3>& uinf);

@@ -1517,7 +1517,12 @@ void orc_remove_udef_momenta(orc_sim* s, double out[13])
   const double c0 = J[1] * J[2] - J[5] * J[5], c1 = J[0] * J[2] - J[4] * J[4], c2 = J[0] * J[1] - J[3] * J[3];
   const double c3 = J[4] * J[5] - J[3] * J[2], c4 = J[3] * J[5] - J[4] * J[1], c5 = J[3] * J[4] - J[0] * J[5];
   const double det = J[0] * c0 + J[3] * c3 + J[4] * c4;
-  const double iJ[6] = {c0 / det, c1 / det, c2 / det, c3 / det, c4 / det, c5 / det};
+  /* invertSym (MatArrayMath.h:18-38): a zero inverse when |det| <= DBL_MIN */
+  double iJ[6] = {0, 0, 0, 0, 0, 0};
+  if (!(fabs(det) <= DBL_MIN)) {
+    iJ[0] = c0 / det; iJ[1] = c1 / det; iJ[2] = c2 / det;
+    iJ[3] = c3 / det; iJ[4] = c4 / det; iJ[5] = c5 / det;
+  }
   const double T[3] = {M[4], M[5], M[6]};
   double ct[3], cw[3];
   for (int d = 0; d < 3; d++) ct[d] = M[1 + d] / M[0];

@@ -97,6 +97,37 @@ def test_overlapping_bodies_are_rejected(cup):
     assert G.body_count() == 1
 
 
+def test_rejected_move_keeps_the_body(cup, oracle):
+    """A move refused because the body's obstacle blocks would overlap another's (or because it
+    would touch no block) leaves every body as it was: the count, each centre of mass and mass, the
+    obstacle blocks and the grid chi; a later valid move still matches the oracle."""
+    print("rejected move keeps the body", flush=True)
+    G, O = pair(cup, oracle)
+    chi0 = G.get("chi")
+    blocks = [G.obstacle_blocks(k) for k in range(2)]
+    with pytest.raises(ValueError, match="overlap"):
+        G.move_body(1, (0.75, 0.5, 0.48))
+    with pytest.raises(ValueError, match="no block"):
+        G.move_body(0, (50.0, 50.0, 50.0))
+    assert G.body_count() == 2
+    np.testing.assert_array_equal(G.get("chi"), chi0)
+    for k in range(2):
+        np.testing.assert_array_equal(G.obstacle_blocks(k), blocks[k])
+    m = G.move_body(1, (1.40, 0.52, 0.5))
+    O.move_body(1, (1.40, 0.52, 0.5))
+    with O.body(1):
+        info = O.sphere_info()
+    np.testing.assert_allclose(m["com"], info["com"], rtol=1e-13)
+    assert np.abs(G.get("chi") - O.get("chi")).max() < 1e-13
+    # the merged obstacle tables follow the bodies: the masked and corrected RHS of both
+    rng = np.random.default_rng(11)
+    u = tuple(0.3 * rng.standard_normal(G.shape) for _ in range(3))
+    G.set_velocity(*u); O.set_velocity(*u)
+    G.pressure_rhs(1e-3); O.pressure_rhs(1e-3)
+    G.poisson_periodic(); O.poisson_solve()
+    assert rel(G.get("p"), O.get("p")) < 1e-12
+
+
 
 PERIODIC = ["-BC_x", "periodic", "-BC_y", "periodic", "-BC_z", "periodic", "-muteAll", "1",
             "-freqDiagnostics", "0", "-precision", "double"]
@@ -272,6 +303,52 @@ def test_udef_in_penalization_and_momenta(cup, oracle, decomp):
         assert rel(a, b) < 1e-11
     with pytest.raises(cup.CupError, match="udef"):  # the body adjoint does not differentiate udef
         G.adjoint_step_body(1e-3, 1e-3, (0, 0, 0), sph)
+
+
+@pytest.mark.parametrize("case", ["towed_and_resting", "two_frame_fixing"])
+def test_spheres_that_move_in_the_frame_take_the_pipeline(cup, oracle, case):
+    """Every sphere forced at a constant velocity, yet some move relative to the grid: a resting
+    sphere beside a towed one that fixes the frame (U_inf = 1 carries it along x), or two spheres
+    fixing the frame at different x velocities (U_inf = 0.75, each moves). Simulation.run in the
+    fused decomposition must move and rebuild them every step as the oracle's pipeline does:
+    positions, chi, fields and each body's force after 4 CFL steps."""
+    print(f"moving forced spheres in the Simulation case={case}", flush=True)
+    nu, CFL, n, lam = 1e-3, 0.3, 4, 1e6
+    argv = ["-bpdx", str(BPD[0]), "-bpdy", str(BPD[1]), "-bpdz", str(BPD[2]), "-extentx", str(EXT), "-CFL",
+            str(CFL), "-nu", str(nu), "-nsteps", str(n), "-rampup", "0", "-lambda", str(lam),
+            "-implicitPenalization", "1", "-decomposition", "fused", "-initCond", "zero"] + PERIODIC
+    if case == "towed_and_resting":
+        vel = [(-1.0, 0.0, 0.0), (0.0, 0.0, 0.0)]
+        fix = [(True, True, True), (False, False, False)]
+    else:
+        vel = [(-1.0, 0.0, 0.0), (-0.5, 0.0, 0.0)]
+        fix = [(True, True, True), (True, True, True)]
+    S = cup.Simulation(cup.SimulationData(argv=argv))
+    for (c, R, hemi), v, f in zip(BODIES, vel, fix):
+        S.add_obstacle(cup.Sphere(radius=R, position=c, enforcedVelocity=v, bForcedInSimFrame=True,
+                                  bFixFrameOfRef=f, bHemi=hemi))
+    S.run()
+    O = oracle.OracleSim(BPD, maxextent=EXT)
+    for c, R, hemi in BODIES:
+        O.add_sphere(R, c, hemi=hemi)
+    bodies = [dict(radius=R, position=list(c), transVel=list(v), angVel=[0.0] * 3, forced=[True] * 3,
+                   fix=list(f), block_rot=[True] * 3, imposed=list(v))
+              for (c, R, _), v, f in zip(BODIES, vel, fix)]
+    uinf = [0.0] * 3
+    for step in range(n):
+        dt, _ = O.calc_max_timestep(CFL, nu, uinf, step, 0)
+        uinf, ft, q = O.bodies_timestep(dt, nu, bodies, lam, True)
+    assert uinf == pytest.approx([1.0 if case == "towed_and_resting" else 0.75, 0.0, 0.0])
+    moved = 0
+    for k, b in enumerate(bodies):
+        ob = S.obstacle(k)
+        np.testing.assert_allclose(ob["position"], b["position"], rtol=1e-14)
+        moved += np.abs(np.asarray(ob["position"]) - BODIES[k][0]).max() > 0
+        np.testing.assert_allclose(ob["force"], ft[k][:3], rtol=1e-9, atol=1e-11 * np.abs(ft[k]).max())
+    assert moved == (1 if case == "towed_and_resting" else 2)
+    assert np.abs(S.field("chi") - O.get("chi")).max() < 1e-13
+    for name in ("u", "v", "w"):
+        assert rel(S.field(name), O.get(name)) < 1e-10, name
 
 
 @pytest.mark.parametrize("decomp", ["plain", "fused"])
