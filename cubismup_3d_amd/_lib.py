@@ -18,6 +18,7 @@ CUP_E_DEVICE = -2
 CUP_E_STATE = -3
 CUP_E_NOMEM = -4
 CUP_E_IO = -5
+CUP_NOALLOC_MAX_PARTS = 32768
 
 CUP_CHI, CUP_U, CUP_V, CUP_W, CUP_P, CUP_TMPU, CUP_TMPV, CUP_TMPW = range(8)
 FIELD_IDS = {"chi": 0, "u": 1, "v": 2, "w": 3, "p": 4, "tmpU": 5, "tmpV": 6, "tmpW": 7,
@@ -84,6 +85,7 @@ SIGNATURES = {
     "cup_add_uniform_velocity": (_i, [_p, _dp]),
     "cup_ic_hit_spectrum": (_i, [_p, _dp, _dp, _i, _d, _u64]),
     "cup_diagnostics": (_i, [_p, _d, _dp]),
+    "cup_diagnostics_noalloc": (_i, [C.POINTER(GridDesc), C.POINTER(C.c_void_p), _d, _p, _p]),
     "cup_ic_taylor_green": (_i, [_p, _d]),
     "cup_ic_hit": (_i, [_p, _d, _d, _u64]),
     "cup_spectral_forcing": (_i, [_p, _d, _d, _d, _d, _d, _dp]),

@@ -122,7 +122,8 @@ using namespace cup;
   Ctx& c = (ctx)->c;                                   \
   if (hipSetDevice(c.dev) != hipSuccess) return set_error(c, CUP_E_DEVICE, "hipSetDevice");
 
-// Message of this thread's last failed cup_create (no context exists to hold it).
+// Message of this thread's last failed call that has no context to hold it (cup_create,
+// cup_diagnostics_noalloc).
 static thread_local std::string g_create_error;
 
 static int create_error(int code, const std::string& msg)
@@ -130,6 +131,8 @@ static int create_error(int code, const std::string& msg)
   g_create_error = msg;
   return code;
 }
+
+int cup::free_error(int code, const std::string& msg) { return create_error(code, msg); }
 
 extern "C" {
 

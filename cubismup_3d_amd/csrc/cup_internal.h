@@ -223,6 +223,8 @@ int launch_fill(Ctx& c, void* ptr, size_t n, double value);
 int launch_copy(Ctx& c, void* dst, const void* src, size_t n);
 int launch_set_dt(Ctx& c, double dt);
 int launch_reduce(Ctx& c, int n, int width, unsigned maxmask, double* out_dev, const double* part = nullptr);
+// Records the message of a failed call that has no context (read by cup_last_error(NULL)).
+int free_error(int code, const std::string& msg);
 int launch_reset_umax(Ctx& c);
 int launch_finalize_step(Ctx& c, double CFL, double nu, int rampup, double fixed_dt, int advance);
 // cup_fft.hip

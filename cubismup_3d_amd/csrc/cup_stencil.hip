@@ -8,6 +8,7 @@
 #include "cup_internal.h"
 
 #include <cmath>
+#include <mutex>
 
 namespace cup {
 
@@ -1229,7 +1230,8 @@ __global__ __launch_bounds__(NT) void k_diagnostics(const T* __restrict__ u, con
 }
 
 // Blocks per diagnostics workgroup: 8 when the x rows allow it.
-inline int diag_nbw(const Ctx& c) { return c.bpd[0] % 8 == 0 ? 8 : (c.bpd[0] % 4 == 0 ? 4 : (c.bpd[0] % 2 == 0 ? 2 : 1)); }
+inline int diag_nbw_x(int bpdx) { return bpdx % 8 == 0 ? 8 : (bpdx % 4 == 0 ? 4 : (bpdx % 2 == 0 ? 2 : 1)); }
+inline int diag_nbw(const Ctx& c) { return diag_nbw_x(c.bpd[0]); }
 
 template <typename T>
 static int diagnostics_t(Ctx& c, double nu)
@@ -1303,6 +1305,48 @@ int launch_diagnostics(Ctx& c, double nu, int* nparts)
   return c.precision == 4 ? diagnostics_t<float>(c, nu) : diagnostics_t<double>(c, nu);
 }
 
+// ------------------------------------------------- diagnostics without allocation
+
+// The per-workgroup partial sums of cup_diagnostics_noalloc: a file-scope device array, one
+// copy per GPU, held until the process ends. Host code never passes this name to a kernel:
+// each device's address comes from hipGetSymbolAddress (scratch_ptr below).
+static __device__ __align__(256) double scratch[(size_t)CUP_NOALLOC_MAX_PARTS * 20];
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+
+// The device address of `scratch` on the current device `dev`, looked up once per device.
+static int scratch_ptr(int dev, double** out)
+{
+  static std::mutex mu;
+  static double* ptr[SCRATCH_MAX_DEVICES] = {};
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return free_error(CUP_E_DEVICE, std::string("diagnostics_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e));
+    ptr[dev] = (double*)p;
+  }
+  *out = ptr[dev];
+  return CUP_OK;
+}
+
+template <typename T>
+static int diagnostics_noalloc_t(const void* const fields[5], const GridArgs& g, const double half_extent[3],
+                                 double nu, int nbw, double* part, hipStream_t s, double* out_dev)
+{
+  hipLaunchKernelGGL((k_diagnostics<T>), dim3(g.nb / nbw), dim3(NT), 0, s, (const T*)fields[CUP_U],
+                     (const T*)fields[CUP_V], (const T*)fields[CUP_W], (const T*)fields[CUP_P],
+                     (const T*)fields[CUP_CHI], g, nu, half_extent[0], half_extent[1], half_extent[2], part, nbw);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_reduce, dim3(20), dim3(NT), 0, s, (const double*)part, g.nb / nbw, 20, 0u, out_dev);
+    e = hipGetLastError();
+  }
+  if (e != hipSuccess) return free_error(CUP_E_DEVICE, std::string("diagnostics_noalloc: launch: ") + hipGetErrorString(e));
+  return CUP_OK;
+}
+
 // ----------------------------------------------------------- initial conditions
 
 // KernelIC_taylorGreen (InitialConditions.cpp:32-52): every field cleared, then
@@ -1346,3 +1390,43 @@ int launch_ic_taylor_green(Ctx& c, double uMax)
 }
 
 }  // namespace cup
+
+extern "C" int cup_diagnostics_noalloc(const cup_grid_desc* d, const void* const fields[5], double nu,
+                                       void* hip_stream, double* out_dev)
+{
+  using namespace cup;
+  if (!d || !fields || !out_dev) return free_error(CUP_E_INVALID, "diagnostics_noalloc: null argument");
+  for (int k = 0; k < 5; ++k)
+    if (!fields[k]) return free_error(CUP_E_INVALID, "diagnostics_noalloc: null field pointer");
+  for (int a = 0; a < 3; ++a)
+    if (d->bpd[a] < 1) return free_error(CUP_E_INVALID, "diagnostics_noalloc: blocks per dimension must be >= 1");
+  if (d->precision != 4 && d->precision != 8)
+    return free_error(CUP_E_INVALID, "diagnostics_noalloc: precision must be 4 (FP32) or 8 (FP64)");
+  if (!(d->maxextent > 0)) return free_error(CUP_E_INVALID, "diagnostics_noalloc: maxextent must be > 0");
+  if (d->device < 0 || d->device >= SCRATCH_MAX_DEVICES)
+    return free_error(CUP_E_INVALID, "diagnostics_noalloc: device ordinal out of range");
+  const long long nb = (long long)d->bpd[0] * d->bpd[1] * d->bpd[2];
+  const int nbw = diag_nbw_x(d->bpd[0]);
+  if (nb / nbw > CUP_NOALLOC_MAX_PARTS)
+    return free_error(CUP_E_INVALID, "diagnostics_noalloc: the grid needs " + std::to_string(nb / nbw) +
+                                         " partial sums, more than the static scratch holds (" +
+                                         std::to_string(CUP_NOALLOC_MAX_PARTS) + ")");
+  GridArgs g;
+  g.nbx = d->bpd[0]; g.nby = d->bpd[1]; g.nbz = d->bpd[2];
+  g.Nx = g.nbx * BS; g.Ny = g.nby * BS; g.Nz = g.nbz * BS;
+  g.nb = (int)nb;
+  const int maxN = std::max(g.Nx, std::max(g.Ny, g.Nz));
+  g.h = d->maxextent / maxN;
+  const double half_extent[3] = {0.5 * ((double)g.Nx / maxN) * d->maxextent, 0.5 * ((double)g.Ny / maxN) * d->maxextent,
+                                 0.5 * ((double)g.Nz / maxN) * d->maxextent};
+  int prev = 0;
+  if (hipGetDevice(&prev) != hipSuccess) return free_error(CUP_E_DEVICE, "diagnostics_noalloc: hipGetDevice");
+  if (hipSetDevice(d->device) != hipSuccess) return free_error(CUP_E_DEVICE, "diagnostics_noalloc: hipSetDevice");
+  double* part = nullptr;
+  int rc = scratch_ptr(d->device, &part);
+  if (!rc)
+    rc = d->precision == 4 ? diagnostics_noalloc_t<float>(fields, g, half_extent, nu, nbw, part, (hipStream_t)hip_stream, out_dev)
+                           : diagnostics_noalloc_t<double>(fields, g, half_extent, nu, nbw, part, (hipStream_t)hip_stream, out_dev);
+  hipSetDevice(prev);
+  return rc;
+}

@@ -196,6 +196,22 @@ int cup_add_uniform_velocity(cup_ctx* ctx, const double du[3]);
 /* KernelDissipation (ComputeDissipation.cpp:17-116): out[20] in the RDX order (:136-155). */
 int cup_diagnostics(cup_ctx* ctx, double nu, double out[20]);
 
+/* cup_diagnostics without any device allocation. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc). No context: fields[5] are the caller's device arrays chi,
+ * u, v, w, p (cup_field order), each nb*512 cells of desc->precision in the block-major layout
+ * of desc's grid (bpd, maxextent, device; decomposition ignored); out_dev receives the same 20
+ * integrals as cup_diagnostics, as device doubles. Stream-ordered on hip_stream (NULL: the
+ * null stream); nothing is copied to the host and nothing is synchronized. The per-workgroup
+ * partial sums live in a static device array of CUP_NOALLOC_MAX_PARTS x 20 doubles (5 MiB on
+ * each GPU the call runs on, held until the process ends); a grid that needs more partial
+ * sums (more than 262144 blocks when bpdx is a multiple of 8, 32768 when bpdx is odd) is
+ * refused with CUP_E_INVALID. Every call on a GPU shares that array: calls that overlap on
+ * one GPU (two streams, two threads) race. Errors: cup_last_error(NULL). */
+#define CUP_NOALLOC_MAX_PARTS 32768
+int cup_diagnostics_noalloc(const cup_grid_desc* desc, const void* const fields[5], double nu, void* hip_stream,
+                            double* out_dev);
+
 /* The HIT branch of Analysis::operator() (Analysis.cpp:159-196): KernelAnalysis_gradStats
  * (:18-52) and SpectralAnalysis::run, i.e. SpectralManipFFTW::_compute_analysis
  * (SpectralManipFFTW.cpp:110-207) on the velocity's forward FFT. out[9] = tke, eps (viscous

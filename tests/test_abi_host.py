@@ -143,3 +143,31 @@ def test_command_line_program_rejects_bad_flags(cup):
     r = subprocess.run([exe, "-bpdx", "2", "-bpdy", "2", "-bpdz", "2", "-nu", "0.01"], capture_output=True, text=True,
                        timeout=60)
     assert r.returncode == 1 and "periodic" in r.stderr
+
+
+def test_noalloc_diagnostics_rejects_bad_arguments_before_touching_a_device(cup):
+    """cup_diagnostics_noalloc checks its arguments and the static scratch's capacity first;
+    these calls return CUP_E_INVALID with a message and make no HIP call."""
+    from cubismup_3d_amd import _lib
+    L = _lib.lib()
+    out = C.c_void_p(16)  # never written: every call below is refused
+    ptrs = (C.c_void_p * 5)(*([C.c_void_p(16)] * 5))
+    ok = _lib.GridDesc((C.c_int * 3)(4, 4, 4), 1.0, 8, 1, 0)
+    assert L.cup_diagnostics_noalloc(None, ptrs, 0.01, None, out) == _lib.CUP_E_INVALID
+    assert b"null" in L.cup_last_error(None)
+    assert L.cup_diagnostics_noalloc(C.byref(ok), ptrs, 0.01, None, None) == _lib.CUP_E_INVALID
+    holes = (C.c_void_p * 5)(C.c_void_p(16), None, C.c_void_p(16), C.c_void_p(16), C.c_void_p(16))
+    assert L.cup_diagnostics_noalloc(C.byref(ok), holes, 0.01, None, out) == _lib.CUP_E_INVALID
+    assert b"field" in L.cup_last_error(None)
+    for bpd, prec, word in (((0, 4, 4), 8, b"blocks"), ((4, 4, 4), 2, b"precision")):
+        d = _lib.GridDesc((C.c_int * 3)(*bpd), 1.0, prec, 1, 0)
+        assert L.cup_diagnostics_noalloc(C.byref(d), ptrs, 0.01, None, out) == _lib.CUP_E_INVALID
+        assert word in L.cup_last_error(None)
+    # 64 x 64 x 72 blocks (bpdx a multiple of 8) need 36864 partial sums: beyond the 32768
+    # the scratch holds; 64^3 blocks (512^3 cells) need exactly 32768 and pass this check
+    big = _lib.GridDesc((C.c_int * 3)(64, 64, 72), 1.0, 4, 1, 0)
+    assert L.cup_diagnostics_noalloc(C.byref(big), ptrs, 0.01, None, out) == _lib.CUP_E_INVALID
+    assert b"scratch" in L.cup_last_error(None)
+    odd = _lib.GridDesc((C.c_int * 3)(33, 33, 33), 1.0, 4, 1, 0)  # 35937 blocks, one per workgroup
+    assert L.cup_diagnostics_noalloc(C.byref(odd), ptrs, 0.01, None, out) == _lib.CUP_E_INVALID
+    assert b"scratch" in L.cup_last_error(None)
