@@ -18,17 +18,29 @@
  *   lab fill          -- TestBoundaries shift test restated, exact (tests/cubismup3d/TestBoundaries.cpp:19-75)
  *   advect_diffuse    -- shear-wave closed form (exact one-step factor), advected-mode symbol,
  *                        brute force against an independent numpy restatement on 16^3
- *   pressure_rhs      -- single-mode closed form (2i sin(kh) h^2/(2dt) symbol)
+ *   pressure_rhs      -- single-mode closed form (2i sin(kh) h^2/(2dt) symbol); with bodies, the
+ *                        masking and perimeter correction: plain divergence outside the obstacle
+ *                        blocks, (1-chi) rhs off the surface, one common factor on the surface cells
+ *                        of sumRHS's sign, the total restored to zero, for both signs and two bodies
  *   fft3d             -- numpy.fft golden vectors (tests/golden/fft_*.npz), round trip
  *   poisson_solve     -- closed-form symbol per mode, the 8 zeroed modes, div_c(u_new)=0
  *   grad_p            -- single-mode closed form, projection idempotence
  *   find_max_u / dt   -- closed form incl. ramp
- *   dissipation       -- KE vs analytic TGV energy, linear momentum of a constant field
+ *   dissipation       -- a numpy restatement from vector definitions (np.cross, full strain
+ *                        tensor) on random fields; single-mode closed forms of helicity (ABC flow),
+ *                        viscous power, pressure power and KE; the impulse identities (linear
+ *                        impulse = momentum, angular momentum = 2 sum A = angular impulse) on
+ *                        compact fields
  *   spectral_forcing  -- Parseval (tke vs physical-space mean), 32-mode count, injected energy
  *   ic_hit            -- Philox-4x32-10 known-answer vectors, solenoidality, u_rms, spectrum shape
  *   sphere chi        -- Towers formula on a planar SDF (closed form), sphere volume vs 4/3 pi R^3
  *   penalize          -- per-cell closed form; force = sum of per-cell terms
- *   surface_forces    -- uniform pressure gives zero net force (closed surface), linear shear
+ *   surface_forces    -- the divergence theorem against exact ball integrals of Fourier modes:
+ *                        pressure force, viscous force, viscous torque and circulation, converging
+ *                        from 64^3 to 128^3; thrust - drag, Pout and pLocomotion against the force
+ *                        and torque; zero force for uniform pressure
+ *   The pins of pressure_rhs's masking, surface_forces and dissipation plant mistakes here
+ *   (orc_set_plant) and show that their checks fail for them.
  */
 #include "cup_oracle.h"
 
@@ -42,6 +54,11 @@
 
 #define BS 8
 #define SURFDH 1 /* ObstacleBlock.h:16 */
+
+/* Planted mistakes (orc_set_plant): the pin tests switch one on to show that their checks
+ * fail for it, and switch it off again. 0, the default, is the restatement as written. */
+static int g_plant = 0;
+void orc_set_plant(int code) { g_plant = code; }
 
 typedef struct {
   double chi, u, v, w, p, tmpU, tmpV, tmpW;
@@ -546,7 +563,7 @@ static void rhs_obstacle(orc_sim* s)
   }
   free(part);
   const double corrDenom = sumRHS > 0 ? posRHS : negRHS;
-  const double corr = sumRHS / (corrDenom > EPS ? corrDenom : EPS);
+  const double corr = (g_plant == ORC_PLANT_RHS_CORR_SIGN ? -1 : 1) * sumRHS / (corrDenom > EPS ? corrDenom : EPS);
   #pragma omp parallel for schedule(static)
   for (int b = 0; b < s->nb; b++) {
     if (s->ob[b] == NULL) continue;
@@ -782,24 +799,25 @@ void orc_dissipation(const orc_sim* s, double nu, double out[20])
             const double D12 = inv2h * (LN->u - LS->u + LE->v - LW->v) / 2;
             const double D13 = inv2h * (LB->u - LF->u + LE->w - LW->w) / 2;
             const double D23 = inv2h * (LN->w - LS->w + LB->v - LF->v) / 2;
-            const double V2 = D11 * D11 + D22 * D22 + D33 * D33 + 2 * (D12 * D12 + D13 * D13 + D23 * D23);
+            const double V2 = D11 * D11 + D22 * D22 + D33 * D33 +
+                              (g_plant == ORC_PLANT_DISS_SHEAR_WEIGHT ? 1 : 2) * (D12 * D12 + D13 * D13 + D23 * D23);
             R[16] += (1 - X) * hCube * nu * (V1 + 2 * V2);
             R[0] += hCube * WX;
             R[1] += hCube * WY;
             R[2] += hCube * WZ;
-            R[3] += hCube / 2 * (PY * WZ - PZ * WY);
+            R[3] += hCube / 2 * (PY * WZ - (g_plant == ORC_PLANT_DISS_IMPULSE_SIGN ? -1 : 1) * PZ * WY);
             R[4] += hCube / 2 * (PZ * WX - PX * WZ);
             R[5] += hCube / 2 * (PX * WY - PY * WX);
             R[6] += hCube * L->u;
             R[7] += hCube * L->v;
             R[8] += hCube * L->w;
-            R[9] += hCube / 3 * (PX * (PY * WY + PZ * WZ) - WX * (PY * PY + PZ * PZ));
+            R[9] += hCube / 3 * (PX * (PY * WY + PZ * WZ) - WX * (PY * PY + (g_plant == ORC_PLANT_DISS_ANGIMP_TERM ? 0 : 1) * PZ * PZ));
             R[10] += hCube / 3 * (PY * (PX * WX + PZ * WZ) - WY * (PX * PX + PZ * PZ));
             R[11] += hCube / 3 * (PZ * (PX * WX + PY * WY) - WZ * (PX * PX + PY * PY));
             R[12] += hCube * (PY * L->w - PZ * L->v);
             R[13] += hCube * (PZ * L->u - PX * L->w);
             R[14] += hCube * (PX * L->v - PY * L->u);
-            R[17] += hCube * (WX * L->u + WY * L->v + WZ * L->w);
+            R[17] += hCube * (WX * L->u + WY * L->v + (g_plant == ORC_PLANT_DISS_HELICITY ? -1 : 1) * WZ * L->w);
             R[18] += hCube * (L->u * L->u + L->v * L->v + L->w * L->w) / 2;
             R[19] += sqrt(WX * WX + WY * WY + WZ * WZ);
           }
@@ -1706,12 +1724,13 @@ void orc_surface_forces(const orc_sim* s, double nu, const double uTrans[3], con
         const double D11 = _1oH * (LAB(&l, ix + 1, iy, iz)->u - LAB(&l, ix - 1, iy, iz)->u);
         const double D22 = _1oH * (LAB(&l, ix, iy + 1, iz)->v - LAB(&l, ix, iy - 1, iz)->v);
         const double D33 = _1oH * (LAB(&l, ix, iy, iz + 1)->w - LAB(&l, ix, iy, iz - 1)->w);
-        const double D12 = .5 * _1oH * (LAB(&l, ix, iy + 1, iz)->u - LAB(&l, ix, iy - 1, iz)->u +
-                                        LAB(&l, ix + 1, iy, iz)->v - LAB(&l, ix - 1, iy, iz)->v);
-        const double D13 = .5 * _1oH * (LAB(&l, ix, iy, iz + 1)->u - LAB(&l, ix, iy, iz - 1)->u +
-                                        LAB(&l, ix + 1, iy, iz)->w - LAB(&l, ix - 1, iy, iz)->w);
-        const double D23 = .5 * _1oH * (LAB(&l, ix, iy + 1, iz)->w - LAB(&l, ix, iy - 1, iz)->w +
-                                        LAB(&l, ix, iy, iz + 1)->v - LAB(&l, ix, iy, iz - 1)->v);
+        const double half = g_plant == ORC_PLANT_FORCES_SHEAR_HALF ? 1.0 : .5;
+        const double D12 = half * _1oH * (LAB(&l, ix, iy + 1, iz)->u - LAB(&l, ix, iy - 1, iz)->u +
+                                          LAB(&l, ix + 1, iy, iz)->v - LAB(&l, ix - 1, iy, iz)->v);
+        const double D13 = half * _1oH * (LAB(&l, ix, iy, iz + 1)->u - LAB(&l, ix, iy, iz - 1)->u +
+                                          LAB(&l, ix + 1, iy, iz)->w - LAB(&l, ix - 1, iy, iz)->w);
+        const double D23 = half * _1oH * (LAB(&l, ix, iy + 1, iz)->w - LAB(&l, ix, iy - 1, iz)->w +
+                                          LAB(&l, ix, iy, iz + 1)->v - LAB(&l, ix, iy, iz - 1)->v);
         const FE* C = LAB(&l, ix, iy, iz);
         const double P = C->p;
         const double normX = o->dchidx[i], normY = o->dchidy[i], normZ = o->dchidz[i];
@@ -1731,8 +1750,9 @@ void orc_surface_forces(const orc_sim* s, double nu, const double uTrans[3], con
         Q[13] += normZ * vX - normX * vZ;
         Q[14] += normX * vY - normY * vX;
         const double forcePar = fXT * velUnit[0] + fYT * velUnit[1] + fZT * velUnit[2];
-        Q[16] += .5 * (forcePar + fabs(forcePar));
-        Q[15] -= .5 * (forcePar - fabs(forcePar));
+        const int swap = g_plant == ORC_PLANT_FORCES_THRUST_DRAG;
+        Q[swap ? 15 : 16] += .5 * (forcePar + fabs(forcePar));
+        Q[swap ? 16 : 15] -= .5 * (forcePar - fabs(forcePar));
         const double powOut = fXT * vX + fYT * vY + fZT * vZ;
         const double powDef = fXT * o->udef[0][iz][iy][ix] + fYT * o->udef[1][iz][iy][ix] +
                               fZT * o->udef[2][iz][iy][ix]; /* FluidSolidForces.cpp:110-112,145 */

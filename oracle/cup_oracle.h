@@ -24,6 +24,19 @@ enum { ORC_CHI = 0, ORC_U, ORC_V, ORC_W, ORC_P, ORC_TMPU, ORC_TMPV, ORC_TMPW };
 
 typedef struct orc_sim orc_sim;
 
+/* Planted mistakes, for the tests that show their pins catch them (0: none). */
+enum {
+  ORC_PLANT_NONE = 0,
+  ORC_PLANT_RHS_CORR_SIGN = 1,       /* the perimeter correction's factor with the wrong sign */
+  ORC_PLANT_FORCES_SHEAR_HALF = 2,   /* the surface forces' D12, D13, D23 without their 1/2 */
+  ORC_PLANT_FORCES_THRUST_DRAG = 3,  /* thrust and drag swapped */
+  ORC_PLANT_DISS_SHEAR_WEIGHT = 4,   /* D:D with the off-diagonal terms counted once */
+  ORC_PLANT_DISS_IMPULSE_SIGN = 5,   /* the linear impulse's x component with r_z w_y added */
+  ORC_PLANT_DISS_ANGIMP_TERM = 6,    /* the angular impulse's x component without r_z^2 w_x */
+  ORC_PLANT_DISS_HELICITY = 7        /* the helicity with w_z u_z subtracted */
+};
+void orc_set_plant(int code);
+
 /* Grid: bpd blocks of 8^3 per dimension, h = maxextent / max(cells)
  * (SimulationData.cpp:184-195). */
 orc_sim* orc_create(int bpdx, int bpdy, int bpdz, double maxextent);

@@ -45,6 +45,7 @@ def lib():
         "orc_h": (d, [p]),
         "orc_extent": (None, [p, dp]),
         "orc_set_threads": (None, [i]),
+        "orc_set_plant": (None, [i]),
         "orc_set_field": (None, [p, i, dp]),
         "orc_get_field": (None, [p, i, dp]),
         "orc_lab_shift": (None, [p, i, i, i, i]),
@@ -115,6 +116,25 @@ def from_blocks(b: np.ndarray, shape) -> np.ndarray:
     nbz, nby, nbx = Z // BS, Y // BS, X // BS
     return np.ascontiguousarray(
         b.reshape(nbz, nby, nbx, BS, BS, BS).transpose(0, 3, 1, 4, 2, 5).reshape(Z, Y, X))
+
+
+PLANTS = {"rhs_corr_sign": 1, "forces_shear_half": 2, "forces_thrust_drag": 3, "diss_shear_weight": 4,
+          "diss_impulse_sign": 5, "diss_angimp_term": 6, "diss_helicity": 7}
+
+
+def planted(name):
+    """Context manager: the oracle runs with the named mistake planted inside (the pin tests
+    show with it that their checks fail for that mistake)."""
+    import contextlib
+
+    @contextlib.contextmanager
+    def scope():
+        lib().orc_set_plant(PLANTS[name])
+        try:
+            yield
+        finally:
+            lib().orc_set_plant(0)
+    return scope()
 
 
 def philox4x32_10(ctr, key):

@@ -132,5 +132,6 @@ def test_dissipated_energy_gradient_128(cup, oracle):
           f"({sum(e < 1e-4 for e in e32s)} of 8 within 1e-4), fp64 arithmetic on fp32 states {max(emix):.1e}",
           flush=True)
     assert max(e64s) < 1e-6
-    # The FP32 gradient's error is that of the FP32 trajectory (see the module docstring).
-    assert max(e32s) < 5e-4
+    # The FP32 gradient's error is that of the FP32 trajectory (see the module docstring):
+    # measured 2e-6 to 8.6e-5 over the 8 directions, within the 1e-4 target on every one.
+    assert max(e32s) < 1e-4
