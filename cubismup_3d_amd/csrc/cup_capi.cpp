@@ -57,6 +57,8 @@ static void release(Ctx& c)
     if (*b) { hipFree(*b); *b = nullptr; }
   for (auto& p : c.cos4)
     if (p) { hipFree(p); p = nullptr; }
+  for (auto& p : c.sin2)
+    if (p) { hipFree(p); p = nullptr; }
   if (c.partials) hipFree(c.partials);
   if (c.ke_part) hipFree(c.ke_part);
   c.ke_part = nullptr;
@@ -780,16 +782,28 @@ int cup_run_steps(cup_ctx* ctx, const cup_step_params* p, int nsteps)
   }
   // FUSED: measure the current velocity and set dt for the coming step on the device.
   if (int rc = launch_reset_umax(c)) return rc;
+  c.umax_bank = 0;
   if (int rc = launch_max_u(c, p->uinf, true)) return rc;
   if (int rc = launch_finalize_step(c, p->CFL, p->nu, p->rampup, p->fixed_dt, 0)) return rc;
+  // With the central scheme each step's bookkeeping (k_finalize_step) runs inside the next
+  // step's advection; the ramp needs the device step counter, read once here.
+  const bool fold_ok = p->scheme == CUP_CENTRAL_EULER;
+  long long step0 = 0;
+  if (fold_ok && p->rampup > 0) {
+    if (int rc = read_scalars(c)) return rc;
+    step0 = c.sc_host->step;
+  }
   const bool body = c.body.present && p->has_body;
   // The band-limited forcing stays on the device when its low-mode set fits the fused
   // kernels; otherwise the reference decomposition of SpectralForcing runs (host round trip).
   const bool force_fused = p->spectral_forcing && c.low_fused_ok;
   for (int s = 0; s < nsteps; ++s) {
+    const bool fold = fold_ok && s > 0;
+    const StepFold sf{umax_slots_dev(c, c.umax_bank), step0 + s, p->CFL, p->rampup, p->fixed_dt};
     stage_mark(c, ST_ADVDIFF, false);
-    if (int rc = launch_advdiff(c, p->nu, p->uinf, p->scheme)) return rc;
+    if (int rc = launch_advdiff(c, p->nu, p->uinf, p->scheme, fold ? &sf : nullptr)) return rc;
     stage_mark(c, ST_ADVDIFF, true);
+    if (fold) c.umax_bank ^= 1;  // this step's gradient fills the other bank and clears the folded one
     if (p->spectral_forcing && !force_fused) {
       double st[5];
       if (int rc = forcing_plain(c, -1.0, -1.0, p->nu, p->tke_target, p->injection_rate, st)) return rc;
@@ -809,7 +823,8 @@ int cup_run_steps(cup_ctx* ctx, const cup_step_params* p, int nsteps)
                                         c.body_ft + 6 * k))
         return rc;
     }
-    if (int rc = launch_finalize_step(c, p->CFL, p->nu, p->rampup, p->fixed_dt, 1)) return rc;
+    if (!fold_ok || s == nsteps - 1)
+      if (int rc = launch_finalize_step(c, p->CFL, p->nu, p->rampup, p->fixed_dt, 1)) return rc;
   }
   return CUP_OK;
 }

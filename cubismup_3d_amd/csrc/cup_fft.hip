@@ -529,7 +529,9 @@ template <typename T, int N, int AXIS, int MODE, bool FORCE = false>
 __global__ __launch_bounds__(NT) void k_fft_colr(C2<T>* __restrict__ S, const C2<T>* __restrict__ tw, int ncols,
                                                  int Ny, int Nz, double norm, const double* __restrict__ cx,
                                                  const double* __restrict__ cy, const double* __restrict__ cz,
-                                                 const LowModes* __restrict__ lmp, const double* __restrict__ coef)
+                                                 const LowModes* __restrict__ lmp, const double* __restrict__ coef,
+                                                 const float* __restrict__ sx, const float* __restrict__ sy,
+                                                 const float* __restrict__ sz)
 {
   using CR = ColReg<T, N>;
   constexpr int TK = CR::TK, NTH = CR::NTH, RPT = CR::RPT;
@@ -583,9 +585,29 @@ __global__ __launch_bounds__(NT) void k_fft_colr(C2<T>* __restrict__ S, const C2
     return;
   }
   // ZM_POISSON (z): PoissonSolverPeriodic::_solve's grid-consistent branch
-  // (PoissonSolverPeriodic.cpp:23-61), multiply by norm/(2D-6) in FP64, zero the modes with
-  // every index in {0, N/2}; then the inverse transform.
-  {
+  // (PoissonSolverPeriodic.cpp:23-61), multiply by norm/(2D-6), zero the modes with every
+  // index in {0, N/2}; then the inverse transform. FP64 contexts form the factor as the
+  // reference writes it, in FP64 from the cosine tables. FP32 contexts use the same value as
+  // -norm/(4 S), S = sum_i sin^2(2 pi k_i/N_i) from FP32 tables (2 cos(2a) - 2 = -4 sin^2 a):
+  // a sum of non-negative terms, so it keeps FP32's relative precision down to k = 1, where
+  // 2D - 6 in FP32 would cancel.
+  if constexpr (sizeof(T) == 4) {
+    const int kx = kx0 + f, ky = other;
+    const float sxy = live ? sx[kx] + sy[ky] : 1.0f;
+    const bool zxy = kx == 0 && (ky == 0 || 2 * ky == Ny);
+    const float nq = (float)(-0.25 * norm);
+#pragma unroll
+    for (int i = 0; i < RPT; ++i)
+#pragma unroll
+      for (int k2 = 0; k2 < NTH; ++k2) {
+        const int kz = j * RPT + i + 16 * k2;
+        const bool zero = zxy && (kz == 0 || 2 * kz == N);
+        const float fac = zero ? 0.0f : nq * __builtin_amdgcn_rcpf(sxy + sz[kz]);
+        C2<T>& q = v[i * NTH + k2];
+        q.x *= fac;
+        q.y *= fac;
+      }
+  } else {
     const int kx = kx0 + f, ky = other;
     const double dxy = live ? cx[kx] + cy[ky] : 0.0;
     const bool zxy = kx == 0 && (ky == 0 || 2 * ky == Ny);
@@ -676,17 +698,17 @@ struct ColPass {
       const C2<T>* tw = (const C2<T>*)c.tw[AXIS];
       if (mode == ZM_FWD)
         hipLaunchKernelGGL((k_fft_colr<T, N, AXIS, ZM_FWD>), grid, dim3(NT), 0, c.stream, spec, tw, ncols, c.N[1],
-                           c.N[2], norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low_dev, (const double*)nullptr);
+                           c.N[2], norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low_dev, (const double*)nullptr, c.sin2[0], c.sin2[1], c.sin2[2]);
       else if (mode == ZM_INV)
         hipLaunchKernelGGL((k_fft_colr<T, N, AXIS, ZM_INV>), grid, dim3(NT), 0, c.stream, spec, tw, ncols, c.N[1],
-                           c.N[2], norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low_dev, (const double*)nullptr);
+                           c.N[2], norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low_dev, (const double*)nullptr, c.sin2[0], c.sin2[1], c.sin2[2]);
       else if (AXIS == 2 && force)
         hipLaunchKernelGGL((k_fft_colr<T, N, 2, ZM_POISSON, true>), grid, dim3(NT), 0, c.stream, spec, tw, ncols,
                            c.N[1], c.N[2], norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low_dev,
-                           (const double*)c.low_coef);
+                           (const double*)c.low_coef, c.sin2[0], c.sin2[1], c.sin2[2]);
       else if (AXIS == 2)
         hipLaunchKernelGGL((k_fft_colr<T, N, 2, ZM_POISSON, false>), grid, dim3(NT), 0, c.stream, spec, tw, ncols,
-                           c.N[1], c.N[2], norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low_dev, (const double*)nullptr);
+                           c.N[1], c.N[2], norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low_dev, (const double*)nullptr, c.sin2[0], c.sin2[1], c.sin2[2]);
       return;
     }
     constexpr int TK = ColTile<T, N>::TK;
@@ -872,11 +894,13 @@ int fft_setup(Ctx& c)
     if (n < 8 || n > 1024 || (n & (n - 1)) != 0)
       return set_error(c, CUP_E_INVALID, "FFT sizes must be powers of two between 8 and 1024 cells per axis");
     std::vector<double> tw(2 * (size_t)n), cs((size_t)n);
+    std::vector<float> s2((size_t)n);
     for (int t = 0; t < n; ++t) {
       const double ang = 2.0 * M_PI * (double)t / (double)n;
       tw[2 * t] = cos(ang);
       tw[2 * t + 1] = -sin(ang);
       cs[t] = cos(4.0 * M_PI * (double)t / (double)n);
+      s2[t] = (float)(sin(ang) * sin(ang));
     }
     if (check_hip(c, hipMalloc(&c.tw[a], (size_t)n * 2 * c.esz), "hipMalloc twiddles")) return CUP_E_DEVICE;
     if (c.precision == 4) {
@@ -888,6 +912,8 @@ int fft_setup(Ctx& c)
     }
     if (check_hip(c, hipMalloc(&c.cos4[a], (size_t)n * sizeof(double)), "hipMalloc cos4")) return CUP_E_DEVICE;
     hipMemcpy(c.cos4[a], cs.data(), cs.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (check_hip(c, hipMalloc(&c.sin2[a], (size_t)n * sizeof(float)), "hipMalloc sin2")) return CUP_E_DEVICE;
+    hipMemcpy(c.sin2[a], s2.data(), s2.size() * sizeof(float), hipMemcpyHostToDevice);
   }
   return CUP_OK;
 }

@@ -48,8 +48,9 @@ struct DevScalars {
   double tke, tke_low, eps, lint;
   double red[32];    // generic finalized reduction results
   // max |u+uinf| of the fused step, folded per workgroup by atomic max on the bit pattern
-  // of a non-negative double, spread over UMAX_SLOTS addresses to avoid contention
-  unsigned long long umax_slots[256];
+  // of a non-negative double, spread over UMAX_SLOTS addresses to avoid contention. Two
+  // banks: a step's gradient writes one while the next step's advection reads the other.
+  unsigned long long umax_slots[2][256];
 };
 constexpr int UMAX_SLOTS = 256;
 
@@ -108,6 +109,7 @@ struct Ctx {
   void* f[14] = {};       // block-layout fields chi,u,v,w,p,tmpU,tmpV,tmpW; adjoint u,v,w (lazy);
                           // F_ADJ2..+2: the upwind midpoint adjoint's scratch (lazy)
   int ustar = CUP_TMPU;   // first of the three fields holding the step's u* after advection
+  int umax_bank = 0;      // the bank of DevScalars::umax_slots that the max |u| writers fill
   bool plain_rhs = false; // the fused x pass ignores the obstacle RHS (the adjoint's projection)
   void* spec = nullptr;   // packed half spectrum, complex, [z][y][kx], kx < Nx/2
   double* ana = nullptr;  // HIT analysis: per-line sums [3 + nbin][Ny*Nz], then the row totals (lazy)
@@ -116,6 +118,7 @@ struct Ctx {
   void* rbuf = nullptr;   // block-layout real scratch (plain RHS / solution)
   void* tw[3] = {};       // FFT twiddles per axis, e^{-2 pi i t / N}, context precision
   double* cos4[3] = {};   // cos(4 pi k / N) per axis (FP64), the Poisson symbol
+  float* sin2[3] = {};    // sin^2(2 pi k / N) per axis (FP32): 2 cos(4 pi k/N) - 2 = -4 sin^2(2 pi k/N)
   double* partials = nullptr;
   size_t partials_cap = 0;  // doubles
   double* ke_part = nullptr;  // [nb][4] partial sums |u|^2 of the fused step's velocity (per wave)
@@ -191,7 +194,17 @@ __device__ __forceinline__ T wave_max(T v)
 // cup_stencil.hip
 int launch_lab_shift(Ctx& c, int g, int dx, int dy, int dz);
 int launch_max_u(Ctx& c, const double uinf[3], bool prime = false);  // partials[nb][2]; prime: atomic max + ke_part
-int launch_advdiff(Ctx& c, double nu, const double uinf[3], int scheme);  // -> tmp
+// The previous step's bookkeeping folded into the advection (central scheme only): every
+// workgroup forms dt from the max |u| slots of bank `slots` by the CFL rule for step `step`
+// (0-based) and workgroup 0 advances step, time, last_dt, umax and dt as k_finalize_step does.
+struct StepFold {
+  const unsigned long long* slots;
+  long long step;
+  double CFL;
+  int rampup;
+  double fixed_dt;
+};
+int launch_advdiff(Ctx& c, double nu, const double uinf[3], int scheme, const StepFold* fold = nullptr);  // -> tmp
 int launch_copy_tmp_to_u(Ctx& c);
 int launch_copy_u_to_tmp(Ctx& c);
 int launch_adjoint_advdiff(Ctx& c, double nu, const double uinf[3]);
@@ -225,7 +238,9 @@ int launch_set_dt(Ctx& c, double dt);
 int launch_reduce(Ctx& c, int n, int width, unsigned maxmask, double* out_dev, const double* part = nullptr);
 // Records the message of a failed call that has no context (read by cup_last_error(NULL)).
 int free_error(int code, const std::string& msg);
-int launch_reset_umax(Ctx& c);
+int launch_reset_umax(Ctx& c);  // clears both banks
+// Device address of bank k of the max |u| slots (host pointer arithmetic, no access).
+inline unsigned long long* umax_slots_dev(const Ctx& c, int k) { return &c.sc->umax_slots[k][0]; }
 int launch_finalize_step(Ctx& c, double CFL, double nu, int rampup, double fixed_dt, int advance);
 // cup_fft.hip
 int fft_setup(Ctx& c);

@@ -462,7 +462,7 @@ __global__ __launch_bounds__(NT) void k_penalize(const int* __restrict__ ob_ids,
                                                  double lambda, int implicit, double vx, double vy, double vz,
                                                  double ox, double oy, double oz, double cmx, double cmy, double cmz,
                                                  double Ux, double Uy, double Uz, double* __restrict__ part,
-                                                 double* __restrict__ ke_part)
+                                                 double* __restrict__ ke_part, unsigned long long* __restrict__ slots)
 {
   const int s = blockIdx.x, b = ob_ids[s];
   int bx, by, bz;
@@ -517,7 +517,7 @@ __global__ __launch_bounds__(NT) void k_penalize(const int* __restrict__ ob_ids,
     if (threadIdx.x == 0) {
       double a = red[0][0], bsum = red[0][1];
       for (int q = 1; q < NT / 64; ++q) { a = fmax(a, red[q][0]); bsum += red[q][1]; }
-      atomicMax(&sc->umax_slots[b & (UMAX_SLOTS - 1)], (unsigned long long)__double_as_longlong(a));
+      atomicMax(&slots[b & (UMAX_SLOTS - 1)], (unsigned long long)__double_as_longlong(a));
       ke_part[4 * (size_t)b] = bsum;  // the block's partials: [4b] its sum, [4b+1..3] zero
       ke_part[4 * (size_t)b + 1] = ke_part[4 * (size_t)b + 2] = ke_part[4 * (size_t)b + 3] = 0.0;
     }
@@ -787,7 +787,7 @@ static int penalize_t(Ctx& c, double lambda, int implicit, const double vel[3], 
                      (const double*)B.ob_udef, (const T*)c.f[CUP_CHI],
                      (T*)c.f[CUP_U], (T*)c.f[CUP_V], (T*)c.f[CUP_W], c.grid(), c.sc, lambda, implicit, vel[0], vel[1],
                      vel[2], om[0], om[1], om[2], B.com[0], B.com[1], B.com[2], uinf[0], uinf[1], uinf[2], bp,
-                     refresh ? c.ke_part : (double*)nullptr);
+                     refresh ? c.ke_part : (double*)nullptr, umax_slots_dev(c, c.umax_bank));
   hipLaunchKernelGGL(k_sum_rows, dim3(6), dim3(NT), 0, c.stream, (const double*)bp, B.nob, 6,
                      ft_dev ? ft_dev : c.sc->red);
   return check_hip(c, hipGetLastError(), "k_penalize");

@@ -207,6 +207,24 @@ __device__ __forceinline__ void block_reduce(double (&v)[NV], const unsigned max
   }
 }
 
+// The CFL rule of Simulation::calcMaxTimestep (Simulation.cpp:403-424), ramp included:
+// the dt of step `step` (0-based) from the velocity's max |u + uinf|.
+__device__ __forceinline__ double cfl_dt(double umax, long long step, double CFL, double nu, double h, int rampup,
+                                         double fixed_dt)
+{
+  if (fixed_dt > 0) return fixed_dt;
+  const double dtDif = nu > 0 ? h * h / nu : INFINITY;
+  const double dtAdv = h / (umax + 1e-8);
+  const double mm = fmin(dtDif, dtAdv);
+  double dt = CFL * mm;
+  if (step < rampup) {
+    const double x = (step + 1.0) / rampup;
+    const double rampCFL = exp(log(1e-3) * (1 - x) + log(CFL) * x);
+    dt = rampCFL * mm;
+  }
+  return dt;
+}
+
 // ------------------------------------------------------------------- lab shift test
 
 template <typename T, int G>
@@ -254,7 +272,7 @@ int launch_lab_shift(Ctx& c, int g, int dx, int dy, int dz)
 template <typename T>
 __global__ __launch_bounds__(NT) void k_max_u(const T* __restrict__ u, const T* __restrict__ v,
                                               const T* __restrict__ w, int nb, double ux, double uy, double uz,
-                                              double* __restrict__ part, DevScalars* __restrict__ sc,
+                                              double* __restrict__ part, unsigned long long* __restrict__ slots,
                                               double* __restrict__ ke_part)
 {
   const int b = blockIdx.x;
@@ -271,8 +289,8 @@ __global__ __launch_bounds__(NT) void k_max_u(const T* __restrict__ u, const T* 
   if (threadIdx.x == 0) {
     part[2 * (size_t)b] = out[0];
     part[2 * (size_t)b + 1] = out[1];
-    if (sc) {
-      atomicMax(&sc->umax_slots[b & (UMAX_SLOTS - 1)], (unsigned long long)__double_as_longlong(out[0]));
+    if (slots) {
+      atomicMax(&slots[b & (UMAX_SLOTS - 1)], (unsigned long long)__double_as_longlong(out[0]));
       ke_part[4 * (size_t)b] = out[1];
       ke_part[4 * (size_t)b + 1] = ke_part[4 * (size_t)b + 2] = ke_part[4 * (size_t)b + 3] = 0.0;
     }
@@ -283,8 +301,8 @@ template <typename T>
 static int max_u_t(Ctx& c, const double uinf[3], bool prime)
 {
   hipLaunchKernelGGL((k_max_u<T>), dim3(c.nb), dim3(NT), 0, c.stream, (const T*)c.f[CUP_U], (const T*)c.f[CUP_V],
-                     (const T*)c.f[CUP_W], c.nb, uinf[0], uinf[1], uinf[2], c.partials, prime ? c.sc : nullptr,
-                     c.ke_part);
+                     (const T*)c.f[CUP_W], c.nb, uinf[0], uinf[1], uinf[2], c.partials,
+                     prime ? umax_slots_dev(c, c.umax_bank) : nullptr, c.ke_part);
   return check_hip(c, hipGetLastError(), "k_max_u");
 }
 
@@ -294,7 +312,7 @@ int launch_max_u(Ctx& c, const double uinf[3], bool prime)
   return c.precision == 4 ? max_u_t<float>(c, uinf, prime) : max_u_t<double>(c, uinf, prime);
 }
 
-__global__ void k_reset_umax(DevScalars* sc) { sc->umax_slots[threadIdx.x] = 0ull; }
+__global__ void k_reset_umax(DevScalars* sc) { sc->umax_slots[0][threadIdx.x] = sc->umax_slots[1][threadIdx.x] = 0ull; }
 
 int launch_reset_umax(Ctx& c)
 {
@@ -365,7 +383,8 @@ template <typename T, int NBW, int PD>
 __global__ __launch_bounds__(NT) void k_advdiff_xwalk(const T* __restrict__ u, const T* __restrict__ v,
                                                       const T* __restrict__ w, T* __restrict__ ou,
                                                       T* __restrict__ ov, T* __restrict__ ow, GridArgs g,
-                                                      const DevScalars* __restrict__ sc, T Ux, T Uy, T Uz, double nu)
+                                                      DevScalars* __restrict__ sc, T Ux, T Uy, T Uz, double nu,
+                                                      StepFold fold)
 {
   __shared__ __align__(16) T lab[3][PLC];
   const LabSlots s = lab_slots();
@@ -375,7 +394,26 @@ __global__ __launch_bounds__(NT) void k_advdiff_xwalk(const T* __restrict__ u, c
   LabRegs<T, 3> r[NS];
 #pragma unroll
   for (int p = 0; p < PD && p < NBW; ++p) lab_fetch_xw(r[p], src, s, g, b0 + p, p == 0, p == NBW - 1);
-  const double dt = sc->dt;
+  double dt;
+  if (fold.slots) {
+    // the previous step's finalize: no workgroup reads the scalars workgroup 0 updates
+    double m[1] = {__longlong_as_double((long long)fold.slots[threadIdx.x])};
+    __shared__ double fold_umax[1];
+    block_reduce<1>(m, 1u, fold_umax);
+    __syncthreads();
+    const double umax = fold_umax[0];
+    dt = cfl_dt(umax, fold.step, fold.CFL, nu, g.h, fold.rampup, fold.fixed_dt);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      const double dt_prev = sc->dt;
+      sc->umax = umax;
+      sc->step += 1;
+      sc->time += dt_prev;
+      sc->last_dt = dt_prev;
+      sc->dt = dt;
+    }
+  } else {
+    dt = sc->dt;
+  }
   const T facA = (T)(-dt / (2 * g.h));
   const T facD = (T)((nu / g.h) * (dt / g.h));
   const int c0 = 2 * threadIdx.x;
@@ -422,21 +460,23 @@ __global__ __launch_bounds__(NT) void k_advdiff_xwalk(const T* __restrict__ u, c
 }
 
 template <typename T, int NBW>
-static void launch_advdiff_xwalk(Ctx& c, double nu, const double uinf[3])
+static void launch_advdiff_xwalk(Ctx& c, double nu, const double uinf[3], const StepFold& fold)
 {
   hipLaunchKernelGGL((k_advdiff_xwalk<T, NBW, ADV_PD>), dim3(c.nb / NBW), dim3(NT), 0, c.stream,
                      (const T*)c.f[CUP_U], (const T*)c.f[CUP_V], (const T*)c.f[CUP_W], (T*)c.f[CUP_TMPU],
-                     (T*)c.f[CUP_TMPV], (T*)c.f[CUP_TMPW], c.grid(), c.sc, (T)uinf[0], (T)uinf[1], (T)uinf[2], nu);
+                     (T*)c.f[CUP_TMPV], (T*)c.f[CUP_TMPW], c.grid(), c.sc, (T)uinf[0], (T)uinf[1], (T)uinf[2], nu,
+                     fold);
 }
 
 template <typename T>
-static int advdiff_t(Ctx& c, double nu, const double uinf[3])
+static int advdiff_t(Ctx& c, double nu, const double uinf[3], const StepFold* fold_in)
 {
+  const StepFold fold = fold_in ? *fold_in : StepFold{nullptr, 0, 0.0, 0, 0.0};
   const int nbx = c.bpd[0];
-  if (nbx % 8 == 0) launch_advdiff_xwalk<T, 8>(c, nu, uinf);
-  else if (nbx % 4 == 0) launch_advdiff_xwalk<T, 4>(c, nu, uinf);
-  else if (nbx % 2 == 0) launch_advdiff_xwalk<T, 2>(c, nu, uinf);
-  else launch_advdiff_xwalk<T, 1>(c, nu, uinf);
+  if (nbx % 8 == 0) launch_advdiff_xwalk<T, 8>(c, nu, uinf, fold);
+  else if (nbx % 4 == 0) launch_advdiff_xwalk<T, 4>(c, nu, uinf, fold);
+  else if (nbx % 2 == 0) launch_advdiff_xwalk<T, 2>(c, nu, uinf, fold);
+  else launch_advdiff_xwalk<T, 1>(c, nu, uinf, fold);
   return check_hip(c, hipGetLastError(), "k_advdiff_xwalk");
 }
 
@@ -552,13 +592,14 @@ static int up3_stage(Ctx& c, int src0, int base0, int out0, double nu, const dou
 // and upwind Euler forms write u* to tmp; the midpoint form writes u_half to tmp and then u*
 // over u. c.ustar names the field set holding u* afterwards (the fused step's consumers read
 // it; the plain decomposition copies it so that u = tmp = u* as after the reference's copy).
-int launch_advdiff(Ctx& c, double nu, const double uinf[3], int scheme)
+int launch_advdiff(Ctx& c, double nu, const double uinf[3], int scheme, const StepFold* fold)
 {
   c.ustar = CUP_TMPU;
   if (scheme == CUP_CENTRAL_EULER)
-    return c.precision == 4 ? advdiff_t<float>(c, nu, uinf) : advdiff_t<double>(c, nu, uinf);
+    return c.precision == 4 ? advdiff_t<float>(c, nu, uinf, fold) : advdiff_t<double>(c, nu, uinf, fold);
   if (scheme != CUP_UPWIND3_EULER && scheme != CUP_UPWIND3_RK2)
     return set_error(c, CUP_E_INVALID, "advect_diffuse: unknown scheme");
+  if (fold) return set_error(c, CUP_E_STATE, "advect_diffuse: the folded step bookkeeping takes the central scheme");
   auto stage = [&](int src0, int base0, int out0, double mult) {
     return c.precision == 4 ? up3_stage<float>(c, src0, base0, out0, nu, uinf, mult)
                             : up3_stage<double>(c, src0, base0, out0, nu, uinf, mult);
@@ -743,7 +784,9 @@ __global__ __launch_bounds__(NT) void k_gradp(const T* __restrict__ p, const T* 
                                               T* __restrict__ v, T* __restrict__ w, GridArgs g,
                                               DevScalars* __restrict__ sc, double Ux, double Uy, double Uz,
                                               double* __restrict__ ke_part, const int* __restrict__ skip_slot,
-                                              const T* __restrict__ rows, const C2<T>* __restrict__ twx, int nbw)
+                                              const T* __restrict__ rows, const C2<T>* __restrict__ twx, int nbw,
+                                              unsigned long long* __restrict__ slots,
+                                              unsigned long long* __restrict__ clear_slots)
 {
   using V2 = typename Vec2<T>::type;
   using V4 = typename Vec4<T>::type;
@@ -847,7 +890,9 @@ __global__ __launch_bounds__(NT) void k_gradp(const T* __restrict__ p, const T* 
     double m[1] = {mx};
     __shared__ double out[1];
     block_reduce<1>(m, 1u, out);
-    if (t == 0) atomicMax(&sc->umax_slots[b0 & (UMAX_SLOTS - 1)], (unsigned long long)__double_as_longlong(out[0]));
+    if (t == 0) atomicMax(&slots[b0 & (UMAX_SLOTS - 1)], (unsigned long long)__double_as_longlong(out[0]));
+    // the other bank, read by this step's advection, is cleared for the next step's gradient
+    if (blockIdx.x == 0) clear_slots[t] = 0ull;
   }
 }
 
@@ -862,16 +907,18 @@ static int gradp_t(Ctx& c, bool fused, const double uinf[3], const int* skip, bo
   if (fused && force)
     hipLaunchKernelGGL((k_gradp<T, true, true>), grid, dim3(NT), 0, c.stream, p, (const T*)c.f[s],
                        (const T*)c.f[s + 1], (const T*)c.f[s + 2], (T*)c.f[CUP_U], (T*)c.f[CUP_V], (T*)c.f[CUP_W],
-                       c.grid(), c.sc, uinf[0], uinf[1], uinf[2], c.ke_part, skip, (const T*)c.low_rows, tx, nbw);
+                       c.grid(), c.sc, uinf[0], uinf[1], uinf[2], c.ke_part, skip, (const T*)c.low_rows, tx, nbw,
+                       umax_slots_dev(c, c.umax_bank), umax_slots_dev(c, c.umax_bank ^ 1));
   else if (fused)
     hipLaunchKernelGGL((k_gradp<T, true, false>), grid, dim3(NT), 0, c.stream, p, (const T*)c.f[s],
                        (const T*)c.f[s + 1], (const T*)c.f[s + 2], (T*)c.f[CUP_U], (T*)c.f[CUP_V], (T*)c.f[CUP_W],
-                       c.grid(), c.sc, uinf[0], uinf[1], uinf[2], c.ke_part, skip, (const T*)nullptr, tx, nbw);
+                       c.grid(), c.sc, uinf[0], uinf[1], uinf[2], c.ke_part, skip, (const T*)nullptr, tx, nbw,
+                       umax_slots_dev(c, c.umax_bank), umax_slots_dev(c, c.umax_bank ^ 1));
   else
     hipLaunchKernelGGL((k_gradp<T, false, false>), grid, dim3(NT), 0, c.stream, p, (const T*)c.f[s],
                        (const T*)c.f[s + 1], (const T*)c.f[s + 2], (T*)c.f[CUP_U], (T*)c.f[CUP_V], (T*)c.f[CUP_W],
                        c.grid(), c.sc, 0.0, 0.0, 0.0, (double*)nullptr, (const int*)nullptr, (const T*)nullptr, tx,
-                       nbw);
+                       nbw, (unsigned long long*)nullptr, (unsigned long long*)nullptr);
   return check_hip(c, hipGetLastError(), "k_gradp");
 }
 
@@ -885,12 +932,14 @@ int launch_gradp_inplace(Ctx& c, int field0)
     float* f[3] = {(float*)c.f[field0], (float*)c.f[field0 + 1], (float*)c.f[field0 + 2]};
     hipLaunchKernelGGL((k_gradp<float, false, false>), grid, dim3(NT), 0, c.stream, (const float*)c.f[CUP_P], f[0],
                        f[1], f[2], f[0], f[1], f[2], c.grid(), c.sc, 0.0, 0.0, 0.0, (double*)nullptr,
-                       (const int*)nullptr, (const float*)nullptr, (const C2<float>*)c.tw[0], nbw);
+                       (const int*)nullptr, (const float*)nullptr, (const C2<float>*)c.tw[0], nbw,
+                       (unsigned long long*)nullptr, (unsigned long long*)nullptr);
   } else {
     double* f[3] = {(double*)c.f[field0], (double*)c.f[field0 + 1], (double*)c.f[field0 + 2]};
     hipLaunchKernelGGL((k_gradp<double, false, false>), grid, dim3(NT), 0, c.stream, (const double*)c.f[CUP_P],
                        f[0], f[1], f[2], f[0], f[1], f[2], c.grid(), c.sc, 0.0, 0.0, 0.0, (double*)nullptr,
-                       (const int*)nullptr, (const double*)nullptr, (const C2<double>*)c.tw[0], nbw);
+                       (const int*)nullptr, (const double*)nullptr, (const C2<double>*)c.tw[0], nbw,
+                       (unsigned long long*)nullptr, (unsigned long long*)nullptr);
   }
   return check_hip(c, hipGetLastError(), "k_gradp");
 }
@@ -1110,13 +1159,14 @@ int launch_reduce(Ctx& c, int n, int width, unsigned maxmask, double* out_dev, c
 }
 
 // Step bookkeeping after the fused gradient: umax of the new velocity from the atomic
-// max (then cleared for the next step), step += 1 and time += dt when advancing, then the
-// next dt by Simulation::calcMaxTimestep (Simulation.cpp:403-424), ramp included.
-__global__ __launch_bounds__(UMAX_SLOTS) void k_finalize_step(DevScalars* sc, double CFL, double nu, double h,
-                                                               int rampup, double fixed_dt, int advance)
+// max over one bank of slots (then cleared), step += 1 and time += dt when advancing, then
+// the next dt by the CFL rule.
+__global__ __launch_bounds__(UMAX_SLOTS) void k_finalize_step(DevScalars* sc, unsigned long long* slots, double CFL,
+                                                               double nu, double h, int rampup, double fixed_dt,
+                                                               int advance)
 {
-  double m[1] = {__longlong_as_double((long long)sc->umax_slots[threadIdx.x])};
-  sc->umax_slots[threadIdx.x] = 0ull;
+  double m[1] = {__longlong_as_double((long long)slots[threadIdx.x])};
+  slots[threadIdx.x] = 0ull;
   __shared__ double out[1];
   block_reduce<1>(m, 1u, out);
   __syncthreads();
@@ -1128,27 +1178,13 @@ __global__ __launch_bounds__(UMAX_SLOTS) void k_finalize_step(DevScalars* sc, do
     sc->time += sc->dt;
     sc->last_dt = sc->dt;
   }
-  double dt;
-  if (fixed_dt > 0) {
-    dt = fixed_dt;
-  } else {
-    const double dtDif = nu > 0 ? h * h / nu : INFINITY;
-    const double dtAdv = h / (umax + 1e-8);
-    const double mm = fmin(dtDif, dtAdv);
-    dt = CFL * mm;
-    if (sc->step < rampup) {
-      const double x = (sc->step + 1.0) / rampup;
-      const double rampCFL = exp(log(1e-3) * (1 - x) + log(CFL) * x);
-      dt = rampCFL * mm;
-    }
-  }
-  sc->dt = dt;
+  sc->dt = cfl_dt(umax, sc->step, CFL, nu, h, rampup, fixed_dt);
 }
 
 int launch_finalize_step(Ctx& c, double CFL, double nu, int rampup, double fixed_dt, int advance)
 {
-  hipLaunchKernelGGL(k_finalize_step, dim3(1), dim3(UMAX_SLOTS), 0, c.stream, c.sc, CFL, nu, c.h, rampup, fixed_dt,
-                     advance);
+  hipLaunchKernelGGL(k_finalize_step, dim3(1), dim3(UMAX_SLOTS), 0, c.stream, c.sc, umax_slots_dev(c, c.umax_bank), CFL,
+                     nu, c.h, rampup, fixed_dt, advance);
   return check_hip(c, hipGetLastError(), "k_finalize_step");
 }
 

@@ -850,6 +850,49 @@ void Analysis::operator()(double dt)
 // layout is not available here): a 64-byte header -- the 8 bytes "CUPFIELD", int32 version
 // (1), int32 ncomp, int32 cells x, y, z, int32 step, int32 0, FP64 time, 20 bytes of zeros --
 // then ncomp FP64 arrays in [comp][z][y][x] order, little-endian.
+// The XDMF 2 descriptor beside a field file, so that ParaView and VisIt open the fields as the
+// reference's HDF5 dumps open (Simulation.cpp:475-505, Cubism's DumpHDF5_MPI writes <name>.h5
+// and <name>.xmf): a 3DCoRectMesh of the cell corners with origin 0 and spacing h, and
+// cell-centred attributes read from the raw FP64 arrays after the 64-byte header (component q
+// of n cells at byte 64 + 8 n q). A vector field gets its three components and the vector
+// joined from them. The binary file is named relative to the descriptor.
+static void write_xdmf(const std::string& path, const std::string& binary, const SimulationData& sim,
+                       const char* const* names, int ncomp, const char* vector_name)
+{
+  const int N[3] = {sim.bpdx * 8, sim.bpdy * 8, sim.bpdz * 8};
+  const int maxN = std::max(N[0], std::max(N[1], N[2]));
+  const double h = sim.maxextent / maxN;
+  const size_t n = (size_t)N[0] * N[1] * N[2];
+  FILE* f = fopen(path.c_str(), "w");
+  if (!f) throw IOError("cannot write " + path);
+  const std::string base = binary.substr(binary.find_last_of('/') + 1);
+  fprintf(f, "<?xml version=\"1.0\" ?>\n<!DOCTYPE Xdmf SYSTEM \"Xdmf.dtd\" []>\n<Xdmf Version=\"2.0\">\n");
+  fprintf(f, " <Domain>\n  <Grid Name=\"cells\" GridType=\"Uniform\">\n");
+  fprintf(f, "   <Time Value=\"%.17e\"/>\n", sim.time);
+  fprintf(f, "   <Topology TopologyType=\"3DCoRectMesh\" Dimensions=\"%d %d %d\"/>\n", N[2] + 1, N[1] + 1, N[0] + 1);
+  fprintf(f, "   <Geometry GeometryType=\"ORIGIN_DXDYDZ\">\n");
+  fprintf(f, "    <DataItem Name=\"Origin\" Dimensions=\"3\" NumberType=\"Float\" Precision=\"8\" Format=\"XML\">0 0 0</DataItem>\n");
+  fprintf(f, "    <DataItem Name=\"Spacing\" Dimensions=\"3\" NumberType=\"Float\" Precision=\"8\" Format=\"XML\">%.17e %.17e %.17e</DataItem>\n", h, h, h);
+  fprintf(f, "   </Geometry>\n");
+  auto item = [&](int q, const char* indent) {
+    fprintf(f, "%s<DataItem Dimensions=\"%d %d %d\" NumberType=\"Float\" Precision=\"8\" Format=\"Binary\" Endian=\"Little\" Seek=\"%zu\">%s</DataItem>\n",
+            indent, N[2], N[1], N[0], 64 + 8 * n * (size_t)q, base.c_str());
+  };
+  for (int q = 0; q < ncomp; ++q) {
+    fprintf(f, "   <Attribute Name=\"%s\" AttributeType=\"Scalar\" Center=\"Cell\">\n", names[q]);
+    item(q, "    ");
+    fprintf(f, "   </Attribute>\n");
+  }
+  if (vector_name) {
+    fprintf(f, "   <Attribute Name=\"%s\" AttributeType=\"Vector\" Center=\"Cell\">\n", vector_name);
+    fprintf(f, "    <DataItem ItemType=\"Function\" Function=\"JOIN($0, $1, $2)\" Dimensions=\"%d %d %d 3\">\n", N[2], N[1], N[0]);
+    for (int q = 0; q < 3; ++q) item(q, "     ");
+    fprintf(f, "    </DataItem>\n   </Attribute>\n");
+  }
+  fprintf(f, "  </Grid>\n </Domain>\n</Xdmf>\n");
+  if (fclose(f) != 0) throw IOError("cannot write " + path);
+}
+
 static void write_field_file(const std::string& path, const SimulationData& sim, const int* fields, int ncomp)
 {
   const int N[3] = {sim.bpdx * 8, sim.bpdy * 8, sim.bpdz * 8};
@@ -933,9 +976,16 @@ void Simulation::_serialize(const std::string& append)
   status(fpath + ".status");
   if (sim.b3Ddump) {
     const int vel[3] = {CUP_U, CUP_V, CUP_W}, pres[1] = {CUP_P}, chi[1] = {CUP_CHI};
-    write_field_file(sim.path4serialization + "/vel_" + name + ".cupf", sim, vel, 3);
-    write_field_file(sim.path4serialization + "/pres_" + name + ".cupf", sim, pres, 1);
-    write_field_file(sim.path4serialization + "/chi_" + name + ".cupf", sim, chi, 1);
+    const char* const vel_names[3] = {"u", "v", "w"};
+    const char* const pres_names[1] = {"pressure"};
+    const char* const chi_names[1] = {"chi"};
+    const std::string root = sim.path4serialization + "/";
+    write_field_file(root + "vel_" + name + ".cupf", sim, vel, 3);
+    write_xdmf(root + "vel_" + name + ".xmf", root + "vel_" + name + ".cupf", sim, vel_names, 3, "velocity");
+    write_field_file(root + "pres_" + name + ".cupf", sim, pres, 1);
+    write_xdmf(root + "pres_" + name + ".xmf", root + "pres_" + name + ".cupf", sim, pres_names, 1, nullptr);
+    write_field_file(root + "chi_" + name + ".cupf", sim, chi, 1);
+    write_xdmf(root + "chi_" + name + ".xmf", root + "chi_" + name + ".cupf", sim, chi_names, 1, nullptr);
   }
   status(sim.path4serialization + "/restart.status");
 }

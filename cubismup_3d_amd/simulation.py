@@ -37,6 +37,47 @@ def read_field_file(path):
     return dict(time=time, step=step, data=data.reshape(ncomp, nz, ny, nx))
 
 
+def read_xdmf(path):
+    """The XDMF descriptor written beside each field file of the saves (vel_/pres_/chi_<name>.xmf):
+    parses the 3DCoRectMesh topology, the ORIGIN_DXDYDZ geometry and each cell-centred
+    attribute's binary DataItems (file, byte offset, dimensions, FP64 little endian), and reads
+    the arrays the way a visualisation tool does. Returns dict(time, cells (x, y, z), origin,
+    spacing (both z, y, x as written), attributes {name: array [z][y][x], or [z][y][x][3] for a
+    vector joined from three items})."""
+    import os
+    import xml.etree.ElementTree as ET
+    import numpy as np
+    root = ET.parse(path).getroot()
+    grid = root.find("Domain/Grid")
+    topo = grid.find("Topology")
+    if topo.get("TopologyType") != "3DCoRectMesh":
+        raise ValueError(f"{path}: topology {topo.get('TopologyType')}")
+    nz, ny, nx = (int(v) - 1 for v in topo.get("Dimensions").split())
+    geo = {d.get("Name"): np.array([float(v) for v in d.text.split()]) for d in grid.find("Geometry")}
+    here = os.path.dirname(os.path.abspath(path))
+
+    def item(d):
+        dims = tuple(int(v) for v in d.get("Dimensions").split())
+        if d.get("Format") != "Binary" or d.get("NumberType") != "Float" or d.get("Precision") != "8":
+            raise ValueError(f"{path}: unsupported DataItem")
+        dt = "<f8" if d.get("Endian", "Little") == "Little" else ">f8"
+        a = np.fromfile(os.path.join(here, d.text.strip()), dtype=dt, count=int(np.prod(dims)),
+                        offset=int(d.get("Seek", "0")))
+        if a.size != np.prod(dims):
+            raise ValueError(f"{path}: {d.text.strip()} is truncated")
+        return a.reshape(dims)
+    attrs = {}
+    for at in grid.findall("Attribute"):
+        d = at.find("DataItem")
+        if d.get("ItemType") == "Function":
+            attrs[at.get("Name")] = np.stack([item(s) for s in d.findall("DataItem")], axis=-1)
+        else:
+            attrs[at.get("Name")] = item(d)
+    t = grid.find("Time")
+    return dict(time=float(t.get("Value")) if t is not None else None, cells=(nx, ny, nz),
+                origin=geo["Origin"], spacing=geo["Spacing"], attributes=attrs)
+
+
 class Sphere:
     """ObstacleAndSphereArguments of the bindings (bindings/Obstacles.cpp:12-71, 79-128):
     ``radius`` or ``length``; ``position``; ``enforcedVelocity``, the velocity of the axes

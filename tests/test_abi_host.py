@@ -171,3 +171,38 @@ def test_noalloc_diagnostics_rejects_bad_arguments_before_touching_a_device(cup)
     odd = _lib.GridDesc((C.c_int * 3)(33, 33, 33), 1.0, 4, 1, 0)  # 35937 blocks, one per workgroup
     assert L.cup_diagnostics_noalloc(C.byref(odd), ptrs, 0.01, None, out) == _lib.CUP_E_INVALID
     assert b"scratch" in L.cup_last_error(None)
+
+
+def test_read_xdmf_follows_the_descriptor(cup, tmp_path):
+    """read_xdmf on a descriptor of the saves' form over a field file written here: the header
+    offset, component offsets and dimensions in the descriptor select the arrays (CPU)."""
+    import numpy as np
+    nx, ny, nz, h = 16, 8, 24, 0.25
+    rng = np.random.default_rng(1)
+    data = rng.standard_normal((3, nz, ny, nx))
+    binf = tmp_path / "vel_restart_000000004.cupf"
+    with open(binf, "wb") as f:
+        f.write(b"CUPFIELD" + bytes(56))
+        f.write(data.astype("<f8").tobytes())
+    n = nx * ny * nz
+
+    def item(q):
+        return (f'<DataItem Dimensions="{nz} {ny} {nx}" NumberType="Float" Precision="8" Format="Binary" '
+                f'Endian="Little" Seek="{64 + 8 * n * q}">{binf.name}</DataItem>')
+    xml = (f'<?xml version="1.0" ?><Xdmf Version="2.0"><Domain><Grid Name="cells" GridType="Uniform">'
+           f'<Time Value="0.5"/><Topology TopologyType="3DCoRectMesh" Dimensions="{nz + 1} {ny + 1} {nx + 1}"/>'
+           f'<Geometry GeometryType="ORIGIN_DXDYDZ"><DataItem Name="Origin" Dimensions="3" NumberType="Float" '
+           f'Precision="8" Format="XML">0 0 0</DataItem><DataItem Name="Spacing" Dimensions="3" NumberType="Float" '
+           f'Precision="8" Format="XML">{h} {h} {h}</DataItem></Geometry>'
+           + "".join(f'<Attribute Name="{c}" AttributeType="Scalar" Center="Cell">{item(q)}</Attribute>'
+                     for q, c in enumerate("uvw"))
+           + f'<Attribute Name="velocity" AttributeType="Vector" Center="Cell"><DataItem ItemType="Function" '
+             f'Function="JOIN($0, $1, $2)" Dimensions="{nz} {ny} {nx} 3">{item(0)}{item(1)}{item(2)}</DataItem>'
+             f'</Attribute></Grid></Domain></Xdmf>')
+    (tmp_path / "vel.xmf").write_text(xml)
+    x = cup.read_xdmf(str(tmp_path / "vel.xmf"))
+    assert x["time"] == 0.5 and x["cells"] == (nx, ny, nz)
+    assert list(x["spacing"]) == [h, h, h]
+    for q, c in enumerate("uvw"):
+        np.testing.assert_array_equal(x["attributes"][c], data[q])
+    np.testing.assert_array_equal(x["attributes"]["velocity"], np.moveaxis(data, 0, -1))

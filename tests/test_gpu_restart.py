@@ -47,6 +47,19 @@ def test_saves_and_restart_continue_exactly(cup, tmp_path):
         np.testing.assert_array_equal(vel["data"][q], Sa.field(name))
     chi = cup.read_field_file(str(a / "chi_restart_000000006.cupf"))
     np.testing.assert_array_equal(chi["data"][0], Sa.field("chi"))
+    # the XDMF descriptors open the same arrays on the grid of cell corners (Simulation.cpp:475-505)
+    h = 1.0 / Sa.field("u").shape[2]  # -extentx 1.0 along the longest side
+    for stem, names in (("vel", ("u", "v", "w")), ("pres", ("p",)), ("chi", ("chi",))):
+        x = cup.read_xdmf(str(a / f"{stem}_restart_000000006.xmf"))
+        assert x["time"] == SDa.time and x["cells"] == Sa.field("u").shape[::-1]
+        np.testing.assert_allclose(x["spacing"], [h, h, h], rtol=1e-15)
+        assert not x["origin"].any()
+        for name in names:
+            key = {"p": "pressure"}.get(name, name)
+            np.testing.assert_array_equal(x["attributes"][key], Sa.field(name))
+        if stem == "vel":
+            np.testing.assert_array_equal(x["attributes"]["velocity"],
+                                          np.stack([Sa.field(n) for n in "uvw"], axis=-1))
     ob = Sa.obstacle(0)
     with open(a / "restart_000000006_0.txt") as f:
         rows = [[float(x) for x in ln.split()] for ln in f.read().splitlines()]
