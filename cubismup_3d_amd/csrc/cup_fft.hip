@@ -116,11 +116,47 @@ __device__ __forceinline__ void dft16(C2<T>* v)
   }
 }
 
+// 3- and 5-point DFTs, W = e^{DIR 2 pi i / R}: X_q = sum_r v_r W^{q r}. The conjugate pairs
+// W^r + W^{-r} and W^r - W^{-r} are formed once (for R = 5 with c_m = cos(2 pi m/5),
+// s_m = DIR sin(2 pi m/5)).
+template <int DIR, typename T>
+__device__ __forceinline__ void dft3(C2<T>* v)
+{
+  const T s = (T)DIR * (T)0.86602540378443864676;  // sin(2 pi/3)
+  const C2<T> a = v[0], t1 = cadd(v[1], v[2]), t3 = csub(v[1], v[2]);
+  const C2<T> m = C2<T>{a.x - (T)0.5 * t1.x, a.y - (T)0.5 * t1.y};
+  const C2<T> n = C2<T>{-s * t3.y, s * t3.x};  // i s (v1 - v2)
+  v[0] = cadd(a, t1);
+  v[1] = cadd(m, n);
+  v[2] = csub(m, n);
+}
+
+template <int DIR, typename T>
+__device__ __forceinline__ void dft5(C2<T>* v)
+{
+  const T c1 = (T)0.30901699437494742410, c2 = (T)-0.80901699437494742410;
+  const T s1 = (T)DIR * (T)0.95105651629515357212, s2 = (T)DIR * (T)0.58778525229247312917;
+  const C2<T> a = v[0];
+  const C2<T> t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]), t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+  const C2<T> m1 = C2<T>{a.x + c1 * t1.x + c2 * t2.x, a.y + c1 * t1.y + c2 * t2.y};
+  const C2<T> m2 = C2<T>{a.x + c2 * t1.x + c1 * t2.x, a.y + c2 * t1.y + c1 * t2.y};
+  // i (s1 t3 + s2 t4) and i (s2 t3 - s1 t4)
+  const C2<T> n1 = C2<T>{-(s1 * t3.y + s2 * t4.y), s1 * t3.x + s2 * t4.x};
+  const C2<T> n2 = C2<T>{-(s2 * t3.y - s1 * t4.y), s2 * t3.x - s1 * t4.x};
+  v[0] = cadd(a, cadd(t1, t2));
+  v[1] = cadd(m1, n1);
+  v[4] = csub(m1, n1);
+  v[2] = cadd(m2, n2);
+  v[3] = csub(m2, n2);
+}
+
 template <int R, int DIR, typename T>
 __device__ __forceinline__ void dftR(C2<T>* v)
 {
   if constexpr (R == 8) dft8<DIR>(v);
   else if constexpr (R == 4) dft4<DIR>(v[0], v[1], v[2], v[3]);
+  else if constexpr (R == 5) dft5<DIR>(v);
+  else if constexpr (R == 3) dft3<DIR>(v);
   else dft2<DIR>(v);
 }
 
@@ -185,6 +221,75 @@ __device__ __forceinline__ void lds_fft(C2<T>* buf, const C2<T>* __restrict__ tw
   fft_stages<N, NFFT, 1, DIR>(buf, tw);
 }
 
+// The same Stockham stages for a length known at run time (the sizes that are not powers of
+// two: N = 8 x blocks with factors 2, 3 and 5, up to FFT_MAX_N), radices from the plan.
+// A stage's butterflies are held in registers across the barrier, as in fft_stage: at most
+// ITM = ceil(NFFT FFT_MAX_N / (R NT)) per thread, predicated on the actual count.
+template <int R, int NFFT, int DIR, typename T>
+__device__ __forceinline__ void fft_stage_rt(C2<T>* buf, const C2<T>* __restrict__ tw, int n, int Ns)
+{
+  constexpr int ITM = (NFFT * FFT_MAX_N / R + NT - 1) / NT;
+  const int NB = NFFT * (n / R), m = n / R, st = n / (Ns * R);
+  C2<T> v[ITM][R];
+#pragma unroll
+  for (int it = 0; it < ITM; ++it) {
+    const int t = threadIdx.x + it * NT;
+    if (t < NB) {
+      const int f = t % NFFT, j = t / NFFT, k = j % Ns;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[it][r] = buf[f + (j + r * m) * NFFT];
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        C2<T> w = tw[r * k * st];
+        if (DIR > 0) w.y = -w.y;
+        v[it][r] = cmul(v[it][r], w);
+      }
+      dftR<R, DIR>(v[it]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < ITM; ++it) {
+    const int t = threadIdx.x + it * NT;
+    if (t < NB) {
+      const int f = t % NFFT, j = t / NFFT, k = j % Ns;
+      const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[f + (base + r * Ns) * NFFT] = v[it][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <int NFFT, int DIR, typename T>
+__device__ __forceinline__ void lds_fft_rt(C2<T>* buf, const C2<T>* __restrict__ tw, const FftPlan& p)
+{
+  int Ns = 1;
+  for (int s = 0; s < p.ns; ++s) {
+    const int R = p.r[s];
+    if (R == 8) fft_stage_rt<8, NFFT, DIR>(buf, tw, p.n, Ns);
+    else if (R == 4) fft_stage_rt<4, NFFT, DIR>(buf, tw, p.n, Ns);
+    else if (R == 2) fft_stage_rt<2, NFFT, DIR>(buf, tw, p.n, Ns);
+    else if (R == 3) fft_stage_rt<3, NFFT, DIR>(buf, tw, p.n, Ns);
+    else fft_stage_rt<5, NFFT, DIR>(buf, tw, p.n, Ns);
+    Ns *= R;
+  }
+}
+
+// The LDS tile of a pass: a static array of COUNT elements, or for COUNT == 0 (a length known
+// at run time) the dynamic shared memory the launch sizes.
+template <typename T, int COUNT>
+__device__ __forceinline__ C2<T>* lds_tile()
+{
+  if constexpr (COUNT > 0) {
+    __shared__ __align__(32) C2<T> tile[COUNT > 0 ? COUNT : 1];
+    return tile;
+  } else {
+    extern __shared__ __align__(32) unsigned char dyn_lds[];
+    return reinterpret_cast<C2<T>*>(dyn_lds);
+  }
+}
+
 // --------------------------------------------------------------------- x passes
 
 constexpr int XROWS = 8;         // rows per workgroup = one block row of a z plane
@@ -206,9 +311,11 @@ __global__ __launch_bounds__(NT) void k_fft_x_fwd(const T* __restrict__ src0, co
                                                   const T* __restrict__ src2, C2<T>* __restrict__ S,
                                                   const C2<T>* __restrict__ tw, GridArgs g,
                                                   const DevScalars* __restrict__ sc, const int* __restrict__ ob_slot,
-                                                  const T* __restrict__ ob_rhs, double* __restrict__ rowbuf)
+                                                  const T* __restrict__ ob_rhs, double* __restrict__ rowbuf,
+                                                  FftPlan pl)
 {
-  __shared__ C2<T> buf[N * XFFT];
+  const int n = N > 0 ? N : pl.n;
+  C2<T>* buf = lds_tile<T, N * XFFT>();
   constexpr int NA = 1 + 2 * LOW_KX;  // row DFT values per component: kx=0 (real), kx=1..: re, im
   static_assert(LOW_KX == 2, "the row DFT below evaluates kx = 1, 2");
   static_assert(3 * NA + 1 == 16, "16 row values");
@@ -226,7 +333,7 @@ __global__ __launch_bounds__(NT) void k_fft_x_fwd(const T* __restrict__ src0, co
   // iy = (q/2)%8, x0 = 8 bx + 4 (q%2), with 16-byte loads of u, of v at y +- 1, of w at
   // z +- 1, and single loads of u across the quad's two x ends.
   using V4 = typename Vec4<T>::type;
-  for (int q4 = threadIdx.x; q4 < N * XROWS / 4; q4 += NT) {
+  for (int q4 = threadIdx.x; q4 < n * XROWS / 4; q4 += NT) {
     const int bx = q4 >> 4, iy = (q4 >> 1) & 7, half = q4 & 1, ix0 = 4 * half;
     const int x0 = bx * BS + ix0;
     const int b = bx + g.nbx * (by + g.nby * bz);
@@ -264,7 +371,7 @@ __global__ __launch_bounds__(NT) void k_fft_x_fwd(const T* __restrict__ src0, co
         const T* cv = vN;
         const T* cw = wB;
         // e^{-2 pi i x/Nx} at x0, rotated by the table's first entries for x0 + 1..3
-        const C2<T> e0 = tw[x0 & (N - 1)];
+        const C2<T> e0 = tw[x0];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
           C2<T> e1 = e0;
@@ -311,52 +418,55 @@ __global__ __launch_bounds__(NT) void k_fft_x_fwd(const T* __restrict__ src0, co
     const int zz = (q >= 2 * NA && q < 3 * NA) ? (z + 1 < g.Nz ? z + 1 : 0) : z;
     rowbuf[((size_t)zz * g.Ny + yy) * 16 + q] = v;
   }
-  lds_fft<N, XFFT, -1>(buf, tw);
+  if constexpr (N > 0) lds_fft<N, XFFT, -1>(buf, tw);
+  else lds_fft_rt<XFFT, -1>(buf, tw, pl);
   // Separate the two real transforms of each complex FFT and pack kx = N/2 into slot 0.
-  for (int t = threadIdx.x; t < XROWS * (N / 2); t += NT) {
-    const int row = t / (N / 2), k = t % (N / 2);
+  for (int t = threadIdx.x; t < XROWS * (n / 2); t += NT) {
+    const int row = t / (n / 2), k = t % (n / 2);
     const int q = row >> 1, odd = row & 1;
     const C2<T> Z = buf[k * XFFT + q];
     C2<T> out;
     if (k == 0) {
-      const C2<T> Zn = buf[(N / 2) * XFFT + q];
+      const C2<T> Zn = buf[(n / 2) * XFFT + q];
       out = odd ? C2<T>{Z.y, Zn.y} : C2<T>{Z.x, Zn.x};
     } else {
-      const C2<T> Zm = buf[(N - k) * XFFT + q];
+      const C2<T> Zm = buf[(n - k) * XFFT + q];
       // Xa = (Z + conj Zm)/2 ; Xb = -i (Z - conj Zm)/2
       out = odd ? C2<T>{(T)0.5 * (Z.y + Zm.y), (T)0.5 * (Zm.x - Z.x)}
                 : C2<T>{(T)0.5 * (Z.x + Zm.x), (T)0.5 * (Z.y - Zm.y)};
     }
     const int y = by * BS + row;
-    S[((size_t)z * g.Ny + y) * (N / 2) + k] = out;
+    S[((size_t)z * g.Ny + y) * (n / 2) + k] = out;
   }
 }
 
 // Packed half spectrum -> real rows of a block-layout field. grid = (Ny/8, Nz).
 template <typename T, int N>
 __global__ __launch_bounds__(NT) void k_fft_x_inv(const C2<T>* __restrict__ S, T* __restrict__ dst,
-                                                  const C2<T>* __restrict__ tw, GridArgs g)
+                                                  const C2<T>* __restrict__ tw, GridArgs g, FftPlan pl)
 {
-  __shared__ C2<T> buf[N * XFFT];
+  const int n = N > 0 ? N : pl.n;
+  C2<T>* buf = lds_tile<T, N * XFFT>();
   const int by = blockIdx.x, z = blockIdx.y;
   const int bz = z >> 3, iz = z & 7;
-  for (int t = threadIdx.x; t < XFFT * (N / 2); t += NT) {
-    const int q = t / (N / 2), k = t % (N / 2);
+  for (int t = threadIdx.x; t < XFFT * (n / 2); t += NT) {
+    const int q = t / (n / 2), k = t % (n / 2);
     const int ya = by * BS + 2 * q;
-    const C2<T> A = S[((size_t)z * g.Ny + ya) * (N / 2) + k];
-    const C2<T> B = S[((size_t)z * g.Ny + ya + 1) * (N / 2) + k];
+    const C2<T> A = S[((size_t)z * g.Ny + ya) * (n / 2) + k];
+    const C2<T> B = S[((size_t)z * g.Ny + ya + 1) * (n / 2) + k];
     if (k == 0) {
       buf[0 * XFFT + q] = C2<T>{A.x, B.x};
-      buf[(N / 2) * XFFT + q] = C2<T>{A.y, B.y};
+      buf[(n / 2) * XFFT + q] = C2<T>{A.y, B.y};
     } else {
       buf[k * XFFT + q] = C2<T>{A.x - B.y, A.y + B.x};
-      buf[(N - k) * XFFT + q] = C2<T>{A.x + B.y, B.x - A.y};
+      buf[(n - k) * XFFT + q] = C2<T>{A.x + B.y, B.x - A.y};
     }
   }
   __syncthreads();
-  lds_fft<N, XFFT, +1>(buf, tw);
+  if constexpr (N > 0) lds_fft<N, XFFT, +1>(buf, tw);
+  else lds_fft_rt<XFFT, +1>(buf, tw, pl);
   const T* rb = (const T*)buf;
-  for (int t = threadIdx.x; t < N * XROWS; t += NT) {
+  for (int t = threadIdx.x; t < n * XROWS; t += NT) {
     const int bx = t >> 6, iy = (t >> 3) & 7, ix = t & 7;
     const int x = bx * BS + ix;
     const int b = bx + g.nbx * (by + g.nby * bz);
@@ -368,8 +478,9 @@ __global__ __launch_bounds__(NT) void k_fft_x_inv(const C2<T>* __restrict__ S, T
 
 template <typename T, int N>
 struct ColTile {
-  // columns per workgroup: LDS tile at most 64 KiB, at most 16 columns
-  static constexpr int raw = 65536 / (N * (int)sizeof(C2<T>));
+  // columns per workgroup: LDS tile at most 64 KiB, at most 16 columns; a length known at run
+  // time (N = 0, at most FFT_MAX_N) takes 4 (FP32) or 2 (FP64) columns, 32 KiB at most
+  static constexpr int raw = N > 0 ? 65536 / (N * (int)sizeof(C2<T>)) : (sizeof(T) == 4 ? 4 : 2);
   static constexpr int TK = raw >= 16 ? 16 : (raw < 1 ? 1 : raw);
 };
 
@@ -381,30 +492,34 @@ template <typename T, int N, int AXIS, int MODE, bool FORCE = false>
 __global__ __launch_bounds__(NT) void k_fft_col(C2<T>* __restrict__ S, const C2<T>* __restrict__ tw, int ncols,
                                                 int Ny, int Nz, double norm, const double* __restrict__ cx,
                                                 const double* __restrict__ cy, const double* __restrict__ cz,
-                                                const LowModes* __restrict__ lmp, const double* __restrict__ coef)
+                                                const LowModes* __restrict__ lmp, const double* __restrict__ coef,
+                                                FftPlan pl)
 {
   constexpr int TK = ColTile<T, N>::TK;
   static_assert(TK % 2 == 0, "columns move in pairs");
-  __shared__ __align__(32) C2<T> buf[N * TK];
+  const int n = N > 0 ? N : pl.n;
+  C2<T>* buf = lds_tile<T, N * TK>();
   const int kx0 = blockIdx.x * TK, other = blockIdx.y;
   const int nc = min(TK, ncols - kx0);
   // element (f, n): AXIS=1: S[(other*Ny + n)*ncols + kx0 + f]; AXIS=2: S[(n*Ny + other)*ncols + kx0 + f].
   // Columns move in pairs (ncols, TK and so nc are even): one 2-element vector access each.
   using V4 = typename Vec4<T>::type;
-  for (int t = threadIdx.x; t < N * TK / 2; t += NT) {
-    const int f = 2 * (t % (TK / 2)), n = t / (TK / 2);
+  for (int t = threadIdx.x; t < n * TK / 2; t += NT) {
+    const int f = 2 * (t % (TK / 2)), e = t / (TK / 2);
     V4 a = {0, 0, 0, 0};
     if (f < nc) {
-      const size_t gi = AXIS == 1 ? ((size_t)other * Ny + n) * ncols + kx0 + f : ((size_t)n * Ny + other) * ncols + kx0 + f;
+      const size_t gi = AXIS == 1 ? ((size_t)other * Ny + e) * ncols + kx0 + f : ((size_t)e * Ny + other) * ncols + kx0 + f;
       a = *reinterpret_cast<const V4*>(S + gi);
     }
-    *reinterpret_cast<V4*>(buf + f + n * TK) = a;
+    *reinterpret_cast<V4*>(buf + f + e * TK) = a;
   }
   __syncthreads();
   if (MODE == ZM_INV) {
-    lds_fft<N, TK, +1>(buf, tw);
+    if constexpr (N > 0) lds_fft<N, TK, +1>(buf, tw);
+    else lds_fft_rt<TK, +1>(buf, tw, pl);
   } else {
-    lds_fft<N, TK, -1>(buf, tw);
+    if constexpr (N > 0) lds_fft<N, TK, -1>(buf, tw);
+    else lds_fft_rt<TK, -1>(buf, tw, pl);
   }
   if (FORCE) {
     // fused forcing: add the forcing's RHS modes (coefficients from k_forcing_reduce)
@@ -425,13 +540,13 @@ __global__ __launch_bounds__(NT) void k_fft_col(C2<T>* __restrict__ S, const C2<
     // multiply by norm/(2D-6), D = cos(4 pi kx/Nx) + cos(4 pi ky/Ny) + cos(4 pi kz/Nz),
     // in FP64; zero the modes with every index in {0, N/2} (kx slot 0 carries kx=0 and Nx/2).
     const int ky = other;
-    for (int t = threadIdx.x; t < N * TK; t += NT) {
+    for (int t = threadIdx.x; t < n * TK; t += NT) {
       const int f = t % TK, kz = t / TK;
       const int kx = kx0 + f;
       if (f < nc) {
         const double D = cx[kx] + cy[ky] + cz[kz];
         double fac = norm / (2 * D - 6);
-        const bool zero = kx == 0 && (ky == 0 || 2 * ky == Ny) && (kz == 0 || 2 * kz == N);
+        const bool zero = kx == 0 && (ky == 0 || 2 * ky == Ny) && (kz == 0 || 2 * kz == n);
         if (zero) fac = 0;
         C2<T> v = buf[f + kz * TK];
         v.x = (T)((double)v.x * fac);
@@ -441,9 +556,10 @@ __global__ __launch_bounds__(NT) void k_fft_col(C2<T>* __restrict__ S, const C2<
       }
     }
     __syncthreads();
-    lds_fft<N, TK, +1>(buf, tw);
+    if constexpr (N > 0) lds_fft<N, TK, +1>(buf, tw);
+    else lds_fft_rt<TK, +1>(buf, tw, pl);
   }
-  for (int t = threadIdx.x; t < N * TK / 2; t += NT) {
+  for (int t = threadIdx.x; t < n * TK / 2; t += NT) {
     const int f = 2 * (t % (TK / 2)), n = t / (TK / 2);
     if (f < nc) {
       const size_t gi = AXIS == 1 ? ((size_t)other * Ny + n) * ncols + kx0 + f : ((size_t)n * Ny + other) * ncols + kx0 + f;
@@ -656,33 +772,36 @@ __global__ void k_divide(C2<T>* __restrict__ S, int ncols, int Ny, int Nz, doubl
 
 template <typename T, int N>
 struct XPass {
+  // dynamic LDS of the passes whose length is known at run time (N = 0)
+  static size_t lds(const Ctx& c) { return N > 0 ? 0 : (size_t)c.N[0] * XFFT * sizeof(C2<T>); }
   static void fwd(Ctx& c, const T* s0, const T* s1, const T* s2, int src, bool force)
   {
     dim3 grid(c.N[1] / BS, c.N[2]);
     const int* slot = c.body.present && !c.plain_rhs ? c.ob_slot_all : nullptr;
     if (src == XSRC_FIELD)
-      hipLaunchKernelGGL((k_fft_x_fwd<T, N, XSRC_FIELD, false>), grid, dim3(NT), 0, c.stream, s0, s1, s2,
+      hipLaunchKernelGGL((k_fft_x_fwd<T, N, XSRC_FIELD, false>), grid, dim3(NT), lds(c), c.stream, s0, s1, s2,
                          (C2<T>*)c.spec, (const C2<T>*)c.tw[0], c.grid(), c.sc, (const int*)nullptr,
-                         (const T*)nullptr, (double*)nullptr);
+                         (const T*)nullptr, (double*)nullptr, c.plan[0]);
     else if (force)
-      hipLaunchKernelGGL((k_fft_x_fwd<T, N, XSRC_DIV, true>), grid, dim3(NT), 0, c.stream, s0, s1, s2, (C2<T>*)c.spec,
-                         (const C2<T>*)c.tw[0], c.grid(), c.sc, slot, (const T*)c.ob_rhs_all, c.low_part);
+      hipLaunchKernelGGL((k_fft_x_fwd<T, N, XSRC_DIV, true>), grid, dim3(NT), lds(c), c.stream, s0, s1, s2, (C2<T>*)c.spec,
+                         (const C2<T>*)c.tw[0], c.grid(), c.sc, slot, (const T*)c.ob_rhs_all, c.low_part, c.plan[0]);
     else
-      hipLaunchKernelGGL((k_fft_x_fwd<T, N, XSRC_DIV, false>), grid, dim3(NT), 0, c.stream, s0, s1, s2,
+      hipLaunchKernelGGL((k_fft_x_fwd<T, N, XSRC_DIV, false>), grid, dim3(NT), lds(c), c.stream, s0, s1, s2,
                          (C2<T>*)c.spec, (const C2<T>*)c.tw[0], c.grid(), c.sc, slot, (const T*)c.ob_rhs_all,
-                         (double*)nullptr);
+                         (double*)nullptr, c.plan[0]);
   }
   static void fwd_to(Ctx& c, const T* s0, C2<T>* spec)
   {
     dim3 grid(c.N[1] / BS, c.N[2]);
-    hipLaunchKernelGGL((k_fft_x_fwd<T, N, XSRC_FIELD, false>), grid, dim3(NT), 0, c.stream, s0, s0, s0, spec,
+    hipLaunchKernelGGL((k_fft_x_fwd<T, N, XSRC_FIELD, false>), grid, dim3(NT), lds(c), c.stream, s0, s0, s0, spec,
                        (const C2<T>*)c.tw[0], c.grid(), c.sc, (const int*)nullptr, (const T*)nullptr,
-                       (double*)nullptr);
+                       (double*)nullptr, c.plan[0]);
   }
   static void inv(Ctx& c, const C2<T>* spec, T* dst)
   {
     dim3 grid(c.N[1] / BS, c.N[2]);
-    hipLaunchKernelGGL((k_fft_x_inv<T, N>), grid, dim3(NT), 0, c.stream, spec, dst, (const C2<T>*)c.tw[0], c.grid());
+    hipLaunchKernelGGL((k_fft_x_inv<T, N>), grid, dim3(NT), lds(c), c.stream, spec, dst, (const C2<T>*)c.tw[0], c.grid(),
+                       c.plan[0]);
   }
 };
 
@@ -716,18 +835,19 @@ struct ColPass {
     const int other = AXIS == 1 ? c.N[2] : c.N[1];
     dim3 grid((ncols + TK - 1) / TK, other);
     const C2<T>* tw = (const C2<T>*)c.tw[AXIS];
+    const size_t lds = N > 0 ? 0 : (size_t)c.N[AXIS] * TK * sizeof(C2<T>);
     if (mode == ZM_FWD)
-      hipLaunchKernelGGL((k_fft_col<T, N, AXIS, ZM_FWD>), grid, dim3(NT), 0, c.stream, spec, tw, ncols, c.N[1], c.N[2],
-                         norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low_dev, (const double*)nullptr);
+      hipLaunchKernelGGL((k_fft_col<T, N, AXIS, ZM_FWD>), grid, dim3(NT), lds, c.stream, spec, tw, ncols, c.N[1], c.N[2],
+                         norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low_dev, (const double*)nullptr, c.plan[AXIS]);
     else if (mode == ZM_INV)
-      hipLaunchKernelGGL((k_fft_col<T, N, AXIS, ZM_INV>), grid, dim3(NT), 0, c.stream, spec, tw, ncols, c.N[1], c.N[2],
-                         norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low_dev, (const double*)nullptr);
+      hipLaunchKernelGGL((k_fft_col<T, N, AXIS, ZM_INV>), grid, dim3(NT), lds, c.stream, spec, tw, ncols, c.N[1], c.N[2],
+                         norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low_dev, (const double*)nullptr, c.plan[AXIS]);
     else if (AXIS == 2 && force)
-      hipLaunchKernelGGL((k_fft_col<T, N, 2, ZM_POISSON, true>), grid, dim3(NT), 0, c.stream, spec, tw, ncols, c.N[1],
-                         c.N[2], norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low_dev, (const double*)c.low_coef);
+      hipLaunchKernelGGL((k_fft_col<T, N, 2, ZM_POISSON, true>), grid, dim3(NT), lds, c.stream, spec, tw, ncols, c.N[1],
+                         c.N[2], norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low_dev, (const double*)c.low_coef, c.plan[AXIS]);
     else if (AXIS == 2)
-      hipLaunchKernelGGL((k_fft_col<T, N, 2, ZM_POISSON, false>), grid, dim3(NT), 0, c.stream, spec, tw, ncols, c.N[1],
-                         c.N[2], norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low_dev, (const double*)nullptr);
+      hipLaunchKernelGGL((k_fft_col<T, N, 2, ZM_POISSON, false>), grid, dim3(NT), lds, c.stream, spec, tw, ncols, c.N[1],
+                         c.N[2], norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low_dev, (const double*)nullptr, c.plan[AXIS]);
   }
 };
 
@@ -744,7 +864,7 @@ static bool dispatch_n(int n, A&&... a)
     case 256: F<T, 256>::go(a...); return true;
     case 512: F<T, 512>::go(a...); return true;
     case 1024: F<T, 1024>::go(a...); return true;
-    default: return false;
+    default: F<T, 0>::go(a...); return true;  // the plan's radices, fft_setup checked the length
   }
 }
 
@@ -891,8 +1011,21 @@ int fft_setup(Ctx& c)
 {
   for (int a = 0; a < 3; ++a) {
     const int n = c.N[a];
-    if (n < 8 || n > 1024 || (n & (n - 1)) != 0)
-      return set_error(c, CUP_E_INVALID, "FFT sizes must be powers of two between 8 and 1024 cells per axis");
+    // powers of two take the kernels specialised for their length; other lengths the
+    // run-time radix plan (8, 4, 2, then 3 and 5), as FFTW plans any size
+    // (PoissonSolverPeriodic.cpp:108-121)
+    FftPlan& pl = c.plan[a];
+    pl = FftPlan{};
+    pl.n = n;
+    int rem = n;
+    for (int r : {8, 4, 2, 3, 5})
+      while (rem % r == 0 && pl.ns < 16) {
+        pl.r[pl.ns++] = r;
+        rem /= r;
+      }
+    if (n < 8 || n > FFT_MAX_N || rem != 1)
+      return set_error(c, CUP_E_INVALID,
+                       "FFT sizes must be between 8 and 1024 cells per axis with no prime factor above 5");
     std::vector<double> tw(2 * (size_t)n), cs((size_t)n);
     std::vector<float> s2((size_t)n);
     for (int t = 0; t < n; ++t) {

@@ -54,6 +54,14 @@ struct DevScalars {
 };
 constexpr int UMAX_SLOTS = 256;
 
+// Radix plan of a transform length that is not a power of two (FFT passes with the length
+// known at run time): n = r[0] r[1] ... r[ns-1], radices 8, 4, 2, 3, 5.
+constexpr int FFT_MAX_N = 1024;
+struct FftPlan {
+  int n = 0, ns = 0;
+  int r[16] = {};
+};
+
 // Canonical low modes of the band-limited forcing (0 < |k|^2 <= 4, one member of each
 // conjugate pair) and their positions in the packed half spectrum; passed by value.
 constexpr int LOW_MAX = 32;  // modes supported by the fused forcing
@@ -119,6 +127,7 @@ struct Ctx {
   void* tw[3] = {};       // FFT twiddles per axis, e^{-2 pi i t / N}, context precision
   double* cos4[3] = {};   // cos(4 pi k / N) per axis (FP64), the Poisson symbol
   float* sin2[3] = {};    // sin^2(2 pi k / N) per axis (FP32): 2 cos(4 pi k/N) - 2 = -4 sin^2(2 pi k/N)
+  FftPlan plan[3];        // per axis, for the lengths that are not powers of two
   double* partials = nullptr;
   size_t partials_cap = 0;  // doubles
   double* ke_part = nullptr;  // [nb][4] partial sums |u|^2 of the fused step's velocity (per wave)

@@ -8,13 +8,15 @@
 
 namespace cup {
 
-// The forcing's plane stage splits each z plane into up to FR_PARTS_MAX row ranges of at
-// least 64 rows (Ny is a power of two), one workgroup each.
+// The forcing's plane stage splits each z plane into up to FR_PARTS_MAX equal row ranges of
+// at least 64 rows, one workgroup each.
 constexpr int FR_PARTS_MAX = 4;
 inline int forcing_plane_parts(int ny)
 {
-  const int p = ny / 64;
-  return p < 1 ? 1 : (p > FR_PARTS_MAX ? FR_PARTS_MAX : p);
+  int p = ny / 64;
+  p = p < 1 ? 1 : (p > FR_PARTS_MAX ? FR_PARTS_MAX : p);
+  while (ny % p) --p;
+  return p;
 }
 
 __device__ __forceinline__ int signed_k(int i, int n) { return i <= n / 2 ? i : i - n; }
@@ -541,7 +543,7 @@ __global__ __launch_bounds__(NT) void k_forcing_planes(const double* __restrict_
 #pragma unroll 4
   for (int y = y0 + grp; y < y0 + ny; y += 16) {
     const double R = rowbuf[((size_t)z * Ny + y) * 16 + sl];
-    const C2<T> e1 = twy[y], e2 = twy[(2 * y) & (Ny - 1)];  // e^{-i a y}, e^{-2 i a y}
+    const C2<T> e1 = twy[y], e2 = twy[2 * y < Ny ? 2 * y : 2 * y - Ny];  // e^{-i a y}, e^{-2 i a y}
     a[0] += R;
     a[1] += R * (double)e1.x;
     a[2] += R * (double)e1.y;
@@ -608,7 +610,7 @@ __global__ __launch_bounds__(NT) void k_forcing_reduce(const double* __restrict_
       }
       // U = (rr + i ri) + i sg (ir + i ii)
       const double Ur = rr - sg * ii, Ui = ri + sg * ir;
-      const C2<T> e = twz[(kz * z) & (N2 - 1)];
+      const C2<T> e = twz[((kz * z) % N2 + N2) % N2];
       const double er = (double)e.x, ei = (double)e.y;
       ar += Ur * er - Ui * ei;
       ai += Ur * ei + Ui * er;
@@ -722,14 +724,14 @@ __global__ __launch_bounds__(FROWS_NT) void k_forcing_rows(const LowModes* __res
 {
   const int row = blockIdx.x * FROWS_NT + threadIdx.x;
   if (row >= Ny * Nz) return;
-  const int y = row & (Ny - 1), z = row / Ny;
+  const int y = row % Ny, z = row / Ny;
   const LowModes& lm = *lmp;
   double r[16];
 #pragma unroll
   for (int q = 0; q < 16; ++q) r[q] = 0;
   for (int m = 0; m < lm.n; ++m) {
     const int kx = lm.k[m][0], ky = lm.k[m][1], kz = lm.k[m][2];
-    const C2<T> ey = twy[(ky * y) & (Ny - 1)], ez = twz[(kz * z) & (Nz - 1)];
+    const C2<T> ey = twy[((ky * y) % Ny + Ny) % Ny], ez = twz[((kz * z) % Nz + Nz) % Nz];
     const double co = (double)ey.x * ez.x - (double)ey.y * ez.y;
     const double si = -((double)ey.x * ez.y + (double)ey.y * ez.x);
     const double* cm = coef + 8 * m;

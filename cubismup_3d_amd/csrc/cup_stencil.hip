@@ -823,7 +823,7 @@ __global__ __launch_bounds__(NT) void k_gradp(const T* __restrict__ p, const T* 
     const int x0 = (b0 % g.nbx) * BS + ix0;
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
-      e1[k] = twx[(x0 + k) & (g.Nx - 1)];                                      // e^{-i a x}
+      e1[k] = twx[x0 + k];                                                     // e^{-i a x}
       e2[k] = C2<T>{e1[k].x * e1[k].x - e1[k].y * e1[k].y, 2 * e1[k].x * e1[k].y};  // e^{-2 i a x}
     }
   }

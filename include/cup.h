@@ -64,7 +64,9 @@ typedef struct cup_ctx cup_ctx;
 
 /* Grid and block store. Replaces the Cubism FluidGridMPI allocation in
  * Simulation::setupGrid (Simulation.cpp:221-243) and the PoissonSolverPeriodic buffers and
- * plans (PoissonSolverPeriodic.cpp:90-127). Only fully periodic boxes are supported. */
+ * plans (PoissonSolverPeriodic.cpp:90-127). Only fully periodic boxes are supported. Cells per
+ * axis (8 x blocks) from 8 to 1024 with no prime factor above 5; other sizes are refused with
+ * CUP_E_INVALID (FFTW plans any size). */
 int cup_create(const cup_grid_desc* desc, cup_ctx** out);
 int cup_destroy(cup_ctx* ctx);
 /* The context's last error message; with ctx == NULL, the message of the calling thread's

@@ -637,6 +637,56 @@ static void fft_line(double complex* a, int n, const double complex* tw)
   }
 }
 
+/* Mixed-radix FFT for lengths that are not powers of two (FFTW plans any size,
+ * PoissonSolverPeriodic.cpp:108-121): recursive decimation in time by the smallest prime
+ * factor p of n. The p interleaved subsequences are transformed into out[r m .. r m + m),
+ * m = n/p, then X[k + m q] = sum_r W_n^{r (k + m q)} Y_r[k]. twf[j] = exp(sign 2 pi i j / N)
+ * for the top-level length N; at a level of length n the root W_n is twf[ts], ts = N/n. */
+static void fft_mixed(const double complex* in, double complex* out, int n, int stride, const double complex* twf,
+                      int ts)
+{
+  if (n == 1) {
+    out[0] = in[0];
+    return;
+  }
+  int p = 2;
+  while (n % p) p++;
+  const int m = n / p;
+  for (int r = 0; r < p; r++) fft_mixed(in + (size_t)r * stride, out + (size_t)r * m, m, stride * p, twf, ts * p);
+  double complex y[p], x[p];
+  for (int k = 0; k < m; k++) {
+    for (int r = 0; r < p; r++) y[r] = out[(size_t)r * m + k];
+    for (int q = 0; q < p; q++) {
+      double complex s = 0;
+      for (int r = 0; r < p; r++) s += y[r] * twf[(size_t)(((long)r * (k + (long)m * q)) % n) * ts];
+      x[q] = s;
+    }
+    for (int q = 0; q < p; q++) out[(size_t)q * m + k] = x[q];
+  }
+}
+
+/* One line of any length n: the radix-2 loop above for powers of two, else fft_mixed. */
+static void fft_any(double complex* a, int n, const double complex* tw, const double complex* twf,
+                    double complex* scratch)
+{
+  if ((n & (n - 1)) == 0) {
+    fft_line(a, n, tw);
+    return;
+  }
+  fft_mixed(a, scratch, n, 1, twf, 1);
+  memcpy(a, scratch, sizeof(double complex) * (size_t)n);
+}
+
+static double complex* make_tw_full(int n, int sign)
+{
+  double complex* tw = (double complex*)malloc(sizeof(double complex) * (size_t)n);
+  for (int k = 0; k < n; k++) {
+    const double ang = 2.0 * M_PI * (double)k / (double)n;
+    tw[k] = cos(ang) + I * (sign * sin(ang));
+  }
+  return tw;
+}
+
 static double complex* make_tw(int n, int sign)
 {
   double complex* tw = (double complex*)malloc(sizeof(double complex) * (n / 2 + 1));
@@ -653,31 +703,37 @@ static void fft3d_c(double complex* c, int nx, int ny, int nz, int sign)
   double complex* twx = make_tw(nx, sign);
   double complex* twy = make_tw(ny, sign);
   double complex* twz = make_tw(nz, sign);
+  double complex* tfx = make_tw_full(nx, sign);
+  double complex* tfy = make_tw_full(ny, sign);
+  double complex* tfz = make_tw_full(nz, sign);
   #pragma omp parallel
   {
     int nmax = nx > ny ? nx : ny;
     nmax = nmax > nz ? nmax : nz;
     double complex* line = (double complex*)malloc(sizeof(double complex) * nmax);
+    double complex* scratch = (double complex*)malloc(sizeof(double complex) * nmax);
     #pragma omp for schedule(static)
-    for (int xy = 0; xy < nx * ny; xy++) fft_line(c + (size_t)xy * nz, nz, twz);
+    for (int xy = 0; xy < nx * ny; xy++) fft_any(c + (size_t)xy * nz, nz, twz, tfz, scratch);
     #pragma omp for schedule(static)
     for (int xz = 0; xz < nx * nz; xz++) {
       const int x = xz / nz, z = xz % nz;
       double complex* base = c + (size_t)x * ny * nz + z;
       for (int y = 0; y < ny; y++) line[y] = base[(size_t)y * nz];
-      fft_line(line, ny, twy);
+      fft_any(line, ny, twy, tfy, scratch);
       for (int y = 0; y < ny; y++) base[(size_t)y * nz] = line[y];
     }
     #pragma omp for schedule(static)
     for (int yz = 0; yz < ny * nz; yz++) {
       double complex* base = c + yz;
       for (int x = 0; x < nx; x++) line[x] = base[(size_t)x * ny * nz];
-      fft_line(line, nx, twx);
+      fft_any(line, nx, twx, tfx, scratch);
       for (int x = 0; x < nx; x++) base[(size_t)x * ny * nz] = line[x];
     }
     free(line);
+    free(scratch);
   }
   free(twx); free(twy); free(twz);
+  free(tfx); free(tfy); free(tfz);
 }
 
 void orc_fft3d(int nx, int ny, int nz, double* re, double* im, int sign)

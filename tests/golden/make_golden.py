@@ -5,6 +5,10 @@ plain numpy algebra), independent of the oracle and of the device code:
   poisson_golden.npz  a random periodic RHS on 16^3 (h = 2pi/16) and the solution of the
                       grid-consistent periodic Poisson problem of
                       PoissonSolverPeriodic.cpp:15-61, computed with numpy.fft
+  fft_mixed_golden.npz  lengths that are not powers of two: random complex lines of 12, 24,
+                      48 and 96 points and a random real 24 x 40 x 12 array with their
+                      numpy.fft transforms, and the Poisson solution of a random RHS on
+                      24 x 16 x 40 cells
   philox_kat.npz      Philox-4x32-10 known-answer vectors (Random123 kat_vectors)
 
 Run: python tests/golden/make_golden.py
@@ -43,6 +47,20 @@ def main():
     rhs = rng.standard_normal((16, 16, 16))
     h = 2 * np.pi / 16
     np.savez_compressed(os.path.join(HERE, "poisson_golden.npz"), rhs=rhs, h=h, p=poisson_numpy(rhs, h))
+    mixed = {}
+    for n in (12, 24, 48, 96):
+        line = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        mixed[f"line{n}"] = line
+        mixed[f"LINE{n}"] = np.fft.fft(line)
+    m = rng.standard_normal((24, 40, 12))
+    mixed["m"] = m
+    mixed["M"] = np.fft.fftn(m)
+    rhs2 = rng.standard_normal((24, 16, 40))
+    h2 = 2 * np.pi / 40
+    mixed["rhs"] = rhs2
+    mixed["h"] = h2
+    mixed["p"] = poisson_numpy(rhs2, h2)
+    np.savez_compressed(os.path.join(HERE, "fft_mixed_golden.npz"), **mixed)
     kat = np.array([
         [0, 0, 0, 0, 0, 0, 0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8],
         [0xffffffff] * 6 + [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd],

@@ -37,8 +37,6 @@ def test_lab_periodic_shift_device(cup, prec, g, shift):
 
 @pytest.mark.parametrize("bpd", [(2, 2, 2), (4, 2, 3)])
 def test_advdiff_fp64_parity(cup, oracle, bpd):
-    if bpd[2] == 3:
-        pytest.skip("FFT sizes are powers of two; advection alone is size-agnostic")
     print(f"advdiff parity bpd={bpd}", flush=True)
     rng = np.random.default_rng(11)
     G = cup.Grid(bpd, precision=8)
