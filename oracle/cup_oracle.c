@@ -104,6 +104,8 @@ struct orc_sim {
   OBlock** ob;
   double CoM[3], mass;
   double corr[6];
+  int bcf[3];        /* per axis: ORC_BC_PERIODIC, ORC_BC_OPEN, ORC_BC_WALL */
+  double umax_meas;  /* SimulationData::uMax_measured, set by orc_calc_max_timestep */
 };
 
 static const double EPS = DBL_EPSILON;
@@ -127,6 +129,14 @@ orc_sim* orc_create(int bpdx, int bpdy, int bpdz, double maxextent)
   s->rhs = (double*)calloc((size_t)s->N[0] * s->N[1] * s->N[2], sizeof(double));
   return s;
 }
+
+/* Boundary conditions per axis (SimulationData's BCx/y/z_flag, SimulationData.cpp:123-166). */
+void orc_set_bc(orc_sim* s, const int bc[3])
+{
+  for (int d = 0; d < 3; d++) s->bcf[d] = bc[d];
+}
+
+void orc_set_umax_measured(orc_sim* s, double umax) { s->umax_meas = umax; }
 
 static void free_sphere(orc_sim* s)
 {
@@ -257,6 +267,43 @@ static void lab_load(const orc_sim* s, int b, Lab* l)
       for (int ix = -g; ix < BS + g; ix++)
         l->c[((iz + g) * L + (iy + g)) * L + (ix + g)] =
             *gcell(s, bc[0] * BS + ix, bc[1] * BS + iy, bc[2] * BS + iz);
+  /* BlockLabBC::_apply_bc (Definitions.h:349-377) on the domain faces of the non-periodic
+   * axes, x then y then z, over the face's own extent [0, BS) in the other two axes:
+   * open (the reference's dirichlet == absorbing == freespace, applyBCfaceOpen :278-301)
+   * copies the boundary cell into every halo layer; wall (applyBCfaceWall :302-334) mirrors
+   * the cells across the face with the velocities (u, v, w and tmp) negated, copies the
+   * boundary cell's p and sets chi = 0. */
+  for (int d = 0; d < 3; d++) {
+    if (s->bcf[d] == ORC_BC_PERIODIC) continue;
+    for (int side = 0; side < 2; side++) {
+      if (bc[d] != (side ? s->bpd[d] - 1 : 0)) continue;
+      for (int q = 1; q <= g; q++) {
+        const int dst = side ? BS - 1 + q : -q;       /* halo index along d */
+        const int bnd = side ? BS - 1 : 0;            /* the boundary cell */
+        const int mir = side ? 2 * BS - 1 - dst : -1 - dst;  /* its mirror image */
+        for (int a = 0; a < BS; a++)
+          for (int e = 0; e < BS; e++) {
+            int id[3], ib[3], im[3];
+            const int o1 = (d + 1) % 3, o2 = (d + 2) % 3;
+            id[d] = dst; ib[d] = bnd; im[d] = mir;
+            id[o1] = ib[o1] = im[o1] = a;
+            id[o2] = ib[o2] = im[o2] = e;
+            FE* D = &l->c[((id[2] + g) * L + (id[1] + g)) * L + (id[0] + g)];
+            const FE* B = &l->c[((ib[2] + g) * L + (ib[1] + g)) * L + (ib[0] + g)];
+            const FE* M = &l->c[((im[2] + g) * L + (im[1] + g)) * L + (im[0] + g)];
+            if (s->bcf[d] == ORC_BC_OPEN) {
+              *D = *B;
+            } else {
+              D->p = B->p;
+              D->chi = 0;
+              D->u = -M->u; D->tmpU = -M->tmpU;
+              D->v = -M->v; D->tmpV = -M->tmpV;
+              D->w = -M->w; D->tmpW = -M->tmpW;
+            }
+          }
+      }
+    }
+  }
 }
 
 static inline const FE* LAB(const Lab* l, int ix, int iy, int iz)
@@ -303,11 +350,12 @@ double orc_find_max_u(const orc_sim* s, const double uinf[3])
 }
 
 /* calcMaxTimestep (Simulation.cpp:403-424), uniform grid so hMin = h. */
-double orc_calc_max_timestep(const orc_sim* s, double CFL, double nu, const double uinf[3],
+double orc_calc_max_timestep(orc_sim* s, double CFL, double nu, const double uinf[3],
                              int step, int rampup, double* umax_out)
 {
   const double hMin = s->h;
   const double umax = orc_find_max_u(s, uinf);
+  s->umax_meas = umax;  /* sim.uMax_measured (Simulation.cpp:407) */
   if (umax_out) *umax_out = umax;
   const double dtDif = hMin * hMin / nu;
   const double dtAdv = hMin / (umax + 1e-8);
@@ -326,6 +374,92 @@ double orc_calc_max_timestep(const orc_sim* s, double CFL, double nu, const doub
 /* KernelAdvectDiffuse (AdvectionDiffusion.cpp:106-145) into tmpU/V/W, then the copy
  * tmp -> u of UpdateAndCorrectInflow_nonUniform::operate (:315-328); with every face
  * periodic the inflow sums are zero and the correction returns early (:363-365). */
+/* KernelAdvectDiffuseBase::applyBC* (AdvectionDiffusion.cpp:20-104): on an open domain face
+ * (neither periodic nor wall) the halo layer's velocity, edges included, is scaled by the
+ * face's fade 1 - fac (max or min of uinf_d over |uinf|_max)^2, fac = min(1, uMax_measured dt/h),
+ * when the fade is below 1 (inflow faces). */
+static void lab_fade(const orc_sim* s, int b, Lab* l, double dt, const double uinf[3])
+{
+  int bc[3];
+  bcoords(s, b, bc);
+  double fac = s->umax_meas * dt / s->h;
+  fac = fac < 1 ? fac : 1;
+  double nor = EPS;
+  for (int d = 0; d < 3; d++) nor = fabs(uinf[d]) > nor ? fabs(uinf[d]) : nor;
+  for (int d = 0; d < 3; d++) {
+    if (s->bcf[d] != ORC_BC_OPEN) continue;
+    for (int side = 0; side < 2; side++) {
+      if (bc[d] != (side ? s->bpd[d] - 1 : 0)) continue;
+      const double q = side ? (uinf[d] < 0 ? uinf[d] : 0) : (uinf[d] > 0 ? uinf[d] : 0);
+      const double fade = 1 - fac * pow(q / nor, 2);
+      if (fade >= 1) continue;
+      const int dst = side ? BS : -1;
+      for (int a = -1; a <= BS; a++)
+        for (int e = -1; e <= BS; e++) {
+          int id[3];
+          id[d] = dst; id[(d + 1) % 3] = a; id[(d + 2) % 3] = e;
+          FE* D = &l->c[((id[2] + 1) * l->L + (id[1] + 1)) * l->L + (id[0] + 1)];
+          D->u *= fade; D->v *= fade; D->w *= fade;
+        }
+    }
+  }
+}
+
+/* UpdateAndCorrectInflow::operate (AdvectionDiffusion.cpp:209-296) after the copy tmp -> u:
+ * the net outflow through the open faces (not wall, not periodic), divided by the cell count
+ * of all six faces, is removed from the face-normal velocity of their boundary cells. */
+static void inflow_correction(orc_sim* s)
+{
+  if (s->bcf[0] != ORC_BC_OPEN && s->bcf[1] != ORC_BC_OPEN && s->bcf[2] != ORC_BC_OPEN) return;
+  double* part = (double*)calloc((size_t)s->nb, sizeof(double));
+  #pragma omp parallel for schedule(static)
+  for (int b = 0; b < s->nb; b++) {
+    int bc[3];
+    bcoords(s, b, bc);
+    double sum = 0;
+    for (int d = 0; d < 3; d++) {
+      if (s->bcf[d] != ORC_BC_OPEN) continue;
+      for (int side = 0; side < 2; side++) {
+        if (bc[d] != (side ? s->bpd[d] - 1 : 0)) continue;
+        for (int a = 0; a < BS; a++)
+          for (int e = 0; e < BS; e++) {
+            int id[3];
+            id[d] = side ? BS - 1 : 0; id[(d + 1) % 3] = a; id[(d + 2) % 3] = e;
+            const FE* c = &s->blk[b].d[id[2]][id[1]][id[0]];
+            const double un = d == 0 ? c->u : (d == 1 ? c->v : c->w);
+            sum += side ? un : -un;
+          }
+      }
+    }
+    part[b] = sum;
+  }
+  double sumInflow = 0;
+  for (int b = 0; b < s->nb; b++) sumInflow += part[b];
+  free(part);
+  const double nx = s->N[0], ny = s->N[1], nz = s->N[2];
+  const double corr = sumInflow / (2 * (nx * ny + nx * nz + ny * nz));
+  if (fabs(corr) < EPS) return;
+  #pragma omp parallel for schedule(static)
+  for (int b = 0; b < s->nb; b++) {
+    int bc[3];
+    bcoords(s, b, bc);
+    for (int d = 0; d < 3; d++) {
+      if (s->bcf[d] != ORC_BC_OPEN) continue;
+      for (int side = 0; side < 2; side++) {
+        if (bc[d] != (side ? s->bpd[d] - 1 : 0)) continue;
+        for (int a = 0; a < BS; a++)
+          for (int e = 0; e < BS; e++) {
+            int id[3];
+            id[d] = side ? BS - 1 : 0; id[(d + 1) % 3] = a; id[(d + 2) % 3] = e;
+            FE* c = &s->blk[b].d[id[2]][id[1]][id[0]];
+            double* un = d == 0 ? &c->u : (d == 1 ? &c->v : &c->w);
+            *un += side ? -corr : corr;
+          }
+      }
+    }
+  }
+}
+
 void orc_advect_diffuse(orc_sim* s, double dt, double nu, const double uinf[3])
 {
   const double h = s->h;
@@ -337,6 +471,7 @@ void orc_advect_diffuse(orc_sim* s, double dt, double nu, const double uinf[3])
     #pragma omp for schedule(static)
     for (int b = 0; b < s->nb; b++) {
       lab_load(s, b, &l);
+      lab_fade(s, b, &l, dt, uinf);
       Block* o = &s->blk[b];
       for (int iz = 0; iz < BS; iz++)
         for (int iy = 0; iy < BS; iy++)
@@ -371,6 +506,7 @@ void orc_advect_diffuse(orc_sim* s, double dt, double nu, const double uinf[3])
       e[i].w = e[i].tmpW;
     }
   }
+  inflow_correction(s);
 }
 
 /* Third-order upwind-biased scheme with central diffusion (no reference counterpart: the
@@ -789,6 +925,119 @@ void orc_poisson_solve(orc_sim* s)
 }
 
 /* KernelGradP (PressureProjection.cpp:27-51). */
+/* PoissonSolverMixed::solve (PoissonSolverMixed.cpp:66-103, _solve PoissonSolverMixed.h:27-80)
+ * on the RHS buffer: real-to-real transforms per axis as FFTW's r2r plans make them (R2HC /
+ * HC2R on periodic axes, REDFT10 / REDFT01 on the others, PoissonSolverMixed.cpp:37-42), each
+ * applied here as a dense N x N matrix; every coefficient times
+ * norm / (denX + denY + denZ), den = (1 - tol)(cos(2 w i) - 1)/2 - tol ((k + s) w)^2 with
+ * w = 2 pi/N, s = 0, k = min(i, N - i) (periodic) or w = pi/N, s = 1/2, k = i; tol = 0.01,
+ * norm = (normX/h) normY normZ, norm = 1/N (periodic) or 1/(2N); the first coefficient zeroed.
+ * The result goes to p (_fftw2cub, PoissonSolver.cpp:116-130). No FFT: N^4 work per axis, a
+ * checker for small grids. */
+static double* r2r_matrix(int n, int periodic, int forward)
+{
+  double* m = (double*)malloc(sizeof(double) * (size_t)n * n);  /* m[k][j]: out_k = sum_j m[k][j] in_j */
+  for (int k = 0; k < n; k++)
+    for (int j = 0; j < n; j++) {
+      double v;
+      if (periodic && forward) {        /* R2HC, halfcomplex out: r_0..r_{n/2}, i_{(n+1)/2-1}..i_1 */
+        v = k <= n / 2 ? cos(2 * M_PI * (double)j * k / n) : -sin(2 * M_PI * (double)j * (n - k) / n);
+      } else if (periodic) {            /* HC2R: in_j is halfcomplex, out_k real */
+        if (j == 0) v = 1;
+        else if (2 * j == n) v = cos(M_PI * (double)k);
+        else if (j < n / 2 || (j <= n / 2 && n % 2)) v = 2 * cos(2 * M_PI * (double)j * k / n);
+        else v = -2 * sin(2 * M_PI * (double)(n - j) * k / n);
+      } else if (forward) {             /* REDFT10 */
+        v = 2 * cos(M_PI * (j + 0.5) * k / n);
+      } else {                          /* REDFT01 */
+        v = j == 0 ? 1 : 2 * cos(M_PI * (double)j * (k + 0.5) / n);
+      }
+      m[(size_t)k * n + j] = v;
+    }
+  return m;
+}
+
+/* a[x][y][z] (z fastest) along axis d by the matrix m. */
+static void r2r_axis(double* a, const int N[3], int d, const double* m)
+{
+  const int n = N[d];
+  const size_t sz = 1, sy = (size_t)N[2], sx = (size_t)N[1] * N[2];
+  const size_t stride = d == 0 ? sx : (d == 1 ? sy : sz);
+  const int o1 = d == 0 ? 1 : 0, o2 = d == 2 ? 1 : 2;  /* the other two axes */
+  const size_t s1 = o1 == 0 ? sx : sy, s2 = o2 == 1 ? sy : sz;
+  #pragma omp parallel
+  {
+    double* in = (double*)malloc(sizeof(double) * n);
+    #pragma omp for schedule(static)
+    for (int p = 0; p < N[o1] * N[o2]; p++) {
+      double* base = a + (size_t)(p / N[o2]) * s1 + (size_t)(p % N[o2]) * s2;
+      for (int j = 0; j < n; j++) in[j] = base[(size_t)j * stride];
+      for (int k = 0; k < n; k++) {
+        double acc = 0;
+        for (int j = 0; j < n; j++) acc += m[(size_t)k * n + j] * in[j];
+        base[(size_t)k * stride] = acc;
+      }
+    }
+    free(in);
+  }
+}
+
+void orc_poisson_mixed(orc_sim* s)
+{
+  const int* N = s->N;
+  const double tol = 0.01, h = s->h;
+  double* den[3];
+  double norm = 1 / h;
+  for (int d = 0; d < 3; d++) {
+    const int per = s->bcf[d] == ORC_BC_PERIODIC, n = N[d];
+    norm *= (per ? 1.0 : 0.5) / n;
+    const double wf = (per ? 2 : 1) * M_PI / n;
+    den[d] = (double*)malloc(sizeof(double) * n);
+    for (int i = 0; i < n; i++) {
+      const int k = per ? (i <= n / 2 ? i : n - i) : i;
+      const double rk2 = pow((k + (per ? 0 : 0.5)) * wf, 2);
+      den[d][i] = (1 - tol) * (cos(2 * wf * i) - 1) / 2 - tol * rk2;
+    }
+  }
+  for (int d = 0; d < 3; d++) {
+    double* m = r2r_matrix(N[d], s->bcf[d] == ORC_BC_PERIODIC, 1);
+    r2r_axis(s->rhs, N, d, m);
+    free(m);
+  }
+  #pragma omp parallel for schedule(static)
+  for (int i = 0; i < N[0]; i++)
+    for (int j = 0; j < N[1]; j++)
+      for (int k = 0; k < N[2]; k++) {
+        double* v = &s->rhs[((size_t)i * N[1] + j) * N[2] + k];
+        *v *= norm / (den[0][i] + den[1][j] + den[2][k]);
+      }
+  s->rhs[0] = 0;
+  for (int d = 0; d < 3; d++) {
+    double* m = r2r_matrix(N[d], s->bcf[d] == ORC_BC_PERIODIC, 0);
+    r2r_axis(s->rhs, N, d, m);
+    free(m);
+    free(den[d]);
+  }
+  #pragma omp parallel for schedule(static)
+  for (int b = 0; b < s->nb; b++) {
+    int bc[3];
+    bcoords(s, b, bc);
+    for (int iz = 0; iz < BS; iz++)
+      for (int iy = 0; iy < BS; iy++)
+        for (int ix = 0; ix < BS; ix++)
+          s->blk[b].d[iz][iy][ix].p = s->rhs[ridx(s, bc[0] * BS + ix, bc[1] * BS + iy, bc[2] * BS + iz)];
+  }
+}
+
+/* One r2r transform of a line (tests): kind 0 R2HC, 1 HC2R, 2 REDFT10, 3 REDFT01. */
+void orc_r2r_line(double* a, int n, int kind)
+{
+  double* m = r2r_matrix(n, kind < 2, kind % 2 == 0);
+  const int N[3] = {1, 1, n};
+  r2r_axis(a, N, 2, m);
+  free(m);
+}
+
 void orc_grad_p(orc_sim* s, double dt)
 {
   const double fac = -0.5 * dt / s->h;

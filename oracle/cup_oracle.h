@@ -40,6 +40,15 @@ void orc_set_plant(int code);
 /* Grid: bpd blocks of 8^3 per dimension, h = maxextent / max(cells)
  * (SimulationData.cpp:184-195). */
 orc_sim* orc_create(int bpdx, int bpdy, int bpdz, double maxextent);
+/* Boundary conditions per axis (SimulationData.cpp:123-166): periodic, open (the reference's
+ * dirichlet, absorbing and freespace labs) or wall; SimulationData::uMax_measured. */
+enum { ORC_BC_PERIODIC = 0, ORC_BC_OPEN = 1, ORC_BC_WALL = 2 };
+void orc_set_bc(orc_sim* s, const int bc[3]);
+void orc_set_umax_measured(orc_sim* s, double umax);
+/* PoissonSolverMixed's solve of the RHS buffer into p; one r2r line (0 R2HC, 1 HC2R,
+ * 2 REDFT10, 3 REDFT01). */
+void orc_poisson_mixed(orc_sim* s);
+void orc_r2r_line(double* a, int n, int kind);
 void orc_destroy(orc_sim* s);
 double orc_h(const orc_sim* s);
 void orc_extent(const orc_sim* s, double out[3]);
@@ -67,7 +76,7 @@ void orc_lab_shift(orc_sim* s, int g, int dx, int dy, int dz);
 
 /* Reductions / dt (ProcessHelpers.h:74-99, Simulation.cpp:403-424). */
 double orc_find_max_u(const orc_sim* s, const double uinf[3]);
-double orc_calc_max_timestep(const orc_sim* s, double CFL, double nu, const double uinf[3],
+double orc_calc_max_timestep(orc_sim* s, double CFL, double nu, const double uinf[3],
                              int step, int rampup, double* umax_out);
 
 /* Operators. */

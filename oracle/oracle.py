@@ -45,6 +45,10 @@ def lib():
         "orc_h": (d, [p]),
         "orc_extent": (None, [p, dp]),
         "orc_set_threads": (None, [i]),
+        "orc_set_bc": (None, [p, C.POINTER(C.c_int)]),
+        "orc_set_umax_measured": (None, [p, d]),
+        "orc_poisson_mixed": (None, [p]),
+        "orc_r2r_line": (None, [dp, i, i]),
         "orc_set_plant": (None, [i]),
         "orc_set_field": (None, [p, i, dp]),
         "orc_get_field": (None, [p, i, dp]),
@@ -137,6 +141,16 @@ def planted(name):
     return scope()
 
 
+BC = {"periodic": 0, "dirichlet": 1, "open": 1, "absorbing": 1, "freespace": 1, "wall": 2}
+
+
+def r2r_line(a, kind):
+    """One r2r transform as FFTW defines it: kind 'R2HC', 'HC2R', 'REDFT10' or 'REDFT01'."""
+    b = np.array(a, dtype=np.float64)
+    lib().orc_r2r_line(_dp(b), b.size, ("R2HC", "HC2R", "REDFT10", "REDFT01").index(kind))
+    return b
+
+
 def philox4x32_10(ctr, key):
     c = (C.c_uint32 * 4)(*ctr)
     k = (C.c_uint32 * 2)(*key)
@@ -202,13 +216,15 @@ def compute_velocities(M, implicit, forced, imposed, block_rot):
 class OracleSim:
     """FP64 CPU restatement of the reference's periodic block grid and operators."""
 
-    def __init__(self, bpd, maxextent: float = 2 * np.pi):
+    def __init__(self, bpd, maxextent: float = 2 * np.pi, bc=("periodic", "periodic", "periodic")):
         self.bpd = tuple(int(b) for b in bpd)
         self.cells = tuple(BS * b for b in self.bpd)
         self.shape = (self.cells[2], self.cells[1], self.cells[0])  # [z][y][x]
         self._h = lib().orc_create(self.bpd[0], self.bpd[1], self.bpd[2], float(maxextent))
         if not self._h:
             raise ValueError(f"invalid blocks per dimension {bpd}")
+        self.bc = tuple(bc)
+        lib().orc_set_bc(self._h, (C.c_int * 3)(*[BC[b] for b in self.bc]))
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -305,6 +321,13 @@ class OracleSim:
 
     def poisson_solve(self):
         lib().orc_poisson_solve(self._h)
+
+    def poisson_mixed(self):
+        """PoissonSolverMixed's solve (DFT on periodic axes, DCT on the others) of the RHS into p."""
+        lib().orc_poisson_mixed(self._h)
+
+    def set_umax_measured(self, umax):
+        lib().orc_set_umax_measured(self._h, float(umax))
 
     def grad_p(self, dt):
         lib().orc_grad_p(self._h, float(dt))
