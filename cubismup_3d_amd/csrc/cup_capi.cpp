@@ -57,7 +57,10 @@ static void release(Ctx& c)
     if (*b) { hipFree(*b); *b = nullptr; }
   for (auto& p : c.cos4)
     if (p) { hipFree(p); p = nullptr; }
-  for (auto& p : c.sin2)
+  for (auto& row : c.r2r)
+    for (auto& p : row)
+      if (p) { hipFree(p); p = nullptr; }
+  for (auto& p : c.mden)
     if (p) { hipFree(p); p = nullptr; }
   if (c.partials) hipFree(c.partials);
   if (c.ke_part) hipFree(c.ke_part);
@@ -149,10 +152,19 @@ int cup_create(const cup_grid_desc* d, cup_ctx** out)
   if (d->decomposition != CUP_PLAIN && d->decomposition != CUP_FUSED)
     return create_error(CUP_E_INVALID, "cup_create: decomposition must be CUP_PLAIN or CUP_FUSED");
   if (!(d->maxextent > 0)) return create_error(CUP_E_INVALID, "cup_create: maxextent must be > 0");
+  bool periodic = true;
+  for (int a = 0; a < 3; ++a) {
+    if (d->bc[a] != CUP_BC_PERIODIC && d->bc[a] != CUP_BC_OPEN && d->bc[a] != CUP_BC_WALL)
+      return create_error(CUP_E_INVALID, "cup_create: boundary conditions must be CUP_BC_PERIODIC, _OPEN or _WALL");
+    periodic = periodic && d->bc[a] == CUP_BC_PERIODIC;
+  }
+  if (!periodic && d->decomposition != CUP_PLAIN)
+    return create_error(CUP_E_INVALID, "cup_create: non-periodic boundaries take the CUP_PLAIN decomposition");
   cup_ctx* h = new (std::nothrow) cup_ctx();
   if (!h) return create_error(CUP_E_NOMEM, "cup_create: out of host memory");
   Ctx& c = h->c;
   c.dev = d->device;
+  for (int a = 0; a < 3; ++a) c.bc[a] = d->bc[a];
   c.precision = d->precision;
   c.esz = (size_t)d->precision;
   c.decomposition = d->decomposition;
@@ -318,6 +330,7 @@ int cup_get_field_device(cup_ctx* ctx, int field, void* dst)
 int cup_lab_shift(cup_ctx* ctx, int g, int dx, int dy, int dz)
 {
   CUP_GUARD(ctx);
+  if (!c.periodic()) return set_error(c, CUP_E_INVALID, "lab_shift: needs a fully periodic box");
   if (int rc = launch_lab_shift(c, g, dx, dy, dz)) return rc;
   return check_hip(c, hipStreamSynchronize(c.stream), "lab_shift");
 }
@@ -340,6 +353,7 @@ int cup_calc_max_timestep(cup_ctx* ctx, double CFL, double nu, const double uinf
   if (!(CFL > 0) || nu < 0) return set_error(c, CUP_E_INVALID, "calc_max_timestep: CFL must be > 0 and nu >= 0");
   double umax = 0;
   if (int rc = cup_max_velocity(ctx, uinf, &umax)) return rc;
+  c.umax_measured = umax;  // sim.uMax_measured (Simulation.cpp:407)
   const double hMin = c.h;
   const double dtDif = nu > 0 ? hMin * hMin / nu : INFINITY;
   const double dtAdv = hMin / (umax + 1e-8);
@@ -361,8 +375,11 @@ int cup_advect_diffuse(cup_ctx* ctx, double dt, double nu, const double uinf[3],
   if (!uinf || !(dt > 0)) return set_error(c, CUP_E_INVALID, "advect_diffuse: dt must be > 0");
   if (int rc = launch_set_dt(c, dt)) return rc;
   if (int rc = launch_advdiff(c, nu, uinf, scheme)) return rc;
-  if (c.decomposition == CUP_PLAIN)  // u = tmp = u*, as after the reference's copy
+  if (c.decomposition == CUP_PLAIN) {  // u = tmp = u*, as after the reference's copy
     if (int rc = c.ustar == CUP_U ? launch_copy_u_to_tmp(c) : launch_copy_tmp_to_u(c)) return rc;
+    if (!c.periodic())
+      if (int rc = launch_inflow_correction(c)) return rc;  // on u, AdvectionDiffusion.cpp:209-296
+  }
   return CUP_OK;
 }
 
@@ -392,11 +409,21 @@ int cup_pressure_rhs(cup_ctx* ctx, double dt)
 int cup_poisson_periodic(cup_ctx* ctx)
 {
   CUP_GUARD(ctx);
+  if (!c.periodic()) return set_error(c, CUP_E_INVALID, "poisson_periodic: non-periodic boundaries take cup_poisson_mixed");
   if (c.decomposition == CUP_PLAIN) {
     if (int rc = launch_poisson_plain(c)) return rc;
     return launch_copy(c, c.f[CUP_P], c.rbuf, c.ncell);  // PoissonSolver::_fftw2cub
   }
   return launch_poisson_fused(c);
+}
+
+int cup_poisson_mixed(cup_ctx* ctx)
+{
+  CUP_GUARD(ctx);
+  if (c.decomposition != CUP_PLAIN)
+    return set_error(c, CUP_E_STATE, "poisson_mixed: takes the CUP_PLAIN decomposition's RHS buffer");
+  if (int rc = launch_poisson_mixed(c)) return rc;
+  return launch_copy(c, c.f[CUP_P], c.rbuf, c.ncell);  // PoissonSolver::_fftw2cub
 }
 
 int cup_pressure_project(cup_ctx* ctx, double dt)
@@ -424,6 +451,7 @@ int cup_diagnostics(cup_ctx* ctx, double nu, double out[20])
 int cup_hit_analysis(cup_ctx* ctx, double nu, double out[9], double* spectrum, int nbin)
 {
   CUP_GUARD(ctx);
+  if (!c.periodic()) return set_error(c, CUP_E_INVALID, "hit_analysis: needs a fully periodic box");
   const int maxN = std::max(c.N[0], std::max(c.N[1], c.N[2]));
   const int nb = maxN / 2 - 1;
   if (!out || !(nu > 0)) return set_error(c, CUP_E_INVALID, "hit_analysis: bad argument");
@@ -462,6 +490,7 @@ int cup_ic_taylor_green(cup_ctx* ctx, double uMax)
 int cup_ic_hit(cup_ctx* ctx, double k0, double u_rms, uint64_t seed)
 {
   CUP_GUARD(ctx);
+  if (!c.periodic()) return set_error(c, CUP_E_INVALID, "ic_hit: needs a fully periodic box");
   if (!(k0 > 0) || !(u_rms > 0)) return set_error(c, CUP_E_INVALID, "ic_hit: k0 and u_rms must be > 0");
   if (int rc = launch_ic_hit(c, k0, nullptr, nullptr, 0, u_rms, seed)) return rc;
   return check_hip(c, hipStreamSynchronize(c.stream), "ic_hit");
@@ -470,6 +499,7 @@ int cup_ic_hit(cup_ctx* ctx, double k0, double u_rms, uint64_t seed)
 int cup_ic_hit_spectrum(cup_ctx* ctx, const double* K, const double* E, int n, double u_rms, uint64_t seed)
 {
   CUP_GUARD(ctx);
+  if (!c.periodic()) return set_error(c, CUP_E_INVALID, "ic_hit_spectrum: needs a fully periodic box");
   if (!K || !E || n < 2) return set_error(c, CUP_E_INVALID, "ic_hit_spectrum: the table needs two entries or more");
   for (int i = 1; i < n; ++i)
     if (!(K[i] > K[i - 1])) return set_error(c, CUP_E_INVALID, "ic_hit_spectrum: K must increase");
@@ -481,6 +511,7 @@ int cup_spectral_forcing(cup_ctx* ctx, double dt, double sim_dt, double nu, doub
                          double injection_rate, double stats_out[5])
 {
   CUP_GUARD(ctx);
+  if (!c.periodic()) return set_error(c, CUP_E_INVALID, "spectral_forcing: needs a fully periodic box");
   if (!(dt > 0)) return set_error(c, CUP_E_INVALID, "spectral_forcing: dt must be > 0");
   double st[5];
   if (int rc = forcing_plain(c, dt, sim_dt, nu, tke_target, injection_rate, st)) return rc;
@@ -740,7 +771,7 @@ static int plain_step(cup_ctx* ctx, const cup_step_params* p)
   if (p->spectral_forcing)
     if (int rc = cup_spectral_forcing(ctx, dt, dt, p->nu, p->tke_target, p->injection_rate, nullptr)) return rc;
   if (int rc = cup_pressure_rhs(ctx, dt)) return rc;
-  if (int rc = cup_poisson_periodic(ctx)) return rc;
+  if (int rc = c.periodic() ? cup_poisson_periodic(ctx) : cup_poisson_mixed(ctx)) return rc;
   if (int rc = cup_pressure_project(ctx, dt)) return rc;
   if (p->has_body && c.body.present) {
     if (int rc = cup_penalize(ctx, dt, p->lambda, p->implicit_penalization, p->body_vel, p->body_angvel, nullptr))
@@ -897,6 +928,7 @@ int cup_adjoint_step_forcing(cup_ctx* ctx, double dt, double nu, const double ui
                              double tke_target, double injection_rate)
 {
   CUP_GUARD(ctx);
+  if (!c.periodic()) return set_error(c, CUP_E_INVALID, "adjoint_step: needs a fully periodic box");
   if (!uinf || !(dt > 0)) return set_error(c, CUP_E_INVALID, "adjoint_step: dt must be > 0");
   if (tke_target < 0 || injection_rate < 0)
     return set_error(c, CUP_E_INVALID, "adjoint_step: tke_target and injection_rate must be >= 0");
@@ -965,6 +997,7 @@ int cup_adjoint_step_body(cup_ctx* ctx, double dt, double nu, const double uinf[
                           const double body_vel[3], double wdrag, double dJdV[3], double dJdc[3])
 {
   CUP_GUARD(ctx);
+  if (!c.periodic()) return set_error(c, CUP_E_INVALID, "adjoint_step_body: needs a fully periodic box");
   if (!uinf || !body_vel || !dJdV || !(dt > 0)) return set_error(c, CUP_E_INVALID, "adjoint_step_body: bad argument");
   if (!c.body.present) return set_error(c, CUP_E_STATE, "adjoint_step_body: no obstacle");
   if (body_count(c) > 1) return set_error(c, CUP_E_STATE, "adjoint_step_body: one body only");

@@ -645,9 +645,7 @@ template <typename T, int N, int AXIS, int MODE, bool FORCE = false>
 __global__ __launch_bounds__(NT) void k_fft_colr(C2<T>* __restrict__ S, const C2<T>* __restrict__ tw, int ncols,
                                                  int Ny, int Nz, double norm, const double* __restrict__ cx,
                                                  const double* __restrict__ cy, const double* __restrict__ cz,
-                                                 const LowModes* __restrict__ lmp, const double* __restrict__ coef,
-                                                 const float* __restrict__ sx, const float* __restrict__ sy,
-                                                 const float* __restrict__ sz)
+                                                 const LowModes* __restrict__ lmp, const double* __restrict__ coef)
 {
   using CR = ColReg<T, N>;
   constexpr int TK = CR::TK, NTH = CR::NTH, RPT = CR::RPT;
@@ -701,29 +699,11 @@ __global__ __launch_bounds__(NT) void k_fft_colr(C2<T>* __restrict__ S, const C2
     return;
   }
   // ZM_POISSON (z): PoissonSolverPeriodic::_solve's grid-consistent branch
-  // (PoissonSolverPeriodic.cpp:23-61), multiply by norm/(2D-6), zero the modes with every
-  // index in {0, N/2}; then the inverse transform. FP64 contexts form the factor as the
-  // reference writes it, in FP64 from the cosine tables. FP32 contexts use the same value as
-  // -norm/(4 S), S = sum_i sin^2(2 pi k_i/N_i) from FP32 tables (2 cos(2a) - 2 = -4 sin^2 a):
-  // a sum of non-negative terms, so it keeps FP32's relative precision down to k = 1, where
-  // 2D - 6 in FP32 would cancel.
-  if constexpr (sizeof(T) == 4) {
-    const int kx = kx0 + f, ky = other;
-    const float sxy = live ? sx[kx] + sy[ky] : 1.0f;
-    const bool zxy = kx == 0 && (ky == 0 || 2 * ky == Ny);
-    const float nq = (float)(-0.25 * norm);
-#pragma unroll
-    for (int i = 0; i < RPT; ++i)
-#pragma unroll
-      for (int k2 = 0; k2 < NTH; ++k2) {
-        const int kz = j * RPT + i + 16 * k2;
-        const bool zero = zxy && (kz == 0 || 2 * kz == N);
-        const float fac = zero ? 0.0f : nq * __builtin_amdgcn_rcpf(sxy + sz[kz]);
-        C2<T>& q = v[i * NTH + k2];
-        q.x *= fac;
-        q.y *= fac;
-      }
-  } else {
+  // (PoissonSolverPeriodic.cpp:23-61), multiply by norm/(2D-6) in FP64 (also for FP32 fields:
+  // an FP32 factor, even formed without cancellation as -norm/(4 sum sin^2), adds rounding to
+  // the lowest modes that long FP32 trajectories and their adjoints amplify), zero the modes
+  // with every index in {0, N/2}; then the inverse transform.
+  {
     const int kx = kx0 + f, ky = other;
     const double dxy = live ? cx[kx] + cy[ky] : 0.0;
     const bool zxy = kx == 0 && (ky == 0 || 2 * ky == Ny);
@@ -817,17 +797,17 @@ struct ColPass {
       const C2<T>* tw = (const C2<T>*)c.tw[AXIS];
       if (mode == ZM_FWD)
         hipLaunchKernelGGL((k_fft_colr<T, N, AXIS, ZM_FWD>), grid, dim3(NT), 0, c.stream, spec, tw, ncols, c.N[1],
-                           c.N[2], norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low_dev, (const double*)nullptr, c.sin2[0], c.sin2[1], c.sin2[2]);
+                           c.N[2], norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low_dev, (const double*)nullptr);
       else if (mode == ZM_INV)
         hipLaunchKernelGGL((k_fft_colr<T, N, AXIS, ZM_INV>), grid, dim3(NT), 0, c.stream, spec, tw, ncols, c.N[1],
-                           c.N[2], norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low_dev, (const double*)nullptr, c.sin2[0], c.sin2[1], c.sin2[2]);
+                           c.N[2], norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low_dev, (const double*)nullptr);
       else if (AXIS == 2 && force)
         hipLaunchKernelGGL((k_fft_colr<T, N, 2, ZM_POISSON, true>), grid, dim3(NT), 0, c.stream, spec, tw, ncols,
                            c.N[1], c.N[2], norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low_dev,
-                           (const double*)c.low_coef, c.sin2[0], c.sin2[1], c.sin2[2]);
+                           (const double*)c.low_coef);
       else if (AXIS == 2)
         hipLaunchKernelGGL((k_fft_colr<T, N, 2, ZM_POISSON, false>), grid, dim3(NT), 0, c.stream, spec, tw, ncols,
-                           c.N[1], c.N[2], norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low_dev, (const double*)nullptr, c.sin2[0], c.sin2[1], c.sin2[2]);
+                           c.N[1], c.N[2], norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low_dev, (const double*)nullptr);
       return;
     }
     constexpr int TK = ColTile<T, N>::TK;
@@ -1005,6 +985,117 @@ int launch_fft_inverse_field(Ctx& c, void* spec, void* real_blocks)
   return c.precision == 4 ? inv_field_t<float>(c, spec, real_blocks) : inv_field_t<double>(c, spec, real_blocks);
 }
 
+// ------------------------------------------------------- PoissonSolverMixed
+// PoissonSolverMixed::solve (PoissonSolverMixed.cpp:66-103, _solve PoissonSolverMixed.h:27-80):
+// FFTW r2r transforms per axis, R2HC / HC2R on periodic axes and REDFT10 / REDFT01 (DCT-II /
+// DCT-III) on the others (PoissonSolverMixed.cpp:37-42), here as dense N x N matrices applied
+// to each line in LDS (N^2 work per line: the path for non-periodic boxes, not a benchmark
+// path); the coefficient (i, j, k) times norm / (denX_i + denY_j + denZ_k) in FP64, the first
+// one zeroed. Works on the RHS buffer in block layout.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_r2r_axis(T* __restrict__ a, const T* __restrict__ mt, GridArgs g, int axis)
+{
+  extern __shared__ __align__(16) unsigned char r2r_lds[];
+  T* line = reinterpret_cast<T*>(r2r_lds);
+  const int n = axis == 0 ? g.Nx : (axis == 1 ? g.Ny : g.Nz);
+  const int n1 = axis == 0 ? g.Ny : g.Nx;  // the first of the other two axes
+  const int p = blockIdx.x, o1 = p % n1, o2 = p / n1;
+  auto at = [&](int k) -> size_t {
+    return axis == 0 ? cell_index(g, k, o1, o2) : (axis == 1 ? cell_index(g, o1, k, o2) : cell_index(g, o1, o2, k));
+  };
+  for (int j = threadIdx.x; j < n; j += NT) line[j] = a[at(j)];
+  __syncthreads();
+  for (int k = threadIdx.x; k < n; k += NT) {
+    T acc = 0;
+    for (int j = 0; j < n; ++j) acc += mt[(size_t)j * n + k] * line[j];
+    a[at(k)] = acc;
+  }
+}
+
+template <typename T>
+__global__ void k_mixed_divide(T* __restrict__ a, GridArgs g, double norm, const double* __restrict__ dx,
+                               const double* __restrict__ dy, const double* __restrict__ dz)
+{
+  const size_t n = (size_t)g.nb * BS3;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const int b = (int)(i / BS3), c = (int)(i % BS3);
+    const int x = (b % g.nbx) * BS + (c & 7), y = ((b / g.nbx) % g.nby) * BS + ((c >> 3) & 7),
+              z = (b / (g.nbx * g.nby)) * BS + (c >> 6);
+    if (x == 0 && y == 0 && z == 0) a[i] = 0;
+    else a[i] = (T)((double)a[i] * (norm / (dx[x] + dy[y] + dz[z])));
+  }
+}
+
+template <typename T>
+static int mixed_setup(Ctx& c)
+{
+  const double tol = 0.01;  // PoissonSolverMixed.h:50
+  for (int d = 0; d < 3; ++d) {
+    if (c.mden[d]) continue;
+    const int n = c.N[d];
+    const bool per = c.bc[d] == CUP_BC_PERIODIC;
+    std::vector<T> fw((size_t)n * n), bw((size_t)n * n);  // transposed: m[j][k] = M[k][j]
+    for (int k = 0; k < n; ++k)
+      for (int j = 0; j < n; ++j) {
+        double f, b;
+        if (per) {
+          // R2HC (halfcomplex out: r_0..r_{n/2}, then i_{n/2-1}..i_1) and HC2R
+          f = k <= n / 2 ? cos(2 * M_PI * (double)j * k / n) : -sin(2 * M_PI * (double)j * (n - k) / n);
+          if (j == 0) b = 1;
+          else if (2 * j == n) b = cos(M_PI * (double)k);
+          else if (j < n / 2) b = 2 * cos(2 * M_PI * (double)j * k / n);
+          else b = -2 * sin(2 * M_PI * (double)(n - j) * k / n);
+        } else {
+          f = 2 * cos(M_PI * (j + 0.5) * k / n);                      // REDFT10
+          b = j == 0 ? 1 : 2 * cos(M_PI * (double)j * (k + 0.5) / n);  // REDFT01
+        }
+        fw[(size_t)j * n + k] = (T)f;
+        bw[(size_t)j * n + k] = (T)b;
+      }
+    std::vector<double> den((size_t)n);
+    const double wf = (per ? 2 : 1) * M_PI / n;
+    for (int i = 0; i < n; ++i) {
+      const int kk = per ? (i <= n / 2 ? i : n - i) : i;
+      const double rk = (kk + (per ? 0.0 : 0.5)) * wf;
+      den[i] = (1 - tol) * (cos(2 * wf * i) - 1) / 2 - tol * rk * rk;
+    }
+    for (int q = 0; q < 2; ++q) {
+      if (check_hip(c, hipMalloc(&c.r2r[d][q], fw.size() * sizeof(T)), "hipMalloc r2r")) return CUP_E_NOMEM;
+      hipMemcpy(c.r2r[d][q], q ? bw.data() : fw.data(), fw.size() * sizeof(T), hipMemcpyHostToDevice);
+    }
+    if (check_hip(c, hipMalloc(&c.mden[d], den.size() * sizeof(double)), "hipMalloc mixed symbol")) return CUP_E_NOMEM;
+    hipMemcpy(c.mden[d], den.data(), den.size() * sizeof(double), hipMemcpyHostToDevice);
+  }
+  return CUP_OK;
+}
+
+template <typename T>
+static int poisson_mixed_t(Ctx& c)
+{
+  if (int rc = mixed_setup<T>(c)) return rc;
+  T* a = (T*)c.rbuf;
+  const GridArgs g = c.grid();
+  auto axis_pass = [&](int d, int q) {
+    const int n = c.N[d], lines = (int)(c.ncell / (size_t)n);
+    hipLaunchKernelGGL((k_r2r_axis<T>), dim3(lines), dim3(NT), (size_t)n * sizeof(T), c.stream, a,
+                       (const T*)c.r2r[d][q], g, d);
+  };
+  for (int d = 0; d < 3; ++d) axis_pass(d, 0);
+  double norm = 1 / c.h;
+  for (int d = 0; d < 3; ++d) norm *= (c.bc[d] == CUP_BC_PERIODIC ? 1.0 : 0.5) / c.N[d];
+  size_t gsz = (c.ncell + NT - 1) / NT;
+  if (gsz > 65536) gsz = 65536;
+  hipLaunchKernelGGL((k_mixed_divide<T>), dim3((unsigned)gsz), dim3(NT), 0, c.stream, a, g, norm,
+                     (const double*)c.mden[0], (const double*)c.mden[1], (const double*)c.mden[2]);
+  for (int d = 0; d < 3; ++d) axis_pass(d, 1);
+  return check_hip(c, hipGetLastError(), "poisson_mixed");
+}
+
+int launch_poisson_mixed(Ctx& c)
+{
+  return c.precision == 4 ? poisson_mixed_t<float>(c) : poisson_mixed_t<double>(c);
+}
+
 // Twiddle tables e^{-2 pi i t/N} (t < N) per axis and the Poisson cosine tables, built in
 // FP64 on the host.
 int fft_setup(Ctx& c)
@@ -1027,13 +1118,11 @@ int fft_setup(Ctx& c)
       return set_error(c, CUP_E_INVALID,
                        "FFT sizes must be between 8 and 1024 cells per axis with no prime factor above 5");
     std::vector<double> tw(2 * (size_t)n), cs((size_t)n);
-    std::vector<float> s2((size_t)n);
     for (int t = 0; t < n; ++t) {
       const double ang = 2.0 * M_PI * (double)t / (double)n;
       tw[2 * t] = cos(ang);
       tw[2 * t + 1] = -sin(ang);
       cs[t] = cos(4.0 * M_PI * (double)t / (double)n);
-      s2[t] = (float)(sin(ang) * sin(ang));
     }
     if (check_hip(c, hipMalloc(&c.tw[a], (size_t)n * 2 * c.esz), "hipMalloc twiddles")) return CUP_E_DEVICE;
     if (c.precision == 4) {
@@ -1045,8 +1134,6 @@ int fft_setup(Ctx& c)
     }
     if (check_hip(c, hipMalloc(&c.cos4[a], (size_t)n * sizeof(double)), "hipMalloc cos4")) return CUP_E_DEVICE;
     hipMemcpy(c.cos4[a], cs.data(), cs.size() * sizeof(double), hipMemcpyHostToDevice);
-    if (check_hip(c, hipMalloc(&c.sin2[a], (size_t)n * sizeof(float)), "hipMalloc sin2")) return CUP_E_DEVICE;
-    hipMemcpy(c.sin2[a], s2.data(), s2.size() * sizeof(float), hipMemcpyHostToDevice);
   }
   return CUP_OK;
 }

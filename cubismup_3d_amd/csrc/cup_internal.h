@@ -80,7 +80,14 @@ struct GridArgs {
   int Nx, Ny, Nz;     // cells per dimension
   int nb;             // blocks
   double h;
+  int bc[3];          // per axis: CUP_BC_PERIODIC, CUP_BC_OPEN, CUP_BC_WALL
+  int bcany;          // some axis is not periodic
 };
+
+// How a halo value is formed at a wall face (BlockLabBC::applyBCfaceWall,
+// Definitions.h:302-334): velocities (and the SDF the obstacle build keeps in tmpU) are the
+// negated mirror image, chi is zero, everything else (p) copies. Open faces copy every field.
+enum HaloKind { HK_COPY = 0, HK_VEL = 1, HK_CHI = 2 };
 
 // A sphere turned into chi and surface data (static body: built once).
 struct Body {
@@ -118,6 +125,10 @@ struct Ctx {
                           // F_ADJ2..+2: the upwind midpoint adjoint's scratch (lazy)
   int ustar = CUP_TMPU;   // first of the three fields holding the step's u* after advection
   int umax_bank = 0;      // the bank of DevScalars::umax_slots that the max |u| writers fill
+  int bc[3] = {CUP_BC_PERIODIC, CUP_BC_PERIODIC, CUP_BC_PERIODIC};
+  double umax_measured = 0;  // SimulationData::uMax_measured of the last cup_calc_max_timestep
+  double host_dt = 0;        // dt of the last host-driven operator call (the fades' fac)
+  bool periodic() const { return bc[0] == CUP_BC_PERIODIC && bc[1] == CUP_BC_PERIODIC && bc[2] == CUP_BC_PERIODIC; }
   bool plain_rhs = false; // the fused x pass ignores the obstacle RHS (the adjoint's projection)
   void* spec = nullptr;   // packed half spectrum, complex, [z][y][kx], kx < Nx/2
   double* ana = nullptr;  // HIT analysis: per-line sums [3 + nbin][Ny*Nz], then the row totals (lazy)
@@ -126,8 +137,10 @@ struct Ctx {
   void* rbuf = nullptr;   // block-layout real scratch (plain RHS / solution)
   void* tw[3] = {};       // FFT twiddles per axis, e^{-2 pi i t / N}, context precision
   double* cos4[3] = {};   // cos(4 pi k / N) per axis (FP64), the Poisson symbol
-  float* sin2[3] = {};    // sin^2(2 pi k / N) per axis (FP32): 2 cos(4 pi k/N) - 2 = -4 sin^2(2 pi k/N)
   FftPlan plan[3];        // per axis, for the lengths that are not powers of two
+  void* r2r[3][2] = {};   // PoissonSolverMixed: per axis the forward and backward r2r matrix,
+                          // transposed (m[j][k] = M[k][j]), context precision (lazy)
+  double* mden[3] = {};   // PoissonSolverMixed's per-axis symbol terms (lazy)
   double* partials = nullptr;
   size_t partials_cap = 0;  // doubles
   double* ke_part = nullptr;  // [nb][4] partial sums |u|^2 of the fused step's velocity (per wave)
@@ -163,6 +176,11 @@ struct Ctx {
     g.nbx = bpd[0]; g.nby = bpd[1]; g.nbz = bpd[2];
     g.Nx = N[0]; g.Ny = N[1]; g.Nz = N[2];
     g.nb = nb; g.h = h;
+    g.bcany = 0;
+    for (int a = 0; a < 3; ++a) {
+      g.bc[a] = bc[a];
+      g.bcany |= bc[a] != CUP_BC_PERIODIC;
+    }
     return g;
   }
 };
@@ -183,6 +201,52 @@ __device__ __forceinline__ size_t cell_index(const GridArgs& g, int gx, int gy, 
 {
   const int b = (gx >> 3) + g.nbx * ((gy >> 3) + g.nby * (gz >> 3));
   return (size_t)b * BS3 + (((gz & 7) * BS + (gy & 7)) * BS + (gx & 7));
+}
+
+// The boundary condition of a block face (0..5 = x-, x+, y-, y+, z-, z+) when it lies on a
+// non-periodic domain face, else 0 (the halo comes from the periodic neighbour block).
+__device__ __forceinline__ int domain_face(const GridArgs& g, int bx, int by, int bz, int face)
+{
+  const int a = face >> 1;
+  if (g.bc[a] == CUP_BC_PERIODIC) return 0;
+  const int bi = a == 0 ? bx : (a == 1 ? by : bz), nbk = a == 0 ? g.nbx : (a == 1 ? g.nby : g.nbz);
+  return ((face & 1) ? bi == nbk - 1 : bi == 0) ? g.bc[a] : 0;
+}
+
+// A halo value from its source cell at a domain face of type bcf (0: not a domain face).
+template <typename T>
+__device__ __forceinline__ T halo_value(T v, int kind, int bcf)
+{
+  if (bcf != CUP_BC_WALL) return v;
+  return kind == HK_VEL ? -v : (kind == HK_CHI ? (T)0 : v);
+}
+
+// The neighbour of global cell (gx, gy, gz) one cell along axis a in direction dir = +-1
+// through the grid's boundaries (BlockLabBC, Definitions.h:261-378): the periodic image, or
+// at a non-periodic domain face the boundary cell itself (open faces copy it; a wall mirrors
+// across the face, and one cell out the mirror image is the boundary cell). *bcf is the face
+// type crossed (0 inside the domain or across a periodic face).
+__device__ __forceinline__ size_t nb_cell(const GridArgs& g, int gx, int gy, int gz, int a, int dir, int* bcf)
+{
+  int c[3] = {gx, gy, gz};
+  const int n = a == 0 ? g.Nx : (a == 1 ? g.Ny : g.Nz);
+  int v = c[a] + dir, f = 0;
+  if (v < 0 || v >= n) {
+    if (g.bc[a] == CUP_BC_PERIODIC) v = v < 0 ? v + n : v - n;
+    else { f = g.bc[a]; v = c[a]; }
+  }
+  c[a] = v;
+  *bcf = f;
+  return cell_index(g, c[0], c[1], c[2]);
+}
+
+template <typename T>
+__device__ __forceinline__ T nb_value(const T* __restrict__ f, const GridArgs& g, int gx, int gy, int gz, int a,
+                                      int dir, int kind)
+{
+  int bcf;
+  const size_t i = nb_cell(g, gx, gy, gz, a, dir, &bcf);
+  return halo_value(f[i], kind, bcf);
 }
 
 template <typename T>
@@ -248,6 +312,8 @@ int launch_reduce(Ctx& c, int n, int width, unsigned maxmask, double* out_dev, c
 // Records the message of a failed call that has no context (read by cup_last_error(NULL)).
 int free_error(int code, const std::string& msg);
 int launch_reset_umax(Ctx& c);  // clears both banks
+int launch_inflow_correction(Ctx& c);  // UpdateAndCorrectInflow on the open faces of u
+int launch_poisson_mixed(Ctx& c);      // PoissonSolverMixed: rbuf (RHS) -> rbuf (solution)
 // Device address of bank k of the max |u| slots (host pointer arithmetic, no access).
 inline unsigned long long* umax_slots_dev(const Ctx& c, int k) { return &c.sc->umax_slots[k][0]; }
 int launch_finalize_step(Ctx& c, double CFL, double nu, int rampup, double fixed_dt, int advance);

@@ -64,14 +64,27 @@ __global__ void k_sphere_sdf(const int* __restrict__ ob_ids, double* __restrict_
 
 // SDF as the reference's grid tmpU after CreateObstacles' reset (-1) and the block fill
 // (max with the body's SDF): the body's SDF on obstacle blocks, -1 elsewhere.
+// Outside a non-periodic domain face the lab holds the boundary cell's value, negated at a
+// wall (the reference keeps the SDF in tmpU, which BlockLabBC mirrors with its sign flipped).
 __device__ __forceinline__ double sdf_at(const GridArgs& g, const int* __restrict__ slot, const double* __restrict__ sdf,
                                          int gx, int gy, int gz)
 {
+  double sign = 1.0;
+  if (g.bcany) {
+    int c[3] = {gx, gy, gz};
+    const int n[3] = {g.Nx, g.Ny, g.Nz};
+    for (int a = 0; a < 3; ++a)
+      if (g.bc[a] != CUP_BC_PERIODIC && (c[a] < 0 || c[a] >= n[a])) {
+        c[a] = c[a] < 0 ? 0 : n[a] - 1;
+        if (g.bc[a] == CUP_BC_WALL) sign = -sign;
+      }
+    gx = c[0]; gy = c[1]; gz = c[2];
+  }
   gx = wrapi(gx, g.Nx); gy = wrapi(gy, g.Ny); gz = wrapi(gz, g.Nz);
   const int b = (gx >> 3) + g.nbx * ((gy >> 3) + g.nby * (gz >> 3));
   const int s = slot[b];
-  if (s < 0) return -1.0;
-  return sdf[(size_t)s * BS3 + (((gz & 7) * BS + (gy & 7)) * BS + (gx & 7))];
+  if (s < 0) return -sign;
+  return sign * sdf[(size_t)s * BS3 + (((gz & 7) * BS + (gy & 7)) * BS + (gx & 7))];
 }
 
 // Mollified Heaviside of the signed distance d with half-width e (no reference counterpart):
@@ -364,17 +377,17 @@ __global__ __launch_bounds__(NT) void k_rhs_obs_mask(const int* __restrict__ ob_
     if (rhs) {
       r = rhs[o];
     } else {
-      r = fac * (u[cell_index(g, wrapi(gx + 1, g.Nx), gy, gz)] - u[cell_index(g, wrapi(gx - 1, g.Nx), gy, gz)] +
-                 v[cell_index(g, gx, wrapi(gy + 1, g.Ny), gz)] - v[cell_index(g, gx, wrapi(gy - 1, g.Ny), gz)] +
-                 w[cell_index(g, gx, gy, wrapi(gz + 1, g.Nz))] - w[cell_index(g, gx, gy, wrapi(gz - 1, g.Nz))]);
+      r = fac * (nb_value(u, g, gx, gy, gz, 0, 1, HK_VEL) - nb_value(u, g, gx, gy, gz, 0, -1, HK_VEL) +
+                 nb_value(v, g, gx, gy, gz, 1, 1, HK_VEL) - nb_value(v, g, gx, gy, gz, 1, -1, HK_VEL) +
+                 nb_value(w, g, gx, gy, gz, 2, 1, HK_VEL) - nb_value(w, g, gx, gy, gz, 2, -1, HK_VEL));
     }
     const double X = ob_chi[(size_t)s * BS3 + c];
     if (r0_out) r0_out[(size_t)s * BS3 + c] = r;
     unsigned char t = 0;
     if (!(chi[o] > (T)X)) {  // in the grid's precision: the grid chi holds the body's chi rounded to T
-      const double dXx = (double)chi[cell_index(g, wrapi(gx + 1, g.Nx), gy, gz)] - chi[cell_index(g, wrapi(gx - 1, g.Nx), gy, gz)];
-      const double dXy = (double)chi[cell_index(g, gx, wrapi(gy + 1, g.Ny), gz)] - chi[cell_index(g, gx, wrapi(gy - 1, g.Ny), gz)];
-      const double dXz = (double)chi[cell_index(g, gx, gy, wrapi(gz + 1, g.Nz))] - chi[cell_index(g, gx, gy, wrapi(gz - 1, g.Nz))];
+      const double dXx = (double)nb_value(chi, g, gx, gy, gz, 0, 1, HK_CHI) - nb_value(chi, g, gx, gy, gz, 0, -1, HK_CHI);
+      const double dXy = (double)nb_value(chi, g, gx, gy, gz, 1, 1, HK_CHI) - nb_value(chi, g, gx, gy, gz, 1, -1, HK_CHI);
+      const double dXz = (double)nb_value(chi, g, gx, gy, gz, 2, 1, HK_CHI) - nb_value(chi, g, gx, gy, gz, 2, -1, HK_CHI);
       const bool surf = dXx * dXx + dXy * dXy + dXz * dXz > 0;
       t = surf ? 1 : 0;
       const double srcBulk = -X * (double)r;
@@ -1046,16 +1059,19 @@ __global__ __launch_bounds__(NT) void k_surface_forces(const int* __restrict__ o
     if (!(sd[0] > DEPS)) continue;
     const int ix = c & 7, iy = (c >> 3) & 7, iz = c >> 6;
     const int gx = bx * BS + ix, gy = by * BS + iy, gz = bz * BS + iz;
-    const size_t E = cell_index(g, wrapi(gx + 1, g.Nx), gy, gz), W = cell_index(g, wrapi(gx - 1, g.Nx), gy, gz);
-    const size_t N = cell_index(g, gx, wrapi(gy + 1, g.Ny), gz), S = cell_index(g, gx, wrapi(gy - 1, g.Ny), gz);
-    const size_t Bk = cell_index(g, gx, gy, wrapi(gz + 1, g.Nz)), F = cell_index(g, gx, gy, wrapi(gz - 1, g.Nz));
+    // neighbours through the grid's boundaries (a wall face negates the velocities)
+    int fE, fW, fN, fS, fB, fF;
+    const size_t E = nb_cell(g, gx, gy, gz, 0, 1, &fE), W = nb_cell(g, gx, gy, gz, 0, -1, &fW);
+    const size_t N = nb_cell(g, gx, gy, gz, 1, 1, &fN), S = nb_cell(g, gx, gy, gz, 1, -1, &fS);
+    const size_t Bk = nb_cell(g, gx, gy, gz, 2, 1, &fB), F = nb_cell(g, gx, gy, gz, 2, -1, &fF);
     const size_t C = (size_t)b * BS3 + c;
-    const double D11 = _1oH * ((double)u[E] - u[W]);
-    const double D22 = _1oH * ((double)v[N] - v[S]);
-    const double D33 = _1oH * ((double)w[Bk] - w[F]);
-    const double D12 = .5 * _1oH * ((double)u[N] - u[S] + v[E] - v[W]);
-    const double D13 = .5 * _1oH * ((double)u[Bk] - u[F] + w[E] - w[W]);
-    const double D23 = .5 * _1oH * ((double)w[N] - w[S] + v[Bk] - v[F]);
+    auto vel = [](const T* f, size_t i, int bcf) { return (double)halo_value(f[i], HK_VEL, bcf); };
+    const double D11 = _1oH * (vel(u, E, fE) - vel(u, W, fW));
+    const double D22 = _1oH * (vel(v, N, fN) - vel(v, S, fS));
+    const double D33 = _1oH * (vel(w, Bk, fB) - vel(w, F, fF));
+    const double D12 = .5 * _1oH * (vel(u, N, fN) - vel(u, S, fS) + vel(v, E, fE) - vel(v, W, fW));
+    const double D13 = .5 * _1oH * (vel(u, Bk, fB) - vel(u, F, fF) + vel(w, E, fE) - vel(w, W, fW));
+    const double D23 = .5 * _1oH * (vel(w, N, fN) - vel(w, S, fS) + vel(v, Bk, fB) - vel(v, F, fF));
     const double P = p[C];
     const double nX = sd[1], nY = sd[2], nZ = sd[3];
     const double fXV = D11 * nX + D12 * nY + D13 * nZ;

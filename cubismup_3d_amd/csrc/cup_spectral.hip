@@ -143,6 +143,7 @@ int launch_ic_hit(Ctx& c, double k0, const double* K, const double* E, int tn, d
     d.precision = 8;
     d.decomposition = CUP_FUSED;
     d.device = c.dev;
+    d.bc[0] = d.bc[1] = d.bc[2] = CUP_BC_PERIODIC;
     cup_ctx* t = nullptr;
     if (int rc = cup_create(&d, &t)) return set_error(c, rc, "ic_hit: FP64 work context");
     int rc = tn ? cup_ic_hit_spectrum(t, K, E, tn, u_rms, seed) : cup_ic_hit(t, k0, u_rms, seed);

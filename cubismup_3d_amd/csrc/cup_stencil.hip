@@ -28,9 +28,10 @@ struct Vec2<double> {
 };
 
 
+// kinds[f] (HaloKind) says how field f's halo forms at a wall face; periodic grids ignore it.
 template <typename T, int NF>
 __device__ __forceinline__ void gather_g1(const T* const* src, T (*lab)[L1C], const GridArgs& g, int b,
-                                          int bx, int by, int bz)
+                                          int bx, int by, int bz, const int (&kinds)[NF])
 {
   const int tid = threadIdx.x;
   static_assert(2 * NT == BS3, "two cells per thread");
@@ -56,10 +57,18 @@ __device__ __forceinline__ void gather_g1(const T* const* src, T (*lab)[L1C], co
       case 4: qz = wrapi(bz - 1, g.nbz); sx = e; sy = a; sz = 7; lx = e + 1; ly = a + 1; lz = 0; break;
       default: qz = wrapi(bz + 1, g.nbz); sx = e; sy = a; sz = 0; lx = e + 1; ly = a + 1; lz = 9; break;
     }
+    const int bf = g.bcany ? domain_face(g, bx, by, bz, face) : 0;
+    if (bf) {  // a non-periodic domain face: the block's own boundary cell (BlockLabBC)
+      qx = bx; qy = by; qz = bz;
+      const int edge = (face & 1) ? 7 : 0;
+      if (face < 2) sx = edge;
+      else if (face < 4) sy = edge;
+      else sz = edge;
+    }
     const size_t s = (size_t)(qx + g.nbx * (qy + g.nby * qz)) * BS3 + (sz * BS + sy) * BS + sx;
     const int l = (lz * L1 + ly) * L1 + lx;
 #pragma unroll
-    for (int f = 0; f < NF; ++f) lab[f][l] = src[f][s];
+    for (int f = 0; f < NF; ++f) lab[f][l] = halo_value(src[f][s], kinds[f], bf);
   }
 }
 
@@ -86,10 +95,11 @@ constexpr int PZ = PX * L1;       // z stride
 constexpr int PLC = PX * L1 * L1;
 constexpr int ADV_PD = 3;  // blocks in flight in the advection's walk along x
 struct LabSlots {
-  int li;       // LDS index of interior cell 2t
-  int l[2];     // LDS index of the halo cells
-  int off[2];   // their offsets inside the neighbour block
-  int face[2];  // 0..5 = x-, x+, y-, y+, z-, z+
+  int li;        // LDS index of interior cell 2t
+  int l[2];      // LDS index of the halo cells
+  int off[2];    // their offsets inside the neighbour block
+  int off_bc[2]; // at a non-periodic domain face: the offset of the block's own boundary cell
+  int face[2];   // 0..5 = x-, x+, y-, y+, z-, z+
 };
 
 __device__ __forceinline__ bool has_halo2() { return threadIdx.x < 6 * 64 - NT; }
@@ -114,6 +124,9 @@ __device__ __forceinline__ LabSlots lab_slots()
     }
     s.face[k] = face;
     s.off[k] = (sz * BS + sy) * BS + sx;
+    // the boundary cell sits at the other end of the normal axis: 7 - (source index)
+    s.off_bc[k] = face < 2 ? (sz * BS + sy) * BS + (7 - sx) : (face < 4 ? (sz * BS + (7 - sy)) * BS + sx
+                                                                        : ((7 - sz) * BS + sy) * BS + sx);
     s.l[k] = (lz * L1 + ly) * PX + lx + 1;
   }
   return s;
@@ -137,7 +150,7 @@ struct LabRegs {
 
 template <typename T, int NF>
 __device__ __forceinline__ void lab_fetch(LabRegs<T, NF>& r, const T* const* src, const LabSlots& s,
-                                          const GridArgs& g, int b)
+                                          const GridArgs& g, int b, const int (&kinds)[NF])
 {
   using V2 = typename Vec2<T>::type;
   int bx, by, bz;
@@ -153,9 +166,10 @@ __device__ __forceinline__ void lab_fetch(LabRegs<T, NF>& r, const T* const* src
     if (k == 0 || has_halo2()) {
       // a halo slot's face is the same across its wave: the neighbour block is scalar work
       const int face = __builtin_amdgcn_readfirstlane(s.face[k]);
-      const size_t q = (size_t)face_block(g, bx, by, bz, face) * BS3 + s.off[k];
+      const int bf = g.bcany ? domain_face(g, bx, by, bz, face) : 0;
+      const size_t q = bf ? (size_t)b * BS3 + s.off_bc[k] : (size_t)face_block(g, bx, by, bz, face) * BS3 + s.off[k];
 #pragma unroll
-      for (int f = 0; f < NF; ++f) r.hv[f][k] = src[f][q];
+      for (int f = 0; f < NF; ++f) r.hv[f][k] = halo_value(src[f][q], kinds[f], bf);
     }
 }
 
@@ -330,9 +344,11 @@ int launch_reset_umax(Ctx& c)
 // pipeline holds (threads t % 4 == 3 hold ix = 6, 7; t % 4 == 0 hold ix = 0, 1), so only
 // the walk's two end faces are gathered. PD blocks are in flight (register sets rotate
 // through PD + 1 slots; the loop unrolls fully, so the slot indices are constants).
+// fade[face]: the advection's inflow fade of an open domain face (1: none), applied to the
+// velocities of that face's halo (AdvectionDiffusion.cpp:39-104).
 template <typename T, int NF>
 __device__ __forceinline__ void lab_fetch_xw(LabRegs<T, NF>& r, const T* const* src, const LabSlots& s,
-                                             const GridArgs& g, int b, bool x_lo, bool x_hi)
+                                             const GridArgs& g, int b, bool x_lo, bool x_hi, const T (&fade)[6])
 {
   using V2 = typename Vec2<T>::type;
   int bx, by, bz;
@@ -348,9 +364,11 @@ __device__ __forceinline__ void lab_fetch_xw(LabRegs<T, NF>& r, const T* const* 
     if (k == 0 || has_halo2()) {
       const int face = __builtin_amdgcn_readfirstlane(s.face[k]);
       if ((face == 0 && !x_lo) || (face == 1 && !x_hi)) continue;
-      const size_t q = (size_t)face_block(g, bx, by, bz, face) * BS3 + s.off[k];
+      const int bf = g.bcany ? domain_face(g, bx, by, bz, face) : 0;
+      const size_t q = bf ? (size_t)b * BS3 + s.off_bc[k] : (size_t)face_block(g, bx, by, bz, face) * BS3 + s.off[k];
+      const T fd = bf == CUP_BC_OPEN ? fade[face] : (T)1;
 #pragma unroll
-      for (int f = 0; f < NF; ++f) r.hv[f][k] = src[f][q];
+      for (int f = 0; f < NF; ++f) r.hv[f][k] = bf ? halo_value(src[f][q], HK_VEL, bf) * fd : src[f][q];
     }
 }
 
@@ -379,12 +397,17 @@ __device__ __forceinline__ void lab_put_xw(const LabRegs<T, NF>& r, const LabReg
     for (int f = 0; f < NF; ++f) lab[f][s.li + 8] = next.in[f][0];
 }
 
+template <typename T>
+struct Fade6 {
+  T f[6];
+};
+
 template <typename T, int NBW, int PD>
 __global__ __launch_bounds__(NT) void k_advdiff_xwalk(const T* __restrict__ u, const T* __restrict__ v,
                                                       const T* __restrict__ w, T* __restrict__ ou,
                                                       T* __restrict__ ov, T* __restrict__ ow, GridArgs g,
                                                       DevScalars* __restrict__ sc, T Ux, T Uy, T Uz, double nu,
-                                                      StepFold fold)
+                                                      StepFold fold, Fade6<T> fd)
 {
   __shared__ __align__(16) T lab[3][PLC];
   const LabSlots s = lab_slots();
@@ -393,7 +416,7 @@ __global__ __launch_bounds__(NT) void k_advdiff_xwalk(const T* __restrict__ u, c
   constexpr int NS = PD + 1;
   LabRegs<T, 3> r[NS];
 #pragma unroll
-  for (int p = 0; p < PD && p < NBW; ++p) lab_fetch_xw(r[p], src, s, g, b0 + p, p == 0, p == NBW - 1);
+  for (int p = 0; p < PD && p < NBW; ++p) lab_fetch_xw(r[p], src, s, g, b0 + p, p == 0, p == NBW - 1, fd.f);
   double dt;
   if (fold.slots) {
     // the previous step's finalize: no workgroup reads the scalars workgroup 0 updates
@@ -423,7 +446,7 @@ __global__ __launch_bounds__(NT) void k_advdiff_xwalk(const T* __restrict__ u, c
     if (j) __syncthreads();  // the previous block's compute is done with the lab
     lab_put_xw(r[j % NS], r[(j + NS - 1) % NS], r[(j + 1) % NS], lab, s, j == 0, j == NBW - 1);
     __syncthreads();
-    if (j + PD < NBW) lab_fetch_xw(r[(j + PD) % NS], src, s, g, b0 + j + PD, false, j + PD == NBW - 1);
+    if (j + PD < NBW) lab_fetch_xw(r[(j + PD) % NS], src, s, g, b0 + j + PD, false, j + PD == NBW - 1, fd.f);
     T av[3][2];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
@@ -460,23 +483,37 @@ __global__ __launch_bounds__(NT) void k_advdiff_xwalk(const T* __restrict__ u, c
 }
 
 template <typename T, int NBW>
-static void launch_advdiff_xwalk(Ctx& c, double nu, const double uinf[3], const StepFold& fold)
+static void launch_advdiff_xwalk(Ctx& c, double nu, const double uinf[3], const StepFold& fold, const Fade6<T>& fd)
 {
   hipLaunchKernelGGL((k_advdiff_xwalk<T, NBW, ADV_PD>), dim3(c.nb / NBW), dim3(NT), 0, c.stream,
                      (const T*)c.f[CUP_U], (const T*)c.f[CUP_V], (const T*)c.f[CUP_W], (T*)c.f[CUP_TMPU],
                      (T*)c.f[CUP_TMPV], (T*)c.f[CUP_TMPW], c.grid(), c.sc, (T)uinf[0], (T)uinf[1], (T)uinf[2], nu,
-                     fold);
+                     fold, fd);
 }
 
 template <typename T>
 static int advdiff_t(Ctx& c, double nu, const double uinf[3], const StepFold* fold_in)
 {
   const StepFold fold = fold_in ? *fold_in : StepFold{nullptr, 0, 0.0, 0, 0.0};
+  // KernelAdvectDiffuseBase's fades (AdvectionDiffusion.cpp:25-33): per open domain face
+  // 1 - fac (max or min of uinf_d over max |uinf|)^2, fac = min(1, uMax_measured dt / h); the
+  // kernel applies them on open faces only.
+  Fade6<T> fd;
+  {
+    double fac = fmin(1.0, c.umax_measured * c.host_dt / c.h);
+    double nor = 2.220446049250313e-16;
+    for (int a = 0; a < 3; ++a) nor = fmax(nor, fabs(uinf[a]));
+    for (int a = 0; a < 3; ++a) {
+      const double lo = fmax(uinf[a], 0.0) / nor, hi = fmin(uinf[a], 0.0) / nor;
+      fd.f[2 * a] = (T)(1 - fac * lo * lo);
+      fd.f[2 * a + 1] = (T)(1 - fac * hi * hi);
+    }
+  }
   const int nbx = c.bpd[0];
-  if (nbx % 8 == 0) launch_advdiff_xwalk<T, 8>(c, nu, uinf, fold);
-  else if (nbx % 4 == 0) launch_advdiff_xwalk<T, 4>(c, nu, uinf, fold);
-  else if (nbx % 2 == 0) launch_advdiff_xwalk<T, 2>(c, nu, uinf, fold);
-  else launch_advdiff_xwalk<T, 1>(c, nu, uinf, fold);
+  if (nbx % 8 == 0) launch_advdiff_xwalk<T, 8>(c, nu, uinf, fold, fd);
+  else if (nbx % 4 == 0) launch_advdiff_xwalk<T, 4>(c, nu, uinf, fold, fd);
+  else if (nbx % 2 == 0) launch_advdiff_xwalk<T, 2>(c, nu, uinf, fold, fd);
+  else launch_advdiff_xwalk<T, 1>(c, nu, uinf, fold, fd);
   return check_hip(c, hipGetLastError(), "k_advdiff_xwalk");
 }
 
@@ -599,6 +636,7 @@ int launch_advdiff(Ctx& c, double nu, const double uinf[3], int scheme, const St
     return c.precision == 4 ? advdiff_t<float>(c, nu, uinf, fold) : advdiff_t<double>(c, nu, uinf, fold);
   if (scheme != CUP_UPWIND3_EULER && scheme != CUP_UPWIND3_RK2)
     return set_error(c, CUP_E_INVALID, "advect_diffuse: unknown scheme");
+  if (!c.periodic()) return set_error(c, CUP_E_INVALID, "advect_diffuse: the upwind schemes need a periodic box");
   if (fold) return set_error(c, CUP_E_STATE, "advect_diffuse: the folded step bookkeeping takes the central scheme");
   auto stage = [&](int src0, int base0, int out0, double mult) {
     return c.precision == 4 ? up3_stage<float>(c, src0, base0, out0, nu, uinf, mult)
@@ -609,6 +647,84 @@ int launch_advdiff(Ctx& c, double nu, const double uinf[3], int scheme, const St
   if (int rc = stage(CUP_TMPU, CUP_U, CUP_U, 1.0)) return rc;
   c.ustar = CUP_U;
   return CUP_OK;
+}
+
+// ------------------------------------------------------------- inflow correction
+// UpdateAndCorrectInflow::operate (AdvectionDiffusion.cpp:209-296) after the copy tmp -> u:
+// the net outflow sum_faces (+-) u_n over the boundary cells of the open domain faces (neither
+// periodic nor wall), then corr = sum / (2 (NxNy + NxNz + NyNz)) is added to the inflow
+// faces' and removed from the outflow faces' normal velocity, unless |corr| < DBL_EPSILON.
+// One workgroup per block; blocks off the open faces return at once.
+__device__ __forceinline__ bool open_face(const GridArgs& g, int bx, int by, int bz, int d, int side)
+{
+  if (g.bc[d] != CUP_BC_OPEN) return false;
+  const int bi = d == 0 ? bx : (d == 1 ? by : bz), nbk = d == 0 ? g.nbx : (d == 1 ? g.nby : g.nbz);
+  return side ? bi == nbk - 1 : bi == 0;
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_inflow_sum(const T* __restrict__ u, const T* __restrict__ v,
+                                                   const T* __restrict__ w, GridArgs g, double* __restrict__ part)
+{
+  const int b = blockIdx.x;
+  int bx, by, bz;
+  block_coords(g, b, bx, by, bz);
+  const T* f[3] = {u, v, w};
+  double acc[1] = {0.0};
+  for (int d = 0; d < 3; ++d)
+    for (int side = 0; side < 2; ++side) {
+      if (!open_face(g, bx, by, bz, d, side)) continue;
+      if (threadIdx.x < BS * BS) {
+        int id[3];
+        id[d] = side ? BS - 1 : 0;
+        id[(d + 1) % 3] = threadIdx.x & 7;
+        id[(d + 2) % 3] = threadIdx.x >> 3;
+        const double un = (double)f[d][(size_t)b * BS3 + (id[2] * BS + id[1]) * BS + id[0]];
+        acc[0] += side ? un : -un;
+      }
+    }
+  block_reduce<1>(acc, 0u, part + b);
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_inflow_apply(T* __restrict__ u, T* __restrict__ v, T* __restrict__ w,
+                                                     GridArgs g, const DevScalars* __restrict__ sc)
+{
+  const int b = blockIdx.x;
+  int bx, by, bz;
+  block_coords(g, b, bx, by, bz);
+  const double nx = g.Nx, ny = g.Ny, nz = g.Nz;
+  const double corr = sc->red[0] / (2 * (nx * ny + nx * nz + ny * nz));
+  if (fabs(corr) < 2.220446049250313e-16) return;
+  T* f[3] = {u, v, w};
+  for (int d = 0; d < 3; ++d)
+    for (int side = 0; side < 2; ++side) {
+      if (!open_face(g, bx, by, bz, d, side) || threadIdx.x >= BS * BS) continue;
+      int id[3];
+      id[d] = side ? BS - 1 : 0;
+      id[(d + 1) % 3] = threadIdx.x & 7;
+      id[(d + 2) % 3] = threadIdx.x >> 3;
+      T& un = f[d][(size_t)b * BS3 + (id[2] * BS + id[1]) * BS + id[0]];
+      un = (T)((double)un + (side ? -corr : corr));
+    }
+}
+
+template <typename T>
+static int inflow_t(Ctx& c)
+{
+  hipLaunchKernelGGL((k_inflow_sum<T>), dim3(c.nb), dim3(NT), 0, c.stream, (const T*)c.f[CUP_U],
+                     (const T*)c.f[CUP_V], (const T*)c.f[CUP_W], c.grid(), c.partials);
+  if (int rc = launch_reduce(c, c.nb, 1, 0u, c.sc->red)) return rc;
+  hipLaunchKernelGGL((k_inflow_apply<T>), dim3(c.nb), dim3(NT), 0, c.stream, (T*)c.f[CUP_U], (T*)c.f[CUP_V],
+                     (T*)c.f[CUP_W], c.grid(), c.sc);
+  return check_hip(c, hipGetLastError(), "inflow correction");
+}
+
+int launch_inflow_correction(Ctx& c)
+{
+  if (c.bc[0] != CUP_BC_OPEN && c.bc[1] != CUP_BC_OPEN && c.bc[2] != CUP_BC_OPEN) return CUP_OK;
+  if (int e = ensure_partials(c, (size_t)c.nb)) return e;
+  return c.precision == 4 ? inflow_t<float>(c) : inflow_t<double>(c);
 }
 
 // -------------------------------------------------------------------- copies, fills
@@ -720,6 +836,7 @@ __global__ void k_set_dt(DevScalars* sc, double dt) { sc->dt = dt; }
 
 int launch_set_dt(Ctx& c, double dt)
 {
+  c.host_dt = dt;
   hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, c.stream, c.sc, dt);
   return check_hip(c, hipGetLastError(), "k_set_dt");
 }
@@ -738,7 +855,7 @@ __global__ __launch_bounds__(NT) void k_rhs(const T* __restrict__ u, const T* __
   int bx, by, bz;
   block_coords(g, b, bx, by, bz);
   const T* src[3] = {u, v, w};
-  gather_g1<T, 3>(src, lab, g, b, bx, by, bz);
+  gather_g1<T, 3>(src, lab, g, b, bx, by, bz, {HK_VEL, HK_VEL, HK_VEL});
   __syncthreads();
   const T fac = (T)(0.5 * g.h * g.h / sc->dt);
   for (int c = threadIdx.x; c < BS3; c += NT) {
@@ -802,7 +919,7 @@ __global__ __launch_bounds__(NT) void k_gradp(const T* __restrict__ p, const T* 
   T us[3][2];
   T rr[4];
   auto fetch = [&](int b) {
-    lab_fetch(r, src, s, g, b);
+    lab_fetch(r, src, s, g, b, {HK_COPY});
     const size_t o = (size_t)b * BS3 + c0;
     const V2 a = *reinterpret_cast<const V2*>(su + o);
     const V2 bb = *reinterpret_cast<const V2*>(sv + o);
@@ -964,7 +1081,7 @@ __global__ __launch_bounds__(NT) void k_adj_advdiff(const T* __restrict__ u, con
   int bx, by, bz;
   block_coords(g, b, bx, by, bz);
   const T* src[6] = {u, v, w, lu, lv, lw};
-  gather_g1<T, 6>(src, lab, g, b, bx, by, bz);
+  gather_g1<T, 6>(src, lab, g, b, bx, by, bz, {HK_VEL, HK_VEL, HK_VEL, HK_VEL, HK_VEL, HK_VEL});
   __syncthreads();
   const double dt = sc->dt;
   const T facA = (T)(-dt / (2 * g.h));
@@ -1210,7 +1327,7 @@ __global__ __launch_bounds__(NT) void k_diagnostics(const T* __restrict__ u, con
     int bx, by, bz;
     block_coords(g, b, bx, by, bz);
     if (j) __syncthreads();  // the previous block is done with the lab
-    gather_g1<T, 4>(src, lab, g, b, bx, by, bz);
+    gather_g1<T, 4>(src, lab, g, b, bx, by, bz, {HK_VEL, HK_VEL, HK_VEL, HK_COPY});
     __syncthreads();
     for (int c = threadIdx.x; c < BS3; c += NT) {
       const int ix = c & 7, iy = (c >> 3) & 7, iz = c >> 6;
@@ -1297,7 +1414,7 @@ __global__ __launch_bounds__(NT) void k_grad_stats(const T* __restrict__ u, cons
     int bx, by, bz;
     block_coords(g, b, bx, by, bz);
     if (j) __syncthreads();  // the previous block is done with the lab
-    gather_g1<T, 3>(src, lab, g, b, bx, by, bz);
+    gather_g1<T, 3>(src, lab, g, b, bx, by, bz, {HK_VEL, HK_VEL, HK_VEL});
     __syncthreads();
     for (int c = threadIdx.x; c < BS3; c += NT) {
       const int l = (((c >> 6) + 1) * L1 + ((c >> 3) & 7) + 1) * L1 + (c & 7) + 1;
@@ -1451,6 +1568,13 @@ extern "C" int cup_diagnostics_noalloc(const cup_grid_desc* d, const void* const
   g.nbx = d->bpd[0]; g.nby = d->bpd[1]; g.nbz = d->bpd[2];
   g.Nx = g.nbx * BS; g.Ny = g.nby * BS; g.Nz = g.nbz * BS;
   g.nb = (int)nb;
+  g.bcany = 0;
+  for (int a = 0; a < 3; ++a) {
+    if (d->bc[a] != CUP_BC_PERIODIC && d->bc[a] != CUP_BC_OPEN && d->bc[a] != CUP_BC_WALL)
+      return free_error(CUP_E_INVALID, "diagnostics_noalloc: unknown boundary condition");
+    g.bc[a] = d->bc[a];
+    g.bcany |= d->bc[a] != CUP_BC_PERIODIC;
+  }
   const int maxN = std::max(g.Nx, std::max(g.Ny, g.Nz));
   g.h = d->maxextent / maxN;
   const double half_extent[3] = {0.5 * ((double)g.Nx / maxN) * d->maxextent, 0.5 * ((double)g.Ny / maxN) * d->maxextent,

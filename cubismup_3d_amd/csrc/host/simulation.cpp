@@ -208,7 +208,8 @@ SimulationData::SimulationData(int argc, const char* const* argv) : SimulationDa
   bUseFourierBC = BCx_flag == periodic && BCy_flag == periodic && BCz_flag == periodic;
   const std::string prec = parser("-precision").asString("double");
   precision = (prec == "float" || prec == "single" || prec == "4" || prec == "fp32") ? 4 : 8;
-  const std::string dec = parser("-decomposition").asString(precision == 4 ? "fused" : "plain");
+  // FUSED (the device-resident step) takes periodic boxes; non-periodic ones default to PLAIN
+  const std::string dec = parser("-decomposition").asString(precision == 4 && bUseFourierBC ? "fused" : "plain");
   decomposition = dec == "fused" ? CUP_FUSED : CUP_PLAIN;
   device = parser("-device").asInt(0);
   logger->set_dir(path4serialization);
@@ -261,8 +262,17 @@ void SimulationData::_preprocessArguments()
     throw std::invalid_argument("the fixed mass flux of channel flows needs walls; this build has periodic boxes");
   if (analysis == "HIT" && (freqAnalysis > 0 || timeAnalysis > 0) && !(nu > 0))
     throw std::invalid_argument("HIT analysis needs nu > 0");
-  if (BCx_flag != periodic || BCy_flag != periodic || BCz_flag != periodic)
-    throw std::invalid_argument("only fully periodic boxes are supported (-BC_x/y/z periodic)");
+  if (BCx_flag == freespace || BCy_flag == freespace || BCz_flag == freespace)
+    throw std::invalid_argument("freespace boundaries need PoissonSolverUnbounded, which this build does not have "
+                                "(periodic, dirichlet and wall are supported)");
+  if (!bUseFourierBC) {  // the non-periodic path (PoissonSolverMixed)
+    if (decomposition == CUP_FUSED)
+      throw std::invalid_argument("non-periodic boundaries take -decomposition plain");
+    if (spectralForcing || initCond == "HITurbulence" || analysis == "HIT")
+      throw std::invalid_argument("spectral forcing, the HIT initial condition and the HIT analysis need a periodic box");
+    if (scheme != CUP_CENTRAL_EULER)
+      throw std::invalid_argument("the upwind schemes need a periodic box");
+  }
   hmin = hmax = hmean = maxextent / maxbpd;
   for (auto& s : obstacles) {
     if (s->position[1] < 0) s->position[1] = extent[1] / 2;  // ObstacleArguments defaults
@@ -665,7 +675,9 @@ void PressureRHS::operator()(double dt) { check(cup_pressure_rhs(sim.ctx, dt), "
 // PressureProjection::operator() (PressureProjection.cpp:107-129).
 void PressureProjection::operator()(double dt)
 {
-  check(cup_poisson_periodic(sim.ctx), "PressureProjection solve");
+  // PressureProjection's solver (PressureProjection.cpp:86-105): the FFT solver when every face
+  // is periodic (bUseFourierBC), PoissonSolverMixed otherwise
+  check(sim.bUseFourierBC ? cup_poisson_periodic(sim.ctx) : cup_poisson_mixed(sim.ctx), "PressureProjection solve");
   check(cup_pressure_project(sim.ctx, dt), "PressureProjection gradient");
 }
 
@@ -1071,6 +1083,8 @@ void Simulation::setupGrid()
   d.precision = sim.precision;
   d.decomposition = sim.decomposition;
   d.device = sim.device;
+  const BCflag f[3] = {sim.BCx_flag, sim.BCy_flag, sim.BCz_flag};
+  for (int a = 0; a < 3; ++a) d.bc[a] = f[a] == periodic ? CUP_BC_PERIODIC : (f[a] == wall ? CUP_BC_WALL : CUP_BC_OPEN);
   const int rc = cup_create(&d, &sim.ctx);
   if (rc != CUP_OK) throw std::runtime_error("cup_create failed (" + std::to_string(rc) + ")");
 }

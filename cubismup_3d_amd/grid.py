@@ -33,11 +33,14 @@ def from_blocks(b: np.ndarray, shape) -> np.ndarray:
 class Grid:
     """A device context of the per-timestep fluid update (cup_create / cup_destroy)."""
 
-    def __init__(self, bpd, maxextent=2 * np.pi, precision=8, decomposition="plain", device=0):
+    def __init__(self, bpd, maxextent=2 * np.pi, precision=8, decomposition="plain", device=0,
+                 bc=("periodic", "periodic", "periodic")):
         self.bpd = tuple(int(b) for b in bpd)
+        self.bc = tuple(bc)
         d = L.GridDesc()
         for a in range(3):
             d.bpd[a] = self.bpd[a]
+            d.bc[a] = L.BC_IDS[self.bc[a]]
         d.maxextent = float(maxextent)
         d.precision = int(precision)
         d.decomposition = L.CUP_FUSED if decomposition == "fused" else L.CUP_PLAIN
@@ -196,6 +199,9 @@ class Grid:
 
     def poisson_periodic(self):
         self._c(L.lib().cup_poisson_periodic(self._h))
+
+    def poisson_mixed(self):
+        self._c(L.lib().cup_poisson_mixed(self._h))
 
     def pressure_project(self, dt):
         self._c(L.lib().cup_pressure_project(self._h, float(dt)))

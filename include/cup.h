@@ -52,21 +52,31 @@ enum cup_decomposition { CUP_PLAIN = 0, CUP_FUSED = 1 };
  * midpoint rule u* = u + dt L(u + dt/2 L(u)). */
 enum cup_scheme { CUP_CENTRAL_EULER = 0, CUP_UPWIND3_EULER = 1, CUP_UPWIND3_RK2 = 2 };
 
+/* Boundary condition of an axis (SimulationData's BCx/y/z_flag, SimulationData.cpp:123-166):
+ * periodic; open, the reference's dirichlet (its labs treat dirichlet, absorbing and
+ * freespace alike: the boundary cell is copied outward, BlockLabBC::applyBCfaceOpen,
+ * Definitions.h:278-301); wall (no slip: velocities mirrored with their sign flipped, p
+ * copied, chi zero, applyBCfaceWall :302-334). */
+enum cup_bc { CUP_BC_PERIODIC = 0, CUP_BC_OPEN = 1, CUP_BC_WALL = 2 };
+
 typedef struct cup_grid_desc {
   int bpd[3];          /* blocks per dimension (SimulationData::bpdx/y/z, SimulationData.h:88) */
   double maxextent;    /* longest side; h = maxextent / max cells (SimulationData.cpp:184-195) */
   int precision;       /* 4: FP32 bulk fields, 8: FP64 */
   int decomposition;   /* cup_decomposition */
   int device;          /* HIP device ordinal */
+  int bc[3];           /* cup_bc per axis; non-periodic axes need CUP_PLAIN */
 } cup_grid_desc;
 
 typedef struct cup_ctx cup_ctx;
 
 /* Grid and block store. Replaces the Cubism FluidGridMPI allocation in
  * Simulation::setupGrid (Simulation.cpp:221-243) and the PoissonSolverPeriodic buffers and
- * plans (PoissonSolverPeriodic.cpp:90-127). Only fully periodic boxes are supported. Cells per
- * axis (8 x blocks) from 8 to 1024 with no prime factor above 5; other sizes are refused with
- * CUP_E_INVALID (FFTW plans any size). */
+ * plans (PoissonSolverPeriodic.cpp:90-127). Cells per axis (8 x blocks) from 8 to 1024 with no
+ * prime factor above 5; other sizes are refused with CUP_E_INVALID (FFTW plans any size).
+ * Non-periodic axes (desc->bc) take the PLAIN decomposition, the central scheme and
+ * cup_poisson_mixed; the spectral operations (forcing, HIT IC and analysis), the upwind schemes
+ * and the adjoint need a fully periodic box (CUP_E_INVALID otherwise). */
 int cup_create(const cup_grid_desc* desc, cup_ctx** out);
 int cup_destroy(cup_ctx* ctx);
 /* The context's last error message; with ctx == NULL, the message of the calling thread's
@@ -100,9 +110,11 @@ int cup_calc_max_timestep(cup_ctx* ctx, double CFL, double nu, const double uinf
                           int rampup, double* dt_out, double* umax_out);
 
 /* AdvectionDiffusion::operator() (AdvectionDiffusion.cpp:402-428): KernelAdvectDiffuse
- * (:106-145) into tmpU/V/W and the copy tmp -> u (:315-328; the inflow correction is a
- * no-op on periodic faces). With CUP_FUSED the copy is skipped: u keeps the old velocity
- * until cup_pressure_project writes u = tmp - dt grad p. */
+ * (:106-145) into tmpU/V/W and the copy tmp -> u with UpdateAndCorrectInflow's correction on
+ * the open faces (:209-296, none on periodic boxes). On open inflow faces the halo velocity is
+ * faded by 1 - min(1, uMax_measured dt/h) (uinf_d/max|uinf|)^2 (:20-104), uMax_measured being
+ * the last cup_calc_max_timestep's maximum. With CUP_FUSED the copy is skipped: u keeps the
+ * old velocity until cup_pressure_project writes u = tmp - dt grad p. */
 int cup_advect_diffuse(cup_ctx* ctx, double dt, double nu, const double uinf[3], int scheme);
 
 /* PressureRHS::operator() (PressureRHS.cpp:273-346): h^2/(2dt) * central divergence of
@@ -114,6 +126,15 @@ int cup_pressure_rhs(cup_ctx* ctx, double dt);
  * (PoissonSolver.cpp:116-130): hand-written r2c FFT, grid-consistent spectral division
  * (:15-61), c2r, result in p. */
 int cup_poisson_periodic(cup_ctx* ctx);
+
+/* PoissonSolverMixed::solve (PoissonSolverMixed.cpp:66-103, _solve PoissonSolverMixed.h:27-80)
+ * and _fftw2cub: r2r transforms per axis (the real DFT on periodic axes, DCT-II / DCT-III on
+ * the others), division by the blended symbol (1 - tol) grid-consistent + tol spectral,
+ * tol = 0.01, the first coefficient zeroed; result in p. The solver of non-periodic boxes
+ * (PressureProjection.cpp:86-105), and of periodic boxes on the reference's Python path
+ * (its SimulationData never sets bUseFourierBC). CUP_PLAIN decomposition only (CUP_E_STATE):
+ * it solves the RHS of cup_pressure_rhs. */
+int cup_poisson_mixed(cup_ctx* ctx);
 
 /* KernelGradP (PressureProjection.cpp:27-51): u -= dt/(2h) (p_E - p_W), etc. */
 int cup_pressure_project(cup_ctx* ctx, double dt);

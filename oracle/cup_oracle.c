@@ -2553,7 +2553,10 @@ void orc_timestep(orc_sim* s, double dt, const orc_params* prm, double forcing_s
   if (prm->spectral_forcing)
     orc_spectral_forcing(s, dt, dt, prm->nu, prm->tke_target, prm->injection_rate, forcing_stats);
   orc_pressure_rhs(s, dt);
-  orc_poisson_solve(s);
+  /* PressureProjection's solver (PressureProjection.cpp:86-105): FFT with every face periodic
+   * (bUseFourierBC), else PoissonSolverMixed */
+  if (s->bcf[0] == ORC_BC_PERIODIC && s->bcf[1] == ORC_BC_PERIODIC && s->bcf[2] == ORC_BC_PERIODIC) orc_poisson_solve(s);
+  else orc_poisson_mixed(s);
   orc_grad_p(s, dt);
   if (prm->has_sphere && s->has_sphere) {
     double ft[6];

@@ -322,6 +322,14 @@ class OracleSim:
     def poisson_solve(self):
         lib().orc_poisson_solve(self._h)
 
+    def poisson(self):
+        """The solver PressureProjection picks (PressureProjection.cpp:86-105): FFT for a fully
+        periodic box, PoissonSolverMixed otherwise."""
+        if all(b == "periodic" for b in self.bc):
+            self.poisson_solve()
+        else:
+            self.poisson_mixed()
+
     def poisson_mixed(self):
         """PoissonSolverMixed's solve (DFT on periodic axes, DCT on the others) of the RHS into p."""
         lib().orc_poisson_mixed(self._h)
@@ -449,7 +457,7 @@ class OracleSim:
         self.sphere_create(body["radius"], body["position"])
         self.advect_diffuse(dt, nu, uinf)
         self.pressure_rhs(dt)
-        self.poisson_solve()
+        self.poisson()
         self.grad_p(dt)
         M = self.obstacle_momenta(dt, lambda_, implicit)
         V, Om = compute_velocities(M, implicit, body["forced"], body["imposed"], body["block_rot"])
@@ -476,7 +484,7 @@ class OracleSim:
             b["position"] = new
         self.advect_diffuse(dt, nu, uinf)
         self.pressure_rhs(dt)
-        self.poisson_solve()
+        self.poisson()
         self.grad_p(dt)
         for k, b in enumerate(bodies):
             with self.body(k):
@@ -500,7 +508,7 @@ class OracleSim:
         if spectral_forcing:
             out["forcing"] = self.spectral_forcing(dt, nu, tke_target=tke_target)
         self.pressure_rhs(dt)
-        self.poisson_solve()
+        self.poisson()
         self.grad_p(dt)
         if sphere is not None:
             out["penal_ft"] = self.penalize(dt, sphere["lambda_"], sphere["implicit"],
