@@ -89,6 +89,7 @@ SIGNATURES = {
     "cup_ic_hit_spectrum": (_i, [_p, _dp, _dp, _i, _d, _u64]),
     "cup_diagnostics": (_i, [_p, _d, _dp]),
     "cup_diagnostics_noalloc": (_i, [C.POINTER(GridDesc), C.POINTER(C.c_void_p), _d, _p, _p]),
+    "cup_check_nan": (_i, [_p, C.POINTER(C.c_longlong)]),
     "cup_ic_taylor_green": (_i, [_p, _d]),
     "cup_ic_hit": (_i, [_p, _d, _d, _u64]),
     "cup_spectral_forcing": (_i, [_p, _d, _d, _d, _d, _d, _dp]),

@@ -2,6 +2,7 @@
 // and the operator entry points, each launching the hand-written kernels on the
 // context's stream. No exception crosses this boundary.
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <cmath>
 #include <cstring>
@@ -103,8 +104,20 @@ static int read_scalars(Ctx& c)
   return check_hip(c, hipStreamSynchronize(c.stream), "synchronize");
 }
 
+// roctx range names of the fused step's stages, after the reference's profiler regions
+// (AdvectionDiffusion.cpp:406, PressureRHS.cpp:285, PoissonSolverPeriodic.cpp:131-143,
+// PressureProjection.cpp:115, Penalization.cpp:133)
+static const char* const kStageName[] = {"AdvDiff Kernel", "PresRHS Kernel + FFTW r2c x", "Forcing reduce",
+                                         "FFTW r2c y",     "FFTW solve z",                "FFTW c2r y",
+                                         "FFTW c2r x",     "GradP",                       "Penalization",
+                                         "Step finalize"};
+
 void stage_mark(Ctx& c, int stage, bool end)
 {
+  if (end)
+    roctxRangePop();
+  else
+    roctxRangePushA(kStageName[stage]);
   if (stage != c.timing_stage || c.timing_count >= c.timing_cap) return;
   hipEventRecord(c.tev[2 * c.timing_count + (end ? 1 : 0)], c.stream);
   if (end) c.timing_count++;
@@ -446,6 +459,13 @@ int cup_diagnostics(cup_ctx* ctx, double nu, double out[20])
   if (int rc = read_scalars(c)) return rc;
   for (int i = 0; i < 20; ++i) out[i] = c.sc_host->red[i];
   return CUP_OK;
+}
+
+int cup_check_nan(cup_ctx* ctx, long long counts[4])
+{
+  CUP_GUARD(ctx);
+  if (!counts) return set_error(c, CUP_E_INVALID, "check_nan: null output");
+  return launch_count_nan(c, counts);
 }
 
 int cup_hit_analysis(cup_ctx* ctx, double nu, double out[9], double* spectrum, int nbin)

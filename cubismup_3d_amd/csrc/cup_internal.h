@@ -47,6 +47,7 @@ struct DevScalars {
   double forcing_fac;
   double tke, tke_low, eps, lint;
   double red[32];    // generic finalized reduction results
+  unsigned long long nan_count[4];  // cup_check_nan: NaN cells of u, v, w, p
   // max |u+uinf| of the fused step, folded per workgroup by atomic max on the bit pattern
   // of a non-negative double, spread over UMAX_SLOTS addresses to avoid contention. Two
   // banks: a step's gradient writes one while the next step's advection reads the other.
@@ -309,6 +310,7 @@ int launch_fill(Ctx& c, void* ptr, size_t n, double value);
 int launch_copy(Ctx& c, void* dst, const void* src, size_t n);
 int launch_set_dt(Ctx& c, double dt);
 int launch_reduce(Ctx& c, int n, int width, unsigned maxmask, double* out_dev, const double* part = nullptr);
+int launch_count_nan(Ctx& c, long long out[4]);  // NaN cells of u, v, w, p (synchronizes)
 // Records the message of a failed call that has no context (read by cup_last_error(NULL)).
 int free_error(int code, const std::string& msg);
 int launch_reset_umax(Ctx& c);  // clears both banks

@@ -1275,6 +1275,63 @@ int launch_reduce(Ctx& c, int n, int width, unsigned maxmask, double* out_dev, c
   return check_hip(c, hipGetLastError(), "k_reduce");
 }
 
+// ----------------------------------------------------------------- NaN check
+
+// NaN by its bit pattern (all-ones exponent, non-zero mantissa), which no floating-point
+// flag of the build can fold away.
+__device__ inline unsigned is_nan_bits(float x) { return (__float_as_uint(x) & 0x7fffffffu) > 0x7f800000u; }
+__device__ inline unsigned is_nan_bits(double x)
+{
+  return ((unsigned long long)__double_as_longlong(x) & 0x7fffffffffffffffull) > 0x7ff0000000000000ull;
+}
+
+// Operator::check's scan (Operator.h:23-50) as counts: NaN cells of u, v, w and p, summed per
+// lane over a grid-stride loop, per wave by shuffles, then one atomic per wave and field.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_count_nan(const T* __restrict__ u, const T* __restrict__ v,
+                                                  const T* __restrict__ w, const T* __restrict__ p, size_t n,
+                                                  unsigned long long* __restrict__ out)
+{
+  unsigned cnt[4] = {0, 0, 0, 0};
+  for (size_t i = (size_t)blockIdx.x * NT + threadIdx.x; i < n; i += (size_t)gridDim.x * NT) {
+    cnt[0] += is_nan_bits(u[i]);
+    cnt[1] += is_nan_bits(v[i]);
+    cnt[2] += is_nan_bits(w[i]);
+    cnt[3] += is_nan_bits(p[i]);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    unsigned x = cnt[k];
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+    if ((threadIdx.x & 63) == 0 && x) atomicAdd(out + k, (unsigned long long)x);
+  }
+}
+
+__global__ void k_clear_u64(unsigned long long* p, int n)
+{
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+int launch_count_nan(Ctx& c, long long out[4])
+{
+  unsigned long long* cnt = c.sc->nan_count;
+  hipLaunchKernelGGL(k_clear_u64, dim3(1), dim3(64), 0, c.stream, cnt, 4);
+  const int grid = (int)std::min<size_t>(4096, (c.ncell + NT - 1) / NT);
+  if (c.precision == 4)
+    hipLaunchKernelGGL(k_count_nan<float>, dim3(grid), dim3(NT), 0, c.stream, (const float*)c.f[CUP_U],
+                       (const float*)c.f[CUP_V], (const float*)c.f[CUP_W], (const float*)c.f[CUP_P], c.ncell, cnt);
+  else
+    hipLaunchKernelGGL(k_count_nan<double>, dim3(grid), dim3(NT), 0, c.stream, (const double*)c.f[CUP_U],
+                       (const double*)c.f[CUP_V], (const double*)c.f[CUP_W], (const double*)c.f[CUP_P], c.ncell, cnt);
+  if (int rc = check_hip(c, hipGetLastError(), "k_count_nan")) return rc;
+  unsigned long long h[4];
+  if (int rc = check_hip(c, hipMemcpyAsync(h, cnt, sizeof(h), hipMemcpyDeviceToHost, c.stream), "nan counts"))
+    return rc;
+  if (int rc = check_hip(c, hipStreamSynchronize(c.stream), "synchronize")) return rc;
+  for (int k = 0; k < 4; ++k) out[k] = (long long)h[k];
+  return CUP_OK;
+}
+
 // Step bookkeeping after the fused gradient: umax of the new velocity from the atomic
 // max over one bank of slots (then cleared), step += 1 and time += dt when advancing, then
 // the next dt by the CFL rule.

@@ -13,6 +13,8 @@
 #include <cstring>
 #include <sys/stat.h>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 namespace cubismup3d {
 
 BCflag string2BCflag(const std::string& s)
@@ -107,7 +109,7 @@ void BufferedLogger::flush()
 
 // ------------------------------------------------------------- SimulationData
 
-SimulationData::SimulationData() : logger(std::make_shared<BufferedLogger>()) {}
+SimulationData::SimulationData() : logger(std::make_shared<BufferedLogger>()), profiler(std::make_shared<Profiler>()) {}
 
 SimulationData::SimulationData(const SimulationData& o) { *this = o; }
 
@@ -133,7 +135,9 @@ SimulationData& SimulationData::operator=(const SimulationData& o)
   precision = o.precision; decomposition = o.decomposition; scheme = o.scheme; device = o.device;
   obstacles.clear();
   for (const auto& s : o.obstacles) obstacles.push_back(std::make_shared<Sphere>(*s));
+  bCheckNaN = o.bCheckNaN;
   logger = std::make_shared<BufferedLogger>(o.path4serialization);
+  profiler = std::make_shared<Profiler>();
   ctx = nullptr;  // a copy never shares the device context
   return *this;
 }
@@ -195,6 +199,7 @@ SimulationData::SimulationData(int argc, const char* const* argv) : SimulationDa
   uinf[2] = parser("-uinfz").asDouble(0.0);
   verbose = parser("-verbose").asBool(false);
   muteAll = parser("-muteAll").asBool(false);
+  bCheckNaN = parser("-checkNaN").asBool(false);
   rampup = parser("-rampup").asInt(100);
   nsteps = parser("-nsteps").asInt(0);
   endTime = parser("-tend").asDouble(0);
@@ -298,6 +303,86 @@ void Operator::check(int rc, const char* what)
   if (rc == CUP_E_INVALID) throw std::invalid_argument(m);
   throw std::runtime_error(m);
 }
+
+// Operator::check's scan on the device when the run asks for it.
+static void checkNaN(const SimulationData& sim, const std::string& infoText)
+{
+  if (!sim.bCheckNaN) return;
+  long long n[4];
+  if (cup_check_nan(sim.ctx, n) != CUP_OK) throw std::runtime_error(std::string("check: ") + cup_last_error(sim.ctx));
+  if (n[0] || n[1] || n[2] || n[3]) {
+    const std::string m = "GenericCoordinator::check isnan " + infoText;
+    fflush(stderr);
+    printf("%s\n", m.c_str());
+    fflush(stdout);
+    throw std::runtime_error(m);
+  }
+}
+
+void Operator::check(const std::string& infoText) { checkNaN(sim, infoText); }
+
+// ------------------------------------------------------------------ profiler
+
+void Profiler::push_start(const std::string& name, cup_ctx* ctx)
+{
+  roctxRangePushA(name.c_str());
+  const bool timed = timing && ctx;
+  if (timed) cup_synchronize(ctx);
+  stack_.push_back({timed ? name : std::string(), std::chrono::steady_clock::now()});
+}
+
+void Profiler::pop_stop(cup_ctx* ctx)
+{
+  if (stack_.empty()) return;
+  const auto& top = stack_.back();
+  if (!top.first.empty()) {  // a timed region
+    if (ctx) cup_synchronize(ctx);
+    Agent& a = agents_[top.first];
+    a.seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - top.second).count();
+    a.count++;
+  }
+  stack_.pop_back();
+  roctxRangePop();
+}
+
+void Profiler::printSummary(FILE* out) const
+{
+  double total = 0;
+  for (const auto& kv : agents_) total += kv.second.seconds;
+  std::vector<std::pair<std::string, Agent>> rows(agents_.begin(), agents_.end());
+  std::sort(rows.begin(), rows.end(), [](const auto& a, const auto& b) { return a.second.seconds > b.second.seconds; });
+  fprintf(out, "%-30s %12s %12s %7s %8s\n", "region", "total[ms]", "mean[ms]", "share", "count");
+  for (const auto& r : rows)
+    fprintf(out, "%-30s %12.3f %12.4f %6.1f%% %8ld\n", r.first.c_str(), 1e3 * r.second.seconds,
+            1e3 * r.second.seconds / std::max(1L, r.second.count), total > 0 ? 100 * r.second.seconds / total : 0.0,
+            r.second.count);
+  fflush(out);
+}
+
+void Profiler::reset() { agents_.clear(); }
+
+void SimulationData::startProfiler(const std::string& name) const
+{
+  profiler->timing = verbose;
+  profiler->push_start(name, ctx);
+}
+
+void SimulationData::stopProfiler() const { profiler->pop_stop(ctx); }
+
+void SimulationData::printResetProfiler()
+{
+  profiler->printSummary();
+  profiler->reset();
+}
+
+namespace {
+// One profiler region for the lifetime of the object, closed also when an operator throws.
+struct Region {
+  const SimulationData& sim;
+  Region(const SimulationData& s, const char* name) : sim(s) { sim.startProfiler(name); }
+  ~Region() { sim.stopProfiler(); }
+};
+}  // namespace
 
 // InitialConditions::operator() (InitialConditions.cpp:164-230).
 // SpectralIcGenerator::_generateTarget (SpectralIcGenerator.cpp:21-98): the target spectrum
@@ -602,6 +687,7 @@ void CreateObstacles::operator()(double dt)
     build();
     return;
   }
+  Region r(sim, "Obst CHI");  // ObstaclesCreate.cpp:466-493: the SDF, chi and centre of mass
   for (int k : moved) {
     auto& s = *sim.obstacles[k];
     double cm[4];
@@ -614,6 +700,7 @@ void CreateObstacles::operator()(double dt)
 // chi, surface points and centre of mass of every body where it is (set-up, and after an IC)
 void CreateObstacles::build()
 {
+  Region r(sim, "Obst CHI");
   for (size_t k = 0; k < sim.obstacles.size(); ++k) {
     auto& s = *sim.obstacles[k];
     double cm[4];
@@ -636,11 +723,14 @@ void ExternalForcing::operator()(double dt)
   const int dir = sim.BCy_flag == wall ? 1 : 2;
   const double H = sim.extent[dir];
   const double du[3] = {8 * sim.uMax_forced * sim.nu / H / H * dt, 0, 0};
+  Region r(sim, "Forcing Kernel");
   check(cup_add_uniform_velocity(sim.ctx, du), "ExternalForcing");
 }
 
+// The kernel and the copy back to u (AdvectionDiffusion.cpp:406-413) are one call here.
 void AdvectionDiffusion::operator()(double dt)
 {
+  Region r(sim, "AdvDiff Kernel");
   check(cup_advect_diffuse(sim.ctx, dt, sim.nu, sim.uinf.data(), sim.scheme), "AdvectionDiffusion");
 }
 
@@ -648,6 +738,7 @@ void AdvectionDiffusion::operator()(double dt)
 void SpectralForcing::operator()(double dt)
 {
   double st[5];
+  Region r(sim, "SpectralForcing");
   check(cup_spectral_forcing(sim.ctx, dt, sim.dt, sim.nu, sim.turbKinEn_target, sim.enInjectionRate, st),
         "SpectralForcing");
   const double totalKinEn = st[0], viscousDissip = st[1], largeModesKinEn = st[2];
@@ -670,15 +761,30 @@ void SpectralForcing::operator()(double dt)
   totalKinEnPrev = st[4] > 0 ? totalKinEn + dt * sim.actualInjectionRate : totalKinEn;
 }
 
-void PressureRHS::operator()(double dt) { check(cup_pressure_rhs(sim.ctx, dt), "PressureRHS"); }
+// The RHS and its obstacle correction (PressureRHS.cpp:285-345) are one call here.
+void PressureRHS::operator()(double dt)
+{
+  {
+    Region r(sim, "PresRHS Kernel");
+    check(cup_pressure_rhs(sim.ctx, dt), "PressureRHS");
+  }
+  check("PressureRHS");
+}
 
 // PressureProjection::operator() (PressureProjection.cpp:107-129).
 void PressureProjection::operator()(double dt)
 {
   // PressureProjection's solver (PressureProjection.cpp:86-105): the FFT solver when every face
   // is periodic (bUseFourierBC), PoissonSolverMixed otherwise
-  check(sim.bUseFourierBC ? cup_poisson_periodic(sim.ctx) : cup_poisson_mixed(sim.ctx), "PressureProjection solve");
-  check(cup_pressure_project(sim.ctx, dt), "PressureProjection gradient");
+  {
+    Region r(sim, sim.bUseFourierBC ? "FFTW solve" : "MFFTW solve");
+    check(sim.bUseFourierBC ? cup_poisson_periodic(sim.ctx) : cup_poisson_mixed(sim.ctx), "PressureProjection solve");
+  }
+  {
+    Region r(sim, "GradP");
+    check(cup_pressure_project(sim.ctx, dt), "PressureProjection gradient");
+  }
+  check("PressureProjection");
 }
 
 // UpdateObstacles (ObstaclesUpdate.cpp:202-236): the body's momentum integrals on the
@@ -690,8 +796,12 @@ void UpdateObstacles::operator()(double dt)
     auto& s = *sim.obstacles[k];
     if (s.isStatic() && sim.muteAll) continue;
     double M[29];
-    check(cup_obstacle_momenta_body(sim.ctx, (int)k, dt, sim.lambda, sim.bImplicitPenalization ? 1 : 0, M),
-          "UpdateObstacles");
+    {
+      Region r(sim, "Obst Int Vel");
+      check(cup_obstacle_momenta_body(sim.ctx, (int)k, dt, sim.lambda, sim.bImplicitPenalization ? 1 : 0, M),
+            "UpdateObstacles");
+    }
+    Region r(sim, "Obst Upd Vel");
     s.computeVelocities(M, sim.bImplicitPenalization, sim.time);
   }
 }
@@ -699,6 +809,7 @@ void UpdateObstacles::operator()(double dt)
 // Penalization::operator() (Penalization.cpp:129-156), body by body.
 void Penalization::operator()(double dt)
 {
+  Region r(sim, "Penalization");
   for (size_t k = 0; k < sim.obstacles.size(); ++k) {
     auto& s = *sim.obstacles[k];
     double ft[6];
@@ -714,6 +825,7 @@ void Penalization::operator()(double dt)
 // (Obstacle.cpp:518-545).
 void ComputeForces::operator()(double)
 {
+  Region r(sim, "Obst. Forces");
   for (size_t k = 0; k < sim.obstacles.size(); ++k) write((int)k);
 }
 
@@ -786,7 +898,11 @@ void ComputeDissipation::operator()(double)
 {
   if (sim.freqDiagnostics == 0 || sim.step % sim.freqDiagnostics) return;
   double R[20];
-  check(cup_diagnostics(sim.ctx, sim.nu, R), "ComputeDissipation");
+  {
+    Region r(sim, "Dissip Kernel");
+    check(cup_diagnostics(sim.ctx, sim.nu, R), "ComputeDissipation");
+  }
+  check("ComputeDissipation");
   for (int i = 0; i < 20; ++i) last[i] = R[i];
   if (sim.muteAll) return;
   std::stringstream& f = sim.logger->get_stream("diagnostics.dat");
@@ -816,7 +932,10 @@ void Analysis::operator()(double dt)
   }
   const int nb = std::max({sim.bpdx, sim.bpdy, sim.bpdz}) * 4 - 1;  // max(N)/2 - 1 shells
   spectrum.assign(nb, 0.0);
-  check(cup_hit_analysis(sim.ctx, sim.nu, last.data(), spectrum.data(), nb), "Analysis");
+  {
+    Region r(sim, "HIT Analysis");
+    check(cup_hit_analysis(sim.ctx, sim.nu, last.data(), spectrum.data(), nb), "Analysis");
+  }
   sim.grad_mean = last[7];
   sim.grad_std = last[8];
   ++files;
@@ -1126,7 +1245,10 @@ void Simulation::setupOperators()
 void Simulation::_ic()
 {
   InitialConditions ic(sim);
-  ic(0);
+  {
+    Region r(sim, ic.getName().c_str());
+    ic(0);
+  }
   // The zero and Taylor-Green ICs clear every field, chi included. A body that does not move
   // keeps its chi (the reference rebuilds it in every step's CreateObstacles), so it is built
   // again here.
@@ -1146,6 +1268,7 @@ void Simulation::reset()
 // Simulation::calcMaxTimestep (Simulation.cpp:403-424).
 double Simulation::calcMaxTimestep()
 {
+  Region r(sim, "DT");
   double dt = 0, umax = 0;
   const int rc =
       cup_calc_max_timestep(sim.ctx, sim.CFL, sim.nu, sim.uinf.data(), sim.step, sim.rampup, &dt, &umax);
@@ -1167,8 +1290,12 @@ bool Simulation::timestep(double dt)
   sim.step++;
   sim.time += dt;
   cup_set_step_state(sim.ctx, sim.step, sim.time);
-  if (sim.bDump) _serialize();
   if (sim.verbose) printf("%d : %f uInf {%f %f %f}\n", sim.step, sim.time, sim.uinf[0], sim.uinf[1], sim.uinf[2]);
+  {
+    Region r(sim, "Save");
+    if (sim.bDump) _serialize();
+  }
+  if (sim.step % 50 == 0 && sim.verbose) sim.printResetProfiler();
   if ((sim.endTime > 0 && sim.time > sim.endTime) || (sim.nsteps != 0 && sim.step >= sim.nsteps)) return true;
   return false;
 }
@@ -1237,8 +1364,13 @@ bool Simulation::advance(int n)
     if (cup_body_velocity(sim.ctx, (int)k, sim.obstacles[k]->transVel.data(), sim.obstacles[k]->angVel.data()) !=
         CUP_OK)
       throw std::runtime_error(std::string("advance: ") + cup_last_error(sim.ctx));
-  if (cup_run_steps(sim.ctx, &p, n) != CUP_OK)
-    throw std::runtime_error(std::string("advance: ") + cup_last_error(sim.ctx));
+  {
+    // the device-resident loop: its stages are roctx ranges of their own
+    Region r(sim, "Fused steps");
+    if (cup_run_steps(sim.ctx, &p, n) != CUP_OK)
+      throw std::runtime_error(std::string("advance: ") + cup_last_error(sim.ctx));
+  }
+  checkNaN(sim, "Fused steps");
   syncStepState();
   for (size_t k = 0; k < sim.obstacles.size(); ++k) {  // each body's last penalization force and surface QoIs
     auto& s = *sim.obstacles[k];
@@ -1445,6 +1577,16 @@ static int* sim_ifield(SimulationData& s, const std::string& n)
   return nullptr;
 }
 
+static bool* sim_bfield(SimulationData& s, const std::string& n)
+{
+  if (n == "verbose") return &s.verbose;
+  if (n == "muteAll") return &s.muteAll;
+  if (n == "bCheckNaN") return &s.bCheckNaN;
+  if (n == "bImplicitPenalization") return &s.bImplicitPenalization;
+  if (n == "spectralForcing") return &s.spectralForcing;
+  return nullptr;
+}
+
 int cup_sim_get(cup_sim* h, const char* name, double* v)
 {
   return guarded(h, [&] {
@@ -1452,6 +1594,7 @@ int cup_sim_get(cup_sim* h, const char* name, double* v)
     if (!s || !name || !v) throw std::invalid_argument("bad argument");
     if (double* p = sim_field(*s, name)) *v = *p;
     else if (int* q = sim_ifield(*s, name)) *v = *q;
+    else if (bool* b = sim_bfield(*s, name)) *v = *b ? 1 : 0;
     else throw std::invalid_argument(std::string("unknown member ") + name);
   });
 }
@@ -1466,6 +1609,7 @@ int cup_sim_set(cup_sim* h, const char* name, double v)
       throw std::invalid_argument(n + " is fixed once the simulation exists");
     if (double* p = sim_field(*s, n)) *p = v;
     else if (int* q = sim_ifield(*s, n)) *q = (int)v;
+    else if (bool* b = sim_bfield(*s, n)) *b = v != 0;
     else throw std::invalid_argument("unknown member " + n);
     if (h->S && n == "step") cup_set_step_state(s->ctx, s->step, s->time);
     if (h->S && n == "time") cup_set_step_state(s->ctx, s->step, s->time);

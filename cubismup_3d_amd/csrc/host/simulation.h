@@ -6,7 +6,9 @@
 #pragma once
 
 #include <array>
+#include <chrono>
 #include <cstdint>
+#include <cstdio>
 #include <map>
 #include <memory>
 #include <sstream>
@@ -55,6 +57,32 @@ class ArgumentParser
 
  private:
   std::map<std::string, std::string> args_;
+};
+
+// cubism::Profiler as SimulationData drives it (SimulationData.cpp:293-305): nested named
+// regions opened by push_start and closed by pop_stop. Every region is a roctx range, so a
+// rocprofv3 run with --marker-trace files each kernel under the region that launched it. With
+// timing on, a region also adds its wall-clock duration to its name's total; both ends wait
+// for the context's stream first, so the time is that of the device work the region issued
+// (the reference's regions time synchronous host code the same way).
+class Profiler
+{
+ public:
+  struct Agent {
+    double seconds = 0;
+    long count = 0;
+  };
+  bool timing = false;
+  void push_start(const std::string& name, cup_ctx* ctx);
+  void pop_stop(cup_ctx* ctx);
+  // a table of the regions by total time: name, total and mean milliseconds, share, count
+  void printSummary(FILE* out = stdout) const;
+  void reset();
+  const std::map<std::string, Agent>& agents() const { return agents_; }
+
+ private:
+  std::map<std::string, Agent> agents_;
+  std::vector<std::pair<std::string, std::chrono::steady_clock::time_point>> stack_;
 };
 
 // BufferedLogger (utils/BufferedLogger.cpp:18-76): per-file string streams appended to
@@ -165,6 +193,9 @@ struct SimulationData {
   int freqAnalysis = 0;
   double grad_mean = 0, grad_std = 0, nu_sgs = 0, cs2_avg = 0;
   bool verbose = false, muteAll = false;
+  // Operator::check (Operator.h:23-50) after PressureRHS, PressureProjection and
+  // ComputeDissipation; the reference compiles it into debug builds, here -checkNaN 1 turns it on
+  bool bCheckNaN = false;
   std::string path4serialization = "./";
   BCflag BCx_flag = dirichlet, BCy_flag = dirichlet, BCz_flag = dirichlet;
   bool bUseFourierBC = false;
@@ -179,6 +210,7 @@ struct SimulationData {
   std::vector<std::shared_ptr<Sphere>> obstacles;
   std::vector<std::unique_ptr<Operator>> pipeline;
   std::shared_ptr<BufferedLogger> logger;
+  std::shared_ptr<Profiler> profiler;  // timing on for verbose runs
 
   SimulationData();
   SimulationData(int argc, const char* const* argv);
@@ -188,6 +220,10 @@ struct SimulationData {
   void _preprocessArguments();
   void setCells(int nx, int ny, int nz);
   Real uniformH() const { return hmean; }
+  // SimulationData::startProfiler / stopProfiler / printResetProfiler (SimulationData.cpp:293-305)
+  void startProfiler(const std::string& name) const;
+  void stopProfiler() const;
+  void printResetProfiler();
 };
 
 class Operator
@@ -195,6 +231,10 @@ class Operator
  protected:
   SimulationData& sim;
   void check(int rc, const char* what);
+  // Operator::check (Operator.h:23-50) when sim.bCheckNaN: a NaN in u, v, w or p prints
+  // "GenericCoordinator::check isnan <infoText>" and throws std::runtime_error with that text
+  // (the reference aborts the MPI job).
+  void check(const std::string& infoText);
 
  public:
   explicit Operator(SimulationData& s) : sim(s) {}

@@ -211,6 +211,12 @@ class Grid:
         self._c(L.lib().cup_diagnostics(self._h, float(nu), L.dptr(out)))
         return out
 
+    def check_nan(self) -> np.ndarray:
+        """Operator::check's scan (Operator.h:23-50): the NaN cells of u, v, w, p."""
+        out = np.zeros(4, dtype=np.int64)
+        self._c(L.lib().cup_check_nan(self._h, out.ctypes.data_as(C.POINTER(C.c_longlong))))
+        return out
+
     def hit_analysis(self, nu):
         """The HIT analysis of Analysis.cpp (gradient statistics and the spectral analysis of
         SpectralManipFFTW::_compute_analysis): dict of tke, eps, uprime, lambda, Re_lambda,

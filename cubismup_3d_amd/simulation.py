@@ -14,7 +14,7 @@ import ctypes as C
 import math
 
 from . import _lib as L
-from .grid import Grid, from_blocks
+from .grid import Grid, from_blocks, to_blocks
 
 
 def read_field_file(path):
@@ -133,6 +133,7 @@ class SimulationData:
     (SimulationData.cpp:32-172). ``argv`` takes precedence over keyword settings."""
 
     _INT = {"nsteps", "rampup", "freqDiagnostics", "freqAnalysis", "bpdx", "bpdy", "bpdz"}
+    _BOOL = {"verbose", "muteAll", "bCheckNaN", "bImplicitPenalization", "spectralForcing"}
 
     def __init__(self, cells=None, CFL=None, uinf=(0.0, 0.0, 0.0), argv=None, **kw):
         if argv is None:
@@ -167,6 +168,8 @@ class SimulationData:
         rc = L.lib().cup_sim_get(self.__dict__["_h"], name.encode(), C.byref(v))
         if rc != L.CUP_OK:
             raise AttributeError(name)
+        if name in self._BOOL:
+            return bool(v.value)
         return int(v.value) if name in self._INT or name == "step" else v.value
 
     def __setattr__(self, name, value):
@@ -229,6 +232,15 @@ class Simulation:
         b = np.empty(int(np.prod(self.shape)), dtype=np.float64)
         L.check(L.lib().cup_get_field(self._ctx, L.FIELD_IDS[name], L.dptr(b)), ctx=self._ctx)
         return from_blocks(b, self.shape)
+
+    def set_field(self, name, g):
+        """Write a [z][y][x] field (chi, u, v, w, p) into the simulation's device context."""
+        import numpy as np
+        g = np.asarray(g, dtype=np.float64)
+        if g.shape != self.shape:
+            raise ValueError(f"field shape {g.shape} != grid {self.shape}")
+        b = to_blocks(g)
+        L.check(L.lib().cup_set_field(self._ctx, L.FIELD_IDS[name], L.dptr(b)), ctx=self._ctx)
 
     def obstacle(self, idx=0):
         import numpy as np

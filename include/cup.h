@@ -235,6 +235,13 @@ int cup_diagnostics(cup_ctx* ctx, double nu, double out[20]);
 int cup_diagnostics_noalloc(const cup_grid_desc* desc, const void* const fields[5], double nu, void* hip_stream,
                             double* out_dev);
 
+/* Operator::check (operators/Operator.h:23-50): the scan for NaN in u, v, w and p that the
+ * reference's debug builds run after PressureRHS, PressureProjection and ComputeDissipation.
+ * counts[4] receives the number of NaN cells of u, v, w and p (cup_field's velocity and
+ * pressure, whatever the last operation left there); infinities are not counted, as isnan
+ * does not count them. Synchronizes the context's stream. */
+int cup_check_nan(cup_ctx* ctx, long long counts[4]);
+
 /* The HIT branch of Analysis::operator() (Analysis.cpp:159-196): KernelAnalysis_gradStats
  * (:18-52) and SpectralAnalysis::run, i.e. SpectralManipFFTW::_compute_analysis
  * (SpectralManipFFTW.cpp:110-207) on the velocity's forward FFT. out[9] = tke, eps (viscous

@@ -97,6 +97,13 @@ def test_reference_defaults_and_flags(cup):
     assert (sd3.freqAnalysis, sd3.timeAnalysis) == (5, 0.25)
     with pytest.raises(AttributeError):
         sd2.no_such_member
+    # Operator::check's NaN scan: off unless -checkNaN; verbose (profiler timing) off by default
+    assert (sd.bCheckNaN, sd.verbose) == (False, False)
+    sd4 = cup.SimulationData(argv=["-bpdx", "2", "-bpdy", "2", "-bpdz", "2", "-nu", "0.01", "-checkNaN", "1",
+                                   "-verbose", "1"])
+    assert (sd4.bCheckNaN, sd4.verbose) == (True, True)
+    sd4.bCheckNaN = False
+    assert sd4.bCheckNaN is False
 
 
 def test_sphere_arguments(cup):
