@@ -103,6 +103,44 @@ def cpu_baseline(n, warm=2, timed=5):
                       f"{warm} warm-up + {timed} timed steps ({el:.1f} s)"}
 
 
+def c1_rows(cup, with_cpu):
+    """C1, the reference's CPU-runnable case: the 64^3 Taylor-Green vortex (the TGV IC with
+    uMax = -1), FP64 plain, nu = 0.01, fixed dt = 1e-3, no forcing, all 20 steps timed, on the
+    GPU in the FP64 plain and FP32 fused forms and, with the CPU baseline, on the host cores (the
+    oracle)."""
+    n, steps, nu, dt = 64, 20, 0.01, 1e-3
+    rows = {"workload": "Taylor-Green vortex, periodic 2pi box 64^3, nu=0.01, dt=1e-3, 20 steps", "cells": n ** 3,
+            "steps": steps}
+    for key, prec, dec in (("gpu_fp64_plain", 8, "plain"), ("gpu_fp32_fused", 4, "fused")):
+        G = cup.Grid((n // 8,) * 3, precision=prec, decomposition=dec)
+        G.ic_taylor_green(-1.0)
+        G.run_steps(2, nu=nu, fixed_dt=dt)  # warm-up: first launches
+        G.ic_taylor_green(-1.0)
+        G.synchronize()
+        t0 = time.perf_counter()
+        G.run_steps(steps, nu=nu, fixed_dt=dt)
+        G.synchronize()
+        el = time.perf_counter() - t0
+        rows[key] = {"value": round(n ** 3 * steps / el / 1e6, 1), "unit": "Mcell-steps/s",
+                     "ms_per_step": round(el / steps * 1e3, 4)}
+        del G
+    if with_cpu:
+        from oracle import oracle as O
+        O.build()
+        threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+        O.lib().orc_set_threads(threads)
+        s = O.OracleSim((n // 8,) * 3)
+        s.ic_taylor_green(-1.0)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            s.timestep(dt, nu)
+        el = time.perf_counter() - t0
+        rows["cpu_fp64_oracle"] = {"value": round(n ** 3 * steps / el / 1e6, 2), "unit": "Mcell-steps/s",
+                                   "ms_per_step": round(el / steps * 1e3, 3), "cores": threads, "kind": "port",
+                                   "cpu": cpu_model()}
+    return rows
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,6 +150,7 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=20, help="timed steps of the FP64 plain form")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--no-c1", action="store_true", help="skip the 64^3 Taylor-Green rows")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -204,6 +243,8 @@ def main():
     if not args.no_cpu_baseline:
         cb = cpu_baseline(n, 2, args.cpu_steps)
         out["cpu_baseline"] = cb
+    if not args.no_c1:
+        out["c1_tgv64"] = c1_rows(cup, not args.no_cpu_baseline)
     print(json.dumps(out), flush=True)
 
 

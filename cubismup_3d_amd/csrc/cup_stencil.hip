@@ -148,7 +148,9 @@ struct LabRegs {
   T hv[NF][2];
 };
 
-template <typename T, int NF>
+// BC: the grid has a non-periodic axis (compiled separately, so periodic grids run none of
+// the domain-face tests).
+template <bool BC, typename T, int NF>
 __device__ __forceinline__ void lab_fetch(LabRegs<T, NF>& r, const T* const* src, const LabSlots& s,
                                           const GridArgs& g, int b, const int (&kinds)[NF])
 {
@@ -166,7 +168,13 @@ __device__ __forceinline__ void lab_fetch(LabRegs<T, NF>& r, const T* const* src
     if (k == 0 || has_halo2()) {
       // a halo slot's face is the same across its wave: the neighbour block is scalar work
       const int face = __builtin_amdgcn_readfirstlane(s.face[k]);
-      const int bf = g.bcany ? domain_face(g, bx, by, bz, face) : 0;
+      if (!BC) {
+        const size_t q = (size_t)face_block(g, bx, by, bz, face) * BS3 + s.off[k];
+#pragma unroll
+        for (int f = 0; f < NF; ++f) r.hv[f][k] = src[f][q];
+        continue;
+      }
+      const int bf = domain_face(g, bx, by, bz, face);
       const size_t q = bf ? (size_t)b * BS3 + s.off_bc[k] : (size_t)face_block(g, bx, by, bz, face) * BS3 + s.off[k];
 #pragma unroll
       for (int f = 0; f < NF; ++f) r.hv[f][k] = halo_value(src[f][q], kinds[f], bf);
@@ -346,7 +354,7 @@ int launch_reset_umax(Ctx& c)
 // through PD + 1 slots; the loop unrolls fully, so the slot indices are constants).
 // fade[face]: the advection's inflow fade of an open domain face (1: none), applied to the
 // velocities of that face's halo (AdvectionDiffusion.cpp:39-104).
-template <typename T, int NF>
+template <bool BC, typename T, int NF>
 __device__ __forceinline__ void lab_fetch_xw(LabRegs<T, NF>& r, const T* const* src, const LabSlots& s,
                                              const GridArgs& g, int b, bool x_lo, bool x_hi, const T (&fade)[6])
 {
@@ -364,7 +372,13 @@ __device__ __forceinline__ void lab_fetch_xw(LabRegs<T, NF>& r, const T* const* 
     if (k == 0 || has_halo2()) {
       const int face = __builtin_amdgcn_readfirstlane(s.face[k]);
       if ((face == 0 && !x_lo) || (face == 1 && !x_hi)) continue;
-      const int bf = g.bcany ? domain_face(g, bx, by, bz, face) : 0;
+      if (!BC) {
+        const size_t q = (size_t)face_block(g, bx, by, bz, face) * BS3 + s.off[k];
+#pragma unroll
+        for (int f = 0; f < NF; ++f) r.hv[f][k] = src[f][q];
+        continue;
+      }
+      const int bf = domain_face(g, bx, by, bz, face);
       const size_t q = bf ? (size_t)b * BS3 + s.off_bc[k] : (size_t)face_block(g, bx, by, bz, face) * BS3 + s.off[k];
       const T fd = bf == CUP_BC_OPEN ? fade[face] : (T)1;
 #pragma unroll
@@ -397,12 +411,27 @@ __device__ __forceinline__ void lab_put_xw(const LabRegs<T, NF>& r, const LabReg
     for (int f = 0; f < NF; ++f) lab[f][s.li + 8] = next.in[f][0];
 }
 
+// First block of logical walk L along x (NBW blocks of a row of blocks). Rows of blocks
+// (by, bz) are ordered in tiles of ZT planes of blocks with bz fastest, so that a walk's z
+// neighbours run a few walks earlier or later (on the same XCD, whose consecutive logical ids
+// xcd_remap keeps together) and its y and z halo faces are still in L2.
+template <int NBW>
+__device__ __forceinline__ int xwalk_first(const GridArgs& g, int L)
+{
+  const int nq = g.nbx / NBW;  // walks per row of blocks
+  const int xq = L % nq, row = L / nq;
+  const int zt = g.nbz % 4 == 0 ? 4 : (g.nbz % 2 == 0 ? 2 : 1);
+  const int tile = row / (g.nby * zt), within = row - tile * (g.nby * zt);
+  const int by = within / zt, bz = tile * zt + (within - (within / zt) * zt);
+  return xq * NBW + g.nbx * (by + g.nby * bz);
+}
+
 template <typename T>
 struct Fade6 {
   T f[6];
 };
 
-template <typename T, int NBW, int PD>
+template <typename T, int NBW, int PD, bool BC>
 __global__ __launch_bounds__(NT) void k_advdiff_xwalk(const T* __restrict__ u, const T* __restrict__ v,
                                                       const T* __restrict__ w, T* __restrict__ ou,
                                                       T* __restrict__ ov, T* __restrict__ ow, GridArgs g,
@@ -412,11 +441,11 @@ __global__ __launch_bounds__(NT) void k_advdiff_xwalk(const T* __restrict__ u, c
   __shared__ __align__(16) T lab[3][PLC];
   const LabSlots s = lab_slots();
   const T* src[3] = {u, v, w};
-  const int b0 = xcd_remap(blockIdx.x, gridDim.x) * NBW;
+  const int b0 = xwalk_first<NBW>(g, xcd_remap(blockIdx.x, gridDim.x));
   constexpr int NS = PD + 1;
   LabRegs<T, 3> r[NS];
 #pragma unroll
-  for (int p = 0; p < PD && p < NBW; ++p) lab_fetch_xw(r[p], src, s, g, b0 + p, p == 0, p == NBW - 1, fd.f);
+  for (int p = 0; p < PD && p < NBW; ++p) lab_fetch_xw<BC>(r[p], src, s, g, b0 + p, p == 0, p == NBW - 1, fd.f);
   double dt;
   if (fold.slots) {
     // the previous step's finalize: no workgroup reads the scalars workgroup 0 updates
@@ -446,7 +475,7 @@ __global__ __launch_bounds__(NT) void k_advdiff_xwalk(const T* __restrict__ u, c
     if (j) __syncthreads();  // the previous block's compute is done with the lab
     lab_put_xw(r[j % NS], r[(j + NS - 1) % NS], r[(j + 1) % NS], lab, s, j == 0, j == NBW - 1);
     __syncthreads();
-    if (j + PD < NBW) lab_fetch_xw(r[(j + PD) % NS], src, s, g, b0 + j + PD, false, j + PD == NBW - 1, fd.f);
+    if (j + PD < NBW) lab_fetch_xw<BC>(r[(j + PD) % NS], src, s, g, b0 + j + PD, false, j + PD == NBW - 1, fd.f);
     T av[3][2];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
@@ -485,10 +514,16 @@ __global__ __launch_bounds__(NT) void k_advdiff_xwalk(const T* __restrict__ u, c
 template <typename T, int NBW>
 static void launch_advdiff_xwalk(Ctx& c, double nu, const double uinf[3], const StepFold& fold, const Fade6<T>& fd)
 {
-  hipLaunchKernelGGL((k_advdiff_xwalk<T, NBW, ADV_PD>), dim3(c.nb / NBW), dim3(NT), 0, c.stream,
-                     (const T*)c.f[CUP_U], (const T*)c.f[CUP_V], (const T*)c.f[CUP_W], (T*)c.f[CUP_TMPU],
-                     (T*)c.f[CUP_TMPV], (T*)c.f[CUP_TMPW], c.grid(), c.sc, (T)uinf[0], (T)uinf[1], (T)uinf[2], nu,
-                     fold, fd);
+  if (c.periodic())
+    hipLaunchKernelGGL((k_advdiff_xwalk<T, NBW, ADV_PD, false>), dim3(c.nb / NBW), dim3(NT), 0, c.stream,
+                       (const T*)c.f[CUP_U], (const T*)c.f[CUP_V], (const T*)c.f[CUP_W], (T*)c.f[CUP_TMPU],
+                       (T*)c.f[CUP_TMPV], (T*)c.f[CUP_TMPW], c.grid(), c.sc, (T)uinf[0], (T)uinf[1], (T)uinf[2], nu,
+                       fold, fd);
+  else
+    hipLaunchKernelGGL((k_advdiff_xwalk<T, NBW, ADV_PD, true>), dim3(c.nb / NBW), dim3(NT), 0, c.stream,
+                       (const T*)c.f[CUP_U], (const T*)c.f[CUP_V], (const T*)c.f[CUP_W], (T*)c.f[CUP_TMPU],
+                       (T*)c.f[CUP_TMPV], (T*)c.f[CUP_TMPW], c.grid(), c.sc, (T)uinf[0], (T)uinf[1], (T)uinf[2], nu,
+                       fold, fd);
 }
 
 template <typename T>
@@ -895,7 +930,7 @@ int launch_rhs_plain(Ctx& c)
 // block's 64 rows x 16 coefficients are staged in LDS by one coalesced 4-value load per
 // thread. A thread's x is the same in every block of its walk, so its e^{-i a x} and
 // e^{-2 i a x} are formed once.
-template <typename T, bool FUSED, bool FORCE>
+template <typename T, bool FUSED, bool FORCE, bool BC>
 __global__ __launch_bounds__(NT) void k_gradp(const T* __restrict__ p, const T* __restrict__ su,
                                               const T* __restrict__ sv, const T* __restrict__ sw, T* __restrict__ u,
                                               T* __restrict__ v, T* __restrict__ w, GridArgs g,
@@ -919,7 +954,7 @@ __global__ __launch_bounds__(NT) void k_gradp(const T* __restrict__ p, const T* 
   T us[3][2];
   T rr[4];
   auto fetch = [&](int b) {
-    lab_fetch(r, src, s, g, b, {HK_COPY});
+    lab_fetch<BC>(r, src, s, g, b, {HK_COPY});
     const size_t o = (size_t)b * BS3 + c0;
     const V2 a = *reinterpret_cast<const V2*>(su + o);
     const V2 bb = *reinterpret_cast<const V2*>(sv + o);
@@ -1022,17 +1057,22 @@ static int gradp_t(Ctx& c, bool fused, const double uinf[3], const int* skip, bo
   const int nbw = blocks_per_wg(c.bpd[2]);
   const dim3 grid(c.nb / nbw);
   if (fused && force)
-    hipLaunchKernelGGL((k_gradp<T, true, true>), grid, dim3(NT), 0, c.stream, p, (const T*)c.f[s],
+    hipLaunchKernelGGL((k_gradp<T, true, true, false>), grid, dim3(NT), 0, c.stream, p, (const T*)c.f[s],
                        (const T*)c.f[s + 1], (const T*)c.f[s + 2], (T*)c.f[CUP_U], (T*)c.f[CUP_V], (T*)c.f[CUP_W],
                        c.grid(), c.sc, uinf[0], uinf[1], uinf[2], c.ke_part, skip, (const T*)c.low_rows, tx, nbw,
                        umax_slots_dev(c, c.umax_bank), umax_slots_dev(c, c.umax_bank ^ 1));
   else if (fused)
-    hipLaunchKernelGGL((k_gradp<T, true, false>), grid, dim3(NT), 0, c.stream, p, (const T*)c.f[s],
+    hipLaunchKernelGGL((k_gradp<T, true, false, false>), grid, dim3(NT), 0, c.stream, p, (const T*)c.f[s],
                        (const T*)c.f[s + 1], (const T*)c.f[s + 2], (T*)c.f[CUP_U], (T*)c.f[CUP_V], (T*)c.f[CUP_W],
                        c.grid(), c.sc, uinf[0], uinf[1], uinf[2], c.ke_part, skip, (const T*)nullptr, tx, nbw,
                        umax_slots_dev(c, c.umax_bank), umax_slots_dev(c, c.umax_bank ^ 1));
+  else if (c.periodic())
+    hipLaunchKernelGGL((k_gradp<T, false, false, false>), grid, dim3(NT), 0, c.stream, p, (const T*)c.f[s],
+                       (const T*)c.f[s + 1], (const T*)c.f[s + 2], (T*)c.f[CUP_U], (T*)c.f[CUP_V], (T*)c.f[CUP_W],
+                       c.grid(), c.sc, 0.0, 0.0, 0.0, (double*)nullptr, (const int*)nullptr, (const T*)nullptr, tx,
+                       nbw, (unsigned long long*)nullptr, (unsigned long long*)nullptr);
   else
-    hipLaunchKernelGGL((k_gradp<T, false, false>), grid, dim3(NT), 0, c.stream, p, (const T*)c.f[s],
+    hipLaunchKernelGGL((k_gradp<T, false, false, true>), grid, dim3(NT), 0, c.stream, p, (const T*)c.f[s],
                        (const T*)c.f[s + 1], (const T*)c.f[s + 2], (T*)c.f[CUP_U], (T*)c.f[CUP_V], (T*)c.f[CUP_W],
                        c.grid(), c.sc, 0.0, 0.0, 0.0, (double*)nullptr, (const int*)nullptr, (const T*)nullptr, tx,
                        nbw, (unsigned long long*)nullptr, (unsigned long long*)nullptr);
@@ -1047,13 +1087,13 @@ int launch_gradp_inplace(Ctx& c, int field0)
   const dim3 grid(c.nb / nbw);
   if (c.precision == 4) {
     float* f[3] = {(float*)c.f[field0], (float*)c.f[field0 + 1], (float*)c.f[field0 + 2]};
-    hipLaunchKernelGGL((k_gradp<float, false, false>), grid, dim3(NT), 0, c.stream, (const float*)c.f[CUP_P], f[0],
+    hipLaunchKernelGGL((k_gradp<float, false, false, false>), grid, dim3(NT), 0, c.stream, (const float*)c.f[CUP_P], f[0],
                        f[1], f[2], f[0], f[1], f[2], c.grid(), c.sc, 0.0, 0.0, 0.0, (double*)nullptr,
                        (const int*)nullptr, (const float*)nullptr, (const C2<float>*)c.tw[0], nbw,
                        (unsigned long long*)nullptr, (unsigned long long*)nullptr);
   } else {
     double* f[3] = {(double*)c.f[field0], (double*)c.f[field0 + 1], (double*)c.f[field0 + 2]};
-    hipLaunchKernelGGL((k_gradp<double, false, false>), grid, dim3(NT), 0, c.stream, (const double*)c.f[CUP_P],
+    hipLaunchKernelGGL((k_gradp<double, false, false, false>), grid, dim3(NT), 0, c.stream, (const double*)c.f[CUP_P],
                        f[0], f[1], f[2], f[0], f[1], f[2], c.grid(), c.sc, 0.0, 0.0, 0.0, (double*)nullptr,
                        (const int*)nullptr, (const double*)nullptr, (const C2<double>*)c.tw[0], nbw,
                        (unsigned long long*)nullptr, (unsigned long long*)nullptr);
