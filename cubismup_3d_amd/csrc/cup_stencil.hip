@@ -442,6 +442,7 @@ __global__ __launch_bounds__(NT) void k_advdiff_xwalk(const T* __restrict__ u, c
   const LabSlots s = lab_slots();
   const T* src[3] = {u, v, w};
   const int b0 = xwalk_first<NBW>(g, xcd_remap(blockIdx.x, gridDim.x));
+  static_assert(PD >= 2, "a block's x+ halo comes from the next block, which must be in registers");
   constexpr int NS = PD + 1;
   LabRegs<T, 3> r[NS];
 #pragma unroll
@@ -650,14 +651,218 @@ __global__ __launch_bounds__(NT) void k_advdiff_up3(const T* __restrict__ su, co
   }
 }
 
+// The same stage as a walk of NBW blocks along x with PD blocks in flight, as the central
+// kernel walks: the lab pads x to PX = 12 (interior at x = 2..9, two halo layers each side),
+// y and z to L2 = 12. A block's x halos are the neighbouring blocks' ix = 6, 7 and 0, 1 planes,
+// held in registers by the threads t % 4 == 3 and t % 4 == 0; the walk's two end faces (two
+// layers, 256 cells: one per thread) and the four y/z faces (two layers, 512 cells: two per
+// thread, one face and layer per wave) are gathered. Each thread's cell pair and all its
+// stencil neighbours are 8-byte aligned pairs in LDS (one 64-bit access each). SEP: base is
+// not src (read at the cells one block ahead); else base is the lab's own centre.
+constexpr int U3Z = PX * L2;       // z stride of the walk's lab
+constexpr int U3C = PX * L2 * L2;  // cells per field
+struct Up3Slots {
+  int li;       // LDS index of interior cell 2t
+  int l[2];     // the thread's two y/z face cells: LDS index,
+  int off[2];   // offset inside the neighbour block,
+  int face[2];  // face 2..5
+  int lx, offx, facex;  // its walk-end x face cell (face 0 or 1)
+};
+
+__device__ __forceinline__ Up3Slots up3_slots()
+{
+  Up3Slots s;
+  const int t = threadIdx.x, c0 = 2 * t;
+  s.li = (((c0 >> 6) + 2) * L2 + ((c0 >> 3) & 7) + 2) * PX + (c0 & 7) + 2;
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int c = t + NT * k;  // 0..511: face 2 + (c >> 7), layer (c >> 6) & 1
+    const int face = 2 + (c >> 7), layer = (c >> 6) & 1, a = (c >> 3) & 7, e = c & 7;
+    const bool plus = face & 1;
+    const int sd = plus ? layer : 6 + layer, ld = plus ? 10 + layer : layer;
+    int sx = e, sy, sz, ly, lz;
+    if (face < 4) { sy = sd; sz = a; ly = ld; lz = a + 2; }
+    else { sy = a; sz = sd; ly = a + 2; lz = ld; }
+    s.face[k] = face;
+    s.off[k] = (sz * BS + sy) * BS + sx;
+    s.l[k] = (lz * L2 + ly) * PX + e + 2;
+  }
+  {
+    const int face = t >> 7, layer = (t >> 6) & 1, a = (t >> 3) & 7, e = t & 7;
+    const bool plus = face & 1;
+    const int sd = plus ? layer : 6 + layer, ld = plus ? 10 + layer : layer;
+    s.facex = face;
+    s.offx = (a * BS + e) * BS + sd;
+    s.lx = ((a + 2) * L2 + e + 2) * PX + ld;
+  }
+  return s;
+}
+
+template <typename T>
+struct Up3Regs {
+  T in[3][2];
+  T hv[3][2];
+  T hx[3];
+};
+
+template <typename T>
+__device__ __forceinline__ void up3_fetch(Up3Regs<T>& r, const T* const* src, const Up3Slots& s, const GridArgs& g,
+                                          int b, bool x_lo, bool x_hi)
+{
+  using V2 = typename Vec2<T>::type;
+  int bx, by, bz;
+  block_coords(g, b, bx, by, bz);
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    const V2 a = *reinterpret_cast<const V2*>(src[f] + (size_t)b * BS3 + 2 * threadIdx.x);
+    r.in[f][0] = a.x;
+    r.in[f][1] = a.y;
+  }
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int face = __builtin_amdgcn_readfirstlane(s.face[k]);  // one face per wave
+    const size_t q = (size_t)face_block(g, bx, by, bz, face) * BS3 + s.off[k];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) r.hv[f][k] = src[f][q];
+  }
+  const int fx = __builtin_amdgcn_readfirstlane(s.facex);
+  if ((fx == 0 && x_lo) || (fx == 1 && x_hi)) {
+    const size_t q = (size_t)face_block(g, bx, by, bz, fx) * BS3 + s.offx;
+#pragma unroll
+    for (int f = 0; f < 3; ++f) r.hx[f] = src[f][q];
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void up3_put(const Up3Regs<T>& r, const Up3Regs<T>& prev, const Up3Regs<T>& next,
+                                        T (*lab)[U3C], const Up3Slots& s, bool x_lo, bool x_hi)
+{
+  using V2 = typename Vec2<T>::type;
+#pragma unroll
+  for (int f = 0; f < 3; ++f) *reinterpret_cast<V2*>(&lab[f][s.li]) = V2{r.in[f][0], r.in[f][1]};
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+#pragma unroll
+    for (int f = 0; f < 3; ++f) lab[f][s.l[k]] = r.hv[f][k];
+  const int fx = __builtin_amdgcn_readfirstlane(s.facex);
+  if ((fx == 0 && x_lo) || (fx == 1 && x_hi))
+#pragma unroll
+    for (int f = 0; f < 3; ++f) lab[f][s.lx] = r.hx[f];
+  const int t4 = threadIdx.x & 3;
+  if (!x_lo && t4 == 3)  // the previous block's ix = 6, 7 at x = 0, 1
+#pragma unroll
+    for (int f = 0; f < 3; ++f) *reinterpret_cast<V2*>(&lab[f][s.li - 8]) = V2{prev.in[f][0], prev.in[f][1]};
+  if (!x_hi && t4 == 0)  // the next block's ix = 0, 1 at x = 10, 11
+#pragma unroll
+    for (int f = 0; f < 3; ++f) *reinterpret_cast<V2*>(&lab[f][s.li + 8]) = V2{next.in[f][0], next.in[f][1]};
+}
+
+// (Without SEP the walk asks for five waves per SIMD, which costs it a few bytes of spill.)
+template <typename T, int NBW, int PD, bool SEP>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SEP ? 1 : 5))) void k_advdiff_up3_xwalk(const T* __restrict__ su, const T* __restrict__ sv,
+                                                          const T* __restrict__ sw, const T* bu, const T* bv,
+                                                          const T* bw, T* ou, T* ov, T* ow, GridArgs g,
+                                                          const DevScalars* __restrict__ sc, T Ux, T Uy, T Uz,
+                                                          double nu, double mult)
+{
+  using V2 = typename Vec2<T>::type;
+  __shared__ __align__(16) T lab[3][U3C];
+  __shared__ __align__(16) T bl[SEP ? 3 : 1][BS3];  // the block's base (SEP)
+  const Up3Slots s = up3_slots();
+  const T* src[3] = {su, sv, sw};
+  const T* base[3] = {bu, bv, bw};
+  const int b0 = xwalk_first<NBW>(g, xcd_remap(blockIdx.x, gridDim.x));
+  static_assert(PD >= 2, "a block's x+ halo comes from the next block, which must be in registers");
+  constexpr int NS = PD + 1;
+  Up3Regs<T> r[NS];
+#pragma unroll
+  for (int p = 0; p < PD && p < NBW; ++p) up3_fetch(r[p], src, s, g, b0 + p, p == 0, p == NBW - 1);
+  const double fac = mult * sc->dt;
+  const T facA = (T)(-fac / (6 * g.h));
+  const T facD = (T)(fac * nu / (g.h * g.h));
+  const int l = s.li;
+  // base at the cells (SEP), loaded one block ahead and staged in LDS with the lab
+  V2 bnext[SEP ? 3 : 1];
+  if (SEP)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) bnext[q] = *reinterpret_cast<const V2*>(base[q] + (size_t)b0 * BS3 + 2 * threadIdx.x);
+#pragma unroll
+  for (int j = 0; j < NBW; ++j) {
+    if (j) __syncthreads();  // the previous block's compute is done with the lab
+    up3_put(r[j % NS], r[(j + NS - 1) % NS], r[(j + 1) % NS], lab, s, j == 0, j == NBW - 1);
+    if (SEP)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) *reinterpret_cast<V2*>(&bl[SEP ? q : 0][2 * threadIdx.x]) = bnext[q];
+    __syncthreads();
+    if (j + PD < NBW) up3_fetch(r[(j + PD) % NS], src, s, g, b0 + j + PD, false, j + PD == NBW - 1);
+    const size_t o = (size_t)(b0 + j) * BS3 + 2 * threadIdx.x;
+    if (SEP && j + 1 < NBW)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) bnext[q] = *reinterpret_cast<const V2*>(base[q] + o + BS3);
+    T a0[2], a1[2], a2[2];
+    {
+      const V2 A = *reinterpret_cast<const V2*>(&lab[0][l]), B = *reinterpret_cast<const V2*>(&lab[1][l]),
+               C = *reinterpret_cast<const V2*>(&lab[2][l]);
+      a0[0] = A.x + Ux; a0[1] = A.y + Ux;
+      a1[0] = B.x + Uy; a1[1] = B.y + Uy;
+      a2[0] = C.x + Uz; a2[1] = C.y + Uz;
+    }
+    // one field at a time (not unrolled: the stencil's 22 values per field stay in registers
+    // once, which keeps the walk's in-flight blocks within the register budget)
+#pragma unroll 1
+    for (int q = 0; q < 3; ++q) {
+      const T* F = lab[q];
+      const V2 xm = *reinterpret_cast<const V2*>(F + l - 2), xc = *reinterpret_cast<const V2*>(F + l),
+               xp = *reinterpret_cast<const V2*>(F + l + 2);
+      const V2 ym2 = *reinterpret_cast<const V2*>(F + l - 2 * PX), ym1 = *reinterpret_cast<const V2*>(F + l - PX),
+               yp1 = *reinterpret_cast<const V2*>(F + l + PX), yp2 = *reinterpret_cast<const V2*>(F + l + 2 * PX);
+      const V2 zm2 = *reinterpret_cast<const V2*>(F + l - 2 * U3Z), zm1 = *reinterpret_cast<const V2*>(F + l - U3Z),
+               zp1 = *reinterpret_cast<const V2*>(F + l + U3Z), zp2 = *reinterpret_cast<const V2*>(F + l + 2 * U3Z);
+      // x neighbours of cell 0: xm.x xm.y | xc.x | xc.y xp.x; of cell 1: xm.y xc.x | xc.y | xp.x xp.y
+      const T X[6] = {xm.x, xm.y, xc.x, xc.y, xp.x, xp.y};
+      const T Ym2[2] = {ym2.x, ym2.y}, Ym1[2] = {ym1.x, ym1.y}, Yp1[2] = {yp1.x, yp1.y}, Yp2[2] = {yp2.x, yp2.y};
+      const T Zm2[2] = {zm2.x, zm2.y}, Zm1[2] = {zm1.x, zm1.y}, Zp1[2] = {zp1.x, zp1.y}, Zp2[2] = {zp2.x, zp2.y};
+      T res[2];
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const T fc = X[2 + k];
+        const T adv = a0[k] * up3(a0[k], X[k], X[k + 1], fc, X[k + 3], X[k + 4]) +
+                      a1[k] * up3(a1[k], Ym2[k], Ym1[k], fc, Yp1[k], Yp2[k]) +
+                      a2[k] * up3(a2[k], Zm2[k], Zm1[k], fc, Zp1[k], Zp2[k]);
+        const T lap = (X[k + 1] + X[k + 3] - 2 * fc) + (Ym1[k] + Yp1[k] - 2 * fc) + (Zm1[k] + Zp1[k] - 2 * fc);
+        res[k] = facA * adv + facD * lap;
+      }
+      const V2 bb = SEP ? *reinterpret_cast<const V2*>(&bl[SEP ? q : 0][2 * threadIdx.x]) : xc;
+      T* dst = q == 0 ? ou : (q == 1 ? ov : ow);
+      *reinterpret_cast<V2*>(dst + o) = V2{bb.x + res[0], bb.y + res[1]};
+    }
+  }
+}
+
+template <typename T, int NBW>
+static void up3_walk_launch(Ctx& c, int src0, int base0, int out0, double nu, const double uinf[3], double mult)
+{
+  constexpr int PD = 2;
+  const bool sep = base0 != src0;
+  auto args = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(c.nb / NBW), dim3(NT), 0, c.stream, (const T*)c.f[src0], (const T*)c.f[src0 + 1],
+                       (const T*)c.f[src0 + 2], (const T*)c.f[base0], (const T*)c.f[base0 + 1],
+                       (const T*)c.f[base0 + 2], (T*)c.f[out0], (T*)c.f[out0 + 1], (T*)c.f[out0 + 2], c.grid(), c.sc,
+                       (T)uinf[0], (T)uinf[1], (T)uinf[2], nu, mult);
+  };
+  if (sep) args(k_advdiff_up3_xwalk<T, NBW, PD, true>);
+  else args(k_advdiff_up3_xwalk<T, NBW, PD, false>);
+}
+
 template <typename T>
 static int up3_stage(Ctx& c, int src0, int base0, int out0, double nu, const double uinf[3], double mult)
 {
-  hipLaunchKernelGGL((k_advdiff_up3<T>), dim3(c.nb), dim3(NT), 0, c.stream, (const T*)c.f[src0],
-                     (const T*)c.f[src0 + 1], (const T*)c.f[src0 + 2], (const T*)c.f[base0], (const T*)c.f[base0 + 1],
-                     (const T*)c.f[base0 + 2], (T*)c.f[out0], (T*)c.f[out0 + 1], (T*)c.f[out0 + 2], c.grid(), c.sc,
-                     (T)uinf[0], (T)uinf[1], (T)uinf[2], nu, mult);
-  return check_hip(c, hipGetLastError(), "k_advdiff_up3");
+  const int nbx = c.bpd[0];
+  if (nbx % 8 == 0) up3_walk_launch<T, 8>(c, src0, base0, out0, nu, uinf, mult);
+  else if (nbx % 4 == 0) up3_walk_launch<T, 4>(c, src0, base0, out0, nu, uinf, mult);
+  else if (nbx % 2 == 0) up3_walk_launch<T, 2>(c, src0, base0, out0, nu, uinf, mult);
+  else up3_walk_launch<T, 1>(c, src0, base0, out0, nu, uinf, mult);
+  return check_hip(c, hipGetLastError(), "k_advdiff_up3_xwalk");
 }
 
 // Advection-diffusion of the scheme into the step's intermediate velocity u*: the central
