@@ -24,8 +24,8 @@ CUP_CHI, CUP_U, CUP_V, CUP_W, CUP_P, CUP_TMPU, CUP_TMPV, CUP_TMPW = range(8)
 FIELD_IDS = {"chi": 0, "u": 1, "v": 2, "w": 3, "p": 4, "tmpU": 5, "tmpV": 6, "tmpW": 7,
              "adjU": 8, "adjV": 9, "adjW": 10}
 CUP_PLAIN, CUP_FUSED = 0, 1
-CUP_BC_PERIODIC, CUP_BC_OPEN, CUP_BC_WALL = 0, 1, 2
-BC_IDS = {"periodic": 0, "dirichlet": 1, "open": 1, "absorbing": 1, "wall": 2}
+CUP_BC_PERIODIC, CUP_BC_OPEN, CUP_BC_WALL, CUP_BC_FREESPACE = 0, 1, 2, 3
+BC_IDS = {"periodic": 0, "dirichlet": 1, "open": 1, "absorbing": 1, "wall": 2, "freespace": 3, "unbounded": 3}
 CUP_CENTRAL_EULER, CUP_UPWIND3_EULER, CUP_UPWIND3_RK2 = 0, 1, 2
 
 
@@ -65,6 +65,7 @@ SIGNATURES = {
     "cup_pressure_rhs": (_i, [_p, _d]),
     "cup_poisson_periodic": (_i, [_p]),
     "cup_poisson_mixed": (_i, [_p]),
+    "cup_poisson_unbounded": (_i, [_p]),
     "cup_pressure_project": (_i, [_p, _d]),
     "cup_chi_from_sphere": (_i, [_p, _dp, _d, _dp]),
     "cup_chi_from_hemisphere": (_i, [_p, _dp, _d, _dp]),

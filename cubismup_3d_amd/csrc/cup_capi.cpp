@@ -63,6 +63,7 @@ static void release(Ctx& c)
       if (p) { hipFree(p); p = nullptr; }
   for (auto& p : c.mden)
     if (p) { hipFree(p); p = nullptr; }
+  unbounded_free(c);
   if (c.partials) hipFree(c.partials);
   if (c.ke_part) hipFree(c.ke_part);
   c.ke_part = nullptr;
@@ -166,18 +167,27 @@ int cup_create(const cup_grid_desc* d, cup_ctx** out)
     return create_error(CUP_E_INVALID, "cup_create: decomposition must be CUP_PLAIN or CUP_FUSED");
   if (!(d->maxextent > 0)) return create_error(CUP_E_INVALID, "cup_create: maxextent must be > 0");
   bool periodic = true;
+  int nfree = 0;
   for (int a = 0; a < 3; ++a) {
-    if (d->bc[a] != CUP_BC_PERIODIC && d->bc[a] != CUP_BC_OPEN && d->bc[a] != CUP_BC_WALL)
-      return create_error(CUP_E_INVALID, "cup_create: boundary conditions must be CUP_BC_PERIODIC, _OPEN or _WALL");
+    if (d->bc[a] != CUP_BC_PERIODIC && d->bc[a] != CUP_BC_OPEN && d->bc[a] != CUP_BC_WALL &&
+        d->bc[a] != CUP_BC_FREESPACE)
+      return create_error(CUP_E_INVALID,
+                          "cup_create: boundary conditions must be CUP_BC_PERIODIC, _OPEN, _WALL or _FREESPACE");
     periodic = periodic && d->bc[a] == CUP_BC_PERIODIC;
+    nfree += d->bc[a] == CUP_BC_FREESPACE;
   }
+  if (nfree != 0 && nfree != 3)  // SimulationData.cpp:136-143
+    return create_error(CUP_E_INVALID, "cup_create: either all or no boundary condition can be CUP_BC_FREESPACE");
   if (!periodic && d->decomposition != CUP_PLAIN)
     return create_error(CUP_E_INVALID, "cup_create: non-periodic boundaries take the CUP_PLAIN decomposition");
   cup_ctx* h = new (std::nothrow) cup_ctx();
   if (!h) return create_error(CUP_E_NOMEM, "cup_create: out of host memory");
   Ctx& c = h->c;
   c.dev = d->device;
-  for (int a = 0; a < 3; ++a) c.bc[a] = d->bc[a];
+  // freespace faces are open faces for every stencil (Definitions.h:355-373); they select
+  // PoissonSolverUnbounded
+  for (int a = 0; a < 3; ++a) c.bc[a] = d->bc[a] == CUP_BC_FREESPACE ? CUP_BC_OPEN : d->bc[a];
+  c.unbounded = nfree == 3;
   c.precision = d->precision;
   c.esz = (size_t)d->precision;
   c.decomposition = d->decomposition;
@@ -437,6 +447,14 @@ int cup_poisson_mixed(cup_ctx* ctx)
     return set_error(c, CUP_E_STATE, "poisson_mixed: takes the CUP_PLAIN decomposition's RHS buffer");
   if (int rc = launch_poisson_mixed(c)) return rc;
   return launch_copy(c, c.f[CUP_P], c.rbuf, c.ncell);  // PoissonSolver::_fftw2cub
+}
+
+int cup_poisson_unbounded(cup_ctx* ctx)
+{
+  CUP_GUARD(ctx);
+  if (c.decomposition != CUP_PLAIN)
+    return set_error(c, CUP_E_STATE, "poisson_unbounded: takes the CUP_PLAIN decomposition's RHS buffer");
+  return launch_poisson_unbounded(c);
 }
 
 int cup_pressure_project(cup_ctx* ctx, double dt)

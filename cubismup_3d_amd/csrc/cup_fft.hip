@@ -20,7 +20,10 @@
 // division (PoissonSolverPeriodic.cpp:23-35).
 #include "cup_internal.h"
 
+#include <algorithm>
 #include <cmath>
+#include <new>
+#include <vector>
 
 namespace cup {
 
@@ -1098,6 +1101,166 @@ int launch_poisson_mixed(Ctx& c)
 
 // Twiddle tables e^{-2 pi i t/N} (t < N) per axis and the Poisson cosine tables, built in
 // FP64 on the host.
+// ------------------------------------------------------ PoissonSolverUnbounded
+// PoissonSolverUnbounded::solve (PoissonSolverUnbounded.cpp:126-157): Eastwood and
+// Brownrigg's cyclic convolution of the zero-padded RHS with the free-space Green's function
+// of the uniform mesh, -h^2/(4 pi r) (r in cells; r = 0: -0.1924173658 h^2, :216-218), times
+// 1/h^3 (:50). The reference pads to 2N-1 points per axis; here the period is 2N (the FFT
+// passes' lengths), with the Green's function mirrored about N: on the physical points the
+// cyclic convolution of either period is the linear one (distances stay below N, and the
+// entry at distance N never meets a non-zero RHS value), so the solution is the same. The
+// padded grid is a periodic context of its own (2 bpd blocks per axis) that shares the
+// parent's stream; its Green's function spectrum (real: the function is even) is computed
+// once.
+
+// RHS blocks into the padded grid (the padding blocks stay zero), or the solution back.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_pad_blocks(const T* __restrict__ src, T* __restrict__ dst, GridArgs g, int out)
+{
+  const int b = blockIdx.x;
+  const int bx = b % g.nbx, by = (b / g.nbx) % g.nby, bz = b / (g.nbx * g.nby);
+  const size_t pb = (size_t)bx + 2 * g.nbx * (by + 2 * (size_t)g.nby * bz);
+  using V4 = typename Vec4<T>::type;
+  if (threadIdx.x >= BS3 / 4) return;
+  const size_t o = (size_t)b * BS3 + 4 * threadIdx.x, po = pb * BS3 + 4 * threadIdx.x;
+  if (out) *reinterpret_cast<V4*>(dst + o) = *reinterpret_cast<const V4*>(src + po);
+  else *reinterpret_cast<V4*>(dst + po) = *reinterpret_cast<const V4*>(src + o);
+}
+
+// The free-space Green's function on the padded grid (block layout), distances mirrored
+// about N per axis, times 1/h^3: -1/(4 pi h r), r = 0: -0.1924173658/h.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_green(T* __restrict__ G, GridArgs pg, double h)
+{
+  const int b = blockIdx.x;
+  const int bx = b % pg.nbx, by = (b / pg.nbx) % pg.nby, bz = b / (pg.nbx * pg.nby);
+  for (int c = threadIdx.x; c < BS3; c += NT) {
+    const int X = bx * BS + (c & 7), Y = by * BS + ((c >> 3) & 7), Z = bz * BS + (c >> 6);
+    const double xi = X <= pg.Nx / 2 ? X : pg.Nx - X, yi = Y <= pg.Ny / 2 ? Y : pg.Ny - Y,
+                 zi = Z <= pg.Nz / 2 ? Z : pg.Nz - Z;
+    const double r = sqrt(xi * xi + yi * yi + zi * zi);
+    G[(size_t)b * BS3 + c] = (T)(r > 0 ? -1.0 / (4 * M_PI * h * r) : -0.1924173658 / h);
+  }
+}
+
+// The spectrum times the Green's function's (FP64, normalization included). Slots kx >= 1:
+// S *= G. Slot 0 packs the kx = 0 and kx = Nx/2 planes as A + iB; with G0 = Re Gs, GN = Im Gs
+// there (both real and even), the product is a S(k) + b conj S(-k), a = (G0 + GN)/2,
+// b = (G0 - GN)/2, so each thread takes a (k, -k) pair of slot 0.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_green_mul(C2<T>* __restrict__ S, const C2<double>* __restrict__ Gs,
+                                                  int ncols, int Ny, int Nz)
+{
+  const size_t n = (size_t)ncols * Ny * Nz;
+  for (size_t i = (size_t)blockIdx.x * NT + threadIdx.x; i < n; i += (size_t)gridDim.x * NT) {
+    const int kx = (int)(i % ncols);
+    if (kx) {
+      const double gg = Gs[i].x;
+      C2<T> v = S[i];
+      S[i] = C2<T>{(T)(v.x * gg), (T)(v.y * gg)};
+      continue;
+    }
+    const int ky = (int)((i / ncols) % Ny), kz = (int)(i / ((size_t)ncols * Ny));
+    const int my = (Ny - ky) % Ny, mz = (Nz - kz) % Nz;
+    const size_t m = ((size_t)mz * Ny + my) * ncols;
+    if (m < i) continue;  // the pair's lower index does both
+    const double a = 0.5 * (Gs[i].x + Gs[i].y), bq = 0.5 * (Gs[i].x - Gs[i].y);
+    const C2<T> z = S[i], zm = S[m];
+    // a z + b conj(zm), and at the mirror a zm + b conj(z) (G is even: the same a, b)
+    const C2<T> o1{(T)(a * z.x + bq * zm.x), (T)(a * z.y - bq * zm.y)};
+    const C2<T> o2{(T)(a * zm.x + bq * z.x), (T)(a * zm.y - bq * z.y)};
+    S[i] = o1;
+    if (m != i) S[m] = o2;
+  }
+}
+
+void unbounded_free(Ctx& c)
+{
+  if (!c.unb) return;
+  Ctx& p = *c.unb;
+  void* bufs[] = {p.spec, p.rbuf, p.f[CUP_P], p.ghat, p.tw[0], p.tw[1], p.tw[2]};
+  for (void* b : bufs)
+    if (b) hipFree(b);
+  for (auto* q : p.cos4)
+    if (q) hipFree(q);
+  delete c.unb;
+  c.unb = nullptr;
+}
+
+template <typename T>
+static int unbounded_setup_t(Ctx& c)
+{
+  Ctx* p = new (std::nothrow) Ctx();
+  if (!p) return set_error(c, CUP_E_NOMEM, "poisson_unbounded: out of host memory");
+  c.unb = p;
+  p->dev = c.dev;
+  p->stream = c.stream;
+  p->precision = c.precision;
+  p->esz = c.esz;
+  p->decomposition = CUP_PLAIN;
+  for (int a = 0; a < 3; ++a) {
+    p->bpd[a] = 2 * c.bpd[a];
+    p->N[a] = 2 * c.N[a];
+    p->extent[a] = 2 * c.extent[a];
+  }
+  p->nb = 8 * c.nb;
+  p->ncell = 8 * c.ncell;
+  p->h = c.h;
+  p->maxextent = 2 * c.maxextent;
+  p->sc = c.sc;
+  if (int rc = fft_setup(*p)) return set_error(c, rc, "poisson_unbounded: " + p->err);
+  const size_t nspec = (size_t)(p->N[0] / 2) * p->N[1] * p->N[2];
+  if (check_hip(c, hipMalloc(&p->spec, nspec * sizeof(C2<T>)), "hipMalloc unbounded spectrum") ||
+      check_hip(c, hipMalloc(&p->rbuf, p->ncell * sizeof(T)), "hipMalloc unbounded rhs") ||
+      check_hip(c, hipMalloc(&p->f[CUP_P], p->ncell * sizeof(T)), "hipMalloc unbounded solution") ||
+      check_hip(c, hipMalloc(&p->ghat, nspec * sizeof(C2<double>)), "hipMalloc unbounded kernel"))
+    return CUP_E_NOMEM;
+  // the Green's function's spectrum, normalized by the padded size, kept in FP64
+  hipLaunchKernelGGL((k_green<T>), dim3(p->nb), dim3(NT), 0, c.stream, (T*)p->rbuf, p->grid(), c.h);
+  if (int rc = launch_fft_forward_field(*p, p->rbuf, p->spec)) return set_error(c, rc, "poisson_unbounded: " + p->err);
+  const double norm = 1.0 / ((double)p->N[0] * p->N[1] * p->N[2]);
+  std::vector<C2<T>> hs(nspec);
+  std::vector<C2<double>> gd(nspec);
+  if (check_hip(c, hipMemcpyAsync(hs.data(), p->spec, nspec * sizeof(C2<T>), hipMemcpyDeviceToHost, c.stream),
+                "unbounded kernel") ||
+      check_hip(c, hipStreamSynchronize(c.stream), "synchronize"))
+    return CUP_E_DEVICE;
+  for (size_t i = 0; i < nspec; ++i) gd[i] = C2<double>{(double)hs[i].x * norm, (double)hs[i].y * norm};
+  if (check_hip(c, hipMemcpy(p->ghat, gd.data(), nspec * sizeof(C2<double>), hipMemcpyHostToDevice), "unbounded kernel") ||
+      check_hip(c, hipMemsetAsync(p->rbuf, 0, p->ncell * sizeof(T), c.stream), "unbounded zero"))
+    return CUP_E_DEVICE;
+  return CUP_OK;
+}
+
+template <typename T>
+static int poisson_unbounded_t(Ctx& c)
+{
+  if (!c.unb)
+    if (int rc = unbounded_setup_t<T>(c)) {
+      unbounded_free(c);
+      return rc;
+    }
+  Ctx& p = *c.unb;
+  p.stream = c.stream;
+  hipLaunchKernelGGL((k_pad_blocks<T>), dim3(c.nb), dim3(NT), 0, c.stream, (const T*)c.rbuf, (T*)p.rbuf, c.grid(), 0);
+  if (int rc = launch_fft_forward_field(p, p.rbuf, p.spec)) return set_error(c, rc, "poisson_unbounded: " + p.err);
+  const size_t n = (size_t)(p.N[0] / 2) * p.N[1] * p.N[2];
+  const unsigned gsz = (unsigned)std::min<size_t>(65536, (n + NT - 1) / NT);
+  hipLaunchKernelGGL((k_green_mul<T>), dim3(gsz), dim3(NT), 0, c.stream, (C2<T>*)p.spec, (const C2<double>*)p.ghat,
+                     p.N[0] / 2, p.N[1], p.N[2]);
+  if (int rc = launch_fft_inverse_field(p, p.spec, p.f[CUP_P])) return set_error(c, rc, "poisson_unbounded: " + p.err);
+  hipLaunchKernelGGL((k_pad_blocks<T>), dim3(c.nb), dim3(NT), 0, c.stream, (const T*)p.f[CUP_P], (T*)c.f[CUP_P],
+                     c.grid(), 1);
+  return check_hip(c, hipGetLastError(), "poisson_unbounded");
+}
+
+int launch_poisson_unbounded(Ctx& c)
+{
+  if (2 * std::max(c.N[0], std::max(c.N[1], c.N[2])) > FFT_MAX_N)
+    return set_error(c, CUP_E_INVALID, "poisson_unbounded: the padded grid needs at most 512 cells per axis");
+  return c.precision == 4 ? poisson_unbounded_t<float>(c) : poisson_unbounded_t<double>(c);
+}
+
 int fft_setup(Ctx& c)
 {
   for (int a = 0; a < 3; ++a) {

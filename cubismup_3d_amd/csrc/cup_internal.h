@@ -142,6 +142,9 @@ struct Ctx {
   void* r2r[3][2] = {};   // PoissonSolverMixed: per axis the forward and backward r2r matrix,
                           // transposed (m[j][k] = M[k][j]), context precision (lazy)
   double* mden[3] = {};   // PoissonSolverMixed's per-axis symbol terms (lazy)
+  bool unbounded = false; // freespace faces on every axis: PoissonSolverUnbounded (bc[] holds them as open)
+  Ctx* unb = nullptr;     // PoissonSolverUnbounded's padded periodic grid (lazy)
+  void* ghat = nullptr;   // (padded grid) the Green's function's spectrum, FP64 complex
   double* partials = nullptr;
   size_t partials_cap = 0;  // doubles
   double* ke_part = nullptr;  // [nb][4] partial sums |u|^2 of the fused step's velocity (per wave)
@@ -316,6 +319,8 @@ int free_error(int code, const std::string& msg);
 int launch_reset_umax(Ctx& c);  // clears both banks
 int launch_inflow_correction(Ctx& c);  // UpdateAndCorrectInflow on the open faces of u
 int launch_poisson_mixed(Ctx& c);      // PoissonSolverMixed: rbuf (RHS) -> rbuf (solution)
+int launch_poisson_unbounded(Ctx& c);  // PoissonSolverUnbounded: rbuf (RHS) -> p
+void unbounded_free(Ctx& c);
 // Device address of bank k of the max |u| slots (host pointer arithmetic, no access).
 inline unsigned long long* umax_slots_dev(const Ctx& c, int k) { return &c.sc->umax_slots[k][0]; }
 int launch_finalize_step(Ctx& c, double CFL, double nu, int rampup, double fixed_dt, int advance);

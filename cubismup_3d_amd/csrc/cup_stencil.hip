@@ -605,53 +605,7 @@ __device__ __forceinline__ T up3(T a, T m2, T m1, T c, T p1, T p2)
   return a > 0 ? 2 * p1 + 3 * c - 6 * m1 + m2 : -p2 + 6 * p1 - 3 * c - 2 * m1;
 }
 
-template <typename T>
-__global__ __launch_bounds__(NT) void k_advdiff_up3(const T* __restrict__ su, const T* __restrict__ sv,
-                                                    const T* __restrict__ sw, const T* bu, const T* bv, const T* bw,
-                                                    T* ou, T* ov, T* ow, GridArgs g,
-                                                    const DevScalars* __restrict__ sc, T Ux, T Uy, T Uz, double nu,
-                                                    double mult)
-{
-  __shared__ T lab[3][L2C];
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
-  int bx, by, bz;
-  block_coords(g, b, bx, by, bz);
-  const T* src[3] = {su, sv, sw};
-  gather_g2<T>(src, lab, g, b, bx, by, bz);
-  __syncthreads();
-  const double fac = mult * sc->dt;
-  const T facA = (T)(-fac / (6 * g.h));
-  const T facD = (T)(fac * nu / (g.h * g.h));
-  const int c0 = 2 * threadIdx.x, ix0 = c0 & 7, iy = (c0 >> 3) & 7, iz = c0 >> 6;
-  const size_t o = (size_t)b * BS3 + c0;
-  const T* base[3] = {bu, bv, bw};
-  T* out[3] = {ou, ov, ow};
-  T r[3][2];
-#pragma unroll
-  for (int k = 0; k < 2; ++k) {
-    const int l = ((iz + 2) * L2 + iy + 2) * L2 + ix0 + k + 2;
-    const T a0 = lab[0][l] + Ux, a1 = lab[1][l] + Uy, a2 = lab[2][l] + Uz;
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      const T* F = lab[q];
-      const T fc = F[l];
-      const T adv = a0 * up3(a0, F[l - 2], F[l - 1], fc, F[l + 1], F[l + 2]) +
-                    a1 * up3(a1, F[l - 2 * L2], F[l - L2], fc, F[l + L2], F[l + 2 * L2]) +
-                    a2 * up3(a2, F[l - 2 * L2 * L2], F[l - L2 * L2], fc, F[l + L2 * L2], F[l + 2 * L2 * L2]);
-      const T lap = (F[l - 1] + F[l + 1] - 2 * fc) + (F[l - L2] + F[l + L2] - 2 * fc) +
-                    (F[l - L2 * L2] + F[l + L2 * L2] - 2 * fc);
-      r[q][k] = facA * adv + facD * lap;
-    }
-  }
-  using V2 = typename Vec2<T>::type;
-#pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    const V2 bb = *reinterpret_cast<const V2*>(base[q] + o);
-    *reinterpret_cast<V2*>(out[q] + o) = V2{bb.x + r[q][0], bb.y + r[q][1]};
-  }
-}
-
-// The same stage as a walk of NBW blocks along x with PD blocks in flight, as the central
+// The stage runs as a walk of NBW blocks along x with PD blocks in flight, as the central
 // kernel walks: the lab pads x to PX = 12 (interior at x = 2..9, two halo layers each side),
 // y and z to L2 = 12. A block's x halos are the neighbouring blocks' ix = 6, 7 and 0, 1 planes,
 // held in registers by the threads t % 4 == 3 and t % 4 == 0; the walk's two end faces (two

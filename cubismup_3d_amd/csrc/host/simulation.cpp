@@ -132,6 +132,7 @@ SimulationData& SimulationData::operator=(const SimulationData& o)
   cs2_avg = o.cs2_avg;
   path4serialization = o.path4serialization;
   BCx_flag = o.BCx_flag; BCy_flag = o.BCy_flag; BCz_flag = o.BCz_flag; bUseFourierBC = o.bUseFourierBC;
+  bUseUnboundedBC = o.bUseUnboundedBC;
   precision = o.precision; decomposition = o.decomposition; scheme = o.scheme; device = o.device;
   obstacles.clear();
   for (const auto& s : o.obstacles) obstacles.push_back(std::make_shared<Sphere>(*s));
@@ -267,9 +268,12 @@ void SimulationData::_preprocessArguments()
     throw std::invalid_argument("the fixed mass flux of channel flows needs walls; this build has periodic boxes");
   if (analysis == "HIT" && (freqAnalysis > 0 || timeAnalysis > 0) && !(nu > 0))
     throw std::invalid_argument("HIT analysis needs nu > 0");
-  if (BCx_flag == freespace || BCy_flag == freespace || BCz_flag == freespace)
-    throw std::invalid_argument("freespace boundaries need PoissonSolverUnbounded, which this build does not have "
-                                "(periodic, dirichlet and wall are supported)");
+  // SimulationData.cpp:136-143: freespace faces select PoissonSolverUnbounded, on every axis
+  if (BCx_flag == freespace || BCy_flag == freespace || BCz_flag == freespace) {
+    if (!(BCx_flag == freespace && BCy_flag == freespace && BCz_flag == freespace))
+      throw std::invalid_argument("either all or no BC can be freespace/unbounded");
+    bUseUnboundedBC = true;
+  }
   if (!bUseFourierBC) {  // the non-periodic path (PoissonSolverMixed)
     if (decomposition == CUP_FUSED)
       throw std::invalid_argument("non-periodic boundaries take -decomposition plain");
@@ -775,10 +779,13 @@ void PressureRHS::operator()(double dt)
 void PressureProjection::operator()(double dt)
 {
   // PressureProjection's solver (PressureProjection.cpp:86-105): the FFT solver when every face
-  // is periodic (bUseFourierBC), PoissonSolverMixed otherwise
+  // is periodic (bUseFourierBC), PoissonSolverUnbounded with freespace faces (bUseUnboundedBC),
+  // PoissonSolverMixed otherwise
   {
-    Region r(sim, sim.bUseFourierBC ? "FFTW solve" : "MFFTW solve");
-    check(sim.bUseFourierBC ? cup_poisson_periodic(sim.ctx) : cup_poisson_mixed(sim.ctx), "PressureProjection solve");
+    Region r(sim, sim.bUseFourierBC ? "FFTW solve" : (sim.bUseUnboundedBC ? "UFFTW solve" : "MFFTW solve"));
+    check(sim.bUseFourierBC ? cup_poisson_periodic(sim.ctx)
+                            : (sim.bUseUnboundedBC ? cup_poisson_unbounded(sim.ctx) : cup_poisson_mixed(sim.ctx)),
+          "PressureProjection solve");
   }
   {
     Region r(sim, "GradP");
@@ -1203,7 +1210,9 @@ void Simulation::setupGrid()
   d.decomposition = sim.decomposition;
   d.device = sim.device;
   const BCflag f[3] = {sim.BCx_flag, sim.BCy_flag, sim.BCz_flag};
-  for (int a = 0; a < 3; ++a) d.bc[a] = f[a] == periodic ? CUP_BC_PERIODIC : (f[a] == wall ? CUP_BC_WALL : CUP_BC_OPEN);
+  for (int a = 0; a < 3; ++a)
+    d.bc[a] = f[a] == periodic ? CUP_BC_PERIODIC
+                               : (f[a] == wall ? CUP_BC_WALL : (f[a] == freespace ? CUP_BC_FREESPACE : CUP_BC_OPEN));
   const int rc = cup_create(&d, &sim.ctx);
   if (rc != CUP_OK) throw std::runtime_error("cup_create failed (" + std::to_string(rc) + ")");
 }

@@ -199,6 +199,7 @@ struct SimulationData {
   std::string path4serialization = "./";
   BCflag BCx_flag = dirichlet, BCy_flag = dirichlet, BCz_flag = dirichlet;
   bool bUseFourierBC = false;
+  bool bUseUnboundedBC = false;  // SimulationData.h:157: every face freespace
   // device-specific: context precision (4/8), step decomposition, advection scheme
   int precision = 8;
   int decomposition = CUP_PLAIN;

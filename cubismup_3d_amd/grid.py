@@ -203,6 +203,10 @@ class Grid:
     def poisson_mixed(self):
         self._c(L.lib().cup_poisson_mixed(self._h))
 
+    def poisson_unbounded(self):
+        """PoissonSolverUnbounded: the free-space solution of the RHS into p (freespace boxes)."""
+        self._c(L.lib().cup_poisson_unbounded(self._h))
+
     def pressure_project(self, dt):
         self._c(L.lib().cup_pressure_project(self._h, float(dt)))
 

@@ -56,8 +56,9 @@ enum cup_scheme { CUP_CENTRAL_EULER = 0, CUP_UPWIND3_EULER = 1, CUP_UPWIND3_RK2 
  * periodic; open, the reference's dirichlet (its labs treat dirichlet, absorbing and
  * freespace alike: the boundary cell is copied outward, BlockLabBC::applyBCfaceOpen,
  * Definitions.h:278-301); wall (no slip: velocities mirrored with their sign flipped, p
- * copied, chi zero, applyBCfaceWall :302-334). */
-enum cup_bc { CUP_BC_PERIODIC = 0, CUP_BC_OPEN = 1, CUP_BC_WALL = 2 };
+ * copied, chi zero, applyBCfaceWall :302-334); freespace, open faces for every stencil whose
+ * pressure solve is PoissonSolverUnbounded (all three axes or none, SimulationData.cpp:136-143). */
+enum cup_bc { CUP_BC_PERIODIC = 0, CUP_BC_OPEN = 1, CUP_BC_WALL = 2, CUP_BC_FREESPACE = 3 };
 
 typedef struct cup_grid_desc {
   int bpd[3];          /* blocks per dimension (SimulationData::bpdx/y/z, SimulationData.h:88) */
@@ -75,7 +76,7 @@ typedef struct cup_ctx cup_ctx;
  * plans (PoissonSolverPeriodic.cpp:90-127). Cells per axis (8 x blocks) from 8 to 1024 with no
  * prime factor above 5; other sizes are refused with CUP_E_INVALID (FFTW plans any size).
  * Non-periodic axes (desc->bc) take the PLAIN decomposition, the central scheme and
- * cup_poisson_mixed; the spectral operations (forcing, HIT IC and analysis), the upwind schemes
+ * cup_poisson_mixed (cup_poisson_unbounded with freespace faces); the spectral operations (forcing, HIT IC and analysis), the upwind schemes
  * and the adjoint need a fully periodic box (CUP_E_INVALID otherwise). */
 int cup_create(const cup_grid_desc* desc, cup_ctx** out);
 int cup_destroy(cup_ctx* ctx);
@@ -135,6 +136,14 @@ int cup_poisson_periodic(cup_ctx* ctx);
  * (its SimulationData never sets bUseFourierBC). CUP_PLAIN decomposition only (CUP_E_STATE):
  * it solves the RHS of cup_pressure_rhs. */
 int cup_poisson_mixed(cup_ctx* ctx);
+
+/* PoissonSolverUnbounded::solve (PoissonSolverUnbounded.cpp:126-157, the Green's function
+ * :195-246): the free-space solution of the RHS of cup_pressure_rhs, Eastwood and Brownrigg's
+ * convolution with the uniform mesh's Green's function -h^2/(4 pi r) (r = 0: -0.1924173658 h^2)
+ * over h^3, by FFTs of a zero-padded grid of twice the cells per axis (at most 512 cells per
+ * axis, CUP_E_INVALID otherwise; the padded buffers and the Green's function's spectrum are
+ * made on the first call); result in p. CUP_PLAIN decomposition only (CUP_E_STATE). */
+int cup_poisson_unbounded(cup_ctx* ctx);
 
 /* KernelGradP (PressureProjection.cpp:27-51): u -= dt/(2h) (p_E - p_W), etc. */
 int cup_pressure_project(cup_ctx* ctx, double dt);

@@ -106,6 +106,7 @@ struct orc_sim {
   double corr[6];
   int bcf[3];        /* per axis: ORC_BC_PERIODIC, ORC_BC_OPEN, ORC_BC_WALL */
   double umax_meas;  /* SimulationData::uMax_measured, set by orc_calc_max_timestep */
+  int unbounded;     /* freespace faces on every axis: PoissonSolverUnbounded */
 };
 
 static const double EPS = DBL_EPSILON;
@@ -133,7 +134,14 @@ orc_sim* orc_create(int bpdx, int bpdy, int bpdz, double maxextent)
 /* Boundary conditions per axis (SimulationData's BCx/y/z_flag, SimulationData.cpp:123-166). */
 void orc_set_bc(orc_sim* s, const int bc[3])
 {
-  for (int d = 0; d < 3; d++) s->bcf[d] = bc[d];
+  /* freespace faces are open faces for the labs (Definitions.h:355-373); all three make the
+     pressure solve PoissonSolverUnbounded (SimulationData.cpp:136-143) */
+  int nfree = 0;
+  for (int d = 0; d < 3; d++) {
+    nfree += bc[d] == ORC_BC_FREESPACE;
+    s->bcf[d] = bc[d] == ORC_BC_FREESPACE ? ORC_BC_OPEN : bc[d];
+  }
+  s->unbounded = nfree == 3;
 }
 
 void orc_set_umax_measured(orc_sim* s, double umax) { s->umax_meas = umax; }
@@ -1027,6 +1035,55 @@ void orc_poisson_mixed(orc_sim* s)
         for (int ix = 0; ix < BS; ix++)
           s->blk[b].d[iz][iy][ix].p = s->rhs[ridx(s, bc[0] * BS + ix, bc[1] * BS + iy, bc[2] * BS + iz)];
   }
+}
+
+/* PoissonSolverUnbounded::solve (PoissonSolverUnbounded.cpp:126-157, Green's function
+ * :195-246): Eastwood and Brownrigg's cyclic convolution of the RHS, zero-padded to 2N-1
+ * points per axis, with the free-space Green's function of the uniform mesh,
+ * -h^2/(4 pi r) (r in cells; at r = 0 -0.1924173658 h^2, :216-218), normalized by
+ * 1/(h^3 prod(2N-1)) (:50). On that period the cyclic convolution equals the linear one on
+ * the physical points (distances stay below N), so the oracle evaluates the linear
+ * convolution directly: p_i = sum_j g(|i - j|) rhs_j with g = -1/(4 pi h r), g(0) =
+ * -0.1924173658/h. O(N^6): test grids only. */
+void orc_poisson_unbounded(orc_sim* s)
+{
+  const int* N = s->N;
+  const double h = s->h;
+  const size_t nt = (size_t)N[0] * N[1] * N[2];
+  double* g = (double*)malloc(sizeof(double) * nt);  /* g[dx][dy][dz], distances in cells */
+  for (int i = 0; i < N[0]; i++)
+    for (int j = 0; j < N[1]; j++)
+      for (int k = 0; k < N[2]; k++) {
+        const double r = sqrt((double)i * i + (double)j * j + (double)k * k);
+        g[((size_t)i * N[1] + j) * N[2] + k] = r > 0 ? -1.0 / (4 * M_PI * h * r) : -0.1924173658 / h;
+      }
+  double* out = (double*)malloc(sizeof(double) * nt);
+  #pragma omp parallel for schedule(dynamic, 16)
+  for (size_t o = 0; o < nt; o++) {
+    const int xi = (int)(o / ((size_t)N[1] * N[2])), yi = (int)((o / N[2]) % N[1]), zi = (int)(o % N[2]);
+    double acc = 0;
+    for (int x = 0; x < N[0]; x++) {
+      const int dx = abs(xi - x);
+      for (int y = 0; y < N[1]; y++) {
+        const int dy = abs(yi - y);
+        const double* gr = g + ((size_t)dx * N[1] + dy) * N[2];
+        const double* rr = s->rhs + ((size_t)x * N[1] + y) * N[2];
+        for (int z = 0; z < N[2]; z++) acc += gr[abs(zi - z)] * rr[z];
+      }
+    }
+    out[o] = acc;
+  }
+  #pragma omp parallel for schedule(static)
+  for (int b = 0; b < s->nb; b++) {
+    int bc[3];
+    bcoords(s, b, bc);
+    for (int iz = 0; iz < BS; iz++)
+      for (int iy = 0; iy < BS; iy++)
+        for (int ix = 0; ix < BS; ix++)
+          s->blk[b].d[iz][iy][ix].p = out[ridx(s, bc[0] * BS + ix, bc[1] * BS + iy, bc[2] * BS + iz)];
+  }
+  free(out);
+  free(g);
 }
 
 /* One r2r transform of a line (tests): kind 0 R2HC, 1 HC2R, 2 REDFT10, 3 REDFT01. */
@@ -2556,6 +2613,7 @@ void orc_timestep(orc_sim* s, double dt, const orc_params* prm, double forcing_s
   /* PressureProjection's solver (PressureProjection.cpp:86-105): FFT with every face periodic
    * (bUseFourierBC), else PoissonSolverMixed */
   if (s->bcf[0] == ORC_BC_PERIODIC && s->bcf[1] == ORC_BC_PERIODIC && s->bcf[2] == ORC_BC_PERIODIC) orc_poisson_solve(s);
+  else if (s->unbounded) orc_poisson_unbounded(s);
   else orc_poisson_mixed(s);
   orc_grad_p(s, dt);
   if (prm->has_sphere && s->has_sphere) {

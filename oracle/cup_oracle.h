@@ -42,12 +42,14 @@ void orc_set_plant(int code);
 orc_sim* orc_create(int bpdx, int bpdy, int bpdz, double maxextent);
 /* Boundary conditions per axis (SimulationData.cpp:123-166): periodic, open (the reference's
  * dirichlet, absorbing and freespace labs) or wall; SimulationData::uMax_measured. */
-enum { ORC_BC_PERIODIC = 0, ORC_BC_OPEN = 1, ORC_BC_WALL = 2 };
+enum { ORC_BC_PERIODIC = 0, ORC_BC_OPEN = 1, ORC_BC_WALL = 2, ORC_BC_FREESPACE = 3 };
 void orc_set_bc(orc_sim* s, const int bc[3]);
 void orc_set_umax_measured(orc_sim* s, double umax);
 /* PoissonSolverMixed's solve of the RHS buffer into p; one r2r line (0 R2HC, 1 HC2R,
  * 2 REDFT10, 3 REDFT01). */
 void orc_poisson_mixed(orc_sim* s);
+/* PoissonSolverUnbounded (freespace on every axis): the RHS's free-space convolution into p. */
+void orc_poisson_unbounded(orc_sim* s);
 void orc_r2r_line(double* a, int n, int kind);
 void orc_destroy(orc_sim* s);
 double orc_h(const orc_sim* s);

@@ -46,6 +46,7 @@ def lib():
         "orc_extent": (None, [p, dp]),
         "orc_set_threads": (None, [i]),
         "orc_set_bc": (None, [p, C.POINTER(C.c_int)]),
+        "orc_poisson_unbounded": (None, [p]),
         "orc_set_umax_measured": (None, [p, d]),
         "orc_poisson_mixed": (None, [p]),
         "orc_r2r_line": (None, [dp, i, i]),
@@ -141,7 +142,7 @@ def planted(name):
     return scope()
 
 
-BC = {"periodic": 0, "dirichlet": 1, "open": 1, "absorbing": 1, "freespace": 1, "wall": 2}
+BC = {"periodic": 0, "dirichlet": 1, "open": 1, "absorbing": 1, "wall": 2, "freespace": 3, "unbounded": 3}
 
 
 def r2r_line(a, kind):
@@ -324,11 +325,17 @@ class OracleSim:
 
     def poisson(self):
         """The solver PressureProjection picks (PressureProjection.cpp:86-105): FFT for a fully
-        periodic box, PoissonSolverMixed otherwise."""
+        periodic box, PoissonSolverUnbounded with freespace faces, PoissonSolverMixed otherwise."""
         if all(b == "periodic" for b in self.bc):
             self.poisson_solve()
+        elif all(BC[b] == 3 for b in self.bc):
+            self.poisson_unbounded()
         else:
             self.poisson_mixed()
+
+    def poisson_unbounded(self):
+        """PoissonSolverUnbounded's solve (the free-space convolution) of the RHS into p."""
+        lib().orc_poisson_unbounded(self._h)
 
     def poisson_mixed(self):
         """PoissonSolverMixed's solve (DFT on periodic axes, DCT on the others) of the RHS into p."""

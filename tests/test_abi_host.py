@@ -144,12 +144,12 @@ def test_buffered_logger_flushes_like_the_reference(cup, tmp_path):
 
 def test_command_line_program_rejects_bad_flags(cup):
     """cup_simulation (source/main.cpp) ends with status 1 and the reason on stderr where the
-    reference aborts; here freespace boundaries ask for PoissonSolverUnbounded, which this
-    build does not have, and a non-periodic box with the fused decomposition; both are found
-    before any device call."""
+    reference aborts; here freespace on some axes only (all or none, SimulationData.cpp:136-143)
+    and a non-periodic box with the fused decomposition; both are found before any device
+    call."""
     exe = os.path.join(os.path.dirname(cup.__file__), "cup_simulation")
     base = [exe, "-bpdx", "2", "-bpdy", "2", "-bpdz", "2", "-nu", "0.01"]
-    r = subprocess.run(base + ["-BC_x", "freespace", "-BC_y", "freespace", "-BC_z", "freespace"], capture_output=True,
+    r = subprocess.run(base + ["-BC_x", "freespace", "-BC_y", "dirichlet", "-BC_z", "freespace"], capture_output=True,
                        text=True, timeout=60)
     assert r.returncode == 1 and "freespace" in r.stderr
     r = subprocess.run(base + ["-BC_x", "wall", "-decomposition", "fused"], capture_output=True, text=True, timeout=60)

@@ -169,3 +169,60 @@ def test_simulation_sphere_preset_boundaries(cup, oracle, bc):
         assert e < 1e-10, name
     ob = S.obstacle(0)
     np.testing.assert_allclose(ob["force"], out["penal_ft"][:3], rtol=1e-9, atol=1e-12 * np.abs(out["penal_ft"]).max())
+
+
+@pytest.mark.parametrize("bpd", [(2, 2, 2), (3, 2, 4)])
+@pytest.mark.parametrize("precision,tol", [(8, 1e-11), (4, 2e-6)])
+def test_unbounded_solve_matches_the_direct_sum(cup, oracle, bpd, precision, tol):
+    """PoissonSolverUnbounded on the device (FFTs of the grid padded to 2N per axis, 48 cells
+    for the 24-cell axis: radix 3) against the oracle's direct free-space sum, from the pressure
+    RHS of a random velocity."""
+    print(f"unbounded solve bpd={bpd} precision={precision}", flush=True)
+    rng = np.random.default_rng(12)
+    bc = ("freespace",) * 3
+    G = cup.Grid(bpd, maxextent=1.0, precision=precision, decomposition="plain", bc=bc)
+    O = oracle.OracleSim(bpd, maxextent=1.0, bc=bc)
+    u, v, w = (rng.standard_normal(G.shape) for _ in range(3))
+    G.set_velocity(u, v, w)
+    O.set_velocity(u, v, w)
+    G.pressure_rhs(1e-3); G.poisson_unbounded(); G.poisson_unbounded()  # the second call reuses the kernel
+    O.pressure_rhs(1e-3); O.poisson_unbounded()
+    e = rel(G.get("p"), O.get("p"))
+    print(f"  p rel L2 {e:.2e}", flush=True)
+    assert e < tol
+
+
+def test_freespace_refusals(cup):
+    with pytest.raises(ValueError):  # all or none (SimulationData.cpp:136-143)
+        cup.Grid((2, 2, 2), bc=("freespace", "dirichlet", "freespace"))
+    G = cup.Grid((2, 2, 2), bc=("freespace",) * 3, decomposition="plain")
+    with pytest.raises(ValueError):
+        G.poisson_periodic()
+
+
+def test_simulation_sphere_in_freespace(cup, oracle):
+    """The drop-in Simulation with -BC_x/y/z freespace (PressureProjection picks
+    PoissonSolverUnbounded, PressureProjection.cpp:86-105): a forced sphere, 3 CFL steps on
+    32 x 16 x 16 cells against the oracle's steps."""
+    print("Simulation sphere freespace", flush=True)
+    nu, CFL, n, R = 1e-3, 0.1, 3, 0.1
+    argv = ["-bpdx", "4", "-bpdy", "2", "-bpdz", "2", "-extentx", "1.0", "-CFL", str(CFL), "-nu", str(nu),
+            "-nsteps", str(n), "-rampup", "0", "-sphereL", str(2 * R), "-xpos", "0.35", "-xvel", "0.125",
+            "-bForcedInSimFrame", "1", "-bFixFrameOfRef", "1", "-lambda", "1e5", "-implicitPenalization", "1",
+            "-BC_x", "freespace", "-BC_y", "freespace", "-BC_z", "freespace", "-muteAll", "1",
+            "-freqDiagnostics", "0"]
+    S = cup.Simulation(cup.SimulationData(argv=argv))
+    S.run()
+    O = oracle.OracleSim((4, 2, 2), maxextent=1.0, bc=("freespace",) * 3)
+    O.sphere_create(R, (0.35, 0.25, 0.25))
+    U = (0.125, 0.0, 0.0)
+    sph = dict(lambda_=1e5, implicit=True, trans_vel=(-0.125, 0.0, 0.0))
+    u_cfl = (0.0, 0.0, 0.0)
+    for step in range(n):
+        dt, _ = O.calc_max_timestep(CFL, nu, u_cfl, step, 0)
+        O.timestep(dt, nu, uinf=U, sphere=sph)
+        u_cfl = U
+    for name in ("u", "v", "w", "p"):
+        e = rel(S.field(name), O.get(name))
+        print(f"  {name} rel L2 {e:.2e}", flush=True)
+        assert e < 1e-10, name

@@ -209,3 +209,89 @@ def test_inflow_correction_closes_the_flux_with_every_face_open(oracle):
             + ww[-1].sum() - ww[0].sum())
     scale = np.abs(uu).sum()
     assert abs(flux) < 1e-13 * scale
+
+
+# PoissonSolverUnbounded: the oracle's direct free-space convolution
+
+def _g(r, h):
+    """The Green's function per unit RHS: -1/(4 pi h r), -0.1924173658/h at r = 0
+    (PoissonSolverUnbounded.cpp:216-218, over the h^3 of :50)."""
+    return np.where(r > 0, -1.0 / (4 * np.pi * h * np.maximum(r, 1e-300)), -0.1924173658 / h)
+
+
+def _cells(shape):
+    z, y, x = np.meshgrid(*(np.arange(n) for n in shape), indexing="ij")
+    return x, y, z
+
+
+def test_unbounded_point_sources_give_the_green_function(oracle):
+    """Two point sources: p is the sum of the Green's function around each, with the r = 0
+    value at the sources (fixes the kernel, its self term and the direct sum's indexing)."""
+    s = oracle.OracleSim((2, 3, 2), maxextent=0.8, bc=("freespace",) * 3)
+    h = s.h
+    x, y, z = _cells(s.shape)
+    rhs = np.zeros(s.shape)
+    src = [((3, 5, 11), 1.5), ((12, 20, 2), -0.7)]  # (x, y, z), q
+    for (sx, sy, sz), q in src:
+        rhs[sz, sy, sx] = q
+    s.set_rhs(rhs)
+    s.poisson_unbounded()
+    exp = sum(q * _g(np.sqrt((x - sx) ** 2 + (y - sy) ** 2 + (z - sz) ** 2), h) for (sx, sy, sz), q in src)
+    np.testing.assert_allclose(s.get("p"), exp, rtol=1e-13, atol=0)
+
+
+def test_unbounded_far_field_of_a_gaussian_is_the_monopole(oracle):
+    """A cubically symmetric Gaussian charge Q (sum of the RHS): far from it, p = -Q/(4 pi R)
+    with R the physical distance (its dipole and quadrupole vanish; the next term is
+    O((sigma/R)^4)). Fixes the sign, the 4 pi and the powers of h."""
+    s = oracle.OracleSim((3, 3, 3), maxextent=2.4, bc=("freespace",) * 3)
+    h = s.h
+    x, y, z = _cells(s.shape)
+    c = 4
+    rho = np.exp(-((x - c) ** 2 + (y - c) ** 2 + (z - c) ** 2) / 2.0)
+    rho[np.abs(x - c) > 3] = 0
+    rho[np.abs(y - c) > 3] = 0
+    rho[np.abs(z - c) > 3] = 0
+    Q = rho.sum()
+    s.set_rhs(rho)
+    s.poisson_unbounded()
+    p = s.get("p")
+    R = h * np.sqrt((x - c) ** 2 + (y - c) ** 2 + (z - c) ** 2)
+    far = R > 12 * h
+    ratio = p[far] * (-4 * np.pi * R[far]) / Q
+    assert np.abs(ratio - 1).max() < 2e-3
+    # h = 0.1 here, so a kernel with a wrong power of h would be off by a factor of 10
+    assert abs(h - 0.1) < 1e-12
+
+
+def test_unbounded_solution_solves_the_poisson_equation(oracle):
+    """Away from a smooth source the discrete Laplacian of p is the density (RHS / h^3) up to
+    the O(h^2) error of sampling the continuous Green's function: < 3% of the peak here."""
+    s = oracle.OracleSim((3, 3, 3), maxextent=2.4, bc=("freespace",) * 3)
+    h = s.h
+    x, y, z = _cells(s.shape)
+    c, sig = 12, 3.0
+    rho = np.exp(-((x - c) ** 2 + (y - c) ** 2 + (z - c) ** 2) / (2 * sig ** 2))
+    s.set_rhs(rho * h ** 3)  # the RHS carries h^3 (PressureRHS.cpp:54)
+    s.poisson_unbounded()
+    p = s.get("p")
+    lap = (np.roll(p, 1, 0) + np.roll(p, -1, 0) + np.roll(p, 1, 1) + np.roll(p, -1, 1) + np.roll(p, 1, 2)
+           + np.roll(p, -1, 2) - 6 * p) / h ** 2
+    inner = (slice(6, -6),) * 3
+    assert np.abs(lap[inner] - rho[inner]).max() < 0.03 * rho.max()
+
+
+def test_freespace_is_open_for_the_labs(oracle):
+    """Freespace faces fill the labs as open faces: the same advection as dirichlet."""
+    rng = np.random.default_rng(11)
+    a = oracle.OracleSim((2, 2, 2), maxextent=1.0, bc=("freespace",) * 3)
+    b = oracle.OracleSim((2, 2, 2), maxextent=1.0, bc=("dirichlet",) * 3)
+    u, v, w = (0.3 * rng.standard_normal(a.shape) for _ in range(3))
+    for s in (a, b):
+        s.set_velocity(u, v, w)
+        s.calc_max_timestep(0.3, 0.01, (0.2, 0.0, 0.0))
+        s.advect_diffuse(1e-3, 0.01, (0.2, 0.0, 0.0))
+        s.pressure_rhs(1e-3)
+    for f, g in zip(a.velocity(), b.velocity()):
+        assert np.array_equal(f, g)
+    assert np.array_equal(a.rhs(), b.rhs())
