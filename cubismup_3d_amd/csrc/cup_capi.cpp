@@ -86,12 +86,6 @@ static void release(Ctx& c)
   if (c.host_red) hipHostFree(c.host_red);
   c.partials = nullptr; c.low_coef = nullptr; c.sc = nullptr; c.sc_host = nullptr;
   c.host_red = nullptr;
-  if (c.side) hipStreamSynchronize(c.side);
-  if (c.ev_fork) hipEventDestroy(c.ev_fork);
-  if (c.ev_rows) hipEventDestroy(c.ev_rows);
-  if (c.side) hipStreamDestroy(c.side);
-  c.ev_fork = c.ev_rows = nullptr;
-  c.side = nullptr;
   if (c.stream) hipStreamDestroy(c.stream);
   c.stream = nullptr;
 }

@@ -894,20 +894,12 @@ static int poisson_fused_t(Ctx& c, bool force, double tke_target, double inj_rat
                         (const T*)c.f[c.ustar + 2], (int)XSRC_DIV, force);
   stage_mark(c, ST_XFWD, true);
   if (force) {
-    // The forcing reduction (a few microseconds) runs in order; the per-row synthesis
-    // coefficients are first read by the gradient, so they are made on the side stream
-    // (default priority) beside the z, inverse y and inverse x passes and joined after them.
+    // the forcing's mode coefficients and, from them, the per-row synthesis coefficients the
+    // gradient reads: small kernels in order (a second stream's fork and join cost more here)
     stage_mark(c, ST_FREDUCE, false);
     if (int rc = launch_forcing_reduce(c, tke_target, inj_rate)) return rc;
+    if (int rc = launch_forcing_rows(c)) return rc;
     stage_mark(c, ST_FREDUCE, true);
-    if (int rc = check_hip(c, hipEventRecord(c.ev_fork, c.stream), "fork")) return rc;
-    if (int rc = check_hip(c, hipStreamWaitEvent(c.side, c.ev_fork, 0), "fork")) return rc;
-    hipStream_t main = c.stream;
-    c.stream = c.side;
-    int rc = launch_forcing_rows(c);
-    if (!rc) rc = check_hip(c, hipEventRecord(c.ev_rows, c.side), "join");
-    c.stream = main;
-    if (rc) return rc;
   }
   stage_mark(c, ST_YFWD, false);
   dispatch_n<GoY, T>(c.N[1], c, S, (int)ZM_FWD, norm);
@@ -922,8 +914,6 @@ static int poisson_fused_t(Ctx& c, bool force, double tke_target, double inj_rat
   stage_mark(c, ST_XINV, false);
   dispatch_n<GoXInv, T>(c.N[0], c, (const C2<T>*)S, (T*)c.f[CUP_P]);
   stage_mark(c, ST_XINV, true);
-  if (force)
-    if (int rc = check_hip(c, hipStreamWaitEvent(c.stream, c.ev_rows, 0), "join")) return rc;
   return check_hip(c, hipGetLastError(), "poisson_fused");
 }
 

@@ -164,8 +164,6 @@ struct Ctx {
   double* body_ft = nullptr;   // device: 6 per body, each body's last penalization force/torque in the loop
   int body_ft_n = 0;
   void* adjf = nullptr;        // the forcing adjoint's 9 fields: n P lam, n P u*, u_n kept aside (lazy)
-  hipStream_t side = nullptr;  // the forcing row coefficients' stream (default priority)
-  hipEvent_t ev_fork = nullptr, ev_rows = nullptr;
   Body body;                // the active body: body 0, or body k inside a body-indexed call
   std::vector<Body> more;   // the other bodies (more[k-1] holds body k while body 0 is active)
   int* ob_slot_all = nullptr;     // device [nb]: every body's obstacle blocks, slot into ob_rhs_all

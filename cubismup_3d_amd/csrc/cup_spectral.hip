@@ -516,10 +516,6 @@ int spectral_setup(Ctx& c)
       check_hip(c, hipMalloc(&c.low_dev, sizeof(LowModes)), "hipMalloc low modes") ||
       check_hip(c, hipMalloc(&c.low_rows, (size_t)c.N[1] * c.N[2] * 16 * c.esz), "hipMalloc low rows"))
     return CUP_E_NOMEM;
-  if (check_hip(c, hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking), "hipStreamCreate side") ||
-      check_hip(c, hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming), "hipEventCreate") ||
-      check_hip(c, hipEventCreateWithFlags(&c.ev_rows, hipEventDisableTiming), "hipEventCreate"))
-    return CUP_E_DEVICE;
   hipMemcpy(c.low_dev, &c.low, sizeof(LowModes), hipMemcpyHostToDevice);
   hipMemsetAsync(c.low_counter, 0, sizeof(unsigned), c.stream);
   hipMemsetAsync(c.low_coef, 0, (size_t)c.nlow * 8 * sizeof(double), c.stream);
@@ -713,57 +709,91 @@ int launch_forcing_reduce(Ctx& c, double tke_target, double inj_rate)
 // Fused forcing, stage 3: for every grid row (y, z) the coefficients of the velocity
 // modes' synthesis along x, fac*u_low(x) = R0 + 2 Re(R1 e^{2 pi i x/Nx}) + 2 Re(R2 e^{4 pi i x/Nx})
 // per component (5 values x 3 = 15, padded to 16), so the gradient pass evaluates the
-// forcing with a handful of FMAs per cell. The row phase e^{2 pi i (ky y/Ny + kz z/Nz)} is
-// the conjugate of the product of two twiddles e^{-2 pi i t/N}.
-constexpr int FROWS_NT = 64;
-
+// forcing with a handful of FMAs per cell. Mode m (velocity coefficient c, component by
+// component) contributes c e^{2 pi i (ky y/Ny + kz z/Nz)} to the entries of |kx|: 2 Re at
+// kx = 0, Re and (sign kx) Im at |kx| = 1, 2. Taking a mode with kx < 0 as its conjugate
+// (conj c, -ky, -kz; the real parts are unchanged), the row value is a sum over ky of
+// A_ky(z) e^{2 pi i ky y/Ny} with A_ky(z) = sum over the modes of that ky of c e^{2 pi i kz z/Nz}:
+// one workgroup per z plane forms B_m = c_m e^{...z} per mode, then the 5 x 3 x 3 sums A (in mode
+// order), then each thread the 16 entries of one row from two y twiddles. All in FP64; the
+// phases are conjugates of the twiddles e^{-2 pi i t/N}.
 template <typename T>
-__global__ __launch_bounds__(FROWS_NT) void k_forcing_rows(const LowModes* __restrict__ lmp,
-                                                           const double* __restrict__ coef, const C2<T>* __restrict__ twy,
-                                                           const C2<T>* __restrict__ twz, T* __restrict__ rows, int Ny,
-                                                           int Nz)
+__global__ __launch_bounds__(NT) void k_forcing_rows(const LowModes* __restrict__ lmp,
+                                                     const double* __restrict__ coef, const C2<T>* __restrict__ twy,
+                                                     const C2<T>* __restrict__ twz, T* __restrict__ rows, int Ny,
+                                                     int Nz)
 {
-  const int row = blockIdx.x * FROWS_NT + threadIdx.x;
-  if (row >= Ny * Nz) return;
-  const int y = row % Ny, z = row / Ny;
+  __shared__ int mk[LOW_MAX][3];           // canonical (kx >= 0) wavenumbers
+  __shared__ double B[LOW_MAX][3][2];      // c_m e^{2 pi i kz z/Nz} per component
+  __shared__ double A[5][3][3][2];         // [ky + 2][component][|kx|]: re, im
   const LowModes& lm = *lmp;
-  double r[16];
-#pragma unroll
-  for (int q = 0; q < 16; ++q) r[q] = 0;
-  for (int m = 0; m < lm.n; ++m) {
-    const int kx = lm.k[m][0], ky = lm.k[m][1], kz = lm.k[m][2];
-    const C2<T> ey = twy[((ky * y) % Ny + Ny) % Ny], ez = twz[((kz * z) % Nz + Nz) % Nz];
-    const double co = (double)ey.x * ez.x - (double)ey.y * ez.y;
-    const double si = -((double)ey.x * ez.y + (double)ey.y * ez.x);
-    const double* cm = coef + 8 * m;
-    const int ak = kx < 0 ? -kx : kx;
+  const int n = lm.n, t = threadIdx.x, z = blockIdx.x;
+  if (t < n) {
+    const int kx = lm.k[t][0], sg = kx < 0 ? -1 : 1;
+    const int ky = sg * lm.k[t][1], kz = sg * lm.k[t][2];
+    mk[t][0] = sg * kx;
+    mk[t][1] = ky;
+    mk[t][2] = kz;
+    const C2<T> e = twz[((kz * z) % Nz + Nz) % Nz];
+    const double ec = (double)e.x, es = -(double)e.y;  // e^{+2 pi i kz z/Nz}
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-      const double cr = cm[2 + 2 * q], ci = cm[3 + 2 * q];
-      const double rr = cr * co - ci * si, ri = cr * si + ci * co;
-      if (ak == 0) {
-        r[5 * q] += 2 * rr;
-      } else {
-        r[5 * q + 2 * ak - 1] += rr;
-        r[5 * q + 2 * ak] += kx > 0 ? ri : -ri;
-      }
+      const double cr = coef[8 * t + 2 + 2 * q], ci = sg * coef[8 * t + 3 + 2 * q];
+      B[t][q][0] = cr * ec - ci * es;
+      B[t][q][1] = cr * es + ci * ec;
     }
   }
-  T* out = rows + (size_t)row * 16;
+  __syncthreads();
+  if (t < 5 * 3 * 3) {
+    const int ky = t / 9 - 2, q = (t / 3) % 3, ak = t % 3;
+    double ar = 0, ai = 0;
+    for (int m = 0; m < n; ++m)
+      if (mk[m][1] == ky && mk[m][0] == ak) {
+        ar += B[m][q][0];
+        ai += B[m][q][1];
+      }
+    A[ky + 2][q][ak][0] = ar;
+    A[ky + 2][q][ak][1] = ai;
+  }
+  __syncthreads();
+  for (int y = t; y < Ny; y += NT) {
+    const C2<T> t1 = twy[y], t2 = twy[(2 * y) % Ny];
+    // e^{+2 pi i ky y/Ny} for ky = -2 .. 2
+    const double er[5] = {(double)t2.x, (double)t1.x, 1.0, (double)t1.x, (double)t2.x};
+    const double ei[5] = {(double)t2.y, (double)t1.y, 0.0, -(double)t1.y, -(double)t2.y};
+    double out[16];
 #pragma unroll
-  for (int q = 0; q < 16; ++q) out[q] = (T)r[q];
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+      for (int ak = 0; ak < 3; ++ak) {
+        double xr = 0, xi = 0;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+          xr += A[k][q][ak][0] * er[k] - A[k][q][ak][1] * ei[k];
+          xi += A[k][q][ak][0] * ei[k] + A[k][q][ak][1] * er[k];
+        }
+        if (ak == 0) out[5 * q] = 2 * xr;
+        else {
+          out[5 * q + 2 * ak - 1] = xr;
+          out[5 * q + 2 * ak] = xi;
+        }
+      }
+    out[15] = 0;
+    T* o = rows + ((size_t)z * Ny + y) * 16;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) o[q] = (T)out[q];
+  }
 }
 
 int launch_forcing_rows(Ctx& c)
 {
-  const int nrows = c.N[1] * c.N[2];
-  const dim3 grid((nrows + FROWS_NT - 1) / FROWS_NT);
+  const dim3 grid(c.N[2]);
   if (c.precision == 4)
-    hipLaunchKernelGGL((k_forcing_rows<float>), grid, dim3(FROWS_NT), 0, c.stream, (const LowModes*)c.low_dev,
+    hipLaunchKernelGGL((k_forcing_rows<float>), grid, dim3(NT), 0, c.stream, (const LowModes*)c.low_dev,
                        (const double*)c.low_coef, (const C2<float>*)c.tw[1], (const C2<float>*)c.tw[2],
                        (float*)c.low_rows, c.N[1], c.N[2]);
   else
-    hipLaunchKernelGGL((k_forcing_rows<double>), grid, dim3(FROWS_NT), 0, c.stream, (const LowModes*)c.low_dev,
+    hipLaunchKernelGGL((k_forcing_rows<double>), grid, dim3(NT), 0, c.stream, (const LowModes*)c.low_dev,
                        (const double*)c.low_coef, (const C2<double>*)c.tw[1], (const C2<double>*)c.tw[2],
                        (double*)c.low_rows, c.N[1], c.N[2]);
   return check_hip(c, hipGetLastError(), "k_forcing_rows");
