@@ -86,13 +86,20 @@ __device__ __forceinline__ void block_coords(const GridArgs& g, int b, int& bx, 
 // flight per CU of a load-sync-compute workgroup.
 // A thread's cells: interior c = 2t, 2t+1; halo (faces only, 6*64 cells) c = t (faces
 // x-, x+, y-, y+: one face per wave) and, for t < 128, c = NT + t (faces z-, z+).
-// The pipelined lab pads x to PX = 12 (interior at x = 2..9, halo at 1 and 10): a thread's
-// x-adjacent cell pair, and its y and z neighbour pairs, are 8-byte aligned (one 64-bit LDS
-// access each).
-constexpr int PX = 12;
-constexpr int PY = PX;            // y stride
-constexpr int PZ = PX * L1;       // z stride
-constexpr int PLC = PX * L1 * L1;
+// The pipelined lab pads x to a row stride PXV of at least 12 (interior at x = 2..9, halo at
+// 1 and 10): a thread's x-adjacent cell pair, and its y and z neighbour pairs, are 8-byte
+// aligned (one 64-bit LDS access each). A wave's half covers the 8 rows of one z plane; with
+// rows 24 apart their 8-float interiors fall on distinct LDS banks (24 r mod 64 is 8 times a
+// permutation of 0..7), rows 14 apart share fewer banks than 12; the stride is chosen per
+// kernel by measurement (the advection's lab of three fields at 24, the gradient's at 14).
+template <int PXV>
+struct LabGeom {
+  static constexpr int PY = PXV;            // y stride
+  static constexpr int PZ = PXV * L1;       // z stride
+  static constexpr int PLC = PXV * L1 * L1;  // cells per field
+};
+constexpr int PX_ADV = 24;
+constexpr int PX_GRAD = 14;
 constexpr int ADV_PD = 3;  // blocks in flight in the advection's walk along x
 struct LabSlots {
   int li;        // LDS index of interior cell 2t
@@ -104,8 +111,10 @@ struct LabSlots {
 
 __device__ __forceinline__ bool has_halo2() { return threadIdx.x < 6 * 64 - NT; }
 
+template <int PXV>
 __device__ __forceinline__ LabSlots lab_slots()
 {
+  constexpr int PX = PXV;
   LabSlots s;
   const int t = threadIdx.x, c0 = 2 * t;
   s.li = (((c0 >> 6) + 1) * L1 + ((c0 >> 3) & 7) + 1) * PX + (c0 & 7) + 2;
@@ -181,8 +190,8 @@ __device__ __forceinline__ void lab_fetch(LabRegs<T, NF>& r, const T* const* src
     }
 }
 
-template <typename T, int NF>
-__device__ __forceinline__ void lab_put(const LabRegs<T, NF>& r, T (*lab)[PLC], const LabSlots& s)
+template <int PXV, typename T, int NF>
+__device__ __forceinline__ void lab_put(const LabRegs<T, NF>& r, T (*lab)[LabGeom<PXV>::PLC], const LabSlots& s)
 {
   using V2 = typename Vec2<T>::type;
 #pragma unroll
@@ -386,9 +395,10 @@ __device__ __forceinline__ void lab_fetch_xw(LabRegs<T, NF>& r, const T* const* 
     }
 }
 
-template <typename T, int NF>
+template <int PXV, typename T, int NF>
 __device__ __forceinline__ void lab_put_xw(const LabRegs<T, NF>& r, const LabRegs<T, NF>& prev,
-                                           const LabRegs<T, NF>& next, T (*lab)[PLC], const LabSlots& s, bool x_lo,
+                                           const LabRegs<T, NF>& next, T (*lab)[LabGeom<PXV>::PLC], const LabSlots& s,
+                                           bool x_lo,
                                            bool x_hi)
 {
   using V2 = typename Vec2<T>::type;
@@ -438,8 +448,9 @@ __global__ __launch_bounds__(NT) void k_advdiff_xwalk(const T* __restrict__ u, c
                                                       DevScalars* __restrict__ sc, T Ux, T Uy, T Uz, double nu,
                                                       StepFold fold, Fade6<T> fd)
 {
-  __shared__ __align__(16) T lab[3][PLC];
-  const LabSlots s = lab_slots();
+  using LG = LabGeom<PX_ADV>;
+  __shared__ __align__(16) T lab[3][LG::PLC];
+  const LabSlots s = lab_slots<PX_ADV>();
   const T* src[3] = {u, v, w};
   const int b0 = xwalk_first<NBW>(g, xcd_remap(blockIdx.x, gridDim.x));
   static_assert(PD >= 2, "a block's x+ halo comes from the next block, which must be in registers");
@@ -474,7 +485,7 @@ __global__ __launch_bounds__(NT) void k_advdiff_xwalk(const T* __restrict__ u, c
 #pragma unroll
   for (int j = 0; j < NBW; ++j) {
     if (j) __syncthreads();  // the previous block's compute is done with the lab
-    lab_put_xw(r[j % NS], r[(j + NS - 1) % NS], r[(j + 1) % NS], lab, s, j == 0, j == NBW - 1);
+    lab_put_xw<PX_ADV>(r[j % NS], r[(j + NS - 1) % NS], r[(j + 1) % NS], lab, s, j == 0, j == NBW - 1);
     __syncthreads();
     if (j + PD < NBW) lab_fetch_xw<BC>(r[(j + PD) % NS], src, s, g, b0 + j + PD, false, j + PD == NBW - 1, fd.f);
     T av[3][2];
@@ -491,8 +502,8 @@ __global__ __launch_bounds__(NT) void k_advdiff_xwalk(const T* __restrict__ u, c
     for (int q = 0; q < 3; ++q) {
       const T* F = lab[q];
       const int l = s.li;
-      const V2 Nn = *reinterpret_cast<const V2*>(F + l + PY), Sn = *reinterpret_cast<const V2*>(F + l - PY);
-      const V2 Bn = *reinterpret_cast<const V2*>(F + l + PZ), Fn = *reinterpret_cast<const V2*>(F + l - PZ);
+      const V2 Nn = *reinterpret_cast<const V2*>(F + l + LG::PY), Sn = *reinterpret_cast<const V2*>(F + l - LG::PY);
+      const V2 Bn = *reinterpret_cast<const V2*>(F + l + LG::PZ), Fn = *reinterpret_cast<const V2*>(F + l - LG::PZ);
       const T Wx = F[l - 1], Ex = F[l + 2];
       const T fc[2] = {av[q][0], av[q][1]};
       const T fe[2] = {av[q][1], Ex}, fw[2] = {Wx, av[q][0]};
@@ -606,15 +617,16 @@ __device__ __forceinline__ T up3(T a, T m2, T m1, T c, T p1, T p2)
 }
 
 // The stage runs as a walk of NBW blocks along x with PD blocks in flight, as the central
-// kernel walks: the lab pads x to PX = 12 (interior at x = 2..9, two halo layers each side),
+// kernel walks: the lab pads x to U3X = 14 (interior at x = 2..9, two halo layers each side),
 // y and z to L2 = 12. A block's x halos are the neighbouring blocks' ix = 6, 7 and 0, 1 planes,
 // held in registers by the threads t % 4 == 3 and t % 4 == 0; the walk's two end faces (two
 // layers, 256 cells: one per thread) and the four y/z faces (two layers, 512 cells: two per
 // thread, one face and layer per wave) are gathered. Each thread's cell pair and all its
 // stencil neighbours are 8-byte aligned pairs in LDS (one 64-bit access each). SEP: base is
 // not src (read at the cells one block ahead); else base is the lab's own centre.
-constexpr int U3Z = PX * L2;       // z stride of the walk's lab
-constexpr int U3C = PX * L2 * L2;  // cells per field
+constexpr int U3X = 14;            // y stride of the walk's lab (x padded to 14)
+constexpr int U3Z = U3X * L2;      // z stride
+constexpr int U3C = U3X * L2 * L2; // cells per field
 struct Up3Slots {
   int li;       // LDS index of interior cell 2t
   int l[2];     // the thread's two y/z face cells: LDS index,
@@ -627,7 +639,7 @@ __device__ __forceinline__ Up3Slots up3_slots()
 {
   Up3Slots s;
   const int t = threadIdx.x, c0 = 2 * t;
-  s.li = (((c0 >> 6) + 2) * L2 + ((c0 >> 3) & 7) + 2) * PX + (c0 & 7) + 2;
+  s.li = (((c0 >> 6) + 2) * L2 + ((c0 >> 3) & 7) + 2) * U3X + (c0 & 7) + 2;
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
     const int c = t + NT * k;  // 0..511: face 2 + (c >> 7), layer (c >> 6) & 1
@@ -639,7 +651,7 @@ __device__ __forceinline__ Up3Slots up3_slots()
     else { sy = a; sz = sd; ly = a + 2; lz = ld; }
     s.face[k] = face;
     s.off[k] = (sz * BS + sy) * BS + sx;
-    s.l[k] = (lz * L2 + ly) * PX + e + 2;
+    s.l[k] = (lz * L2 + ly) * U3X + e + 2;
   }
   {
     const int face = t >> 7, layer = (t >> 6) & 1, a = (t >> 3) & 7, e = t & 7;
@@ -647,7 +659,7 @@ __device__ __forceinline__ Up3Slots up3_slots()
     const int sd = plus ? layer : 6 + layer, ld = plus ? 10 + layer : layer;
     s.facex = face;
     s.offx = (a * BS + e) * BS + sd;
-    s.lx = ((a + 2) * L2 + e + 2) * PX + ld;
+    s.lx = ((a + 2) * L2 + e + 2) * U3X + ld;
   }
   return s;
 }
@@ -768,8 +780,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SEP ? 1 : 5)
       const T* F = lab[q];
       const V2 xm = *reinterpret_cast<const V2*>(F + l - 2), xc = *reinterpret_cast<const V2*>(F + l),
                xp = *reinterpret_cast<const V2*>(F + l + 2);
-      const V2 ym2 = *reinterpret_cast<const V2*>(F + l - 2 * PX), ym1 = *reinterpret_cast<const V2*>(F + l - PX),
-               yp1 = *reinterpret_cast<const V2*>(F + l + PX), yp2 = *reinterpret_cast<const V2*>(F + l + 2 * PX);
+      const V2 ym2 = *reinterpret_cast<const V2*>(F + l - 2 * U3X), ym1 = *reinterpret_cast<const V2*>(F + l - U3X),
+               yp1 = *reinterpret_cast<const V2*>(F + l + U3X), yp2 = *reinterpret_cast<const V2*>(F + l + 2 * U3X);
       const V2 zm2 = *reinterpret_cast<const V2*>(F + l - 2 * U3Z), zm1 = *reinterpret_cast<const V2*>(F + l - U3Z),
                zp1 = *reinterpret_cast<const V2*>(F + l + U3Z), zp2 = *reinterpret_cast<const V2*>(F + l + 2 * U3Z);
       // x neighbours of cell 0: xm.x xm.y | xc.x | xc.y xp.x; of cell 1: xm.y xc.x | xc.y | xp.x xp.y
@@ -1101,10 +1113,11 @@ __global__ __launch_bounds__(NT) void k_gradp(const T* __restrict__ p, const T* 
 {
   using V2 = typename Vec2<T>::type;
   using V4 = typename Vec4<T>::type;
-  __shared__ __align__(16) T lab[1][PLC];
+  using LG = LabGeom<PX_GRAD>;
+  __shared__ __align__(16) T lab[1][LG::PLC];
   __shared__ __align__(16) T rl[FORCE ? 64 * 16 : 4];
   const int t = threadIdx.x;
-  const LabSlots s = lab_slots();
+  const LabSlots s = lab_slots<PX_GRAD>();
   const T* src[1] = {p};
   const int nxy = g.nbx * g.nby;
   const int b0 = walk_first(g, xcd_remap(blockIdx.x, gridDim.x), nbw);
@@ -1143,7 +1156,7 @@ __global__ __launch_bounds__(NT) void k_gradp(const T* __restrict__ p, const T* 
   for (int j = 0; j < nbw; ++j) {
     const int b = b0 + j * nxy;
     if (j) __syncthreads();  // the previous block is done with the lab
-    lab_put(r, lab, s);
+    lab_put<PX_GRAD>(r, lab, s);
     if (FORCE) *reinterpret_cast<V4*>(rl + 4 * t) = V4{rr[0], rr[1], rr[2], rr[3]};
     T nu_[2] = {us[0][0], us[0][1]}, nv_[2] = {us[1][0], us[1][1]}, nw_[2] = {us[2][0], us[2][1]};
     __syncthreads();
@@ -1152,8 +1165,8 @@ __global__ __launch_bounds__(NT) void k_gradp(const T* __restrict__ p, const T* 
       const T* P = lab[0];
       const int l = s.li;
       const V2 C = *reinterpret_cast<const V2*>(P + l);
-      const V2 Nn = *reinterpret_cast<const V2*>(P + l + PY), Sn = *reinterpret_cast<const V2*>(P + l - PY);
-      const V2 Bn = *reinterpret_cast<const V2*>(P + l + PZ), Fn = *reinterpret_cast<const V2*>(P + l - PZ);
+      const V2 Nn = *reinterpret_cast<const V2*>(P + l + LG::PY), Sn = *reinterpret_cast<const V2*>(P + l - LG::PY);
+      const V2 Bn = *reinterpret_cast<const V2*>(P + l + LG::PZ), Fn = *reinterpret_cast<const V2*>(P + l - LG::PZ);
       const T Wx = P[l - 1], Ex = P[l + 2];
       nu_[0] += fac * (C.y - Wx);
       nu_[1] += fac * (Ex - C.x);
