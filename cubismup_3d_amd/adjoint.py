@@ -84,15 +84,18 @@ def rollout(u0, grid: Grid, nsteps: int, nu: float, uinf=(0.0, 0.0, 0.0), dts=No
 
 class PenalizedRollout(torch.autograd.Function):
     """Time-averaged drag of a rollout with the penalized sphere (forced at the body velocity
-    V), J = sum_n (dt_n / T) F_x^n with F^n the step's penalization force; differentiable with
-    respect to u0 and V, and, when a centre is given (the sphere is then rebuilt there with the
-    mollified chi), with respect to the sphere centre. Without a centre the grid's chi from
+    V and, with angvel, rotating at W about its centre of mass), J = sum_n (dt_n / T) F_x^n
+    with F^n the step's penalization force; differentiable with respect to u0, V and W, and,
+    when a centre is given (the sphere is then rebuilt there with the mollified chi) and W is
+    zero, with respect to the sphere centre. Without a centre the grid's chi from
     Grid.chi_from_sphere is used as it stands. Fixed timesteps."""
 
     @staticmethod
-    def forward(ctx, u0, vel, centre, grid: Grid, nsteps: int, nu: float, uinf, dts, every: int, lambda_: float,
-                implicit: bool, radius: float, mollify: float):
+    def forward(ctx, u0, vel, centre, angvel, grid: Grid, nsteps: int, nu: float, uinf, dts, every: int,
+                lambda_: float, implicit: bool, radius: float, mollify: float):
         sphere = dict(lambda_=lambda_, implicit=implicit, trans_vel=tuple(float(x) for x in vel.tolist()))
+        if angvel is not None:
+            sphere["ang_vel"] = tuple(float(x) for x in angvel.tolist())
         body = None
         if centre is not None:
             body = (tuple(float(x) for x in centre.tolist()), float(radius), float(mollify))
@@ -109,18 +112,21 @@ class PenalizedRollout(torch.autograd.Function):
         ctx.grid, ctx.nsteps, ctx.nu, ctx.uinf, ctx.dts, ctx.every = grid, nsteps, nu, uinf, list(dts), every
         ctx.sphere, ctx.T, ctx.ckpts, ctx.u0_dtype, ctx.vdev, ctx.body = sphere, T, ckpts, u0.dtype, vel.device, body
         ctx.cdev = centre.device if centre is not None else None
+        ctx.wdev = angvel.device if angvel is not None else None
         return torch.tensor(J, dtype=torch.float64, device=u0.device)
 
     @staticmethod
     def backward(ctx, gJ):
         grid, every, sphere = ctx.grid, ctx.every, ctx.sphere
         want_c = ctx.body is not None and ctx.needs_input_grad[2]
+        want_w = ctx.wdev is not None and ctx.needs_input_grad[3]
         if ctx.body is not None:
             grid.chi_from_sphere(ctx.body[0], ctx.body[1], mollify=ctx.body[2])
         g = float(gJ)
         lam = torch.zeros((3,) + grid.shape, dtype=grid._torch_dtype(), device="cuda")
         dV = torch.zeros(3, dtype=torch.float64)
         dc = torch.zeros(3, dtype=torch.float64)
+        dW = torch.zeros(3, dtype=torch.float64)
         for s0 in sorted(ctx.ckpts, reverse=True):
             s1 = min(s0 + every, ctx.nsteps)
             states = [ctx.ckpts[s0]]
@@ -132,27 +138,31 @@ class PenalizedRollout(torch.autograd.Function):
                 grid.set_torch_velocity(states[n - s0])
                 grid.set_torch_adjoint(lam)
                 out = grid.adjoint_step_body(ctx.dts[n], ctx.nu, ctx.uinf, sphere, g * ctx.dts[n] / ctx.T,
-                                             centre=want_c)
+                                             centre=want_c, rotation=True)
+                dV += torch.from_numpy(out[0])
+                dW += torch.from_numpy(out[1])
                 if want_c:
-                    dV += torch.from_numpy(out[0])
-                    dc += torch.from_numpy(out[1])
-                else:
-                    dV += torch.from_numpy(out)
+                    dc += torch.from_numpy(out[2])
                 lam = grid.get_torch_adjoint()
         ctx.ckpts = None
         return (lam.to(ctx.u0_dtype), dV.to(ctx.vdev), dc.to(ctx.cdev) if want_c else None,
-                None, None, None, None, None, None, None, None, None, None)
+                dW.to(ctx.wdev) if want_w else None, None, None, None, None, None, None, None, None, None, None)
 
 
 def mean_drag(u0, vel, grid: Grid, nsteps: int, nu: float, uinf, dts, every=10, lambda_=1e6, implicit=True,
-              centre=None, radius=None, mollify=1.0):
+              centre=None, radius=None, mollify=1.0, angvel=None):
     """Time-averaged drag (penalization force, x) of a rollout with the sphere; differentiable
-    with respect to u0 ([3][Z][Y][X], cuda) and vel (the body velocity, 3 values). With centre
-    (3 values) and radius the sphere is rebuilt at centre with the mollified chi of half-width
-    mollify cells, and J is differentiable with respect to the centre as well."""
+    with respect to u0 ([3][Z][Y][X], cuda), vel (the body velocity, 3 values) and angvel (its
+    rotation about the centre of mass, 3 values; None: no rotation). With centre (3 values)
+    and radius the sphere is rebuilt at centre with the mollified chi of half-width mollify
+    cells, and J is differentiable with respect to the centre as well (a body without
+    rotation only)."""
     if centre is not None and radius is None:
         raise ValueError("mean_drag: a centre needs the sphere's radius")
-    return PenalizedRollout.apply(u0, vel, centre, grid, int(nsteps), float(nu), tuple(float(x) for x in uinf),
+    if (centre is not None and centre.requires_grad and angvel is not None
+            and bool(torch.any(torch.as_tensor(angvel) != 0))):
+        raise ValueError("mean_drag: the centre gradient is formed for a body without rotation only")
+    return PenalizedRollout.apply(u0, vel, centre, angvel, grid, int(nsteps), float(nu), tuple(float(x) for x in uinf),
                                   list(dts), int(every), float(lambda_), bool(implicit),
                                   float(radius) if radius is not None else 0.0, float(mollify))
 

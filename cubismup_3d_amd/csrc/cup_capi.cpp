@@ -1028,9 +1028,21 @@ int cup_adjoint_step_forcing(cup_ctx* ctx, double dt, double nu, const double ui
 int cup_adjoint_step_body(cup_ctx* ctx, double dt, double nu, const double uinf[3], double lambda, int implicit,
                           const double body_vel[3], double wdrag, double dJdV[3], double dJdc[3])
 {
+  const double zero[3] = {0, 0, 0};
+  return cup_adjoint_step_body_rot(ctx, dt, nu, uinf, lambda, implicit, body_vel, zero, wdrag, dJdV, nullptr, dJdc);
+}
+
+int cup_adjoint_step_body_rot(cup_ctx* ctx, double dt, double nu, const double uinf[3], double lambda,
+                              int implicit, const double body_vel[3], const double body_angvel[3], double wdrag,
+                              double dJdV[3], double dJdW[3], double dJdc[3])
+{
   CUP_GUARD(ctx);
   if (!c.periodic()) return set_error(c, CUP_E_INVALID, "adjoint_step_body: needs a fully periodic box");
-  if (!uinf || !body_vel || !dJdV || !(dt > 0)) return set_error(c, CUP_E_INVALID, "adjoint_step_body: bad argument");
+  if (!uinf || !body_vel || !body_angvel || !dJdV || !(dt > 0))
+    return set_error(c, CUP_E_INVALID, "adjoint_step_body: bad argument");
+  if (dJdc && (body_angvel[0] != 0 || body_angvel[1] != 0 || body_angvel[2] != 0))
+    return set_error(c, CUP_E_INVALID,
+                     "adjoint_step_body: the centre gradient is formed for a body without rotation only");
   if (!c.body.present) return set_error(c, CUP_E_STATE, "adjoint_step_body: no obstacle");
   if (body_count(c) > 1) return set_error(c, CUP_E_STATE, "adjoint_step_body: one body only");
   if (c.body.ob_udef) return set_error(c, CUP_E_STATE, "adjoint_step_body: the body's udef is not differentiated");
@@ -1074,6 +1086,8 @@ int cup_adjoint_step_body(cup_ctx* ctx, double dt, double nu, const double uinf[
   for (int q = 0; q < 3; ++q) std::swap(c.f[CUP_TMPU + q], c.f[CUP_ADJ_U + q]);
   if (int e = read_scalars(c)) return e;
   for (int d = 0; d < 3; ++d) dJdV[d] = c.sc_host->red[8 + d];
+  if (dJdW)
+    for (int d = 0; d < 3; ++d) dJdW[d] = c.sc_host->red[17 + d];
   if (dJdc)
     for (int d = 0; d < 3; ++d) dJdc[d] = c.sc_host->red[11 + d] + c.sc_host->red[14 + d];
   return CUP_OK;

@@ -542,37 +542,47 @@ __global__ void k_sum_rows(const double* __restrict__ part, int n, int width, do
 // ------------------------------------------------------------ adjoint with a body
 // (no reference counterpart) The transposes of the penalization, of its drag F_x and of the
 // RHS masking with its perimeter correction, for the adjoint step with a forced sphere
-// (velocity V, no rotation). Partials per obstacle block, summed by every workgroup in the
-// same order where a total is needed (deterministic).
+// (velocity V and rotation W about its centre of mass). Partials per obstacle block, summed
+// by every workgroup in the same order where a total is needed (deterministic).
 
-// lam' = (1 - dt pf) lam - wdrag h^3 pf e_x on penalized cells; part[3s+d] += this block's
-// dJ/dV_d = sum dt pf lam_d + wdrag h^3 pf [d = x].
+// lam' = (1 - dt pf) lam - wdrag h^3 pf e_x on penalized cells. The target velocity
+// V + W x r (r = x - CoM) enters as g . (V + W x r) with g_d = dt pf lam_d + wdrag h^3 pf [d = x]:
+// part[6s+d] += this block's dJ/dV_d = sum g_d, part[6s+3+d] += its dJ/dW_d = sum (r x g)_d.
 template <typename T>
 __global__ __launch_bounds__(NT) void k_adj_penalize(const int* __restrict__ ob_ids, const double* __restrict__ ob_chi,
                                                      const T* __restrict__ chi, T* __restrict__ lu, T* __restrict__ lv,
                                                      T* __restrict__ lw, GridArgs g, const DevScalars* __restrict__ sc,
-                                                     double lambda, int implicit, double wdrag,
-                                                     double* __restrict__ part)
+                                                     double lambda, int implicit, double wdrag, double cmx,
+                                                     double cmy, double cmz, double* __restrict__ part)
 {
   const int s = blockIdx.x, b = ob_ids[s];
+  int bx, by, bz;
+  bcoords(g, b, bx, by, bz);
   const double dt = sc->dt, h3 = g.h * g.h * g.h;
   const double lf = implicit ? lambda : 1.0 / dt;
   T* L[3] = {lu, lv, lw};
-  double acc[3] = {0, 0, 0};
+  double acc[6] = {0, 0, 0, 0, 0, 0};
   for (int c = threadIdx.x; c < BS3; c += NT) {
     const size_t o = (size_t)b * BS3 + c;
     const double X = ob_chi[(size_t)s * BS3 + c];
     if (chi[o] > (T)X || !(X > 0)) continue;
     const double pf = implicit ? X * lf / (1 + lf * dt * X) : X * lf;
+    double gv[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       const double l = L[d][o];
       const double dr = d == 0 ? wdrag * h3 * pf : 0.0;
-      acc[d] += dt * pf * l + dr;
+      gv[d] = dt * pf * l + dr;
+      acc[d] += gv[d];
       L[d][o] = (T)((1 - dt * pf) * l - dr);
     }
+    const double px = (bx * BS + (c & 7) + 0.5) * g.h - cmx, py = (by * BS + ((c >> 3) & 7) + 0.5) * g.h - cmy,
+                 pz = (bz * BS + (c >> 6) + 0.5) * g.h - cmz;
+    acc[3] += py * gv[2] - pz * gv[1];
+    acc[4] += pz * gv[0] - px * gv[2];
+    acc[5] += px * gv[1] - py * gv[0];
   }
-  breduce<3>(acc, part + 3 * (size_t)s);
+  breduce<6>(acc, part + 6 * (size_t)s);
 }
 
 // Totals of the forward masking sums (S, pos, neg per block) and the correction factor,
@@ -723,7 +733,7 @@ static int adj_body_t(Ctx& c, AdjBodyPhase phase, const AdjBodyArgs& a)
   double* msum = c.partials + 2 * (size_t)c.nb;  // written by k_rhs_obs_mask
   double* gpart = msum + 3 * (size_t)B.nob;
   double* vpart = gpart + B.nob;
-  double* cpart = vpart + 3 * (size_t)B.nob;
+  double* cpart = vpart + 6 * (size_t)B.nob;
   const double e = B.moll * c.h;
   switch (phase) {
     case ADJB_FWD_MASK: {  // the forward masking of u*: r_new in ob_rhs, touch flags, sums
@@ -749,11 +759,13 @@ static int adj_body_t(Ctx& c, AdjBodyPhase phase, const AdjBodyArgs& a)
                          cpart);
       hipLaunchKernelGGL(k_sum_rows, dim3(3), dim3(NT), 0, c.stream, (const double*)cpart, B.nob, 3, c.sc->red + 11);
       break;
-    case ADJB_PENALIZE:  // penalization and drag, dJ/dV into sc->red[8..10]
+    case ADJB_PENALIZE:  // penalization and drag, dJ/dV into sc->red[8..10], dJ/dW into red[17..19]
       hipLaunchKernelGGL((k_adj_penalize<T>), dim3(B.nob), dim3(NT), 0, c.stream, B.ob_ids, B.ob_chi,
                          (const T*)c.f[CUP_CHI], (T*)c.f[CUP_ADJ_U], (T*)c.f[CUP_ADJ_V], (T*)c.f[CUP_ADJ_W],
-                         c.grid(), c.sc, a.lambda, a.implicit, a.wdrag, vpart);
-      hipLaunchKernelGGL(k_sum_rows, dim3(3), dim3(NT), 0, c.stream, (const double*)vpart, B.nob, 3, c.sc->red + 8);
+                         c.grid(), c.sc, a.lambda, a.implicit, a.wdrag, B.com[0], B.com[1], B.com[2], vpart);
+      hipLaunchKernelGGL(k_sum_rows, dim3(3), dim3(NT), 0, c.stream, (const double*)vpart, B.nob, 6, c.sc->red + 8);
+      hipLaunchKernelGGL(k_sum_rows, dim3(3), dim3(NT), 0, c.stream, (const double*)(vpart + 3), B.nob, 6,
+                         c.sc->red + 17);
       break;
     case ADJB_MASK: {  // the masking's transpose on the adjoint pressure; with the centre, red[14..16]
       const double yscale = a.dt * a.dt / (c.h * c.h * c.h);

@@ -155,15 +155,19 @@ class Grid:
         self._c(L.lib().cup_adjoint_step_forcing(self._h, float(dt), float(nu), L.vec3(uinf), int(scheme),
                                                  float(tke_target), float(injection_rate)))
 
-    def adjoint_step_body(self, dt, nu, uinf, sphere, wdrag=0.0, centre=False):
-        """With the sphere: adj <- dJ/du_n; returns this step's dJ/d(body velocity), J holding
-        wdrag times the step's drag (penalization force, x). centre=True (mollified chi) returns
-        (dJ/dV, dJ/d(sphere centre))."""
-        out, dc = np.zeros(3), np.zeros(3)
-        self._c(L.lib().cup_adjoint_step_body(self._h, float(dt), float(nu), L.vec3(uinf), float(sphere["lambda_"]),
-                                              int(bool(sphere["implicit"])), L.vec3(sphere["trans_vel"]),
-                                              float(wdrag), L.dptr(out), L.dptr(dc) if centre else None))
-        return (out, dc) if centre else out
+    def adjoint_step_body(self, dt, nu, uinf, sphere, wdrag=0.0, centre=False, rotation=False):
+        """With the sphere (rotating at sphere["ang_vel"] about its centre of mass when that is
+        given): adj <- dJ/du_n; returns this step's dJ/d(body velocity), J holding wdrag times
+        the step's drag (penalization force, x). rotation=True returns (dJ/dV, dJ/d(ang_vel));
+        centre=True (mollified chi, no rotation) appends dJ/d(sphere centre)."""
+        out, dw, dc = np.zeros(3), np.zeros(3), np.zeros(3)
+        self._c(L.lib().cup_adjoint_step_body_rot(
+            self._h, float(dt), float(nu), L.vec3(uinf), float(sphere["lambda_"]), int(bool(sphere["implicit"])),
+            L.vec3(sphere["trans_vel"]), L.vec3(sphere.get("ang_vel", (0.0, 0.0, 0.0))), float(wdrag), L.dptr(out),
+            L.dptr(dw), L.dptr(dc) if centre else None))
+        res = (out, dw) if rotation else (out,)
+        res = res + (dc,) if centre else res
+        return res if len(res) > 1 else out
 
     def last_body_force(self) -> np.ndarray:
         """Penalization force and torque of the last run_steps step with the sphere."""

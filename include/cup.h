@@ -328,7 +328,7 @@ int cup_adjoint_step(cup_ctx* ctx, double dt, double nu, const double uinf[3], i
  * cup_adjoint_step. Synchronous (two scalar read-backs). */
 int cup_adjoint_step_forcing(cup_ctx* ctx, double dt, double nu, const double uinf[3], int scheme,
                              double tke_target, double injection_rate);
-/* The same with the sphere (forced at body_vel, no rotation; chi from cup_chi_from_sphere),
+/* The same with the sphere (forced at body_vel; chi from cup_chi_from_sphere),
  * no reference counterpart. The step adds the implicit (or explicit) penalization of
  * Penalization.cpp:40-98 after the projection, whose RHS is masked and corrected on the
  * body as in PressureRHS.cpp:78-112,322-343. The objective may hold wdrag times this step's
@@ -339,6 +339,16 @@ int cup_adjoint_step_forcing(cup_ctx* ctx, double dt, double nu, const double ui
  * RHS masking, whose surface flags are held; this re-solves the forward pressure once. */
 int cup_adjoint_step_body(cup_ctx* ctx, double dt, double nu, const double uinf[3], double lambda, int implicit,
                           const double body_vel[3], double wdrag, double dJdV[3], double dJdc[3]);
+/* cup_adjoint_step_body for a sphere that also rotates at body_angvel about its centre of mass
+ * (the penalization's target V + Omega x (x - CoM), Penalization.cpp:74-80), no reference
+ * counterpart. dJdW (may be NULL) returns this step's dJ/dOmega = sum over penalized cells of
+ * (x - CoM) x g, g_d = dt pf lam_d + wdrag h^3 pf [d = x] the cell's dJ/d(target velocity).
+ * dJdc must be NULL when body_angvel is nonzero (CUP_E_INVALID: the lever arms move with the
+ * centre, which this gradient does not differentiate). cup_adjoint_step_body is this call with
+ * body_angvel = 0 and dJdW = NULL. */
+int cup_adjoint_step_body_rot(cup_ctx* ctx, double dt, double nu, const double uinf[3], double lambda,
+                              int implicit, const double body_vel[3], const double body_angvel[3], double wdrag,
+                              double dJdV[3], double dJdW[3], double dJdc[3]);
 /* Penalization force and torque (6 values) of the last step cup_run_steps made with a body. */
 int cup_last_body_force(cup_ctx* ctx, double out[6]);
 /* Several bodies in cup_run_steps (has_body): body 0 is penalized with the params' body_vel and

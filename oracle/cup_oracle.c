@@ -2425,9 +2425,19 @@ void orc_adjoint_step(orc_sim* s, double dt, double nu, const double uinf[3], do
 void orc_adjoint_step_body(orc_sim* s, double dt, double nu, const double uinf[3], double lambda, int implicit,
                            const double V[3], double wdrag, double* lam, double dJdV[3], double dJdc[3])
 {
+  const double W[3] = {0, 0, 0};
+  orc_adjoint_step_body_rot(s, dt, nu, uinf, lambda, implicit, V, W, wdrag, lam, dJdV, NULL, dJdc);
+}
+
+void orc_adjoint_step_body_rot(orc_sim* s, double dt, double nu, const double uinf[3], double lambda, int implicit,
+                               const double V[3], const double W[3], double wdrag, double* lam, double dJdV[3],
+                               double dJdW[3], double dJdc[3])
+{
   if (!s->has_sphere) { orc_adjoint_step(s, dt, nu, uinf, lam); return; }
-  const int want_c = dJdc != NULL && s->moll > 0;
-  if (dJdc && !want_c) dJdc[0] = dJdc[1] = dJdc[2] = NAN;  /* Towers' chi: no centre gradient */
+  const int rot = W[0] != 0 || W[1] != 0 || W[2] != 0;
+  /* Towers' chi, or a rotating body (its lever arms move with the centre): no centre gradient */
+  const int want_c = dJdc != NULL && s->moll > 0 && !rot;
+  if (dJdc && !want_c) dJdc[0] = dJdc[1] = dJdc[2] = NAN;
   const size_t nc = (size_t)s->nb * BS * BS * BS;
   const double h = s->h, h3 = h * h * h;
   /* forward state u_n kept aside; u* = A(u_n) for the masking's linearisation point */
@@ -2499,16 +2509,18 @@ void orc_adjoint_step_body(orc_sim* s, double dt, double nu, const double uinf[3
   }
   for (int q = 0; q < 4; q++) orc_set_field(s, keep[q], save + q * nc);  /* u_n back */
   /* penalization and drag: lam' = (1 - dt pf) lam - wdrag h^3 pf e_x on penalized cells; with
-   * the centre, dJ/dchi = dpf/dchi (dt sum_d lam_d (V_d - u_in,d) + wdrag h^3 (V_x - u_in,x)) */
+   * the centre, dJ/dchi = dpf/dchi (dt sum_d lam_d (V_d - u_in,d) + wdrag h^3 (V_x - u_in,x)).
+   * The target velocity V + W x r (r = x - CoM) enters as g . (V + W x r) with g_d =
+   * dt pf lam_d + wdrag h^3 pf [d = x]: dJ/dV = sum g, dJ/dW = sum r x g. */
   const double lf = implicit ? lambda : 1.0 / dt;
-  double* pv = (double*)calloc(6 * (size_t)s->nb, sizeof(double));
+  double* pv = (double*)calloc(9 * (size_t)s->nb, sizeof(double));
   #pragma omp parallel for schedule(static)
   for (int b = 0; b < s->nb; b++) {
     const OBlock* o = s->ob[b];
     if (!o) continue;
     int bc[3];
     bcoords(s, b, bc);
-    double a[6] = {0, 0, 0, 0, 0, 0};
+    double a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int iz = 0; iz < BS; iz++)
       for (int iy = 0; iy < BS; iy++)
         for (int ix = 0; ix < BS; ix++) {
@@ -2526,18 +2538,27 @@ void orc_adjoint_step_body(orc_sim* s, double dt, double nu, const double uinf[3
             moll_dchi_dc(s, p, dX);
             for (int k = 0; k < 3; k++) a[3 + k] += dpf * gx * dX[k];
           }
+          double gv[3];
           for (int d = 0; d < 3; d++) {
-            a[d] += dt * pf * lam[d * nc + c] + (d == 0 ? wdrag * h3 * pf : 0.0);
+            gv[d] = dt * pf * lam[d * nc + c] + (d == 0 ? wdrag * h3 * pf : 0.0);
+            a[d] += gv[d];
             lam[d * nc + c] = (1 - dt * pf) * lam[d * nc + c] - (d == 0 ? wdrag * h3 * pf : 0.0);
           }
+          double r[3];
+          cell_pos(s, bc, ix, iy, iz, r);
+          for (int d = 0; d < 3; d++) r[d] -= s->CoM[d];
+          a[6] += r[1] * gv[2] - r[2] * gv[1];
+          a[7] += r[2] * gv[0] - r[0] * gv[2];
+          a[8] += r[0] * gv[1] - r[1] * gv[0];
         }
-    for (int d = 0; d < 6; d++) pv[6 * b + d] = a[d];
+    for (int d = 0; d < 9; d++) pv[9 * b + d] = a[d];
   }
   double dc[3] = {0, 0, 0};
   for (int b = 0; b < s->nb; b++)
     for (int d = 0; d < 3; d++) {
-      dJdV[d] += pv[6 * b + d];
-      dc[d] += pv[6 * b + 3 + d];
+      dJdV[d] += pv[9 * b + d];
+      dc[d] += pv[9 * b + 3 + d];
+      if (dJdW) dJdW[d] += pv[9 * b + 6 + d];
     }
   free(pv);
   free(uin);

@@ -62,6 +62,7 @@ def lib():
         "orc_adjoint_step_scheme": (None, [p, d, d, dp, i, dp]),
         "orc_adjoint_step_forcing": (None, [p, d, d, dp, i, d, d, dp]),
         "orc_adjoint_step_body": (None, [p, d, d, dp, d, i, dp, d, dp, dp, dp]),
+        "orc_adjoint_step_body_rot": (None, [p, d, d, dp, d, i, dp, dp, d, dp, dp, dp, dp]),
         "orc_pressure_rhs": (None, [p, d]),
         "orc_poisson_solve": (None, [p]),
         "orc_grad_p": (None, [p, d]),
@@ -284,18 +285,27 @@ class OracleSim:
         n = buf.size // 3
         return tuple(from_blocks(buf[q * n:(q + 1) * n], self.shape) for q in range(3))
 
-    def adjoint_step_body(self, dt, nu, lam, vel, lambda_, implicit, wdrag, uinf=(0, 0, 0), centre=False):
+    def adjoint_step_body(self, dt, nu, lam, vel, lambda_, implicit, wdrag, uinf=(0, 0, 0), centre=False,
+                          ang_vel=None):
         """With the sphere: (dJ/du_n, dJ/dV of this step) from lam = dJ/du_{n+1}, J containing
         wdrag * F_x of this step (F the penalization force); u_n is this sim's velocity. With
-        centre=True (mollified chi) also this step's dJ/d(sphere centre) as a third value."""
+        ang_vel (the sphere rotating about its centre of mass) this step's dJ/d(ang_vel) follows
+        dJ/dV. With centre=True (mollified chi) also this step's dJ/d(sphere centre) as the last
+        value (NaN with a nonzero ang_vel)."""
         buf = np.concatenate([to_blocks(np.asarray(f, dtype=np.float64)) for f in lam])
-        dv, dc = np.zeros(3), np.zeros(3)
-        lib().orc_adjoint_step_body(self._h, float(dt), float(nu), _dp(_vec(uinf)), float(lambda_),
-                                    int(bool(implicit)), _dp(_vec(vel)), float(wdrag), _dp(buf), _dp(dv),
-                                    _dp(dc) if centre else None)
+        dv, dw, dc = np.zeros(3), np.zeros(3), np.zeros(3)
+        if ang_vel is None:
+            lib().orc_adjoint_step_body(self._h, float(dt), float(nu), _dp(_vec(uinf)), float(lambda_),
+                                        int(bool(implicit)), _dp(_vec(vel)), float(wdrag), _dp(buf), _dp(dv),
+                                        _dp(dc) if centre else None)
+        else:
+            lib().orc_adjoint_step_body_rot(self._h, float(dt), float(nu), _dp(_vec(uinf)), float(lambda_),
+                                            int(bool(implicit)), _dp(_vec(vel)), _dp(_vec(ang_vel)), float(wdrag),
+                                            _dp(buf), _dp(dv), _dp(dw), _dp(dc) if centre else None)
         n = buf.size // 3
         lam0 = tuple(from_blocks(buf[q * n:(q + 1) * n], self.shape) for q in range(3))
-        return (lam0, dv, dc) if centre else (lam0, dv)
+        out = (lam0, dv) if ang_vel is None else (lam0, dv, dw)
+        return out + (dc,) if centre else out
 
     def hit_analysis(self, nu):
         """HIT analysis: dict of tke, eps, uprime, lambda, Re_lambda, l_integral, tau_integral,

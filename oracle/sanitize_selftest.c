@@ -157,6 +157,10 @@ static void bodies(void)
   double dJdV[3] = {0, 0, 0}, dJdc[3] = {0, 0, 0};
   orc_adjoint_step_body(s, 1e-3, 0.01, uinf, 1e5, 1, V, 1.0, lam, dJdV, dJdc);
   EXPECT(isfinite(dJdV[0]) && isfinite(dJdc[0]));
+  const double Wr[3] = {0.1, -0.2, 0.3};
+  double dJdW[3] = {0, 0, 0};
+  orc_adjoint_step_body_rot(s, 1e-3, 0.01, uinf, 1e5, 1, V, Wr, 1.0, lam, dJdV, dJdW, dJdc);
+  EXPECT(isfinite(dJdW[0]) && isfinite(dJdW[2]) && isnan(dJdc[0]));
   orc_params p;
   memset(&p, 0, sizeof(p));
   p.nu = 0.01;

@@ -1,8 +1,9 @@
-"""The adjoint through penalization (cup_adjoint_step_body, the PenalizedRollout built on
-it) against the oracle's (pinned by the dot-product identity with central differences of
-the forward step with the sphere), and, at the 256 x 128 x 128 sphere case, the gradient of
-the time-averaged drag with respect to the body velocity against finite differences of the
-FP64 forward."""
+"""The adjoint through penalization (cup_adjoint_step_body and its rotating-body form
+cup_adjoint_step_body_rot, the PenalizedRollout built on them) against the oracle's (pinned
+by the dot-product identity with central differences of the forward step with the sphere),
+and, at the 256 x 128 x 128 sphere case, the gradient of the time-averaged drag with respect
+to the body velocity and centre (at 128 x 64 x 64, its rotation) against finite differences
+of the FP64 forward."""
 import numpy as np
 import pytest
 
@@ -72,6 +73,88 @@ def test_adjoint_step_body_centre_matches_oracle(cup, oracle, decomp, mollify):
     for n, b in zip(("adjU", "adjV", "adjW"), lam0):
         assert rel(G.get(n), b) < 1e-11
     np.testing.assert_allclose(dv2, dvo, rtol=1e-11, atol=1e-12 * np.abs(dvo).max())
+
+
+@pytest.mark.parametrize("decomp", ["plain", "fused"])
+def test_adjoint_step_rotating_body_matches_oracle(cup, oracle, decomp):
+    """The sphere rotating at W about its centre of mass: lam_n, dJ/dV and dJ/dW against the
+    oracle's (pinned by central differences of its forward step in u, V and W)."""
+    print(f"adjoint step with a rotating sphere decomp={decomp}", flush=True)
+    rng = np.random.default_rng(47)
+    bpd, ext = (4, 2, 2), 2.0
+    G = cup.Grid(bpd, maxextent=ext, precision=8, decomposition=decomp)
+    O = oracle.OracleSim(bpd, maxextent=ext)
+    G.chi_from_sphere((0.9, 0.5, 0.5), 0.3)
+    O.sphere_create(0.3, (0.9, 0.5, 0.5))
+    shape = G.shape
+    U, V, W = (1.0, 0.0, 0.0), (-1.0, 0.1, 0.0), (0.7, -1.3, 2.1)
+    u = tuple(0.3 * rng.standard_normal(shape) + (1.0 if q == 0 else 0.0) for q in range(3))
+    lam = tuple(rng.standard_normal(shape) for _ in range(3))
+    dt, nu, w = 0.2 * G.h, 1e-3, 3.7
+    sph = dict(lambda_=1e6, implicit=True, trans_vel=V, ang_vel=W)
+    G.set_velocity(*u); O.set_velocity(*u)
+    for n, f in zip(("adjU", "adjV", "adjW"), lam):
+        G.set(n, f)
+    dv, dw = G.adjoint_step_body(dt, nu, U, sph, w, rotation=True)
+    lam0, dvo, dwo = O.adjoint_step_body(dt, nu, lam, V, 1e6, True, w, U, ang_vel=W)
+    for n, b in zip(("adjU", "adjV", "adjW"), lam0):
+        assert rel(G.get(n), b) < 1e-11
+    np.testing.assert_allclose(dv, dvo, rtol=1e-11, atol=1e-12 * np.abs(dvo).max())
+    print(f"dJ/dW gpu {dw} oracle {dwo}", flush=True)
+    np.testing.assert_allclose(dw, dwo, rtol=1e-10, atol=1e-11 * np.abs(dwo).max())
+    with pytest.raises(ValueError, match="without rotation"):
+        G.adjoint_step_body(dt, nu, U, sph, w, centre=True)
+
+
+def test_mean_drag_rotation_gradient_vs_finite_differences(cup):
+    """Sphere D = 0.2 rotating at W = (0.5, -1, 4) about its centre of mass in a periodic
+    2 x 1 x 1 box at 128 x 64 x 64, inflow U = 1, Re = 200, lambda = 1e6, 40 steps of
+    dt = 2e-3 from rest, checkpoints every 10: dJ/dW of the time-averaged drag, FP32 and FP64
+    adjoints against central differences (eps = 1e-4) of the FP64 forward. The x drag is
+    nearly even in W (a sphere's spin changes it at second order), so dJ/dW is small, about
+    3e-4 of dJ/dV, and J has kinks where the masking's sign pattern changes along the
+    trajectory: steps of 1e-2 and above cross them and miss by up to 1e-3 of |dJ/dW|, steps
+    of 3e-4 and below agree to 1e-6, rounding keeping 1e-4's error near 1e-8."""
+    import torch
+    from cubismup_3d_amd.adjoint import mean_drag
+    bpd, ext, R = (16, 8, 8), 2.0, 0.1
+    c0 = (0.4982, 0.4988, 0.4999)
+    U, nu, nsteps, every = (1.0, 0.0, 0.0), 1.0 * 2 * R / 200, 40, 10
+    V0, W0 = np.array([-1.0, 0.0, 0.0]), np.array([0.5, -1.0, 4.0])
+    dts = [2e-3] * nsteps
+    gradw, gradv = {}, {}
+    for prec in (4, 8):
+        G = cup.Grid(bpd, maxextent=ext, precision=prec, decomposition="fused")
+        G.chi_from_sphere(c0, R)
+        dtype = torch.float32 if prec == 4 else torch.float64
+        u0 = torch.zeros((3,) + G.shape, dtype=dtype, device="cuda", requires_grad=True)
+        V = torch.tensor(V0, dtype=torch.float64, requires_grad=True)
+        W = torch.tensor(W0, dtype=torch.float64, requires_grad=True)
+        J = mean_drag(u0, V, G, nsteps, nu, U, dts, every=every, angvel=W)
+        J.backward()
+        gradw[prec], gradv[prec] = W.grad.numpy().copy(), V.grad.numpy().copy()
+        print(f"precision {prec}: mean drag {J.item():.8e}, dJ/dV {gradv[prec]}, dJ/dW {gradw[prec]}", flush=True)
+    G8 = cup.Grid(bpd, maxextent=ext, precision=8, decomposition="fused")
+    G8.chi_from_sphere(c0, R)
+
+    def J64(Wv):
+        G8.set_velocity(*(np.zeros(G8.shape) for _ in range(3)))
+        sph = dict(lambda_=1e6, implicit=True, trans_vel=tuple(V0), ang_vel=tuple(Wv))
+        J = 0.0
+        for dt in dts:
+            G8.run_steps(1, nu=nu, uinf=U, fixed_dt=dt, sphere=sph)
+            J += dt / sum(dts) * G8.last_body_force()[0]
+        return J
+    eps = 1e-4
+    wscale = np.linalg.norm(gradw[8])
+    assert wscale > 0
+    for k in range(3):
+        e = np.eye(3)[k] * eps
+        fd = (J64(W0 + e) - J64(W0 - e)) / (2 * eps)
+        print(f"dJ/dW_{k}: fd {fd:.10e} fp64 adjoint {gradw[8][k]:.10e} ({abs(gradw[8][k] - fd) / wscale:.1e}) "
+              f"fp32 adjoint {gradw[4][k]:.10e} ({abs(gradw[4][k] - fd) / wscale:.1e} of |dJ/dW|)", flush=True)
+        assert abs(gradw[8][k] - fd) < 1e-6 * wscale
+        assert abs(gradw[4][k] - fd) < 1e-4 * wscale
 
 
 def test_centre_gradient_needs_mollified_chi(cup):

@@ -100,11 +100,11 @@ def test_adjoint_of_pure_projection_is_symmetric(oracle):
     assert max(np.max(np.abs(x - y)) for x, y in zip(ppa, pa)) < 1e-12 * max(np.max(np.abs(x)) for x in pa)
 
 
-def fwd_step_body(oracle, bpd, u, dt, nu, U, V, lam_, ext, centre=(0.9, 0.5, 0.5), mollify=0.0):
+def fwd_step_body(oracle, bpd, u, dt, nu, U, V, lam_, ext, centre=(0.9, 0.5, 0.5), mollify=0.0, W=(0, 0, 0)):
     s = oracle.OracleSim(bpd, maxextent=ext)
     s.sphere_create(0.3, centre, mollify=mollify)
     s.set_velocity(*u)
-    out = s.timestep(dt, nu, uinf=U, sphere=dict(lambda_=lam_, implicit=True, trans_vel=V))
+    out = s.timestep(dt, nu, uinf=U, sphere=dict(lambda_=lam_, implicit=True, trans_vel=V, ang_vel=W))
     return s.velocity(), out["penal_ft"][0]
 
 
@@ -175,6 +175,64 @@ def test_adjoint_sphere_centre_dot_product_identity(oracle, mollify):
         fdc = (J(u, V, c0 + e) - J(u, V, c0 - e)) / (2 * ec)
         print(f"mollify {mollify} dJ/dc_{k}: fd {fdc:.10e} adjoint {dJdc[k]:.10e}")
         assert abs(fdc - dJdc[k]) < 1e-6 * max(abs(fdc), 1.0)
+
+
+@pytest.mark.parametrize("lam_", [1e4, 1e6])
+def test_adjoint_rotating_sphere_dot_product_identity(oracle, lam_):
+    """The sphere rotating at W about its centre of mass (target V + W x (x - CoM)): the
+    adjoint's dJ/du_n, dJ/dV and dJ/dW against central differences of the forward oracle step
+    (J = <lam, u_{n+1}> + w F_x, eps = 1e-5, agreement 1e-7). A lever-arm sign or a swapped
+    cross-product component fails the dJ/dW comparison."""
+    rng = np.random.default_rng(47)
+    bpd, ext = (4, 2, 2), 2.0
+    s = oracle.OracleSim(bpd, maxextent=ext)
+    s.sphere_create(0.3, (0.9, 0.5, 0.5))
+    shape = s.shape
+    U, V, W = (1.0, 0.0, 0.0), (-1.0, 0.1, 0.0), (0.7, -1.3, 2.1)
+    u = tuple(0.3 * rng.standard_normal(shape) + (1.0 if q == 0 else 0.0) for q in range(3))
+    d = tuple(rng.standard_normal(shape) for _ in range(3))
+    lam = tuple(rng.standard_normal(shape) for _ in range(3))
+    dt, nu, w, eps = 0.2 * s.h, 1e-3, 3.7, 1e-5
+
+    def J(uu, VV, WW):
+        u1, F = fwd_step_body(oracle, bpd, uu, dt, nu, U, VV, lam_, ext, W=WW)
+        return dot(lam, u1) + w * F
+    s.set_velocity(*u)
+    lam0, dJdV, dJdW = s.adjoint_step_body(dt, nu, lam, V, lam_, True, w, U, ang_vel=W)
+    fd = (J(tuple(a + eps * b for a, b in zip(u, d)), V, W) - J(tuple(a - eps * b for a, b in zip(u, d)), V, W)) / (2 * eps)
+    assert abs(fd - dot(d, lam0)) < 1e-7 * abs(fd)
+    for k in range(3):
+        e = np.eye(3)[k] * eps
+        fdv = (J(u, tuple(np.add(V, e)), W) - J(u, tuple(np.subtract(V, e)), W)) / (2 * eps)
+        assert abs(fdv - dJdV[k]) < 1e-7 * max(abs(fdv), 1e-3)
+        fdw = (J(u, V, tuple(np.add(W, e))) - J(u, V, tuple(np.subtract(W, e)))) / (2 * eps)
+        assert abs(fdw - dJdW[k]) < 1e-7 * max(abs(fdw), 1e-3)
+    # the lever arms are not all zero: W's gradient is not a multiple of V's
+    assert np.max(np.abs(dJdW)) > 1e-3 * np.max(np.abs(dJdV))
+
+
+def test_adjoint_rotation_zero_matches_the_translating_body(oracle):
+    """W = 0 through the rotating entry point: lam_n and dJ/dV as the translating body's."""
+    rng = np.random.default_rng(5)
+    s = oracle.OracleSim((2, 2, 2), maxextent=1.0)
+    s.sphere_create(0.2, (0.5, 0.5, 0.5))
+    u = tuple(0.3 * rng.standard_normal(s.shape) for _ in range(3))
+    lam = tuple(rng.standard_normal(s.shape) for _ in range(3))
+    s.set_velocity(*u)
+    a0, av = s.adjoint_step_body(1e-3, 1e-3, lam, (0.2, 0, 0), 1e5, True, 1.5)
+    s.set_velocity(*u)
+    b0, bv, bw = s.adjoint_step_body(1e-3, 1e-3, lam, (0.2, 0, 0), 1e5, True, 1.5, ang_vel=(0, 0, 0))
+    assert all(np.array_equal(x, y) for x, y in zip(a0, b0)) and np.array_equal(av, bv)
+    assert np.all(np.isfinite(bw))
+
+
+def test_adjoint_centre_is_nan_with_rotation(oracle):
+    s = oracle.OracleSim((2, 2, 2), maxextent=1.0)
+    s.sphere_create(0.2, (0.5, 0.5, 0.5), mollify=1.0)
+    s.set_velocity(*(np.zeros(s.shape) for _ in range(3)))
+    lam = tuple(np.zeros(s.shape) for _ in range(3))
+    _, _, _, dc = s.adjoint_step_body(1e-3, 1e-3, lam, (0, 0, 0), 1e6, True, 1.0, centre=True, ang_vel=(0, 0, 1))
+    assert np.isnan(dc).all()
 
 
 def test_adjoint_centre_is_nan_with_towers_chi(oracle):
