@@ -88,12 +88,16 @@ class PenalizedRollout(torch.autograd.Function):
     with F^n the step's penalization force; differentiable with respect to u0, V and W, and,
     when a centre is given (the sphere is then rebuilt there with the mollified chi) and W is
     zero, with respect to the sphere centre. Without a centre the grid's chi from
-    Grid.chi_from_sphere is used as it stands. Fixed timesteps."""
+    Grid.chi_from_sphere is used as it stands. The steps advect with any of the three schemes
+    and may apply the linear forcing after the advection. Fixed timesteps."""
 
     @staticmethod
     def forward(ctx, u0, vel, centre, angvel, grid: Grid, nsteps: int, nu: float, uinf, dts, every: int,
-                lambda_: float, implicit: bool, radius: float, mollify: float):
+                lambda_: float, implicit: bool, radius: float, mollify: float, scheme: int, forcing: tuple):
+        """forcing: (tke_target, injection_rate), both 0 for none."""
         sphere = dict(lambda_=lambda_, implicit=implicit, trans_vel=tuple(float(x) for x in vel.tolist()))
+        fkw = dict(spectral_forcing=any(f > 0 for f in forcing), tke_target=forcing[0], injection_rate=forcing[1],
+                   scheme=scheme)
         if angvel is not None:
             sphere["ang_vel"] = tuple(float(x) for x in angvel.tolist())
         body = None
@@ -105,7 +109,7 @@ class PenalizedRollout(torch.autograd.Function):
         ckpts = {0: u0.detach().clone()}
         J = 0.0
         for n in range(nsteps):
-            grid.run_steps(1, nu=nu, uinf=uinf, fixed_dt=dts[n], sphere=sphere)
+            grid.run_steps(1, nu=nu, uinf=uinf, fixed_dt=dts[n], sphere=sphere, **fkw)
             J += dts[n] / T * grid.last_body_force()[0]
             if (n + 1) % every == 0 and n + 1 < nsteps:
                 ckpts[n + 1] = grid.get_torch_velocity()
@@ -113,6 +117,7 @@ class PenalizedRollout(torch.autograd.Function):
         ctx.sphere, ctx.T, ctx.ckpts, ctx.u0_dtype, ctx.vdev, ctx.body = sphere, T, ckpts, u0.dtype, vel.device, body
         ctx.cdev = centre.device if centre is not None else None
         ctx.wdev = angvel.device if angvel is not None else None
+        ctx.fkw, ctx.forcing, ctx.scheme = fkw, forcing, scheme
         return torch.tensor(J, dtype=torch.float64, device=u0.device)
 
     @staticmethod
@@ -132,13 +137,14 @@ class PenalizedRollout(torch.autograd.Function):
             states = [ctx.ckpts[s0]]
             grid.set_torch_velocity(ctx.ckpts[s0])
             for n in range(s0, s1 - 1):
-                grid.run_steps(1, nu=ctx.nu, uinf=ctx.uinf, fixed_dt=ctx.dts[n], sphere=sphere)
+                grid.run_steps(1, nu=ctx.nu, uinf=ctx.uinf, fixed_dt=ctx.dts[n], sphere=sphere, **ctx.fkw)
                 states.append(grid.get_torch_velocity())
             for n in reversed(range(s0, s1)):
                 grid.set_torch_velocity(states[n - s0])
                 grid.set_torch_adjoint(lam)
                 out = grid.adjoint_step_body(ctx.dts[n], ctx.nu, ctx.uinf, sphere, g * ctx.dts[n] / ctx.T,
-                                             centre=want_c, rotation=True)
+                                             centre=want_c, rotation=True, scheme=ctx.scheme,
+                                             tke_target=ctx.forcing[0], injection_rate=ctx.forcing[1])
                 dV += torch.from_numpy(out[0])
                 dW += torch.from_numpy(out[1])
                 if want_c:
@@ -146,25 +152,30 @@ class PenalizedRollout(torch.autograd.Function):
                 lam = grid.get_torch_adjoint()
         ctx.ckpts = None
         return (lam.to(ctx.u0_dtype), dV.to(ctx.vdev), dc.to(ctx.cdev) if want_c else None,
-                dW.to(ctx.wdev) if want_w else None, None, None, None, None, None, None, None, None, None, None)
+                dW.to(ctx.wdev) if want_w else None, None, None, None, None, None, None, None, None, None, None,
+                None, None)
 
 
 def mean_drag(u0, vel, grid: Grid, nsteps: int, nu: float, uinf, dts, every=10, lambda_=1e6, implicit=True,
-              centre=None, radius=None, mollify=1.0, angvel=None):
+              centre=None, radius=None, mollify=1.0, angvel=None, scheme=0, tke_target=0.0, injection_rate=0.0):
     """Time-averaged drag (penalization force, x) of a rollout with the sphere; differentiable
     with respect to u0 ([3][Z][Y][X], cuda), vel (the body velocity, 3 values) and angvel (its
     rotation about the centre of mass, 3 values; None: no rotation). With centre (3 values)
     and radius the sphere is rebuilt at centre with the mollified chi of half-width mollify
     cells, and J is differentiable with respect to the centre as well (a body without
-    rotation only)."""
+    rotation only). scheme: 0 central Euler, 1 upwind Euler, 2 upwind midpoint; tke_target or
+    injection_rate > 0: each step applies the linear forcing after the advection."""
     if centre is not None and radius is None:
         raise ValueError("mean_drag: a centre needs the sphere's radius")
+    if tke_target < 0 or injection_rate < 0:
+        raise ValueError("tke_target and injection_rate must be >= 0")
     if (centre is not None and centre.requires_grad and angvel is not None
             and bool(torch.any(torch.as_tensor(angvel) != 0))):
         raise ValueError("mean_drag: the centre gradient is formed for a body without rotation only")
     return PenalizedRollout.apply(u0, vel, centre, angvel, grid, int(nsteps), float(nu), tuple(float(x) for x in uinf),
                                   list(dts), int(every), float(lambda_), bool(implicit),
-                                  float(radius) if radius is not None else 0.0, float(mollify))
+                                  float(radius) if radius is not None else 0.0, float(mollify), int(scheme),
+                                  (float(tke_target), float(injection_rate)))
 
 
 def kinetic_energy(u, h):

@@ -859,7 +859,9 @@ int cup_run_steps(cup_ctx* ctx, const cup_step_params* p, int nsteps)
   const bool body = c.body.present && p->has_body;
   // The band-limited forcing stays on the device when its low-mode set fits the fused
   // kernels; otherwise the reference decomposition of SpectralForcing runs (host round trip).
-  const bool force_fused = p->spectral_forcing && c.low_fused_ok;
+  // With a body the forced u* must be in place before the RHS masking reads it (the fused
+  // forcing adds its modes after), so the reference decomposition runs then as well.
+  const bool force_fused = p->spectral_forcing && c.low_fused_ok && !body;
   for (int s = 0; s < nsteps; ++s) {
     const bool fold = fold_ok && s > 0;
     const StepFold sf{umax_slots_dev(c, c.umax_bank), step0 + s, p->CFL, p->rampup, p->fixed_dt};
@@ -951,65 +953,53 @@ int cup_set_step_state(cup_ctx* ctx, int step, double time)
   return CUP_OK;
 }
 
-int cup_adjoint_step(cup_ctx* ctx, double dt, double nu, const double uinf[3], int scheme)
+// u_n (the velocity fields) kept aside in the forcing adjoint's last three fields while the
+// midpoint form computes u* in their place, and put back.
+static int keep_velocity(Ctx& c, bool back)
 {
-  return cup_adjoint_step_forcing(ctx, dt, nu, uinf, scheme, 0.0, 0.0);
+  const size_t fb = c.ncell * c.esz;
+  char* keep = (char*)c.adjf + 6 * fb;
+  for (int q = 0; q < 3; ++q) {
+    void* dst = back ? c.f[CUP_U + q] : (void*)(keep + q * fb);
+    const void* src = back ? (const void*)(keep + q * fb) : c.f[CUP_U + q];
+    if (int e = check_hip(c, hipMemcpyAsync(dst, src, fb, hipMemcpyDeviceToDevice, c.stream), "keep velocity"))
+      return e;
+  }
+  return CUP_OK;
 }
 
-int cup_adjoint_step_forcing(cup_ctx* ctx, double dt, double nu, const double uinf[3], int scheme,
-                             double tke_target, double injection_rate)
+static int alloc_adjf(Ctx& c)
 {
-  CUP_GUARD(ctx);
-  if (!c.periodic()) return set_error(c, CUP_E_INVALID, "adjoint_step: needs a fully periodic box");
-  if (!uinf || !(dt > 0)) return set_error(c, CUP_E_INVALID, "adjoint_step: dt must be > 0");
-  if (tke_target < 0 || injection_rate < 0)
-    return set_error(c, CUP_E_INVALID, "adjoint_step: tke_target and injection_rate must be >= 0");
-  if (scheme != CUP_CENTRAL_EULER && scheme != CUP_UPWIND3_EULER && scheme != CUP_UPWIND3_RK2)
-    return set_error(c, CUP_E_INVALID, "adjoint_step: unknown scheme");
-  if (c.body.present) return set_error(c, CUP_E_STATE, "adjoint_step: obstacles are not supported");
-  if (scheme == CUP_UPWIND3_RK2)
-    for (int q = 0; q < 3; ++q)
-      if (!c.f[F_ADJ2 + q] &&
-          check_hip(c, hipMalloc(&c.f[F_ADJ2 + q], c.ncell * c.esz), "hipMalloc adjoint scratch"))
-        return CUP_E_NOMEM;
-  for (int f = CUP_ADJ_U; f <= CUP_ADJ_W; ++f)
-    if (int rc = field_ready(c, f, "adjoint_step")) return rc;
-  if (int rc = launch_set_dt(c, dt)) return rc;
-  // lam* = P lam: the forward projection on the adjoint fields (P is symmetric)
+  if (!c.adjf && check_hip(c, hipMalloc(&c.adjf, 9 * c.ncell * c.esz), "hipMalloc forcing adjoint"))
+    return CUP_E_NOMEM;
+  return CUP_OK;
+}
+
+// The adjoint fields += the linear forcing's transpose at u* = A(u_n), the scheme's advection of
+// the velocity fields, which are left as they were.
+static int adjoint_forcing_at_ustar(Ctx& c, double dt, double nu, const double uinf[3], int scheme,
+                                    double tke_target, double injection_rate)
+{
+  if (int e = alloc_adjf(c)) return e;
+  const bool midpoint = scheme == CUP_UPWIND3_RK2;
+  if (midpoint)
+    if (int e = keep_velocity(c, false)) return e;
   const int saved = c.ustar;
-  c.ustar = CUP_ADJ_U;
-  int rc = launch_poisson_fused(c, false, 0.0, 0.0);
+  int e = launch_advdiff(c, nu, uinf, scheme);
+  const int us0 = c.ustar;
   c.ustar = saved;
-  if (rc) return rc;
-  if (int e = launch_gradp_inplace(c, CUP_ADJ_U)) return e;
-  if (tke_target > 0 || injection_rate > 0) {
-    // the forcing's transpose at u* = A(u_n); the midpoint form computes u* in place of u_n,
-    // which is kept aside and put back
-    const size_t fb = c.ncell * c.esz;
-    if (!c.adjf && check_hip(c, hipMalloc(&c.adjf, 9 * fb), "hipMalloc forcing adjoint")) return CUP_E_NOMEM;
-    void* keep = nullptr;
-    if (scheme == CUP_UPWIND3_RK2) {
-      keep = (char*)c.adjf + 6 * fb;
-      for (int q = 0; q < 3; ++q)
-        if (int e = check_hip(c, hipMemcpyAsync((char*)keep + q * fb, c.f[CUP_U + q], fb, hipMemcpyDeviceToDevice,
-                                                c.stream),
-                              "forcing adjoint"))
-          return e;
-    }
-    int e = launch_advdiff(c, nu, uinf, scheme);
-    const int us0 = c.ustar;
-    c.ustar = saved;
-    if (!e) e = launch_adjoint_forcing(c, us0, dt, tke_target, injection_rate);
-    if (keep)  // u_n back in place, also after a failure
-      for (int q = 0; q < 3; ++q) {
-        const int r = check_hip(c, hipMemcpyAsync(c.f[CUP_U + q], (char*)keep + q * fb, fb, hipMemcpyDeviceToDevice,
-                                                  c.stream),
-                                "forcing adjoint");
-        if (!e) e = r;
-      }
-    if (e) return e;
+  if (!e) e = launch_adjoint_forcing(c, us0, dt, tke_target, injection_rate);
+  if (midpoint) {  // u_n back in place, also after a failure
+    const int r = keep_velocity(c, true);
+    if (!e) e = r;
   }
-  // lam_n = A'(u_n)^T lam* into tmp, then tmp and the adjoint fields trade buffers
+  return e;
+}
+
+// lam_n = A'(u_n)^T lam* for the scheme's advection, into tmp; then tmp and the adjoint fields
+// trade buffers.
+static int adjoint_advection(Ctx& c, double nu, const double uinf[3], int scheme)
+{
   if (scheme == CUP_CENTRAL_EULER) {
     if (int e = launch_adjoint_advdiff(c, nu, uinf)) return e;
   } else if (scheme == CUP_UPWIND3_EULER) {
@@ -1025,21 +1015,67 @@ int cup_adjoint_step_forcing(cup_ctx* ctx, double dt, double nu, const double ui
   return CUP_OK;
 }
 
+// Argument checks shared by the adjoint steps; allocates the midpoint form's scratch fields.
+static int adjoint_scheme_ready(Ctx& c, int scheme, double tke_target, double injection_rate)
+{
+  if (tke_target < 0 || injection_rate < 0)
+    return set_error(c, CUP_E_INVALID, "adjoint_step: tke_target and injection_rate must be >= 0");
+  if (scheme != CUP_CENTRAL_EULER && scheme != CUP_UPWIND3_EULER && scheme != CUP_UPWIND3_RK2)
+    return set_error(c, CUP_E_INVALID, "adjoint_step: unknown scheme");
+  if (scheme == CUP_UPWIND3_RK2)
+    for (int q = 0; q < 3; ++q)
+      if (!c.f[F_ADJ2 + q] &&
+          check_hip(c, hipMalloc(&c.f[F_ADJ2 + q], c.ncell * c.esz), "hipMalloc adjoint scratch"))
+        return CUP_E_NOMEM;
+  return CUP_OK;
+}
+
+int cup_adjoint_step(cup_ctx* ctx, double dt, double nu, const double uinf[3], int scheme)
+{
+  return cup_adjoint_step_forcing(ctx, dt, nu, uinf, scheme, 0.0, 0.0);
+}
+
+int cup_adjoint_step_forcing(cup_ctx* ctx, double dt, double nu, const double uinf[3], int scheme,
+                             double tke_target, double injection_rate)
+{
+  CUP_GUARD(ctx);
+  if (!c.periodic()) return set_error(c, CUP_E_INVALID, "adjoint_step: needs a fully periodic box");
+  if (!uinf || !(dt > 0)) return set_error(c, CUP_E_INVALID, "adjoint_step: dt must be > 0");
+  if (c.body.present) return set_error(c, CUP_E_STATE, "adjoint_step: obstacles are not supported");
+  if (int rc = adjoint_scheme_ready(c, scheme, tke_target, injection_rate)) return rc;
+  for (int f = CUP_ADJ_U; f <= CUP_ADJ_W; ++f)
+    if (int rc = field_ready(c, f, "adjoint_step")) return rc;
+  if (int rc = launch_set_dt(c, dt)) return rc;
+  // lam* = P lam: the forward projection on the adjoint fields (P is symmetric)
+  const int saved = c.ustar;
+  c.ustar = CUP_ADJ_U;
+  int rc = launch_poisson_fused(c, false, 0.0, 0.0);
+  c.ustar = saved;
+  if (rc) return rc;
+  if (int e = launch_gradp_inplace(c, CUP_ADJ_U)) return e;
+  if (tke_target > 0 || injection_rate > 0)
+    if (int e = adjoint_forcing_at_ustar(c, dt, nu, uinf, scheme, tke_target, injection_rate)) return e;
+  return adjoint_advection(c, nu, uinf, scheme);
+}
+
 int cup_adjoint_step_body(cup_ctx* ctx, double dt, double nu, const double uinf[3], double lambda, int implicit,
                           const double body_vel[3], double wdrag, double dJdV[3], double dJdc[3])
 {
   const double zero[3] = {0, 0, 0};
-  return cup_adjoint_step_body_rot(ctx, dt, nu, uinf, lambda, implicit, body_vel, zero, wdrag, dJdV, nullptr, dJdc);
+  return cup_adjoint_step_body_full(ctx, dt, nu, uinf, CUP_CENTRAL_EULER, 0.0, 0.0, lambda, implicit, body_vel, zero,
+                                    wdrag, dJdV, nullptr, dJdc);
 }
 
-int cup_adjoint_step_body_rot(cup_ctx* ctx, double dt, double nu, const double uinf[3], double lambda,
-                              int implicit, const double body_vel[3], const double body_angvel[3], double wdrag,
-                              double dJdV[3], double dJdW[3], double dJdc[3])
+int cup_adjoint_step_body_full(cup_ctx* ctx, double dt, double nu, const double uinf[3], int scheme,
+                               double tke_target, double injection_rate, double lambda, int implicit,
+                               const double body_vel[3], const double body_angvel[3], double wdrag, double dJdV[3],
+                               double dJdW[3], double dJdc[3])
 {
   CUP_GUARD(ctx);
   if (!c.periodic()) return set_error(c, CUP_E_INVALID, "adjoint_step_body: needs a fully periodic box");
   if (!uinf || !body_vel || !body_angvel || !dJdV || !(dt > 0))
     return set_error(c, CUP_E_INVALID, "adjoint_step_body: bad argument");
+  if (int rc = adjoint_scheme_ready(c, scheme, tke_target, injection_rate)) return rc;
   if (dJdc && (body_angvel[0] != 0 || body_angvel[1] != 0 || body_angvel[2] != 0))
     return set_error(c, CUP_E_INVALID,
                      "adjoint_step_body: the centre gradient is formed for a body without rotation only");
@@ -1058,9 +1094,19 @@ int cup_adjoint_step_body_rot(cup_ctx* ctx, double dt, double nu, const double u
   a.wdrag = wdrag;
   for (int d = 0; d < 3; ++d) a.vel[d] = body_vel[d];
   a.centre = dJdc != nullptr;
+  const bool forced = tke_target > 0 || injection_rate > 0, midpoint = scheme == CUP_UPWIND3_RK2;
   if (int rc = launch_set_dt(c, dt)) return rc;
-  // forward state at this step: u* = A(u_n) in tmp
-  if (int rc = launch_advdiff(c, nu, uinf, CUP_CENTRAL_EULER)) return rc;
+  // forward state at this step: u* = F(A(u_n)), the scheme's advection (the midpoint form writes
+  // it over u_n, which is kept aside) and the forcing
+  if (midpoint) {
+    if (int rc = alloc_adjf(c)) return rc;
+    if (int rc = keep_velocity(c, false)) return rc;
+  }
+  if (int rc = launch_advdiff(c, nu, uinf, scheme)) return rc;
+  if (forced) {
+    double st[5];
+    if (int rc = forcing_plain(c, dt, dt, nu, tke_target, injection_rate, st, c.ustar)) return rc;
+  }
   if (a.centre) {
     // the penalization's input P_M(u*) needs the forward pressure: masked and corrected RHS,
     // Poisson solve into P; then the chi sensitivity of the penalization and its drag
@@ -1074,16 +1120,21 @@ int cup_adjoint_step_body_rot(cup_ctx* ctx, double dt, double nu, const double u
   // penalization and the drag's weight; dJ/dV of this step
   if (int rc = launch_adjoint_body(c, ADJB_PENALIZE, a)) return rc;
   // transposed projection: y = L^{-1} R lam' (no obstacle RHS), y <- M'^T y, lam* = lam' - (dt/2h) grad y
+  const int us0 = c.ustar;
   c.ustar = CUP_ADJ_U;
   c.plain_rhs = true;
   int rc = launch_poisson_fused(c, false, 0.0, 0.0);
   c.plain_rhs = false;
-  c.ustar = CUP_TMPU;
+  c.ustar = us0;
   if (rc) return rc;
   if (int e = launch_adjoint_body(c, ADJB_MASK, a)) return e;
   if (int e = launch_gradp_inplace(c, CUP_ADJ_U)) return e;
-  if (int e = launch_adjoint_advdiff(c, nu, uinf)) return e;
-  for (int q = 0; q < 3; ++q) std::swap(c.f[CUP_TMPU + q], c.f[CUP_ADJ_U + q]);
+  if (midpoint)
+    if (int e = keep_velocity(c, true)) return e;
+  // the forcing's transpose at A(u_n), then the advection's
+  if (forced)
+    if (int e = adjoint_forcing_at_ustar(c, dt, nu, uinf, scheme, tke_target, injection_rate)) return e;
+  if (int e = adjoint_advection(c, nu, uinf, scheme)) return e;
   if (int e = read_scalars(c)) return e;
   for (int d = 0; d < 3; ++d) dJdV[d] = c.sc_host->red[8 + d];
   if (dJdW)

@@ -335,8 +335,9 @@ int launch_fft_inverse_field(Ctx& c, void* spec, void* real_blocks);        // 3
 // cup_spectral.hip
 int spectral_setup(Ctx& c);
 int launch_ic_hit(Ctx& c, double k0, const double* K, const double* E, int tn, double u_rms, uint64_t seed);
+// field: the first of the three velocity fields forced in place (-1: u* under FUSED, u under PLAIN)
 int forcing_plain(Ctx& c, double dt, double sim_dt, double nu, double tke_target, double inj,
-                  double stats[5]);
+                  double stats[5], int field = -1);
 int launch_forcing_reduce(Ctx& c, double tke_target, double inj_rate);
 int launch_forcing_rows(Ctx& c);
 // adjoint fields += the forcing's transpose at u* (fields us0..us0+2); synchronous

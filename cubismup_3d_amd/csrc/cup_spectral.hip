@@ -306,10 +306,10 @@ __global__ void k_axpy_blocks(T* __restrict__ y, const T* __restrict__ x, size_t
 
 template <typename T>
 static int forcing_plain_t(Ctx& c, double dt, double sim_dt, double nu, double tke_target, double inj_rate,
-                           double stats[5])
+                           double stats[5], int field)
 {
   if (int rc = alloc_extra_spectra(c)) return rc;
-  const int v0 = c.decomposition == CUP_FUSED ? c.ustar : CUP_U;
+  const int v0 = field >= 0 ? field : (c.decomposition == CUP_FUSED ? c.ustar : CUP_U);
   // SpectralForcing::_cub2fftw + runFwd: three forward transforms.
   if (int rc = launch_fft_forward_field(c, c.f[v0], c.spec)) return rc;
   if (int rc = launch_fft_forward_field(c, c.f[v0 + 1], c.spec2)) return rc;
@@ -352,10 +352,11 @@ static int forcing_plain_t(Ctx& c, double dt, double sim_dt, double nu, double t
   return CUP_OK;
 }
 
-int forcing_plain(Ctx& c, double dt, double sim_dt, double nu, double tke_target, double inj, double stats[5])
+int forcing_plain(Ctx& c, double dt, double sim_dt, double nu, double tke_target, double inj, double stats[5],
+                  int field)
 {
-  return c.precision == 4 ? forcing_plain_t<float>(c, dt, sim_dt, nu, tke_target, inj, stats)
-                          : forcing_plain_t<double>(c, dt, sim_dt, nu, tke_target, inj, stats);
+  return c.precision == 4 ? forcing_plain_t<float>(c, dt, sim_dt, nu, tke_target, inj, stats, field)
+                          : forcing_plain_t<double>(c, dt, sim_dt, nu, tke_target, inj, stats, field);
 }
 
 // sum over cells of a0 b0 + a1 b1 + a2 b2 into part[blockIdx.x] (fixed grid: deterministic)

@@ -155,16 +155,20 @@ class Grid:
         self._c(L.lib().cup_adjoint_step_forcing(self._h, float(dt), float(nu), L.vec3(uinf), int(scheme),
                                                  float(tke_target), float(injection_rate)))
 
-    def adjoint_step_body(self, dt, nu, uinf, sphere, wdrag=0.0, centre=False, rotation=False):
+    def adjoint_step_body(self, dt, nu, uinf, sphere, wdrag=0.0, centre=False, rotation=False, scheme=0,
+                          tke_target=0.0, injection_rate=0.0):
         """With the sphere (rotating at sphere["ang_vel"] about its centre of mass when that is
-        given): adj <- dJ/du_n; returns this step's dJ/d(body velocity), J holding wdrag times
-        the step's drag (penalization force, x). rotation=True returns (dJ/dV, dJ/d(ang_vel));
-        centre=True (mollified chi, no rotation) appends dJ/d(sphere centre)."""
+        given): adj <- dJ/du_n for the step with scheme's advection and, with tke_target or
+        injection_rate > 0, the linear forcing after it; returns this step's dJ/d(body
+        velocity), J holding wdrag times the step's drag (penalization force, x).
+        rotation=True returns (dJ/dV, dJ/d(ang_vel)); centre=True (mollified chi, no rotation)
+        appends dJ/d(sphere centre)."""
         out, dw, dc = np.zeros(3), np.zeros(3), np.zeros(3)
-        self._c(L.lib().cup_adjoint_step_body_rot(
-            self._h, float(dt), float(nu), L.vec3(uinf), float(sphere["lambda_"]), int(bool(sphere["implicit"])),
-            L.vec3(sphere["trans_vel"]), L.vec3(sphere.get("ang_vel", (0.0, 0.0, 0.0))), float(wdrag), L.dptr(out),
-            L.dptr(dw), L.dptr(dc) if centre else None))
+        self._c(L.lib().cup_adjoint_step_body_full(
+            self._h, float(dt), float(nu), L.vec3(uinf), int(scheme), float(tke_target), float(injection_rate),
+            float(sphere["lambda_"]), int(bool(sphere["implicit"])), L.vec3(sphere["trans_vel"]),
+            L.vec3(sphere.get("ang_vel", (0.0, 0.0, 0.0))), float(wdrag), L.dptr(out), L.dptr(dw),
+            L.dptr(dc) if centre else None))
         res = (out, dw) if rotation else (out,)
         res = res + (dc,) if centre else res
         return res if len(res) > 1 else out

@@ -339,16 +339,20 @@ int cup_adjoint_step_forcing(cup_ctx* ctx, double dt, double nu, const double ui
  * RHS masking, whose surface flags are held; this re-solves the forward pressure once. */
 int cup_adjoint_step_body(cup_ctx* ctx, double dt, double nu, const double uinf[3], double lambda, int implicit,
                           const double body_vel[3], double wdrag, double dJdV[3], double dJdc[3]);
-/* cup_adjoint_step_body for a sphere that also rotates at body_angvel about its centre of mass
- * (the penalization's target V + Omega x (x - CoM), Penalization.cpp:74-80), no reference
+/* cup_adjoint_step_body for the step with the scheme's advection (CUP_CENTRAL_EULER,
+ * CUP_UPWIND3_EULER, CUP_UPWIND3_RK2), the linear forcing after it (tke_target or
+ * injection_rate > 0, as cup_adjoint_step_forcing; the masking is then linearised at the forced
+ * u*) and a sphere that also rotates at body_angvel about its centre of mass (the
+ * penalization's target V + Omega x (x - CoM), Penalization.cpp:74-80); no reference
  * counterpart. dJdW (may be NULL) returns this step's dJ/dOmega = sum over penalized cells of
  * (x - CoM) x g, g_d = dt pf lam_d + wdrag h^3 pf [d = x] the cell's dJ/d(target velocity).
  * dJdc must be NULL when body_angvel is nonzero (CUP_E_INVALID: the lever arms move with the
  * centre, which this gradient does not differentiate). cup_adjoint_step_body is this call with
- * body_angvel = 0 and dJdW = NULL. */
-int cup_adjoint_step_body_rot(cup_ctx* ctx, double dt, double nu, const double uinf[3], double lambda,
-                              int implicit, const double body_vel[3], const double body_angvel[3], double wdrag,
-                              double dJdV[3], double dJdW[3], double dJdc[3]);
+ * the central scheme, no forcing, body_angvel = 0 and dJdW = NULL. */
+int cup_adjoint_step_body_full(cup_ctx* ctx, double dt, double nu, const double uinf[3], int scheme,
+                               double tke_target, double injection_rate, double lambda, int implicit,
+                               const double body_vel[3], const double body_angvel[3], double wdrag, double dJdV[3],
+                               double dJdW[3], double dJdc[3]);
 /* Penalization force and torque (6 values) of the last step cup_run_steps made with a body. */
 int cup_last_body_force(cup_ctx* ctx, double out[6]);
 /* Several bodies in cup_run_steps (has_body): body 0 is penalized with the params' body_vel and

@@ -2287,6 +2287,8 @@ static void up3_jt(const orc_sim* s, const double* phi, const double* lam, doubl
  * midpoint: u_h = u_n + dt/2 L(u_n), mu = dt L'(u_h)^T lam*, lam_n = lam* + mu + dt/2 L'(u_n)^T mu.
  * scheme 0 is the central Euler step (orc_adjoint_step). */
 static void adj_advection(orc_sim* s, double dt, double nu, const double uinf[3], int scheme, double* lam);
+static void adj_forcing(orc_sim* s, double dt, double nu, const double uinf[3], int scheme, double tke_target,
+                        double injection_rate, double* lam);
 
 void orc_adjoint_step_scheme(orc_sim* s, double dt, double nu, const double uinf[3], int scheme, double* lam)
 {
@@ -2343,9 +2345,18 @@ void orc_adjoint_step_forcing(orc_sim* s, double dt, double nu, const double uin
     orc_adjoint_step_scheme(s, dt, nu, uinf, scheme, lam);
     return;
   }
+  adj_project(s, dt, lam);
+  adj_forcing(s, dt, nu, uinf, scheme, tke_target, injection_rate, lam);
+  adj_advection(s, dt, nu, uinf, scheme, lam);
+}
+
+/* lam <- F'(u*)^T lam for the linear forcing at u* = A(u_n) (the scheme's advection of the
+ * sim's velocity u_n, which is kept): lam += fac P lam + <P u*, lam> grad fac. */
+static void adj_forcing(orc_sim* s, double dt, double nu, const double uinf[3], int scheme, double tke_target,
+                        double injection_rate, double* lam)
+{
   const size_t nc = (size_t)s->nb * BS * BS * BS;
   const double n = (double)s->N[0] * s->N[1] * s->N[2];
-  adj_project(s, dt, lam);
   double* un = (double*)malloc(3 * nc * sizeof(double));
   double* us = (double*)malloc(3 * nc * sizeof(double));
   double* Pu = (double*)malloc(3 * nc * sizeof(double));
@@ -2375,7 +2386,6 @@ void orc_adjoint_step_forcing(orc_sim* s, double dt, double nu, const double uin
   free(us);
   free(Pu);
   free(Pl);
-  adj_advection(s, dt, nu, uinf, scheme, lam);
 }
 
 /* lam <- A'(u_n)^T lam for the scheme's advection, u_n the sim's velocity. */
@@ -2426,25 +2436,32 @@ void orc_adjoint_step_body(orc_sim* s, double dt, double nu, const double uinf[3
                            const double V[3], double wdrag, double* lam, double dJdV[3], double dJdc[3])
 {
   const double W[3] = {0, 0, 0};
-  orc_adjoint_step_body_rot(s, dt, nu, uinf, lambda, implicit, V, W, wdrag, lam, dJdV, NULL, dJdc);
+  orc_adjoint_step_body_full(s, dt, nu, uinf, 0, 0.0, 0.0, lambda, implicit, V, W, wdrag, lam, dJdV, NULL, dJdc);
 }
 
-void orc_adjoint_step_body_rot(orc_sim* s, double dt, double nu, const double uinf[3], double lambda, int implicit,
-                               const double V[3], const double W[3], double wdrag, double* lam, double dJdV[3],
-                               double dJdW[3], double dJdc[3])
+void orc_adjoint_step_body_full(orc_sim* s, double dt, double nu, const double uinf[3], int scheme,
+                                double tke_target, double injection_rate, double lambda, int implicit,
+                                const double V[3], const double W[3], double wdrag, double* lam, double dJdV[3],
+                                double dJdW[3], double dJdc[3])
 {
-  if (!s->has_sphere) { orc_adjoint_step(s, dt, nu, uinf, lam); return; }
+  if (!s->has_sphere) { orc_adjoint_step_forcing(s, dt, nu, uinf, scheme, tke_target, injection_rate, lam); return; }
+  const int forced = tke_target > 0 || injection_rate > 0;
   const int rot = W[0] != 0 || W[1] != 0 || W[2] != 0;
   /* Towers' chi, or a rotating body (its lever arms move with the centre): no centre gradient */
   const int want_c = dJdc != NULL && s->moll > 0 && !rot;
   if (dJdc && !want_c) dJdc[0] = dJdc[1] = dJdc[2] = NAN;
   const size_t nc = (size_t)s->nb * BS * BS * BS;
   const double h = s->h, h3 = h * h * h;
-  /* forward state u_n kept aside; u* = A(u_n) for the masking's linearisation point */
+  /* forward state u_n kept aside; u* = F(A(u_n)), the scheme's advection and the forcing, is
+   * the masking's linearisation point */
   double* save = (double*)malloc(4 * nc * sizeof(double));
   const int keep[4] = {ORC_U, ORC_V, ORC_W, ORC_P};
   for (int q = 0; q < 4; q++) orc_get_field(s, keep[q], save + q * nc);
-  orc_advect_diffuse(s, dt, nu, uinf);
+  orc_advect_diffuse_scheme(s, dt, nu, uinf, scheme);
+  if (forced) {
+    double st[5];
+    orc_spectral_forcing(s, dt, dt, nu, tke_target, injection_rate, st);
+  }
   /* masked RHS of u* on obstacle blocks, as orc_pressure_rhs forms it: per cell the flags
    * (masked, surface), the unmasked rhs r0 and rhs_new; per block the sums S, pos, neg */
   double* rnew = (double*)calloc(nc, sizeof(double));
@@ -2614,7 +2631,8 @@ void orc_adjoint_step_body_rot(orc_sim* s, double dt, double nu, const double ui
   for (int q = 0; q < 3; q++) orc_get_field(s, ORC_U + q, lam + q * nc);  /* lam* */
   for (int q = 0; q < 4; q++) orc_set_field(s, keep[q], save + q * nc);
   free(save); free(y); free(rnew); free(flag); free(r0c);
-  adj_advect(s, dt, nu, uinf, lam);
+  if (forced) adj_forcing(s, dt, nu, uinf, scheme, tke_target, injection_rate, lam);
+  adj_advection(s, dt, nu, uinf, scheme, lam);
 }
 
 /* ---------------------------------------------------------------- timestep */

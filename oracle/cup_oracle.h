@@ -149,13 +149,17 @@ void orc_adjoint_step(orc_sim* s, double dt, double nu, const double uinf[3], do
 void orc_adjoint_step_body(orc_sim* s, double dt, double nu, const double uinf[3], double lambda, int implicit,
                            const double V[3], double wdrag, double* lam, double dJdV[3], double dJdc[3]);
 
-/* orc_adjoint_step_body with the sphere also rotating at W about its centre of mass (the
- * penalization's target V + W x (x - CoM)): dJdW, when non-NULL, accumulates this step's
- * dJ/dW = sum over penalized cells of (x - CoM) x g, g the cell's dJ/d(target velocity).
- * With W != 0 the centre gradient is not formed (dJdc set to NaN). No reference counterpart. */
-void orc_adjoint_step_body_rot(orc_sim* s, double dt, double nu, const double uinf[3], double lambda, int implicit,
-                               const double V[3], const double W[3], double wdrag, double* lam, double dJdV[3],
-                               double dJdW[3], double dJdc[3]);
+/* orc_adjoint_step_body for the step with the scheme's advection (0 central Euler, 1 upwind
+ * Euler, 2 upwind midpoint), the linear forcing after it (tke_target or injection_rate > 0, as
+ * orc_adjoint_step_forcing), and the sphere also rotating at W about its centre of mass (the
+ * penalization's target V + W x (x - CoM)). The masking is linearised at the forced u*. dJdW,
+ * when non-NULL, accumulates this step's dJ/dW = sum over penalized cells of (x - CoM) x g,
+ * g the cell's dJ/d(target velocity). With W != 0 the centre gradient is not formed (dJdc set
+ * to NaN). No reference counterpart. */
+void orc_adjoint_step_body_full(orc_sim* s, double dt, double nu, const double uinf[3], int scheme,
+                                double tke_target, double injection_rate, double lambda, int implicit,
+                                const double V[3], const double W[3], double wdrag, double* lam, double dJdV[3],
+                                double dJdW[3], double dJdc[3]);
 
 /* Plain 3-D complex FFT, unnormalized, sign -1 forward / +1 backward; arrays [x][y][z]. */
 void orc_fft3d(int nx, int ny, int nz, double* re, double* im, int sign);

@@ -62,7 +62,7 @@ def lib():
         "orc_adjoint_step_scheme": (None, [p, d, d, dp, i, dp]),
         "orc_adjoint_step_forcing": (None, [p, d, d, dp, i, d, d, dp]),
         "orc_adjoint_step_body": (None, [p, d, d, dp, d, i, dp, d, dp, dp, dp]),
-        "orc_adjoint_step_body_rot": (None, [p, d, d, dp, d, i, dp, dp, d, dp, dp, dp, dp]),
+        "orc_adjoint_step_body_full": (None, [p, d, d, dp, i, d, d, d, i, dp, dp, d, dp, dp, dp, dp]),
         "orc_pressure_rhs": (None, [p, d]),
         "orc_poisson_solve": (None, [p]),
         "orc_grad_p": (None, [p, d]),
@@ -286,22 +286,20 @@ class OracleSim:
         return tuple(from_blocks(buf[q * n:(q + 1) * n], self.shape) for q in range(3))
 
     def adjoint_step_body(self, dt, nu, lam, vel, lambda_, implicit, wdrag, uinf=(0, 0, 0), centre=False,
-                          ang_vel=None):
+                          ang_vel=None, scheme=0, tke_target=0.0, injection_rate=0.0):
         """With the sphere: (dJ/du_n, dJ/dV of this step) from lam = dJ/du_{n+1}, J containing
-        wdrag * F_x of this step (F the penalization force); u_n is this sim's velocity. With
-        ang_vel (the sphere rotating about its centre of mass) this step's dJ/d(ang_vel) follows
-        dJ/dV. With centre=True (mollified chi) also this step's dJ/d(sphere centre) as the last
-        value (NaN with a nonzero ang_vel)."""
+        wdrag * F_x of this step (F the penalization force); u_n is this sim's velocity. The
+        step advects with scheme and, with tke_target or injection_rate > 0, applies the linear
+        forcing after the advection. With ang_vel (the sphere rotating about its centre of mass)
+        this step's dJ/d(ang_vel) follows dJ/dV. With centre=True (mollified chi) also this
+        step's dJ/d(sphere centre) as the last value (NaN with a nonzero ang_vel)."""
         buf = np.concatenate([to_blocks(np.asarray(f, dtype=np.float64)) for f in lam])
         dv, dw, dc = np.zeros(3), np.zeros(3), np.zeros(3)
-        if ang_vel is None:
-            lib().orc_adjoint_step_body(self._h, float(dt), float(nu), _dp(_vec(uinf)), float(lambda_),
-                                        int(bool(implicit)), _dp(_vec(vel)), float(wdrag), _dp(buf), _dp(dv),
-                                        _dp(dc) if centre else None)
-        else:
-            lib().orc_adjoint_step_body_rot(self._h, float(dt), float(nu), _dp(_vec(uinf)), float(lambda_),
-                                            int(bool(implicit)), _dp(_vec(vel)), _dp(_vec(ang_vel)), float(wdrag),
-                                            _dp(buf), _dp(dv), _dp(dw), _dp(dc) if centre else None)
+        lib().orc_adjoint_step_body_full(self._h, float(dt), float(nu), _dp(_vec(uinf)), int(scheme),
+                                         float(tke_target), float(injection_rate), float(lambda_),
+                                         int(bool(implicit)), _dp(_vec(vel)),
+                                         _dp(_vec(ang_vel if ang_vel is not None else (0, 0, 0))), float(wdrag),
+                                         _dp(buf), _dp(dv), _dp(dw), _dp(dc) if centre else None)
         n = buf.size // 3
         lam0 = tuple(from_blocks(buf[q * n:(q + 1) * n], self.shape) for q in range(3))
         out = (lam0, dv) if ang_vel is None else (lam0, dv, dw)
@@ -517,13 +515,13 @@ class OracleSim:
         return uinf, ft, q
 
     def timestep(self, dt, nu, uinf=(0, 0, 0), spectral_forcing=False, tke_target=0.0,
-                 sphere=None, scheme=0):
+                 sphere=None, scheme=0, injection_rate=0.0):
         """One step in the reference pipeline order. ``sphere``: dict(lambda_, implicit,
         trans_vel, ang_vel) when a sphere was created with sphere_create."""
         out = {}
         self.advect_diffuse(dt, nu, uinf, scheme)
         if spectral_forcing:
-            out["forcing"] = self.spectral_forcing(dt, nu, tke_target=tke_target)
+            out["forcing"] = self.spectral_forcing(dt, nu, tke_target=tke_target, injection_rate=injection_rate)
         self.pressure_rhs(dt)
         self.poisson()
         self.grad_p(dt)

@@ -159,8 +159,11 @@ static void bodies(void)
   EXPECT(isfinite(dJdV[0]) && isfinite(dJdc[0]));
   const double Wr[3] = {0.1, -0.2, 0.3};
   double dJdW[3] = {0, 0, 0};
-  orc_adjoint_step_body_rot(s, 1e-3, 0.01, uinf, 1e5, 1, V, Wr, 1.0, lam, dJdV, dJdW, dJdc);
+  orc_adjoint_step_body_full(s, 1e-3, 0.01, uinf, 0, 0.0, 0.0, 1e5, 1, V, Wr, 1.0, lam, dJdV, dJdW, dJdc);
   EXPECT(isfinite(dJdW[0]) && isfinite(dJdW[2]) && isnan(dJdc[0]));
+  for (int scheme = 1; scheme < 3; ++scheme)
+    orc_adjoint_step_body_full(s, 1e-3, 0.01, uinf, scheme, 0.5, 0.0, 1e5, 1, V, Wr, 1.0, lam, dJdV, dJdW, NULL);
+  orc_adjoint_step_body_full(s, 1e-3, 0.01, uinf, 0, 0.0, 0.1, 1e5, 1, V, Wr, 1.0, lam, dJdV, dJdW, NULL);
   orc_params p;
   memset(&p, 0, sizeof(p));
   p.nu = 0.01;

@@ -1,5 +1,5 @@
-"""The adjoint through penalization (cup_adjoint_step_body and its rotating-body form
-cup_adjoint_step_body_rot, the PenalizedRollout built on them) against the oracle's (pinned
+"""The adjoint through penalization (cup_adjoint_step_body and its general form
+cup_adjoint_step_body_full, the PenalizedRollout built on them) against the oracle's (pinned
 by the dot-product identity with central differences of the forward step with the sphere),
 and, at the 256 x 128 x 128 sphere case, the gradient of the time-averaged drag with respect
 to the body velocity and centre (at 128 x 64 x 64, its rotation) against finite differences
@@ -104,6 +104,52 @@ def test_adjoint_step_rotating_body_matches_oracle(cup, oracle, decomp):
     np.testing.assert_allclose(dw, dwo, rtol=1e-10, atol=1e-11 * np.abs(dwo).max())
     with pytest.raises(ValueError, match="without rotation"):
         G.adjoint_step_body(dt, nu, U, sph, w, centre=True)
+
+
+@pytest.mark.parametrize("decomp,scheme,target,rate", [("fused", 1, 0.0, 0.0), ("fused", 2, 0.0, 0.0),
+                                                        ("plain", 2, 0.0, 0.0), ("fused", 0, 0.0, 0.4),
+                                                        ("plain", 0, 0.0, 0.4), ("fused", 2, 1.5, 0.0)])
+def test_step_and_adjoint_body_scheme_forcing_match_oracle(cup, oracle, decomp, scheme, target, rate):
+    """The step with the rotating sphere, the upwind schemes and the linear forcing: the forward
+    step (run_steps) and its adjoint (lam_n, dJ/dV, dJ/dW) against the oracle's, which central
+    differences of its forward step pin (2pi x pi x pi box, where the forcing's band holds
+    modes; velocity means (1, 0.8, -0.9) as in the oracle's check)."""
+    print(f"body step and adjoint decomp={decomp} scheme={scheme} target={target} rate={rate}", flush=True)
+    rng = np.random.default_rng(53 + scheme)
+    bpd, ext = (4, 2, 2), 2 * np.pi
+    centre, R = tuple(np.pi * np.array([0.9, 0.5, 0.5])), 0.3 * np.pi
+    G = cup.Grid(bpd, maxextent=ext, precision=8, decomposition=decomp)
+    O = oracle.OracleSim(bpd, maxextent=ext)
+    G.chi_from_sphere(centre, R)
+    O.sphere_create(R, centre)
+    shape = G.shape
+    U, V, W = (1.0, 0.0, 0.0), (-1.0, 0.1, 0.0), (0.3, -0.4, 0.8)
+    u = tuple(0.3 * rng.standard_normal(shape) + m for m in (1.0, 0.8, -0.9))
+    lam = tuple(rng.standard_normal(shape) for _ in range(3))
+    dt, nu, w, lam_ = 0.2 * G.h, 1e-3, 3.7, 1e5
+    sph = dict(lambda_=lam_, implicit=True, trans_vel=V, ang_vel=W)
+    fkw = dict(spectral_forcing=target > 0 or rate > 0, tke_target=target, injection_rate=rate)
+    # the forward step
+    G.set_velocity(*u); O.set_velocity(*u)
+    G.run_steps(1, nu=nu, uinf=U, fixed_dt=dt, sphere=sph, scheme=scheme, **fkw)
+    out = O.timestep(dt, nu, uinf=U, sphere=sph, scheme=scheme, **fkw)
+    for a, b in zip(G.velocity(), O.velocity()):
+        assert rel(a, b) < 1e-11
+    np.testing.assert_allclose(G.last_body_force(), out["penal_ft"], rtol=1e-10, atol=1e-12)
+    # its adjoint
+    G.set_velocity(*u); O.set_velocity(*u)
+    for n, f in zip(("adjU", "adjV", "adjW"), lam):
+        G.set(n, f)
+    dv, dw = G.adjoint_step_body(dt, nu, U, sph, w, rotation=True, scheme=scheme, tke_target=target,
+                                 injection_rate=rate)
+    lam0, dvo, dwo = O.adjoint_step_body(dt, nu, lam, V, lam_, True, w, U, ang_vel=W, scheme=scheme,
+                                         tke_target=target, injection_rate=rate)
+    for n, b in zip(("adjU", "adjV", "adjW"), lam0):
+        assert rel(G.get(n), b) < 1e-11
+    for a, b in zip(G.velocity(), u):  # u_n is left in place
+        assert np.array_equal(a, b)
+    np.testing.assert_allclose(dv, dvo, rtol=1e-11, atol=1e-12 * np.abs(dvo).max())
+    np.testing.assert_allclose(dw, dwo, rtol=1e-10, atol=1e-11 * np.abs(dwo).max())
 
 
 def test_mean_drag_rotation_gradient_vs_finite_differences(cup):

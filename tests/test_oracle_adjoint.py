@@ -100,11 +100,13 @@ def test_adjoint_of_pure_projection_is_symmetric(oracle):
     assert max(np.max(np.abs(x - y)) for x, y in zip(ppa, pa)) < 1e-12 * max(np.max(np.abs(x)) for x in pa)
 
 
-def fwd_step_body(oracle, bpd, u, dt, nu, U, V, lam_, ext, centre=(0.9, 0.5, 0.5), mollify=0.0, W=(0, 0, 0)):
+def fwd_step_body(oracle, bpd, u, dt, nu, U, V, lam_, ext, centre=(0.9, 0.5, 0.5), mollify=0.0, W=(0, 0, 0),
+                  scheme=0, target=0.0, rate=0.0, radius=0.3):
     s = oracle.OracleSim(bpd, maxextent=ext)
-    s.sphere_create(0.3, centre, mollify=mollify)
+    s.sphere_create(radius, centre, mollify=mollify)
     s.set_velocity(*u)
-    out = s.timestep(dt, nu, uinf=U, sphere=dict(lambda_=lam_, implicit=True, trans_vel=V, ang_vel=W))
+    out = s.timestep(dt, nu, uinf=U, sphere=dict(lambda_=lam_, implicit=True, trans_vel=V, ang_vel=W),
+                     scheme=scheme, spectral_forcing=target > 0 or rate > 0, tke_target=target, injection_rate=rate)
     return s.velocity(), out["penal_ft"][0]
 
 
@@ -209,6 +211,51 @@ def test_adjoint_rotating_sphere_dot_product_identity(oracle, lam_):
         assert abs(fdw - dJdW[k]) < 1e-7 * max(abs(fdw), 1e-3)
     # the lever arms are not all zero: W's gradient is not a multiple of V's
     assert np.max(np.abs(dJdW)) > 1e-3 * np.max(np.abs(dJdV))
+
+
+@pytest.mark.parametrize("scheme,target,rate", [(1, 0.0, 0.0), (2, 0.0, 0.0), (0, 0.0, 0.4), (2, 1.5, 0.0)])
+def test_adjoint_body_scheme_forcing_dot_product_identity(oracle, scheme, target, rate):
+    """The step with the sphere, the upwind schemes (Euler, midpoint) and the linear forcing
+    (constant injection rate, or the TKE target): dJ/du_n, dJ/dV and dJ/dW against central
+    differences of the forward oracle step (J = <lam, u_{n+1}> + w F_x, eps = 1e-5, agreement
+    1e-7). The forcing's factor must be positive at u_n (it switches the forcing on) for the
+    forcing's transpose to be exercised, which the test checks. The velocity components have
+    means (1, 0.8, -0.9) so that no advecting velocity comes within the differences' step of
+    zero, where the upwind stencil switches (a kink: with zero means the same check misses by
+    1e-5 at eps = 1e-5 and agrees to 1e-10 at eps = 1e-6). The box is 2pi x pi x pi, where the
+    forcing's band (0 < |k|^2 <= 4) holds modes."""
+    rng = np.random.default_rng(53 + scheme)
+    bpd, ext = (4, 2, 2), 2 * np.pi
+    centre, R = tuple(np.pi * np.array([0.9, 0.5, 0.5])), 0.3 * np.pi
+    s = oracle.OracleSim(bpd, maxextent=ext)
+    s.sphere_create(R, centre)
+    shape = s.shape
+    U, V, W = (1.0, 0.0, 0.0), (-1.0, 0.1, 0.0), (0.3, -0.4, 0.8)
+    u = tuple(0.3 * rng.standard_normal(shape) + m for m in (1.0, 0.8, -0.9))
+    d = tuple(rng.standard_normal(shape) for _ in range(3))
+    lam = tuple(rng.standard_normal(shape) for _ in range(3))
+    dt, nu, w, lam_, eps = 0.2 * s.h, 1e-3, 3.7, 1e5, 1e-5
+    if target > 0 or rate > 0:
+        f = oracle.OracleSim(bpd, maxextent=ext)
+        f.set_velocity(*u)
+        f.advect_diffuse(dt, nu, U, scheme)
+        assert f.spectral_forcing(dt, nu, tke_target=target, injection_rate=rate)["fac"] > 0
+
+    def J(uu, VV, WW):
+        u1, F = fwd_step_body(oracle, bpd, uu, dt, nu, U, VV, lam_, ext, centre=centre, radius=R, W=WW,
+                              scheme=scheme, target=target, rate=rate)
+        return dot(lam, u1) + w * F
+    s.set_velocity(*u)
+    lam0, dJdV, dJdW = s.adjoint_step_body(dt, nu, lam, V, lam_, True, w, U, ang_vel=W, scheme=scheme,
+                                           tke_target=target, injection_rate=rate)
+    fd = (J(tuple(a + eps * b for a, b in zip(u, d)), V, W) - J(tuple(a - eps * b for a, b in zip(u, d)), V, W)) / (2 * eps)
+    assert abs(fd - dot(d, lam0)) < 1e-7 * abs(fd)
+    for k in range(3):
+        e = np.eye(3)[k] * eps
+        fdv = (J(u, tuple(np.add(V, e)), W) - J(u, tuple(np.subtract(V, e)), W)) / (2 * eps)
+        assert abs(fdv - dJdV[k]) < 1e-7 * max(abs(fdv), 1e-3)
+        fdw = (J(u, V, tuple(np.add(W, e))) - J(u, V, tuple(np.subtract(W, e)))) / (2 * eps)
+        assert abs(fdw - dJdW[k]) < 1e-7 * max(abs(fdw), 1e-3)
 
 
 def test_adjoint_rotation_zero_matches_the_translating_body(oracle):
