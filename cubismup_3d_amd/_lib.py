@@ -101,6 +101,7 @@ SIGNATURES = {
     "cup_adjoint_step_forcing": (_i, [_p, _d, _d, _dp, _i, _d, _d]),
     "cup_adjoint_step_body": (_i, [_p, _d, _d, _dp, _d, _i, _dp, _d, _dp, _dp]),
     "cup_adjoint_step_body_full": (_i, [_p, _d, _d, _dp, _i, _d, _d, _d, _i, _dp, _dp, _d, _dp, _dp, _dp]),
+    "cup_adjoint_step_bodies": (_i, [_p, _d, _d, _dp, _i, _d, _d, _d, _i, _i, _dp, _dp, _dp, _dp, _dp]),
     "cup_last_body_force": (_i, [_p, _dp]),
     "cup_stream": (_i, [_p, C.POINTER(C.c_void_p)]),
     "cup_stage_timing_begin": (_i, [_p, _i, _i]),

@@ -178,6 +178,87 @@ def mean_drag(u0, vel, grid: Grid, nsteps: int, nu: float, uinf, dts, every=10, 
                                   (float(tke_target), float(injection_rate)))
 
 
+class BodiesRollout(torch.autograd.Function):
+    """J = sum_n (dt_n / T) sum_k w_k F_x^{k,n} for the grid's bodies (added with Grid.add_sphere
+    or built with Grid.chi_from_sphere), body k forced at vels[k] and rotating at angvels[k]
+    about its centre of mass; differentiable with respect to u0, vels and angvels ([nbody][3]
+    float64). Fixed timesteps, checkpoints every ``every`` steps."""
+
+    @staticmethod
+    def forward(ctx, u0, vels, angvels, grid: Grid, nsteps: int, nu: float, uinf, dts, every: int, weights,
+                lambda_: float, implicit: bool, scheme: int, forcing: tuple):
+        n = grid.body_count()
+        bodies = [dict(trans_vel=tuple(float(x) for x in vels[k].tolist()),
+                       ang_vel=tuple(float(x) for x in angvels[k].tolist())) for k in range(n)]
+        sphere = dict(lambda_=lambda_, implicit=implicit, **bodies[0])
+        fkw = dict(spectral_forcing=any(f > 0 for f in forcing), tke_target=forcing[0], injection_rate=forcing[1],
+                   scheme=scheme)
+        for k in range(1, n):
+            grid.body_velocity(k, bodies[k]["trans_vel"], bodies[k]["ang_vel"])
+        grid.set_torch_velocity(u0.detach())
+        T = float(sum(dts))
+        ckpts = {0: u0.detach().clone()}
+        J = 0.0
+        for s in range(nsteps):
+            grid.run_steps(1, nu=nu, uinf=uinf, fixed_dt=dts[s], sphere=sphere, **fkw)
+            J += dts[s] / T * sum(weights[k] * grid.last_body_force_body(k)[0] for k in range(n))
+            if (s + 1) % every == 0 and s + 1 < nsteps:
+                ckpts[s + 1] = grid.get_torch_velocity()
+        ctx.grid, ctx.nsteps, ctx.nu, ctx.uinf, ctx.dts, ctx.every = grid, nsteps, nu, uinf, list(dts), every
+        ctx.sphere, ctx.bodies, ctx.weights, ctx.fkw, ctx.forcing, ctx.scheme = \
+            sphere, bodies, list(weights), fkw, forcing, scheme
+        ctx.T, ctx.ckpts, ctx.u0_dtype, ctx.vdev = T, ckpts, u0.dtype, vels.device
+        return torch.tensor(J, dtype=torch.float64, device=u0.device)
+
+    @staticmethod
+    def backward(ctx, gJ):
+        grid, every, n = ctx.grid, ctx.every, len(ctx.bodies)
+        g = float(gJ)
+        lam = torch.zeros((3,) + grid.shape, dtype=grid._torch_dtype(), device="cuda")
+        dV = torch.zeros((n, 3), dtype=torch.float64)
+        dW = torch.zeros((n, 3), dtype=torch.float64)
+        for s0 in sorted(ctx.ckpts, reverse=True):
+            s1 = min(s0 + every, ctx.nsteps)
+            states = [ctx.ckpts[s0]]
+            grid.set_torch_velocity(ctx.ckpts[s0])
+            for s in range(s0, s1 - 1):
+                grid.run_steps(1, nu=ctx.nu, uinf=ctx.uinf, fixed_dt=ctx.dts[s], sphere=ctx.sphere, **ctx.fkw)
+                states.append(grid.get_torch_velocity())
+            for s in reversed(range(s0, s1)):
+                grid.set_torch_velocity(states[s - s0])
+                grid.set_torch_adjoint(lam)
+                w = [g * ctx.dts[s] / ctx.T * wk for wk in ctx.weights]
+                dv, dw = grid.adjoint_step_bodies(ctx.dts[s], ctx.nu, ctx.uinf, ctx.bodies, ctx.sphere["lambda_"],
+                                                  ctx.sphere["implicit"], w, scheme=ctx.scheme,
+                                                  tke_target=ctx.forcing[0], injection_rate=ctx.forcing[1])
+                dV += torch.from_numpy(dv)
+                dW += torch.from_numpy(dw)
+                lam = grid.get_torch_adjoint()
+        ctx.ckpts = None
+        return (lam.to(ctx.u0_dtype), dV.to(ctx.vdev), dW.to(ctx.vdev), None, None, None, None, None, None, None,
+                None, None, None, None)
+
+
+def bodies_drag(u0, vels, angvels, grid: Grid, nsteps: int, nu: float, uinf, dts, weights=None, every=10,
+                lambda_=1e6, implicit=True, scheme=0, tke_target=0.0, injection_rate=0.0):
+    """Time-averaged weighted drag sum_k w_k F_x^k of a rollout with every body of the grid
+    (weights default to 1); differentiable with respect to u0 ([3][Z][Y][X], cuda), vels and
+    angvels ([nbody][3], float64: each body's velocity and rotation about its centre of mass)."""
+    n = grid.body_count()
+    if n < 1:
+        raise ValueError("bodies_drag: the grid has no body")
+    if tuple(vels.shape) != (n, 3) or tuple(angvels.shape) != (n, 3):
+        raise ValueError(f"bodies_drag: vels and angvels must be [{n}][3]")
+    if tke_target < 0 or injection_rate < 0:
+        raise ValueError("tke_target and injection_rate must be >= 0")
+    w = [1.0] * n if weights is None else [float(x) for x in weights]
+    if len(w) != n:
+        raise ValueError(f"bodies_drag: {n} weights needed")
+    return BodiesRollout.apply(u0, vels, angvels, grid, int(nsteps), float(nu), tuple(float(x) for x in uinf),
+                               list(dts), int(every), w, float(lambda_), bool(implicit), int(scheme),
+                               (float(tke_target), float(injection_rate)))
+
+
 def kinetic_energy(u, h):
     """KE = h^3/2 sum |u|^2 (ComputeDissipation.cpp:109)."""
     return 0.5 * h ** 3 * (u * u).sum()

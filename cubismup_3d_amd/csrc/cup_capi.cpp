@@ -1066,34 +1066,25 @@ int cup_adjoint_step_body(cup_ctx* ctx, double dt, double nu, const double uinf[
                                     wdrag, dJdV, nullptr, dJdc);
 }
 
-int cup_adjoint_step_body_full(cup_ctx* ctx, double dt, double nu, const double uinf[3], int scheme,
-                               double tke_target, double injection_rate, double lambda, int implicit,
-                               const double body_vel[3], const double body_angvel[3], double wdrag, double dJdV[3],
-                               double dJdW[3], double dJdc[3])
+// The adjoint step with bodies 0..nbody-1 (nbody == body_count, their obstacle blocks apart, so
+// each body's transposes act on its own blocks): body k forced at vel[3k..], its drag weighted by
+// wdrag[k]; dJdV[3k..] and dJdW[3k..] (may be NULL) set. dJdc: the centre of body 0 (nbody == 1).
+static int adjoint_bodies(Ctx& c, double dt, double nu, const double uinf[3], int scheme, double tke_target,
+                          double injection_rate, double lambda, int implicit, int nbody, const double* vel,
+                          const double* wdrag, double* dJdV, double* dJdW, double* dJdc)
 {
-  CUP_GUARD(ctx);
-  if (!c.periodic()) return set_error(c, CUP_E_INVALID, "adjoint_step_body: needs a fully periodic box");
-  if (!uinf || !body_vel || !body_angvel || !dJdV || !(dt > 0))
-    return set_error(c, CUP_E_INVALID, "adjoint_step_body: bad argument");
-  if (int rc = adjoint_scheme_ready(c, scheme, tke_target, injection_rate)) return rc;
-  if (dJdc && (body_angvel[0] != 0 || body_angvel[1] != 0 || body_angvel[2] != 0))
-    return set_error(c, CUP_E_INVALID,
-                     "adjoint_step_body: the centre gradient is formed for a body without rotation only");
-  if (!c.body.present) return set_error(c, CUP_E_STATE, "adjoint_step_body: no obstacle");
-  if (body_count(c) > 1) return set_error(c, CUP_E_STATE, "adjoint_step_body: one body only");
-  if (c.body.ob_udef) return set_error(c, CUP_E_STATE, "adjoint_step_body: the body's udef is not differentiated");
-  if (dJdc && !(c.body.moll > 0))
-    return set_error(c, CUP_E_STATE,
-                     "adjoint_step_body: the centre gradient needs the mollified chi (cup_chi_from_sphere_mollified)");
   for (int f = CUP_ADJ_U; f <= CUP_ADJ_W; ++f)
     if (int rc = field_ready(c, f, "adjoint_step_body")) return rc;
-  AdjBodyArgs a;
-  a.dt = dt;
-  a.lambda = lambda;
-  a.implicit = implicit;
-  a.wdrag = wdrag;
-  for (int d = 0; d < 3; ++d) a.vel[d] = body_vel[d];
-  a.centre = dJdc != nullptr;
+  auto args = [&](int k) {
+    AdjBodyArgs a;
+    a.dt = dt;
+    a.lambda = lambda;
+    a.implicit = implicit;
+    a.wdrag = wdrag[k];
+    for (int d = 0; d < 3; ++d) a.vel[d] = vel[3 * k + d];
+    a.centre = dJdc != nullptr;
+    return a;
+  };
   const bool forced = tke_target > 0 || injection_rate > 0, midpoint = scheme == CUP_UPWIND3_RK2;
   if (int rc = launch_set_dt(c, dt)) return rc;
   // forward state at this step: u* = F(A(u_n)), the scheme's advection (the midpoint form writes
@@ -1107,19 +1098,28 @@ int cup_adjoint_step_body_full(cup_ctx* ctx, double dt, double nu, const double 
     double st[5];
     if (int rc = forcing_plain(c, dt, dt, nu, tke_target, injection_rate, st, c.ustar)) return rc;
   }
-  if (a.centre) {
+  if (dJdc) {
     // the penalization's input P_M(u*) needs the forward pressure: masked and corrected RHS,
     // Poisson solve into P; then the chi sensitivity of the penalization and its drag
+    const AdjBodyArgs a = args(0);
     if (int rc = launch_adjoint_body(c, ADJB_FWD_MASK, a)) return rc;
     if (int rc = launch_adjoint_body(c, ADJB_FWD_CORRECT, a)) return rc;
     if (int rc = launch_poisson_fused(c, false, 0.0, 0.0)) return rc;
     if (int rc = launch_adjoint_body(c, ADJB_CHI_PEN, a)) return rc;
   }
-  // the masked RHS of u* on the obstacle blocks (the masking's linearisation point)
-  if (int rc = launch_adjoint_body(c, ADJB_FWD_MASK, a)) return rc;
-  // penalization and the drag's weight; dJ/dV of this step
-  if (int rc = launch_adjoint_body(c, ADJB_PENALIZE, a)) return rc;
-  // transposed projection: y = L^{-1} R lam' (no obstacle RHS), y <- M'^T y, lam* = lam' - (dt/2h) grad y
+  // each body's penalization and drag weight; its dJ/dV and dJ/dW of this step
+  for (int k = 0; k < nbody; ++k) {
+    BodyScope scope(c, k);
+    if (int rc = launch_adjoint_body(c, ADJB_PENALIZE, args(k))) return rc;
+    if (int rc = read_scalars(c)) return rc;
+    for (int d = 0; d < 3; ++d) {
+      dJdV[3 * k + d] = c.sc_host->red[8 + d];
+      if (dJdW) dJdW[3 * k + d] = c.sc_host->red[17 + d];
+    }
+  }
+  // transposed projection: y = L^{-1} R lam' (no obstacle RHS), y <- M'^T y body by body (each
+  // body's masked RHS of u* on its blocks, the linearisation point, formed first), lam* = lam' -
+  // (dt/2h) grad y
   const int us0 = c.ustar;
   c.ustar = CUP_ADJ_U;
   c.plain_rhs = true;
@@ -1127,7 +1127,11 @@ int cup_adjoint_step_body_full(cup_ctx* ctx, double dt, double nu, const double 
   c.plain_rhs = false;
   c.ustar = us0;
   if (rc) return rc;
-  if (int e = launch_adjoint_body(c, ADJB_MASK, a)) return e;
+  for (int k = 0; k < nbody; ++k) {
+    BodyScope scope(c, k);
+    if (int e = launch_adjoint_body(c, ADJB_FWD_MASK, args(k))) return e;
+    if (int e = launch_adjoint_body(c, ADJB_MASK, args(k))) return e;
+  }
   if (int e = launch_gradp_inplace(c, CUP_ADJ_U)) return e;
   if (midpoint)
     if (int e = keep_velocity(c, true)) return e;
@@ -1135,13 +1139,60 @@ int cup_adjoint_step_body_full(cup_ctx* ctx, double dt, double nu, const double 
   if (forced)
     if (int e = adjoint_forcing_at_ustar(c, dt, nu, uinf, scheme, tke_target, injection_rate)) return e;
   if (int e = adjoint_advection(c, nu, uinf, scheme)) return e;
-  if (int e = read_scalars(c)) return e;
-  for (int d = 0; d < 3; ++d) dJdV[d] = c.sc_host->red[8 + d];
-  if (dJdW)
-    for (int d = 0; d < 3; ++d) dJdW[d] = c.sc_host->red[17 + d];
-  if (dJdc)
+  if (dJdc) {
+    if (int e = read_scalars(c)) return e;
     for (int d = 0; d < 3; ++d) dJdc[d] = c.sc_host->red[11 + d] + c.sc_host->red[14 + d];
+  }
   return CUP_OK;
+}
+
+// Checks shared by the adjoint steps with bodies.
+static int adjoint_body_ready(Ctx& c, double dt, const double uinf[3], int scheme, double tke_target,
+                              double injection_rate)
+{
+  if (!c.periodic()) return set_error(c, CUP_E_INVALID, "adjoint_step_body: needs a fully periodic box");
+  if (!uinf || !(dt > 0)) return set_error(c, CUP_E_INVALID, "adjoint_step_body: bad argument");
+  if (int rc = adjoint_scheme_ready(c, scheme, tke_target, injection_rate)) return rc;
+  if (!c.body.present) return set_error(c, CUP_E_STATE, "adjoint_step_body: no obstacle");
+  for (int k = 0; k < body_count(c); ++k) {
+    BodyScope scope(c, k);
+    if (c.body.ob_udef) return set_error(c, CUP_E_STATE, "adjoint_step_body: a body's udef is not differentiated");
+  }
+  return CUP_OK;
+}
+
+int cup_adjoint_step_body_full(cup_ctx* ctx, double dt, double nu, const double uinf[3], int scheme,
+                               double tke_target, double injection_rate, double lambda, int implicit,
+                               const double body_vel[3], const double body_angvel[3], double wdrag, double dJdV[3],
+                               double dJdW[3], double dJdc[3])
+{
+  CUP_GUARD(ctx);
+  if (!body_vel || !body_angvel || !dJdV) return set_error(c, CUP_E_INVALID, "adjoint_step_body: bad argument");
+  if (int rc = adjoint_body_ready(c, dt, uinf, scheme, tke_target, injection_rate)) return rc;
+  if (dJdc && (body_angvel[0] != 0 || body_angvel[1] != 0 || body_angvel[2] != 0))
+    return set_error(c, CUP_E_INVALID,
+                     "adjoint_step_body: the centre gradient is formed for a body without rotation only");
+  if (body_count(c) > 1)
+    return set_error(c, CUP_E_STATE, "adjoint_step_body: one body only (cup_adjoint_step_bodies takes several)");
+  if (dJdc && !(c.body.moll > 0))
+    return set_error(c, CUP_E_STATE,
+                     "adjoint_step_body: the centre gradient needs the mollified chi (cup_chi_from_sphere_mollified)");
+  return adjoint_bodies(c, dt, nu, uinf, scheme, tke_target, injection_rate, lambda, implicit, 1, body_vel, &wdrag,
+                        dJdV, dJdW, dJdc);
+}
+
+int cup_adjoint_step_bodies(cup_ctx* ctx, double dt, double nu, const double uinf[3], int scheme, double tke_target,
+                            double injection_rate, double lambda, int implicit, int nbody, const double* body_vel,
+                            const double* body_angvel, const double* wdrag, double* dJdV, double* dJdW)
+{
+  CUP_GUARD(ctx);
+  if (!body_vel || !body_angvel || !wdrag || !dJdV)
+    return set_error(c, CUP_E_INVALID, "adjoint_step_bodies: bad argument");
+  if (int rc = adjoint_body_ready(c, dt, uinf, scheme, tke_target, injection_rate)) return rc;
+  if (nbody != body_count(c))
+    return set_error(c, CUP_E_INVALID, "adjoint_step_bodies: nbody must be the number of bodies");
+  return adjoint_bodies(c, dt, nu, uinf, scheme, tke_target, injection_rate, lambda, implicit, nbody, body_vel, wdrag,
+                        dJdV, dJdW, nullptr);
 }
 
 int cup_body_velocity(cup_ctx* ctx, int id, const double vel[3], const double angvel[3])

@@ -173,6 +173,22 @@ class Grid:
         res = res + (dc,) if centre else res
         return res if len(res) > 1 else out
 
+    def adjoint_step_bodies(self, dt, nu, uinf, bodies, lambda_, implicit, wdrag, scheme=0, tke_target=0.0,
+                            injection_rate=0.0):
+        """With every body (bodies[k] = dict(trans_vel, ang_vel) of body k, as run_steps
+        penalized them): adj <- dJ/du_n for J holding wdrag[k] times body k's drag of this step;
+        returns (dJ/dV, dJ/d(ang_vel)), each [nbody][3]."""
+        n = len(bodies)
+        vel = np.array([b["trans_vel"] for b in bodies], dtype=np.float64).reshape(n, 3)
+        om = np.array([b.get("ang_vel", (0.0, 0.0, 0.0)) for b in bodies], dtype=np.float64).reshape(n, 3)
+        w = np.ascontiguousarray(np.asarray(wdrag, dtype=np.float64).reshape(n))
+        dv, dw = np.zeros((n, 3)), np.zeros((n, 3))
+        self._c(L.lib().cup_adjoint_step_bodies(self._h, float(dt), float(nu), L.vec3(uinf), int(scheme),
+                                                float(tke_target), float(injection_rate), float(lambda_),
+                                                int(bool(implicit)), n, L.dptr(vel), L.dptr(om), L.dptr(w),
+                                                L.dptr(dv), L.dptr(dw)))
+        return dv, dw
+
     def last_body_force(self) -> np.ndarray:
         """Penalization force and torque of the last run_steps step with the sphere."""
         out = np.zeros(6)

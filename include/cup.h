@@ -353,6 +353,16 @@ int cup_adjoint_step_body_full(cup_ctx* ctx, double dt, double nu, const double 
                                double tke_target, double injection_rate, double lambda, int implicit,
                                const double body_vel[3], const double body_angvel[3], double wdrag, double dJdV[3],
                                double dJdW[3], double dJdc[3]);
+/* cup_adjoint_step_body_full with several bodies (nbody == cup_body_count, CUP_E_INVALID
+ * otherwise; their obstacle blocks are apart, so each body's transposes act on its own blocks),
+ * no reference counterpart. Body k is forced at body_vel[3k..3k+2] and rotates at
+ * body_angvel[3k..3k+2] (the velocities cup_run_steps was given: body 0's in the step
+ * parameters, the others' through cup_body_velocity); J holds wdrag[k] times body k's drag.
+ * dJdV[3k..3k+2] and dJdW[3k..3k+2] (may be NULL) return body k's gradients of this step. No
+ * centre gradient. Synchronous (one scalar read-back per body). */
+int cup_adjoint_step_bodies(cup_ctx* ctx, double dt, double nu, const double uinf[3], int scheme, double tke_target,
+                            double injection_rate, double lambda, int implicit, int nbody, const double* body_vel,
+                            const double* body_angvel, const double* wdrag, double* dJdV, double* dJdW);
 /* Penalization force and torque (6 values) of the last step cup_run_steps made with a body. */
 int cup_last_body_force(cup_ctx* ctx, double out[6]);
 /* Several bodies in cup_run_steps (has_body): body 0 is penalized with the params' body_vel and

@@ -2439,36 +2439,17 @@ void orc_adjoint_step_body(orc_sim* s, double dt, double nu, const double uinf[3
   orc_adjoint_step_body_full(s, dt, nu, uinf, 0, 0.0, 0.0, lambda, implicit, V, W, wdrag, lam, dJdV, NULL, dJdc);
 }
 
-void orc_adjoint_step_body_full(orc_sim* s, double dt, double nu, const double uinf[3], int scheme,
-                                double tke_target, double injection_rate, double lambda, int implicit,
-                                const double V[3], const double W[3], double wdrag, double* lam, double dJdV[3],
-                                double dJdW[3], double dJdc[3])
+/* The active body's masking linearised at u* (the sim's velocity): per cell of its blocks the
+ * flags (bit 0 masked, bit 1 surface), rhs_new and, for the centre gradient, r0; its totals S,
+ * D and the correction, as rhs_obstacle forms them. */
+typedef struct {
+  double S, D, denom, corr, sg;
+} MaskTot;
+
+static MaskTot mask_linearise(orc_sim* s, double dt, double* rnew, double* r0c, unsigned char* flag)
 {
-  if (!s->has_sphere) { orc_adjoint_step_forcing(s, dt, nu, uinf, scheme, tke_target, injection_rate, lam); return; }
-  const int forced = tke_target > 0 || injection_rate > 0;
-  const int rot = W[0] != 0 || W[1] != 0 || W[2] != 0;
-  /* Towers' chi, or a rotating body (its lever arms move with the centre): no centre gradient */
-  const int want_c = dJdc != NULL && s->moll > 0 && !rot;
-  if (dJdc && !want_c) dJdc[0] = dJdc[1] = dJdc[2] = NAN;
-  const size_t nc = (size_t)s->nb * BS * BS * BS;
-  const double h = s->h, h3 = h * h * h;
-  /* forward state u_n kept aside; u* = F(A(u_n)), the scheme's advection and the forcing, is
-   * the masking's linearisation point */
-  double* save = (double*)malloc(4 * nc * sizeof(double));
-  const int keep[4] = {ORC_U, ORC_V, ORC_W, ORC_P};
-  for (int q = 0; q < 4; q++) orc_get_field(s, keep[q], save + q * nc);
-  orc_advect_diffuse_scheme(s, dt, nu, uinf, scheme);
-  if (forced) {
-    double st[5];
-    orc_spectral_forcing(s, dt, dt, nu, tke_target, injection_rate, st);
-  }
-  /* masked RHS of u* on obstacle blocks, as orc_pressure_rhs forms it: per cell the flags
-   * (masked, surface), the unmasked rhs r0 and rhs_new; per block the sums S, pos, neg */
-  double* rnew = (double*)calloc(nc, sizeof(double));
-  double* r0c = want_c ? (double*)calloc(nc, sizeof(double)) : NULL;
-  unsigned char* flag = (unsigned char*)calloc(nc, 1);  /* bit 0 masked, bit 1 surface */
+  const double h = s->h, fac = 0.5 * h * h / dt;
   double* part = (double*)calloc(3 * (size_t)s->nb, sizeof(double));
-  const double fac = 0.5 * h * h / dt;
   #pragma omp parallel
   {
     Lab l = lab_alloc(1);
@@ -2511,24 +2492,26 @@ void orc_adjoint_step_body_full(orc_sim* s, double dt, double nu, const double u
     neg += part[3 * b + 2];
   }
   free(part);
-  const double denom = S > 0 ? pos : neg;
-  const double D = denom > EPS ? denom : EPS;
-  const double corr = S / D;
-  const double sg = corr > 0 ? 1.0 : (corr < 0 ? -1.0 : 0.0);
-  /* the penalization's input u_in = P_M(u*), for its chi sensitivity */
-  double* uin = NULL;
-  if (want_c) {
-    orc_pressure_rhs(s, dt);
-    orc_poisson_solve(s);
-    orc_grad_p(s, dt);
-    uin = (double*)malloc(3 * nc * sizeof(double));
-    for (int q = 0; q < 3; q++) orc_get_field(s, ORC_U + q, uin + q * nc);
-  }
-  for (int q = 0; q < 4; q++) orc_set_field(s, keep[q], save + q * nc);  /* u_n back */
-  /* penalization and drag: lam' = (1 - dt pf) lam - wdrag h^3 pf e_x on penalized cells; with
-   * the centre, dJ/dchi = dpf/dchi (dt sum_d lam_d (V_d - u_in,d) + wdrag h^3 (V_x - u_in,x)).
-   * The target velocity V + W x r (r = x - CoM) enters as g . (V + W x r) with g_d =
-   * dt pf lam_d + wdrag h^3 pf [d = x]: dJ/dV = sum g, dJ/dW = sum r x g. */
+  MaskTot m;
+  m.S = S;
+  m.denom = S > 0 ? pos : neg;
+  m.D = m.denom > EPS ? m.denom : EPS;
+  m.corr = S / m.D;
+  m.sg = m.corr > 0 ? 1.0 : (m.corr < 0 ? -1.0 : 0.0);
+  return m;
+}
+
+/* The active body's penalization and drag transposed: lam' = (1 - dt pf) lam - wdrag h^3 pf e_x
+ * on its penalized cells. The target velocity V + W x r (r = x - CoM) enters as
+ * g . (V + W x r) with g_d = dt pf lam_d + wdrag h^3 pf [d = x]: acc[0..2] += dJ/dV = sum g,
+ * acc[6..8] += dJ/dW = sum r x g. With uin (the penalization's input P_M(u*)), acc[3..5] +=
+ * the chi part of dJ/d(centre): dpf/dchi (dt sum_d lam_d (V_d - u_in,d) + wdrag h^3 (V_x -
+ * u_in,x)) dchi/dc. */
+static void pen_transpose(orc_sim* s, double dt, double lambda, int implicit, const double V[3], double wdrag,
+                          const double* uin, double* lam, double acc[9])
+{
+  const size_t nc = (size_t)s->nb * BS * BS * BS;
+  const double h3 = s->h * s->h * s->h;
   const double lf = implicit ? lambda : 1.0 / dt;
   double* pv = (double*)calloc(9 * (size_t)s->nb, sizeof(double));
   #pragma omp parallel for schedule(static)
@@ -2545,7 +2528,7 @@ void orc_adjoint_step_body_full(orc_sim* s, double dt, double nu, const double u
           if (s->blk[b].d[iz][iy][ix].chi > X || X <= 0) continue;
           const double pf = implicit ? X * lf / (1 + lf * dt * X) : X * lf;
           const size_t c = (size_t)b * BS * BS * BS + (iz * BS + iy) * BS + ix;
-          if (want_c) {
+          if (uin) {
             const double den = 1 + lf * dt * X;
             const double dpf = implicit ? lf / (den * den) : lf;
             double gx = wdrag * h3 * (V[0] - uin[c]);
@@ -2570,32 +2553,30 @@ void orc_adjoint_step_body_full(orc_sim* s, double dt, double nu, const double u
         }
     for (int d = 0; d < 9; d++) pv[9 * b + d] = a[d];
   }
-  double dc[3] = {0, 0, 0};
   for (int b = 0; b < s->nb; b++)
-    for (int d = 0; d < 3; d++) {
-      dJdV[d] += pv[9 * b + d];
-      dc[d] += pv[9 * b + 3 + d];
-      if (dJdW) dJdW[d] += pv[9 * b + 6 + d];
-    }
+    for (int d = 0; d < 9; d++) acc[d] += pv[9 * b + d];
   free(pv);
-  free(uin);
-  /* transposed projection: y = L^{-1} R lam' (plain RHS), q = M'^T y, lam* = lam' - (dt/2h) grad_c q */
-  for (int q = 0; q < 3; q++) orc_set_field(s, ORC_U + q, lam + q * nc);
-  const int sphere = s->has_sphere;
-  s->has_sphere = 0;
-  orc_pressure_rhs(s, dt);
-  orc_poisson_solve(s);
-  double* y = (double*)malloc(nc * sizeof(double));
-  orc_get_field(s, ORC_P, y);
+}
+
+/* y <- M'^T y on the active body's blocks (the masking's transpose at its linearisation m):
+ * with g = sum over its corrected cells Q of y (-sg rhs_new), a_S = g/D and a_D = -g corr/D,
+ * adj(rhs_new) = y (1 - sg corr) + a_D sg on Q and y elsewhere, and the unmasked RHS's adjoint is
+ * (1 - X) adj(rhs_new) - X a_S. With dc (the centre gradient): rhs_new = (1 - X) r0 and S =
+ * -sum X r0 give dJ/dX = -r0 (adj(rhs_new) + a_S), scaled by yscale and contracted with dX/dc. */
+static void mask_transpose(orc_sim* s, const MaskTot* m, const double* rnew, const double* r0c,
+                           const unsigned char* flag, double yscale, double* y, double dc[3])
+{
+  const int n3 = BS * BS * BS;
   double g = 0;
-  for (size_t c = 0; c < nc; c++)
-    if ((flag[c] & 2) && ((rnew[c] > 0 && corr > 0) || (rnew[c] < 0 && corr < 0))) g += y[c] * (-sg * rnew[c]);
-  const double aS = g / D, aD = denom > EPS ? -g * corr / D : 0.0;
-  /* with the centre: rhs_new = (1 - X) r0 and S = -sum X r0 give dJ/dX = -r0 (adj(rhs_new) + aS).
-   * y is the adjoint of the masked RHS times h^3/dt^2: the transposed projection applies the
-   * RHS operator (h^2/2dt) div where the transpose of the gradient step is (dt/2h) div, and the
-   * gradient step after it applies the inverse factor, so lam* is exact; the chi term is scaled. */
-  const double yscale = dt * dt / (h * h * h);
+  for (int b = 0; b < s->nb; b++) {
+    if (!s->ob[b]) continue;
+    for (int i = 0; i < n3; i++) {
+      const size_t c = (size_t)b * n3 + i;
+      if ((flag[c] & 2) && ((rnew[c] > 0 && m->corr > 0) || (rnew[c] < 0 && m->corr < 0)))
+        g += y[c] * (-m->sg * rnew[c]);
+    }
+  }
+  const double aS = g / m->D, aD = m->denom > EPS ? -g * m->corr / m->D : 0.0;
   double* pm = (double*)calloc(3 * (size_t)s->nb, sizeof(double));
   #pragma omp parallel for schedule(static)
   for (int b = 0; b < s->nb; b++) {
@@ -2604,13 +2585,13 @@ void orc_adjoint_step_body_full(orc_sim* s, double dt, double nu, const double u
     int bc[3];
     bcoords(s, b, bc);
     double a[3] = {0, 0, 0};
-    for (int i = 0; i < BS * BS * BS; i++) {
-      const size_t c = (size_t)b * BS * BS * BS + i;
+    for (int i = 0; i < n3; i++) {
+      const size_t c = (size_t)b * n3 + i;
       if (!(flag[c] & 1)) continue;
       const double X = (&o->chi[0][0][0])[i];
-      const int inQ = (flag[c] & 2) && ((rnew[c] > 0 && corr > 0) || (rnew[c] < 0 && corr < 0));
-      const double an = inQ ? y[c] * (1 - sg * corr) + aD * sg : y[c];
-      if (want_c) {
+      const int inQ = (flag[c] & 2) && ((rnew[c] > 0 && m->corr > 0) || (rnew[c] < 0 && m->corr < 0));
+      const double an = inQ ? y[c] * (1 - m->sg * m->corr) + aD * m->sg : y[c];
+      if (dc) {
         double p[3], dX[3];
         cell_pos(s, bc, i % BS, (i / BS) % BS, i / (BS * BS), p);
         moll_dchi_dc(s, p, dX);
@@ -2620,19 +2601,124 @@ void orc_adjoint_step_body_full(orc_sim* s, double dt, double nu, const double u
     }
     for (int k = 0; k < 3; k++) pm[3 * b + k] = a[k];
   }
-  for (int b = 0; b < s->nb; b++)
-    for (int k = 0; k < 3; k++) dc[k] += pm[3 * b + k];
+  if (dc)
+    for (int b = 0; b < s->nb; b++)
+      for (int k = 0; k < 3; k++) dc[k] += pm[3 * b + k];
   free(pm);
+}
+
+/* The adjoint step with nbody bodies: nbody == 1 takes the active body, nbody > 1 bodies
+ * 0..nbody-1 (the sim's bodies, whose obstacle blocks are apart, so each body's transposes act
+ * on its own blocks). Body k has velocity V[3k..], rotation W[3k..] and drag weight wdrag[k];
+ * dJdV[3k..] and dJdW[3k..] (if non-NULL) accumulate. dJdc: the active body's centre (nbody ==
+ * 1, mollified chi, no rotation), else NaN. */
+static void adjoint_bodies(orc_sim* s, double dt, double nu, const double uinf[3], int scheme, double tke_target,
+                           double injection_rate, double lambda, int implicit, int nbody, const double* V,
+                           const double* W, const double* wdrag, double* lam, double* dJdV, double* dJdW,
+                           double dJdc[3])
+{
+  const int forced = tke_target > 0 || injection_rate > 0;
+  const int rot = W[0] != 0 || W[1] != 0 || W[2] != 0;
+  /* Towers' chi, or a rotating body (its lever arms move with the centre): no centre gradient */
+  const int want_c = dJdc != NULL && nbody == 1 && s->moll > 0 && !rot;
+  if (dJdc && !want_c) dJdc[0] = dJdc[1] = dJdc[2] = NAN;
+  const size_t nc = (size_t)s->nb * BS * BS * BS;
+  const double h = s->h;
+#define SEL(k) do { if (nbody > 1) orc_select_body(s, (k)); } while (0)
+  /* forward state u_n kept aside; u* = F(A(u_n)), the scheme's advection and the forcing, is
+   * the masking's linearisation point */
+  double* save = (double*)malloc(4 * nc * sizeof(double));
+  const int keep[4] = {ORC_U, ORC_V, ORC_W, ORC_P};
+  for (int q = 0; q < 4; q++) orc_get_field(s, keep[q], save + q * nc);
+  orc_advect_diffuse_scheme(s, dt, nu, uinf, scheme);
+  if (forced) {
+    double st[5];
+    orc_spectral_forcing(s, dt, dt, nu, tke_target, injection_rate, st);
+  }
+  double* rnew = (double*)calloc(nc, sizeof(double));
+  double* r0c = want_c ? (double*)calloc(nc, sizeof(double)) : NULL;
+  unsigned char* flag = (unsigned char*)calloc(nc, 1);
+  MaskTot* mt = (MaskTot*)malloc((size_t)nbody * sizeof(MaskTot));
+  for (int k = 0; k < nbody; k++) {
+    SEL(k);
+    mt[k] = mask_linearise(s, dt, rnew, r0c, flag);
+    SEL(k);
+  }
+  /* the penalization's input u_in = P_M(u*), for its chi sensitivity */
+  double* uin = NULL;
+  if (want_c) {
+    orc_pressure_rhs(s, dt);
+    orc_poisson_solve(s);
+    orc_grad_p(s, dt);
+    uin = (double*)malloc(3 * nc * sizeof(double));
+    for (int q = 0; q < 3; q++) orc_get_field(s, ORC_U + q, uin + q * nc);
+  }
+  for (int q = 0; q < 4; q++) orc_set_field(s, keep[q], save + q * nc);  /* u_n back */
+  double dc[3] = {0, 0, 0};
+  for (int k = 0; k < nbody; k++) {
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    SEL(k);
+    pen_transpose(s, dt, lambda, implicit, V + 3 * k, wdrag[k], uin, lam, acc);
+    SEL(k);
+    for (int d = 0; d < 3; d++) {
+      dJdV[3 * k + d] += acc[d];
+      dc[d] += acc[3 + d];
+      if (dJdW) dJdW[3 * k + d] += acc[6 + d];
+    }
+  }
+  free(uin);
+  /* transposed projection: y = L^{-1} R lam' (plain RHS), y <- M'^T y, lam* = lam' - (dt/2h) grad y.
+   * y is the adjoint of the masked RHS times h^3/dt^2: the transposed projection applies the RHS
+   * operator (h^2/2dt) div where the transpose of the gradient step is (dt/2h) div, and the
+   * gradient step after it applies the inverse factor, so lam* is exact; the chi term is scaled. */
+  for (int q = 0; q < 3; q++) orc_set_field(s, ORC_U + q, lam + q * nc);
+  const int sphere = s->has_sphere;
+  s->has_sphere = 0;
+  orc_pressure_rhs(s, dt);
+  orc_poisson_solve(s);
+  double* y = (double*)malloc(nc * sizeof(double));
+  orc_get_field(s, ORC_P, y);
+  s->has_sphere = sphere;
+  const double yscale = dt * dt / (h * h * h);
+  for (int k = 0; k < nbody; k++) {
+    SEL(k);
+    mask_transpose(s, &mt[k], rnew, r0c, flag, yscale, y, want_c ? dc : NULL);
+    SEL(k);
+  }
+#undef SEL
   if (want_c)
     for (int k = 0; k < 3; k++) dJdc[k] += dc[k];
   orc_set_field(s, ORC_P, y);
+  s->has_sphere = 0;
   orc_grad_p(s, dt);
   s->has_sphere = sphere;
   for (int q = 0; q < 3; q++) orc_get_field(s, ORC_U + q, lam + q * nc);  /* lam* */
   for (int q = 0; q < 4; q++) orc_set_field(s, keep[q], save + q * nc);
-  free(save); free(y); free(rnew); free(flag); free(r0c);
+  free(save); free(y); free(rnew); free(flag); free(r0c); free(mt);
   if (forced) adj_forcing(s, dt, nu, uinf, scheme, tke_target, injection_rate, lam);
   adj_advection(s, dt, nu, uinf, scheme, lam);
+}
+
+void orc_adjoint_step_body_full(orc_sim* s, double dt, double nu, const double uinf[3], int scheme,
+                                double tke_target, double injection_rate, double lambda, int implicit,
+                                const double V[3], const double W[3], double wdrag, double* lam, double dJdV[3],
+                                double dJdW[3], double dJdc[3])
+{
+  if (!s->has_sphere) { orc_adjoint_step_forcing(s, dt, nu, uinf, scheme, tke_target, injection_rate, lam); return; }
+  adjoint_bodies(s, dt, nu, uinf, scheme, tke_target, injection_rate, lambda, implicit, 1, V, W, &wdrag, lam, dJdV,
+                 dJdW, dJdc);
+}
+
+void orc_adjoint_step_bodies(orc_sim* s, double dt, double nu, const double uinf[3], int scheme, double tke_target,
+                             double injection_rate, double lambda, int implicit, int nbody, const double* V,
+                             const double* W, const double* wdrag, double* lam, double* dJdV, double* dJdW)
+{
+  if (nbody != orc_body_count(s) || nbody < 1) {
+    for (int i = 0; i < 3 * (nbody > 0 ? nbody : 0); i++) dJdV[i] = dJdW[i] = NAN;
+    return;
+  }
+  adjoint_bodies(s, dt, nu, uinf, scheme, tke_target, injection_rate, lambda, implicit, nbody, V, W, wdrag, lam,
+                 dJdV, dJdW, NULL);
 }
 
 /* ---------------------------------------------------------------- timestep */

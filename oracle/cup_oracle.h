@@ -161,6 +161,14 @@ void orc_adjoint_step_body_full(orc_sim* s, double dt, double nu, const double u
                                 const double V[3], const double W[3], double wdrag, double* lam, double dJdV[3],
                                 double dJdW[3], double dJdc[3]);
 
+/* orc_adjoint_step_body_full with nbody bodies (nbody == orc_body_count, else the outputs are
+ * NaN and lam is left), their obstacle blocks apart: body k forced at V[3k..3k+2], rotating at
+ * W[3k..3k+2] about its centre of mass, J holding wdrag[k] times its drag F_x; dJdV[3k..] and
+ * dJdW[3k..] accumulate its gradients. No centre gradient. No reference counterpart. */
+void orc_adjoint_step_bodies(orc_sim* s, double dt, double nu, const double uinf[3], int scheme, double tke_target,
+                             double injection_rate, double lambda, int implicit, int nbody, const double* V,
+                             const double* W, const double* wdrag, double* lam, double* dJdV, double* dJdW);
+
 /* Plain 3-D complex FFT, unnormalized, sign -1 forward / +1 backward; arrays [x][y][z]. */
 void orc_fft3d(int nx, int ny, int nz, double* re, double* im, int sign);
 

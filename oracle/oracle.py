@@ -63,6 +63,7 @@ def lib():
         "orc_adjoint_step_forcing": (None, [p, d, d, dp, i, d, d, dp]),
         "orc_adjoint_step_body": (None, [p, d, d, dp, d, i, dp, d, dp, dp, dp]),
         "orc_adjoint_step_body_full": (None, [p, d, d, dp, i, d, d, d, i, dp, dp, d, dp, dp, dp, dp]),
+        "orc_adjoint_step_bodies": (None, [p, d, d, dp, i, d, d, d, i, i, dp, dp, dp, dp, dp, dp]),
         "orc_pressure_rhs": (None, [p, d]),
         "orc_poisson_solve": (None, [p]),
         "orc_grad_p": (None, [p, d]),
@@ -304,6 +305,23 @@ class OracleSim:
         lam0 = tuple(from_blocks(buf[q * n:(q + 1) * n], self.shape) for q in range(3))
         out = (lam0, dv) if ang_vel is None else (lam0, dv, dw)
         return out + (dc,) if centre else out
+
+    def adjoint_step_bodies(self, dt, nu, lam, vels, ang_vels, lambda_, implicit, wdrag, uinf=(0, 0, 0), scheme=0,
+                            tke_target=0.0, injection_rate=0.0):
+        """With every body (vels[k], ang_vels[k] those of body k): (dJ/du_n, dJ/dV, dJ/dW) from
+        lam = dJ/du_{n+1}, J containing wdrag[k] times body k's drag of this step; dJ/dV and
+        dJ/dW are [nbody][3]."""
+        n = len(wdrag)
+        buf = np.concatenate([to_blocks(np.asarray(f, dtype=np.float64)) for f in lam])
+        V = np.ascontiguousarray(np.asarray(vels, dtype=np.float64).reshape(n, 3))
+        W = np.ascontiguousarray(np.asarray(ang_vels, dtype=np.float64).reshape(n, 3))
+        w = np.ascontiguousarray(np.asarray(wdrag, dtype=np.float64).reshape(n))
+        dv, dw = np.zeros((n, 3)), np.zeros((n, 3))
+        lib().orc_adjoint_step_bodies(self._h, float(dt), float(nu), _dp(_vec(uinf)), int(scheme), float(tke_target),
+                                      float(injection_rate), float(lambda_), int(bool(implicit)), n, _dp(V), _dp(W),
+                                      _dp(w), _dp(buf), _dp(dv), _dp(dw))
+        m = buf.size // 3
+        return tuple(from_blocks(buf[q * m:(q + 1) * m], self.shape) for q in range(3)), dv, dw
 
     def hit_analysis(self, nu):
         """HIT analysis: dict of tke, eps, uprime, lambda, Re_lambda, l_integral, tau_integral,

@@ -152,6 +152,124 @@ def test_step_and_adjoint_body_scheme_forcing_match_oracle(cup, oracle, decomp, 
     np.testing.assert_allclose(dw, dwo, rtol=1e-10, atol=1e-11 * np.abs(dwo).max())
 
 
+TWO_BODIES = [(0.2 * np.pi, (0.5 * np.pi, 0.5 * np.pi, 0.5 * np.pi)),
+              (0.2 * np.pi, (1.5 * np.pi, 0.45 * np.pi, 0.55 * np.pi))]
+
+
+@pytest.mark.parametrize("decomp,scheme,rate", [("plain", 0, 0.0), ("fused", 0, 0.0), ("fused", 2, 0.4)])
+def test_step_and_adjoint_two_bodies_match_oracle(cup, oracle, decomp, scheme, rate):
+    """Two spheres with their own velocities and rotations (body 1's through cup_body_velocity):
+    the forward step (run_steps), each body's penalization force, and the adjoint with both
+    (lam_n, each body's dJ/dV and dJ/dW) against the oracle's, which central differences of its
+    forward step pin."""
+    print(f"two-body step and adjoint decomp={decomp} scheme={scheme} rate={rate}", flush=True)
+    rng = np.random.default_rng(59 + scheme)
+    bpd, ext = (4, 2, 2), 2 * np.pi
+    G = cup.Grid(bpd, maxextent=ext, precision=8, decomposition=decomp)
+    O = oracle.OracleSim(bpd, maxextent=ext)
+    for R, c in TWO_BODIES:
+        G.add_sphere(c, R)
+        O.add_sphere(R, c)
+    shape = G.shape
+    U = (1.0, 0.0, 0.0)
+    Vs, Ws, w = [(-1.0, 0.1, 0.0), (-0.8, 0.0, 0.2)], [(0.3, -0.4, 0.8), (-0.5, 0.2, 0.6)], [3.7, -1.9]
+    u = tuple(0.3 * rng.standard_normal(shape) + m for m in (1.0, 0.8, -0.9))
+    lam = tuple(rng.standard_normal(shape) for _ in range(3))
+    dt, nu, lam_ = 0.2 * G.h, 1e-3, 1e5
+    fkw = dict(spectral_forcing=rate > 0, injection_rate=rate)
+    sph = dict(lambda_=lam_, implicit=True, trans_vel=Vs[0], ang_vel=Ws[0])
+    G.body_velocity(1, Vs[1], Ws[1])
+    G.set_velocity(*u); O.set_velocity(*u)
+    G.run_steps(1, nu=nu, uinf=U, fixed_dt=dt, sphere=sph, scheme=scheme, **fkw)
+    O.advect_diffuse(dt, nu, U, scheme)
+    if rate > 0:
+        O.spectral_forcing(dt, nu, injection_rate=rate)
+    O.pressure_rhs(dt)
+    O.poisson()
+    O.grad_p(dt)
+    for k in range(2):
+        with O.body(k):
+            ft = O.penalize(dt, lam_, True, Vs[k], Ws[k])
+        np.testing.assert_allclose(G.last_body_force_body(k), ft, rtol=1e-10, atol=1e-12 * np.abs(ft).max())
+    for a, b in zip(G.velocity(), O.velocity()):
+        assert rel(a, b) < 1e-11
+    G.set_velocity(*u); O.set_velocity(*u)
+    for n, f in zip(("adjU", "adjV", "adjW"), lam):
+        G.set(n, f)
+    bodies = [dict(trans_vel=Vs[k], ang_vel=Ws[k]) for k in range(2)]
+    dv, dw = G.adjoint_step_bodies(dt, nu, U, bodies, lam_, True, w, scheme=scheme, injection_rate=rate)
+    lam0, dvo, dwo = O.adjoint_step_bodies(dt, nu, lam, Vs, Ws, lam_, True, w, U, scheme=scheme, injection_rate=rate)
+    for n, b in zip(("adjU", "adjV", "adjW"), lam0):
+        assert rel(G.get(n), b) < 1e-11
+    print(f"dJ/dV gpu {dv.tolist()} oracle {dvo.tolist()}", flush=True)
+    np.testing.assert_allclose(dv, dvo, rtol=1e-11, atol=1e-12 * np.abs(dvo).max())
+    np.testing.assert_allclose(dw, dwo, rtol=1e-10, atol=1e-11 * np.abs(dwo).max())
+    with pytest.raises(RuntimeError, match="one body only"):
+        G.adjoint_step_body(dt, nu, U, sph, 1.0)
+    with pytest.raises(ValueError, match="number of bodies"):
+        G.adjoint_step_bodies(dt, nu, U, bodies[:1], lam_, True, w[:1])
+
+
+def test_bodies_drag_gradient_vs_finite_differences(cup):
+    """Two spheres D = 0.2 in a periodic 2 x 1 x 1 box at 128 x 64 x 64 (the second behind and
+    beside the first), inflow U = 1, Re = 200, lambda = 1e6, 30 steps of dt = 2e-3 from rest,
+    checkpoints every 10, J = sum_n (dt/T) (F_x^0 + 0.5 F_x^1): every body's dJ/dV and dJ/dW,
+    FP32 and FP64 adjoints, against central differences (eps = 1e-4, below the masking's kinks)
+    of the FP64 forward."""
+    import torch
+    from cubismup_3d_amd.adjoint import bodies_drag
+    bpd, ext, R = (16, 8, 8), 2.0, 0.1
+    cs = [(0.5, 0.4988, 0.4999), (1.2, 0.62, 0.47)]
+    U, nu, nsteps, every = (1.0, 0.0, 0.0), 1.0 * 2 * R / 200, 30, 10
+    V0 = np.array([[-1.0, 0.0, 0.0], [-0.9, 0.1, 0.0]])
+    W0 = np.array([[0.5, -1.0, 4.0], [0.0, 2.0, -1.0]])
+    weights = [1.0, 0.5]
+    dts = [2e-3] * nsteps
+
+    def grid(prec):
+        G = cup.Grid(bpd, maxextent=ext, precision=prec, decomposition="fused")
+        for c in cs:
+            G.add_sphere(c, R)
+        return G
+    gv, gw = {}, {}
+    for prec in (4, 8):
+        G = grid(prec)
+        dtype = torch.float32 if prec == 4 else torch.float64
+        u0 = torch.zeros((3,) + G.shape, dtype=dtype, device="cuda", requires_grad=True)
+        V = torch.tensor(V0, dtype=torch.float64, requires_grad=True)
+        W = torch.tensor(W0, dtype=torch.float64, requires_grad=True)
+        J = bodies_drag(u0, V, W, G, nsteps, nu, U, dts, weights=weights, every=every)
+        J.backward()
+        gv[prec], gw[prec] = V.grad.numpy().copy(), W.grad.numpy().copy()
+        print(f"precision {prec}: J {J.item():.8e}, dJ/dV {gv[prec].tolist()}, dJ/dW {gw[prec].tolist()}", flush=True)
+    G8 = grid(8)
+
+    def J64(V, W):
+        G8.set_velocity(*(np.zeros(G8.shape) for _ in range(3)))
+        G8.body_velocity(1, V[1], W[1])
+        sph = dict(lambda_=1e6, implicit=True, trans_vel=tuple(V[0]), ang_vel=tuple(W[0]))
+        J = 0.0
+        for dt in dts:
+            G8.run_steps(1, nu=nu, uinf=U, fixed_dt=dt, sphere=sph)
+            J += dt / sum(dts) * sum(weights[k] * G8.last_body_force_body(k)[0] for k in range(2))
+        return J
+    eps = 1e-4
+    for name, P0, grads in (("V", V0, gv), ("W", W0, gw)):
+        scale = np.linalg.norm(grads[8])
+        assert scale > 0
+        for k in range(2):
+            for j in range(3):
+                e = np.zeros((2, 3))
+                e[k, j] = eps
+                fd = ((J64(V0 + e, W0) - J64(V0 - e, W0)) if name == "V" else
+                      (J64(V0, W0 + e) - J64(V0, W0 - e))) / (2 * eps)
+                print(f"dJ/d{name}[{k}][{j}]: fd {fd:.10e} fp64 {grads[8][k, j]:.10e} "
+                      f"({abs(grads[8][k, j] - fd) / scale:.1e}) fp32 {grads[4][k, j]:.10e} "
+                      f"({abs(grads[4][k, j] - fd) / scale:.1e} of |dJ/d{name}|)", flush=True)
+                assert abs(grads[8][k, j] - fd) < 1e-6 * scale
+                assert abs(grads[4][k, j] - fd) < 1e-4 * scale
+
+
 def test_mean_drag_rotation_gradient_vs_finite_differences(cup):
     """Sphere D = 0.2 rotating at W = (0.5, -1, 4) about its centre of mass in a periodic
     2 x 1 x 1 box at 128 x 64 x 64, inflow U = 1, Re = 200, lambda = 1e6, 40 steps of

@@ -258,6 +258,69 @@ def test_adjoint_body_scheme_forcing_dot_product_identity(oracle, scheme, target
         assert abs(fdw - dJdW[k]) < 1e-7 * max(abs(fdw), 1e-3)
 
 
+TWO_BODIES = [(0.2 * np.pi, (0.5 * np.pi, 0.5 * np.pi, 0.5 * np.pi)),
+              (0.2 * np.pi, (1.5 * np.pi, 0.45 * np.pi, 0.55 * np.pi))]
+
+
+def fwd_step_bodies(oracle, bpd, u, dt, nu, U, Vs, Ws, lam_, ext, scheme=0, target=0.0, rate=0.0):
+    s = oracle.OracleSim(bpd, maxextent=ext)
+    for R, c in TWO_BODIES:
+        s.add_sphere(R, c)
+    s.set_velocity(*u)
+    s.advect_diffuse(dt, nu, U, scheme)
+    if target > 0 or rate > 0:
+        s.spectral_forcing(dt, nu, tke_target=target, injection_rate=rate)
+    s.pressure_rhs(dt)
+    s.poisson()
+    s.grad_p(dt)
+    F = []
+    for k in range(len(TWO_BODIES)):
+        with s.body(k):
+            F.append(s.penalize(dt, lam_, True, Vs[k], Ws[k])[0])
+    return s.velocity(), F
+
+
+@pytest.mark.parametrize("scheme,rate", [(0, 0.0), (2, 0.4)])
+def test_adjoint_two_bodies_dot_product_identity(oracle, scheme, rate):
+    """Two spheres with their own velocities and rotations, J = <lam, u_{n+1}> + sum_k w_k F_x^k:
+    dJ/du_n and each body's dJ/dV, dJ/dW against central differences of the forward oracle
+    step (the reference order: every body's masking in the RHS, then each body penalized;
+    eps = 1e-5, agreement 1e-7). Each body's masking transpose acts on its own blocks, so a sum
+    over every body's flagged cells in one body's correction fails it."""
+    rng = np.random.default_rng(59 + scheme)
+    bpd, ext = (4, 2, 2), 2 * np.pi
+    s = oracle.OracleSim(bpd, maxextent=ext)
+    for R, c in TWO_BODIES:
+        s.add_sphere(R, c)
+    assert s.body_count() == 2
+    shape = s.shape
+    U = (1.0, 0.0, 0.0)
+    Vs, Ws, w = [(-1.0, 0.1, 0.0), (-0.8, 0.0, 0.2)], [(0.3, -0.4, 0.8), (-0.5, 0.2, 0.6)], [3.7, -1.9]
+    u = tuple(0.3 * rng.standard_normal(shape) + m for m in (1.0, 0.8, -0.9))
+    d = tuple(rng.standard_normal(shape) for _ in range(3))
+    lam = tuple(rng.standard_normal(shape) for _ in range(3))
+    dt, nu, lam_, eps = 0.2 * s.h, 1e-3, 1e5, 1e-5
+
+    def J(uu, VV, WW):
+        u1, F = fwd_step_bodies(oracle, bpd, uu, dt, nu, U, VV, WW, lam_, ext, scheme=scheme, rate=rate)
+        return dot(lam, u1) + sum(wk * Fk for wk, Fk in zip(w, F))
+    s.set_velocity(*u)
+    lam0, dV, dW = s.adjoint_step_bodies(dt, nu, lam, Vs, Ws, lam_, True, w, U, scheme=scheme, injection_rate=rate)
+    fd = (J(tuple(a + eps * b for a, b in zip(u, d)), Vs, Ws) - J(tuple(a - eps * b for a, b in zip(u, d)), Vs, Ws)) / (2 * eps)
+    assert abs(fd - dot(d, lam0)) < 1e-7 * abs(fd)
+    for k in range(2):
+        for j in range(3):
+            e = np.eye(3)[j] * eps
+            Vp, Vm = list(Vs), list(Vs)
+            Vp[k], Vm[k] = tuple(np.add(Vs[k], e)), tuple(np.subtract(Vs[k], e))
+            fdv = (J(u, Vp, Ws) - J(u, Vm, Ws)) / (2 * eps)
+            assert abs(fdv - dV[k][j]) < 1e-7 * max(abs(fdv), 1e-3), (k, j, fdv, dV[k][j])
+            Wp, Wm = list(Ws), list(Ws)
+            Wp[k], Wm[k] = tuple(np.add(Ws[k], e)), tuple(np.subtract(Ws[k], e))
+            fdw = (J(u, Vs, Wp) - J(u, Vs, Wm)) / (2 * eps)
+            assert abs(fdw - dW[k][j]) < 1e-7 * max(abs(fdw), 1e-3), (k, j, fdw, dW[k][j])
+
+
 def test_adjoint_rotation_zero_matches_the_translating_body(oracle):
     """W = 0 through the rotating entry point: lam_n and dJ/dV as the translating body's."""
     rng = np.random.default_rng(5)
