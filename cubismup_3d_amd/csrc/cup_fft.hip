@@ -473,7 +473,9 @@ __global__ __launch_bounds__(NT) void k_fft_x_inv(const C2<T>* __restrict__ S, T
     const int bx = t >> 6, iy = (t >> 3) & 7, ix = t & 7;
     const int x = bx * BS + ix;
     const int b = bx + g.nbx * (by + g.nby * bz);
-    dst[(size_t)b * BS3 + (iz * BS + iy) * BS + ix] = rb[2 * (x * XFFT + (iy >> 1)) + (iy & 1)];
+    // streamed past the caches: the gradient pass reads p with its own halo order
+    __builtin_nontemporal_store(rb[2 * (x * XFFT + (iy >> 1)) + (iy & 1)],
+                                &dst[(size_t)b * BS3 + (iz * BS + iy) * BS + ix]);
   }
 }
 

@@ -23,6 +23,24 @@ struct C2 {
   T x, y;
 };
 
+// Two consecutive values stored with the non-temporal hint (a vector store streamed past the
+// caches, for outputs the next kernel reads in another order), and loaded with it (for inputs
+// read for the last time).
+template <typename T>
+__device__ __forceinline__ void store2_nt(T* p, T a, T b)
+{
+  typedef T v2 __attribute__((ext_vector_type(2)));
+  __builtin_nontemporal_store(v2{a, b}, reinterpret_cast<v2*>(p));
+}
+template <typename T>
+__device__ __forceinline__ void load2_nt(const T* p, T& a, T& b)
+{
+  typedef T v2 __attribute__((ext_vector_type(2)));
+  const v2 v = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p));
+  a = v.x;
+  b = v.y;
+}
+
 // Four-wide vector of the context precision (16- or 32-byte loads of 4 consecutive cells).
 template <typename T>
 struct Vec4;

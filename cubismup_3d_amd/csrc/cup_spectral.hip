@@ -644,24 +644,24 @@ __global__ __launch_bounds__(NT) void k_forcing_reduce(const double* __restrict_
   const double* r = fin;
   const double dt = sc->dt;
   const double norm = (double)N0 * N1 * N2;
-  if (t == 0) {
-    *counter = 0u;
-    const double tke = 0.5 * r[W - 1] / ncell;
-    double tke_low = 0;
-    for (int m = 0; m < lm.n; ++m)
-      for (int q = 0; q < 3; ++q) {
-        const double re = r[2 * (3 * m + q)], im = r[2 * (3 * m + q) + 1];
-        tke_low += re * re + im * im;
-      }
-    tke_low /= norm * norm;
-    double inj = 0;
-    if (tke_target > 0) inj = (tke_target - tke) / dt;
-    else if (inj_rate > 0) inj = inj_rate;
-    const double fac = dt * inj / (2 * tke_low);
-    sc->tke = tke;
-    sc->tke_low = tke_low;
-    sc->forcing_fac = fac;
-    facs[0] = fac;
+  if (t < 64) {
+    // sum over the low modes' 3n columns of |c|^2, lanes over columns and a fixed shuffle tree
+    double e = 0;
+    for (int o2 = t; o2 < nq; o2 += 64) e += r[2 * o2] * r[2 * o2] + r[2 * o2 + 1] * r[2 * o2 + 1];
+    e = wave_sum(e);
+    if (t == 0) {
+      *counter = 0u;
+      const double tke = 0.5 * r[W - 1] / ncell;
+      const double tke_low = e / (norm * norm);
+      double inj = 0;
+      if (tke_target > 0) inj = (tke_target - tke) / dt;
+      else if (inj_rate > 0) inj = inj_rate;
+      const double fac = dt * inj / (2 * tke_low);
+      sc->tke = tke;
+      sc->tke_low = tke_low;
+      sc->forcing_fac = fac;
+      facs[0] = fac;
+    }
   }
   __syncthreads();
   const double fac = facs[0];

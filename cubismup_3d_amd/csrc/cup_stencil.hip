@@ -517,9 +517,9 @@ __global__ __launch_bounds__(NT) void k_advdiff_xwalk(const T* __restrict__ u, c
       }
     }
     const size_t q = (size_t)(b0 + j) * BS3 + c0;
-    *reinterpret_cast<V2*>(ou + q) = V2{o[0][0], o[0][1]};
-    *reinterpret_cast<V2*>(ov + q) = V2{o[1][0], o[1][1]};
-    *reinterpret_cast<V2*>(ow + q) = V2{o[2][0], o[2][1]};
+    store2_nt(ou + q, o[0][0], o[0][1]);
+    store2_nt(ov + q, o[1][0], o[1][1]);
+    store2_nt(ow + q, o[2][0], o[2][1]);
   }
 }
 
@@ -751,7 +751,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SEP ? 1 : 5)
   V2 bnext[SEP ? 3 : 1];
   if (SEP)
 #pragma unroll
-    for (int q = 0; q < 3; ++q) bnext[q] = *reinterpret_cast<const V2*>(base[q] + (size_t)b0 * BS3 + 2 * threadIdx.x);
+    for (int q = 0; q < 3; ++q) load2_nt(base[q] + (size_t)b0 * BS3 + 2 * threadIdx.x, bnext[q].x, bnext[q].y);
 #pragma unroll
   for (int j = 0; j < NBW; ++j) {
     if (j) __syncthreads();  // the previous block's compute is done with the lab
@@ -764,7 +764,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SEP ? 1 : 5)
     const size_t o = (size_t)(b0 + j) * BS3 + 2 * threadIdx.x;
     if (SEP && j + 1 < NBW)
 #pragma unroll
-      for (int q = 0; q < 3; ++q) bnext[q] = *reinterpret_cast<const V2*>(base[q] + o + BS3);
+      for (int q = 0; q < 3; ++q) load2_nt(base[q] + o + BS3, bnext[q].x, bnext[q].y);
     T a0[2], a1[2], a2[2];
     {
       const V2 A = *reinterpret_cast<const V2*>(&lab[0][l]), B = *reinterpret_cast<const V2*>(&lab[1][l]),
@@ -800,7 +800,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SEP ? 1 : 5)
       }
       const V2 bb = SEP ? *reinterpret_cast<const V2*>(&bl[SEP ? q : 0][2 * threadIdx.x]) : xc;
       T* dst = q == 0 ? ou : (q == 1 ? ov : ow);
-      *reinterpret_cast<V2*>(dst + o) = V2{bb.x + res[0], bb.y + res[1]};
+      store2_nt(dst + o, bb.x + res[0], bb.y + res[1]);
     }
   }
 }
@@ -1128,10 +1128,9 @@ __global__ __launch_bounds__(NT) void k_gradp(const T* __restrict__ p, const T* 
   auto fetch = [&](int b) {
     lab_fetch<BC>(r, src, s, g, b, {HK_COPY});
     const size_t o = (size_t)b * BS3 + c0;
-    const V2 a = *reinterpret_cast<const V2*>(su + o);
-    const V2 bb = *reinterpret_cast<const V2*>(sv + o);
-    const V2 cc = *reinterpret_cast<const V2*>(sw + o);
-    us[0][0] = a.x; us[0][1] = a.y; us[1][0] = bb.x; us[1][1] = bb.y; us[2][0] = cc.x; us[2][1] = cc.y;
+    load2_nt(su + o, us[0][0], us[0][1]);
+    load2_nt(sv + o, us[1][0], us[1][1]);
+    load2_nt(sw + o, us[2][0], us[2][1]);
     if (FORCE) {
       // rows (y = by*8 + iy', z = bz*8 + t/32): 8 runs of 128 contiguous values
       int bx, by, bz;
