@@ -792,11 +792,23 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SEP ? 1 : 5)
 #pragma unroll
       for (int k = 0; k < 2; ++k) {
         const T fc = X[2 + k];
-        const T adv = a0[k] * up3(a0[k], X[k], X[k + 1], fc, X[k + 3], X[k + 4]) +
-                      a1[k] * up3(a1[k], Ym2[k], Ym1[k], fc, Yp1[k], Yp2[k]) +
-                      a2[k] * up3(a2[k], Zm2[k], Zm1[k], fc, Zp1[k], Zp2[k]);
-        const T lap = (X[k + 1] + X[k + 3] - 2 * fc) + (Ym1[k] + Yp1[k] - 2 * fc) + (Zm1[k] + Zp1[k] - 2 * fc);
-        res[k] = facA * adv + facD * lap;
+        // a D(a) per axis without a branch: with S = D+ + D- and T = D+ - D- (up3's two sides),
+        // a D(a) = (a S + |a| T) / 2 for either sign; m1 + p1 is shared with the Laplacian
+        T pm[3], adv2 = 0;
+        {
+          const T m2[3] = {X[k], Ym2[k], Zm2[k]}, m1[3] = {X[k + 1], Ym1[k], Zm1[k]},
+                  p1[3] = {X[k + 3], Yp1[k], Zp1[k]}, p2[3] = {X[k + 4], Yp2[k], Zp2[k]};
+          const T a[3] = {a0[k], a1[k], a2[k]};
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            pm[d] = m1[d] + p1[d];
+            const T S = 8 * (p1[d] - m1[d]) + (m2[d] - p2[d]);
+            const T Tq = (m2[d] + p2[d]) - 4 * pm[d] + 6 * fc;
+            adv2 += a[d] * S + fabs(a[d]) * Tq;
+          }
+        }
+        const T lap = pm[0] + pm[1] + pm[2] - 6 * fc;
+        res[k] = (facA * (T)0.5) * adv2 + facD * lap;
       }
       const V2 bb = SEP ? *reinterpret_cast<const V2*>(&bl[SEP ? q : 0][2 * threadIdx.x]) : xc;
       T* dst = q == 0 ? ou : (q == 1 ? ov : ow);
