@@ -1093,48 +1093,56 @@ static int adjoint_bodies(Ctx& c, double dt, double nu, const double uinf[3], in
     if (int rc = alloc_adjf(c)) return rc;
     if (int rc = keep_velocity(c, false)) return rc;
   }
-  if (int rc = launch_advdiff(c, nu, uinf, scheme)) return rc;
-  if (forced) {
-    double st[5];
-    if (int rc = forcing_plain(c, dt, dt, nu, tke_target, injection_rate, st, c.ustar)) return rc;
-  }
-  if (dJdc) {
-    // the penalization's input P_M(u*) needs the forward pressure: masked and corrected RHS,
-    // Poisson solve into P; then the chi sensitivity of the penalization and its drag
-    const AdjBodyArgs a = args(0);
-    if (int rc = launch_adjoint_body(c, ADJB_FWD_MASK, a)) return rc;
-    if (int rc = launch_adjoint_body(c, ADJB_FWD_CORRECT, a)) return rc;
-    if (int rc = launch_poisson_fused(c, false, 0.0, 0.0)) return rc;
-    if (int rc = launch_adjoint_body(c, ADJB_CHI_PEN, a)) return rc;
-  }
-  // each body's penalization and drag weight; its dJ/dV and dJ/dW of this step
-  for (int k = 0; k < nbody; ++k) {
-    BodyScope scope(c, k);
-    if (int rc = launch_adjoint_body(c, ADJB_PENALIZE, args(k))) return rc;
-    if (int rc = read_scalars(c)) return rc;
-    for (int d = 0; d < 3; ++d) {
-      dJdV[3 * k + d] = c.sc_host->red[8 + d];
-      if (dJdW) dJdW[3 * k + d] = c.sc_host->red[17 + d];
+  // from the forward state to lam* = P_M^T Pen^T lam; u_n is put back afterwards, also after a failure
+  auto transposes = [&]() -> int {
+    if (int rc = launch_advdiff(c, nu, uinf, scheme)) return rc;
+    if (forced) {
+      double st[5];
+      if (int rc = forcing_plain(c, dt, dt, nu, tke_target, injection_rate, st, c.ustar)) return rc;
     }
+    if (dJdc) {
+      // the penalization's input P_M(u*) needs the forward pressure: masked and corrected RHS,
+      // Poisson solve into P; then the chi sensitivity of the penalization and its drag
+      const AdjBodyArgs a = args(0);
+      if (int rc = launch_adjoint_body(c, ADJB_FWD_MASK, a)) return rc;
+      if (int rc = launch_adjoint_body(c, ADJB_FWD_CORRECT, a)) return rc;
+      if (int rc = launch_poisson_fused(c, false, 0.0, 0.0)) return rc;
+      if (int rc = launch_adjoint_body(c, ADJB_CHI_PEN, a)) return rc;
+    }
+    // each body's penalization and drag weight; its dJ/dV and dJ/dW of this step
+    for (int k = 0; k < nbody; ++k) {
+      BodyScope scope(c, k);
+      if (int rc = launch_adjoint_body(c, ADJB_PENALIZE, args(k))) return rc;
+      if (int rc = read_scalars(c)) return rc;
+      for (int d = 0; d < 3; ++d) {
+        dJdV[3 * k + d] = c.sc_host->red[8 + d];
+        if (dJdW) dJdW[3 * k + d] = c.sc_host->red[17 + d];
+      }
+    }
+    // transposed projection: y = L^{-1} R lam' (no obstacle RHS), y <- M'^T y body by body (each
+    // body's masked RHS of u* on its blocks, the linearisation point, formed first), lam* = lam' -
+    // (dt/2h) grad y
+    const int us0 = c.ustar;
+    c.ustar = CUP_ADJ_U;
+    c.plain_rhs = true;
+    int rc = launch_poisson_fused(c, false, 0.0, 0.0);
+    c.plain_rhs = false;
+    c.ustar = us0;
+    if (rc) return rc;
+    for (int k = 0; k < nbody; ++k) {
+      BodyScope scope(c, k);
+      if (int e = launch_adjoint_body(c, ADJB_FWD_MASK, args(k))) return e;
+      if (int e = launch_adjoint_body(c, ADJB_MASK, args(k))) return e;
+    }
+    if (int e = launch_gradp_inplace(c, CUP_ADJ_U)) return e;
+    return CUP_OK;
+  };
+  int rc = transposes();
+  if (midpoint) {
+    const int r = keep_velocity(c, true);
+    if (!rc) rc = r;
   }
-  // transposed projection: y = L^{-1} R lam' (no obstacle RHS), y <- M'^T y body by body (each
-  // body's masked RHS of u* on its blocks, the linearisation point, formed first), lam* = lam' -
-  // (dt/2h) grad y
-  const int us0 = c.ustar;
-  c.ustar = CUP_ADJ_U;
-  c.plain_rhs = true;
-  int rc = launch_poisson_fused(c, false, 0.0, 0.0);
-  c.plain_rhs = false;
-  c.ustar = us0;
   if (rc) return rc;
-  for (int k = 0; k < nbody; ++k) {
-    BodyScope scope(c, k);
-    if (int e = launch_adjoint_body(c, ADJB_FWD_MASK, args(k))) return e;
-    if (int e = launch_adjoint_body(c, ADJB_MASK, args(k))) return e;
-  }
-  if (int e = launch_gradp_inplace(c, CUP_ADJ_U)) return e;
-  if (midpoint)
-    if (int e = keep_velocity(c, true)) return e;
   // the forcing's transpose at A(u_n), then the advection's
   if (forced)
     if (int e = adjoint_forcing_at_ustar(c, dt, nu, uinf, scheme, tke_target, injection_rate)) return e;
