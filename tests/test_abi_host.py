@@ -217,3 +217,40 @@ def test_read_xdmf_follows_the_descriptor(cup, tmp_path):
     for q, c in enumerate("uvw"):
         np.testing.assert_array_equal(x["attributes"][c], data[q])
     np.testing.assert_array_equal(x["attributes"]["velocity"], np.moveaxis(data, 0, -1))
+
+
+class _StubGrid:
+    """Stands in for a Grid where the rollouts' argument checks run before any device call."""
+
+    def __init__(self, nbody):
+        self.nbody = nbody
+
+    def body_count(self):
+        return self.nbody
+
+
+def test_adjoint_rollouts_check_their_arguments(cup):
+    """mean_drag refuses a differentiable centre with a nonzero rotation and negative forcing
+    rates; bodies_drag checks the body count and the shapes of its velocities and weights,
+    all before any device call."""
+    import torch
+    from cubismup_3d_amd.adjoint import bodies_drag, mean_drag
+    u0 = torch.zeros(3, 8, 8, 8)
+    V = torch.zeros(3, dtype=torch.float64)
+    c = torch.zeros(3, dtype=torch.float64, requires_grad=True)
+    W = torch.tensor([0.0, 0.0, 1.0], dtype=torch.float64)
+    with pytest.raises(ValueError, match="without rotation"):
+        mean_drag(u0, V, _StubGrid(1), 1, 1e-3, (1, 0, 0), [1e-3], centre=c, radius=0.1, angvel=W)
+    with pytest.raises(ValueError, match="radius"):
+        mean_drag(u0, V, _StubGrid(1), 1, 1e-3, (1, 0, 0), [1e-3], centre=c)
+    with pytest.raises(ValueError, match=">= 0"):
+        mean_drag(u0, V, _StubGrid(1), 1, 1e-3, (1, 0, 0), [1e-3], tke_target=-1.0)
+    V2, W2 = torch.zeros(2, 3, dtype=torch.float64), torch.zeros(2, 3, dtype=torch.float64)
+    with pytest.raises(ValueError, match="no body"):
+        bodies_drag(u0, V2, W2, _StubGrid(0), 1, 1e-3, (1, 0, 0), [1e-3])
+    with pytest.raises(ValueError, match=r"\[3\]\[3\]"):
+        bodies_drag(u0, V2, W2, _StubGrid(3), 1, 1e-3, (1, 0, 0), [1e-3])
+    with pytest.raises(ValueError, match="2 weights"):
+        bodies_drag(u0, V2, W2, _StubGrid(2), 1, 1e-3, (1, 0, 0), [1e-3], weights=[1.0])
+    with pytest.raises(ValueError, match=">= 0"):
+        bodies_drag(u0, V2, W2, _StubGrid(2), 1, 1e-3, (1, 0, 0), [1e-3], injection_rate=-0.5)
