@@ -725,7 +725,7 @@ __device__ __forceinline__ void up3_put(const Up3Regs<T>& r, const Up3Regs<T>& p
 
 // (Without SEP the walk asks for five waves per SIMD, which costs it a few bytes of spill.)
 template <typename T, int NBW, int PD, bool SEP>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SEP ? 1 : 5))) void k_advdiff_up3_xwalk(const T* __restrict__ su, const T* __restrict__ sv,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SEP || sizeof(T) > 4 ? 1 : 5))) void k_advdiff_up3_xwalk(const T* __restrict__ su, const T* __restrict__ sv,
                                                           const T* __restrict__ sw, const T* bu, const T* bv,
                                                           const T* bw, T* ou, T* ov, T* ow, GridArgs g,
                                                           const DevScalars* __restrict__ sc, T Ux, T Uy, T Uz,
