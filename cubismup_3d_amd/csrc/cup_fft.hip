@@ -19,6 +19,7 @@
 // Unnormalized in both directions like FFTW's r2c/c2r; the 1/N^3 and h factors live in the
 // division (PoissonSolverPeriodic.cpp:23-35).
 #include "cup_internal.h"
+#include "cup_forcing_dev.h"
 
 #include <algorithm>
 #include <cmath>
@@ -645,17 +646,33 @@ __device__ __forceinline__ void fft_b_to_a(C2<T> (&v)[16], C2<T>* buf, const C2<
   dft16<DIR>(v);
 }
 
-// The column pass of k_fft_col in the register form (same arguments and modes).
-template <typename T, int N, int AXIS, int MODE, bool FORCE = false>
+// The column pass of k_fft_col in the register form (same arguments and modes). EXTRA: the
+// grid's first ex.ey rows are workgroups of the forcing's plane stage (1, beside the y forward
+// pass) or row stage (2, beside the z pass; coef is the forcing's coefficient table), in the
+// column tile's LDS.
+template <typename T, int N, int AXIS, int MODE, bool FORCE = false, int EXTRA = 0>
 __global__ __launch_bounds__(NT) void k_fft_colr(C2<T>* __restrict__ S, const C2<T>* __restrict__ tw, int ncols,
                                                  int Ny, int Nz, double norm, const double* __restrict__ cx,
                                                  const double* __restrict__ cy, const double* __restrict__ cz,
-                                                 const LowModes* __restrict__ lmp, const double* __restrict__ coef)
+                                                 const LowModes* __restrict__ lmp, const double* __restrict__ coef,
+                                                 ColExtra ex)
 {
   using CR = ColReg<T, N>;
   constexpr int TK = CR::TK, NTH = CR::NTH, RPT = CR::RPT;
-  __shared__ C2<T> buf[16 * CR::ROWP];
-  const int kx0 = blockIdx.x * TK, other = blockIdx.y;
+  __shared__ __align__(16) C2<T> buf[16 * CR::ROWP];
+  static_assert(EXTRA == 0 || (sizeof(buf) >= FORCING_PLANES_LDS && sizeof(buf) >= FORCING_ROWS_LDS),
+                "the forcing stages fit the column tile's LDS");
+  if constexpr (EXTRA != 0) {
+    if ((int)blockIdx.y < ex.ey) {
+      const int w = blockIdx.y * gridDim.x + blockIdx.x;  // uniform over the workgroup
+      if (EXTRA == 1 && w < ex.nz * ex.parts)
+        forcing_planes_wg<T>(ex.rowbuf, ex.plane, Ny, (const C2<T>*)ex.twy, w / ex.parts, w % ex.parts, ex.parts, buf);
+      else if (EXTRA == 2 && w < ex.nz)
+        forcing_rows_wg<T>(lmp, coef, (const C2<T>*)ex.twy, (const C2<T>*)ex.twz, (T*)ex.rows, Ny, Nz, w, buf);
+      return;
+    }
+  }
+  const int kx0 = blockIdx.x * TK, other = blockIdx.y - (EXTRA != 0 ? ex.ey : 0);
   const int f = threadIdx.x % TK, j = threadIdx.x / TK;
   const bool live = kx0 + f < ncols;
   // element n of column kx0 + f
@@ -792,27 +809,45 @@ struct XPass {
 
 template <typename T, int N, int AXIS>
 struct ColPass {
-  static void run(Ctx& c, C2<T>* spec, int mode, double norm, bool force = false)
+  // extra: 1 (y forward) or 2 (z pass with the forcing) runs the forcing's plane or row stage
+  // beside the transforms; only the register form takes it (col_extra_ok)
+  static void run(Ctx& c, C2<T>* spec, int mode, double norm, bool force = false, int extra = 0)
   {
     if constexpr (ColReg<T, N>::ok) {
       constexpr int TK = ColReg<T, N>::TK;
       const int ncols = c.N[0] / 2;
       const int other = AXIS == 1 ? c.N[2] : c.N[1];
-      dim3 grid((ncols + TK - 1) / TK, other);
+      const int gx = (ncols + TK - 1) / TK;
+      ColExtra ex{};
+      if (extra) {
+        ex = forcing_planes_extra(c);
+        const int work = extra == 1 ? ex.nz * ex.parts : ex.nz;
+        ex.ey = (work + gx - 1) / gx;
+      }
+      dim3 grid(gx, other + ex.ey);
       const C2<T>* tw = (const C2<T>*)c.tw[AXIS];
-      if (mode == ZM_FWD)
+      if (mode == ZM_FWD && extra == 1)
+        hipLaunchKernelGGL((k_fft_colr<T, N, AXIS, ZM_FWD, false, 1>), grid, dim3(NT), 0, c.stream, spec, tw, ncols,
+                           c.N[1], c.N[2], norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low_dev, (const double*)nullptr,
+                           ex);
+      else if (mode == ZM_FWD)
         hipLaunchKernelGGL((k_fft_colr<T, N, AXIS, ZM_FWD>), grid, dim3(NT), 0, c.stream, spec, tw, ncols, c.N[1],
-                           c.N[2], norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low_dev, (const double*)nullptr);
+                           c.N[2], norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low_dev, (const double*)nullptr, ex);
       else if (mode == ZM_INV)
         hipLaunchKernelGGL((k_fft_colr<T, N, AXIS, ZM_INV>), grid, dim3(NT), 0, c.stream, spec, tw, ncols, c.N[1],
-                           c.N[2], norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low_dev, (const double*)nullptr);
+                           c.N[2], norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low_dev, (const double*)nullptr, ex);
+      else if (AXIS == 2 && force && extra == 2)
+        hipLaunchKernelGGL((k_fft_colr<T, N, 2, ZM_POISSON, true, 2>), grid, dim3(NT), 0, c.stream, spec, tw, ncols,
+                           c.N[1], c.N[2], norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low_dev,
+                           (const double*)c.low_coef, ex);
       else if (AXIS == 2 && force)
         hipLaunchKernelGGL((k_fft_colr<T, N, 2, ZM_POISSON, true>), grid, dim3(NT), 0, c.stream, spec, tw, ncols,
                            c.N[1], c.N[2], norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low_dev,
-                           (const double*)c.low_coef);
+                           (const double*)c.low_coef, ex);
       else if (AXIS == 2)
         hipLaunchKernelGGL((k_fft_colr<T, N, 2, ZM_POISSON, false>), grid, dim3(NT), 0, c.stream, spec, tw, ncols,
-                           c.N[1], c.N[2], norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low_dev, (const double*)nullptr);
+                           c.N[1], c.N[2], norm, c.cos4[0], c.cos4[1], c.cos4[2], c.low_dev, (const double*)nullptr,
+                           ex);
       return;
     }
     constexpr int TK = ColTile<T, N>::TK;
@@ -880,6 +915,18 @@ template <typename T, int N>
 struct GoZF {
   static void go(Ctx& c, C2<T>* spec, int mode, double norm) { ColPass<T, N, 2>::run(c, spec, mode, norm, true); }
 };
+// the same passes with the forcing's plane stage (y forward) or row stage (z) beside them
+template <typename T, int N>
+struct GoYPlanes {
+  static void go(Ctx& c, C2<T>* spec, int mode, double norm) { ColPass<T, N, 1>::run(c, spec, mode, norm, false, 1); }
+};
+template <typename T, int N>
+struct GoZFRows {
+  static void go(Ctx& c, C2<T>* spec, int mode, double norm) { ColPass<T, N, 2>::run(c, spec, mode, norm, true, 2); }
+};
+
+// The column lengths whose passes take the register form (ColReg::ok over dispatch_n's cases).
+static bool col_extra_ok(int n) { return n == 64 || n == 128 || n == 256; }
 
 static double poisson_norm(const Ctx& c)
 {
@@ -895,19 +942,29 @@ static int poisson_fused_t(Ctx& c, bool force, double tke_target, double inj_rat
   dispatch_n<GoXFwd, T>(c.N[0], c, (const T*)c.f[c.ustar], (const T*)c.f[c.ustar + 1],
                         (const T*)c.f[c.ustar + 2], (int)XSRC_DIV, force);
   stage_mark(c, ST_XFWD, true);
-  if (force) {
-    // the forcing's mode coefficients and, from them, the per-row synthesis coefficients the
-    // gradient reads: small kernels in order (a second stream's fork and join cost more here)
+  // With the register form of the column passes, the forcing's plane stage runs as extra
+  // workgroups of the y forward pass and its row stage beside the z pass, leaving the mode sums
+  // (k_forcing_reduce) the only kernel of the forcing between them. Otherwise the three stages
+  // are small kernels in order (a second stream's fork and join cost more here).
+  const bool extra = force && col_extra_ok(c.N[1]) && col_extra_ok(c.N[2]);
+  if (force && !extra) {
     stage_mark(c, ST_FREDUCE, false);
     if (int rc = launch_forcing_reduce(c, tke_target, inj_rate)) return rc;
     if (int rc = launch_forcing_rows(c)) return rc;
     stage_mark(c, ST_FREDUCE, true);
   }
   stage_mark(c, ST_YFWD, false);
-  dispatch_n<GoY, T>(c.N[1], c, S, (int)ZM_FWD, norm);
+  if (extra) dispatch_n<GoYPlanes, T>(c.N[1], c, S, (int)ZM_FWD, norm);
+  else dispatch_n<GoY, T>(c.N[1], c, S, (int)ZM_FWD, norm);
   stage_mark(c, ST_YFWD, true);
+  if (extra) {
+    stage_mark(c, ST_FREDUCE, false);
+    if (int rc = launch_forcing_reduce(c, tke_target, inj_rate, false)) return rc;
+    stage_mark(c, ST_FREDUCE, true);
+  }
   stage_mark(c, ST_ZPASS, false);
-  if (force) dispatch_n<GoZF, T>(c.N[2], c, S, (int)ZM_POISSON, norm);
+  if (extra) dispatch_n<GoZFRows, T>(c.N[2], c, S, (int)ZM_POISSON, norm);
+  else if (force) dispatch_n<GoZF, T>(c.N[2], c, S, (int)ZM_POISSON, norm);
   else dispatch_n<GoZ, T>(c.N[2], c, S, (int)ZM_POISSON, norm);
   stage_mark(c, ST_ZPASS, true);
   stage_mark(c, ST_YINV, false);

@@ -356,7 +356,10 @@ int launch_ic_hit(Ctx& c, double k0, const double* K, const double* E, int tn, d
 // field: the first of the three velocity fields forced in place (-1: u* under FUSED, u under PLAIN)
 int forcing_plain(Ctx& c, double dt, double sim_dt, double nu, double tke_target, double inj,
                   double stats[5], int field = -1);
-int launch_forcing_reduce(Ctx& c, double tke_target, double inj_rate);
+// planes = false: the plane stage ran beside the y pass (forcing_planes_extra)
+int launch_forcing_reduce(Ctx& c, double tke_target, double inj_rate, bool planes = true);
+struct ColExtra;
+ColExtra forcing_planes_extra(Ctx& c);  // the plane and row stages' buffers, for the column passes
 int launch_forcing_rows(Ctx& c);
 // adjoint fields += the forcing's transpose at u* (fields us0..us0+2); synchronous
 int launch_adjoint_forcing(Ctx& c, int us0, double dt, double tke_target, double inj_rate);

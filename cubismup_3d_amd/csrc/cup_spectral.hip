@@ -1,6 +1,7 @@
 // cup_spectral.hip -- spectral pieces of isotropic turbulence: the seeded initial
 // condition and band-limited linear forcing in the reference's decomposition.
 #include "cup_internal.h"
+#include "cup_forcing_dev.h"
 
 #include <algorithm>
 #include <cmath>
@@ -523,40 +524,14 @@ int spectral_setup(Ctx& c)
   return CUP_OK;
 }
 
-// Fused forcing, stage 2 (plane DFTs in y): workgroup (z, part) over the x pass's row values
-// R(y, z, s) (16 real values per row, rowbuf[(z*Ny + y)*16 + s]) of rows y in part's quarter
-// (gridDim.y parts). For each slot s it forms P(z, part, s) = sum_y R e^{-i ky y} at
-// ky = 0, 1, 2 (5 real values: the ky = 0 sum, then re/im at ky = 1 and 2; ky < 0 are their
-// conjugates, the slots being real) into plane[((z*parts + part)*16 + s)*5 + k]. Thread t takes
-// slot t % 16 and every 16th row from t/16; the 16 row groups are added in a fixed order.
+// Fused forcing, stage 2 (plane DFTs in y, forcing_planes_wg) as a kernel of its own:
+// grid (Nz, parts).
 template <typename T>
 __global__ __launch_bounds__(NT) void k_forcing_planes(const double* __restrict__ rowbuf, double* __restrict__ plane,
                                                        int Ny, const C2<T>* __restrict__ twy)
 {
-  static_assert(NT == 256, "16 slots x 16 row groups");
-  __shared__ double acc[16][16][5];
-  const int t = threadIdx.x, z = blockIdx.x, sl = t & 15, grp = t >> 4;
-  const int parts = gridDim.y, ny = Ny / parts, y0 = blockIdx.y * ny;
-  double a[5] = {0, 0, 0, 0, 0};
-#pragma unroll 4
-  for (int y = y0 + grp; y < y0 + ny; y += 16) {
-    const double R = rowbuf[((size_t)z * Ny + y) * 16 + sl];
-    const C2<T> e1 = twy[y], e2 = twy[2 * y < Ny ? 2 * y : 2 * y - Ny];  // e^{-i a y}, e^{-2 i a y}
-    a[0] += R;
-    a[1] += R * (double)e1.x;
-    a[2] += R * (double)e1.y;
-    a[3] += R * (double)e2.x;
-    a[4] += R * (double)e2.y;
-  }
-#pragma unroll
-  for (int k = 0; k < 5; ++k) acc[grp][sl][k] = a[k];
-  __syncthreads();
-  if (t < 16 * 5) {
-    const int s2 = t / 5, k = t % 5;
-    double v = 0;
-    for (int g2 = 0; g2 < 16; ++g2) v += acc[g2][s2][k];
-    plane[(((size_t)z * parts + blockIdx.y) * 16 + s2) * 5 + k] = v;
-  }
+  __shared__ double acc[FORCING_PLANES_LDS / 8];
+  forcing_planes_wg<T>(rowbuf, plane, Ny, twy, blockIdx.x, blockIdx.y, gridDim.y, acc);
 }
 
 // Fused forcing, stage 3 (the z sums): one workgroup per column owner, (mode m, component q)
@@ -684,7 +659,20 @@ __global__ __launch_bounds__(NT) void k_forcing_reduce(const double* __restrict_
   }
 }
 
-int launch_forcing_reduce(Ctx& c, double tke_target, double inj_rate)
+ColExtra forcing_planes_extra(Ctx& c)
+{
+  ColExtra e{};
+  e.rowbuf = c.low_part;
+  e.plane = c.low_red;
+  e.twy = c.tw[1];
+  e.twz = c.tw[2];
+  e.rows = c.low_rows;
+  e.parts = forcing_plane_parts(c.N[1]);
+  e.nz = c.N[2];
+  return e;
+}
+
+int launch_forcing_reduce(Ctx& c, double tke_target, double inj_rate, bool planes)
 {
   const int W = 6 * c.nlow + 1;
   const int nz = c.N[2];
@@ -692,14 +680,16 @@ int launch_forcing_reduce(Ctx& c, double tke_target, double inj_rate)
   double* plane = c.low_red;  // [Nz][parts][16][5] plane DFTs, then the W column sums
   double* red = plane + (size_t)nz * FR_PARTS_MAX * 16 * 5;
   if (c.precision == 4) {
-    hipLaunchKernelGGL((k_forcing_planes<float>), dim3(nz, parts), dim3(NT), 0, c.stream, (const double*)c.low_part,
-                       plane, c.N[1], (const C2<float>*)c.tw[1]);
+    if (planes)
+      hipLaunchKernelGGL((k_forcing_planes<float>), dim3(nz, parts), dim3(NT), 0, c.stream,
+                         (const double*)c.low_part, plane, c.N[1], (const C2<float>*)c.tw[1]);
     hipLaunchKernelGGL((k_forcing_reduce<float>), dim3(3 * c.nlow + 1), dim3(NT), 0, c.stream, (const double*)plane, W,
                        red, c.low_counter, c.low_dev, c.sc, c.low_coef, tke_target, inj_rate, c.h, (double)c.ncell,
                        c.N[0], c.N[1], c.N[2], (const C2<float>*)c.tw[2], parts);
   } else {
-    hipLaunchKernelGGL((k_forcing_planes<double>), dim3(nz, parts), dim3(NT), 0, c.stream, (const double*)c.low_part,
-                       plane, c.N[1], (const C2<double>*)c.tw[1]);
+    if (planes)
+      hipLaunchKernelGGL((k_forcing_planes<double>), dim3(nz, parts), dim3(NT), 0, c.stream,
+                         (const double*)c.low_part, plane, c.N[1], (const C2<double>*)c.tw[1]);
     hipLaunchKernelGGL((k_forcing_reduce<double>), dim3(3 * c.nlow + 1), dim3(NT), 0, c.stream, (const double*)plane,
                        W, red, c.low_counter, c.low_dev, c.sc, c.low_coef, tke_target, inj_rate, c.h,
                        (double)c.ncell, c.N[0], c.N[1], c.N[2], (const C2<double>*)c.tw[2], parts);
@@ -707,83 +697,16 @@ int launch_forcing_reduce(Ctx& c, double tke_target, double inj_rate)
   return check_hip(c, hipGetLastError(), "k_forcing_reduce");
 }
 
-// Fused forcing, stage 3: for every grid row (y, z) the coefficients of the velocity
-// modes' synthesis along x, fac*u_low(x) = R0 + 2 Re(R1 e^{2 pi i x/Nx}) + 2 Re(R2 e^{4 pi i x/Nx})
-// per component (5 values x 3 = 15, padded to 16), so the gradient pass evaluates the
-// forcing with a handful of FMAs per cell. Mode m (velocity coefficient c, component by
-// component) contributes c e^{2 pi i (ky y/Ny + kz z/Nz)} to the entries of |kx|: 2 Re at
-// kx = 0, Re and (sign kx) Im at |kx| = 1, 2. Taking a mode with kx < 0 as its conjugate
-// (conj c, -ky, -kz; the real parts are unchanged), the row value is a sum over ky of
-// A_ky(z) e^{2 pi i ky y/Ny} with A_ky(z) = sum over the modes of that ky of c e^{2 pi i kz z/Nz}:
-// one workgroup per z plane forms B_m = c_m e^{...z} per mode, then the 5 x 3 x 3 sums A (in mode
-// order), then each thread the 16 entries of one row from two y twiddles. All in FP64; the
-// phases are conjugates of the twiddles e^{-2 pi i t/N}.
+// Fused forcing, stage 4 (the per-row synthesis coefficients, forcing_rows_wg) as a kernel of
+// its own: one workgroup per z plane.
 template <typename T>
 __global__ __launch_bounds__(NT) void k_forcing_rows(const LowModes* __restrict__ lmp,
                                                      const double* __restrict__ coef, const C2<T>* __restrict__ twy,
                                                      const C2<T>* __restrict__ twz, T* __restrict__ rows, int Ny,
                                                      int Nz)
 {
-  __shared__ int mk[LOW_MAX][3];           // canonical (kx >= 0) wavenumbers
-  __shared__ double B[LOW_MAX][3][2];      // c_m e^{2 pi i kz z/Nz} per component
-  __shared__ double A[5][3][3][2];         // [ky + 2][component][|kx|]: re, im
-  const LowModes& lm = *lmp;
-  const int n = lm.n, t = threadIdx.x, z = blockIdx.x;
-  if (t < n) {
-    const int kx = lm.k[t][0], sg = kx < 0 ? -1 : 1;
-    const int ky = sg * lm.k[t][1], kz = sg * lm.k[t][2];
-    mk[t][0] = sg * kx;
-    mk[t][1] = ky;
-    mk[t][2] = kz;
-    const C2<T> e = twz[((kz * z) % Nz + Nz) % Nz];
-    const double ec = (double)e.x, es = -(double)e.y;  // e^{+2 pi i kz z/Nz}
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      const double cr = coef[8 * t + 2 + 2 * q], ci = sg * coef[8 * t + 3 + 2 * q];
-      B[t][q][0] = cr * ec - ci * es;
-      B[t][q][1] = cr * es + ci * ec;
-    }
-  }
-  __syncthreads();
-  if (t < 5 * 3 * 3) {
-    const int ky = t / 9 - 2, q = (t / 3) % 3, ak = t % 3;
-    double ar = 0, ai = 0;
-    for (int m = 0; m < n; ++m)
-      if (mk[m][1] == ky && mk[m][0] == ak) {
-        ar += B[m][q][0];
-        ai += B[m][q][1];
-      }
-    A[ky + 2][q][ak][0] = ar;
-    A[ky + 2][q][ak][1] = ai;
-  }
-  __syncthreads();
-  for (int y = t; y < Ny; y += NT) {
-    const C2<T> t1 = twy[y], t2 = twy[(2 * y) % Ny];
-    // e^{+2 pi i ky y/Ny} for ky = -2 .. 2
-    const double er[5] = {(double)t2.x, (double)t1.x, 1.0, (double)t1.x, (double)t2.x};
-    const double ei[5] = {(double)t2.y, (double)t1.y, 0.0, -(double)t1.y, -(double)t2.y};
-    double out[16];
-#pragma unroll
-    for (int q = 0; q < 3; ++q)
-#pragma unroll
-      for (int ak = 0; ak < 3; ++ak) {
-        double xr = 0, xi = 0;
-#pragma unroll
-        for (int k = 0; k < 5; ++k) {
-          xr += A[k][q][ak][0] * er[k] - A[k][q][ak][1] * ei[k];
-          xi += A[k][q][ak][0] * ei[k] + A[k][q][ak][1] * er[k];
-        }
-        if (ak == 0) out[5 * q] = 2 * xr;
-        else {
-          out[5 * q + 2 * ak - 1] = xr;
-          out[5 * q + 2 * ak] = xi;
-        }
-      }
-    out[15] = 0;
-    T* o = rows + ((size_t)z * Ny + y) * 16;
-#pragma unroll
-    for (int q = 0; q < 16; ++q) o[q] = (T)out[q];
-  }
+  __shared__ double lds[(FORCING_ROWS_LDS + 7) / 8];
+  forcing_rows_wg<T>(lmp, coef, twy, twz, rows, Ny, Nz, blockIdx.x, lds);
 }
 
 int launch_forcing_rows(Ctx& c)
