@@ -53,7 +53,7 @@ static void release(Ctx& c)
     if (p) { hipFree(p); p = nullptr; }
   if (c.ana) hipFree(c.ana);
   c.ana = nullptr;
-  void** bufs[] = {&c.spec, &c.spec2, &c.spec3, &c.rbuf, &c.tw[0], &c.tw[1], &c.tw[2]};
+  void** bufs[] = {&c.spec, &c.spec2, &c.spec3, &c.rbuf, &c.xface, &c.tw[0], &c.tw[1], &c.tw[2]};
   for (void** b : bufs)
     if (*b) { hipFree(*b); *b = nullptr; }
   for (auto& p : c.cos4)
@@ -207,6 +207,7 @@ int cup_create(const cup_grid_desc* d, cup_ctx** out)
     if (rc) break;
     if ((rc = check_hip(c, hipMalloc(&c.spec, c.ncell * c.esz), "hipMalloc spectrum"))) break;
     if ((rc = check_hip(c, hipMalloc(&c.rbuf, c.ncell * c.esz), "hipMalloc rhs"))) break;
+    if ((rc = check_hip(c, hipMalloc(&c.xface, (size_t)c.nb * 2 * 64 * c.esz), "hipMalloc x faces"))) break;
     if ((rc = check_hip(c, hipMalloc(&c.sc, sizeof(DevScalars)), "hipMalloc scalars"))) break;
     if ((rc = check_hip(c, hipMemsetAsync(c.sc, 0, sizeof(DevScalars), c.stream), "hipMemset scalars"))) break;
     if ((rc = check_hip(c, hipHostMalloc((void**)&c.sc_host, sizeof(DevScalars), hipHostMallocDefault),
@@ -875,9 +876,9 @@ int cup_run_steps(cup_ctx* ctx, const cup_step_params* p, int nsteps)
     }
     if (body)
       if (int rc = launch_rhs_obstacle(c, nullptr)) return rc;
-    if (int rc = launch_poisson_fused(c, force_fused, p->tke_target, p->injection_rate)) return rc;
+    if (int rc = launch_poisson_fused(c, force_fused, p->tke_target, p->injection_rate, true)) return rc;
     stage_mark(c, ST_GRADP, false);
-    if (int rc = launch_gradp_fused(c, p->uinf, body ? c.ob_slot_all : nullptr, force_fused)) return rc;
+    if (int rc = launch_gradp_fused(c, p->uinf, body ? c.ob_slot_all : nullptr, force_fused, true)) return rc;
     stage_mark(c, ST_GRADP, true);
     if (body)
       if (int rc = launch_penalize_step(c, p->lambda, p->implicit_penalization, p->body_vel, p->body_angvel, p->uinf))

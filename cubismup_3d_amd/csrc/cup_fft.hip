@@ -445,9 +445,12 @@ __global__ __launch_bounds__(NT) void k_fft_x_fwd(const T* __restrict__ src0, co
 }
 
 // Packed half spectrum -> real rows of a block-layout field. grid = (Ny/8, Nz).
-template <typename T, int N>
+// XF: also write each block's x faces (ix = 0 and 7) to xf[(b*2 + side)*64 + iz*8 + iy], so that
+// the gradient reads them as 64 contiguous values instead of one value per 32-byte sector.
+template <typename T, int N, bool XF = false>
 __global__ __launch_bounds__(NT) void k_fft_x_inv(const C2<T>* __restrict__ S, T* __restrict__ dst,
-                                                  const C2<T>* __restrict__ tw, GridArgs g, FftPlan pl)
+                                                  const C2<T>* __restrict__ tw, GridArgs g, FftPlan pl,
+                                                  T* __restrict__ xf = nullptr)
 {
   const int n = N > 0 ? N : pl.n;
   C2<T>* buf = lds_tile<T, N * XFFT>();
@@ -475,8 +478,9 @@ __global__ __launch_bounds__(NT) void k_fft_x_inv(const C2<T>* __restrict__ S, T
     const int x = bx * BS + ix;
     const int b = bx + g.nbx * (by + g.nby * bz);
     // streamed past the caches: the gradient pass reads p with its own halo order
-    __builtin_nontemporal_store(rb[2 * (x * XFFT + (iy >> 1)) + (iy & 1)],
-                                &dst[(size_t)b * BS3 + (iz * BS + iy) * BS + ix]);
+    const T v = rb[2 * (x * XFFT + (iy >> 1)) + (iy & 1)];
+    __builtin_nontemporal_store(v, &dst[(size_t)b * BS3 + (iz * BS + iy) * BS + ix]);
+    if (XF && (ix == 0 || ix == BS - 1)) xf[((size_t)b * 2 + (ix != 0)) * 64 + iz * BS + iy] = v;
   }
 }
 
@@ -799,11 +803,15 @@ struct XPass {
                        (const C2<T>*)c.tw[0], c.grid(), c.sc, (const int*)nullptr, (const T*)nullptr,
                        (double*)nullptr, c.plan[0]);
   }
-  static void inv(Ctx& c, const C2<T>* spec, T* dst)
+  static void inv(Ctx& c, const C2<T>* spec, T* dst, T* xf = nullptr)
   {
     dim3 grid(c.N[1] / BS, c.N[2]);
-    hipLaunchKernelGGL((k_fft_x_inv<T, N>), grid, dim3(NT), lds(c), c.stream, spec, dst, (const C2<T>*)c.tw[0], c.grid(),
-                       c.plan[0]);
+    if (xf)
+      hipLaunchKernelGGL((k_fft_x_inv<T, N, true>), grid, dim3(NT), lds(c), c.stream, spec, dst, (const C2<T>*)c.tw[0],
+                         c.grid(), c.plan[0], xf);
+    else
+      hipLaunchKernelGGL((k_fft_x_inv<T, N>), grid, dim3(NT), lds(c), c.stream, spec, dst, (const C2<T>*)c.tw[0],
+                         c.grid(), c.plan[0], (T*)nullptr);
   }
 };
 
@@ -901,7 +909,7 @@ struct GoXFwdTo {
 };
 template <typename T, int N>
 struct GoXInv {
-  static void go(Ctx& c, const C2<T>* spec, T* dst) { XPass<T, N>::inv(c, spec, dst); }
+  static void go(Ctx& c, const C2<T>* spec, T* dst, T* xf = nullptr) { XPass<T, N>::inv(c, spec, dst, xf); }
 };
 template <typename T, int N>
 struct GoY {
@@ -934,7 +942,7 @@ static double poisson_norm(const Ctx& c)
 }
 
 template <typename T>
-static int poisson_fused_t(Ctx& c, bool force, double tke_target, double inj_rate)
+static int poisson_fused_t(Ctx& c, bool force, double tke_target, double inj_rate, bool xface)
 {
   C2<T>* S = (C2<T>*)c.spec;
   const double norm = poisson_norm(c);
@@ -971,8 +979,9 @@ static int poisson_fused_t(Ctx& c, bool force, double tke_target, double inj_rat
   dispatch_n<GoY, T>(c.N[1], c, S, (int)ZM_INV, norm);
   stage_mark(c, ST_YINV, true);
   stage_mark(c, ST_XINV, false);
-  dispatch_n<GoXInv, T>(c.N[0], c, (const C2<T>*)S, (T*)c.f[CUP_P]);
+  dispatch_n<GoXInv, T>(c.N[0], c, (const C2<T>*)S, (T*)c.f[CUP_P], xface ? (T*)c.xface : (T*)nullptr);
   stage_mark(c, ST_XINV, true);
+  c.xface_ok = xface;
   return check_hip(c, hipGetLastError(), "poisson_fused");
 }
 
@@ -995,11 +1004,12 @@ static int poisson_plain_t(Ctx& c)
   return check_hip(c, hipGetLastError(), "poisson_plain");
 }
 
-int launch_poisson_fused(Ctx& c, bool force, double tke_target, double inj_rate)
+int launch_poisson_fused(Ctx& c, bool force, double tke_target, double inj_rate, bool xface)
 {
+  c.xface_ok = false;
   if (force && !c.low_fused_ok) return set_error(c, CUP_E_INVALID, "fused forcing: low-mode set not supported");
-  return c.precision == 4 ? poisson_fused_t<float>(c, force, tke_target, inj_rate)
-                          : poisson_fused_t<double>(c, force, tke_target, inj_rate);
+  return c.precision == 4 ? poisson_fused_t<float>(c, force, tke_target, inj_rate, xface)
+                          : poisson_fused_t<double>(c, force, tke_target, inj_rate, xface);
 }
 
 int launch_poisson_plain(Ctx& c)

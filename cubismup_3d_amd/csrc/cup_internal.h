@@ -154,6 +154,10 @@ struct Ctx {
   void* spec2 = nullptr;  // extra spectra for the plain forcing path
   void* spec3 = nullptr;
   void* rbuf = nullptr;   // block-layout real scratch (plain RHS / solution)
+  // p's x faces as the fused step's x inverse pass writes them beside p: per block, side 0
+  // (ix = 0) and side 1 (ix = 7), 64 values each at iz*8 + iy, read by the gradient that follows
+  void* xface = nullptr;
+  bool xface_ok = false;  // xface holds the faces of the current p
   void* tw[3] = {};       // FFT twiddles per axis, e^{-2 pi i t / N}, context precision
   double* cos4[3] = {};   // cos(4 pi k / N) per axis (FP64), the Poisson symbol
   FftPlan plan[3];        // per axis, for the lengths that are not powers of two
@@ -322,7 +326,9 @@ struct AdjBodyArgs {
 int launch_adjoint_body(Ctx& c, AdjBodyPhase phase, const AdjBodyArgs& a);
 int launch_rhs_plain(Ctx& c);                                   // -> rbuf
 int launch_gradp_plain(Ctx& c);                                 // u += -dt/2h grad p
-int launch_gradp_fused(Ctx& c, const double uinf[3], const int* skip_slot, bool force = false);  // u = tmp - dt/2h grad p; max, ke_part
+// xface: take p's x faces from c.xface when the fused Poisson solve just wrote them
+int launch_gradp_fused(Ctx& c, const double uinf[3], const int* skip_slot, bool force = false,
+                       bool xface = false);  // u = tmp - dt/2h grad p; max, ke_part
 int launch_diagnostics(Ctx& c, double nu, int* nparts);  // partials[nparts][20]
 int launch_ic_taylor_green(Ctx& c, double uMax);
 int launch_fill(Ctx& c, void* ptr, size_t n, double value);
@@ -342,7 +348,9 @@ inline unsigned long long* umax_slots_dev(const Ctx& c, int k) { return &c.sc->u
 int launch_finalize_step(Ctx& c, double CFL, double nu, int rampup, double fixed_dt, int advance);
 // cup_fft.hip
 int fft_setup(Ctx& c);
-int launch_poisson_fused(Ctx& c, bool force = false, double tke_target = 0, double inj_rate = 0);  // div(tmp) -> p
+// xface: the x inverse pass also writes p's x faces to c.xface (for launch_gradp_fused)
+int launch_poisson_fused(Ctx& c, bool force = false, double tke_target = 0, double inj_rate = 0,
+                         bool xface = false);  // div(tmp) -> p
 int launch_poisson_plain(Ctx& c);   // rhs in rbuf -> solution in rbuf (separate divide pass)
 int launch_fft_forward_field(Ctx& c, const void* real_blocks, void* spec);  // 3 passes
 // HIT analysis: out[0..2] = raw sums tke, eps, l_integral numerator; E[nbin] raw shell sums;

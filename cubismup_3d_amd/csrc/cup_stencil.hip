@@ -1098,6 +1098,30 @@ int launch_rhs_plain(Ctx& c)
 
 // --------------------------------------------------------------- pressure gradient
 
+// lab_fetch of p on a periodic grid with the x faces taken from the x inverse pass's face rows
+// (xf[(b*2 + side)*64 + iz*8 + iy], side 1 = ix 7): a wave reads its face as 64 contiguous values.
+// The x faces are the halo slots k = 0 of waves 0 (x-) and 1 (x+), where the slot's (sz, sy) is
+// (c >> 3 & 7, c & 7) with c = t, i.e. iz*8 + iy = t & 63.
+template <typename T>
+__device__ __forceinline__ void lab_fetch_xf(LabRegs<T, 1>& r, const T* __restrict__ p, const T* __restrict__ xf,
+                                             const LabSlots& s, const GridArgs& g, int b)
+{
+  using V2 = typename Vec2<T>::type;
+  int bx, by, bz;
+  block_coords(g, b, bx, by, bz);
+  const V2 a = *reinterpret_cast<const V2*>(p + (size_t)b * BS3 + 2 * threadIdx.x);
+  r.in[0][0] = a.x;
+  r.in[0][1] = a.y;
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+    if (k == 0 || has_halo2()) {
+      const int face = __builtin_amdgcn_readfirstlane(s.face[k]);
+      const size_t nb = (size_t)face_block(g, bx, by, bz, face);
+      if (face < 2) r.hv[0][k] = xf[(nb * 2 + (face == 0)) * 64 + (threadIdx.x & 63)];
+      else r.hv[0][k] = p[nb * BS3 + s.off[k]];
+    }
+}
+
 // KernelGradP (PressureProjection.cpp:38-50): u += -dt/(2h) (p_E - p_W), etc.
 // PLAIN: in place on u (which holds u* after the advection copy).
 // FUSED: u = tmp + fac*grad p, plus the step's reductions: max |u+uinf| over the
@@ -1113,7 +1137,7 @@ int launch_rhs_plain(Ctx& c)
 // block's 64 rows x 16 coefficients are staged in LDS by one coalesced 4-value load per
 // thread. A thread's x is the same in every block of its walk, so its e^{-i a x} and
 // e^{-2 i a x} are formed once.
-template <typename T, bool FUSED, bool FORCE, bool BC>
+template <typename T, bool FUSED, bool FORCE, bool BC, bool XF = false>
 __global__ __launch_bounds__(NT) void k_gradp(const T* __restrict__ p, const T* __restrict__ su,
                                               const T* __restrict__ sv, const T* __restrict__ sw, T* __restrict__ u,
                                               T* __restrict__ v, T* __restrict__ w, GridArgs g,
@@ -1121,7 +1145,7 @@ __global__ __launch_bounds__(NT) void k_gradp(const T* __restrict__ p, const T* 
                                               double* __restrict__ ke_part, const int* __restrict__ skip_slot,
                                               const T* __restrict__ rows, const C2<T>* __restrict__ twx, int nbw,
                                               unsigned long long* __restrict__ slots,
-                                              unsigned long long* __restrict__ clear_slots)
+                                              unsigned long long* __restrict__ clear_slots, const T* __restrict__ xf)
 {
   using V2 = typename Vec2<T>::type;
   using V4 = typename Vec4<T>::type;
@@ -1138,7 +1162,8 @@ __global__ __launch_bounds__(NT) void k_gradp(const T* __restrict__ p, const T* 
   T us[3][2];
   T rr[4];
   auto fetch = [&](int b) {
-    lab_fetch<BC>(r, src, s, g, b, {HK_COPY});
+    if (XF) lab_fetch_xf(r, p, xf, s, g, b);
+    else lab_fetch<BC>(r, src, s, g, b, {HK_COPY});
     const size_t o = (size_t)b * BS3 + c0;
     load2_nt(su + o, us[0][0], us[0][1]);
     load2_nt(sv + o, us[1][0], us[1][1]);
@@ -1232,33 +1257,44 @@ __global__ __launch_bounds__(NT) void k_gradp(const T* __restrict__ p, const T* 
 }
 
 template <typename T>
-static int gradp_t(Ctx& c, bool fused, const double uinf[3], const int* skip, bool force)
+static int gradp_t(Ctx& c, bool fused, const double uinf[3], const int* skip, bool force, bool xface = false)
 {
   const T* p = (const T*)c.f[CUP_P];
   const int s = fused ? c.ustar : CUP_U;
   const C2<T>* tx = (const C2<T>*)c.tw[0];
   const int nbw = blocks_per_wg(c.bpd[2]);
   const dim3 grid(c.nb / nbw);
-  if (fused && force)
+  const T* xf = (const T*)c.xface;
+  if (fused && force && xface)
+    hipLaunchKernelGGL((k_gradp<T, true, true, false, true>), grid, dim3(NT), 0, c.stream, p, (const T*)c.f[s],
+                       (const T*)c.f[s + 1], (const T*)c.f[s + 2], (T*)c.f[CUP_U], (T*)c.f[CUP_V], (T*)c.f[CUP_W],
+                       c.grid(), c.sc, uinf[0], uinf[1], uinf[2], c.ke_part, skip, (const T*)c.low_rows, tx, nbw,
+                       umax_slots_dev(c, c.umax_bank), umax_slots_dev(c, c.umax_bank ^ 1), xf);
+  else if (fused && xface)
+    hipLaunchKernelGGL((k_gradp<T, true, false, false, true>), grid, dim3(NT), 0, c.stream, p, (const T*)c.f[s],
+                       (const T*)c.f[s + 1], (const T*)c.f[s + 2], (T*)c.f[CUP_U], (T*)c.f[CUP_V], (T*)c.f[CUP_W],
+                       c.grid(), c.sc, uinf[0], uinf[1], uinf[2], c.ke_part, skip, (const T*)nullptr, tx, nbw,
+                       umax_slots_dev(c, c.umax_bank), umax_slots_dev(c, c.umax_bank ^ 1), xf);
+  else if (fused && force)
     hipLaunchKernelGGL((k_gradp<T, true, true, false>), grid, dim3(NT), 0, c.stream, p, (const T*)c.f[s],
                        (const T*)c.f[s + 1], (const T*)c.f[s + 2], (T*)c.f[CUP_U], (T*)c.f[CUP_V], (T*)c.f[CUP_W],
                        c.grid(), c.sc, uinf[0], uinf[1], uinf[2], c.ke_part, skip, (const T*)c.low_rows, tx, nbw,
-                       umax_slots_dev(c, c.umax_bank), umax_slots_dev(c, c.umax_bank ^ 1));
+                       umax_slots_dev(c, c.umax_bank), umax_slots_dev(c, c.umax_bank ^ 1), nullptr);
   else if (fused)
     hipLaunchKernelGGL((k_gradp<T, true, false, false>), grid, dim3(NT), 0, c.stream, p, (const T*)c.f[s],
                        (const T*)c.f[s + 1], (const T*)c.f[s + 2], (T*)c.f[CUP_U], (T*)c.f[CUP_V], (T*)c.f[CUP_W],
                        c.grid(), c.sc, uinf[0], uinf[1], uinf[2], c.ke_part, skip, (const T*)nullptr, tx, nbw,
-                       umax_slots_dev(c, c.umax_bank), umax_slots_dev(c, c.umax_bank ^ 1));
+                       umax_slots_dev(c, c.umax_bank), umax_slots_dev(c, c.umax_bank ^ 1), nullptr);
   else if (c.periodic())
     hipLaunchKernelGGL((k_gradp<T, false, false, false>), grid, dim3(NT), 0, c.stream, p, (const T*)c.f[s],
                        (const T*)c.f[s + 1], (const T*)c.f[s + 2], (T*)c.f[CUP_U], (T*)c.f[CUP_V], (T*)c.f[CUP_W],
                        c.grid(), c.sc, 0.0, 0.0, 0.0, (double*)nullptr, (const int*)nullptr, (const T*)nullptr, tx,
-                       nbw, (unsigned long long*)nullptr, (unsigned long long*)nullptr);
+                       nbw, (unsigned long long*)nullptr, (unsigned long long*)nullptr, nullptr);
   else
     hipLaunchKernelGGL((k_gradp<T, false, false, true>), grid, dim3(NT), 0, c.stream, p, (const T*)c.f[s],
                        (const T*)c.f[s + 1], (const T*)c.f[s + 2], (T*)c.f[CUP_U], (T*)c.f[CUP_V], (T*)c.f[CUP_W],
                        c.grid(), c.sc, 0.0, 0.0, 0.0, (double*)nullptr, (const int*)nullptr, (const T*)nullptr, tx,
-                       nbw, (unsigned long long*)nullptr, (unsigned long long*)nullptr);
+                       nbw, (unsigned long long*)nullptr, (unsigned long long*)nullptr, nullptr);
   return check_hip(c, hipGetLastError(), "k_gradp");
 }
 
@@ -1273,13 +1309,13 @@ int launch_gradp_inplace(Ctx& c, int field0)
     hipLaunchKernelGGL((k_gradp<float, false, false, false>), grid, dim3(NT), 0, c.stream, (const float*)c.f[CUP_P], f[0],
                        f[1], f[2], f[0], f[1], f[2], c.grid(), c.sc, 0.0, 0.0, 0.0, (double*)nullptr,
                        (const int*)nullptr, (const float*)nullptr, (const C2<float>*)c.tw[0], nbw,
-                       (unsigned long long*)nullptr, (unsigned long long*)nullptr);
+                       (unsigned long long*)nullptr, (unsigned long long*)nullptr, nullptr);
   } else {
     double* f[3] = {(double*)c.f[field0], (double*)c.f[field0 + 1], (double*)c.f[field0 + 2]};
     hipLaunchKernelGGL((k_gradp<double, false, false, false>), grid, dim3(NT), 0, c.stream, (const double*)c.f[CUP_P],
                        f[0], f[1], f[2], f[0], f[1], f[2], c.grid(), c.sc, 0.0, 0.0, 0.0, (double*)nullptr,
                        (const int*)nullptr, (const double*)nullptr, (const C2<double>*)c.tw[0], nbw,
-                       (unsigned long long*)nullptr, (unsigned long long*)nullptr);
+                       (unsigned long long*)nullptr, (unsigned long long*)nullptr, nullptr);
   }
   return check_hip(c, hipGetLastError(), "k_gradp");
 }
@@ -1469,10 +1505,13 @@ int launch_gradp_plain(Ctx& c)
                           : gradp_t<double>(c, false, nullptr, nullptr, false);
 }
 
-int launch_gradp_fused(Ctx& c, const double uinf[3], const int* skip_slot, bool force)
+int launch_gradp_fused(Ctx& c, const double uinf[3], const int* skip_slot, bool force, bool xface)
 {
-  return c.precision == 4 ? gradp_t<float>(c, true, uinf, skip_slot, force)
-                          : gradp_t<double>(c, true, uinf, skip_slot, force);
+  // the faces are those of the p the last fused Poisson solve wrote, used once
+  const bool xf = xface && c.xface_ok && c.periodic();
+  c.xface_ok = false;
+  return c.precision == 4 ? gradp_t<float>(c, true, uinf, skip_slot, force, xf)
+                          : gradp_t<double>(c, true, uinf, skip_slot, force, xf);
 }
 
 // ------------------------------------------------------------------- reductions
