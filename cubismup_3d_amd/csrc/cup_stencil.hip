@@ -424,7 +424,9 @@ __device__ __forceinline__ void lab_put_xw(const LabRegs<T, NF>& r, const LabReg
 // First block of logical walk L along x (NBW blocks of a row of blocks). Rows of blocks
 // (by, bz) are ordered in tiles of ZT planes of blocks with bz fastest, so that a walk's z
 // neighbours run a few walks earlier or later (on the same XCD, whose consecutive logical ids
-// xcd_remap keeps together) and its y and z halo faces are still in L2.
+// xcd_remap keeps together) and its y and z halo faces are still in L2. The rows run from the
+// last by down to 0: the gradient writes u from by = 0 up, so the advection of the next step
+// starts on the rows written last, which the last-level cache still holds.
 template <int NBW>
 __device__ __forceinline__ int xwalk_first(const GridArgs& g, int L)
 {
@@ -432,7 +434,7 @@ __device__ __forceinline__ int xwalk_first(const GridArgs& g, int L)
   const int xq = L % nq, row = L / nq;
   const int zt = g.nbz % 4 == 0 ? 4 : (g.nbz % 2 == 0 ? 2 : 1);
   const int tile = row / (g.nby * zt), within = row - tile * (g.nby * zt);
-  const int by = within / zt, bz = tile * zt + (within - (within / zt) * zt);
+  const int by = g.nby - 1 - within / zt, bz = tile * zt + (within - (within / zt) * zt);
   return xq * NBW + g.nbx * (by + g.nby * bz);
 }
 
