@@ -172,7 +172,8 @@ def main():
         time_steps(G, 3, tke)
         stage_ms[st] = G.stage_timing_end()[0]
     dominant = max(STAGE_BYTES_PER_CELL, key=lambda k: stage_ms[k])
-    # timed region: K steps, events around the dominant kernel in every one of them
+    # timed region: K steps, events around the dominant kernel in every one of them after the
+    # first (whose events follow the idle stream of the opening synchronize)
     G.stage_timing_begin(dominant, args.steps)
     el = time_steps(G, args.steps, tke)
     dom_ms, dom_n = G.stage_timing_end()

@@ -113,7 +113,7 @@ void stage_mark(Ctx& c, int stage, bool end)
     roctxRangePop();
   else
     roctxRangePushA(kStageName[stage]);
-  if (stage != c.timing_stage || c.timing_count >= c.timing_cap) return;
+  if (stage != c.timing_stage || c.timing_skip || c.timing_count >= c.timing_cap) return;
   hipEventRecord(c.tev[2 * c.timing_count + (end ? 1 : 0)], c.stream);
   if (end) c.timing_count++;
 }
@@ -863,7 +863,15 @@ int cup_run_steps(cup_ctx* ctx, const cup_step_params* p, int nsteps)
   // With a body the forced u* must be in place before the RHS masking reads it (the fused
   // forcing adds its modes after), so the reference decomposition runs then as well.
   const bool force_fused = p->spectral_forcing && c.low_fused_ok && !body;
+  struct SkipReset {
+    Ctx& c;
+    ~SkipReset() { c.timing_skip = false; }
+  } skip_reset{c};
   for (int s = 0; s < nsteps; ++s) {
+    // The first step of the call is left out of the stage timing: its events follow an idle
+    // stream (they would hold the host's launch latency) and it runs without the folded
+    // bookkeeping of a previous step.
+    c.timing_skip = s == 0;
     const bool fold = fold_ok && s > 0;
     const StepFold sf{umax_slots_dev(c, c.umax_bank), step0 + s, p->CFL, p->rampup, p->fixed_dt};
     stage_mark(c, ST_ADVDIFF, false);
@@ -907,6 +915,7 @@ int cup_stage_timing_begin(cup_ctx* ctx, int stage, int max_steps)
   c.timing_stage = stage;
   c.timing_cap = max_steps;
   c.timing_count = 0;
+  c.timing_skip = false;
   return CUP_OK;
 }
 

@@ -194,6 +194,7 @@ struct Ctx {
   std::string err;
   // measurement hook: HIP events around one stage of the fused step
   int timing_stage = -1, timing_cap = 0, timing_count = 0;
+  bool timing_skip = false;  // the first step of a cup_run_steps call is not timed
   std::vector<hipEvent_t> tev;
   GridArgs grid() const {
     GridArgs g;

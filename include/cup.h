@@ -376,7 +376,9 @@ int cup_last_body_force_body(cup_ctx* ctx, int id, double out[6]);
 /* Measurement: the stream the context launches on, and HIP events recorded on it around
  * one stage of the FUSED step (0 advection-diffusion, 1 x-forward FFT with the RHS,
  * 2 forcing reduction, 3 y forward, 4 z pass with the division, 5 y inverse, 6 x inverse,
- * 7 gradient) for up to max_steps steps of the following cup_run_steps calls; _end
+ * 7 gradient) for up to max_steps steps of the following cup_run_steps calls, leaving out the
+ * first step of each call (it follows an idle stream, so its events would hold the host's
+ * launch latency, and it runs without the previous step's folded bookkeeping); _end
  * synchronizes and returns the average duration (ms) and the number of launches timed.
  * No counterpart in the reference (its cubism::Profiler regions time host phases). */
 int cup_stream(cup_ctx* ctx, void** hip_stream);
